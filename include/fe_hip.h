@@ -1,0 +1,191 @@
+/*
+ * fe_hip.h -- C ABI of libfe_hip.so: the MI355X (gfx950) assemble-and-solve
+ * path of feNG.  One context = one GPU-resident linear system: mesh, element
+ * -> DOF maps, device-built CSR, the (bi-)linear forms, the solution vector
+ * and the Krylov workspace.
+ *
+ * The entry points map one-to-one onto the reference's feLinearSystem
+ * virtuals (feLinearSystem.h:85-168) and its createLinearSystem factory
+ * (feLinearSystem.h:33-37); the element kernels replace the feSysElm
+ * computeAe/computeBe virtuals (feSysElm.h:187-251) behind feBilinearForm.
+ *
+ * Conventions: every function returns an int feStatus (feMessage.h:44-62,
+ * 0 = OK, negative = error); no exception crosses the ABI; host pointers are
+ * borrowed for the duration of the call; device work is ordered on the
+ * context's stream and a call returns after its work has completed (unless
+ * stated otherwise).  One host thread per context.
+ */
+#ifndef FE_HIP_H
+#define FE_HIP_H
+
+#include <stdint.h>
+
+#include "fe_mesh.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fe_hip_ctx fe_hip_ctx;
+
+/* Weak-form ids: the reference's elementSystemType values (feSysElm.h:13-77)
+ * for the terms the device kernels implement. */
+#define FE_SYS_SOURCE 0
+#define FE_SYS_VECTOR_SOURCE 2
+#define FE_SYS_DIFFUSION 16
+#define FE_SYS_VECTOR_DIFFUSION 18
+#define FE_SYS_VECTOR_CONVECTIVE_ACCELERATION 22
+#define FE_SYS_DIV_NEWTONIAN_STRESS 25
+#define FE_SYS_MIXED_GRADIENT 26
+#define FE_SYS_MIXED_DIVERGENCE 31
+
+/* Coefficient kinds: the feFunction callbacks (feFunction.h:75-110) the
+ * device evaluates at quadrature points, or caller tables. */
+#define FE_FN_CONST 0   /* c0 */
+#define FE_FN_SINSIN 1  /* c0 sin(pi x) sin(pi y) */
+#define FE_FN_BLOCKS 2  /* exp(0.5(((7 bi + 13 bj) mod 11) - 5)) on c0 x c0 blocks */
+#define FE_FN_SMOOTH3 3 /* exp(c0 sin(2 pi x) sin(2 pi y) sin(2 pi z)) */
+#define FE_FN_ELEM 4    /* data[e] (count = E) */
+#define FE_FN_QP 5      /* data[e nq + q] (count = E nq) */
+#define FE_FN_KOV_U 6   /* Kovasznay velocity component c1 (0 or 1), Re = c0 */
+#define FE_FN_KOV_P 7   /* Kovasznay pressure, Re = c0 */
+
+typedef struct {
+  int kind;
+  double c0, c1;
+  const double *data; /* host table for FE_FN_ELEM / FE_FN_QP, copied at add time */
+  int64_t count;
+} fe_hip_fn;
+
+/* Krylov method / preconditioner / precision (the KSP and PC types the
+ * reference selects through PETSc options, feLinearSystemPETSc.cpp:900-1031). */
+#define FE_KSP_CG 0
+#define FE_KSP_GMRES 1
+
+#define FE_PC_NONE 0
+#define FE_PC_JACOBI 1
+#define FE_PC_IC0 2
+#define FE_PC_ILU0 3
+#define FE_PC_BJ_ILU0 4
+#define FE_PC_LEARNED 5
+
+#define FE_PREC_FP64 0    /* FP64 Krylov */
+#define FE_PREC_FP32_IR 1 /* FP32 inner Krylov + FP64 iterative refinement */
+
+typedef struct {
+  int ksp;           /* FE_KSP_* */
+  int pc;            /* FE_PC_* */
+  int precision;     /* FE_PREC_* */
+  int restart;       /* GMRES restart length */
+  int max_iter;      /* total inner iterations (reference default 1e4) */
+  double rtol;       /* ||b - A x||_2 / ||b||_2 target, checked in FP64 */
+  double atol;       /* absolute ||b - A x||_2 target (reference default 1e-14) */
+  double dtol;       /* divergence tolerance (reference default 1e6) */
+  double inner_rtol; /* FP32_IR: relative reduction per inner solve */
+  int check_every;   /* iterations between host convergence checks */
+} fe_hip_solver_opts;
+
+typedef struct {
+  int iterations;       /* total Krylov iterations */
+  int outer_iterations; /* refinement steps (FP32_IR) */
+  int converged;        /* 1 when the FP64 true residual met the target */
+  int reason;           /* >0 converged, <0 diverged (PETSc KSPConvergedReason style) */
+  double relres_fp64;   /* ||b - A x||_2 / ||b||_2 in FP64 */
+  double norm_dx_inf;   /* ||du||_inf */
+  double norm_rhs_inf;  /* ||b||_inf */
+  double norm_axb_inf;  /* ||A du - b||_inf */
+  double ms_solve;      /* device time of the solve (hipEvents) */
+  double ms_setup;      /* preconditioner setup included in ms_solve */
+} fe_hip_solve_stats;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* createLinearSystem (feLinearSystem.cpp:7-67) counterpart: a context on HIP
+ * device `device`.  Fails with FE_STATUS_ERROR when no GPU is present. */
+int fe_hip_ctx_create(int device, fe_hip_ctx **out);
+int fe_hip_ctx_destroy(fe_hip_ctx *ctx);
+/* Last error message of this context (static storage owned by the context). */
+const char *fe_hip_last_error(const fe_hip_ctx *ctx);
+
+/* Mesh geometry (feMesh getCoord, feMesh.cpp:119-134): dim 2 triangles or
+ * dim 3 tets, straight (P1) geometry. */
+int fe_hip_upload_mesh(fe_hip_ctx *ctx, int dim, int V, const double *xyz, int E, int nvpe,
+                       const int *conn);
+/* Element coloring (feCncGeo.cpp:735-794 semantics): colorOf[E]; elements of
+ * one color share no vertex, so each color is one atomic-free scatter sweep. */
+int fe_hip_set_coloring(fe_hip_ctx *ctx, const int *colorOf);
+/* One field (an feSpace): kind FE_TRI_* / FE_TET_P1, element -> DOF map
+ * adr[E nloc] from feMetaNumber (initializeAddressingVector order). Returns
+ * the field index in *field. */
+int fe_hip_add_field(fe_hip_ctx *ctx, int kind, const int *adr, int *field);
+/* System size n = nInc and total DOFs (feMetaNumber::getNbUnknowns /
+ * getNbDOFs, feNumber.h:202-203). */
+int fe_hip_set_sizes(fe_hip_ctx *ctx, int nInc, int nDofs);
+
+/* One (bi-)linear form (createBilinearForm, feBilinearForm.cpp:6-73):
+ * sysType FE_SYS_*, the fields it acts on (same order as the reference's
+ * space vector), quadrature degree, up to 4 coefficient callbacks in the
+ * order the reference's feSysElm constructor takes them. Returns the form id. */
+int fe_hip_add_form(fe_hip_ctx *ctx, int sysType, int nfields, const int *fields,
+                    int quad_degree, int ncoeff, const fe_hip_fn *coeffs, int *form);
+
+/* feEZCompressedRowStorage (feCompressedRowStorage.cpp:15-133) built on the
+ * device from the element -> DOF maps of the matrix forms: ia[n+1], ja[nnz],
+ * rows sorted, diagonal always present, only unknown rows and columns. */
+int fe_hip_build_csr(fe_hip_ctx *ctx, int64_t *nnz);
+int fe_hip_get_csr(fe_hip_ctx *ctx, int *ia, int *ja);
+
+/* ---- solution, assembly ------------------------------------------------- */
+
+/* feSolution values (nDofs doubles, essential DOFs holding their BC values). */
+int fe_hip_set_solution(fe_hip_ctx *ctx, const double *u);
+int fe_hip_get_solution(fe_hip_ctx *ctx, double *u);
+
+/* setToZero / setMatrixToZero / setResidualToZero (feLinearSystem.h:112-114). */
+int fe_hip_set_to_zero(fe_hip_ctx *ctx, int matrix, int residual);
+
+/* assembleMatrices / assembleResiduals / assemble (feLinearSystemPETSc.cpp:
+ * 543-805): every form's element kernel on every element, one colored sweep
+ * per color, adding into the CSR values and the RHS.  what: 1 = matrix,
+ * 2 = residual, 3 = both (fused: one kernel per form and color computes Ae
+ * and Be together).  precision: FE_PREC_FP64 computes in FP64;
+ * FE_PREC_FP32_IR additionally keeps an FP32 copy of the values for the
+ * inner solve. */
+int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision);
+
+/* getRHSMaxNorm (feLinearSystem.h:104). */
+int fe_hip_rhs_max_norm(fe_hip_ctx *ctx, double *norm);
+
+/* Copy back the assembled system: val[nnz] (may be NULL), rhs[n] (may be NULL). */
+int fe_hip_get_system(fe_hip_ctx *ctx, double *val, double *rhs);
+/* Overwrite the system values / RHS from the host (used for externally
+ * assembled or perturbed systems). */
+int fe_hip_set_system(fe_hip_ctx *ctx, const double *val, const double *rhs);
+
+/* ---- solve -------------------------------------------------------------- */
+
+void fe_hip_default_opts(fe_hip_solver_opts *opts);
+
+/* solve (feLinearSystemPETSc.cpp:900-1031): du = A^-1 rhs with the Krylov
+ * method and preconditioner of opts, x0 = 0, stopped on the FP64 true
+ * residual.  du (n doubles, may be NULL) receives the correction. */
+int fe_hip_solve(fe_hip_ctx *ctx, const fe_hip_solver_opts *opts, double *du,
+                 fe_hip_solve_stats *stats);
+
+/* correctSolution (feLinearSystemPETSc.cpp:1033-1060): u[i] += du[i], i < n. */
+int fe_hip_correct_solution(fe_hip_ctx *ctx);
+
+/* ---- measurement -------------------------------------------------------- */
+
+/* Average device time (ms, hipEvents on the context stream) of the last
+ * `fe_hip_solve`'s dominant kernel (the SpMV sweep) and its launch count. */
+int fe_hip_kernel_stats(fe_hip_ctx *ctx, int which, double *avg_ms, int64_t *launches);
+/* Enable per-launch event timing of the dominant kernel (adds two events per
+ * launch). */
+int fe_hip_set_profiling(fe_hip_ctx *ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,970 @@
+/* CPU oracle for the feNG assemble-and-solve path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference algorithm, used by tests/ (as the
+ * parity checker), by __graft_entry__.smoke() and by the cpu_baseline leg of
+ * bench.py.  The product never links or calls it.
+ *
+ * Parity pins (tests/test_oracle_*.py): every function below is checked
+ * against vectors dumped from the reference itself (oracle/_ref, fixtures in
+ * tests/golden/) and/or against closed forms:
+ *   - or_tri_edges, or_number, or_csr: bit-exact vs reference fixtures.
+ *   - or_quad_*: exact integration of monomials (quadratureTri.cpp closed form).
+ *   - or_assemble_diffusion: <=1e-12 vs reference fixtures; P1 textbook stiffness.
+ *   - or_assemble_ns: <=1e-12 vs reference fixtures (Taylor-Hood, Kovasznay).
+ *   - solvers: residual ||b - A x|| checked directly, CG vs dense solve on
+ *     tiny systems; IC(0)/ILU(0) reproduce the exact factor on a tridiagonal
+ *     matrix (no fill), where they equal the complete Cholesky / LU factor.
+ *   - or_l2_error_sinsin: reference convergence table scalarFE_diffusion.output.
+ */
+#include "fe_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Edges: feMeshRead.cpp:1412-1456.  An edge is keyed on its (min, max) vertex
+ * pair; the tag counter starts at 1 and increments on each new insertion; the
+ * element stores +tag for the first encounter and -tag afterwards.  Edge k of a
+ * triangle joins local vertices (k, k+1 mod 3). */
+typedef struct {
+  int64_t *keys;
+  int *vals;
+  int64_t cap;
+} ihash;
+
+static void ih_init(ihash *h, int64_t n)
+{
+  int64_t cap = 16;
+  while(cap < 2 * n) cap <<= 1;
+  h->cap = cap;
+  h->keys = (int64_t *)malloc(sizeof(int64_t) * cap);
+  h->vals = (int *)malloc(sizeof(int) * cap);
+  for(int64_t i = 0; i < cap; ++i) h->keys[i] = -1;
+}
+static void ih_free(ihash *h)
+{
+  free(h->keys);
+  free(h->vals);
+}
+/* returns pointer to value slot; *found set when key existed */
+static int *ih_get(ihash *h, int64_t key, int *found)
+{
+  uint64_t x = (uint64_t)key * 0x9E3779B97F4A7C15ull;
+  int64_t i = (int64_t)(x >> 17) & (h->cap - 1);
+  while(h->keys[i] != -1 && h->keys[i] != key) i = (i + 1) & (h->cap - 1);
+  *found = (h->keys[i] == key);
+  if(!*found) h->keys[i] = key;
+  return &h->vals[i];
+}
+
+int or_tri_edges(int E, const int *conn, int *edges)
+{
+  ihash h;
+  ih_init(&h, (int64_t)E * 2 + 16);
+  int tag = 1;
+  for(int e = 0; e < E; ++e) {
+    for(int k = 0; k < 3; ++k) {
+      int a = conn[3 * e + k], b = conn[3 * e + (k + 1) % 3];
+      int lo = a < b ? a : b, hi = a < b ? b : a;
+      int found;
+      int *slot = ih_get(&h, ((int64_t)lo << 32) | (int64_t)hi, &found);
+      if(!found) {
+        *slot = tag;
+        edges[3 * e + k] = tag++;
+      } else {
+        edges[3 * e + k] = -*slot;
+      }
+    }
+  }
+  ih_free(&h);
+  return tag - 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Coloring: feCncGeo.cpp:735-794 (algorithm 1).  Each pass walks the
+ * uncolored elements in order; an element still unmarked when visited joins
+ * the current color and marks its uncolored vertex-neighbours for later
+ * passes. */
+int or_color_elements(int E, int nvpe, const int *conn, int V, int *colorOf)
+{
+  int *cnt = (int *)calloc((size_t)V + 1, sizeof(int));
+  for(int i = 0; i < E * nvpe; ++i) cnt[conn[i] + 1]++;
+  for(int v = 0; v < V; ++v) cnt[v + 1] += cnt[v];
+  int *lst = (int *)malloc(sizeof(int) * (size_t)E * nvpe);
+  int *pos = (int *)malloc(sizeof(int) * (size_t)V);
+  memcpy(pos, cnt, sizeof(int) * V);
+  for(int e = 0; e < E; ++e)
+    for(int j = 0; j < nvpe; ++j) lst[pos[conn[e * nvpe + j]]++] = e;
+  for(int e = 0; e < E; ++e) colorOf[e] = -1;
+  int ncolor = 0, remaining = E;
+  while(remaining > 0) {
+    for(int k = 0; k < E; ++k) {
+      if(colorOf[k] != -1) continue;
+      for(int i = 0; i < nvpe; ++i) {
+        int s = conn[k * nvpe + i];
+        for(int j = cnt[s]; j < cnt[s + 1]; ++j) {
+          int m = lst[j];
+          if(m != k && colorOf[m] < 0) colorOf[m] = -2;
+        }
+      }
+    }
+    for(int k = 0; k < E; ++k) {
+      if(colorOf[k] == -1) {
+        colorOf[k] = ncolor;
+        remaining--;
+      } else if(colorOf[k] == -2) {
+        colorOf[k] = -1;
+      }
+    }
+    ncolor++;
+  }
+  free(cnt);
+  free(lst);
+  free(pos);
+  return ncolor;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Numbering: feNumber.cpp:370-483 (numberUnknowns / numberEssential) driven by
+ * feMetaNumber (feNumber.cpp:677-843).  Per field, in order: unknown vertex
+ * DOFs (vertex ascending, components interleaved), unknown edge DOFs (edge
+ * index ascending); then, after all fields' unknowns, per field: essential
+ * vertex DOFs, essential boundary-segment DOFs in boundary element order
+ * (they are the 1D "element" DOFs of the boundary space, copied to their edge,
+ * feNumber.cpp:452-466), then any remaining essential edge DOFs. */
+static int kind_vdofs(int kind) { return kind == OR_TRI_VP2 ? 2 : 1; }
+static int kind_edofs(int kind) { return kind == OR_TRI_P2 ? 1 : (kind == OR_TRI_VP2 ? 2 : 0); }
+static int kind_nloc(int kind)
+{
+  switch(kind) {
+    case OR_TRI_P1: return 3;
+    case OR_TRI_P2: return 6;
+    case OR_TRI_VP2: return 12;
+    case OR_TET_P1: return 4;
+  }
+  return 0;
+}
+
+int or_number(int V, int E, int nvpe, const int *conn, int nEdges, const int *edges,
+              int nB, int bnv, const int *bconn, const int *bedge,
+              int nfields, const int *kinds, const int *essential,
+              int **adr_out, int *nInc, int *nDofs)
+{
+  const int UNK = -1, ESS = -2;
+  int **vnum = (int **)malloc(sizeof(int *) * nfields);
+  int **enum_ = (int **)malloc(sizeof(int *) * nfields);
+  for(int f = 0; f < nfields; ++f) {
+    int nv = kind_vdofs(kinds[f]), ne = kind_edofs(kinds[f]);
+    vnum[f] = (int *)malloc(sizeof(int) * (size_t)V * nv);
+    for(int i = 0; i < V * nv; ++i) vnum[f][i] = UNK; /* every vertex carries the field */
+    enum_[f] = NULL;
+    if(ne > 0) {
+      enum_[f] = (int *)malloc(sizeof(int) * (size_t)nEdges * ne);
+      for(int i = 0; i < nEdges * ne; ++i) enum_[f][i] = UNK;
+    }
+    if(essential[f]) {
+      for(int b = 0; b < nB; ++b) {
+        for(int j = 0; j < bnv; ++j) {
+          int v = bconn[b * bnv + j];
+          for(int c = 0; c < nv; ++c) vnum[f][v * nv + c] = ESS;
+        }
+        if(ne > 0)
+          for(int c = 0; c < ne; ++c) enum_[f][bedge[b] * ne + c] = ESS;
+      }
+    }
+  }
+  (void)conn;
+  (void)nvpe;
+  (void)E;
+  int g = 0;
+  for(int f = 0; f < nfields; ++f) {
+    int nv = kind_vdofs(kinds[f]), ne = kind_edofs(kinds[f]);
+    for(int i = 0; i < V * nv; ++i)
+      if(vnum[f][i] == UNK) vnum[f][i] = g++;
+    for(int i = 0; i < nEdges * ne; ++i)
+      if(enum_[f][i] == UNK) enum_[f][i] = g++;
+  }
+  *nInc = g;
+  for(int f = 0; f < nfields; ++f) {
+    int nv = kind_vdofs(kinds[f]), ne = kind_edofs(kinds[f]);
+    for(int i = 0; i < V * nv; ++i)
+      if(vnum[f][i] == ESS) vnum[f][i] = g++;
+    if(ne > 0) {
+      for(int b = 0; b < nB; ++b)
+        for(int c = 0; c < ne; ++c)
+          if(enum_[f][bedge[b] * ne + c] == ESS) enum_[f][bedge[b] * ne + c] = g++;
+      for(int i = 0; i < nEdges * ne; ++i)
+        if(enum_[f][i] == ESS) enum_[f][i] = g++;
+    }
+  }
+  *nDofs = g;
+  /* Addressing vectors: feSpace_2D.cpp:154-159 (P1), :772-786 (P2),
+   * :1017-1041 (vector P2: v0x v0y v1x v1y v2x v2y e0x e0y e1x e1y e2x e2y),
+   * feSpace_3D.cpp:293-345 (tet). */
+  for(int f = 0; f < nfields; ++f) {
+    int kind = kinds[f], nl = kind_nloc(kind);
+    int *adr = adr_out[f];
+    for(int e = 0; e < E; ++e) {
+      int *a = adr + (size_t)e * nl;
+      if(kind == OR_TRI_P1 || kind == OR_TET_P1) {
+        for(int j = 0; j < nl; ++j) a[j] = vnum[f][conn[e * nvpe + j]];
+      } else if(kind == OR_TRI_P2) {
+        for(int j = 0; j < 3; ++j) a[j] = vnum[f][conn[e * 3 + j]];
+        for(int k = 0; k < 3; ++k) a[3 + k] = enum_[f][abs(edges[e * 3 + k]) - 1];
+      } else {
+        for(int j = 0; j < 3; ++j)
+          for(int c = 0; c < 2; ++c) a[2 * j + c] = vnum[f][conn[e * 3 + j] * 2 + c];
+        for(int k = 0; k < 3; ++k)
+          for(int c = 0; c < 2; ++c) a[6 + 2 * k + c] = enum_[f][(abs(edges[e * 3 + k]) - 1) * 2 + c];
+      }
+    }
+  }
+  for(int f = 0; f < nfields; ++f) {
+    free(vnum[f]);
+    free(enum_[f]);
+  }
+  free(vnum);
+  free(enum_);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* CSR: feCompressedRowStorage.cpp:15-133.  Row i holds i itself plus every
+ * adrJ[j] < n that some element couples to a row adrI[i] = i (i < n); sorted,
+ * unique, 0-based. */
+static int cmp_int(const void *a, const void *b)
+{
+  int x = *(const int *)a, y = *(const int *)b;
+  return (x > y) - (x < y);
+}
+
+int64_t or_csr(int n, int nforms, const int *E, const int *nI, const int *nJ,
+               const int *const *adrI, const int *const *adrJ, int *ia, int *ja)
+{
+  int64_t *cnt = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+  for(int i = 0; i < n; ++i) cnt[i + 1] = 1;
+  for(int f = 0; f < nforms; ++f)
+    for(int e = 0; e < E[f]; ++e)
+      for(int i = 0; i < nI[f]; ++i) {
+        int r = adrI[f][(size_t)e * nI[f] + i];
+        if(r >= n) continue;
+        for(int j = 0; j < nJ[f]; ++j)
+          if(adrJ[f][(size_t)e * nJ[f] + j] < n) cnt[r + 1]++;
+      }
+  for(int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+  int *buf = (int *)malloc(sizeof(int) * (size_t)(cnt[n] > 0 ? cnt[n] : 1));
+  int64_t *pos = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+  for(int i = 0; i < n; ++i) {
+    pos[i] = cnt[i];
+    buf[pos[i]++] = i;
+  }
+  for(int f = 0; f < nforms; ++f)
+    for(int e = 0; e < E[f]; ++e)
+      for(int i = 0; i < nI[f]; ++i) {
+        int r = adrI[f][(size_t)e * nI[f] + i];
+        if(r >= n) continue;
+        for(int j = 0; j < nJ[f]; ++j) {
+          int c = adrJ[f][(size_t)e * nJ[f] + j];
+          if(c < n) buf[pos[r]++] = c;
+        }
+      }
+  int64_t nnz = 0;
+  for(int i = 0; i < n; ++i) {
+    int64_t b = cnt[i], len = cnt[i + 1] - cnt[i];
+    qsort(buf + b, (size_t)len, sizeof(int), cmp_int);
+    int64_t u = 0;
+    for(int64_t k = 0; k < len; ++k)
+      if(k == 0 || buf[b + k] != buf[b + k - 1]) buf[b + u++] = buf[b + k];
+    if(ia) ia[i] = (int)nnz;
+    if(ja)
+      for(int64_t k = 0; k < u; ++k) ja[nnz + k] = buf[b + k];
+    nnz += u;
+  }
+  if(ia) ia[n] = (int)nnz;
+  free(cnt);
+  free(buf);
+  free(pos);
+  return nnz;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Quadrature tables (data, as in feQuadratureTri.cpp:788-920 -- MFEM's
+ * symmetric triangle rules -- and the Solin tet tables of
+ * feQuadratureTet.cpp:96-110). */
+static int tri_pt(double *r, double *s, double *w, int off, double a, double b, double wt)
+{
+  r[off] = a;
+  s[off] = b;
+  w[off] = wt;
+  return off + 1;
+}
+static int tri_p3(double *r, double *s, double *w, int off, double a, double wt)
+{
+  double b = 1. - 2. * a;
+  off = tri_pt(r, s, w, off, a, a, wt);
+  off = tri_pt(r, s, w, off, a, b, wt);
+  return tri_pt(r, s, w, off, b, a, wt);
+}
+static int tri_p3r(double *r, double *s, double *w, int off, double a, double b, double c,
+                   double wt)
+{
+  off = tri_pt(r, s, w, off, a, b, wt);
+  off = tri_pt(r, s, w, off, c, a, wt);
+  return tri_pt(r, s, w, off, b, c, wt);
+}
+static int tri_p6(double *r, double *s, double *w, int off, double a, double b, double wt)
+{
+  double c = 1. - a - b;
+  off = tri_pt(r, s, w, off, a, b, wt);
+  off = tri_pt(r, s, w, off, b, a, wt);
+  off = tri_pt(r, s, w, off, a, c, wt);
+  off = tri_pt(r, s, w, off, c, a, wt);
+  off = tri_pt(r, s, w, off, b, c, wt);
+  return tri_pt(r, s, w, off, c, b, wt);
+}
+
+int or_quad_tri(int degree, double *r, double *s, double *w)
+{
+  int o = 0;
+  switch(degree) {
+    case 0:
+    case 1: o = tri_pt(r, s, w, 0, 1. / 3., 1. / 3., 0.5); break;
+    case 2: o = tri_p3(r, s, w, 0, 1. / 6., 1. / 6.); break;
+    case 3:
+      o = tri_pt(r, s, w, 0, 1. / 3., 1. / 3., -0.28125);
+      o = tri_p3(r, s, w, o, 0.2, 25. / 96.);
+      break;
+    case 4:
+      o = tri_p3(r, s, w, 0, 0.091576213509770743460, 0.054975871827660933819);
+      o = tri_p3(r, s, w, o, 0.44594849091596488632, 0.11169079483900573285);
+      break;
+    case 5:
+      o = tri_pt(r, s, w, 0, 1. / 3., 1. / 3., 0.1125);
+      o = tri_p3(r, s, w, o, 0.10128650732345633880, 0.062969590272413576298);
+      o = tri_p3(r, s, w, o, 0.47014206410511508977, 0.066197076394253090369);
+      break;
+    case 6:
+      o = tri_p3(r, s, w, 0, 0.063089014491502228340, 0.025422453185103408460);
+      o = tri_p3(r, s, w, o, 0.24928674517091042129, 0.058393137863189683013);
+      o = tri_p6(r, s, w, o, 0.053145049844816947353, 0.31035245103378440542,
+                 0.041425537809186787597);
+      break;
+    case 7:
+      o = tri_p3r(r, s, w, 0, 0.062382265094402118174, 0.067517867073916085443,
+                  1. - 0.062382265094402118174 - 0.067517867073916085443,
+                  0.026517028157436251429);
+      o = tri_p3r(r, s, w, o, 0.055225456656926611737, 0.32150249385198182267,
+                  1. - 0.055225456656926611737 - 0.32150249385198182267,
+                  0.043881408714446055037);
+      o = tri_p3r(r, s, w, o, 0.034324302945097146470, 0.66094919618673565761,
+                  0.30472650086816719592, 0.028775042784981585738);
+      o = tri_p3r(r, s, w, o, 0.51584233435359177926, 0.27771616697639178257,
+                  0.20644149867001643817, 0.067493187009802774463);
+      break;
+    case 8:
+      o = tri_pt(r, s, w, 0, 1. / 3., 1. / 3., 0.0721578038388935841255455552445323);
+      o = tri_p3(r, s, w, o, 0.170569307751760206622293501491464,
+                 0.0516086852673591251408957751460645);
+      o = tri_p3(r, s, w, o, 0.0505472283170309754584235505965989,
+                 0.0162292488115990401554629641708902);
+      o = tri_p3(r, s, w, o, 0.459292588292723156028815514494169,
+                 0.0475458171336423123969480521942921);
+      o = tri_p6(r, s, w, o, 0.008394777409957605337213834539296,
+                 0.263112829634638113421785786284643, 0.0136151570872174971324223450369544);
+      break;
+    default: return -1;
+  }
+  return o;
+}
+
+int or_quad_tet(int degree, double *r, double *s, double *t, double *w)
+{
+  if(degree <= 1) {
+    r[0] = s[0] = t[0] = 0.25;
+    w[0] = 0.166666666666667;
+    return 1;
+  }
+  if(degree == 2) {
+    const double a = 0.138196601125, b = 0.585410196625;
+    const double P[4][3] = {{a, a, a}, {b, a, a}, {a, b, a}, {a, a, b}};
+    for(int q = 0; q < 4; ++q) {
+      r[q] = P[q][0];
+      s[q] = P[q][1];
+      t[q] = P[q][2];
+      w[q] = 0.0416666666666667;
+    }
+    return 4;
+  }
+  return -1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Reference-element bases (feSpace_2D.cpp:118-134, :536-553, :727-741;
+ * feSpace_3D.cpp:104-242 for n = 1). */
+static int basis_eval(int kind, double r, double s, double t, double *L, double *dr,
+                      double *ds, double *dt)
+{
+  if(kind == OR_TRI_P1) {
+    L[0] = 1. - r - s; L[1] = r; L[2] = s;
+    dr[0] = -1.; dr[1] = 1.; dr[2] = 0.;
+    ds[0] = -1.; ds[1] = 0.; ds[2] = 1.;
+    return 3;
+  }
+  if(kind == OR_TRI_P2 || kind == OR_TRI_VP2) {
+    L[0] = (1. - r - s) * (1. - 2. * r - 2. * s);
+    L[1] = r * (2. * r - 1.);
+    L[2] = s * (2. * s - 1.);
+    L[3] = 4. * r * (1. - r - s);
+    L[4] = 4. * r * s;
+    L[5] = 4. * s * (1. - r - s);
+    dr[0] = 4. * (r + s) - 3.; dr[1] = 4. * r - 1.; dr[2] = 0.;
+    dr[3] = 4. * (1. - 2. * r - s); dr[4] = 4. * s; dr[5] = -4. * s;
+    ds[0] = 4. * (r + s) - 3.; ds[1] = 0.; ds[2] = 4. * s - 1.;
+    ds[3] = -4. * r; ds[4] = 4. * r; ds[5] = 4. * (1. - r - 2. * s);
+    return 6;
+  }
+  if(kind == OR_TET_P1) {
+    L[0] = 1. - r - s - t; L[1] = r; L[2] = s; L[3] = t;
+    dr[0] = -1.; dr[1] = 1.; dr[2] = 0.; dr[3] = 0.;
+    ds[0] = -1.; ds[1] = 0.; ds[2] = 1.; ds[3] = 0.;
+    dt[0] = -1.; dt[1] = 0.; dt[2] = 0.; dt[3] = 1.;
+    return 4;
+  }
+  return 0;
+}
+
+/* Geometry of a straight simplex: feCncGeo.cpp:318-340 (2D detJ), :380-396
+ * (3D det3x3), :651-692 (inverse Jacobian, constant for P1 geometry). */
+typedef struct {
+  double J;
+  double drdx[3], drdy[3], drdz[3];
+} or_geo;
+
+static int geo_tri(const double *c /* 3 vertices x 3 */, or_geo *g)
+{
+  double dxdr[2] = {-c[0] + c[3], -c[1] + c[4]};
+  double dxds[2] = {-c[0] + c[6], -c[1] + c[7]};
+  double J = dxdr[0] * dxds[1] - dxdr[1] * dxds[0];
+  g->J = J;
+  g->drdx[0] = dxds[1] / J;
+  g->drdx[1] = -dxdr[1] / J;
+  g->drdy[0] = -dxds[0] / J;
+  g->drdy[1] = dxdr[0] / J;
+  return J > 0 ? 0 : -1;
+}
+
+static int geo_tet(const double *c /* 4 vertices x 3 */, or_geo *g)
+{
+  double m[3][3];
+  for(int i = 0; i < 3; ++i) {
+    m[i][0] = -c[i] + c[3 + i];
+    m[i][1] = -c[i] + c[6 + i];
+    m[i][2] = -c[i] + c[9 + i];
+  }
+  double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+               m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+               m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+  g->J = det;
+  double ud = 1. / det, inv[3][3];
+  inv[0][0] = (m[1][1] * m[2][2] - m[1][2] * m[2][1]) * ud;
+  inv[1][0] = -(m[1][0] * m[2][2] - m[1][2] * m[2][0]) * ud;
+  inv[2][0] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) * ud;
+  inv[0][1] = -(m[0][1] * m[2][2] - m[0][2] * m[2][1]) * ud;
+  inv[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * ud;
+  inv[2][1] = -(m[0][0] * m[2][1] - m[0][1] * m[2][0]) * ud;
+  inv[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * ud;
+  inv[1][2] = -(m[0][0] * m[1][2] - m[0][2] * m[1][0]) * ud;
+  inv[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * ud;
+  for(int i = 0; i < 3; ++i) {
+    g->drdx[i] = inv[i][0];
+    g->drdy[i] = inv[i][1];
+    g->drdz[i] = inv[i][2];
+  }
+  return det > 0 ? 0 : -1;
+}
+
+/* Coefficient callbacks evaluated at physical quadrature points
+ * (feSysElm.cpp:540-542: pos interpolated from the geometry). */
+static double kov_lambda(double Re) { return Re / 2. - sqrt(Re * Re / 4. + 4. * M_PI * M_PI); }
+
+static double fn_eval(or_fn f, const double *x, int e, int nq, int q)
+{
+  switch(f.kind) {
+    case OR_FN_CONST: return f.c0;
+    case OR_FN_SINSIN: return f.c0 * sin(M_PI * x[0]) * sin(M_PI * x[1]);
+    case OR_FN_BLOCKS: {
+      int B = (int)f.c0;
+      int bi = (int)floor(x[0] * B), bj = (int)floor(x[1] * B);
+      if(bi >= B) bi = B - 1;
+      if(bj >= B) bj = B - 1;
+      return exp(0.5 * (double)(((7 * bi + 13 * bj) % 11) - 5));
+    }
+    case OR_FN_SMOOTH3:
+      return exp(f.c0 * sin(2. * M_PI * x[0]) * sin(2. * M_PI * x[1]) * sin(2. * M_PI * x[2]));
+    case OR_FN_ELEM: return f.data[e];
+    case OR_FN_QP: return f.data[(size_t)e * nq + q];
+    case OR_FN_KOV_U: {
+      double l = kov_lambda(f.c0);
+      if(f.c1 == 0.) return 1. - exp(l * x[0]) * cos(2. * M_PI * x[1]);
+      return l / (2. * M_PI) * exp(l * x[0]) * sin(2. * M_PI * x[1]);
+    }
+    case OR_FN_KOV_P: {
+      double l = kov_lambda(f.c0);
+      return 0.5 * (1. - exp(2. * l * x[0]));
+    }
+  }
+  return 0.;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Scalar diffusion + source on one element: feSysElm.cpp:530-555 (Ae, loop
+ * order q, i, j, d), :557-583 (Be from grad u_h), :16-27 (source). */
+typedef struct {
+  int nf, nq, dim, nv;
+  double qr[16], qs[16], qt[16], qw[16];
+  double L[16][12], dLr[16][12], dLs[16][12], dLt[16][12];
+  double Lg[16][4]; /* P1 geometry basis at the quadrature points */
+} or_ref;
+
+static int ref_setup(int kind, int quad_degree, or_ref *R)
+{
+  R->dim = (kind == OR_TET_P1) ? 3 : 2;
+  R->nv = R->dim + 1;
+  if(R->dim == 2)
+    R->nq = or_quad_tri(quad_degree, R->qr, R->qs, R->qw);
+  else
+    R->nq = or_quad_tet(quad_degree, R->qr, R->qs, R->qt, R->qw);
+  if(R->nq <= 0) return -1;
+  for(int q = 0; q < R->nq; ++q) {
+    double t = R->dim == 3 ? R->qt[q] : 0.;
+    R->nf = basis_eval(kind == OR_TRI_VP2 ? OR_TRI_P2 : kind, R->qr[q], R->qs[q], t, R->L[q],
+                       R->dLr[q], R->dLs[q], R->dLt[q]);
+    double d1[4], d2[4], d3[4];
+    basis_eval(R->dim == 3 ? OR_TET_P1 : OR_TRI_P1, R->qr[q], R->qs[q], t, R->Lg[q], d1, d2, d3);
+  }
+  return 0;
+}
+
+static int elem_geo(const or_ref *R, const double *c, or_geo *g)
+{
+  return R->dim == 2 ? geo_tri(c, g) : geo_tet(c, g);
+}
+
+static void qp_pos(const or_ref *R, const double *c, int q, double *x)
+{
+  x[0] = x[1] = x[2] = 0.;
+  for(int j = 0; j < R->nv; ++j)
+    for(int d = 0; d < 3; ++d) x[d] += c[3 * j + d] * R->Lg[q][j];
+}
+
+static void phys_grad(const or_ref *R, const or_geo *g, int q, double *gp /* nf*dim */)
+{
+  for(int i = 0; i < R->nf; ++i) {
+    if(R->dim == 2) {
+      gp[i * 2 + 0] = R->dLr[q][i] * g->drdx[0] + R->dLs[q][i] * g->drdx[1];
+      gp[i * 2 + 1] = R->dLr[q][i] * g->drdy[0] + R->dLs[q][i] * g->drdy[1];
+    } else {
+      gp[i * 3 + 0] = R->dLr[q][i] * g->drdx[0] + R->dLs[q][i] * g->drdx[1] + R->dLt[q][i] * g->drdx[2];
+      gp[i * 3 + 1] = R->dLr[q][i] * g->drdy[0] + R->dLs[q][i] * g->drdy[1] + R->dLt[q][i] * g->drdy[2];
+      gp[i * 3 + 2] = R->dLr[q][i] * g->drdz[0] + R->dLs[q][i] * g->drdz[1] + R->dLt[q][i] * g->drdz[2];
+    }
+  }
+}
+
+static void diffusion_elem(const or_ref *R, const double *c, int e, or_fn coeff, or_fn source,
+                           const double *ul, double *Ae, double *Be)
+{
+  or_geo g;
+  elem_geo(R, c, &g);
+  const int nf = R->nf, dim = R->dim;
+  double gp[36];
+  for(int k = 0; k < R->nq; ++k) {
+    double x[3];
+    qp_pos(R, c, k, x);
+    double kc = fn_eval(coeff, x, e, R->nq, k);
+    phys_grad(R, &g, k, gp);
+    if(Ae)
+      for(int i = 0; i < nf; ++i)
+        for(int j = 0; j < nf; ++j)
+          for(int d = 0; d < dim; ++d)
+            Ae[i * nf + j] += kc * gp[i * dim + d] * gp[j * dim + d] * g.J * R->qw[k];
+    if(Be) {
+      double gu[3] = {0., 0., 0.};
+      for(int i = 0; i < nf; ++i)
+        for(int d = 0; d < dim; ++d) gu[d] += ul[i] * gp[i * dim + d];
+      for(int i = 0; i < nf; ++i)
+        for(int d = 0; d < dim; ++d) Be[i] -= kc * gp[i * dim + d] * gu[d] * g.J * R->qw[k];
+    }
+  }
+  if(Be) {
+    for(int k = 0; k < R->nq; ++k) {
+      double x[3];
+      qp_pos(R, c, k, x);
+      double S = fn_eval(source, x, e, R->nq, k);
+      for(int i = 0; i < nf; ++i) Be[i] -= R->L[k][i] * S * g.J * R->qw[k];
+    }
+  }
+}
+
+int or_element_diffusion(int kind, const double *coords, int quad_degree, or_fn coeff,
+                         int elem, double *Ae)
+{
+  or_ref R;
+  if(ref_setup(kind, quad_degree, &R)) return -1;
+  memset(Ae, 0, sizeof(double) * R.nf * R.nf);
+  or_fn zero = {OR_FN_CONST, 0., 0., NULL};
+  diffusion_elem(&R, coords, elem, coeff, zero, NULL, Ae, NULL);
+  return R.nf;
+}
+
+static int find_pos(const int *ia, const int *ja, int row, int col)
+{
+  for(int k = ia[row]; k < ia[row + 1]; ++k)
+    if(ja[k] == col) return k;
+  return -1;
+}
+
+/* Assembly: the diffusion form's matrix + residual, then the source form's
+ * residual, each scattered with the Pardiso semantics
+ * (feLinearSystemMklPardiso.cpp:561-660 matrix, :729-738 RHS). */
+int or_assemble_diffusion(int kind, int V, const double *xyz, int E, const int *conn,
+                          const int *adr, int n, const int *ia, const int *ja,
+                          int quad_degree, or_fn coeff, or_fn source, const double *u,
+                          const int *elem_order, double *val, double *rhs)
+{
+  (void)V;
+  or_ref R;
+  if(ref_setup(kind, quad_degree, &R)) return -1;
+  const int nf = R.nf, nv = R.nv;
+  double c[12], Ae[144], Be[12], ul[12];
+  or_fn zero = {OR_FN_CONST, 0., 0., NULL};
+  for(int pass = 0; pass < 2; ++pass) {
+    for(int ee = 0; ee < E; ++ee) {
+      int e = elem_order ? elem_order[ee] : ee;
+      for(int j = 0; j < nv; ++j)
+        for(int d = 0; d < 3; ++d) c[3 * j + d] = xyz[3 * conn[e * nv + j] + d];
+      const int *a = adr + (size_t)e * nf;
+      for(int i = 0; i < nf; ++i) ul[i] = u[a[i]];
+      memset(Ae, 0, sizeof(Ae));
+      memset(Be, 0, sizeof(Be));
+      or_geo g;
+      if(elem_geo(&R, c, &g)) return -2 - e;
+      if(pass == 0)
+        diffusion_elem(&R, c, e, coeff, zero, ul, Ae, Be);
+      else {
+        or_fn nocoef = {OR_FN_CONST, 0., 0., NULL};
+        diffusion_elem(&R, c, e, nocoef, source, ul, NULL, Be);
+      }
+      for(int i = 0; i < nf; ++i) {
+        if(a[i] >= n) continue;
+        rhs[a[i]] += Be[i];
+        if(pass == 0)
+          for(int j = 0; j < nf; ++j) {
+            if(a[j] >= n) continue;
+            int p = find_pos(ia, ja, a[i], a[j]);
+            if(p < 0) return -1;
+            val[p] += Ae[i * nf + j];
+          }
+      }
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Taylor-Hood Navier-Stokes: filled in by or_ns.c-style helpers below. */
+
+/* ------------------------------------------------------------------------ */
+/* Krylov solvers. */
+int or_spmv(int n, const int *ia, const int *ja, const double *val, const double *x, double *y)
+{
+  for(int i = 0; i < n; ++i) {
+    double s = 0.;
+    for(int k = ia[i]; k < ia[i + 1]; ++k) s += val[k] * x[ja[k]];
+    y[i] = s;
+  }
+  return 0;
+}
+
+static double dot(int n, const double *a, const double *b)
+{
+  double s = 0.;
+  for(int i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+/* Preconditioned CG (Hestenes-Stiefel) from x0 = 0 with a Jacobi or IC(0)
+ * preconditioner, stopped on the true-residual-equivalent recursive residual
+ * ||r||_2 <= rtol ||b||_2. */
+static void ic0_apply(int n, const int *ia, const int *ja, const double *lval, const double *r,
+                      double *z);
+
+static int pcg(int n, const int *ia, const int *ja, const double *val, const double *lval,
+               const double *b, double *x, double rtol, int maxit, double *relres)
+{
+  double *r = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n);
+  double *p = malloc(sizeof(double) * n), *q = malloc(sizeof(double) * n);
+  double *dinv = malloc(sizeof(double) * n);
+  for(int i = 0; i < n; ++i) {
+    double d = 0.;
+    for(int k = ia[i]; k < ia[i + 1]; ++k)
+      if(ja[k] == i) d = val[k];
+    dinv[i] = d != 0. ? 1. / d : 1.;
+    x[i] = 0.;
+    r[i] = b[i];
+  }
+  double bn = sqrt(dot(n, b, b));
+  if(bn == 0.) bn = 1.;
+  if(lval) ic0_apply(n, ia, ja, lval, r, z);
+  else for(int i = 0; i < n; ++i) z[i] = dinv[i] * r[i];
+  memcpy(p, z, sizeof(double) * n);
+  double rz = dot(n, r, z);
+  int it = 0;
+  double rn = sqrt(dot(n, r, r));
+  while(rn > rtol * bn && it < maxit) {
+    or_spmv(n, ia, ja, val, p, q);
+    double alpha = rz / dot(n, p, q);
+    for(int i = 0; i < n; ++i) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * q[i];
+    }
+    if(lval) ic0_apply(n, ia, ja, lval, r, z);
+    else for(int i = 0; i < n; ++i) z[i] = dinv[i] * r[i];
+    double rz1 = dot(n, r, z);
+    double beta = rz1 / rz;
+    rz = rz1;
+    for(int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    rn = sqrt(dot(n, r, r));
+    it++;
+  }
+  /* true residual */
+  or_spmv(n, ia, ja, val, x, q);
+  double tr = 0.;
+  for(int i = 0; i < n; ++i) tr += (b[i] - q[i]) * (b[i] - q[i]);
+  if(relres) *relres = sqrt(tr) / bn;
+  free(r); free(z); free(p); free(q); free(dinv);
+  return it;
+}
+
+int or_cg_jacobi(int n, const int *ia, const int *ja, const double *val, const double *b,
+                 double *x, double rtol, int maxit, double *relres)
+{
+  return pcg(n, ia, ja, val, NULL, b, x, rtol, maxit, relres);
+}
+
+/* IC(0): incomplete Cholesky with the pattern of the lower triangle of A
+ * (the IC(0) pattern).  lval holds L on A's pattern (upper entries zero). */
+int or_ic0(int n, const int *ia, const int *ja, const double *val, double *lval)
+{
+  int64_t nnz = ia[n];
+  for(int64_t k = 0; k < nnz; ++k) lval[k] = 0.;
+  double *diag = malloc(sizeof(double) * n);
+  for(int i = 0; i < n; ++i) {
+    /* row i of L: for j < i in pattern, L_ij = (a_ij - sum_{k<j} L_ik L_jk) / L_jj */
+    for(int p = ia[i]; p < ia[i + 1]; ++p) {
+      int j = ja[p];
+      if(j > i) break;
+      double s = val[p];
+      /* sparse dot of rows i and j of L over columns < j */
+      int pi = ia[i], pj = ia[j];
+      while(pi < ia[i + 1] && pj < ia[j + 1]) {
+        int ci = ja[pi], cj = ja[pj];
+        if(ci >= j || cj >= j) break;
+        if(ci == cj) { s -= lval[pi] * lval[pj]; pi++; pj++; }
+        else if(ci < cj) pi++;
+        else pj++;
+      }
+      if(j < i) {
+        lval[p] = s / diag[j];
+      } else {
+        if(s <= 0.) { free(diag); return -1 - i; }
+        diag[i] = sqrt(s);
+        lval[p] = diag[i];
+      }
+    }
+  }
+  free(diag);
+  return 0;
+}
+
+static void ic0_apply(int n, const int *ia, const int *ja, const double *lval, const double *r,
+                      double *z)
+{
+  /* L y = r */
+  for(int i = 0; i < n; ++i) {
+    double s = r[i], d = 1.;
+    for(int p = ia[i]; p < ia[i + 1]; ++p) {
+      int j = ja[p];
+      if(j < i) s -= lval[p] * z[j];
+      else if(j == i) d = lval[p];
+    }
+    z[i] = s / d;
+  }
+  /* L^T z = y (column sweep from the bottom) */
+  for(int i = n - 1; i >= 0; --i) {
+    double d = 1.;
+    for(int p = ia[i]; p < ia[i + 1]; ++p)
+      if(ja[p] == i) d = lval[p];
+    z[i] /= d;
+    for(int p = ia[i]; p < ia[i + 1]; ++p) {
+      int j = ja[p];
+      if(j < i) z[j] -= lval[p] * z[i];
+    }
+  }
+}
+
+int or_pcg_ic0(int n, const int *ia, const int *ja, const double *val, const double *lval,
+               const double *b, double *x, double rtol, int maxit, double *relres)
+{
+  return pcg(n, ia, ja, val, lval, b, x, rtol, maxit, relres);
+}
+
+/* ILU(0) on A's pattern (IKJ variant); unit-lower L and U share luval. */
+int or_ilu0(int n, const int *ia, const int *ja, const double *val, double *lu)
+{
+  int64_t nnz = ia[n];
+  memcpy(lu, val, sizeof(double) * nnz);
+  int *diag = malloc(sizeof(int) * n), *iw = malloc(sizeof(int) * n);
+  for(int i = 0; i < n; ++i) {
+    diag[i] = -1;
+    iw[i] = -1;
+    for(int p = ia[i]; p < ia[i + 1]; ++p)
+      if(ja[p] == i) diag[i] = p;
+  }
+  for(int i = 0; i < n; ++i) {
+    for(int p = ia[i]; p < ia[i + 1]; ++p) iw[ja[p]] = p;
+    for(int p = ia[i]; p < ia[i + 1]; ++p) {
+      int k = ja[p];
+      if(k >= i) break;
+      double piv = lu[diag[k]];
+      if(piv == 0.) { free(diag); free(iw); return -1 - k; }
+      lu[p] /= piv;
+      double l = lu[p];
+      for(int q = diag[k] + 1; q < ia[k + 1]; ++q) {
+        int w = iw[ja[q]];
+        if(w >= 0) lu[w] -= l * lu[q];
+      }
+    }
+    for(int p = ia[i]; p < ia[i + 1]; ++p) iw[ja[p]] = -1;
+    if(diag[i] < 0 || lu[diag[i]] == 0.) { free(diag); free(iw); return -1 - i; }
+  }
+  free(diag);
+  free(iw);
+  return 0;
+}
+
+static void ilu0_apply(int n, const int *ia, const int *ja, const double *lu, const double *r,
+                       double *z)
+{
+  for(int i = 0; i < n; ++i) {
+    double s = r[i];
+    for(int p = ia[i]; p < ia[i + 1] && ja[p] < i; ++p) s -= lu[p] * z[ja[p]];
+    z[i] = s;
+  }
+  for(int i = n - 1; i >= 0; --i) {
+    double s = z[i], d = 1.;
+    for(int p = ia[i]; p < ia[i + 1]; ++p) {
+      if(ja[p] > i) s -= lu[p] * z[ja[p]];
+      else if(ja[p] == i) d = lu[p];
+    }
+    z[i] = s / d;
+  }
+}
+
+/* Right-preconditioned restarted GMRES(m) with modified Gram-Schmidt and
+ * Givens rotations, x0 = 0, stopped on ||b - A x||_2 <= rtol ||b||_2. */
+int or_gmres_ilu0(int n, const int *ia, const int *ja, const double *val, const double *lu,
+                  const double *b, double *x, int m, double rtol, int maxit, double *relres)
+{
+  double *Vb = malloc(sizeof(double) * (size_t)n * (m + 1));
+  double *Z = malloc(sizeof(double) * (size_t)n * m);
+  double *H = calloc((size_t)(m + 1) * m, sizeof(double));
+  double *cs = malloc(sizeof(double) * m), *sn = malloc(sizeof(double) * m);
+  double *gg = malloc(sizeof(double) * (m + 1)), *w = malloc(sizeof(double) * n);
+  double *yv = malloc(sizeof(double) * m);
+  double bn = sqrt(dot(n, b, b));
+  if(bn == 0.) bn = 1.;
+  for(int i = 0; i < n; ++i) x[i] = 0.;
+  int it = 0;
+  double res = 0.;
+  for(;;) {
+    or_spmv(n, ia, ja, val, x, w);
+    for(int i = 0; i < n; ++i) Vb[i] = b[i] - w[i];
+    double beta = sqrt(dot(n, Vb, Vb));
+    res = beta / bn;
+    if(res <= rtol || it >= maxit) break;
+    for(int i = 0; i < n; ++i) Vb[i] /= beta;
+    for(int k = 0; k <= m; ++k) gg[k] = 0.;
+    gg[0] = beta;
+    int j;
+    for(j = 0; j < m && it < maxit; ++j, ++it) {
+      ilu0_apply(n, ia, ja, lu, Vb + (size_t)j * n, Z + (size_t)j * n);
+      or_spmv(n, ia, ja, val, Z + (size_t)j * n, w);
+      for(int k = 0; k <= j; ++k) {
+        double h = dot(n, w, Vb + (size_t)k * n);
+        H[k * m + j] = h;
+        for(int i = 0; i < n; ++i) w[i] -= h * Vb[(size_t)k * n + i];
+      }
+      double hn = sqrt(dot(n, w, w));
+      H[(j + 1) * m + j] = hn;
+      for(int i = 0; i < n; ++i) Vb[(size_t)(j + 1) * n + i] = hn > 0. ? w[i] / hn : 0.;
+      for(int k = 0; k < j; ++k) {
+        double t = cs[k] * H[k * m + j] + sn[k] * H[(k + 1) * m + j];
+        H[(k + 1) * m + j] = -sn[k] * H[k * m + j] + cs[k] * H[(k + 1) * m + j];
+        H[k * m + j] = t;
+      }
+      double a = H[j * m + j], bb = H[(j + 1) * m + j];
+      double r = sqrt(a * a + bb * bb);
+      cs[j] = r > 0. ? a / r : 1.;
+      sn[j] = r > 0. ? bb / r : 0.;
+      H[j * m + j] = r;
+      H[(j + 1) * m + j] = 0.;
+      gg[j + 1] = -sn[j] * gg[j];
+      gg[j] = cs[j] * gg[j];
+      if(fabs(gg[j + 1]) / bn <= rtol) { ++j; ++it; break; }
+    }
+    for(int k = j - 1; k >= 0; --k) {
+      double s = gg[k];
+      for(int l = k + 1; l < j; ++l) s -= H[k * m + l] * yv[l];
+      yv[k] = s / H[k * m + k];
+    }
+    for(int k = 0; k < j; ++k)
+      for(int i = 0; i < n; ++i) x[i] += yv[k] * Z[(size_t)k * n + i];
+  }
+  if(relres) *relres = res;
+  free(Vb); free(Z); free(H); free(cs); free(sn); free(gg); free(w); free(yv);
+  return it;
+}
+
+/* ------------------------------------------------------------------------ */
+/* L2 error vs u = sin(pi x) sin(pi y) (feNorm.cpp:323-398: sum over elements
+ * and quadrature points of (u_h - u)^2 J w, square root at the end). */
+double or_l2_error_sinsin(int kind, const double *xyz, int E, const int *conn, const int *adr,
+                          const double *u, int quad_degree)
+{
+  or_ref R;
+  if(ref_setup(kind, quad_degree, &R)) return -1.;
+  double acc = 0., c[9];
+  for(int e = 0; e < E; ++e) {
+    for(int j = 0; j < 3; ++j)
+      for(int d = 0; d < 3; ++d) c[3 * j + d] = xyz[3 * conn[e * 3 + j] + d];
+    or_geo g;
+    geo_tri(c, &g);
+    for(int k = 0; k < R.nq; ++k) {
+      double x[3];
+      qp_pos(&R, c, k, x);
+      double uh = 0.;
+      for(int i = 0; i < R.nf; ++i) uh += u[adr[(size_t)e * R.nf + i]] * R.L[k][i];
+      double ue = sin(M_PI * x[0]) * sin(M_PI * x[1]);
+      acc += (uh - ue) * (uh - ue) * g.J * R.qw[k];
+    }
+  }
+  return sqrt(acc);
+}
+
+/* Taylor-Hood NS forms: implemented in fe_oracle_ns.c. */
