@@ -1,0 +1,381 @@
+// Element kernels and the colored, atomic-free scatter (the device
+// counterpart of feLinearSystemPETSc::assembleMatrices/assembleResiduals,
+// feLinearSystemPETSc.cpp:543-805, with the Pardiso CSR scatter semantics of
+// feLinearSystemMklPardiso.cpp:501-749).
+//
+// Layout: elements are processed in color-sorted order; one launch per
+// color, one thread per element.  Within a color no two elements share a
+// vertex, hence no two threads add into the same CSR value or RHS entry, so
+// the scatter uses plain read-modify-write (deterministic, no atomics).  The
+// reference-element tables are staged in LDS; geometry is recomputed from the
+// vertex coordinates in registers.
+#include "fe_hip_internal.h"
+
+#include <cmath>
+
+namespace {
+
+__device__ __forceinline__ double feKovLambda(double Re)
+{
+  return Re / 2. - sqrt(Re * Re / 4. + 4. * M_PI * M_PI);
+}
+
+// The feFunction callbacks of the configs (feFunction.h:75-110), evaluated at
+// a physical quadrature point.
+__device__ __forceinline__ double feEvalFn(const FeFn &f, const double *x, int e, int nq, int q)
+{
+  switch(f.kind) {
+    case FE_FN_CONST: return f.c0;
+    case FE_FN_SINSIN: return f.c0 * sin(M_PI * x[0]) * sin(M_PI * x[1]);
+    case FE_FN_BLOCKS: {
+      const int B = (int)f.c0;
+      int bi = (int)floor(x[0] * B), bj = (int)floor(x[1] * B);
+      bi = bi >= B ? B - 1 : bi;
+      bj = bj >= B ? B - 1 : bj;
+      return exp(0.5 * (double)(((7 * bi + 13 * bj) % 11) - 5));
+    }
+    case FE_FN_SMOOTH3:
+      return exp(f.c0 * sin(2. * M_PI * x[0]) * sin(2. * M_PI * x[1]) * sin(2. * M_PI * x[2]));
+    case FE_FN_ELEM: return f.data[e];
+    case FE_FN_QP: return f.data[(size_t)e * nq + q];
+    case FE_FN_KOV_U: {
+      const double l = feKovLambda(f.c0);
+      if(f.c1 == 0.) return 1. - exp(l * x[0]) * cos(2. * M_PI * x[1]);
+      return l / (2. * M_PI) * exp(l * x[0]) * sin(2. * M_PI * x[1]);
+    }
+    case FE_FN_KOV_P: {
+      const double l = feKovLambda(f.c0);
+      return 0.5 * (1. - exp(2. * l * x[0]));
+    }
+  }
+  return 0.;
+}
+
+struct ScalarAsmArgs {
+  int begin, count, E, n, what;
+  const int *perm, *conn, *adr, *pos;
+  const double *xyz, *tab, *u;
+  FeFn coeff, src;
+  int hasDiff, hasSrc;
+  double *val, *rhs;
+};
+
+// Straight-simplex geometry: detJ (feCncGeo.cpp:318-340 / :380-396) and the
+// constant inverse Jacobian (feCncGeo.cpp:651-692).
+template <int DIM>
+__device__ __forceinline__ void feGeometry(const double (&c)[DIM + 1][3], double &J,
+                                           double (&drdx)[3], double (&drdy)[3], double (&drdz)[3])
+{
+  if(DIM == 2) {
+    const double dxdr0 = -c[0][0] + c[1][0], dxdr1 = -c[0][1] + c[1][1];
+    const double dxds0 = -c[0][0] + c[2][0], dxds1 = -c[0][1] + c[2][1];
+    J = dxdr0 * dxds1 - dxdr1 * dxds0;
+    drdx[0] = dxds1 / J;
+    drdx[1] = -dxdr1 / J;
+    drdy[0] = -dxds0 / J;
+    drdy[1] = dxdr0 / J;
+    drdx[2] = drdy[2] = drdz[0] = drdz[1] = drdz[2] = 0.;
+  } else {
+    double m[3][3];
+#pragma unroll
+    for(int i = 0; i < 3; ++i) {
+      m[i][0] = -c[0][i] + c[1][i];
+      m[i][1] = -c[0][i] + c[2][i];
+      m[i][2] = -c[0][i] + c[3][i];
+    }
+    J = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+        m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+    const double ud = 1. / J;
+    drdx[0] = (m[1][1] * m[2][2] - m[1][2] * m[2][1]) * ud;
+    drdx[1] = -(m[1][0] * m[2][2] - m[1][2] * m[2][0]) * ud;
+    drdx[2] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) * ud;
+    drdy[0] = -(m[0][1] * m[2][2] - m[0][2] * m[2][1]) * ud;
+    drdy[1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * ud;
+    drdy[2] = -(m[0][0] * m[2][1] - m[0][1] * m[2][0]) * ud;
+    drdz[0] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * ud;
+    drdz[1] = -(m[0][0] * m[1][2] - m[0][2] * m[1][0]) * ud;
+    drdz[2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * ud;
+  }
+}
+
+// Scalar diffusion (feSysElm.cpp:530-583) fused with a scalar source
+// (feSysElm.cpp:16-27) on one element per thread.
+template <int NF, int DIM>
+__global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
+{
+  __shared__ FeRefTables T;
+  {
+    const double *src = a.tab;
+    double *dst = (double *)&T;
+    for(int i = threadIdx.x; i < (int)(sizeof(FeRefTables) / sizeof(double)); i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if(t >= a.count) return;
+  const int ee = a.begin + t;
+  const int e = a.perm[ee];
+  constexpr int NV = DIM + 1;
+  double c[NV][3];
+#pragma unroll
+  for(int j = 0; j < NV; ++j) {
+    const int v = a.conn[(size_t)e * NV + j];
+#pragma unroll
+    for(int d = 0; d < 3; ++d) c[j][d] = a.xyz[3 * (size_t)v + d];
+  }
+  double J, drdx[3], drdy[3], drdz[3];
+  feGeometry<DIM>(c, J, drdx, drdy, drdz);
+  int ad[NF];
+  double ul[NF];
+#pragma unroll
+  for(int i = 0; i < NF; ++i) {
+    ad[i] = a.adr[(size_t)ee * NF + i];
+    ul[i] = a.u[ad[i]];
+  }
+  double Ae[NF][NF], Be[NF];
+#pragma unroll
+  for(int i = 0; i < NF; ++i) {
+    Be[i] = 0.;
+#pragma unroll
+    for(int j = 0; j < NF; ++j) Ae[i][j] = 0.;
+  }
+  const bool doMat = (a.what & 1) && a.hasDiff, doRes = (a.what & 2);
+  const int nq = T.nq;
+  if(a.hasDiff) {
+    for(int k = 0; k < nq; ++k) {
+      double x[3] = {0., 0., 0.};
+#pragma unroll
+      for(int j = 0; j < NV; ++j)
+#pragma unroll
+        for(int d = 0; d < 3; ++d) x[d] += c[j][d] * T.Lg[k][j];
+      const double kc = feEvalFn(a.coeff, x, e, nq, k);
+      double gp[NF][DIM];
+#pragma unroll
+      for(int i = 0; i < NF; ++i) {
+        if(DIM == 2) {
+          gp[i][0] = T.dLr[k][i] * drdx[0] + T.dLs[k][i] * drdx[1];
+          gp[i][1 % DIM] = T.dLr[k][i] * drdy[0] + T.dLs[k][i] * drdy[1];
+        } else {
+          gp[i][0] = T.dLr[k][i] * drdx[0] + T.dLs[k][i] * drdx[1] + T.dLt[k][i] * drdx[2];
+          gp[i][1 % DIM] = T.dLr[k][i] * drdy[0] + T.dLs[k][i] * drdy[1] + T.dLt[k][i] * drdy[2];
+          gp[i][2 % DIM] = T.dLr[k][i] * drdz[0] + T.dLs[k][i] * drdz[1] + T.dLt[k][i] * drdz[2];
+        }
+      }
+      const double jw = J * T.w[k];
+      if(doMat) {
+#pragma unroll
+        for(int i = 0; i < NF; ++i)
+#pragma unroll
+          for(int j = 0; j < NF; ++j)
+#pragma unroll
+            for(int d = 0; d < DIM; ++d) Ae[i][j] += kc * gp[i][d] * gp[j][d] * jw;
+      }
+      if(doRes) {
+        double gu[DIM];
+#pragma unroll
+        for(int d = 0; d < DIM; ++d) gu[d] = 0.;
+#pragma unroll
+        for(int i = 0; i < NF; ++i)
+#pragma unroll
+          for(int d = 0; d < DIM; ++d) gu[d] += ul[i] * gp[i][d];
+#pragma unroll
+        for(int i = 0; i < NF; ++i)
+#pragma unroll
+          for(int d = 0; d < DIM; ++d) Be[i] -= kc * gp[i][d] * gu[d] * jw;
+      }
+    }
+  }
+  if(doRes && a.hasSrc) {
+    for(int k = 0; k < nq; ++k) {
+      double x[3] = {0., 0., 0.};
+#pragma unroll
+      for(int j = 0; j < NV; ++j)
+#pragma unroll
+        for(int d = 0; d < 3; ++d) x[d] += c[j][d] * T.Lg[k][j];
+      const double S = feEvalFn(a.src, x, e, nq, k);
+      const double jw = J * T.w[k];
+#pragma unroll
+      for(int i = 0; i < NF; ++i) Be[i] -= T.L[k][i] * S * jw;
+    }
+  }
+  // Atomic-free scatter (exclusive ownership within the color).
+#pragma unroll
+  for(int i = 0; i < NF; ++i) {
+    if(doRes && ad[i] < a.n) a.rhs[ad[i]] += Be[i];
+    if(doMat) {
+#pragma unroll
+      for(int j = 0; j < NF; ++j) {
+        const int p = a.pos[(size_t)(i * NF + j) * a.E + ee];
+        if(p >= 0) a.val[p] += Ae[i][j];
+      }
+    }
+  }
+}
+
+__global__ void toFloat(int64_t n, const double *__restrict__ x, float *__restrict__ y)
+{
+  for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (float)x[i];
+}
+
+__global__ void maxAbs(int n, const double *__restrict__ x, double *partials, unsigned *counter,
+                       double *out)
+{
+  __shared__ double lds[16];
+  double m = 0.;
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) m = fmax(m, fabs(x[i]));
+  // max via wave shuffles
+#pragma unroll
+  for(int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if(lane == 0) lds[wid] = m;
+  __syncthreads();
+  __shared__ bool amLast;
+  if(threadIdx.x == 0) {
+    double b = 0.;
+    for(int w = 0; w < (int)(blockDim.x >> 6); ++w) b = fmax(b, lds[w]);
+    partials[blockIdx.x] = b;
+    __threadfence();
+    amLast = atomicAdd(counter, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if(amLast && threadIdx.x == 0) {
+    __threadfence();
+    double r = 0.;
+    for(int b = 0; b < (int)gridDim.x; ++b) r = fmax(r, partials[b]);
+    *out = r;
+    *counter = 0u;
+  }
+}
+
+__global__ void addCorrection(int n, double *__restrict__ u, const double *__restrict__ du)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) u[i] += du[i];
+}
+
+template <int NF, int DIM>
+void launchScalar(fe_hip_ctx *ctx, ScalarAsmArgs a)
+{
+  for(size_t c = 0; c + 1 < ctx->colorStart.size(); ++c) {
+    a.begin = ctx->colorStart[c];
+    a.count = ctx->colorStart[c + 1] - a.begin;
+    if(a.count == 0) continue;
+    hipLaunchKernelGGL((asmScalarFP64<NF, DIM>), dim3((a.count + 255) / 256), dim3(256), 0, ctx->stream, a);
+  }
+}
+
+}  // namespace
+
+extern "C" int fe_hip_set_to_zero(fe_hip_ctx *ctx, int matrix, int residual)
+{
+  if(!ctx) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, ctx->csrBuilt, "build the CSR first");
+  if(matrix) FE_HIP_TRY(ctx, hipMemsetAsync(ctx->d_val, 0, sizeof(double) * ctx->nnz, ctx->stream));
+  if(residual) FE_HIP_TRY(ctx, hipMemsetAsync(ctx->d_rhs, 0, sizeof(double) * ctx->n, ctx->stream));
+  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return FE_STATUS_OK;
+}
+
+extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
+{
+  if(!ctx) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, ctx->csrBuilt, "build the CSR first");
+  FE_REQUIRE(ctx, what >= 1 && what <= 3, "what must be 1 (matrix), 2 (residual) or 3 (both)");
+  FE_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipEvent_t t0, t1;
+  FE_HIP_TRY(ctx, hipEventCreate(&t0));
+  FE_HIP_TRY(ctx, hipEventCreate(&t1));
+  FE_HIP_TRY(ctx, hipEventRecord(t0, ctx->stream));
+  // Fuse every SOURCE form into the DIFFUSION form of the same field and
+  // quadrature rule (one element sweep computes Ae and Be of both).
+  std::vector<bool> done(ctx->forms.size(), false);
+  for(size_t fi = 0; fi < ctx->forms.size(); ++fi) {
+    if(done[fi]) continue;
+    FeForm &F = ctx->forms[fi];
+    if(F.sysType != FE_SYS_DIFFUSION && F.sysType != FE_SYS_SOURCE) continue;
+    ScalarAsmArgs a{};
+    a.E = ctx->E;
+    a.n = ctx->n;
+    a.what = what;
+    a.perm = ctx->d_perm;
+    a.conn = ctx->d_conn;
+    a.xyz = ctx->d_xyz;
+    a.u = ctx->d_u;
+    a.val = ctx->d_val;
+    a.rhs = ctx->d_rhs;
+    a.tab = F.d_tab;
+    const int field = F.fields[0];
+    if(F.sysType == FE_SYS_DIFFUSION) {
+      a.hasDiff = 1;
+      a.coeff = F.coeffs[0];
+      a.pos = ctx->groups[F.group].d_pos;
+      a.adr = ctx->groups[F.group].d_adrI;
+      for(size_t fj = 0; fj < ctx->forms.size(); ++fj) {
+        FeForm &S = ctx->forms[fj];
+        if(!done[fj] && fj != fi && S.sysType == FE_SYS_SOURCE && S.fields[0] == field && S.degree == F.degree &&
+           !a.hasSrc) {
+          a.hasSrc = 1;
+          a.src = S.coeffs[0];
+          done[fj] = true;
+        }
+      }
+    } else {
+      // A lone source form: its row map comes from a diffusion group on the
+      // same field when one exists.
+      int g = -1;
+      for(size_t k = 0; k < ctx->groups.size(); ++k)
+        if(ctx->groups[k].layoutI.size() == 1 && ctx->groups[k].layoutI[0] == field) g = (int)k;
+      FE_REQUIRE(ctx, g >= 0, "a SOURCE form needs a DIFFUSION form on the same field");
+      a.hasSrc = 1;
+      a.src = F.coeffs[0];
+      a.adr = ctx->groups[g].d_adrI;
+      a.pos = ctx->groups[g].d_pos;
+      if(!(what & 2)) { done[fi] = true; continue; }
+    }
+    done[fi] = true;
+    const int kind = ctx->fields[field].kind;
+    if(kind == FE_TRI_P1) launchScalar<3, 2>(ctx, a);
+    else if(kind == FE_TRI_P2) launchScalar<6, 2>(ctx, a);
+    else if(kind == FE_TET_P1) launchScalar<4, 3>(ctx, a);
+    else FE_REQUIRE(ctx, false, "scalar forms need a scalar field");
+  }
+  for(size_t fi = 0; fi < ctx->forms.size(); ++fi)
+    FE_REQUIRE(ctx, done[fi], "form type has no device kernel");
+  FE_HIP_TRY(ctx, hipGetLastError());
+  if(precision == FE_PREC_FP32_IR && (what & 1)) {
+    if(!ctx->d_val32 && feAlloc(ctx, &ctx->d_val32, (size_t)ctx->nnz)) return FE_STATUS_OUT_OF_MEMORY;
+    hipLaunchKernelGGL(toFloat, dim3(2048), dim3(256), 0, ctx->stream, ctx->nnz, ctx->d_val, ctx->d_val32);
+  }
+  FE_HIP_TRY(ctx, hipEventRecord(t1, ctx->stream));
+  FE_HIP_TRY(ctx, hipEventSynchronize(t1));
+  float ms = 0.f;
+  FE_HIP_TRY(ctx, hipEventElapsedTime(&ms, t0, t1));
+  ctx->lastAsmMs = ms;
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  return FE_STATUS_OK;
+}
+
+extern "C" int fe_hip_rhs_max_norm(fe_hip_ctx *ctx, double *norm)
+{
+  if(!ctx || !norm) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, ctx->csrBuilt, "build the CSR first");
+  double *out = ctx->d_partials + 3 * ctx->maxBlocks;
+  const int nb = std::min(1024, (ctx->n + 255) / 256);
+  hipLaunchKernelGGL(maxAbs, dim3(nb), dim3(256), 0, ctx->stream, ctx->n, ctx->d_rhs, ctx->d_partials,
+                     ctx->d_counter + 8, out);
+  FE_HIP_TRY(ctx, hipGetLastError());
+  FE_HIP_TRY(ctx, hipMemcpyAsync(norm, out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return FE_STATUS_OK;
+}
+
+extern "C" int fe_hip_correct_solution(fe_hip_ctx *ctx)
+{
+  if(!ctx) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, ctx->csrBuilt, "build the CSR first");
+  hipLaunchKernelGGL(addCorrection, dim3((ctx->n + 255) / 256), dim3(256), 0, ctx->stream, ctx->n, ctx->d_u, ctx->d_du);
+  FE_HIP_TRY(ctx, hipGetLastError());
+  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return FE_STATUS_OK;
+}

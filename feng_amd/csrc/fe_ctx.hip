@@ -1,0 +1,419 @@
+// Context lifetime, uploads, fields, forms and solution I/O of libfe_hip.so.
+#include "fe_hip_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+// ---------------------------------------------------------------------------
+// Quadrature rules (data of feQuadratureTri.cpp:788-920 and
+// feQuadratureTet.cpp:96-110) and reference bases (feSpace_2D.cpp:118-134,
+// :536-553, :727-741; feSpace_3D.cpp:104-242 with n = 1).
+namespace {
+
+struct QuadBuilder {
+  double r[16], s[16], t[16], w[16];
+  int n = 0;
+  void pt(double a, double b, double wt) { r[n] = a; s[n] = b; t[n] = 0.; w[n] = wt; ++n; }
+  void orbit3(double a, double wt) { double b = 1. - 2. * a; pt(a, a, wt); pt(a, b, wt); pt(b, a, wt); }
+  void orbit3r(double a, double b, double c, double wt) { pt(a, b, wt); pt(c, a, wt); pt(b, c, wt); }
+  void orbit6(double a, double b, double wt)
+  {
+    double c = 1. - a - b;
+    pt(a, b, wt); pt(b, a, wt); pt(a, c, wt); pt(c, a, wt); pt(b, c, wt); pt(c, b, wt);
+  }
+};
+
+bool triRule(int degree, QuadBuilder &q)
+{
+  switch(degree) {
+    case 0:
+    case 1: q.pt(1. / 3., 1. / 3., 0.5); return true;
+    case 2: q.orbit3(1. / 6., 1. / 6.); return true;
+    case 3: q.pt(1. / 3., 1. / 3., -0.28125); q.orbit3(0.2, 25. / 96.); return true;
+    case 4:
+      q.orbit3(0.091576213509770743460, 0.054975871827660933819);
+      q.orbit3(0.44594849091596488632, 0.11169079483900573285);
+      return true;
+    case 5:
+      q.pt(1. / 3., 1. / 3., 0.1125);
+      q.orbit3(0.10128650732345633880, 0.062969590272413576298);
+      q.orbit3(0.47014206410511508977, 0.066197076394253090369);
+      return true;
+    case 6:
+      q.orbit3(0.063089014491502228340, 0.025422453185103408460);
+      q.orbit3(0.24928674517091042129, 0.058393137863189683013);
+      q.orbit6(0.053145049844816947353, 0.31035245103378440542, 0.041425537809186787597);
+      return true;
+    case 7:
+      q.orbit3r(0.062382265094402118174, 0.067517867073916085443,
+                1. - 0.062382265094402118174 - 0.067517867073916085443, 0.026517028157436251429);
+      q.orbit3r(0.055225456656926611737, 0.32150249385198182267,
+                1. - 0.055225456656926611737 - 0.32150249385198182267, 0.043881408714446055037);
+      q.orbit3r(0.034324302945097146470, 0.66094919618673565761, 0.30472650086816719592,
+                0.028775042784981585738);
+      q.orbit3r(0.51584233435359177926, 0.27771616697639178257, 0.20644149867001643817,
+                0.067493187009802774463);
+      return true;
+    case 8:
+      q.pt(1. / 3., 1. / 3., 0.0721578038388935841255455552445323);
+      q.orbit3(0.170569307751760206622293501491464, 0.0516086852673591251408957751460645);
+      q.orbit3(0.0505472283170309754584235505965989, 0.0162292488115990401554629641708902);
+      q.orbit3(0.459292588292723156028815514494169, 0.0475458171336423123969480521942921);
+      q.orbit6(0.008394777409957605337213834539296, 0.263112829634638113421785786284643,
+               0.0136151570872174971324223450369544);
+      return true;
+  }
+  return false;
+}
+
+bool tetRule(int degree, QuadBuilder &q)
+{
+  if(degree <= 1) {
+    q.r[0] = q.s[0] = q.t[0] = 0.25;
+    q.w[0] = 0.166666666666667;
+    q.n = 1;
+    return true;
+  }
+  if(degree == 2) {
+    const double a = 0.138196601125, b = 0.585410196625;
+    const double P[4][3] = {{a, a, a}, {b, a, a}, {a, b, a}, {a, a, b}};
+    for(int i = 0; i < 4; ++i) {
+      q.r[i] = P[i][0]; q.s[i] = P[i][1]; q.t[i] = P[i][2]; q.w[i] = 0.0416666666666667;
+    }
+    q.n = 4;
+    return true;
+  }
+  return false;
+}
+
+}  // namespace
+
+int feBuildRefTables(int kind, int degree, FeRefTables *T)
+{
+  memset(T, 0, sizeof(*T));
+  QuadBuilder q;
+  const bool tet = (kind == FE_TET_P1);
+  if(!(tet ? tetRule(degree, q) : triRule(degree, q))) return -1;
+  T->nq = q.n;
+  T->dim = tet ? 3 : 2;
+  T->nv = tet ? 4 : 3;
+  T->nf = (kind == FE_TRI_P1) ? 3 : (tet ? 4 : 6);  // vector P2 uses the scalar P2 tables
+  for(int k = 0; k < q.n; ++k) {
+    const double r = q.r[k], s = q.s[k], t = q.t[k];
+    T->w[k] = q.w[k];
+    if(kind == FE_TRI_P1) {
+      const double L[3] = {1. - r - s, r, s}, dr[3] = {-1., 1., 0.}, ds[3] = {-1., 0., 1.};
+      for(int i = 0; i < 3; ++i) { T->L[k][i] = L[i]; T->dLr[k][i] = dr[i]; T->dLs[k][i] = ds[i]; }
+    } else if(tet) {
+      const double L[4] = {1. - r - s - t, r, s, t};
+      const double dr[4] = {-1., 1., 0., 0.}, ds[4] = {-1., 0., 1., 0.}, dt[4] = {-1., 0., 0., 1.};
+      for(int i = 0; i < 4; ++i) {
+        T->L[k][i] = L[i]; T->dLr[k][i] = dr[i]; T->dLs[k][i] = ds[i]; T->dLt[k][i] = dt[i];
+      }
+    } else {
+      const double L[6] = {(1. - r - s) * (1. - 2. * r - 2. * s), r * (2. * r - 1.), s * (2. * s - 1.),
+                           4. * r * (1. - r - s), 4. * r * s, 4. * s * (1. - r - s)};
+      const double dr[6] = {4. * (r + s) - 3., 4. * r - 1., 0., 4. * (1. - 2. * r - s), 4. * s, -4. * s};
+      const double ds[6] = {4. * (r + s) - 3., 0., 4. * s - 1., -4. * r, 4. * r, 4. * (1. - r - 2. * s)};
+      for(int i = 0; i < 6; ++i) { T->L[k][i] = L[i]; T->dLr[k][i] = dr[i]; T->dLs[k][i] = ds[i]; }
+    }
+    if(tet) {
+      T->Lg[k][0] = 1. - r - s - t; T->Lg[k][1] = r; T->Lg[k][2] = s; T->Lg[k][3] = t;
+    } else {
+      T->Lg[k][0] = 1. - r - s; T->Lg[k][1] = r; T->Lg[k][2] = s;
+    }
+  }
+  return q.n;
+}
+
+void feFree(fe_hip_ctx *ctx, void *p)
+{
+  if(!p) return;
+  auto it = std::find(ctx->owned.begin(), ctx->owned.end(), p);
+  if(it != ctx->owned.end()) ctx->owned.erase(it);
+  (void)hipFree(p);
+}
+
+static int nlocOf(int kind)
+{
+  switch(kind) {
+    case FE_TRI_P1: return 3;
+    case FE_TRI_P2: return 6;
+    case FE_TRI_VP2: return 12;
+    case FE_TET_P1: return 4;
+  }
+  return 0;
+}
+
+extern "C" {
+
+int fe_hip_ctx_create(int device, fe_hip_ctx **out)
+{
+  if(!out) return FE_STATUS_POINTER_ERROR;
+  *out = nullptr;
+  int count = 0;
+  if(hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0)
+    return FE_STATUS_ERROR;
+  fe_hip_ctx *ctx = new fe_hip_ctx;
+  ctx->device = device;
+  if(hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return FE_STATUS_ERROR;
+  }
+  ctx->maxBlocks = 65536;
+  if(feAlloc(ctx, &ctx->d_partials, (size_t)ctx->maxBlocks * 4) ||
+     feAlloc(ctx, &ctx->d_counter, 64)) {
+    for(void *p : ctx->owned) (void)hipFree(p);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return FE_STATUS_OUT_OF_MEMORY;
+  }
+  if(hipMemset(ctx->d_counter, 0, 64 * sizeof(unsigned)) != hipSuccess) {
+    for(void *p : ctx->owned) (void)hipFree(p);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return FE_STATUS_ERROR;
+  }
+  *out = ctx;
+  return FE_STATUS_OK;
+}
+
+int fe_hip_ctx_destroy(fe_hip_ctx *ctx)
+{
+  if(!ctx) return FE_STATUS_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for(hipEvent_t e : ctx->spmvTimer.ev) (void)hipEventDestroy(e);
+  for(void *p : ctx->owned) (void)hipFree(p);
+  if(ctx->h_state) (void)hipHostFree(ctx->h_state);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return FE_STATUS_OK;
+}
+
+const char *fe_hip_last_error(const fe_hip_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int fe_hip_upload_mesh(fe_hip_ctx *ctx, int dim, int V, const double *xyz, int E, int nvpe,
+                       const int *conn)
+{
+  if(!ctx || !xyz || !conn) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, (dim == 2 && nvpe == 3) || (dim == 3 && nvpe == 4), "unsupported element type");
+  FE_REQUIRE(ctx, V > 0 && E > 0, "empty mesh");
+  for(size_t i = 0; i < (size_t)E * nvpe; ++i)
+    FE_REQUIRE(ctx, conn[i] >= 0 && conn[i] < V, "connectivity out of range");
+  FE_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  ctx->dim = dim; ctx->V = V; ctx->E = E; ctx->nvpe = nvpe;
+  if(feAlloc(ctx, &ctx->d_xyz, (size_t)3 * V) || feAlloc(ctx, &ctx->d_conn, (size_t)E * nvpe))
+    return FE_STATUS_OUT_OF_MEMORY;
+  FE_HIP_TRY(ctx, hipMemcpy(ctx->d_xyz, xyz, sizeof(double) * 3 * V, hipMemcpyHostToDevice));
+  FE_HIP_TRY(ctx, hipMemcpy(ctx->d_conn, conn, sizeof(int) * (size_t)E * nvpe, hipMemcpyHostToDevice));
+  // Default coloring (first-fit over vertex adjacency); replaced by fe_hip_set_coloring.
+  std::vector<int> colorOf(E);
+  std::vector<uint64_t> used((size_t)V * 4, 0);
+  for(int e = 0; e < E; ++e) {
+    uint64_t forb[4] = {0, 0, 0, 0};
+    for(int j = 0; j < nvpe; ++j)
+      for(int w = 0; w < 4; ++w) forb[w] |= used[(size_t)conn[(size_t)e * nvpe + j] * 4 + w];
+    int c = -1;
+    for(int w = 0; w < 4 && c < 0; ++w)
+      if(~forb[w]) c = 64 * w + __builtin_ctzll(~forb[w]);
+    FE_REQUIRE(ctx, c >= 0, "more than 256 colors");
+    colorOf[e] = c;
+    for(int j = 0; j < nvpe; ++j) used[(size_t)conn[(size_t)e * nvpe + j] * 4 + c / 64] |= 1ull << (c % 64);
+  }
+  return fe_hip_set_coloring(ctx, colorOf.data());
+}
+
+int fe_hip_set_coloring(fe_hip_ctx *ctx, const int *colorOf)
+{
+  if(!ctx || !colorOf) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, ctx->E > 0, "upload the mesh first");
+  FE_REQUIRE(ctx, !ctx->csrBuilt, "coloring must be set before fe_hip_build_csr");
+  int nc = 0;
+  for(int e = 0; e < ctx->E; ++e) {
+    FE_REQUIRE(ctx, colorOf[e] >= 0, "negative color");
+    nc = std::max(nc, colorOf[e] + 1);
+  }
+  ctx->colorStart.assign(nc + 1, 0);
+  for(int e = 0; e < ctx->E; ++e) ctx->colorStart[colorOf[e] + 1]++;
+  for(int c = 0; c < nc; ++c) ctx->colorStart[c + 1] += ctx->colorStart[c];
+  std::vector<int> pos(ctx->colorStart.begin(), ctx->colorStart.end() - 1);
+  ctx->h_perm.assign(ctx->E, 0);
+  for(int e = 0; e < ctx->E; ++e) ctx->h_perm[pos[colorOf[e]]++] = e;
+  if(!ctx->d_perm && feAlloc(ctx, &ctx->d_perm, ctx->E)) return FE_STATUS_OUT_OF_MEMORY;
+  FE_HIP_TRY(ctx, hipMemcpy(ctx->d_perm, ctx->h_perm.data(), sizeof(int) * ctx->E, hipMemcpyHostToDevice));
+  ctx->colored = true;
+  return FE_STATUS_OK;
+}
+
+int fe_hip_add_field(fe_hip_ctx *ctx, int kind, const int *adr, int *field)
+{
+  if(!ctx || !adr) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, ctx->E > 0, "upload the mesh first");
+  const int nl = nlocOf(kind);
+  FE_REQUIRE(ctx, nl > 0, "unknown field kind");
+  FE_REQUIRE(ctx, (ctx->dim == 3) == (kind == FE_TET_P1), "field kind does not match the mesh");
+  FeField f;
+  f.kind = kind;
+  f.nloc = nl;
+  if(feAlloc(ctx, &f.adr, (size_t)ctx->E * nl)) return FE_STATUS_OUT_OF_MEMORY;
+  FE_HIP_TRY(ctx, hipMemcpy(f.adr, adr, sizeof(int) * (size_t)ctx->E * nl, hipMemcpyHostToDevice));
+  ctx->fields.push_back(f);
+  if(field) *field = (int)ctx->fields.size() - 1;
+  return FE_STATUS_OK;
+}
+
+int fe_hip_set_sizes(fe_hip_ctx *ctx, int nInc, int nDofs)
+{
+  if(!ctx) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, nInc > 0 && nDofs >= nInc, "invalid system sizes (no unknowns)");
+  FE_REQUIRE(ctx, !ctx->csrBuilt, "sizes are fixed once the CSR is built");
+  ctx->n = nInc;
+  ctx->nDofs = nDofs;
+  if(feAlloc(ctx, &ctx->d_u, nDofs) || feAlloc(ctx, &ctx->d_rhs, nInc) || feAlloc(ctx, &ctx->d_du, nInc))
+    return FE_STATUS_OUT_OF_MEMORY;
+  FE_HIP_TRY(ctx, hipMemset(ctx->d_u, 0, sizeof(double) * nDofs));
+  FE_HIP_TRY(ctx, hipMemset(ctx->d_rhs, 0, sizeof(double) * nInc));
+  FE_HIP_TRY(ctx, hipMemset(ctx->d_du, 0, sizeof(double) * nInc));
+  return FE_STATUS_OK;
+}
+
+int fe_hip_add_form(fe_hip_ctx *ctx, int sysType, int nfields, const int *fields, int quad_degree,
+                    int ncoeff, const fe_hip_fn *coeffs, int *form)
+{
+  if(!ctx || !fields || (ncoeff > 0 && !coeffs)) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, !ctx->csrBuilt, "forms must be added before fe_hip_build_csr");
+  FE_REQUIRE(ctx, nfields >= 1 && nfields <= 2 && ncoeff >= 0 && ncoeff <= 4, "bad form arity");
+  for(int i = 0; i < nfields; ++i)
+    FE_REQUIRE(ctx, fields[i] >= 0 && fields[i] < (int)ctx->fields.size(), "unknown field");
+  FeForm F;
+  F.sysType = sysType;
+  F.fields.assign(fields, fields + nfields);
+  F.degree = quad_degree;
+  const int k0 = ctx->fields[fields[0]].kind;
+  // Layouts as set by each feSysElm::createElementarySystem.
+  switch(sysType) {
+    case FE_SYS_DIFFUSION:
+      FE_REQUIRE(ctx, nfields == 1 && k0 != FE_TRI_VP2, "DIFFUSION takes one scalar field");
+      F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = true;
+      FE_REQUIRE(ctx, ncoeff == 1, "DIFFUSION takes one coefficient");
+      break;
+    case FE_SYS_SOURCE:
+      FE_REQUIRE(ctx, nfields == 1 && k0 != FE_TRI_VP2, "SOURCE takes one scalar field");
+      F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = false;
+      FE_REQUIRE(ctx, ncoeff == 1, "SOURCE takes one function");
+      break;
+    default:
+      FE_REQUIRE(ctx, false, "weak form not supported on the device: " + std::to_string(sysType));
+  }
+  int nq = feBuildRefTables(k0, quad_degree, &F.tab);
+  FE_REQUIRE(ctx, nq > 0, "unsupported quadrature degree " + std::to_string(quad_degree));
+  for(int c = 0; c < ncoeff; ++c) {
+    FeFn fn;
+    fn.kind = coeffs[c].kind;
+    fn.c0 = coeffs[c].c0;
+    fn.c1 = coeffs[c].c1;
+    fn.data = nullptr;
+    FE_REQUIRE(ctx, fn.kind >= FE_FN_CONST && fn.kind <= FE_FN_KOV_P, "unknown function kind");
+    if(fn.kind == FE_FN_ELEM || fn.kind == FE_FN_QP) {
+      const int64_t need = (fn.kind == FE_FN_ELEM) ? ctx->E : (int64_t)ctx->E * nq;
+      FE_REQUIRE(ctx, coeffs[c].data && coeffs[c].count == need, "coefficient table has the wrong size");
+      if(feAlloc(ctx, &fn.data, (size_t)need)) return FE_STATUS_OUT_OF_MEMORY;
+      FE_HIP_TRY(ctx, hipMemcpy(fn.data, coeffs[c].data, sizeof(double) * need, hipMemcpyHostToDevice));
+    }
+    F.coeffs.push_back(fn);
+  }
+  if(feAlloc(ctx, &F.d_tab, sizeof(FeRefTables) / sizeof(double) + 1)) return FE_STATUS_OUT_OF_MEMORY;
+  FE_HIP_TRY(ctx, hipMemcpy(F.d_tab, &F.tab, sizeof(FeRefTables), hipMemcpyHostToDevice));
+  for(int f : F.layoutI) F.nI += ctx->fields[f].nloc;
+  for(int f : F.layoutJ) F.nJ += ctx->fields[f].nloc;
+  ctx->forms.push_back(F);
+  if(form) *form = (int)ctx->forms.size() - 1;
+  return FE_STATUS_OK;
+}
+
+int fe_hip_set_solution(fe_hip_ctx *ctx, const double *u)
+{
+  if(!ctx || !u) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, ctx->d_u, "set the sizes first");
+  FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_u, u, sizeof(double) * ctx->nDofs, hipMemcpyHostToDevice, ctx->stream));
+  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return FE_STATUS_OK;
+}
+
+int fe_hip_get_solution(fe_hip_ctx *ctx, double *u)
+{
+  if(!ctx || !u) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, ctx->d_u, "set the sizes first");
+  FE_HIP_TRY(ctx, hipMemcpyAsync(u, ctx->d_u, sizeof(double) * ctx->nDofs, hipMemcpyDeviceToHost, ctx->stream));
+  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return FE_STATUS_OK;
+}
+
+int fe_hip_get_system(fe_hip_ctx *ctx, double *val, double *rhs)
+{
+  if(!ctx) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, ctx->csrBuilt, "build the CSR first");
+  if(val) FE_HIP_TRY(ctx, hipMemcpyAsync(val, ctx->d_val, sizeof(double) * ctx->nnz, hipMemcpyDeviceToHost, ctx->stream));
+  if(rhs) FE_HIP_TRY(ctx, hipMemcpyAsync(rhs, ctx->d_rhs, sizeof(double) * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return FE_STATUS_OK;
+}
+
+int fe_hip_set_system(fe_hip_ctx *ctx, const double *val, const double *rhs)
+{
+  if(!ctx) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, ctx->csrBuilt, "build the CSR first");
+  if(val) FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_val, val, sizeof(double) * ctx->nnz, hipMemcpyHostToDevice, ctx->stream));
+  if(rhs) FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_rhs, rhs, sizeof(double) * ctx->n, hipMemcpyHostToDevice, ctx->stream));
+  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return FE_STATUS_OK;
+}
+
+int fe_hip_set_profiling(fe_hip_ctx *ctx, int on)
+{
+  if(!ctx) return FE_STATUS_POINTER_ERROR;
+  ctx->profiling = on != 0;
+  return FE_STATUS_OK;
+}
+
+int fe_hip_kernel_stats(fe_hip_ctx *ctx, int which, double *avg_ms, int64_t *launches)
+{
+  if(!ctx) return FE_STATUS_POINTER_ERROR;
+  if(which == 0) {
+    if(avg_ms) *avg_ms = ctx->spmvTimer.launches ? ctx->spmvTimer.total_ms / ctx->spmvTimer.launches : 0.;
+    if(launches) *launches = ctx->spmvTimer.launches;
+  } else if(which == 1) {
+    if(avg_ms) *avg_ms = ctx->lastAsmMs;
+    if(launches) *launches = 1;
+  } else {
+    return FE_STATUS_RANGE_ERROR;
+  }
+  return FE_STATUS_OK;
+}
+
+void fe_hip_default_opts(fe_hip_solver_opts *o)
+{
+  if(!o) return;
+  o->ksp = FE_KSP_CG;
+  o->pc = FE_PC_JACOBI;
+  o->precision = FE_PREC_FP64;
+  o->restart = 50;
+  o->max_iter = 10000;
+  o->rtol = 1e-8;
+  o->atol = 1e-14;
+  o->dtol = 1e6;
+  o->inner_rtol = 1e-3;
+  o->check_every = 32;
+}
+
+int fe_hip_solve(fe_hip_ctx *ctx, const fe_hip_solver_opts *opts, double *du, fe_hip_solve_stats *stats)
+{
+  if(!ctx || !opts) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, ctx->csrBuilt, "build the CSR first");
+  FE_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  return feSolve(ctx, opts, du, stats);
+}
+
+}  // extern "C"

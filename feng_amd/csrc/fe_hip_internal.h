@@ -1,0 +1,234 @@
+// Internal definitions shared by the libfe_hip.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "fe_hip.h"
+
+#define FE_WAVE 64
+
+// ---------------------------------------------------------------------------
+// Error handling: every HIP call is checked; a failure records the message in
+// the context and returns FE_STATUS_ERROR through the C ABI.
+#define FE_HIP_TRY(ctx, call)                                                                   \
+  do {                                                                                          \
+    hipError_t err_ = (call);                                                                   \
+    if(err_ != hipSuccess) {                                                                    \
+      (ctx)->setError(std::string(#call) + ": " + hipGetErrorString(err_), __FILE__, __LINE__); \
+      return FE_STATUS_ERROR;                                                                   \
+    }                                                                                           \
+  } while(0)
+
+#define FE_REQUIRE(ctx, cond, msg)                         \
+  do {                                                     \
+    if(!(cond)) {                                          \
+      (ctx)->setError(std::string(msg), __FILE__, __LINE__); \
+      return FE_STATUS_ERROR;                              \
+    }                                                      \
+  } while(0)
+
+// ---------------------------------------------------------------------------
+// Reference-element tables of one (element kind, quadrature degree) pair,
+// staged into LDS by the element kernels.
+struct FeRefTables {
+  int nq, nf, dim, nv;
+  double w[16];
+  double L[16][6];
+  double dLr[16][6], dLs[16][6], dLt[16][6];
+  double Lg[16][4];  // P1 geometric basis at the quadrature points
+};
+
+int feBuildRefTables(int kind, int degree, FeRefTables *t);  // host, returns nq or <0
+
+struct FeFn {
+  int kind;
+  double c0, c1;
+  double *data;  // device copy for FE_FN_ELEM / FE_FN_QP (color-sorted element order)
+};
+
+struct FeField {
+  int kind, nloc;
+  int *adr = nullptr;  // device, E * nloc, original element order
+};
+
+struct FeForm {
+  int sysType;
+  std::vector<int> fields;  // indices into ctx->fields, reference order
+  int degree;
+  std::vector<FeFn> coeffs;
+  FeRefTables tab;
+  double *d_tab = nullptr;  // device copy of tab
+  bool hasMatrix;
+  // layouts (feSysElm createElementarySystem): field ids of rows / cols
+  std::vector<int> layoutI, layoutJ;
+  int nI = 0, nJ = 0;
+  int group = -1;  // CSR/position-map group (identical layouts share one)
+};
+
+// Distinct (layoutI, layoutJ) pairs of the matrix forms: the CSR candidates
+// and the scatter-position maps are built per group.
+struct FeGroup {
+  std::vector<int> layoutI, layoutJ;
+  int nI, nJ;
+  int *d_adrI = nullptr;  // E * nI, color-sorted element order
+  int *d_adrJ = nullptr;  // E * nJ
+  int *d_pos = nullptr;   // (nI*nJ) * E, slot-major, -1 when filtered
+};
+
+struct FeKernelTimer {
+  std::vector<hipEvent_t> ev;  // pairs
+  size_t used = 0;
+  double total_ms = 0.;
+  int64_t launches = 0;
+};
+
+struct fe_hip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // mesh
+  int dim = 0, V = 0, E = 0, nvpe = 0;
+  double *d_xyz = nullptr;
+  int *d_conn = nullptr;  // original element order
+
+  // coloring: elements sorted by color
+  std::vector<int> colorStart;  // ncolors + 1
+  int *d_perm = nullptr;        // color-sorted position -> element id
+  std::vector<int> h_perm;
+  bool colored = false;
+
+  std::vector<FeField> fields;
+  std::vector<FeForm> forms;
+  std::vector<FeGroup> groups;
+
+  int n = 0, nDofs = 0;
+
+  // CSR
+  int64_t nnz = 0;
+  int *d_ia = nullptr, *d_ja = nullptr, *d_diag = nullptr;
+  double *d_val = nullptr;
+  float *d_val32 = nullptr;
+  double *d_rhs = nullptr, *d_u = nullptr, *d_du = nullptr;
+  bool csrBuilt = false;
+  int spmvLanes = 8;
+
+  // reductions
+  double *d_partials = nullptr;
+  unsigned *d_counter = nullptr;
+  int maxBlocks = 0;
+
+  // Krylov workspace (allocated on first solve)
+  void *d_work = nullptr;
+  size_t workBytes = 0;
+  void *d_state = nullptr;
+  void *h_state = nullptr;  // pinned mirror
+
+  // measurement
+  bool profiling = false;
+  FeKernelTimer spmvTimer;
+  double lastAsmMs = 0.;
+
+  std::vector<void *> owned;  // device allocations freed at destroy
+
+  void setError(const std::string &m, const char *file, int line)
+  {
+    char buf[64];
+    snprintf(buf, sizeof(buf), " (%s:%d)", file, line);
+    err = m + buf;
+  }
+};
+
+// Allocation helper: device buffer owned by the context.
+template <typename T>
+int feAlloc(fe_hip_ctx *ctx, T **p, size_t count)
+{
+  void *q = nullptr;
+  size_t bytes = count * sizeof(T);
+  if(bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(&q, bytes);
+  if(e != hipSuccess) {
+    ctx->setError(std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e),
+                  __FILE__, __LINE__);
+    return FE_STATUS_OUT_OF_MEMORY;
+  }
+  ctx->owned.push_back(q);
+  *p = (T *)q;
+  return FE_STATUS_OK;
+}
+void feFree(fe_hip_ctx *ctx, void *p);
+
+// Exclusive scan of n int32 counts into out (n+1 entries, out[n] = total).
+// Returns the total through *total (host), or an error when it overflows int32.
+int feExclusiveScan(fe_hip_ctx *ctx, const int *d_in, int *d_out, int n, int64_t *total);
+
+// Krylov entry (fe_krylov.hip)
+int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *opts, double *du,
+            fe_hip_solve_stats *stats);
+
+// ---------------------------------------------------------------------------
+// Device helpers
+__device__ __forceinline__ double feWaveSum(double v)
+{
+#pragma unroll
+  for(int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block sum: result valid in thread 0. lds must hold blockDim/64 doubles.
+__device__ __forceinline__ double feBlockSum(double v, double *lds)
+{
+  v = feWaveSum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if(lane == 0) lds[wid] = v;
+  __syncthreads();
+  double s = 0.;
+  if(threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for(int i = 0; i < nw; ++i) s += lds[i];
+  }
+  return s;
+}
+
+// Grid-wide sum of up to NV values with the last-block-done pattern.  Every
+// block writes its partials; the block that arrives last (atomic ticket)
+// folds them in fixed block order -- deterministic -- and returns true in
+// thread 0 with the totals in out[].  The counter is reset for the next use.
+template <int NV>
+__device__ __forceinline__ bool feGridSum(const double (&v)[NV], double *partials,
+                                          unsigned *counter, double (&out)[NV], double *lds)
+{
+  __shared__ bool amLast;
+  double bs[NV];
+#pragma unroll
+  for(int k = 0; k < NV; ++k) {
+    bs[k] = feBlockSum(v[k], lds);
+    __syncthreads();
+  }
+  if(threadIdx.x == 0) {
+#pragma unroll
+    for(int k = 0; k < NV; ++k) partials[(size_t)k * gridDim.x + blockIdx.x] = bs[k];
+    __threadfence();
+    unsigned t = atomicAdd(counter, 1u);
+    amLast = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if(!amLast) return false;
+  __threadfence();
+#pragma unroll
+  for(int k = 0; k < NV; ++k) {
+    double s = 0.;
+    for(int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+      s += partials[(size_t)k * gridDim.x + b];
+    s = feBlockSum(s, lds);
+    __syncthreads();
+    out[k] = s;
+  }
+  if(threadIdx.x == 0) *counter = 0u;
+  return threadIdx.x == 0;
+}

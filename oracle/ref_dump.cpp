@@ -192,6 +192,8 @@ int main(int argc, char **argv)
     putInts("conn", cnc->getVerticesConnectivity());
     putInts("edges", cnc->getEdgeConnectivity());
     putInt("nvpe", cnc->getNumVerticesPerElem());
+    const feCncGeo *bnd = mesh.getCncGeoByName("Bord");
+    putInts("bconn", bnd->getVerticesConnectivity());
     std::vector<int> colorOf(cnc->getNumElements(), -1);
     for(int c = 0; c < cnc->getNbColor(); ++c)
       for(int e : cnc->getListElmPerColorI(c)) colorOf[e] = c;

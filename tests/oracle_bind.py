@@ -1,0 +1,171 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so) -- the checker.
+
+Loaded only by tests, smoke() and bench.py's cpu_baseline leg.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE = ROOT / "oracle" / "liboracle.so"
+
+ip = C.POINTER(C.c_int)
+dp = C.POINTER(C.c_double)
+
+
+class OrFn(C.Structure):
+    _fields_ = [("kind", C.c_int), ("c0", C.c_double), ("c1", C.c_double), ("data", dp)]
+
+
+def P(a, t):
+    return a.ctypes.data_as(C.POINTER(t)) if a is not None else None
+
+
+class Oracle:
+    def __init__(self, lib):
+        self.L = lib
+        L = lib
+        L.or_tri_edges.argtypes = [C.c_int, ip, ip]
+        L.or_color_elements.argtypes = [C.c_int, C.c_int, ip, C.c_int, ip]
+        L.or_number.argtypes = [C.c_int, C.c_int, C.c_int, ip, C.c_int, ip, C.c_int, C.c_int, ip, ip,
+                                C.c_int, ip, ip, C.POINTER(ip), ip, ip]
+        L.or_csr.restype = C.c_int64
+        L.or_csr.argtypes = [C.c_int, C.c_int, ip, ip, ip, C.POINTER(ip), C.POINTER(ip), ip, ip]
+        L.or_quad_tri.argtypes = [C.c_int, dp, dp, dp]
+        L.or_quad_tet.argtypes = [C.c_int, dp, dp, dp, dp]
+        L.or_assemble_diffusion.argtypes = [C.c_int, C.c_int, dp, C.c_int, ip, ip, C.c_int, ip, ip, C.c_int,
+                                            OrFn, OrFn, dp, ip, dp, dp]
+        L.or_element_diffusion.argtypes = [C.c_int, dp, C.c_int, OrFn, C.c_int, dp]
+        L.or_assemble_ns.argtypes = [C.c_int, dp, C.c_int, ip, ip, ip, C.c_int, ip, ip, C.c_int, C.c_double,
+                                     dp, ip, dp, dp]
+        L.or_spmv.argtypes = [C.c_int, ip, ip, dp, dp, dp]
+        L.or_cg_jacobi.argtypes = [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_int, dp]
+        L.or_ic0.argtypes = [C.c_int, ip, ip, dp, dp]
+        L.or_pcg_ic0.argtypes = [C.c_int, ip, ip, dp, dp, dp, dp, C.c_double, C.c_int, dp]
+        L.or_ilu0.argtypes = [C.c_int, ip, ip, dp, dp]
+        L.or_gmres_ilu0.argtypes = [C.c_int, ip, ip, dp, dp, dp, dp, C.c_int, C.c_double, C.c_int, dp]
+        L.or_l2_error_sinsin.restype = C.c_double
+        L.or_l2_error_sinsin.argtypes = [C.c_int, dp, C.c_int, ip, ip, dp, C.c_int]
+
+    # -- helpers ------------------------------------------------------------
+    def tri_edges(self, conn):
+        conn = np.ascontiguousarray(conn, np.int32)
+        E = conn.size // 3
+        edges = np.zeros(3 * E, np.int32)
+        ne = self.L.or_tri_edges(E, P(conn, C.c_int), P(edges, C.c_int))
+        return edges, ne
+
+    def color(self, conn, nvpe, V):
+        conn = np.ascontiguousarray(conn, np.int32)
+        E = conn.size // nvpe
+        col = np.zeros(E, np.int32)
+        nc = self.L.or_color_elements(E, nvpe, P(conn, C.c_int), V, P(col, C.c_int))
+        return col, nc
+
+    def number(self, V, conn, nvpe, edges, nEdges, bconn, bnv, bedge, kinds, essential):
+        nloc = {1: 3, 2: 6, 3: 12, 4: 4}
+        conn = np.ascontiguousarray(conn, np.int32)
+        E = conn.size // nvpe
+        outs = [np.zeros(E * nloc[k], np.int32) for k in kinds]
+        arr = (ip * len(kinds))(*[P(o, C.c_int) for o in outs])
+        k = np.array(kinds, np.int32)
+        e = np.array(essential, np.int32)
+        nInc, nDofs = C.c_int(), C.c_int()
+        edges = np.ascontiguousarray(edges if edges is not None and len(edges) else np.zeros(1), np.int32)
+        bedge = np.ascontiguousarray(bedge if bedge is not None and len(bedge) else np.zeros(1), np.int32)
+        bconn = np.ascontiguousarray(bconn, np.int32)
+        rc = self.L.or_number(V, E, nvpe, P(conn, C.c_int), nEdges, P(edges, C.c_int), bconn.size // bnv, bnv,
+                              P(bconn, C.c_int), P(bedge, C.c_int), len(kinds), P(k, C.c_int), P(e, C.c_int),
+                              arr, C.byref(nInc), C.byref(nDofs))
+        assert rc == 0
+        return nInc.value, nDofs.value, outs
+
+    def csr(self, n, forms):
+        """forms: list of (adrI, nI, adrJ, nJ) with E elements each."""
+        nf = len(forms)
+        Es = (C.c_int * nf)(*[f[0].size // f[1] for f in forms])
+        nI = (C.c_int * nf)(*[f[1] for f in forms])
+        nJ = (C.c_int * nf)(*[f[3] for f in forms])
+        keep = [np.ascontiguousarray(f[0], np.int32) for f in forms] + [np.ascontiguousarray(f[2], np.int32) for f in forms]
+        aI = (ip * nf)(*[P(keep[i], C.c_int) for i in range(nf)])
+        aJ = (ip * nf)(*[P(keep[nf + i], C.c_int) for i in range(nf)])
+        nnz = self.L.or_csr(n, nf, Es, nI, nJ, aI, aJ, None, None)
+        ia = np.zeros(n + 1, np.int32)
+        ja = np.zeros(nnz, np.int32)
+        self.L.or_csr(n, nf, Es, nI, nJ, aI, aJ, P(ia, C.c_int), P(ja, C.c_int))
+        return ia, ja
+
+    def assemble_diffusion(self, kind, xyz, conn, adr, n, ia, ja, deg, coeff, src, u, order=None):
+        nvpe = 4 if kind == 4 else 3
+        E = conn.size // nvpe
+        val = np.zeros(ja.size)
+        rhs = np.zeros(n)
+        keep = []
+
+        def fn(f):
+            kind_, c0, c1, data = f
+            o = OrFn(kind_, c0, c1, None)
+            if data is not None:
+                d = np.ascontiguousarray(data, np.float64)
+                keep.append(d)
+                o.data = P(d, C.c_double)
+            return o
+        order = None if order is None else np.ascontiguousarray(order, np.int32)
+        rc = self.L.or_assemble_diffusion(kind, xyz.size // 3, P(xyz, C.c_double), E, P(conn, C.c_int),
+                                          P(adr, C.c_int), n, P(ia, C.c_int), P(ja, C.c_int), deg, fn(coeff),
+                                          fn(src), P(u, C.c_double), P(order, C.c_int), P(val, C.c_double),
+                                          P(rhs, C.c_double))
+        assert rc == 0, rc
+        return val, rhs
+
+    def spmv(self, ia, ja, val, x):
+        y = np.zeros(ia.size - 1)
+        self.L.or_spmv(ia.size - 1, P(ia, C.c_int), P(ja, C.c_int), P(val, C.c_double), P(x, C.c_double),
+                       P(y, C.c_double))
+        return y
+
+    def cg_jacobi(self, ia, ja, val, b, rtol, maxit):
+        x = np.zeros(b.size)
+        rr = C.c_double()
+        it = self.L.or_cg_jacobi(b.size, P(ia, C.c_int), P(ja, C.c_int), P(val, C.c_double), P(b, C.c_double),
+                                 P(x, C.c_double), rtol, maxit, C.byref(rr))
+        return x, it, rr.value
+
+    def ic0(self, ia, ja, val):
+        l = np.zeros(val.size)
+        rc = self.L.or_ic0(ia.size - 1, P(ia, C.c_int), P(ja, C.c_int), P(val, C.c_double), P(l, C.c_double))
+        return l, rc
+
+    def pcg_ic0(self, ia, ja, val, lval, b, rtol, maxit):
+        x = np.zeros(b.size)
+        rr = C.c_double()
+        it = self.L.or_pcg_ic0(b.size, P(ia, C.c_int), P(ja, C.c_int), P(val, C.c_double), P(lval, C.c_double),
+                               P(b, C.c_double), P(x, C.c_double), rtol, maxit, C.byref(rr))
+        return x, it, rr.value
+
+    def ilu0(self, ia, ja, val):
+        lu = np.zeros(val.size)
+        rc = self.L.or_ilu0(ia.size - 1, P(ia, C.c_int), P(ja, C.c_int), P(val, C.c_double), P(lu, C.c_double))
+        return lu, rc
+
+    def gmres_ilu0(self, ia, ja, val, lu, b, restart, rtol, maxit):
+        x = np.zeros(b.size)
+        rr = C.c_double()
+        it = self.L.or_gmres_ilu0(b.size, P(ia, C.c_int), P(ja, C.c_int), P(val, C.c_double), P(lu, C.c_double),
+                                  P(b, C.c_double), P(x, C.c_double), restart, rtol, maxit, C.byref(rr))
+        return x, it, rr.value
+
+    def l2_error_sinsin(self, kind, xyz, conn, adr, u, deg):
+        E = conn.size // 3
+        return self.L.or_l2_error_sinsin(kind, P(xyz, C.c_double), E, P(conn, C.c_int), P(adr, C.c_int),
+                                         P(u, C.c_double), deg)
+
+
+def load_oracle():
+    if not ORACLE.exists():
+        subprocess.run(["make", "-C", str(ROOT / "oracle"), "liboracle.so"], check=True)
+    return Oracle(C.CDLL(str(ORACLE)))

@@ -1,0 +1,132 @@
+"""GPU parity of the diffusion path (libfe_hip.so through the C ABI) against
+the reference fixtures and the CPU oracle."""
+import math
+
+import numpy as np
+import pytest
+
+from tests.cases import DIFF_CASES, coeffs_of, kind_of, load, product_mesh, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _fn(t):
+    from feng_amd.api import Fn
+    return Fn(t[0], t[1], t[2], t[3])
+
+
+@pytest.mark.parametrize("name", DIFF_CASES)
+def test_device_csr_and_assembly_match_reference(name):
+    from feng_amd.problems import diffusion_system
+    print("case", name, flush=True)
+    g = load(name)
+    m = product_mesh(g)
+    k, s = coeffs_of(g)
+    S, num = diffusion_system(m, kind_of(g), g["deg"], _fn(k), _fn(s))
+    ia, ja = S.get_csr()
+    assert np.array_equal(ia, g["ia"]) and np.array_equal(ja, g["ja"])
+    S.assemble(3)
+    val, rhs = S.get_system()
+    assert relerr(val, g["val"]) <= 1e-12
+    assert relerr(rhs, g["rhs"]) <= 1e-12
+    S.close()
+
+
+def _oracle_system(oracle, m, num, kind, deg, k, s):
+    ia, ja = oracle.csr(num.nInc, [(num.adr[0], num.adr[0].size // m.E, num.adr[0], num.adr[0].size // m.E)])
+    val, rhs = oracle.assemble_diffusion(kind, m.xyz, m.conn, num.adr[0], num.nInc, ia, ja, deg, k, s,
+                                         np.zeros(num.nDofs))
+    return ia, ja, val, rhs
+
+
+@pytest.mark.parametrize("cfg", ["c1_p1_64", "p2_blocks_48", "tet_grf_12"])
+def test_end_to_end_vs_oracle(oracle, cfg):
+    """Device CSR bit-exact vs the oracle, values <= 1e-12, CG reaches the
+    FP64 true-residual target (checked on the host with the oracle SpMV)."""
+    from feng_amd.api import FN_ELEM, Fn, Mesh
+    from feng_amd.problems import diffusion_system
+    print("case", cfg, flush=True)
+    if cfg == "c1_p1_64":
+        m, kind, deg = Mesh.square(64), 1, 2
+        k, s = (0, 1.0, 0.0, None), (1, -2 * math.pi ** 2, 0.0, None)
+        rtol = 1e-10
+    elif cfg == "p2_blocks_48":
+        m, kind, deg = Mesh.square(48), 2, 2
+        kk = m.coeff_blocks(16, 1e-2, 1e2, 9)
+        k, s = (4, 0.0, 0.0, kk), (0, -1.0, 0.0, None)
+        rtol = 1e-10
+    else:
+        m, kind, deg = Mesh.cube(12), 4, 1
+        kk = m.coeff_grf(0.1, 1.5, 256, 4)
+        k, s = (4, 0.0, 0.0, kk), (0, -1.0, 0.0, None)
+        rtol = 1e-8
+    S, num = diffusion_system(m, kind, deg, _fn(k), _fn(s))
+    ia0, ja0, val0, rhs0 = _oracle_system(oracle, m, num, kind, deg, k, s)
+    ia, ja = S.get_csr()
+    assert np.array_equal(ia, ia0) and np.array_equal(ja, ja0)
+    S.assemble(3)
+    val, rhs = S.get_system()
+    assert relerr(val, val0) <= 1e-12 and relerr(rhs, rhs0) <= 1e-12
+    r, du = S.solve(rtol=rtol, want_du=True)
+    assert r.converged and r.relres <= rtol
+    res = rhs0 - oracle.spmv(ia0, ja0, val0, du)
+    assert np.linalg.norm(res) <= 1.01 * rtol * np.linalg.norm(rhs0)
+    x0, it0, rr0 = oracle.cg_jacobi(ia0, ja0, val0, rhs0, rtol * 0.1, 20000)
+    assert np.abs(du - x0).max() <= 1e-5 * np.abs(x0).max()
+    S.close()
+
+
+def test_fp32_refinement_reaches_fp64_tolerance(oracle):
+    from feng_amd.api import PREC_FP32_IR, Mesh
+    from feng_amd.problems import diffusion_system
+    m = Mesh.square(64, 0.2, 0)
+    S, num = diffusion_system(m, 2, 2, _fn((0, 1.0, 0.0, None)), _fn((1, -2 * math.pi ** 2, 0.0, None)))
+    S.assemble(3, PREC_FP32_IR)
+    r = S.solve(precision=PREC_FP32_IR, rtol=1e-10)
+    assert r.converged and r.relres <= 1e-10 and r.outer_iterations >= 2
+    r64 = S.solve(rtol=1e-10)
+    assert r64.converged
+    S.close()
+
+
+def test_assembly_is_deterministic():
+    from feng_amd.api import Mesh
+    from feng_amd.problems import diffusion_system
+    m = Mesh.square(40, 0.2, 2)
+    S, num = diffusion_system(m, 2, 5, _fn((2, 4.0, 0.0, None)), _fn((0, -1.0, 0.0, None)))
+    S.assemble(3)
+    v1, r1 = S.get_system()
+    S.setToZero()
+    S.assemble(3)
+    v2, r2 = S.get_system()
+    assert np.array_equal(v1, v2) and np.array_equal(r1, r2)
+    S.close()
+
+
+# scalarFE_diffusion.output (tests/withLinearSolver/ of the reference): L2
+# error of P1 and P2 Lagrange diffusion on mmsMeshes/square1-4, degree-8
+# quadrature, u = sin(pi x) sin(pi y), printed with 6 significant digits.
+REF_TABLE = {
+    1: ["7.907546e-02", "2.113277e-02", "5.377435e-03", "1.350436e-03"],
+    2: ["4.327628e-03", "5.480619e-04", "6.873916e-05", "8.600535e-06"],
+}
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_reference_convergence_table(oracle, order):
+    from feng_amd.api import Mesh
+    from feng_amd.problems import NLSolverOptions, diffusion_system, solve_newton
+    from tests.conftest import GOLDEN
+    out = []
+    for lvl in range(1, 5):
+        m = Mesh.read_msh(GOLDEN / "meshes" / f"square{lvl}.msh")
+        bc = lambda x, y, z, c: np.sin(np.pi * x) * np.sin(np.pi * y)
+        S, num = diffusion_system(m, order, 8, _fn((0, 1.0, 0.0, None)), _fn((1, -2 * math.pi ** 2, 0.0, None)),
+                                  u_bc=bc)
+        rep = solve_newton(S, NLSolverOptions(), rtol=1e-13)
+        assert rep.converged
+        u = S.get_solution()
+        err = oracle.l2_error_sinsin(order, m.xyz, m.conn, num.adr[0], u, 8)
+        out.append(f"{err:.6e}")
+        S.close()
+    assert out == REF_TABLE[order]
