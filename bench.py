@@ -139,14 +139,15 @@ def main():
     plain_ms = np.mean([r["wall_s"] for r in plain]) * 1e3
     plain_dofs = n / (plain_ms / 1e3)
 
-    # roofline of the dominant kernel (FP32 SpMV sweep of the inner CG):
-    # algorithmic bytes per launch = values 4 nnz + column ids 4 nnz + row
-    # pointers 4 (n+1) + read z, p_old (8 n) + write p_new, q (8 n)
+    # roofline of the dominant kernel (the fused pipelined-CG iteration,
+    # FP32): algorithmic bytes per launch = values 4 nnz + column ids 4 nnz +
+    # row pointers 4 (n+1) + own-row reads of D^-1, r, p, w, s_old, x (6 x 4 n)
+    # + writes of x, p, s, r_new, w_new (5 x 4 n)
     spmv_ms = float(np.mean([r["spmv_ms"] for r in fast]))
-    alg_bytes = 8 * nnz + 4 * (n + 1) + 16 * n
+    alg_bytes = 8 * nnz + 4 * (n + 1) + 44 * n
     achieved = alg_bytes / (spmv_ms * 1e-3) / 1e9
     p_spmv_ms = float(np.mean([r["spmv_ms"] for r in plain]))
-    p_alg = 12 * nnz + 4 * (n + 1) + 32 * n
+    p_alg = 12 * nnz + 4 * (n + 1) + 88 * n
 
     cpu = None
     if not args.no_cpu:
@@ -182,7 +183,7 @@ def main():
                    "plain_fp64_dofs": plain_dofs, "plain_fp64_ms": plain_ms,
                    "plain_iterations": plain[-1]["iters"], "speedup_vs_plain_fp64": plain_ms / ms_step,
                    "plain_spmv_roofline_frac": p_alg / (p_spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if p_spmv_ms else None},
-        "roofline": {"bound": "hbm", "kernel": "cgSpmv<float,16>", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": "pipeIter<float>", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "avg_launch_ms": spmv_ms, "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,

@@ -292,6 +292,14 @@ class LinearSystemHIP:
         self._c(self.L.fe_hip_kernel_stats(self.ctx, which, C.byref(ms), C.byref(cnt)), "fe_hip_kernel_stats")
         return ms.value, cnt.value
 
+    def set_tuning(self, key, value):
+        self._c(self.L.fe_hip_set_tuning(self.ctx, key.encode(), int(value)), "fe_hip_set_tuning")
+
+    def get_tuning(self, key):
+        v = C.c_int()
+        self._c(self.L.fe_hip_get_tuning(self.ctx, key.encode(), C.byref(v)), "fe_hip_get_tuning")
+        return v.value
+
     def close(self):
         if self.ctx:
             self.L.fe_hip_ctx_destroy(self.ctx)

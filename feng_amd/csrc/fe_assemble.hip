@@ -211,39 +211,228 @@ __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Taylor-Hood P2-P1 Navier-Stokes form stack fused in one kernel (divergence
+// formulation of the reference's navier_stokes tests, navier_stokes.cpp:83-99):
+//   VectorConvectiveAcceleration  feVectorSysElm.cpp:1153-1246
+//   MixedDivergence {p, u}        feVectorSysElm.cpp:669-753
+//   VectorSource                  feVectorSysElm.cpp:101-132
+//   DivergenceNewtonianStress     feVectorSysElm.cpp:1434-1536
+// 16 lanes per element: lane i < 12 owns velocity row i = 2a + c (function
+// psi_a e_c), lanes 12..14 own the pressure rows, lane 15 only loads.  The
+// element's local solution sits in LDS; every lane rebuilds the P2 gradients
+// at each quadrature point from the LDS tables and accumulates its own row of
+// every local matrix in registers, then scatters it through the position
+// maps (exclusive ownership within the color: no atomics).
+struct NsAsmArgs {
+  int begin, count, E, n, what;
+  const int *perm, *conn;
+  const int *adrU, *adrP;  // color-sorted element maps (12, 3 per element)
+  const int *posConv, *posDiv, *posStr;
+  const double *xyz, *tab, *u;
+  int hasConv, hasDiv, hasStr, hasSrc;
+  FeFn cConv, cDiv, cStr, mu, src0, src1;
+  double *val, *rhs;
+};
+
+constexpr int kNsElemPerBlock = 16;
+
+__global__ __launch_bounds__(256) void asmNavierStokesFP64(NsAsmArgs a)
+{
+  __shared__ FeRefTables T;
+  __shared__ double su[kNsElemPerBlock][16];  // 12 velocity + 3 pressure values
+  __shared__ double geo[kNsElemPerBlock][8];  // J, drdx0, drdx1, drdy0, drdy1, x0, y0 (pos origin)
+  __shared__ double cxy[kNsElemPerBlock][6];
+  {
+    const double *src = a.tab;
+    double *dst = (double *)&T;
+    for(int i = threadIdx.x; i < (int)(sizeof(FeRefTables) / sizeof(double)); i += blockDim.x) dst[i] = src[i];
+  }
+  const int le = threadIdx.x >> 4, lane = threadIdx.x & 15;
+  const int t = blockIdx.x * kNsElemPerBlock + le;
+  const bool valid = t < a.count;
+  const int ee = a.begin + (valid ? t : 0);
+  const int e = a.perm[ee];
+  int myDof = 0;
+  if(lane < 12) myDof = a.adrU[(size_t)ee * 12 + lane];
+  else if(lane < 15) myDof = a.adrP[(size_t)ee * 3 + lane - 12];
+  if(lane < 15) su[le][lane] = a.u[myDof];
+  if(lane < 6) {
+    const int v = a.conn[(size_t)e * 3 + lane / 2];
+    cxy[le][lane] = a.xyz[3 * (size_t)v + (lane & 1)];
+  }
+  __syncthreads();
+  if(lane == 0) {
+    const double *c = cxy[le];
+    const double dxdr0 = -c[0] + c[2], dxdr1 = -c[1] + c[3];
+    const double dxds0 = -c[0] + c[4], dxds1 = -c[1] + c[5];
+    const double J = dxdr0 * dxds1 - dxdr1 * dxds0;
+    geo[le][0] = J;
+    geo[le][1] = dxds1 / J;
+    geo[le][2] = -dxdr1 / J;
+    geo[le][3] = -dxds0 / J;
+    geo[le][4] = dxdr0 / J;
+  }
+  __syncthreads();
+  if(!valid || lane == 15) return;
+  const double J = geo[le][0], drdx0 = geo[le][1], drdx1 = geo[le][2], drdy0 = geo[le][3], drdy1 = geo[le][4];
+  const bool uRow = lane < 12;
+  const int ai = lane >> 1, ci = lane & 1, ip = lane - 12;
+  const bool doMat = (a.what & 1), doRes = (a.what & 2);
+  double Ac[12], As[15], Ad[12], Be = 0.;
+#pragma unroll
+  for(int j = 0; j < 12; ++j) { Ac[j] = 0.; Ad[j] = 0.; }
+#pragma unroll
+  for(int j = 0; j < 15; ++j) As[j] = 0.;
+  const int nq = T.nq;
+  for(int k = 0; k < nq; ++k) {
+    double gx[6], gy[6], ps[6];
+#pragma unroll
+    for(int b = 0; b < 6; ++b) {
+      gx[b] = T.dLr[k][b] * drdx0 + T.dLs[k][b] * drdx1;
+      gy[b] = T.dLr[k][b] * drdy0 + T.dLs[k][b] * drdy1;
+      ps[b] = T.L[k][b];
+    }
+    const double pp[3] = {T.Lg[k][0], T.Lg[k][1], T.Lg[k][2]};
+    // u, grad u (grad[m*2+n] = du_n/dx_m), p at the quadrature point
+    double ux = 0., uy = 0., g0 = 0., g1 = 0., g2 = 0., g3 = 0.;
+#pragma unroll
+    for(int b = 0; b < 6; ++b) {
+      const double vx = su[le][2 * b], vy = su[le][2 * b + 1];
+      ux += vx * ps[b];
+      uy += vy * ps[b];
+      g0 += vx * gx[b];
+      g1 += vy * gx[b];
+      g2 += vx * gy[b];
+      g3 += vy * gy[b];
+    }
+    const double pq = su[le][12] * pp[0] + su[le][13] * pp[1] + su[le][14] * pp[2];
+    double x[3] = {0., 0., 0.};
+#pragma unroll
+    for(int j = 0; j < 3; ++j) {
+      x[0] += cxy[le][2 * j] * T.Lg[k][j];
+      x[1] += cxy[le][2 * j + 1] * T.Lg[k][j];
+    }
+    const double jw = J * T.w[k];
+    if(uRow) {
+      const double gi[2] = {gx[ai], gy[ai]};
+      const double psi = ps[ai];
+      if(a.hasConv) {
+        const double cc = feEvalFn(a.cConv, x, e, nq, k);
+        const double gu[4] = {g0, g1, g2, g3};
+        if(doMat) {
+#pragma unroll
+          for(int b = 0; b < 6; ++b) {
+            const double udg = ux * gx[b] + uy * gy[b];
+#pragma unroll
+            for(int cj = 0; cj < 2; ++cj) {
+              const double v = ((ci == cj) ? psi * udg : 0.) + psi * ps[b] * gu[cj * 2 + ci];
+              Ac[2 * b + cj] += cc * v * jw;
+            }
+          }
+        }
+        if(doRes) {
+          const double ugu = ci == 0 ? (ux * g0 + uy * g2) : (ux * g1 + uy * g3);
+          Be -= cc * psi * ugu * jw;
+        }
+      }
+      if(a.hasStr) {
+        const double cs = feEvalFn(a.cStr, x, e, nq, k);
+        const double mu = feEvalFn(a.mu, x, e, nq, k);
+        if(doMat) {
+#pragma unroll
+          for(int b = 0; b < 6; ++b) {
+            const double gg = gi[0] * gx[b] + gi[1] * gy[b];
+            const double gb[2] = {gx[b], gy[b]};
+#pragma unroll
+            for(int cj = 0; cj < 2; ++cj) {
+              const double v = ((ci == cj) ? gg : 0.) + gi[cj] * gb[ci];
+              As[2 * b + cj] += -cs * mu * v * jw;
+            }
+          }
+#pragma unroll
+          for(int j = 0; j < 3; ++j) As[12 + j] += cs * pp[j] * gi[ci] * jw;
+        }
+        if(doRes) {
+          // sym = grad u + grad u^T; contraction with grad phi_i picks column ci
+          const double s0c = ci == 0 ? 2. * g0 : g1 + g2;   // sym[0][ci]
+          const double s1c = ci == 0 ? g2 + g1 : 2. * g3;   // sym[1][ci]
+          const double dcs = gi[0] * s0c + gi[1] * s1c;
+          Be -= cs * (pq * gi[ci] - mu * dcs) * jw;
+        }
+      }
+      if(a.hasSrc && doRes) {
+        const double S = ci == 0 ? feEvalFn(a.src0, x, e, nq, k) : feEvalFn(a.src1, x, e, nq, k);
+        Be -= S * psi * jw;
+      }
+    } else if(a.hasDiv) {
+      const double cd = feEvalFn(a.cDiv, x, e, nq, k);
+      if(doMat) {
+#pragma unroll
+        for(int b = 0; b < 6; ++b) {
+          Ad[2 * b] += cd * pp[ip] * gx[b] * jw;
+          Ad[2 * b + 1] += cd * pp[ip] * gy[b] * jw;
+        }
+      }
+      if(doRes) Be -= cd * (g0 + g3) * pp[ip] * jw;
+    }
+  }
+  if(doRes && myDof < a.n) a.rhs[myDof] += Be;
+  if(!doMat) return;
+  if(uRow) {
+    if(a.hasConv)
+#pragma unroll
+      for(int j = 0; j < 12; ++j) {
+        const int p = a.posConv[(size_t)(lane * 12 + j) * a.E + ee];
+        if(p >= 0) a.val[p] += Ac[j];
+      }
+    if(a.hasStr)
+#pragma unroll
+      for(int j = 0; j < 15; ++j) {
+        const int p = a.posStr[(size_t)(lane * 15 + j) * a.E + ee];
+        if(p >= 0) a.val[p] += As[j];
+      }
+  } else if(a.hasDiv) {
+#pragma unroll
+    for(int j = 0; j < 12; ++j) {
+      const int p = a.posDiv[(size_t)(ip * 12 + j) * a.E + ee];
+      if(p >= 0) a.val[p] += Ad[j];
+    }
+  }
+}
+
 __global__ void toFloat(int64_t n, const double *__restrict__ x, float *__restrict__ y)
 {
   for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     y[i] = (float)x[i];
 }
 
-__global__ void maxAbs(int n, const double *__restrict__ x, double *partials, unsigned *counter,
-                       double *out)
+// Per-block max |x| (plain stores), folded by maxFinal after the kernel
+// boundary -- no device-scope fences inside the kernels.
+__global__ void maxAbsBlocks(int n, const double *__restrict__ x, double *__restrict__ partials)
 {
   __shared__ double lds[16];
   double m = 0.;
   for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) m = fmax(m, fabs(x[i]));
-  // max via wave shuffles
 #pragma unroll
   for(int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if(lane == 0) lds[wid] = m;
   __syncthreads();
-  __shared__ bool amLast;
   if(threadIdx.x == 0) {
     double b = 0.;
     for(int w = 0; w < (int)(blockDim.x >> 6); ++w) b = fmax(b, lds[w]);
     partials[blockIdx.x] = b;
-    __threadfence();
-    amLast = atomicAdd(counter, 1u) == gridDim.x - 1;
   }
-  __syncthreads();
-  if(amLast && threadIdx.x == 0) {
-    __threadfence();
+}
+
+__global__ void maxFinal(const double *__restrict__ partials, int G, double *__restrict__ out)
+{
+  if(threadIdx.x == 0) {
     double r = 0.;
-    for(int b = 0; b < (int)gridDim.x; ++b) r = fmax(r, partials[b]);
+    for(int b = 0; b < G; ++b) r = fmax(r, partials[b]);
     *out = r;
-    *counter = 0u;
   }
 }
 
@@ -339,6 +528,60 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
     else if(kind == FE_TET_P1) launchScalar<4, 3>(ctx, a);
     else FE_REQUIRE(ctx, false, "scalar forms need a scalar field");
   }
+  // Taylor-Hood stack: all vector/mixed forms fused into one sweep per color.
+  {
+    NsAsmArgs a{};
+    int any = 0, vfield = -1;
+    for(size_t fi = 0; fi < ctx->forms.size(); ++fi) {
+      FeForm &F = ctx->forms[fi];
+      if(done[fi]) continue;
+      switch(F.sysType) {
+        case FE_SYS_VECTOR_CONVECTIVE_ACCELERATION:
+          FE_REQUIRE(ctx, !a.hasConv, "two convective forms");
+          a.hasConv = 1; a.cConv = F.coeffs[0]; a.posConv = ctx->groups[F.group].d_pos;
+          a.adrU = ctx->groups[F.group].d_adrI; vfield = F.fields[0];
+          break;
+        case FE_SYS_MIXED_DIVERGENCE:
+          FE_REQUIRE(ctx, !a.hasDiv, "two divergence forms");
+          a.hasDiv = 1; a.cDiv = F.coeffs[0]; a.posDiv = ctx->groups[F.group].d_pos;
+          a.adrP = ctx->groups[F.group].d_adrI; vfield = F.fields[1];
+          break;
+        case FE_SYS_DIV_NEWTONIAN_STRESS:
+          FE_REQUIRE(ctx, !a.hasStr, "two stress forms");
+          a.hasStr = 1; a.cStr = F.coeffs[0]; a.mu = F.coeffs[1]; a.posStr = ctx->groups[F.group].d_pos;
+          a.adrU = ctx->groups[F.group].d_adrI; vfield = F.fields[0];
+          a.tab = F.d_tab;
+          break;
+        case FE_SYS_VECTOR_SOURCE:
+          FE_REQUIRE(ctx, !a.hasSrc, "two vector sources");
+          a.hasSrc = 1; a.src0 = F.coeffs[0]; a.src1 = F.coeffs[1];
+          break;
+        default: continue;
+      }
+      done[fi] = true;
+      any = 1;
+    }
+    if(any) {
+      FE_REQUIRE(ctx, a.hasStr && a.hasDiv, "the device Navier-Stokes kernel needs the stress and divergence forms");
+      (void)vfield;
+      a.E = ctx->E;
+      a.n = ctx->n;
+      a.what = what;
+      a.perm = ctx->d_perm;
+      a.conn = ctx->d_conn;
+      a.xyz = ctx->d_xyz;
+      a.u = ctx->d_u;
+      a.val = ctx->d_val;
+      a.rhs = ctx->d_rhs;
+      for(size_t c = 0; c + 1 < ctx->colorStart.size(); ++c) {
+        a.begin = ctx->colorStart[c];
+        a.count = ctx->colorStart[c + 1] - a.begin;
+        if(a.count == 0) continue;
+        hipLaunchKernelGGL(asmNavierStokesFP64, dim3((a.count + kNsElemPerBlock - 1) / kNsElemPerBlock), dim3(256), 0,
+                           ctx->stream, a);
+      }
+    }
+  }
   for(size_t fi = 0; fi < ctx->forms.size(); ++fi)
     FE_REQUIRE(ctx, done[fi], "form type has no device kernel");
   FE_HIP_TRY(ctx, hipGetLastError());
@@ -362,8 +605,8 @@ extern "C" int fe_hip_rhs_max_norm(fe_hip_ctx *ctx, double *norm)
   FE_REQUIRE(ctx, ctx->csrBuilt, "build the CSR first");
   double *out = ctx->d_partials + 3 * ctx->maxBlocks;
   const int nb = std::min(1024, (ctx->n + 255) / 256);
-  hipLaunchKernelGGL(maxAbs, dim3(nb), dim3(256), 0, ctx->stream, ctx->n, ctx->d_rhs, ctx->d_partials,
-                     ctx->d_counter + 8, out);
+  hipLaunchKernelGGL(maxAbsBlocks, dim3(nb), dim3(256), 0, ctx->stream, ctx->n, ctx->d_rhs, ctx->d_partials);
+  hipLaunchKernelGGL(maxFinal, dim3(1), dim3(64), 0, ctx->stream, ctx->d_partials, nb, out);
   FE_HIP_TRY(ctx, hipGetLastError());
   FE_HIP_TRY(ctx, hipMemcpyAsync(norm, out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));

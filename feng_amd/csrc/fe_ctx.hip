@@ -304,10 +304,39 @@ int fe_hip_add_form(fe_hip_ctx *ctx, int sysType, int nfields, const int *fields
       F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = false;
       FE_REQUIRE(ctx, ncoeff == 1, "SOURCE takes one function");
       break;
+    case FE_SYS_VECTOR_SOURCE:
+      FE_REQUIRE(ctx, nfields == 1 && k0 == FE_TRI_VP2, "VECTOR_SOURCE takes one vector field");
+      F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = false;
+      FE_REQUIRE(ctx, ncoeff == 2, "VECTOR_SOURCE takes one function per component");
+      break;
+    case FE_SYS_VECTOR_CONVECTIVE_ACCELERATION:
+      FE_REQUIRE(ctx, nfields == 1 && k0 == FE_TRI_VP2, "VECTOR_CONVECTIVE_ACCELERATION takes one vector field");
+      F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = true;
+      FE_REQUIRE(ctx, ncoeff == 1, "VECTOR_CONVECTIVE_ACCELERATION takes one coefficient");
+      break;
+    case FE_SYS_MIXED_DIVERGENCE:
+      // fields {p, u}: scalar test functions, vector unknown (feVectorSysElm.cpp:669-683)
+      FE_REQUIRE(ctx, nfields == 2 && ctx->fields[fields[0]].kind == FE_TRI_P1 &&
+                        ctx->fields[fields[1]].kind == FE_TRI_VP2, "MIXED_DIVERGENCE takes {P1 p, vector P2 u}");
+      F.layoutI = {fields[0]}; F.layoutJ = {fields[1]}; F.hasMatrix = true;
+      FE_REQUIRE(ctx, ncoeff == 1, "MIXED_DIVERGENCE takes one coefficient");
+      break;
+    case FE_SYS_DIV_NEWTONIAN_STRESS:
+      // fields {u, p} (feVectorSysElm.cpp:1434-1450)
+      FE_REQUIRE(ctx, nfields == 2 && k0 == FE_TRI_VP2 && ctx->fields[fields[1]].kind == FE_TRI_P1,
+                 "DIV_NEWTONIAN_STRESS takes {vector P2 u, P1 p}");
+      F.layoutI = {fields[0]}; F.layoutJ = {fields[0], fields[1]}; F.hasMatrix = true;
+      FE_REQUIRE(ctx, ncoeff == 2, "DIV_NEWTONIAN_STRESS takes (coefficient, viscosity)");
+      break;
     default:
       FE_REQUIRE(ctx, false, "weak form not supported on the device: " + std::to_string(sysType));
   }
-  int nq = feBuildRefTables(k0, quad_degree, &F.tab);
+  // Forms touching a vector P2 field use the P2 tables (the pressure basis is
+  // the P1 geometric basis, also tabulated).
+  int tabKind = k0;
+  for(int f : F.fields)
+    if(ctx->fields[f].kind == FE_TRI_VP2) tabKind = FE_TRI_VP2;
+  int nq = feBuildRefTables(tabKind, quad_degree, &F.tab);
   FE_REQUIRE(ctx, nq > 0, "unsupported quadrature degree " + std::to_string(quad_degree));
   for(int c = 0; c < ncoeff; ++c) {
     FeFn fn;
@@ -368,6 +397,42 @@ int fe_hip_set_system(fe_hip_ctx *ctx, const double *val, const double *rhs)
   if(val) FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_val, val, sizeof(double) * ctx->nnz, hipMemcpyHostToDevice, ctx->stream));
   if(rhs) FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_rhs, rhs, sizeof(double) * ctx->n, hipMemcpyHostToDevice, ctx->stream));
   FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return FE_STATUS_OK;
+}
+
+int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value)
+{
+  if(!ctx || !key) return FE_STATUS_POINTER_ERROR;
+  const std::string k(key);
+  if(k == "spmv_lanes") {
+    FE_REQUIRE(ctx, value == 1 || value == 2 || value == 4 || value == 8 || value == 16 || value == 32,
+               "spmv_lanes must be 1, 2, 4, 8, 16 or 32");
+    ctx->spmvLanes = value;
+    ctx->lanesFixed = true;
+  } else if(k == "spmv_unroll") {
+    FE_REQUIRE(ctx, value == 1 || value == 2, "spmv_unroll must be 1 or 2");
+    ctx->spmvUnroll = value;
+  } else if(k == "iter_block") {
+    FE_REQUIRE(ctx, value == 256 || value == 512 || value == 1024, "iter_block must be 256, 512 or 1024");
+    ctx->iterBlock = value;
+  } else if(k == "iter_grid") {
+    FE_REQUIRE(ctx, value >= 8 && value <= 4096 && value % 8 == 0, "iter_grid must be a multiple of 8 in [8, 4096]");
+    ctx->iterGrid = value;
+  } else {
+    FE_REQUIRE(ctx, false, "unknown tuning key " + k);
+  }
+  return FE_STATUS_OK;
+}
+
+int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value)
+{
+  if(!ctx || !key || !value) return FE_STATUS_POINTER_ERROR;
+  const std::string k(key);
+  if(k == "spmv_lanes") *value = ctx->spmvLanes;
+  else if(k == "spmv_unroll") *value = ctx->spmvUnroll;
+  else if(k == "iter_block") *value = ctx->iterBlock;
+  else if(k == "iter_grid") *value = ctx->iterGrid;
+  else FE_REQUIRE(ctx, false, "unknown tuning key " + k);
   return FE_STATUS_OK;
 }
 

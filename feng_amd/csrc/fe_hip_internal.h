@@ -116,7 +116,12 @@ struct fe_hip_ctx {
   float *d_val32 = nullptr;
   double *d_rhs = nullptr, *d_u = nullptr, *d_du = nullptr;
   bool csrBuilt = false;
-  int spmvLanes = 8;
+  // launch shapes of the Krylov iteration kernels (fe_hip_set_tuning)
+  int spmvLanes = 4;
+  int spmvUnroll = 1;
+  int iterBlock = 512;
+  int iterGrid = 1024;
+  bool lanesFixed = false;
 
   // reductions
   double *d_partials = nullptr;
@@ -195,40 +200,38 @@ __device__ __forceinline__ double feBlockSum(double v, double *lds)
   return s;
 }
 
-// Grid-wide sum of up to NV values with the last-block-done pattern.  Every
-// block writes its partials; the block that arrives last (atomic ticket)
-// folds them in fixed block order -- deterministic -- and returns true in
-// thread 0 with the totals in out[].  The counter is reset for the next use.
-template <int NV>
-__device__ __forceinline__ bool feGridSum(const double (&v)[NV], double *partials,
-                                          unsigned *counter, double (&out)[NV], double *lds)
+// Cross-block reductions never use per-block device-scope fences: on this
+// multi-XCD part an agent-scope release is an L2 write-back (buffer_wbl2),
+// which costs tens of microseconds when every block issues one.  Instead each
+// block stores its partial with a plain store, and the partials are folded
+// after the kernel boundary (which is the only release/acquire needed): by
+// the next kernel of the iteration, or by feFinalizeSums.
+
+// Sum of G partials (fixed order per thread, then block tree): every thread
+// of the block gets the total.  lds must hold blockDim/64 + 1 doubles.
+__device__ __forceinline__ double feFoldPartials(const double *__restrict__ part, int G, double *lds)
 {
-  __shared__ bool amLast;
-  double bs[NV];
-#pragma unroll
-  for(int k = 0; k < NV; ++k) {
-    bs[k] = feBlockSum(v[k], lds);
-    __syncthreads();
-  }
+  double s = 0.;
+  for(int b = threadIdx.x; b < G; b += blockDim.x) s += part[b];
+  s = feWaveSum(s);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if(lane == 0) lds[wid] = s;
+  __syncthreads();
   if(threadIdx.x == 0) {
-#pragma unroll
-    for(int k = 0; k < NV; ++k) partials[(size_t)k * gridDim.x + blockIdx.x] = bs[k];
-    __threadfence();
-    unsigned t = atomicAdd(counter, 1u);
-    amLast = (t == gridDim.x - 1);
+    double t = 0.;
+    const int nw = (blockDim.x + 63) >> 6;
+    for(int i = 0; i < nw; ++i) t += lds[i];
+    lds[nw] = t;
   }
   __syncthreads();
-  if(!amLast) return false;
-  __threadfence();
-#pragma unroll
-  for(int k = 0; k < NV; ++k) {
-    double s = 0.;
-    for(int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-      s += partials[(size_t)k * gridDim.x + b];
-    s = feBlockSum(s, lds);
-    __syncthreads();
-    out[k] = s;
-  }
-  if(threadIdx.x == 0) *counter = 0u;
-  return threadIdx.x == 0;
+  return lds[(blockDim.x + 63) >> 6];
+}
+
+// Block partial store: thread 0 writes the block sum of v into part[blockIdx].
+__device__ __forceinline__ void feStoreBlockSum(double v, double *__restrict__ part, double *lds)
+{
+  const double b = feBlockSum(v, lds);
+  if(threadIdx.x == 0) part[blockIdx.x] = b;
+  __syncthreads();
 }

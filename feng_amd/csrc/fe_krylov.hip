@@ -3,14 +3,17 @@
 //
 // Preconditioned CG is two fused kernels per iteration with every scalar kept
 // on the device (no host round trip inside the iteration):
-//   K1 cgSpmv   : p_new = z + beta p_old (computed on the fly, also inside the
-//                 gather), q = A p_new, pq = p_new . q, alpha = rz / pq
-//   K2 cgUpdate : x += alpha p, r -= alpha q, z = D^-1 r, rz' = r . z,
-//                 rr = r . r, beta = rz' / rz
+//   K1 cgSpmv   : beta = rz / rz_old, stop test on r.r, p_new = z + beta p_old
+//                 (computed on the fly, also inside the gather), q = A p_new,
+//                 partials of p_new . q
+//   K2 cgUpdate : alpha = rz / (p.q), x += alpha p, r -= alpha q, z = D^-1 r,
+//                 partials of r.z and r.r
 // p is double-buffered (the gather of K1 reads p_old while rows write p_new).
-// Grid reductions use the last-block-done pattern (deterministic order).  The
-// host polls the iteration state every `check_every` iterations; kernels
-// launched after convergence return immediately.
+// Dot products: every block stores its partial sum; the next kernel of the
+// iteration folds the partials at its start (fixed order: deterministic), so
+// no block ever issues a device-scope fence.  The host polls the iteration
+// state every `check_every` iterations; kernels launched after the stop test
+// fired return immediately.
 //
 // Convergence is judged on the FP64 true residual ||b - A x||_2 / ||b||_2:
 // when the recursive residual meets the target, b - A x is recomputed in FP64
@@ -25,20 +28,58 @@
 
 namespace {
 
+// Iteration state mirrored to the host at every check.
 struct CgState {
-  double rz, alpha, beta, rr, pq;
-  double tol2;
-  int iter, done, maxit, pad;
-  double bb;
+  double rr;      // ||r||^2 at the last checked iteration
+  int iter;       // iterations completed when `done` was set
+  int done;       // 0 running, 1 converged, 2 iteration budget, 3 breakdown, 4 NaN
+  double hist[64];
 };
 
-constexpr int kTB = 256;
-constexpr int kMaxGrid = 4096;
+constexpr int kTB = 256;       // block size of the one-off kernels
+constexpr int kGrid = 1024;    // grid of the one-off reduction kernels
+constexpr int kMaxGrid = 4096; // partial buffers hold this many blocks
 
 template <typename T>
 struct CgVecs {
   T *x, *r, *z, *p[2], *q, *dinv;
 };
+
+// Partial-sum buffers: rz and rr double-buffered by iteration parity.
+struct CgParts {
+  double *rz[2], *rr[2], *pq;
+};
+
+// Folds up to three partial arrays (G entries each) in one pass; every
+// thread gets the totals.
+template <int TB, int NV>
+__device__ __forceinline__ void feFold(const double *const (&parts)[NV], int G, double (&out)[NV])
+{
+  __shared__ double lds[NV][TB / 64 + 1];
+  double s[NV];
+#pragma unroll
+  for(int k = 0; k < NV; ++k) s[k] = 0.;
+  for(int b = threadIdx.x; b < G; b += TB)
+#pragma unroll
+    for(int k = 0; k < NV; ++k) s[k] += parts[k][b];
+#pragma unroll
+  for(int k = 0; k < NV; ++k) s[k] = feWaveSum(s[k]);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if(lane == 0)
+#pragma unroll
+    for(int k = 0; k < NV; ++k) lds[k][wid] = s[k];
+  __syncthreads();
+  if(threadIdx.x == 0)
+#pragma unroll
+    for(int k = 0; k < NV; ++k) {
+      double t = 0.;
+      for(int i = 0; i < TB / 64; ++i) t += lds[k][i];
+      lds[k][TB / 64] = t;
+    }
+  __syncthreads();
+#pragma unroll
+  for(int k = 0; k < NV; ++k) out[k] = lds[k][TB / 64];
+}
 
 // XCD-aware block order: hardware dispatches block b to XCD b % 8; map the 8
 // XCDs onto 8 contiguous row ranges so each XCD's L2 sees neighbouring rows.
@@ -60,14 +101,14 @@ __global__ __launch_bounds__(kTB) void extractDinv(int n, const int *__restrict_
   dinv[i] = (T)(d != 0. ? 1. / d : 1.);
 }
 
-// r = b (T), z = D^-1 r, p_old = 0, x = 0; rz, rr; state init.
-template <typename T>
-__global__ __launch_bounds__(kTB) void cgInit(int n, const double *__restrict__ b, CgVecs<T> v,
-                                              CgState *st, double tol2, int maxit, double *partials,
-                                              unsigned *counter, int fromResidual)
+// r = b (or keep r), z = D^-1 r, p_0 = 0 (and x = 0); partials of r.z, r.r
+// into slot 0.
+template <typename T, int TB>
+__global__ __launch_bounds__(TB) void cgInit(int n, const double *__restrict__ b, CgVecs<T> v, CgParts P,
+                                             CgState *st, int fromResidual)
 {
-  __shared__ double lds[kTB / 64];
-  double acc[2] = {0., 0.};
+  __shared__ double lds[TB / 64 + 1];
+  double a0 = 0., a1 = 0.;
   for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const T ri = fromResidual ? v.r[i] : (T)b[i];
     const T zi = v.dinv[i] * ri;
@@ -75,38 +116,256 @@ __global__ __launch_bounds__(kTB) void cgInit(int n, const double *__restrict__ 
     v.z[i] = zi;
     v.p[0][i] = (T)0;
     if(!fromResidual) v.x[i] = (T)0;
-    acc[0] += (double)ri * (double)zi;
-    acc[1] += (double)ri * (double)ri;
+    a0 += (double)ri * (double)zi;
+    a1 += (double)ri * (double)ri;
   }
-  double out[2];
-  if(feGridSum<2>(acc, partials, counter, out, lds)) {
-    st->rz = out[0];
-    st->rr = out[1];
-    st->beta = 0.;
-    st->alpha = 0.;
-    st->tol2 = tol2;
-    st->maxit = maxit;
-    st->done = (out[1] <= tol2) ? 1 : 0;
-    if(!fromResidual) {
-      st->iter = 0;
-      st->bb = out[1];
-    }
+  feStoreBlockSum(a0, P.rz[0], lds);
+  feStoreBlockSum(a1, P.rr[0], lds);
+  if(blockIdx.x == 0 && threadIdx.x == 0) {
+    st->done = 0;
+    st->iter = 0;
   }
 }
 
-template <typename T, int LPR>
-__global__ __launch_bounds__(kTB) void cgSpmv(int n, const int *__restrict__ ia, const int *__restrict__ ja,
-                                              const T *__restrict__ val, const T *__restrict__ z,
-                                              const T *__restrict__ pOld, T *__restrict__ pNew,
-                                              T *__restrict__ q, CgState *st, double *partials,
-                                              unsigned *counter)
+// Iteration `it`: beta = rz_it / rz_{it-1} (0 at it = 0), stop test on
+// rr_it, then p_new = z + beta p_old (on the fly inside the gather too),
+// q = A p_new, partials of p_new . q.  LPR lanes per row; UNR row groups per
+// lane group in flight (independent load chains for latency hiding).
+template <typename T, int LPR, int TB, int UNR>
+__global__ __launch_bounds__(TB) void cgSpmv(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                             const T *__restrict__ val, const T *__restrict__ z,
+                                             const T *__restrict__ pOld, T *__restrict__ pNew,
+                                             T *__restrict__ q, CgParts P, CgState *st, int it, int maxit,
+                                             double tol2)
 {
-  __shared__ double lds[kTB / 64];
+  __shared__ double lds[TB / 64 + 1];
   if(st->done) return;
-  const T beta = (T)st->beta;
-  constexpr int RPB = kTB / LPR;
+  const int s = it & 1;
+  double f[3];
+  const double *const parts[3] = {P.rr[s], P.rz[s], P.rz[s ^ 1]};
+  feFold<TB, 3>(parts, gridDim.x, f);
+  const double rr = f[0];
+  const bool stop = !(rr > tol2) || it >= maxit;  // also stops on NaN
+  if(stop) {
+    if(blockIdx.x == 0 && threadIdx.x == 0) {
+      st->rr = rr;
+      st->iter = it;
+      st->done = (rr <= tol2) ? 1 : (rr == rr ? 2 : 4);
+    }
+    return;
+  }
+  if(blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rr = rr;
+    st->hist[it & 63] = rr;
+  }
+  const T beta = (T)(it > 0 ? f[1] / f[2] : 0.);
+  constexpr int RPB = TB / LPR;
   const int sub = threadIdx.x % LPR;
   double acc = 0.;
+  const int nChunks = (n + RPB - 1) / RPB;
+  const int stride = gridDim.x * UNR;
+  for(int c0 = feXcdBlock() * UNR; c0 < nChunks; c0 += stride) {
+    int row[UNR], kb[UNR], ke[UNR];
+    T sum[UNR];
+#pragma unroll
+    for(int u = 0; u < UNR; ++u) {
+      row[u] = (c0 + u) * RPB + threadIdx.x / LPR;
+      kb[u] = 0;
+      ke[u] = 0;
+      if(row[u] < n) {
+        kb[u] = ia[row[u]] + sub;
+        ke[u] = ia[row[u] + 1];
+      }
+      sum[u] = (T)0;
+    }
+    bool more = true;
+    while(more) {
+      more = false;
+#pragma unroll
+      for(int u = 0; u < UNR; ++u) {
+        if(kb[u] < ke[u]) {
+          const int j = ja[kb[u]];
+          sum[u] += val[kb[u]] * (z[j] + beta * pOld[j]);
+          kb[u] += LPR;
+          more |= kb[u] < ke[u];
+        }
+      }
+    }
+#pragma unroll
+    for(int u = 0; u < UNR; ++u) {
+#pragma unroll
+      for(int o = LPR / 2; o > 0; o >>= 1) sum[u] += __shfl_xor(sum[u], o, LPR);
+      if(sub == 0 && row[u] < n) {
+        const T pn = z[row[u]] + beta * pOld[row[u]];
+        pNew[row[u]] = pn;
+        q[row[u]] = sum[u];
+        acc += (double)pn * (double)sum[u];
+      }
+    }
+  }
+  feStoreBlockSum(acc, P.pq, lds);
+}
+
+// alpha = rz_it / (p.q); x += alpha p, r -= alpha q, z = D^-1 r; partials of
+// r.z and r.r into slot it+1.
+template <typename T, int TB>
+__global__ __launch_bounds__(TB) void cgUpdate(int n, CgVecs<T> v, const T *__restrict__ p, CgParts P,
+                                               CgState *st, int it)
+{
+  __shared__ double lds[TB / 64 + 1];
+  if(st->done) return;
+  const int s = it & 1;
+  double f[2];
+  const double *const parts[2] = {P.pq, P.rz[s]};
+  feFold<TB, 2>(parts, gridDim.x, f);
+  const double pq = f[0], rz = f[1];
+  if(!(pq > 0.)) {
+    if(blockIdx.x == 0 && threadIdx.x == 0) {
+      st->iter = it;
+      st->done = 3;  // breakdown: p.Ap <= 0, the matrix is not SPD
+    }
+    return;
+  }
+  const T alpha = (T)(rz / pq);
+  double a0 = 0., a1 = 0.;
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    v.x[i] += alpha * p[i];
+    const T ri = v.r[i] - alpha * v.q[i];
+    const T zi = v.dinv[i] * ri;
+    v.r[i] = ri;
+    v.z[i] = zi;
+    a0 += (double)ri * (double)zi;
+    a1 += (double)ri * (double)ri;
+  }
+  feStoreBlockSum(a0, P.rz[s ^ 1], lds);
+  feStoreBlockSum(a1, P.rr[s ^ 1], lds);
+}
+
+// ---------------------------------------------------------------------------
+// Pipelined Jacobi-PCG (Chronopoulos & Gear 1989): one fused kernel per
+// iteration and one reduction phase (gamma = r.u, delta = w.u, rho = r.r with
+// u = D^-1 r, w = A u).  Kernel i, with alpha_i / beta_i from the folded
+// partials of iteration i and the scalars of iteration i-1:
+//   p_i = u_i + beta_i p_{i-1},   s_i = w_i + beta_i s_{i-1}
+//   x_{i+1} = x_i + alpha_i p_i,  r_{i+1} = r_i - alpha_i s_i
+//   w_{i+1} = A u_{i+1}, u_{i+1} = D^-1 r_{i+1} recomputed on the fly for
+//   every gathered column from r_i, w_i, s_{i-1} (read-only this kernel).
+// r, w and s are double-buffered by iteration parity; x, p are own-row only.
+// beta_i = gamma_i / gamma_{i-1}, alpha_i = gamma_i / (delta_i - beta_i
+// gamma_i / alpha_{i-1}); the start writes gamma_{-1} = 1, alpha_{-1} = inf,
+// s_{-1} = p_{-1} = 0, so every iteration runs the same code.
+template <typename T>
+struct PipeVecs {
+  T *x, *p, *dinv, *r[2], *w[2], *s[2];
+};
+
+struct PipeParts {
+  double *g[2], *d[2], *rho[2];  // partials of gamma, delta, rho by parity
+};
+
+struct PipeScalars {
+  double gamma[2], alpha[2];  // by parity, written by block 0
+};
+
+// w_0 = A (D^-1 r_0); r_0 = b (or the given residual); x = 0 unless
+// restarting; s_{-1} = p_{-1} = 0; partials of iteration 0.
+template <typename T, int LPR, int TB>
+__global__ __launch_bounds__(TB) void pipeInit(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                               const T *__restrict__ val, const double *__restrict__ b,
+                                               PipeVecs<T> v, PipeParts P, PipeScalars *sc, CgState *st,
+                                               int fromResidual)
+{
+  __shared__ double lds[TB / 64 + 1];
+  // r_0 (slot 0) must be complete before the gather: when starting from b the
+  // gather reads b directly; when restarting, r was written by an earlier
+  // kernel.
+  constexpr int RPB = TB / LPR;
+  const int sub = threadIdx.x % LPR;
+  double g = 0., d = 0., rho = 0.;
+  const int nChunks = (n + RPB - 1) / RPB;
+  const T *r0 = v.r[0];
+  for(int chunk = feXcdBlock(); chunk < nChunks; chunk += gridDim.x) {
+    const int row = chunk * RPB + threadIdx.x / LPR;
+    T sum = (T)0;
+    if(row < n) {
+      const int e = ia[row + 1];
+      for(int k = ia[row] + sub; k < e; k += LPR) {
+        const int j = ja[k];
+        const T rj = fromResidual ? r0[j] : (T)b[j];
+        sum += val[k] * (v.dinv[j] * rj);
+      }
+    }
+#pragma unroll
+    for(int o = LPR / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, LPR);
+    if(sub == 0 && row < n) {
+      const T ri = fromResidual ? r0[row] : (T)b[row];
+      const T ui = v.dinv[row] * ri;
+      if(!fromResidual) {
+        v.r[0][row] = ri;
+        v.x[row] = (T)0;
+      }
+      v.w[0][row] = sum;
+      v.s[1][row] = (T)0;
+      v.p[row] = (T)0;
+      g += (double)ri * (double)ui;
+      d += (double)sum * (double)ui;
+      rho += (double)ri * (double)ri;
+    }
+  }
+  feStoreBlockSum(g, P.g[0], lds);
+  feStoreBlockSum(d, P.d[0], lds);
+  feStoreBlockSum(rho, P.rho[0], lds);
+  if(blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->gamma[1] = 1.;
+    sc->alpha[1] = __builtin_inf();
+    st->done = 0;
+    st->iter = 0;
+  }
+}
+
+template <typename T, int LPR, int TB>
+__global__ __launch_bounds__(TB) void pipeIter(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                               const T *__restrict__ val, PipeVecs<T> v, PipeParts P,
+                                               PipeScalars *sc, CgState *st, int it, int maxit, double tol2)
+{
+  __shared__ double lds[TB / 64 + 1];
+  if(st->done) return;
+  const int c = it & 1, o = c ^ 1;
+  double f[3];
+  const double *const parts[3] = {P.g[c], P.d[c], P.rho[c]};
+  feFold<TB, 3>(parts, gridDim.x, f);
+  const double gamma = f[0], delta = f[1], rho = f[2];
+  if(!(rho > tol2) || it >= maxit) {
+    if(blockIdx.x == 0 && threadIdx.x == 0) {
+      st->rr = rho;
+      st->iter = it;
+      st->done = (rho <= tol2) ? 1 : (rho == rho ? 2 : 4);
+    }
+    return;
+  }
+  const double betaD = gamma / sc->gamma[o];
+  const double den = delta - betaD * gamma / sc->alpha[o];
+  if(!(den > 0.)) {
+    if(blockIdx.x == 0 && threadIdx.x == 0) {
+      st->iter = it;
+      st->done = 3;  // breakdown: the preconditioned operator is not SPD
+    }
+    return;
+  }
+  const double alphaD = gamma / den;
+  if(blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->gamma[c] = gamma;
+    sc->alpha[c] = alphaD;
+    st->rr = rho;
+    st->hist[it & 63] = rho;
+  }
+  const T alpha = (T)alphaD, beta = (T)betaD;
+  const T *__restrict__ rC = v.r[c];
+  const T *__restrict__ wC = v.w[c];
+  const T *__restrict__ sO = v.s[o];
+  constexpr int RPB = TB / LPR;
+  const int sub = threadIdx.x % LPR;
+  double g = 0., d = 0., rh = 0.;
   const int nChunks = (n + RPB - 1) / RPB;
   for(int chunk = feXcdBlock(); chunk < nChunks; chunk += gridDim.x) {
     const int row = chunk * RPB + threadIdx.x / LPR;
@@ -115,73 +374,45 @@ __global__ __launch_bounds__(kTB) void cgSpmv(int n, const int *__restrict__ ia,
       const int e = ia[row + 1];
       for(int k = ia[row] + sub; k < e; k += LPR) {
         const int j = ja[k];
-        sum += val[k] * (z[j] + beta * pOld[j]);
+        const T uj = v.dinv[j] * (rC[j] - alpha * (wC[j] + beta * sO[j]));
+        sum += val[k] * uj;
       }
     }
 #pragma unroll
-    for(int o = LPR / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, LPR);
+    for(int q = LPR / 2; q > 0; q >>= 1) sum += __shfl_xor(sum, q, LPR);
     if(sub == 0 && row < n) {
-      const T pn = z[row] + beta * pOld[row];
-      pNew[row] = pn;
-      q[row] = sum;
-      acc += (double)pn * (double)sum;
+      const T di = v.dinv[row];
+      const T ri = rC[row];
+      const T pi = di * ri + beta * v.p[row];
+      const T si = wC[row] + beta * sO[row];
+      const T rn = ri - alpha * si;
+      const T un = di * rn;
+      v.x[row] += alpha * pi;
+      v.p[row] = pi;
+      v.s[c][row] = si;
+      v.r[o][row] = rn;
+      v.w[o][row] = sum;
+      g += (double)rn * (double)un;
+      d += (double)sum * (double)un;
+      rh += (double)rn * (double)rn;
     }
   }
-  double in[1] = {acc}, out[1];
-  if(feGridSum<1>(in, partials, counter, out, lds)) {
-    st->pq = out[0];
-    if(out[0] > 0.) {
-      st->alpha = st->rz / out[0];
-    } else {
-      st->alpha = 0.;
-      st->done = 3;  // breakdown: A p . p <= 0 (matrix not SPD)
-    }
-  }
+  feStoreBlockSum(g, P.g[o], lds);
+  feStoreBlockSum(d, P.d[o], lds);
+  feStoreBlockSum(rh, P.rho[o], lds);
 }
 
-template <typename T>
-__global__ __launch_bounds__(kTB) void cgUpdate(int n, CgVecs<T> v, const T *__restrict__ p,
-                                                CgState *st, double *partials, unsigned *counter,
-                                                double *hist)
-{
-  __shared__ double lds[kTB / 64];
-  if(st->done) return;
-  const T alpha = (T)st->alpha;
-  double acc[2] = {0., 0.};
-  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    v.x[i] += alpha * p[i];
-    const T ri = v.r[i] - alpha * v.q[i];
-    const T zi = v.dinv[i] * ri;
-    v.r[i] = ri;
-    v.z[i] = zi;
-    acc[0] += (double)ri * (double)zi;
-    acc[1] += (double)ri * (double)ri;
-  }
-  double out[2];
-  if(feGridSum<2>(acc, partials, counter, out, lds)) {
-    st->beta = out[0] / st->rz;
-    st->rz = out[0];
-    st->rr = out[1];
-    const int it = st->iter + 1;
-    st->iter = it;
-    if(hist) hist[it & 1023] = out[1];
-    if(out[1] <= st->tol2) st->done = 1;
-    else if(it >= st->maxit) st->done = 2;
-    else if(!(out[1] == out[1])) st->done = 4;  // NaN
-  }
-}
-
-// r = b - A x (FP64), returns ||r||^2 and ||r||_inf into res[0..1].
+// r = b - A x (FP64); partials of ||r||^2.
 template <int LPR>
 __global__ __launch_bounds__(kTB) void residual64(int n, const int *__restrict__ ia, const int *__restrict__ ja,
                                                   const double *__restrict__ val, const double *__restrict__ x,
                                                   const double *__restrict__ b, double *__restrict__ r,
-                                                  double *partials, unsigned *counter, double *res)
+                                                  double *__restrict__ part)
 {
-  __shared__ double lds[kTB / 64];
+  __shared__ double lds[kTB / 64 + 1];
   constexpr int RPB = kTB / LPR;
   const int sub = threadIdx.x % LPR;
-  double acc[2] = {0., 0.};
+  double acc = 0.;
   const int nChunks = (n + RPB - 1) / RPB;
   for(int chunk = feXcdBlock(); chunk < nChunks; chunk += gridDim.x) {
     const int row = chunk * RPB + threadIdx.x / LPR;
@@ -195,21 +426,21 @@ __global__ __launch_bounds__(kTB) void residual64(int n, const int *__restrict__
     if(sub == 0 && row < n) {
       const double ri = b[row] - sum;
       r[row] = ri;
-      acc[0] += ri * ri;
+      acc += ri * ri;
     }
   }
-  double out[2];
-  // second slot unused by the sum; the inf-norm is not needed on this path
-  if(feGridSum<2>(acc, partials, counter, out, lds)) {
-    res[0] = out[0];
-    res[1] = out[1];
-  }
+  feStoreBlockSum(acc, part, lds);
 }
 
-template <typename T>
-__global__ void castVec(int n, const double *__restrict__ a, T *__restrict__ b)
+// One block folds `count` partial arrays of G entries into out[0..count).
+__global__ __launch_bounds__(kTB) void finalizeSums(const double *__restrict__ part, int G, int count,
+                                                    double *__restrict__ out)
 {
-  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = (T)a[i];
+  __shared__ double lds[kTB / 64 + 1];
+  for(int k = 0; k < count; ++k) {
+    const double s = feFoldPartials(part + (size_t)k * G, G, lds);
+    if(threadIdx.x == 0) out[k] = s;
+  }
 }
 
 template <typename T>
@@ -218,27 +449,20 @@ __global__ void addVec(int n, double *__restrict__ x, const T *__restrict__ d)
   for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] += (double)d[i];
 }
 
-__global__ void normsKernel(int n, const double *__restrict__ a, double *partials, unsigned *counter,
-                            double *res)
+__global__ __launch_bounds__(kTB) void sumSquares(int n, const double *__restrict__ a, double *__restrict__ part)
 {
-  __shared__ double lds[kTB / 64];
-  double acc[2] = {0., 0.};
-  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    acc[0] += a[i] * a[i];
-  }
-  double out[2];
-  if(feGridSum<2>(acc, partials, counter, out, lds)) {
-    res[0] = out[0];
-    res[1] = out[1];
-  }
+  __shared__ double lds[kTB / 64 + 1];
+  double acc = 0.;
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += a[i] * a[i];
+  feStoreBlockSum(acc, part, lds);
 }
 
-// Infinity norms of du, b and A du - b (feLinearSystemPETSc.cpp:1004-1015).
-__global__ void infNorms(int n, const double *__restrict__ a, const double *__restrict__ b,
-                         const double *__restrict__ c, double *partials, unsigned *counter, double *res)
+// Per-block maxima of |a|, |b|, |c| (the infinity norms of du, b, A du - b,
+// feLinearSystemPETSc.cpp:1004-1015), folded by finalizeMax.
+__global__ __launch_bounds__(kTB) void blockMaxAbs3(int n, const double *__restrict__ a, const double *__restrict__ b,
+                                                    const double *__restrict__ c, double *__restrict__ part)
 {
   __shared__ double lds[3][kTB / 64];
-  __shared__ bool amLast;
   double m[3] = {0., 0., 0.};
   for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     m[0] = fmax(m[0], fabs(a[i]));
@@ -251,40 +475,42 @@ __global__ void infNorms(int n, const double *__restrict__ a, const double *__re
     if((threadIdx.x & 63) == 0) lds[k][threadIdx.x >> 6] = m[k];
   }
   __syncthreads();
-  if(threadIdx.x == 0) {
-    for(int k = 0; k < 3; ++k) {
-      double b2 = 0.;
-      for(int w = 0; w < kTB / 64; ++w) b2 = fmax(b2, lds[k][w]);
-      partials[(size_t)k * gridDim.x + blockIdx.x] = b2;
-    }
-    __threadfence();
-    amLast = atomicAdd(counter, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if(amLast && threadIdx.x == 0) {
-    __threadfence();
+  if(threadIdx.x == 0)
     for(int k = 0; k < 3; ++k) {
       double r = 0.;
-      for(int bb = 0; bb < (int)gridDim.x; ++bb) r = fmax(r, partials[(size_t)k * gridDim.x + bb]);
-      res[k] = r;
+      for(int w = 0; w < kTB / 64; ++w) r = fmax(r, lds[k][w]);
+      part[(size_t)k * gridDim.x + blockIdx.x] = r;
     }
-    *counter = 0u;
-  }
 }
 
-int gridFor(int n, int perBlock) { return std::max(1, std::min(kMaxGrid, (n + perBlock - 1) / perBlock)); }
+__global__ __launch_bounds__(64) void finalizeMax(const double *__restrict__ part, int G, int count,
+                                                   double *__restrict__ out)
+{
+  for(int k = 0; k < count; ++k) {
+    double r = 0.;
+    for(int b = threadIdx.x; b < G; b += 64) r = fmax(r, part[(size_t)k * G + b]);
+#pragma unroll
+    for(int o = 32; o > 0; o >>= 1) r = fmax(r, __shfl_xor(r, o, 64));
+    if(threadIdx.x == 0) out[k] = r;
+  }
+}
 
 struct Workspace {
   CgVecs<double> v64;
   CgVecs<float> v32;
-  double *x64, *r64, *tmp64, *hist, *res;
+  PipeVecs<double> q64;
+  PipeVecs<float> q32;
+  PipeParts pparts;
+  PipeScalars *sc;
+  CgParts parts;
+  double *x64, *r64, *tmp64, *res, *part;
   CgState *st;
 };
 
 int ensureWorkspace(fe_hip_ctx *ctx, Workspace &w)
 {
   const size_t n = ctx->n;
-  const size_t need = n * (8 * 8 + 8 * 4 + 3 * 8) + 1024 * 8 + 64 * 8 + 4096;
+  const size_t need = n * (8 * 8 + 8 * 4 + 3 * 8 + 9 * 8 + 9 * 4) + 24 * kMaxGrid * 8 + 64 * 8 + 8192 + 64 * 512;
   if(!ctx->d_work || ctx->workBytes < need) {
     if(ctx->d_work) feFree(ctx, ctx->d_work);
     char *p = nullptr;
@@ -298,9 +524,34 @@ int ensureWorkspace(fe_hip_ctx *ctx, Workspace &w)
     p += (bytes + 255) & ~(size_t)255;
     return q;
   };
-  w.st = (CgState *)take(sizeof(CgState) * 2);
+  w.st = (CgState *)take(sizeof(CgState));
   w.res = (double *)take(64 * 8);
-  w.hist = (double *)take(1024 * 8);
+  w.parts.rz[0] = (double *)take(kMaxGrid * 8);
+  w.parts.rz[1] = (double *)take(kMaxGrid * 8);
+  w.parts.rr[0] = (double *)take(kMaxGrid * 8);
+  w.parts.rr[1] = (double *)take(kMaxGrid * 8);
+  w.parts.pq = (double *)take(kMaxGrid * 8);
+  w.part = (double *)take(4 * kGrid * 8);
+  w.sc = (PipeScalars *)take(sizeof(PipeScalars));
+  for(int k = 0; k < 2; ++k) {
+    w.pparts.g[k] = (double *)take(kMaxGrid * 8);
+    w.pparts.d[k] = (double *)take(kMaxGrid * 8);
+    w.pparts.rho[k] = (double *)take(kMaxGrid * 8);
+  }
+  w.q64.x = (double *)take(n * 8);
+  w.q64.p = (double *)take(n * 8);
+  w.q64.dinv = (double *)take(n * 8);
+  w.q32.x = (float *)take(n * 4);
+  w.q32.p = (float *)take(n * 4);
+  w.q32.dinv = (float *)take(n * 4);
+  for(int k = 0; k < 2; ++k) {
+    w.q64.r[k] = (double *)take(n * 8);
+    w.q64.w[k] = (double *)take(n * 8);
+    w.q64.s[k] = (double *)take(n * 8);
+    w.q32.r[k] = (float *)take(n * 4);
+    w.q32.w[k] = (float *)take(n * 4);
+    w.q32.s[k] = (float *)take(n * 4);
+  }
   double **d64[] = {&w.v64.x, &w.v64.r, &w.v64.z, &w.v64.p[0], &w.v64.p[1], &w.v64.q, &w.v64.dinv};
   for(auto pp : d64) *pp = (double *)take(n * 8);
   float **d32[] = {&w.v32.x, &w.v32.r, &w.v32.z, &w.v32.p[0], &w.v32.p[1], &w.v32.q, &w.v32.dinv};
@@ -312,38 +563,68 @@ int ensureWorkspace(fe_hip_ctx *ctx, Workspace &w)
   return FE_STATUS_OK;
 }
 
-template <typename T>
-void launchSpmv(fe_hip_ctx *ctx, int lanes, int n, const T *val, const T *z, const T *pOld, T *pNew, T *q,
-                CgState *st)
+template <typename T, int TB>
+void launchSpmvTB(fe_hip_ctx *ctx, Workspace &w, const T *val, const T *z, const T *pOld, T *pNew, T *q, int it,
+                  int maxit, double tol2)
 {
-  const int rpb = kTB / lanes;
-  const int grid = gridFor(n, rpb);
-#define FE_SPMV_CASE(L)                                                                                       \
-  case L:                                                                                                     \
-    hipLaunchKernelGGL((cgSpmv<T, L>), dim3(grid), dim3(kTB), 0, ctx->stream, n, ctx->d_ia, ctx->d_ja, val, z, \
-                       pOld, pNew, q, st, ctx->d_partials, ctx->d_counter);                                   \
-    break;
-  switch(lanes) {
-    FE_SPMV_CASE(2)
-    FE_SPMV_CASE(4)
-    FE_SPMV_CASE(8)
-    FE_SPMV_CASE(16)
-    FE_SPMV_CASE(32)
-    default: FE_SPMV_CASE(64)
+  const int n = ctx->n, G = ctx->iterGrid;
+#define FE_SPMV_CASE(L, U)                                                                                    \
+  hipLaunchKernelGGL((cgSpmv<T, L, TB, U>), dim3(G), dim3(TB), 0, ctx->stream, n, ctx->d_ia, ctx->d_ja, val, \
+                     z, pOld, pNew, q, w.parts, w.st, it, maxit, tol2)
+  const int L = ctx->spmvLanes, U = ctx->spmvUnroll;
+  if(U == 2) {
+    switch(L) {
+      case 1: FE_SPMV_CASE(1, 2); break;
+      case 2: FE_SPMV_CASE(2, 2); break;
+      case 4: FE_SPMV_CASE(4, 2); break;
+      case 8: FE_SPMV_CASE(8, 2); break;
+      case 16: FE_SPMV_CASE(16, 2); break;
+      default: FE_SPMV_CASE(32, 2); break;
+    }
+  } else {
+    switch(L) {
+      case 1: FE_SPMV_CASE(1, 1); break;
+      case 2: FE_SPMV_CASE(2, 1); break;
+      case 4: FE_SPMV_CASE(4, 1); break;
+      case 8: FE_SPMV_CASE(8, 1); break;
+      case 16: FE_SPMV_CASE(16, 1); break;
+      default: FE_SPMV_CASE(32, 1); break;
+    }
   }
 #undef FE_SPMV_CASE
 }
 
-void launchResidual(fe_hip_ctx *ctx, const double *x, const double *b, double *r, double *res)
+template <typename T>
+void launchSpmv(fe_hip_ctx *ctx, Workspace &w, const T *val, const T *z, const T *pOld, T *pNew, T *q, int it,
+                int maxit, double tol2)
 {
-  const int lanes = ctx->spmvLanes;
-  const int grid = gridFor(ctx->n, kTB / lanes);
-#define FE_RES_CASE(L)                                                                                         \
-  case L:                                                                                                      \
-    hipLaunchKernelGGL((residual64<L>), dim3(grid), dim3(kTB), 0, ctx->stream, ctx->n, ctx->d_ia, ctx->d_ja,    \
-                       ctx->d_val, x, b, r, ctx->d_partials, ctx->d_counter + 1, res);                           \
+  if(ctx->iterBlock == 1024) launchSpmvTB<T, 1024>(ctx, w, val, z, pOld, pNew, q, it, maxit, tol2);
+  else if(ctx->iterBlock == 512) launchSpmvTB<T, 512>(ctx, w, val, z, pOld, pNew, q, it, maxit, tol2);
+  else launchSpmvTB<T, 256>(ctx, w, val, z, pOld, pNew, q, it, maxit, tol2);
+}
+
+template <typename T>
+void launchUpdate(fe_hip_ctx *ctx, Workspace &w, CgVecs<T> &v, const T *p, int it)
+{
+  const int G = ctx->iterGrid;
+  if(ctx->iterBlock == 1024)
+    hipLaunchKernelGGL((cgUpdate<T, 1024>), dim3(G), dim3(1024), 0, ctx->stream, ctx->n, v, p, w.parts, w.st, it);
+  else if(ctx->iterBlock == 512)
+    hipLaunchKernelGGL((cgUpdate<T, 512>), dim3(G), dim3(512), 0, ctx->stream, ctx->n, v, p, w.parts, w.st, it);
+  else
+    hipLaunchKernelGGL((cgUpdate<T, 256>), dim3(G), dim3(256), 0, ctx->stream, ctx->n, v, p, w.parts, w.st, it);
+}
+
+// r = b - A x in FP64; returns ||r||_2^2 (synchronous).
+int residualNorm2(fe_hip_ctx *ctx, Workspace &w, const double *x, const double *b, double *r, double *out)
+{
+#define FE_RES_CASE(L)                                                                                       \
+  case L:                                                                                                    \
+    hipLaunchKernelGGL((residual64<L>), dim3(kGrid), dim3(kTB), 0, ctx->stream, ctx->n, ctx->d_ia, ctx->d_ja, \
+                       ctx->d_val, x, b, r, w.part);                                                         \
     break;
-  switch(lanes) {
+  switch(ctx->spmvLanes) {
+    FE_RES_CASE(1)
     FE_RES_CASE(2)
     FE_RES_CASE(4)
     FE_RES_CASE(8)
@@ -352,39 +633,41 @@ void launchResidual(fe_hip_ctx *ctx, const double *x, const double *b, double *r
     default: FE_RES_CASE(64)
   }
 #undef FE_RES_CASE
+  hipLaunchKernelGGL(finalizeSums, dim3(1), dim3(kTB), 0, ctx->stream, w.part, kGrid, 1, w.res);
+  FE_HIP_TRY(ctx, hipGetLastError());
+  FE_HIP_TRY(ctx, hipMemcpyAsync(out, w.res, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return FE_STATUS_OK;
 }
 
-// Runs CG iterations until the recursive residual meets tol2, the iteration
-// budget is spent, or breakdown.  Returns the state's done code through *done.
+// Launches CG iterations [0, maxit) in chunks of checkEvery, polling the state
+// after each chunk.  *iters: iterations completed, *done: stop code.
 template <typename T>
-int runCg(fe_hip_ctx *ctx, Workspace &w, CgVecs<T> &v, const T *val, int checkEvery, int *itersOut,
-          int *done)
+int runCg(fe_hip_ctx *ctx, Workspace &w, CgVecs<T> &v, const T *val, int checkEvery, int maxit, double tol2,
+          int *iters, int *done)
 {
-  const int n = ctx->n;
-  const int gridV = gridFor(n, kTB);
   CgState *hs = (CgState *)ctx->h_state;
-  int parity = 0;
+  FeKernelTimer &tm = ctx->spmvTimer;
+  tm.used = 0;
+  int it = 0;
   for(;;) {
-    for(int k = 0; k < checkEvery; ++k) {
+    for(int k = 0; k < checkEvery; ++k, ++it) {
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if(ctx->profiling) {
-        FeKernelTimer &tm = ctx->spmvTimer;
-        if(tm.used + 2 > tm.ev.size()) {
+        if(tm.used + 2 > tm.ev.size())
           for(int a = 0; a < 64; ++a) {
             hipEvent_t e;
             FE_HIP_TRY(ctx, hipEventCreate(&e));
             tm.ev.push_back(e);
           }
-        }
         e0 = tm.ev[tm.used++];
         e1 = tm.ev[tm.used++];
         FE_HIP_TRY(ctx, hipEventRecord(e0, ctx->stream));
       }
-      launchSpmv<T>(ctx, ctx->spmvLanes, n, val, v.z, v.p[parity], v.p[parity ^ 1], v.q, w.st);
+      const int par = it & 1;
+      launchSpmv<T>(ctx, w, val, v.z, v.p[par], v.p[par ^ 1], v.q, it, maxit, tol2);
       if(ctx->profiling) FE_HIP_TRY(ctx, hipEventRecord(e1, ctx->stream));
-      hipLaunchKernelGGL((cgUpdate<T>), dim3(gridV), dim3(kTB), 0, ctx->stream, n, v, v.p[parity ^ 1], w.st,
-                         ctx->d_partials, ctx->d_counter + 2, w.hist);
-      parity ^= 1;
+      launchUpdate<T>(ctx, w, v, v.p[par ^ 1], it);
     }
     FE_HIP_TRY(ctx, hipGetLastError());
     FE_HIP_TRY(ctx, hipMemcpyAsync(hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
@@ -392,11 +675,8 @@ int runCg(fe_hip_ctx *ctx, Workspace &w, CgVecs<T> &v, const T *val, int checkEv
     if(hs->done) break;
   }
   if(ctx->profiling) {
-    // Only launches that did work are counted (the early-exit launches after
-    // convergence are skipped by clipping to the iteration count).
-    FeKernelTimer &tm = ctx->spmvTimer;
-    const size_t pairs = tm.used / 2;
-    const size_t valid = std::min<size_t>(pairs, (size_t)std::max(0, hs->iter - *itersOut));
+    // launches after the stop did no work: count the first hs->iter ones
+    const size_t valid = std::min<size_t>(tm.used / 2, (size_t)std::max(0, hs->iter));
     for(size_t pi = 0; pi < valid; ++pi) {
       float ms = 0.f;
       FE_HIP_TRY(ctx, hipEventElapsedTime(&ms, tm.ev[2 * pi], tm.ev[2 * pi + 1]));
@@ -405,20 +685,106 @@ int runCg(fe_hip_ctx *ctx, Workspace &w, CgVecs<T> &v, const T *val, int checkEv
     }
     tm.used = 0;
   }
-  *itersOut = hs->iter;
+  *iters = hs->iter;
   *done = hs->done;
   return FE_STATUS_OK;
 }
 
 template <typename T>
-int cgStart(fe_hip_ctx *ctx, Workspace &w, CgVecs<T> &v, const double *b, double tol2, int maxit,
-            int fromResidual)
+int cgStart(fe_hip_ctx *ctx, Workspace &w, CgVecs<T> &v, const double *b, int fromResidual)
 {
-  const int gridV = gridFor(ctx->n, kTB);
-  hipLaunchKernelGGL((cgInit<T>), dim3(gridV), dim3(kTB), 0, ctx->stream, ctx->n, b, v, w.st, tol2, maxit,
-                     ctx->d_partials, ctx->d_counter + 3, fromResidual);
+  const int G = ctx->iterGrid;
+  if(ctx->iterBlock == 1024)
+    hipLaunchKernelGGL((cgInit<T, 1024>), dim3(G), dim3(1024), 0, ctx->stream, ctx->n, b, v, w.parts, w.st, fromResidual);
+  else if(ctx->iterBlock == 512)
+    hipLaunchKernelGGL((cgInit<T, 512>), dim3(G), dim3(512), 0, ctx->stream, ctx->n, b, v, w.parts, w.st, fromResidual);
+  else
+    hipLaunchKernelGGL((cgInit<T, 256>), dim3(G), dim3(256), 0, ctx->stream, ctx->n, b, v, w.parts, w.st, fromResidual);
   FE_HIP_TRY(ctx, hipGetLastError());
   return FE_STATUS_OK;
+}
+
+
+template <typename T, int TB>
+void launchPipeTB(fe_hip_ctx *ctx, Workspace &w, PipeVecs<T> &v, const T *val, int it, int maxit, double tol2,
+                  bool init, const double *b, int fromResidual)
+{
+  const int n = ctx->n, G = ctx->iterGrid;
+#define FE_PIPE_CASE(L)                                                                                           \
+  if(init)                                                                                                        \
+    hipLaunchKernelGGL((pipeInit<T, L, TB>), dim3(G), dim3(TB), 0, ctx->stream, n, ctx->d_ia, ctx->d_ja, val, b, v, \
+                       w.pparts, w.sc, w.st, fromResidual);                                                       \
+  else                                                                                                            \
+    hipLaunchKernelGGL((pipeIter<T, L, TB>), dim3(G), dim3(TB), 0, ctx->stream, n, ctx->d_ia, ctx->d_ja, val, v,    \
+                       w.pparts, w.sc, w.st, it, maxit, tol2);
+  switch(ctx->spmvLanes) {
+    case 1: FE_PIPE_CASE(1) break;
+    case 2: FE_PIPE_CASE(2) break;
+    case 4: FE_PIPE_CASE(4) break;
+    case 8: FE_PIPE_CASE(8) break;
+    case 16: FE_PIPE_CASE(16) break;
+    default: FE_PIPE_CASE(32) break;
+  }
+#undef FE_PIPE_CASE
+}
+
+template <typename T>
+void launchPipe(fe_hip_ctx *ctx, Workspace &w, PipeVecs<T> &v, const T *val, int it, int maxit, double tol2,
+                bool init = false, const double *b = nullptr, int fromResidual = 0)
+{
+  if(ctx->iterBlock == 1024) launchPipeTB<T, 1024>(ctx, w, v, val, it, maxit, tol2, init, b, fromResidual);
+  else if(ctx->iterBlock == 512) launchPipeTB<T, 512>(ctx, w, v, val, it, maxit, tol2, init, b, fromResidual);
+  else launchPipeTB<T, 256>(ctx, w, v, val, it, maxit, tol2, init, b, fromResidual);
+}
+
+// Pipelined Jacobi-PCG iterations until the stop test fires; profiling times
+// every iteration kernel (the dominant kernel of the solve).
+template <typename T>
+int runPipe(fe_hip_ctx *ctx, Workspace &w, PipeVecs<T> &v, const T *val, int checkEvery, int maxit, double tol2,
+            int *iters, int *done)
+{
+  CgState *hs = (CgState *)ctx->h_state;
+  FeKernelTimer &tm = ctx->spmvTimer;
+  tm.used = 0;
+  int it = 0;
+  for(;;) {
+    for(int k = 0; k < checkEvery; ++k, ++it) {
+      if(ctx->profiling) {
+        if(tm.used + 2 > tm.ev.size())
+          for(int a = 0; a < 64; ++a) {
+            hipEvent_t e;
+            FE_HIP_TRY(ctx, hipEventCreate(&e));
+            tm.ev.push_back(e);
+          }
+        FE_HIP_TRY(ctx, hipEventRecord(tm.ev[tm.used++], ctx->stream));
+      }
+      launchPipe<T>(ctx, w, v, val, it, maxit, tol2);
+      if(ctx->profiling) FE_HIP_TRY(ctx, hipEventRecord(tm.ev[tm.used++], ctx->stream));
+    }
+    FE_HIP_TRY(ctx, hipGetLastError());
+    FE_HIP_TRY(ctx, hipMemcpyAsync(hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
+    FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if(hs->done) break;
+  }
+  if(ctx->profiling) {
+    const size_t valid = std::min<size_t>(tm.used / 2, (size_t)std::max(0, hs->iter));
+    for(size_t pi = 0; pi < valid; ++pi) {
+      float ms = 0.f;
+      FE_HIP_TRY(ctx, hipEventElapsedTime(&ms, tm.ev[2 * pi], tm.ev[2 * pi + 1]));
+      tm.total_ms += ms;
+      tm.launches++;
+    }
+    tm.used = 0;
+  }
+  *iters = hs->iter;
+  *done = hs->done;
+  return FE_STATUS_OK;
+}
+
+template <typename T>
+__global__ void castResidual(int n, const double *__restrict__ a, T *__restrict__ b)
+{
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = (T)a[i];
 }
 
 }  // namespace
@@ -430,7 +796,6 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
   Workspace w;
   if(int rc = ensureWorkspace(ctx, w)) return rc;
   const int n = ctx->n;
-  const int gridV = gridFor(n, kTB);
   const int checkEvery = std::max(1, o->check_every);
   hipEvent_t t0, t1;
   FE_HIP_TRY(ctx, hipEventCreate(&t0));
@@ -438,11 +803,9 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
   FE_HIP_TRY(ctx, hipEventRecord(t0, ctx->stream));
   ctx->spmvTimer.total_ms = 0.;
   ctx->spmvTimer.launches = 0;
-  ctx->spmvTimer.used = 0;
 
-  // ||b||^2
-  hipLaunchKernelGGL(normsKernel, dim3(gridV), dim3(kTB), 0, ctx->stream, n, ctx->d_rhs, ctx->d_partials,
-                     ctx->d_counter + 4, w.res + 8);
+  hipLaunchKernelGGL(sumSquares, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, ctx->d_rhs, w.part);
+  hipLaunchKernelGGL(finalizeSums, dim3(1), dim3(kTB), 0, ctx->stream, w.part, kGrid, 1, w.res + 8);
   FE_HIP_TRY(ctx, hipGetLastError());
   double bb = 0.;
   FE_HIP_TRY(ctx, hipMemcpyAsync(&bb, w.res + 8, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -459,31 +822,33 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
     relres = 0.;
   } else if(o->precision == FE_PREC_FP64) {
     hipLaunchKernelGGL(extractDinv<double>, dim3((n + kTB - 1) / kTB), dim3(kTB), 0, ctx->stream, n, ctx->d_diag,
-                       ctx->d_val, w.v64.dinv);
-    double tol2 = target * target * 0.25;  // recursive-residual target (true residual checked below)
-    if(int rc = cgStart<double>(ctx, w, w.v64, ctx->d_rhs, tol2, o->max_iter, 0)) return rc;
+                       ctx->d_val, w.q64.dinv);
+    // recursive-residual target; the FP64 true residual is checked after
+    double tol2 = target * target * 0.25;
+    launchPipe<double>(ctx, w, w.q64, ctx->d_val, 0, 0, 0., true, ctx->d_rhs, 0);
     for(;;) {
-      if(int rc = runCg<double>(ctx, w, w.v64, ctx->d_val, checkEvery, &iters, &done)) return rc;
-      launchResidual(ctx, w.v64.x, ctx->d_rhs, w.v64.r, w.res);
-      double res[2];
-      FE_HIP_TRY(ctx, hipMemcpyAsync(res, w.res, sizeof(res), hipMemcpyDeviceToHost, ctx->stream));
-      FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-      relres = std::sqrt(res[0]) / bn;
+      int it = 0;
+      if(int rc = runPipe<double>(ctx, w, w.q64, ctx->d_val, checkEvery, o->max_iter - iters, tol2, &it, &done))
+        return rc;
+      iters += it;
+      double rr = 0.;
+      if(int rc = residualNorm2(ctx, w, w.q64.x, ctx->d_rhs, w.q64.r[0], &rr)) return rc;
+      relres = std::sqrt(rr) / bn;
       outer++;
-      if(std::sqrt(res[0]) <= target) { converged = 1; reason = 2; break; }
-      if(done == 3) { reason = -3; break; }  // indefinite / breakdown
-      if(done == 4) { reason = -9; break; }  // NaN
+      if(std::sqrt(rr) <= target) { converged = 1; reason = 2; break; }
+      if(done == 3) { reason = -3; break; }
+      if(done == 4 || !(rr == rr)) { reason = -9; break; }
       if(iters >= o->max_iter) { reason = -3; break; }
       if(relres > o->dtol) { reason = -4; break; }
-      // restart from the true residual (r already holds b - A x)
+      // restart from the true residual (r slot 0 holds b - A x)
       tol2 *= 0.01;
-      if(int rc = cgStart<double>(ctx, w, w.v64, ctx->d_rhs, tol2, o->max_iter, 1)) return rc;
+      launchPipe<double>(ctx, w, w.q64, ctx->d_val, 0, 0, 0., true, ctx->d_rhs, 1);
     }
-    FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_du, w.v64.x, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_du, w.q64.x, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
   } else {
     FE_REQUIRE(ctx, ctx->d_val32, "assemble with FE_PREC_FP32_IR to create the FP32 values");
     hipLaunchKernelGGL(extractDinv<float>, dim3((n + kTB - 1) / kTB), dim3(kTB), 0, ctx->stream, n, ctx->d_diag,
-                       ctx->d_val, w.v32.dinv);
+                       ctx->d_val, w.q32.dinv);
     FE_HIP_TRY(ctx, hipMemsetAsync(w.x64, 0, sizeof(double) * n, ctx->stream));
     FE_HIP_TRY(ctx, hipMemcpyAsync(w.r64, ctx->d_rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
     double rn = bn;
@@ -492,20 +857,17 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
       relres = rn / bn;
       if(rn <= target) { converged = 1; reason = 2; break; }
       if(iters >= o->max_iter) { reason = -3; break; }
-      // inner FP32 CG on A32 d = r
       double tol2 = (o->inner_rtol * rn) * (o->inner_rtol * rn);
       tol2 = std::max(tol2, target * target * 0.01);
-      hipLaunchKernelGGL(castVec<float>, dim3(gridV), dim3(kTB), 0, ctx->stream, n, w.r64, w.v32.r);
-      if(int rc = cgStart<float>(ctx, w, w.v32, w.r64, tol2, o->max_iter - iters, 0)) return rc;
+      launchPipe<float>(ctx, w, w.q32, ctx->d_val32, 0, 0, 0., true, w.r64, 0);
       int it0 = 0;
-      if(int rc = runCg<float>(ctx, w, w.v32, ctx->d_val32, checkEvery, &it0, &done)) return rc;
+      if(int rc = runPipe<float>(ctx, w, w.q32, ctx->d_val32, checkEvery, o->max_iter - iters, tol2, &it0, &done))
+        return rc;
       iters += it0;
-      hipLaunchKernelGGL(addVec<float>, dim3(gridV), dim3(kTB), 0, ctx->stream, n, w.x64, w.v32.x);
-      launchResidual(ctx, w.x64, ctx->d_rhs, w.r64, w.res);
-      double res[2];
-      FE_HIP_TRY(ctx, hipMemcpyAsync(res, w.res, sizeof(res), hipMemcpyDeviceToHost, ctx->stream));
-      FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-      rn = std::sqrt(res[0]);
+      hipLaunchKernelGGL(addVec<float>, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, w.x64, w.q32.x);
+      double rr = 0.;
+      if(int rc = residualNorm2(ctx, w, w.x64, ctx->d_rhs, w.r64, &rr)) return rc;
+      rn = std::sqrt(rr);
       outer++;
       if(done == 3 || done == 4) { reason = -3; break; }
       if(rn / bn > o->dtol) { reason = -4; break; }
@@ -523,10 +885,11 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
 
   if(stats) {
     memset(stats, 0, sizeof(*stats));
-    // ||du||_inf, ||b||_inf, ||A du - b||_inf
-    launchResidual(ctx, ctx->d_du, ctx->d_rhs, w.tmp64, w.res);
-    hipLaunchKernelGGL(infNorms, dim3(gridV), dim3(kTB), 0, ctx->stream, n, ctx->d_du, ctx->d_rhs, w.tmp64,
-                       ctx->d_partials, ctx->d_counter + 5, w.res + 16);
+    double dummy = 0.;
+    if(int rc = residualNorm2(ctx, w, ctx->d_du, ctx->d_rhs, w.tmp64, &dummy)) return rc;
+    hipLaunchKernelGGL(blockMaxAbs3, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, ctx->d_du, ctx->d_rhs, w.tmp64,
+                       w.part);
+    hipLaunchKernelGGL(finalizeMax, dim3(1), dim3(64), 0, ctx->stream, w.part, kGrid, 3, w.res + 16);
     FE_HIP_TRY(ctx, hipGetLastError());
     double inf[3];
     FE_HIP_TRY(ctx, hipMemcpyAsync(inf, w.res + 16, sizeof(inf), hipMemcpyDeviceToHost, ctx->stream));

@@ -79,3 +79,42 @@ def solve_newton(S: A.LinearSystemHIP, opts: NLSolverOptions = NLSolverOptions()
             break
     rep.iterations = it
     return rep
+
+
+def kovasznay(Re):
+    """Kovasznay flow: lambda = Re/2 - sqrt(Re^2/4 + 4 pi^2), u = 1 - e^{lx}
+    cos 2 pi y, v = l/(2 pi) e^{lx} sin 2 pi y, p = (1 - e^{2 l x}) / 2."""
+    lam = Re / 2.0 - np.sqrt(Re * Re / 4.0 + 4.0 * np.pi ** 2)
+
+    def u(x, y, z, comp):
+        ex = np.exp(lam * x)
+        return np.where(comp == 0, 1.0 - ex * np.cos(2 * np.pi * y), lam / (2 * np.pi) * ex * np.sin(2 * np.pi * y))
+
+    def p(x, y, z, comp):
+        return 0.5 * (1.0 - np.exp(2.0 * lam * x))
+
+    return lam, u, p
+
+
+def navier_stokes_system(mesh: A.Mesh, Re: float, degree: int = 5, coloring=None, device=0, source=None):
+    """Steady incompressible Navier-Stokes, Taylor-Hood P2-P1, divergence
+    formulation (the reference's navier_stokes.cpp:66-99 form stack):
+    convective acceleration (coeff -1), mixed divergence {p, u} (1), vector
+    source, divergence of the Newtonian stress {u, p} (1, mu = 1/Re).  u and p
+    are essential on the boundary with the Kovasznay values; the interior
+    starts from zero."""
+    num = A.Numbering(mesh, [A.TRI_VP2, A.TRI_P1], [1, 1])
+    S = A.LinearSystemHIP(device)
+    S.upload_mesh(mesh, coloring)
+    fu = S.add_field(A.TRI_VP2, num.adr[0])
+    fp = S.add_field(A.TRI_P1, num.adr[1])
+    S.set_sizes(num.nInc, num.nDofs)
+    S.add_form(A.VECTOR_CONVECTIVE_ACCELERATION, [fu], degree, [A.Fn(A.FN_CONST, -1.0)])
+    S.add_form(A.MIXED_DIVERGENCE, [fp, fu], degree, [A.Fn(A.FN_CONST, 1.0)])
+    src = source or [A.Fn(A.FN_CONST, 0.0), A.Fn(A.FN_CONST, 0.0)]
+    S.add_form(A.VECTOR_SOURCE, [fu], degree, src)
+    S.add_form(A.DIV_NEWTONIAN_STRESS, [fu, fp], degree, [A.Fn(A.FN_CONST, 1.0), A.Fn(A.FN_CONST, 1.0 / Re)])
+    S.build_csr()
+    _, ufn, pfn = kovasznay(Re)
+    S.set_solution(A.essential_values(num, [ufn, pfn]))
+    return S, num

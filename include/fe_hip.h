@@ -184,6 +184,14 @@ int fe_hip_kernel_stats(fe_hip_ctx *ctx, int which, double *avg_ms, int64_t *lau
  * launch). */
 int fe_hip_set_profiling(fe_hip_ctx *ctx, int on);
 
+/* Launch shapes of the Krylov iteration kernels (no reference counterpart:
+ * the reference delegates the solve to PETSc).  Keys: "spmv_lanes" (lanes
+ * per CSR row: 4, 8, 16, 32), "spmv_unroll" (row groups in flight per lane
+ * group: 1, 2), "iter_block" (256, 512, 1024), "iter_grid" (blocks, a
+ * multiple of 8 up to 4096). */
+int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value);
+int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value);
+
 #ifdef __cplusplus
 }
 #endif

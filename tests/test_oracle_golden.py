@@ -59,3 +59,23 @@ def test_coloring_is_a_valid_vertex_coloring(oracle):
     for c in range(nc):
         verts = conn[col == c].ravel()
         assert np.unique(verts).size == verts.size
+
+
+@pytest.mark.parametrize("name", NS_CASES)
+def test_navier_stokes_assembly_matches_reference(oracle, name):
+    """Taylor-Hood P2-P1 with the Kovasznay boundary data (Re = 40): the four
+    NS forms (convective, divergence, source, Newtonian stress) assembled by
+    the oracle against the reference's matrix and residual."""
+    import ctypes as C
+    from tests.oracle_bind import P
+    g = load(name)
+    n = g["n"]
+    val = np.zeros(g["ja"].size)
+    rhs = np.zeros(n)
+    rc = oracle.L.or_assemble_ns(g["xyz"].size // 3, P(g["xyz"], C.c_double), g["conn"].size // 3,
+                                 P(g["conn"], C.c_int), P(g["adr0"], C.c_int), P(g["adr1"], C.c_int), n,
+                                 P(g["ia"], C.c_int), P(g["ja"], C.c_int), g["deg"], 1.0 / float(g["extra"]),
+                                 P(g["u0"], C.c_double), None, P(val, C.c_double), P(rhs, C.c_double))
+    assert rc == 0
+    assert relerr(val, g["val"]) <= 1e-12
+    assert relerr(rhs, g["rhs"]) <= 1e-12
