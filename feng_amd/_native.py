@@ -84,6 +84,8 @@ HIP_API = {
     "fe_hip_kernel_stats": (C.c_int, [C.c_void_p, C.c_int, c_double_p, C.POINTER(C.c_int64)]),
     "fe_hip_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "fe_hip_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+    "fe_hip_pc_info": (C.c_int, [C.c_void_p, c_int_p, c_int_p]),
+    "fe_hip_get_factor": (C.c_int, [C.c_void_p, c_double_p]),
     "fe_hip_get_tuning": (C.c_int, [C.c_void_p, C.c_char_p, c_int_p]),
 }
 

@@ -292,6 +292,16 @@ class LinearSystemHIP:
         self._c(self.L.fe_hip_kernel_stats(self.ctx, which, C.byref(ms), C.byref(cnt)), "fe_hip_kernel_stats")
         return ms.value, cnt.value
 
+    def pc_info(self):
+        a, b = C.c_int(), C.c_int()
+        self._c(self.L.fe_hip_pc_info(self.ctx, C.byref(a), C.byref(b)), "fe_hip_pc_info")
+        return a.value, b.value
+
+    def get_factor(self):
+        lu = np.zeros(self.nnz)
+        self._c(self.L.fe_hip_get_factor(self.ctx, N.ptr(lu, C.c_double)), "fe_hip_get_factor")
+        return lu
+
     def set_tuning(self, key, value):
         self._c(self.L.fe_hip_set_tuning(self.ctx, key.encode(), int(value)), "fe_hip_set_tuning")
 

@@ -87,6 +87,22 @@ struct FeKernelTimer {
   int64_t launches = 0;
 };
 
+// Preconditioner state: ILU(0)/IC(0) factor on A's pattern, row levels, and
+// the split factors G (lower CSR, unit diagonal) / G^T (upper CSR) / D^-1.
+struct FePrecond {
+  bool patternReady = false, factored = false;
+  std::vector<int> levelPtr;
+  int *d_levelRows = nullptr, *d_iaL = nullptr, *d_jaL = nullptr, *d_iaU = nullptr, *d_jaU = nullptr;
+  int *d_nShift = nullptr;
+  int64_t nnzL = 0, nnzU = 0;
+  double *d_lu = nullptr, *d_gL64 = nullptr, *d_gU64 = nullptr, *d_dinv64 = nullptr, *d_dval = nullptr;
+  float *d_gL32 = nullptr, *d_gU32 = nullptr, *d_dinv32 = nullptr;
+  int *d_tposL = nullptr;  // for each lower entry of G, its position in the upper CSR of G^T
+  int lastShifts = 0;
+  double lastSetupMs = 0.;
+  int sweeps = 2;  // Jacobi sweeps per triangular factor in GMRES/ILU(0)
+};
+
 struct fe_hip_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -134,6 +150,8 @@ struct fe_hip_ctx {
   void *d_state = nullptr;
   void *h_state = nullptr;  // pinned mirror
 
+  FePrecond pc;
+
   // measurement
   bool profiling = false;
   FeKernelTimer spmvTimer;
@@ -171,6 +189,16 @@ void feFree(fe_hip_ctx *ctx, void *p);
 // Exclusive scan of n int32 counts into out (n+1 entries, out[n] = total).
 // Returns the total through *total (host), or an error when it overflows int32.
 int feExclusiveScan(fe_hip_ctx *ctx, const int *d_in, int *d_out, int n, int64_t *total);
+
+// Preconditioner set-up (fe_precond.hip)
+int fePrecondSetupPattern(fe_hip_ctx *ctx);
+int fePrecondFactor(fe_hip_ctx *ctx);
+
+// Learned correction of the split factor (fe_learned.hip)
+int feLearnedCorrect(fe_hip_ctx *ctx);
+
+// GMRES (fe_gmres.hip)
+int feSolveGmres(fe_hip_ctx *ctx, const fe_hip_solver_opts *opts, double *du, fe_hip_solve_stats *stats);
 
 // Krylov entry (fe_krylov.hip)
 int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *opts, double *du,

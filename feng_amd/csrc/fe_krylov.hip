@@ -241,6 +241,129 @@ __global__ __launch_bounds__(TB) void cgUpdate(int n, CgVecs<T> v, const T *__re
   feStoreBlockSum(a1, P.rr[s ^ 1], lds);
 }
 
+
+// ---------------------------------------------------------------------------
+// PCG with the split factored preconditioner M^-1 = G^T D^-1 G (G unit lower
+// on the IC(0) pattern): three kernels per iteration, every vector update
+// fused into one of them and every scalar folded from block partials.
+//   cgSpmv  : beta, p_new = z + beta p_old, q = A p_new, partials of p.q
+//   applyG  : alpha = rz / p.q, x += alpha p, r_new = r - alpha q,
+//             y = G r_new (r_new gathered on the fly from r, q), partials r.r
+//   applyGT : z = G^T D^-1 y, partials of r_new . z
+template <typename T>
+struct SplitVecs {
+  T *x, *r[2], *p[2], *q, *y, *z;
+  const T *gL, *gU, *dinv;
+  const int *iaL, *jaL, *iaU, *jaU;
+};
+
+template <typename T, int LPR, int TB>
+__global__ __launch_bounds__(TB) void splitInit(int n, const double *__restrict__ b, SplitVecs<T> v, CgParts P,
+                                                CgState *st)
+{
+  __shared__ double lds[TB / 64 + 1];
+  constexpr int RPB = TB / LPR;
+  const int sub = threadIdx.x % LPR;
+  double rr = 0.;
+  const int nChunks = (n + RPB - 1) / RPB;
+  for(int chunk = feXcdBlock(); chunk < nChunks; chunk += gridDim.x) {
+    const int row = chunk * RPB + threadIdx.x / LPR;
+    T sum = (T)0;
+    if(row < n)
+      for(int k = v.iaL[row] + sub; k < v.iaL[row + 1]; k += LPR) sum += v.gL[k] * (T)b[v.jaL[k]];
+#pragma unroll
+    for(int o = LPR / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, LPR);
+    if(sub == 0 && row < n) {
+      const T bi = (T)b[row];
+      v.r[0][row] = bi;
+      v.x[row] = (T)0;
+      v.p[0][row] = (T)0;
+      v.p[1][row] = (T)0;
+      v.y[row] = sum;
+      rr += (double)bi * (double)bi;
+    }
+  }
+  feStoreBlockSum(rr, P.rr[0], lds);
+  if(threadIdx.x == 0) P.rz[1][blockIdx.x] = (blockIdx.x == 0) ? 1. : 0.;
+  if(blockIdx.x == 0 && threadIdx.x == 0) {
+    st->done = 0;
+    st->iter = 0;
+  }
+}
+
+template <typename T, int LPR, int TB>
+__global__ __launch_bounds__(TB) void splitApplyG(int n, SplitVecs<T> v, CgParts P, CgState *st, int it)
+{
+  __shared__ double lds[TB / 64 + 1];
+  if(st->done) return;
+  const int c = it & 1, o = c ^ 1;
+  double f[2];
+  const double *const parts[2] = {P.pq, P.rz[c]};
+  feFold<TB, 2>(parts, gridDim.x, f);
+  if(!(f[0] > 0.)) {
+    if(blockIdx.x == 0 && threadIdx.x == 0) {
+      st->iter = it;
+      st->done = 3;
+    }
+    return;
+  }
+  const T alpha = (T)(f[1] / f[0]);
+  const T *__restrict__ rc = v.r[c];
+  const T *__restrict__ pn = v.p[o];  // p_new written by cgSpmv of this iteration
+  constexpr int RPB = TB / LPR;
+  const int sub = threadIdx.x % LPR;
+  double rr = 0.;
+  const int nChunks = (n + RPB - 1) / RPB;
+  for(int chunk = feXcdBlock(); chunk < nChunks; chunk += gridDim.x) {
+    const int row = chunk * RPB + threadIdx.x / LPR;
+    T sum = (T)0;
+    if(row < n)
+      for(int k = v.iaL[row] + sub; k < v.iaL[row + 1]; k += LPR) {
+        const int j = v.jaL[k];
+        sum += v.gL[k] * (rc[j] - alpha * v.q[j]);
+      }
+#pragma unroll
+    for(int q = LPR / 2; q > 0; q >>= 1) sum += __shfl_xor(sum, q, LPR);
+    if(sub == 0 && row < n) {
+      const T rn = rc[row] - alpha * v.q[row];
+      v.x[row] += alpha * pn[row];
+      v.r[o][row] = rn;
+      v.y[row] = sum;
+      rr += (double)rn * (double)rn;
+    }
+  }
+  feStoreBlockSum(rr, P.rr[o], lds);
+}
+
+// z = G^T D^-1 y; partials of r[rs] . z into rz[slot].
+template <typename T, int LPR, int TB>
+__global__ __launch_bounds__(TB) void splitApplyGT(int n, SplitVecs<T> v, CgParts P, CgState *st, int rs, int slot,
+                                                   int checkDone)
+{
+  __shared__ double lds[TB / 64 + 1];
+  if(checkDone && st->done) return;
+  constexpr int RPB = TB / LPR;
+  const int sub = threadIdx.x % LPR;
+  double rz = 0.;
+  const int nChunks = (n + RPB - 1) / RPB;
+  for(int chunk = feXcdBlock(); chunk < nChunks; chunk += gridDim.x) {
+    const int row = chunk * RPB + threadIdx.x / LPR;
+    T sum = (T)0;
+    if(row < n)
+      for(int k = v.iaU[row] + sub; k < v.iaU[row + 1]; k += LPR) {
+        const int j = v.jaU[k];
+        sum += v.gU[k] * (v.dinv[j] * v.y[j]);
+      }
+#pragma unroll
+    for(int q = LPR / 2; q > 0; q >>= 1) sum += __shfl_xor(sum, q, LPR);
+    if(sub == 0 && row < n) {
+      v.z[row] = sum;
+      rz += (double)v.r[rs][row] * (double)sum;
+    }
+  }
+  feStoreBlockSum(rz, P.rz[slot], lds);
+}
+
 // ---------------------------------------------------------------------------
 // Pipelined Jacobi-PCG (Chronopoulos & Gear 1989): one fused kernel per
 // iteration and one reduction phase (gamma = r.u, delta = w.u, rho = r.r with
@@ -502,6 +625,8 @@ struct Workspace {
   PipeVecs<float> q32;
   PipeParts pparts;
   PipeScalars *sc;
+  SplitVecs<double> s64;
+  SplitVecs<float> s32;
   CgParts parts;
   double *x64, *r64, *tmp64, *res, *part;
   CgState *st;
@@ -510,7 +635,8 @@ struct Workspace {
 int ensureWorkspace(fe_hip_ctx *ctx, Workspace &w)
 {
   const size_t n = ctx->n;
-  const size_t need = n * (8 * 8 + 8 * 4 + 3 * 8 + 9 * 8 + 9 * 4) + 24 * kMaxGrid * 8 + 64 * 8 + 8192 + 64 * 512;
+  const size_t need = n * (8 * 8 + 8 * 4 + 3 * 8 + 9 * 8 + 9 * 4 + 8 * 8 + 8 * 4) + 24 * kMaxGrid * 8 + 64 * 8 + 8192 +
+                      64 * 768;
   if(!ctx->d_work || ctx->workBytes < need) {
     if(ctx->d_work) feFree(ctx, ctx->d_work);
     char *p = nullptr;
@@ -556,6 +682,12 @@ int ensureWorkspace(fe_hip_ctx *ctx, Workspace &w)
   for(auto pp : d64) *pp = (double *)take(n * 8);
   float **d32[] = {&w.v32.x, &w.v32.r, &w.v32.z, &w.v32.p[0], &w.v32.p[1], &w.v32.q, &w.v32.dinv};
   for(auto pp : d32) *pp = (float *)take(n * 4);
+  {
+    double **a[] = {&w.s64.x, &w.s64.r[0], &w.s64.r[1], &w.s64.p[0], &w.s64.p[1], &w.s64.q, &w.s64.y, &w.s64.z};
+    for(auto pp : a) *pp = (double *)take(n * 8);
+    float **b[] = {&w.s32.x, &w.s32.r[0], &w.s32.r[1], &w.s32.p[0], &w.s32.p[1], &w.s32.q, &w.s32.y, &w.s32.z};
+    for(auto pp : b) *pp = (float *)take(n * 4);
+  }
   w.x64 = (double *)take(n * 8);
   w.r64 = (double *)take(n * 8);
   w.tmp64 = (double *)take(n * 8);
@@ -781,6 +913,89 @@ int runPipe(fe_hip_ctx *ctx, Workspace &w, PipeVecs<T> &v, const T *val, int che
   return FE_STATUS_OK;
 }
 
+
+template <typename T, int TB>
+void launchSplitTB(fe_hip_ctx *ctx, Workspace &w, SplitVecs<T> &v, int kind, int it, const double *b, int rs, int slot,
+                   int checkDone)
+{
+  const int n = ctx->n, G = ctx->iterGrid;
+#define FE_SPLIT_CASE(L)                                                                                          \
+  if(kind == 0)                                                                                                   \
+    hipLaunchKernelGGL((splitInit<T, L, TB>), dim3(G), dim3(TB), 0, ctx->stream, n, b, v, w.parts, w.st);          \
+  else if(kind == 1)                                                                                              \
+    hipLaunchKernelGGL((splitApplyG<T, L, TB>), dim3(G), dim3(TB), 0, ctx->stream, n, v, w.parts, w.st, it);       \
+  else                                                                                                            \
+    hipLaunchKernelGGL((splitApplyGT<T, L, TB>), dim3(G), dim3(TB), 0, ctx->stream, n, v, w.parts, w.st, rs, slot, \
+                       checkDone);
+  // the factors have about half the row length of A
+  const int L = std::max(1, ctx->spmvLanes / 2);
+  switch(L) {
+    case 1: FE_SPLIT_CASE(1) break;
+    case 2: FE_SPLIT_CASE(2) break;
+    case 4: FE_SPLIT_CASE(4) break;
+    case 8: FE_SPLIT_CASE(8) break;
+    default: FE_SPLIT_CASE(16) break;
+  }
+#undef FE_SPLIT_CASE
+}
+
+template <typename T>
+void launchSplit(fe_hip_ctx *ctx, Workspace &w, SplitVecs<T> &v, int kind, int it, const double *b = nullptr,
+                 int rs = 0, int slot = 0, int checkDone = 1)
+{
+  if(ctx->iterBlock == 1024) launchSplitTB<T, 1024>(ctx, w, v, kind, it, b, rs, slot, checkDone);
+  else if(ctx->iterBlock == 512) launchSplitTB<T, 512>(ctx, w, v, kind, it, b, rs, slot, checkDone);
+  else launchSplitTB<T, 256>(ctx, w, v, kind, it, b, rs, slot, checkDone);
+}
+
+// Split-preconditioned PCG from x = 0 on A x = b (b in FP64).
+template <typename T>
+int runSplit(fe_hip_ctx *ctx, Workspace &w, SplitVecs<T> &v, const T *val, const double *b, int checkEvery,
+             int maxit, double tol2, int *iters, int *done)
+{
+  CgState *hs = (CgState *)ctx->h_state;
+  FeKernelTimer &tm = ctx->spmvTimer;
+  tm.used = 0;
+  launchSplit<T>(ctx, w, v, 0, 0, b);
+  launchSplit<T>(ctx, w, v, 2, 0, nullptr, 0, 0, 0);
+  int it = 0;
+  for(;;) {
+    for(int k = 0; k < checkEvery; ++k, ++it) {
+      const int c = it & 1;
+      if(ctx->profiling) {
+        if(tm.used + 2 > tm.ev.size())
+          for(int a = 0; a < 64; ++a) {
+            hipEvent_t e;
+            FE_HIP_TRY(ctx, hipEventCreate(&e));
+            tm.ev.push_back(e);
+          }
+        FE_HIP_TRY(ctx, hipEventRecord(tm.ev[tm.used++], ctx->stream));
+      }
+      launchSpmv<T>(ctx, w, val, v.z, v.p[c], v.p[c ^ 1], v.q, it, maxit, tol2);
+      if(ctx->profiling) FE_HIP_TRY(ctx, hipEventRecord(tm.ev[tm.used++], ctx->stream));
+      launchSplit<T>(ctx, w, v, 1, it);
+      launchSplit<T>(ctx, w, v, 2, it, nullptr, c ^ 1, c ^ 1, 1);
+    }
+    FE_HIP_TRY(ctx, hipGetLastError());
+    FE_HIP_TRY(ctx, hipMemcpyAsync(hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
+    FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if(hs->done) break;
+  }
+  if(ctx->profiling) {
+    const size_t valid = std::min<size_t>(tm.used / 2, (size_t)std::max(0, hs->iter));
+    for(size_t pi = 0; pi < valid; ++pi) {
+      float ms = 0.f;
+      FE_HIP_TRY(ctx, hipEventElapsedTime(&ms, tm.ev[2 * pi], tm.ev[2 * pi + 1]));
+      tm.total_ms += ms;
+      tm.launches++;
+    }
+    tm.used = 0;
+  }
+  *iters = hs->iter;
+  *done = hs->done;
+  return FE_STATUS_OK;
+}
+
 template <typename T>
 __global__ void castResidual(int n, const double *__restrict__ a, T *__restrict__ b)
 {
@@ -791,8 +1006,11 @@ __global__ void castResidual(int n, const double *__restrict__ a, T *__restrict_
 
 int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_solve_stats *stats)
 {
-  FE_REQUIRE(ctx, o->ksp == FE_KSP_CG, "only CG is available on this path");
-  FE_REQUIRE(ctx, o->pc == FE_PC_JACOBI, "only the Jacobi preconditioner is available on this path");
+  if(o->ksp == FE_KSP_GMRES) return feSolveGmres(ctx, o, du, stats);
+  FE_REQUIRE(ctx, o->ksp == FE_KSP_CG, "unknown Krylov method");
+  FE_REQUIRE(ctx, o->pc == FE_PC_JACOBI || o->pc == FE_PC_IC0 || o->pc == FE_PC_LEARNED,
+             "CG takes the JACOBI, IC0 or LEARNED preconditioner");
+  const bool split = (o->pc != FE_PC_JACOBI);
   Workspace w;
   if(int rc = ensureWorkspace(ctx, w)) return rc;
   const int n = ctx->n;
@@ -815,11 +1033,68 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
   int iters = 0, outer = 0, converged = 0, reason = 0, done = 0;
   double relres = bn > 0. ? 1. : 0.;
 
+  double setupMs = 0.;
+  if(split && bn != 0.) {
+    hipEvent_t s0, s1;
+    FE_HIP_TRY(ctx, hipEventCreate(&s0));
+    FE_HIP_TRY(ctx, hipEventCreate(&s1));
+    FE_HIP_TRY(ctx, hipEventRecord(s0, ctx->stream));
+    if(int rc = fePrecondFactor(ctx)) return rc;
+    if(o->pc == FE_PC_LEARNED)
+      if(int rc = feLearnedCorrect(ctx)) return rc;
+    FE_HIP_TRY(ctx, hipEventRecord(s1, ctx->stream));
+    FE_HIP_TRY(ctx, hipEventSynchronize(s1));
+    float sm = 0.f;
+    FE_HIP_TRY(ctx, hipEventElapsedTime(&sm, s0, s1));
+    setupMs = sm;
+    (void)hipEventDestroy(s0);
+    (void)hipEventDestroy(s1);
+    FePrecond &P = ctx->pc;
+    w.s64.gL = P.d_gL64; w.s64.gU = P.d_gU64; w.s64.dinv = P.d_dinv64;
+    w.s32.gL = P.d_gL32; w.s32.gU = P.d_gU32; w.s32.dinv = P.d_dinv32;
+    w.s64.iaL = w.s32.iaL = P.d_iaL; w.s64.jaL = w.s32.jaL = P.d_jaL;
+    w.s64.iaU = w.s32.iaU = P.d_iaU; w.s64.jaU = w.s32.jaU = P.d_jaU;
+  }
+
   if(bn == 0.) {
     FE_HIP_TRY(ctx, hipMemsetAsync(ctx->d_du, 0, sizeof(double) * n, ctx->stream));
     converged = 1;
     reason = 2;
     relres = 0.;
+  } else if(split) {
+    // split-preconditioned PCG: FP64 directly, or FP32 inner + FP64 refinement
+    const bool fp32 = (o->precision == FE_PREC_FP32_IR);
+    FE_REQUIRE(ctx, !fp32 || ctx->d_val32, "assemble with FE_PREC_FP32_IR to create the FP32 values");
+    FE_HIP_TRY(ctx, hipMemsetAsync(w.x64, 0, sizeof(double) * n, ctx->stream));
+    FE_HIP_TRY(ctx, hipMemcpyAsync(w.r64, ctx->d_rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    double rn = bn;
+    while(outer < 60) {
+      relres = rn / bn;
+      if(rn <= target) { converged = 1; reason = 2; break; }
+      if(iters >= o->max_iter) { reason = -3; break; }
+      double tol2 = fp32 ? (o->inner_rtol * rn) * (o->inner_rtol * rn) : target * target * 0.25;
+      tol2 = std::max(tol2, target * target * 0.01);
+      int it0 = 0;
+      if(fp32) {
+        if(int rc = runSplit<float>(ctx, w, w.s32, ctx->d_val32, w.r64, checkEvery, o->max_iter - iters, tol2, &it0, &done))
+          return rc;
+        hipLaunchKernelGGL(addVec<float>, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, w.x64, w.s32.x);
+      } else {
+        if(int rc = runSplit<double>(ctx, w, w.s64, ctx->d_val, w.r64, checkEvery, o->max_iter - iters, tol2, &it0, &done))
+          return rc;
+        hipLaunchKernelGGL(addVec<double>, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, w.x64, w.s64.x);
+      }
+      iters += it0;
+      double rr = 0.;
+      if(int rc = residualNorm2(ctx, w, w.x64, ctx->d_rhs, w.r64, &rr)) return rc;
+      rn = std::sqrt(rr);
+      outer++;
+      if(done == 3 || done == 4) { reason = -3; break; }
+      if(rn / bn > o->dtol) { reason = -4; break; }
+    }
+    relres = rn / bn;
+    if(!converged && rn <= target) { converged = 1; reason = 2; }
+    FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_du, w.x64, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
   } else if(o->precision == FE_PREC_FP64) {
     hipLaunchKernelGGL(extractDinv<double>, dim3((n + kTB - 1) / kTB), dim3(kTB), 0, ctx->stream, n, ctx->d_diag,
                        ctx->d_val, w.q64.dinv);
@@ -903,7 +1178,7 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
     stats->norm_rhs_inf = inf[1];
     stats->norm_axb_inf = inf[2];
     stats->ms_solve = ms;
-    stats->ms_setup = 0.;
+    stats->ms_setup = setupMs;
   }
   if(du) {
     FE_HIP_TRY(ctx, hipMemcpyAsync(du, ctx->d_du, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));

@@ -1,0 +1,234 @@
+// Preconditioners of the device Krylov layer.
+//
+// ILU(0) / IC(0) factorization (the PC_ILU / PC_ICC the reference selects
+// through PETSc options, feLinearSystemPETSc.cpp:337-352): exact IKJ
+// incomplete factorization on the CSR pattern of A, row-level scheduled: the
+// host computes the dependency level of every row (level(i) = 1 + max level
+// of the columns j < i of row i) once per pattern; the device factors one
+// level per launch, one wavefront per row, with the row held in LDS.
+//
+// Split factored approximate inverse on the IC(0) pattern:
+//   M^-1 = G^T D^-1 G,  G unit lower triangular on the lower pattern of A.
+// From IC(0) (A ~ L D L^T): G = I - L_strict (the first Neumann term of
+// L^-1), D = the IC(0) pivots.  G (lower CSR) and G^T (upper CSR, with the
+// transpose map of the symmetric pattern) are stored explicitly so the
+// learned correction (G + dG) uses the same apply kernels.
+#include "fe_hip_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace {
+
+constexpr int kRowCap = 256;  // max row length held in LDS by the factor kernel
+
+// One wavefront per row of the level: IKJ ILU(0) of row i, in place in lu.
+__global__ __launch_bounds__(64) void iluFactorLevel(const int *__restrict__ rows, int count,
+                                                     const int *__restrict__ ia, const int *__restrict__ ja,
+                                                     const int *__restrict__ diag, double *__restrict__ lu,
+                                                     double shiftRel, int *__restrict__ nShift)
+{
+  __shared__ int cols[kRowCap];
+  __shared__ double vals[kRowCap];
+  if((int)blockIdx.x >= count) return;
+  const int i = rows[blockIdx.x];
+  const int b = ia[i], e = ia[i + 1], len = e - b;
+  const int lane = threadIdx.x;
+  if(len > kRowCap) {
+    if(lane == 0) atomicAdd(nShift, 1 << 20);  // flagged: row too long
+    return;
+  }
+  double rowAbs = 0.;
+  for(int t = lane; t < len; t += 64) {
+    cols[t] = ja[b + t];
+    vals[t] = lu[b + t];
+    rowAbs = fmax(rowAbs, fabs(vals[t]));
+  }
+#pragma unroll
+  for(int o = 32; o > 0; o >>= 1) rowAbs = fmax(rowAbs, __shfl_xor(rowAbs, o, 64));
+  __syncthreads();
+  for(int t = 0; t < len; ++t) {
+    const int k = cols[t];
+    if(k >= i) break;
+    const double piv = lu[diag[k]];
+    const double l = vals[t] / piv;
+    __syncthreads();
+    if(lane == 0) vals[t] = l;
+    // row k beyond its diagonal: subtract l * u_kj where (i, j) is in the pattern
+    const int kb = diag[k] + 1, ke = ia[k + 1];
+    for(int q = kb + lane; q < ke; q += 64) {
+      const int j = ja[q];
+      int lo = t + 1, hi = len - 1;
+      while(lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int c = cols[mid];
+        if(c == j) {
+          vals[mid] -= l * lu[q];
+          break;
+        }
+        if(c < j) lo = mid + 1; else hi = mid - 1;
+      }
+    }
+    __syncthreads();
+  }
+  // zero pivot (saddle-point rows): shift it, relative to the row scale
+  for(int t = lane; t < len; t += 64) {
+    if(cols[t] == i && fabs(vals[t]) <= 1e-14 * rowAbs) {
+      vals[t] = shiftRel * (rowAbs > 0. ? rowAbs : 1.);
+      atomicAdd(nShift, 1);
+    }
+  }
+  __syncthreads();
+  for(int t = lane; t < len; t += 64) lu[b + t] = vals[t];
+}
+
+// Row counts of the lower (j <= i) and upper (j >= i) parts.
+__global__ void countParts(int n, const int *__restrict__ ia, const int *__restrict__ ja, const int *__restrict__ diag,
+                           int *__restrict__ cL, int *__restrict__ cU)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  cL[i] = diag[i] - ia[i] + 1;
+  cU[i] = ia[i + 1] - diag[i];
+}
+
+// Lower part with the unit diagonal and values g_ij = -l_ij (G = I - L_strict);
+// upper part G^T with values -u_ij / u_ii and unit diagonal; D = u_ii.
+template <typename T>
+__global__ void buildSplitFactor(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                 const int *__restrict__ diag, const double *__restrict__ lu,
+                                 const int *__restrict__ iaL, int *__restrict__ jaL, T *__restrict__ gL,
+                                 const int *__restrict__ iaU, int *__restrict__ jaU, T *__restrict__ gU,
+                                 T *__restrict__ dinv, double *__restrict__ dval)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const int d = diag[i];
+  const double di = lu[d];
+  int o = iaL[i];
+  for(int p = ia[i]; p < d; ++p) {
+    jaL[o] = ja[p];
+    gL[o++] = (T)(-lu[p]);
+  }
+  jaL[o] = i;
+  gL[o] = (T)1;
+  o = iaU[i];
+  jaU[o] = i;
+  gU[o++] = (T)1;
+  for(int p = d + 1; p < ia[i + 1]; ++p) {
+    jaU[o] = ja[p];
+    gU[o++] = (T)(-lu[p] / di);
+  }
+  dinv[i] = (T)(1. / di);
+  dval[i] = di;
+}
+
+}  // namespace
+
+// Host: dependency levels of the lower-triangular DAG of the pattern.
+int feComputeLevels(fe_hip_ctx *ctx, std::vector<int> &levelPtr, std::vector<int> &levelRows)
+{
+  const int n = ctx->n;
+  std::vector<int> ia(n + 1), ja(ctx->nnz);
+  FE_HIP_TRY(ctx, hipMemcpy(ia.data(), ctx->d_ia, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+  FE_HIP_TRY(ctx, hipMemcpy(ja.data(), ctx->d_ja, sizeof(int) * ctx->nnz, hipMemcpyDeviceToHost));
+  std::vector<int> lev(n, 0);
+  int nl = 0;
+  for(int i = 0; i < n; ++i) {
+    int l = 0;
+    for(int p = ia[i]; p < ia[i + 1]; ++p) {
+      const int j = ja[p];
+      if(j >= i) break;
+      l = std::max(l, lev[j] + 1);
+    }
+    lev[i] = l;
+    nl = std::max(nl, l + 1);
+  }
+  levelPtr.assign(nl + 1, 0);
+  for(int i = 0; i < n; ++i) levelPtr[lev[i] + 1]++;
+  for(int l = 0; l < nl; ++l) levelPtr[l + 1] += levelPtr[l];
+  std::vector<int> pos(levelPtr.begin(), levelPtr.end() - 1);
+  levelRows.assign(n, 0);
+  for(int i = 0; i < n; ++i) levelRows[pos[lev[i]]++] = i;
+  return FE_STATUS_OK;
+}
+
+int fePrecondSetupPattern(fe_hip_ctx *ctx)
+{
+  FePrecond &P = ctx->pc;
+  if(P.patternReady) return FE_STATUS_OK;
+  std::vector<int> lrows;
+  if(int rc = feComputeLevels(ctx, P.levelPtr, lrows)) return rc;
+  const int n = ctx->n;
+  if(feAlloc(ctx, &P.d_levelRows, n) || feAlloc(ctx, &P.d_lu, (size_t)ctx->nnz) || feAlloc(ctx, &P.d_iaL, n + 1) ||
+     feAlloc(ctx, &P.d_iaU, n + 1) || feAlloc(ctx, &P.d_nShift, 4))
+    return FE_STATUS_OUT_OF_MEMORY;
+  FE_HIP_TRY(ctx, hipMemcpy(P.d_levelRows, lrows.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  int *cL = nullptr, *cU = nullptr;
+  if(feAlloc(ctx, &cL, n) || feAlloc(ctx, &cU, n)) return FE_STATUS_OUT_OF_MEMORY;
+  hipLaunchKernelGGL(countParts, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, ctx->d_ia, ctx->d_ja, ctx->d_diag,
+                     cL, cU);
+  int64_t nnzL = 0, nnzU = 0;
+  if(int rc = feExclusiveScan(ctx, cL, P.d_iaL, n, &nnzL)) return rc;
+  if(int rc = feExclusiveScan(ctx, cU, P.d_iaU, n, &nnzU)) return rc;
+  feFree(ctx, cL);
+  feFree(ctx, cU);
+  P.nnzL = nnzL;
+  P.nnzU = nnzU;
+  if(feAlloc(ctx, &P.d_jaL, nnzL) || feAlloc(ctx, &P.d_jaU, nnzU) || feAlloc(ctx, &P.d_gL64, nnzL) ||
+     feAlloc(ctx, &P.d_gU64, nnzU) || feAlloc(ctx, &P.d_gL32, nnzL) || feAlloc(ctx, &P.d_gU32, nnzU) ||
+     feAlloc(ctx, &P.d_dinv64, n) || feAlloc(ctx, &P.d_dinv32, n) || feAlloc(ctx, &P.d_dval, n))
+    return FE_STATUS_OUT_OF_MEMORY;
+  P.patternReady = true;
+  return FE_STATUS_OK;
+}
+
+// Exact ILU(0) of the current FP64 values into P.d_lu (level by level), then
+// the split factors G, G^T and D^-1 in both precisions.
+int fePrecondFactor(fe_hip_ctx *ctx)
+{
+  if(int rc = fePrecondSetupPattern(ctx)) return rc;
+  FePrecond &P = ctx->pc;
+  const int n = ctx->n;
+  FE_HIP_TRY(ctx, hipMemcpyAsync(P.d_lu, ctx->d_val, sizeof(double) * ctx->nnz, hipMemcpyDeviceToDevice, ctx->stream));
+  FE_HIP_TRY(ctx, hipMemsetAsync(P.d_nShift, 0, sizeof(int), ctx->stream));
+  const int nl = (int)P.levelPtr.size() - 1;
+  for(int l = 0; l < nl; ++l) {
+    const int b = P.levelPtr[l], cnt = P.levelPtr[l + 1] - b;
+    hipLaunchKernelGGL(iluFactorLevel, dim3(cnt), dim3(64), 0, ctx->stream, P.d_levelRows + b, cnt, ctx->d_ia,
+                       ctx->d_ja, ctx->d_diag, P.d_lu, 1e-8, P.d_nShift);
+  }
+  FE_HIP_TRY(ctx, hipGetLastError());
+  const int g = (n + 255) / 256;
+  hipLaunchKernelGGL(buildSplitFactor<double>, dim3(g), dim3(256), 0, ctx->stream, n, ctx->d_ia, ctx->d_ja,
+                     ctx->d_diag, P.d_lu, P.d_iaL, P.d_jaL, P.d_gL64, P.d_iaU, P.d_jaU, P.d_gU64, P.d_dinv64,
+                     P.d_dval);
+  hipLaunchKernelGGL(buildSplitFactor<float>, dim3(g), dim3(256), 0, ctx->stream, n, ctx->d_ia, ctx->d_ja,
+                     ctx->d_diag, P.d_lu, P.d_iaL, P.d_jaL, P.d_gL32, P.d_iaU, P.d_jaU, P.d_gU32, P.d_dinv32,
+                     P.d_dval);
+  FE_HIP_TRY(ctx, hipGetLastError());
+  int shifts = 0;
+  FE_HIP_TRY(ctx, hipMemcpyAsync(&shifts, P.d_nShift, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  FE_REQUIRE(ctx, shifts < (1 << 20), "ILU(0): a row is longer than the 256 entries held in LDS");
+  P.lastShifts = shifts;
+  P.factored = true;
+  return FE_STATUS_OK;
+}
+
+extern "C" int fe_hip_pc_info(fe_hip_ctx *ctx, int *levels, int *shifts)
+{
+  if(!ctx) return FE_STATUS_POINTER_ERROR;
+  if(levels) *levels = ctx->pc.levelPtr.empty() ? 0 : (int)ctx->pc.levelPtr.size() - 1;
+  if(shifts) *shifts = ctx->pc.lastShifts;
+  return FE_STATUS_OK;
+}
+
+extern "C" int fe_hip_get_factor(fe_hip_ctx *ctx, double *lu)
+{
+  if(!ctx || !lu) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, ctx->pc.factored, "no factorization yet");
+  FE_HIP_TRY(ctx, hipMemcpy(lu, ctx->pc.d_lu, sizeof(double) * ctx->nnz, hipMemcpyDeviceToHost));
+  return FE_STATUS_OK;
+}

@@ -175,6 +175,13 @@ int fe_hip_solve(fe_hip_ctx *ctx, const fe_hip_solver_opts *opts, double *du,
 /* correctSolution (feLinearSystemPETSc.cpp:1033-1060): u[i] += du[i], i < n. */
 int fe_hip_correct_solution(fe_hip_ctx *ctx);
 
+/* ILU(0)/IC(0) factor diagnostics: number of row levels of the factorization
+ * schedule and zero pivots shifted in the last factorization. */
+int fe_hip_pc_info(fe_hip_ctx *ctx, int *levels, int *shifts);
+/* Copy back the last ILU(0) factor (L strict lower, unit diagonal implied,
+ * and U, stored on the CSR pattern of A: nnz doubles). */
+int fe_hip_get_factor(fe_hip_ctx *ctx, double *lu);
+
 /* ---- measurement -------------------------------------------------------- */
 
 /* Average device time (ms, hipEvents on the context stream) of the last
