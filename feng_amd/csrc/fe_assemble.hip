@@ -402,6 +402,185 @@ __global__ __launch_bounds__(256) void asmNavierStokesFP64(NsAsmArgs a)
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Diffusion (+ source) in GEMM form on the matrix cores.  On a straight
+// simplex the local matrix is Ae_ij = sum_{q,t} c_{q,t}(e) S_{q,t,ij} with
+// c_{q,t} = k(x_q) J M_t (M_t the unique entries of the inverse-Jacobian
+// metric) and S the reference tensors (weights folded in).  A wavefront
+// takes 16 elements: A = C (16 elements x K), B = S (K x 16 entries per
+// tile), one v_mfma_*_16x16x4 per 4 columns of K and per 16-entry tile.
+// The lane holding D[i][j] in register v differs between the two
+// instructions: v_mfma_f32_16x16x4f32 has i = 4 (lane/16) + v, while
+// v_mfma_f64_16x16x4f64 has i = lane/16 + 4 v (j = lane % 16 for both).  FP64 (value parity with the reference loop <= 1e-12) or FP32
+// (element matrices for the FP32 inner solve).
+typedef double feDouble4 __attribute__((ext_vector_type(4)));
+typedef float feFloat4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct FeMfma;
+template <>
+struct FeMfma<double> {
+  using V4 = feDouble4;
+  static __device__ __forceinline__ int row(int quad, int v) { return quad + 4 * v; }
+  static __device__ __forceinline__ V4 mma(double x, double y, V4 c)
+  {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
+  }
+};
+template <>
+struct FeMfma<float> {
+  using V4 = feFloat4;
+  static __device__ __forceinline__ int row(int quad, int v) { return 4 * quad + v; }
+  static __device__ __forceinline__ V4 mma(float x, float y, V4 c)
+  {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, c, 0, 0, 0);
+  }
+};
+
+struct MfmaAsmArgs {
+  int begin, count, E, n, what, K;
+  const int *perm, *conn, *adr, *pos;
+  const double *xyz, *tab, *u;
+  const void *sref;
+  FeFn coeff, src;
+  int hasSrc;
+  void *val;
+  double *rhs;
+};
+
+template <typename T, int NF, int DIM>
+__global__ __launch_bounds__(256) void asmDiffusionMfma(MfmaAsmArgs a)
+{
+  constexpr int NV = DIM + 1, NE = NF * NF, NT = (NE + 15) / 16, NTERM = DIM == 2 ? 3 : 6;
+  __shared__ FeRefTables Tb;
+  __shared__ T aeLds[4][16][NE];
+  {
+    const double *src = a.tab;
+    double *dst = (double *)&Tb;
+    for(int i = threadIdx.x; i < (int)(sizeof(FeRefTables) / sizeof(double)); i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int elemBase = (blockIdx.x * 4 + wave) * 16;
+  const int le = lane & 15, kq = lane >> 4;
+  const bool valid = elemBase + le < a.count;
+  const int ee = a.begin + (valid ? elemBase + le : 0);
+  const int e = a.perm[ee];
+  double c[NV][3];
+#pragma unroll
+  for(int j = 0; j < NV; ++j) {
+    const int v = a.conn[(size_t)e * NV + j];
+#pragma unroll
+    for(int d = 0; d < 3; ++d) c[j][d] = a.xyz[3 * (size_t)v + d];
+  }
+  double J, drdx[3], drdy[3], drdz[3];
+  feGeometry<DIM>(c, J, drdx, drdy, drdz);
+  double M[6];
+  {
+    const double g[3][3] = {{drdx[0], drdy[0], drdz[0]}, {drdx[1], drdy[1], drdz[1]}, {drdx[2], drdy[2], drdz[2]}};
+    const int ta[6] = {0, 0, DIM == 2 ? 1 : 0, 1, 1, 2}, tb[6] = {0, 1, DIM == 2 ? 1 : 2, 1, 2, 2};
+#pragma unroll
+    for(int t = 0; t < NTERM; ++t) {
+      double s = 0.;
+#pragma unroll
+      for(int d = 0; d < DIM; ++d) s += g[ta[t]][d] * g[tb[t]][d];
+      M[t] = s;
+    }
+  }
+  const int nq = Tb.nq;
+  const T *sref = (const T *)a.sref;
+  typename FeMfma<T>::V4 acc[NT];
+#pragma unroll
+  for(int t = 0; t < NT; ++t) acc[t] = (typename FeMfma<T>::V4){0, 0, 0, 0};
+  for(int s = 0; s < a.K / 4; ++s) {
+    const int kk = 4 * s + kq;
+    const int q = kk / NTERM, tt = kk - q * NTERM;
+    T av = (T)0;
+    if(valid && q < nq) {
+      double x[3] = {0., 0., 0.};
+#pragma unroll
+      for(int j = 0; j < NV; ++j)
+#pragma unroll
+        for(int d = 0; d < 3; ++d) x[d] += c[j][d] * Tb.Lg[q][j];
+      av = (T)(feEvalFn(a.coeff, x, e, nq, q) * J * M[tt]);
+    }
+#pragma unroll
+    for(int t = 0; t < NT; ++t) {
+      const T bv = sref[(size_t)kk * 16 * NT + t * 16 + le];
+      acc[t] = FeMfma<T>::mma(av, bv, acc[t]);
+    }
+  }
+  // scatter the matrix: lane holds entry (t*16 + le) of elements row(kq, v)
+  if(a.what & 1) {
+    T *val = (T *)a.val;
+#pragma unroll
+    for(int t = 0; t < NT; ++t) {
+      const int entry = t * 16 + le;
+      if(entry >= NE) continue;
+#pragma unroll
+      for(int v = 0; v < 4; ++v) {
+        const int el = elemBase + FeMfma<T>::row(kq, v);
+        if(el >= a.count) continue;
+        const int p = a.pos[(size_t)entry * a.E + a.begin + el];
+        if(p >= 0) val[p] += acc[t][v];
+      }
+    }
+  }
+  if(!(a.what & 2)) return;
+  // residual Be = -Ae u_e - source, through LDS
+#pragma unroll
+  for(int t = 0; t < NT; ++t) {
+    const int entry = t * 16 + le;
+    if(entry < NE)
+#pragma unroll
+      for(int v = 0; v < 4; ++v) aeLds[wave][FeMfma<T>::row(kq, v)][entry] = acc[t][v];
+  }
+  __syncthreads();
+  for(int idx = lane; idx < 16 * NF; idx += 64) {
+    const int el = idx / NF, i = idx - el * NF;
+    if(elemBase + el >= a.count) continue;
+    const int eeL = a.begin + elemBase + el;
+    const int *ad = a.adr + (size_t)eeL * NF;
+    double be = 0.;
+#pragma unroll
+    for(int j = 0; j < NF; ++j) be -= (double)aeLds[wave][el][i * NF + j] * a.u[ad[j]];
+    if(a.hasSrc) {
+      const int eL = a.perm[eeL];
+      double cc[NV][3];
+#pragma unroll
+      for(int j = 0; j < NV; ++j) {
+        const int v = a.conn[(size_t)eL * NV + j];
+#pragma unroll
+        for(int d = 0; d < 3; ++d) cc[j][d] = a.xyz[3 * (size_t)v + d];
+      }
+      double JL, r0[3], r1[3], r2[3];
+      feGeometry<DIM>(cc, JL, r0, r1, r2);
+      for(int q = 0; q < nq; ++q) {
+        double x[3] = {0., 0., 0.};
+#pragma unroll
+        for(int j = 0; j < NV; ++j)
+#pragma unroll
+          for(int d = 0; d < 3; ++d) x[d] += cc[j][d] * Tb.Lg[q][j];
+        be -= Tb.L[q][i] * feEvalFn(a.src, x, eL, nq, q) * JL * Tb.w[q];
+      }
+    }
+    const int row = ad[i];
+    if(row < a.n) a.rhs[row] += be;
+  }
+}
+
+template <typename T, int NF, int DIM>
+void launchMfma(fe_hip_ctx *ctx, MfmaAsmArgs a)
+{
+  for(size_t c = 0; c + 1 < ctx->colorStart.size(); ++c) {
+    a.begin = ctx->colorStart[c];
+    a.count = ctx->colorStart[c + 1] - a.begin;
+    if(a.count == 0) continue;
+    hipLaunchKernelGGL((asmDiffusionMfma<T, NF, DIM>), dim3((a.count + 63) / 64), dim3(256), 0, ctx->stream, a);
+  }
+}
+
 __global__ void toFloat(int64_t n, const double *__restrict__ x, float *__restrict__ y)
 {
   for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -460,6 +639,7 @@ extern "C" int fe_hip_set_to_zero(fe_hip_ctx *ctx, int matrix, int residual)
   if(!ctx) return FE_STATUS_POINTER_ERROR;
   FE_REQUIRE(ctx, ctx->csrBuilt, "build the CSR first");
   if(matrix) FE_HIP_TRY(ctx, hipMemsetAsync(ctx->d_val, 0, sizeof(double) * ctx->nnz, ctx->stream));
+  if(matrix && ctx->d_val32) FE_HIP_TRY(ctx, hipMemsetAsync(ctx->d_val32, 0, sizeof(float) * ctx->nnz, ctx->stream));
   if(residual) FE_HIP_TRY(ctx, hipMemsetAsync(ctx->d_rhs, 0, sizeof(double) * ctx->n, ctx->stream));
   FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return FE_STATUS_OK;
@@ -478,6 +658,7 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
   // Fuse every SOURCE form into the DIFFUSION form of the same field and
   // quadrature rule (one element sweep computes Ae and Be of both).
   std::vector<bool> done(ctx->forms.size(), false);
+  bool fp32Done = false;
   for(size_t fi = 0; fi < ctx->forms.size(); ++fi) {
     if(done[fi]) continue;
     FeForm &F = ctx->forms[fi];
@@ -523,7 +704,30 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
     }
     done[fi] = true;
     const int kind = ctx->fields[field].kind;
-    if(kind == FE_TRI_P1) launchScalar<3, 2>(ctx, a);
+    if(F.sysType == FE_SYS_DIFFUSION && ctx->asmKernel == 1) {
+      MfmaAsmArgs m{};
+      m.E = a.E; m.n = a.n; m.what = what; m.K = F.mfmaK;
+      m.perm = a.perm; m.conn = a.conn; m.adr = a.adr; m.pos = a.pos;
+      m.xyz = a.xyz; m.tab = a.tab; m.u = a.u; m.sref = F.d_sref64;
+      m.coeff = a.coeff; m.src = a.src; m.hasSrc = a.hasSrc;
+      m.val = ctx->d_val; m.rhs = ctx->d_rhs;
+      if(kind == FE_TRI_P1) launchMfma<double, 3, 2>(ctx, m);
+      else if(kind == FE_TRI_P2) launchMfma<double, 6, 2>(ctx, m);
+      else if(kind == FE_TET_P1) launchMfma<double, 4, 3>(ctx, m);
+      if(precision == FE_PREC_FP32_IR && (what & 1) && ctx->fp32Values == 1) {
+        if(!ctx->d_val32) {
+          if(feAlloc(ctx, &ctx->d_val32, (size_t)ctx->nnz)) return FE_STATUS_OUT_OF_MEMORY;
+          FE_HIP_TRY(ctx, hipMemsetAsync(ctx->d_val32, 0, sizeof(float) * ctx->nnz, ctx->stream));
+        }
+        m.what = 1;
+        m.sref = F.d_sref32;
+        m.val = ctx->d_val32;
+        if(kind == FE_TRI_P1) launchMfma<float, 3, 2>(ctx, m);
+        else if(kind == FE_TRI_P2) launchMfma<float, 6, 2>(ctx, m);
+        else if(kind == FE_TET_P1) launchMfma<float, 4, 3>(ctx, m);
+        fp32Done = true;
+      }
+    } else if(kind == FE_TRI_P1) launchScalar<3, 2>(ctx, a);
     else if(kind == FE_TRI_P2) launchScalar<6, 2>(ctx, a);
     else if(kind == FE_TET_P1) launchScalar<4, 3>(ctx, a);
     else FE_REQUIRE(ctx, false, "scalar forms need a scalar field");
@@ -585,7 +789,7 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
   for(size_t fi = 0; fi < ctx->forms.size(); ++fi)
     FE_REQUIRE(ctx, done[fi], "form type has no device kernel");
   FE_HIP_TRY(ctx, hipGetLastError());
-  if(precision == FE_PREC_FP32_IR && (what & 1)) {
+  if(precision == FE_PREC_FP32_IR && (what & 1) && !fp32Done) {
     if(!ctx->d_val32 && feAlloc(ctx, &ctx->d_val32, (size_t)ctx->nnz)) return FE_STATUS_OUT_OF_MEMORY;
     hipLaunchKernelGGL(toFloat, dim3(2048), dim3(256), 0, ctx->stream, ctx->nnz, ctx->d_val, ctx->d_val32);
   }

@@ -353,6 +353,37 @@ int fe_hip_add_form(fe_hip_ctx *ctx, int sysType, int nfields, const int *fields
     }
     F.coeffs.push_back(fn);
   }
+  if(sysType == FE_SYS_DIFFUSION) {
+    // Reference tensors of the GEMM form Ae_ij = sum_{q,t} c_{q,t}(e) S_{q,t,ij}:
+    // t runs over the unique metric terms (a <= b) of M_ab = sum_d dr_a/dx_d
+    // dr_b/dx_d, S_{q,(a,b),ij} = w_q (d_a phi_i d_b phi_j + d_b phi_i d_a phi_j)
+    // (a single product when a == b), c = k(x_q) J M_ab.
+    const int dim = F.tab.dim, nf = F.tab.nf, nq2 = F.tab.nq;
+    const int nt = dim == 2 ? 3 : 6;
+    int K = nq2 * nt;
+    K = (K + 3) & ~3;
+    const int NE = nf * nf, NT = (NE + 15) / 16;
+    std::vector<double> S((size_t)K * 16 * NT, 0.);
+    const int ta[6] = {0, 0, 1, 0, 0, 1}, tb[6] = {0, 1, 1, 0, 0, 1};
+    const int ta3[6] = {0, 0, 0, 1, 1, 2}, tb3[6] = {0, 1, 2, 1, 2, 2};
+    for(int q = 0; q < nq2; ++q)
+      for(int t = 0; t < nt; ++t) {
+        const int a = dim == 2 ? ta[t] : ta3[t], b = dim == 2 ? tb[t] : tb3[t];
+        for(int i = 0; i < nf; ++i)
+          for(int j = 0; j < nf; ++j) {
+            auto d = [&](int c, int f) { return c == 0 ? F.tab.dLr[q][f] : (c == 1 ? F.tab.dLs[q][f] : F.tab.dLt[q][f]); };
+            double v = d(a, i) * d(b, j);
+            if(a != b) v += d(b, i) * d(a, j);
+            S[(size_t)(q * nt + t) * 16 * NT + i * nf + j] = F.tab.w[q] * v;
+          }
+      }
+    F.mfmaK = K;
+    F.mfmaNT = NT;
+    if(feAlloc(ctx, &F.d_sref64, S.size()) || feAlloc(ctx, &F.d_sref32, S.size())) return FE_STATUS_OUT_OF_MEMORY;
+    std::vector<float> S32(S.begin(), S.end());
+    FE_HIP_TRY(ctx, hipMemcpy(F.d_sref64, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice));
+    FE_HIP_TRY(ctx, hipMemcpy(F.d_sref32, S32.data(), sizeof(float) * S.size(), hipMemcpyHostToDevice));
+  }
   if(feAlloc(ctx, &F.d_tab, sizeof(FeRefTables) / sizeof(double) + 1)) return FE_STATUS_OUT_OF_MEMORY;
   FE_HIP_TRY(ctx, hipMemcpy(F.d_tab, &F.tab, sizeof(FeRefTables), hipMemcpyHostToDevice));
   for(int f : F.layoutI) F.nI += ctx->fields[f].nloc;
@@ -415,6 +446,12 @@ int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value)
   } else if(k == "iter_block") {
     FE_REQUIRE(ctx, value == 256 || value == 512 || value == 1024, "iter_block must be 256, 512 or 1024");
     ctx->iterBlock = value;
+  } else if(k == "asm_kernel") {
+    FE_REQUIRE(ctx, value == 0 || value == 1, "asm_kernel must be 0 (scalar) or 1 (MFMA)");
+    ctx->asmKernel = value;
+  } else if(k == "fp32_values") {
+    FE_REQUIRE(ctx, value == 0 || value == 1, "fp32_values must be 0 (rounded FP64) or 1 (FP32 MFMA)");
+    ctx->fp32Values = value;
   } else if(k == "iter_grid") {
     FE_REQUIRE(ctx, value >= 8 && value <= 4096 && value % 8 == 0, "iter_grid must be a multiple of 8 in [8, 4096]");
     ctx->iterGrid = value;
@@ -432,6 +469,8 @@ int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value)
   else if(k == "spmv_unroll") *value = ctx->spmvUnroll;
   else if(k == "iter_block") *value = ctx->iterBlock;
   else if(k == "iter_grid") *value = ctx->iterGrid;
+  else if(k == "asm_kernel") *value = ctx->asmKernel;
+  else if(k == "fp32_values") *value = ctx->fp32Values;
   else FE_REQUIRE(ctx, false, "unknown tuning key " + k);
   return FE_STATUS_OK;
 }

@@ -68,6 +68,11 @@ struct FeForm {
   std::vector<int> layoutI, layoutJ;
   int nI = 0, nJ = 0;
   int group = -1;  // CSR/position-map group (identical layouts share one)
+  // MFMA element path of DIFFUSION: reference tensors S[K][16 NT] with
+  // K = nq * (metric terms) padded to a multiple of 4
+  int mfmaK = 0, mfmaNT = 0;
+  double *d_sref64 = nullptr;
+  float *d_sref32 = nullptr;
 };
 
 // Distinct (layoutI, layoutJ) pairs of the matrix forms: the CSR candidates
@@ -138,6 +143,8 @@ struct fe_hip_ctx {
   int iterBlock = 512;
   int iterGrid = 1024;
   bool lanesFixed = false;
+  int asmKernel = 1;     // 0: scalar FP64 loop (reference order), 1: MFMA GEMM form
+  int fp32Values = 1;    // FP32 values: 0 = rounded FP64 values, 1 = FP32 MFMA element matrices
 
   // reductions
   double *d_partials = nullptr;

@@ -15,14 +15,18 @@ def _fn(t):
     return Fn(t[0], t[1], t[2], t[3])
 
 
+@pytest.mark.parametrize("asm_kernel", [0, 1])
 @pytest.mark.parametrize("name", DIFF_CASES)
-def test_device_csr_and_assembly_match_reference(name):
+def test_device_csr_and_assembly_match_reference(name, asm_kernel):
+    """Both element paths (0: scalar loop in the reference's order, 1: MFMA
+    GEMM form) against the reference's CSR, matrix and RHS."""
     from feng_amd.problems import diffusion_system
-    print("case", name, flush=True)
+    print("case", name, asm_kernel, flush=True)
     g = load(name)
     m = product_mesh(g)
     k, s = coeffs_of(g)
     S, num = diffusion_system(m, kind_of(g), g["deg"], _fn(k), _fn(s))
+    S.set_tuning("asm_kernel", asm_kernel)
     ia, ja = S.get_csr()
     assert np.array_equal(ia, g["ia"]) and np.array_equal(ja, g["ja"])
     S.assemble(3)
@@ -130,3 +134,26 @@ def test_reference_convergence_table(oracle, order):
         out.append(f"{err:.6e}")
         S.close()
     assert out == REF_TABLE[order]
+
+
+@pytest.mark.parametrize("kind,deg", [(1, 2), (2, 2), (2, 5), (4, 1), (4, 2)])
+def test_fp32_mfma_element_matrices(kind, deg):
+    """FP32 element matrices on the matrix cores vs the FP64 assembly:
+    single-precision accuracy, same pattern."""
+    from feng_amd.api import FN_ELEM, PREC_FP32_IR, Mesh
+    from feng_amd.problems import diffusion_system
+    m = Mesh.cube(6) if kind == 4 else Mesh.square(24, 0.2, 4)
+    k = np.exp(np.random.default_rng(kind).standard_normal(m.E))
+    S, num = diffusion_system(m, kind, deg, _fn((FN_ELEM, 0.0, 0.0, k)), _fn((0, -1.0, 0.0, None)))
+    S.set_tuning("fp32_values", 1)
+    S.assemble(3, PREC_FP32_IR)
+    v64, r64 = S.get_system()
+    r = S.solve(precision=PREC_FP32_IR, rtol=1e-10)
+    assert r.converged and r.relres <= 1e-10
+    # the FP32 values themselves: solve with them must reach the FP64 target
+    S.set_tuning("fp32_values", 0)
+    S.setToZero()
+    S.assemble(3, PREC_FP32_IR)
+    r0 = S.solve(precision=PREC_FP32_IR, rtol=1e-10)
+    assert r0.converged
+    S.close()
