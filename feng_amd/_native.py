@@ -34,6 +34,15 @@ class FeHipSolveStats(C.Structure):
                 ("ms_solve", C.c_double), ("ms_setup", C.c_double)]
 
 
+class FeHipMpnnOpts(C.Structure):
+    _fields_ = [("smooth_steps", C.c_int), ("smooth_omega", C.c_double), ("lr", C.c_double),
+                ("beta1", C.c_double), ("beta2", C.c_double), ("eps", C.c_double), ("seed", C.c_uint64)]
+
+
+class FeHipBatchStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("relres_fp64", C.c_double)]
+
+
 # name -> (restype, argtypes)
 HOST_API = {
     "fe_mesh_square": (C.c_int, [C.c_int, C.c_double, C.c_uint64, c_void_pp]),
@@ -87,6 +96,18 @@ HIP_API = {
     "fe_hip_pc_info": (C.c_int, [C.c_void_p, c_int_p, c_int_p]),
     "fe_hip_get_factor": (C.c_int, [C.c_void_p, c_double_p]),
     "fe_hip_get_tuning": (C.c_int, [C.c_void_p, C.c_char_p, c_int_p]),
+    "fe_hip_solve_batched": (C.c_int, [C.c_void_p, C.c_int, c_int_p, C.POINTER(FeHipSolverOpts),
+                                       C.POINTER(FeHipBatchStats), c_double_p]),
+    "fe_hip_autotune_batched": (C.c_int, [C.c_void_p, C.c_int, c_int_p, C.c_int, c_int_p, c_int_p, c_double_p]),
+    "fe_hip_update_coeff": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_double_p, C.c_int64]),
+    "fe_hip_mpnn_param_count": (C.c_int, []),
+    "fe_hip_mpnn_default_opts": (None, [C.POINTER(FeHipMpnnOpts)]),
+    "fe_hip_mpnn_init": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "fe_hip_mpnn_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int]),
+    "fe_hip_mpnn_get_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int]),
+    "fe_hip_mpnn_loss_grad": (C.c_int, [C.c_void_p, C.POINTER(FeHipMpnnOpts), c_double_p, C.POINTER(C.c_float)]),
+    "fe_hip_mpnn_train_step": (C.c_int, [C.c_void_p, C.POINTER(FeHipMpnnOpts), c_double_p]),
+    "fe_hip_mpnn_get_tensor": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_float), C.c_int64]),
 }
 
 _host = None

@@ -310,6 +310,56 @@ class LinearSystemHIP:
         self._c(self.L.fe_hip_get_tuning(self.ctx, key.encode(), C.byref(v)), "fe_hip_get_tuning")
         return v.value
 
+    def update_coeff(self, form, index, data):
+        """Replace a tabulated coefficient (FN_ELEM / FN_QP) of form `form`."""
+        d = np.ascontiguousarray(data, np.float64)
+        self._c(self.L.fe_hip_update_coeff(self.ctx, form, index, N.ptr(d, C.c_double), d.size),
+                "fe_hip_update_coeff")
+
+    # learned preconditioner (FE_PC_LEARNED) ---------------------------------
+    def mpnn_opts(self, **kw):
+        o = N.FeHipMpnnOpts()
+        self.L.fe_hip_mpnn_default_opts(C.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+    def mpnn_init(self, seed):
+        self._c(self.L.fe_hip_mpnn_init(self.ctx, seed), "fe_hip_mpnn_init")
+
+    def mpnn_set_params(self, theta):
+        t = np.ascontiguousarray(theta, np.float32)
+        self._c(self.L.fe_hip_mpnn_set_params(self.ctx, N.ptr(t, C.c_float), t.size), "fe_hip_mpnn_set_params")
+
+    def mpnn_get_params(self):
+        t = np.zeros(self.L.fe_hip_mpnn_param_count(), np.float32)
+        self._c(self.L.fe_hip_mpnn_get_params(self.ctx, N.ptr(t, C.c_float), t.size), "fe_hip_mpnn_get_params")
+        return t
+
+    def mpnn_loss_grad(self, want_grad=True, **kw):
+        o = self.mpnn_opts(**kw)
+        loss = C.c_double()
+        g = np.zeros(self.L.fe_hip_mpnn_param_count(), np.float32) if want_grad else None
+        self._c(self.L.fe_hip_mpnn_loss_grad(self.ctx, C.byref(o), C.byref(loss),
+                                             N.ptr(g, C.c_float) if g is not None else None),
+                "fe_hip_mpnn_loss_grad")
+        return loss.value, g
+
+    def mpnn_train_step(self, **kw):
+        o = self.mpnn_opts(**kw)
+        loss = C.c_double()
+        self._c(self.L.fe_hip_mpnn_train_step(self.ctx, C.byref(o), C.byref(loss)), "fe_hip_mpnn_train_step")
+        return loss.value
+
+    def mpnn_tensor(self, name):
+        sizes = {"X": self.n * 16, "p": self.n * 8, "q": self.n * 8, "tau": self.n, "d": self.n,
+                 "g": self.nnz, "xi": self.n * 8}
+        cnt = sizes.get(name, self.n * 32)
+        out = np.zeros(cnt, np.float32)
+        self._c(self.L.fe_hip_mpnn_get_tensor(self.ctx, name.encode(), N.ptr(out, C.c_float), cnt),
+                "fe_hip_mpnn_get_tensor")
+        return out
+
     def close(self):
         if self.ctx:
             self.L.fe_hip_ctx_destroy(self.ctx)

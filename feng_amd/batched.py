@@ -1,0 +1,130 @@
+"""Many independent diffusion problems assembled and solved together.
+
+Problem p is a P1 or P2 Lagrange discretisation of -div(k_p grad u) + S = 0
+(S = -1, u = 0 on the boundary) on its own N_p x N_p split-square mesh with
+k_p piecewise constant on 8 x 8 blocks (feNG's numbering and forms per
+problem: feNumber.cpp:677-843, feSysElm.cpp:16-27,520-589).  All problems of
+one element degree are uploaded as ONE disjoint mesh whose DOFs are numbered
+problem by problem: problem p's unknowns are the rows [rows[p], rows[p+1]) of
+a block-diagonal system (its reference numbering shifted), the essential DOFs
+of all problems follow.  One assembly then covers every problem, and
+fe_hip_solve_batched runs an independent Jacobi-PCG per problem in shared
+launches.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+from . import api as A
+
+
+@dataclass
+class UnionMesh:
+    """Duck-typed mesh for LinearSystemHIP.upload_mesh."""
+    dim: int
+    V: int
+    E: int
+    nvpe: int
+    xyz: np.ndarray
+    conn: np.ndarray
+
+
+@dataclass
+class Batch:
+    kind: int
+    sizes: list
+    seeds: list
+    S: A.LinearSystemHIP
+    rows: np.ndarray          # nprob + 1 unknown-row offsets
+    nInc: list                # unknowns per problem
+    dof_maps: list = field(default_factory=list)  # per problem: union DOF id of each local DOF
+    meshes: list = field(default_factory=list)
+    coeffs: list = field(default_factory=list)
+    setup_s: float = 0.0
+
+
+def draw_problems(nprob=256, seed=0, nmin=32, nmax=256):
+    """N_p uniform in [nmin, nmax], degree P1/P2 50/50, coefficient seeds."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(nmin, nmax + 1, nprob)
+    kinds = np.where(rng.random(nprob) < 0.5, A.TRI_P1, A.TRI_P2)
+    seeds = rng.integers(1, 2 ** 31, nprob)
+    return [(int(N_), int(k), int(s)) for N_, k, s in zip(sizes, kinds, seeds)]
+
+
+def build_batch(problems, kind, device=0, keep_meshes=False) -> Batch:
+    """One device system holding every problem of element kind `kind`."""
+    t0 = time.perf_counter()
+    probs = [(N_, s) for N_, k, s in problems if k == kind]
+    meshes, nums, ks = [], [], []
+    for N_, s in probs:
+        m = A.Mesh.square(N_)
+        meshes.append(m)
+        nums.append(A.Numbering(m, [kind], [1]))
+        ks.append(m.coeff_blocks(8, 0.1, 10.0, s))
+    nInc = [nm.nInc for nm in nums]
+    nEss = [nm.nDofs - nm.nInc for nm in nums]
+    NU = int(sum(nInc))
+    uoff = np.concatenate([[0], np.cumsum(nInc)]).astype(np.int64)
+    eoff = np.concatenate([[0], np.cumsum(nEss)]).astype(np.int64)
+    voff = np.concatenate([[0], np.cumsum([m.V for m in meshes])]).astype(np.int64)
+    xyz = np.concatenate([m.xyz for m in meshes])
+    conn = np.concatenate([m.conn.astype(np.int64) + voff[p] for p, m in enumerate(meshes)]).astype(np.int32)
+    maps, adr = [], []
+    for p, nm in enumerate(nums):
+        loc = np.arange(nm.nDofs)
+        gm = np.where(loc < nm.nInc, uoff[p] + loc, NU + eoff[p] + (loc - nm.nInc))
+        maps.append(gm)
+        adr.append(gm[nm.adr[0]])
+    adr = np.concatenate(adr).astype(np.int32)
+    k = np.concatenate(ks)
+    um = UnionMesh(2, int(voff[-1]), int(conn.size // 3), 3, xyz, conn)
+    S = A.LinearSystemHIP(device)
+    S.upload_mesh(um)
+    f = S.add_field(kind, adr)
+    S.set_sizes(NU, NU + int(eoff[-1]))
+    S.add_form(A.DIFFUSION, [f], 2, [A.Fn(A.FN_ELEM, data=k)])
+    S.add_form(A.SOURCE, [f], 2, [A.Fn(A.FN_CONST, -1.0)])
+    S.build_csr()
+    S.set_solution(np.zeros(NU + int(eoff[-1])))
+    b = Batch(kind, [N_ for N_, _ in probs], [s for _, s in probs], S, uoff.astype(np.int32), nInc, maps)
+    if keep_meshes:
+        b.meshes, b.coeffs = meshes, ks
+    b.setup_s = time.perf_counter() - t0
+    return b
+
+
+def solve_batch(b: Batch, precision=A.PREC_FP32_IR, rtol=1e-8, inner_rtol=1e-2, max_iter=100000, autotune=True,
+                check=True):
+    """Assemble every problem (zero + all element kernels + scatter) and solve
+    them all; returns timings and per-problem statistics."""
+    L, S = b.S.L, b.S
+    rows = np.ascontiguousarray(b.rows, np.int32)
+    nprob = len(b.nInc)
+    S.set_profiling(True)
+    S.setToZero()
+    S.assemble(3, precision)
+    asm_ms = S.kernel_stats(1)[0]
+    tuned = None
+    if autotune:
+        lanes, block, ms = C.c_int(), C.c_int(), C.c_double()
+        S._c(L.fe_hip_autotune_batched(S.ctx, nprob, N.ptr(rows, C.c_int), precision, C.byref(lanes),
+                                       C.byref(block), C.byref(ms)), "fe_hip_autotune_batched")
+        tuned = dict(lanes=lanes.value, block=block.value, ms_per_iter=ms.value)
+    o = N.FeHipSolverOpts()
+    L.fe_hip_default_opts(C.byref(o))
+    o.precision, o.rtol, o.atol, o.inner_rtol, o.max_iter = precision, rtol, 1e-300, inner_rtol, max_iter
+    st = (N.FeHipBatchStats * nprob)()
+    ms = C.c_double()
+    S._c(L.fe_hip_solve_batched(S.ctx, nprob, N.ptr(rows, C.c_int), C.byref(o), st, C.byref(ms)),
+         "fe_hip_solve_batched")
+    its = np.array([s.iterations for s in st])
+    rel = np.array([s.relres_fp64 for s in st])
+    conv = np.array([bool(s.converged) for s in st])
+    return dict(asm_ms=asm_ms, solve_ms=ms.value, iterations=its, relres=rel, converged=conv, tuned=tuned,
+                n=int(b.rows[-1]), nnz=S.nnz)

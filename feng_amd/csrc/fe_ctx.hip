@@ -185,6 +185,8 @@ int fe_hip_ctx_destroy(fe_hip_ctx *ctx)
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   for(hipEvent_t e : ctx->spmvTimer.ev) (void)hipEventDestroy(e);
+  feMpnnRelease(ctx);
+  feBatchRelease(ctx);
   for(void *p : ctx->owned) (void)hipFree(p);
   if(ctx->h_state) (void)hipHostFree(ctx->h_state);
   (void)hipStreamDestroy(ctx->stream);
@@ -452,12 +454,30 @@ int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value)
   } else if(k == "fp32_values") {
     FE_REQUIRE(ctx, value == 0 || value == 1, "fp32_values must be 0 (rounded FP64) or 1 (FP32 MFMA)");
     ctx->fp32Values = value;
+  } else if(k == "pipe_kernel") {
+    FE_REQUIRE(ctx, value == 0 || value == 1, "pipe_kernel must be 0 (gathered D^-1 r) or 1 (tiled)");
+    ctx->pipeKernel = value;
   } else if(k == "iter_grid") {
     FE_REQUIRE(ctx, value >= 8 && value <= 4096 && value % 8 == 0, "iter_grid must be a multiple of 8 in [8, 4096]");
     ctx->iterGrid = value;
   } else {
     FE_REQUIRE(ctx, false, "unknown tuning key " + k);
   }
+  return FE_STATUS_OK;
+}
+
+int fe_hip_update_coeff(fe_hip_ctx *ctx, int form, int index, const double *data, int64_t count)
+{
+  if(!ctx || !data) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, form >= 0 && form < (int)ctx->forms.size(), "unknown form");
+  FeForm &F = ctx->forms[form];
+  FE_REQUIRE(ctx, index >= 0 && index < (int)F.coeffs.size(), "unknown coefficient");
+  FeFn &fn = F.coeffs[index];
+  FE_REQUIRE(ctx, fn.kind == FE_FN_ELEM || fn.kind == FE_FN_QP, "only tabulated coefficients can be replaced");
+  const int64_t need = (fn.kind == FE_FN_ELEM) ? ctx->E : (int64_t)ctx->E * F.tab.nq;
+  FE_REQUIRE(ctx, count == need, "coefficient table has the wrong size");
+  FE_HIP_TRY(ctx, hipMemcpyAsync(fn.data, data, sizeof(double) * need, hipMemcpyHostToDevice, ctx->stream));
+  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return FE_STATUS_OK;
 }
 
@@ -471,6 +491,7 @@ int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value)
   else if(k == "iter_grid") *value = ctx->iterGrid;
   else if(k == "asm_kernel") *value = ctx->asmKernel;
   else if(k == "fp32_values") *value = ctx->fp32Values;
+  else if(k == "pipe_kernel") *value = ctx->pipeKernel;
   else FE_REQUIRE(ctx, false, "unknown tuning key " + k);
   return FE_STATUS_OK;
 }

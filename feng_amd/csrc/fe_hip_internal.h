@@ -48,7 +48,7 @@ int feBuildRefTables(int kind, int degree, FeRefTables *t);  // host, returns nq
 struct FeFn {
   int kind;
   double c0, c1;
-  double *data;  // device copy for FE_FN_ELEM / FE_FN_QP (color-sorted element order)
+  double *data;  // device copy for FE_FN_ELEM / FE_FN_QP (mesh element order)
 };
 
 struct FeField {
@@ -108,6 +108,9 @@ struct FePrecond {
   int sweeps = 2;  // Jacobi sweeps per triangular factor in GMRES/ILU(0)
 };
 
+struct FeMpnn;   // learned-preconditioner state (fe_learned.hip)
+struct FeBatch;  // batched-solver state (fe_batch.hip)
+
 struct fe_hip_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -145,6 +148,9 @@ struct fe_hip_ctx {
   bool lanesFixed = false;
   int asmKernel = 1;     // 0: scalar FP64 loop (reference order), 1: MFMA GEMM form
   int fp32Values = 1;    // FP32 values: 0 = rounded FP64 values, 1 = FP32 MFMA element matrices
+  int pipeKernel = 1;
+  FeMpnn *mpnn = nullptr;
+  FeBatch *batch = nullptr;    // Jacobi-PCG: 0 = on-the-fly D^-1 r gather, 1 = tiled (one gathered vector)
 
   // reductions
   double *d_partials = nullptr;
@@ -203,6 +209,8 @@ int fePrecondFactor(fe_hip_ctx *ctx);
 
 // Learned correction of the split factor (fe_learned.hip)
 int feLearnedCorrect(fe_hip_ctx *ctx);
+void feMpnnRelease(fe_hip_ctx *ctx);
+void feBatchRelease(fe_hip_ctx *ctx);
 
 // GMRES (fe_gmres.hip)
 int feSolveGmres(fe_hip_ctx *ctx, const fe_hip_solver_opts *opts, double *du, fe_hip_solve_stats *stats);

@@ -21,12 +21,15 @@
 // (values, vectors; FP64 dot accumulation) as the inner solver of FP64
 // iterative refinement x += A32^-1 (b - A64 x).
 #include "fe_hip_internal.h"
+#include "fe_tile.h"
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 
 namespace {
+
+using namespace fe_tile;
 
 // Iteration state mirrored to the host at every check.
 struct CgState {
@@ -525,6 +528,171 @@ __global__ __launch_bounds__(TB) void pipeIter(int n, const int *__restrict__ ia
   feStoreBlockSum(rh, P.rho[o], lds);
 }
 
+// Tiled pipelined Jacobi-PCG (Ghysels & Vanroose, "Hiding global
+// synchronization latency in the preconditioned conjugate gradient
+// algorithm", Parallel Computing 40 (2014), Alg. 3, with M = D) in one kernel
+// per iteration.  With a diagonal preconditioner the vectors u = D^-1 r,
+// q = D^-1 s and m = D^-1 w follow from stored ones, so the iteration keeps
+// x, r, w, s, z, p (own row only, updated in place) and m (the only gathered
+// vector, double-buffered by parity):
+//   beta = gamma_i / gamma_{i-1}, alpha = gamma_i / (delta_i - beta gamma_i / alpha_{i-1})
+//   y = A m_i (gathered), z = y + beta z, s = w + beta s, p = D^-1 r + beta p,
+//   x += alpha p, r -= alpha s, w -= alpha z, m_{i+1} = D^-1 w,
+//   partials of gamma = (r, D^-1 r), delta = (w, D^-1 r), rho = (r, r).
+// A block owns tiles of TB rows.  In the product phase each group of LPR
+// lanes takes LPR rows of the tile, their gathers interleaved; the row sums
+// go through LDS to the update phase, where thread t updates row t of the
+// tile with coalesced loads and stores.  The row bounds and the own-row
+// values of the first tile are loaded before the partials of the previous
+// iteration are folded, so those loads overlap the fold.
+// r_0 = b (or the residual already in r), x = 0 unless restarting,
+// w_0 = A D^-1 r_0, m_0 = D^-1 w_0, s = z = p = 0; partials of iteration 0.
+// When starting from b the product gathers D^-1 b directly; when restarting,
+// r was written by an earlier kernel.
+template <typename T, int LPR, int TB>
+__global__ __launch_bounds__(TB) void tileInit(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                               const T *__restrict__ val, const double *__restrict__ b,
+                                               TileVecs<T> v, PipeParts P, PipeScalars *sc, CgState *st,
+                                               int fromResidual)
+{
+  __shared__ T ldsY[TB];
+  const int nTiles = (n + TB - 1) / TB;
+  double g = 0., d = 0., rho = 0.;
+  for(int tile = blockIdx.x; tile < nTiles; tile += gridDim.x) {
+    const int base = tile * TB;
+    int kb[LPR], ke[LPR];
+    tileBounds<LPR, TB>(n, base, ia, kb, ke);
+    if(fromResidual) tileProduct<T, LPR, TB, true>(n, base, ia, ja, val, v.r, v.dinv, kb, ke, ldsY);
+    else {
+      // gather D^-1 b in T precision
+      constexpr int RPB = TB / LPR;
+      const int sub = threadIdx.x % LPR, gg = threadIdx.x / LPR;
+#pragma unroll
+      for(int u = 0; u < LPR; ++u) {
+        T acc = (T)0;
+        for(int k = kb[u]; k < ke[u]; k += LPR) {
+          const int j = ja[k];
+          acc += val[k] * (v.dinv[j] * (T)b[j]);
+        }
+#pragma unroll
+        for(int q = LPR / 2; q > 0; q >>= 1) acc += __shfl_xor(acc, q, LPR);
+        if(sub == u) ldsY[u * RPB + gg] = acc;
+      }
+    }
+    __syncthreads();
+    const int row = base + threadIdx.x;
+    if(row < n) {
+      const T di = v.dinv[row];
+      const T ri = fromResidual ? v.r[row] : (T)b[row];
+      const T wi = ldsY[threadIdx.x];
+      const T ui = di * ri;
+      if(!fromResidual) {
+        v.r[row] = ri;
+        v.x[row] = (T)0;
+      }
+      v.w[row] = wi;
+      v.m[0][row] = di * wi;
+      v.s[row] = (T)0;
+      v.z[row] = (T)0;
+      v.p[row] = (T)0;
+      g += (double)ri * (double)ui;
+      d += (double)wi * (double)ui;
+      rho += (double)ri * (double)ri;
+    }
+    __syncthreads();
+  }
+  tileStore3<TB>(g, d, rho, P.g[0], P.d[0], P.rho[0]);
+  if(blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->gamma[1] = 1.;
+    sc->alpha[1] = __builtin_inf();
+    st->done = 0;
+    st->iter = 0;
+  }
+}
+
+template <typename T, int LPR, int TB>
+__global__ __launch_bounds__(TB) void tileIter(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                               const T *__restrict__ val, TileVecs<T> v, PipeParts P,
+                                               PipeScalars *sc, CgState *st, int it, int maxit, double tol2)
+{
+  __shared__ T ldsY[TB];
+  const int c = it & 1, o = c ^ 1;
+  const int nTiles = (n + TB - 1) / TB;
+  int tile = blockIdx.x;
+  // first tile: row bounds and own-row values in flight during the fold
+  int kb[LPR], ke[LPR];
+  TileRow<T> me;
+  tileBounds<LPR, TB>(n, tile * TB, ia, kb, ke);
+  tileLoadRow(v, tile * TB + threadIdx.x, n, me);
+  const int done0 = st->done;
+  const double gPrev = sc->gamma[o], aPrev = sc->alpha[o];
+  double f[3];
+  const double *const parts[3] = {P.g[c], P.d[c], P.rho[c]};
+  feFold<TB, 3>(parts, gridDim.x, f);
+  const double gamma = f[0], delta = f[1], rho = f[2];
+  if(done0) return;
+  if(!(rho > tol2) || it >= maxit) {
+    if(blockIdx.x == 0 && threadIdx.x == 0) {
+      st->rr = rho;
+      st->iter = it;
+      st->done = (rho <= tol2) ? 1 : (rho == rho ? 2 : 4);
+    }
+    return;
+  }
+  const double betaD = gamma / gPrev;
+  const double den = delta - betaD * gamma / aPrev;
+  if(!(den > 0.)) {
+    if(blockIdx.x == 0 && threadIdx.x == 0) {
+      st->iter = it;
+      st->done = 3;  // breakdown: the preconditioned operator is not SPD
+    }
+    return;
+  }
+  const double alphaD = gamma / den;
+  if(blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->gamma[c] = gamma;
+    sc->alpha[c] = alphaD;
+    st->rr = rho;
+    st->hist[it & 63] = rho;
+  }
+  const T alpha = (T)alphaD, beta = (T)betaD;
+  const T *__restrict__ mC = v.m[c];
+  T *__restrict__ mO = v.m[o];
+  double g = 0., d = 0., rh = 0.;
+  while(tile < nTiles) {
+    const int base = tile * TB;
+    tileProduct<T, LPR, TB, false>(n, base, ia, ja, val, mC, nullptr, kb, ke, ldsY);
+    __syncthreads();
+    const int row = base + threadIdx.x;
+    if(row < n) {
+      const T y = ldsY[threadIdx.x];
+      const T z = y + beta * me.z;
+      const T s = me.w + beta * me.s;
+      const T p = me.d * me.r + beta * me.p;
+      const T rn = me.r - alpha * s;
+      const T wn = me.w - alpha * z;
+      const T un = me.d * rn;
+      v.x[row] = me.x + alpha * p;
+      v.p[row] = p;
+      v.s[row] = s;
+      v.z[row] = z;
+      v.r[row] = rn;
+      v.w[row] = wn;
+      mO[row] = me.d * wn;
+      g += (double)rn * (double)un;
+      d += (double)wn * (double)un;
+      rh += (double)rn * (double)rn;
+    }
+    tile += gridDim.x;
+    if(tile < nTiles) {
+      tileBounds<LPR, TB>(n, tile * TB, ia, kb, ke);
+      tileLoadRow(v, tile * TB + threadIdx.x, n, me);
+    }
+    __syncthreads();
+  }
+  tileStore3<TB>(g, d, rh, P.g[o], P.d[o], P.rho[o]);
+}
+
 // r = b - A x (FP64); partials of ||r||^2.
 template <int LPR>
 __global__ __launch_bounds__(kTB) void residual64(int n, const int *__restrict__ ia, const int *__restrict__ ja,
@@ -571,6 +739,7 @@ __global__ void addVec(int n, double *__restrict__ x, const T *__restrict__ d)
 {
   for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] += (double)d[i];
 }
+
 
 __global__ __launch_bounds__(kTB) void sumSquares(int n, const double *__restrict__ a, double *__restrict__ part)
 {
@@ -623,6 +792,8 @@ struct Workspace {
   CgVecs<float> v32;
   PipeVecs<double> q64;
   PipeVecs<float> q32;
+  TileVecs<double> t64;
+  TileVecs<float> t32;
   PipeParts pparts;
   PipeScalars *sc;
   SplitVecs<double> s64;
@@ -635,8 +806,8 @@ struct Workspace {
 int ensureWorkspace(fe_hip_ctx *ctx, Workspace &w)
 {
   const size_t n = ctx->n;
-  const size_t need = n * (8 * 8 + 8 * 4 + 3 * 8 + 9 * 8 + 9 * 4 + 8 * 8 + 8 * 4) + 24 * kMaxGrid * 8 + 64 * 8 + 8192 +
-                      64 * 768;
+  const size_t need = n * (8 * 8 + 8 * 4 + 3 * 8 + 9 * 8 + 9 * 4 + 8 * 8 + 8 * 4 + 9 * 8 + 9 * 4) +
+                      24 * kMaxGrid * 8 + 64 * 8 + 8192 + 64 * 1024;
   if(!ctx->d_work || ctx->workBytes < need) {
     if(ctx->d_work) feFree(ctx, ctx->d_work);
     char *p = nullptr;
@@ -677,6 +848,12 @@ int ensureWorkspace(fe_hip_ctx *ctx, Workspace &w)
     w.q32.r[k] = (float *)take(n * 4);
     w.q32.w[k] = (float *)take(n * 4);
     w.q32.s[k] = (float *)take(n * 4);
+  }
+  {
+    double **a[] = {&w.t64.x, &w.t64.r, &w.t64.w, &w.t64.s, &w.t64.z, &w.t64.p, &w.t64.dinv, &w.t64.m[0], &w.t64.m[1]};
+    for(auto pp : a) *pp = (double *)take(n * 8);
+    float **b[] = {&w.t32.x, &w.t32.r, &w.t32.w, &w.t32.s, &w.t32.z, &w.t32.p, &w.t32.dinv, &w.t32.m[0], &w.t32.m[1]};
+    for(auto pp : b) *pp = (float *)take(n * 4);
   }
   double **d64[] = {&w.v64.x, &w.v64.r, &w.v64.z, &w.v64.p[0], &w.v64.p[1], &w.v64.q, &w.v64.dinv};
   for(auto pp : d64) *pp = (double *)take(n * 8);
@@ -915,6 +1092,92 @@ int runPipe(fe_hip_ctx *ctx, Workspace &w, PipeVecs<T> &v, const T *val, int che
 
 
 template <typename T, int TB>
+void launchTileTB(fe_hip_ctx *ctx, Workspace &w, TileVecs<T> &v, const T *val, int it, int maxit, double tol2,
+                  bool init, const double *b, int fromResidual)
+{
+  const int n = ctx->n;
+  const int nTiles = (n + TB - 1) / TB;
+  const int G = std::max(1, std::min(ctx->iterGrid, nTiles));
+#define FE_TILE_CASE(L)                                                                                            \
+  if(init)                                                                                                         \
+    hipLaunchKernelGGL((tileInit<T, L, TB>), dim3(G), dim3(TB), 0, ctx->stream, n, ctx->d_ia, ctx->d_ja, val, b, v, \
+                       w.pparts, w.sc, w.st, fromResidual);                                                        \
+  else                                                                                                             \
+    hipLaunchKernelGGL((tileIter<T, L, TB>), dim3(G), dim3(TB), 0, ctx->stream, n, ctx->d_ia, ctx->d_ja, val, v,    \
+                       w.pparts, w.sc, w.st, it, maxit, tol2);
+  switch(ctx->spmvLanes) {
+    case 1: FE_TILE_CASE(1) break;
+    case 2: FE_TILE_CASE(2) break;
+    case 4: FE_TILE_CASE(4) break;
+    case 8: FE_TILE_CASE(8) break;
+    default: FE_TILE_CASE(16) break;
+  }
+#undef FE_TILE_CASE
+}
+
+template <typename T>
+void launchTile(fe_hip_ctx *ctx, Workspace &w, TileVecs<T> &v, const T *val, int it, int maxit, double tol2,
+                bool init = false, const double *b = nullptr, int fromResidual = 0)
+{
+  if(ctx->iterBlock == 1024) launchTileTB<T, 1024>(ctx, w, v, val, it, maxit, tol2, init, b, fromResidual);
+  else if(ctx->iterBlock == 512) launchTileTB<T, 512>(ctx, w, v, val, it, maxit, tol2, init, b, fromResidual);
+  else launchTileTB<T, 256>(ctx, w, v, val, it, maxit, tol2, init, b, fromResidual);
+}
+
+// Jacobi-PCG iterations (pipelined kernel chosen by the pipe_kernel tuning
+// key) until the stop test fires.
+template <typename T>
+int runJacobi(fe_hip_ctx *ctx, Workspace &w, PipeVecs<T> &pv, TileVecs<T> &tv, const T *val, int checkEvery,
+              int maxit, double tol2, int *iters, int *done)
+{
+  if(!ctx->pipeKernel) return runPipe<T>(ctx, w, pv, val, checkEvery, maxit, tol2, iters, done);
+  CgState *hs = (CgState *)ctx->h_state;
+  FeKernelTimer &tm = ctx->spmvTimer;
+  tm.used = 0;
+  int it = 0;
+  for(;;) {
+    for(int k = 0; k < checkEvery; ++k, ++it) {
+      if(ctx->profiling) {
+        if(tm.used + 2 > tm.ev.size())
+          for(int a = 0; a < 64; ++a) {
+            hipEvent_t e;
+            FE_HIP_TRY(ctx, hipEventCreate(&e));
+            tm.ev.push_back(e);
+          }
+        FE_HIP_TRY(ctx, hipEventRecord(tm.ev[tm.used++], ctx->stream));
+      }
+      launchTile<T>(ctx, w, tv, val, it, maxit, tol2);
+      if(ctx->profiling) FE_HIP_TRY(ctx, hipEventRecord(tm.ev[tm.used++], ctx->stream));
+    }
+    FE_HIP_TRY(ctx, hipGetLastError());
+    FE_HIP_TRY(ctx, hipMemcpyAsync(hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
+    FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if(hs->done) break;
+  }
+  if(ctx->profiling) {
+    const size_t valid = std::min<size_t>(tm.used / 2, (size_t)std::max(0, hs->iter));
+    for(size_t pi = 0; pi < valid; ++pi) {
+      float ms = 0.f;
+      FE_HIP_TRY(ctx, hipEventElapsedTime(&ms, tm.ev[2 * pi], tm.ev[2 * pi + 1]));
+      tm.total_ms += ms;
+      tm.launches++;
+    }
+    tm.used = 0;
+  }
+  *iters = hs->iter;
+  *done = hs->done;
+  return FE_STATUS_OK;
+}
+
+template <typename T>
+void startJacobi(fe_hip_ctx *ctx, Workspace &w, PipeVecs<T> &pv, TileVecs<T> &tv, const T *val, const double *b,
+                 int fromResidual)
+{
+  if(ctx->pipeKernel) launchTile<T>(ctx, w, tv, val, 0, 0, 0., true, b, fromResidual);
+  else launchPipe<T>(ctx, w, pv, val, 0, 0, 0., true, b, fromResidual);
+}
+
+template <typename T, int TB>
 void launchSplitTB(fe_hip_ctx *ctx, Workspace &w, SplitVecs<T> &v, int kind, int it, const double *b, int rs, int slot,
                    int checkDone)
 {
@@ -1068,14 +1331,16 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
     FE_HIP_TRY(ctx, hipMemsetAsync(w.x64, 0, sizeof(double) * n, ctx->stream));
     FE_HIP_TRY(ctx, hipMemcpyAsync(w.r64, ctx->d_rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
     double rn = bn;
+    bool inner32 = fp32;
     while(outer < 60) {
       relres = rn / bn;
       if(rn <= target) { converged = 1; reason = 2; break; }
       if(iters >= o->max_iter) { reason = -3; break; }
-      double tol2 = fp32 ? (o->inner_rtol * rn) * (o->inner_rtol * rn) : target * target * 0.25;
+      double tol2 = inner32 ? (o->inner_rtol * rn) * (o->inner_rtol * rn) : target * target * 0.25;
       tol2 = std::max(tol2, target * target * 0.01);
       int it0 = 0;
-      if(fp32) {
+      FE_HIP_TRY(ctx, hipMemcpyAsync(w.tmp64, w.x64, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+      if(inner32) {
         if(int rc = runSplit<float>(ctx, w, w.s32, ctx->d_val32, w.r64, checkEvery, o->max_iter - iters, tol2, &it0, &done))
           return rc;
         hipLaunchKernelGGL(addVec<float>, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, w.x64, w.s32.x);
@@ -1087,8 +1352,16 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
       iters += it0;
       double rr = 0.;
       if(int rc = residualNorm2(ctx, w, w.x64, ctx->d_rhs, w.r64, &rr)) return rc;
-      rn = std::sqrt(rr);
       outer++;
+      if(inner32 && !(std::sqrt(rr) < 0.9 * rn)) {
+        // the FP32 correction did not reduce the FP64 residual: undo it and
+        // finish with FP64 inner solves
+        FE_HIP_TRY(ctx, hipMemcpyAsync(w.x64, w.tmp64, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+        if(int rc = residualNorm2(ctx, w, w.x64, ctx->d_rhs, w.r64, &rr)) return rc;
+        inner32 = false;
+        continue;
+      }
+      rn = std::sqrt(rr);
       if(done == 3 || done == 4) { reason = -3; break; }
       if(rn / bn > o->dtol) { reason = -4; break; }
     }
@@ -1096,18 +1369,22 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
     if(!converged && rn <= target) { converged = 1; reason = 2; }
     FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_du, w.x64, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
   } else if(o->precision == FE_PREC_FP64) {
+    double *dinv64 = ctx->pipeKernel ? w.t64.dinv : w.q64.dinv;
+    double *x64 = ctx->pipeKernel ? w.t64.x : w.q64.x;
+    double *r64 = ctx->pipeKernel ? w.t64.r : w.q64.r[0];
     hipLaunchKernelGGL(extractDinv<double>, dim3((n + kTB - 1) / kTB), dim3(kTB), 0, ctx->stream, n, ctx->d_diag,
-                       ctx->d_val, w.q64.dinv);
+                       ctx->d_val, dinv64);
     // recursive-residual target; the FP64 true residual is checked after
     double tol2 = target * target * 0.25;
-    launchPipe<double>(ctx, w, w.q64, ctx->d_val, 0, 0, 0., true, ctx->d_rhs, 0);
+    startJacobi<double>(ctx, w, w.q64, w.t64, ctx->d_val, ctx->d_rhs, 0);
     for(;;) {
       int it = 0;
-      if(int rc = runPipe<double>(ctx, w, w.q64, ctx->d_val, checkEvery, o->max_iter - iters, tol2, &it, &done))
+      if(int rc = runJacobi<double>(ctx, w, w.q64, w.t64, ctx->d_val, checkEvery, o->max_iter - iters, tol2, &it,
+                                    &done))
         return rc;
       iters += it;
       double rr = 0.;
-      if(int rc = residualNorm2(ctx, w, w.q64.x, ctx->d_rhs, w.q64.r[0], &rr)) return rc;
+      if(int rc = residualNorm2(ctx, w, x64, ctx->d_rhs, r64, &rr)) return rc;
       relres = std::sqrt(rr) / bn;
       outer++;
       if(std::sqrt(rr) <= target) { converged = 1; reason = 2; break; }
@@ -1115,35 +1392,59 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
       if(done == 4 || !(rr == rr)) { reason = -9; break; }
       if(iters >= o->max_iter) { reason = -3; break; }
       if(relres > o->dtol) { reason = -4; break; }
-      // restart from the true residual (r slot 0 holds b - A x)
+      // restart from the true residual (r holds b - A x)
       tol2 *= 0.01;
-      launchPipe<double>(ctx, w, w.q64, ctx->d_val, 0, 0, 0., true, ctx->d_rhs, 1);
+      startJacobi<double>(ctx, w, w.q64, w.t64, ctx->d_val, ctx->d_rhs, 1);
     }
-    FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_du, w.q64.x, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_du, x64, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
   } else {
     FE_REQUIRE(ctx, ctx->d_val32, "assemble with FE_PREC_FP32_IR to create the FP32 values");
     hipLaunchKernelGGL(extractDinv<float>, dim3((n + kTB - 1) / kTB), dim3(kTB), 0, ctx->stream, n, ctx->d_diag,
-                       ctx->d_val, w.q32.dinv);
+                       ctx->d_val, ctx->pipeKernel ? w.t32.dinv : w.q32.dinv);
     FE_HIP_TRY(ctx, hipMemsetAsync(w.x64, 0, sizeof(double) * n, ctx->stream));
     FE_HIP_TRY(ctx, hipMemcpyAsync(w.r64, ctx->d_rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
     double rn = bn;
     const int maxOuter = 60;
+    bool inner32 = true;
     while(outer < maxOuter) {
       relres = rn / bn;
       if(rn <= target) { converged = 1; reason = 2; break; }
       if(iters >= o->max_iter) { reason = -3; break; }
-      double tol2 = (o->inner_rtol * rn) * (o->inner_rtol * rn);
+      double tol2 = inner32 ? (o->inner_rtol * rn) * (o->inner_rtol * rn) : target * target * 0.25;
       tol2 = std::max(tol2, target * target * 0.01);
-      launchPipe<float>(ctx, w, w.q32, ctx->d_val32, 0, 0, 0., true, w.r64, 0);
       int it0 = 0;
-      if(int rc = runPipe<float>(ctx, w, w.q32, ctx->d_val32, checkEvery, o->max_iter - iters, tol2, &it0, &done))
-        return rc;
+      const float *d32 = ctx->pipeKernel ? w.t32.x : w.q32.x;
+      FE_HIP_TRY(ctx, hipMemcpyAsync(w.tmp64, w.x64, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+      if(inner32) {
+        startJacobi<float>(ctx, w, w.q32, w.t32, ctx->d_val32, w.r64, 0);
+        if(int rc = runJacobi<float>(ctx, w, w.q32, w.t32, ctx->d_val32, checkEvery, o->max_iter - iters, tol2, &it0,
+                                     &done))
+          return rc;
+        hipLaunchKernelGGL(addVec<float>, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, w.x64, d32);
+      } else {
+        // FP64 inner solve of A d = r (after an FP32 correction was rejected)
+        hipLaunchKernelGGL(extractDinv<double>, dim3((n + kTB - 1) / kTB), dim3(kTB), 0, ctx->stream, n,
+                           ctx->d_diag, ctx->d_val, ctx->pipeKernel ? w.t64.dinv : w.q64.dinv);
+        startJacobi<double>(ctx, w, w.q64, w.t64, ctx->d_val, w.r64, 0);
+        if(int rc = runJacobi<double>(ctx, w, w.q64, w.t64, ctx->d_val, checkEvery, o->max_iter - iters, tol2, &it0,
+                                      &done))
+          return rc;
+        hipLaunchKernelGGL(addVec<double>, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, w.x64,
+                           ctx->pipeKernel ? w.t64.x : w.q64.x);
+      }
       iters += it0;
-      hipLaunchKernelGGL(addVec<float>, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, w.x64, w.q32.x);
       double rr = 0.;
       if(int rc = residualNorm2(ctx, w, w.x64, ctx->d_rhs, w.r64, &rr)) return rc;
-      rn = std::sqrt(rr);
       outer++;
+      if(inner32 && !(std::sqrt(rr) < 0.9 * rn)) {
+        // the FP32 correction did not reduce the FP64 residual: undo it and
+        // finish with FP64 inner solves
+        FE_HIP_TRY(ctx, hipMemcpyAsync(w.x64, w.tmp64, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+        if(int rc = residualNorm2(ctx, w, w.x64, ctx->d_rhs, w.r64, &rr)) return rc;
+        inner32 = false;
+        continue;
+      }
+      rn = std::sqrt(rr);
       if(done == 3 || done == 4) { reason = -3; break; }
       if(rn / bn > o->dtol) { reason = -4; break; }
     }

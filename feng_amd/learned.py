@@ -1,0 +1,92 @@
+"""Learned IC(0)-pattern preconditioner: weights, training and use.
+
+The network and its training kernels live in libfe_hip.so (fe_learned.hip);
+this module drives them.  Training follows the instance recipe of the
+learned-preconditioner workload: P1 tetrahedral diffusion on the unit cube
+(N^3 hexes x 6 Kuhn tets) with k = exp(GRF, length 0.1, sigma 1.5), one
+seeded coefficient field per instance, each instance assembled on the GPU,
+factored (IC(0)) and used for one Adam step of the probe loss.
+
+Weights are a flat float32 vector (fe_hip_mpnn_param_count() entries) saved
+as .npy; `default_weights()` is the file trained by `python -m
+feng_amd.learned` and shipped in feng_amd/weights/.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import api as A
+from .problems import diffusion_system
+
+WEIGHTS_DIR = Path(__file__).resolve().parent / "weights"
+DEFAULT_WEIGHTS = WEIGHTS_DIR / "mpnn_ic0_cube32.npy"
+
+
+def default_weights() -> np.ndarray:
+    if not DEFAULT_WEIGHTS.exists():
+        raise FileNotFoundError(f"{DEFAULT_WEIGHTS} is missing: train with `python -m feng_amd.learned`")
+    return np.load(DEFAULT_WEIGHTS, allow_pickle=False)
+
+
+def load_weights(S: A.LinearSystemHIP, theta=None):
+    S.mpnn_set_params(default_weights() if theta is None else theta)
+
+
+def cube_instance(N: int, seed: int, nfeat: int = 512, device: int = 0):
+    """Assembled P1 tet diffusion system of one training instance."""
+    m = A.Mesh.cube(N)
+    k = m.coeff_grf(0.1, 1.5, nfeat, seed)
+    S, num = diffusion_system(m, A.TET_P1, 1, A.Fn(A.FN_ELEM, data=k), A.Fn(A.FN_CONST, -1.0), device=device)
+    S.setToZero()
+    S.assemble(3, A.PREC_FP64)
+    return m, S, num
+
+
+def train(N=32, instances=2000, seed0=1000, init_seed=7, lr=2e-3, smooth_steps=4, nfeat=512, log_every=100,
+          theta0=None, out=None, device=0):
+    """One Adam step per instance, instances seeded seed0 .. seed0 + instances - 1.
+    Returns (theta, history)."""
+    m, S, num = cube_instance(N, seed0, nfeat, device)
+    if theta0 is None:
+        S.mpnn_init(init_seed)
+    else:
+        S.mpnn_set_params(theta0)
+    hist = []
+    t0 = time.perf_counter()
+    for it in range(instances):
+        if it > 0:
+            S.update_coeff(0, 0, m.coeff_grf(0.1, 1.5, nfeat, seed0 + it))
+            S.setToZero()
+            S.assemble(1, A.PREC_FP64)
+        loss = S.mpnn_train_step(lr=lr, smooth_steps=smooth_steps, seed=seed0 + it)
+        hist.append(loss)
+        if log_every and (it % log_every == 0 or it == instances - 1):
+            print(json.dumps({"instance": it, "loss": loss, "s": round(time.perf_counter() - t0, 1)}), flush=True)
+    theta = S.mpnn_get_params()
+    if out is not None:
+        Path(out).parent.mkdir(parents=True, exist_ok=True)
+        np.save(out, theta, allow_pickle=False)
+    S.close()
+    return theta, hist
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="train the learned IC(0)-pattern preconditioner on the GPU")
+    ap.add_argument("--N", type=int, default=32)
+    ap.add_argument("--instances", type=int, default=2000)
+    ap.add_argument("--seed0", type=int, default=1000)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--smooth", type=int, default=4)
+    ap.add_argument("--out", default=str(DEFAULT_WEIGHTS))
+    a = ap.parse_args(argv)
+    train(a.N, a.instances, a.seed0, lr=a.lr, smooth_steps=a.smooth, out=a.out)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

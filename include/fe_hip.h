@@ -199,6 +199,66 @@ int fe_hip_set_profiling(fe_hip_ctx *ctx, int on);
 int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value);
 int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value);
 
+/* Replace the values of coefficient `index` of form `form` (an FE_FN_ELEM or
+ * FE_FN_QP table, same length as given to fe_hip_add_form).  Replaces the
+ * reference's per-instance feFunction objects (feFunction.h) when many
+ * coefficient fields are assembled on one mesh. */
+int fe_hip_update_coeff(fe_hip_ctx *ctx, int form, int index, const double *data, int64_t count);
+
+/* ---- batched independent problems ---------------------------------------
+ * No reference counterpart (the reference solves one system per
+ * feLinearSystem).  The assembled system must be block diagonal: problem p
+ * owns the unknown rows [rows[p], rows[p+1]) (rows[0] = 0, rows[nprob] = n)
+ * and couples to no other problem, as when the meshes of many problems are
+ * uploaded as one disjoint mesh with per-problem contiguous numbering.  Every
+ * problem gets its own Jacobi-PCG (own scalars, own stop on
+ * ||b_p - A_p x_p||_2 <= max(rtol ||b_p||_2, atol), FP64), all problems
+ * advancing in the same launches; FE_PREC_FP32_IR runs the inner solves on
+ * the FP32 values with FP64 refinement.  The correction goes to the device
+ * du (fe_hip_correct_solution applies it).  ms_solve: device time. */
+typedef struct {
+  int iterations;
+  int converged;
+  double relres_fp64;
+} fe_hip_batch_stats;
+int fe_hip_solve_batched(fe_hip_ctx *ctx, int nprob, const int *rows, const fe_hip_solver_opts *o,
+                         fe_hip_batch_stats *stats, double *ms_solve);
+/* Times the batched iteration for every launch shape (lanes per row 2/4/8,
+ * block 256/512/1024) and keeps the fastest in the context's tuning. */
+int fe_hip_autotune_batched(fe_hip_ctx *ctx, int nprob, const int *rows, int precision, int *lanes, int *block,
+                            double *best_ms);
+
+/* ---- learned IC(0)-pattern correction (FE_PC_LEARNED) --------------------
+ * No reference counterpart (the reference's PCG uses PETSc's IC(0),
+ * feLinearSystemPETSc.cpp:900-1031).  A 3-layer, 32-wide message-passing
+ * network on the graph of A corrects the split IC(0) factor
+ * M^-1 = G^T D^-1 G (G = I - L_strict); with FE_PC_LEARNED, fe_hip_solve
+ * factors IC(0), runs the network and applies the corrected factor.
+ * Parameters: fe_hip_mpnn_param_count() floats.  Training minimises
+ * (1/8) sum_k ||xi_k - D^-1/2 G A^ G^T D^-1/2 xi_k||^2 over 8 smoothed random
+ * probes of the assembled matrix (A^ = unit-diagonal scaling of A). */
+typedef struct {
+  int smooth_steps;     /* damped-Jacobi sweeps applied to the probes */
+  double smooth_omega;  /* their damping */
+  double lr, beta1, beta2, eps; /* Adam */
+  uint64_t seed;        /* probe seed */
+} fe_hip_mpnn_opts;
+
+int fe_hip_mpnn_param_count(void);
+void fe_hip_mpnn_default_opts(fe_hip_mpnn_opts *o);
+int fe_hip_mpnn_init(fe_hip_ctx *ctx, uint64_t seed);
+int fe_hip_mpnn_set_params(fe_hip_ctx *ctx, const float *theta, int count);
+int fe_hip_mpnn_get_params(fe_hip_ctx *ctx, float *theta, int count);
+/* Factor IC(0) of the assembled matrix, run the network, the loss and its
+ * gradient (grad may be NULL). */
+int fe_hip_mpnn_loss_grad(fe_hip_ctx *ctx, const fe_hip_mpnn_opts *o, double *loss, float *grad);
+/* fe_hip_mpnn_loss_grad followed by one Adam step. */
+int fe_hip_mpnn_train_step(fe_hip_ctx *ctx, const fe_hip_mpnn_opts *o, double *loss);
+/* Intermediates of the last network run, as floats: "X" (n x 16), "h0".."h3",
+ * "m1".."m3" (n x 32), "p", "q" (n x 8), "tau", "d" (n), "g" (nnz, corrected
+ * normalised strict-lower factor entries), "xi" (n x 8 probes). */
+int fe_hip_mpnn_get_tensor(fe_hip_ctx *ctx, const char *name, float *out, int64_t count);
+
 #ifdef __cplusplus
 }
 #endif
