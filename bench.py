@@ -8,11 +8,18 @@ Dirichlet boundary, preconditioned CG stopped on the FP64 true residual
 ||b - A x||_2 / ||b||_2 <= 1e-8.
 
 One step = zero + assemble (all element kernels, colored scatter into the
-device CSR) + solve, inputs already resident in HBM.  Two paths are run:
-  plain : FP64 assembly + FP64 Jacobi-PCG
-  fast  : FP64 assembly (+FP32 copy) + FP32 Jacobi-PCG inner solves with FP64
-          iterative refinement
-`value` is the fast path's DOF/s; the plain path is reported beside it.
+device CSR) + solve, inputs already resident in HBM.  Paths:
+  plain   : FP64 assembly + FP64 Jacobi-PCG (pipelined, one kernel/iteration)
+  fast    : FP64 + FP32 (matrix-core) assembly, FP32 Jacobi-PCG inner solves,
+            FP64 iterative refinement
+  learned : as fast, with the learned IC(0)-pattern split preconditioner
+            (weights from feng_amd/weights/, skipped when absent)
+  ic0     : FP64 split IC(0)-pattern PCG (the learned path without the net)
+`value` is the DOF/s of the faster of fast/learned (named in config.path);
+every path is reported in config.paths.
+
+--workload batched runs the 256-problem batched workload instead;
+--copy-probe runs a plain device copy (counter calibration for rocprofv3).
 
 The cpu_baseline leg times the CPU oracle (a single-threaded C restatement of
 the reference element loop + CG/Jacobi) on the same problem.
@@ -32,6 +39,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the dominant kernels from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE (scaled by the same counters on a plain copy of known size).
+TRAFFIC = {}
 
 
 def log(*a):
@@ -50,16 +60,27 @@ def make_problem(N, nfeat, seed):
     return m, k, S, num
 
 
-def run_step(S, path, rtol, profile=False):
-    from feng_amd.api import PREC_FP32_IR, PREC_FP64
+PATHS = {
+    # name: (precision, preconditioner, inner rtol)
+    "plain": ("fp64", "jacobi", 1e-3),
+    "fast": ("fp32", "jacobi", 1e-2),
+    "learned": ("fp32", "learned", 1e-2),
+    "ic0": ("fp64", "ic0", 1e-3),
+}
 
-    prec = PREC_FP32_IR if path == "fast" else PREC_FP64
+
+def run_step(S, path, rtol, profile=False):
+    from feng_amd.api import PC_IC0, PC_JACOBI, PC_LEARNED, PREC_FP32_IR, PREC_FP64
+
+    precn, pcn, inner = PATHS[path]
+    prec = PREC_FP32_IR if precn == "fp32" else PREC_FP64
+    pc = {"jacobi": PC_JACOBI, "learned": PC_LEARNED, "ic0": PC_IC0}[pcn]
     S.set_profiling(profile)
     t0 = time.perf_counter()
     S.setToZero()
     S.assemble(3, prec)
     asm_ms, _ = S.kernel_stats(1)
-    r = S.solve(precision=prec, rtol=rtol, inner_rtol=1e-2 if path == "fast" else 1e-3, check_every=32)
+    r = S.solve(pc=pc, precision=prec, rtol=rtol, inner_rtol=inner, check_every=32)
     wall = time.perf_counter() - t0
     spmv_ms, launches = S.kernel_stats(0)
     return dict(wall_s=wall, asm_ms=asm_ms, solve_ms=r.ms_solve, iters=r.iterations, outer=r.outer_iterations,
@@ -98,6 +119,49 @@ def cpu_baseline(m, k, num, rtol, budget_s=25.0):
                 relres=rr)
 
 
+def copy_probe(mib=512, reps=5):
+    """A plain device copy of known size (calibrates the FETCH_SIZE and
+    WRITE_SIZE counters: each launch reads and writes `mib` MiB)."""
+    import torch
+
+    x = torch.empty(mib * 2 ** 20 // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "reps": reps}), flush=True)
+
+
+def bench_batched(args):
+    """256 independent P1/P2 diffusion problems (N_p in [32, 256]), FP32
+    assembly + batched FP32 Jacobi-PCG with FP64 refinement to rel-res 1e-8."""
+    from feng_amd.api import PREC_FP32_IR
+    from feng_amd.batched import build_batch, draw_problems, solve_batch
+
+    probs = draw_problems(256, args.seed)
+    out = {"workload": "batched", "problems": len(probs)}
+    tot_n, tot_ms, detail = 0, 0.0, {}
+    for kind, name in ((1, "P1"), (2, "P2")):
+        b = build_batch(probs, kind)
+        solve_batch(b, PREC_FP32_IR, rtol=1e-8, autotune=True)  # warmup + autotune
+        runs = [solve_batch(b, PREC_FP32_IR, rtol=1e-8, autotune=False) for _ in range(max(1, args.steps))]
+        r = runs[-1]
+        if not r["converged"].all():
+            raise SystemExit(f"batched {name}: {int((~r['converged']).sum())} problems did not converge")
+        ms = float(np.mean([x["asm_ms"] + x["solve_ms"] for x in runs]))
+        detail[name] = {"problems": len(b.nInc), "n": r["n"], "nnz": r["nnz"], "asm_ms": r["asm_ms"],
+                        "solve_ms": r["solve_ms"], "max_iterations": int(r["iterations"].max()),
+                        "max_relres": float(r["relres"].max()), "tuned": r["tuned"], "setup_s": b.setup_s}
+        tot_n += r["n"]
+        tot_ms += ms
+        b.S.close()
+        log(name, detail[name])
+    out.update({"metric": "DOF/s assemble+solve to FP64 rel-res 1e-8 (256 batched problems)",
+                "value": tot_n / (tot_ms / 1e3), "unit": "DOF/s", "ms_per_step": tot_ms, "n_total": tot_n,
+                "batches": detail, "dtype": "fp32-inner/fp64-refine", "n_gpus": 1})
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,51 +172,64 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=25.0)
+    ap.add_argument("--workload", default="cube", choices=["cube", "batched"])
+    ap.add_argument("--copy-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        return copy_probe()
+    if args.workload == "batched":
+        return bench_batched(args)
     rtol = 1e-8
     m, k, S, num = make_problem(args.N, args.nfeat, args.seed)
     n, nnz = num.nInc, S.nnz
+    paths = ["plain", "fast", "ic0"]
+    try:
+        from feng_amd.learned import load_weights
 
-    # plain FP64 path (reported beside the headline)
-    for _ in range(max(1, args.warmup)):
-        run_step(S, "plain", rtol)
-    plain = [run_step(S, "plain", rtol, profile=True) for _ in range(args.steps)]
-    log("plain", plain[-1])
-    # fast path: warmup, then the timed steps
-    for _ in range(args.warmup):
-        run_step(S, "fast", rtol)
-    t0 = time.perf_counter()
-    fast = [run_step(S, "fast", rtol, profile=True) for _ in range(args.steps)]
-    total = time.perf_counter() - t0
-    log("fast", fast[-1])
-    for r in fast + plain:
-        if not r["converged"] or r["relres"] > rtol:
-            raise SystemExit(f"solve did not reach FP64 rel-res {rtol}: {r}")
+        load_weights(S)
+        paths.append("learned")
+    except FileNotFoundError as e:
+        log("learned path skipped:", e)
 
-    ms_step = 1e3 * total / args.steps
-    dev_ms = np.mean([r["asm_ms"] + r["solve_ms"] for r in fast])
-    value = n / (ms_step / 1e3)
-    plain_ms = np.mean([r["wall_s"] for r in plain]) * 1e3
-    plain_dofs = n / (plain_ms / 1e3)
+    res = {}
+    for path in paths:
+        for _ in range(max(1, args.warmup)):
+            run_step(S, path, rtol)
+        t0 = time.perf_counter()
+        steps = [run_step(S, path, rtol, profile=True) for _ in range(args.steps)]
+        wall = (time.perf_counter() - t0) / args.steps
+        for r in steps:
+            if not r["converged"] or r["relres"] > rtol:
+                raise SystemExit(f"{path}: solve did not reach FP64 rel-res {rtol}: {r}")
+        res[path] = dict(ms=1e3 * wall, dofs=n / wall, steps=steps, last=steps[-1])
+        log(path, f"{1e3 * wall:.3f} ms/step", steps[-1])
 
-    # roofline of the dominant kernel (the fused pipelined-CG iteration,
-    # FP32): algorithmic bytes per launch = values 4 nnz + column ids 4 nnz +
-    # row pointers 4 (n+1) + own-row reads of D^-1, r, p, w, s_old, x (6 x 4 n)
-    # + writes of x, p, s, r_new, w_new (5 x 4 n)
-    spmv_ms = float(np.mean([r["spmv_ms"] for r in fast]))
-    alg_bytes = 8 * nnz + 4 * (n + 1) + 44 * n
-    achieved = alg_bytes / (spmv_ms * 1e-3) / 1e9
-    p_spmv_ms = float(np.mean([r["spmv_ms"] for r in plain]))
-    p_alg = 12 * nnz + 4 * (n + 1) + 88 * n
+    fast_paths = [p for p in ("fast", "learned") if p in res]
+    best = min(fast_paths, key=lambda p: res[p]["ms"])
+    B = res[best]
+    plain = res["plain"]
+
+    # roofline of the dominant kernel of the reported path: one PCG iteration
+    # launch.  Algorithmic bytes per iteration (minimal unfused PCG-Jacobi):
+    # values s nnz + column ids 4 nnz + row pointers 4 (n+1) + 18 vector
+    # passes of s n.  FP32 inner solve: s = 4.
+    kern_ms = float(np.mean([r["spmv_ms"] for r in B["steps"]]))
+    alg_bytes = 8 * nnz + 4 * (n + 1) + 72 * n
+    kname = "tileIter<float>" if best == "fast" else "cgSpmv<float> (split PCG product)"
+    if best == "learned":
+        alg_bytes = 8 * nnz + 4 * (n + 1) + 16 * n
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    p_kern = float(np.mean([r["spmv_ms"] for r in plain["steps"]]))
+    p_alg = 12 * nnz + 4 * (n + 1) + 144 * n
 
     cpu = None
     if not args.no_cpu:
         c = cpu_baseline(m, k, num, rtol, args.cpu_budget)
-        iters = c["iters"] if c["converged"] else plain[-1]["iters"]
+        iters = c["iters"] if c["converged"] else plain["last"]["iters"]
         t_cpu = c["t_asm"] + c["t_iter"] * iters
         cpu = {"value": n / t_cpu, "unit": "DOF/s", "cores": 1, "kind": "port",
                "sample": (f"oracle assembly of all {m.E} tets ({c['t_asm']:.2f}s) + CG/Jacobi "
@@ -160,14 +237,21 @@ def main():
                           f"{iters} iterations)")}
         log("cpu", c)
 
+    def summary(p):
+        r = res[p]
+        return {"ms": r["ms"], "dofs": r["dofs"], "iterations": r["last"]["iters"], "outer": r["last"]["outer"],
+                "relres": r["last"]["relres"], "asm_ms": float(np.mean([x["asm_ms"] for x in r["steps"]])),
+                "solve_ms": float(np.mean([x["solve_ms"] for x in r["steps"]])),
+                "iter_kernel_ms": float(np.mean([x["spmv_ms"] for x in r["steps"]]))}
+
     out = {
         "metric": "DOF/s assemble+solve to FP64 rel-res 1e-8; learned/FP32 vs plain FP64 PCG",
-        "value": value,
+        "value": B["dofs"],
         "unit": "DOF/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": B["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -175,17 +259,14 @@ def main():
         "data": "synthetic (seeded GRF coefficient on a generated Kuhn-tet cube)",
         "config": {"workload": f"P1 tet diffusion, unit cube {args.N}^3 x6 Kuhn tets, k=exp(GRF l=0.1 s=1.5), "
                                f"PCG to FP64 rel-res 1e-8", "n": n, "nnz": nnz, "elements": m.E,
-                   "path": "FP32 Jacobi-PCG + FP64 iterative refinement",
-                   "device_ms_per_step": dev_ms,
-                   "asm_ms": float(np.mean([r["asm_ms"] for r in fast])),
-                   "solve_ms": float(np.mean([r["solve_ms"] for r in fast])),
-                   "iterations": fast[-1]["iters"], "outer": fast[-1]["outer"], "relres": fast[-1]["relres"],
-                   "plain_fp64_dofs": plain_dofs, "plain_fp64_ms": plain_ms,
-                   "plain_iterations": plain[-1]["iters"], "speedup_vs_plain_fp64": plain_ms / ms_step,
-                   "plain_spmv_roofline_frac": p_alg / (p_spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if p_spmv_ms else None},
-        "roofline": {"bound": "hbm", "kernel": "pipeIter<float>", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "avg_launch_ms": spmv_ms, "alg_bytes_per_launch": alg_bytes},
+                   "path": {"fast": "FP32 Jacobi-PCG + FP64 iterative refinement",
+                            "learned": "FP32 learned-IC(0) split PCG + FP64 iterative refinement"}[best],
+                   "paths": {p: summary(p) for p in res},
+                   "speedup_vs_plain_fp64": plain["ms"] / B["ms"],
+                   "plain_iter_roofline_frac": p_alg / (p_kern * 1e-3) / 1e9 / HBM_PEAK_GBS if p_kern else None},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(kname),
+                     "avg_launch_ms": kern_ms, "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -35,8 +35,7 @@ class FeHipSolveStats(C.Structure):
 
 
 class FeHipMpnnOpts(C.Structure):
-    _fields_ = [("smooth_steps", C.c_int), ("smooth_omega", C.c_double), ("lr", C.c_double),
-                ("beta1", C.c_double), ("beta2", C.c_double), ("eps", C.c_double), ("seed", C.c_uint64)]
+    _fields_ = [("lr", C.c_double), ("beta1", C.c_double), ("beta2", C.c_double), ("eps", C.c_double)]
 
 
 class FeHipBatchStats(C.Structure):
@@ -105,7 +104,7 @@ HIP_API = {
     "fe_hip_mpnn_init": (C.c_int, [C.c_void_p, C.c_uint64]),
     "fe_hip_mpnn_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int]),
     "fe_hip_mpnn_get_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int]),
-    "fe_hip_mpnn_loss_grad": (C.c_int, [C.c_void_p, C.POINTER(FeHipMpnnOpts), c_double_p, C.POINTER(C.c_float)]),
+    "fe_hip_mpnn_loss_grad": (C.c_int, [C.c_void_p, c_double_p, C.POINTER(C.c_float)]),
     "fe_hip_mpnn_train_step": (C.c_int, [C.c_void_p, C.POINTER(FeHipMpnnOpts), c_double_p]),
     "fe_hip_mpnn_get_tensor": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_float), C.c_int64]),
 }

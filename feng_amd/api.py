@@ -336,12 +336,10 @@ class LinearSystemHIP:
         self._c(self.L.fe_hip_mpnn_get_params(self.ctx, N.ptr(t, C.c_float), t.size), "fe_hip_mpnn_get_params")
         return t
 
-    def mpnn_loss_grad(self, want_grad=True, **kw):
-        o = self.mpnn_opts(**kw)
+    def mpnn_loss_grad(self, want_grad=True):
         loss = C.c_double()
         g = np.zeros(self.L.fe_hip_mpnn_param_count(), np.float32) if want_grad else None
-        self._c(self.L.fe_hip_mpnn_loss_grad(self.ctx, C.byref(o), C.byref(loss),
-                                             N.ptr(g, C.c_float) if g is not None else None),
+        self._c(self.L.fe_hip_mpnn_loss_grad(self.ctx, C.byref(loss), N.ptr(g, C.c_float) if g is not None else None),
                 "fe_hip_mpnn_loss_grad")
         return loss.value, g
 
@@ -352,8 +350,8 @@ class LinearSystemHIP:
         return loss.value
 
     def mpnn_tensor(self, name):
-        sizes = {"X": self.n * 16, "p": self.n * 8, "q": self.n * 8, "tau": self.n, "d": self.n,
-                 "g": self.nnz, "xi": self.n * 8}
+        sizes = {"X": self.n * 16, "p": self.n * 8, "q": self.n * 8, "heads": self.n * 5, "B": self.n,
+                 "d": self.n, "g": self.nnz, "w0": self.nnz, "w": self.nnz}
         cnt = sizes.get(name, self.n * 32)
         out = np.zeros(cnt, np.float32)
         self._c(self.L.fe_hip_mpnn_get_tensor(self.ctx, name.encode(), N.ptr(out, C.c_float), cnt),

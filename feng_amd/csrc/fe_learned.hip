@@ -5,23 +5,30 @@
 // IC(0) pivots; it is SPD for any G with a unit diagonal and any D > 0, so a
 // correction of G and D can never make PCG break down.  A message-passing
 // network on the matrix graph predicts the correction, in the coordinates of
-// the unit-diagonal matrix A^ = S A S (s_i = a_ii^-1/2):
-//   g^_ij = g^0_ij + a^_ij (p_i . q_j)       strict lower entries j < i
-//   d^_i  = d^0_i exp(clamp(tau_i, -4, 4))
-//   x_i   = [1, d^0_i, sum |a^_ij|, sum a^_ij, deg_i/16, sum_{j<i} g^0_ij^2, 0..]
+// the unit-diagonal matrix A^ = S A S (s_i = a_ii^-1/2), on the strict lower
+// entries (i, j), j < i:
+//   w0_ij = (A^ g^0_i)_j                   (row i of G^0 times A^)
+//   g^_ij = g^0_ij - (sigma_i + rho_j) w0_ij + a^_ij (p_i . q_j + s_i + r_j)
+//   d^_i  = B_ii exp(clamp(tau_i, -4, 4)),  B_ii = g^_i^T A^ g^_i
+//   x_i   = [1, d^0_i, sum |a^_ij|, sum a^_ij, deg_i/16, sum_{j<i} g^0_ij^2,
+//            sum_{j<i} w0_ij^2, 0..]
 //   h0 = relu(x W0 + b0),  m_l = A^_off h_{l-1},
 //   h_l = relu([h_{l-1}, m_l] W_l + b_l)  (l = 1..3),
-//   p = h3 P, q = h3 Q, tau = h3 wt + bt            (H = 32, p/q 8 wide)
+//   p = h3 P, q = h3 Q (8 wide), [tau, s, r, sigma, rho] = h3 Hs + hb.
+// w0 is the gradient of the loss below at the IC(0) factor, so the network
+// learns step lengths along it plus corrections proportional to a^.
 // Activations are stored as bfloat16; the dense layers and both GEMMs of
 // their backward run on v_mfma_f32_16x16x16bf16_1k with FP32 accumulation,
-// the heads and the loss in FP32.
+// the heads, the factor and the loss in FP32.
 //
-// Training minimises, over K = 8 probe vectors (random signs smoothed by
-// damped-Jacobi sweeps of A^, then normalised), the loss
-//   L = (1/K) sum_k || xi_k - D^^-1/2 G^ A^ G^^T D^^-1/2 xi_k ||^2,
-// which weights the low end of the spectrum of the preconditioned operator.
-// The backward is written out by hand; every parameter gradient is a
-// per-block partial folded by a second kernel (no atomics, no fences).
+// Training minimises Kaporin's condition number of the preconditioned
+// operator A~ = D^^-1/2 G^ A^ G^^T D^^-1/2, K = (tr A~ / n)^n / det A~, a
+// bound on the CG iteration count (Kaporin 1994); up to a constant,
+//   L = log K / n = log((1/n) sum_i e^-tau_i) + mean(tau) + mean(log B_ii),
+// whose minimiser over the pattern is the FSAI factor (Kolotilina & Yeremin
+// 1993).  dL/dg^_ij = 2 w_ij / (n B_ii) with w_ij = (A^ g^_i)_j.  The backward
+// is written out by hand; every parameter gradient is a per-block partial
+// folded by a second kernel (no atomics, no fences).
 #include "fe_hip_internal.h"
 
 #include <cmath>
@@ -31,16 +38,18 @@
 
 namespace {
 
-constexpr int kH = 32, kNin = 16, kNpq = 8, kK = 8;
+constexpr int kH = 32, kNin = 16, kNpq = 8, kNs = 5;
 constexpr int oW0 = 0, ob0 = oW0 + kNin * kH, oW1 = ob0 + kH, ob1 = oW1 + 2 * kH * kH, oW2 = ob1 + kH,
               ob2 = oW2 + 2 * kH * kH, oW3 = ob2 + kH, ob3 = oW3 + 2 * kH * kH, oP = ob3 + kH,
-              oQ = oP + kH * kNpq, owt = oQ + kH * kNpq, obt = owt + kH, kParams = obt + 1;
+              oQ = oP + kH * kNpq, oHs = oQ + kH * kNpq, ohb = oHs + kH * kNs, kParams = ohb + kNs;
+constexpr int kHeadW = kParams - oP;           // P, Q, Hs, hb: contiguous
 constexpr int kW16 = kNin * kH + 6 * kH * kH;  // bf16 copies of W0..W3, packed
 constexpr int kW16Off[4] = {0, kNin * kH, kNin * kH + 2 * kH * kH, kNin * kH + 4 * kH * kH};
 constexpr int kWOff[4] = {oW0, oW1, oW2, oW3};
 constexpr int kBOff[4] = {ob0, ob1, ob2, ob3};
 constexpr float kTauClamp = 4.f;
 constexpr int kGemmGrid = 128;  // blocks of the parameter-gradient GEMMs
+constexpr int kLossGrid = 256;  // blocks of the loss reduction
 
 typedef short feShort4 __attribute__((ext_vector_type(4)));
 typedef float feF4 __attribute__((ext_vector_type(4)));
@@ -82,31 +91,76 @@ __global__ void mpScale(int n, const int *__restrict__ diag, const double *__res
   d0[i] = lu[diag[i]] / a;
 }
 
-// a^ on the pattern, g^0 on the strict lower entries, input features.
-__global__ void mpFeatures(int n, const int *__restrict__ ia, const int *__restrict__ ja,
-                           const double *__restrict__ val, const double *__restrict__ lu,
-                           const double *__restrict__ s, const double *__restrict__ d0, float *__restrict__ ahat,
-                           float *__restrict__ g0, bf16_t *__restrict__ X)
+// a^ on the pattern, g^0 on the strict lower entries.
+__global__ void mpNormalise(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                            const double *__restrict__ val, const double *__restrict__ lu,
+                            const double *__restrict__ s, float *__restrict__ ahat, float *__restrict__ g0)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if(i >= n) return;
   const double si = s[i];
-  double sa = 0., ss = 0., sg = 0.;
-  int deg = 0;
   for(int p = ia[i]; p < ia[i + 1]; ++p) {
     const int j = ja[p];
-    const double ah = val[p] * si * s[j];
-    ahat[p] = (float)ah;
-    if(j != i) {
-      sa += fabs(ah);
-      ss += ah;
-      ++deg;
+    ahat[p] = (float)(val[p] * si * s[j]);
+    if(j < i) g0[p] = (float)(-lu[p] * si / s[j]);
+  }
+}
+
+// B_ii = g^_i^T A^ g^_i and w_e = (A^ g^_i)_j on the strict lower entries
+// e = (i, j) of row i, for G^ = I + g (strict lower values on the pattern).
+// Row i's factor pattern P_i = its entries up to the diagonal; (A^ g^_i)_j
+// merges row j of A^ with P_i (both sorted).
+__global__ void mpRows(int n, const int *__restrict__ ia, const int *__restrict__ ja, const int *__restrict__ diag,
+                       const float *__restrict__ ahat, const float *__restrict__ g, float *__restrict__ B,
+                       float *__restrict__ w)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const int p0 = ia[i], pd = diag[i];
+  double b = 0.;
+  for(int e = p0; e <= pd; ++e) {
+    const int j = ja[e];
+    double t = 0.;
+    int a = ia[j], q = p0;
+    const int ae = ia[j + 1];
+    while(a < ae && q <= pd) {
+      const int ca = ja[a], cq = ja[q];
+      if(ca == cq) {
+        t += (double)ahat[a] * (q == pd ? 1. : (double)g[q]);
+        ++a;
+        ++q;
+      } else if(ca < cq) {
+        ++a;
+      } else {
+        ++q;
+      }
     }
-    if(j < i) {
-      const double g = -lu[p] * si / s[j];
-      g0[p] = (float)g;
-      sg += g * g;
-    }
+    const double ge = (e == pd) ? 1. : (double)g[e];
+    b += ge * t;
+    if(e < pd) w[e] = (float)t;
+  }
+  B[i] = (float)b;
+}
+
+// Input features.
+__global__ void mpFeatures(int n, const int *__restrict__ ia, const int *__restrict__ ja, const int *__restrict__ diag,
+                           const float *__restrict__ ahat, const float *__restrict__ g0, const float *__restrict__ w0,
+                           const double *__restrict__ d0, bf16_t *__restrict__ X)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  double sa = 0., ss = 0., sg = 0., sw = 0.;
+  int deg = 0;
+  for(int p = ia[i]; p < ia[i + 1]; ++p) {
+    if(ja[p] == i) continue;
+    const double ah = ahat[p];
+    sa += fabs(ah);
+    ss += ah;
+    ++deg;
+  }
+  for(int p = ia[i]; p < diag[i]; ++p) {
+    sg += (double)g0[p] * g0[p];
+    sw += (double)w0[p] * w0[p];
   }
   bf16_t *x = X + (size_t)i * kNin;
   x[0] = toBf16(1.f);
@@ -115,8 +169,9 @@ __global__ void mpFeatures(int n, const int *__restrict__ ia, const int *__restr
   x[3] = toBf16((float)ss);
   x[4] = toBf16((float)(deg / 16.));
   x[5] = toBf16((float)sg);
+  x[6] = toBf16((float)sw);
 #pragma unroll
-  for(int k = 6; k < kNin; ++k) x[k] = 0;
+  for(int k = 7; k < kNin; ++k) x[k] = 0;
 }
 
 // Position of the mirrored entry (j, i) of every entry (i, j); -1 if absent.
@@ -208,22 +263,18 @@ __global__ __launch_bounds__(256) void mpAggregate(int n, const int *__restrict_
   *(unsigned *)(Mout + (size_t)row * 32 + f) = (unsigned)toBf16(s0) | ((unsigned)toBf16(s1) << 16);
 }
 
-// p = h3 P, q = h3 Q, tau = h3 wt + bt (FP32, one thread per row).
+// p = h3 P, q = h3 Q, hs = h3 Hs + hb (FP32, one thread per row).
 __global__ __launch_bounds__(256) void mpHeads(int n, const bf16_t *__restrict__ h3, const float *__restrict__ theta,
-                                               float *__restrict__ p, float *__restrict__ q, float *__restrict__ tau)
+                                               float *__restrict__ p, float *__restrict__ q, float *__restrict__ hs)
 {
-  __shared__ float w[obt + 1 - oP];
-  for(int k = threadIdx.x; k < obt + 1 - oP; k += blockDim.x) w[k] = theta[oP + k];
+  __shared__ float w[kHeadW];
+  for(int k = threadIdx.x; k < kHeadW; k += blockDim.x) w[k] = theta[oP + k];
   __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if(i >= n) return;
   float h[kH];
 #pragma unroll
   for(int f = 0; f < kH; ++f) h[f] = fromBf16(h3[(size_t)i * kH + f]);
-  float t = w[obt - oP];
-#pragma unroll
-  for(int f = 0; f < kH; ++f) t += h[f] * w[owt - oP + f];
-  tau[i] = t;
 #pragma unroll
   for(int k = 0; k < kNpq; ++k) {
     float a = 0.f, b = 0.f;
@@ -235,28 +286,44 @@ __global__ __launch_bounds__(256) void mpHeads(int n, const bf16_t *__restrict__
     p[(size_t)i * kNpq + k] = a;
     q[(size_t)i * kNpq + k] = b;
   }
+#pragma unroll
+  for(int c = 0; c < kNs; ++c) {
+    float a = w[ohb - oP + c];
+#pragma unroll
+    for(int f = 0; f < kH; ++f) a += h[f] * w[oHs - oP + f * kNs + c];
+    hs[(size_t)i * kNs + c] = a;
+  }
 }
 
-// g^ = g^0 + a^ (p_i . q_j) on strict lower entries; d^ = d^0 exp(clamp(tau)).
+// g^ = g^0 - (sigma_i + rho_j) w0 + a^ (p_i . q_j + s_i + r_j) on strict lower
+// entries.
 __global__ void mpCorrect(int n, const int *__restrict__ ia, const int *__restrict__ ja, const int *__restrict__ diag,
-                          const float *__restrict__ ahat, const float *__restrict__ g0, const float *__restrict__ p,
-                          const float *__restrict__ q, const float *__restrict__ tau, const double *__restrict__ d0,
-                          float *__restrict__ g, float *__restrict__ d)
+                          const float *__restrict__ ahat, const float *__restrict__ g0, const float *__restrict__ w0,
+                          const float *__restrict__ p, const float *__restrict__ q, const float *__restrict__ hs,
+                          float *__restrict__ g)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if(i >= n) return;
   float pi[kNpq];
 #pragma unroll
   for(int k = 0; k < kNpq; ++k) pi[k] = p[(size_t)i * kNpq + k];
+  const float si = hs[(size_t)i * kNs + 1], sigi = hs[(size_t)i * kNs + 3];
   for(int e = ia[i]; e < diag[i]; ++e) {
     const int j = ja[e];
-    float c = 0.f;
+    float c = si + hs[(size_t)j * kNs + 2];
 #pragma unroll
     for(int k = 0; k < kNpq; ++k) c += pi[k] * q[(size_t)j * kNpq + k];
-    g[e] = g0[e] + ahat[e] * c;
+    g[e] = g0[e] - (sigi + hs[(size_t)j * kNs + 4]) * w0[e] + ahat[e] * c;
   }
-  const float t = fminf(fmaxf(tau[i], -kTauClamp), kTauClamp);
-  d[i] = (float)(d0[i] * exp((double)t));
+}
+
+// d^_i = B_ii exp(clamp(tau_i)).
+__global__ void mpPivots(int n, const float *__restrict__ B, const float *__restrict__ hs, float *__restrict__ d)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const float t = fminf(fmaxf(hs[(size_t)i * kNs], -kTauClamp), kTauClamp);
+  d[i] = (float)((double)B[i] * exp((double)t));
 }
 
 // Corrected split factor in the original coordinates, written over the IC(0)
@@ -288,174 +355,122 @@ __global__ void mpWriteFactor(int n, const int *__restrict__ ia, const int *__re
 }
 
 // ---------------------------------------------------------------------------
-// Loss: n x K row-major FP32 blocks, 8 lanes per row (lane k = probe k).
+// Loss and its gradient.
 
-__global__ void mpProbes(int n, uint64_t seed, float *__restrict__ xi)
+// Per-block partials of sum log B_i, sum e^-tau_i, sum tau_i (tau clamped).
+__global__ __launch_bounds__(256) void mpLossParts(int n, const float *__restrict__ B, const float *__restrict__ hs,
+                                                   double *__restrict__ part)
 {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if(t >= n * kK) return;
-  const uint64_t h = splitmix64(((seed & 0xFFFFFFFFull) << 32) + (uint64_t)t);
-  xi[t] = (h >> 63) ? -1.f : 1.f;
-}
-
-// out = beta x + alpha A^ x
-__global__ void mpSpmmA(int n, const int *__restrict__ ia, const int *__restrict__ ja, const float *__restrict__ ahat,
-                        const float *__restrict__ x, float alpha, float beta, float *__restrict__ out)
-{
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = t >> 3, k = t & 7;
-  if(i >= n) return;
-  float s = 0.f;
-  for(int p = ia[i]; p < ia[i + 1]; ++p) s += ahat[p] * x[(size_t)ja[p] * kK + k];
-  out[(size_t)i * kK + k] = beta * x[(size_t)i * kK + k] + alpha * s;
-}
-
-// y = G^^T x: y_j = x_j + sum_{i > j} g^_ij x_i (the upper entries of row j,
-// values at the mirrored lower position).
-__global__ void mpApplyGT(int n, const int *__restrict__ ia, const int *__restrict__ ja, const int *__restrict__ diag,
-                          const int *__restrict__ tpos, const float *__restrict__ g, const float *__restrict__ x,
-                          float *__restrict__ y)
-{
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = t >> 3, k = t & 7;
-  if(j >= n) return;
-  float s = x[(size_t)j * kK + k];
-  for(int p = diag[j] + 1; p < ia[j + 1]; ++p) s += g[tpos[p]] * x[(size_t)ja[p] * kK + k];
-  y[(size_t)j * kK + k] = s;
-}
-
-// y = G^ x: y_i = x_i + sum_{j < i} g^_ij x_j.
-__global__ void mpApplyG(int n, const int *__restrict__ ia, const int *__restrict__ ja, const int *__restrict__ diag,
-                         const float *__restrict__ g, const float *__restrict__ x, float *__restrict__ y)
-{
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = t >> 3, k = t & 7;
-  if(i >= n) return;
-  float s = x[(size_t)i * kK + k];
-  for(int p = ia[i]; p < diag[i]; ++p) s += g[p] * x[(size_t)ja[p] * kK + k];
-  y[(size_t)i * kK + k] = s;
-}
-
-// Per-block partial sums of squares of each of the K columns.
-__global__ __launch_bounds__(256) void mpColSq(int n, const float *__restrict__ x, double *__restrict__ part)
-{
-  __shared__ double lds[256];
-  const int k = threadIdx.x & 7;
-  double s = 0.;
-  for(int t = blockIdx.x * 256 + threadIdx.x; t < n * kK; t += gridDim.x * 256) s += (double)x[t] * (double)x[t];
-  lds[threadIdx.x] = s;
-  __syncthreads();
-  if(threadIdx.x < kK) {
-    double a = 0.;
-    for(int r = threadIdx.x; r < 256; r += kK) a += lds[r];
-    part[blockIdx.x * kK + k] = a;
+  __shared__ double lds[3][4];
+  double lb = 0., et = 0., tt = 0.;
+  for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const double t = fmin(fmax((double)hs[(size_t)i * kNs], -(double)kTauClamp), (double)kTauClamp);
+    lb += log((double)B[i]);
+    et += exp(-t);
+    tt += t;
   }
-}
-
-// x[:, k] /= sqrt(sum of the partials of column k)
-__global__ void mpNormalize(int n, int G, const double *__restrict__ part, float *__restrict__ x)
-{
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if(t >= n * kK) return;
-  const int k = t & 7;
-  double s = 0.;
-  for(int b = 0; b < G; ++b) s += part[b * kK + k];
-  x[t] = (float)((double)x[t] / sqrt(s));
-}
-
-// a = xi / sqrt(d)
-__global__ void mpScaleProbes(int n, const float *__restrict__ xi, const float *__restrict__ d, float *__restrict__ a)
-{
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if(t >= n * kK) return;
-  a[t] = xi[t] / sqrtf(d[t >> 3]);
-}
-
-// v = u / sqrt(d), e = xi - v; loss partials sum e^2; gu = -(2/K) e / sqrt(d);
-// q1_i = sum_k gv_ik v_ik with gv = -(2/K) e.
-__global__ __launch_bounds__(256) void mpLossFwd(int n, const float *__restrict__ xi, const float *__restrict__ u,
-                                                 const float *__restrict__ d, float *__restrict__ gu,
-                                                 float *__restrict__ q1, double *__restrict__ part)
-{
-  __shared__ double lds[4];
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  double e2 = 0.;
-  float qv = 0.f;
-  if(t < n * kK) {
-    const float rs = 1.f / sqrtf(d[t >> 3]);
-    const float v = u[t] * rs;
-    const float e = xi[t] - v;
-    const float gv = -2.f / kK * e;
-    gu[t] = gv * rs;
-    qv = gv * v;
-    e2 = (double)e * (double)e;
+  lb = feWaveSum(lb);
+  et = feWaveSum(et);
+  tt = feWaveSum(tt);
+  if((threadIdx.x & 63) == 0) {
+    lds[0][threadIdx.x >> 6] = lb;
+    lds[1][threadIdx.x >> 6] = et;
+    lds[2][threadIdx.x >> 6] = tt;
   }
-#pragma unroll
-  for(int o = 4; o > 0; o >>= 1) qv += __shfl_xor(qv, o, 8);
-  if(t < n * kK && (t & 7) == 0) q1[t >> 3] = qv;
-  e2 = feWaveSum(e2);
-  if((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = e2;
   __syncthreads();
-  if(threadIdx.x == 0) part[blockIdx.x] = lds[0] + lds[1] + lds[2] + lds[3];
+  if(threadIdx.x < 3)
+    part[blockIdx.x * 3 + threadIdx.x] =
+        lds[threadIdx.x][0] + lds[threadIdx.x][1] + lds[threadIdx.x][2] + lds[threadIdx.x][3];
 }
 
-// Head gradients into gHead (n x 32: gp at 0..7, gq at 8..15, gtau at 16):
-//   ge_ij = sum_k gu_ik t_jk + a_ik gy_jk,  gc_ij = a^_ij ge_ij  (j < i),
-//   gp_i = sum_{j<i} gc_ij q_j,  gq_i = sum_{j>i} gc_ji p_j,
-//   gtau_i = -1/2 (sum_k ga_ik a_ik + q1_i) inside the clamp, else 0.
+// out[0] = L, out[1] = sum e^-tau (for the tau gradient).
+__global__ void mpLossFold(int n, int G, const double *__restrict__ part, double *__restrict__ out)
+{
+  if(threadIdx.x != 0 || blockIdx.x != 0) return;
+  double lb = 0., et = 0., tt = 0.;
+  for(int b = 0; b < G; ++b) {
+    lb += part[3 * b];
+    et += part[3 * b + 1];
+    tt += part[3 * b + 2];
+  }
+  out[0] = log(et / n) + tt / n + lb / n;
+  out[1] = et;
+}
+
+// Head gradients into gHead (n x 32: gp at 0..7, gq at 8..15, then gtau, gs,
+// gr, gsigma, grho at 16..20), with ge_ij = 2 w_ij / (n B_ii) on the strict
+// lower entries, gc = ge a^, gsr = -ge w0:
+//   gp_i = sum_{j<i} gc_ij q_j, gs_i = sum_{j<i} gc_ij, gsigma_i = sum_{j<i} gsr_ij,
+//   gq_i = sum_{j>i} gc_ji p_j, gr_i = sum_{j>i} gc_ji, grho_i = sum_{j>i} gsr_ji,
+//   gtau_i = (1 - n e^-tau_i / sum e^-tau) / n inside the clamp, else 0.
 __global__ __launch_bounds__(256) void mpEdgeGrad(int n, const int *__restrict__ ia, const int *__restrict__ ja,
                                                   const int *__restrict__ diag, const int *__restrict__ tpos,
-                                                  const float *__restrict__ ahat, const float *__restrict__ gu,
-                                                  const float *__restrict__ tt, const float *__restrict__ a,
-                                                  const float *__restrict__ gy, const float *__restrict__ ga,
-                                                  const float *__restrict__ q1, const float *__restrict__ p,
-                                                  const float *__restrict__ q, const float *__restrict__ tau,
+                                                  const float *__restrict__ ahat, const float *__restrict__ w,
+                                                  const float *__restrict__ w0, const float *__restrict__ B,
+                                                  const float *__restrict__ p, const float *__restrict__ q,
+                                                  const float *__restrict__ hs, const double *__restrict__ lossOut,
                                                   float *__restrict__ gHead)
 {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = t >> 3, k = t & 7;
-  if(i >= n) return;  // whole 8-lane groups exit together
-  const float gui = gu[(size_t)i * kK + k], ai = a[(size_t)i * kK + k];
-  const float ti = tt[(size_t)i * kK + k], gyi = gy[(size_t)i * kK + k];
-  float gp = 0.f, gq = 0.f;
-  for(int e = ia[i]; e < ia[i + 1]; ++e) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const float inv = 2.f / (float)n;
+  float gp[kNpq], gq[kNpq];
+#pragma unroll
+  for(int k = 0; k < kNpq; ++k) gp[k] = gq[k] = 0.f;
+  float gs = 0.f, gr = 0.f, gsig = 0.f, grho = 0.f;
+  const float bi = B[i];
+  for(int e = ia[i]; e < diag[i]; ++e) {
     const int j = ja[e];
-    if(j == i) continue;
-    float c;
-    if(j < i) c = gui * tt[(size_t)j * kK + k] + ai * gy[(size_t)j * kK + k];
-    else c = gu[(size_t)j * kK + k] * ti + a[(size_t)j * kK + k] * gyi;
+    const float ge = inv * w[e] / bi;
+    const float gc = ge * ahat[e];
 #pragma unroll
-    for(int o = 4; o > 0; o >>= 1) c += __shfl_xor(c, o, 8);
-    if(j < i) gp += c * ahat[e] * q[(size_t)j * kNpq + k];
-    else gq += c * ahat[tpos[e]] * p[(size_t)j * kNpq + k];
+    for(int k = 0; k < kNpq; ++k) gp[k] += gc * q[(size_t)j * kNpq + k];
+    gs += gc;
+    gsig -= ge * w0[e];
   }
-  float ta = ga[(size_t)i * kK + k] * ai;
+  for(int e = diag[i] + 1; e < ia[i + 1]; ++e) {
+    const int j = ja[e];
+    const int t = tpos[e];  // entry (j, i), strict lower in row j
+    const float ge = inv * w[t] / B[j];
+    const float gc = ge * ahat[t];
 #pragma unroll
-  for(int o = 4; o > 0; o >>= 1) ta += __shfl_xor(ta, o, 8);
+    for(int k = 0; k < kNpq; ++k) gq[k] += gc * p[(size_t)j * kNpq + k];
+    gr += gc;
+    grho -= ge * w0[t];
+  }
   float *gh = gHead + (size_t)i * 32;
-  gh[k] = gp;
-  gh[8 + k] = gq;
-  if(k == 0) {
-    const float tv = tau[i];
-    gh[16] = (tv > -kTauClamp && tv < kTauClamp) ? -0.5f * (ta + q1[i]) : 0.f;
+#pragma unroll
+  for(int k = 0; k < kNpq; ++k) {
+    gh[k] = gp[k];
+    gh[8 + k] = gq[k];
   }
-  (void)diag;
+  const float tv = hs[(size_t)i * kNs];
+  const double S = lossOut[1];
+  gh[16] = (tv > -kTauClamp && tv < kTauClamp)
+               ? (float)((1. - (double)n * exp(-(double)tv) / S) / (double)n)
+               : 0.f;
+  gh[17] = gs;
+  gh[18] = gr;
+  gh[19] = gsig;
+  gh[20] = grho;
 }
 
-// gz3 = bf16(mask(h3 > 0) (gp P^T + gq Q^T + gtau wt)), 32 lanes per row.
+// gz3 = bf16(mask(h3 > 0) (gp P^T + gq Q^T + ghs Hs^T)), 32 lanes per row.
 __global__ __launch_bounds__(256) void mpHeadsBwd(int n, const bf16_t *__restrict__ h3, const float *__restrict__ gHead,
                                                   const float *__restrict__ theta, bf16_t *__restrict__ gz)
 {
-  __shared__ float w[obt + 1 - oP];
-  for(int k = threadIdx.x; k < obt + 1 - oP; k += blockDim.x) w[k] = theta[oP + k];
+  __shared__ float w[kHeadW];
+  for(int k = threadIdx.x; k < kHeadW; k += blockDim.x) w[k] = theta[oP + k];
   __syncthreads();
   const int t = blockIdx.x * 256 + threadIdx.x;
   const int i = t >> 5, f = t & 31;
   if(i >= n) return;
   const float *gh = gHead + (size_t)i * 32;
-  float s = gh[16] * w[owt - oP + f];
+  float s = 0.f;
 #pragma unroll
   for(int k = 0; k < kNpq; ++k) s += gh[k] * w[f * kNpq + k] + gh[8 + k] * w[oQ - oP + f * kNpq + k];
+#pragma unroll
+  for(int c = 0; c < kNs; ++c) s += gh[16 + c] * w[oHs - oP + f * kNs + c];
   gz[(size_t)i * 32 + f] = fromBf16(h3[(size_t)i * 32 + f]) > 0.f ? toBf16(s) : (bf16_t)0;
 }
 
@@ -591,7 +606,7 @@ __global__ __launch_bounds__(256) void mpGemmTN(int n, int R, const bf16_t *__re
 
 // Fold the partials of mpGemmTN into the gradient vector.  mode 0: a dense
 // layer (weights at wOff, F1 x 32 row-major; biases at bOff); mode 1: the
-// heads (columns 0-7 -> P, 8-15 -> Q, 16 -> wt; column sum 16 -> bt).
+// heads (columns 0-7 -> P, 8-15 -> Q, 16-20 -> Hs; column sums 16-20 -> hb).
 __global__ void mpFoldGrad(int G, int F1, int mode, int wOff, int bOff, const float *__restrict__ part,
                            float *__restrict__ grad)
 {
@@ -603,27 +618,17 @@ __global__ void mpFoldGrad(int G, int F1, int mode, int wOff, int bOff, const fl
   if(mode == 0) {
     if(e < F1 * 32) grad[wOff + e] = (float)s;
     else grad[bOff + (e - F1 * 32)] = (float)s;
+  } else if(e < F1 * 32) {
+    const int f = e / 32, c = e % 32;
+    if(c < 8) grad[oP + f * kNpq + c] = (float)s;
+    else if(c < 16) grad[oQ + f * kNpq + c - 8] = (float)s;
+    else if(c < 16 + kNs) grad[oHs + f * kNs + c - 16] = (float)s;
   } else {
-    if(e < F1 * 32) {
-      const int f = e / 32, c = e % 32;
-      if(c < 8) grad[oP + f * kNpq + c] = (float)s;
-      else if(c < 16) grad[oQ + f * kNpq + c - 8] = (float)s;
-      else if(c == 16) grad[owt + f] = (float)s;
-    } else if(e - F1 * 32 == 16) {
-      grad[obt] = (float)s;
-    }
+    const int c = e - F1 * 32;
+    if(c >= 16 && c < 16 + kNs) grad[ohb + c - 16] = (float)s;
   }
 }
 
-__global__ void mpFoldLoss(int G, const double *__restrict__ part, double *__restrict__ out)
-{
-  if(threadIdx.x != 0 || blockIdx.x != 0) return;
-  double s = 0.;
-  for(int b = 0; b < G; ++b) s += part[b];
-  out[0] = s / kK;
-}
-
-// Adam (Kingma & Ba 2015, Alg. 1); c1 = 1 - beta1^t, c2 = 1 - beta2^t.
 __global__ void mpAdam(int count, float *__restrict__ theta, const float *__restrict__ grad, float *__restrict__ m,
                        float *__restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2)
 {
@@ -661,13 +666,12 @@ struct FeMpnn {
   int64_t nnz = 0;
   bool tposReady = false;
   float *theta = nullptr, *grad = nullptr, *adm = nullptr, *adv = nullptr;
-  double *s = nullptr, *d0 = nullptr, *lossPart = nullptr, *loss = nullptr, *colPart = nullptr;
-  float *ahat = nullptr, *g0 = nullptr, *g = nullptr, *d = nullptr;
+  double *s = nullptr, *d0 = nullptr, *lossPart = nullptr, *loss = nullptr;
+  float *ahat = nullptr, *g0 = nullptr, *w0 = nullptr, *g = nullptr, *w = nullptr, *B = nullptr, *B0 = nullptr,
+        *d = nullptr;
   int *tpos = nullptr, *missing = nullptr;
   unsigned short *X = nullptr, *h[4] = {}, *m[4] = {}, *gz = nullptr, *w16 = nullptr;
-  float *p = nullptr, *q = nullptr, *tau = nullptr;
-  float *xi = nullptr, *a = nullptr, *y = nullptr, *t = nullptr, *u = nullptr, *gu = nullptr, *gt = nullptr,
-        *gy = nullptr, *ga = nullptr, *q1 = nullptr, *gHead = nullptr, *gX = nullptr, *gpart = nullptr;
+  float *p = nullptr, *q = nullptr, *hs = nullptr, *gHead = nullptr, *gX = nullptr, *gpart = nullptr;
 };
 
 void feMpnnRelease(fe_hip_ctx *ctx)
@@ -684,7 +688,8 @@ int mpState(fe_hip_ctx *ctx, FeMpnn *&S)
     ctx->mpnn = new FeMpnn();
     FeMpnn &M = *ctx->mpnn;
     if(feAlloc(ctx, &M.theta, kParams) || feAlloc(ctx, &M.grad, kParams) || feAlloc(ctx, &M.adm, kParams) ||
-       feAlloc(ctx, &M.adv, kParams) || feAlloc(ctx, &M.w16, kW16) || feAlloc(ctx, &M.loss, 8))
+       feAlloc(ctx, &M.adv, kParams) || feAlloc(ctx, &M.w16, kW16) || feAlloc(ctx, &M.loss, 8) ||
+       feAlloc(ctx, &M.lossPart, 3 * kLossGrid))
       return FE_STATUS_OUT_OF_MEMORY;
   }
   S = ctx->mpnn;
@@ -701,26 +706,21 @@ int mpBuffers(fe_hip_ctx *ctx, FeMpnn &M)
   FE_REQUIRE(ctx, M.n == 0, "the learned preconditioner is bound to the first system it saw");
   M.n = n;
   M.nnz = nnz;
-  const size_t nK = (size_t)n * kK;
   const int gemmOut = 64 * 32 + 32;
   if(feAlloc(ctx, &M.s, n) || feAlloc(ctx, &M.d0, n) || feAlloc(ctx, &M.ahat, nnz) || feAlloc(ctx, &M.g0, nnz) ||
-     feAlloc(ctx, &M.g, nnz) || feAlloc(ctx, &M.d, n) || feAlloc(ctx, &M.tpos, nnz) || feAlloc(ctx, &M.missing, 1) ||
-     feAlloc(ctx, &M.X, (size_t)n * kNin) || feAlloc(ctx, &M.gz, (size_t)n * 32) || feAlloc(ctx, &M.p, (size_t)n * kNpq) ||
-     feAlloc(ctx, &M.q, (size_t)n * kNpq) || feAlloc(ctx, &M.tau, n) || feAlloc(ctx, &M.q1, n) ||
-     feAlloc(ctx, &M.gHead, (size_t)n * 32) || feAlloc(ctx, &M.gX, (size_t)n * 64) ||
-     feAlloc(ctx, &M.gpart, (size_t)kGemmGrid * gemmOut) || feAlloc(ctx, &M.lossPart, 4096) ||
-     feAlloc(ctx, &M.colPart, 256 * kK))
+     feAlloc(ctx, &M.w0, nnz) || feAlloc(ctx, &M.g, nnz) || feAlloc(ctx, &M.w, nnz) || feAlloc(ctx, &M.B, n) ||
+     feAlloc(ctx, &M.B0, n) || feAlloc(ctx, &M.d, n) || feAlloc(ctx, &M.tpos, nnz) || feAlloc(ctx, &M.missing, 1) ||
+     feAlloc(ctx, &M.X, (size_t)n * kNin) || feAlloc(ctx, &M.gz, (size_t)n * 32) ||
+     feAlloc(ctx, &M.p, (size_t)n * kNpq) || feAlloc(ctx, &M.q, (size_t)n * kNpq) ||
+     feAlloc(ctx, &M.hs, (size_t)n * kNs) || feAlloc(ctx, &M.gHead, (size_t)n * 32) ||
+     feAlloc(ctx, &M.gX, (size_t)n * 64) || feAlloc(ctx, &M.gpart, (size_t)kGemmGrid * gemmOut))
     return FE_STATUS_OUT_OF_MEMORY;
   for(int l = 0; l < 4; ++l)
     if(feAlloc(ctx, &M.h[l], (size_t)n * 32)) return FE_STATUS_OUT_OF_MEMORY;
   for(int l = 1; l < 4; ++l)
     if(feAlloc(ctx, &M.m[l], (size_t)n * 32)) return FE_STATUS_OUT_OF_MEMORY;
-  float **vk[] = {&M.xi, &M.a, &M.y, &M.t, &M.u, &M.gu, &M.gt, &M.gy, &M.ga};
-  for(auto pp : vk)
-    if(feAlloc(ctx, pp, nK)) return FE_STATUS_OUT_OF_MEMORY;
   // entries never written by the kernels stay zero
-  FE_HIP_TRY(ctx, hipMemsetAsync(M.g0, 0, sizeof(float) * nnz, ctx->stream));
-  FE_HIP_TRY(ctx, hipMemsetAsync(M.g, 0, sizeof(float) * nnz, ctx->stream));
+  for(float *a : {M.g0, M.w0, M.g, M.w}) FE_HIP_TRY(ctx, hipMemsetAsync(a, 0, sizeof(float) * nnz, ctx->stream));
   FE_HIP_TRY(ctx, hipMemsetAsync(M.gHead, 0, sizeof(float) * (size_t)n * 32, ctx->stream));
   M.tposReady = false;
   return FE_STATUS_OK;
@@ -741,8 +741,8 @@ int mpTranspose(fe_hip_ctx *ctx, FeMpnn &M)
   return FE_STATUS_OK;
 }
 
-// Features, forward, corrected factor (g, d) of the current matrix and IC(0)
-// factor (ctx->pc.d_lu must hold it).
+// Features, forward, corrected factor (g, B, d) of the current matrix and its
+// IC(0) factor (ctx->pc.d_lu must hold it).
 int mpForward(fe_hip_ctx *ctx, FeMpnn &M)
 {
   FE_REQUIRE(ctx, M.hasParams, "no learned preconditioner weights are loaded");
@@ -750,55 +750,39 @@ int mpForward(fe_hip_ctx *ctx, FeMpnn &M)
   if(int rc = mpTranspose(ctx, M)) return rc;
   const int n = ctx->n;
   hipStream_t st = ctx->stream;
-  hipLaunchKernelGGL(mpScale, dim3(cdiv(n, 256)), dim3(256), 0, st, n, ctx->d_diag, ctx->d_val, ctx->pc.d_lu, M.s,
-                     M.d0);
-  hipLaunchKernelGGL(mpFeatures, dim3(cdiv(n, 256)), dim3(256), 0, st, n, ctx->d_ia, ctx->d_ja, ctx->d_val,
-                     ctx->pc.d_lu, M.s, M.d0, M.ahat, M.g0, M.X);
+  const int gN = cdiv(n, 256);
+  const int *ia = ctx->d_ia, *ja = ctx->d_ja, *dg = ctx->d_diag;
+  hipLaunchKernelGGL(mpScale, dim3(gN), dim3(256), 0, st, n, dg, ctx->d_val, ctx->pc.d_lu, M.s, M.d0);
+  hipLaunchKernelGGL(mpNormalise, dim3(gN), dim3(256), 0, st, n, ia, ja, ctx->d_val, ctx->pc.d_lu, M.s, M.ahat, M.g0);
+  hipLaunchKernelGGL(mpRows, dim3(gN), dim3(256), 0, st, n, ia, ja, dg, M.ahat, M.g0, M.B0, M.w0);
+  hipLaunchKernelGGL(mpFeatures, dim3(gN), dim3(256), 0, st, n, ia, ja, dg, M.ahat, M.g0, M.w0, M.d0, M.X);
   const int gD = cdiv(n, 64);
   hipLaunchKernelGGL(mpDense<16>, dim3(gD), dim3(256), 0, st, n, M.X, (const bf16_t *)nullptr, M.w16 + kW16Off[0],
                      M.theta + ob0, M.h[0]);
   for(int l = 1; l <= 3; ++l) {
-    hipLaunchKernelGGL(mpAggregate, dim3(cdiv((int64_t)n * 16, 256)), dim3(256), 0, st, n, ctx->d_ia, ctx->d_ja,
-                       M.ahat, M.h[l - 1], M.m[l]);
+    hipLaunchKernelGGL(mpAggregate, dim3(cdiv((int64_t)n * 16, 256)), dim3(256), 0, st, n, ia, ja, M.ahat,
+                       M.h[l - 1], M.m[l]);
     hipLaunchKernelGGL(mpDense<64>, dim3(gD), dim3(256), 0, st, n, M.h[l - 1], M.m[l], M.w16 + kW16Off[l],
                        M.theta + kBOff[l], M.h[l]);
   }
-  hipLaunchKernelGGL(mpHeads, dim3(cdiv(n, 256)), dim3(256), 0, st, n, M.h[3], M.theta, M.p, M.q, M.tau);
-  hipLaunchKernelGGL(mpCorrect, dim3(cdiv(n, 256)), dim3(256), 0, st, n, ctx->d_ia, ctx->d_ja, ctx->d_diag, M.ahat,
-                     M.g0, M.p, M.q, M.tau, M.d0, M.g, M.d);
+  hipLaunchKernelGGL(mpHeads, dim3(gN), dim3(256), 0, st, n, M.h[3], M.theta, M.p, M.q, M.hs);
+  hipLaunchKernelGGL(mpCorrect, dim3(gN), dim3(256), 0, st, n, ia, ja, dg, M.ahat, M.g0, M.w0, M.p, M.q, M.hs, M.g);
+  hipLaunchKernelGGL(mpRows, dim3(gN), dim3(256), 0, st, n, ia, ja, dg, M.ahat, M.g, M.B, M.w);
+  hipLaunchKernelGGL(mpPivots, dim3(gN), dim3(256), 0, st, n, M.B, M.hs, M.d);
   FE_HIP_TRY(ctx, hipGetLastError());
   return FE_STATUS_OK;
 }
 
-int mpLossGrad(fe_hip_ctx *ctx, FeMpnn &M, const fe_hip_mpnn_opts *o, double *lossOut)
+int mpLossGrad(fe_hip_ctx *ctx, FeMpnn &M, double *lossOut)
 {
   const int n = ctx->n;
   hipStream_t st = ctx->stream;
-  const int gK = cdiv((int64_t)n * kK, 256);
   const int *ia = ctx->d_ia, *ja = ctx->d_ja, *dg = ctx->d_diag;
-  FE_REQUIRE(ctx, gK <= 4096, "system too large for the learned-preconditioner loss");
-  // probes
-  hipLaunchKernelGGL(mpProbes, dim3(gK), dim3(256), 0, st, n, o->seed, M.xi);
-  for(int s = 0; s < o->smooth_steps; ++s) {
-    hipLaunchKernelGGL(mpSpmmA, dim3(gK), dim3(256), 0, st, n, ia, ja, M.ahat, M.xi, (float)-o->smooth_omega, 1.f, M.y);
-    std::swap(M.xi, M.y);
-  }
-  const int gC = std::min(256, gK);
-  hipLaunchKernelGGL(mpColSq, dim3(gC), dim3(256), 0, st, n, M.xi, M.colPart);
-  hipLaunchKernelGGL(mpNormalize, dim3(gK), dim3(256), 0, st, n, gC, M.colPart, M.xi);
-  // forward of the loss
-  hipLaunchKernelGGL(mpScaleProbes, dim3(gK), dim3(256), 0, st, n, M.xi, M.d, M.a);
-  hipLaunchKernelGGL(mpApplyGT, dim3(gK), dim3(256), 0, st, n, ia, ja, dg, M.tpos, M.g, M.a, M.y);
-  hipLaunchKernelGGL(mpSpmmA, dim3(gK), dim3(256), 0, st, n, ia, ja, M.ahat, M.y, 1.f, 0.f, M.t);
-  hipLaunchKernelGGL(mpApplyG, dim3(gK), dim3(256), 0, st, n, ia, ja, dg, M.g, M.t, M.u);
-  hipLaunchKernelGGL(mpLossFwd, dim3(gK), dim3(256), 0, st, n, M.xi, M.u, M.d, M.gu, M.q1, M.lossPart);
-  hipLaunchKernelGGL(mpFoldLoss, dim3(1), dim3(64), 0, st, gK, M.lossPart, M.loss);
-  // backward of the loss
-  hipLaunchKernelGGL(mpApplyGT, dim3(gK), dim3(256), 0, st, n, ia, ja, dg, M.tpos, M.g, M.gu, M.gt);
-  hipLaunchKernelGGL(mpSpmmA, dim3(gK), dim3(256), 0, st, n, ia, ja, M.ahat, M.gt, 1.f, 0.f, M.gy);
-  hipLaunchKernelGGL(mpApplyG, dim3(gK), dim3(256), 0, st, n, ia, ja, dg, M.g, M.gy, M.ga);
-  hipLaunchKernelGGL(mpEdgeGrad, dim3(gK), dim3(256), 0, st, n, ia, ja, dg, M.tpos, M.ahat, M.gu, M.t, M.a, M.gy,
-                     M.ga, M.q1, M.p, M.q, M.tau, M.gHead);
+  const int gL = std::min(kLossGrid, cdiv(n, 256));
+  hipLaunchKernelGGL(mpLossParts, dim3(gL), dim3(256), 0, st, n, M.B, M.hs, M.lossPart);
+  hipLaunchKernelGGL(mpLossFold, dim3(1), dim3(64), 0, st, n, gL, M.lossPart, M.loss);
+  hipLaunchKernelGGL(mpEdgeGrad, dim3(cdiv(n, 256)), dim3(256), 0, st, n, ia, ja, dg, M.tpos, M.ahat, M.w, M.w0, M.B,
+                     M.p, M.q, M.hs, M.loss, M.gHead);
   // heads
   const int R = ((cdiv(n, kGemmGrid) + 63) / 64) * 64;
   const int G = cdiv(n, R);
@@ -871,13 +855,10 @@ int fe_hip_mpnn_param_count(void) { return kParams; }
 void fe_hip_mpnn_default_opts(fe_hip_mpnn_opts *o)
 {
   if(!o) return;
-  o->smooth_steps = 4;
-  o->smooth_omega = 2.0 / 3.0;
-  o->lr = 1e-3;
+  o->lr = 3e-3;
   o->beta1 = 0.9;
   o->beta2 = 0.999;
   o->eps = 1e-8;
-  o->seed = 1;
 }
 
 int fe_hip_mpnn_set_params(fe_hip_ctx *ctx, const float *theta, int count)
@@ -917,16 +898,15 @@ int fe_hip_mpnn_get_params(fe_hip_ctx *ctx, float *theta, int count)
   return FE_STATUS_OK;
 }
 
-int fe_hip_mpnn_loss_grad(fe_hip_ctx *ctx, const fe_hip_mpnn_opts *o, double *loss, float *grad)
+int fe_hip_mpnn_loss_grad(fe_hip_ctx *ctx, double *loss, float *grad)
 {
-  if(!ctx || !o) return FE_STATUS_POINTER_ERROR;
-  FE_REQUIRE(ctx, o->smooth_steps >= 0 && o->smooth_steps <= 64, "smooth_steps must be in [0, 64]");
+  if(!ctx) return FE_STATUS_POINTER_ERROR;
   FE_REQUIRE(ctx, ctx->mpnn && ctx->mpnn->hasParams, "no learned preconditioner weights are loaded");
   FE_HIP_TRY(ctx, hipSetDevice(ctx->device));
   if(int rc = fePrecondFactor(ctx)) return rc;
   FeMpnn &M = *ctx->mpnn;
   if(int rc = mpForward(ctx, M)) return rc;
-  if(int rc = mpLossGrad(ctx, M, o, loss)) return rc;
+  if(int rc = mpLossGrad(ctx, M, loss)) return rc;
   if(grad) {
     FE_HIP_TRY(ctx, hipMemcpyAsync(grad, M.grad, sizeof(float) * kParams, hipMemcpyDeviceToHost, ctx->stream));
     FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -936,7 +916,8 @@ int fe_hip_mpnn_loss_grad(fe_hip_ctx *ctx, const fe_hip_mpnn_opts *o, double *lo
 
 int fe_hip_mpnn_train_step(fe_hip_ctx *ctx, const fe_hip_mpnn_opts *o, double *loss)
 {
-  if(int rc = fe_hip_mpnn_loss_grad(ctx, o, loss, nullptr)) return rc;
+  if(!ctx || !o) return FE_STATUS_POINTER_ERROR;
+  if(int rc = fe_hip_mpnn_loss_grad(ctx, loss, nullptr)) return rc;
   FeMpnn &M = *ctx->mpnn;
   M.step++;
   const double c1 = 1. - std::pow(o->beta1, M.step), c2 = 1. - std::pow(o->beta2, M.step);
@@ -966,10 +947,12 @@ int fe_hip_mpnn_get_tensor(fe_hip_ctx *ctx, const char *name, float *out, int64_
   }
   else if(k == "p") { f32 = M.p; need = n * kNpq; }
   else if(k == "q") { f32 = M.q; need = n * kNpq; }
-  else if(k == "tau") { f32 = M.tau; need = n; }
+  else if(k == "heads") { f32 = M.hs; need = n * kNs; }
   else if(k == "g") { f32 = M.g; need = M.nnz; }
+  else if(k == "w0") { f32 = M.w0; need = M.nnz; }
+  else if(k == "w") { f32 = M.w; need = M.nnz; }
+  else if(k == "B") { f32 = M.B; need = n; }
   else if(k == "d") { f32 = M.d; need = n; }
-  else if(k == "xi") { f32 = M.xi; need = n * kK; }
   else FE_REQUIRE(ctx, false, "unknown tensor " + k);
   FE_REQUIRE(ctx, count == need, "tensor " + k + " has " + std::to_string(need) + " entries");
   FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));

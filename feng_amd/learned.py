@@ -5,7 +5,8 @@ this module drives them.  Training follows the instance recipe of the
 learned-preconditioner workload: P1 tetrahedral diffusion on the unit cube
 (N^3 hexes x 6 Kuhn tets) with k = exp(GRF, length 0.1, sigma 1.5), one
 seeded coefficient field per instance, each instance assembled on the GPU,
-factored (IC(0)) and used for one Adam step of the probe loss.
+factored (IC(0)) and used for one Adam step of the Kaporin condition-number
+loss.
 
 Weights are a flat float32 vector (fe_hip_mpnn_param_count() entries) saved
 as .npy; `default_weights()` is the file trained by `python -m
@@ -48,8 +49,8 @@ def cube_instance(N: int, seed: int, nfeat: int = 512, device: int = 0):
     return m, S, num
 
 
-def train(N=32, instances=2000, seed0=1000, init_seed=7, lr=2e-3, smooth_steps=4, nfeat=512, log_every=100,
-          theta0=None, out=None, device=0):
+def train(N=32, instances=2000, seed0=1000, init_seed=7, lr=3e-3, nfeat=512, log_every=100, theta0=None,
+          out=None, device=0):
     """One Adam step per instance, instances seeded seed0 .. seed0 + instances - 1.
     Returns (theta, history)."""
     m, S, num = cube_instance(N, seed0, nfeat, device)
@@ -64,7 +65,7 @@ def train(N=32, instances=2000, seed0=1000, init_seed=7, lr=2e-3, smooth_steps=4
             S.update_coeff(0, 0, m.coeff_grf(0.1, 1.5, nfeat, seed0 + it))
             S.setToZero()
             S.assemble(1, A.PREC_FP64)
-        loss = S.mpnn_train_step(lr=lr, smooth_steps=smooth_steps, seed=seed0 + it)
+        loss = S.mpnn_train_step(lr=lr)
         hist.append(loss)
         if log_every and (it % log_every == 0 or it == instances - 1):
             print(json.dumps({"instance": it, "loss": loss, "s": round(time.perf_counter() - t0, 1)}), flush=True)
@@ -81,11 +82,10 @@ def main(argv=None):
     ap.add_argument("--N", type=int, default=32)
     ap.add_argument("--instances", type=int, default=2000)
     ap.add_argument("--seed0", type=int, default=1000)
-    ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--smooth", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--out", default=str(DEFAULT_WEIGHTS))
     a = ap.parse_args(argv)
-    train(a.N, a.instances, a.seed0, lr=a.lr, smooth_steps=a.smooth, out=a.out)
+    train(a.N, a.instances, a.seed0, lr=a.lr, out=a.out)
 
 
 if __name__ == "__main__":

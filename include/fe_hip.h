@@ -234,14 +234,11 @@ int fe_hip_autotune_batched(fe_hip_ctx *ctx, int nprob, const int *rows, int pre
  * network on the graph of A corrects the split IC(0) factor
  * M^-1 = G^T D^-1 G (G = I - L_strict); with FE_PC_LEARNED, fe_hip_solve
  * factors IC(0), runs the network and applies the corrected factor.
- * Parameters: fe_hip_mpnn_param_count() floats.  Training minimises
- * (1/8) sum_k ||xi_k - D^-1/2 G A^ G^T D^-1/2 xi_k||^2 over 8 smoothed random
- * probes of the assembled matrix (A^ = unit-diagonal scaling of A). */
+ * Parameters: fe_hip_mpnn_param_count() floats.  Training minimises the
+ * logarithm of Kaporin's condition number of the preconditioned matrix (a
+ * bound on the CG iteration count), exactly, on the assembled matrix. */
 typedef struct {
-  int smooth_steps;     /* damped-Jacobi sweeps applied to the probes */
-  double smooth_omega;  /* their damping */
   double lr, beta1, beta2, eps; /* Adam */
-  uint64_t seed;        /* probe seed */
 } fe_hip_mpnn_opts;
 
 int fe_hip_mpnn_param_count(void);
@@ -251,12 +248,13 @@ int fe_hip_mpnn_set_params(fe_hip_ctx *ctx, const float *theta, int count);
 int fe_hip_mpnn_get_params(fe_hip_ctx *ctx, float *theta, int count);
 /* Factor IC(0) of the assembled matrix, run the network, the loss and its
  * gradient (grad may be NULL). */
-int fe_hip_mpnn_loss_grad(fe_hip_ctx *ctx, const fe_hip_mpnn_opts *o, double *loss, float *grad);
+int fe_hip_mpnn_loss_grad(fe_hip_ctx *ctx, double *loss, float *grad);
 /* fe_hip_mpnn_loss_grad followed by one Adam step. */
 int fe_hip_mpnn_train_step(fe_hip_ctx *ctx, const fe_hip_mpnn_opts *o, double *loss);
 /* Intermediates of the last network run, as floats: "X" (n x 16), "h0".."h3",
- * "m1".."m3" (n x 32), "p", "q" (n x 8), "tau", "d" (n), "g" (nnz, corrected
- * normalised strict-lower factor entries), "xi" (n x 8 probes). */
+ * "m1".."m3" (n x 32), "p", "q" (n x 8), "heads" (n x 5: tau, s, r, sigma,
+ * rho), "B", "d" (n), "g", "w0", "w" (nnz; normalised strict-lower factor
+ * entries, the IC(0) row products and the corrected ones). */
 int fe_hip_mpnn_get_tensor(fe_hip_ctx *ctx, const char *name, float *out, int64_t count);
 
 #ifdef __cplusplus

@@ -11,29 +11,37 @@ corrects the factor of the split IC(0) preconditioner
 that the device path applies.  This module restates, in float64 numpy with
 optional rounding at the points where the device kernels store bfloat16, the
 whole computation: features, the 3-layer 32-wide message-passing net, the
-corrected factor, the low-frequency-weighted loss and its gradient (written
-out by hand, in the same order as the device backward kernels).  It is pinned
-by central finite differences of its own forward (the gradient), by the
-zero-head case (the untrained net reproduces the IC(0) factor exactly), and by
-the SPD property of G^T D^-1 G (tests/test_oracle_mpnn.py).
+corrected factor, the Kaporin loss and its gradient (written out by hand, in
+the order of the device backward kernels).  Pins (tests/test_oracle_mpnn.py):
+central finite differences of its own forward (the gradient), the zero-head
+net reproduces the IC(0) factor exactly, the per-row minimiser of the loss is
+the FSAI factor (closed form by dense solves), and G^T D^-1 G is SPD.
 
 Normalised coordinates.  With s_i = a_ii^-1/2 and S = diag(s):
     A^ = S A S (unit diagonal),  G^ = S G S^-1 (g^_ij = g_ij s_i / s_j),
-    D^ = S D S (d^_i = d_i / a_ii),
-so that G^T D^-1 G = S G^^T D^^-1 G^ S.  The net works on A^ and predicts
-    g^_ij = g^0_ij + a^_ij (p_i . q_j)   (strict lower entries j < i),
-    d^_i  = d^0_i exp(clamp(tau_i, -4, 4)).
+    D^ = S D S,
+so that G^T D^-1 G = S G^^T D^^-1 G^ S.
 
 Net (H = 32 hidden, parameters in one float32 vector, layout `LAYOUT`):
     x_i  = [1, d^0_i, sum_j!=i |a^_ij|, sum_j!=i a^_ij, deg_i / 16,
-            sum_j<i g^0_ij^2, 0 ... 0]                      (16 features)
+            sum_j<i g^0_ij^2, sum_j<i w0_ij^2, 0 ... 0]   (16 features)
     h0   = relu(x W0 + b0)
     m_l  = A^_off h_{l-1};  h_l = relu([h_{l-1}, m_l] W_l + b_l),  l = 1..3
-    p = h3 P, q = h3 Q (8 wide), tau = h3 wt + bt.
+    p = h3 P, q = h3 Q (8 wide), [tau, s, r, sigma, rho] = h3 Hs + hb.
+Corrected factor, on the strict lower entries (i, j), j < i:
+    w0_ij = (A^ g^0_i)_j   (row i of G^0 times A^, at column j)
+    g^_ij = g^0_ij - (sigma_i + rho_j) w0_ij + a^_ij (p_i . q_j + s_i + r_j)
+    d^_i  = B_ii exp(clamp(tau_i, -4, 4)),  B_ii = g^_i^T A^ g^_i.
+The w0 term moves g^ along the negative gradient of the loss at the IC(0)
+factor, so the net learns step lengths; the a^ term adds learned corrections.
 
-Loss on K probes (Rademacher signs from SplitMix64, smoothed by nu damped
-Jacobi sweeps xi <- xi - omega A^ xi, then normalised to unit length):
-    A~ = D^^-1/2 G^ A^ G^^T D^^-1/2,   L = (1/K) sum_k ||xi_k - A~ xi_k||^2.
+Loss: Kaporin's condition number of A~ = D^^-1/2 G^ A^ G^^T D^^-1/2,
+    K = (tr A~ / n)^n / det A~,  which bounds the CG iteration count
+(Kaporin, Numer. Linear Algebra Appl. 1 (1994)); up to the constant
+-log det A^ / n,
+    L = log K / n = log((1/n) sum_i e^-tau_i) + mean(tau) + mean(log B_ii).
+Its minimiser over the pattern is the factorized sparse approximate inverse
+(FSAI) of A^ (Kolotilina & Yeremin 1993), with tau = 0.
 """
 from __future__ import annotations
 
@@ -43,7 +51,7 @@ import scipy.sparse as sp
 H = 32
 NIN = 16
 NPQ = 8
-K_DEFAULT = 8
+NS = 5  # scalar heads: tau, s, r, sigma, rho
 TAU_CLAMP = 4.0
 
 # (name, shape) in storage order
@@ -52,7 +60,7 @@ LAYOUT = [("W0", (NIN, H)), ("b0", (H,)),
           ("W2", (2 * H, H)), ("b2", (H,)),
           ("W3", (2 * H, H)), ("b3", (H,)),
           ("P", (H, NPQ)), ("Q", (H, NPQ)),
-          ("wt", (H,)), ("bt", (1,))]
+          ("Hs", (H, NS)), ("hb", (NS,))]
 
 
 def param_count() -> int:
@@ -83,7 +91,7 @@ def splitmix64(x):
 def init_params(seed: int) -> np.ndarray:
     """Uniform init: weights U(-r, r) with r = sqrt(6 / fan_in) (heads P, Q
     scaled by 0.05 so the untrained net stays close to IC(0)), biases and the
-    tau head zero.  u_q = (splitmix64(seed * 2^32 + q) >> 11) 2^-53."""
+    scalar heads zero.  u_q = (splitmix64(seed * 2^32 + q) >> 11) 2^-53."""
     theta = np.zeros(param_count(), np.float64)
     o = 0
     with np.errstate(over="ignore"):
@@ -108,16 +116,6 @@ def bf16(x):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def probes(n: int, K: int, seed: int) -> np.ndarray:
-    """xi_ik = +1 or -1 from the top bit of splitmix64(seed * 2^32 + i K + k)."""
-    idx = np.arange(n * K, dtype=np.uint64)
-    with np.errstate(over="ignore"):
-        key = (np.uint64(seed & 0xFFFFFFFF) << np.uint64(32)) + idx
-        h = splitmix64(key)
-    sign = 1.0 - 2.0 * (h >> np.uint64(63)).astype(np.float64)
-    return sign.reshape(n, K)
-
-
 class Graph:
     """Matrix graph quantities of one SPD CSR matrix and its IC(0) factor
     (lu on the same pattern: strict lower = L, upper incl. diagonal = U)."""
@@ -129,6 +127,7 @@ class Graph:
         lu = np.asarray(lu, np.float64)
         n = len(ia) - 1
         self.n = n
+        self.ia = ia
         rows = np.repeat(np.arange(n), np.diff(ia))
         self.rows, self.cols = rows, ja
         diag_mask = rows == ja
@@ -138,15 +137,14 @@ class Graph:
         self.s = 1.0 / np.sqrt(a_ii)
         self.ahat = val * self.s[rows] * self.s[ja]
         self.lower = rows > ja
-        self.upper = rows < ja
         off = ~diag_mask
         self.Aoff = sp.csr_matrix((np.where(off, self.ahat, 0.0), ja, ia), shape=(n, n))
         self.Ahat = sp.csr_matrix((self.ahat, ja, ia), shape=(n, n))
-        # IC(0) split factor, normalised
         self.d0 = lu[dpos] / a_ii
         self.g0 = np.where(self.lower, -lu * self.s[rows] / self.s[ja], 0.0)
         self.deg = np.bincount(rows[off], minlength=n).astype(np.float64)
         self.a_ii = a_ii
+        _, self.w0 = row_products(self, self.g0)
 
     def features(self, round_bf16=True):
         n = self.n
@@ -158,6 +156,7 @@ class Graph:
         X[:, 3] = np.bincount(self.rows[off], weights=self.ahat[off], minlength=n)
         X[:, 4] = self.deg / 16.0
         X[:, 5] = np.bincount(self.rows, weights=self.g0 ** 2, minlength=n)
+        X[:, 6] = np.bincount(self.rows, weights=self.w0 ** 2, minlength=n)
         return bf16(X) if round_bf16 else X
 
     def factor(self, g):
@@ -167,9 +166,19 @@ class Graph:
         return (sp.csr_matrix((g[m], (self.rows[m], self.cols[m])), shape=(n, n)) + sp.identity(n)).tocsr()
 
 
+def row_products(graph: Graph, g):
+    """B_ii = g^_i^T A^ g^_i for every row of G^ = I + strict lower g, and
+    w_e = (A^ g^_i)_j on every strict lower entry e = (i, j) (0 elsewhere)."""
+    G = graph.factor(g)
+    AG = (G @ graph.Ahat).tocsr()  # row i holds (A^ g^_i)^T (A^ symmetric)
+    B = np.asarray(AG.multiply(G).sum(axis=1)).ravel()
+    w = np.asarray(AG[graph.rows, graph.cols]).ravel()
+    return B, np.where(graph.lower, w, 0.0)
+
+
 def forward(graph: Graph, theta, round_bf16=True):
-    """Net forward.  Returns a dict of every intermediate the device kernels
-    store (X, h0..h3, m1..m3, p, q, tau) and the corrected factor (g, d)."""
+    """Net forward and corrected factor.  Returns every intermediate the
+    device kernels store (X, h0..h3, m1..m3, p, q, heads) and g, B, d."""
     R = bf16 if round_bf16 else (lambda v: np.asarray(v, np.float64))
     W = unpack(theta)
     out = {}
@@ -185,79 +194,59 @@ def forward(graph: Graph, theta, round_bf16=True):
         out[f"m{l}"] = m
         h = R(np.maximum(z, 0.0))
         out[f"h{l}"] = h
-    p = h @ W["P"]
-    q = h @ W["Q"]
-    tau = h @ W["wt"] + W["bt"][0]
-    out["p"], out["q"], out["tau"] = p, q, tau
-    c = np.einsum("ek,ek->e", p[graph.rows], q[graph.cols])
-    out["g"] = np.where(graph.lower, graph.g0 + graph.ahat * c, 0.0)
-    out["d"] = graph.d0 * np.exp(np.clip(tau, -TAU_CLAMP, TAU_CLAMP))
+    out["p"] = h @ W["P"]
+    out["q"] = h @ W["Q"]
+    hs = h @ W["Hs"] + W["hb"]
+    out["heads"] = hs
+    tau, sv, rv, sig, rho = (hs[:, k] for k in range(NS))
+    rows, cols = graph.rows, graph.cols
+    c = np.einsum("ek,ek->e", out["p"][rows], out["q"][cols]) + sv[rows] + rv[cols]
+    g = graph.g0 - (sig[rows] + rho[cols]) * graph.w0 + graph.ahat * c
+    out["g"] = np.where(graph.lower, g, 0.0)
+    B, w = row_products(graph, out["g"])
+    out["B"], out["w"] = B, w
+    out["d"] = B * np.exp(np.clip(tau, -TAU_CLAMP, TAU_CLAMP))
     return out
 
 
-def smoothed_probes(graph: Graph, K: int, seed: int, nu: int, omega: float):
-    xi = probes(graph.n, K, seed)
-    for _ in range(nu):
-        xi = xi - omega * (graph.Ahat @ xi)
-    return xi / np.linalg.norm(xi, axis=0, keepdims=True)
-
-
-def factor_loss(graph: Graph, g, d, xi):
-    """L = (1/K) sum_k ||xi_k - D^-1/2 G^ A^ G^^T D^-1/2 xi_k||^2 for a given
-    normalised factor (strict-lower values g on the pattern, pivots d)."""
-    G = graph.factor(g)
-    rs = 1.0 / np.sqrt(d)[:, None]
-    v = (G @ (graph.Ahat @ (G.T @ (xi * rs)))) * rs
-    e = xi - v
-    return float(np.sum(e * e) / xi.shape[1])
-
-
-def loss_and_grad(graph: Graph, theta, K=K_DEFAULT, seed=1, nu=4, omega=2.0 / 3.0, round_bf16=True,
-                  want_grad=True):
-    """Loss and its gradient with respect to theta (float64), following the
-    device backward: heads, then layers 3..1 (dense backward, aggregation
-    backward), then layer 0."""
+def loss_and_grad(graph: Graph, theta, round_bf16=True, want_grad=True):
+    """Kaporin loss L = log(mean e^-tau) + mean(tau) + mean(log B) and its
+    gradient with respect to theta, backward in the device kernels' order:
+    edges -> heads -> layers 3..1 (dense, then aggregation) -> layer 0."""
     R = bf16 if round_bf16 else (lambda v: np.asarray(v, np.float64))
     W = unpack(theta)
     f = forward(graph, theta, round_bf16)
-    G = graph.factor(f["g"])
-    GT = G.T.tocsr()
-    A = graph.Ahat
-    xi = smoothed_probes(graph, K, seed, nu, omega)
-    rs = 1.0 / np.sqrt(f["d"])[:, None]
-    a = xi * rs
-    y = GT @ a
-    t = A @ y
-    u = G @ t
-    v = u * rs
-    e = xi - v
-    loss = float(np.sum(e * e) / K)
+    n = graph.n
+    hs = f["heads"]
+    tau = np.clip(hs[:, 0], -TAU_CLAMP, TAU_CLAMP)
+    et = np.exp(-tau)
+    S = et.sum()
+    B, w = f["B"], f["w"]
+    loss = float(np.log(S / n) + tau.mean() + np.log(B).mean())
     if not want_grad:
         return loss, None, f
-    gv = -2.0 / K * e
-    gu = gv * rs
-    gt = GT @ gu
-    gy = A @ gt
-    ga = G @ gy
-    tau = f["tau"]
-    active = (tau > -TAU_CLAMP) & (tau < TAU_CLAMP)
-    gtau = np.where(active, -0.5 * np.sum(ga * a + gv * v, axis=1), 0.0)
-    # edge gradients (strict lower (i, j)): dL/dg^_ij = sum_k gu_ik t_jk + a_ik gy_jk
     rows, cols, low = graph.rows, graph.cols, graph.lower
-    ge = np.sum(gu[rows] * t[cols] + a[rows] * gy[cols], axis=1)
-    gc = np.where(low, ge * graph.ahat, 0.0)
-    n = graph.n
+    ghs = np.zeros((n, NS))
+    active = (hs[:, 0] > -TAU_CLAMP) & (hs[:, 0] < TAU_CLAMP)
+    ghs[:, 0] = np.where(active, (1.0 - n * et / S) / n, 0.0)
+    ge = np.where(low, 2.0 * w / B[rows] / n, 0.0)      # dL/dg^_ij
+    gc = ge * graph.ahat                                # dL/dc_ij
+    gsr = -ge * graph.w0                                # dL/d(sigma_i + rho_j)
     gp = np.zeros((n, NPQ))
     gq = np.zeros((n, NPQ))
     np.add.at(gp, rows, gc[:, None] * f["q"][cols])
     np.add.at(gq, cols, gc[:, None] * f["p"][rows])
+    ghs[:, 1] = np.bincount(rows, weights=gc, minlength=n)
+    ghs[:, 2] = np.bincount(cols, weights=gc, minlength=n)
+    ghs[:, 3] = np.bincount(rows, weights=gsr, minlength=n)
+    ghs[:, 4] = np.bincount(cols, weights=gsr, minlength=n)
     grads = {k: np.zeros_like(v) for k, v in W.items()}
     h3 = f["h3"]
     grads["P"] = h3.T @ gp
     grads["Q"] = h3.T @ gq
-    grads["wt"] = h3.T @ gtau
-    grads["bt"] = np.array([gtau.sum()])
-    gh = gp @ W["P"].T + gq @ W["Q"].T + np.outer(gtau, W["wt"])
+    grads["Hs"] = h3.T @ ghs
+    grads["hb"] = ghs.sum(axis=0)
+    gh = gp @ W["P"].T + gq @ W["Q"].T + ghs @ W["Hs"].T
     for l in (3, 2, 1):
         hprev = f[f"h{l - 1}"]
         gz = R(np.where(f[f"h{l}"] > 0, gh, 0.0))
@@ -270,6 +259,23 @@ def loss_and_grad(graph: Graph, theta, K=K_DEFAULT, seed=1, nu=4, omega=2.0 / 3.
     grads["W0"] = f["X"].T @ gz
     grads["b0"] = gz.sum(axis=0)
     return loss, pack(grads), f
+
+
+def fsai(graph: Graph):
+    """Closed-form minimiser of mean(log B) over the strict lower pattern:
+    row i solves A^[P_i, P_i] y = e_i (P_i = {j <= i in the pattern}) and
+    g^_i = y / y_i (Kolotilina & Yeremin 1993)."""
+    g = np.zeros(len(graph.cols))
+    Ah = graph.Ahat.tocsr()
+    for i in range(graph.n):
+        P = np.arange(graph.ia[i], graph.ia[i + 1])
+        P = P[graph.cols[P] <= i]
+        c = graph.cols[P]
+        e = np.zeros(len(c))
+        e[-1] = 1.0
+        y = np.linalg.solve(Ah[c][:, c].toarray(), e)
+        g[P[:-1]] = y[:-1] / y[-1]
+    return np.where(graph.lower, g, 0.0)
 
 
 def preconditioner(graph: Graph, f):
