@@ -122,11 +122,7 @@ __global__ __launch_bounds__(TB) void bInit(const int *__restrict__ ia, const in
   const int sub = threadIdx.x % LPR, gg = threadIdx.x / LPR;
 #pragma unroll
   for(int u = 0; u < LPR; ++u) {
-    T acc = (T)0;
-    for(int k = kb[u]; k < ke[u]; k += LPR) {
-      const int j = ja[k];
-      acc += val[k] * (v.dinv[j] * (T)b[j]);
-    }
+    T acc = feRowDot<FeKs<LPR>::value, LPR>(ja, val, kb[u], ke[u], [&](int j) { return v.dinv[j] * (T)b[j]; });
 #pragma unroll
     for(int q = LPR / 2; q > 0; q >>= 1) acc += __shfl_xor(acc, q, LPR);
     if(sub == u) ldsY[u * RPB + gg] = acc;
@@ -542,7 +538,9 @@ extern "C" int fe_hip_solve_batched(fe_hip_ctx *ctx, int nprob, const int *rows,
     for(int p = 0; p < nprob; ++p) {
       if(outerDone[p]) continue;
       its[p] += hIter[p];
-      if(hDone[p] == 3 || hDone[p] == 4) reason[p] = -3;
+      // a breakdown only ends this inner solve (the next outer iteration
+      // restarts from the FP64 residual); NaN ends the problem
+      if(hDone[p] == 4) reason[p] = -9;
       if(inner32 && !(rnew[p] < 0.9 * rn[p])) {
         // undo this problem's FP32 correction
         const size_t r0 = B.rows[p], cnt = B.rows[p + 1] - B.rows[p];

@@ -37,7 +37,7 @@ __global__ __launch_bounds__(kTB) void spmv64(int n, const int *__restrict__ ia,
     const int row = chunk * RPB + threadIdx.x / LPR;
     double s = 0.;
     if(row < n)
-      for(int k = ia[row] + sub; k < ia[row + 1]; k += LPR) s += val[k] * x[ja[k]];
+      s = feRowDot<FeKs<LPR>::value, LPR>(ja, val, ia[row] + sub, ia[row + 1], [&](int j) { return x[j]; });
 #pragma unroll
     for(int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LPR);
     if(sub == 0 && row < n) y[row] = s;

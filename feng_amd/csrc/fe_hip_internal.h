@@ -228,6 +228,38 @@ __device__ __forceinline__ double feWaveSum(double v)
   return v;
 }
 
+// sum over k = kb, kb + STRIDE, ... < ke of val[k] * f(ja[k]).  The column
+// indices and values of KS entries are loaded before any gathered value is
+// used, so a lane keeps KS independent gathers in flight instead of one
+// dependent pair per entry; the accumulation order is that of the plain loop.
+template <int KS, int STRIDE, typename T, typename F>
+__device__ __forceinline__ T feRowDot(const int *__restrict__ ja, const T *__restrict__ val, int kb, int ke, F f)
+{
+  T acc = (T)0;
+  for(int k = kb; k < ke; k += KS * STRIDE) {
+    int j[KS];
+    T v[KS];
+    bool ok[KS];
+#pragma unroll
+    for(int s = 0; s < KS; ++s) {
+      const int kk = k + s * STRIDE;
+      ok[s] = kk < ke;
+      j[s] = ok[s] ? ja[kk] : 0;
+      v[s] = ok[s] ? val[kk] : (T)0;
+    }
+#pragma unroll
+    for(int s = 0; s < KS; ++s)
+      if(ok[s]) acc += v[s] * f(j[s]);
+  }
+  return acc;
+}
+
+// Entries a lane of an LPR-lane row group keeps in flight: 16 per lane.
+template <int LPR>
+struct FeKs {
+  static constexpr int value = LPR >= 16 ? 1 : 16 / LPR;
+};
+
 // Block sum: result valid in thread 0. lds must hold blockDim/64 doubles.
 __device__ __forceinline__ double feBlockSum(double v, double *lds)
 {

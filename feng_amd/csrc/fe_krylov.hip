@@ -181,19 +181,9 @@ __global__ __launch_bounds__(TB) void cgSpmv(int n, const int *__restrict__ ia, 
       }
       sum[u] = (T)0;
     }
-    bool more = true;
-    while(more) {
-      more = false;
 #pragma unroll
-      for(int u = 0; u < UNR; ++u) {
-        if(kb[u] < ke[u]) {
-          const int j = ja[kb[u]];
-          sum[u] += val[kb[u]] * (z[j] + beta * pOld[j]);
-          kb[u] += LPR;
-          more |= kb[u] < ke[u];
-        }
-      }
-    }
+    for(int u = 0; u < UNR; ++u)
+      sum[u] = feRowDot<FeKs<LPR>::value, LPR>(ja, val, kb[u], ke[u], [&](int j) { return z[j] + beta * pOld[j]; });
 #pragma unroll
     for(int u = 0; u < UNR; ++u) {
 #pragma unroll
@@ -273,7 +263,8 @@ __global__ __launch_bounds__(TB) void splitInit(int n, const double *__restrict_
     const int row = chunk * RPB + threadIdx.x / LPR;
     T sum = (T)0;
     if(row < n)
-      for(int k = v.iaL[row] + sub; k < v.iaL[row + 1]; k += LPR) sum += v.gL[k] * (T)b[v.jaL[k]];
+      sum = feRowDot<FeKs<LPR>::value, LPR>(v.jaL, v.gL, v.iaL[row] + sub, v.iaL[row + 1],
+                                             [&](int j) { return (T)b[j]; });
 #pragma unroll
     for(int o = LPR / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, LPR);
     if(sub == 0 && row < n) {
@@ -321,10 +312,8 @@ __global__ __launch_bounds__(TB) void splitApplyG(int n, SplitVecs<T> v, CgParts
     const int row = chunk * RPB + threadIdx.x / LPR;
     T sum = (T)0;
     if(row < n)
-      for(int k = v.iaL[row] + sub; k < v.iaL[row + 1]; k += LPR) {
-        const int j = v.jaL[k];
-        sum += v.gL[k] * (rc[j] - alpha * v.q[j]);
-      }
+      sum = feRowDot<FeKs<LPR>::value, LPR>(v.jaL, v.gL, v.iaL[row] + sub, v.iaL[row + 1],
+                                             [&](int j) { return rc[j] - alpha * v.q[j]; });
 #pragma unroll
     for(int q = LPR / 2; q > 0; q >>= 1) sum += __shfl_xor(sum, q, LPR);
     if(sub == 0 && row < n) {
@@ -353,10 +342,8 @@ __global__ __launch_bounds__(TB) void splitApplyGT(int n, SplitVecs<T> v, CgPart
     const int row = chunk * RPB + threadIdx.x / LPR;
     T sum = (T)0;
     if(row < n)
-      for(int k = v.iaU[row] + sub; k < v.iaU[row + 1]; k += LPR) {
-        const int j = v.jaU[k];
-        sum += v.gU[k] * (v.dinv[j] * v.y[j]);
-      }
+      sum = feRowDot<FeKs<LPR>::value, LPR>(v.jaU, v.gU, v.iaU[row] + sub, v.iaU[row + 1],
+                                             [&](int j) { return v.dinv[j] * v.y[j]; });
 #pragma unroll
     for(int q = LPR / 2; q > 0; q >>= 1) sum += __shfl_xor(sum, q, LPR);
     if(sub == 0 && row < n) {
@@ -413,14 +400,11 @@ __global__ __launch_bounds__(TB) void pipeInit(int n, const int *__restrict__ ia
   for(int chunk = feXcdBlock(); chunk < nChunks; chunk += gridDim.x) {
     const int row = chunk * RPB + threadIdx.x / LPR;
     T sum = (T)0;
-    if(row < n) {
-      const int e = ia[row + 1];
-      for(int k = ia[row] + sub; k < e; k += LPR) {
-        const int j = ja[k];
+    if(row < n)
+      sum = feRowDot<FeKs<LPR>::value, LPR>(ja, val, ia[row] + sub, ia[row + 1], [&](int j) {
         const T rj = fromResidual ? r0[j] : (T)b[j];
-        sum += val[k] * (v.dinv[j] * rj);
-      }
-    }
+        return v.dinv[j] * rj;
+      });
 #pragma unroll
     for(int o = LPR / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, LPR);
     if(sub == 0 && row < n) {
@@ -496,14 +480,10 @@ __global__ __launch_bounds__(TB) void pipeIter(int n, const int *__restrict__ ia
   for(int chunk = feXcdBlock(); chunk < nChunks; chunk += gridDim.x) {
     const int row = chunk * RPB + threadIdx.x / LPR;
     T sum = (T)0;
-    if(row < n) {
-      const int e = ia[row + 1];
-      for(int k = ia[row] + sub; k < e; k += LPR) {
-        const int j = ja[k];
-        const T uj = v.dinv[j] * (rC[j] - alpha * (wC[j] + beta * sO[j]));
-        sum += val[k] * uj;
-      }
-    }
+    if(row < n)
+      sum = feRowDot<FeKs<LPR>::value, LPR>(ja, val, ia[row] + sub, ia[row + 1], [&](int j) {
+        return v.dinv[j] * (rC[j] - alpha * (wC[j] + beta * sO[j]));
+      });
 #pragma unroll
     for(int q = LPR / 2; q > 0; q >>= 1) sum += __shfl_xor(sum, q, LPR);
     if(sub == 0 && row < n) {
@@ -569,11 +549,8 @@ __global__ __launch_bounds__(TB) void tileInit(int n, const int *__restrict__ ia
       const int sub = threadIdx.x % LPR, gg = threadIdx.x / LPR;
 #pragma unroll
       for(int u = 0; u < LPR; ++u) {
-        T acc = (T)0;
-        for(int k = kb[u]; k < ke[u]; k += LPR) {
-          const int j = ja[k];
-          acc += val[k] * (v.dinv[j] * (T)b[j]);
-        }
+        T acc =
+            feRowDot<FeKs<LPR>::value, LPR>(ja, val, kb[u], ke[u], [&](int j) { return v.dinv[j] * (T)b[j]; });
 #pragma unroll
         for(int q = LPR / 2; q > 0; q >>= 1) acc += __shfl_xor(acc, q, LPR);
         if(sub == u) ldsY[u * RPB + gg] = acc;
@@ -693,6 +670,188 @@ __global__ __launch_bounds__(TB) void tileIter(int n, const int *__restrict__ ia
   tileStore3<TB>(g, d, rh, P.g[o], P.d[o], P.rho[o]);
 }
 
+// Tiled pipelined PCG with the split preconditioner M^-1 = G^T D^-1 G
+// (Ghysels & Vanroose 2014, Alg. 3), three kernels per iteration and one
+// reduction phase:
+//   K1  t = D^-1 (G w)          gathers w (G: lower CSR with unit diagonal)
+//   K2  m = G^T t               gathers t (G^T: upper CSR)
+//   K3  fold gamma, delta, rho; alpha, beta; y = A m (gathers m);
+//       z = y + beta z, q = m + beta q, s = w + beta s, p = u + beta p,
+//       x += alpha p, r -= alpha s, u -= alpha q, w -= alpha z;
+//       partials of gamma = (r, u), delta = (w, u), rho = (r, r).
+// Every gathered vector is written by an earlier kernel and not modified by
+// the kernel that gathers it, so no vector is double-buffered.
+template <typename T>
+struct SpVecs {
+  T *x, *r, *u, *w, *z, *q, *s, *p, *m, *t;
+};
+
+template <typename T>
+struct SpRow {
+  T r, u, w, z, q, s, p, x, m;
+};
+
+// y = scale .* (M x) over row tiles (scale may be null); skipped once the
+// iteration has stopped when checkDone is set.
+template <typename T, int LPR, int TB>
+__global__ __launch_bounds__(TB) void spSpmv(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                             const T *__restrict__ val, const T *__restrict__ x,
+                                             const T *__restrict__ scale, T *__restrict__ y, const CgState *st,
+                                             int checkDone)
+{
+  __shared__ T ldsY[TB];
+  if(checkDone && st->done) return;
+  const int nTiles = (n + TB - 1) / TB;
+  for(int tile = blockIdx.x; tile < nTiles; tile += gridDim.x) {
+    const int base = tile * TB;
+    int kb[LPR], ke[LPR];
+    tileBounds<LPR, TB>(n, base, ia, kb, ke);
+    tileProduct<T, LPR, TB, false>(n, base, ia, ja, val, x, nullptr, kb, ke, ldsY);
+    __syncthreads();
+    const int row = base + threadIdx.x;
+    if(row < n) y[row] = scale ? scale[row] * ldsY[threadIdx.x] : ldsY[threadIdx.x];
+    __syncthreads();
+  }
+}
+
+// r = b, x = 0 (FP64 b rounded to T)
+template <typename T>
+__global__ void spStart(int n, const double *__restrict__ b, SpVecs<T> v)
+{
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    v.r[i] = (T)b[i];
+    v.x[i] = (T)0;
+  }
+}
+
+// w = A u; z = q = s = p = 0; partials of iteration 0; scalar start values.
+template <typename T, int LPR, int TB>
+__global__ __launch_bounds__(TB) void spInitW(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                              const T *__restrict__ val, SpVecs<T> v, PipeParts P, PipeScalars *sc,
+                                              CgState *st)
+{
+  __shared__ T ldsY[TB];
+  const int nTiles = (n + TB - 1) / TB;
+  double g = 0., d = 0., rho = 0.;
+  for(int tile = blockIdx.x; tile < nTiles; tile += gridDim.x) {
+    const int base = tile * TB;
+    int kb[LPR], ke[LPR];
+    tileBounds<LPR, TB>(n, base, ia, kb, ke);
+    tileProduct<T, LPR, TB, false>(n, base, ia, ja, val, v.u, nullptr, kb, ke, ldsY);
+    __syncthreads();
+    const int row = base + threadIdx.x;
+    if(row < n) {
+      const T wi = ldsY[threadIdx.x], ri = v.r[row], ui = v.u[row];
+      v.w[row] = wi;
+      v.z[row] = v.q[row] = v.s[row] = v.p[row] = (T)0;
+      g += (double)ri * (double)ui;
+      d += (double)wi * (double)ui;
+      rho += (double)ri * (double)ri;
+    }
+    __syncthreads();
+  }
+  tileStore3<TB>(g, d, rho, P.g[0], P.d[0], P.rho[0]);
+  if(blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->gamma[1] = 1.;
+    sc->alpha[1] = __builtin_inf();
+    st->done = 0;
+    st->iter = 0;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void spLoadRow(const SpVecs<T> &v, int row, int n, SpRow<T> &o)
+{
+  if(row < n) {
+    o.r = v.r[row]; o.u = v.u[row]; o.w = v.w[row]; o.z = v.z[row]; o.q = v.q[row];
+    o.s = v.s[row]; o.p = v.p[row]; o.x = v.x[row]; o.m = v.m[row];
+  } else {
+    o.r = o.u = o.w = o.z = o.q = o.s = o.p = o.x = o.m = (T)0;
+  }
+}
+
+template <typename T, int LPR, int TB>
+__global__ __launch_bounds__(TB) void spIter(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                             const T *__restrict__ val, SpVecs<T> v, PipeParts P, PipeScalars *sc,
+                                             CgState *st, int it, int maxit, double tol2)
+{
+  __shared__ T ldsY[TB];
+  const int c = it & 1, o = c ^ 1;
+  const int nTiles = (n + TB - 1) / TB;
+  int tile = blockIdx.x;
+  int kb[LPR], ke[LPR];
+  SpRow<T> me;
+  tileBounds<LPR, TB>(n, tile * TB, ia, kb, ke);
+  spLoadRow(v, tile * TB + threadIdx.x, n, me);
+  const int done0 = st->done;
+  const double gPrev = sc->gamma[o], aPrev = sc->alpha[o];
+  double f[3];
+  const double *const parts[3] = {P.g[c], P.d[c], P.rho[c]};
+  feFold<TB, 3>(parts, gridDim.x, f);
+  const double gamma = f[0], delta = f[1], rho = f[2];
+  if(done0) return;
+  if(!(rho > tol2) || it >= maxit) {
+    if(blockIdx.x == 0 && threadIdx.x == 0) {
+      st->rr = rho;
+      st->iter = it;
+      st->done = (rho <= tol2) ? 1 : (rho == rho ? 2 : 4);
+    }
+    return;
+  }
+  const double betaD = gamma / gPrev;
+  const double den = delta - betaD * gamma / aPrev;
+  if(!(den > 0.)) {
+    if(blockIdx.x == 0 && threadIdx.x == 0) {
+      st->iter = it;
+      st->done = 3;
+    }
+    return;
+  }
+  const double alphaD = gamma / den;
+  if(blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->gamma[c] = gamma;
+    sc->alpha[c] = alphaD;
+    st->rr = rho;
+    st->hist[it & 63] = rho;
+  }
+  const T alpha = (T)alphaD, beta = (T)betaD;
+  double g = 0., d = 0., rh = 0.;
+  while(tile < nTiles) {
+    const int base = tile * TB;
+    tileProduct<T, LPR, TB, false>(n, base, ia, ja, val, v.m, nullptr, kb, ke, ldsY);
+    __syncthreads();
+    const int row = base + threadIdx.x;
+    if(row < n) {
+      const T y = ldsY[threadIdx.x];
+      const T z = y + beta * me.z;
+      const T q = me.m + beta * me.q;
+      const T s = me.w + beta * me.s;
+      const T p = me.u + beta * me.p;
+      const T rn = me.r - alpha * s;
+      const T un = me.u - alpha * q;
+      const T wn = me.w - alpha * z;
+      v.x[row] = me.x + alpha * p;
+      v.z[row] = z;
+      v.q[row] = q;
+      v.s[row] = s;
+      v.p[row] = p;
+      v.r[row] = rn;
+      v.u[row] = un;
+      v.w[row] = wn;
+      g += (double)rn * (double)un;
+      d += (double)wn * (double)un;
+      rh += (double)rn * (double)rn;
+    }
+    tile += gridDim.x;
+    if(tile < nTiles) {
+      tileBounds<LPR, TB>(n, tile * TB, ia, kb, ke);
+      spLoadRow(v, tile * TB + threadIdx.x, n, me);
+    }
+    __syncthreads();
+  }
+  tileStore3<TB>(g, d, rh, P.g[o], P.d[o], P.rho[o]);
+}
+
 // r = b - A x (FP64); partials of ||r||^2.
 template <int LPR>
 __global__ __launch_bounds__(kTB) void residual64(int n, const int *__restrict__ ia, const int *__restrict__ ja,
@@ -708,10 +867,7 @@ __global__ __launch_bounds__(kTB) void residual64(int n, const int *__restrict__
   for(int chunk = feXcdBlock(); chunk < nChunks; chunk += gridDim.x) {
     const int row = chunk * RPB + threadIdx.x / LPR;
     double sum = 0.;
-    if(row < n) {
-      const int e = ia[row + 1];
-      for(int k = ia[row] + sub; k < e; k += LPR) sum += val[k] * x[ja[k]];
-    }
+    if(row < n) sum = feRowDot<FeKs<LPR>::value, LPR>(ja, val, ia[row] + sub, ia[row + 1], [&](int j) { return x[j]; });
 #pragma unroll
     for(int o = LPR / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, LPR);
     if(sub == 0 && row < n) {
@@ -794,6 +950,8 @@ struct Workspace {
   PipeVecs<float> q32;
   TileVecs<double> t64;
   TileVecs<float> t32;
+  SpVecs<double> sp64;
+  SpVecs<float> sp32;
   PipeParts pparts;
   PipeScalars *sc;
   SplitVecs<double> s64;
@@ -806,7 +964,7 @@ struct Workspace {
 int ensureWorkspace(fe_hip_ctx *ctx, Workspace &w)
 {
   const size_t n = ctx->n;
-  const size_t need = n * (8 * 8 + 8 * 4 + 3 * 8 + 9 * 8 + 9 * 4 + 8 * 8 + 8 * 4 + 9 * 8 + 9 * 4) +
+  const size_t need = n * (8 * 8 + 8 * 4 + 3 * 8 + 9 * 8 + 9 * 4 + 8 * 8 + 8 * 4 + 9 * 8 + 9 * 4 + 10 * 8 + 10 * 4) +
                       24 * kMaxGrid * 8 + 64 * 8 + 8192 + 64 * 1024;
   if(!ctx->d_work || ctx->workBytes < need) {
     if(ctx->d_work) feFree(ctx, ctx->d_work);
@@ -853,6 +1011,14 @@ int ensureWorkspace(fe_hip_ctx *ctx, Workspace &w)
     double **a[] = {&w.t64.x, &w.t64.r, &w.t64.w, &w.t64.s, &w.t64.z, &w.t64.p, &w.t64.dinv, &w.t64.m[0], &w.t64.m[1]};
     for(auto pp : a) *pp = (double *)take(n * 8);
     float **b[] = {&w.t32.x, &w.t32.r, &w.t32.w, &w.t32.s, &w.t32.z, &w.t32.p, &w.t32.dinv, &w.t32.m[0], &w.t32.m[1]};
+    for(auto pp : b) *pp = (float *)take(n * 4);
+  }
+  {
+    double **a[] = {&w.sp64.x, &w.sp64.r, &w.sp64.u, &w.sp64.w, &w.sp64.z,
+                     &w.sp64.q, &w.sp64.s, &w.sp64.p, &w.sp64.m, &w.sp64.t};
+    for(auto pp : a) *pp = (double *)take(n * 8);
+    float **b[] = {&w.sp32.x, &w.sp32.r, &w.sp32.u, &w.sp32.w, &w.sp32.z,
+                   &w.sp32.q, &w.sp32.s, &w.sp32.p, &w.sp32.m, &w.sp32.t};
     for(auto pp : b) *pp = (float *)take(n * 4);
   }
   double **d64[] = {&w.v64.x, &w.v64.r, &w.v64.z, &w.v64.p[0], &w.v64.p[1], &w.v64.q, &w.v64.dinv};
@@ -1124,6 +1290,84 @@ void launchTile(fe_hip_ctx *ctx, Workspace &w, TileVecs<T> &v, const T *val, int
   else launchTileTB<T, 256>(ctx, w, v, val, it, maxit, tol2, init, b, fromResidual);
 }
 
+// Pipelined split-preconditioned PCG from x = 0 on A x = b (b in FP64) with the
+// factor G (lower CSR gL, upper CSR gU of G^T) and dinv; every iteration is
+// K1, K2, K3 (the A product K3 is the timed kernel).
+template <typename T>
+int runSplitPipe(fe_hip_ctx *ctx, Workspace &w, SpVecs<T> &v, const T *val, const T *gL, const T *gU,
+                 const T *dinv, const double *b, int checkEvery, int maxit, double tol2, int *iters, int *done)
+{
+  const int n = ctx->n;
+  const FePrecond &P = ctx->pc;
+  const int TB = ctx->iterBlock;
+  const int G = std::max(1, std::min(ctx->iterGrid, (n + TB - 1) / TB));
+  const int L = ctx->spmvLanes, LF = std::max(1, L / 2);
+  CgState *hs = (CgState *)ctx->h_state;
+  FeKernelTimer &tm = ctx->spmvTimer;
+  tm.used = 0;
+#define FE_SP_LAUNCH(KER, LL, ...)                                                                                 \
+  do {                                                                                                             \
+    if(TB == 1024) {                                                                                               \
+      if(LL == 1) hipLaunchKernelGGL((KER<T, 1, 1024>), dim3(G), dim3(1024), 0, ctx->stream, __VA_ARGS__);          \
+      else if(LL == 2) hipLaunchKernelGGL((KER<T, 2, 1024>), dim3(G), dim3(1024), 0, ctx->stream, __VA_ARGS__);     \
+      else if(LL == 4) hipLaunchKernelGGL((KER<T, 4, 1024>), dim3(G), dim3(1024), 0, ctx->stream, __VA_ARGS__);     \
+      else hipLaunchKernelGGL((KER<T, 8, 1024>), dim3(G), dim3(1024), 0, ctx->stream, __VA_ARGS__);                 \
+    } else if(TB == 512) {                                                                                         \
+      if(LL == 1) hipLaunchKernelGGL((KER<T, 1, 512>), dim3(G), dim3(512), 0, ctx->stream, __VA_ARGS__);            \
+      else if(LL == 2) hipLaunchKernelGGL((KER<T, 2, 512>), dim3(G), dim3(512), 0, ctx->stream, __VA_ARGS__);       \
+      else if(LL == 4) hipLaunchKernelGGL((KER<T, 4, 512>), dim3(G), dim3(512), 0, ctx->stream, __VA_ARGS__);       \
+      else hipLaunchKernelGGL((KER<T, 8, 512>), dim3(G), dim3(512), 0, ctx->stream, __VA_ARGS__);                   \
+    } else {                                                                                                       \
+      if(LL == 1) hipLaunchKernelGGL((KER<T, 1, 256>), dim3(G), dim3(256), 0, ctx->stream, __VA_ARGS__);            \
+      else if(LL == 2) hipLaunchKernelGGL((KER<T, 2, 256>), dim3(G), dim3(256), 0, ctx->stream, __VA_ARGS__);       \
+      else if(LL == 4) hipLaunchKernelGGL((KER<T, 4, 256>), dim3(G), dim3(256), 0, ctx->stream, __VA_ARGS__);       \
+      else hipLaunchKernelGGL((KER<T, 8, 256>), dim3(G), dim3(256), 0, ctx->stream, __VA_ARGS__);                   \
+    }                                                                                                              \
+  } while(0)
+  hipLaunchKernelGGL(spStart<T>, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, b, v);
+  // u_0 = G^T D^-1 G r_0, w_0 = A u_0
+  FE_SP_LAUNCH(spSpmv, LF, n, P.d_iaL, P.d_jaL, gL, (const T *)v.r, dinv, v.t, (const CgState *)w.st, 0);
+  FE_SP_LAUNCH(spSpmv, LF, n, P.d_iaU, P.d_jaU, gU, (const T *)v.t, (const T *)nullptr, v.u, (const CgState *)w.st, 0);
+  FE_SP_LAUNCH(spInitW, L, n, ctx->d_ia, ctx->d_ja, val, v, w.pparts, w.sc, w.st);
+  int it = 0;
+  for(;;) {
+    for(int k = 0; k < checkEvery; ++k, ++it) {
+      FE_SP_LAUNCH(spSpmv, LF, n, P.d_iaL, P.d_jaL, gL, (const T *)v.w, dinv, v.t, (const CgState *)w.st, 1);
+      FE_SP_LAUNCH(spSpmv, LF, n, P.d_iaU, P.d_jaU, gU, (const T *)v.t, (const T *)nullptr, v.m,
+                   (const CgState *)w.st, 1);
+      if(ctx->profiling) {
+        if(tm.used + 2 > tm.ev.size())
+          for(int a = 0; a < 64; ++a) {
+            hipEvent_t e;
+            FE_HIP_TRY(ctx, hipEventCreate(&e));
+            tm.ev.push_back(e);
+          }
+        FE_HIP_TRY(ctx, hipEventRecord(tm.ev[tm.used++], ctx->stream));
+      }
+      FE_SP_LAUNCH(spIter, L, n, ctx->d_ia, ctx->d_ja, val, v, w.pparts, w.sc, w.st, it, maxit, tol2);
+      if(ctx->profiling) FE_HIP_TRY(ctx, hipEventRecord(tm.ev[tm.used++], ctx->stream));
+    }
+    FE_HIP_TRY(ctx, hipGetLastError());
+    FE_HIP_TRY(ctx, hipMemcpyAsync(hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
+    FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if(hs->done) break;
+  }
+#undef FE_SP_LAUNCH
+  if(ctx->profiling) {
+    const size_t valid = std::min<size_t>(tm.used / 2, (size_t)std::max(0, hs->iter));
+    for(size_t pi = 0; pi < valid; ++pi) {
+      float ms = 0.f;
+      FE_HIP_TRY(ctx, hipEventElapsedTime(&ms, tm.ev[2 * pi], tm.ev[2 * pi + 1]));
+      tm.total_ms += ms;
+      tm.launches++;
+    }
+    tm.used = 0;
+  }
+  *iters = hs->iter;
+  *done = hs->done;
+  return FE_STATUS_OK;
+}
+
 // Jacobi-PCG iterations (pipelined kernel chosen by the pipe_kernel tuning
 // key) until the stop test fires.
 template <typename T>
@@ -1293,7 +1537,7 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
   FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   const double bn = std::sqrt(bb);
   const double target = std::max(o->rtol * bn, o->atol);
-  int iters = 0, outer = 0, converged = 0, reason = 0, done = 0;
+  int iters = 0, outer = 0, converged = 0, reason = 0, done = 0, breakdowns = 0;
   double relres = bn > 0. ? 1. : 0.;
 
   double setupMs = 0.;
@@ -1340,14 +1584,29 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
       tol2 = std::max(tol2, target * target * 0.01);
       int it0 = 0;
       FE_HIP_TRY(ctx, hipMemcpyAsync(w.tmp64, w.x64, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+      const FePrecond &Pc = ctx->pc;
       if(inner32) {
-        if(int rc = runSplit<float>(ctx, w, w.s32, ctx->d_val32, w.r64, checkEvery, o->max_iter - iters, tol2, &it0, &done))
+        if(ctx->pipeKernel) {
+          if(int rc = runSplitPipe<float>(ctx, w, w.sp32, ctx->d_val32, Pc.d_gL32, Pc.d_gU32, Pc.d_dinv32, w.r64,
+                                          checkEvery, o->max_iter - iters, tol2, &it0, &done))
+            return rc;
+        } else if(int rc = runSplit<float>(ctx, w, w.s32, ctx->d_val32, w.r64, checkEvery, o->max_iter - iters, tol2,
+                                           &it0, &done)) {
           return rc;
-        hipLaunchKernelGGL(addVec<float>, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, w.x64, w.s32.x);
+        }
+        hipLaunchKernelGGL(addVec<float>, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, w.x64,
+                           ctx->pipeKernel ? w.sp32.x : w.s32.x);
       } else {
-        if(int rc = runSplit<double>(ctx, w, w.s64, ctx->d_val, w.r64, checkEvery, o->max_iter - iters, tol2, &it0, &done))
+        if(ctx->pipeKernel) {
+          if(int rc = runSplitPipe<double>(ctx, w, w.sp64, ctx->d_val, Pc.d_gL64, Pc.d_gU64, Pc.d_dinv64, w.r64,
+                                           checkEvery, o->max_iter - iters, tol2, &it0, &done))
+            return rc;
+        } else if(int rc = runSplit<double>(ctx, w, w.s64, ctx->d_val, w.r64, checkEvery, o->max_iter - iters, tol2,
+                                            &it0, &done)) {
           return rc;
-        hipLaunchKernelGGL(addVec<double>, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, w.x64, w.s64.x);
+        }
+        hipLaunchKernelGGL(addVec<double>, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, w.x64,
+                           ctx->pipeKernel ? w.sp64.x : w.s64.x);
       }
       iters += it0;
       double rr = 0.;
@@ -1362,7 +1621,10 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
         continue;
       }
       rn = std::sqrt(rr);
-      if(done == 3 || done == 4) { reason = -3; break; }
+      // a breakdown of the recurrences ends the inner solve; the next outer
+      // iteration restarts it from the FP64 residual
+      if(done == 3 && ++breakdowns > 20) { reason = -3; break; }
+      if(done == 4 || !(rn == rn)) { reason = -9; break; }
       if(rn / bn > o->dtol) { reason = -4; break; }
     }
     relres = rn / bn;
@@ -1388,12 +1650,17 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
       relres = std::sqrt(rr) / bn;
       outer++;
       if(std::sqrt(rr) <= target) { converged = 1; reason = 2; break; }
-      if(done == 3) { reason = -3; break; }
       if(done == 4 || !(rr == rr)) { reason = -9; break; }
       if(iters >= o->max_iter) { reason = -3; break; }
       if(relres > o->dtol) { reason = -4; break; }
-      // restart from the true residual (r holds b - A x)
-      tol2 *= 0.01;
+      // restart from the true residual (r holds b - A x): after a breakdown of
+      // the recurrences, or when the recursive residual met a target the true
+      // one has not (then aim lower)
+      if(done == 3) {
+        if(++breakdowns > 20) { reason = -3; break; }
+      } else {
+        tol2 *= 0.01;
+      }
       startJacobi<double>(ctx, w, w.q64, w.t64, ctx->d_val, ctx->d_rhs, 1);
     }
     FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_du, x64, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
@@ -1445,7 +1712,10 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
         continue;
       }
       rn = std::sqrt(rr);
-      if(done == 3 || done == 4) { reason = -3; break; }
+      // a breakdown of the recurrences ends the inner solve; the next outer
+      // iteration restarts it from the FP64 residual
+      if(done == 3 && ++breakdowns > 20) { reason = -3; break; }
+      if(done == 4 || !(rn == rn)) { reason = -9; break; }
       if(rn / bn > o->dtol) { reason = -4; break; }
     }
     relres = rn / bn;

@@ -71,25 +71,35 @@ __device__ __forceinline__ void tileProduct(int n, int base, const int *__restri
 {
   constexpr int RPB = TB / LPR;
   const int sub = threadIdx.x % LPR, g = threadIdx.x / LPR;
+  // the LPR rows of a lane group advance together: per step each lane loads
+  // the column ids and values of KS entries of every row, then gathers
+  constexpr int KS = LPR >= 16 ? 1 : 16 / LPR;
   T acc[LPR];
-  int k[LPR];
+  int k0 = 0;
 #pragma unroll
-  for(int u = 0; u < LPR; ++u) {
-    acc[u] = (T)0;
-    k[u] = kb[u];
-  }
+  for(int u = 0; u < LPR; ++u) acc[u] = (T)0;
   for(;;) {
-    bool more = false;
+    int j[LPR][KS];
+    T v[LPR][KS];
+    bool ok[LPR][KS];
+    bool any = false;
 #pragma unroll
     for(int u = 0; u < LPR; ++u)
-      if(k[u] < ke[u]) {
-        const int j = ja[k[u]];
-        const T xj = SCALE ? scale[j] * src[j] : src[j];
-        acc[u] += val[k[u]] * xj;
-        k[u] += LPR;
-        more = true;
+#pragma unroll
+      for(int s = 0; s < KS; ++s) {
+        const int kk = kb[u] + k0 + s * LPR;
+        ok[u][s] = kk < ke[u];
+        any |= ok[u][s];
+        j[u][s] = ok[u][s] ? ja[kk] : 0;
+        v[u][s] = ok[u][s] ? val[kk] : (T)0;
       }
-    if(!more) break;
+    if(!any) break;
+#pragma unroll
+    for(int u = 0; u < LPR; ++u)
+#pragma unroll
+      for(int s = 0; s < KS; ++s)
+        if(ok[u][s]) acc[u] += v[u][s] * (SCALE ? scale[j[u][s]] * src[j[u][s]] : src[j[u][s]]);
+    k0 += KS * LPR;
   }
 #pragma unroll
   for(int u = 0; u < LPR; ++u) {
