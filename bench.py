@@ -14,8 +14,10 @@ device CSR) + solve, inputs already resident in HBM.  Paths:
             FP64 iterative refinement
   learned : as fast, with the learned IC(0)-pattern split preconditioner
             (weights from feng_amd/weights/, skipped when absent)
+  learned64 : FP64 split PCG with the learned preconditioner
   ic0     : FP64 split IC(0)-pattern PCG (the learned path without the net)
-`value` is the DOF/s of the faster of fast/learned (named in config.path);
+`value` is the DOF/s of the fastest of fast/learned/learned64 (named in
+config.path);
 every path is reported in config.paths.
 
 --workload batched runs the 256-problem batched workload instead;
@@ -65,8 +67,25 @@ PATHS = {
     "plain": ("fp64", "jacobi", 1e-3),
     "fast": ("fp32", "jacobi", 1e-2),
     "learned": ("fp32", "learned", 1e-2),
+    "learned64": ("fp64", "learned", 1e-3),
     "ic0": ("fp64", "ic0", 1e-3),
 }
+
+# Timed kernel of each path (the A product of one PCG iteration) and its
+# algorithmic bytes per launch: s nnz values + 4 nnz column ids + 4 (n+1) row
+# pointers + the vector passes of one iteration (minimal unfused PCG-Jacobi:
+# 18 passes of s n), s = 8 (FP64) or 4 (FP32).
+ITER_KERNEL = {
+    "plain": ("tileIter<double>", 8),
+    "fast": ("pipeIter<float>", 4),
+    "learned": ("cgSpmv<float> (split PCG, A product)", 4),
+    "learned64": ("spIter<double> (split PCG, A product + updates)", 8),
+    "ic0": ("spIter<double> (split PCG, A product + updates)", 8),
+}
+
+
+def iter_bytes(s, n, nnz):
+    return (s + 4) * nnz + 4 * (n + 1) + 18 * s * n
 
 
 def run_step(S, path, rtol, profile=False):
@@ -191,7 +210,7 @@ def main():
         from feng_amd.learned import load_weights
 
         load_weights(S)
-        paths.append("learned")
+        paths += ["learned", "learned64"]
     except FileNotFoundError as e:
         log("learned path skipped:", e)
 
@@ -208,7 +227,7 @@ def main():
         res[path] = dict(ms=1e3 * wall, dofs=n / wall, steps=steps, last=steps[-1])
         log(path, f"{1e3 * wall:.3f} ms/step", steps[-1])
 
-    fast_paths = [p for p in ("fast", "learned") if p in res]
+    fast_paths = [p for p in ("fast", "learned", "learned64") if p in res]
     best = min(fast_paths, key=lambda p: res[p]["ms"])
     B = res[best]
     plain = res["plain"]
@@ -218,13 +237,11 @@ def main():
     # values s nnz + column ids 4 nnz + row pointers 4 (n+1) + 18 vector
     # passes of s n.  FP32 inner solve: s = 4.
     kern_ms = float(np.mean([r["spmv_ms"] for r in B["steps"]]))
-    alg_bytes = 8 * nnz + 4 * (n + 1) + 72 * n
-    kname = "tileIter<float>" if best == "fast" else "cgSpmv<float> (split PCG product)"
-    if best == "learned":
-        alg_bytes = 8 * nnz + 4 * (n + 1) + 16 * n
+    kname, sz = ITER_KERNEL[best]
+    alg_bytes = iter_bytes(sz, n, nnz)
     achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
     p_kern = float(np.mean([r["spmv_ms"] for r in plain["steps"]]))
-    p_alg = 12 * nnz + 4 * (n + 1) + 144 * n
+    p_alg = iter_bytes(8, n, nnz)
 
     cpu = None
     if not args.no_cpu:
@@ -260,7 +277,8 @@ def main():
         "config": {"workload": f"P1 tet diffusion, unit cube {args.N}^3 x6 Kuhn tets, k=exp(GRF l=0.1 s=1.5), "
                                f"PCG to FP64 rel-res 1e-8", "n": n, "nnz": nnz, "elements": m.E,
                    "path": {"fast": "FP32 Jacobi-PCG + FP64 iterative refinement",
-                            "learned": "FP32 learned-IC(0) split PCG + FP64 iterative refinement"}[best],
+                            "learned": "FP32 learned-IC(0) split PCG + FP64 iterative refinement",
+                            "learned64": "FP64 learned-IC(0) split PCG"}[best],
                    "paths": {p: summary(p) for p in res},
                    "speedup_vs_plain_fp64": plain["ms"] / B["ms"],
                    "plain_iter_roofline_frac": p_alg / (p_kern * 1e-3) / 1e9 / HBM_PEAK_GBS if p_kern else None},

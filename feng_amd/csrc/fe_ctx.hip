@@ -455,8 +455,12 @@ int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value)
     FE_REQUIRE(ctx, value == 0 || value == 1, "fp32_values must be 0 (rounded FP64) or 1 (FP32 MFMA)");
     ctx->fp32Values = value;
   } else if(k == "pipe_kernel") {
-    FE_REQUIRE(ctx, value == 0 || value == 1, "pipe_kernel must be 0 (gathered D^-1 r) or 1 (tiled)");
+    FE_REQUIRE(ctx, value >= 0 && value <= 2,
+               "pipe_kernel must be 0 (Chronopoulos-Gear), 1 (tiled Ghysels-Vanroose) or 2 (by precision)");
     ctx->pipeKernel = value;
+  } else if(k == "ic0_sweeps") {
+    FE_REQUIRE(ctx, value >= 0 && value <= 64, "ic0_sweeps must be 0 (exact ILU(0)) or 1..64");
+    ctx->icSweeps = value;
   } else if(k == "iter_grid") {
     FE_REQUIRE(ctx, value >= 8 && value <= 4096 && value % 8 == 0, "iter_grid must be a multiple of 8 in [8, 4096]");
     ctx->iterGrid = value;
@@ -492,6 +496,7 @@ int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value)
   else if(k == "asm_kernel") *value = ctx->asmKernel;
   else if(k == "fp32_values") *value = ctx->fp32Values;
   else if(k == "pipe_kernel") *value = ctx->pipeKernel;
+  else if(k == "ic0_sweeps") *value = ctx->icSweeps;
   else FE_REQUIRE(ctx, false, "unknown tuning key " + k);
   return FE_STATUS_OK;
 }

@@ -103,6 +103,9 @@ struct FePrecond {
   double *d_lu = nullptr, *d_gL64 = nullptr, *d_gU64 = nullptr, *d_dinv64 = nullptr, *d_dval = nullptr;
   float *d_gL32 = nullptr, *d_gU32 = nullptr, *d_dinv32 = nullptr;
   int *d_tposL = nullptr;  // for each lower entry of G, its position in the upper CSR of G^T
+  int *d_tposA = nullptr;  // for each entry (i, j) of A, the position of (j, i)
+  bool symmetricPattern = false;
+  double *d_icL = nullptr, *d_icD = nullptr;  // parallel IC(0) iterates (two of each)
   int lastShifts = 0;
   double lastSetupMs = 0.;
   int sweeps = 2;  // Jacobi sweeps per triangular factor in GMRES/ILU(0)
@@ -148,7 +151,8 @@ struct fe_hip_ctx {
   bool lanesFixed = false;
   int asmKernel = 1;     // 0: scalar FP64 loop (reference order), 1: MFMA GEMM form
   int fp32Values = 1;    // FP32 values: 0 = rounded FP64 values, 1 = FP32 MFMA element matrices
-  int pipeKernel = 1;
+  int pipeKernel = 2;  // pipelined PCG: 0 Chronopoulos-Gear, 1 tiled Ghysels-Vanroose, 2 by precision
+  int icSweeps = 0;    // FE_PC_IC0: 0 exact ILU(0), > 0 parallel IC(0) sweeps
   FeMpnn *mpnn = nullptr;
   FeBatch *batch = nullptr;    // Jacobi-PCG: 0 = on-the-fly D^-1 r gather, 1 = tiled (one gathered vector)
 
@@ -205,7 +209,10 @@ int feExclusiveScan(fe_hip_ctx *ctx, const int *d_in, int *d_out, int n, int64_t
 
 // Preconditioner set-up (fe_precond.hip)
 int fePrecondSetupPattern(fe_hip_ctx *ctx);
-int fePrecondFactor(fe_hip_ctx *ctx);
+// sweeps = 0: exact level-scheduled ILU(0); > 0: parallel IC(0) sweeps.
+int fePrecondFactor(fe_hip_ctx *ctx, int sweeps = 0);
+// Parallel IC(0) sweeps that feed the learned correction.
+constexpr int kLearnedIcSweeps = 3;
 
 // Learned correction of the split factor (fe_learned.hip)
 int feLearnedCorrect(fe_hip_ctx *ctx);

@@ -1371,10 +1371,10 @@ int runSplitPipe(fe_hip_ctx *ctx, Workspace &w, SpVecs<T> &v, const T *val, cons
 // Jacobi-PCG iterations (pipelined kernel chosen by the pipe_kernel tuning
 // key) until the stop test fires.
 template <typename T>
-int runJacobi(fe_hip_ctx *ctx, Workspace &w, PipeVecs<T> &pv, TileVecs<T> &tv, const T *val, int checkEvery,
-              int maxit, double tol2, int *iters, int *done)
+int runJacobi(fe_hip_ctx *ctx, bool tiled, Workspace &w, PipeVecs<T> &pv, TileVecs<T> &tv, const T *val,
+              int checkEvery, int maxit, double tol2, int *iters, int *done)
 {
-  if(!ctx->pipeKernel) return runPipe<T>(ctx, w, pv, val, checkEvery, maxit, tol2, iters, done);
+  if(!tiled) return runPipe<T>(ctx, w, pv, val, checkEvery, maxit, tol2, iters, done);
   CgState *hs = (CgState *)ctx->h_state;
   FeKernelTimer &tm = ctx->spmvTimer;
   tm.used = 0;
@@ -1414,10 +1414,10 @@ int runJacobi(fe_hip_ctx *ctx, Workspace &w, PipeVecs<T> &pv, TileVecs<T> &tv, c
 }
 
 template <typename T>
-void startJacobi(fe_hip_ctx *ctx, Workspace &w, PipeVecs<T> &pv, TileVecs<T> &tv, const T *val, const double *b,
-                 int fromResidual)
+void startJacobi(fe_hip_ctx *ctx, bool tiled, Workspace &w, PipeVecs<T> &pv, TileVecs<T> &tv, const T *val,
+                 const double *b, int fromResidual)
 {
-  if(ctx->pipeKernel) launchTile<T>(ctx, w, tv, val, 0, 0, 0., true, b, fromResidual);
+  if(tiled) launchTile<T>(ctx, w, tv, val, 0, 0, 0., true, b, fromResidual);
   else launchPipe<T>(ctx, w, pv, val, 0, 0, 0., true, b, fromResidual);
 }
 
@@ -1518,6 +1518,11 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
   FE_REQUIRE(ctx, o->pc == FE_PC_JACOBI || o->pc == FE_PC_IC0 || o->pc == FE_PC_LEARNED,
              "CG takes the JACOBI, IC0 or LEARNED preconditioner");
   const bool split = (o->pc != FE_PC_JACOBI);
+  // pipelined form per precision: the tiled Ghysels-Vanroose iteration (one
+  // gathered vector) for FP64 solves, Chronopoulos-Gear (w = A u every
+  // iteration, fewer refinement restarts) for FP32 inner solves
+  const bool tiled64 = ctx->pipeKernel == 2 ? true : ctx->pipeKernel == 1;
+  const bool tiled32 = ctx->pipeKernel == 2 ? false : ctx->pipeKernel == 1;
   Workspace w;
   if(int rc = ensureWorkspace(ctx, w)) return rc;
   const int n = ctx->n;
@@ -1546,7 +1551,7 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
     FE_HIP_TRY(ctx, hipEventCreate(&s0));
     FE_HIP_TRY(ctx, hipEventCreate(&s1));
     FE_HIP_TRY(ctx, hipEventRecord(s0, ctx->stream));
-    if(int rc = fePrecondFactor(ctx)) return rc;
+    if(int rc = fePrecondFactor(ctx, o->pc == FE_PC_LEARNED ? kLearnedIcSweeps : ctx->icSweeps)) return rc;
     if(o->pc == FE_PC_LEARNED)
       if(int rc = feLearnedCorrect(ctx)) return rc;
     FE_HIP_TRY(ctx, hipEventRecord(s1, ctx->stream));
@@ -1586,7 +1591,7 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
       FE_HIP_TRY(ctx, hipMemcpyAsync(w.tmp64, w.x64, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
       const FePrecond &Pc = ctx->pc;
       if(inner32) {
-        if(ctx->pipeKernel) {
+        if(tiled32) {
           if(int rc = runSplitPipe<float>(ctx, w, w.sp32, ctx->d_val32, Pc.d_gL32, Pc.d_gU32, Pc.d_dinv32, w.r64,
                                           checkEvery, o->max_iter - iters, tol2, &it0, &done))
             return rc;
@@ -1595,9 +1600,9 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
           return rc;
         }
         hipLaunchKernelGGL(addVec<float>, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, w.x64,
-                           ctx->pipeKernel ? w.sp32.x : w.s32.x);
+                           tiled32 ? w.sp32.x : w.s32.x);
       } else {
-        if(ctx->pipeKernel) {
+        if(tiled64) {
           if(int rc = runSplitPipe<double>(ctx, w, w.sp64, ctx->d_val, Pc.d_gL64, Pc.d_gU64, Pc.d_dinv64, w.r64,
                                            checkEvery, o->max_iter - iters, tol2, &it0, &done))
             return rc;
@@ -1606,7 +1611,7 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
           return rc;
         }
         hipLaunchKernelGGL(addVec<double>, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, w.x64,
-                           ctx->pipeKernel ? w.sp64.x : w.s64.x);
+                           tiled64 ? w.sp64.x : w.s64.x);
       }
       iters += it0;
       double rr = 0.;
@@ -1631,17 +1636,17 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
     if(!converged && rn <= target) { converged = 1; reason = 2; }
     FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_du, w.x64, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
   } else if(o->precision == FE_PREC_FP64) {
-    double *dinv64 = ctx->pipeKernel ? w.t64.dinv : w.q64.dinv;
-    double *x64 = ctx->pipeKernel ? w.t64.x : w.q64.x;
-    double *r64 = ctx->pipeKernel ? w.t64.r : w.q64.r[0];
+    double *dinv64 = tiled64 ? w.t64.dinv : w.q64.dinv;
+    double *x64 = tiled64 ? w.t64.x : w.q64.x;
+    double *r64 = tiled64 ? w.t64.r : w.q64.r[0];
     hipLaunchKernelGGL(extractDinv<double>, dim3((n + kTB - 1) / kTB), dim3(kTB), 0, ctx->stream, n, ctx->d_diag,
                        ctx->d_val, dinv64);
     // recursive-residual target; the FP64 true residual is checked after
     double tol2 = target * target * 0.25;
-    startJacobi<double>(ctx, w, w.q64, w.t64, ctx->d_val, ctx->d_rhs, 0);
+    startJacobi<double>(ctx, tiled64, w, w.q64, w.t64, ctx->d_val, ctx->d_rhs, 0);
     for(;;) {
       int it = 0;
-      if(int rc = runJacobi<double>(ctx, w, w.q64, w.t64, ctx->d_val, checkEvery, o->max_iter - iters, tol2, &it,
+      if(int rc = runJacobi<double>(ctx, tiled64, w, w.q64, w.t64, ctx->d_val, checkEvery, o->max_iter - iters, tol2, &it,
                                     &done))
         return rc;
       iters += it;
@@ -1661,13 +1666,13 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
       } else {
         tol2 *= 0.01;
       }
-      startJacobi<double>(ctx, w, w.q64, w.t64, ctx->d_val, ctx->d_rhs, 1);
+      startJacobi<double>(ctx, tiled64, w, w.q64, w.t64, ctx->d_val, ctx->d_rhs, 1);
     }
     FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_du, x64, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
   } else {
     FE_REQUIRE(ctx, ctx->d_val32, "assemble with FE_PREC_FP32_IR to create the FP32 values");
     hipLaunchKernelGGL(extractDinv<float>, dim3((n + kTB - 1) / kTB), dim3(kTB), 0, ctx->stream, n, ctx->d_diag,
-                       ctx->d_val, ctx->pipeKernel ? w.t32.dinv : w.q32.dinv);
+                       ctx->d_val, tiled32 ? w.t32.dinv : w.q32.dinv);
     FE_HIP_TRY(ctx, hipMemsetAsync(w.x64, 0, sizeof(double) * n, ctx->stream));
     FE_HIP_TRY(ctx, hipMemcpyAsync(w.r64, ctx->d_rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
     double rn = bn;
@@ -1680,24 +1685,24 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
       double tol2 = inner32 ? (o->inner_rtol * rn) * (o->inner_rtol * rn) : target * target * 0.25;
       tol2 = std::max(tol2, target * target * 0.01);
       int it0 = 0;
-      const float *d32 = ctx->pipeKernel ? w.t32.x : w.q32.x;
+      const float *d32 = tiled32 ? w.t32.x : w.q32.x;
       FE_HIP_TRY(ctx, hipMemcpyAsync(w.tmp64, w.x64, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
       if(inner32) {
-        startJacobi<float>(ctx, w, w.q32, w.t32, ctx->d_val32, w.r64, 0);
-        if(int rc = runJacobi<float>(ctx, w, w.q32, w.t32, ctx->d_val32, checkEvery, o->max_iter - iters, tol2, &it0,
+        startJacobi<float>(ctx, tiled32, w, w.q32, w.t32, ctx->d_val32, w.r64, 0);
+        if(int rc = runJacobi<float>(ctx, tiled32, w, w.q32, w.t32, ctx->d_val32, checkEvery, o->max_iter - iters, tol2, &it0,
                                      &done))
           return rc;
         hipLaunchKernelGGL(addVec<float>, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, w.x64, d32);
       } else {
         // FP64 inner solve of A d = r (after an FP32 correction was rejected)
         hipLaunchKernelGGL(extractDinv<double>, dim3((n + kTB - 1) / kTB), dim3(kTB), 0, ctx->stream, n,
-                           ctx->d_diag, ctx->d_val, ctx->pipeKernel ? w.t64.dinv : w.q64.dinv);
-        startJacobi<double>(ctx, w, w.q64, w.t64, ctx->d_val, w.r64, 0);
-        if(int rc = runJacobi<double>(ctx, w, w.q64, w.t64, ctx->d_val, checkEvery, o->max_iter - iters, tol2, &it0,
+                           ctx->d_diag, ctx->d_val, tiled64 ? w.t64.dinv : w.q64.dinv);
+        startJacobi<double>(ctx, tiled64, w, w.q64, w.t64, ctx->d_val, w.r64, 0);
+        if(int rc = runJacobi<double>(ctx, tiled64, w, w.q64, w.t64, ctx->d_val, checkEvery, o->max_iter - iters, tol2, &it0,
                                       &done))
           return rc;
         hipLaunchKernelGGL(addVec<double>, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, w.x64,
-                           ctx->pipeKernel ? w.t64.x : w.q64.x);
+                           tiled64 ? w.t64.x : w.q64.x);
       }
       iters += it0;
       double rr = 0.;

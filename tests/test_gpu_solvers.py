@@ -78,3 +78,26 @@ def test_gmres_ilu0_kovasznay_newton(oracle):
     ue = ufn(xyz[uf, 0], xyz[uf, 1], xyz[uf, 2], num.dof_comp[uf])
     assert np.abs(u[uf] - ue).max() < 5e-3
     S.close()
+
+
+def test_parallel_ic0_converges_to_the_exact_factor(oracle):
+    """Parallel IC(0) sweeps (Chow & Patel): the fixed point is the exact
+    IC(0) factor (oracle ILU(0) of the symmetric matrix), and three sweeps
+    already precondition as well as the exact factor."""
+    from feng_amd.api import FN_ELEM, PC_IC0, Mesh
+    from feng_amd.problems import diffusion_system
+    m = Mesh.cube(10)
+    k = m.coeff_grf(0.1, 1.5, 64, 3)
+    S, num = diffusion_system(m, 4, 1, _fn(FN_ELEM, data=k), _fn(0, -1.0))
+    S.assemble(3)
+    ia, ja = S.get_csr()
+    val, _ = S.get_system()
+    lu_o, rc = oracle.ilu0(ia, ja, val)
+    r_exact = S.solve(pc=PC_IC0, rtol=1e-8)
+    S.set_tuning("ic0_sweeps", 40)
+    S.solve(pc=PC_IC0, rtol=1e-8)
+    assert relerr(S.get_factor(), lu_o) < 1e-10
+    S.set_tuning("ic0_sweeps", 3)
+    r3 = S.solve(pc=PC_IC0, rtol=1e-8)
+    assert r3.converged and abs(r3.iterations - r_exact.iterations) <= 3
+    S.close()
