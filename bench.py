@@ -43,7 +43,10 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernels from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (scaled by the same counters on a plain copy of known size).
-TRAFFIC = {}
+TRAFFIC = {  # profiles/cube64_learned64_hbm_traffic.md (cube 64^3, learned FP64 path)
+    "spIter<double> (split PCG, A product + updates)": 79.6e6,
+    "tileIter<double>": 72.8e6,
+}
 
 
 def log(*a):
@@ -193,6 +196,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=25.0)
     ap.add_argument("--workload", default="cube", choices=["cube", "batched"])
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--paths", default=None, help="comma-separated subset of " + ",".join(PATHS))
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -214,6 +218,11 @@ def main():
     except FileNotFoundError as e:
         log("learned path skipped:", e)
 
+    if args.paths:
+        want = args.paths.split(",")
+        paths = [p for p in paths if p in want]
+        if "plain" not in paths:
+            paths.insert(0, "plain")
     res = {}
     for path in paths:
         for _ in range(max(1, args.warmup)):
@@ -227,7 +236,7 @@ def main():
         res[path] = dict(ms=1e3 * wall, dofs=n / wall, steps=steps, last=steps[-1])
         log(path, f"{1e3 * wall:.3f} ms/step", steps[-1])
 
-    fast_paths = [p for p in ("fast", "learned", "learned64") if p in res]
+    fast_paths = [p for p in ("fast", "learned", "learned64") if p in res] or ["plain"]
     best = min(fast_paths, key=lambda p: res[p]["ms"])
     B = res[best]
     plain = res["plain"]
@@ -278,7 +287,8 @@ def main():
                                f"PCG to FP64 rel-res 1e-8", "n": n, "nnz": nnz, "elements": m.E,
                    "path": {"fast": "FP32 Jacobi-PCG + FP64 iterative refinement",
                             "learned": "FP32 learned-IC(0) split PCG + FP64 iterative refinement",
-                            "learned64": "FP64 learned-IC(0) split PCG"}[best],
+                            "learned64": "FP64 learned-IC(0) split PCG",
+                            "plain": "FP64 Jacobi-PCG"}[best],
                    "paths": {p: summary(p) for p in res},
                    "speedup_vs_plain_fp64": plain["ms"] / B["ms"],
                    "plain_iter_roofline_frac": p_alg / (p_kern * 1e-3) / 1e9 / HBM_PEAK_GBS if p_kern else None},
