@@ -36,8 +36,11 @@ struct FeBatch {
   TileVecs<float> v32;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int graphKey[4] = {-1, -1, -1, -1};  // TB, lanes, precision, maxit of the captured graph
+  int graphKey[5] = {-1, -1, -1, -1, -1};  // TB, lanes, precision, maxit, active tiles of the captured graph
   size_t cap = 0;  // allocated tiles
+  std::vector<int> hPtile;  // host copy of the per-problem tile ranges
+  int *act = nullptr;       // tiles of the problems still iterating (the iteration grid)
+  int nAct = 0;
 };
 
 void feBatchRelease(fe_hip_ctx *ctx)
@@ -161,10 +164,10 @@ __global__ __launch_bounds__(TB) void bInit(const int *__restrict__ ia, const in
 template <typename T, int LPR, int TB>
 __global__ __launch_bounds__(TB) void bIter(const int *__restrict__ ia, const int *__restrict__ ja,
                                             const T *__restrict__ val, TileVecs<T> v, BTiles bt, BParts P, BState S,
-                                            int k, int maxit)
+                                            const int *__restrict__ act, int k, int maxit)
 {
   __shared__ T ldsY[TB];
-  const int tile = blockIdx.x, p = bt.prob[tile];
+  const int tile = act[blockIdx.x], p = bt.prob[tile];
   if(S.done[p]) return;
   const int it = S.itBase[0] + k;
   const int base = bt.row0[tile], nEnd = base + bt.rows[tile];
@@ -312,8 +315,9 @@ int buildTiles(fe_hip_ctx *ctx, FeBatch &B, int TB)
     for(int k = 0; k < 2; ++k)
       for(void *q : {(void *)B.pg[k], (void *)B.pd[k], (void *)B.prho[k]})
         if(q) feFree(ctx, q);
+    if(B.act) feFree(ctx, B.act);
     if(feAlloc(ctx, &B.tRow0, nt) || feAlloc(ctx, &B.tRows, nt) || feAlloc(ctx, &B.tProb, nt) ||
-       feAlloc(ctx, &B.part, nt))
+       feAlloc(ctx, &B.part, nt) || feAlloc(ctx, &B.act, nt))
       return FE_STATUS_OUT_OF_MEMORY;
     for(int k = 0; k < 2; ++k)
       if(feAlloc(ctx, &B.pg[k], nt) || feAlloc(ctx, &B.pd[k], nt) || feAlloc(ctx, &B.prho[k], nt))
@@ -326,6 +330,8 @@ int buildTiles(fe_hip_ctx *ctx, FeBatch &B, int TB)
   FE_HIP_TRY(ctx, hipMemcpy(B.pTile, ptile.data(), sizeof(int) * (B.nprob + 1), hipMemcpyHostToDevice));
   B.nTiles = (int)nt;
   B.TB = TB;
+  B.hPtile = ptile;
+  B.nAct = 0;
   B.graphKey[0] = -1;  // tiles changed: recapture
   return FE_STATUS_OK;
 }
@@ -375,8 +381,8 @@ void launchInit(fe_hip_ctx *ctx, FeBatch &B, const T *val, TileVecs<T> &v)
 template <typename T, int LPR, int TB>
 void launchIter(fe_hip_ctx *ctx, FeBatch &B, const T *val, TileVecs<T> &v, int k, int maxit)
 {
-  hipLaunchKernelGGL((bIter<T, LPR, TB>), dim3(B.nTiles), dim3(TB), 0, ctx->stream, ctx->d_ia, ctx->d_ja, val, v,
-                     tilesOf(B), partsOf(B), stateOf(B), k, maxit);
+  hipLaunchKernelGGL((bIter<T, LPR, TB>), dim3(B.nAct), dim3(TB), 0, ctx->stream, ctx->d_ia, ctx->d_ja, val, v,
+                     tilesOf(B), partsOf(B), stateOf(B), (const int *)B.act, k, maxit);
 }
 
 // Dispatch over (lanes, block) of a templated launcher.
@@ -411,8 +417,8 @@ struct IterL {
 template <typename T>
 int captureGraph(fe_hip_ctx *ctx, FeBatch &B, const T *val, TileVecs<T> &v, int maxit, int prec)
 {
-  const int key[4] = {B.TB, ctx->spmvLanes, prec, maxit};
-  if(B.exec && std::equal(key, key + 4, B.graphKey)) return FE_STATUS_OK;
+  const int key[5] = {B.TB, ctx->spmvLanes, prec, maxit, B.nAct};
+  if(B.exec && std::equal(key, key + 5, B.graphKey)) return FE_STATUS_OK;
   if(B.exec) (void)hipGraphExecDestroy(B.exec);
   if(B.graph) (void)hipGraphDestroy(B.graph);
   B.exec = nullptr;
@@ -428,15 +434,33 @@ int captureGraph(fe_hip_ctx *ctx, FeBatch &B, const T *val, TileVecs<T> &v, int 
   FE_HIP_TRY(ctx, e);
   B.graph = g;
   FE_HIP_TRY(ctx, hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
-  std::copy(key, key + 4, B.graphKey);
+  std::copy(key, key + 5, B.graphKey);
   return FE_STATUS_OK;
 }
 
-// Runs inner iterations until every problem has stopped (or maxit).
+// Uploads the tiles of the problems with done[p] == 0 as the iteration grid.
+int setActive(fe_hip_ctx *ctx, FeBatch &B, const std::vector<int> &done)
+{
+  std::vector<int> act;
+  act.reserve(B.nTiles);
+  for(int p = 0; p < B.nprob; ++p)
+    if(!done[p])
+      for(int t = B.hPtile[p]; t < B.hPtile[p + 1]; ++t) act.push_back(t);
+  B.nAct = std::max(1, (int)act.size());
+  if(act.empty()) act.push_back(0);  // a stopped problem's tile returns at once
+  FE_HIP_TRY(ctx, hipMemcpyAsync(B.act, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, ctx->stream));
+  return FE_STATUS_OK;
+}
+
+// Runs inner iterations until every problem has stopped (or maxit).  The
+// iteration grid covers only the tiles of problems still iterating: when the
+// active tiles fall below 3/4 of the captured grid, the list is rebuilt and
+// the graph recaptured with the smaller grid.
 template <typename T>
 int runInner(fe_hip_ctx *ctx, FeBatch &B, const T *val, TileVecs<T> &v, int maxit, int prec, std::vector<int> &hDone,
-             std::vector<int> &hIter)
+             std::vector<int> &hIter, const std::vector<int> &outerDone)
 {
+  if(int rc = setActive(ctx, B, outerDone)) return rc;
   if(int rc = captureGraph<T>(ctx, B, val, v, maxit, prec)) return rc;
   FE_HIP_TRY(ctx, hipMemsetAsync(B.itBase, 0, sizeof(int), ctx->stream));
   if(int rc = dispatch<T, InitL>(ctx, ctx->spmvLanes, B.TB, B, val, v)) return rc;
@@ -448,6 +472,13 @@ int runInner(fe_hip_ctx *ctx, FeBatch &B, const T *val, TileVecs<T> &v, int maxi
       FE_HIP_TRY(ctx, hipMemcpyAsync(hDone.data(), B.done, sizeof(int) * B.nprob, hipMemcpyDeviceToHost, ctx->stream));
       FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
       if(std::all_of(hDone.begin(), hDone.end(), [](int d) { return d != 0; })) break;
+      int active = 0;
+      for(int p = 0; p < B.nprob; ++p)
+        if(!hDone[p]) active += B.hPtile[p + 1] - B.hPtile[p];
+      if(active < (3 * B.nAct) / 4) {
+        if(int rc = setActive(ctx, B, hDone)) return rc;
+        if(int rc = captureGraph<T>(ctx, B, val, v, maxit, prec)) return rc;
+      }
     }
   }
   FE_HIP_TRY(ctx, hipMemcpyAsync(hDone.data(), B.done, sizeof(int) * B.nprob, hipMemcpyDeviceToHost, ctx->stream));
@@ -526,10 +557,10 @@ extern "C" int fe_hip_solve_batched(fe_hip_ctx *ctx, int nprob, const int *rows,
     for(int p = 0; p < nprob; ++p)
       if(!outerDone[p]) budget = std::max(budget, o->max_iter - its[p]);
     if(inner32) {
-      if(int rc = runInner<float>(ctx, B, ctx->d_val32, B.v32, budget, 1, hDone, hIter)) return rc;
+      if(int rc = runInner<float>(ctx, B, ctx->d_val32, B.v32, budget, 1, hDone, hIter, outerDone)) return rc;
       hipLaunchKernelGGL(bAddCorrection<float>, dim3(1024), dim3(256), 0, ctx->stream, n, B.x64, B.v32.x);
     } else {
-      if(int rc = runInner<double>(ctx, B, ctx->d_val, B.v64, budget, 0, hDone, hIter)) return rc;
+      if(int rc = runInner<double>(ctx, B, ctx->d_val, B.v64, budget, 0, hDone, hIter, outerDone)) return rc;
       hipLaunchKernelGGL(bAddCorrection<double>, dim3(1024), dim3(256), 0, ctx->stream, n, B.x64, B.v64.x);
     }
     std::vector<double> rnew;
@@ -609,6 +640,7 @@ extern "C" int fe_hip_autotune_batched(fe_hip_ctx *ctx, int nprob, const int *ro
       ctx->spmvLanes = L;
       ctx->iterBlock = TB;
       if(int rc = buildTiles(ctx, B, TB)) return rc;
+      if(int rc = setActive(ctx, B, zeros)) return rc;
       const int prec = fp32 ? 1 : 0;
       if(fp32 ? captureGraph<float>(ctx, B, ctx->d_val32, B.v32, 1 << 30, prec)
               : captureGraph<double>(ctx, B, ctx->d_val, B.v64, 1 << 30, prec))

@@ -104,6 +104,7 @@ struct FePrecond {
   float *d_gL32 = nullptr, *d_gU32 = nullptr, *d_dinv32 = nullptr;
   int *d_tposL = nullptr;  // for each lower entry of G, its position in the upper CSR of G^T
   int *d_tposA = nullptr;  // for each entry (i, j) of A, the position of (j, i)
+  int *d_rowOf = nullptr;  // row of each entry of A
   bool symmetricPattern = false;
   double *d_icL = nullptr, *d_icD = nullptr;  // parallel IC(0) iterates (two of each)
   int lastShifts = 0;

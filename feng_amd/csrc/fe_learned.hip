@@ -91,19 +91,17 @@ __global__ void mpScale(int n, const int *__restrict__ diag, const double *__res
   d0[i] = lu[diag[i]] / a;
 }
 
-// a^ on the pattern, g^0 on the strict lower entries.
-__global__ void mpNormalise(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+// a^ on the pattern, g^0 on the strict lower entries (one thread per entry).
+__global__ void mpNormalise(int64_t nnz, const int *__restrict__ rowOf, const int *__restrict__ ja,
                             const double *__restrict__ val, const double *__restrict__ lu,
                             const double *__restrict__ s, float *__restrict__ ahat, float *__restrict__ g0)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if(i >= n) return;
-  const double si = s[i];
-  for(int p = ia[i]; p < ia[i + 1]; ++p) {
-    const int j = ja[p];
-    ahat[p] = (float)(val[p] * si * s[j]);
-    if(j < i) g0[p] = (float)(-lu[p] * si / s[j]);
-  }
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(p >= nnz) return;
+  const int i = rowOf[p], j = ja[p];
+  const double si = s[i], sj = s[j];
+  ahat[p] = (float)(val[p] * si * sj);
+  if(j < i) g0[p] = (float)(-lu[p] * si / sj);
 }
 
 // B_ii = g^_i^T A^ g^_i and w_e = (A^ g^_i)_j on the strict lower entries
@@ -277,25 +275,20 @@ __global__ __launch_bounds__(256) void mpHeads(int n, const bf16_t *__restrict__
 }
 
 // g^ = g^0 - (sigma_i + rho_j) w0 + a^ (p_i . q_j + s_i + r_j) on strict lower
-// entries.
-__global__ void mpCorrect(int n, const int *__restrict__ ia, const int *__restrict__ ja, const int *__restrict__ diag,
+// entries (one thread per entry).
+__global__ void mpCorrect(int64_t nnz, const int *__restrict__ rowOf, const int *__restrict__ ja,
                           const float *__restrict__ ahat, const float *__restrict__ g0, const float *__restrict__ w0,
                           const float *__restrict__ p, const float *__restrict__ q, const float *__restrict__ hs,
                           float *__restrict__ g)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if(i >= n) return;
-  float pi[kNpq];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(e >= nnz) return;
+  const int i = rowOf[e], j = ja[e];
+  if(j >= i) return;
+  float c = hs[(size_t)i * kNs + 1] + hs[(size_t)j * kNs + 2];
 #pragma unroll
-  for(int k = 0; k < kNpq; ++k) pi[k] = p[(size_t)i * kNpq + k];
-  const float si = hs[(size_t)i * kNs + 1], sigi = hs[(size_t)i * kNs + 3];
-  for(int e = ia[i]; e < diag[i]; ++e) {
-    const int j = ja[e];
-    float c = si + hs[(size_t)j * kNs + 2];
-#pragma unroll
-    for(int k = 0; k < kNpq; ++k) c += pi[k] * q[(size_t)j * kNpq + k];
-    g[e] = g0[e] - (sigi + hs[(size_t)j * kNs + 4]) * w0[e] + ahat[e] * c;
-  }
+  for(int k = 0; k < kNpq; ++k) c += p[(size_t)i * kNpq + k] * q[(size_t)j * kNpq + k];
+  g[e] = g0[e] - (hs[(size_t)i * kNs + 3] + hs[(size_t)j * kNs + 4]) * w0[e] + ahat[e] * c;
 }
 
 // d^_i = B_ii exp(clamp(tau_i)).
@@ -309,30 +302,31 @@ __global__ void mpPivots(int n, const float *__restrict__ B, const float *__rest
 
 // Corrected split factor in the original coordinates, written over the IC(0)
 // one: g_ij = g^_ij s_j / s_i into the lower rows of G and the upper rows of
-// G^T, dinv_i = 1 / (d^_i a_ii).
-__global__ void mpWriteFactor(int n, const int *__restrict__ ia, const int *__restrict__ ja,
-                              const int *__restrict__ diag, const int *__restrict__ tpos, const double *__restrict__ val,
-                              const double *__restrict__ s, const float *__restrict__ g, const float *__restrict__ d,
-                              const int *__restrict__ iaL, const int *__restrict__ iaU, double *__restrict__ gL64,
-                              float *__restrict__ gL32, double *__restrict__ gU64, float *__restrict__ gU32,
-                              double *__restrict__ dinv64, float *__restrict__ dinv32)
+// G^T, dinv_i = 1 / (d^_i a_ii) (one thread per entry of A).
+__global__ void mpWriteFactor(int64_t nnz, const int *__restrict__ rowOf, const int *__restrict__ ia,
+                              const int *__restrict__ ja, const int *__restrict__ diag, const int *__restrict__ tpos,
+                              const double *__restrict__ val, const double *__restrict__ s, const float *__restrict__ g,
+                              const float *__restrict__ d, const int *__restrict__ iaL, const int *__restrict__ iaU,
+                              double *__restrict__ gL64, float *__restrict__ gL32, double *__restrict__ gU64,
+                              float *__restrict__ gU32, double *__restrict__ dinv64, float *__restrict__ dinv32)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if(i >= n) return;
-  for(int e = ia[i]; e < diag[i]; ++e) {
-    const int j = ja[e];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(e >= nnz) return;
+  const int i = rowOf[e], j = ja[e];
+  if(j < i) {
     const double gij = (double)g[e] * s[j] / s[i];
-    const int lpos = iaL[i] + (e - ia[i]);
+    const int lpos = iaL[i] + (int)(e - ia[i]);
     gL64[lpos] = gij;
     gL32[lpos] = (float)gij;
     const int t = tpos[e];  // entry (j, i), an upper entry of row j
     const int upos = iaU[j] + (t - diag[j]);
     gU64[upos] = gij;
     gU32[upos] = (float)gij;
+  } else if(j == i) {
+    const double di = (double)d[i] * val[e];
+    dinv64[i] = 1. / di;
+    dinv32[i] = (float)(1. / di);
   }
-  const double di = (double)d[i] * val[diag[i]];
-  dinv64[i] = 1. / di;
-  dinv32[i] = (float)(1. / di);
 }
 
 // ---------------------------------------------------------------------------
@@ -719,7 +713,11 @@ int mpForward(fe_hip_ctx *ctx, FeMpnn &M)
   const int gN = cdiv(n, 256);
   const int *ia = ctx->d_ia, *ja = ctx->d_ja, *dg = ctx->d_diag;
   hipLaunchKernelGGL(mpScale, dim3(gN), dim3(256), 0, st, n, dg, ctx->d_val, ctx->pc.d_lu, M.s, M.d0);
-  hipLaunchKernelGGL(mpNormalise, dim3(gN), dim3(256), 0, st, n, ia, ja, ctx->d_val, ctx->pc.d_lu, M.s, M.ahat, M.g0);
+  const int64_t nnz = ctx->nnz;
+  const int gE = cdiv(nnz, 256);
+  const int *rowOf = ctx->pc.d_rowOf;
+  hipLaunchKernelGGL(mpNormalise, dim3(gE), dim3(256), 0, st, nnz, rowOf, ja, ctx->d_val, ctx->pc.d_lu, M.s, M.ahat,
+                     M.g0);
   const int g8 = cdiv((int64_t)n * 8, 256);
   hipLaunchKernelGGL(mpRows, dim3(g8), dim3(256), 0, st, n, ia, ja, dg, M.ahat, M.g0, M.B0, M.w0);
   hipLaunchKernelGGL(mpFeatures, dim3(gN), dim3(256), 0, st, n, ia, ja, dg, M.ahat, M.g0, M.w0, M.d0, M.X);
@@ -733,7 +731,7 @@ int mpForward(fe_hip_ctx *ctx, FeMpnn &M)
                        M.theta + kBOff[l], M.h[l]);
   }
   hipLaunchKernelGGL(mpHeads, dim3(gN), dim3(256), 0, st, n, M.h[3], M.theta, M.p, M.q, M.hs);
-  hipLaunchKernelGGL(mpCorrect, dim3(gN), dim3(256), 0, st, n, ia, ja, dg, M.ahat, M.g0, M.w0, M.p, M.q, M.hs, M.g);
+  hipLaunchKernelGGL(mpCorrect, dim3(gE), dim3(256), 0, st, nnz, rowOf, ja, M.ahat, M.g0, M.w0, M.p, M.q, M.hs, M.g);
   hipLaunchKernelGGL(mpRows, dim3(g8), dim3(256), 0, st, n, ia, ja, dg, M.ahat, M.g, M.B, M.w);
   hipLaunchKernelGGL(mpPivots, dim3(gN), dim3(256), 0, st, n, M.B, M.hs, M.d);
   FE_HIP_TRY(ctx, hipGetLastError());
@@ -808,9 +806,9 @@ int feLearnedCorrect(fe_hip_ctx *ctx)
   FeMpnn &M = *ctx->mpnn;
   if(int rc = mpForward(ctx, M)) return rc;
   FePrecond &P = ctx->pc;
-  hipLaunchKernelGGL(mpWriteFactor, dim3(cdiv(ctx->n, 256)), dim3(256), 0, ctx->stream, ctx->n, ctx->d_ia, ctx->d_ja,
-                     ctx->d_diag, M.tpos, ctx->d_val, M.s, M.g, M.d, P.d_iaL, P.d_iaU, P.d_gL64, P.d_gL32, P.d_gU64,
-                     P.d_gU32, P.d_dinv64, P.d_dinv32);
+  hipLaunchKernelGGL(mpWriteFactor, dim3(cdiv(ctx->nnz, 256)), dim3(256), 0, ctx->stream, ctx->nnz, P.d_rowOf,
+                     ctx->d_ia, ctx->d_ja, ctx->d_diag, M.tpos, ctx->d_val, M.s, M.g, M.d, P.d_iaL, P.d_iaU, P.d_gL64,
+                     P.d_gL32, P.d_gU64, P.d_gU32, P.d_dinv64, P.d_dinv32);
   FE_HIP_TRY(ctx, hipGetLastError());
   return FE_STATUS_OK;
 }

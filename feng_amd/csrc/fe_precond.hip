@@ -94,35 +94,44 @@ __global__ void countParts(int n, const int *__restrict__ ia, const int *__restr
 }
 
 // Lower part with the unit diagonal and values g_ij = -l_ij (G = I - L_strict);
-// upper part G^T with values -u_ij / u_ii and unit diagonal; D = u_ii.
+// upper part G^T with values -u_ij / u_ii and unit diagonal; D = u_ii.  One
+// thread per entry of A (row from rowOf): the lower entry (i, j) goes to
+// position iaL[i] + (p - ia[i]) of G (the diagonal last), the upper entry to
+// iaU[i] + (p - diag[i]) of G^T (the diagonal first).
 template <typename T>
-__global__ void buildSplitFactor(int n, const int *__restrict__ ia, const int *__restrict__ ja,
-                                 const int *__restrict__ diag, const double *__restrict__ lu,
-                                 const int *__restrict__ iaL, int *__restrict__ jaL, T *__restrict__ gL,
-                                 const int *__restrict__ iaU, int *__restrict__ jaU, T *__restrict__ gU,
-                                 T *__restrict__ dinv, double *__restrict__ dval)
+__global__ void buildSplitFactor(int64_t nnz, const int *__restrict__ rowOf, const int *__restrict__ ia,
+                                 const int *__restrict__ ja, const int *__restrict__ diag,
+                                 const double *__restrict__ lu, const int *__restrict__ iaL, int *__restrict__ jaL,
+                                 T *__restrict__ gL, const int *__restrict__ iaU, int *__restrict__ jaU,
+                                 T *__restrict__ gU, T *__restrict__ dinv, double *__restrict__ dval)
+{
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(p >= nnz) return;
+  const int i = rowOf[p], j = ja[p], d = diag[i];
+  if(j <= i) {
+    const int o = iaL[i] + (int)(p - ia[i]);
+    jaL[o] = j;
+    gL[o] = (T)(j == i ? 1. : -lu[p]);
+  }
+  if(j >= i) {
+    const int o = iaU[i] + (int)(p - d);
+    jaU[o] = j;
+    gU[o] = (T)(j == i ? 1. : -lu[p] / lu[d]);
+  }
+  if(j == i) {
+    dinv[i] = (T)(1. / lu[p]);
+    dval[i] = lu[p];
+  }
+}
+
+// Row of every entry of A.
+__global__ void rowIndex(int n, const int *__restrict__ ia, int *__restrict__ rowOf)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if(i >= n) return;
-  const int d = diag[i];
-  const double di = lu[d];
-  int o = iaL[i];
-  for(int p = ia[i]; p < d; ++p) {
-    jaL[o] = ja[p];
-    gL[o++] = (T)(-lu[p]);
-  }
-  jaL[o] = i;
-  gL[o] = (T)1;
-  o = iaU[i];
-  jaU[o] = i;
-  gU[o++] = (T)1;
-  for(int p = d + 1; p < ia[i + 1]; ++p) {
-    jaU[o] = ja[p];
-    gU[o++] = (T)(-lu[p] / di);
-  }
-  dinv[i] = (T)(1. / di);
-  dval[i] = di;
+  for(int p = ia[i]; p < ia[i + 1]; ++p) rowOf[p] = i;
 }
+
 
 // Position of the mirrored entry (j, i) of every entry (i, j); -1 (and the
 // flag) when the pattern is not structurally symmetric there.
@@ -156,13 +165,15 @@ __global__ void transposeMap(int n, const int *__restrict__ ia, const int *__res
 //   d_i  = a_ii - sum_{k<i} l_ik^2 d_k,
 // from l_ij = a_ij / a_jj, d_i = a_ii.  Its fixed point is the exact IC(0)
 // factor; a few sweeps give a factor of the same quality as preconditioner.
-__global__ void icInit(int n, const int *__restrict__ ia, const int *__restrict__ ja, const int *__restrict__ diag,
-                       const double *__restrict__ val, double *__restrict__ l, double *__restrict__ d)
+__global__ void icInit(int64_t nnz, const int *__restrict__ rowOf, const int *__restrict__ ja,
+                       const int *__restrict__ diag, const double *__restrict__ val, double *__restrict__ l,
+                       double *__restrict__ d)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if(i >= n) return;
-  for(int e = ia[i]; e < diag[i]; ++e) l[e] = val[e] / val[diag[ja[e]]];
-  d[i] = val[diag[i]];
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(p >= nnz) return;
+  const int i = rowOf[p], j = ja[p];
+  if(j < i) l[p] = val[p] / val[diag[j]];
+  else if(j == i) d[i] = val[p];
 }
 
 // One sweep, 8 lanes per row: lane q handles the strict lower entries
@@ -205,16 +216,15 @@ __global__ __launch_bounds__(256) void icSweep(int n, const int *__restrict__ ia
 }
 
 // ILU(0) storage of the LDL^T factor: lu = l on the strict lower part, d on
-// the diagonal, u_ij = d_i l_ji on the strict upper part.
-__global__ void icToLu(int n, const int *__restrict__ ia, const int *__restrict__ diag, const int *__restrict__ tpos,
-                       const double *__restrict__ l, const double *__restrict__ d, double *__restrict__ lu)
+// the diagonal, u_ij = d_i l_ji on the strict upper part (one thread per entry).
+__global__ void icToLu(int64_t nnz, const int *__restrict__ rowOf, const int *__restrict__ ja,
+                       const int *__restrict__ tpos, const double *__restrict__ l, const double *__restrict__ d,
+                       double *__restrict__ lu)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if(i >= n) return;
-  const int pd = diag[i];
-  for(int e = ia[i]; e < pd; ++e) lu[e] = l[e];
-  lu[pd] = d[i];
-  for(int e = pd + 1; e < ia[i + 1]; ++e) lu[e] = d[i] * l[tpos[e]];
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(p >= nnz) return;
+  const int i = rowOf[p], j = ja[p];
+  lu[p] = j < i ? l[p] : (j == i ? d[i] : d[i] * l[tpos[p]]);
 }
 
 }  // namespace
@@ -258,7 +268,9 @@ int fePrecondSetupPattern(fe_hip_ctx *ctx)
      feAlloc(ctx, &P.d_iaU, n + 1) || feAlloc(ctx, &P.d_nShift, 4))
     return FE_STATUS_OUT_OF_MEMORY;
   FE_HIP_TRY(ctx, hipMemcpy(P.d_levelRows, lrows.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-  if(feAlloc(ctx, &P.d_tposA, (size_t)ctx->nnz)) return FE_STATUS_OUT_OF_MEMORY;
+  if(feAlloc(ctx, &P.d_tposA, (size_t)ctx->nnz) || feAlloc(ctx, &P.d_rowOf, (size_t)ctx->nnz))
+    return FE_STATUS_OUT_OF_MEMORY;
+  hipLaunchKernelGGL(rowIndex, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, ctx->d_ia, P.d_rowOf);
   FE_HIP_TRY(ctx, hipMemsetAsync(P.d_nShift, 0, sizeof(int), ctx->stream));
   hipLaunchKernelGGL(transposeMap, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, ctx->d_ia, ctx->d_ja,
                      P.d_tposA, P.d_nShift);
@@ -309,23 +321,23 @@ int fePrecondFactor(fe_hip_ctx *ctx, int sweeps)
         return FE_STATUS_OUT_OF_MEMORY;
     }
     double *L[2] = {P.d_icL, P.d_icL + ctx->nnz}, *D[2] = {P.d_icD, P.d_icD + n};
-    const int g = (n + 255) / 256;
-    hipLaunchKernelGGL(icInit, dim3(g), dim3(256), 0, ctx->stream, n, ctx->d_ia, ctx->d_ja, ctx->d_diag, ctx->d_val,
-                       L[0], D[0]);
+    const int gE = (int)((ctx->nnz + 255) / 256);
+    hipLaunchKernelGGL(icInit, dim3(gE), dim3(256), 0, ctx->stream, ctx->nnz, P.d_rowOf, ctx->d_ja, ctx->d_diag,
+                       ctx->d_val, L[0], D[0]);
     for(int s = 0; s < sweeps; ++s)
       hipLaunchKernelGGL(icSweep, dim3((int)(((int64_t)n * 8 + 255) / 256)), dim3(256), 0, ctx->stream, n, ctx->d_ia,
                          ctx->d_ja, ctx->d_diag, ctx->d_val, L[s & 1], D[s & 1], L[(s + 1) & 1], D[(s + 1) & 1]);
-    hipLaunchKernelGGL(icToLu, dim3(g), dim3(256), 0, ctx->stream, n, ctx->d_ia, ctx->d_diag, P.d_tposA,
+    hipLaunchKernelGGL(icToLu, dim3(gE), dim3(256), 0, ctx->stream, ctx->nnz, P.d_rowOf, ctx->d_ja, P.d_tposA,
                        L[sweeps & 1], D[sweeps & 1], P.d_lu);
   }
   FE_HIP_TRY(ctx, hipGetLastError());
-  const int g = (n + 255) / 256;
-  hipLaunchKernelGGL(buildSplitFactor<double>, dim3(g), dim3(256), 0, ctx->stream, n, ctx->d_ia, ctx->d_ja,
-                     ctx->d_diag, P.d_lu, P.d_iaL, P.d_jaL, P.d_gL64, P.d_iaU, P.d_jaU, P.d_gU64, P.d_dinv64,
-                     P.d_dval);
-  hipLaunchKernelGGL(buildSplitFactor<float>, dim3(g), dim3(256), 0, ctx->stream, n, ctx->d_ia, ctx->d_ja,
-                     ctx->d_diag, P.d_lu, P.d_iaL, P.d_jaL, P.d_gL32, P.d_iaU, P.d_jaU, P.d_gU32, P.d_dinv32,
-                     P.d_dval);
+  const int gE2 = (int)((ctx->nnz + 255) / 256);
+  hipLaunchKernelGGL(buildSplitFactor<double>, dim3(gE2), dim3(256), 0, ctx->stream, ctx->nnz, P.d_rowOf, ctx->d_ia,
+                     ctx->d_ja, ctx->d_diag, P.d_lu, P.d_iaL, P.d_jaL, P.d_gL64, P.d_iaU, P.d_jaU, P.d_gU64,
+                     P.d_dinv64, P.d_dval);
+  hipLaunchKernelGGL(buildSplitFactor<float>, dim3(gE2), dim3(256), 0, ctx->stream, ctx->nnz, P.d_rowOf, ctx->d_ia,
+                     ctx->d_ja, ctx->d_diag, P.d_lu, P.d_iaL, P.d_jaL, P.d_gL32, P.d_iaU, P.d_jaU, P.d_gU32,
+                     P.d_dinv32, P.d_dval);
   FE_HIP_TRY(ctx, hipGetLastError());
   int shifts = 0;
   FE_HIP_TRY(ctx, hipMemcpyAsync(&shifts, P.d_nShift, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
