@@ -458,6 +458,12 @@ int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value)
     FE_REQUIRE(ctx, value >= 0 && value <= 2,
                "pipe_kernel must be 0 (Chronopoulos-Gear), 1 (tiled Ghysels-Vanroose) or 2 (by precision)");
     ctx->pipeKernel = value;
+  } else if(k == "pc_values32") {
+    FE_REQUIRE(ctx, value == 0 || value == 1, "pc_values32 must be 0 or 1");
+    ctx->pcValues32 = value;
+  } else if(k == "pc_fp32") {
+    FE_REQUIRE(ctx, value == 0 || value == 1, "pc_fp32 must be 0 or 1");
+    ctx->pcFp32 = value;
   } else if(k == "ic0_sweeps") {
     FE_REQUIRE(ctx, value >= 0 && value <= 64, "ic0_sweeps must be 0 (exact ILU(0)) or 1..64");
     ctx->icSweeps = value;
@@ -497,6 +503,8 @@ int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value)
   else if(k == "fp32_values") *value = ctx->fp32Values;
   else if(k == "pipe_kernel") *value = ctx->pipeKernel;
   else if(k == "ic0_sweeps") *value = ctx->icSweeps;
+  else if(k == "pc_fp32") *value = ctx->pcFp32;
+  else if(k == "pc_values32") *value = ctx->pcValues32;
   else FE_REQUIRE(ctx, false, "unknown tuning key " + k);
   return FE_STATUS_OK;
 }

@@ -154,6 +154,9 @@ struct fe_hip_ctx {
   int fp32Values = 1;    // FP32 values: 0 = rounded FP64 values, 1 = FP32 MFMA element matrices
   int pipeKernel = 2;  // pipelined PCG: 0 Chronopoulos-Gear, 1 tiled Ghysels-Vanroose, 2 by precision
   int icSweeps = 0;    // FE_PC_IC0: 0 exact ILU(0), > 0 parallel IC(0) sweeps
+  int pcValues32 = 1;  // FP64 split PCG: factor values read as FP32 (vectors and arithmetic FP64)
+  int pcFp32 = 0;      // FP64 split PCG: apply G^T D^-1 G in FP64 (0) or FP32 (1; the pipelined
+                       // recurrences then drift from M^-1 r: 20% more iterations measured)
   FeMpnn *mpnn = nullptr;
   FeBatch *batch = nullptr;    // Jacobi-PCG: 0 = on-the-fly D^-1 r gather, 1 = tiled (one gathered vector)
 

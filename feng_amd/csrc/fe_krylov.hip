@@ -684,6 +684,9 @@ __global__ __launch_bounds__(TB) void tileIter(int n, const int *__restrict__ ia
 template <typename T>
 struct SpVecs {
   T *x, *r, *u, *w, *z, *q, *s, *p, *m, *t;
+  // FP32 shadows of w, t, m when the preconditioner is applied in FP32 inside
+  // an FP64 solve (null otherwise)
+  float *w32, *t32, *m32;
 };
 
 template <typename T>
@@ -693,10 +696,10 @@ struct SpRow {
 
 // y = scale .* (M x) over row tiles (scale may be null); skipped once the
 // iteration has stopped when checkDone is set.
-template <typename T, int LPR, int TB>
+template <typename T, int LPR, int TB, typename TV = T>
 __global__ __launch_bounds__(TB) void spSpmv(int n, const int *__restrict__ ia, const int *__restrict__ ja,
-                                             const T *__restrict__ val, const T *__restrict__ x,
-                                             const T *__restrict__ scale, T *__restrict__ y, const CgState *st,
+                                             const TV *__restrict__ val, const T *__restrict__ x,
+                                             const TV *__restrict__ scale, T *__restrict__ y, const CgState *st,
                                              int checkDone)
 {
   __shared__ T ldsY[TB];
@@ -706,10 +709,32 @@ __global__ __launch_bounds__(TB) void spSpmv(int n, const int *__restrict__ ia, 
     const int base = tile * TB;
     int kb[LPR], ke[LPR];
     tileBounds<LPR, TB>(n, base, ia, kb, ke);
-    tileProduct<T, LPR, TB, false>(n, base, ia, ja, val, x, nullptr, kb, ke, ldsY);
+    tileProductF<T, LPR, TB>(ja, val, kb, ke, ldsY, [&](int j) { return x[j]; });
     __syncthreads();
     const int row = base + threadIdx.x;
-    if(row < n) y[row] = scale ? scale[row] * ldsY[threadIdx.x] : ldsY[threadIdx.x];
+    if(row < n) y[row] = scale ? (T)scale[row] * ldsY[threadIdx.x] : ldsY[threadIdx.x];
+    __syncthreads();
+  }
+}
+
+// FP64 vectors, FP32 factor values.
+template <typename T, int LPR, int TB>
+__global__ __launch_bounds__(TB) void spSpmvF32(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                                const float *__restrict__ val, const T *__restrict__ x,
+                                                const float *__restrict__ scale, T *__restrict__ y,
+                                                const CgState *st, int checkDone)
+{
+  __shared__ T ldsY[TB];
+  if(checkDone && st->done) return;
+  const int nTiles = (n + TB - 1) / TB;
+  for(int tile = blockIdx.x; tile < nTiles; tile += gridDim.x) {
+    const int base = tile * TB;
+    int kb[LPR], ke[LPR];
+    tileBounds<LPR, TB>(n, base, ia, kb, ke);
+    tileProductF<T, LPR, TB>(ja, val, kb, ke, ldsY, [&](int j) { return x[j]; });
+    __syncthreads();
+    const int row = base + threadIdx.x;
+    if(row < n) y[row] = scale ? (T)scale[row] * ldsY[threadIdx.x] : ldsY[threadIdx.x];
     __syncthreads();
   }
 }
@@ -743,6 +768,7 @@ __global__ __launch_bounds__(TB) void spInitW(int n, const int *__restrict__ ia,
     if(row < n) {
       const T wi = ldsY[threadIdx.x], ri = v.r[row], ui = v.u[row];
       v.w[row] = wi;
+      if(v.w32) v.w32[row] = (float)wi;
       v.z[row] = v.q[row] = v.s[row] = v.p[row] = (T)0;
       g += (double)ri * (double)ui;
       d += (double)wi * (double)ui;
@@ -764,7 +790,7 @@ __device__ __forceinline__ void spLoadRow(const SpVecs<T> &v, int row, int n, Sp
 {
   if(row < n) {
     o.r = v.r[row]; o.u = v.u[row]; o.w = v.w[row]; o.z = v.z[row]; o.q = v.q[row];
-    o.s = v.s[row]; o.p = v.p[row]; o.x = v.x[row]; o.m = v.m[row];
+    o.s = v.s[row]; o.p = v.p[row]; o.x = v.x[row]; o.m = v.m32 ? (T)v.m32[row] : v.m[row];
   } else {
     o.r = o.u = o.w = o.z = o.q = o.s = o.p = o.x = o.m = (T)0;
   }
@@ -818,7 +844,8 @@ __global__ __launch_bounds__(TB) void spIter(int n, const int *__restrict__ ia, 
   double g = 0., d = 0., rh = 0.;
   while(tile < nTiles) {
     const int base = tile * TB;
-    tileProduct<T, LPR, TB, false>(n, base, ia, ja, val, v.m, nullptr, kb, ke, ldsY);
+    if(v.m32) tileProductF<T, LPR, TB>(ja, val, kb, ke, ldsY, [&](int j) { return (T)v.m32[j]; });
+    else tileProduct<T, LPR, TB, false>(n, base, ia, ja, val, v.m, nullptr, kb, ke, ldsY);
     __syncthreads();
     const int row = base + threadIdx.x;
     if(row < n) {
@@ -838,6 +865,7 @@ __global__ __launch_bounds__(TB) void spIter(int n, const int *__restrict__ ia, 
       v.r[row] = rn;
       v.u[row] = un;
       v.w[row] = wn;
+      if(v.w32) v.w32[row] = (float)wn;
       g += (double)rn * (double)un;
       d += (double)wn * (double)un;
       rh += (double)rn * (double)rn;
@@ -952,6 +980,7 @@ struct Workspace {
   TileVecs<float> t32;
   SpVecs<double> sp64;
   SpVecs<float> sp32;
+  float *spShadow[3];
   PipeParts pparts;
   PipeScalars *sc;
   SplitVecs<double> s64;
@@ -964,7 +993,7 @@ struct Workspace {
 int ensureWorkspace(fe_hip_ctx *ctx, Workspace &w)
 {
   const size_t n = ctx->n;
-  const size_t need = n * (8 * 8 + 8 * 4 + 3 * 8 + 9 * 8 + 9 * 4 + 8 * 8 + 8 * 4 + 9 * 8 + 9 * 4 + 10 * 8 + 10 * 4) +
+  const size_t need = n * (8 * 8 + 8 * 4 + 3 * 8 + 9 * 8 + 9 * 4 + 8 * 8 + 8 * 4 + 9 * 8 + 9 * 4 + 10 * 8 + 10 * 4 + 3 * 4) +
                       24 * kMaxGrid * 8 + 64 * 8 + 8192 + 64 * 1024;
   if(!ctx->d_work || ctx->workBytes < need) {
     if(ctx->d_work) feFree(ctx, ctx->d_work);
@@ -1017,6 +1046,11 @@ int ensureWorkspace(fe_hip_ctx *ctx, Workspace &w)
     double **a[] = {&w.sp64.x, &w.sp64.r, &w.sp64.u, &w.sp64.w, &w.sp64.z,
                      &w.sp64.q, &w.sp64.s, &w.sp64.p, &w.sp64.m, &w.sp64.t};
     for(auto pp : a) *pp = (double *)take(n * 8);
+    w.spShadow[0] = (float *)take(n * 4);
+    w.spShadow[1] = (float *)take(n * 4);
+    w.spShadow[2] = (float *)take(n * 4);
+    w.sp64.w32 = w.sp64.t32 = w.sp64.m32 = nullptr;
+    w.sp32.w32 = w.sp32.t32 = w.sp32.m32 = nullptr;
     float **b[] = {&w.sp32.x, &w.sp32.r, &w.sp32.u, &w.sp32.w, &w.sp32.z,
                    &w.sp32.q, &w.sp32.s, &w.sp32.p, &w.sp32.m, &w.sp32.t};
     for(auto pp : b) *pp = (float *)take(n * 4);
@@ -1305,36 +1339,59 @@ int runSplitPipe(fe_hip_ctx *ctx, Workspace &w, SpVecs<T> &v, const T *val, cons
   CgState *hs = (CgState *)ctx->h_state;
   FeKernelTimer &tm = ctx->spmvTimer;
   tm.used = 0;
-#define FE_SP_LAUNCH(KER, LL, ...)                                                                                 \
+#define FE_SP_LAUNCH_T(KER, TT, LL, ...)                                                                                 \
   do {                                                                                                             \
     if(TB == 1024) {                                                                                               \
-      if(LL == 1) hipLaunchKernelGGL((KER<T, 1, 1024>), dim3(G), dim3(1024), 0, ctx->stream, __VA_ARGS__);          \
-      else if(LL == 2) hipLaunchKernelGGL((KER<T, 2, 1024>), dim3(G), dim3(1024), 0, ctx->stream, __VA_ARGS__);     \
-      else if(LL == 4) hipLaunchKernelGGL((KER<T, 4, 1024>), dim3(G), dim3(1024), 0, ctx->stream, __VA_ARGS__);     \
-      else hipLaunchKernelGGL((KER<T, 8, 1024>), dim3(G), dim3(1024), 0, ctx->stream, __VA_ARGS__);                 \
+      if(LL == 1) hipLaunchKernelGGL((KER<TT, 1, 1024>), dim3(G), dim3(1024), 0, ctx->stream, __VA_ARGS__);          \
+      else if(LL == 2) hipLaunchKernelGGL((KER<TT, 2, 1024>), dim3(G), dim3(1024), 0, ctx->stream, __VA_ARGS__);     \
+      else if(LL == 4) hipLaunchKernelGGL((KER<TT, 4, 1024>), dim3(G), dim3(1024), 0, ctx->stream, __VA_ARGS__);     \
+      else hipLaunchKernelGGL((KER<TT, 8, 1024>), dim3(G), dim3(1024), 0, ctx->stream, __VA_ARGS__);                 \
     } else if(TB == 512) {                                                                                         \
-      if(LL == 1) hipLaunchKernelGGL((KER<T, 1, 512>), dim3(G), dim3(512), 0, ctx->stream, __VA_ARGS__);            \
-      else if(LL == 2) hipLaunchKernelGGL((KER<T, 2, 512>), dim3(G), dim3(512), 0, ctx->stream, __VA_ARGS__);       \
-      else if(LL == 4) hipLaunchKernelGGL((KER<T, 4, 512>), dim3(G), dim3(512), 0, ctx->stream, __VA_ARGS__);       \
-      else hipLaunchKernelGGL((KER<T, 8, 512>), dim3(G), dim3(512), 0, ctx->stream, __VA_ARGS__);                   \
+      if(LL == 1) hipLaunchKernelGGL((KER<TT, 1, 512>), dim3(G), dim3(512), 0, ctx->stream, __VA_ARGS__);            \
+      else if(LL == 2) hipLaunchKernelGGL((KER<TT, 2, 512>), dim3(G), dim3(512), 0, ctx->stream, __VA_ARGS__);       \
+      else if(LL == 4) hipLaunchKernelGGL((KER<TT, 4, 512>), dim3(G), dim3(512), 0, ctx->stream, __VA_ARGS__);       \
+      else hipLaunchKernelGGL((KER<TT, 8, 512>), dim3(G), dim3(512), 0, ctx->stream, __VA_ARGS__);                   \
     } else {                                                                                                       \
-      if(LL == 1) hipLaunchKernelGGL((KER<T, 1, 256>), dim3(G), dim3(256), 0, ctx->stream, __VA_ARGS__);            \
-      else if(LL == 2) hipLaunchKernelGGL((KER<T, 2, 256>), dim3(G), dim3(256), 0, ctx->stream, __VA_ARGS__);       \
-      else if(LL == 4) hipLaunchKernelGGL((KER<T, 4, 256>), dim3(G), dim3(256), 0, ctx->stream, __VA_ARGS__);       \
-      else hipLaunchKernelGGL((KER<T, 8, 256>), dim3(G), dim3(256), 0, ctx->stream, __VA_ARGS__);                   \
+      if(LL == 1) hipLaunchKernelGGL((KER<TT, 1, 256>), dim3(G), dim3(256), 0, ctx->stream, __VA_ARGS__);            \
+      else if(LL == 2) hipLaunchKernelGGL((KER<TT, 2, 256>), dim3(G), dim3(256), 0, ctx->stream, __VA_ARGS__);       \
+      else if(LL == 4) hipLaunchKernelGGL((KER<TT, 4, 256>), dim3(G), dim3(256), 0, ctx->stream, __VA_ARGS__);       \
+      else hipLaunchKernelGGL((KER<TT, 8, 256>), dim3(G), dim3(256), 0, ctx->stream, __VA_ARGS__);                   \
     }                                                                                                              \
   } while(0)
+#define FE_SP_LAUNCH(KER, LL, ...) FE_SP_LAUNCH_T(KER, T, LL, __VA_ARGS__)
   hipLaunchKernelGGL(spStart<T>, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, b, v);
   // u_0 = G^T D^-1 G r_0, w_0 = A u_0
-  FE_SP_LAUNCH(spSpmv, LF, n, P.d_iaL, P.d_jaL, gL, (const T *)v.r, dinv, v.t, (const CgState *)w.st, 0);
-  FE_SP_LAUNCH(spSpmv, LF, n, P.d_iaU, P.d_jaU, gU, (const T *)v.t, (const T *)nullptr, v.u, (const CgState *)w.st, 0);
+  if(sizeof(T) == 8 && ctx->pcValues32 && !v.m32) {
+    FE_SP_LAUNCH(spSpmvF32, LF, n, P.d_iaL, P.d_jaL, (const float *)P.d_gL32, (const T *)v.r,
+                 (const float *)P.d_dinv32, v.t, (const CgState *)w.st, 0);
+    FE_SP_LAUNCH(spSpmvF32, LF, n, P.d_iaU, P.d_jaU, (const float *)P.d_gU32, (const T *)v.t, (const float *)nullptr,
+                 v.u, (const CgState *)w.st, 0);
+  } else {
+    FE_SP_LAUNCH(spSpmv, LF, n, P.d_iaL, P.d_jaL, gL, (const T *)v.r, dinv, v.t, (const CgState *)w.st, 0);
+    FE_SP_LAUNCH(spSpmv, LF, n, P.d_iaU, P.d_jaU, gU, (const T *)v.t, (const T *)nullptr, v.u, (const CgState *)w.st,
+                 0);
+  }
   FE_SP_LAUNCH(spInitW, L, n, ctx->d_ia, ctx->d_ja, val, v, w.pparts, w.sc, w.st);
   int it = 0;
   for(;;) {
     for(int k = 0; k < checkEvery; ++k, ++it) {
-      FE_SP_LAUNCH(spSpmv, LF, n, P.d_iaL, P.d_jaL, gL, (const T *)v.w, dinv, v.t, (const CgState *)w.st, 1);
-      FE_SP_LAUNCH(spSpmv, LF, n, P.d_iaU, P.d_jaU, gU, (const T *)v.t, (const T *)nullptr, v.m,
-                   (const CgState *)w.st, 1);
+      if(v.m32) {
+        // preconditioner applied in FP32 (FP32 factor, FP32 shadows of w, t, m)
+        FE_SP_LAUNCH_T(spSpmv, float, LF, n, P.d_iaL, P.d_jaL, (const float *)P.d_gL32, (const float *)v.w32,
+                       (const float *)P.d_dinv32, v.t32, (const CgState *)w.st, 1);
+        FE_SP_LAUNCH_T(spSpmv, float, LF, n, P.d_iaU, P.d_jaU, (const float *)P.d_gU32, (const float *)v.t32,
+                       (const float *)nullptr, v.m32, (const CgState *)w.st, 1);
+      } else if(sizeof(T) == 8 && ctx->pcValues32) {
+        // FP64 vectors, FP32 factor values: an exactly applied SPD operator
+        FE_SP_LAUNCH(spSpmvF32, LF, n, P.d_iaL, P.d_jaL, (const float *)P.d_gL32, (const T *)v.w,
+                     (const float *)P.d_dinv32, v.t, (const CgState *)w.st, 1);
+        FE_SP_LAUNCH(spSpmvF32, LF, n, P.d_iaU, P.d_jaU, (const float *)P.d_gU32, (const T *)v.t,
+                     (const float *)nullptr, v.m, (const CgState *)w.st, 1);
+      } else {
+        FE_SP_LAUNCH(spSpmv, LF, n, P.d_iaL, P.d_jaL, gL, (const T *)v.w, dinv, v.t, (const CgState *)w.st, 1);
+        FE_SP_LAUNCH(spSpmv, LF, n, P.d_iaU, P.d_jaU, gU, (const T *)v.t, (const T *)nullptr, v.m,
+                     (const CgState *)w.st, 1);
+      }
       if(ctx->profiling) {
         if(tm.used + 2 > tm.ev.size())
           for(int a = 0; a < 64; ++a) {
@@ -1353,6 +1410,7 @@ int runSplitPipe(fe_hip_ctx *ctx, Workspace &w, SpVecs<T> &v, const T *val, cons
     if(hs->done) break;
   }
 #undef FE_SP_LAUNCH
+#undef FE_SP_LAUNCH_T
   if(ctx->profiling) {
     const size_t valid = std::min<size_t>(tm.used / 2, (size_t)std::max(0, hs->iter));
     for(size_t pi = 0; pi < valid; ++pi) {
@@ -1603,6 +1661,10 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
                            tiled32 ? w.sp32.x : w.s32.x);
       } else {
         if(tiled64) {
+          const bool mixed = ctx->pcFp32 != 0;
+          w.sp64.w32 = mixed ? w.spShadow[0] : nullptr;
+          w.sp64.t32 = mixed ? w.spShadow[1] : nullptr;
+          w.sp64.m32 = mixed ? w.spShadow[2] : nullptr;
           if(int rc = runSplitPipe<double>(ctx, w, w.sp64, ctx->d_val, Pc.d_gL64, Pc.d_gU64, Pc.d_dinv64, w.r64,
                                            checkEvery, o->max_iter - iters, tol2, &it0, &done))
             return rc;

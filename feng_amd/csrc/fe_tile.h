@@ -61,13 +61,11 @@ __device__ __forceinline__ void tileStore3(double a, double b, double c, double 
   }
 }
 
-// y[row] = sum_k val[k] src[ja[k]] for the LPR rows base + u RPB + g of a
+// y[row] = sum_k val[k] f(ja[k]) for the LPR rows base + u RPB + g of a
 // tile (u < LPR), gathers of the LPR rows interleaved; results into ldsY.
-template <typename T, int LPR, int TB, bool SCALE>
-__device__ __forceinline__ void tileProduct(int n, int base, const int *__restrict__ ia,
-                                            const int *__restrict__ ja, const T *__restrict__ val,
-                                            const T *__restrict__ src, const T *__restrict__ scale,
-                                            const int (&kb)[LPR], const int (&ke)[LPR], T *ldsY)
+template <typename T, int LPR, int TB, typename F, typename TV = T>
+__device__ __forceinline__ void tileProductF(const int *__restrict__ ja, const TV *__restrict__ val,
+                                             const int (&kb)[LPR], const int (&ke)[LPR], T *ldsY, F f)
 {
   constexpr int RPB = TB / LPR;
   const int sub = threadIdx.x % LPR, g = threadIdx.x / LPR;
@@ -91,14 +89,14 @@ __device__ __forceinline__ void tileProduct(int n, int base, const int *__restri
         ok[u][s] = kk < ke[u];
         any |= ok[u][s];
         j[u][s] = ok[u][s] ? ja[kk] : 0;
-        v[u][s] = ok[u][s] ? val[kk] : (T)0;
+        v[u][s] = ok[u][s] ? (T)val[kk] : (T)0;
       }
     if(!any) break;
 #pragma unroll
     for(int u = 0; u < LPR; ++u)
 #pragma unroll
       for(int s = 0; s < KS; ++s)
-        if(ok[u][s]) acc[u] += v[u][s] * (SCALE ? scale[j[u][s]] * src[j[u][s]] : src[j[u][s]]);
+        if(ok[u][s]) acc[u] += v[u][s] * f(j[u][s]);
     k0 += KS * LPR;
   }
 #pragma unroll
@@ -107,6 +105,16 @@ __device__ __forceinline__ void tileProduct(int n, int base, const int *__restri
     for(int q = LPR / 2; q > 0; q >>= 1) acc[u] += __shfl_xor(acc[u], q, LPR);
     if(sub == u) ldsY[u * RPB + g] = acc[u];
   }
+}
+
+// y[row] = sum_k val[k] src[ja[k]] (times scale[ja[k]] when SCALE).
+template <typename T, int LPR, int TB, bool SCALE>
+__device__ __forceinline__ void tileProduct(int n, int base, const int *__restrict__ ia,
+                                            const int *__restrict__ ja, const T *__restrict__ val,
+                                            const T *__restrict__ src, const T *__restrict__ scale,
+                                            const int (&kb)[LPR], const int (&ke)[LPR], T *ldsY)
+{
+  tileProductF<T, LPR, TB>(ja, val, kb, ke, ldsY, [&](int j) { return SCALE ? scale[j] * src[j] : src[j]; });
   (void)n;
   (void)base;
   (void)ia;
