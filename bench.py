@@ -72,6 +72,8 @@ PATHS = {
     "learned": ("fp32", "learned", 1e-2),
     "learned64": ("fp64", "learned", 1e-3),
     "ic0": ("fp64", "ic0", 1e-3),
+    "amg": ("fp64", "amg", 1e-3),
+    "amg32": ("fp32", "amg", 1e-2),
 }
 
 # Timed kernel of each path (the A product of one PCG iteration) and its
@@ -84,6 +86,8 @@ ITER_KERNEL = {
     "learned": ("cgSpmv<float> (split PCG, A product)", 4),
     "learned64": ("spIter<double> (split PCG, A product + updates)", 8),
     "ic0": ("spIter<double> (split PCG, A product + updates)", 8),
+    "amg": ("spIter<double> (AMG PCG, A product + updates)", 8),
+    "amg32": ("spIter<float> (AMG PCG, A product + updates)", 4),
 }
 
 
@@ -92,11 +96,11 @@ def iter_bytes(s, n, nnz):
 
 
 def run_step(S, path, rtol, profile=False):
-    from feng_amd.api import PC_IC0, PC_JACOBI, PC_LEARNED, PREC_FP32_IR, PREC_FP64
+    from feng_amd.api import PC_AMG, PC_IC0, PC_JACOBI, PC_LEARNED, PREC_FP32_IR, PREC_FP64
 
     precn, pcn, inner = PATHS[path]
     prec = PREC_FP32_IR if precn == "fp32" else PREC_FP64
-    pc = {"jacobi": PC_JACOBI, "learned": PC_LEARNED, "ic0": PC_IC0}[pcn]
+    pc = {"jacobi": PC_JACOBI, "learned": PC_LEARNED, "ic0": PC_IC0, "amg": PC_AMG}[pcn]
     S.set_profiling(profile)
     t0 = time.perf_counter()
     S.setToZero()
@@ -209,7 +213,7 @@ def main():
     rtol = 1e-8
     m, k, S, num = make_problem(args.N, args.nfeat, args.seed)
     n, nnz = num.nInc, S.nnz
-    paths = ["plain", "fast", "ic0"]
+    paths = ["plain", "fast", "ic0", "amg", "amg32"]
     try:
         from feng_amd.learned import load_weights
 
@@ -237,9 +241,12 @@ def main():
             if not r["converged"] or r["relres"] > rtol:
                 raise SystemExit(f"{path}: solve did not reach FP64 rel-res {rtol}: {r}")
         res[path] = dict(ms=1e3 * wall, dofs=n / wall, steps=steps, last=steps[-1], prof=prof)
-        log(path, f"{1e3 * wall:.3f} ms/step", steps[-1])
+        if PATHS[path][1] == "amg":
+            rows_l, nnz_l = S.amg_levels()
+            res[path]["extra"] = {"amg_rows": rows_l, "amg_nnz": nnz_l}
+        log(path, f"{1e3 * wall:.3f} ms/step", steps[-1], res[path].get("extra", ""))
 
-    fast_paths = [p for p in ("fast", "learned", "learned64") if p in res] or ["plain"]
+    fast_paths = [p for p in ("fast", "learned", "learned64", "amg", "amg32") if p in res] or ["plain"]
     best = min(fast_paths, key=lambda p: res[p]["ms"])
     B = res[best]
     plain = res["plain"]
@@ -271,7 +278,7 @@ def main():
         return {"ms": r["ms"], "dofs": r["dofs"], "iterations": r["last"]["iters"], "outer": r["last"]["outer"],
                 "relres": r["last"]["relres"], "asm_ms": r["prof"]["asm_ms"],
                 "solve_ms": float(np.mean([x["solve_ms"] for x in r["steps"]])),
-                "iter_kernel_ms": r["prof"]["spmv_ms"]}
+                "iter_kernel_ms": r["prof"]["spmv_ms"], **r.get("extra", {})}
 
     out = {
         "metric": "DOF/s assemble+solve to FP64 rel-res 1e-8; learned/FP32 vs plain FP64 PCG",
@@ -284,13 +291,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if best in ("plain", "learned64", "ic0") else "fp32-inner/fp64-refine",
+        "dtype": "f64" if best in ("plain", "learned64", "ic0", "amg") else "fp32-inner/fp64-refine",
         "data": "synthetic (seeded GRF coefficient on a generated Kuhn-tet cube)",
         "config": {"workload": f"P1 tet diffusion, unit cube {args.N}^3 x6 Kuhn tets, k=exp(GRF l=0.1 s=1.5), "
                                f"PCG to FP64 rel-res 1e-8", "n": n, "nnz": nnz, "elements": m.E,
                    "path": {"fast": "FP32 Jacobi-PCG + FP64 iterative refinement",
                             "learned": "FP32 learned-IC(0) split PCG + FP64 iterative refinement",
                             "learned64": "FP64 learned-IC(0) split PCG",
+                            "amg": "FP64 PCG with the aggregation-multigrid V-cycle",
+                            "amg32": "FP32 PCG with the aggregation-multigrid V-cycle + FP64 iterative refinement",
                             "plain": "FP64 Jacobi-PCG"}[best],
                    "paths": {p: summary(p) for p in res},
                    "speedup_vs_plain_fp64": plain["ms"] / B["ms"],

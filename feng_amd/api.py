@@ -27,7 +27,7 @@ MIXED_GRADIENT, MIXED_DIVERGENCE = 26, 31
 FN_CONST, FN_SINSIN, FN_BLOCKS, FN_SMOOTH3, FN_ELEM, FN_QP, FN_KOV_U, FN_KOV_P = range(8)
 
 KSP_CG, KSP_GMRES = 0, 1
-PC_NONE, PC_JACOBI, PC_IC0, PC_ILU0, PC_BJ_ILU0, PC_LEARNED = range(6)
+PC_NONE, PC_JACOBI, PC_IC0, PC_ILU0, PC_BJ_ILU0, PC_LEARNED, PC_AMG = range(7)
 PREC_FP64, PREC_FP32_IR = 0, 1
 
 
@@ -315,6 +315,25 @@ class LinearSystemHIP:
         d = np.ascontiguousarray(data, np.float64)
         self._c(self.L.fe_hip_update_coeff(self.ctx, form, index, N.ptr(d, C.c_double), d.size),
                 "fe_hip_update_coeff")
+
+    # aggregation multigrid (FE_PC_AMG) ---------------------------------------
+    def amg_levels(self):
+        """(rows, nnz) per level of the AMG hierarchy (empty before the first AMG solve)."""
+        cap = 32
+        lv = C.c_int()
+        rows = np.zeros(cap, np.int32)
+        nnz = np.zeros(cap, np.int64)
+        self._c(self.L.fe_hip_amg_levels(self.ctx, C.byref(lv), N.ptr(rows, C.c_int), N.ptr(nnz, C.c_int64), cap),
+                "fe_hip_amg_levels")
+        k = min(lv.value, cap)
+        return rows[:k].tolist(), nnz[:k].tolist()
+
+    def amg_apply(self, r):
+        """One V-cycle z = M^-1 r with the current values (FP64 vectors)."""
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros_like(r)
+        self._c(self.L.fe_hip_amg_apply(self.ctx, N.ptr(r, C.c_double), N.ptr(z, C.c_double)), "fe_hip_amg_apply")
+        return z
 
     # learned preconditioner (FE_PC_LEARNED) ---------------------------------
     def mpnn_opts(self, **kw):

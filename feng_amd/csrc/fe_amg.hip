@@ -1,0 +1,880 @@
+// Aggregation algebraic multigrid preconditioner for CG (FE_PC_AMG), the
+// counterpart of the algebraic multigrid PETSc selects with `-pc_type gamg`
+// through the runtime options the reference's PETSc back end reads
+// (feLinearSystemPETSc.cpp:900-1031; the solver is configured by options, not
+// by code, so only the role is alike).
+//
+// Hierarchy (built once per CSR pattern, on the host, like the pattern
+// itself): aggregates are a maximal independent set of the matrix graph taken
+// greedily in a hashed priority order, each root with its neighbours, every
+// other node joining the largest-numbered neighbouring root.  Prolongation is
+// the piecewise-constant tentative one (unsmoothed aggregation), so the
+// Galerkin operator A_c = P^T A P is a plain sum of fine entries: each coarse
+// entry keeps the list of fine entries that add into it, and the numeric
+// set-up per matrix is one gather-sum per level.
+//
+// V-cycle (symmetric, so CG can use it): l1-Jacobi smoothing S = c D_l1^-1,
+// D_l1 = diag(sum_j |a_ij|) (A <= D_l1, so S converges for c < 2), one step
+// before and after the coarse correction, which is over-corrected by a
+// factor oc < 2:
+//   u = S b;  b_c = P^T (b - A u);  e = V(b_c);  y = u + oc P e;
+//   x = y + S (b - A y).
+// The first step is fused into the restriction (one kernel per level reads
+// the aggregate's rows and writes b_c), the prolongation into the second
+// smoothing step (one kernel per level gathers u_j + oc e[agg_j]).  Levels of
+// at most `blockRows` rows and the dense coarsest solve run in one
+// single-workgroup kernel, so a V-cycle is 2 L + 1 launches for L large
+// levels.
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "fe_hip_internal.h"
+#include "fe_tile.h"
+
+using namespace fe_tile;
+
+namespace {
+
+constexpr int kAmgDense = 64;       // the coarsest level is inverted densely up to this size
+constexpr int kAmgMaxLevels = 20;
+constexpr int kAmgTileMax = 2048;   // rows of one aggregate tile (LDS)
+constexpr int kAmgTileRows = 256;   // target rows per aggregate tile
+constexpr int kAmgTB = 256;         // block of the level kernels
+constexpr int kAmgLPR = 4;          // lanes per row in the level kernels
+constexpr int kAmgBlockTB = 1024;   // the small-level kernel
+constexpr int kAmgBlockLds = 152 * 1024;  // LDS of the small-level kernel (bytes)
+constexpr int kAmgBlockLev = 8;           // small levels (coarsest excluded) it can hold
+
+// LDS layout of the small-level kernel: levels from .. coarsest - 1 (slot
+// s = l - from) with their CSR (16-bit columns), values, l1, aggregates and
+// the vectors b (then x), u, r (then y); the coarsest right-hand side and
+// solution (its dense inverse is read from global memory).
+struct AmgBlockPlan {
+  int from = 0, coarsest = 0;
+  int n[kAmgBlockLev + 1], nc[kAmgBlockLev], nnz[kAmgBlockLev];
+  int oIa[kAmgBlockLev], oJa[kAmgBlockLev], oVal[kAmgBlockLev], oL1[kAmgBlockLev], oAgg[kAmgBlockLev];
+  int oMptr[kAmgBlockLev], oMlist[kAmgBlockLev], oU[kAmgBlockLev], oR[kAmgBlockLev];
+  int oB[kAmgBlockLev + 1];
+  int oCx = 0, bytes = 0;
+};
+
+template <typename T, typename TV>
+struct AmgLev {
+  int n, nc;
+  const int *ia, *ja;
+  const TV *val, *l1;  // l1[i] = c / sum_j |a_ij|
+  const int *agg, *mptr, *mlist;
+  const int2 *mrange;  // row range [ia[i], ia[i + 1]) of member i = mlist[q]
+  T *b, *u, *x, *r;
+};
+
+}  // namespace
+
+struct FeAmgLevel {
+  int n = 0, nc = 0;
+  int64_t nnz = 0;
+  int *ia = nullptr, *ja = nullptr;  // level 0: the context's CSR
+  double *val64 = nullptr;           // level 0: the context's values
+  float *val32 = nullptr;
+  double *l1_64 = nullptr;
+  float *l1_32 = nullptr;
+  int *agg = nullptr, *mptr = nullptr, *mlist = nullptr;  // to level + 1
+  int2 *mrange = nullptr;
+  int *cptr = nullptr, *clist = nullptr;                  // Galerkin sums from level - 1
+  int nTiles = 0;
+  int *tileRow = nullptr, *tileAgg = nullptr;
+  double *b64 = nullptr, *u64 = nullptr, *x64 = nullptr, *r64 = nullptr;
+  float *b32 = nullptr, *u32 = nullptr, *x32 = nullptr, *r32 = nullptr;
+  int maxAgg = 0;
+};
+
+struct FeAmg {
+  std::vector<FeAmgLevel> lv;
+  AmgBlockPlan plan[3];  // per (vector, value) type combination of feAmgApply
+  double *cinv64 = nullptr;
+  float *cinv32 = nullptr;
+  void *desc[3] = {nullptr, nullptr, nullptr};  // AmgLev<double,double>, <double,float>, <float,float>
+  int *flag = nullptr;
+  double c = 0., oc = 0.;
+  bool valuesReady = false;
+  double lastSetupMs = 0.;
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Host: aggregation and the Galerkin pattern
+
+uint64_t amgMix(uint64_t x)
+{
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// Greedy maximal independent set in descending hashed priority (the fixed
+// point of Luby's rounds with these priorities), roots numbered in row order,
+// every other row joining the largest-numbered neighbouring root.
+int amgAggregate(int n, const std::vector<int> &ia, const std::vector<int> &ja, std::vector<int> &agg)
+{
+  std::vector<uint64_t> pri(n);
+  std::vector<int> order(n);
+  for(int i = 0; i < n; ++i) {
+    pri[i] = amgMix((uint64_t)i);
+    order[i] = i;
+  }
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return pri[a] != pri[b] ? pri[a] > pri[b] : a < b; });
+  std::vector<signed char> state(n, 0);
+  for(int i : order) {
+    if(state[i] != 0) continue;
+    state[i] = 1;
+    for(int k = ia[i]; k < ia[i + 1]; ++k)
+      if(state[ja[k]] == 0) state[ja[k]] = -1;
+  }
+  agg.assign(n, -1);
+  int nc = 0;
+  for(int i = 0; i < n; ++i)
+    if(state[i] == 1) agg[i] = nc++;
+  for(int i = 0; i < n; ++i) {
+    if(state[i] == 1) continue;
+    int best = -1;
+    for(int k = ia[i]; k < ia[i + 1]; ++k) {
+      const int j = ja[k];
+      if(state[j] == 1) best = std::max(best, agg[j]);
+    }
+    agg[i] = best;  // a non-root was excluded by a neighbouring root
+  }
+  return nc;
+}
+
+struct HostLevel {
+  int n = 0, nc = 0;
+  std::vector<int> ia, ja, agg, mptr, mlist, cptr, clist, tileRow, tileAgg;
+  int maxAgg = 0;
+};
+
+// Coarse pattern of P^T A P and, for each coarse entry, the fine entries that
+// sum into it (fine -> coarse position, then grouped by position).
+void amgGalerkinPattern(const HostLevel &f, std::vector<int> &iac, std::vector<int> &jac, std::vector<int> &cptr,
+                        std::vector<int> &clist)
+{
+  const int nc = f.nc;
+  iac.assign(nc + 1, 0);
+  jac.clear();
+  std::vector<int> mark(nc, -1), cols;
+  for(int I = 0; I < nc; ++I) {
+    cols.clear();
+    for(int q = f.mptr[I]; q < f.mptr[I + 1]; ++q) {
+      const int i = f.mlist[q];
+      for(int k = f.ia[i]; k < f.ia[i + 1]; ++k) {
+        const int J = f.agg[f.ja[k]];
+        if(mark[J] != I) {
+          mark[J] = I;
+          cols.push_back(J);
+        }
+      }
+    }
+    std::sort(cols.begin(), cols.end());
+    jac.insert(jac.end(), cols.begin(), cols.end());
+    iac[I + 1] = (int)jac.size();
+  }
+  const int64_t nnzF = f.ia[f.n];
+  std::vector<int> pos(nnzF);
+  for(int i = 0; i < f.n; ++i) {
+    const int I = f.agg[i];
+    const int *b = jac.data() + iac[I], *e = jac.data() + iac[I + 1];
+    for(int k = f.ia[i]; k < f.ia[i + 1]; ++k) pos[k] = (int)(std::lower_bound(b, e, f.agg[f.ja[k]]) - jac.data());
+  }
+  const int nnzC = iac[nc];
+  cptr.assign(nnzC + 1, 0);
+  for(int64_t k = 0; k < nnzF; ++k) cptr[pos[k] + 1]++;
+  for(int p = 0; p < nnzC; ++p) cptr[p + 1] += cptr[p];
+  clist.resize(nnzF);
+  std::vector<int> fill(cptr.begin(), cptr.end() - 1);
+  for(int64_t k = 0; k < nnzF; ++k) clist[fill[pos[k]]++] = (int)k;
+}
+
+// ---------------------------------------------------------------------------
+// Device: numeric set-up
+
+// l1[i] = c / sum_j |a_ij| of the fine level (8 lanes per row), and the FP32
+// rounding of the fine values.
+__global__ __launch_bounds__(256) void amgFineL1(int n, const int *__restrict__ ia, const double *__restrict__ val,
+                                                 double c, double *__restrict__ l1_64, float *__restrict__ l1_32,
+                                                 float *__restrict__ val32)
+{
+  const int sub = threadIdx.x & 7;
+  for(int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 3; i < n; i += (gridDim.x * blockDim.x) >> 3) {
+    double a = 0.;
+    for(int k = ia[i] + sub; k < ia[i + 1]; k += 8) {
+      const double v = val[k];
+      val32[k] = (float)v;
+      a += fabs(v);
+    }
+#pragma unroll
+    for(int o = 4; o > 0; o >>= 1) a += __shfl_xor(a, o, 8);
+    if(sub == 0) {
+      const double s = a > 0. ? c / a : 0.;
+      l1_64[i] = s;
+      l1_32[i] = (float)s;
+    }
+  }
+}
+
+// Coarse values as sums of fine values (fixed order, eight contributions in
+// flight per thread), one thread per coarse entry.
+__global__ __launch_bounds__(256) void amgGalerkin(int nnzC, const int *__restrict__ cptr, const int *__restrict__ clist,
+                                                   const double *__restrict__ valF, double *__restrict__ val64,
+                                                   float *__restrict__ val32)
+{
+  for(int p = blockIdx.x * blockDim.x + threadIdx.x; p < nnzC; p += gridDim.x * blockDim.x) {
+    double s = 0.;
+    const int q1 = cptr[p + 1];
+    for(int q = cptr[p]; q < q1; q += 8) {
+      int kk[8];
+#pragma unroll
+      for(int u = 0; u < 8; ++u) kk[u] = q + u < q1 ? clist[q + u] : -1;
+#pragma unroll
+      for(int u = 0; u < 8; ++u)
+        if(kk[u] >= 0) s += valF[kk[u]];
+    }
+    val64[p] = s;
+    val32[p] = (float)s;
+  }
+}
+
+// l1[i] = c / sum_j |a_ij| of a coarse level (8 lanes per row).
+__global__ __launch_bounds__(256) void amgRowL1(int n, const int *__restrict__ ia, const double *__restrict__ val,
+                                                double c, double *__restrict__ l1_64, float *__restrict__ l1_32)
+{
+  const int sub = threadIdx.x & 7;
+  for(int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 3; i < n; i += (gridDim.x * blockDim.x) >> 3) {
+    double a = 0.;
+    for(int k = ia[i] + sub; k < ia[i + 1]; k += 8) a += fabs(val[k]);
+#pragma unroll
+    for(int o = 4; o > 0; o >>= 1) a += __shfl_xor(a, o, 8);
+    if(sub == 0) {
+      const double s = a > 0. ? c / a : 0.;
+      l1_64[i] = s;
+      l1_32[i] = (float)s;
+    }
+  }
+}
+
+// Dense inverse of the coarsest operator (n <= kAmgDense = 64, SPD) by
+// in-place Gauss-Jordan elimination without pivoting in one workgroup of
+// 32 x 32 threads: thread (ty, tx) keeps entries (ty + 32 a, tx + 32 b),
+// a, b < 2, in registers; the pivot row and column of every step go through
+// LDS.  flag is set when a pivot is not positive.
+__global__ __launch_bounds__(1024) void amgDenseInverse(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                                        const double *__restrict__ val, double *__restrict__ inv64,
+                                                        float *__restrict__ inv32, int *__restrict__ flag)
+{
+  static_assert(kAmgDense == 64, "2 x 2 entries of 32 x 32 threads");
+  __shared__ double D[kAmgDense * kAmgDense];
+  __shared__ double prow[kAmgDense], pcol[kAmgDense];
+  const int t = threadIdx.x, tx = t & 31, ty = t >> 5;
+  for(int e = t; e < n * n; e += 1024) D[e] = 0.;
+  __syncthreads();
+  for(int i = t; i < n; i += 1024)
+    for(int k = ia[i]; k < ia[i + 1]; ++k) D[i * n + ja[k]] = val[k];
+  __syncthreads();
+  double m[2][2];
+#pragma unroll
+  for(int a = 0; a < 2; ++a)
+#pragma unroll
+    for(int b = 0; b < 2; ++b) {
+      const int i = ty + 32 * a, j = tx + 32 * b;
+      m[a][b] = (i < n && j < n) ? D[i * n + j] : 0.;
+    }
+  for(int k = 0; k < n; ++k) {
+    const int ka = k >> 5, kr = k & 31;
+#pragma unroll
+    for(int a = 0; a < 2; ++a)
+#pragma unroll
+      for(int b = 0; b < 2; ++b) {
+        const int i = ty + 32 * a, j = tx + 32 * b;
+        if(a == ka && ty == kr && j < n) prow[j] = m[a][b];
+        if(b == ka && tx == kr && i < n) pcol[i] = m[a][b];
+      }
+    __syncthreads();
+    const double piv = prow[k];
+    const double p = piv > 0. ? 1. / piv : 0.;
+    if(t == 0 && !(piv > 0.)) flag[0] = 1;
+    double pc[2], pr[2];
+#pragma unroll
+    for(int a = 0; a < 2; ++a) pc[a] = (ty + 32 * a < n) ? pcol[ty + 32 * a] : 0.;
+#pragma unroll
+    for(int b = 0; b < 2; ++b) pr[b] = (tx + 32 * b < n) ? prow[tx + 32 * b] * p : 0.;
+#pragma unroll
+    for(int a = 0; a < 2; ++a)
+#pragma unroll
+      for(int b = 0; b < 2; ++b) {
+        const bool rk = (a == ka && ty == kr), ck = (b == ka && tx == kr);
+        if(rk && ck) m[a][b] = p;
+        else if(rk) m[a][b] *= p;
+        else if(ck) m[a][b] = -m[a][b] * p;
+        else m[a][b] -= pc[a] * pr[b];
+      }
+    __syncthreads();
+  }
+#pragma unroll
+  for(int a = 0; a < 2; ++a)
+#pragma unroll
+    for(int b = 0; b < 2; ++b) {
+      const int i = ty + 32 * a, j = tx + 32 * b;
+      if(i < n && j < n) {
+        inv64[i * n + j] = m[a][b];
+        inv32[i * n + j] = (float)m[a][b];
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Device: the V-cycle
+
+// Restriction fused with the first smoothing step, over aggregate tiles:
+// u_i = l1_i b_i, r_i = b_i - sum_j a_ij l1_j b_j for the member rows of the
+// tile (through LDS), then bNext[I] = sum of r over the members of I.
+template <typename T, typename TV, int LPR, int TB>
+__global__ __launch_bounds__(TB) void amgDown(AmgLev<T, TV> L, const T *__restrict__ b, T *__restrict__ bNext,
+                                              const int *__restrict__ tileRow, const int *__restrict__ tileAgg,
+                                              const int *done)
+{
+  __shared__ T ldsY[TB];
+  __shared__ T rb[kAmgTileMax];
+  if(done && *done) return;
+  constexpr int RPB = TB / LPR;
+  const int sub = threadIdx.x % LPR, g = threadIdx.x / LPR;
+  const int tile = blockIdx.x;
+  const int r0 = tileRow[tile], r1 = tileRow[tile + 1];
+  for(int pass = r0; pass < r1; pass += TB) {
+    int kb[LPR], ke[LPR];
+#pragma unroll
+    for(int u = 0; u < LPR; ++u) {
+      const int q = pass + u * RPB + g;
+      if(q < r1) {
+        const int2 rg = L.mrange[q];
+        kb[u] = rg.x + sub;
+        ke[u] = rg.y;
+      } else {
+        kb[u] = ke[u] = 0;
+      }
+    }
+    tileProductF<T, LPR, TB>(L.ja, L.val, kb, ke, ldsY, [&](int j) { return (T)L.l1[j] * b[j]; });
+    __syncthreads();
+    const int q = pass + threadIdx.x;
+    if(q < r1) {
+      const int i = L.mlist[q];
+      const T bi = b[i];
+      L.u[i] = (T)L.l1[i] * bi;
+      rb[q - r0] = bi - ldsY[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  const int I1 = tileAgg[tile + 1];
+  for(int I = tileAgg[tile] + threadIdx.x; I < I1; I += TB) {
+    T s = (T)0;
+    for(int q = L.mptr[I]; q < L.mptr[I + 1]; ++q) s += rb[q - r0];
+    bNext[I] = s;
+  }
+}
+
+// Prolongation fused with the second smoothing step, over row tiles:
+// y_i = u_i + oc e[agg_i], x_i = y_i + l1_i (b_i - sum_j a_ij y_j).
+template <typename T, typename TV, int LPR, int TB>
+__global__ __launch_bounds__(TB) void amgUp(AmgLev<T, TV> L, const T *__restrict__ b, const T *__restrict__ e, T oc,
+                                            T *__restrict__ x, const int *done)
+{
+  __shared__ T ldsY[TB];
+  if(done && *done) return;
+  const int n = L.n;
+  const int nTiles = (n + TB - 1) / TB;
+  for(int tile = blockIdx.x; tile < nTiles; tile += gridDim.x) {
+    const int base = tile * TB;
+    int kb[LPR], ke[LPR];
+    tileBounds<LPR, TB>(n, base, L.ia, kb, ke);
+    tileProductF<T, LPR, TB>(L.ja, L.val, kb, ke, ldsY, [&](int j) { return L.u[j] + oc * e[L.agg[j]]; });
+    __syncthreads();
+    const int row = base + threadIdx.x;
+    if(row < n) {
+      const T yi = L.u[row] + oc * e[L.agg[row]];
+      x[row] = yi + (T)L.l1[row] * (b[row] - ldsY[threadIdx.x]);
+    }
+    __syncthreads();
+  }
+}
+
+// Levels from .. coarsest - 1 and the dense coarsest solve in one workgroup,
+// everything staged in LDS first (plan.bytes <= kAmgBlockLds); the phases are
+// separated by workgroup barriers.  Level `from` reads b (b0 when from == 0)
+// and writes its result to x (x0 when from == 0).
+template <typename T, typename TV>
+__global__ __launch_bounds__(kAmgBlockTB) void amgBlock(const AmgLev<T, TV> *__restrict__ lv, AmgBlockPlan pl,
+                                                        const T *__restrict__ cinv, T oc, const T *b0, T *x0,
+                                                        const int *done)
+{
+  __shared__ __align__(16) unsigned char lds[kAmgBlockLds];
+  if(done && *done) return;
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int nLev = pl.coarsest - pl.from;
+  for(int s = 0; s < nLev; ++s) {
+    const AmgLev<T, TV> L = lv[pl.from + s];
+    const int n = pl.n[s], nnz = pl.nnz[s], nc = pl.nc[s];
+    int *ia = (int *)(lds + pl.oIa[s]);
+    unsigned short *ja = (unsigned short *)(lds + pl.oJa[s]);
+    TV *val = (TV *)(lds + pl.oVal[s]), *l1 = (TV *)(lds + pl.oL1[s]);
+    unsigned short *agg = (unsigned short *)(lds + pl.oAgg[s]), *mptr = (unsigned short *)(lds + pl.oMptr[s]),
+                   *mlist = (unsigned short *)(lds + pl.oMlist[s]);
+    for(int i = t; i <= n; i += nt) ia[i] = L.ia[i];
+    for(int k = t; k < nnz; k += nt) {
+      ja[k] = (unsigned short)L.ja[k];
+      val[k] = L.val[k];
+    }
+    for(int i = t; i < n; i += nt) {
+      l1[i] = L.l1[i];
+      agg[i] = (unsigned short)L.agg[i];
+      mlist[i] = (unsigned short)L.mlist[i];
+    }
+    for(int I = t; I <= nc; I += nt) mptr[I] = (unsigned short)L.mptr[I];
+  }
+  {
+    const T *bsrc = pl.from == 0 ? b0 : lv[pl.from].b;
+    T *b = (T *)(lds + pl.oB[0]);
+    for(int i = t; i < pl.n[0]; i += nt) b[i] = bsrc[i];
+  }
+  __syncthreads();
+  for(int s = 0; s < nLev; ++s) {
+    const int n = pl.n[s], nc = pl.nc[s];
+    const int *ia = (const int *)(lds + pl.oIa[s]);
+    const unsigned short *ja = (const unsigned short *)(lds + pl.oJa[s]);
+    const TV *val = (const TV *)(lds + pl.oVal[s]), *l1 = (const TV *)(lds + pl.oL1[s]);
+    const unsigned short *mptr = (const unsigned short *)(lds + pl.oMptr[s]),
+                         *mlist = (const unsigned short *)(lds + pl.oMlist[s]);
+    const T *b = (const T *)(lds + pl.oB[s]);
+    T *u = (T *)(lds + pl.oU[s]), *r = (T *)(lds + pl.oR[s]);
+    for(int i = t; i < n; i += nt) {
+      T acc = (T)0;
+      for(int k = ia[i]; k < ia[i + 1]; ++k) acc += (T)val[k] * ((T)l1[ja[k]] * b[ja[k]]);
+      const T bi = b[i];
+      u[i] = (T)l1[i] * bi;
+      r[i] = bi - acc;
+    }
+    __syncthreads();
+    T *bn = (T *)(lds + pl.oB[s + 1]);
+    for(int I = t; I < nc; I += nt) {
+      T acc = (T)0;
+      for(int q = mptr[I]; q < mptr[I + 1]; ++q) acc += r[mlist[q]];
+      bn[I] = acc;
+    }
+    __syncthreads();
+  }
+  {
+    const int nC = pl.n[nLev];
+    const T *ci = cinv;  // global: row i of the inverse per thread
+    const T *b = (const T *)(lds + pl.oB[nLev]);
+    T *x = nLev == 0 ? (pl.from == 0 ? x0 : lv[pl.from].x) : (T *)(lds + pl.oCx);
+    for(int i = t; i < nC; i += nt) {
+      T acc = (T)0;
+      for(int j = 0; j < nC; ++j) acc += ci[i * nC + j] * b[j];
+      x[i] = acc;
+    }
+    __syncthreads();
+  }
+  for(int s = nLev - 1; s >= 0; --s) {
+    const int n = pl.n[s];
+    const int *ia = (const int *)(lds + pl.oIa[s]);
+    const unsigned short *ja = (const unsigned short *)(lds + pl.oJa[s]);
+    const TV *val = (const TV *)(lds + pl.oVal[s]), *l1 = (const TV *)(lds + pl.oL1[s]);
+    const unsigned short *agg = (const unsigned short *)(lds + pl.oAgg[s]);
+    T *b = (T *)(lds + pl.oB[s]);
+    const T *u = (const T *)(lds + pl.oU[s]);
+    T *y = (T *)(lds + pl.oR[s]);
+    // e = x of level s + 1: the coarsest solution, or level s + 1's b slot
+    const T *e = (s + 1 == nLev) ? (const T *)(lds + pl.oCx) : (const T *)(lds + pl.oB[s + 1]);
+    for(int i = t; i < n; i += nt) y[i] = u[i] + oc * e[agg[i]];
+    __syncthreads();
+    T *xg = s == 0 ? (pl.from == 0 ? x0 : lv[pl.from].x) : nullptr;
+    for(int i = t; i < n; i += nt) {
+      T acc = (T)0;
+      for(int k = ia[i]; k < ia[i + 1]; ++k) acc += (T)val[k] * y[ja[k]];
+      const T xi = y[i] + (T)l1[i] * (b[i] - acc);
+      if(xg) xg[i] = xi;
+      else b[i] = xi;
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+T *pick(double *a, float *b);
+template <>
+double *pick<double>(double *a, float *) { return a; }
+template <>
+float *pick<float>(double *, float *b) { return b; }
+
+template <typename TV>
+const TV *pickV(const double *a, const float *b);
+template <>
+const double *pickV<double>(const double *a, const float *) { return a; }
+template <>
+const float *pickV<float>(const double *, const float *b) { return b; }
+
+template <typename T, typename TV>
+AmgLev<T, TV> amgView(const FeAmgLevel &h)
+{
+  AmgLev<T, TV> L;
+  L.n = h.n;
+  L.nc = h.nc;
+  L.ia = h.ia;
+  L.ja = h.ja;
+  L.val = pickV<TV>(h.val64, h.val32);
+  L.l1 = pickV<TV>(h.l1_64, h.l1_32);
+  L.agg = h.agg;
+  L.mptr = h.mptr;
+  L.mlist = h.mlist;
+  L.mrange = h.mrange;
+  L.b = pick<T>(h.b64, h.b32);
+  L.u = pick<T>(h.u64, h.u32);
+  L.x = pick<T>(h.x64, h.x32);
+  L.r = pick<T>(h.r64, h.r32);
+  return L;
+}
+
+template <typename T, typename TV>
+int amgUploadDesc(fe_hip_ctx *ctx, FeAmg &A, int slot)
+{
+  std::vector<AmgLev<T, TV>> h;
+  for(const FeAmgLevel &l : A.lv) h.push_back(amgView<T, TV>(l));
+  AmgLev<T, TV> *d = nullptr;
+  if(int rc = feAlloc(ctx, &d, h.size())) return rc;
+  FE_HIP_TRY(ctx, hipMemcpy(d, h.data(), sizeof(AmgLev<T, TV>) * h.size(), hipMemcpyHostToDevice));
+  A.desc[slot] = d;
+  return FE_STATUS_OK;
+}
+
+template <typename T>
+int upload(fe_hip_ctx *ctx, T **d, const std::vector<T> &h)
+{
+  if(int rc = feAlloc(ctx, d, h.size())) return rc;
+  if(!h.empty()) FE_HIP_TRY(ctx, hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  return FE_STATUS_OK;
+}
+
+}  // namespace
+
+// Hierarchy of the current CSR pattern (once per context).
+int feAmgSetup(fe_hip_ctx *ctx)
+{
+  if(ctx->amg) return FE_STATUS_OK;
+  FE_REQUIRE(ctx, ctx->csrBuilt, "build the CSR before an AMG solve");
+  const int n = ctx->n;
+  HostLevel f;
+  f.n = n;
+  f.ia.resize(n + 1);
+  FE_HIP_TRY(ctx, hipMemcpy(f.ia.data(), ctx->d_ia, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+  f.ja.resize(f.ia[n]);
+  FE_HIP_TRY(ctx, hipMemcpy(f.ja.data(), ctx->d_ja, sizeof(int) * f.ja.size(), hipMemcpyDeviceToHost));
+  std::vector<HostLevel> H;
+  H.push_back(std::move(f));
+  while((int)H.size() < kAmgMaxLevels && H.back().n > kAmgDense) {
+    HostLevel &c = H.back();
+    c.nc = amgAggregate(c.n, c.ia, c.ja, c.agg);
+    FE_REQUIRE(ctx, c.nc < c.n, "AMG: the matrix graph has no edges to aggregate");
+    c.mptr.assign(c.nc + 1, 0);
+    for(int i = 0; i < c.n; ++i) c.mptr[c.agg[i] + 1]++;
+    for(int I = 0; I < c.nc; ++I) c.mptr[I + 1] += c.mptr[I];
+    c.mlist.resize(c.n);
+    {
+      std::vector<int> fill(c.mptr.begin(), c.mptr.end() - 1);
+      for(int i = 0; i < c.n; ++i) c.mlist[fill[c.agg[i]]++] = i;
+    }
+    // aggregate tiles of about kAmgTileRows member rows
+    c.tileRow.assign(1, 0);
+    c.tileAgg.assign(1, 0);
+    int rows = 0;
+    for(int I = 0; I < c.nc; ++I) {
+      const int sz = c.mptr[I + 1] - c.mptr[I];
+      c.maxAgg = std::max(c.maxAgg, sz);
+      FE_REQUIRE(ctx, sz <= kAmgTileMax, "AMG: an aggregate is larger than the tile limit");
+      if(rows > 0 && rows + sz > kAmgTileRows) {
+        c.tileRow.push_back(c.mptr[I]);
+        c.tileAgg.push_back(I);
+        rows = 0;
+      }
+      rows += sz;
+    }
+    c.tileRow.push_back(c.n);
+    c.tileAgg.push_back(c.nc);
+    HostLevel nx;
+    nx.n = c.nc;
+    amgGalerkinPattern(c, nx.ia, nx.ja, nx.cptr, nx.clist);
+    H.push_back(std::move(nx));
+  }
+  FE_REQUIRE(ctx, H.back().n <= kAmgDense, "AMG: too many levels");
+
+  FeAmg *A = new FeAmg();
+  ctx->amg = A;
+  A->lv.resize(H.size());
+  for(size_t l = 0; l < H.size(); ++l) {
+    HostLevel &h = H[l];
+    FeAmgLevel &d = A->lv[l];
+    d.n = h.n;
+    d.nc = h.nc;
+    d.nnz = h.ia[h.n];
+    d.maxAgg = h.maxAgg;
+    if(l == 0) {
+      d.ia = ctx->d_ia;
+      d.ja = ctx->d_ja;
+      d.val64 = ctx->d_val;
+    } else {
+      if(int rc = upload(ctx, &d.ia, h.ia)) return rc;
+      if(int rc = upload(ctx, &d.ja, h.ja)) return rc;
+      if(int rc = upload(ctx, &d.cptr, h.cptr)) return rc;
+      if(int rc = upload(ctx, &d.clist, h.clist)) return rc;
+      if(int rc = feAlloc(ctx, &d.val64, d.nnz)) return rc;
+    }
+    if(int rc = feAlloc(ctx, &d.val32, d.nnz)) return rc;
+    if(int rc = feAlloc(ctx, &d.l1_64, d.n)) return rc;
+    if(int rc = feAlloc(ctx, &d.l1_32, d.n)) return rc;
+    if(l + 1 < H.size()) {
+      if(int rc = upload(ctx, &d.agg, h.agg)) return rc;
+      if(int rc = upload(ctx, &d.mptr, h.mptr)) return rc;
+      if(int rc = upload(ctx, &d.mlist, h.mlist)) return rc;
+      {
+        std::vector<int2> rg(h.n);
+        for(int q = 0; q < h.n; ++q) rg[q] = make_int2(h.ia[h.mlist[q]], h.ia[h.mlist[q] + 1]);
+        if(int rc = upload(ctx, &d.mrange, rg)) return rc;
+      }
+      if(int rc = upload(ctx, &d.tileRow, h.tileRow)) return rc;
+      if(int rc = upload(ctx, &d.tileAgg, h.tileAgg)) return rc;
+      d.nTiles = (int)h.tileRow.size() - 1;
+    }
+    // level 0 reads the Krylov vectors for b and x
+    if(l > 0) {
+      if(int rc = feAlloc(ctx, &d.b64, d.n)) return rc;
+      if(int rc = feAlloc(ctx, &d.x64, d.n)) return rc;
+      if(int rc = feAlloc(ctx, &d.b32, d.n)) return rc;
+      if(int rc = feAlloc(ctx, &d.x32, d.n)) return rc;
+    }
+    if(int rc = feAlloc(ctx, &d.u64, d.n)) return rc;
+    if(int rc = feAlloc(ctx, &d.r64, d.n)) return rc;
+    if(int rc = feAlloc(ctx, &d.u32, d.n)) return rc;
+    if(int rc = feAlloc(ctx, &d.r32, d.n)) return rc;
+  }
+  const int nC = A->lv.back().n;
+  if(int rc = feAlloc(ctx, &A->cinv64, (size_t)nC * nC)) return rc;
+  if(int rc = feAlloc(ctx, &A->cinv32, (size_t)nC * nC)) return rc;
+  if(int rc = feAlloc(ctx, &A->flag, 1)) return rc;
+  if(int rc = amgUploadDesc<double, double>(ctx, *A, 0)) return rc;
+  if(int rc = amgUploadDesc<double, float>(ctx, *A, 1)) return rc;
+  if(int rc = amgUploadDesc<float, float>(ctx, *A, 2)) return rc;
+  return FE_STATUS_OK;
+}
+
+void feAmgRelease(fe_hip_ctx *ctx)
+{
+  delete ctx->amg;  // device buffers are owned by the context
+  ctx->amg = nullptr;
+}
+
+namespace {
+
+// LDS plan of the small-level kernel for vectors of T and values of TV: the
+// most trailing levels (each of at most blockRows rows) that fit.  The
+// coarsest level alone always fits (kAmgDense^2 values of T).
+template <typename T, typename TV>
+AmgBlockPlan amgPlan(const FeAmg &A, int blockRows)
+{
+  const int Lc = (int)A.lv.size() - 1;
+  AmgBlockPlan pl;
+  for(int from = 0; from <= Lc; ++from) {
+    if(Lc - from > kAmgBlockLev) continue;
+    bool ok = true;
+    for(int l = from; l < Lc; ++l) ok = ok && A.lv[l].n <= blockRows && A.lv[l].n <= 65535;
+    if(!ok) continue;
+    int bytes = 0;
+    auto take = [&](size_t b) {
+      const int o = bytes;
+      bytes += (int)((b + 15) & ~(size_t)15);
+      return o;
+    };
+    pl = AmgBlockPlan();
+    pl.from = from;
+    pl.coarsest = Lc;
+    for(int l = from; l < Lc; ++l) {
+      const int s = l - from;
+      const FeAmgLevel &h = A.lv[l];
+      pl.n[s] = h.n;
+      pl.nc[s] = h.nc;
+      pl.nnz[s] = (int)h.nnz;
+      pl.oIa[s] = take(sizeof(int) * (h.n + 1));
+      pl.oJa[s] = take(2 * h.nnz);
+      pl.oVal[s] = take(sizeof(TV) * h.nnz);
+      pl.oL1[s] = take(sizeof(TV) * h.n);
+      pl.oAgg[s] = take(2 * h.n);
+      pl.oMptr[s] = take(2 * (h.nc + 1));
+      pl.oMlist[s] = take(2 * h.n);
+      pl.oB[s] = take(sizeof(T) * h.n);
+      pl.oU[s] = take(sizeof(T) * h.n);
+      pl.oR[s] = take(sizeof(T) * h.n);
+    }
+    const int nC = A.lv[Lc].n;
+    pl.n[Lc - from] = nC;
+    pl.oB[Lc - from] = take(sizeof(T) * nC);
+    pl.oCx = take(sizeof(T) * nC);
+    pl.bytes = bytes;
+    if(bytes <= kAmgBlockLds) return pl;
+  }
+  return pl;  // not reached: from = Lc fits
+}
+
+}  // namespace
+
+// Numeric set-up for the current values: fine l1 and FP32 values, the
+// Galerkin sums of every coarser level, the dense coarsest inverse.  A
+// non-positive coarsest pivot is reported by feAmgCheck after the solve.
+int feAmgNumeric(fe_hip_ctx *ctx)
+{
+  FeAmg &A = *ctx->amg;
+  A.c = ctx->amgOmega;
+  A.oc = ctx->amgOverCorrect;
+  A.plan[0] = amgPlan<double, double>(A, ctx->amgBlockRows);
+  A.plan[1] = amgPlan<double, float>(A, ctx->amgBlockRows);
+  A.plan[2] = amgPlan<float, float>(A, ctx->amgBlockRows);
+  FeAmgLevel &f = A.lv[0];
+  f.val64 = ctx->d_val;
+  {
+    const int g = std::min(4096, (f.n * 8 + 255) / 256);
+    hipLaunchKernelGGL(amgFineL1, dim3(g), dim3(256), 0, ctx->stream, f.n, f.ia, f.val64, A.c, f.l1_64, f.l1_32,
+                       f.val32);
+  }
+  for(size_t l = 1; l < A.lv.size(); ++l) {
+    FeAmgLevel &c = A.lv[l];
+    const int ge = std::max(1, std::min(4096, (int)((c.nnz + 255) / 256)));
+    hipLaunchKernelGGL(amgGalerkin, dim3(ge), dim3(256), 0, ctx->stream, (int)c.nnz, c.cptr, c.clist,
+                       A.lv[l - 1].val64, c.val64, c.val32);
+    const int gr = std::max(1, std::min(4096, (c.n * 8 + 255) / 256));
+    hipLaunchKernelGGL(amgRowL1, dim3(gr), dim3(256), 0, ctx->stream, c.n, c.ia, c.val64, A.c, c.l1_64, c.l1_32);
+  }
+  FE_HIP_TRY(ctx, hipMemsetAsync(A.flag, 0, sizeof(int), ctx->stream));
+  const FeAmgLevel &C = A.lv.back();
+  hipLaunchKernelGGL(amgDenseInverse, dim3(1), dim3(1024), 0, ctx->stream, C.n, C.ia, C.ja, C.val64, A.cinv64,
+                     A.cinv32, A.flag);
+  FE_HIP_TRY(ctx, hipGetLastError());
+  A.valuesReady = true;
+  return FE_STATUS_OK;
+}
+
+// Error when the last numeric set-up met a non-positive coarsest pivot
+// (synchronises the stream).
+int feAmgCheck(fe_hip_ctx *ctx)
+{
+  if(!ctx->amg) return FE_STATUS_OK;
+  int flag = 0;
+  FE_HIP_TRY(ctx, hipMemcpyAsync(&flag, ctx->amg->flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  FE_REQUIRE(ctx, flag == 0, "AMG: the coarsest operator is not positive definite");
+  return FE_STATUS_OK;
+}
+
+namespace {
+
+template <typename T, typename TV>
+int amgApplyT(fe_hip_ctx *ctx, const T *in, T *out, const int *done, int slot)
+{
+  FeAmg &A = *ctx->amg;
+  const AmgBlockPlan &pl = A.plan[slot];
+  const int B = pl.from;
+  const T oc = (T)A.oc;
+  for(int l = 0; l < B; ++l) {
+    const FeAmgLevel &h = A.lv[l];
+    AmgLev<T, TV> L = amgView<T, TV>(h);
+    const T *b = l == 0 ? in : L.b;
+    hipLaunchKernelGGL((amgDown<T, TV, kAmgLPR, kAmgTB>), dim3(h.nTiles), dim3(kAmgTB), 0, ctx->stream, L, b,
+                       pick<T>(A.lv[l + 1].b64, A.lv[l + 1].b32), h.tileRow, h.tileAgg, done);
+  }
+  hipLaunchKernelGGL((amgBlock<T, TV>), dim3(1), dim3(kAmgBlockTB), 0, ctx->stream,
+                     (const AmgLev<T, TV> *)A.desc[slot], pl, (const T *)pick<T>(A.cinv64, A.cinv32), oc, in, out,
+                     done);
+  for(int l = B - 1; l >= 0; --l) {
+    const FeAmgLevel &h = A.lv[l];
+    AmgLev<T, TV> L = amgView<T, TV>(h);
+    const T *b = l == 0 ? in : L.b;
+    T *x = l == 0 ? out : L.x;
+    const int G = std::max(1, std::min(ctx->iterGrid * 2, (h.n + kAmgTB - 1) / kAmgTB));
+    hipLaunchKernelGGL((amgUp<T, TV, kAmgLPR, kAmgTB>), dim3(G), dim3(kAmgTB), 0, ctx->stream, L, b,
+                       (const T *)pick<T>(A.lv[l + 1].x64, A.lv[l + 1].x32), oc, x, done);
+  }
+  return FE_STATUS_OK;
+}
+
+}  // namespace
+
+// out = M^-1 in (one V-cycle).  FP64 vectors read FP64 or FP32 level values
+// (values32); FP32 vectors read FP32 values.  Skipped once *done is set.
+int feAmgApply(fe_hip_ctx *ctx, const double *in, double *out, const int *done, bool values32)
+{
+  if(values32) return amgApplyT<double, float>(ctx, in, out, done, 1);
+  return amgApplyT<double, double>(ctx, in, out, done, 0);
+}
+
+int feAmgApply(fe_hip_ctx *ctx, const float *in, float *out, const int *done, bool)
+{
+  return amgApplyT<float, float>(ctx, in, out, done, 2);
+}
+
+int feAmgLevels(const fe_hip_ctx *ctx, int *levels, int *rows, int64_t *nnz, int cap)
+{
+  if(!ctx->amg) {
+    *levels = 0;
+    return FE_STATUS_OK;
+  }
+  const FeAmg &A = *ctx->amg;
+  *levels = (int)A.lv.size();
+  for(int l = 0; l < (int)A.lv.size() && l < cap; ++l) {
+    rows[l] = A.lv[l].n;
+    nnz[l] = A.lv[l].nnz;
+  }
+  return FE_STATUS_OK;
+}
+
+extern "C" {
+
+int fe_hip_amg_levels(fe_hip_ctx *ctx, int *levels, int *rows, int64_t *nnz, int cap)
+{
+  if(!ctx || !levels || (cap > 0 && (!rows || !nnz))) return FE_STATUS_POINTER_ERROR;
+  return feAmgLevels(ctx, levels, rows, nnz, cap);
+}
+
+int fe_hip_amg_apply(fe_hip_ctx *ctx, const double *r, double *z)
+{
+  if(!ctx || !r || !z) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, ctx->csrBuilt, "build the CSR first");
+  FE_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if(int rc = feAmgSetup(ctx)) return rc;
+  if(int rc = feAmgNumeric(ctx)) return rc;
+  const int n = ctx->n;
+  double *d = nullptr;
+  FE_HIP_TRY(ctx, hipMalloc(&d, sizeof(double) * 2 * (size_t)n));
+  int rc = FE_STATUS_OK;
+  if(hipMemcpyAsync(d, r, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+    ctx->setError("fe_hip_amg_apply: copy of r failed", __FILE__, __LINE__);
+    rc = FE_STATUS_ERROR;
+  }
+  if(rc == FE_STATUS_OK) rc = feAmgApply(ctx, (const double *)d, d + n, nullptr, ctx->pcValues32 != 0);
+  if(rc == FE_STATUS_OK && (hipGetLastError() != hipSuccess ||
+                            hipMemcpyAsync(z, d + n, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+                            hipStreamSynchronize(ctx->stream) != hipSuccess)) {
+    ctx->setError("fe_hip_amg_apply: V-cycle failed", __FILE__, __LINE__);
+    rc = FE_STATUS_ERROR;
+  }
+  (void)hipFree(d);
+  if(rc == FE_STATUS_OK) rc = feAmgCheck(ctx);
+  return rc;
+}
+
+}  // extern "C"

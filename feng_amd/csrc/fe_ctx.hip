@@ -187,6 +187,7 @@ int fe_hip_ctx_destroy(fe_hip_ctx *ctx)
   for(hipEvent_t e : ctx->spmvTimer.ev) (void)hipEventDestroy(e);
   feMpnnRelease(ctx);
   feBatchRelease(ctx);
+  feAmgRelease(ctx);
   for(void *p : ctx->owned) (void)hipFree(p);
   if(ctx->h_state) (void)hipHostFree(ctx->h_state);
   (void)hipStreamDestroy(ctx->stream);
@@ -467,6 +468,15 @@ int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value)
   } else if(k == "ic0_sweeps") {
     FE_REQUIRE(ctx, value >= 0 && value <= 64, "ic0_sweeps must be 0 (exact ILU(0)) or 1..64");
     ctx->icSweeps = value;
+  } else if(k == "amg_omega_x100") {
+    FE_REQUIRE(ctx, value > 0 && value < 200, "amg_omega_x100 must be in (0, 200)");
+    ctx->amgOmega = value / 100.;
+  } else if(k == "amg_oc_x100") {
+    FE_REQUIRE(ctx, value > 0 && value < 200, "amg_oc_x100 must be in (0, 200)");
+    ctx->amgOverCorrect = value / 100.;
+  } else if(k == "amg_block_rows") {
+    FE_REQUIRE(ctx, value >= 0 && value <= (1 << 20), "amg_block_rows must be in [0, 2^20]");
+    ctx->amgBlockRows = value;
   } else if(k == "iter_grid") {
     FE_REQUIRE(ctx, value >= 8 && value <= 4096 && value % 8 == 0, "iter_grid must be a multiple of 8 in [8, 4096]");
     ctx->iterGrid = value;
@@ -505,6 +515,9 @@ int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value)
   else if(k == "ic0_sweeps") *value = ctx->icSweeps;
   else if(k == "pc_fp32") *value = ctx->pcFp32;
   else if(k == "pc_values32") *value = ctx->pcValues32;
+  else if(k == "amg_omega_x100") *value = (int)std::lround(ctx->amgOmega * 100.);
+  else if(k == "amg_oc_x100") *value = (int)std::lround(ctx->amgOverCorrect * 100.);
+  else if(k == "amg_block_rows") *value = ctx->amgBlockRows;
   else FE_REQUIRE(ctx, false, "unknown tuning key " + k);
   return FE_STATUS_OK;
 }

@@ -113,6 +113,7 @@ struct FePrecond {
 };
 
 struct FeMpnn;   // learned-preconditioner state (fe_learned.hip)
+struct FeAmg;    // aggregation multigrid hierarchy (fe_amg.hip)
 struct FeBatch;  // batched-solver state (fe_batch.hip)
 
 struct fe_hip_ctx {
@@ -157,6 +158,12 @@ struct fe_hip_ctx {
   int pcValues32 = 1;  // FP64 split PCG: factor values read as FP32 (vectors and arithmetic FP64)
   int pcFp32 = 0;      // FP64 split PCG: apply G^T D^-1 G in FP64 (0) or FP32 (1; the pipelined
                        // recurrences then drift from M^-1 r: 20% more iterations measured)
+  // FE_PC_AMG: l1-Jacobi factor, coarse over-correction, rows up to which a
+  // level runs inside the single-workgroup kernel
+  double amgOmega = 1.8, amgOverCorrect = 1.8;
+  int amgBlockRows = 1024;
+  FeAmg *amg = nullptr;
+  bool amgActive = false;  // the current CG solve applies the AMG V-cycle
   FeMpnn *mpnn = nullptr;
   FeBatch *batch = nullptr;    // Jacobi-PCG: 0 = on-the-fly D^-1 r gather, 1 = tiled (one gathered vector)
 
@@ -221,6 +228,16 @@ constexpr int kLearnedIcSweeps = 3;
 // Learned correction of the split factor (fe_learned.hip)
 int feLearnedCorrect(fe_hip_ctx *ctx);
 void feMpnnRelease(fe_hip_ctx *ctx);
+
+// Aggregation multigrid (fe_amg.hip): hierarchy of the CSR pattern (once),
+// numeric set-up per matrix, one V-cycle out = M^-1 in (skipped once *done).
+int feAmgSetup(fe_hip_ctx *ctx);
+int feAmgNumeric(fe_hip_ctx *ctx);
+int feAmgCheck(fe_hip_ctx *ctx);
+int feAmgApply(fe_hip_ctx *ctx, const double *in, double *out, const int *done, bool values32);
+int feAmgApply(fe_hip_ctx *ctx, const float *in, float *out, const int *done, bool values32);
+int feAmgLevels(const fe_hip_ctx *ctx, int *levels, int *rows, int64_t *nnz, int cap);
+void feAmgRelease(fe_hip_ctx *ctx);
 void feBatchRelease(fe_hip_ctx *ctx);
 
 // GMRES (fe_gmres.hip)

@@ -39,6 +39,22 @@ struct CgState {
   double hist[64];
 };
 
+// Iterations to launch before the next host check: the recursive residual's
+// geometric decay over the last (up to 16) iterations predicts how many are
+// left; launched iterations past convergence only exit early, but each still
+// costs a dispatch.
+int feNextBatch(const CgState *hs, int it, int last, int checkEvery, double tol2)
+{
+  const int m = std::min(last, 16);
+  if(it <= m) return checkEvery;
+  const double r1 = hs->hist[(it - 1) & 63], r0 = hs->hist[(it - 1 - m) & 63];
+  if(!(r1 > 0.) || !(r0 > r1)) return checkEvery;
+  const double rate = std::log(r1 / r0) / m;
+  const double need = std::log(tol2 / r1) / rate;
+  if(!(need > 0.)) return 1;
+  return std::max(1, std::min(checkEvery, (int)std::ceil(need) + 1));
+}
+
 constexpr int kTB = 256;       // block size of the one-off kernels
 constexpr int kGrid = 1024;    // grid of the one-off reduction kernels
 constexpr int kMaxGrid = 4096; // partial buffers hold this many blocks
@@ -1255,9 +1271,9 @@ int runPipe(fe_hip_ctx *ctx, Workspace &w, PipeVecs<T> &v, const T *val, int che
   CgState *hs = (CgState *)ctx->h_state;
   FeKernelTimer &tm = ctx->spmvTimer;
   tm.used = 0;
-  int it = 0;
+  int it = 0, batch = checkEvery;
   for(;;) {
-    for(int k = 0; k < checkEvery; ++k, ++it) {
+    for(int k = 0; k < batch; ++k, ++it) {
       if(ctx->profiling) {
         if(tm.used + 2 > tm.ev.size())
           for(int a = 0; a < 64; ++a) {
@@ -1274,6 +1290,7 @@ int runPipe(fe_hip_ctx *ctx, Workspace &w, PipeVecs<T> &v, const T *val, int che
     FE_HIP_TRY(ctx, hipMemcpyAsync(hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
     FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if(hs->done) break;
+    batch = feNextBatch(hs, it, batch, checkEvery, tol2);
   }
   if(ctx->profiling) {
     const size_t valid = std::min<size_t>(tm.used / 2, (size_t)std::max(0, hs->iter));
@@ -1360,8 +1377,12 @@ int runSplitPipe(fe_hip_ctx *ctx, Workspace &w, SpVecs<T> &v, const T *val, cons
   } while(0)
 #define FE_SP_LAUNCH(KER, LL, ...) FE_SP_LAUNCH_T(KER, T, LL, __VA_ARGS__)
   hipLaunchKernelGGL(spStart<T>, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, b, v);
-  // u_0 = G^T D^-1 G r_0, w_0 = A u_0
-  if(sizeof(T) == 8 && ctx->pcValues32 && !v.m32) {
+  const bool amg = ctx->amgActive;
+  const int *dDone = &((const CgState *)w.st)->done;
+  // u_0 = M^-1 r_0, w_0 = A u_0
+  if(amg) {
+    if(int rc = feAmgApply(ctx, (const T *)v.r, v.u, nullptr, ctx->pcValues32 != 0)) return rc;
+  } else if(sizeof(T) == 8 && ctx->pcValues32 && !v.m32) {
     FE_SP_LAUNCH(spSpmvF32, LF, n, P.d_iaL, P.d_jaL, (const float *)P.d_gL32, (const T *)v.r,
                  (const float *)P.d_dinv32, v.t, (const CgState *)w.st, 0);
     FE_SP_LAUNCH(spSpmvF32, LF, n, P.d_iaU, P.d_jaU, (const float *)P.d_gU32, (const T *)v.t, (const float *)nullptr,
@@ -1372,10 +1393,12 @@ int runSplitPipe(fe_hip_ctx *ctx, Workspace &w, SpVecs<T> &v, const T *val, cons
                  0);
   }
   FE_SP_LAUNCH(spInitW, L, n, ctx->d_ia, ctx->d_ja, val, v, w.pparts, w.sc, w.st);
-  int it = 0;
+  int it = 0, batch = checkEvery;
   for(;;) {
-    for(int k = 0; k < checkEvery; ++k, ++it) {
-      if(v.m32) {
+    for(int k = 0; k < batch; ++k, ++it) {
+      if(amg) {
+        if(int rc = feAmgApply(ctx, (const T *)v.w, v.m, dDone, ctx->pcValues32 != 0)) return rc;
+      } else if(v.m32) {
         // preconditioner applied in FP32 (FP32 factor, FP32 shadows of w, t, m)
         FE_SP_LAUNCH_T(spSpmv, float, LF, n, P.d_iaL, P.d_jaL, (const float *)P.d_gL32, (const float *)v.w32,
                        (const float *)P.d_dinv32, v.t32, (const CgState *)w.st, 1);
@@ -1408,6 +1431,7 @@ int runSplitPipe(fe_hip_ctx *ctx, Workspace &w, SpVecs<T> &v, const T *val, cons
     FE_HIP_TRY(ctx, hipMemcpyAsync(hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
     FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if(hs->done) break;
+    batch = feNextBatch(hs, it, batch, checkEvery, tol2);
   }
 #undef FE_SP_LAUNCH
 #undef FE_SP_LAUNCH_T
@@ -1436,9 +1460,9 @@ int runJacobi(fe_hip_ctx *ctx, bool tiled, Workspace &w, PipeVecs<T> &pv, TileVe
   CgState *hs = (CgState *)ctx->h_state;
   FeKernelTimer &tm = ctx->spmvTimer;
   tm.used = 0;
-  int it = 0;
+  int it = 0, batch = checkEvery;
   for(;;) {
-    for(int k = 0; k < checkEvery; ++k, ++it) {
+    for(int k = 0; k < batch; ++k, ++it) {
       if(ctx->profiling) {
         if(tm.used + 2 > tm.ev.size())
           for(int a = 0; a < 64; ++a) {
@@ -1455,6 +1479,7 @@ int runJacobi(fe_hip_ctx *ctx, bool tiled, Workspace &w, PipeVecs<T> &pv, TileVe
     FE_HIP_TRY(ctx, hipMemcpyAsync(hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
     FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if(hs->done) break;
+    batch = feNextBatch(hs, it, batch, checkEvery, tol2);
   }
   if(ctx->profiling) {
     const size_t valid = std::min<size_t>(tm.used / 2, (size_t)std::max(0, hs->iter));
@@ -1573,9 +1598,11 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
 {
   if(o->ksp == FE_KSP_GMRES) return feSolveGmres(ctx, o, du, stats);
   FE_REQUIRE(ctx, o->ksp == FE_KSP_CG, "unknown Krylov method");
-  FE_REQUIRE(ctx, o->pc == FE_PC_JACOBI || o->pc == FE_PC_IC0 || o->pc == FE_PC_LEARNED,
-             "CG takes the JACOBI, IC0 or LEARNED preconditioner");
+  FE_REQUIRE(ctx, o->pc == FE_PC_JACOBI || o->pc == FE_PC_IC0 || o->pc == FE_PC_LEARNED || o->pc == FE_PC_AMG,
+             "CG takes the JACOBI, IC0, LEARNED or AMG preconditioner");
   const bool split = (o->pc != FE_PC_JACOBI);
+  const bool amg = (o->pc == FE_PC_AMG);
+  ctx->amgActive = amg;
   // pipelined form per precision: the tiled Ghysels-Vanroose iteration (one
   // gathered vector) for FP64 solves, Chronopoulos-Gear (w = A u every
   // iteration, fewer refinement restarts) for FP32 inner solves
@@ -1609,9 +1636,14 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
     FE_HIP_TRY(ctx, hipEventCreate(&s0));
     FE_HIP_TRY(ctx, hipEventCreate(&s1));
     FE_HIP_TRY(ctx, hipEventRecord(s0, ctx->stream));
-    if(int rc = fePrecondFactor(ctx, o->pc == FE_PC_LEARNED ? kLearnedIcSweeps : ctx->icSweeps)) return rc;
-    if(o->pc == FE_PC_LEARNED)
-      if(int rc = feLearnedCorrect(ctx)) return rc;
+    if(amg) {
+      if(int rc = feAmgSetup(ctx)) return rc;
+      if(int rc = feAmgNumeric(ctx)) return rc;
+    } else {
+      if(int rc = fePrecondFactor(ctx, o->pc == FE_PC_LEARNED ? kLearnedIcSweeps : ctx->icSweeps)) return rc;
+      if(o->pc == FE_PC_LEARNED)
+        if(int rc = feLearnedCorrect(ctx)) return rc;
+    }
     FE_HIP_TRY(ctx, hipEventRecord(s1, ctx->stream));
     FE_HIP_TRY(ctx, hipEventSynchronize(s1));
     float sm = 0.f;
@@ -1649,7 +1681,7 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
       FE_HIP_TRY(ctx, hipMemcpyAsync(w.tmp64, w.x64, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
       const FePrecond &Pc = ctx->pc;
       if(inner32) {
-        if(tiled32) {
+        if(tiled32 || amg) {
           if(int rc = runSplitPipe<float>(ctx, w, w.sp32, ctx->d_val32, Pc.d_gL32, Pc.d_gU32, Pc.d_dinv32, w.r64,
                                           checkEvery, o->max_iter - iters, tol2, &it0, &done))
             return rc;
@@ -1658,10 +1690,10 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
           return rc;
         }
         hipLaunchKernelGGL(addVec<float>, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, w.x64,
-                           tiled32 ? w.sp32.x : w.s32.x);
+                           (tiled32 || amg) ? w.sp32.x : w.s32.x);
       } else {
-        if(tiled64) {
-          const bool mixed = ctx->pcFp32 != 0;
+        if(tiled64 || amg) {
+          const bool mixed = ctx->pcFp32 != 0 && !amg;
           w.sp64.w32 = mixed ? w.spShadow[0] : nullptr;
           w.sp64.t32 = mixed ? w.spShadow[1] : nullptr;
           w.sp64.m32 = mixed ? w.spShadow[2] : nullptr;
@@ -1673,7 +1705,7 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
           return rc;
         }
         hipLaunchKernelGGL(addVec<double>, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, w.x64,
-                           tiled64 ? w.sp64.x : w.s64.x);
+                           (tiled64 || amg) ? w.sp64.x : w.s64.x);
       }
       iters += it0;
       double rr = 0.;
@@ -1696,6 +1728,8 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
     }
     relres = rn / bn;
     if(!converged && rn <= target) { converged = 1; reason = 2; }
+    if(amg)
+      if(int rc = feAmgCheck(ctx)) return rc;
     FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_du, w.x64, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
   } else if(o->precision == FE_PREC_FP64) {
     double *dinv64 = tiled64 ? w.t64.dinv : w.q64.dinv;

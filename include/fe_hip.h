@@ -68,6 +68,7 @@ typedef struct {
 #define FE_PC_ILU0 3
 #define FE_PC_BJ_ILU0 4
 #define FE_PC_LEARNED 5
+#define FE_PC_AMG 6 /* aggregation multigrid V-cycle (PETSc -pc_type gamg role) */
 
 #define FE_PREC_FP64 0    /* FP64 Krylov */
 #define FE_PREC_FP32_IR 1 /* FP32 inner Krylov + FP64 iterative refinement */
@@ -195,7 +196,10 @@ int fe_hip_set_profiling(fe_hip_ctx *ctx, int on);
  * the reference delegates the solve to PETSc).  Keys: "spmv_lanes" (lanes
  * per CSR row: 4, 8, 16, 32), "spmv_unroll" (row groups in flight per lane
  * group: 1, 2), "iter_block" (256, 512, 1024), "iter_grid" (blocks, a
- * multiple of 8 up to 4096). */
+ * multiple of 8 up to 4096); FE_PC_AMG: "amg_omega_x100" (l1-Jacobi factor
+ * times 100, default 180), "amg_oc_x100" (coarse over-correction times 100,
+ * default 180), "amg_block_rows" (levels up to this many rows run in one
+ * workgroup, default 1024). */
 int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value);
 int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value);
 
@@ -204,6 +208,17 @@ int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value);
  * reference's per-instance feFunction objects (feFunction.h) when many
  * coefficient fields are assembled on one mesh. */
 int fe_hip_update_coeff(fe_hip_ctx *ctx, int form, int index, const double *data, int64_t count);
+
+/* ---- aggregation multigrid (FE_PC_AMG) ----------------------------------
+ * The algebraic multigrid the reference reaches through PETSc's runtime
+ * options (-pc_type gamg, feLinearSystemPETSc.cpp:900-1031).  The hierarchy
+ * of the CSR pattern is built on the first AMG solve; the numeric set-up runs
+ * in every solve.  fe_hip_amg_levels reports the rows and nonzeros per level
+ * (up to cap levels, *levels = 0 before the first AMG solve);
+ * fe_hip_amg_apply applies one V-cycle z = M^-1 r (host arrays of n values,
+ * FP64), setting the hierarchy up from the current values first. */
+int fe_hip_amg_levels(fe_hip_ctx *ctx, int *levels, int *rows, int64_t *nnz, int cap);
+int fe_hip_amg_apply(fe_hip_ctx *ctx, const double *r, double *z);
 
 /* ---- batched independent problems ---------------------------------------
  * No reference counterpart (the reference solves one system per

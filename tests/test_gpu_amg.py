@@ -1,0 +1,105 @@
+"""Aggregation multigrid on the GPU (fe_amg.hip) against the CPU oracle
+(oracle/fe_oracle_amg.py, pinned in tests/test_oracle_amg.py).
+
+The hierarchy (rows and stored entries per level) is integer data and must
+match exactly.  One V-cycle with FP64 level values matches the oracle to
+1e-11 relative (the device sums the Galerkin entries, the l1 norms and the
+row products in other orders, and inverts the coarsest level by Gauss-Jordan
+instead of LAPACK); with FP32 level values (the default) to 1e-5, the FP32
+rounding of the values carried through the cycle.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import fe_oracle_amg as amg
+
+pytestmark = pytest.mark.gpu
+
+
+def nrel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+def _system(case):
+    from feng_amd.api import FN_CONST, FN_ELEM, Fn, Mesh
+    from feng_amd.problems import diffusion_system
+    if case == "tet_grf_12":
+        m = Mesh.cube(12)
+        k, kind, deg = Fn(FN_ELEM, data=m.coeff_grf(0.1, 1.5, 64, 5)), 4, 1
+    elif case == "p2_blocks_40":
+        m = Mesh.square(40)
+        k, kind, deg = Fn(FN_ELEM, data=m.coeff_blocks(16, 1e-2, 1e2, 9)), 2, 2
+    else:  # p1_square_9: 64 unknowns, one level
+        m = Mesh.square(9)
+        k, kind, deg = Fn(FN_CONST, 1.0), 1, 2
+    S, num = diffusion_system(m, kind, deg, k, Fn(FN_CONST, -1.0))
+    S.assemble(3)
+    return m, S, num
+
+
+def _oracle_levels(S):
+    ia, ja = S.get_csr()
+    val, rhs = S.get_system()
+    n = len(ia) - 1
+    A = sp.csr_matrix((val, ja, ia), shape=(n, n))
+    return A, rhs, amg.hierarchy(A)
+
+
+@pytest.mark.parametrize("case", ["tet_grf_12", "p2_blocks_40", "p1_square_9"])
+def test_vcycle_matches_oracle(case):
+    print("case", case, flush=True)
+    m, S, num = _system(case)
+    A, rhs, lv = _oracle_levels(S)
+    r = np.random.default_rng(3).normal(size=A.shape[0])
+    S.set_tuning("pc_values32", 0)
+    z = S.amg_apply(r)
+    rows, nnz = S.amg_levels()
+    assert rows == [L.A.shape[0] for L in lv]
+    assert nnz == [L.A.nnz for L in lv]
+    zo = amg.vcycle(lv, r)
+    assert nrel(z, zo) <= 1e-11, nrel(z, zo)
+    if len(lv) == 1:
+        assert nrel(A @ z, r) <= 1e-11
+    S.set_tuning("pc_values32", 1)
+    z32 = S.amg_apply(r)
+    assert nrel(z32, zo) <= 1e-5, nrel(z32, zo)
+    S.close()
+
+
+def test_numeric_setup_follows_new_values():
+    m, S, num = _system("tet_grf_12")
+    S.set_tuning("pc_values32", 0)
+    r = np.random.default_rng(4).normal(size=num.nInc)
+    S.amg_apply(r)
+    S.update_coeff(0, 0, m.coeff_grf(0.1, 1.5, 64, 11))
+    S.setToZero()
+    S.assemble(3)
+    A, rhs, lv = _oracle_levels(S)
+    assert nrel(S.amg_apply(r), amg.vcycle(lv, r)) <= 1e-11
+    S.close()
+
+
+@pytest.mark.parametrize("case", ["tet_grf_16", "p2_blocks_64"])
+def test_amg_pcg_converges(oracle, case):
+    from feng_amd.api import FN_CONST, FN_ELEM, PC_AMG, PC_JACOBI, PREC_FP32_IR, Fn, Mesh
+    from feng_amd.problems import diffusion_system
+    print("case", case, flush=True)
+    if case == "tet_grf_16":
+        m = Mesh.cube(16)
+        k, kind, deg = Fn(FN_ELEM, data=m.coeff_grf(0.1, 1.5, 256, 2)), 4, 1
+    else:
+        m = Mesh.square(64)
+        k, kind, deg = Fn(FN_ELEM, data=m.coeff_blocks(16, 1e-2, 1e2, 9)), 2, 2
+    S, num = diffusion_system(m, kind, deg, k, Fn(FN_CONST, -1.0))
+    S.assemble(3, PREC_FP32_IR)
+    rj = S.solve(pc=PC_JACOBI, rtol=1e-8, max_iter=20000)
+    ra, du = S.solve(pc=PC_AMG, rtol=1e-8, want_du=True)
+    ra32 = S.solve(pc=PC_AMG, precision=PREC_FP32_IR, rtol=1e-8, inner_rtol=1e-2)
+    assert rj.converged and ra.converged and ra32.converged
+    assert ra.iterations * 3 < rj.iterations, (ra.iterations, rj.iterations)
+    ia, ja = S.get_csr()
+    val, rhs = S.get_system()
+    res = rhs - oracle.spmv(ia, ja, val, du)
+    assert np.linalg.norm(res) <= 1.01e-8 * np.linalg.norm(rhs)
+    S.close()

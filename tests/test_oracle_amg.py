@@ -1,0 +1,145 @@
+"""Pins of the aggregation-multigrid oracle (oracle/fe_oracle_amg.py).
+
+The reference reaches AMG only through PETSc options, so these checks rest on
+mathematics: an independent algorithm for the aggregation (Luby's rounds),
+sparse products for the Galerkin operator, the dense textbook two-grid error
+propagation for the cycle, symmetry and definiteness of the multilevel cycle.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from oracle import fe_oracle_amg as amg
+from tests.test_oracle_mpnn import grid_matrix
+
+
+def csr(N, seed=3):
+    ia, ja, val = grid_matrix(N, seed)
+    n = len(ia) - 1
+    return sp.csr_matrix((val, ja, ia), shape=(n, n))
+
+
+def luby_mis(ia, ja, pri):
+    """Luby's rounds: an undecided row whose priority is the largest among its
+    undecided neighbours joins the set, its neighbours leave."""
+    n = len(ia) - 1
+    rows = np.repeat(np.arange(n), np.diff(ia))
+    state = np.zeros(n, np.int8)
+    key = pri.astype(object)
+    while (state == 0).any():
+        best = np.full(n, -1, dtype=object)
+        und = state[ja] == 0
+        for r, j in zip(rows[und], ja[und]):
+            if best[r] is None or key[j] > best[r]:
+                best[r] = key[j]
+        newin = [i for i in range(n) if state[i] == 0 and best[i] == key[i]]
+        state[newin] = 1
+        for i in newin:
+            nb = ja[ia[i]:ia[i + 1]]
+            state[nb[state[nb] == 0]] = -1
+    return state == 1
+
+
+@pytest.mark.parametrize("N", [6, 13])
+def test_aggregation_is_the_luby_fixed_point(N):
+    A = csr(N)
+    ia, ja = A.indptr, A.indices
+    agg, nc = amg.aggregate(ia, ja)
+    roots = luby_mis(ia, ja, amg.priorities(A.shape[0]))
+    # roots of the oracle = Luby's set, numbered in row order
+    oracle_roots = np.zeros(A.shape[0], bool)
+    for I in range(nc):
+        members = np.flatnonzero(agg == I)
+        # the root is the member adjacent to every other member's choice: the
+        # one row of the aggregate that is in the independent set
+        oracle_roots[members[roots[members]]] = True
+        assert roots[members].sum() == 1
+    assert np.array_equal(oracle_roots, roots)
+    rid = np.cumsum(roots) - 1
+    for i in range(A.shape[0]):
+        nb = ja[ia[i]:ia[i + 1]]
+        if roots[i]:
+            assert agg[i] == rid[i]
+            assert not roots[nb[nb != i]].any()  # independent
+        else:
+            assert roots[nb].any()  # maximal
+            assert agg[i] == rid[nb[roots[nb]]].max()
+
+
+def test_mix64_known_values():
+    # first output of SplitMix64 seeded with 0 and with 1 (finaliser of seed + golden gamma)
+    assert amg.mix64(0) == 0xE220A8397B1DCDAF
+    assert amg.mix64(1) == 0x910A2DEC89025CC1
+
+
+def test_galerkin_is_pt_a_p():
+    A = csr(13)
+    agg, nc = amg.aggregate(A.indptr, A.indices)
+    P = amg.tentative(agg, nc)
+    C = amg.galerkin(A, agg, nc)
+    ref = (P.T @ A @ P).toarray()
+    assert np.abs(C.toarray() - ref).max() <= 1e-13 * np.abs(ref).max()
+    assert C.nnz >= np.count_nonzero(ref)
+
+
+def test_two_level_cycle_is_the_textbook_two_grid():
+    A = csr(13)  # 169 rows: one aggregation step down to <= 64 rows
+    lv = amg.hierarchy(A)
+    assert len(lv) == 2
+    n = A.shape[0]
+    Ad = A.toarray()
+    L = lv[0]
+    P = amg.tentative(L.agg, L.nc).toarray()
+    Ac = P.T @ Ad @ P
+    S = np.diag(L.l1)
+    I = np.eye(n)
+    E = (I - S @ Ad) @ (I - L.oc * P @ np.linalg.solve(Ac, P.T @ Ad)) @ (I - S @ Ad)
+    Minv = (I - E) @ np.linalg.inv(Ad)
+    rng = np.random.default_rng(0)
+    for _ in range(3):
+        b = rng.normal(size=n)
+        z = amg.vcycle(lv, b)
+        assert np.linalg.norm(z - Minv @ b) <= 1e-11 * np.linalg.norm(z)
+
+
+def test_single_level_is_the_exact_inverse():
+    A = csr(8)  # 64 rows
+    lv = amg.hierarchy(A)
+    assert len(lv) == 1
+    b = np.random.default_rng(1).normal(size=A.shape[0])
+    assert np.allclose(A @ amg.vcycle(lv, b), b, rtol=0, atol=1e-11 * np.abs(b).max())
+
+
+def test_multilevel_cycle_is_spd():
+    A = csr(30)  # 900 rows, three levels
+    lv = amg.hierarchy(A)
+    assert len(lv) >= 3
+    n = A.shape[0]
+    M = np.column_stack([amg.vcycle(lv, e) for e in np.eye(n)])
+    assert np.abs(M - M.T).max() <= 1e-12 * np.abs(M).max()
+    assert np.linalg.eigvalsh(0.5 * (M + M.T)).min() > 0
+
+
+def test_amg_pcg_beats_jacobi_on_a_3d_problem(oracle):
+    import feng_amd.api as A
+    m = A.Mesh.cube(12)
+    num = A.Numbering(m, [A.TET_P1], [1])
+    n = num.nInc
+    ia, ja = oracle.csr(n, [(num.adr[0], 4, num.adr[0], 4)])
+    k = m.coeff_grf(0.1, 1.5, 64, 5)
+    val, rhs = oracle.assemble_diffusion(4, m.xyz, m.conn, num.adr[0], n, ia, ja, 1, (4, 0.0, 0.0, k),
+                                         (0, -1.0, 0.0, None), np.zeros(num.nDofs))
+    K = sp.csr_matrix((val, ja, ia), shape=(n, n))
+    lv = amg.hierarchy(K)
+
+    def its(M):
+        c = [0]
+        x, info = spla.cg(K, rhs, rtol=1e-8, M=spla.LinearOperator((n, n), matvec=M),
+                          callback=lambda _: c.__setitem__(0, c[0] + 1), maxiter=2000)
+        assert info == 0
+        return c[0]
+
+    j = its(lambda r: r / K.diagonal())
+    a = its(lambda r: amg.vcycle(lv, r))
+    assert a * 3 < j, (a, j)
