@@ -165,6 +165,7 @@ class SolveResult:
     norm_rhs_inf: float
     norm_axb_inf: float
     ms_solve: float
+    ms_setup: float = 0.0
 
 
 class LinearSystemHIP:
@@ -277,7 +278,8 @@ class LinearSystemHIP:
         self._c(self.L.fe_hip_solve(self.ctx, C.byref(o), N.ptr(du, C.c_double) if du is not None else None,
                                     C.byref(st)), "fe_hip_solve")
         res = SolveResult(st.iterations, st.outer_iterations, bool(st.converged), st.reason, st.relres_fp64,
-                          st.norm_dx_inf, st.norm_rhs_inf, st.norm_axb_inf, st.ms_solve)
+                          st.norm_dx_inf, st.norm_rhs_inf, st.norm_axb_inf, st.ms_solve,
+                          st.ms_setup)
         return (res, du) if want_du else res
 
     def correctSolution(self):
