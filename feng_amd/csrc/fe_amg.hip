@@ -60,6 +60,16 @@ struct AmgBlockPlan {
   int oCx = 0, bytes = 0;
 };
 
+// Tile of this block: with xcd set and a grid that is a multiple of 8, the
+// 8 XCDs (block b runs on XCD b % 8) take 8 contiguous ranges of tiles, so
+// the gathers of neighbouring rows stay in one XCD's L2.
+__device__ __forceinline__ int amgTile(int xcd)
+{
+  const int G = gridDim.x;
+  if(!xcd || (G & 7)) return blockIdx.x;
+  return (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
+}
+
 template <typename T, typename TV>
 struct AmgLev {
   int n, nc;
@@ -82,6 +92,11 @@ struct FeAmgLevel {
   float *l1_32 = nullptr;
   int *agg = nullptr, *mptr = nullptr, *mlist = nullptr;  // to level + 1
   int2 *mrange = nullptr;
+  // aggregate-aligned tiles of kAmgTB member positions (no aggregate crosses
+  // a tile; padding positions have row -1): row, row range, aggregate
+  int padTiles = 0;
+  int *prow = nullptr, *pagg = nullptr;
+  int2 *prg = nullptr;
   int *cptr = nullptr, *clist = nullptr;                  // Galerkin sums from level - 1
   int nTiles = 0;
   int *tileRow = nullptr, *tileAgg = nullptr;
@@ -336,6 +351,55 @@ __global__ __launch_bounds__(1024) void amgDenseInverse(int n, const int *__rest
 // ---------------------------------------------------------------------------
 // Device: the V-cycle
 
+// Restriction fused with the first smoothing step over aggregate-aligned
+// tiles of TB member positions: u_i = l1_i b_i, r_i = b_i - sum_j a_ij l1_j b_j
+// for the member rows (through LDS), then the first position of every
+// aggregate sums its members into bNext.  The position data are the first
+// loads of the kernel, so the row products start one load later than
+// through a tile header.
+template <typename T, typename TV, int LPR, int TB>
+__global__ __launch_bounds__(TB) void amgDownP(AmgLev<T, TV> L, const T *__restrict__ b, T *__restrict__ bNext,
+                                               const int *__restrict__ prow, const int2 *__restrict__ prg,
+                                               const int *__restrict__ pagg, int nTiles, int xcd, const int *done)
+{
+  __shared__ T ldsY[TB];
+  __shared__ T rb[TB];
+  __shared__ int ag[TB];
+  if(done && *done) return;
+  const int tile = amgTile(xcd);
+  if(tile >= nTiles) return;
+  constexpr int RPB = TB / LPR;
+  const int sub = threadIdx.x % LPR, g = threadIdx.x / LPR;
+  const int base = tile * TB;
+  int kb[LPR], ke[LPR];
+#pragma unroll
+  for(int u = 0; u < LPR; ++u) {
+    const int2 rg = prg[base + u * RPB + g];
+    kb[u] = rg.x + sub;
+    ke[u] = rg.y;
+  }
+  const int t = threadIdx.x;
+  const int i = prow[base + t], a = pagg[base + t];
+  T bi = (T)0, li = (T)0;
+  if(i >= 0) {
+    bi = b[i];
+    li = (T)L.l1[i];
+  }
+  tileProductF<T, LPR, TB>(L.ja, L.val, kb, ke, ldsY, [&](int j) { return (T)L.l1[j] * b[j]; });
+  __syncthreads();
+  if(i >= 0) {
+    L.u[i] = li * bi;
+    rb[t] = bi - ldsY[t];
+  }
+  ag[t] = a;
+  __syncthreads();
+  if(a >= 0 && (t == 0 || ag[t - 1] != a)) {
+    T s = (T)0;
+    for(int q = t; q < TB && ag[q] == a; ++q) s += rb[q];
+    bNext[a] = s;
+  }
+}
+
 // Restriction fused with the first smoothing step, over aggregate tiles:
 // u_i = l1_i b_i, r_i = b_i - sum_j a_ij l1_j b_j for the member rows of the
 // tile (through LDS), then bNext[I] = sum of r over the members of I.
@@ -387,13 +451,13 @@ __global__ __launch_bounds__(TB) void amgDown(AmgLev<T, TV> L, const T *__restri
 // y_i = u_i + oc e[agg_i], x_i = y_i + l1_i (b_i - sum_j a_ij y_j).
 template <typename T, typename TV, int LPR, int TB>
 __global__ __launch_bounds__(TB) void amgUp(AmgLev<T, TV> L, const T *__restrict__ b, const T *__restrict__ e, T oc,
-                                            T *__restrict__ x, const int *done)
+                                            T *__restrict__ x, int xcd, const int *done)
 {
   __shared__ T ldsY[TB];
   if(done && *done) return;
   const int n = L.n;
   const int nTiles = (n + TB - 1) / TB;
-  for(int tile = blockIdx.x; tile < nTiles; tile += gridDim.x) {
+  for(int tile = amgTile(xcd); tile < nTiles; tile += gridDim.x) {
     const int base = tile * TB;
     int kb[LPR], ke[LPR];
     tileBounds<LPR, TB>(n, base, L.ia, kb, ke);
@@ -644,6 +708,35 @@ int feAmgSetup(fe_hip_ctx *ctx)
       if(int rc = upload(ctx, &d.agg, h.agg)) return rc;
       if(int rc = upload(ctx, &d.mptr, h.mptr)) return rc;
       if(int rc = upload(ctx, &d.mlist, h.mlist)) return rc;
+      if(h.maxAgg <= kAmgTB) {
+        std::vector<int> prow, pagg;
+        std::vector<int2> prg;
+        for(int I = 0; I < h.nc; ++I) {
+          const int sz = h.mptr[I + 1] - h.mptr[I];
+          const int used = (int)prow.size() % kAmgTB;
+          if(used + sz > kAmgTB)
+            for(int q = used; q < kAmgTB; ++q) {
+              prow.push_back(-1);
+              pagg.push_back(-1);
+              prg.push_back(make_int2(0, 0));
+            }
+          for(int q = h.mptr[I]; q < h.mptr[I + 1]; ++q) {
+            const int i = h.mlist[q];
+            prow.push_back(i);
+            pagg.push_back(I);
+            prg.push_back(make_int2(h.ia[i], h.ia[i + 1]));
+          }
+        }
+        while(prow.size() % kAmgTB) {
+          prow.push_back(-1);
+          pagg.push_back(-1);
+          prg.push_back(make_int2(0, 0));
+        }
+        d.padTiles = (int)(prow.size() / kAmgTB);
+        if(int rc = upload(ctx, &d.prow, prow)) return rc;
+        if(int rc = upload(ctx, &d.pagg, pagg)) return rc;
+        if(int rc = upload(ctx, &d.prg, prg)) return rc;
+      }
       {
         std::vector<int2> rg(h.n);
         for(int q = 0; q < h.n; ++q) rg[q] = make_int2(h.ia[h.mlist[q]], h.ia[h.mlist[q] + 1]);
@@ -794,8 +887,20 @@ int amgApplyT(fe_hip_ctx *ctx, const T *in, T *out, const int *done, int slot)
     const FeAmgLevel &h = A.lv[l];
     AmgLev<T, TV> L = amgView<T, TV>(h);
     const T *b = l == 0 ? in : L.b;
-    hipLaunchKernelGGL((amgDown<T, TV, kAmgLPR, kAmgTB>), dim3(h.nTiles), dim3(kAmgTB), 0, ctx->stream, L, b,
-                       pick<T>(A.lv[l + 1].b64, A.lv[l + 1].b32), h.tileRow, h.tileAgg, done);
+    T *bn = pick<T>(A.lv[l + 1].b64, A.lv[l + 1].b32);
+    const dim3 gp(ctx->amgXcd ? (h.padTiles + 7) & ~7 : h.padTiles);
+    if(h.padTiles > 0 && ctx->amgLanes == 2)
+      hipLaunchKernelGGL((amgDownP<T, TV, 2, kAmgTB>), gp, dim3(kAmgTB), 0, ctx->stream, L, b, bn, h.prow, h.prg,
+                         h.pagg, h.padTiles, ctx->amgXcd, done);
+    else if(h.padTiles > 0 && ctx->amgLanes == 8)
+      hipLaunchKernelGGL((amgDownP<T, TV, 8, kAmgTB>), gp, dim3(kAmgTB), 0, ctx->stream, L, b, bn, h.prow, h.prg,
+                         h.pagg, h.padTiles, ctx->amgXcd, done);
+    else if(h.padTiles > 0)
+      hipLaunchKernelGGL((amgDownP<T, TV, kAmgLPR, kAmgTB>), gp, dim3(kAmgTB), 0, ctx->stream, L, b, bn, h.prow,
+                         h.prg, h.pagg, h.padTiles, ctx->amgXcd, done);
+    else
+      hipLaunchKernelGGL((amgDown<T, TV, kAmgLPR, kAmgTB>), dim3(h.nTiles), dim3(kAmgTB), 0, ctx->stream, L, b, bn,
+                         h.tileRow, h.tileAgg, done);
   }
   hipLaunchKernelGGL((amgBlock<T, TV>), dim3(1), dim3(kAmgBlockTB), 0, ctx->stream,
                      (const AmgLev<T, TV> *)A.desc[slot], pl, (const T *)pick<T>(A.cinv64, A.cinv32), oc, in, out,
@@ -805,9 +910,18 @@ int amgApplyT(fe_hip_ctx *ctx, const T *in, T *out, const int *done, int slot)
     AmgLev<T, TV> L = amgView<T, TV>(h);
     const T *b = l == 0 ? in : L.b;
     T *x = l == 0 ? out : L.x;
-    const int G = std::max(1, std::min(ctx->iterGrid * 2, (h.n + kAmgTB - 1) / kAmgTB));
-    hipLaunchKernelGGL((amgUp<T, TV, kAmgLPR, kAmgTB>), dim3(G), dim3(kAmgTB), 0, ctx->stream, L, b,
-                       (const T *)pick<T>(A.lv[l + 1].x64, A.lv[l + 1].x32), oc, x, done);
+    int G = std::max(1, (h.n + kAmgTB - 1) / kAmgTB);
+    if(ctx->amgXcd) G = (G + 7) & ~7;
+    const T *e = pick<T>(A.lv[l + 1].x64, A.lv[l + 1].x32);
+    if(ctx->amgLanes == 2)
+      hipLaunchKernelGGL((amgUp<T, TV, 2, kAmgTB>), dim3(G), dim3(kAmgTB), 0, ctx->stream, L, b, e, oc, x, ctx->amgXcd,
+                         done);
+    else if(ctx->amgLanes == 8)
+      hipLaunchKernelGGL((amgUp<T, TV, 8, kAmgTB>), dim3(G), dim3(kAmgTB), 0, ctx->stream, L, b, e, oc, x, ctx->amgXcd,
+                         done);
+    else
+      hipLaunchKernelGGL((amgUp<T, TV, kAmgLPR, kAmgTB>), dim3(G), dim3(kAmgTB), 0, ctx->stream, L, b, e, oc, x,
+                         ctx->amgXcd, done);
   }
   return FE_STATUS_OK;
 }

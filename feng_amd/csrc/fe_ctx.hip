@@ -474,6 +474,15 @@ int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value)
   } else if(k == "amg_oc_x100") {
     FE_REQUIRE(ctx, value > 0 && value < 200, "amg_oc_x100 must be in (0, 200)");
     ctx->amgOverCorrect = value / 100.;
+  } else if(k == "xcd_tiles") {
+    FE_REQUIRE(ctx, value == 0 || value == 1, "xcd_tiles must be 0 or 1");
+    ctx->xcdTiles = value;
+  } else if(k == "amg_lanes") {
+    FE_REQUIRE(ctx, value == 2 || value == 4 || value == 8, "amg_lanes must be 2, 4 or 8");
+    ctx->amgLanes = value;
+  } else if(k == "amg_xcd") {
+    FE_REQUIRE(ctx, value == 0 || value == 1, "amg_xcd must be 0 or 1");
+    ctx->amgXcd = value;
   } else if(k == "amg_block_rows") {
     FE_REQUIRE(ctx, value >= 0 && value <= (1 << 20), "amg_block_rows must be in [0, 2^20]");
     ctx->amgBlockRows = value;
@@ -518,6 +527,9 @@ int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value)
   else if(k == "amg_omega_x100") *value = (int)std::lround(ctx->amgOmega * 100.);
   else if(k == "amg_oc_x100") *value = (int)std::lround(ctx->amgOverCorrect * 100.);
   else if(k == "amg_block_rows") *value = ctx->amgBlockRows;
+  else if(k == "amg_xcd") *value = ctx->amgXcd;
+  else if(k == "xcd_tiles") *value = ctx->xcdTiles;
+  else if(k == "amg_lanes") *value = ctx->amgLanes;
   else FE_REQUIRE(ctx, false, "unknown tuning key " + k);
   return FE_STATUS_OK;
 }

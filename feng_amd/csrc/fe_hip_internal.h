@@ -162,6 +162,9 @@ struct fe_hip_ctx {
   // level runs inside the single-workgroup kernel
   double amgOmega = 1.8, amgOverCorrect = 1.8;
   int amgBlockRows = 1024;
+  int amgXcd = 1;    // XCD-aware tile order in the multigrid level kernels
+  int xcdTiles = 0;  // XCD-aware tile order in the row-tile PCG kernels (measured slower at 512-row tiles)
+  int amgLanes = 4;  // lanes per row in the multigrid level kernels
   FeAmg *amg = nullptr;
   bool amgActive = false;  // the current CG solve applies the AMG V-cycle
   FeMpnn *mpnn = nullptr;

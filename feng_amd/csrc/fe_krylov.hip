@@ -56,8 +56,18 @@ int feNextBatch(const CgState *hs, int it, int last, int checkEvery, double tol2
 }
 
 constexpr int kTB = 256;       // block size of the one-off kernels
-constexpr int kGrid = 1024;    // grid of the one-off reduction kernels
 constexpr int kMaxGrid = 4096; // partial buffers hold this many blocks
+
+// Blocks of a row-tile iteration kernel: one per tile up to iter_grid, rounded
+// up to a multiple of 8 for the XCD-aware tile order (blocks past the last
+// tile store zero partials).
+int feIterGrid(const fe_hip_ctx *ctx, int nTiles)
+{
+  int G = std::max(1, std::min(ctx->iterGrid, nTiles));
+  if(ctx->xcdTiles) G = std::min(kMaxGrid, (G + 7) & ~7);
+  return G;
+}
+constexpr int kGrid = 1024;    // grid of the one-off reduction kernels
 
 template <typename T>
 struct CgVecs {
@@ -554,7 +564,7 @@ __global__ __launch_bounds__(TB) void tileInit(int n, const int *__restrict__ ia
   __shared__ T ldsY[TB];
   const int nTiles = (n + TB - 1) / TB;
   double g = 0., d = 0., rho = 0.;
-  for(int tile = blockIdx.x; tile < nTiles; tile += gridDim.x) {
+  for(int tile = tileStart(); tile < nTiles; tile += gridDim.x) {
     const int base = tile * TB;
     int kb[LPR], ke[LPR];
     tileBounds<LPR, TB>(n, base, ia, kb, ke);
@@ -611,7 +621,7 @@ __global__ __launch_bounds__(TB) void tileIter(int n, const int *__restrict__ ia
   __shared__ T ldsY[TB];
   const int c = it & 1, o = c ^ 1;
   const int nTiles = (n + TB - 1) / TB;
-  int tile = blockIdx.x;
+  int tile = tileStart();
   // first tile: row bounds and own-row values in flight during the fold
   int kb[LPR], ke[LPR];
   TileRow<T> me;
@@ -721,7 +731,7 @@ __global__ __launch_bounds__(TB) void spSpmv(int n, const int *__restrict__ ia, 
   __shared__ T ldsY[TB];
   if(checkDone && st->done) return;
   const int nTiles = (n + TB - 1) / TB;
-  for(int tile = blockIdx.x; tile < nTiles; tile += gridDim.x) {
+  for(int tile = tileStart(); tile < nTiles; tile += gridDim.x) {
     const int base = tile * TB;
     int kb[LPR], ke[LPR];
     tileBounds<LPR, TB>(n, base, ia, kb, ke);
@@ -743,7 +753,7 @@ __global__ __launch_bounds__(TB) void spSpmvF32(int n, const int *__restrict__ i
   __shared__ T ldsY[TB];
   if(checkDone && st->done) return;
   const int nTiles = (n + TB - 1) / TB;
-  for(int tile = blockIdx.x; tile < nTiles; tile += gridDim.x) {
+  for(int tile = tileStart(); tile < nTiles; tile += gridDim.x) {
     const int base = tile * TB;
     int kb[LPR], ke[LPR];
     tileBounds<LPR, TB>(n, base, ia, kb, ke);
@@ -774,7 +784,7 @@ __global__ __launch_bounds__(TB) void spInitW(int n, const int *__restrict__ ia,
   __shared__ T ldsY[TB];
   const int nTiles = (n + TB - 1) / TB;
   double g = 0., d = 0., rho = 0.;
-  for(int tile = blockIdx.x; tile < nTiles; tile += gridDim.x) {
+  for(int tile = tileStart(); tile < nTiles; tile += gridDim.x) {
     const int base = tile * TB;
     int kb[LPR], ke[LPR];
     tileBounds<LPR, TB>(n, base, ia, kb, ke);
@@ -820,7 +830,7 @@ __global__ __launch_bounds__(TB) void spIter(int n, const int *__restrict__ ia, 
   __shared__ T ldsY[TB];
   const int c = it & 1, o = c ^ 1;
   const int nTiles = (n + TB - 1) / TB;
-  int tile = blockIdx.x;
+  int tile = tileStart();
   int kb[LPR], ke[LPR];
   SpRow<T> me;
   tileBounds<LPR, TB>(n, tile * TB, ia, kb, ke);
@@ -1314,7 +1324,7 @@ void launchTileTB(fe_hip_ctx *ctx, Workspace &w, TileVecs<T> &v, const T *val, i
 {
   const int n = ctx->n;
   const int nTiles = (n + TB - 1) / TB;
-  const int G = std::max(1, std::min(ctx->iterGrid, nTiles));
+  const int G = feIterGrid(ctx, nTiles);
 #define FE_TILE_CASE(L)                                                                                            \
   if(init)                                                                                                         \
     hipLaunchKernelGGL((tileInit<T, L, TB>), dim3(G), dim3(TB), 0, ctx->stream, n, ctx->d_ia, ctx->d_ja, val, b, v, \
@@ -1351,7 +1361,7 @@ int runSplitPipe(fe_hip_ctx *ctx, Workspace &w, SpVecs<T> &v, const T *val, cons
   const int n = ctx->n;
   const FePrecond &P = ctx->pc;
   const int TB = ctx->iterBlock;
-  const int G = std::max(1, std::min(ctx->iterGrid, (n + TB - 1) / TB));
+  const int G = feIterGrid(ctx, (n + TB - 1) / TB);
   const int L = ctx->spmvLanes, LF = std::max(1, L / 2);
   CgState *hs = (CgState *)ctx->h_state;
   FeKernelTimer &tm = ctx->spmvTimer;

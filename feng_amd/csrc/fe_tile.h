@@ -9,6 +9,16 @@
 
 namespace fe_tile {
 
+// First row tile of this block: when the grid is a multiple of 8 the 8 XCDs
+// (block b runs on XCD b % 8) take 8 contiguous ranges of tiles, so gathers
+// of neighbouring rows stay in one XCD's L2; tiles then advance by gridDim.
+__device__ __forceinline__ int tileStart()
+{
+  const int G = gridDim.x;
+  if(G & 7) return blockIdx.x;
+  return (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
+}
+
 template <typename T>
 struct TileVecs {
   T *x, *r, *w, *s, *z, *p, *dinv, *m[2];
