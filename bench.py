@@ -43,9 +43,12 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernels from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (scaled by the same counters on a plain copy of known size).
-TRAFFIC = {  # profiles/cube64_learned64_hbm_traffic.md (cube 64^3, learned FP64 path)
+TRAFFIC = {  # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE (cube 64^3):
+    # profiles/cube64_learned64_hbm_traffic.md (learned FP64 path),
+    # profiles/cube64_amg_hbm_traffic.md (multigrid FP64 path)
     "spIter<double> (split PCG, A product + updates)": 79.6e6,
     "tileIter<double>": 72.8e6,
+    "spIter<double> (AMG PCG, A product + updates)": 76.6e6,
 }
 
 

@@ -941,6 +941,11 @@ int feAmgApply(fe_hip_ctx *ctx, const float *in, float *out, const int *done, bo
   return amgApplyT<float, float>(ctx, in, out, done, 2);
 }
 
+int feAmgPlanFrom(const fe_hip_ctx *ctx, int slot)
+{
+  return ctx->amg ? ctx->amg->plan[slot].from : -1;
+}
+
 int feAmgLevels(const fe_hip_ctx *ctx, int *levels, int *rows, int64_t *nnz, int cap)
 {
   if(!ctx->amg) {

@@ -189,6 +189,8 @@ int fe_hip_ctx_destroy(fe_hip_ctx *ctx)
   feBatchRelease(ctx);
   feAmgRelease(ctx);
   for(void *p : ctx->owned) (void)hipFree(p);
+  for(auto &g : ctx->graphs) (void)hipGraphExecDestroy((hipGraphExec_t)g.second);
+  ctx->graphs.clear();
   if(ctx->h_state) (void)hipHostFree(ctx->h_state);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -477,6 +479,9 @@ int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value)
   } else if(k == "xcd_tiles") {
     FE_REQUIRE(ctx, value == 0 || value == 1, "xcd_tiles must be 0 or 1");
     ctx->xcdTiles = value;
+  } else if(k == "graph_iters") {
+    FE_REQUIRE(ctx, value >= 0 && value <= 32, "graph_iters must be in [0, 32]");
+    ctx->graphIters = value;
   } else if(k == "amg_lanes") {
     FE_REQUIRE(ctx, value == 2 || value == 4 || value == 8, "amg_lanes must be 2, 4 or 8");
     ctx->amgLanes = value;
@@ -530,6 +535,7 @@ int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value)
   else if(k == "amg_xcd") *value = ctx->amgXcd;
   else if(k == "xcd_tiles") *value = ctx->xcdTiles;
   else if(k == "amg_lanes") *value = ctx->amgLanes;
+  else if(k == "graph_iters") *value = ctx->graphIters;
   else FE_REQUIRE(ctx, false, "unknown tuning key " + k);
   return FE_STATUS_OK;
 }

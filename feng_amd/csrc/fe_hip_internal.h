@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -180,6 +181,10 @@ struct fe_hip_ctx {
   size_t workBytes = 0;
   void *d_state = nullptr;
   void *h_state = nullptr;  // pinned mirror
+  void *h_iterParams = nullptr;  // pinned: iteration parameters of graph replays
+  int graphIters = 0;  // iterations per captured graph in the split/multigrid PCG (0: direct launches; graphs
+                       // measured no faster here: the iteration kernels, not the host, set the pace)
+  std::map<std::string, void *> graphs;  // instantiated iteration graphs (hipGraphExec_t)
 
   FePrecond pc;
 
@@ -240,6 +245,9 @@ int feAmgCheck(fe_hip_ctx *ctx);
 int feAmgApply(fe_hip_ctx *ctx, const double *in, double *out, const int *done, bool values32);
 int feAmgApply(fe_hip_ctx *ctx, const float *in, float *out, const int *done, bool values32);
 int feAmgLevels(const fe_hip_ctx *ctx, int *levels, int *rows, int64_t *nnz, int cap);
+// First level of the single-workgroup kernel for a (vector, value) type slot
+// (0 f64/f64, 1 f64/f32, 2 f32/f32): part of the launch sequence a graph captures.
+int feAmgPlanFrom(const fe_hip_ctx *ctx, int slot);
 void feAmgRelease(fe_hip_ctx *ctx);
 void feBatchRelease(fe_hip_ctx *ctx);
 

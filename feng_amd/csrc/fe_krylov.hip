@@ -55,6 +55,20 @@ int feNextBatch(const CgState *hs, int it, int last, int checkEvery, double tol2
   return std::max(1, std::min(checkEvery, (int)std::ceil(need) + 1));
 }
 
+// Iteration index base, budget and stop threshold of a graph-replayed
+// iteration batch: the captured kernels carry only their offset in the batch;
+// the last node of the graph advances the base.
+struct IterParams {
+  int itBase;
+  int maxit;
+  double tol2;
+};
+
+__global__ void iterAdvance(IterParams *ip, int k)
+{
+  if(blockIdx.x == 0 && threadIdx.x == 0) ip->itBase += k;
+}
+
 constexpr int kTB = 256;       // block size of the one-off kernels
 constexpr int kMaxGrid = 4096; // partial buffers hold this many blocks
 
@@ -825,9 +839,15 @@ __device__ __forceinline__ void spLoadRow(const SpVecs<T> &v, int row, int n, Sp
 template <typename T, int LPR, int TB>
 __global__ __launch_bounds__(TB) void spIter(int n, const int *__restrict__ ia, const int *__restrict__ ja,
                                              const T *__restrict__ val, SpVecs<T> v, PipeParts P, PipeScalars *sc,
-                                             CgState *st, int it, int maxit, double tol2)
+                                             CgState *st, int it, int maxit, double tol2, const IterParams *ip)
 {
   __shared__ T ldsY[TB];
+  if(ip) {
+    // graph replay: it is the offset in the captured batch
+    it += ip->itBase;
+    maxit = ip->maxit;
+    tol2 = ip->tol2;
+  }
   const int c = it & 1, o = c ^ 1;
   const int nTiles = (n + TB - 1) / TB;
   int tile = tileStart();
@@ -1014,6 +1034,7 @@ struct Workspace {
   CgParts parts;
   double *x64, *r64, *tmp64, *res, *part;
   CgState *st;
+  IterParams *ip;
 };
 
 int ensureWorkspace(fe_hip_ctx *ctx, Workspace &w)
@@ -1035,6 +1056,7 @@ int ensureWorkspace(fe_hip_ctx *ctx, Workspace &w)
     return q;
   };
   w.st = (CgState *)take(sizeof(CgState));
+  w.ip = (IterParams *)take(sizeof(IterParams));
   w.res = (double *)take(64 * 8);
   w.parts.rz[0] = (double *)take(kMaxGrid * 8);
   w.parts.rz[1] = (double *)take(kMaxGrid * 8);
@@ -1095,6 +1117,9 @@ int ensureWorkspace(fe_hip_ctx *ctx, Workspace &w)
   w.r64 = (double *)take(n * 8);
   w.tmp64 = (double *)take(n * 8);
   if(!ctx->h_state) FE_HIP_TRY(ctx, hipHostMalloc(&ctx->h_state, 4096));
+  // second half of the pinned block: the iteration parameters of graph replays
+  static_assert(sizeof(CgState) <= 2048, "pinned state block layout");
+  ctx->h_iterParams = (char *)ctx->h_state + 2048;
   return FE_STATUS_OK;
 }
 
@@ -1356,7 +1381,8 @@ void launchTile(fe_hip_ctx *ctx, Workspace &w, TileVecs<T> &v, const T *val, int
 // K1, K2, K3 (the A product K3 is the timed kernel).
 template <typename T>
 int runSplitPipe(fe_hip_ctx *ctx, Workspace &w, SpVecs<T> &v, const T *val, const T *gL, const T *gU,
-                 const T *dinv, const double *b, int checkEvery, int maxit, double tol2, int *iters, int *done)
+                 const T *dinv, const double *b, int checkEvery, int maxit, double tol2, int *iters, int *done,
+                 int firstBatch = 0)
 {
   const int n = ctx->n;
   const FePrecond &P = ctx->pc;
@@ -1403,9 +1429,70 @@ int runSplitPipe(fe_hip_ctx *ctx, Workspace &w, SpVecs<T> &v, const T *val, cons
                  0);
   }
   FE_SP_LAUNCH(spInitW, L, n, ctx->d_ia, ctx->d_ja, val, v, w.pparts, w.sc, w.st);
-  int it = 0, batch = checkEvery;
+  // m = M^-1 w, skipped once the iteration has stopped
+  auto applyPc = [&]() -> int {
+    if(amg) return feAmgApply(ctx, (const T *)v.w, v.m, dDone, ctx->pcValues32 != 0);
+    if(v.m32) {
+      FE_SP_LAUNCH_T(spSpmv, float, LF, n, P.d_iaL, P.d_jaL, (const float *)P.d_gL32, (const float *)v.w32,
+                     (const float *)P.d_dinv32, v.t32, (const CgState *)w.st, 1);
+      FE_SP_LAUNCH_T(spSpmv, float, LF, n, P.d_iaU, P.d_jaU, (const float *)P.d_gU32, (const float *)v.t32,
+                     (const float *)nullptr, v.m32, (const CgState *)w.st, 1);
+    } else if(sizeof(T) == 8 && ctx->pcValues32) {
+      FE_SP_LAUNCH(spSpmvF32, LF, n, P.d_iaL, P.d_jaL, (const float *)P.d_gL32, (const T *)v.w,
+                   (const float *)P.d_dinv32, v.t, (const CgState *)w.st, 1);
+      FE_SP_LAUNCH(spSpmvF32, LF, n, P.d_iaU, P.d_jaU, (const float *)P.d_gU32, (const T *)v.t, (const float *)nullptr,
+                   v.m, (const CgState *)w.st, 1);
+    } else {
+      FE_SP_LAUNCH(spSpmv, LF, n, P.d_iaL, P.d_jaL, gL, (const T *)v.w, dinv, v.t, (const CgState *)w.st, 1);
+      FE_SP_LAUNCH(spSpmv, LF, n, P.d_iaU, P.d_jaU, gU, (const T *)v.t, (const T *)nullptr, v.m,
+                   (const CgState *)w.st, 1);
+    }
+    return FE_STATUS_OK;
+  };
+  // Graph replay of K iterations (preconditioner + A product each) when not
+  // profiling: one hipGraphLaunch instead of 3 to 8 launches per iteration,
+  // so short kernels are not starved by the host's launch rate.
+  const int K = ctx->profiling ? 0 : ctx->graphIters;
+  hipGraphExec_t gx = nullptr;
+  if(K > 0) {
+    IterParams *hp = (IterParams *)ctx->h_iterParams;
+    hp->itBase = 0;
+    hp->maxit = maxit;
+    hp->tol2 = tol2;
+    FE_HIP_TRY(ctx, hipMemcpyAsync(w.ip, hp, sizeof(IterParams), hipMemcpyHostToDevice, ctx->stream));
+    char key[256];
+    snprintf(key, sizeof(key), "sp%zu a%d v%d m%d L%d TB%d G%d K%d p%d s%p x%p", sizeof(T), (int)amg, ctx->pcValues32,
+             v.m32 ? 1 : 0, L, TB, G, K, amg ? feAmgPlanFrom(ctx, sizeof(T) == 8 ? (ctx->pcValues32 ? 1 : 0) : 2) : -1,
+             (void *)w.st, (void *)v.x);
+    auto f = ctx->graphs.find(key);
+    if(f != ctx->graphs.end()) {
+      gx = (hipGraphExec_t)f->second;
+    } else {
+      hipGraph_t g = nullptr;
+      FE_HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+      int rc = FE_STATUS_OK;
+      for(int k = 0; k < K && rc == FE_STATUS_OK; ++k) {
+        rc = applyPc();
+        FE_SP_LAUNCH(spIter, L, n, ctx->d_ia, ctx->d_ja, val, v, w.pparts, w.sc, w.st, k, maxit, tol2,
+                     (const IterParams *)w.ip);
+      }
+      hipLaunchKernelGGL(iterAdvance, dim3(1), dim3(64), 0, ctx->stream, w.ip, K);
+      const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+      if(rc != FE_STATUS_OK) return rc;
+      FE_HIP_TRY(ctx, ec);
+      FE_HIP_TRY(ctx, hipGraphInstantiate(&gx, g, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(g);
+      ctx->graphs[key] = (void *)gx;
+    }
+  }
+  int it = 0, batch = firstBatch > 0 ? std::min(firstBatch, checkEvery) : checkEvery;
   for(;;) {
-    for(int k = 0; k < batch; ++k, ++it) {
+    if(gx) {
+      const int reps = (batch + K - 1) / K;
+      for(int r = 0; r < reps; ++r) FE_HIP_TRY(ctx, hipGraphLaunch(gx, ctx->stream));
+      it += reps * K;
+    }
+    for(int k = 0; !gx && k < batch; ++k, ++it) {
       if(amg) {
         if(int rc = feAmgApply(ctx, (const T *)v.w, v.m, dDone, ctx->pcValues32 != 0)) return rc;
       } else if(v.m32) {
@@ -1434,7 +1521,8 @@ int runSplitPipe(fe_hip_ctx *ctx, Workspace &w, SpVecs<T> &v, const T *val, cons
           }
         FE_HIP_TRY(ctx, hipEventRecord(tm.ev[tm.used++], ctx->stream));
       }
-      FE_SP_LAUNCH(spIter, L, n, ctx->d_ia, ctx->d_ja, val, v, w.pparts, w.sc, w.st, it, maxit, tol2);
+      FE_SP_LAUNCH(spIter, L, n, ctx->d_ia, ctx->d_ja, val, v, w.pparts, w.sc, w.st, it, maxit, tol2,
+                   (const IterParams *)nullptr);
       if(ctx->profiling) FE_HIP_TRY(ctx, hipEventRecord(tm.ev[tm.used++], ctx->stream));
     }
     FE_HIP_TRY(ctx, hipGetLastError());
@@ -1464,13 +1552,13 @@ int runSplitPipe(fe_hip_ctx *ctx, Workspace &w, SpVecs<T> &v, const T *val, cons
 // key) until the stop test fires.
 template <typename T>
 int runJacobi(fe_hip_ctx *ctx, bool tiled, Workspace &w, PipeVecs<T> &pv, TileVecs<T> &tv, const T *val,
-              int checkEvery, int maxit, double tol2, int *iters, int *done)
+              int checkEvery, int maxit, double tol2, int *iters, int *done, int firstBatch = 0)
 {
   if(!tiled) return runPipe<T>(ctx, w, pv, val, checkEvery, maxit, tol2, iters, done);
   CgState *hs = (CgState *)ctx->h_state;
   FeKernelTimer &tm = ctx->spmvTimer;
   tm.used = 0;
-  int it = 0, batch = checkEvery;
+  int it = 0, batch = firstBatch > 0 ? std::min(firstBatch, checkEvery) : checkEvery;
   for(;;) {
     for(int k = 0; k < batch; ++k, ++it) {
       if(ctx->profiling) {
@@ -1681,6 +1769,7 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
     FE_HIP_TRY(ctx, hipMemcpyAsync(w.r64, ctx->d_rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
     double rn = bn;
     bool inner32 = fp32;
+    int lastInner = 0;  // iterations of the previous inner solve: first check batch of the next
     while(outer < 60) {
       relres = rn / bn;
       if(rn <= target) { converged = 1; reason = 2; break; }
@@ -1693,7 +1782,7 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
       if(inner32) {
         if(tiled32 || amg) {
           if(int rc = runSplitPipe<float>(ctx, w, w.sp32, ctx->d_val32, Pc.d_gL32, Pc.d_gU32, Pc.d_dinv32, w.r64,
-                                          checkEvery, o->max_iter - iters, tol2, &it0, &done))
+                                          checkEvery, o->max_iter - iters, tol2, &it0, &done, lastInner > 0 ? lastInner + 2 : 0))
             return rc;
         } else if(int rc = runSplit<float>(ctx, w, w.s32, ctx->d_val32, w.r64, checkEvery, o->max_iter - iters, tol2,
                                            &it0, &done)) {
@@ -1708,7 +1797,7 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
           w.sp64.t32 = mixed ? w.spShadow[1] : nullptr;
           w.sp64.m32 = mixed ? w.spShadow[2] : nullptr;
           if(int rc = runSplitPipe<double>(ctx, w, w.sp64, ctx->d_val, Pc.d_gL64, Pc.d_gU64, Pc.d_dinv64, w.r64,
-                                           checkEvery, o->max_iter - iters, tol2, &it0, &done))
+                                           checkEvery, o->max_iter - iters, tol2, &it0, &done, lastInner > 0 ? lastInner + 2 : 0))
             return rc;
         } else if(int rc = runSplit<double>(ctx, w, w.s64, ctx->d_val, w.r64, checkEvery, o->max_iter - iters, tol2,
                                             &it0, &done)) {
@@ -1718,6 +1807,7 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
                            (tiled64 || amg) ? w.sp64.x : w.s64.x);
       }
       iters += it0;
+      lastInner = it0;
       double rr = 0.;
       if(int rc = residualNorm2(ctx, w, w.x64, ctx->d_rhs, w.r64, &rr)) return rc;
       outer++;

@@ -103,3 +103,27 @@ def test_amg_pcg_converges(oracle, case):
     res = rhs - oracle.spmv(ia, ja, val, du)
     assert np.linalg.norm(res) <= 1.01e-8 * np.linalg.norm(rhs)
     S.close()
+
+
+@pytest.mark.parametrize("pcname", ["amg", "ic0"])
+def test_graph_replay_matches_direct_launches(pcname):
+    """Iteration batches replayed from a captured HIP graph give the same
+    iterates as direct launches, bit for bit, on every replayed solve (eager
+    solves and host synchronisations in between)."""
+    from feng_amd.api import PC_AMG, PC_IC0, PREC_FP32_IR
+    pc = {"amg": PC_AMG, "ic0": PC_IC0}[pcname]
+    m, S, num = _system("tet_grf_12")
+    S.assemble(3, PREC_FP32_IR)
+    for prec in (0, PREC_FP32_IR):
+        S.set_tuning("graph_iters", 0)
+        r0, d0 = S.solve(pc=pc, precision=prec, rtol=1e-8, want_du=True)
+        assert r0.converged
+        for rep in range(3):
+            S.set_tuning("graph_iters", 8)
+            r1, d1 = S.solve(pc=pc, precision=prec, rtol=1e-8, want_du=True)
+            assert r1.iterations == r0.iterations and r1.outer_iterations == r0.outer_iterations, (prec, rep)
+            assert np.array_equal(d1, d0), (prec, rep)
+            S.set_tuning("graph_iters", 0)
+            r2, d2 = S.solve(pc=pc, precision=prec, rtol=1e-8, want_du=True)
+            assert np.array_equal(d2, d0), (prec, rep)
+    S.close()
