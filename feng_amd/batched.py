@@ -100,7 +100,7 @@ def build_batch(problems, kind, device=0, keep_meshes=False) -> Batch:
 
 
 def solve_batch(b: Batch, precision=A.PREC_FP32_IR, rtol=1e-8, inner_rtol=1e-2, max_iter=100000, autotune=True,
-                check=True):
+                check=True, pc=A.PC_JACOBI):
     """Assemble every problem (zero + all element kernels + scatter) and solve
     them all; returns timings and per-problem statistics."""
     L, S = b.S.L, b.S
@@ -111,7 +111,7 @@ def solve_batch(b: Batch, precision=A.PREC_FP32_IR, rtol=1e-8, inner_rtol=1e-2, 
     S.assemble(3, precision)
     asm_ms = S.kernel_stats(1)[0]
     tuned = None
-    if autotune:
+    if autotune and pc == A.PC_JACOBI:
         lanes, block, ms = C.c_int(), C.c_int(), C.c_double()
         S._c(L.fe_hip_autotune_batched(S.ctx, nprob, N.ptr(rows, C.c_int), precision, C.byref(lanes),
                                        C.byref(block), C.byref(ms)), "fe_hip_autotune_batched")
@@ -119,6 +119,7 @@ def solve_batch(b: Batch, precision=A.PREC_FP32_IR, rtol=1e-8, inner_rtol=1e-2, 
     o = N.FeHipSolverOpts()
     L.fe_hip_default_opts(C.byref(o))
     o.precision, o.rtol, o.atol, o.inner_rtol, o.max_iter = precision, rtol, 1e-300, inner_rtol, max_iter
+    o.pc = pc
     st = (N.FeHipBatchStats * nprob)()
     ms = C.c_double()
     S._c(L.fe_hip_solve_batched(S.ctx, nprob, N.ptr(rows, C.c_int), C.byref(o), st, C.byref(ms)),

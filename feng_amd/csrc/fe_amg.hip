@@ -39,6 +39,8 @@ namespace {
 
 constexpr int kAmgDense = 64;       // the coarsest level is inverted densely up to this size
 constexpr int kAmgMaxLevels = 20;
+constexpr double kAmgStall = 0.9;   // coarsening stops when a level keeps more than this fraction of rows
+constexpr int kAmgCoarseSweeps = 4; // l1-Jacobi sweeps of a stalled coarsest level larger than kAmgDense
 constexpr int kAmgTileMax = 2048;   // rows of one aggregate tile (LDS)
 constexpr int kAmgTileRows = 256;   // target rows per aggregate tile
 constexpr int kAmgTB = 256;         // block of the level kernels
@@ -58,6 +60,8 @@ struct AmgBlockPlan {
   int oMptr[kAmgBlockLev], oMlist[kAmgBlockLev], oU[kAmgBlockLev], oR[kAmgBlockLev];
   int oB[kAmgBlockLev + 1];
   int oCx = 0, bytes = 0;
+  int coarseSweeps = 0;  // 0: dense inverse of the coarsest level; > 0: l1-Jacobi sweeps (global memory)
+  float coarseScale = 1.f;  // 1 / c: turns the stored l1 = c / sum |a_ij| into 1 / sum |a_ij|
 };
 
 // Tile of this block: with xcd set and a grid that is a multiple of 8, the
@@ -108,6 +112,7 @@ struct FeAmgLevel {
 struct FeAmg {
   std::vector<FeAmgLevel> lv;
   AmgBlockPlan plan[3];  // per (vector, value) type combination of feAmgApply
+  int coarseSweeps = 0;  // stalled coarsening: the coarsest level is swept, not inverted
   double *cinv64 = nullptr;
   float *cinv32 = nullptr;
   void *desc[3] = {nullptr, nullptr, nullptr};  // AmgLev<double,double>, <double,float>, <float,float>
@@ -538,13 +543,35 @@ __global__ __launch_bounds__(kAmgBlockTB) void amgBlock(const AmgLev<T, TV> *__r
   }
   {
     const int nC = pl.n[nLev];
-    const T *ci = cinv;  // global: row i of the inverse per thread
     const T *b = (const T *)(lds + pl.oB[nLev]);
     T *x = nLev == 0 ? (pl.from == 0 ? x0 : lv[pl.from].x) : (T *)(lds + pl.oCx);
-    for(int i = t; i < nC; i += nt) {
-      T acc = (T)0;
-      for(int j = 0; j < nC; ++j) acc += ci[i * nC + j] * b[j];
-      x[i] = acc;
+    if(pl.coarseSweeps == 0) {
+      const T *ci = cinv;  // global: row i of the inverse per thread
+      for(int i = t; i < nC; i += nt) {
+        T acc = (T)0;
+        for(int j = 0; j < nC; ++j) acc += ci[i * nC + j] * b[j];
+        x[i] = acc;
+      }
+    } else {
+      // stalled coarsening: l1-Jacobi sweeps from zero on the coarsest CSR
+      // (global memory; its u and r vectors as the two iterates)
+      const AmgLev<T, TV> C = lv[pl.coarsest];
+      const T sc = (T)pl.coarseScale;
+      T *xa = C.u, *xb = C.r;
+      for(int i = t; i < nC; i += nt) xa[i] = sc * (T)C.l1[i] * b[i];
+      __syncthreads();
+      for(int sw = 1; sw < pl.coarseSweeps; ++sw) {
+        for(int i = t; i < nC; i += nt) {
+          T acc = (T)0;
+          for(int k = C.ia[i]; k < C.ia[i + 1]; ++k) acc += (T)C.val[k] * xa[C.ja[k]];
+          xb[i] = xa[i] + sc * (T)C.l1[i] * (b[i] - acc);
+        }
+        __syncthreads();
+        T *tmp = xa;
+        xa = xb;
+        xb = tmp;
+      }
+      for(int i = t; i < nC; i += nt) x[i] = xa[i];
     }
     __syncthreads();
   }
@@ -644,10 +671,18 @@ int feAmgSetup(fe_hip_ctx *ctx)
   FE_HIP_TRY(ctx, hipMemcpy(f.ja.data(), ctx->d_ja, sizeof(int) * f.ja.size(), hipMemcpyDeviceToHost));
   std::vector<HostLevel> H;
   H.push_back(std::move(f));
-  while((int)H.size() < kAmgMaxLevels && H.back().n > kAmgDense) {
+  int coarseSweeps = 0;
+  while(H.back().n > kAmgDense) {
     HostLevel &c = H.back();
     c.nc = amgAggregate(c.n, c.ia, c.ja, c.agg);
-    FE_REQUIRE(ctx, c.nc < c.n, "AMG: the matrix graph has no edges to aggregate");
+    if((int)H.size() >= kAmgMaxLevels || c.nc > kAmgStall * c.n) {
+      // coarsening has stalled (e.g. every problem of a block-diagonal system
+      // is down to a node or two): this level is the coarsest, swept
+      c.nc = 0;
+      c.agg.clear();
+      coarseSweeps = kAmgCoarseSweeps;
+      break;
+    }
     c.mptr.assign(c.nc + 1, 0);
     for(int i = 0; i < c.n; ++i) c.mptr[c.agg[i] + 1]++;
     for(int I = 0; I < c.nc; ++I) c.mptr[I + 1] += c.mptr[I];
@@ -678,10 +713,9 @@ int feAmgSetup(fe_hip_ctx *ctx)
     amgGalerkinPattern(c, nx.ia, nx.ja, nx.cptr, nx.clist);
     H.push_back(std::move(nx));
   }
-  FE_REQUIRE(ctx, H.back().n <= kAmgDense, "AMG: too many levels");
-
   FeAmg *A = new FeAmg();
   ctx->amg = A;
+  A->coarseSweeps = coarseSweeps;
   A->lv.resize(H.size());
   for(size_t l = 0; l < H.size(); ++l) {
     HostLevel &h = H[l];
@@ -758,9 +792,9 @@ int feAmgSetup(fe_hip_ctx *ctx)
     if(int rc = feAlloc(ctx, &d.u32, d.n)) return rc;
     if(int rc = feAlloc(ctx, &d.r32, d.n)) return rc;
   }
-  const int nC = A->lv.back().n;
-  if(int rc = feAlloc(ctx, &A->cinv64, (size_t)nC * nC)) return rc;
-  if(int rc = feAlloc(ctx, &A->cinv32, (size_t)nC * nC)) return rc;
+  const size_t nC = coarseSweeps ? 1 : (size_t)A->lv.back().n;
+  if(int rc = feAlloc(ctx, &A->cinv64, nC * nC)) return rc;
+  if(int rc = feAlloc(ctx, &A->cinv32, nC * nC)) return rc;
   if(int rc = feAlloc(ctx, &A->flag, 1)) return rc;
   if(int rc = amgUploadDesc<double, double>(ctx, *A, 0)) return rc;
   if(int rc = amgUploadDesc<double, float>(ctx, *A, 1)) return rc;
@@ -820,6 +854,8 @@ AmgBlockPlan amgPlan(const FeAmg &A, int blockRows)
     pl.oB[Lc - from] = take(sizeof(T) * nC);
     pl.oCx = take(sizeof(T) * nC);
     pl.bytes = bytes;
+    pl.coarseSweeps = A.coarseSweeps;
+    pl.coarseScale = (float)(1. / A.c);
     if(bytes <= kAmgBlockLds) return pl;
   }
   return pl;  // not reached: from = Lc fits
@@ -838,6 +874,8 @@ int feAmgNumeric(fe_hip_ctx *ctx)
   A.plan[0] = amgPlan<double, double>(A, ctx->amgBlockRows);
   A.plan[1] = amgPlan<double, float>(A, ctx->amgBlockRows);
   A.plan[2] = amgPlan<float, float>(A, ctx->amgBlockRows);
+  for(const AmgBlockPlan &pl : A.plan)
+    FE_REQUIRE(ctx, pl.bytes <= kAmgBlockLds, "AMG: the coarsest level does not fit the small-level kernel's LDS");
   FeAmgLevel &f = A.lv[0];
   f.val64 = ctx->d_val;
   {
@@ -855,8 +893,9 @@ int feAmgNumeric(fe_hip_ctx *ctx)
   }
   FE_HIP_TRY(ctx, hipMemsetAsync(A.flag, 0, sizeof(int), ctx->stream));
   const FeAmgLevel &C = A.lv.back();
-  hipLaunchKernelGGL(amgDenseInverse, dim3(1), dim3(1024), 0, ctx->stream, C.n, C.ia, C.ja, C.val64, A.cinv64,
-                     A.cinv32, A.flag);
+  if(!A.coarseSweeps)
+    hipLaunchKernelGGL(amgDenseInverse, dim3(1), dim3(1024), 0, ctx->stream, C.n, C.ia, C.ja, C.val64, A.cinv64,
+                       A.cinv32, A.flag);
   FE_HIP_TRY(ctx, hipGetLastError());
   A.valuesReady = true;
   return FE_STATUS_OK;

@@ -508,13 +508,57 @@ int residuals(fe_hip_ctx *ctx, FeBatch &B, const double *x, std::vector<double> 
   return FE_STATUS_OK;
 }
 
+// AMG-PCG on the union system.  The system is block diagonal, so the
+// aggregation never joins two problems and one V-cycle applies every
+// problem's own V-cycle in shared launches.  The CG scalars are global: the
+// solve runs until ||r|| <= min_p max(rtol ||b_p||, atol), which bounds every
+// problem's own residual, and the per-problem FP64 residuals are then
+// checked.
+int solveBatchedAmg(fe_hip_ctx *ctx, FeBatch &B, const fe_hip_solver_opts *o, fe_hip_batch_stats *stats,
+                    double *ms_solve)
+{
+  hipEvent_t t0, t1;
+  FE_HIP_TRY(ctx, hipEventCreate(&t0));
+  FE_HIP_TRY(ctx, hipEventCreate(&t1));
+  FE_HIP_TRY(ctx, hipEventRecord(t0, ctx->stream));
+  std::vector<double> bn, rn;
+  if(int rc = residuals(ctx, B, nullptr, bn)) return rc;
+  std::vector<double> target(B.nprob);
+  double tmin = 1e300;
+  for(int p = 0; p < B.nprob; ++p) {
+    target[p] = std::max(o->rtol * bn[p], o->atol);
+    if(bn[p] > 0.) tmin = std::min(tmin, target[p]);
+  }
+  fe_hip_solver_opts o2 = *o;
+  o2.rtol = 0.;
+  o2.atol = tmin < 1e300 ? 0.99 * tmin : o->atol;
+  fe_hip_solve_stats st{};
+  if(int rc = feSolve(ctx, &o2, nullptr, &st)) return rc;
+  if(int rc = residuals(ctx, B, ctx->d_du, rn)) return rc;
+  FE_HIP_TRY(ctx, hipEventRecord(t1, ctx->stream));
+  FE_HIP_TRY(ctx, hipEventSynchronize(t1));
+  float ms = 0.f;
+  FE_HIP_TRY(ctx, hipEventElapsedTime(&ms, t0, t1));
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  if(ms_solve) *ms_solve = ms;
+  if(stats)
+    for(int p = 0; p < B.nprob; ++p) {
+      stats[p].iterations = st.iterations;
+      stats[p].relres_fp64 = bn[p] > 0. ? rn[p] / bn[p] : 0.;
+      stats[p].converged = rn[p] <= target[p];
+    }
+  return FE_STATUS_OK;
+}
+
 }  // namespace
 
 extern "C" int fe_hip_solve_batched(fe_hip_ctx *ctx, int nprob, const int *rows, const fe_hip_solver_opts *o,
                                     fe_hip_batch_stats *stats, double *ms_solve)
 {
   if(!ctx || !rows || !o) return FE_STATUS_POINTER_ERROR;
-  FE_REQUIRE(ctx, o->ksp == FE_KSP_CG && o->pc == FE_PC_JACOBI, "the batched solver runs Jacobi-PCG");
+  FE_REQUIRE(ctx, o->ksp == FE_KSP_CG && (o->pc == FE_PC_JACOBI || o->pc == FE_PC_AMG),
+             "the batched solver runs Jacobi-PCG or AMG-PCG");
   const bool fp32 = (o->precision == FE_PREC_FP32_IR);
   FE_REQUIRE(ctx, !fp32 || ctx->d_val32, "assemble with FE_PREC_FP32_IR to create the FP32 values");
   FE_HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -522,6 +566,7 @@ extern "C" int fe_hip_solve_batched(fe_hip_ctx *ctx, int nprob, const int *rows,
   if(int rc = setupBatch(ctx, nprob, rows, Bp)) return rc;
   FeBatch &B = *Bp;
   if(int rc = buildTiles(ctx, B, ctx->iterBlock)) return rc;
+  if(o->pc == FE_PC_AMG) return solveBatchedAmg(ctx, B, o, stats, ms_solve);
   const int n = ctx->n;
   hipEvent_t t0, t1;
   FE_HIP_TRY(ctx, hipEventCreate(&t0));

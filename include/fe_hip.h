@@ -229,8 +229,12 @@ int fe_hip_amg_apply(fe_hip_ctx *ctx, const double *r, double *z);
  * problem gets its own Jacobi-PCG (own scalars, own stop on
  * ||b_p - A_p x_p||_2 <= max(rtol ||b_p||_2, atol), FP64), all problems
  * advancing in the same launches; FE_PREC_FP32_IR runs the inner solves on
- * the FP32 values with FP64 refinement.  The correction goes to the device
- * du (fe_hip_correct_solution applies it).  ms_solve: device time. */
+ * the FP32 values with FP64 refinement.  With pc = FE_PC_AMG the union system
+ * is solved by one AMG-PCG (the hierarchy of a block-diagonal matrix is block
+ * diagonal) to ||r||_2 <= min_p max(rtol ||b_p||_2, atol), which bounds every
+ * problem's own residual; the per-problem FP64 residuals are then reported.
+ * The correction goes to the device du (fe_hip_correct_solution applies it).
+ * ms_solve: device time. */
 typedef struct {
   int iterations;
   int converged;

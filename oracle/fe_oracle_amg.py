@@ -15,12 +15,15 @@ functions.  They are pinned by mathematics in tests/test_oracle_amg.py:
   - the two-level cycle equals the textbook error propagation
     E = (I - S A)(I - oc P A_c^-1 P^T A)(I - S A), M^-1 = (I - E) A^-1,
     formed densely; a single level is the exact inverse;
-  - the multilevel cycle is symmetric positive definite.
+  - the multilevel cycle is symmetric positive definite, also when coarsening
+    stalls on a block-diagonal matrix and the coarsest level is swept.
 """
 import numpy as np
 import scipy.sparse as sp
 
 DENSE = 64  # the coarsest level is inverted densely up to this many rows
+STALL = 0.9  # coarsening stops when a level keeps more than this fraction of its rows
+SWEEPS = 4   # l1-Jacobi sweeps that solve a stalled coarsest level
 MASK = (1 << 64) - 1
 
 
@@ -90,23 +93,39 @@ def hierarchy(A, c=1.8, oc=1.8, dense=DENSE, max_levels=20):
         L = Level()
         L.A = A
         L.l1 = l1_scale(A, c)
+        L.c = c
         levels.append(L)
-        if A.shape[0] <= dense or len(levels) >= max_levels:
+        if A.shape[0] <= dense:
             L.Ainv = np.linalg.inv(A.toarray())
             break
-        L.agg, L.nc = aggregate(A.indptr, A.indices)
-        assert L.nc < A.shape[0]
+        agg, nc = aggregate(A.indptr, A.indices)
+        if len(levels) >= max_levels or nc > STALL * A.shape[0]:
+            L.sweeps = SWEEPS  # stalled: the coarsest level is swept, not inverted
+            break
+        L.agg, L.nc = agg, nc
         A = galerkin(A, L.agg, L.nc)
     for L in levels:
         L.oc = oc
     return levels
 
 
+def coarse_solve(L, b):
+    """Dense inverse, or (stalled coarsening) SWEEPS l1-Jacobi sweeps from zero
+    with weights 1 / sum_j |a_ij|."""
+    if hasattr(L, "Ainv"):
+        return L.Ainv @ b
+    w = L.l1 / L.c
+    x = w * b
+    for _ in range(L.sweeps - 1):
+        x = x + w * (b - L.A @ x)
+    return x
+
+
 def vcycle(levels, b, l=0):
     """u = S b; b_c = P^T (b - A u); e = V(b_c); y = u + oc P e; x = y + S (b - A y)."""
     L = levels[l]
     if l == len(levels) - 1:
-        return L.Ainv @ b
+        return coarse_solve(L, b)
     u = L.l1 * b
     r = b - L.A @ u
     bc = np.bincount(L.agg, weights=r, minlength=L.nc)

@@ -127,3 +127,24 @@ def test_graph_replay_matches_direct_launches(pcname):
             r2, d2 = S.solve(pc=pc, precision=prec, rtol=1e-8, want_du=True)
             assert np.array_equal(d2, d0), (prec, rep)
     S.close()
+
+
+def test_vcycle_on_a_block_diagonal_batch_matches_oracle():
+    """120 disjoint small problems in one system: every level keeps at least
+    one row per problem, so coarsening stalls above the dense limit and the
+    coarsest level is swept; hierarchy and V-cycle as the oracle's."""
+    from feng_amd.batched import build_batch
+    probs = [(6 + (p % 5), 1, 100 + p) for p in range(120)]
+    b = build_batch([(N_, 1, s) for N_, _, s in probs], 1)
+    S = b.S
+    S.setToZero()
+    S.assemble(3)
+    A, rhs, lv = _oracle_levels(S)
+    assert hasattr(lv[-1], "sweeps")
+    r = np.random.default_rng(5).normal(size=A.shape[0])
+    S.set_tuning("pc_values32", 0)
+    z = S.amg_apply(r)
+    rows, nnz = S.amg_levels()
+    assert rows == [L.A.shape[0] for L in lv] and nnz == [L.A.nnz for L in lv]
+    assert nrel(z, amg.vcycle(lv, r)) <= 1e-11
+    S.close()

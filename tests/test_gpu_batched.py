@@ -56,3 +56,27 @@ def test_batch_matches_individual_systems(kind, precision):
             assert int(res["iterations"][p]) <= 1.1 * r.iterations + 5, (res["iterations"][p], r.iterations)
         S1.close()
     b.S.close()
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_batched_amg_matches_individual_systems(kind):
+    """AMG-PCG on the union system: every problem meets its own FP64 rel-res
+    target and matches its single-system solve."""
+    from feng_amd.api import PC_AMG, PREC_FP64
+    from feng_amd.batched import build_batch, solve_batch
+    probs = [(9, kind, 21), (17, kind, 22), (12, kind, 23), (24, kind, 24), (31, kind, 25)]
+    b = build_batch(probs, kind)
+    res = solve_batch(b, precision=PREC_FP64, rtol=1e-10, pc=PC_AMG)
+    assert res["converged"].all() and (res["relres"] <= 1e-10).all(), res["relres"]
+    b.S.correctSolution()
+    u = b.S.get_solution()
+    for p, (N_, _, seed) in enumerate(probs):
+        S1, num = _alone(N_, kind, seed)
+        S1.setToZero()
+        S1.assemble(3, PREC_FP64)
+        S1.solve(rtol=1e-10)
+        S1.correctSolution()
+        u1 = S1.get_solution()[:num.nInc]
+        assert relerr(u[b.dof_maps[p][:num.nInc]], u1) <= 1e-6
+        S1.close()
+    b.S.close()

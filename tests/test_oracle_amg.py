@@ -143,3 +143,24 @@ def test_amg_pcg_beats_jacobi_on_a_3d_problem(oracle):
     j = its(lambda r: r / K.diagonal())
     a = its(lambda r: amg.vcycle(lv, r))
     assert a * 3 < j, (a, j)
+
+
+def test_stalled_coarsening_on_block_diagonal_matrix_is_spd():
+    """Many small disjoint problems: coarsening stalls with one or two rows
+    per problem, above the dense limit; the swept coarsest level keeps the
+    cycle SPD and every block is preconditioned by its own cycle."""
+    blocks = [csr(N, seed) for N, seed in ((3, 1), (4, 2), (3, 3), (4, 4))] * 25
+    A = sp.block_diag(blocks).tocsr()
+    A.sort_indices()
+    lv = amg.hierarchy(A)
+    assert hasattr(lv[-1], "sweeps") and lv[-1].A.shape[0] > amg.DENSE
+    n = A.shape[0]
+    M = np.column_stack([amg.vcycle(lv, e) for e in np.eye(n)])
+    assert np.abs(M - M.T).max() <= 1e-12 * np.abs(M).max()
+    assert np.linalg.eigvalsh(0.5 * (M + M.T)).min() > 0
+    # block diagonal: no coupling between problems
+    r0 = 0
+    for B in blocks:
+        m = B.shape[0]
+        assert np.abs(M[r0:r0 + m, :r0]).max(initial=0) == 0 and np.abs(M[r0:r0 + m, r0 + m:]).max(initial=0) == 0
+        r0 += m
