@@ -467,6 +467,9 @@ int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value)
   } else if(k == "pc_fp32") {
     FE_REQUIRE(ctx, value == 0 || value == 1, "pc_fp32 must be 0 or 1");
     ctx->pcFp32 = value;
+  } else if(k == "ilu_sweeps") {
+    FE_REQUIRE(ctx, value >= 1 && value <= 64, "ilu_sweeps must be in [1, 64]");
+    ctx->pc.sweeps = value;
   } else if(k == "ic0_sweeps") {
     FE_REQUIRE(ctx, value >= 0 && value <= 64, "ic0_sweeps must be 0 (exact ILU(0)) or 1..64");
     ctx->icSweeps = value;
@@ -527,6 +530,7 @@ int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value)
   else if(k == "fp32_values") *value = ctx->fp32Values;
   else if(k == "pipe_kernel") *value = ctx->pipeKernel;
   else if(k == "ic0_sweeps") *value = ctx->icSweeps;
+  else if(k == "ilu_sweeps") *value = ctx->pc.sweeps;
   else if(k == "pc_fp32") *value = ctx->pcFp32;
   else if(k == "pc_values32") *value = ctx->pcValues32;
   else if(k == "amg_omega_x100") *value = (int)std::lround(ctx->amgOmega * 100.);

@@ -49,10 +49,12 @@ def diffusion_system(mesh: A.Mesh, kind: int, degree: int, k: A.Fn, s: A.Fn, u_b
     return S, num
 
 
-def solve_newton(S: A.LinearSystemHIP, opts: NLSolverOptions = NLSolverOptions(), **solve_kw) -> NewtonReport:
+def solve_newton(S: A.LinearSystemHIP, opts: NLSolverOptions = NLSolverOptions(), progress=None,
+                 **solve_kw) -> NewtonReport:
     """solveNewtonRaphson (feNonLinearSolver.cpp:64-142): residual, stopping
     test, matrix, solve, u += du, until ||NL(u)||_inf <= tolResidual or
-    ||du||_inf <= tolCorrection."""
+    ||du||_inf <= tolCorrection.  progress(iteration, SolveResult) is called
+    after every linear solve (the reference prints the same per-step norms)."""
     rep = NewtonReport()
     it = 0
     while True:
@@ -72,6 +74,8 @@ def solve_newton(S: A.LinearSystemHIP, opts: NLSolverOptions = NLSolverOptions()
             raise A.FeError(f"residual norm {r.norm_rhs_inf:.4e} above the divergence tolerance")
         S.correctSolution()
         it += 1
+        if progress is not None:
+            progress(it, r)
         rep.residual_norms.append(r.norm_rhs_inf)
         rep.correction_norms.append(r.norm_dx_inf)
         if r.norm_rhs_inf <= opts.tolResidual or r.norm_dx_inf <= opts.tolCorrection or it > opts.maxIter:
