@@ -117,35 +117,45 @@ def run_step(S, path, rtol, profile=False):
 
 
 def cpu_baseline(m, k, num, rtol, budget_s=25.0):
-    """Oracle (single-threaded C): assembly of the whole problem, then CG
-    iterations until the budget; DOF/s extrapolated with the oracle's own
-    iteration count when it converges, else with the measured per-iteration
-    time times the GPU plain path's iteration count."""
-    import ctypes as C
-
+    """Oracle (C) on the host cores: assembly of the whole problem, then
+    Jacobi-PCG until convergence or the budget; DOF/s extrapolated with the
+    oracle's own iteration count when it converges, else with the measured
+    per-iteration time times the GPU plain path's iteration count.  Timed
+    twice: single-threaded, and with the colored OpenMP assembly and threaded
+    CG on all cores of this process's affinity (at most 16)."""
     from tests.oracle_bind import load_oracle
 
     O = load_oracle()
     n = num.nInc
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
     t0 = time.perf_counter()
     ia, ja = O.csr(n, [(num.adr[0], 4, num.adr[0], 4)])
     t_csr = time.perf_counter() - t0
     t0 = time.perf_counter()
-    val, rhs = O.assemble_diffusion(4, m.xyz, m.conn, num.adr[0], n, ia, ja, 1, (4, 0.0, 0.0, k),
-                                    (0, -1.0, 0.0, None), np.zeros(num.nDofs))
-    t_asm = time.perf_counter() - t0
-    # time a bounded number of CG iterations
-    probe = 50
-    t0 = time.perf_counter()
-    O.cg_jacobi(ia, ja, val, rhs, 1e-30, probe)
-    t_it = (time.perf_counter() - t0) / probe
-    max_it = int(max(probe, (budget_s - t_asm) / max(t_it, 1e-9)))
-    t0 = time.perf_counter()
-    x, it, rr = O.cg_jacobi(ia, ja, val, rhs, rtol, max_it)
-    t_cg = time.perf_counter() - t0
-    converged = rr <= rtol
-    return dict(t_csr=t_csr, t_asm=t_asm, t_cg=t_cg, iters=it, converged=converged, t_iter=t_cg / max(it, 1),
-                relres=rr)
+    colorOf, ncolors = O.color(m.conn, 4, m.V)
+    t_color = time.perf_counter() - t0
+    args = (4, m.xyz, m.conn, num.adr[0], n, ia, ja, 1, (4, 0.0, 0.0, k), (0, -1.0, 0.0, None), np.zeros(num.nDofs))
+    out = {"t_csr": t_csr, "t_color": t_color, "threads": threads}
+    for tag, T in (("1", 1), ("all", threads)):
+        t0 = time.perf_counter()
+        if T == 1:
+            val, rhs = O.assemble_diffusion(*args)
+        else:
+            val, rhs = O.assemble_diffusion_omp(*args, colorOf, ncolors, T)
+        t_asm = time.perf_counter() - t0
+        probe = 20
+        t0 = time.perf_counter()
+        O.cg_jacobi_omp(ia, ja, val, rhs, 1e-30, probe, T)
+        t_it = (time.perf_counter() - t0) / probe
+        max_it = int(max(probe, (budget_s / 2 - t_asm) / max(t_it, 1e-9)))
+        t0 = time.perf_counter()
+        if T == 1:
+            x, it, rr = O.cg_jacobi(ia, ja, val, rhs, rtol, max_it)
+        else:
+            x, it, rr = O.cg_jacobi_omp(ia, ja, val, rhs, rtol, max_it, T)
+        t_cg = time.perf_counter() - t0
+        out[tag] = dict(t_asm=t_asm, t_cg=t_cg, iters=it, converged=rr <= rtol, t_iter=t_cg / max(it, 1), relres=rr)
+    return out
 
 
 def copy_probe(mib=512, reps=5):
@@ -268,12 +278,20 @@ def main():
     cpu = None
     if not args.no_cpu:
         c = cpu_baseline(m, k, num, rtol, args.cpu_budget)
-        iters = c["iters"] if c["converged"] else plain["last"]["iters"]
-        t_cpu = c["t_asm"] + c["t_iter"] * iters
-        cpu = {"value": n / t_cpu, "unit": "DOF/s", "cores": 1, "kind": "port",
-               "sample": (f"oracle assembly of all {m.E} tets ({c['t_asm']:.2f}s) + CG/Jacobi "
-                          f"{c['iters']} iterations ({c['t_cg']:.1f}s, {'converged' if c['converged'] else 'bounded, extrapolated to'} "
-                          f"{iters} iterations)")}
+
+        def dofs(r):
+            iters = r["iters"] if r["converged"] else plain["last"]["iters"]
+            return n / (r["t_asm"] + r["t_iter"] * iters), iters
+        v_all, it_all = dofs(c["all"])
+        v_1, _ = dofs(c["1"])
+        ca = c["all"]
+        cpu = {"value": v_all, "unit": "DOF/s", "cores": c["threads"], "kind": "port",
+               "value_1core": v_1,
+               "sample": (f"oracle on {c['threads']} threads: colored OpenMP assembly of all {m.E} tets "
+                          f"({ca['t_asm']:.3f}s; coloring {c['t_color']:.2f}s excluded like the CSR build) + threaded "
+                          f"CG/Jacobi {ca['iters']} iterations ({ca['t_cg']:.2f}s, "
+                          f"{'converged' if ca['converged'] else 'bounded, extrapolated to'} {it_all} iterations); "
+                          f"value_1core: the serial oracle on the same problem")}
         log("cpu", c)
 
     def summary(p):

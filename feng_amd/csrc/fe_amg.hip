@@ -60,6 +60,7 @@ struct AmgBlockPlan {
   int oMptr[kAmgBlockLev], oMlist[kAmgBlockLev], oU[kAmgBlockLev], oR[kAmgBlockLev];
   int oB[kAmgBlockLev + 1];
   int oCx = 0, bytes = 0;
+  int mglob[kAmgBlockLev];  // 1: the level's matrix and aggregates are read from global memory, not staged
   int coarseSweeps = 0;  // 0: dense inverse of the coarsest level; > 0: l1-Jacobi sweeps (global memory)
   float coarseScale = 1.f;  // 1 / c: turns the stored l1 = c / sum |a_ij| into 1 / sum |a_ij|
 };
@@ -477,8 +478,74 @@ __global__ __launch_bounds__(TB) void amgUp(AmgLev<T, TV> L, const T *__restrict
   }
 }
 
-// Levels from .. coarsest - 1 and the dense coarsest solve in one workgroup,
-// everything staged in LDS first (plan.bytes <= kAmgBlockLds); the phases are
+// One restriction step of the small-level kernel (all threads): u = l1 b,
+// r = b - A u on the rows, then bn[I] = sum of r over the members of I.
+template <typename T, typename TV, typename IA, typename IJ, typename IX>
+__device__ __forceinline__ void amgBlockDown(int n, int nc, const IA *ia, const IJ *ja, const TV *val, const TV *l1,
+                                             const IX *mptr, const IX *mlist, const T *b, T *u, T *r, T *bn)
+{
+  const int t = threadIdx.x, nt = blockDim.x;
+  for(int i = t; i < n; i += nt) {
+    T acc = (T)0;
+    const int k1 = ia[i + 1];
+    for(int k = ia[i]; k < k1; k += 4) {
+      int j[4];
+      T v[4];
+#pragma unroll
+      for(int q = 0; q < 4; ++q) {
+        j[q] = k + q < k1 ? (int)ja[k + q] : 0;
+        v[q] = k + q < k1 ? (T)val[k + q] : (T)0;
+      }
+#pragma unroll
+      for(int q = 0; q < 4; ++q) acc += v[q] * ((T)l1[j[q]] * b[j[q]]);
+    }
+    const T bi = b[i];
+    u[i] = (T)l1[i] * bi;
+    r[i] = bi - acc;
+  }
+  __syncthreads();
+  for(int I = t; I < nc; I += nt) {
+    T acc = (T)0;
+    for(int q = mptr[I]; q < mptr[I + 1]; ++q) acc += r[mlist[q]];
+    bn[I] = acc;
+  }
+  __syncthreads();
+}
+
+// One prolongation step (all threads): y = u + oc e[agg], x = y + l1 (b - A y);
+// x goes to xg (global) when given, else over b.
+template <typename T, typename TV, typename IA, typename IJ, typename IX>
+__device__ __forceinline__ void amgBlockUp(int n, const IA *ia, const IJ *ja, const TV *val, const TV *l1,
+                                           const IX *agg, T *b, const T *u, T *y, const T *e, T oc, T *xg)
+{
+  const int t = threadIdx.x, nt = blockDim.x;
+  for(int i = t; i < n; i += nt) y[i] = u[i] + oc * e[agg[i]];
+  __syncthreads();
+  for(int i = t; i < n; i += nt) {
+    T acc = (T)0;
+    const int k1 = ia[i + 1];
+    for(int k = ia[i]; k < k1; k += 4) {
+      int j[4];
+      T v[4];
+#pragma unroll
+      for(int q = 0; q < 4; ++q) {
+        j[q] = k + q < k1 ? (int)ja[k + q] : 0;
+        v[q] = k + q < k1 ? (T)val[k + q] : (T)0;
+      }
+#pragma unroll
+      for(int q = 0; q < 4; ++q) acc += v[q] * y[j[q]];
+    }
+    const T xi = y[i] + (T)l1[i] * (b[i] - acc);
+    if(xg) xg[i] = xi;
+    else b[i] = xi;
+  }
+  __syncthreads();
+}
+
+// Levels from .. coarsest - 1 and the coarsest solve in one workgroup.  The
+// vectors of every level live in LDS; a level's matrix, l1 and aggregates are
+// staged in LDS too (16-bit column ids) unless plan.mglob marks them to be
+// read from global memory (L2-resident: the larger levels).  Phases are
 // separated by workgroup barriers.  Level `from` reads b (b0 when from == 0)
 // and writes its result to x (x0 when from == 0).
 template <typename T, typename TV>
@@ -491,6 +558,7 @@ __global__ __launch_bounds__(kAmgBlockTB) void amgBlock(const AmgLev<T, TV> *__r
   const int t = threadIdx.x, nt = blockDim.x;
   const int nLev = pl.coarsest - pl.from;
   for(int s = 0; s < nLev; ++s) {
+    if(pl.mglob[s]) continue;
     const AmgLev<T, TV> L = lv[pl.from + s];
     const int n = pl.n[s], nnz = pl.nnz[s], nc = pl.nc[s];
     int *ia = (int *)(lds + pl.oIa[s]);
@@ -518,28 +586,18 @@ __global__ __launch_bounds__(kAmgBlockTB) void amgBlock(const AmgLev<T, TV> *__r
   __syncthreads();
   for(int s = 0; s < nLev; ++s) {
     const int n = pl.n[s], nc = pl.nc[s];
-    const int *ia = (const int *)(lds + pl.oIa[s]);
-    const unsigned short *ja = (const unsigned short *)(lds + pl.oJa[s]);
-    const TV *val = (const TV *)(lds + pl.oVal[s]), *l1 = (const TV *)(lds + pl.oL1[s]);
-    const unsigned short *mptr = (const unsigned short *)(lds + pl.oMptr[s]),
-                         *mlist = (const unsigned short *)(lds + pl.oMlist[s]);
     const T *b = (const T *)(lds + pl.oB[s]);
     T *u = (T *)(lds + pl.oU[s]), *r = (T *)(lds + pl.oR[s]);
-    for(int i = t; i < n; i += nt) {
-      T acc = (T)0;
-      for(int k = ia[i]; k < ia[i + 1]; ++k) acc += (T)val[k] * ((T)l1[ja[k]] * b[ja[k]]);
-      const T bi = b[i];
-      u[i] = (T)l1[i] * bi;
-      r[i] = bi - acc;
-    }
-    __syncthreads();
     T *bn = (T *)(lds + pl.oB[s + 1]);
-    for(int I = t; I < nc; I += nt) {
-      T acc = (T)0;
-      for(int q = mptr[I]; q < mptr[I + 1]; ++q) acc += r[mlist[q]];
-      bn[I] = acc;
+    if(pl.mglob[s]) {
+      const AmgLev<T, TV> L = lv[pl.from + s];
+      amgBlockDown<T, TV>(n, nc, L.ia, L.ja, L.val, L.l1, L.mptr, L.mlist, b, u, r, bn);
+    } else {
+      amgBlockDown<T, TV>(n, nc, (const int *)(lds + pl.oIa[s]), (const unsigned short *)(lds + pl.oJa[s]),
+                          (const TV *)(lds + pl.oVal[s]), (const TV *)(lds + pl.oL1[s]),
+                          (const unsigned short *)(lds + pl.oMptr[s]), (const unsigned short *)(lds + pl.oMlist[s]),
+                          b, u, r, bn);
     }
-    __syncthreads();
   }
   {
     const int nC = pl.n[nLev];
@@ -577,26 +635,20 @@ __global__ __launch_bounds__(kAmgBlockTB) void amgBlock(const AmgLev<T, TV> *__r
   }
   for(int s = nLev - 1; s >= 0; --s) {
     const int n = pl.n[s];
-    const int *ia = (const int *)(lds + pl.oIa[s]);
-    const unsigned short *ja = (const unsigned short *)(lds + pl.oJa[s]);
-    const TV *val = (const TV *)(lds + pl.oVal[s]), *l1 = (const TV *)(lds + pl.oL1[s]);
-    const unsigned short *agg = (const unsigned short *)(lds + pl.oAgg[s]);
     T *b = (T *)(lds + pl.oB[s]);
     const T *u = (const T *)(lds + pl.oU[s]);
     T *y = (T *)(lds + pl.oR[s]);
     // e = x of level s + 1: the coarsest solution, or level s + 1's b slot
     const T *e = (s + 1 == nLev) ? (const T *)(lds + pl.oCx) : (const T *)(lds + pl.oB[s + 1]);
-    for(int i = t; i < n; i += nt) y[i] = u[i] + oc * e[agg[i]];
-    __syncthreads();
     T *xg = s == 0 ? (pl.from == 0 ? x0 : lv[pl.from].x) : nullptr;
-    for(int i = t; i < n; i += nt) {
-      T acc = (T)0;
-      for(int k = ia[i]; k < ia[i + 1]; ++k) acc += (T)val[k] * y[ja[k]];
-      const T xi = y[i] + (T)l1[i] * (b[i] - acc);
-      if(xg) xg[i] = xi;
-      else b[i] = xi;
+    if(pl.mglob[s]) {
+      const AmgLev<T, TV> L = lv[pl.from + s];
+      amgBlockUp<T, TV>(n, L.ia, L.ja, L.val, L.l1, L.agg, b, u, y, e, oc, xg);
+    } else {
+      amgBlockUp<T, TV>(n, (const int *)(lds + pl.oIa[s]), (const unsigned short *)(lds + pl.oJa[s]),
+                        (const TV *)(lds + pl.oVal[s]), (const TV *)(lds + pl.oL1[s]),
+                        (const unsigned short *)(lds + pl.oAgg[s]), b, u, y, e, oc, xg);
     }
-    __syncthreads();
   }
 }
 
@@ -832,12 +884,31 @@ AmgBlockPlan amgPlan(const FeAmg &A, int blockRows)
     pl = AmgBlockPlan();
     pl.from = from;
     pl.coarsest = Lc;
+    // vectors of every level
     for(int l = from; l < Lc; ++l) {
       const int s = l - from;
       const FeAmgLevel &h = A.lv[l];
       pl.n[s] = h.n;
       pl.nc[s] = h.nc;
       pl.nnz[s] = (int)h.nnz;
+      pl.oB[s] = take(sizeof(T) * h.n);
+      pl.oU[s] = take(sizeof(T) * h.n);
+      pl.oR[s] = take(sizeof(T) * h.n);
+      pl.mglob[s] = 1;
+    }
+    const int nC = A.lv[Lc].n;
+    pl.n[Lc - from] = nC;
+    pl.oB[Lc - from] = take(sizeof(T) * nC);
+    pl.oCx = take(sizeof(T) * nC);
+    if(bytes > kAmgBlockLds) continue;
+    // stage matrices and aggregates from the coarsest level up while they fit
+    for(int l = Lc - 1; l >= from; --l) {
+      const int s = l - from;
+      const FeAmgLevel &h = A.lv[l];
+      const size_t need = ((sizeof(int) * (h.n + 1) + 15) & ~15) + ((2 * h.nnz + 15) & ~15) +
+                          ((sizeof(TV) * h.nnz + 15) & ~15) + ((sizeof(TV) * h.n + 15) & ~15) + ((2 * h.n + 15) & ~15) +
+                          ((2 * (h.nc + 1) + 15) & ~15) + ((2 * h.n + 15) & ~15);
+      if(bytes + (int)need > kAmgBlockLds) break;
       pl.oIa[s] = take(sizeof(int) * (h.n + 1));
       pl.oJa[s] = take(2 * h.nnz);
       pl.oVal[s] = take(sizeof(TV) * h.nnz);
@@ -845,18 +916,12 @@ AmgBlockPlan amgPlan(const FeAmg &A, int blockRows)
       pl.oAgg[s] = take(2 * h.n);
       pl.oMptr[s] = take(2 * (h.nc + 1));
       pl.oMlist[s] = take(2 * h.n);
-      pl.oB[s] = take(sizeof(T) * h.n);
-      pl.oU[s] = take(sizeof(T) * h.n);
-      pl.oR[s] = take(sizeof(T) * h.n);
+      pl.mglob[s] = 0;
     }
-    const int nC = A.lv[Lc].n;
-    pl.n[Lc - from] = nC;
-    pl.oB[Lc - from] = take(sizeof(T) * nC);
-    pl.oCx = take(sizeof(T) * nC);
     pl.bytes = bytes;
     pl.coarseSweeps = A.coarseSweeps;
     pl.coarseScale = (float)(1. / A.c);
-    if(bytes <= kAmgBlockLds) return pl;
+    return pl;
   }
   return pl;  // not reached: from = Lc fits
 }

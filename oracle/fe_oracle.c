@@ -676,6 +676,65 @@ int or_assemble_diffusion(int kind, int V, const double *xyz, int E, const int *
   return 0;
 }
 
+/* Threaded assembly in the reference's OpenMP form: colors one after the
+ * other, the elements of a color in parallel (they share no vertex, so no
+ * two threads add into the same row; feLinearSystemPETSc.cpp:543-707 runs
+ * the colored element loop under `omp parallel for`).  Each element adds its
+ * diffusion and source contributions in one visit. */
+int or_assemble_diffusion_omp(int kind, int V, const double *xyz, int E, const int *conn,
+                              const int *adr, int n, const int *ia, const int *ja,
+                              int quad_degree, or_fn coeff, or_fn source, const double *u,
+                              const int *colorOf, int ncolors, int threads, double *val, double *rhs)
+{
+  (void)V;
+  or_ref R;
+  if(ref_setup(kind, quad_degree, &R)) return -1;
+  const int nf = R.nf, nv = R.nv;
+  int *start = calloc((size_t)ncolors + 1, sizeof(int)), *elems = malloc(sizeof(int) * (size_t)E);
+  for(int e = 0; e < E; ++e) start[colorOf[e] + 1]++;
+  for(int c = 0; c < ncolors; ++c) start[c + 1] += start[c];
+  int *fill = malloc(sizeof(int) * (size_t)ncolors);
+  memcpy(fill, start, sizeof(int) * (size_t)ncolors);
+  for(int e = 0; e < E; ++e) elems[fill[colorOf[e]]++] = e;
+  free(fill);
+  int err = 0;
+#pragma omp parallel num_threads(threads) reduction(| : err)
+  {
+    double c[12], Ae[144], Be[12], ul[12];
+    for(int col = 0; col < ncolors; ++col) {
+#pragma omp for schedule(static)
+      for(int q = start[col]; q < start[col + 1]; ++q) {
+        const int e = elems[q];
+        for(int j = 0; j < nv; ++j)
+          for(int d = 0; d < 3; ++d) c[3 * j + d] = xyz[3 * conn[e * nv + j] + d];
+        const int *a = adr + (size_t)e * nf;
+        for(int i = 0; i < nf; ++i) ul[i] = u[a[i]];
+        memset(Ae, 0, sizeof(Ae));
+        memset(Be, 0, sizeof(Be));
+        or_geo g;
+        if(elem_geo(&R, c, &g)) {
+          err = 1;
+          continue;
+        }
+        diffusion_elem(&R, c, e, coeff, source, ul, Ae, Be);
+        for(int i = 0; i < nf; ++i) {
+          if(a[i] >= n) continue;
+          rhs[a[i]] += Be[i];
+          for(int j = 0; j < nf; ++j) {
+            if(a[j] >= n) continue;
+            const int p = find_pos(ia, ja, a[i], a[j]);
+            if(p < 0) err = 1;
+            else val[p] += Ae[i * nf + j];
+          }
+        }
+      }
+    }
+  }
+  free(start);
+  free(elems);
+  return err ? -1 : 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Taylor-Hood Navier-Stokes: filled in by or_ns.c-style helpers below. */
 
@@ -755,6 +814,70 @@ int or_cg_jacobi(int n, const int *ia, const int *ja, const double *val, const d
                  double *x, double rtol, int maxit, double *relres)
 {
   return pcg(n, ia, ja, val, NULL, b, x, rtol, maxit, relres);
+}
+
+/* The same Jacobi-PCG with every loop (SpMV, dots, updates) split over
+ * `threads` OpenMP threads. */
+int or_cg_jacobi_omp(int n, const int *ia, const int *ja, const double *val, const double *b,
+                     double *x, double rtol, int maxit, int threads, double *relres)
+{
+  double *r = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n);
+  double *p = malloc(sizeof(double) * n), *q = malloc(sizeof(double) * n);
+  double *dinv = malloc(sizeof(double) * n);
+  double bb = 0., rz = 0., rr = 0.;
+#pragma omp parallel for num_threads(threads) reduction(+ : bb, rz, rr)
+  for(int i = 0; i < n; ++i) {
+    double d = 0.;
+    for(int k = ia[i]; k < ia[i + 1]; ++k)
+      if(ja[k] == i) d = val[k];
+    dinv[i] = d != 0. ? 1. / d : 1.;
+    x[i] = 0.;
+    r[i] = b[i];
+    z[i] = dinv[i] * r[i];
+    p[i] = z[i];
+    bb += b[i] * b[i];
+    rz += r[i] * z[i];
+    rr += r[i] * r[i];
+  }
+  double bn = sqrt(bb);
+  if(bn == 0.) bn = 1.;
+  int it = 0;
+  while(sqrt(rr) > rtol * bn && it < maxit) {
+    double pq = 0.;
+#pragma omp parallel for num_threads(threads) reduction(+ : pq)
+    for(int i = 0; i < n; ++i) {
+      double s = 0.;
+      for(int k = ia[i]; k < ia[i + 1]; ++k) s += val[k] * p[ja[k]];
+      q[i] = s;
+      pq += p[i] * s;
+    }
+    const double alpha = rz / pq;
+    double rz1 = 0., rr1 = 0.;
+#pragma omp parallel for num_threads(threads) reduction(+ : rz1, rr1)
+    for(int i = 0; i < n; ++i) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * q[i];
+      z[i] = dinv[i] * r[i];
+      rz1 += r[i] * z[i];
+      rr1 += r[i] * r[i];
+    }
+    const double beta = rz1 / rz;
+    rz = rz1;
+    rr = rr1;
+#pragma omp parallel for num_threads(threads)
+    for(int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    it++;
+  }
+  double tr = 0.;
+#pragma omp parallel for num_threads(threads) reduction(+ : tr)
+  for(int i = 0; i < n; ++i) {
+    double s = 0.;
+    for(int k = ia[i]; k < ia[i + 1]; ++k) s += val[k] * x[ja[k]];
+    tr += (b[i] - s) * (b[i] - s);
+  }
+  if(relres) *relres = sqrt(tr) / bn;
+  free(r); free(z); free(p); free(q); free(dinv);
+  return it;
 }
 
 /* IC(0): incomplete Cholesky with the pattern of the lower triangle of A

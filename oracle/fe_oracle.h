@@ -74,6 +74,13 @@ int or_assemble_diffusion(int kind, int V, const double *xyz, int E, const int *
                           int quad_degree, or_fn coeff, or_fn source, const double *u,
                           const int *elem_order, double *val, double *rhs);
 
+/* The same assembly over the element colors with `threads` OpenMP threads
+ * (colorOf from or_color_elements, ncolors colors). */
+int or_assemble_diffusion_omp(int kind, int V, const double *xyz, int E, const int *conn,
+                              const int *adr, int n, const int *ia, const int *ja,
+                              int quad_degree, or_fn coeff, or_fn source, const double *u,
+                              const int *colorOf, int ncolors, int threads, double *val, double *rhs);
+
 /* Element matrix of one scalar diffusion element (for unit checks). */
 int or_element_diffusion(int kind, const double *coords, int quad_degree, or_fn coeff,
                          int elem, double *Ae);
@@ -92,6 +99,8 @@ int or_spmv(int n, const int *ia, const int *ja, const double *val, const double
             double *y);
 int or_cg_jacobi(int n, const int *ia, const int *ja, const double *val, const double *b,
                  double *x, double rtol, int maxit, double *relres);
+int or_cg_jacobi_omp(int n, const int *ia, const int *ja, const double *val, const double *b,
+                     double *x, double rtol, int maxit, int threads, double *relres);
 int or_ic0(int n, const int *ia, const int *ja, const double *val, double *lval);
 int or_pcg_ic0(int n, const int *ia, const int *ja, const double *val, const double *lval,
                const double *b, double *x, double rtol, int maxit, double *relres);

@@ -44,6 +44,9 @@ class Oracle:
                                      dp, ip, dp, dp]
         L.or_spmv.argtypes = [C.c_int, ip, ip, dp, dp, dp]
         L.or_cg_jacobi.argtypes = [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_int, dp]
+        L.or_cg_jacobi_omp.argtypes = [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_int, C.c_int, dp]
+        L.or_assemble_diffusion_omp.argtypes = [C.c_int, C.c_int, dp, C.c_int, ip, ip, C.c_int, ip, ip, C.c_int,
+                                                OrFn, OrFn, dp, ip, C.c_int, C.c_int, dp, dp]
         L.or_ic0.argtypes = [C.c_int, ip, ip, dp, dp]
         L.or_pcg_ic0.argtypes = [C.c_int, ip, ip, dp, dp, dp, dp, C.c_double, C.c_int, dp]
         L.or_ilu0.argtypes = [C.c_int, ip, ip, dp, dp]
@@ -121,6 +124,37 @@ class Oracle:
                                           P(rhs, C.c_double))
         assert rc == 0, rc
         return val, rhs
+
+    def assemble_diffusion_omp(self, kind, xyz, conn, adr, n, ia, ja, deg, coeff, src, u, colorOf, ncolors,
+                               threads):
+        nvpe = 4 if kind == 4 else 3
+        E = conn.size // nvpe
+        val = np.zeros(ja.size)
+        rhs = np.zeros(n)
+        keep = []
+
+        def fn(f):
+            kind_, c0, c1, data = f
+            o = OrFn(kind_, c0, c1, None)
+            if data is not None:
+                d = np.ascontiguousarray(data, np.float64)
+                keep.append(d)
+                o.data = P(d, C.c_double)
+            return o
+        colorOf = np.ascontiguousarray(colorOf, np.int32)
+        rc = self.L.or_assemble_diffusion_omp(kind, xyz.size // 3, P(xyz, C.c_double), E, P(conn, C.c_int),
+                                              P(adr, C.c_int), n, P(ia, C.c_int), P(ja, C.c_int), deg, fn(coeff),
+                                              fn(src), P(u, C.c_double), P(colorOf, C.c_int), ncolors, threads,
+                                              P(val, C.c_double), P(rhs, C.c_double))
+        assert rc == 0, rc
+        return val, rhs
+
+    def cg_jacobi_omp(self, ia, ja, val, b, rtol, maxit, threads):
+        x = np.zeros(b.size)
+        rr = C.c_double()
+        it = self.L.or_cg_jacobi_omp(b.size, P(ia, C.c_int), P(ja, C.c_int), P(val, C.c_double), P(b, C.c_double),
+                                     P(x, C.c_double), rtol, maxit, threads, C.byref(rr))
+        return x, it, rr.value
 
     def spmv(self, ia, ja, val, x):
         y = np.zeros(ia.size - 1)
