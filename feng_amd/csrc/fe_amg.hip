@@ -993,13 +993,14 @@ int amgApplyT(fe_hip_ctx *ctx, const T *in, T *out, const int *done, int slot)
     const T *b = l == 0 ? in : L.b;
     T *bn = pick<T>(A.lv[l + 1].b64, A.lv[l + 1].b32);
     const dim3 gp(ctx->amgXcd ? (h.padTiles + 7) & ~7 : h.padTiles);
-    if(h.padTiles > 0 && ctx->amgLanes == 2)
+    const bool pad = h.padTiles > 0 && ctx->amgPadTiles;
+    if(pad && ctx->amgLanes == 2)
       hipLaunchKernelGGL((amgDownP<T, TV, 2, kAmgTB>), gp, dim3(kAmgTB), 0, ctx->stream, L, b, bn, h.prow, h.prg,
                          h.pagg, h.padTiles, ctx->amgXcd, done);
-    else if(h.padTiles > 0 && ctx->amgLanes == 8)
+    else if(pad && ctx->amgLanes == 8)
       hipLaunchKernelGGL((amgDownP<T, TV, 8, kAmgTB>), gp, dim3(kAmgTB), 0, ctx->stream, L, b, bn, h.prow, h.prg,
                          h.pagg, h.padTiles, ctx->amgXcd, done);
-    else if(h.padTiles > 0)
+    else if(pad)
       hipLaunchKernelGGL((amgDownP<T, TV, kAmgLPR, kAmgTB>), gp, dim3(kAmgTB), 0, ctx->stream, L, b, bn, h.prow,
                          h.prg, h.pagg, h.padTiles, ctx->amgXcd, done);
     else

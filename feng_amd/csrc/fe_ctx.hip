@@ -485,6 +485,9 @@ int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value)
   } else if(k == "graph_iters") {
     FE_REQUIRE(ctx, value >= 0 && value <= 32, "graph_iters must be in [0, 32]");
     ctx->graphIters = value;
+  } else if(k == "amg_pad_tiles") {
+    FE_REQUIRE(ctx, value == 0 || value == 1, "amg_pad_tiles must be 0 or 1");
+    ctx->amgPadTiles = value;
   } else if(k == "amg_lanes") {
     FE_REQUIRE(ctx, value == 2 || value == 4 || value == 8, "amg_lanes must be 2, 4 or 8");
     ctx->amgLanes = value;
@@ -539,6 +542,7 @@ int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value)
   else if(k == "amg_xcd") *value = ctx->amgXcd;
   else if(k == "xcd_tiles") *value = ctx->xcdTiles;
   else if(k == "amg_lanes") *value = ctx->amgLanes;
+  else if(k == "amg_pad_tiles") *value = ctx->amgPadTiles;
   else if(k == "graph_iters") *value = ctx->graphIters;
   else FE_REQUIRE(ctx, false, "unknown tuning key " + k);
   return FE_STATUS_OK;

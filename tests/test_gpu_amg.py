@@ -47,9 +47,16 @@ def _oracle_levels(S):
 
 
 @pytest.mark.parametrize("case", ["tet_grf_12", "p2_blocks_40", "p1_square_9"])
-def test_vcycle_matches_oracle(case):
-    print("case", case, flush=True)
+@pytest.mark.parametrize("variant", ["default", "header_tiles_big_block"])
+def test_vcycle_matches_oracle(case, variant):
+    print("case", case, variant, flush=True)
     m, S, num = _system(case)
+    if variant != "default":
+        # restriction through tile headers (the kernel for aggregates above
+        # 256 rows) and larger levels in the single-workgroup kernel, whose
+        # matrices it then reads from global memory
+        S.set_tuning("amg_pad_tiles", 0)
+        S.set_tuning("amg_block_rows", 8192)
     A, rhs, lv = _oracle_levels(S)
     r = np.random.default_rng(3).normal(size=A.shape[0])
     S.set_tuning("pc_values32", 0)
