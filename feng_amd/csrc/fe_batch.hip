@@ -34,6 +34,7 @@ struct FeBatch {
   double *x64 = nullptr, *r64 = nullptr, *bak64 = nullptr;
   TileVecs<double> v64;
   TileVecs<float> v32;
+  double *sp[9] = {};  // per-problem AMG-PCG vectors x, r, u, w, z, q, s, p, m (allocated on first use)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graphKey[5] = {-1, -1, -1, -1, -1};  // TB, lanes, precision, maxit, active tiles of the captured graph
@@ -290,6 +291,152 @@ __global__ void bDinv(int n, const int *__restrict__ diag, const double *__restr
   if(i >= n) return;
   const double dd = val[diag[i]];
   dinv[i] = (T)(dd != 0. ? 1. / dd : 1.);
+}
+
+// ---------------------------------------------------------------------------
+// Per-problem pipelined PCG with an external preconditioner (the union AMG
+// V-cycle, block diagonal, so problem p sees its own V-cycle): the split form
+// of the single-system solver (Ghysels & Vanroose, Alg. 3) with the scalars,
+// stopping test and iteration count of each problem, u = M r and m = M w
+// given as vectors.
+
+struct BSpVecs {
+  double *x, *r, *u, *w, *z, *q, *s, *p, *m;
+};
+
+// r = b (the outer FP64 residual), x = 0; rows of problems that have stopped
+// get r = 0.
+__global__ void bSpStart(const double *__restrict__ b, BSpVecs v, BTiles bt, BState S)
+{
+  const int tile = blockIdx.x, p = bt.prob[tile];
+  const int row = bt.row0[tile] + threadIdx.x;
+  if(threadIdx.x < bt.rows[tile]) {
+    v.x[row] = 0.;
+    v.r[row] = S.outerDone[p] ? 0. : b[row];
+  }
+}
+
+// w = A u, z = q = s = p = 0, partials of iteration 0, per-problem start values.
+template <int LPR, int TB>
+__global__ __launch_bounds__(TB) void bSpInit(const int *__restrict__ ia, const int *__restrict__ ja,
+                                              const double *__restrict__ val, BSpVecs v, BTiles bt, BParts P,
+                                              BState S)
+{
+  __shared__ double ldsY[TB];
+  const int tile = blockIdx.x, p = bt.prob[tile];
+  const int base = bt.row0[tile], nEnd = base + bt.rows[tile];
+  const int row = base + threadIdx.x;
+  if(S.outerDone[p]) {
+    if(threadIdx.x == 0) {
+      P.g[0][tile] = P.d[0][tile] = P.rho[0][tile] = 0.;
+      if(tile == bt.ptile[p]) {
+        S.done[p] = 1;
+        S.iter[p] = 0;
+      }
+    }
+    return;
+  }
+  int kb[LPR], ke[LPR];
+  tileBounds<LPR, TB>(nEnd, base, ia, kb, ke);
+  tileProduct<double, LPR, TB, false>(nEnd, base, ia, ja, val, v.u, nullptr, kb, ke, ldsY);
+  __syncthreads();
+  double g = 0., d = 0., rho = 0.;
+  if(row < nEnd) {
+    const double wi = ldsY[threadIdx.x], ri = v.r[row], ui = v.u[row];
+    v.w[row] = wi;
+    v.z[row] = v.q[row] = v.s[row] = v.p[row] = 0.;
+    g = ri * ui;
+    d = wi * ui;
+    rho = ri * ri;
+  }
+  blockSum3<TB>(g, d, rho);
+  if(threadIdx.x == 0) {
+    P.g[0][tile] = g;
+    P.d[0][tile] = d;
+    P.rho[0][tile] = rho;
+    if(tile == bt.ptile[p]) {
+      S.gamma[2 * p + 1] = 1.;
+      S.alpha[2 * p + 1] = __builtin_inf();
+      S.done[p] = 0;
+      S.iter[p] = 0;
+    }
+  }
+}
+
+// Iteration `it` of every problem still iterating (active tiles), m = M w.
+template <int LPR, int TB>
+__global__ __launch_bounds__(TB) void bSpIter(const int *__restrict__ ia, const int *__restrict__ ja,
+                                              const double *__restrict__ val, BSpVecs v, BTiles bt, BParts P,
+                                              BState S, const int *__restrict__ act, int it, int maxit)
+{
+  __shared__ double ldsY[TB];
+  const int tile = act[blockIdx.x], p = bt.prob[tile];
+  if(S.done[p]) return;
+  const int base = bt.row0[tile], nEnd = base + bt.rows[tile];
+  const int row = base + threadIdx.x;
+  int kb[LPR], ke[LPR];
+  tileBounds<LPR, TB>(nEnd, base, ia, kb, ke);
+  double r = 0., u = 0., w = 0., z = 0., q = 0., s = 0., pv = 0., x = 0., m = 0.;
+  if(row < nEnd) {
+    r = v.r[row]; u = v.u[row]; w = v.w[row]; z = v.z[row]; q = v.q[row];
+    s = v.s[row]; pv = v.p[row]; x = v.x[row]; m = v.m[row];
+  }
+  const int c = it & 1, o = c ^ 1;
+  const double gPrev = S.gamma[2 * p + o], aPrev = S.alpha[2 * p + o];
+  const int t0 = bt.ptile[p], t1 = bt.ptile[p + 1];
+  double gamma = 0., delta = 0., rho = 0.;
+  for(int k = t0 + threadIdx.x; k < t1; k += TB) {
+    gamma += P.g[c][k];
+    delta += P.d[c][k];
+    rho += P.rho[c][k];
+  }
+  blockSum3<TB>(gamma, delta, rho);
+  const bool lead = (tile == t0 && threadIdx.x == 0);
+  if(!(rho > S.tol2[p]) || it >= maxit) {
+    if(lead) {
+      S.iter[p] = it;
+      S.done[p] = (rho <= S.tol2[p]) ? 1 : (rho == rho ? 2 : 4);
+    }
+    return;
+  }
+  const double beta = gamma / gPrev;
+  const double den = delta - beta * gamma / aPrev;
+  if(!(den > 0.)) {
+    if(lead) {
+      S.iter[p] = it;
+      S.done[p] = 3;
+    }
+    return;
+  }
+  const double alpha = gamma / den;
+  if(lead) {
+    S.gamma[2 * p + c] = gamma;
+    S.alpha[2 * p + c] = alpha;
+  }
+  tileProduct<double, LPR, TB, false>(nEnd, base, ia, ja, val, v.m, nullptr, kb, ke, ldsY);
+  __syncthreads();
+  double g = 0., d = 0., rh = 0.;
+  if(row < nEnd) {
+    z = ldsY[threadIdx.x] + beta * z;
+    q = m + beta * q;
+    s = w + beta * s;
+    pv = u + beta * pv;
+    x += alpha * pv;
+    r -= alpha * s;
+    u -= alpha * q;
+    w -= alpha * z;
+    v.x[row] = x; v.r[row] = r; v.u[row] = u; v.w[row] = w;
+    v.z[row] = z; v.q[row] = q; v.s[row] = s; v.p[row] = pv;
+    g = r * u;
+    d = w * u;
+    rh = r * r;
+  }
+  blockSum3<TB>(g, d, rh);
+  if(threadIdx.x == 0) {
+    P.g[o][tile] = g;
+    P.d[o][tile] = d;
+    P.rho[o][tile] = rh;
+  }
 }
 
 inline int cdiv(int64_t a, int b) { return (int)((a + b - 1) / b); }
@@ -551,6 +698,105 @@ int solveBatchedAmg(fe_hip_ctx *ctx, FeBatch &B, const fe_hip_solver_opts *o, fe
   return FE_STATUS_OK;
 }
 
+// Per-problem AMG-PCG: each problem its own scalars and stop, the union
+// V-cycle as the preconditioner (FP64 vectors), per-problem FP64 refinement
+// restarts from the true residual when the recursive one met its target
+// first.  Iteration grid = tiles of the problems still iterating.
+int solveBatchedAmgPerProblem(fe_hip_ctx *ctx, FeBatch &B, const fe_hip_solver_opts *o, fe_hip_batch_stats *stats,
+                              double *ms_solve)
+{
+  const int n = ctx->n, nprob = B.nprob, TB = B.TB;
+  FE_REQUIRE(ctx, TB == 256 || TB == 512 || TB == 1024, "batched AMG-PCG: block must be 256, 512 or 1024");
+  for(double *&q : B.sp)
+    if(!q && feAlloc(ctx, &q, n)) return FE_STATUS_OUT_OF_MEMORY;
+  BSpVecs v{B.sp[0], B.sp[1], B.sp[2], B.sp[3], B.sp[4], B.sp[5], B.sp[6], B.sp[7], B.sp[8]};
+  if(int rc = feAmgSetup(ctx)) return rc;
+  hipEvent_t t0, t1;
+  FE_HIP_TRY(ctx, hipEventCreate(&t0));
+  FE_HIP_TRY(ctx, hipEventCreate(&t1));
+  FE_HIP_TRY(ctx, hipEventRecord(t0, ctx->stream));
+  if(int rc = feAmgNumeric(ctx)) return rc;
+  const bool v32 = ctx->pcValues32 != 0;
+  FE_HIP_TRY(ctx, hipMemsetAsync(B.x64, 0, sizeof(double) * n, ctx->stream));
+  std::vector<double> bn, rn;
+  if(int rc = residuals(ctx, B, nullptr, bn)) return rc;  // r = b
+  rn = bn;
+  std::vector<double> target(nprob), tol2(nprob);
+  for(int p = 0; p < nprob; ++p) target[p] = std::max(o->rtol * bn[p], o->atol);
+  std::vector<int> its(nprob, 0), outerDone(nprob, 0), hDone(nprob), hIter(nprob);
+  const BTiles bt = tilesOf(B);
+  const BState S = stateOf(B);
+  const BParts P = partsOf(B);
+  for(int outer = 0; outer < 20; ++outer) {
+    bool any = false;
+    for(int p = 0; p < nprob; ++p) {
+      outerDone[p] = (rn[p] <= target[p] || its[p] >= o->max_iter) ? 1 : 0;
+      any |= !outerDone[p];
+      // recursive-residual target below the true one (restart aims lower)
+      const double tt = (outer == 0 ? 0.5 : 0.05) * target[p];
+      tol2[p] = tt * tt;
+    }
+    if(!any) break;
+    FE_HIP_TRY(ctx, hipMemcpyAsync(B.outerDone, outerDone.data(), sizeof(int) * nprob, hipMemcpyHostToDevice, ctx->stream));
+    FE_HIP_TRY(ctx, hipMemcpyAsync(B.tol2, tol2.data(), sizeof(double) * nprob, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(bSpStart, dim3(B.nTiles), dim3(TB), 0, ctx->stream, B.r64, v, bt, S);
+    if(int rc = feAmgApply(ctx, (const double *)v.r, v.u, nullptr, v32)) return rc;
+#define FE_SP_B(KER, GRID, ...)                                                                                     \
+  do {                                                                                                           \
+    if(TB == 256) hipLaunchKernelGGL((KER<4, 256>), dim3(GRID), dim3(256), 0, ctx->stream, __VA_ARGS__);         \
+    else if(TB == 512) hipLaunchKernelGGL((KER<4, 512>), dim3(GRID), dim3(512), 0, ctx->stream, __VA_ARGS__);    \
+    else hipLaunchKernelGGL((KER<4, 1024>), dim3(GRID), dim3(1024), 0, ctx->stream, __VA_ARGS__);                \
+  } while(0)
+    FE_SP_B(bSpInit, B.nTiles, ctx->d_ia, ctx->d_ja, (const double *)ctx->d_val, v, bt, P, S);
+    if(int rc = setActive(ctx, B, outerDone)) return rc;
+    int budget = 0;
+    for(int p = 0; p < nprob; ++p)
+      if(!outerDone[p]) budget = std::max(budget, o->max_iter - its[p]);
+    for(int it = 0; it <= budget;) {
+      for(int k = 0; k < 16; ++k, ++it) {
+        if(int rc = feAmgApply(ctx, (const double *)v.w, v.m, nullptr, v32)) return rc;
+        FE_SP_B(bSpIter, B.nAct, ctx->d_ia, ctx->d_ja, (const double *)ctx->d_val, v, bt, P, S, (const int *)B.act,
+                it, budget);
+      }
+      FE_HIP_TRY(ctx, hipGetLastError());
+      FE_HIP_TRY(ctx, hipMemcpyAsync(hDone.data(), B.done, sizeof(int) * nprob, hipMemcpyDeviceToHost, ctx->stream));
+      FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+      if(std::all_of(hDone.begin(), hDone.end(), [](int d) { return d != 0; })) break;
+      int active = 0;
+      for(int p = 0; p < nprob; ++p)
+        if(!hDone[p]) active += B.hPtile[p + 1] - B.hPtile[p];
+      if(active < (3 * B.nAct) / 4)
+        if(int rc = setActive(ctx, B, hDone)) return rc;
+    }
+#undef FE_SP_B
+    FE_HIP_TRY(ctx, hipMemcpyAsync(hIter.data(), B.iter, sizeof(int) * nprob, hipMemcpyDeviceToHost, ctx->stream));
+    hipLaunchKernelGGL(bAddCorrection<double>, dim3(1024), dim3(256), 0, ctx->stream, n, B.x64, (const double *)v.x);
+    std::vector<double> rnew;
+    if(int rc = residuals(ctx, B, B.x64, rnew)) return rc;
+    for(int p = 0; p < nprob; ++p) {
+      if(outerDone[p]) continue;
+      its[p] += hIter[p];
+      rn[p] = rnew[p];
+    }
+  }
+  FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_du, B.x64, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+  FE_HIP_TRY(ctx, hipEventRecord(t1, ctx->stream));
+  FE_HIP_TRY(ctx, hipEventSynchronize(t1));
+  if(int rc = feAmgCheck(ctx)) return rc;
+  float ms = 0.f;
+  FE_HIP_TRY(ctx, hipEventElapsedTime(&ms, t0, t1));
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  if(ms_solve) *ms_solve = ms;
+  if(stats)
+    for(int p = 0; p < nprob; ++p) {
+      stats[p].iterations = its[p];
+      stats[p].relres_fp64 = bn[p] > 0. ? rn[p] / bn[p] : 0.;
+      stats[p].converged = rn[p] <= target[p];
+    }
+  return FE_STATUS_OK;
+}
+
 }  // namespace
 
 extern "C" int fe_hip_solve_batched(fe_hip_ctx *ctx, int nprob, const int *rows, const fe_hip_solver_opts *o,
@@ -566,7 +812,9 @@ extern "C" int fe_hip_solve_batched(fe_hip_ctx *ctx, int nprob, const int *rows,
   if(int rc = setupBatch(ctx, nprob, rows, Bp)) return rc;
   FeBatch &B = *Bp;
   if(int rc = buildTiles(ctx, B, ctx->iterBlock)) return rc;
-  if(o->pc == FE_PC_AMG) return solveBatchedAmg(ctx, B, o, stats, ms_solve);
+  if(o->pc == FE_PC_AMG)
+    return ctx->batchAmgGlobal ? solveBatchedAmg(ctx, B, o, stats, ms_solve)
+                               : solveBatchedAmgPerProblem(ctx, B, o, stats, ms_solve);
   const int n = ctx->n;
   hipEvent_t t0, t1;
   FE_HIP_TRY(ctx, hipEventCreate(&t0));

@@ -485,6 +485,9 @@ int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value)
   } else if(k == "graph_iters") {
     FE_REQUIRE(ctx, value >= 0 && value <= 32, "graph_iters must be in [0, 32]");
     ctx->graphIters = value;
+  } else if(k == "batch_amg_global") {
+    FE_REQUIRE(ctx, value == 0 || value == 1, "batch_amg_global must be 0 or 1");
+    ctx->batchAmgGlobal = value;
   } else if(k == "amg_pad_tiles") {
     FE_REQUIRE(ctx, value == 0 || value == 1, "amg_pad_tiles must be 0 or 1");
     ctx->amgPadTiles = value;
@@ -543,6 +546,7 @@ int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value)
   else if(k == "xcd_tiles") *value = ctx->xcdTiles;
   else if(k == "amg_lanes") *value = ctx->amgLanes;
   else if(k == "amg_pad_tiles") *value = ctx->amgPadTiles;
+  else if(k == "batch_amg_global") *value = ctx->batchAmgGlobal;
   else if(k == "graph_iters") *value = ctx->graphIters;
   else FE_REQUIRE(ctx, false, "unknown tuning key " + k);
   return FE_STATUS_OK;

@@ -167,6 +167,7 @@ struct fe_hip_ctx {
   int xcdTiles = 0;  // XCD-aware tile order in the row-tile PCG kernels (measured slower at 512-row tiles)
   int amgLanes = 4;  // lanes per row in the multigrid level kernels
   int amgPadTiles = 1;  // restriction over aggregate-aligned tiles (0: tile-header kernel)
+  int batchAmgGlobal = 0;  // batched AMG: 0 per-problem CG scalars, 1 one CG over the union system
   FeAmg *amg = nullptr;
   bool amgActive = false;  // the current CG solve applies the AMG V-cycle
   FeMpnn *mpnn = nullptr;

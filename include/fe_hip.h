@@ -229,10 +229,12 @@ int fe_hip_amg_apply(fe_hip_ctx *ctx, const double *r, double *z);
  * problem gets its own Jacobi-PCG (own scalars, own stop on
  * ||b_p - A_p x_p||_2 <= max(rtol ||b_p||_2, atol), FP64), all problems
  * advancing in the same launches; FE_PREC_FP32_IR runs the inner solves on
- * the FP32 values with FP64 refinement.  With pc = FE_PC_AMG the union system
- * is solved by one AMG-PCG (the hierarchy of a block-diagonal matrix is block
- * diagonal) to ||r||_2 <= min_p max(rtol ||b_p||_2, atol), which bounds every
- * problem's own residual; the per-problem FP64 residuals are then reported.
+ * the FP32 values with FP64 refinement.  With pc = FE_PC_AMG every problem
+ * runs its own pipelined PCG (own scalars and stop, FP64 vectors) with the
+ * union AMG V-cycle as preconditioner: the hierarchy of a block-diagonal
+ * matrix is block diagonal, so each problem sees its own V-cycle (tuning key
+ * "batch_amg_global" = 1 instead runs one CG over the union system to
+ * ||r||_2 <= min_p max(rtol ||b_p||_2, atol)).
  * The correction goes to the device du (fe_hip_correct_solution applies it).
  * ms_solve: device time. */
 typedef struct {
