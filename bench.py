@@ -228,8 +228,8 @@ def main():
     ap.add_argument("--paths", default=None, help="comma-separated subset of " + ",".join(PATHS))
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the solver has no multi-GPU path "
+              "(row-slab sharding with halo exchange is a design, not code)", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         return copy_probe()

@@ -22,9 +22,12 @@
 // The first step is fused into the restriction (one kernel per level reads
 // the aggregate's rows and writes b_c), the prolongation into the second
 // smoothing step (one kernel per level gathers u_j + oc e[agg_j]).  Levels of
-// at most `blockRows` rows and the dense coarsest solve run in one
+// at most `blockRows` rows and the coarsest solve run in one
 // single-workgroup kernel, so a V-cycle is 2 L + 1 launches for L large
-// levels.
+// levels.  The coarsest level (at most kAmgDense rows) is inverted densely per
+// matrix; when coarsening stalls above that size (a block-diagonal batch of
+// small problems ends at a node or two per problem) it is solved by
+// kAmgCoarseSweeps l1-Jacobi sweeps instead, a fixed SPD polynomial.
 
 #include <algorithm>
 #include <cmath>
@@ -119,8 +122,6 @@ struct FeAmg {
   void *desc[3] = {nullptr, nullptr, nullptr};  // AmgLev<double,double>, <double,float>, <float,float>
   int *flag = nullptr;
   double c = 0., oc = 0.;
-  bool valuesReady = false;
-  double lastSetupMs = 0.;
 };
 
 namespace {
@@ -962,7 +963,6 @@ int feAmgNumeric(fe_hip_ctx *ctx)
     hipLaunchKernelGGL(amgDenseInverse, dim3(1), dim3(1024), 0, ctx->stream, C.n, C.ia, C.ja, C.val64, A.cinv64,
                        A.cinv32, A.flag);
   FE_HIP_TRY(ctx, hipGetLastError());
-  A.valuesReady = true;
   return FE_STATUS_OK;
 }
 
