@@ -19,7 +19,7 @@ TRI_P1, TRI_P2, TRI_VP2, TET_P1 = 1, 2, 3, 4
 NLOC = {TRI_P1: 3, TRI_P2: 6, TRI_VP2: 12, TET_P1: 4}
 
 # weak forms (elementSystemType values, feSysElm.h:13-77)
-SOURCE, VECTOR_SOURCE, DIFFUSION, VECTOR_DIFFUSION = 0, 2, 16, 18
+SOURCE, VECTOR_SOURCE, TRANSIENT_MASS, DIFFUSION, VECTOR_DIFFUSION = 0, 2, 13, 16, 18
 VECTOR_CONVECTIVE_ACCELERATION, DIV_NEWTONIAN_STRESS = 22, 25
 MIXED_GRADIENT, MIXED_DIVERGENCE = 26, 31
 
@@ -313,10 +313,38 @@ class LinearSystemHIP:
         return v.value
 
     def update_coeff(self, form, index, data):
-        """Replace a tabulated coefficient (FN_ELEM / FN_QP) of form `form`."""
-        d = np.ascontiguousarray(data, np.float64)
+        """Replace a tabulated coefficient (FN_ELEM / FN_QP) of form `form`, or
+        the constant c0 of an analytic one (a single value)."""
+        d = np.ascontiguousarray(np.atleast_1d(data), np.float64)
         self._c(self.L.fe_hip_update_coeff(self.ctx, form, index, N.ptr(d, C.c_double), d.size),
                 "fe_hip_update_coeff")
+
+    # solution history of the BDF schemes (feSolutionContainer) ---------------
+    def set_bdf(self, cn):
+        """setC0 + computeBDFCoefficients: u_dot = sum_j cn[j] sol[j]."""
+        d = np.ascontiguousarray(cn, np.float64)
+        self._c(self.L.fe_hip_set_bdf(self.ctx, d.size, N.ptr(d, C.c_double)), "fe_hip_set_bdf")
+
+    def rotate_solutions(self):
+        """feSolutionContainer::rotate: sol[j] = sol[j - 1], j = 4..1."""
+        self._c(self.L.fe_hip_rotate_solutions(self.ctx), "fe_hip_rotate_solutions")
+
+    def set_past_solution(self, j, u):
+        u = np.ascontiguousarray(u, np.float64)
+        assert u.size == self.nDofs
+        self._c(self.L.fe_hip_set_past_solution(self.ctx, j, N.ptr(u, C.c_double)), "fe_hip_set_past_solution")
+
+    def get_past_solution(self, j):
+        u = np.zeros(self.nDofs)
+        self._c(self.L.fe_hip_get_past_solution(self.ctx, j, N.ptr(u, C.c_double)), "fe_hip_get_past_solution")
+        return u
+
+    def set_essential(self, values):
+        """The essential part of the current solution (nDofs - nInc values)."""
+        v = np.ascontiguousarray(values, np.float64)
+        assert v.size == self.nDofs - self.n
+        if v.size:
+            self._c(self.L.fe_hip_set_essential(self.ctx, N.ptr(v, C.c_double)), "fe_hip_set_essential")
 
     # aggregation multigrid (FE_PC_AMG) ---------------------------------------
     def amg_levels(self):

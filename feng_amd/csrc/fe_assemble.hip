@@ -55,8 +55,12 @@ struct ScalarAsmArgs {
   int begin, count, E, n, what;
   const int *perm, *conn, *adr, *pos;
   const double *xyz, *tab, *u;
-  FeFn coeff, src;
-  int hasDiff, hasSrc;
+  FeFn coeff, src, mass;
+  int hasDiff, hasSrc, hasMass;
+  // time derivative u_dot = cn[0] u + sum_{j = 1}^{nsol - 1} cn[j] past[j - 1]
+  int nsol;
+  double cn[fe_hip_ctx::kMaxPast + 1];
+  const double *past[fe_hip_ctx::kMaxPast];
   double *val, *rhs;
 };
 
@@ -98,8 +102,9 @@ __device__ __forceinline__ void feGeometry(const double (&c)[DIM + 1][3], double
   }
 }
 
-// Scalar diffusion (feSysElm.cpp:530-583) fused with a scalar source
-// (feSysElm.cpp:16-27) on one element per thread.
+// Scalar diffusion (feSysElm.cpp:530-583), transient mass
+// (feSysElm.cpp:426-458) and source (feSysElm.cpp:16-27) of one field and
+// quadrature rule, fused, on one element per thread.
 template <int NF, int DIM>
 __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
 {
@@ -138,7 +143,7 @@ __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
 #pragma unroll
     for(int j = 0; j < NF; ++j) Ae[i][j] = 0.;
   }
-  const bool doMat = (a.what & 1) && a.hasDiff, doRes = (a.what & 2);
+  const bool doMat = (a.what & 1) && (a.hasDiff || a.hasMass), doRes = (a.what & 2);
   const int nq = T.nq;
   if(a.hasDiff) {
     for(int k = 0; k < nq; ++k) {
@@ -181,6 +186,41 @@ __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
         for(int i = 0; i < NF; ++i)
 #pragma unroll
           for(int d = 0; d < DIM; ++d) Be[i] -= kc * gp[i][d] * gu[d] * jw;
+      }
+    }
+  }
+  if(a.hasMass) {
+    // nodal time derivative of the element's DOFs
+    double ud[NF];
+    if(doRes) {
+#pragma unroll
+      for(int i = 0; i < NF; ++i) {
+        double v = a.cn[0] * ul[i];
+        for(int j = 1; j < a.nsol; ++j) v += a.cn[j] * a.past[j - 1][ad[i]];
+        ud[i] = v;
+      }
+    }
+    const double c0 = a.cn[0];
+    for(int k = 0; k < nq; ++k) {
+      double x[3] = {0., 0., 0.};
+#pragma unroll
+      for(int j = 0; j < NV; ++j)
+#pragma unroll
+        for(int d = 0; d < 3; ++d) x[d] += c[j][d] * T.Lg[k][j];
+      const double m = feEvalFn(a.mass, x, e, nq, k);
+      const double jw = J * T.w[k];
+      if(doMat) {
+#pragma unroll
+        for(int i = 0; i < NF; ++i)
+#pragma unroll
+          for(int j = 0; j < NF; ++j) Ae[i][j] += T.L[k][i] * m * c0 * T.L[k][j] * jw;
+      }
+      if(doRes) {
+        double uq = 0.;
+#pragma unroll
+        for(int j = 0; j < NF; ++j) uq += T.L[k][j] * ud[j];
+#pragma unroll
+        for(int i = 0; i < NF; ++i) Be[i] -= T.L[k][i] * m * uq * jw;
       }
     }
   }
@@ -655,14 +695,27 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
   FE_HIP_TRY(ctx, hipEventCreate(&t0));
   FE_HIP_TRY(ctx, hipEventCreate(&t1));
   FE_HIP_TRY(ctx, hipEventRecord(t0, ctx->stream));
-  // Fuse every SOURCE form into the DIFFUSION form of the same field and
-  // quadrature rule (one element sweep computes Ae and Be of both).
+  // Scalar forms of one field and quadrature rule (at most one DIFFUSION,
+  // one TRANSIENT_MASS and one SOURCE) are fused: one element sweep computes
+  // Ae and Be of all of them.
   std::vector<bool> done(ctx->forms.size(), false);
   bool fp32Done = false;
+  auto scalarForm = [](int t) {
+    return t == FE_SYS_DIFFUSION || t == FE_SYS_TRANSIENT_MASS || t == FE_SYS_SOURCE;
+  };
   for(size_t fi = 0; fi < ctx->forms.size(); ++fi) {
-    if(done[fi]) continue;
-    FeForm &F = ctx->forms[fi];
-    if(F.sysType != FE_SYS_DIFFUSION && F.sysType != FE_SYS_SOURCE) continue;
+    if(done[fi] || !scalarForm(ctx->forms[fi].sysType)) continue;
+    const int field = ctx->forms[fi].fields[0], degree = ctx->forms[fi].degree;
+    int iDiff = -1, iMass = -1, iSrc = -1;
+    for(size_t fj = fi; fj < ctx->forms.size(); ++fj) {
+      const FeForm &G = ctx->forms[fj];
+      if(done[fj] || !scalarForm(G.sysType) || G.fields[0] != field || G.degree != degree) continue;
+      int &slot = G.sysType == FE_SYS_DIFFUSION ? iDiff : (G.sysType == FE_SYS_TRANSIENT_MASS ? iMass : iSrc);
+      if(slot < 0) {
+        slot = (int)fj;
+        done[fj] = true;
+      }
+    }
     ScalarAsmArgs a{};
     a.E = ctx->E;
     a.n = ctx->n;
@@ -673,38 +726,34 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
     a.u = ctx->d_u;
     a.val = ctx->d_val;
     a.rhs = ctx->d_rhs;
-    a.tab = F.d_tab;
-    const int field = F.fields[0];
-    if(F.sysType == FE_SYS_DIFFUSION) {
+    a.tab = ctx->forms[fi].d_tab;
+    // Row map and scatter positions: the group of a matrix form of the sweep,
+    // else (a lone source) a matrix group on the same field.
+    int g = iDiff >= 0 ? ctx->forms[iDiff].group : (iMass >= 0 ? ctx->forms[iMass].group : -1);
+    for(size_t k = 0; g < 0 && k < ctx->groups.size(); ++k)
+      if(ctx->groups[k].layoutI.size() == 1 && ctx->groups[k].layoutI[0] == field) g = (int)k;
+    FE_REQUIRE(ctx, g >= 0, "a SOURCE form needs a DIFFUSION or TRANSIENT_MASS form on the same field");
+    a.adr = ctx->groups[g].d_adrI;
+    a.pos = ctx->groups[g].d_pos;
+    if(iDiff >= 0) {
       a.hasDiff = 1;
-      a.coeff = F.coeffs[0];
-      a.pos = ctx->groups[F.group].d_pos;
-      a.adr = ctx->groups[F.group].d_adrI;
-      for(size_t fj = 0; fj < ctx->forms.size(); ++fj) {
-        FeForm &S = ctx->forms[fj];
-        if(!done[fj] && fj != fi && S.sysType == FE_SYS_SOURCE && S.fields[0] == field && S.degree == F.degree &&
-           !a.hasSrc) {
-          a.hasSrc = 1;
-          a.src = S.coeffs[0];
-          done[fj] = true;
-        }
-      }
-    } else {
-      // A lone source form: its row map comes from a diffusion group on the
-      // same field when one exists.
-      int g = -1;
-      for(size_t k = 0; k < ctx->groups.size(); ++k)
-        if(ctx->groups[k].layoutI.size() == 1 && ctx->groups[k].layoutI[0] == field) g = (int)k;
-      FE_REQUIRE(ctx, g >= 0, "a SOURCE form needs a DIFFUSION form on the same field");
-      a.hasSrc = 1;
-      a.src = F.coeffs[0];
-      a.adr = ctx->groups[g].d_adrI;
-      a.pos = ctx->groups[g].d_pos;
-      if(!(what & 2)) { done[fi] = true; continue; }
+      a.coeff = ctx->forms[iDiff].coeffs[0];
     }
-    done[fi] = true;
+    if(iMass >= 0) {
+      a.hasMass = 1;
+      a.mass = ctx->forms[iMass].coeffs[0];
+      a.nsol = ctx->nsol;
+      for(int j = 0; j <= fe_hip_ctx::kMaxPast; ++j) a.cn[j] = ctx->bdf[j];
+      for(int j = 0; j < fe_hip_ctx::kMaxPast; ++j) a.past[j] = ctx->d_past[j];
+    }
+    if(iSrc >= 0) {
+      a.hasSrc = 1;
+      a.src = ctx->forms[iSrc].coeffs[0];
+    }
+    if(!(what & 2) && iDiff < 0 && iMass < 0) continue;  // a lone source has no matrix
     const int kind = ctx->fields[field].kind;
-    if(F.sysType == FE_SYS_DIFFUSION && ctx->asmKernel == 1) {
+    if(iDiff >= 0 && iMass < 0 && ctx->asmKernel == 1) {
+      const FeForm &F = ctx->forms[iDiff];
       MfmaAsmArgs m{};
       m.E = a.E; m.n = a.n; m.what = what; m.K = F.mfmaK;
       m.perm = a.perm; m.conn = a.conn; m.adr = a.adr; m.pos = a.pos;

@@ -274,6 +274,7 @@ int fe_hip_set_sizes(fe_hip_ctx *ctx, int nInc, int nDofs)
   if(!ctx) return FE_STATUS_POINTER_ERROR;
   FE_REQUIRE(ctx, nInc > 0 && nDofs >= nInc, "invalid system sizes (no unknowns)");
   FE_REQUIRE(ctx, !ctx->csrBuilt, "sizes are fixed once the CSR is built");
+  FE_REQUIRE(ctx, !ctx->d_past[0], "sizes are fixed once a solution history exists");
   ctx->n = nInc;
   ctx->nDofs = nDofs;
   if(feAlloc(ctx, &ctx->d_u, nDofs) || feAlloc(ctx, &ctx->d_rhs, nInc) || feAlloc(ctx, &ctx->d_du, nInc))
@@ -303,6 +304,11 @@ int fe_hip_add_form(fe_hip_ctx *ctx, int sysType, int nfields, const int *fields
       FE_REQUIRE(ctx, nfields == 1 && k0 != FE_TRI_VP2, "DIFFUSION takes one scalar field");
       F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = true;
       FE_REQUIRE(ctx, ncoeff == 1, "DIFFUSION takes one coefficient");
+      break;
+    case FE_SYS_TRANSIENT_MASS:
+      FE_REQUIRE(ctx, nfields == 1 && k0 != FE_TRI_VP2, "TRANSIENT_MASS takes one scalar field");
+      F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = true;
+      FE_REQUIRE(ctx, ncoeff == 1, "TRANSIENT_MASS takes one coefficient");
       break;
     case FE_SYS_SOURCE:
       FE_REQUIRE(ctx, nfields == 1 && k0 != FE_TRI_VP2, "SOURCE takes one scalar field");
@@ -416,6 +422,78 @@ int fe_hip_get_solution(fe_hip_ctx *ctx, double *u)
   return FE_STATUS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Solution history of the BDF schemes (feSolutionContainer.cpp:82-99,291-305).
+static int fePastReady(fe_hip_ctx *ctx)
+{
+  FE_REQUIRE(ctx, ctx->d_u, "set the sizes first");
+  for(int j = 0; j < fe_hip_ctx::kMaxPast; ++j) {
+    if(ctx->d_past[j]) continue;
+    if(feAlloc(ctx, &ctx->d_past[j], (size_t)ctx->nDofs)) return FE_STATUS_OUT_OF_MEMORY;
+    FE_HIP_TRY(ctx, hipMemsetAsync(ctx->d_past[j], 0, sizeof(double) * ctx->nDofs, ctx->stream));
+  }
+  return FE_STATUS_OK;
+}
+
+int fe_hip_set_bdf(fe_hip_ctx *ctx, int nsol, const double *cn)
+{
+  if(!ctx || !cn) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, nsol >= 1 && nsol <= fe_hip_ctx::kMaxPast + 1, "nsol must be in [1, 5]");
+  if(nsol > 1 && fePastReady(ctx)) return FE_STATUS_ERROR;
+  ctx->nsol = nsol;
+  for(int j = 0; j <= fe_hip_ctx::kMaxPast; ++j) ctx->bdf[j] = j < nsol ? cn[j] : 0.;
+  return FE_STATUS_OK;
+}
+
+int fe_hip_rotate_solutions(fe_hip_ctx *ctx)
+{
+  if(!ctx) return FE_STATUS_POINTER_ERROR;
+  if(fePastReady(ctx)) return FE_STATUS_ERROR;
+  // The oldest buffer is recycled as sol[1]; the others move down one slot.
+  double *last = ctx->d_past[fe_hip_ctx::kMaxPast - 1];
+  for(int j = fe_hip_ctx::kMaxPast - 1; j > 0; --j) ctx->d_past[j] = ctx->d_past[j - 1];
+  ctx->d_past[0] = last;
+  FE_HIP_TRY(ctx, hipMemcpyAsync(last, ctx->d_u, sizeof(double) * ctx->nDofs, hipMemcpyDeviceToDevice,
+                                 ctx->stream));
+  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return FE_STATUS_OK;
+}
+
+int fe_hip_set_past_solution(fe_hip_ctx *ctx, int j, const double *u)
+{
+  if(!ctx || !u) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, j >= 1 && j <= fe_hip_ctx::kMaxPast, "past solution index must be in [1, 4]");
+  if(fePastReady(ctx)) return FE_STATUS_ERROR;
+  FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_past[j - 1], u, sizeof(double) * ctx->nDofs, hipMemcpyHostToDevice,
+                                 ctx->stream));
+  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return FE_STATUS_OK;
+}
+
+int fe_hip_get_past_solution(fe_hip_ctx *ctx, int j, double *u)
+{
+  if(!ctx || !u) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, j >= 1 && j <= fe_hip_ctx::kMaxPast, "past solution index must be in [1, 4]");
+  if(fePastReady(ctx)) return FE_STATUS_ERROR;
+  FE_HIP_TRY(ctx, hipMemcpyAsync(u, ctx->d_past[j - 1], sizeof(double) * ctx->nDofs, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return FE_STATUS_OK;
+}
+
+int fe_hip_set_essential(fe_hip_ctx *ctx, const double *values)
+{
+  if(!ctx || !values) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, ctx->d_u, "set the sizes first");
+  const int ne = ctx->nDofs - ctx->n;
+  if(ne > 0) {
+    FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_u + ctx->n, values, sizeof(double) * ne, hipMemcpyHostToDevice,
+                                   ctx->stream));
+    FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return FE_STATUS_OK;
+}
+
 int fe_hip_get_system(fe_hip_ctx *ctx, double *val, double *rhs)
 {
   if(!ctx) return FE_STATUS_POINTER_ERROR;
@@ -516,7 +594,11 @@ int fe_hip_update_coeff(fe_hip_ctx *ctx, int form, int index, const double *data
   FeForm &F = ctx->forms[form];
   FE_REQUIRE(ctx, index >= 0 && index < (int)F.coeffs.size(), "unknown coefficient");
   FeFn &fn = F.coeffs[index];
-  FE_REQUIRE(ctx, fn.kind == FE_FN_ELEM || fn.kind == FE_FN_QP, "only tabulated coefficients can be replaced");
+  if(fn.kind != FE_FN_ELEM && fn.kind != FE_FN_QP) {
+    FE_REQUIRE(ctx, count == 1, "an analytic coefficient takes one value (its c0)");
+    fn.c0 = data[0];
+    return FE_STATUS_OK;
+  }
   const int64_t need = (fn.kind == FE_FN_ELEM) ? ctx->E : (int64_t)ctx->E * F.tab.nq;
   FE_REQUIRE(ctx, count == need, "coefficient table has the wrong size");
   FE_HIP_TRY(ctx, hipMemcpyAsync(fn.data, data, sizeof(double) * need, hipMemcpyHostToDevice, ctx->stream));

@@ -146,6 +146,13 @@ struct fe_hip_ctx {
   float *d_val32 = nullptr;
   double *d_rhs = nullptr, *d_u = nullptr, *d_du = nullptr;
   bool csrBuilt = false;
+  // Time integration: u_dot = sum_{j < nsol} bdf[j] sol[j] with sol[0] = d_u
+  // (the current iterate) and sol[j] = d_past[j - 1] (nDofs each, allocated
+  // on first use); TRANSIENT_MASS matrices carry c0 = bdf[0].
+  static constexpr int kMaxPast = 4;
+  double *d_past[kMaxPast] = {};
+  int nsol = 1;
+  double bdf[kMaxPast + 1] = {};
   // launch shapes of the Krylov iteration kernels (fe_hip_set_tuning)
   int spmvLanes = 4;
   int spmvUnroll = 1;

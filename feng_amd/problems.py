@@ -2,7 +2,8 @@
 
 `diffusion_system` wires the reference's scalar diffusion set-up
 (createFiniteElementSpace x2, feMetaNumber, feSolution, Diffusion + Source
-forms, createLinearSystem -- exe/example1.cpp:125-160) onto the device, and
+forms, createLinearSystem -- exe/example1.cpp:125-160) onto the device,
+`heat_system` the transient one (TransientMass + Diffusion + Source), and
 `solve_newton` is the Newton-Raphson loop of solveNewtonRaphson
 (feNonLinearSolver.cpp:38-182) driving it.
 """
@@ -47,6 +48,28 @@ def diffusion_system(mesh: A.Mesh, kind: int, degree: int, k: A.Fn, s: A.Fn, u_b
     S.build_csr()
     S.set_solution(A.essential_values(num, [u_bc]))
     return S, num
+
+
+def heat_system(mesh: A.Mesh, kind: int, degree: int, m: A.Fn, k: A.Fn, s: A.Fn, essential=True, coloring=None,
+                device=0):
+    """Transient scalar m u_t - div(k grad u) + S = 0: the form stack of the
+    reference's unsteady scalar tests (TransientMass, Diffusion, Source --
+    scalarFEunsteady.cpp:270-300), on the GPU.  essential: Dirichlet boundary
+    (True) or none (False).  k = None leaves out the diffusion form: with no
+    essential boundary that is the ODE m u_t + S = 0 of
+    scalarFEunsteady.cpp:54-80.  Returns (S, num, forms) with forms =
+    {"mass", "diffusion", "source"} ids for fe_hip_update_coeff."""
+    num = A.Numbering(mesh, [kind], [1 if essential else 0])
+    S = A.LinearSystemHIP(device)
+    S.upload_mesh(mesh, coloring)
+    f = S.add_field(kind, num.adr[0])
+    S.set_sizes(num.nInc, num.nDofs)
+    forms = {"mass": S.add_form(A.TRANSIENT_MASS, [f], degree, [m])}
+    if k is not None:
+        forms["diffusion"] = S.add_form(A.DIFFUSION, [f], degree, [k])
+    forms["source"] = S.add_form(A.SOURCE, [f], degree, [s])
+    S.build_csr()
+    return S, num, forms
 
 
 def solve_newton(S: A.LinearSystemHIP, opts: NLSolverOptions = NLSolverOptions(), progress=None,

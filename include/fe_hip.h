@@ -32,6 +32,7 @@ typedef struct fe_hip_ctx fe_hip_ctx;
  * for the terms the device kernels implement. */
 #define FE_SYS_SOURCE 0
 #define FE_SYS_VECTOR_SOURCE 2
+#define FE_SYS_TRANSIENT_MASS 13
 #define FE_SYS_DIFFUSION 16
 #define FE_SYS_VECTOR_DIFFUSION 18
 #define FE_SYS_VECTOR_CONVECTIVE_ACCELERATION 22
@@ -206,8 +207,34 @@ int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value);
 /* Replace the values of coefficient `index` of form `form` (an FE_FN_ELEM or
  * FE_FN_QP table, same length as given to fe_hip_add_form).  Replaces the
  * reference's per-instance feFunction objects (feFunction.h) when many
- * coefficient fields are assembled on one mesh. */
+ * coefficient fields are assembled on one mesh.  For the analytic kinds
+ * count = 1 replaces c0: a callback of time only (feFunctionArguments::t,
+ * feFunction.h) is set this way before every time step. */
 int fe_hip_update_coeff(fe_hip_ctx *ctx, int form, int index, const double *data, int64_t count);
+
+/* ---- time integration (BDF) ---------------------------------------------
+ * The solution history of feSolutionContainer / BDFContainer
+ * (feSolutionContainer.cpp:82-99,291-305,338-348) kept on the device: sol[0]
+ * is the current solution (fe_hip_set_solution), sol[1..4] are past
+ * solutions.  FE_SYS_TRANSIENT_MASS (feSysElm.cpp:426-458) assembles
+ * Ae += phi_i m c0 phi_j and Be -= phi_i m u_dot with
+ * u_dot = sum_{j < nsol} cn[j] sol[j] and c0 = cn[0], gathered per element
+ * from the history vectors.
+ *
+ * fe_hip_set_bdf: setC0 + computeBDFCoefficients; nsol in [1, 5] (the
+ * reference's order 0 is nsol = 1, cn = {0}).
+ * fe_hip_rotate_solutions: feSolutionContainer::rotate: sol[j] = sol[j - 1]
+ * for j = 4..1; sol[0] keeps its values (Newton's initial guess).
+ * fe_hip_set_past_solution: sol[j] (1 <= j <= 4) from nDofs host values
+ * (initial conditions, the exact-solution BDF2 starter, restarts);
+ * fe_hip_get_past_solution copies it back.
+ * fe_hip_set_essential: the essential part of sol[0] (nDofs - nInc values,
+ * feSolution::initializeEssentialBC at the new time). */
+int fe_hip_set_bdf(fe_hip_ctx *ctx, int nsol, const double *cn);
+int fe_hip_rotate_solutions(fe_hip_ctx *ctx);
+int fe_hip_set_past_solution(fe_hip_ctx *ctx, int j, const double *u);
+int fe_hip_get_past_solution(fe_hip_ctx *ctx, int j, double *u);
+int fe_hip_set_essential(fe_hip_ctx *ctx, const double *values);
 
 /* ---- aggregation multigrid (FE_PC_AMG) ----------------------------------
  * The algebraic multigrid the reference reaches through PETSc's runtime

@@ -15,6 +15,10 @@
  *     tiny systems; IC(0)/ILU(0) reproduce the exact factor on a tridiagonal
  *     matrix (no fill), where they equal the complete Cholesky / LU factor.
  *   - or_l2_error_sinsin: reference convergence table scalarFE_diffusion.output.
+ *   - or_element_mass / or_assemble_transient_mass: closed-form P1 mass
+ *     matrices (|T|/12 (1 + d_ij) triangles, |T|/20 (1 + d_ij) tets), the P2
+ *     row sums (0 at vertices, |T|/3 at edges), and, through the BDF oracle
+ *     (fe_oracle_time.py), the reference's scalarFEunsteady_linearODE tables.
  */
 #include "fe_oracle.h"
 
@@ -671,6 +675,73 @@ int or_assemble_diffusion(int kind, int V, const double *xyz, int E, const int *
             val[p] += Ae[i * nf + j];
           }
       }
+    }
+  }
+  return 0;
+}
+
+/* Transient mass on one element, feSysElm.cpp:426-458: Ae_ij += phi_i m c0
+ * phi_j J w (loop order q, i, j); Be_i -= phi_i m udot_h J w with udot_h the
+ * interpolation of the nodal time derivative. */
+static void transient_mass_elem(const or_ref *R, const double *c, int e, or_fn coeff, double c0,
+                                const double *udl, double *Ae, double *Be)
+{
+  or_geo g;
+  elem_geo(R, c, &g);
+  const int nf = R->nf;
+  for(int k = 0; k < R->nq; ++k) {
+    double x[3];
+    qp_pos(R, c, k, x);
+    const double m = fn_eval(coeff, x, e, R->nq, k);
+    if(Ae)
+      for(int i = 0; i < nf; ++i)
+        for(int j = 0; j < nf; ++j) Ae[i * nf + j] += R->L[k][i] * m * c0 * R->L[k][j] * g.J * R->qw[k];
+    if(Be) {
+      double ud = 0.;
+      for(int j = 0; j < nf; ++j) ud += R->L[k][j] * udl[j];
+      for(int i = 0; i < nf; ++i) Be[i] -= R->L[k][i] * m * ud * g.J * R->qw[k];
+    }
+  }
+}
+
+int or_element_mass(int kind, const double *coords, int quad_degree, or_fn coeff, int elem, double *Ae)
+{
+  or_ref R;
+  if(ref_setup(kind, quad_degree, &R)) return -1;
+  memset(Ae, 0, sizeof(double) * R.nf * R.nf);
+  transient_mass_elem(&R, coords, elem, coeff, 1., NULL, Ae, NULL);
+  return R.nf;
+}
+
+int or_assemble_transient_mass(int kind, int V, const double *xyz, int E, const int *conn,
+                               const int *adr, int n, const int *ia, const int *ja, int quad_degree,
+                               or_fn coeff, double c0, const double *udot, double *val, double *rhs)
+{
+  (void)V;
+  or_ref R;
+  if(ref_setup(kind, quad_degree, &R)) return -1;
+  const int nf = R.nf, nv = R.nv;
+  double c[12], Ae[144], Be[12], udl[12];
+  for(int e = 0; e < E; ++e) {
+    for(int j = 0; j < nv; ++j)
+      for(int d = 0; d < 3; ++d) c[3 * j + d] = xyz[3 * conn[e * nv + j] + d];
+    const int *a = adr + (size_t)e * nf;
+    for(int i = 0; i < nf; ++i) udl[i] = udot[a[i]];
+    memset(Ae, 0, sizeof(Ae));
+    memset(Be, 0, sizeof(Be));
+    or_geo g;
+    if(elem_geo(&R, c, &g)) return -2 - e;
+    transient_mass_elem(&R, c, e, coeff, c0, udl, val ? Ae : NULL, Be);
+    for(int i = 0; i < nf; ++i) {
+      if(a[i] >= n) continue;
+      rhs[a[i]] += Be[i];
+      if(val)
+        for(int j = 0; j < nf; ++j) {
+          if(a[j] >= n) continue;
+          int p = find_pos(ia, ja, a[i], a[j]);
+          if(p < 0) return -1;
+          val[p] += Ae[i * nf + j];
+        }
     }
   }
   return 0;

@@ -85,6 +85,15 @@ int or_assemble_diffusion_omp(int kind, int V, const double *xyz, int E, const i
 int or_element_diffusion(int kind, const double *coords, int quad_degree, or_fn coeff,
                          int elem, double *Ae);
 
+/* Transient mass (feSysElm.cpp:426-458): Ae += phi_i m c0 phi_j, Be -=
+ * phi_i m udot_h with udot the nodal time derivative (nDofs).  val may be
+ * NULL (residual only).  Adds into val/rhs. */
+int or_assemble_transient_mass(int kind, int V, const double *xyz, int E, const int *conn,
+                               const int *adr, int n, const int *ia, const int *ja, int quad_degree,
+                               or_fn coeff, double c0, const double *udot, double *val, double *rhs);
+/* Element mass matrix (coefficient m, c0 = 1) of one element (for unit checks). */
+int or_element_mass(int kind, const double *coords, int quad_degree, or_fn coeff, int elem, double *Ae);
+
 /* Taylor-Hood P2-P1 Navier-Stokes forms (feVectorSysElm.cpp).  adrU: E*12,
  * adrP: E*3.  Form stack: convective (c=-1), mixed divergence (c=1), zero
  * vector source, divergence of the Newtonian stress (c=1, mu). */

@@ -40,6 +40,9 @@ class Oracle:
         L.or_assemble_diffusion.argtypes = [C.c_int, C.c_int, dp, C.c_int, ip, ip, C.c_int, ip, ip, C.c_int,
                                             OrFn, OrFn, dp, ip, dp, dp]
         L.or_element_diffusion.argtypes = [C.c_int, dp, C.c_int, OrFn, C.c_int, dp]
+        L.or_element_mass.argtypes = [C.c_int, dp, C.c_int, OrFn, C.c_int, dp]
+        L.or_assemble_transient_mass.argtypes = [C.c_int, C.c_int, dp, C.c_int, ip, ip, C.c_int, ip, ip, C.c_int,
+                                                 OrFn, C.c_double, dp, dp, dp]
         L.or_assemble_ns.argtypes = [C.c_int, dp, C.c_int, ip, ip, ip, C.c_int, ip, ip, C.c_int, C.c_double,
                                      dp, ip, dp, dp]
         L.or_spmv.argtypes = [C.c_int, ip, ip, dp, dp, dp]
@@ -148,6 +151,31 @@ class Oracle:
                                               P(val, C.c_double), P(rhs, C.c_double))
         assert rc == 0, rc
         return val, rhs
+
+    def assemble_transient_mass(self, kind, xyz, conn, adr, n, ia, ja, deg, coeff, c0, udot, val, rhs):
+        """Adds the transient-mass terms into val (None: residual only) and rhs."""
+        nvpe = 4 if kind == 4 else 3
+        E = conn.size // nvpe
+        kind_, a0, a1, data = coeff
+        o = OrFn(kind_, a0, a1, None)
+        if data is not None:
+            d = np.ascontiguousarray(data, np.float64)
+            o.data = P(d, C.c_double)
+        udot = np.ascontiguousarray(udot, np.float64)
+        rc = self.L.or_assemble_transient_mass(kind, xyz.size // 3, P(xyz, C.c_double), E, P(conn, C.c_int),
+                                               P(adr, C.c_int), n, P(ia, C.c_int), P(ja, C.c_int), deg, o, c0,
+                                               P(udot, C.c_double), None if val is None else P(val, C.c_double),
+                                               P(rhs, C.c_double))
+        assert rc == 0, rc
+
+    def element_mass(self, kind, coords, deg, coeff=(0, 1.0, 0.0, None), elem=0):
+        nf = {1: 3, 2: 6, 4: 4}[kind]
+        Ae = np.zeros(nf * nf)
+        c = np.ascontiguousarray(coords, np.float64)
+        rc = self.L.or_element_mass(kind, P(c, C.c_double), deg, OrFn(coeff[0], coeff[1], coeff[2], None), elem,
+                                    P(Ae, C.c_double))
+        assert rc == nf, rc
+        return Ae.reshape(nf, nf)
 
     def cg_jacobi_omp(self, ia, ja, val, b, rtol, maxit, threads):
         x = np.zeros(b.size)

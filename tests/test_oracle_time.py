@@ -1,0 +1,167 @@
+"""Pins of the BDF oracle (oracle/fe_oracle_time.py) and of the transient
+mass form of the C oracle (or_element_mass / or_assemble_transient_mass).
+
+- BDF coefficients differentiate polynomials of degree <= order exactly on
+  variable steps (the defining property of the BDF formulas).
+- Mass matrices: the closed-form P1 triangle / tet element matrices, the P2
+  triangle row sums (0 at the vertices, |T|/3 at the edge midpoints).
+- The integrator reproduces the reference's golden tables
+  scalarFEunsteady_linearODE.output (BDF1; BDF2 with the exact-solution
+  starter) on u' = 4 t^3, u(0) = 0, through the C oracle's assembly.
+- With the BDF1 starter, BDF2 integrates a solution linear in time exactly.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import fe_oracle_time as ot
+
+# scalarFEunsteady_linearODE.output (tests/withLinearSolver/ of the
+# reference): max over the steps of the L-infinity error of u' = 4 t^3,
+# u(0) = 0 on [0, 1] with 10, 20, 40, 80, 160 steps (P1, degree-8 quadrature).
+LINEAR_ODE_TABLE = {
+    "BDF1": [2.100000e-01, 1.025000e-01, 5.062500e-02, 2.515625e-02, 1.253906e-02],
+    "BDF2": [3.429999e-02, 9.268750e-03, 2.407422e-03, 6.133545e-04, 1.547897e-04],
+}
+
+
+def table_matches(err, ref):
+    """The reference prints %e (7 significant digits)."""
+    return all(abs(e - r) <= 1e-6 * r for e, r in zip(err, ref))
+
+
+class OracleScalarTransient:
+    """Transient scalar problem assembled by the C oracle: TRANSIENT_MASS
+    (coefficient mass), DIFFUSION (coefficient k), SOURCE (src(t) -> an
+    or_fn tuple) on one scalar field."""
+
+    def __init__(self, oracle, mesh, kind, deg, k, mass, src, essential_flag):
+        from feng_amd.api import Numbering
+        self.o, self.m, self.kind, self.deg = oracle, mesh, kind, deg
+        self.num = Numbering(mesh, [kind], [essential_flag])
+        self.n, self.nDofs = self.num.nInc, self.num.nDofs
+        nf = {1: 3, 2: 6, 4: 4}[kind]
+        self.ia, self.ja = oracle.csr(self.n, [(self.num.adr[0], nf, self.num.adr[0], nf)])
+        self.k, self.mass, self.src = k, mass, src
+
+    def assemble(self, u, udot, c0, t):
+        o, m = self.o, self.m
+        val, rhs = o.assemble_diffusion(self.kind, m.xyz, m.conn, self.num.adr[0], self.n, self.ia, self.ja,
+                                        self.deg, self.k, self.src(t), u)
+        o.assemble_transient_mass(self.kind, m.xyz, m.conn, self.num.adr[0], self.n, self.ia, self.ja, self.deg,
+                                  self.mass, c0, udot, val, rhs)
+        return sp.csr_matrix((val, self.ja, self.ia), shape=(self.n, self.n)), rhs
+
+
+def linear_ode_errors(oracle, scheme):
+    from feng_amd.api import Mesh
+    m = Mesh.square(3)
+    prob = OracleScalarTransient(oracle, m, 1, 8, (0, 0.0, 0.0, None), (0, 1.0, 0.0, None),
+                                 lambda t: (0, -4.0 * t ** 3, 0.0, None), 0)
+    out = []
+    for nsteps in (10, 20, 40, 80, 160):
+        order = 1 if scheme == "BDF1" else 2
+        times, sols = ot.integrate(prob.assemble, np.zeros(prob.nDofs), prob.n, order, 0.0, 1.0, nsteps,
+                                   start="InitialCondition", exact=lambda t: np.full(prob.nDofs, t ** 4))
+        out.append(max(float(np.abs(u - t ** 4).max()) for t, u in zip(times, sols)))
+    return out
+
+
+@pytest.mark.parametrize("scheme", ["BDF1", "BDF2"])
+def test_oracle_reproduces_reference_linear_ode_table(oracle, scheme):
+    err = linear_ode_errors(oracle, scheme)
+    assert table_matches(err, LINEAR_ODE_TABLE[scheme]), err
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_bdf_coefficients_are_exact_on_polynomials(order):
+    rng = np.random.default_rng(order)
+    for _ in range(20):
+        h = rng.uniform(0.05, 1.0, 2)
+        t = np.array([0.0, -h[0], -h[0] - h[1]])
+        cn = ot.bdf_coefficients(order, h)
+        for p in range(order + 1):
+            d = sum(c * tj ** p for c, tj in zip(cn, t))
+            assert abs(d - (1.0 if p == 1 else 0.0)) <= 1e-12 * max(1.0, np.abs(cn).max()), (order, p)
+        # and not one degree higher (the scheme's order is exactly `order`)
+        p = order + 1
+        assert abs(sum(c * tj ** p for c, tj in zip(cn, t))) > 1e-6
+
+
+def _area(xy):
+    return 0.5 * abs((xy[1, 0] - xy[0, 0]) * (xy[2, 1] - xy[0, 1]) - (xy[1, 1] - xy[0, 1]) * (xy[2, 0] - xy[0, 0]))
+
+
+@pytest.mark.parametrize("deg", [2, 5, 8])
+def test_p1_triangle_mass_is_closed_form(oracle, deg):
+    rng = np.random.default_rng(deg)
+    for _ in range(4):
+        xy = rng.random((3, 2))
+        c = np.zeros((3, 3))
+        c[:, :2] = xy
+        M = oracle.element_mass(1, c.ravel(), deg, (0, 2.5, 0.0, None))
+        ref = 2.5 * abs(_area(xy)) / 12.0 * (np.ones((3, 3)) + np.eye(3))
+        # the Jacobian is signed: the element matrix carries its orientation
+        assert np.abs(np.abs(M) - ref).max() <= 1e-14 * ref.max()
+
+
+def test_p1_tet_mass_is_closed_form(oracle):
+    rng = np.random.default_rng(7)
+    for _ in range(4):
+        c = rng.random((4, 3))
+        vol = abs(np.linalg.det(c[1:] - c[0])) / 6.0
+        M = oracle.element_mass(4, c.ravel(), 2)
+        ref = vol / 20.0 * (np.ones((4, 4)) + np.eye(4))
+        # degree-2 tet rule with 12-digit points: exact to ~1e-12
+        assert np.abs(np.abs(M) - ref).max() <= 1e-11 * ref.max()
+
+
+def test_p2_triangle_mass_row_sums(oracle):
+    rng = np.random.default_rng(3)
+    for _ in range(3):
+        xy = rng.random((3, 2))
+        c = np.zeros((3, 3))
+        c[:, :2] = xy
+        M = oracle.element_mass(2, c.ravel(), 8)
+        s = M.sum(axis=1) * np.sign(M.sum())
+        A = _area(xy)
+        assert np.abs(s[:3]).max() <= 1e-14 * A
+        assert np.abs(s[3:] - A / 3.0).max() <= 1e-14 * A
+
+
+def test_bdf2_with_bdf1_starter_is_exact_for_linear_time(oracle):
+    """u(x, t) = (1 + 2t) g(x) with g linear in space solves u_t = 2 g with
+    zero diffusion flux divergence; every BDF step is exact for it."""
+    from feng_amd.api import Mesh
+    m = Mesh.square(4)
+    prob = OracleScalarTransient(oracle, m, 1, 2, (0, 1.0, 0.0, None), (0, 3.0, 0.0, None), None, 1)
+    xyz = prob.num.dof_xyz.reshape(-1, 3)
+    g = 1.0 + 0.5 * xyz[:, 0] - 0.25 * xyz[:, 1]
+    # m u_t - div(k grad u) + S = 0 with S = -m u_t = -6 g, given at the
+    # quadrature points (degree 2 integrates phi_i g exactly)
+    tab = _qp_table(m, 2, lambda x, y: -6.0 * (1.0 + 0.5 * x - 0.25 * y))
+    prob.src = lambda t: (5, 0.0, 0.0, tab)
+    ex = lambda t: (1.0 + 2.0 * t) * g
+    times, sols = ot.integrate(prob.assemble, ex(0.0), prob.n, 2, 0.0, 1.0, 5, start="BDF1",
+                               essential=lambda t: ex(t)[prob.n:])
+    for t, u in zip(times, sols):
+        assert np.abs(u - ex(t)).max() <= 1e-11, t
+
+
+def _qp_table(mesh, deg, f):
+    """f at the physical quadrature points of every triangle (E x nq, the
+    FE_FN_QP layout), with the reference's degree-`deg` triangle rule."""
+    import ctypes as C
+    from tests.oracle_bind import P, load_oracle
+    o = load_oracle()
+    r, s, w = (np.zeros(16) for _ in range(3))
+    nq = o.L.or_quad_tri(deg, P(r, C.c_double), P(s, C.c_double), P(w, C.c_double))
+    xy = mesh.xyz.reshape(-1, 3)[:, :2]
+    conn = mesh.conn.reshape(-1, 3)
+    p0, p1, p2 = xy[conn[:, 0]], xy[conn[:, 1]], xy[conn[:, 2]]
+    out = np.zeros((conn.shape[0], nq))
+    for q in range(nq):
+        l0 = 1.0 - r[q] - s[q]
+        x = l0 * p0 + r[q] * p1 + s[q] * p2
+        out[:, q] = f(x[:, 0], x[:, 1])
+    return out.ravel()
