@@ -21,6 +21,7 @@ config.path);
 every path is reported in config.paths.
 
 --workload batched runs the 256-problem batched workload instead;
+--workload transient runs BDF2 steps of the heat equation on the same cube;
 --copy-probe runs a plain device copy (counter calibration for rocprofv3).
 
 The cpu_baseline leg times the CPU oracle (a single-threaded C restatement of
@@ -213,6 +214,103 @@ def bench_batched(args):
     print(json.dumps(out), flush=True)
 
 
+def transient_alg_bytes(E, V, nDofs, n, nnz, nsol):
+    """Algorithmic HBM bytes of one fused residual + matrix sweep of
+    TRANSIENT_MASS + DIFFUSION + SOURCE over P1 tets: per element the color
+    permutation (4), connectivity (16), row map (16), scatter positions (64)
+    and coefficient (8); the vertex coordinates (24 V), the current and past
+    solutions (8 nDofs nsol) read once; the CSR values (8 nnz) and RHS (8 n)
+    written once."""
+    return E * (4 + 16 + 16 + 64 + 8) + 24 * V + 8 * nDofs * nsol + 8 * nnz + 8 * n
+
+
+def cpu_transient_step(m, k, num, dt, threads):
+    """One BDF1 step of the heat equation by the oracle on the host cores:
+    colored OpenMP diffusion + source assembly, the transient-mass terms
+    (serial), threaded Jacobi-PCG to rel-res 1e-8."""
+    from tests.oracle_bind import load_oracle
+
+    O = load_oracle()
+    n = num.nInc
+    ia, ja = O.csr(n, [(num.adr[0], 4, num.adr[0], 4)])
+    colorOf, ncolors = O.color(m.conn, 4, m.V)
+    u = np.zeros(num.nDofs)
+    t0 = time.perf_counter()
+    val, rhs = O.assemble_diffusion_omp(4, m.xyz, m.conn, num.adr[0], n, ia, ja, 1, (4, 0.0, 0.0, k),
+                                        (0, -1.0, 0.0, None), u, colorOf, ncolors, threads)
+    O.assemble_transient_mass(4, m.xyz, m.conn, num.adr[0], n, ia, ja, 1, (0, 1.0, 0.0, None), 1.0 / dt,
+                              np.zeros(num.nDofs), val, rhs)
+    t_asm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    x, it, rr = O.cg_jacobi_omp(ia, ja, val, rhs, 1e-8, 20000, threads)
+    t_cg = time.perf_counter() - t0
+    return dict(t_asm=t_asm, t_cg=t_cg, iters=it, relres=rr)
+
+
+def bench_transient(args):
+    """Heat equation m u_t - div(k grad u) + S = 0 on the headline cube
+    (P1 tets, k = the same exp(GRF) field, m = 1, S = -1, u = 0 on the
+    boundary and at t = 0), BDF2 (BDF1 starter) with dt = 1e-3.  One step =
+    history rotation + BDF coefficients + Newton: fused residual + matrix
+    sweep, PCG to FP64 rel-res 1e-8, correction, residual sweep and check.
+    Paths: FP64 Jacobi-PCG and FP64 AMG-PCG; value = DOF-steps/s of the
+    faster."""
+    from feng_amd.api import FN_CONST, FN_ELEM, PC_AMG, PC_JACOBI, Fn, Mesh, TET_P1
+    from feng_amd.problems import NLSolverOptions, heat_system
+    from feng_amd.timeint import TimeIntegrator
+
+    dt = 1e-3
+    nsteps = max(args.steps, 20)
+    m = Mesh.cube(args.N)
+    k = m.coeff_grf(0.1, 1.5, args.nfeat, args.seed)
+    res = {}
+    for name, pc in (("jacobi", PC_JACOBI), ("amg", PC_AMG)):
+        S, num, forms = heat_system(m, TET_P1, 1, Fn(FN_CONST, 1.0), Fn(FN_ELEM, data=k), Fn(FN_CONST, -1.0))
+        ti = TimeIntegrator(S, 2, 0.0, dt * (args.warmup + 1 + nsteps), args.warmup + 1 + nsteps,
+                            np.zeros(num.nDofs), nl=NLSolverOptions(1e-10, 1e-10, 1e4, 10),
+                            solve_kw=dict(pc=pc, rtol=1e-8, check_every=32))
+        ti.make_steps(args.warmup + 1)  # the BDF2 starting step and warm-up steps
+        S.set_profiling(False)
+        t0 = time.perf_counter()
+        ti.make_steps(args.warmup + 1 + nsteps)
+        wall = (time.perf_counter() - t0) / nsteps
+        reps = ti.reports[-nsteps:]
+        res[name] = {"ms_per_step": 1e3 * wall, "dof_steps": num.nInc / wall,
+                     "asm_ms": float(np.mean([r.asm_ms for r in reps])),
+                     "solve_ms": float(np.mean([r.solve_ms for r in reps])),
+                     "newton_iterations": float(np.mean([r.newton_iterations for r in reps])),
+                     "linear_iterations": float(np.mean([r.linear_iterations for r in reps])),
+                     "sweeps_per_step": 2}
+        n, nnz = num.nInc, S.nnz
+        log("transient", name, res[name])
+        S.close()
+    best = max(res, key=lambda p: res[p]["dof_steps"])
+    B = res[best]
+    alg = transient_alg_bytes(m.E, m.V, num.nDofs, n, nnz, 3)
+    sweep_ms = B["asm_ms"] / B["sweeps_per_step"]
+    achieved = alg / (sweep_ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        c = cpu_transient_step(m, k, num, dt, threads)
+        cpu = {"value": n / (c["t_asm"] + c["t_cg"]), "unit": "DOF-steps/s", "cores": threads, "kind": "port",
+               "sample": (f"one BDF1 step of the same problem by the oracle: OpenMP assembly + serial transient "
+                          f"mass {c['t_asm']:.2f}s, threaded Jacobi-PCG {c['iters']} iterations {c['t_cg']:.2f}s")}
+        log("cpu", c)
+    out = {"metric": "DOF-steps/s of BDF2 heat-equation steps (assemble + solve to FP64 rel-res 1e-8 per step)",
+           "value": B["dof_steps"], "unit": "DOF-steps/s", "n_gpus": 1, "steps": nsteps, "warmup": args.warmup,
+           "ms_per_step": B["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic (seeded GRF coefficient on a generated Kuhn-tet cube)",
+           "config": {"workload": f"transient P1 tet heat equation, unit cube {args.N}^3 x6 Kuhn tets, "
+                                  f"k=exp(GRF l=0.1 s=1.5), BDF2 dt={dt}", "n": n, "nnz": nnz, "elements": m.E,
+                      "path": best, "paths": res},
+           "roofline": {"bound": "hbm", "kernel": "asmScalarFP64<4, 3> (fused residual + matrix sweep, all colors)",
+                        "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": None, "avg_sweep_ms": sweep_ms, "alg_bytes_per_sweep": alg},
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,7 +321,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=25.0)
-    ap.add_argument("--workload", default="cube", choices=["cube", "batched"])
+    ap.add_argument("--workload", default="cube", choices=["cube", "batched", "transient"])
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--paths", default=None, help="comma-separated subset of " + ",".join(PATHS))
     args = ap.parse_args()
@@ -235,6 +333,8 @@ def main():
         return copy_probe()
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "transient":
+        return bench_transient(args)
     rtol = 1e-8
     m, k, S, num = make_problem(args.N, args.nfeat, args.seed)
     n, nnz = num.nInc, S.nnz

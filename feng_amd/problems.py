@@ -31,6 +31,7 @@ class NewtonReport:
     residual_norms: list = field(default_factory=list)
     correction_norms: list = field(default_factory=list)
     linear: list = field(default_factory=list)
+    asm_ms: float = 0.0  # device time of the element sweeps of all iterations
     converged: bool = False
 
 
@@ -73,22 +74,32 @@ def heat_system(mesh: A.Mesh, kind: int, degree: int, m: A.Fn, k: A.Fn, s: A.Fn,
 
 
 def solve_newton(S: A.LinearSystemHIP, opts: NLSolverOptions = NLSolverOptions(), progress=None,
-                 **solve_kw) -> NewtonReport:
+                 fused=False, **solve_kw) -> NewtonReport:
     """solveNewtonRaphson (feNonLinearSolver.cpp:64-142): residual, stopping
     test, matrix, solve, u += du, until ||NL(u)||_inf <= tolResidual or
     ||du||_inf <= tolCorrection.  progress(iteration, SolveResult) is called
-    after every linear solve (the reference prints the same per-step norms)."""
+    after every linear solve (the reference prints the same per-step norms).
+    fused: the first iteration, which always needs the matrix, assembles
+    residual and matrix in one element sweep per form (same values)."""
     rep = NewtonReport()
     it = 0
+    prec = solve_kw.get("precision", A.PREC_FP64)
     while True:
         S.setToZero()
-        S.assembleResiduals()
+        both = fused and it == 0
+        if both:
+            S.assemble(3, prec)
+        else:
+            S.assembleResiduals()
+        rep.asm_ms += S.kernel_stats(1)[0]
         nres = S.getRHSMaxNorm()
         if it > 0 and nres <= opts.tolResidual:
             rep.residual_norms.append(nres)
             rep.converged = True
             break
-        S.assembleMatrices(solve_kw.get("precision", A.PREC_FP64))
+        if not both:
+            S.assembleMatrices(prec)
+            rep.asm_ms += S.kernel_stats(1)[0]
         r = S.solve(**solve_kw)
         rep.linear.append(r)
         if not r.converged and r.reason < 0 and r.reason != -3:

@@ -48,6 +48,8 @@ class StepReport:
     time: float
     newton_iterations: int
     linear_iterations: int
+    asm_ms: float = 0.0    # device time of the element sweeps
+    solve_ms: float = 0.0  # device time of the linear solves
 
 
 @dataclass
@@ -71,6 +73,7 @@ class TimeIntegrator:
     exact: object = None
     start: str = BDF2_START_BDF1
     solve_kw: dict = field(default_factory=dict)
+    fused_assembly: bool = True  # first Newton iteration: residual and matrix in one sweep
 
     def __post_init__(self):
         if self.order not in (1, 2):
@@ -99,10 +102,11 @@ class TimeIntegrator:
         self.S.rotate_solutions()
 
     def _newton(self):
-        rep = solve_newton(self.S, self.nl, **self.solve_kw)
+        rep = solve_newton(self.S, self.nl, fused=self.fused_assembly, **self.solve_kw)
         if not rep.converged:
             raise A.FeError(f"Newton did not converge at t = {self.time:.6e}")
-        self.reports.append(StepReport(self.time, rep.iterations, sum(r.iterations for r in rep.linear)))
+        self.reports.append(StepReport(self.time, rep.iterations, sum(r.iterations for r in rep.linear), rep.asm_ms,
+                                       sum(r.ms_solve for r in rep.linear)))
 
     def _post_process(self):
         self.post[0].append(self.time)
