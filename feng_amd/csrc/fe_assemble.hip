@@ -259,6 +259,7 @@ __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
 //   MixedDivergence {p, u}        feVectorSysElm.cpp:669-753
 //   VectorSource                  feVectorSysElm.cpp:101-132
 //   DivergenceNewtonianStress     feVectorSysElm.cpp:1434-1536
+//   TransientVectorMass           feVectorSysElm.cpp:391-425
 // 16 lanes per element: lane i < 12 owns velocity row i = 2a + c (function
 // psi_a e_c), lanes 12..14 own the pressure rows, lane 15 only loads.  The
 // element's local solution sits in LDS; every lane rebuilds the P2 gradients
@@ -268,11 +269,15 @@ __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
 struct NsAsmArgs {
   int begin, count, E, n, what;
   const int *perm, *conn;
-  const int *adrU, *adrP;  // color-sorted element maps (12, 3 per element)
-  const int *posConv, *posDiv, *posStr;
+  const int *adrU, *adrP;  // color-sorted element maps (12, 3 per element); adrP null: no pressure
+  const int *posUU, *posDiv, *posStr;  // posUU: the velocity-velocity group (convection and mass)
   const double *xyz, *tab, *u;
-  int hasConv, hasDiv, hasStr, hasSrc;
-  FeFn cConv, cDiv, cStr, mu, src0, src1;
+  int hasConv, hasDiv, hasStr, hasSrc, hasMass;
+  FeFn cConv, cDiv, cStr, mu, src0, src1, cMass;
+  // time derivative u_dot = cn[0] u + sum_{j = 1}^{nsol - 1} cn[j] past[j - 1]
+  int nsol;
+  double cn[fe_hip_ctx::kMaxPast + 1];
+  const double *past[fe_hip_ctx::kMaxPast];
   double *val, *rhs;
 };
 
@@ -282,6 +287,7 @@ __global__ __launch_bounds__(256) void asmNavierStokesFP64(NsAsmArgs a)
 {
   __shared__ FeRefTables T;
   __shared__ double su[kNsElemPerBlock][16];  // 12 velocity + 3 pressure values
+  __shared__ double sud[kNsElemPerBlock][12];  // velocity time derivative (TransientVectorMass)
   __shared__ double geo[kNsElemPerBlock][8];  // J, drdx0, drdx1, drdy0, drdy1, x0, y0 (pos origin)
   __shared__ double cxy[kNsElemPerBlock][6];
   {
@@ -294,10 +300,16 @@ __global__ __launch_bounds__(256) void asmNavierStokesFP64(NsAsmArgs a)
   const bool valid = t < a.count;
   const int ee = a.begin + (valid ? t : 0);
   const int e = a.perm[ee];
+  const bool hasP = a.adrP != nullptr;
   int myDof = 0;
   if(lane < 12) myDof = a.adrU[(size_t)ee * 12 + lane];
-  else if(lane < 15) myDof = a.adrP[(size_t)ee * 3 + lane - 12];
-  if(lane < 15) su[le][lane] = a.u[myDof];
+  else if(lane < 15 && hasP) myDof = a.adrP[(size_t)ee * 3 + lane - 12];
+  if(lane < 15) su[le][lane] = (lane < 12 || hasP) ? a.u[myDof] : 0.;
+  if(lane < 12 && a.hasMass) {
+    double v = a.cn[0] * su[le][lane];
+    for(int j = 1; j < a.nsol; ++j) v += a.cn[j] * a.past[j - 1][myDof];
+    sud[le][lane] = v;
+  }
   if(lane < 6) {
     const int v = a.conn[(size_t)e * 3 + lane / 2];
     cxy[le][lane] = a.xyz[3 * (size_t)v + (lane & 1)];
@@ -406,6 +418,21 @@ __global__ __launch_bounds__(256) void asmNavierStokesFP64(NsAsmArgs a)
         const double S = ci == 0 ? feEvalFn(a.src0, x, e, nq, k) : feEvalFn(a.src1, x, e, nq, k);
         Be -= S * psi * jw;
       }
+      if(a.hasMass) {
+        // phi_i . phi_j = psi_a psi_b delta(c_i, c_j)
+        const double cm = feEvalFn(a.cMass, x, e, nq, k);
+        if(doMat) {
+          const double f = cm * a.cn[0] * psi * jw;
+#pragma unroll
+          for(int b = 0; b < 6; ++b) Ac[2 * b + ci] += f * ps[b];
+        }
+        if(doRes) {
+          double udq = 0.;
+#pragma unroll
+          for(int b = 0; b < 6; ++b) udq += ps[b] * sud[le][2 * b + ci];
+          Be -= cm * udq * psi * jw;
+        }
+      }
     } else if(a.hasDiv) {
       const double cd = feEvalFn(a.cDiv, x, e, nq, k);
       if(doMat) {
@@ -418,13 +445,13 @@ __global__ __launch_bounds__(256) void asmNavierStokesFP64(NsAsmArgs a)
       if(doRes) Be -= cd * (g0 + g3) * pp[ip] * jw;
     }
   }
-  if(doRes && myDof < a.n) a.rhs[myDof] += Be;
+  if(doRes && (uRow || hasP) && myDof < a.n) a.rhs[myDof] += Be;
   if(!doMat) return;
   if(uRow) {
-    if(a.hasConv)
+    if(a.hasConv || a.hasMass)
 #pragma unroll
       for(int j = 0; j < 12; ++j) {
-        const int p = a.posConv[(size_t)(lane * 12 + j) * a.E + ee];
+        const int p = a.posUU[(size_t)(lane * 12 + j) * a.E + ee];
         if(p >= 0) a.val[p] += Ac[j];
       }
     if(a.hasStr)
@@ -646,13 +673,15 @@ __global__ void maxAbsBlocks(int n, const double *__restrict__ x, double *__rest
   }
 }
 
+// One wavefront folds the block maxima (strided loads in flight together,
+// then a shuffle tree); launched with 64 threads.
 __global__ void maxFinal(const double *__restrict__ partials, int G, double *__restrict__ out)
 {
-  if(threadIdx.x == 0) {
-    double r = 0.;
-    for(int b = 0; b < G; ++b) r = fmax(r, partials[b]);
-    *out = r;
-  }
+  double r = 0.;
+  for(int b = threadIdx.x; b < G; b += 64) r = fmax(r, partials[b]);
+#pragma unroll
+  for(int o = 32; o > 0; o >>= 1) r = fmax(r, __shfl_xor(r, o, 64));
+  if(threadIdx.x == 0) *out = r;
 }
 
 __global__ void addCorrection(int n, double *__restrict__ u, const double *__restrict__ du)
@@ -781,29 +810,35 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
     else if(kind == FE_TET_P1) launchScalar<4, 3>(ctx, a);
     else FE_REQUIRE(ctx, false, "scalar forms need a scalar field");
   }
-  // Taylor-Hood stack: all vector/mixed forms fused into one sweep per color.
+  // Taylor-Hood stack: all vector/mixed forms fused into one sweep per color
+  // (one quadrature rule: the forms must share their degree).
   {
     NsAsmArgs a{};
-    int any = 0, vfield = -1;
+    int any = 0, degree = -1;
+    const int *posConv = nullptr, *posMass = nullptr;
     for(size_t fi = 0; fi < ctx->forms.size(); ++fi) {
       FeForm &F = ctx->forms[fi];
       if(done[fi]) continue;
       switch(F.sysType) {
         case FE_SYS_VECTOR_CONVECTIVE_ACCELERATION:
           FE_REQUIRE(ctx, !a.hasConv, "two convective forms");
-          a.hasConv = 1; a.cConv = F.coeffs[0]; a.posConv = ctx->groups[F.group].d_pos;
-          a.adrU = ctx->groups[F.group].d_adrI; vfield = F.fields[0];
+          a.hasConv = 1; a.cConv = F.coeffs[0]; posConv = ctx->groups[F.group].d_pos;
+          a.adrU = ctx->groups[F.group].d_adrI;
+          break;
+        case FE_SYS_TRANSIENT_VECTOR_MASS:
+          FE_REQUIRE(ctx, !a.hasMass, "two vector transient mass forms");
+          a.hasMass = 1; a.cMass = F.coeffs[0]; posMass = ctx->groups[F.group].d_pos;
+          a.adrU = ctx->groups[F.group].d_adrI;
           break;
         case FE_SYS_MIXED_DIVERGENCE:
           FE_REQUIRE(ctx, !a.hasDiv, "two divergence forms");
           a.hasDiv = 1; a.cDiv = F.coeffs[0]; a.posDiv = ctx->groups[F.group].d_pos;
-          a.adrP = ctx->groups[F.group].d_adrI; vfield = F.fields[1];
+          a.adrP = ctx->groups[F.group].d_adrI;
           break;
         case FE_SYS_DIV_NEWTONIAN_STRESS:
           FE_REQUIRE(ctx, !a.hasStr, "two stress forms");
           a.hasStr = 1; a.cStr = F.coeffs[0]; a.mu = F.coeffs[1]; a.posStr = ctx->groups[F.group].d_pos;
-          a.adrU = ctx->groups[F.group].d_adrI; vfield = F.fields[0];
-          a.tab = F.d_tab;
+          a.adrU = ctx->groups[F.group].d_adrI;
           break;
         case FE_SYS_VECTOR_SOURCE:
           FE_REQUIRE(ctx, !a.hasSrc, "two vector sources");
@@ -811,12 +846,23 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
           break;
         default: continue;
       }
+      FE_REQUIRE(ctx, degree < 0 || F.degree == degree, "the vector forms must share one quadrature degree");
+      degree = F.degree;
+      if(!a.tab) a.tab = F.d_tab;
       done[fi] = true;
       any = 1;
     }
     if(any) {
-      FE_REQUIRE(ctx, a.hasStr && a.hasDiv, "the device Navier-Stokes kernel needs the stress and divergence forms");
-      (void)vfield;
+      FE_REQUIRE(ctx, a.adrU, "the vector forms need a velocity matrix form (stress, convection or transient mass)");
+      FE_REQUIRE(ctx, !a.hasStr || a.hasDiv, "the stress form needs the divergence form (pressure map)");
+      // convection and transient mass share the velocity-velocity layout, hence one group
+      FE_REQUIRE(ctx, !(posConv && posMass) || posConv == posMass, "velocity forms on different groups");
+      a.posUU = posConv ? posConv : posMass;
+      if(a.hasMass) {
+        a.nsol = ctx->nsol;
+        for(int j = 0; j <= fe_hip_ctx::kMaxPast; ++j) a.cn[j] = ctx->bdf[j];
+        for(int j = 0; j < fe_hip_ctx::kMaxPast; ++j) a.past[j] = ctx->d_past[j];
+      }
       a.E = ctx->E;
       a.n = ctx->n;
       a.what = what;

@@ -325,6 +325,11 @@ int fe_hip_add_form(fe_hip_ctx *ctx, int sysType, int nfields, const int *fields
       F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = true;
       FE_REQUIRE(ctx, ncoeff == 1, "VECTOR_CONVECTIVE_ACCELERATION takes one coefficient");
       break;
+    case FE_SYS_TRANSIENT_VECTOR_MASS:
+      FE_REQUIRE(ctx, nfields == 1 && k0 == FE_TRI_VP2, "TRANSIENT_VECTOR_MASS takes one vector field");
+      F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = true;
+      FE_REQUIRE(ctx, ncoeff == 1, "TRANSIENT_VECTOR_MASS takes one coefficient");
+      break;
     case FE_SYS_MIXED_DIVERGENCE:
       // fields {p, u}: scalar test functions, vector unknown (feVectorSysElm.cpp:669-683)
       FE_REQUIRE(ctx, nfields == 2 && ctx->fields[fields[0]].kind == FE_TRI_P1 &&

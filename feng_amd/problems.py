@@ -134,19 +134,23 @@ def kovasznay(Re):
     return lam, u, p
 
 
-def navier_stokes_system(mesh: A.Mesh, Re: float, degree: int = 5, coloring=None, device=0, source=None):
+def navier_stokes_system(mesh: A.Mesh, Re: float, degree: int = 5, coloring=None, device=0, source=None,
+                         mass=None):
     """Steady incompressible Navier-Stokes, Taylor-Hood P2-P1, divergence
     formulation (the reference's navier_stokes.cpp:66-99 form stack):
     convective acceleration (coeff -1), mixed divergence {p, u} (1), vector
     source, divergence of the Newtonian stress {u, p} (1, mu = 1/Re).  u and p
     are essential on the boundary with the Kovasznay values; the interior
-    starts from zero."""
+    starts from zero.  mass (an Fn): the unsteady equations, with a
+    TransientVectorMass form first (vectorFEunsteady.cpp:562)."""
     num = A.Numbering(mesh, [A.TRI_VP2, A.TRI_P1], [1, 1])
     S = A.LinearSystemHIP(device)
     S.upload_mesh(mesh, coloring)
     fu = S.add_field(A.TRI_VP2, num.adr[0])
     fp = S.add_field(A.TRI_P1, num.adr[1])
     S.set_sizes(num.nInc, num.nDofs)
+    if mass is not None:
+        S.add_form(A.TRANSIENT_VECTOR_MASS, [fu], degree, [mass])
     S.add_form(A.VECTOR_CONVECTIVE_ACCELERATION, [fu], degree, [A.Fn(A.FN_CONST, -1.0)])
     S.add_form(A.MIXED_DIVERGENCE, [fp, fu], degree, [A.Fn(A.FN_CONST, 1.0)])
     src = source or [A.Fn(A.FN_CONST, 0.0), A.Fn(A.FN_CONST, 0.0)]

@@ -33,6 +33,7 @@ typedef struct fe_hip_ctx fe_hip_ctx;
 #define FE_SYS_SOURCE 0
 #define FE_SYS_VECTOR_SOURCE 2
 #define FE_SYS_TRANSIENT_MASS 13
+#define FE_SYS_TRANSIENT_VECTOR_MASS 15
 #define FE_SYS_DIFFUSION 16
 #define FE_SYS_VECTOR_DIFFUSION 18
 #define FE_SYS_VECTOR_CONVECTIVE_ACCELERATION 22
@@ -219,7 +220,9 @@ int fe_hip_update_coeff(fe_hip_ctx *ctx, int form, int index, const double *data
  * solutions.  FE_SYS_TRANSIENT_MASS (feSysElm.cpp:426-458) assembles
  * Ae += phi_i m c0 phi_j and Be -= phi_i m u_dot with
  * u_dot = sum_{j < nsol} cn[j] sol[j] and c0 = cn[0], gathered per element
- * from the history vectors.
+ * from the history vectors; FE_SYS_TRANSIENT_VECTOR_MASS
+ * (feVectorSysElm.cpp:391-425) the same with phi_i . phi_j on the vector P2
+ * field, fused into the Taylor-Hood element sweep.
  *
  * fe_hip_set_bdf: setC0 + computeBDFCoefficients; nsol in [1, 5] (the
  * reference's order 0 is nsol = 1, cn = {0}).

@@ -91,6 +91,11 @@ int or_element_diffusion(int kind, const double *coords, int quad_degree, or_fn 
 int or_assemble_transient_mass(int kind, int V, const double *xyz, int E, const int *conn,
                                const int *adr, int n, const int *ia, const int *ja, int quad_degree,
                                or_fn coeff, double c0, const double *udot, double *val, double *rhs);
+/* TransientVectorMass (feVectorSysElm.cpp:391-425) on the vector P2 field
+ * (adrU: E*12), constant coefficient m; val may be NULL.  Adds into val/rhs. */
+int or_assemble_vector_transient_mass(int V, const double *xyz, int E, const int *conn, const int *adrU, int n,
+                                      const int *ia, const int *ja, int quad_degree, double m, double c0,
+                                      const double *udot, double *val, double *rhs);
 /* Element mass matrix (coefficient m, c0 = 1) of one element (for unit checks). */
 int or_element_mass(int kind, const double *coords, int quad_degree, or_fn coeff, int elem, double *Ae);
 

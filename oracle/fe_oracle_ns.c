@@ -15,6 +15,9 @@
  *
  * Parity pin: tests/test_oracle_golden.py compares the assembled matrix and
  * residual with the reference's own values (<= 1e-12) on jittered meshes.
+ * or_assemble_vector_transient_mass: its matrix is the scalar P2 mass
+ * matrix (or_element_mass) on each component, uncoupled
+ * (tests/test_oracle_time.py).
  */
 #include "fe_oracle.h"
 
@@ -187,6 +190,54 @@ int or_assemble_ns(int V, const double *xyz, int E, const int *conn, const int *
           if(aU[i] < n) rhs[aU[i]] += Bu[i];
       }
     }
+  }
+  return 0;
+}
+
+/* TransientVectorMass (feVectorSysElm.cpp:391-425) on the vector P2 field:
+ * Ae_ij += m c0 (phi_i . phi_j) J w (dotProductShapeShape), Be_i -= m
+ * (udot_h . phi_i) J w with udot_h interpolated from the nodal time
+ * derivative (dotProductShapeOther).  Constant coefficient m. */
+int or_assemble_vector_transient_mass(int V, const double *xyz, int E, const int *conn, const int *adrU, int n,
+                                      const int *ia, const int *ja, int quad_degree, double m, double c0,
+                                      const double *udot, double *val, double *rhs)
+{
+  (void)V;
+  double qr[16], qs[16], qw[16];
+  const int nq = or_quad_tri(quad_degree, qr, qs, qw);
+  if(nq <= 0) return -1;
+  for(int e = 0; e < E; ++e) {
+    double c[3][3];
+    for(int j = 0; j < 3; ++j)
+      for(int d = 0; d < 3; ++d) c[j][d] = xyz[3 * conn[3 * e + j] + d];
+    const double J = (-c[0][0] + c[1][0]) * (-c[0][1] + c[2][1]) - (-c[0][1] + c[1][1]) * (-c[0][0] + c[2][0]);
+    const int *aU = adrU + (size_t)e * 12;
+    double ud[12], A[144], B[12];
+    for(int i = 0; i < 12; ++i) ud[i] = udot[aU[i]];
+    memset(A, 0, sizeof(A));
+    memset(B, 0, sizeof(B));
+    for(int k = 0; k < nq; ++k) {
+      double L[6], dr[6], ds[6];
+      p2_tables(qr[k], qs[k], L, dr, ds);
+      double Ph[12][2];
+      for(int a = 0; a < 6; ++a) {
+        Ph[2 * a][0] = L[a]; Ph[2 * a][1] = 0.;
+        Ph[2 * a + 1][0] = 0.; Ph[2 * a + 1][1] = L[a];
+      }
+      double uq[2] = {0., 0.};
+      for(int f = 0; f < 12; ++f) {
+        uq[0] += ud[f] * Ph[f][0];
+        uq[1] += ud[f] * Ph[f][1];
+      }
+      const double jw = J * qw[k];
+      for(int i = 0; i < 12; ++i) {
+        for(int j = 0; j < 12; ++j) A[i * 12 + j] += m * c0 * (Ph[i][0] * Ph[j][0] + Ph[i][1] * Ph[j][1]) * jw;
+        B[i] -= m * (uq[0] * Ph[i][0] + uq[1] * Ph[i][1]) * jw;
+      }
+    }
+    for(int i = 0; i < 12; ++i)
+      if(aU[i] < n) rhs[aU[i]] += B[i];
+    if(val && scatter(ia, ja, n, aU, 12, aU, 12, A, val)) return -1;
   }
   return 0;
 }

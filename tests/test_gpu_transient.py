@@ -9,6 +9,10 @@ BDF history gathered per element, driven by feng_amd.timeint.
   at every step to 1e-10 relative: the same scheme, the device solving to
   rtol 1e-13 where the oracle solves directly.
 - The device history: rotation and set/get of past solutions.
+- TRANSIENT_VECTOR_MASS fused into the Taylor-Hood sweep: one residual +
+  matrix at a random state and BDF2 history equals the oracle to 1e-12, and
+  BDF2 steps of the unsteady Navier-Stokes stack (Newton with GMRES/ILU(0)
+  on the device, Newton with direct solves in the oracle) agree to 1e-9.
 """
 import numpy as np
 import pytest
@@ -120,4 +124,79 @@ def test_history_rotation_and_transfer():
     S.set_essential(v[5][num.nInc:])
     u = S.get_solution()
     assert np.array_equal(u[:num.nInc], v[0][:num.nInc]) and np.array_equal(u[num.nInc:], v[5][num.nInc:])
+    S.close()
+
+
+def _ns_oracle(oracle, m, num, Re, deg, mass):
+    import ctypes as C
+    import scipy.sparse as sp
+    from tests.oracle_bind import P
+    uU, pP = num.adr
+    forms = [(uU, 12, uU, 12), (pP, 3, uU, 12),
+             (uU, 12, np.hstack([uU.reshape(-1, 12), pP.reshape(-1, 3)]).ravel(), 15)]
+    ia, ja = oracle.csr(num.nInc, forms)
+    n = num.nInc
+
+    def assemble(u, udot, c0, t):
+        val, rhs = np.zeros(ja.size), np.zeros(n)
+        rc = oracle.L.or_assemble_ns(m.V, P(m.xyz, C.c_double), m.E, P(m.conn, C.c_int), P(uU, C.c_int),
+                                     P(pP, C.c_int), n, P(ia, C.c_int), P(ja, C.c_int), deg, 1.0 / Re,
+                                     P(np.ascontiguousarray(u), C.c_double), None, P(val, C.c_double),
+                                     P(rhs, C.c_double))
+        assert rc == 0
+        oracle.assemble_vector_transient_mass(m.xyz, m.conn, uU, n, ia, ja, deg, mass, c0, udot, val, rhs)
+        return sp.csr_matrix((val, ja, ia), shape=(n, n)), rhs
+    return ia, ja, assemble
+
+
+def test_vector_mass_assembly_matches_oracle(oracle):
+    """One BDF2 residual + matrix of the unsteady Taylor-Hood stack at a
+    random state and history: device vs oracle."""
+    from feng_amd.api import FN_CONST, Fn, Mesh
+    from feng_amd.problems import navier_stokes_system
+    m = Mesh.square(12, 0.2, 1)
+    S, num = navier_stokes_system(m, 40.0, 5, mass=Fn(FN_CONST, 1.5))
+    ia0, ja0, assemble = _ns_oracle(oracle, m, num, 40.0, 5, 1.5)
+    ia, ja = S.get_csr()
+    assert np.array_equal(ia, ia0) and np.array_equal(ja, ja0)
+    rng = np.random.default_rng(5)
+    u = S.get_solution()
+    u[:num.nInc] = 0.3 * rng.normal(size=num.nInc)
+    h1, h2 = u + 0.1 * rng.normal(size=u.size), u + 0.2 * rng.normal(size=u.size)
+    cn = [1.0 / 0.1 + 1.0 / 0.2, -1.0 / 0.1 - 1.0 / 0.1, 0.1 / 0.1 / 0.2]
+    S.set_solution(u)
+    S.set_past_solution(1, h1)
+    S.set_past_solution(2, h2)
+    S.set_bdf(cn)
+    S.setToZero()
+    S.assemble(3)
+    val, rhs = S.get_system()
+    A0, b0 = assemble(u, cn[0] * u + cn[1] * h1 + cn[2] * h2, cn[0], 0.0)
+    assert np.linalg.norm(val - A0.data) <= 1e-12 * np.linalg.norm(A0.data)
+    assert np.linalg.norm(rhs - b0) <= 1e-12 * np.linalg.norm(b0)
+    S.close()
+
+
+def test_unsteady_navier_stokes_matches_oracle(oracle):
+    """BDF2 (BDF1 starter) on the unsteady Taylor-Hood stack from rest with
+    Kovasznay boundary values: the device Newton + GMRES/ILU(0) steps equal
+    the oracle's Newton + direct solves at every step."""
+    from feng_amd.api import FN_CONST, KSP_GMRES, PC_ILU0, Fn, Mesh
+    from feng_amd.problems import NLSolverOptions, navier_stokes_system
+    from feng_amd.timeint import TimeIntegrator
+    print("unsteady NS", flush=True)
+    m = Mesh.square(10, 0.2, 2)
+    S, num = navier_stokes_system(m, 40.0, 5, mass=Fn(FN_CONST, 1.0))
+    _, _, assemble = _ns_oracle(oracle, m, num, 40.0, 5, 1.0)
+    u0 = S.get_solution()
+    sols = []
+    nl = NLSolverOptions(1e-11, 1e-12, 1e4, 12)
+    ti = TimeIntegrator(S, 2, 0.0, 0.2, 4, u0, nl=nl, norms=[lambda S_, t: sols.append(S_.get_solution()) or 0.0],
+                        solve_kw=dict(ksp=KSP_GMRES, pc=PC_ILU0, rtol=1e-13, restart=50, max_iter=20000))
+    ti.make_steps()
+    times, ref = ot.integrate(assemble, u0, num.nInc, 2, 0.0, 0.2, 4, start="BDF1", tol=1e-11, max_newton=12)
+    assert len(sols) == len(ref) == 5
+    for t, u, r in zip(times, sols, ref):
+        assert np.linalg.norm(u - r) <= 1e-9 * np.linalg.norm(r), (t, np.linalg.norm(u - r) / np.linalg.norm(r))
+    assert np.linalg.norm(sols[-1][:num.nInc]) > 1e-2 * np.linalg.norm(sols[-1])
     S.close()

@@ -4,7 +4,9 @@ mass form of the C oracle (or_element_mass / or_assemble_transient_mass).
 - BDF coefficients differentiate polynomials of degree <= order exactly on
   variable steps (the defining property of the BDF formulas).
 - Mass matrices: the closed-form P1 triangle / tet element matrices, the P2
-  triangle row sums (0 at the vertices, |T|/3 at the edge midpoints).
+  triangle row sums (0 at the vertices, |T|/3 at the edge midpoints); the
+  vector P2 mass is the scalar P2 mass on each component, uncoupled; every
+  residual equals -(mass matrix) x (time derivative) / c0.
 - The integrator reproduces the reference's golden tables
   scalarFEunsteady_linearODE.output (BDF1; BDF2 with the exact-solution
   starter) on u' = 4 t^3, u(0) = 0, through the C oracle's assembly.
@@ -127,6 +129,36 @@ def test_p2_triangle_mass_row_sums(oracle):
         A = _area(xy)
         assert np.abs(s[:3]).max() <= 1e-14 * A
         assert np.abs(s[3:] - A / 3.0).max() <= 1e-14 * A
+
+
+def test_vector_mass_is_componentwise_scalar_mass_and_residual_consistent(oracle):
+    rng = np.random.default_rng(11)
+    xy = rng.random((3, 2))
+    xyz = np.zeros((3, 3))
+    xyz[:, :2] = xy
+    xyz = xyz.ravel()
+    conn = np.array([0, 1, 2], np.int32)
+    ia = np.arange(0, 145, 12, dtype=np.int32)
+    ja = np.tile(np.arange(12, dtype=np.int32), 12)
+    c0 = 3.5
+    udot = rng.normal(size=12)
+    val, rhs = np.zeros(144), np.zeros(12)
+    oracle.assemble_vector_transient_mass(xyz, conn, np.arange(12, dtype=np.int32), 12, ia, ja, 5, 2.0, c0,
+                                          udot, val, rhs)
+    Ms = oracle.element_mass(2, xyz, 5, (0, 2.0, 0.0, None))
+    ref = np.zeros((12, 12))
+    for c in range(2):
+        ref[c::2, c::2] = c0 * Ms  # f = 2a + c
+    assert np.abs(val.reshape(12, 12) - ref).max() <= 1e-14 * np.abs(ref).max()
+    assert np.abs(rhs + ref @ udot / c0).max() <= 1e-13 * np.abs(rhs).max()
+    # scalar transient mass: residual = -M udot / c0 as well
+    ia6 = np.arange(0, 37, 6, dtype=np.int32)
+    ja6 = np.tile(np.arange(6, dtype=np.int32), 6)
+    v6, r6 = np.zeros(36), np.zeros(6)
+    oracle.assemble_transient_mass(2, xyz, conn, np.arange(6, dtype=np.int32), 6, ia6, ja6, 5, (0, 2.0, 0.0, None),
+                                   c0, udot[:6], v6, r6)
+    assert np.abs(v6.reshape(6, 6) - c0 * Ms).max() <= 1e-14 * c0 * np.abs(Ms).max()
+    assert np.abs(r6 + Ms @ udot[:6]).max() <= 1e-13 * np.abs(r6).max()
 
 
 def test_bdf2_with_bdf1_starter_is_exact_for_linear_time(oracle):
