@@ -142,7 +142,11 @@ def navier_stokes_system(mesh: A.Mesh, Re: float, degree: int = 5, coloring=None
     source, divergence of the Newtonian stress {u, p} (1, mu = 1/Re).  u and p
     are essential on the boundary with the Kovasznay values; the interior
     starts from zero.  mass (an Fn): the unsteady equations, with a
-    TransientVectorMass form first (vectorFEunsteady.cpp:562)."""
+    TransientVectorMass form first (vectorFEunsteady.cpp:561).  The momentum
+    rows of this stack carry -(u . grad u - div sigma), so a well-posed
+    unsteady problem takes the mass coefficient -1 (the reference's unsteady
+    stack flips the convection and stress coefficients instead,
+    vectorFEunsteady.cpp:560-572)."""
     num = A.Numbering(mesh, [A.TRI_VP2, A.TRI_P1], [1, 1])
     S = A.LinearSystemHIP(device)
     S.upload_mesh(mesh, coloring)

@@ -186,8 +186,11 @@ def test_unsteady_navier_stokes_matches_oracle(oracle):
     from feng_amd.timeint import TimeIntegrator
     print("unsteady NS", flush=True)
     m = Mesh.square(10, 0.2, 2)
-    S, num = navier_stokes_system(m, 40.0, 5, mass=Fn(FN_CONST, 1.0))
-    _, _, assemble = _ns_oracle(oracle, m, num, 40.0, 5, 1.0)
+    # this stack's momentum rows carry -(u . grad u - div sigma): the mass
+    # coefficient -1 keeps u_t with the same sign (the reference's unsteady
+    # stack flips convection and stress instead, vectorFEunsteady.cpp:560-572)
+    S, num = navier_stokes_system(m, 40.0, 5, mass=Fn(FN_CONST, -1.0))
+    _, _, assemble = _ns_oracle(oracle, m, num, 40.0, 5, -1.0)
     u0 = S.get_solution()
     sols = []
     nl = NLSolverOptions(1e-11, 1e-12, 1e4, 12)
