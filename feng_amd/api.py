@@ -30,6 +30,11 @@ KSP_CG, KSP_GMRES = 0, 1
 PC_NONE, PC_JACOBI, PC_IC0, PC_ILU0, PC_BJ_ILU0, PC_LEARNED, PC_AMG = range(7)
 PREC_FP64, PREC_FP32_IR = 0, 1
 
+# norm types (the reference's normType values, feNorm.h:10-85)
+(NORM_L1, NORM_L1_ERROR, NORM_L2, NORM_VECTOR_L2, NORM_L2_ERROR, NORM_VECTOR_L2_ERROR, NORM_LINF, NORM_LINF_ERROR,
+ NORM_LINF_UDOT_ERROR, NORM_VECTOR_LINF_ERROR, NORM_VECTOR_LINF_UDOT_ERROR, NORM_SEMI_H1, NORM_SEMI_H1_ERROR,
+ NORM_H1, NORM_H1_ERROR, NORM_AREA) = (1, 2, 3, 4, 5, 7, 8, 9, 10, 11, 12, 13, 14, 17, 18, 19)
+
 
 class FeError(RuntimeError):
     pass
@@ -318,6 +323,23 @@ class LinearSystemHIP:
         d = np.ascontiguousarray(np.atleast_1d(data), np.float64)
         self._c(self.L.fe_hip_update_coeff(self.ctx, form, index, N.ptr(d, C.c_double), d.size),
                 "fe_hip_update_coeff")
+
+    def norm(self, type_, field, degree, exact=()):
+        """feNorm::compute on the device: norm `type_` (NORM_*) of field
+        `field` with the degree-`degree` rule; exact: the Fn callbacks the
+        type takes (see fe_hip_norm)."""
+        arr = (N.FeHipFn * max(1, len(exact)))()
+        keep = []
+        for i, fn in enumerate(exact):
+            arr[i].kind, arr[i].c0, arr[i].c1 = fn.kind, fn.c0, fn.c1
+            if fn.data is not None:
+                d = np.ascontiguousarray(fn.data, np.float64)
+                keep.append(d)
+                arr[i].data = N.ptr(d, C.c_double)
+                arr[i].count = d.size
+        out = C.c_double()
+        self._c(self.L.fe_hip_norm(self.ctx, type_, field, degree, len(exact), arr, C.byref(out)), "fe_hip_norm")
+        return out.value
 
     # solution history of the BDF schemes (feSolutionContainer) ---------------
     def set_bdf(self, cn):

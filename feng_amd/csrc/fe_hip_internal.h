@@ -153,6 +153,11 @@ struct fe_hip_ctx {
   double *d_past[kMaxPast] = {};
   int nsol = 1;
   double bdf[kMaxPast + 1] = {};
+  // norms: reference tables per (kind, degree) and a buffer for exact-solution tables
+  std::map<int, double *> normTabs;
+  std::map<int, int> normNq;
+  double *d_normData = nullptr;
+  size_t normDataCap = 0;
   // launch shapes of the Krylov iteration kernels (fe_hip_set_tuning)
   int spmvLanes = 4;
   int spmvUnroll = 1;

@@ -1,0 +1,267 @@
+// Norms and error norms of one field on the device (feNorm::compute,
+// feNorm.cpp:124-246, with computeLpNorm :323-398, computeVectorLpNorm
+// :1399-1444, computeVectorLInfNorm :1445-1484, the time-derivative
+// L-infinity norms :1485-1545 / :1618-1642, computeLInfNorm :1593-1617,
+// computeH1SemiNorm :1643-1733, computeH1Norm :1848-1853, computeArea
+// :1862-1873).
+//
+// One thread per element (mesh order, no coloring: nothing is scattered):
+// the element's contribution over its quadrature points, a block reduction
+// into one partial per block, and one wavefront folding the partials in a
+// fixed order, so the value is bitwise repeatable.  Exact solutions are
+// coefficient callbacks (analytic kinds or quadrature-point tables) as in
+// the assembly; the gradient of the exact solution is passed as its
+// components, like the reference's vector callback of the H1 norms.
+#include "fe_element.h"
+
+#include <cmath>
+
+namespace {
+
+struct NormArgs {
+  int E, type, nq, nsol;
+  const int *conn, *adr;  // adr: E x NF, mesh element order
+  const double *xyz, *tab, *u;
+  FeFn ex[4];
+  double cn[fe_hip_ctx::kMaxPast + 1];
+  const double *past[fe_hip_ctx::kMaxPast];
+  double *partials;
+};
+
+__device__ __forceinline__ bool normIsMax(int t)
+{
+  return t == FE_NORM_LINF || t == FE_NORM_LINF_ERROR || t == FE_NORM_LINF_UDOT_ERROR ||
+         t == FE_NORM_VECTOR_LINF_ERROR || t == FE_NORM_VECTOR_LINF_UDOT_ERROR;
+}
+
+// NC = 1: scalar field with NF functions; NC = 2: the vector P2 field,
+// function f = 2 a + c is psi_a e_c (NF = 12, six scalar functions).
+template <int NF, int DIM, int NC>
+__global__ __launch_bounds__(256) void normElements(NormArgs a)
+{
+  __shared__ FeRefTables T;
+  __shared__ double red[4];
+  {
+    const double *src = a.tab;
+    double *dst = (double *)&T;
+    for(int i = threadIdx.x; i < (int)(sizeof(FeRefTables) / sizeof(double)); i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
+  constexpr int NV = DIM + 1, NS = NF / NC;
+  const int t = a.type;
+  const bool isMax = normIsMax(t);
+  const bool udot = t == FE_NORM_LINF_UDOT_ERROR || t == FE_NORM_VECTOR_LINF_UDOT_ERROR;
+  const bool err = t == FE_NORM_L1_ERROR || t == FE_NORM_L2_ERROR || t == FE_NORM_LINF_ERROR || udot ||
+                   t == FE_NORM_VECTOR_L2_ERROR || t == FE_NORM_VECTOR_LINF_ERROR || t == FE_NORM_H1_ERROR;
+  const bool grad = t == FE_NORM_SEMI_H1 || t == FE_NORM_SEMI_H1_ERROR || t == FE_NORM_H1 || t == FE_NORM_H1_ERROR;
+  const bool val = t != FE_NORM_SEMI_H1 && t != FE_NORM_SEMI_H1_ERROR && t != FE_NORM_AREA;
+  // exact callbacks: value(s) first, then the gradient components
+  const int g0 = (t == FE_NORM_H1_ERROR) ? 1 : 0;
+  const bool gradErr = t == FE_NORM_SEMI_H1_ERROR || t == FE_NORM_H1_ERROR;
+  double acc = 0.;
+  for(int e = blockIdx.x * blockDim.x + threadIdx.x; e < a.E; e += gridDim.x * blockDim.x) {
+    double c[NV][3];
+#pragma unroll
+    for(int j = 0; j < NV; ++j) {
+      const int v = a.conn[(size_t)e * NV + j];
+#pragma unroll
+      for(int d = 0; d < 3; ++d) c[j][d] = a.xyz[3 * (size_t)v + d];
+    }
+    double J, drdx[3], drdy[3], drdz[3];
+    feGeometry<DIM>(c, J, drdx, drdy, drdz);
+    double ul[NF];
+#pragma unroll
+    for(int i = 0; i < NF; ++i) {
+      const int ad = a.adr[(size_t)e * NF + i];
+      double v = a.u[ad];
+      if(udot) {
+        v *= a.cn[0];
+        for(int j = 1; j < a.nsol; ++j) v += a.cn[j] * a.past[j - 1][ad];
+      }
+      ul[i] = v;
+    }
+    for(int k = 0; k < T.nq; ++k) {
+      double x[3] = {0., 0., 0.};
+#pragma unroll
+      for(int j = 0; j < NV; ++j)
+#pragma unroll
+        for(int d = 0; d < 3; ++d) x[d] += c[j][d] * T.Lg[k][j];
+      const double jw = J * T.w[k];
+      double q = 0.;  // this point's integrand (sum norms) or value (max norms)
+      if(val) {
+#pragma unroll
+        for(int cc = 0; cc < NC; ++cc) {
+          double uh = 0.;
+#pragma unroll
+          for(int b = 0; b < NS; ++b) uh += T.L[k][b] * ul[NC * b + cc];
+          const double d = (err ? feEvalFn(a.ex[cc], x, e, T.nq, k) : 0.) - uh;
+          if(isMax) q = fmax(q, fabs(d));
+          else if(t == FE_NORM_L1 || t == FE_NORM_L1_ERROR) q += fabs(d);
+          else q += d * d;
+        }
+      }
+      if(grad) {
+        double gh[3] = {0., 0., 0.};
+#pragma unroll
+        for(int b = 0; b < NS; ++b) {
+          const double gr = T.dLr[k][b], gs = T.dLs[k][b], gt = DIM == 3 ? T.dLt[k][b] : 0.;
+          gh[0] += ul[b] * (gr * drdx[0] + gs * drdx[1] + gt * drdx[2]);
+          gh[1] += ul[b] * (gr * drdy[0] + gs * drdy[1] + gt * drdy[2]);
+          if(DIM == 3) gh[2] += ul[b] * (gr * drdz[0] + gs * drdz[1] + gt * drdz[2]);
+        }
+#pragma unroll
+        for(int d = 0; d < DIM; ++d) {
+          const double gd = (gradErr ? feEvalFn(a.ex[g0 + d], x, e, T.nq, k) : 0.) - gh[d];
+          q += gd * gd;
+        }
+      }
+      if(t == FE_NORM_AREA) q = 1.;
+      if(isMax) acc = fmax(acc, q);
+      else acc += q * jw;
+    }
+  }
+  // block reduction: wavefront shuffles, then the 4 wavefronts in order
+#pragma unroll
+  for(int o = 32; o > 0; o >>= 1) {
+    const double v = __shfl_xor(acc, o, 64);
+    acc = isMax ? fmax(acc, v) : acc + v;
+  }
+  if((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if(threadIdx.x == 0) {
+    double r = red[0];
+    for(int w = 1; w < (int)(blockDim.x >> 6); ++w) r = isMax ? fmax(r, red[w]) : r + red[w];
+    a.partials[blockIdx.x] = r;
+  }
+}
+
+// One wavefront folds the G block partials in a fixed order.
+__global__ __launch_bounds__(64) void normFinal(const double *__restrict__ partials, int G, int isMax,
+                                                double *__restrict__ out)
+{
+  double r = 0.;
+  for(int b = threadIdx.x; b < G; b += 64) r = isMax ? fmax(r, partials[b]) : r + partials[b];
+#pragma unroll
+  for(int o = 32; o > 0; o >>= 1) {
+    const double v = __shfl_xor(r, o, 64);
+    r = isMax ? fmax(r, v) : r + v;
+  }
+  if(threadIdx.x == 0) *out = r;
+}
+
+// Number of exact callbacks each norm takes for a field with nc components
+// in dimension dim (-1: not supported on the device).
+int normExactCount(int type, int nc, int dim)
+{
+  switch(type) {
+    case FE_NORM_L1: case FE_NORM_L2: case FE_NORM_LINF: case FE_NORM_SEMI_H1: case FE_NORM_H1:
+    case FE_NORM_AREA:
+      return nc == 1 || type == FE_NORM_AREA ? 0 : -1;
+    case FE_NORM_VECTOR_L2: return nc == 2 ? 0 : -1;
+    case FE_NORM_L1_ERROR: case FE_NORM_L2_ERROR: case FE_NORM_LINF_ERROR: case FE_NORM_LINF_UDOT_ERROR:
+      return nc == 1 ? 1 : -1;
+    case FE_NORM_VECTOR_L2_ERROR: case FE_NORM_VECTOR_LINF_ERROR: case FE_NORM_VECTOR_LINF_UDOT_ERROR:
+      return nc == 2 ? 2 : -1;
+    case FE_NORM_SEMI_H1_ERROR: return nc == 1 ? dim : -1;
+    case FE_NORM_H1_ERROR: return nc == 1 ? 1 + dim : -1;
+  }
+  return -1;
+}
+
+}  // namespace
+
+extern "C" int fe_hip_norm(fe_hip_ctx *ctx, int type, int field, int quad_degree, int nexact,
+                           const fe_hip_fn *exact, double *value)
+{
+  if(!ctx || !value || (nexact > 0 && !exact)) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, ctx->d_u && ctx->E > 0, "upload the mesh and set the sizes first");
+  FE_REQUIRE(ctx, field >= 0 && field < (int)ctx->fields.size(), "unknown field");
+  const int kind = ctx->fields[field].kind;
+  const int nc = kind == FE_TRI_VP2 ? 2 : 1;
+  const int need = normExactCount(type, nc, ctx->dim);
+  FE_REQUIRE(ctx, need >= 0, "norm type not available on the device for this field: " + std::to_string(type));
+  FE_REQUIRE(ctx, nexact == need, "norm type " + std::to_string(type) + " takes " + std::to_string(need) +
+                                      " exact callback(s)");
+  const bool isMax = type == FE_NORM_LINF || type == FE_NORM_LINF_ERROR || type == FE_NORM_LINF_UDOT_ERROR ||
+                     type == FE_NORM_VECTOR_LINF_ERROR || type == FE_NORM_VECTOR_LINF_UDOT_ERROR;
+  if(type == FE_NORM_LINF_UDOT_ERROR || type == FE_NORM_VECTOR_LINF_UDOT_ERROR)
+    FE_REQUIRE(ctx, ctx->nsol == 1 || ctx->d_past[0], "the time derivative needs the solution history");
+  // reference tables of (kind, degree), cached
+  const int key = kind * 64 + quad_degree;
+  double *tab = nullptr;
+  auto it = ctx->normTabs.find(key);
+  if(it != ctx->normTabs.end()) tab = it->second;
+  else {
+    FeRefTables T;
+    const int nq = feBuildRefTables(kind, quad_degree, &T);
+    FE_REQUIRE(ctx, nq > 0, "unsupported quadrature degree " + std::to_string(quad_degree));
+    if(feAlloc(ctx, &tab, sizeof(FeRefTables) / sizeof(double) + 1)) return FE_STATUS_OUT_OF_MEMORY;
+    FE_HIP_TRY(ctx, hipMemcpy(tab, &T, sizeof(FeRefTables), hipMemcpyHostToDevice));
+    ctx->normTabs[key] = tab;
+    ctx->normNq[key] = nq;
+  }
+  const int nq = ctx->normNq[key];
+  NormArgs a{};
+  a.E = ctx->E;
+  a.type = type;
+  a.nq = nq;
+  a.conn = ctx->d_conn;
+  a.adr = ctx->fields[field].adr;
+  a.xyz = ctx->d_xyz;
+  a.tab = tab;
+  a.u = ctx->d_u;
+  a.nsol = ctx->nsol;
+  for(int j = 0; j <= fe_hip_ctx::kMaxPast; ++j) a.cn[j] = ctx->bdf[j];
+  for(int j = 0; j < fe_hip_ctx::kMaxPast; ++j) a.past[j] = ctx->d_past[j];
+  // exact callbacks; tables go to one cached device buffer
+  size_t tabDoubles = 0;
+  for(int i = 0; i < nexact; ++i) {
+    FE_REQUIRE(ctx, exact[i].kind >= FE_FN_CONST && exact[i].kind <= FE_FN_KOV_P, "unknown function kind");
+    if(exact[i].kind == FE_FN_ELEM || exact[i].kind == FE_FN_QP) {
+      const int64_t cnt = exact[i].kind == FE_FN_ELEM ? ctx->E : (int64_t)ctx->E * nq;
+      FE_REQUIRE(ctx, exact[i].data && exact[i].count == cnt, "exact-solution table has the wrong size");
+      tabDoubles += (size_t)cnt;
+    }
+  }
+  if(tabDoubles > ctx->normDataCap) {
+    if(ctx->d_normData) feFree(ctx, ctx->d_normData);
+    ctx->d_normData = nullptr;
+    ctx->normDataCap = 0;
+    if(feAlloc(ctx, &ctx->d_normData, tabDoubles)) return FE_STATUS_OUT_OF_MEMORY;
+    ctx->normDataCap = tabDoubles;
+  }
+  size_t off = 0;
+  for(int i = 0; i < nexact; ++i) {
+    FeFn &f = a.ex[i];
+    f.kind = exact[i].kind;
+    f.c0 = exact[i].c0;
+    f.c1 = exact[i].c1;
+    f.data = nullptr;
+    if(f.kind == FE_FN_ELEM || f.kind == FE_FN_QP) {
+      f.data = ctx->d_normData + off;
+      FE_HIP_TRY(ctx, hipMemcpyAsync(f.data, exact[i].data, sizeof(double) * exact[i].count, hipMemcpyHostToDevice,
+                                     ctx->stream));
+      off += (size_t)exact[i].count;
+    }
+  }
+  const int G = std::max(1, std::min(1024, (ctx->E + 255) / 256));
+  a.partials = ctx->d_partials;
+  double *out = ctx->d_partials + 3 * ctx->maxBlocks + 8;
+  if(kind == FE_TRI_P1) hipLaunchKernelGGL((normElements<3, 2, 1>), dim3(G), dim3(256), 0, ctx->stream, a);
+  else if(kind == FE_TRI_P2) hipLaunchKernelGGL((normElements<6, 2, 1>), dim3(G), dim3(256), 0, ctx->stream, a);
+  else if(kind == FE_TET_P1) hipLaunchKernelGGL((normElements<4, 3, 1>), dim3(G), dim3(256), 0, ctx->stream, a);
+  else if(kind == FE_TRI_VP2) hipLaunchKernelGGL((normElements<12, 2, 2>), dim3(G), dim3(256), 0, ctx->stream, a);
+  else FE_REQUIRE(ctx, false, "field kind without a device norm");
+  hipLaunchKernelGGL(normFinal, dim3(1), dim3(64), 0, ctx->stream, ctx->d_partials, G, isMax ? 1 : 0, out);
+  FE_HIP_TRY(ctx, hipGetLastError());
+  double r = 0.;
+  FE_HIP_TRY(ctx, hipMemcpyAsync(&r, out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  // tiny negative sums from negative weights (feNorm.cpp:390-393)
+  if(!isMax && r < 0. && fabs(r) < 1e-14) r = fabs(r);
+  const bool root = type == FE_NORM_L2 || type == FE_NORM_L2_ERROR || type == FE_NORM_VECTOR_L2 ||
+                    type == FE_NORM_VECTOR_L2_ERROR || type == FE_NORM_SEMI_H1 || type == FE_NORM_SEMI_H1_ERROR ||
+                    type == FE_NORM_H1 || type == FE_NORM_H1_ERROR;
+  *value = root ? sqrt(r) : r;
+  return FE_STATUS_OK;
+}

@@ -239,6 +239,38 @@ int fe_hip_set_past_solution(fe_hip_ctx *ctx, int j, const double *u);
 int fe_hip_get_past_solution(fe_hip_ctx *ctx, int j, double *u);
 int fe_hip_set_essential(fe_hip_ctx *ctx, const double *values);
 
+/* ---- norms ---------------------------------------------------------------
+ * feNorm::compute (feNorm.cpp:124-246) on the device: norm `type` (the
+ * reference's normType values, feNorm.h:10-85) of field `field` of the
+ * current solution with the quadrature rule of degree quad_degree.  exact
+ * holds the exact-solution callbacks the type needs (the reference's scalar
+ * or vector feFunction; quadrature-point tables are E nq of this degree):
+ *   L1/L2/LINF/SEMI_H1/H1/AREA, VECTOR_L2: none;
+ *   L1_ERROR, L2_ERROR, LINF_ERROR: u; LINF_UDOT_ERROR: du/dt (the time
+ *   derivative of the solution from the BDF history, fe_hip_set_bdf);
+ *   VECTOR_L2_ERROR, VECTOR_LINF_ERROR, VECTOR_LINF_UDOT_ERROR: the two
+ *   components of the vector P2 field;
+ *   SEMI_H1_ERROR: the dim components of grad u; H1_ERROR: u, then grad u.
+ * The value is bitwise repeatable (fixed reduction order). */
+#define FE_NORM_L1 1
+#define FE_NORM_L1_ERROR 2
+#define FE_NORM_L2 3
+#define FE_NORM_VECTOR_L2 4
+#define FE_NORM_L2_ERROR 5
+#define FE_NORM_VECTOR_L2_ERROR 7
+#define FE_NORM_LINF 8
+#define FE_NORM_LINF_ERROR 9
+#define FE_NORM_LINF_UDOT_ERROR 10
+#define FE_NORM_VECTOR_LINF_ERROR 11
+#define FE_NORM_VECTOR_LINF_UDOT_ERROR 12
+#define FE_NORM_SEMI_H1 13
+#define FE_NORM_SEMI_H1_ERROR 14
+#define FE_NORM_H1 17
+#define FE_NORM_H1_ERROR 18
+#define FE_NORM_AREA 19
+int fe_hip_norm(fe_hip_ctx *ctx, int type, int field, int quad_degree, int nexact, const fe_hip_fn *exact,
+                double *value);
+
 /* ---- aggregation multigrid (FE_PC_AMG) ----------------------------------
  * The algebraic multigrid the reference reaches through PETSc's runtime
  * options (-pc_type gamg, feLinearSystemPETSc.cpp:900-1031).  The hierarchy

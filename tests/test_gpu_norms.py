@@ -1,0 +1,131 @@
+"""Device norms (fe_hip_norm, feNorm::compute) against the reference's golden
+convergence table, the C oracle's L2 error and closed forms.
+
+- L2_ERROR of the device solution on the reference's meshes square1-4.msh
+  reproduces scalarFE_diffusion.output (P1 and P2, degree-8 quadrature) and
+  equals the oracle's or_l2_error_sinsin to 1e-12.
+- Closed forms: the area of the unit square / cube; the L1, L2, semi-H1 and
+  H1 norms of a linear field (represented exactly by P1, P2 and P1 tets);
+  the L2 and semi-H1 errors of the zero field against sin(pi x) sin(pi y)
+  (1/2 and pi/sqrt(2)); vector norms of constant fields; the time-derivative
+  error of a BDF1 history with a known slope.
+- Bitwise repeatable (fixed reduction order).
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# scalarFE_diffusion.output (tests/withLinearSolver/ of the reference)
+REF_TABLE = {
+    1: ["7.907546e-02", "2.113277e-02", "5.377435e-03", "1.350436e-03"],
+    2: ["4.327628e-03", "5.480619e-04", "6.873916e-05", "8.600535e-06"],
+}
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_l2_error_reproduces_reference_table(oracle, order):
+    from feng_amd.api import FN_CONST, FN_SINSIN, NORM_L2_ERROR, Fn, Mesh
+    from feng_amd.problems import NLSolverOptions, diffusion_system, solve_newton
+    from tests.conftest import GOLDEN
+    print("L2 table", order, flush=True)
+    out = []
+    for lvl in range(1, 5):
+        m = Mesh.read_msh(GOLDEN / "meshes" / f"square{lvl}.msh")
+        bc = lambda x, y, z, c: np.sin(np.pi * x) * np.sin(np.pi * y)
+        S, num = diffusion_system(m, order, 8, Fn(FN_CONST, 1.0), Fn(FN_SINSIN, -2 * math.pi ** 2), u_bc=bc)
+        rep = solve_newton(S, NLSolverOptions(), rtol=1e-13)
+        assert rep.converged
+        err = S.norm(NORM_L2_ERROR, 0, 8, [Fn(FN_SINSIN, 1.0)])
+        ref = oracle.l2_error_sinsin(order, m.xyz, m.conn, num.adr[0], S.get_solution(), 8)
+        assert abs(err - ref) <= 1e-12 * ref
+        assert err == S.norm(NORM_L2_ERROR, 0, 8, [Fn(FN_SINSIN, 1.0)])  # repeatable
+        out.append(f"{err:.6e}")
+        S.close()
+    assert out == REF_TABLE[order]
+
+
+def _linear_case(kind):
+    from feng_amd.api import FN_CONST, Fn, Mesh
+    from feng_amd.problems import diffusion_system
+    m = Mesh.cube(6) if kind == 4 else Mesh.square(9, 0.2, 4)
+    S, num = diffusion_system(m, kind, 2, Fn(FN_CONST, 1.0), Fn(FN_CONST, 0.0))
+    xyz = num.dof_xyz.reshape(-1, 3)
+    return m, S, num, xyz
+
+
+@pytest.mark.parametrize("kind", [1, 2, 4])
+def test_closed_forms_of_a_linear_field(kind):
+    from feng_amd.api import (NORM_AREA, NORM_H1, NORM_L1, NORM_L2, NORM_L2_ERROR, NORM_LINF, NORM_SEMI_H1,
+                              FN_QP, Fn)
+    from tests.test_oracle_time import _qp_table
+    m, S, num, xyz = _linear_case(kind)
+    a, b, d, c = 0.7, -0.4, 0.3 if kind == 4 else 0.0, 1.5
+    g = a * xyz[:, 0] + b * xyz[:, 1] + d * xyz[:, 2] + c
+    S.set_solution(g)
+    assert abs(S.norm(NORM_AREA, 0, 2) - 1.0) <= 1e-13
+    # int over the unit square / cube of (a x + b y + d z + c)^2 and of g (> 0)
+    l2sq = (a * a + b * b + d * d) / 3 + c * c + (a * b + a * d + b * d) / 2 + (a + b + d) * c
+    l1 = (a + b + d) / 2 + c
+    grad2 = a * a + b * b + d * d
+    tol = 1e-11 if kind == 4 else 1e-13  # tet rules carry 12-digit points
+    assert abs(S.norm(NORM_L2, 0, 2) - math.sqrt(l2sq)) <= tol
+    assert abs(S.norm(NORM_L1, 0, 2) - l1) <= tol
+    assert abs(S.norm(NORM_SEMI_H1, 0, 2) - math.sqrt(grad2)) <= 1e-12
+    assert abs(S.norm(NORM_H1, 0, 2) - math.sqrt(l2sq + grad2)) <= tol
+    if kind != 4:
+        tab = _qp_table(m, 2, lambda x, y: a * x + b * y + c)
+        assert S.norm(NORM_L2_ERROR, 0, 2, [Fn(FN_QP, data=tab)]) <= 1e-13
+    S.set_solution(np.full(num.nDofs, -2.5))
+    assert abs(S.norm(NORM_LINF, 0, 2) - 2.5) <= 1e-15
+    S.close()
+
+
+def test_error_norms_of_the_zero_field_against_sinsin():
+    from feng_amd.api import (FN_QP, FN_SINSIN, NORM_H1_ERROR, NORM_L2_ERROR, NORM_LINF_ERROR, NORM_SEMI_H1_ERROR,
+                              Fn)
+    from feng_amd.api import FN_CONST, Mesh
+    from feng_amd.problems import diffusion_system
+    from tests.test_oracle_time import _qp_table
+    m = Mesh.square(32)
+    S, num = diffusion_system(m, 2, 8, Fn(FN_CONST, 1.0), Fn(FN_CONST, 0.0))
+    S.set_solution(np.zeros(num.nDofs))
+    gx = _qp_table(m, 8, lambda x, y: np.pi * np.cos(np.pi * x) * np.sin(np.pi * y))
+    gy = _qp_table(m, 8, lambda x, y: np.pi * np.sin(np.pi * x) * np.cos(np.pi * y))
+    assert abs(S.norm(NORM_L2_ERROR, 0, 8, [Fn(FN_SINSIN, 1.0)]) - 0.5) <= 1e-10
+    semi = S.norm(NORM_SEMI_H1_ERROR, 0, 8, [Fn(FN_QP, data=gx), Fn(FN_QP, data=gy)])
+    assert abs(semi - np.pi / math.sqrt(2.0)) <= 1e-9
+    h1 = S.norm(NORM_H1_ERROR, 0, 8, [Fn(FN_SINSIN, 1.0), Fn(FN_QP, data=gx), Fn(FN_QP, data=gy)])
+    assert abs(h1 - math.sqrt(0.25 + np.pi ** 2 / 2)) <= 1e-9
+    # the maximum over quadrature points of sin sin is below 1 and close to it
+    linf = S.norm(NORM_LINF_ERROR, 0, 8, [Fn(FN_SINSIN, 1.0)])
+    assert 0.99 < linf < 1.0
+    S.close()
+
+
+def test_vector_norms_and_time_derivative_error():
+    from feng_amd.api import (FN_CONST, NORM_LINF_UDOT_ERROR, NORM_VECTOR_L2, NORM_VECTOR_L2_ERROR,
+                              NORM_VECTOR_LINF_ERROR, NORM_VECTOR_LINF_UDOT_ERROR, Fn, Mesh)
+    from feng_amd.problems import navier_stokes_system
+    m = Mesh.square(8, 0.2, 5)
+    S, num = navier_stokes_system(m, 40.0, 5)
+    uf = num.dof_field == 0
+    u = np.zeros(num.nDofs)
+    u[uf] = np.where(num.dof_comp[uf] == 0, 0.6, -0.8)
+    S.set_solution(u)
+    assert abs(S.norm(NORM_VECTOR_L2, 0, 5) - 1.0) <= 1e-13
+    err = S.norm(NORM_VECTOR_L2_ERROR, 0, 5, [Fn(FN_CONST, 0.1), Fn(FN_CONST, -0.8)])
+    assert abs(err - 0.5) <= 1e-13
+    assert abs(S.norm(NORM_VECTOR_LINF_ERROR, 0, 5, [Fn(FN_CONST, 0.6), Fn(FN_CONST, 0.0)]) - 0.8) <= 1e-15
+    # BDF1 history with slope (0.6, -0.8) / dt: u_dot = (60, -80) for dt = 0.01
+    S.set_past_solution(1, np.zeros(num.nDofs))
+    S.set_bdf([100.0, -100.0])
+    e = S.norm(NORM_VECTOR_LINF_UDOT_ERROR, 0, 5, [Fn(FN_CONST, 60.0), Fn(FN_CONST, -80.0)])
+    assert e <= 1e-12
+    e = S.norm(NORM_VECTOR_LINF_UDOT_ERROR, 0, 5, [Fn(FN_CONST, 60.0), Fn(FN_CONST, -79.5)])
+    assert abs(e - 0.5) <= 1e-12
+    # the scalar P1 pressure field: zero solution and history
+    assert S.norm(NORM_LINF_UDOT_ERROR, 1, 5, [Fn(FN_CONST, 0.25)]) == 0.25
+    S.close()
