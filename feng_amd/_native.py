@@ -105,6 +105,10 @@ HIP_API = {
     "fe_hip_get_past_solution": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "fe_hip_set_essential": (C.c_int, [C.c_void_p, c_double_p]),
     "fe_hip_norm": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, c_double_p]),
+    "fe_hip_set_constrained_dofs": (C.c_int, [C.c_void_p, C.c_int, c_int_p]),
+    "fe_hip_constrain_essential_components": (C.c_int, [C.c_void_p]),
+    "fe_hip_set_periodic_dofs": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p]),
+    "fe_hip_apply_periodicity": (C.c_int, [C.c_void_p]),
     "fe_hip_amg_levels": (C.c_int, [C.c_void_p, c_int_p, c_int_p, C.POINTER(C.c_int64), C.c_int]),
     "fe_hip_amg_apply": (C.c_int, [C.c_void_p, c_double_p, c_double_p]),
     "fe_hip_mpnn_param_count": (C.c_int, []),

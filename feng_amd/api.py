@@ -287,6 +287,25 @@ class LinearSystemHIP:
                           st.ms_setup)
         return (res, du) if want_du else res
 
+    # constraints (the reference's constrainEssentialComponents / applyPeriodicity)
+    def set_constrained_dofs(self, dofs):
+        d = np.ascontiguousarray(dofs, np.int32)
+        self._c(self.L.fe_hip_set_constrained_dofs(self.ctx, d.size, N.ptr(d, C.c_int)),
+                "fe_hip_set_constrained_dofs")
+
+    def constrainEssentialComponents(self):
+        self._c(self.L.fe_hip_constrain_essential_components(self.ctx), "fe_hip_constrain_essential_components")
+
+    def set_periodic_dofs(self, master, slave):
+        m = np.ascontiguousarray(master, np.int32)
+        s_ = np.ascontiguousarray(slave, np.int32)
+        assert m.size == s_.size
+        self._c(self.L.fe_hip_set_periodic_dofs(self.ctx, m.size, N.ptr(m, C.c_int), N.ptr(s_, C.c_int)),
+                "fe_hip_set_periodic_dofs")
+
+    def applyPeriodicity(self):
+        self._c(self.L.fe_hip_apply_periodicity(self.ctx), "fe_hip_apply_periodicity")
+
     def correctSolution(self):
         self._c(self.L.fe_hip_correct_solution(self.ctx), "fe_hip_correct_solution")
 

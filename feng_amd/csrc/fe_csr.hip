@@ -147,9 +147,12 @@ __device__ __forceinline__ int waveExclusiveScan(int v, int lane, int *total)
 }
 
 // One 64-thread block (a single wavefront) per unknown row.
+// Extra columns of a row (the periodic (slave, master) entries,
+// feCompressedRowStorage.cpp:96-108): xPtr[n + 1] / xCol, or null.
 template <int CAP>
 __global__ __launch_bounds__(64) void csrRows(int n, const int *__restrict__ refPtr,
                                               const int *__restrict__ refs, int G, GroupCols gc,
+                                              const int *__restrict__ xPtr, const int *__restrict__ xCol,
                                               int mode, int *__restrict__ uniq,
                                               const int *__restrict__ ia, int *__restrict__ ja,
                                               int *__restrict__ errFlag)
@@ -181,12 +184,14 @@ __global__ __launch_bounds__(64) void csrRows(int n, const int *__restrict__ ref
     }
     base += tot;
   }
-  if(overflow) {
+  const int xb = xPtr ? xPtr[r] : 0, xn = xPtr ? xPtr[r + 1] - xb : 0;
+  if(overflow || base + 1 + xn > CAP) {
     if(lane == 0) atomicMax(errFlag, 1);
     return;
   }
   if(lane == 0) buf[base] = r;  // the diagonal is always stored
-  const int N = base + 1;
+  for(int k = lane; k < xn; k += 64) buf[base + 1 + k] = xCol[xb + k];
+  const int N = base + 1 + xn;
   int P = 1;
   while(P < N) P <<= 1;
   for(int i = N + lane; i < P; i += 64) buf[i] = INT_MAX;
@@ -360,8 +365,20 @@ extern "C" int fe_hip_build_csr(fe_hip_ctx *ctx, int64_t *nnzOut)
   }
   if(feAlloc(ctx, &ctx->d_ia, (size_t)n + 1)) return FE_STATUS_OUT_OF_MEMORY;
   constexpr int CAP = 2048;
+  // periodic (slave, master) entries as extra columns of the slave rows
+  int *xPtr = nullptr, *xCol = nullptr;
+  if(!ctx->perSlave.empty()) {
+    std::vector<int> hp(n + 1, 0);
+    for(size_t k = 0; k < ctx->perSlave.size(); ++k) hp[ctx->perSlave[k] + 1]++;
+    for(int i = 0; i < n; ++i) hp[i + 1] += hp[i];
+    std::vector<int> hc(ctx->perSlave.size()), fill(hp.begin(), hp.end() - 1);
+    for(size_t k = 0; k < ctx->perSlave.size(); ++k) hc[fill[ctx->perSlave[k]]++] = ctx->perMaster[k];
+    if(feAlloc(ctx, &xPtr, (size_t)n + 1) || feAlloc(ctx, &xCol, hc.size())) return FE_STATUS_OUT_OF_MEMORY;
+    FE_HIP_TRY(ctx, hipMemcpy(xPtr, hp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    FE_HIP_TRY(ctx, hipMemcpy(xCol, hc.data(), sizeof(int) * hc.size(), hipMemcpyHostToDevice));
+  }
   hipLaunchKernelGGL(csrRows<CAP>, dim3(n), dim3(64), 0, ctx->stream, n, refPtr, refs, std::max(G, 1),
-                     gc, 0, cnt, (const int *)nullptr, (int *)nullptr, err);
+                     gc, (const int *)xPtr, (const int *)xCol, 0, cnt, (const int *)nullptr, (int *)nullptr, err);
   FE_HIP_TRY(ctx, hipGetLastError());
   int hErr = 0;
   FE_HIP_TRY(ctx, hipMemcpyAsync(&hErr, err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -372,7 +389,7 @@ extern "C" int fe_hip_build_csr(fe_hip_ctx *ctx, int64_t *nnzOut)
   ctx->nnz = nnz;
   if(feAlloc(ctx, &ctx->d_ja, (size_t)std::max<int64_t>(nnz, 1))) return FE_STATUS_OUT_OF_MEMORY;
   hipLaunchKernelGGL(csrRows<CAP>, dim3(n), dim3(64), 0, ctx->stream, n, refPtr, refs, std::max(G, 1),
-                     gc, 1, (int *)nullptr, ctx->d_ia, ctx->d_ja, err);
+                     gc, (const int *)xPtr, (const int *)xCol, 1, (int *)nullptr, ctx->d_ia, ctx->d_ja, err);
   FE_HIP_TRY(ctx, hipGetLastError());
   // diagonal positions and scatter maps
   if(feAlloc(ctx, &ctx->d_diag, n)) return FE_STATUS_OUT_OF_MEMORY;
@@ -391,6 +408,10 @@ extern "C" int fe_hip_build_csr(fe_hip_ctx *ctx, int64_t *nnzOut)
   feFree(ctx, cursor);
   feFree(ctx, refs);
   feFree(ctx, err);
+  if(xPtr) {
+    feFree(ctx, xPtr);
+    feFree(ctx, xCol);
+  }
   if(feAlloc(ctx, &ctx->d_val, (size_t)std::max<int64_t>(nnz, 1))) return FE_STATUS_OUT_OF_MEMORY;
   FE_HIP_TRY(ctx, hipMemsetAsync(ctx->d_val, 0, sizeof(double) * nnz, ctx->stream));
   FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));

@@ -153,6 +153,14 @@ struct fe_hip_ctx {
   double *d_past[kMaxPast] = {};
   int nsol = 1;
   double bdf[kMaxPast + 1] = {};
+  // Constraints applied to the assembled system: rows of essential vector
+  // components (sorted, < n; conFlag[n] marks them) and periodic
+  // (master, slave) pairs of unknowns, in the reference's map order.
+  std::vector<int> conRows;
+  int *d_conFlag = nullptr;
+  std::vector<int> perMaster, perSlave;
+  int *d_perSlave = nullptr, *d_perMaster = nullptr;  // slave rows and the master each one ends with
+  int nPerRows = 0;
   // norms: reference tables per (kind, degree) and a buffer for exact-solution tables
   std::map<int, double *> normTabs;
   std::map<int, int> normNq;

@@ -52,11 +52,12 @@ def diffusion_system(mesh: A.Mesh, kind: int, degree: int, k: A.Fn, s: A.Fn, u_b
 
 
 def heat_system(mesh: A.Mesh, kind: int, degree: int, m: A.Fn, k: A.Fn, s: A.Fn, essential=True, coloring=None,
-                device=0):
+                device=0, periodic=None):
     """Transient scalar m u_t - div(k grad u) + S = 0: the form stack of the
     reference's unsteady scalar tests (TransientMass, Diffusion, Source --
     scalarFEunsteady.cpp:270-300), on the GPU.  essential: Dirichlet boundary
-    (True) or none (False).  k = None leaves out the diffusion form: with no
+    (True) or none (False); periodic: (master, slave) DOF pairs entering the
+    pattern.  k = None leaves out the diffusion form: with no
     essential boundary that is the ODE m u_t + S = 0 of
     scalarFEunsteady.cpp:54-80.  Returns (S, num, forms) with forms =
     {"mass", "diffusion", "source"} ids for fe_hip_update_coeff."""
@@ -69,6 +70,8 @@ def heat_system(mesh: A.Mesh, kind: int, degree: int, m: A.Fn, k: A.Fn, s: A.Fn,
     if k is not None:
         forms["diffusion"] = S.add_form(A.DIFFUSION, [f], degree, [k])
     forms["source"] = S.add_form(A.SOURCE, [f], degree, [s])
+    if periodic is not None:
+        S.set_periodic_dofs(*periodic)
     S.build_csr()
     return S, num, forms
 
@@ -100,6 +103,8 @@ def solve_newton(S: A.LinearSystemHIP, opts: NLSolverOptions = NLSolverOptions()
         if not both:
             S.assembleMatrices(prec)
             rep.asm_ms += S.kernel_stats(1)[0]
+        S.constrainEssentialComponents()  # no-ops without constraints (feNonLinearSolver.cpp:94-95)
+        S.applyPeriodicity()
         r = S.solve(**solve_kw)
         rep.linear.append(r)
         if not r.converged and r.reason < 0 and r.reason != -3:
@@ -135,7 +140,7 @@ def kovasznay(Re):
 
 
 def navier_stokes_system(mesh: A.Mesh, Re: float, degree: int = 5, coloring=None, device=0, source=None,
-                         mass=None):
+                         mass=None, essential=True, periodic=None):
     """Steady incompressible Navier-Stokes, Taylor-Hood P2-P1, divergence
     formulation (the reference's navier_stokes.cpp:66-99 form stack):
     convective acceleration (coeff -1), mixed divergence {p, u} (1), vector
@@ -146,8 +151,10 @@ def navier_stokes_system(mesh: A.Mesh, Re: float, degree: int = 5, coloring=None
     rows of this stack carry -(u . grad u - div sigma), so a well-posed
     unsteady problem takes the mass coefficient -1 (the reference's unsteady
     stack flips the convection and stress coefficients instead,
-    vectorFEunsteady.cpp:560-572)."""
-    num = A.Numbering(mesh, [A.TRI_VP2, A.TRI_P1], [1, 1])
+    vectorFEunsteady.cpp:560-572).  essential=False: no Dirichlet boundary
+    (constraints through set_constrained_dofs instead); periodic: (master,
+    slave) DOF pairs entering the pattern."""
+    num = A.Numbering(mesh, [A.TRI_VP2, A.TRI_P1], [1, 1] if essential else [0, 0])
     S = A.LinearSystemHIP(device)
     S.upload_mesh(mesh, coloring)
     fu = S.add_field(A.TRI_VP2, num.adr[0])
@@ -160,6 +167,8 @@ def navier_stokes_system(mesh: A.Mesh, Re: float, degree: int = 5, coloring=None
     src = source or [A.Fn(A.FN_CONST, 0.0), A.Fn(A.FN_CONST, 0.0)]
     S.add_form(A.VECTOR_SOURCE, [fu], degree, src)
     S.add_form(A.DIV_NEWTONIAN_STRESS, [fu, fp], degree, [A.Fn(A.FN_CONST, 1.0), A.Fn(A.FN_CONST, 1.0 / Re)])
+    if periodic is not None:
+        S.set_periodic_dofs(*periodic)
     S.build_csr()
     _, ufn, pfn = kovasznay(Re)
     S.set_solution(A.essential_values(num, [ufn, pfn]))

@@ -178,6 +178,29 @@ int fe_hip_solve(fe_hip_ctx *ctx, const fe_hip_solver_opts *opts, double *du,
 /* correctSolution (feLinearSystemPETSc.cpp:1033-1060): u[i] += du[i], i < n. */
 int fe_hip_correct_solution(fe_hip_ctx *ctx);
 
+/* ---- constraints on the assembled system ------------------------------------
+ * The linear-system hooks Newton calls after the matrix assembly
+ * (feNonLinearSolver.cpp:94-95).
+ * fe_hip_set_constrained_dofs: the DOFs of the essential components of vector
+ * spaces (feSpace::setEssentialComponent; the rows the reference collects in
+ * feLinearSystemMklPardiso.cpp:1000-1037); DOFs >= nInc are skipped as there.
+ * fe_hip_constrain_essential_components: constrainEssentialComponents
+ * (feLinearSystemPETSc.cpp:816-853 MatZeroRowsColumns with zero values,
+ * feLinearSystemMklPardiso.cpp:994-1097): columns and rows of those DOFs
+ * zeroed, 1 on their diagonal, 0 in their RHS (the FP32 value copy too).
+ * fe_hip_set_periodic_dofs: the numbering's periodic (master, slave) pairs
+ * (feMetaNumber::PeriodicDOF, one slave per master), before
+ * fe_hip_build_csr: each slave row of two unknowns gains the master column
+ * (feCompressedRowStorage.cpp:96-108).
+ * fe_hip_apply_periodicity: applyPeriodicity
+ * (feLinearSystemMklPardiso.cpp:1099-1127): slave row = 1 at the slave,
+ * -1 at the master, 0 elsewhere, RHS 0.  Both apply hooks are no-ops
+ * without constraints. */
+int fe_hip_set_constrained_dofs(fe_hip_ctx *ctx, int count, const int *dofs);
+int fe_hip_constrain_essential_components(fe_hip_ctx *ctx);
+int fe_hip_set_periodic_dofs(fe_hip_ctx *ctx, int count, const int *master, const int *slave);
+int fe_hip_apply_periodicity(fe_hip_ctx *ctx);
+
 /* ILU(0)/IC(0) factor diagnostics: number of row levels of the factorization
  * schedule and zero pivots shifted in the last factorization. */
 int fe_hip_pc_info(fe_hip_ctx *ctx, int *levels, int *shifts);
