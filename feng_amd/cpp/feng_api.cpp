@@ -177,6 +177,12 @@ void feLinearSystemHIP::correctSolution(std::vector<double> &sol)
   _status = fe_hip_get_solution(_ctx, sol.data());
 }
 
+// The DOF lists come through fe_hip_set_constrained_dofs /
+// fe_hip_set_periodic_dofs on context(); without them both are no-ops.
+void feLinearSystemHIP::constrainEssentialComponents() { _status = fe_hip_constrain_essential_components(_ctx); }
+
+void feLinearSystemHIP::applyPeriodicity() { _status = fe_hip_apply_periodicity(_ctx); }
+
 feStatus createLinearSystem(feLinearSystem *&system, linearSolverType type, const feMesh &mesh,
                             const std::vector<feBilinearForm *> &forms, const feMetaNumber *numbering)
 {
@@ -207,6 +213,8 @@ feStatus solveNewtonRaphson(feLinearSystem *system, std::vector<double> &sol, co
     system->getRHSMaxNorm(&nres);
     if(it > 0 && nres <= tol.tolResidual) break;
     system->assembleMatrices(sol);
+    system->constrainEssentialComponents();
+    system->applyPeriodicity();
     double ndx = 0., nrhs = 0., naxb = 0.;
     int lin = 0;
     if(!system->solve(&ndx, &nrhs, &naxb, &lin)) return FE_STATUS_FAILED;

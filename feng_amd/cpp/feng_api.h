@@ -112,6 +112,8 @@ public:
   virtual void getRHSMaxNorm(double *norm) const = 0;
   virtual bool solve(double *normDx, double *normResidual, double *normAxb, int *nIter) = 0;
   virtual void correctSolution(std::vector<double> &sol) = 0;
+  virtual void constrainEssentialComponents() = 0;
+  virtual void applyPeriodicity() = 0;
 
 protected:
   double _rel_tol = 1e-8, _abs_tol = 1e-14, _div_tol = 1e6;
@@ -135,6 +137,8 @@ public:
   void getRHSMaxNorm(double *norm) const override;
   bool solve(double *normDx, double *normResidual, double *normAxb, int *nIter) override;
   void correctSolution(std::vector<double> &sol) override;
+  void constrainEssentialComponents() override;
+  void applyPeriodicity() override;
   // Krylov method / preconditioner / precision (what PETSc reads from
   // -ksp_type / -pc_type in the reference)
   void setSolverOptions(int ksp, int pc, int precision) { _ksp = ksp; _pc = pc; _prec = precision; }
