@@ -19,7 +19,7 @@ TRI_P1, TRI_P2, TRI_VP2, TET_P1 = 1, 2, 3, 4
 NLOC = {TRI_P1: 3, TRI_P2: 6, TRI_VP2: 12, TET_P1: 4}
 
 # weak forms (elementSystemType values, feSysElm.h:13-77)
-SOURCE, VECTOR_SOURCE, TRANSIENT_MASS, TRANSIENT_VECTOR_MASS, DIFFUSION, VECTOR_DIFFUSION = 0, 2, 13, 15, 16, 18
+SOURCE, VECTOR_SOURCE, MASS, TRANSIENT_MASS, TRANSIENT_VECTOR_MASS, DIFFUSION, VECTOR_DIFFUSION = 0, 2, 6, 13, 15, 16, 18
 VECTOR_CONVECTIVE_ACCELERATION, DIV_NEWTONIAN_STRESS = 22, 25
 MIXED_GRADIENT, MIXED_DIVERGENCE = 26, 31
 

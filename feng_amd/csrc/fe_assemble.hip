@@ -20,8 +20,9 @@ struct ScalarAsmArgs {
   int begin, count, E, n, what;
   const int *perm, *conn, *adr, *pos;
   const double *xyz, *tab, *u;
-  FeFn coeff, src, mass;
-  int hasDiff, hasSrc, hasMass;
+  FeFn coeff, src, mass, react;
+  int hasDiff, hasSrc, hasMass, hasReact;
+  double reactPow;  // MASS form: c u^p (p = 1: the plain mass)
   // time derivative u_dot = cn[0] u + sum_{j = 1}^{nsol - 1} cn[j] past[j - 1]
   int nsol;
   double cn[fe_hip_ctx::kMaxPast + 1];
@@ -30,8 +31,9 @@ struct ScalarAsmArgs {
 };
 
 // Scalar diffusion (feSysElm.cpp:530-583), transient mass
-// (feSysElm.cpp:426-458) and source (feSysElm.cpp:16-27) of one field and
-// quadrature rule, fused, on one element per thread.
+// (feSysElm.cpp:426-458), mass / mass power (feSysElm.cpp:186-225,
+// :260-278) and source (feSysElm.cpp:16-27) of one field and quadrature
+// rule, fused, on one element per thread.
 template <int NF, int DIM>
 __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
 {
@@ -70,7 +72,7 @@ __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
 #pragma unroll
     for(int j = 0; j < NF; ++j) Ae[i][j] = 0.;
   }
-  const bool doMat = (a.what & 1) && (a.hasDiff || a.hasMass), doRes = (a.what & 2);
+  const bool doMat = (a.what & 1) && (a.hasDiff || a.hasMass || a.hasReact), doRes = (a.what & 2);
   const int nq = T.nq;
   if(a.hasDiff) {
     for(int k = 0; k < nq; ++k) {
@@ -148,6 +150,35 @@ __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
         for(int j = 0; j < NF; ++j) uq += T.L[k][j] * ud[j];
 #pragma unroll
         for(int i = 0; i < NF; ++i) Be[i] -= T.L[k][i] * m * uq * jw;
+      }
+    }
+  }
+  if(a.hasReact) {
+    // Be -= c u^p phi_i; Ae += c p u^(p-1) phi_i phi_j (the derivative the
+    // reference approximates by finite differences for p != 1)
+    const double pw = a.reactPow;
+    for(int k = 0; k < nq; ++k) {
+      double x[3] = {0., 0., 0.};
+#pragma unroll
+      for(int j = 0; j < NV; ++j)
+#pragma unroll
+        for(int d = 0; d < 3; ++d) x[d] += c[j][d] * T.Lg[k][j];
+      const double cr = feEvalFn(a.react, x, e, nq, k);
+      const double jw = J * T.w[k];
+      double uq = 0.;
+#pragma unroll
+      for(int j = 0; j < NF; ++j) uq += T.L[k][j] * ul[j];
+      if(doMat) {
+        const double d = pw == 1. ? cr : cr * pw * pow(uq, pw - 1.);
+#pragma unroll
+        for(int i = 0; i < NF; ++i)
+#pragma unroll
+          for(int j = 0; j < NF; ++j) Ae[i][j] += d * T.L[k][i] * T.L[k][j] * jw;
+      }
+      if(doRes) {
+        const double up = pw == 1. ? uq : pow(uq, pw);
+#pragma unroll
+        for(int i = 0; i < NF; ++i) Be[i] -= cr * up * T.L[k][i] * jw;
       }
     }
   }
@@ -657,16 +688,19 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
   std::vector<bool> done(ctx->forms.size(), false);
   bool fp32Done = false;
   auto scalarForm = [](int t) {
-    return t == FE_SYS_DIFFUSION || t == FE_SYS_TRANSIENT_MASS || t == FE_SYS_SOURCE;
+    return t == FE_SYS_DIFFUSION || t == FE_SYS_TRANSIENT_MASS || t == FE_SYS_MASS || t == FE_SYS_SOURCE;
   };
   for(size_t fi = 0; fi < ctx->forms.size(); ++fi) {
     if(done[fi] || !scalarForm(ctx->forms[fi].sysType)) continue;
     const int field = ctx->forms[fi].fields[0], degree = ctx->forms[fi].degree;
-    int iDiff = -1, iMass = -1, iSrc = -1;
+    int iDiff = -1, iMass = -1, iReact = -1, iSrc = -1;
     for(size_t fj = fi; fj < ctx->forms.size(); ++fj) {
       const FeForm &G = ctx->forms[fj];
       if(done[fj] || !scalarForm(G.sysType) || G.fields[0] != field || G.degree != degree) continue;
-      int &slot = G.sysType == FE_SYS_DIFFUSION ? iDiff : (G.sysType == FE_SYS_TRANSIENT_MASS ? iMass : iSrc);
+      int &slot = G.sysType == FE_SYS_DIFFUSION         ? iDiff
+                  : G.sysType == FE_SYS_TRANSIENT_MASS ? iMass
+                  : G.sysType == FE_SYS_MASS           ? iReact
+                                                       : iSrc;
       if(slot < 0) {
         slot = (int)fj;
         done[fj] = true;
@@ -685,10 +719,13 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
     a.tab = ctx->forms[fi].d_tab;
     // Row map and scatter positions: the group of a matrix form of the sweep,
     // else (a lone source) a matrix group on the same field.
-    int g = iDiff >= 0 ? ctx->forms[iDiff].group : (iMass >= 0 ? ctx->forms[iMass].group : -1);
+    int g = iDiff >= 0 ? ctx->forms[iDiff].group
+            : iMass >= 0 ? ctx->forms[iMass].group
+            : iReact >= 0 ? ctx->forms[iReact].group
+                          : -1;
     for(size_t k = 0; g < 0 && k < ctx->groups.size(); ++k)
       if(ctx->groups[k].layoutI.size() == 1 && ctx->groups[k].layoutI[0] == field) g = (int)k;
-    FE_REQUIRE(ctx, g >= 0, "a SOURCE form needs a DIFFUSION or TRANSIENT_MASS form on the same field");
+    FE_REQUIRE(ctx, g >= 0, "a SOURCE form needs a matrix form (DIFFUSION, TRANSIENT_MASS, MASS) on the same field");
     a.adr = ctx->groups[g].d_adrI;
     a.pos = ctx->groups[g].d_pos;
     if(iDiff >= 0) {
@@ -702,13 +739,18 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
       for(int j = 0; j <= fe_hip_ctx::kMaxPast; ++j) a.cn[j] = ctx->bdf[j];
       for(int j = 0; j < fe_hip_ctx::kMaxPast; ++j) a.past[j] = ctx->d_past[j];
     }
+    if(iReact >= 0) {
+      a.hasReact = 1;
+      a.react = ctx->forms[iReact].coeffs[0];
+      a.reactPow = ctx->forms[iReact].coeffs.size() > 1 ? ctx->forms[iReact].coeffs[1].c0 : 1.;
+    }
     if(iSrc >= 0) {
       a.hasSrc = 1;
       a.src = ctx->forms[iSrc].coeffs[0];
     }
-    if(!(what & 2) && iDiff < 0 && iMass < 0) continue;  // a lone source has no matrix
+    if(!(what & 2) && iDiff < 0 && iMass < 0 && iReact < 0) continue;  // a lone source has no matrix
     const int kind = ctx->fields[field].kind;
-    if(iDiff >= 0 && iMass < 0 && ctx->asmKernel == 1) {
+    if(iDiff >= 0 && iMass < 0 && iReact < 0 && ctx->asmKernel == 1) {
       const FeForm &F = ctx->forms[iDiff];
       MfmaAsmArgs m{};
       m.E = a.E; m.n = a.n; m.what = what; m.K = F.mfmaK;

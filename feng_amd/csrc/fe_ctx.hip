@@ -305,6 +305,14 @@ int fe_hip_add_form(fe_hip_ctx *ctx, int sysType, int nfields, const int *fields
       F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = true;
       FE_REQUIRE(ctx, ncoeff == 1, "DIFFUSION takes one coefficient");
       break;
+    case FE_SYS_MASS:
+      // feSysElm_Mass (coefficient) or feSysElm_MassPower (coefficient,
+      // exponent as an FE_FN_CONST), which reuses the MASS id (feSysElm.h:408-409)
+      FE_REQUIRE(ctx, nfields == 1 && k0 != FE_TRI_VP2, "MASS takes one scalar field");
+      F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = true;
+      FE_REQUIRE(ctx, ncoeff == 1 || (ncoeff == 2 && coeffs[1].kind == FE_FN_CONST),
+                 "MASS takes a coefficient and optionally a constant exponent");
+      break;
     case FE_SYS_TRANSIENT_MASS:
       FE_REQUIRE(ctx, nfields == 1 && k0 != FE_TRI_VP2, "TRANSIENT_MASS takes one scalar field");
       F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = true;

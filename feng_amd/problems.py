@@ -52,14 +52,15 @@ def diffusion_system(mesh: A.Mesh, kind: int, degree: int, k: A.Fn, s: A.Fn, u_b
 
 
 def heat_system(mesh: A.Mesh, kind: int, degree: int, m: A.Fn, k: A.Fn, s: A.Fn, essential=True, coloring=None,
-                device=0, periodic=None):
+                device=0, periodic=None, react=None):
     """Transient scalar m u_t - div(k grad u) + S = 0: the form stack of the
     reference's unsteady scalar tests (TransientMass, Diffusion, Source --
     scalarFEunsteady.cpp:270-300), on the GPU.  essential: Dirichlet boundary
     (True) or none (False); periodic: (master, slave) DOF pairs entering the
     pattern.  k = None leaves out the diffusion form: with no
     essential boundary that is the ODE m u_t + S = 0 of
-    scalarFEunsteady.cpp:54-80.  Returns (S, num, forms) with forms =
+    scalarFEunsteady.cpp:54-80.  react = (c, p): a MassPower form c u^p (with
+    m u_t alone, the nonlinear ODE of scalarFEunsteady.cpp:134-165).  Returns (S, num, forms) with forms =
     {"mass", "diffusion", "source"} ids for fe_hip_update_coeff."""
     num = A.Numbering(mesh, [kind], [1 if essential else 0])
     S = A.LinearSystemHIP(device)
@@ -67,9 +68,12 @@ def heat_system(mesh: A.Mesh, kind: int, degree: int, m: A.Fn, k: A.Fn, s: A.Fn,
     f = S.add_field(kind, num.adr[0])
     S.set_sizes(num.nInc, num.nDofs)
     forms = {"mass": S.add_form(A.TRANSIENT_MASS, [f], degree, [m])}
+    if react is not None:
+        forms["react"] = S.add_form(A.MASS, [f], degree, [react[0], A.Fn(A.FN_CONST, float(react[1]))])
     if k is not None:
         forms["diffusion"] = S.add_form(A.DIFFUSION, [f], degree, [k])
-    forms["source"] = S.add_form(A.SOURCE, [f], degree, [s])
+    if s is not None:
+        forms["source"] = S.add_form(A.SOURCE, [f], degree, [s])
     if periodic is not None:
         S.set_periodic_dofs(*periodic)
     S.build_csr()

@@ -704,6 +704,56 @@ static void transient_mass_elem(const or_ref *R, const double *c, int e, or_fn c
   }
 }
 
+/* Mass (feSysElm.cpp:186-225, p = 1) and mass power (feSysElm.cpp:260-278,
+ * Be_i -= c u_h^p phi_i J w) on a whole mesh; the matrix is the derivative
+ * c p u_h^(p-1) phi_i phi_j J w (the reference takes it by finite
+ * differences of the residual for the power form).  val may be NULL. */
+int or_assemble_mass_power(int kind, int V, const double *xyz, int E, const int *conn, const int *adr, int n,
+                           const int *ia, const int *ja, int quad_degree, or_fn coeff, double p, const double *u,
+                           double *val, double *rhs)
+{
+  (void)V;
+  or_ref R;
+  if(ref_setup(kind, quad_degree, &R)) return -1;
+  const int nf = R.nf, nv = R.nv;
+  double c[12], Ae[144], Be[12], ul[12];
+  for(int e = 0; e < E; ++e) {
+    for(int j = 0; j < nv; ++j)
+      for(int d = 0; d < 3; ++d) c[3 * j + d] = xyz[3 * conn[e * nv + j] + d];
+    const int *a = adr + (size_t)e * nf;
+    for(int i = 0; i < nf; ++i) ul[i] = u[a[i]];
+    memset(Ae, 0, sizeof(Ae));
+    memset(Be, 0, sizeof(Be));
+    or_geo g;
+    if(elem_geo(&R, c, &g)) return -2 - e;
+    for(int k = 0; k < R.nq; ++k) {
+      double x[3];
+      qp_pos(&R, c, k, x);
+      const double cr = fn_eval(coeff, x, e, R.nq, k);
+      double uq = 0.;
+      for(int j = 0; j < nf; ++j) uq += R.L[k][j] * ul[j];
+      const double d = p == 1. ? cr : cr * p * pow(uq, p - 1.);
+      const double up = p == 1. ? uq : pow(uq, p);
+      for(int i = 0; i < nf; ++i) {
+        for(int j = 0; j < nf; ++j) Ae[i * nf + j] += d * R.L[k][i] * R.L[k][j] * g.J * R.qw[k];
+        Be[i] -= cr * up * R.L[k][i] * g.J * R.qw[k];
+      }
+    }
+    for(int i = 0; i < nf; ++i) {
+      if(a[i] >= n) continue;
+      rhs[a[i]] += Be[i];
+      if(val)
+        for(int j = 0; j < nf; ++j) {
+          if(a[j] >= n) continue;
+          int q = find_pos(ia, ja, a[i], a[j]);
+          if(q < 0) return -1;
+          val[q] += Ae[i * nf + j];
+        }
+    }
+  }
+  return 0;
+}
+
 int or_element_mass(int kind, const double *coords, int quad_degree, or_fn coeff, int elem, double *Ae)
 {
   or_ref R;

@@ -96,6 +96,11 @@ int or_assemble_transient_mass(int kind, int V, const double *xyz, int E, const 
 int or_assemble_vector_transient_mass(int V, const double *xyz, int E, const int *conn, const int *adrU, int n,
                                       const int *ia, const int *ja, int quad_degree, double m, double c0,
                                       const double *udot, double *val, double *rhs);
+/* Mass (p = 1) / mass power c u^p (feSysElm.cpp:186-225, :260-278) with
+ * the derivative matrix c p u^(p-1) phi_i phi_j; val may be NULL. */
+int or_assemble_mass_power(int kind, int V, const double *xyz, int E, const int *conn, const int *adr, int n,
+                           const int *ia, const int *ja, int quad_degree, or_fn coeff, double p, const double *u,
+                           double *val, double *rhs);
 /* Element mass matrix (coefficient m, c0 = 1) of one element (for unit checks). */
 int or_element_mass(int kind, const double *coords, int quad_degree, or_fn coeff, int elem, double *Ae);
 

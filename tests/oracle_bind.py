@@ -41,6 +41,8 @@ class Oracle:
                                             OrFn, OrFn, dp, ip, dp, dp]
         L.or_element_diffusion.argtypes = [C.c_int, dp, C.c_int, OrFn, C.c_int, dp]
         L.or_element_mass.argtypes = [C.c_int, dp, C.c_int, OrFn, C.c_int, dp]
+        L.or_assemble_mass_power.argtypes = [C.c_int, C.c_int, dp, C.c_int, ip, ip, C.c_int, ip, ip, C.c_int,
+                                             OrFn, C.c_double, dp, dp, dp]
         L.or_assemble_vector_transient_mass.argtypes = [C.c_int, dp, C.c_int, ip, ip, C.c_int, ip, ip, C.c_int,
                                                         C.c_double, C.c_double, dp, dp, dp]
         L.or_assemble_transient_mass.argtypes = [C.c_int, C.c_int, dp, C.c_int, ip, ip, C.c_int, ip, ip, C.c_int,
@@ -177,6 +179,16 @@ class Oracle:
                                                       P(ja, C.c_int), deg, m, c0, P(udot, C.c_double),
                                                       None if val is None else P(val, C.c_double),
                                                       P(rhs, C.c_double))
+        assert rc == 0, rc
+
+    def assemble_mass_power(self, kind, xyz, conn, adr, n, ia, ja, deg, coeff, p, u, val, rhs):
+        nvpe = 4 if kind == 4 else 3
+        o = OrFn(coeff[0], coeff[1], coeff[2], None)
+        u = np.ascontiguousarray(u, np.float64)
+        rc = self.L.or_assemble_mass_power(kind, xyz.size // 3, P(xyz, C.c_double), conn.size // nvpe,
+                                           P(conn, C.c_int), P(adr, C.c_int), n, P(ia, C.c_int), P(ja, C.c_int),
+                                           deg, o, p, P(u, C.c_double), None if val is None else P(val, C.c_double),
+                                           P(rhs, C.c_double))
         assert rc == 0, rc
 
     def element_mass(self, kind, coords, deg, coeff=(0, 1.0, 0.0, None), elem=0):

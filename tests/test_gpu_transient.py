@@ -2,7 +2,9 @@
 BDF history gathered per element, driven by feng_amd.timeint.
 
 - The reference's golden tables scalarFEunsteady_linearODE.output (BDF1;
-  BDF2 with the exact-solution starter), to the 7 printed digits.
+  BDF2 with the exact-solution starter), to the 7 printed digits, and
+  scalarFEunsteady_nonlinearODE.output (u' = -u^2 through the mass-power
+  form) to 7 digits or 5e-11 absolute (the reference's Newton tolerance).
 - Heat equations (P2 triangles with block coefficients and a time-dependent
   quadrature-point source and Dirichlet data; P1 tets with AMG-PCG) match
   the BDF oracle (oracle/fe_oracle_time.py over the C oracle's assembly)
@@ -18,7 +20,8 @@ import numpy as np
 import pytest
 
 from oracle import fe_oracle_time as ot
-from tests.test_oracle_time import LINEAR_ODE_TABLE, OracleScalarTransient, _qp_table, table_matches
+from tests.test_oracle_time import (LINEAR_ODE_TABLE, NONLINEAR_ODE_TABLE, OracleScalarTransient, _qp_table,
+                                    table_matches)
 
 pytestmark = pytest.mark.gpu
 
@@ -44,6 +47,31 @@ def test_linear_ode_reference_table(scheme):
         err.append(max(post[1]))
     S.close()
     assert table_matches(err, LINEAR_ODE_TABLE[scheme]), err
+
+
+@pytest.mark.parametrize("scheme", ["BDF1", "BDF2"])
+def test_nonlinear_ode_reference_table(scheme):
+    """u' = -u^2 through TRANSIENT_MASS + the MASS form with exponent 2
+    (feSysElm_MassPower), Newton on the device."""
+    from feng_amd.api import FN_CONST, Fn, Mesh
+    from feng_amd.problems import NLSolverOptions, heat_system
+    from feng_amd.timeint import BDF2_START_EXACT, TimeIntegrator, linf_nodal_error
+    print("nonlinear ODE", scheme, flush=True)
+    m = Mesh.square(3)
+    S, num, forms = heat_system(m, 1, 8, Fn(FN_CONST, 1.0), None, None, essential=False,
+                                react=(Fn(FN_CONST, 1.0), 2.0))
+    err = []
+    for nsteps in (10, 20, 40, 80, 160):
+        ti = TimeIntegrator(S, 1 if scheme == "BDF1" else 2, 0.0, 1.0, nsteps, np.ones(num.nDofs),
+                            nl=NLSolverOptions(1e-13, 1e-13, 1e4, 10),
+                            norms=[linf_nodal_error(num, lambda t, x, y, z: 1.0 / (1.0 + t) + 0 * x)],
+                            exact=lambda t: np.full(num.nDofs, 1.0 / (1.0 + t)), start=BDF2_START_EXACT,
+                            solve_kw=dict(rtol=1e-14))
+        post = ti.make_steps()
+        assert all(r.newton_iterations <= 6 for r in ti.reports)
+        err.append(max(post[1]))
+    S.close()
+    assert table_matches(err, NONLINEAR_ODE_TABLE[scheme], atol=5e-11), err
 
 
 def _heat_case(case, oracle):
