@@ -210,6 +210,184 @@ __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
 }
 
 
+// The same fused forms with one lane per local row: LPE lanes (4 for P1, 8
+// for P2) share an element, lane i owning row i of Ae and Be.  The element's
+// vertices, DOF values and time derivatives are loaded once by the lane
+// that owns them and exchanged with shuffles; every lane rebuilds the
+// reference-element gradients from the LDS tables and accumulates its own
+// row in the scalar kernel's order.  A color of ~50k elements becomes
+// ~3,100 wavefronts instead of ~780, so the dependent-load chain of the
+// colored sweep is hidden by four times the occupancy.
+template <int NF, int DIM>
+__global__ __launch_bounds__(256) void asmScalarLanes(ScalarAsmArgs a)
+{
+  __shared__ FeRefTables T;
+  {
+    const double *src = a.tab;
+    double *dst = (double *)&T;
+    for(int i = threadIdx.x; i < (int)(sizeof(FeRefTables) / sizeof(double)); i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
+  constexpr int LPE = NF <= 4 ? 4 : 8;
+  constexpr int NV = DIM + 1;
+  const int lane = threadIdx.x & 63;
+  const int i = lane % LPE, base = lane - i;
+  const int t = (int)((blockIdx.x * blockDim.x + threadIdx.x) / LPE);
+  const bool valid = t < a.count;
+  const int ee = a.begin + (valid ? t : 0);
+  const int e = a.perm[ee];
+  double vc[3] = {0., 0., 0.};
+  if(i < NV) {
+    const int v = a.conn[(size_t)e * NV + i];
+#pragma unroll
+    for(int d = 0; d < 3; ++d) vc[d] = a.xyz[3 * (size_t)v + d];
+  }
+  double c[NV][3];
+#pragma unroll
+  for(int j = 0; j < NV; ++j)
+#pragma unroll
+    for(int d = 0; d < 3; ++d) c[j][d] = __shfl(vc[d], base + j, 64);
+  double J, drdx[3], drdy[3], drdz[3];
+  feGeometry<DIM>(c, J, drdx, drdy, drdz);
+  const bool row = i < NF;
+  const int ad = row ? a.adr[(size_t)ee * NF + i] : 0;
+  const double uo = row ? a.u[ad] : 0.;
+  double udo = 0.;
+  const bool doMat = (a.what & 1) && (a.hasDiff || a.hasMass || a.hasReact), doRes = (a.what & 2);
+  if(a.hasMass && doRes && row) {
+    udo = a.cn[0] * uo;
+    for(int j = 1; j < a.nsol; ++j) udo += a.cn[j] * a.past[j - 1][ad];
+  }
+  double ul[NF], ud[NF];
+#pragma unroll
+  for(int j = 0; j < NF; ++j) {
+    ul[j] = __shfl(uo, base + j, 64);
+    ud[j] = __shfl(udo, base + j, 64);
+  }
+  const int ir = row ? i : 0;  // LDS column of this lane's row
+  double Ae[NF], Be = 0.;
+#pragma unroll
+  for(int j = 0; j < NF; ++j) Ae[j] = 0.;
+  const int nq = T.nq;
+  if(a.hasDiff) {
+    for(int k = 0; k < nq; ++k) {
+      double x[3] = {0., 0., 0.};
+#pragma unroll
+      for(int j = 0; j < NV; ++j)
+#pragma unroll
+        for(int d = 0; d < 3; ++d) x[d] += c[j][d] * T.Lg[k][j];
+      const double kc = feEvalFn(a.coeff, x, e, nq, k);
+      double gp[NF][DIM], gi[DIM];
+#pragma unroll
+      for(int j = 0; j < NF; ++j) {
+        if(DIM == 2) {
+          gp[j][0] = T.dLr[k][j] * drdx[0] + T.dLs[k][j] * drdx[1];
+          gp[j][1 % DIM] = T.dLr[k][j] * drdy[0] + T.dLs[k][j] * drdy[1];
+        } else {
+          gp[j][0] = T.dLr[k][j] * drdx[0] + T.dLs[k][j] * drdx[1] + T.dLt[k][j] * drdx[2];
+          gp[j][1 % DIM] = T.dLr[k][j] * drdy[0] + T.dLs[k][j] * drdy[1] + T.dLt[k][j] * drdy[2];
+          gp[j][2 % DIM] = T.dLr[k][j] * drdz[0] + T.dLs[k][j] * drdz[1] + T.dLt[k][j] * drdz[2];
+        }
+      }
+      if(DIM == 2) {
+        gi[0] = T.dLr[k][ir] * drdx[0] + T.dLs[k][ir] * drdx[1];
+        gi[1 % DIM] = T.dLr[k][ir] * drdy[0] + T.dLs[k][ir] * drdy[1];
+      } else {
+        gi[0] = T.dLr[k][ir] * drdx[0] + T.dLs[k][ir] * drdx[1] + T.dLt[k][ir] * drdx[2];
+        gi[1 % DIM] = T.dLr[k][ir] * drdy[0] + T.dLs[k][ir] * drdy[1] + T.dLt[k][ir] * drdy[2];
+        gi[2 % DIM] = T.dLr[k][ir] * drdz[0] + T.dLs[k][ir] * drdz[1] + T.dLt[k][ir] * drdz[2];
+      }
+      const double jw = J * T.w[k];
+      if(doMat) {
+#pragma unroll
+        for(int j = 0; j < NF; ++j)
+#pragma unroll
+          for(int d = 0; d < DIM; ++d) Ae[j] += kc * gi[d] * gp[j][d] * jw;
+      }
+      if(doRes) {
+        double gu[DIM];
+#pragma unroll
+        for(int d = 0; d < DIM; ++d) gu[d] = 0.;
+#pragma unroll
+        for(int j = 0; j < NF; ++j)
+#pragma unroll
+          for(int d = 0; d < DIM; ++d) gu[d] += ul[j] * gp[j][d];
+#pragma unroll
+        for(int d = 0; d < DIM; ++d) Be -= kc * gi[d] * gu[d] * jw;
+      }
+    }
+  }
+  if(a.hasMass) {
+    const double c0 = a.cn[0];
+    for(int k = 0; k < nq; ++k) {
+      double x[3] = {0., 0., 0.};
+#pragma unroll
+      for(int j = 0; j < NV; ++j)
+#pragma unroll
+        for(int d = 0; d < 3; ++d) x[d] += c[j][d] * T.Lg[k][j];
+      const double m = feEvalFn(a.mass, x, e, nq, k);
+      const double jw = J * T.w[k];
+      const double Li = T.L[k][ir];
+      if(doMat) {
+#pragma unroll
+        for(int j = 0; j < NF; ++j) Ae[j] += Li * m * c0 * T.L[k][j] * jw;
+      }
+      if(doRes) {
+        double uq = 0.;
+#pragma unroll
+        for(int j = 0; j < NF; ++j) uq += T.L[k][j] * ud[j];
+        Be -= Li * m * uq * jw;
+      }
+    }
+  }
+  if(a.hasReact) {
+    const double pw = a.reactPow;
+    for(int k = 0; k < nq; ++k) {
+      double x[3] = {0., 0., 0.};
+#pragma unroll
+      for(int j = 0; j < NV; ++j)
+#pragma unroll
+        for(int d = 0; d < 3; ++d) x[d] += c[j][d] * T.Lg[k][j];
+      const double cr = feEvalFn(a.react, x, e, nq, k);
+      const double jw = J * T.w[k];
+      const double Li = T.L[k][ir];
+      double uq = 0.;
+#pragma unroll
+      for(int j = 0; j < NF; ++j) uq += T.L[k][j] * ul[j];
+      if(doMat) {
+        const double dd = pw == 1. ? cr : cr * pw * pow(uq, pw - 1.);
+#pragma unroll
+        for(int j = 0; j < NF; ++j) Ae[j] += dd * Li * T.L[k][j] * jw;
+      }
+      if(doRes) {
+        const double up = pw == 1. ? uq : pow(uq, pw);
+        Be -= cr * up * Li * jw;
+      }
+    }
+  }
+  if(doRes && a.hasSrc) {
+    for(int k = 0; k < nq; ++k) {
+      double x[3] = {0., 0., 0.};
+#pragma unroll
+      for(int j = 0; j < NV; ++j)
+#pragma unroll
+        for(int d = 0; d < 3; ++d) x[d] += c[j][d] * T.Lg[k][j];
+      const double S = feEvalFn(a.src, x, e, nq, k);
+      const double jw = J * T.w[k];
+      Be -= T.L[k][ir] * S * jw;
+    }
+  }
+  if(!valid || !row) return;
+  if(doRes && ad < a.n) a.rhs[ad] += Be;
+  if(doMat) {
+#pragma unroll
+    for(int j = 0; j < NF; ++j) {
+      const int p = a.pos[(size_t)(i * NF + j) * a.E + ee];
+      if(p >= 0) a.val[p] += Ae[j];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Taylor-Hood P2-P1 Navier-Stokes form stack fused in one kernel (divergence
 // formulation of the reference's navier_stokes tests, navier_stokes.cpp:83-99):
@@ -651,11 +829,15 @@ __global__ void addCorrection(int n, double *__restrict__ u, const double *__res
 template <int NF, int DIM>
 void launchScalar(fe_hip_ctx *ctx, ScalarAsmArgs a)
 {
+  constexpr int LPE = NF <= 4 ? 4 : 8;
   for(size_t c = 0; c + 1 < ctx->colorStart.size(); ++c) {
     a.begin = ctx->colorStart[c];
     a.count = ctx->colorStart[c + 1] - a.begin;
     if(a.count == 0) continue;
-    hipLaunchKernelGGL((asmScalarFP64<NF, DIM>), dim3((a.count + 255) / 256), dim3(256), 0, ctx->stream, a);
+    if(ctx->scalarLanes)
+      hipLaunchKernelGGL((asmScalarLanes<NF, DIM>), dim3((a.count * LPE + 255) / 256), dim3(256), 0, ctx->stream, a);
+    else
+      hipLaunchKernelGGL((asmScalarFP64<NF, DIM>), dim3((a.count + 255) / 256), dim3(256), 0, ctx->stream, a);
   }
 }
 
@@ -668,8 +850,7 @@ extern "C" int fe_hip_set_to_zero(fe_hip_ctx *ctx, int matrix, int residual)
   if(matrix) FE_HIP_TRY(ctx, hipMemsetAsync(ctx->d_val, 0, sizeof(double) * ctx->nnz, ctx->stream));
   if(matrix && ctx->d_val32) FE_HIP_TRY(ctx, hipMemsetAsync(ctx->d_val32, 0, sizeof(float) * ctx->nnz, ctx->stream));
   if(residual) FE_HIP_TRY(ctx, hipMemsetAsync(ctx->d_rhs, 0, sizeof(double) * ctx->n, ctx->stream));
-  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  return FE_STATUS_OK;
+  return FE_STATUS_OK;  // stream-ordered before the next assembly
 }
 
 extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
@@ -678,10 +859,14 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
   FE_REQUIRE(ctx, ctx->csrBuilt, "build the CSR first");
   FE_REQUIRE(ctx, what >= 1 && what <= 3, "what must be 1 (matrix), 2 (residual) or 3 (both)");
   FE_HIP_TRY(ctx, hipSetDevice(ctx->device));
-  hipEvent_t t0, t1;
-  FE_HIP_TRY(ctx, hipEventCreate(&t0));
-  FE_HIP_TRY(ctx, hipEventCreate(&t1));
-  FE_HIP_TRY(ctx, hipEventRecord(t0, ctx->stream));
+  // the sweep is bracketed by two events and not waited for: its time is
+  // read when fe_hip_kernel_stats asks (the next host read waits anyway)
+  if(!ctx->asmEv[0]) {
+    FE_HIP_TRY(ctx, hipEventCreate(&ctx->asmEv[0]));
+    FE_HIP_TRY(ctx, hipEventCreate(&ctx->asmEv[1]));
+  }
+  FE_HIP_TRY(ctx, hipEventRecord(ctx->asmEv[0], ctx->stream));
+  ctx->asmPending = false;
   // Scalar forms of one field and quadrature rule (at most one DIFFUSION,
   // one TRANSIENT_MASS and one SOURCE) are fused: one element sweep computes
   // Ae and Be of all of them.
@@ -857,13 +1042,8 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
     if(!ctx->d_val32 && feAlloc(ctx, &ctx->d_val32, (size_t)ctx->nnz)) return FE_STATUS_OUT_OF_MEMORY;
     hipLaunchKernelGGL(toFloat, dim3(2048), dim3(256), 0, ctx->stream, ctx->nnz, ctx->d_val, ctx->d_val32);
   }
-  FE_HIP_TRY(ctx, hipEventRecord(t1, ctx->stream));
-  FE_HIP_TRY(ctx, hipEventSynchronize(t1));
-  float ms = 0.f;
-  FE_HIP_TRY(ctx, hipEventElapsedTime(&ms, t0, t1));
-  ctx->lastAsmMs = ms;
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
+  FE_HIP_TRY(ctx, hipEventRecord(ctx->asmEv[1], ctx->stream));
+  ctx->asmPending = true;
   return FE_STATUS_OK;
 }
 
@@ -886,7 +1066,6 @@ extern "C" int fe_hip_correct_solution(fe_hip_ctx *ctx)
   if(!ctx) return FE_STATUS_POINTER_ERROR;
   FE_REQUIRE(ctx, ctx->csrBuilt, "build the CSR first");
   hipLaunchKernelGGL(addCorrection, dim3((ctx->n + 255) / 256), dim3(256), 0, ctx->stream, ctx->n, ctx->d_u, ctx->d_du);
-  FE_HIP_TRY(ctx, hipGetLastError());
-  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  FE_HIP_TRY(ctx, hipGetLastError());  // stream-ordered: the next read of u waits for it
   return FE_STATUS_OK;
 }

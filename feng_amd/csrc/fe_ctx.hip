@@ -185,6 +185,8 @@ int fe_hip_ctx_destroy(fe_hip_ctx *ctx)
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   for(hipEvent_t e : ctx->spmvTimer.ev) (void)hipEventDestroy(e);
+  for(hipEvent_t e : ctx->asmEv)
+    if(e) (void)hipEventDestroy(e);
   feMpnnRelease(ctx);
   feBatchRelease(ctx);
   feAmgRelease(ctx);
@@ -467,8 +469,7 @@ int fe_hip_rotate_solutions(fe_hip_ctx *ctx)
   for(int j = fe_hip_ctx::kMaxPast - 1; j > 0; --j) ctx->d_past[j] = ctx->d_past[j - 1];
   ctx->d_past[0] = last;
   FE_HIP_TRY(ctx, hipMemcpyAsync(last, ctx->d_u, sizeof(double) * ctx->nDofs, hipMemcpyDeviceToDevice,
-                                 ctx->stream));
-  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+                                 ctx->stream));  // stream-ordered before the next sweep
   return FE_STATUS_OK;
 }
 
@@ -542,6 +543,9 @@ int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value)
   } else if(k == "iter_block") {
     FE_REQUIRE(ctx, value == 256 || value == 512 || value == 1024, "iter_block must be 256, 512 or 1024");
     ctx->iterBlock = value;
+  } else if(k == "scalar_lanes") {
+    FE_REQUIRE(ctx, value == 0 || value == 1, "scalar_lanes must be 0 or 1");
+    ctx->scalarLanes = value;
   } else if(k == "asm_kernel") {
     FE_REQUIRE(ctx, value == 0 || value == 1, "asm_kernel must be 0 (scalar) or 1 (MFMA)");
     ctx->asmKernel = value;
@@ -628,6 +632,7 @@ int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value)
   else if(k == "iter_block") *value = ctx->iterBlock;
   else if(k == "iter_grid") *value = ctx->iterGrid;
   else if(k == "asm_kernel") *value = ctx->asmKernel;
+  else if(k == "scalar_lanes") *value = ctx->scalarLanes;
   else if(k == "fp32_values") *value = ctx->fp32Values;
   else if(k == "pipe_kernel") *value = ctx->pipeKernel;
   else if(k == "ic0_sweeps") *value = ctx->icSweeps;
@@ -661,6 +666,13 @@ int fe_hip_kernel_stats(fe_hip_ctx *ctx, int which, double *avg_ms, int64_t *lau
     if(avg_ms) *avg_ms = ctx->spmvTimer.launches ? ctx->spmvTimer.total_ms / ctx->spmvTimer.launches : 0.;
     if(launches) *launches = ctx->spmvTimer.launches;
   } else if(which == 1) {
+    if(ctx->asmPending) {
+      float ms = 0.f;
+      FE_HIP_TRY(ctx, hipEventSynchronize(ctx->asmEv[1]));
+      FE_HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->asmEv[0], ctx->asmEv[1]));
+      ctx->lastAsmMs = ms;
+      ctx->asmPending = false;
+    }
     if(avg_ms) *avg_ms = ctx->lastAsmMs;
     if(launches) *launches = 1;
   } else {

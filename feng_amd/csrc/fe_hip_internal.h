@@ -173,6 +173,7 @@ struct fe_hip_ctx {
   int iterGrid = 1024;
   bool lanesFixed = false;
   int asmKernel = 1;     // 0: scalar FP64 loop (reference order), 1: MFMA GEMM form
+  int scalarLanes = 0;   // scalar sweeps: 1 = one lane per local row (measured no faster), 0 = one thread per element
   int fp32Values = 1;    // FP32 values: 0 = rounded FP64 values, 1 = FP32 MFMA element matrices
   int pipeKernel = 2;  // pipelined PCG: 0 Chronopoulos-Gear, 1 tiled Ghysels-Vanroose, 2 by precision
   int icSweeps = 0;    // FE_PC_IC0: 0 exact ILU(0), > 0 parallel IC(0) sweeps
@@ -214,6 +215,8 @@ struct fe_hip_ctx {
   bool profiling = false;
   FeKernelTimer spmvTimer;
   double lastAsmMs = 0.;
+  hipEvent_t asmEv[2] = {nullptr, nullptr};  // bracket the last fe_hip_assemble, read lazily
+  bool asmPending = false;
 
   std::vector<void *> owned;  // device allocations freed at destroy
 
