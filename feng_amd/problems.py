@@ -129,6 +129,31 @@ def solve_newton(S: A.LinearSystemHIP, opts: NLSolverOptions = NLSolverOptions()
     return rep
 
 
+def taylor_hood_system(mesh: A.Mesh, degree: int, mu: float = 1.0, convection=None, source=None, u_bc=None,
+                       p_bc=None, coloring=None, device=0):
+    """Taylor-Hood P2-P1 Stokes or Navier-Stokes with Dirichlet velocity and
+    pressure on the boundary: the form stacks of the reference's MMS tests
+    (stokes.cpp:76-85, divergence formulation: MixedDivergence {p, u} (1),
+    VectorSource, DivergenceNewtonianStress {u, p} (1, mu); navier_stokes.cpp
+    adds VectorConvectiveAcceleration first with coefficient `convection`).
+    source: the two Fn components of the vector source (None: zero);
+    u_bc(x, y, z, comp), p_bc(x, y, z, comp): the essential values."""
+    num = A.Numbering(mesh, [A.TRI_VP2, A.TRI_P1], [1, 1])
+    S = A.LinearSystemHIP(device)
+    S.upload_mesh(mesh, coloring)
+    fu = S.add_field(A.TRI_VP2, num.adr[0])
+    fp = S.add_field(A.TRI_P1, num.adr[1])
+    S.set_sizes(num.nInc, num.nDofs)
+    if convection is not None:
+        S.add_form(A.VECTOR_CONVECTIVE_ACCELERATION, [fu], degree, [A.Fn(A.FN_CONST, float(convection))])
+    S.add_form(A.MIXED_DIVERGENCE, [fp, fu], degree, [A.Fn(A.FN_CONST, 1.0)])
+    S.add_form(A.VECTOR_SOURCE, [fu], degree, source or [A.Fn(A.FN_CONST, 0.0), A.Fn(A.FN_CONST, 0.0)])
+    S.add_form(A.DIV_NEWTONIAN_STRESS, [fu, fp], degree, [A.Fn(A.FN_CONST, 1.0), A.Fn(A.FN_CONST, float(mu))])
+    S.build_csr()
+    S.set_solution(A.essential_values(num, [u_bc, p_bc]))
+    return S, num
+
+
 def kovasznay(Re):
     """Kovasznay flow: lambda = Re/2 - sqrt(Re^2/4 + 4 pi^2), u = 1 - e^{lx}
     cos 2 pi y, v = l/(2 pi) e^{lx} sin 2 pi y, p = (1 - e^{2 l x}) / 2."""
