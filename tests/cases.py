@@ -10,7 +10,7 @@ from tests.conftest import golden
 
 DIFF_CASES = ["sq8_p1_sinsin", "sq6j_p1_blocks_q5", "sq8_p2_sinsin", "sq6j_p2_blocks", "sq4_p2_sinsin_q8",
               "cube3_p1_smooth_q1", "cube3_p1_smooth_q2", "mms1_p1_sinsin", "mms2_p2_sinsin", "mms3_p2_blocks"]
-NS_CASES = ["sq4_th_kovasznay", "sq5j_th_kovasznay"]
+NS_CASES = ["sq4_th_kovasznay", "sq5j_th_kovasznay", "data2_th_kovasznay_q8"]
 
 FN_CONST, FN_SINSIN, FN_BLOCKS, FN_SMOOTH3 = 0, 1, 2, 3
 

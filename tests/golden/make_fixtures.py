@@ -39,6 +39,7 @@ CASES = [
     ("mms3_p2_blocks", ("file", "square3.msh"), "diff2d_blocks", 2, 2, "4"),
     ("sq4_th_kovasznay", ("square", 4, 0.0, 0), "ns_kovasznay", 2, 5, "40"),
     ("sq5j_th_kovasznay", ("square", 5, 0.2, 1), "ns_kovasznay", 2, 5, "40"),
+    ("data2_th_kovasznay_q8", ("file", "data/square2.msh"), "ns_kovasznay", 2, 8, "1"),
 ]
 
 

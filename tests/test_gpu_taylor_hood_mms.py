@@ -2,7 +2,8 @@
 
 stokes_MMS.output and navier_stokes_MMS.output (tests/withLinearSolver/ of
 the reference), P2-P1 divergence formulation rows: u = (x^4 y^4,
--4/5 x^3 y^5), p = x^2 y^2 on the reference's square1-4.msh, degree-8
+-4/5 x^3 y^5), p = x^2 y^2 on the reference's data/square1-4.msh (copied
+to tests/golden/meshes/data/; the scalar tables use data/mmsMeshes/), degree-8
 quadrature, u and p essential on "Bord", the manufactured source tabulated at
 the quadrature points; Newton with GMRES/ILU(0) on the device; the L2 errors
 of u (vector) and p by the device norms.  Printed by the reference with 7
@@ -50,7 +51,7 @@ def test_reference_mms_table(case):
     eu, ep = [], []
     for lvl in range(1, 5):
         print(case, "square", lvl, flush=True)
-        m = Mesh.read_msh(GOLDEN / "meshes" / f"square{lvl}.msh")
+        m = Mesh.read_msh(GOLDEN / "meshes" / "data" / f"square{lvl}.msh")
         src = [Fn(FN_QP, data=_qp_table(m, 8, lambda x, y, c=c: _src(x, y, conv)[c])) for c in range(2)]
         ubc = lambda x, y, z, comp: np.where(comp == 0, x ** 4 * y ** 4, -4.0 / 5.0 * x ** 3 * y ** 5)
         pbc = lambda x, y, z, comp: x * x * y * y
