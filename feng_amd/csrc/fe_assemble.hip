@@ -20,8 +20,8 @@ struct ScalarAsmArgs {
   int begin, count, E, n, what;
   const int *perm, *conn, *adr, *pos;
   const double *xyz, *tab, *u;
-  FeFn coeff, src, mass, react;
-  int hasDiff, hasSrc, hasMass, hasReact;
+  FeFn coeff, src, mass, react, adv[3];
+  int hasDiff, hasSrc, hasMass, hasReact, hasAdv;
   double reactPow;  // MASS form: c u^p (p = 1: the plain mass)
   // time derivative u_dot = cn[0] u + sum_{j = 1}^{nsol - 1} cn[j] past[j - 1]
   int nsol;
@@ -30,10 +30,10 @@ struct ScalarAsmArgs {
   double *val, *rhs;
 };
 
-// Scalar diffusion (feSysElm.cpp:530-583), transient mass
-// (feSysElm.cpp:426-458), mass / mass power (feSysElm.cpp:186-225,
-// :260-278) and source (feSysElm.cpp:16-27) of one field and quadrature
-// rule, fused, on one element per thread.
+// Scalar diffusion (feSysElm.cpp:530-583), advection (feSysElm.cpp:766-835),
+// transient mass (feSysElm.cpp:426-458), mass / mass power
+// (feSysElm.cpp:186-225, :260-278) and source (feSysElm.cpp:16-27) of one
+// field and quadrature rule, fused, on one element per thread.
 template <int NF, int DIM>
 __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
 {
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
 #pragma unroll
     for(int j = 0; j < NF; ++j) Ae[i][j] = 0.;
   }
-  const bool doMat = (a.what & 1) && (a.hasDiff || a.hasMass || a.hasReact), doRes = (a.what & 2);
+  const bool doMat = (a.what & 1) && (a.hasDiff || a.hasMass || a.hasReact || a.hasAdv), doRes = (a.what & 2);
   const int nq = T.nq;
   if(a.hasDiff) {
     for(int k = 0; k < nq; ++k) {
@@ -115,6 +115,50 @@ __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
         for(int i = 0; i < NF; ++i)
 #pragma unroll
           for(int d = 0; d < DIM; ++d) Be[i] -= kc * gp[i][d] * gu[d] * jw;
+      }
+    }
+  }
+  if(a.hasAdv) {
+    // Ae_ij -= c . grad(phi_i) phi_j, Be_i += u c . grad(phi_i) (conservative
+    // form, feSysElm.cpp:766-835)
+    for(int k = 0; k < nq; ++k) {
+      double x[3] = {0., 0., 0.};
+#pragma unroll
+      for(int j = 0; j < NV; ++j)
+#pragma unroll
+        for(int d = 0; d < 3; ++d) x[d] += c[j][d] * T.Lg[k][j];
+      double cv[DIM];
+#pragma unroll
+      for(int d = 0; d < DIM; ++d) cv[d] = feEvalFn(a.adv[d], x, e, nq, k);
+      double gp[NF][DIM];
+#pragma unroll
+      for(int i = 0; i < NF; ++i) {
+        if(DIM == 2) {
+          gp[i][0] = T.dLr[k][i] * drdx[0] + T.dLs[k][i] * drdx[1];
+          gp[i][1 % DIM] = T.dLr[k][i] * drdy[0] + T.dLs[k][i] * drdy[1];
+        } else {
+          gp[i][0] = T.dLr[k][i] * drdx[0] + T.dLs[k][i] * drdx[1] + T.dLt[k][i] * drdx[2];
+          gp[i][1 % DIM] = T.dLr[k][i] * drdy[0] + T.dLs[k][i] * drdy[1] + T.dLt[k][i] * drdy[2];
+          gp[i][2 % DIM] = T.dLr[k][i] * drdz[0] + T.dLs[k][i] * drdz[1] + T.dLt[k][i] * drdz[2];
+        }
+      }
+      const double jw = J * T.w[k];
+      if(doMat) {
+#pragma unroll
+        for(int i = 0; i < NF; ++i)
+#pragma unroll
+          for(int j = 0; j < NF; ++j)
+#pragma unroll
+            for(int d = 0; d < DIM; ++d) Ae[i][j] -= cv[d] * gp[i][d] * T.L[k][j] * jw;
+      }
+      if(doRes) {
+        double uq = 0.;
+#pragma unroll
+        for(int j = 0; j < NF; ++j) uq += T.L[k][j] * ul[j];
+#pragma unroll
+        for(int i = 0; i < NF; ++i)
+#pragma unroll
+          for(int d = 0; d < DIM; ++d) Be[i] += cv[d] * uq * gp[i][d] * jw;
       }
     }
   }
@@ -253,7 +297,7 @@ __global__ __launch_bounds__(256) void asmScalarLanes(ScalarAsmArgs a)
   const int ad = row ? a.adr[(size_t)ee * NF + i] : 0;
   const double uo = row ? a.u[ad] : 0.;
   double udo = 0.;
-  const bool doMat = (a.what & 1) && (a.hasDiff || a.hasMass || a.hasReact), doRes = (a.what & 2);
+  const bool doMat = (a.what & 1) && (a.hasDiff || a.hasMass || a.hasReact || a.hasAdv), doRes = (a.what & 2);
   if(a.hasMass && doRes && row) {
     udo = a.cn[0] * uo;
     for(int j = 1; j < a.nsol; ++j) udo += a.cn[j] * a.past[j - 1][ad];
@@ -314,6 +358,40 @@ __global__ __launch_bounds__(256) void asmScalarLanes(ScalarAsmArgs a)
           for(int d = 0; d < DIM; ++d) gu[d] += ul[j] * gp[j][d];
 #pragma unroll
         for(int d = 0; d < DIM; ++d) Be -= kc * gi[d] * gu[d] * jw;
+      }
+    }
+  }
+  if(a.hasAdv) {
+    for(int k = 0; k < nq; ++k) {
+      double x[3] = {0., 0., 0.};
+#pragma unroll
+      for(int j = 0; j < NV; ++j)
+#pragma unroll
+        for(int d = 0; d < 3; ++d) x[d] += c[j][d] * T.Lg[k][j];
+      double cv[DIM], gi[DIM];
+#pragma unroll
+      for(int d = 0; d < DIM; ++d) cv[d] = feEvalFn(a.adv[d], x, e, nq, k);
+      if(DIM == 2) {
+        gi[0] = T.dLr[k][ir] * drdx[0] + T.dLs[k][ir] * drdx[1];
+        gi[1 % DIM] = T.dLr[k][ir] * drdy[0] + T.dLs[k][ir] * drdy[1];
+      } else {
+        gi[0] = T.dLr[k][ir] * drdx[0] + T.dLs[k][ir] * drdx[1] + T.dLt[k][ir] * drdx[2];
+        gi[1 % DIM] = T.dLr[k][ir] * drdy[0] + T.dLs[k][ir] * drdy[1] + T.dLt[k][ir] * drdy[2];
+        gi[2 % DIM] = T.dLr[k][ir] * drdz[0] + T.dLs[k][ir] * drdz[1] + T.dLt[k][ir] * drdz[2];
+      }
+      const double jw = J * T.w[k];
+      if(doMat) {
+#pragma unroll
+        for(int j = 0; j < NF; ++j)
+#pragma unroll
+          for(int d = 0; d < DIM; ++d) Ae[j] -= cv[d] * gi[d] * T.L[k][j] * jw;
+      }
+      if(doRes) {
+        double uq = 0.;
+#pragma unroll
+        for(int j = 0; j < NF; ++j) uq += T.L[k][j] * ul[j];
+#pragma unroll
+        for(int d = 0; d < DIM; ++d) Be += cv[d] * uq * gi[d] * jw;
       }
     }
   }
@@ -873,18 +951,20 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
   std::vector<bool> done(ctx->forms.size(), false);
   bool fp32Done = false;
   auto scalarForm = [](int t) {
-    return t == FE_SYS_DIFFUSION || t == FE_SYS_TRANSIENT_MASS || t == FE_SYS_MASS || t == FE_SYS_SOURCE;
+    return t == FE_SYS_DIFFUSION || t == FE_SYS_TRANSIENT_MASS || t == FE_SYS_MASS || t == FE_SYS_ADVECTION ||
+           t == FE_SYS_SOURCE;
   };
   for(size_t fi = 0; fi < ctx->forms.size(); ++fi) {
     if(done[fi] || !scalarForm(ctx->forms[fi].sysType)) continue;
     const int field = ctx->forms[fi].fields[0], degree = ctx->forms[fi].degree;
-    int iDiff = -1, iMass = -1, iReact = -1, iSrc = -1;
+    int iDiff = -1, iMass = -1, iReact = -1, iAdv = -1, iSrc = -1;
     for(size_t fj = fi; fj < ctx->forms.size(); ++fj) {
       const FeForm &G = ctx->forms[fj];
       if(done[fj] || !scalarForm(G.sysType) || G.fields[0] != field || G.degree != degree) continue;
       int &slot = G.sysType == FE_SYS_DIFFUSION         ? iDiff
                   : G.sysType == FE_SYS_TRANSIENT_MASS ? iMass
                   : G.sysType == FE_SYS_MASS           ? iReact
+                  : G.sysType == FE_SYS_ADVECTION      ? iAdv
                                                        : iSrc;
       if(slot < 0) {
         slot = (int)fj;
@@ -907,6 +987,7 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
     int g = iDiff >= 0 ? ctx->forms[iDiff].group
             : iMass >= 0 ? ctx->forms[iMass].group
             : iReact >= 0 ? ctx->forms[iReact].group
+            : iAdv >= 0   ? ctx->forms[iAdv].group
                           : -1;
     for(size_t k = 0; g < 0 && k < ctx->groups.size(); ++k)
       if(ctx->groups[k].layoutI.size() == 1 && ctx->groups[k].layoutI[0] == field) g = (int)k;
@@ -929,13 +1010,17 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
       a.react = ctx->forms[iReact].coeffs[0];
       a.reactPow = ctx->forms[iReact].coeffs.size() > 1 ? ctx->forms[iReact].coeffs[1].c0 : 1.;
     }
+    if(iAdv >= 0) {
+      a.hasAdv = 1;
+      for(size_t d = 0; d < ctx->forms[iAdv].coeffs.size() && d < 3; ++d) a.adv[d] = ctx->forms[iAdv].coeffs[d];
+    }
     if(iSrc >= 0) {
       a.hasSrc = 1;
       a.src = ctx->forms[iSrc].coeffs[0];
     }
-    if(!(what & 2) && iDiff < 0 && iMass < 0 && iReact < 0) continue;  // a lone source has no matrix
+    if(!(what & 2) && iDiff < 0 && iMass < 0 && iReact < 0 && iAdv < 0) continue;  // a lone source has no matrix
     const int kind = ctx->fields[field].kind;
-    if(iDiff >= 0 && iMass < 0 && iReact < 0 && ctx->asmKernel == 1) {
+    if(iDiff >= 0 && iMass < 0 && iReact < 0 && iAdv < 0 && ctx->asmKernel == 1) {
       const FeForm &F = ctx->forms[iDiff];
       MfmaAsmArgs m{};
       m.E = a.E; m.n = a.n; m.what = what; m.K = F.mfmaK;

@@ -315,6 +315,12 @@ int fe_hip_add_form(fe_hip_ctx *ctx, int sysType, int nfields, const int *fields
       FE_REQUIRE(ctx, ncoeff == 1 || (ncoeff == 2 && coeffs[1].kind == FE_FN_CONST),
                  "MASS takes a coefficient and optionally a constant exponent");
       break;
+    case FE_SYS_ADVECTION:
+      // velocity given by its dim components
+      FE_REQUIRE(ctx, nfields == 1 && k0 != FE_TRI_VP2, "ADVECTION takes one scalar field");
+      F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = true;
+      FE_REQUIRE(ctx, ncoeff == ctx->dim, "ADVECTION takes one coefficient per velocity component");
+      break;
     case FE_SYS_TRANSIENT_MASS:
       FE_REQUIRE(ctx, nfields == 1 && k0 != FE_TRI_VP2, "TRANSIENT_MASS takes one scalar field");
       F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = true;

@@ -51,6 +51,27 @@ def diffusion_system(mesh: A.Mesh, kind: int, degree: int, k: A.Fn, s: A.Fn, u_b
     return S, num
 
 
+def advection_diffusion_system(mesh: A.Mesh, kind: int, degree: int, velocity, k: A.Fn, s: A.Fn, reaction=None,
+                               u_bc=None, coloring=None, device=0):
+    """Steady advection-diffusion(-reaction) with essential boundary: the form
+    stack of the reference's scalarFE tests (scalarFE.cpp:241-246 and
+    :345-353): Advection (velocity components), Diffusion (k), Mass
+    (reaction coefficient, optional), Source."""
+    num = A.Numbering(mesh, [kind], [1])
+    S = A.LinearSystemHIP(device)
+    S.upload_mesh(mesh, coloring)
+    f = S.add_field(kind, num.adr[0])
+    S.set_sizes(num.nInc, num.nDofs)
+    S.add_form(A.ADVECTION, [f], degree, list(velocity))
+    S.add_form(A.DIFFUSION, [f], degree, [k])
+    if reaction is not None:
+        S.add_form(A.MASS, [f], degree, [reaction])
+    S.add_form(A.SOURCE, [f], degree, [s])
+    S.build_csr()
+    S.set_solution(A.essential_values(num, [u_bc]))
+    return S, num
+
+
 def heat_system(mesh: A.Mesh, kind: int, degree: int, m: A.Fn, k: A.Fn, s: A.Fn, essential=True, coloring=None,
                 device=0, periodic=None, react=None):
     """Transient scalar m u_t - div(k grad u) + S = 0: the form stack of the

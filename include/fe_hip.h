@@ -36,6 +36,7 @@ typedef struct fe_hip_ctx fe_hip_ctx;
 #define FE_SYS_TRANSIENT_MASS 13
 #define FE_SYS_TRANSIENT_VECTOR_MASS 15
 #define FE_SYS_DIFFUSION 16
+#define FE_SYS_ADVECTION 20 /* coefficients: the dim velocity components */
 #define FE_SYS_VECTOR_DIFFUSION 18
 #define FE_SYS_VECTOR_CONVECTIVE_ACCELERATION 22
 #define FE_SYS_DIV_NEWTONIAN_STRESS 25

@@ -1211,4 +1211,78 @@ double or_l2_error_sinsin(int kind, const double *xyz, int E, const int *conn, c
   return sqrt(acc);
 }
 
+/* L2 error against an exact solution tabulated at the quadrature points
+ * (ue[e nq + q]), feNorm.cpp:323-398 with p = 2, triangles. */
+double or_l2_error_qp(int kind, const double *xyz, int E, const int *conn, const int *adr, const double *u,
+                      int quad_degree, const double *ue)
+{
+  or_ref R;
+  if(ref_setup(kind, quad_degree, &R)) return -1.;
+  double acc = 0., c[9];
+  for(int e = 0; e < E; ++e) {
+    for(int j = 0; j < 3; ++j)
+      for(int d = 0; d < 3; ++d) c[3 * j + d] = xyz[3 * conn[e * 3 + j] + d];
+    or_geo g;
+    geo_tri(c, &g);
+    for(int k = 0; k < R.nq; ++k) {
+      double uh = 0.;
+      for(int i = 0; i < R.nf; ++i) uh += u[adr[(size_t)e * R.nf + i]] * R.L[k][i];
+      const double d = uh - ue[(size_t)e * R.nq + k];
+      acc += d * d * g.J * R.qw[k];
+    }
+  }
+  return sqrt(acc);
+}
+
+/* Advection in conservative form (feSysElm.cpp:766-835): Ae_ij -=
+ * sum_d c_d dphi_i/dx_d phi_j J w, Be_i += sum_d c_d u_h dphi_i/dx_d J w,
+ * the velocity given by its components cx, cy (cz for tets).  val may be
+ * NULL.  Adds into val/rhs. */
+int or_assemble_advection(int kind, int V, const double *xyz, int E, const int *conn, const int *adr, int n,
+                          const int *ia, const int *ja, int quad_degree, or_fn cx, or_fn cy, or_fn cz,
+                          const double *u, double *val, double *rhs)
+{
+  (void)V;
+  or_ref R;
+  if(ref_setup(kind, quad_degree, &R)) return -1;
+  const int nf = R.nf, nv = R.nv, dim = R.dim;
+  double c[12], Ae[144], Be[12], ul[12], gp[36];
+  for(int e = 0; e < E; ++e) {
+    for(int j = 0; j < nv; ++j)
+      for(int d = 0; d < 3; ++d) c[3 * j + d] = xyz[3 * conn[e * nv + j] + d];
+    const int *a = adr + (size_t)e * nf;
+    for(int i = 0; i < nf; ++i) ul[i] = u[a[i]];
+    memset(Ae, 0, sizeof(Ae));
+    memset(Be, 0, sizeof(Be));
+    or_geo g;
+    if(elem_geo(&R, c, &g)) return -2 - e;
+    for(int k = 0; k < R.nq; ++k) {
+      double x[3];
+      qp_pos(&R, c, k, x);
+      const double cv[3] = {fn_eval(cx, x, e, R.nq, k), fn_eval(cy, x, e, R.nq, k),
+                            dim == 3 ? fn_eval(cz, x, e, R.nq, k) : 0.};
+      phys_grad(&R, &g, k, gp);
+      double uq = 0.;
+      for(int j = 0; j < nf; ++j) uq += R.L[k][j] * ul[j];
+      for(int i = 0; i < nf; ++i) {
+        for(int j = 0; j < nf; ++j)
+          for(int d = 0; d < dim; ++d) Ae[i * nf + j] -= cv[d] * gp[i * dim + d] * R.L[k][j] * g.J * R.qw[k];
+        for(int d = 0; d < dim; ++d) Be[i] += cv[d] * uq * gp[i * dim + d] * g.J * R.qw[k];
+      }
+    }
+    for(int i = 0; i < nf; ++i) {
+      if(a[i] >= n) continue;
+      rhs[a[i]] += Be[i];
+      if(val)
+        for(int j = 0; j < nf; ++j) {
+          if(a[j] >= n) continue;
+          int q = find_pos(ia, ja, a[i], a[j]);
+          if(q < 0) return -1;
+          val[q] += Ae[i * nf + j];
+        }
+    }
+  }
+  return 0;
+}
+
 /* Taylor-Hood NS forms: implemented in fe_oracle_ns.c. */

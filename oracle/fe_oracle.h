@@ -101,6 +101,13 @@ int or_assemble_vector_transient_mass(int V, const double *xyz, int E, const int
 int or_assemble_mass_power(int kind, int V, const double *xyz, int E, const int *conn, const int *adr, int n,
                            const int *ia, const int *ja, int quad_degree, or_fn coeff, double p, const double *u,
                            double *val, double *rhs);
+/* Advection in conservative form (feSysElm.cpp:766-835), velocity (cx, cy, cz). */
+int or_assemble_advection(int kind, int V, const double *xyz, int E, const int *conn, const int *adr, int n,
+                          const int *ia, const int *ja, int quad_degree, or_fn cx, or_fn cy, or_fn cz,
+                          const double *u, double *val, double *rhs);
+/* L2 error (triangles) against exact values at the quadrature points ue[E nq]. */
+double or_l2_error_qp(int kind, const double *xyz, int E, const int *conn, const int *adr, const double *u,
+                      int quad_degree, const double *ue);
 /* Element mass matrix (coefficient m, c0 = 1) of one element (for unit checks). */
 int or_element_mass(int kind, const double *coords, int quad_degree, or_fn coeff, int elem, double *Ae);
 

@@ -41,6 +41,10 @@ class Oracle:
                                             OrFn, OrFn, dp, ip, dp, dp]
         L.or_element_diffusion.argtypes = [C.c_int, dp, C.c_int, OrFn, C.c_int, dp]
         L.or_element_mass.argtypes = [C.c_int, dp, C.c_int, OrFn, C.c_int, dp]
+        L.or_assemble_advection.argtypes = [C.c_int, C.c_int, dp, C.c_int, ip, ip, C.c_int, ip, ip, C.c_int,
+                                            OrFn, OrFn, OrFn, dp, dp, dp]
+        L.or_l2_error_qp.argtypes = [C.c_int, dp, C.c_int, ip, ip, dp, C.c_int, dp]
+        L.or_l2_error_qp.restype = C.c_double
         L.or_assemble_mass_power.argtypes = [C.c_int, C.c_int, dp, C.c_int, ip, ip, C.c_int, ip, ip, C.c_int,
                                              OrFn, C.c_double, dp, dp, dp]
         L.or_assemble_vector_transient_mass.argtypes = [C.c_int, dp, C.c_int, ip, ip, C.c_int, ip, ip, C.c_int,
@@ -190,6 +194,30 @@ class Oracle:
                                            deg, o, p, P(u, C.c_double), None if val is None else P(val, C.c_double),
                                            P(rhs, C.c_double))
         assert rc == 0, rc
+
+    def assemble_advection(self, kind, xyz, conn, adr, n, ia, ja, deg, vel, u, val, rhs):
+        nvpe = 4 if kind == 4 else 3
+        keep = []
+
+        def fn(f):
+            o = OrFn(f[0], f[1], f[2], None)
+            if f[3] is not None:
+                d = np.ascontiguousarray(f[3], np.float64)
+                keep.append(d)
+                o.data = P(d, C.c_double)
+            return o
+        v = [fn(f) for f in vel] + [OrFn(0, 0.0, 0.0, None)] * (3 - len(vel))
+        u = np.ascontiguousarray(u, np.float64)
+        rc = self.L.or_assemble_advection(kind, xyz.size // 3, P(xyz, C.c_double), conn.size // nvpe,
+                                          P(conn, C.c_int), P(adr, C.c_int), n, P(ia, C.c_int), P(ja, C.c_int),
+                                          deg, v[0], v[1], v[2], P(u, C.c_double),
+                                          None if val is None else P(val, C.c_double), P(rhs, C.c_double))
+        assert rc == 0, rc
+
+    def l2_error_qp(self, kind, xyz, conn, adr, u, deg, ue):
+        ue = np.ascontiguousarray(ue, np.float64)
+        return self.L.or_l2_error_qp(kind, P(xyz, C.c_double), conn.size // 3, P(conn, C.c_int), P(adr, C.c_int),
+                                     P(np.ascontiguousarray(u, np.float64), C.c_double), deg, P(ue, C.c_double))
 
     def element_mass(self, kind, coords, deg, coeff=(0, 1.0, 0.0, None), elem=0):
         nf = {1: 3, 2: 6, 4: 4}[kind]
