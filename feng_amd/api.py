@@ -34,7 +34,8 @@ PREC_FP64, PREC_FP32_IR = 0, 1
 # norm types (the reference's normType values, feNorm.h:10-85)
 (NORM_L1, NORM_L1_ERROR, NORM_L2, NORM_VECTOR_L2, NORM_L2_ERROR, NORM_VECTOR_L2_ERROR, NORM_LINF, NORM_LINF_ERROR,
  NORM_LINF_UDOT_ERROR, NORM_VECTOR_LINF_ERROR, NORM_VECTOR_LINF_UDOT_ERROR, NORM_SEMI_H1, NORM_SEMI_H1_ERROR,
- NORM_H1, NORM_H1_ERROR, NORM_AREA) = (1, 2, 3, 4, 5, 7, 8, 9, 10, 11, 12, 13, 14, 17, 18, 19)
+ NORM_VECTOR_SEMI_H1_ERROR, NORM_H1, NORM_H1_ERROR, NORM_AREA) = (1, 2, 3, 4, 5, 7, 8, 9, 10, 11, 12, 13, 14, 16, 17,
+                                                                  18, 19)
 
 
 class FeError(RuntimeError):

@@ -474,6 +474,8 @@ __global__ __launch_bounds__(256) void asmScalarLanes(ScalarAsmArgs a)
 //   VectorSource                  feVectorSysElm.cpp:101-132
 //   DivergenceNewtonianStress     feVectorSysElm.cpp:1434-1536
 //   TransientVectorMass           feVectorSysElm.cpp:391-425
+//   VectorDiffusion               feVectorSysElm.cpp:450-503
+//   MixedGradient {u, p}          feVectorSysElm.cpp:529-576
 // 16 lanes per element: lane i < 12 owns velocity row i = 2a + c (function
 // psi_a e_c), lanes 12..14 own the pressure rows, lane 15 only loads.  The
 // element's local solution sits in LDS; every lane rebuilds the P2 gradients
@@ -484,10 +486,10 @@ struct NsAsmArgs {
   int begin, count, E, n, what;
   const int *perm, *conn;
   const int *adrU, *adrP;  // color-sorted element maps (12, 3 per element); adrP null: no pressure
-  const int *posUU, *posDiv, *posStr;  // posUU: the velocity-velocity group (convection and mass)
+  const int *posUU, *posDiv, *posStr, *posGrad;  // posUU: the velocity-velocity group (convection, mass, diffusion)
   const double *xyz, *tab, *u;
-  int hasConv, hasDiv, hasStr, hasSrc, hasMass;
-  FeFn cConv, cDiv, cStr, mu, src0, src1, cMass;
+  int hasConv, hasDiv, hasStr, hasSrc, hasMass, hasVDiff, hasGrad;
+  FeFn cConv, cDiv, cStr, mu, src0, src1, cMass, cVDiff, dVDiff, cGrad;
   // time derivative u_dot = cn[0] u + sum_{j = 1}^{nsol - 1} cn[j] past[j - 1]
   int nsol;
   double cn[fe_hip_ctx::kMaxPast + 1];
@@ -546,9 +548,11 @@ __global__ __launch_bounds__(256) void asmNavierStokesFP64(NsAsmArgs a)
   const bool uRow = lane < 12;
   const int ai = lane >> 1, ci = lane & 1, ip = lane - 12;
   const bool doMat = (a.what & 1), doRes = (a.what & 2);
-  double Ac[12], As[15], Ad[12], Be = 0.;
+  double Ac[12], As[15], Ad[12], Ag[3], Be = 0.;
 #pragma unroll
   for(int j = 0; j < 12; ++j) { Ac[j] = 0.; Ad[j] = 0.; }
+#pragma unroll
+  for(int j = 0; j < 3; ++j) Ag[j] = 0.;
 #pragma unroll
   for(int j = 0; j < 15; ++j) As[j] = 0.;
   const int nq = T.nq;
@@ -632,6 +636,27 @@ __global__ __launch_bounds__(256) void asmNavierStokesFP64(NsAsmArgs a)
         const double S = ci == 0 ? feEvalFn(a.src0, x, e, nq, k) : feEvalFn(a.src1, x, e, nq, k);
         Be -= S * psi * jw;
       }
+      if(a.hasVDiff) {
+        // grad(phi_i) : grad(phi_j) = delta(c_i, c_j) grad(psi_a) . grad(psi_b)
+        const double cd = feEvalFn(a.cVDiff, x, e, nq, k) * feEvalFn(a.dVDiff, x, e, nq, k);
+        if(doMat) {
+#pragma unroll
+          for(int b = 0; b < 6; ++b) Ac[2 * b + ci] += cd * (gi[0] * gx[b] + gi[1] * gy[b]) * jw;
+        }
+        if(doRes) {
+          const double gdu = gi[0] * (ci == 0 ? g0 : g1) + gi[1] * (ci == 0 ? g2 : g3);
+          Be -= cd * gdu * jw;
+        }
+      }
+      if(a.hasGrad) {
+        // div(phi_i) = d psi_a / d x_ci
+        const double cg = feEvalFn(a.cGrad, x, e, nq, k);
+        if(doMat) {
+#pragma unroll
+          for(int j = 0; j < 3; ++j) Ag[j] -= cg * pp[j] * gi[ci] * jw;
+        }
+        if(doRes) Be += cg * pq * gi[ci] * jw;
+      }
       if(a.hasMass) {
         // phi_i . phi_j = psi_a psi_b delta(c_i, c_j)
         const double cm = feEvalFn(a.cMass, x, e, nq, k);
@@ -662,11 +687,17 @@ __global__ __launch_bounds__(256) void asmNavierStokesFP64(NsAsmArgs a)
   if(doRes && (uRow || hasP) && myDof < a.n) a.rhs[myDof] += Be;
   if(!doMat) return;
   if(uRow) {
-    if(a.hasConv || a.hasMass)
+    if(a.hasConv || a.hasMass || a.hasVDiff)
 #pragma unroll
       for(int j = 0; j < 12; ++j) {
         const int p = a.posUU[(size_t)(lane * 12 + j) * a.E + ee];
         if(p >= 0) a.val[p] += Ac[j];
+      }
+    if(a.hasGrad)
+#pragma unroll
+      for(int j = 0; j < 3; ++j) {
+        const int p = a.posGrad[(size_t)(lane * 3 + j) * a.E + ee];
+        if(p >= 0) a.val[p] += Ag[j];
       }
     if(a.hasStr)
 #pragma unroll
@@ -1054,7 +1085,8 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
   {
     NsAsmArgs a{};
     int any = 0, degree = -1;
-    const int *posConv = nullptr, *posMass = nullptr;
+    const int *posConv = nullptr, *posMass = nullptr, *posVDiff = nullptr;
+    const int *adrPgrad = nullptr;
     for(size_t fi = 0; fi < ctx->forms.size(); ++fi) {
       FeForm &F = ctx->forms[fi];
       if(done[fi]) continue;
@@ -1068,6 +1100,16 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
           FE_REQUIRE(ctx, !a.hasMass, "two vector transient mass forms");
           a.hasMass = 1; a.cMass = F.coeffs[0]; posMass = ctx->groups[F.group].d_pos;
           a.adrU = ctx->groups[F.group].d_adrI;
+          break;
+        case FE_SYS_VECTOR_DIFFUSION:
+          FE_REQUIRE(ctx, !a.hasVDiff, "two vector diffusion forms");
+          a.hasVDiff = 1; a.cVDiff = F.coeffs[0]; a.dVDiff = F.coeffs[1]; posVDiff = ctx->groups[F.group].d_pos;
+          a.adrU = ctx->groups[F.group].d_adrI;
+          break;
+        case FE_SYS_MIXED_GRADIENT:
+          FE_REQUIRE(ctx, !a.hasGrad, "two gradient forms");
+          a.hasGrad = 1; a.cGrad = F.coeffs[0]; a.posGrad = ctx->groups[F.group].d_pos;
+          a.adrU = ctx->groups[F.group].d_adrI; adrPgrad = ctx->groups[F.group].d_adrJ;
           break;
         case FE_SYS_MIXED_DIVERGENCE:
           FE_REQUIRE(ctx, !a.hasDiv, "two divergence forms");
@@ -1095,8 +1137,14 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
       FE_REQUIRE(ctx, a.adrU, "the vector forms need a velocity matrix form (stress, convection or transient mass)");
       FE_REQUIRE(ctx, !a.hasStr || a.hasDiv, "the stress form needs the divergence form (pressure map)");
       // convection and transient mass share the velocity-velocity layout, hence one group
-      FE_REQUIRE(ctx, !(posConv && posMass) || posConv == posMass, "velocity forms on different groups");
-      a.posUU = posConv ? posConv : posMass;
+      const int *uu[3] = {posConv, posMass, posVDiff};
+      for(const int *p : uu)
+        if(p) {
+          FE_REQUIRE(ctx, !a.posUU || a.posUU == p, "velocity forms on different groups");
+          a.posUU = p;
+        }
+      // the pressure map: the divergence form's rows, else the gradient form's columns
+      if(!a.adrP) a.adrP = adrPgrad;
       if(a.hasMass) {
         a.nsol = ctx->nsol;
         for(int j = 0; j <= fe_hip_ctx::kMaxPast; ++j) a.cn[j] = ctx->bdf[j];

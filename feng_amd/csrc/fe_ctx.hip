@@ -346,6 +346,18 @@ int fe_hip_add_form(fe_hip_ctx *ctx, int sysType, int nfields, const int *fields
       F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = true;
       FE_REQUIRE(ctx, ncoeff == 1, "TRANSIENT_VECTOR_MASS takes one coefficient");
       break;
+    case FE_SYS_VECTOR_DIFFUSION:
+      FE_REQUIRE(ctx, nfields == 1 && k0 == FE_TRI_VP2, "VECTOR_DIFFUSION takes one vector field");
+      F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = true;
+      FE_REQUIRE(ctx, ncoeff == 2, "VECTOR_DIFFUSION takes (coefficient, diffusivity)");
+      break;
+    case FE_SYS_MIXED_GRADIENT:
+      // fields {u, p}: vector test functions, scalar unknown (feSysElm_MixedGradient)
+      FE_REQUIRE(ctx, nfields == 2 && k0 == FE_TRI_VP2 && ctx->fields[fields[1]].kind == FE_TRI_P1,
+                 "MIXED_GRADIENT takes {vector P2 u, P1 p}");
+      F.layoutI = {fields[0]}; F.layoutJ = {fields[1]}; F.hasMatrix = true;
+      FE_REQUIRE(ctx, ncoeff == 1, "MIXED_GRADIENT takes one coefficient");
+      break;
     case FE_SYS_MIXED_DIVERGENCE:
       // fields {p, u}: scalar test functions, vector unknown (feVectorSysElm.cpp:669-683)
       FE_REQUIRE(ctx, nfields == 2 && ctx->fields[fields[0]].kind == FE_TRI_P1 &&

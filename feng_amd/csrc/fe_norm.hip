@@ -2,8 +2,8 @@
 // feNorm.cpp:124-246, with computeLpNorm :323-398, computeVectorLpNorm
 // :1399-1444, computeVectorLInfNorm :1445-1484, the time-derivative
 // L-infinity norms :1485-1545 / :1618-1642, computeLInfNorm :1593-1617,
-// computeH1SemiNorm :1643-1733, computeH1Norm :1848-1853, computeArea
-// :1862-1873).
+// computeH1SemiNorm :1643-1733, computeVectorH1SemiNorm :1794-1847,
+// computeH1Norm :1848-1853, computeArea :1862-1873).
 //
 // One thread per element (mesh order, no coloring: nothing is scattered):
 // the element's contribution over its quadrature points, a block reduction
@@ -53,11 +53,13 @@ __global__ __launch_bounds__(256) void normElements(NormArgs a)
   const bool udot = t == FE_NORM_LINF_UDOT_ERROR || t == FE_NORM_VECTOR_LINF_UDOT_ERROR;
   const bool err = t == FE_NORM_L1_ERROR || t == FE_NORM_L2_ERROR || t == FE_NORM_LINF_ERROR || udot ||
                    t == FE_NORM_VECTOR_L2_ERROR || t == FE_NORM_VECTOR_LINF_ERROR || t == FE_NORM_H1_ERROR;
-  const bool grad = t == FE_NORM_SEMI_H1 || t == FE_NORM_SEMI_H1_ERROR || t == FE_NORM_H1 || t == FE_NORM_H1_ERROR;
-  const bool val = t != FE_NORM_SEMI_H1 && t != FE_NORM_SEMI_H1_ERROR && t != FE_NORM_AREA;
+  const bool grad = t == FE_NORM_SEMI_H1 || t == FE_NORM_SEMI_H1_ERROR || t == FE_NORM_H1 || t == FE_NORM_H1_ERROR ||
+                    t == FE_NORM_VECTOR_SEMI_H1_ERROR;
+  const bool val = t != FE_NORM_SEMI_H1 && t != FE_NORM_SEMI_H1_ERROR && t != FE_NORM_AREA &&
+                   t != FE_NORM_VECTOR_SEMI_H1_ERROR;
   // exact callbacks: value(s) first, then the gradient components
   const int g0 = (t == FE_NORM_H1_ERROR) ? 1 : 0;
-  const bool gradErr = t == FE_NORM_SEMI_H1_ERROR || t == FE_NORM_H1_ERROR;
+  const bool gradErr = t == FE_NORM_SEMI_H1_ERROR || t == FE_NORM_H1_ERROR || t == FE_NORM_VECTOR_SEMI_H1_ERROR;
   double acc = 0.;
   for(int e = blockIdx.x * blockDim.x + threadIdx.x; e < a.E; e += gridDim.x * blockDim.x) {
     double c[NV][3];
@@ -100,7 +102,23 @@ __global__ __launch_bounds__(256) void normElements(NormArgs a)
           else q += d * d;
         }
       }
-      if(grad) {
+      if(grad && NC == 2) {
+        // grad(u)[m 2 + n] = d u_n / d x_m, exact components in that order
+#pragma unroll
+        for(int m = 0; m < 2; ++m)
+#pragma unroll
+          for(int cc = 0; cc < 2; ++cc) {
+            double gh = 0.;
+#pragma unroll
+            for(int b = 0; b < NS; ++b) {
+              const double gphys = m == 0 ? T.dLr[k][b] * drdx[0] + T.dLs[k][b] * drdx[1]
+                                          : T.dLr[k][b] * drdy[0] + T.dLs[k][b] * drdy[1];
+              gh += ul[NC * b + cc] * gphys;
+            }
+            const double gd = (gradErr ? feEvalFn(a.ex[m * 2 + cc], x, e, T.nq, k) : 0.) - gh;
+            q += gd * gd;
+          }
+      } else if(grad) {
         double gh[3] = {0., 0., 0.};
 #pragma unroll
         for(int b = 0; b < NS; ++b) {
@@ -163,6 +181,7 @@ int normExactCount(int type, int nc, int dim)
     case FE_NORM_VECTOR_L2_ERROR: case FE_NORM_VECTOR_LINF_ERROR: case FE_NORM_VECTOR_LINF_UDOT_ERROR:
       return nc == 2 ? 2 : -1;
     case FE_NORM_SEMI_H1_ERROR: return nc == 1 ? dim : -1;
+    case FE_NORM_VECTOR_SEMI_H1_ERROR: return nc == 2 ? 4 : -1;
     case FE_NORM_H1_ERROR: return nc == 1 ? 1 + dim : -1;
   }
   return -1;
@@ -261,7 +280,7 @@ extern "C" int fe_hip_norm(fe_hip_ctx *ctx, int type, int field, int quad_degree
   if(!isMax && r < 0. && fabs(r) < 1e-14) r = fabs(r);
   const bool root = type == FE_NORM_L2 || type == FE_NORM_L2_ERROR || type == FE_NORM_VECTOR_L2 ||
                     type == FE_NORM_VECTOR_L2_ERROR || type == FE_NORM_SEMI_H1 || type == FE_NORM_SEMI_H1_ERROR ||
-                    type == FE_NORM_H1 || type == FE_NORM_H1_ERROR;
+                    type == FE_NORM_H1 || type == FE_NORM_H1_ERROR || type == FE_NORM_VECTOR_SEMI_H1_ERROR;
   *value = root ? sqrt(r) : r;
   return FE_STATUS_OK;
 }

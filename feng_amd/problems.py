@@ -151,14 +151,16 @@ def solve_newton(S: A.LinearSystemHIP, opts: NLSolverOptions = NLSolverOptions()
 
 
 def taylor_hood_system(mesh: A.Mesh, degree: int, mu: float = 1.0, convection=None, source=None, u_bc=None,
-                       p_bc=None, coloring=None, device=0):
+                       p_bc=None, coloring=None, device=0, laplacian=False):
     """Taylor-Hood P2-P1 Stokes or Navier-Stokes with Dirichlet velocity and
     pressure on the boundary: the form stacks of the reference's MMS tests
     (stokes.cpp:76-85, divergence formulation: MixedDivergence {p, u} (1),
     VectorSource, DivergenceNewtonianStress {u, p} (1, mu); navier_stokes.cpp
     adds VectorConvectiveAcceleration first with coefficient `convection`).
     source: the two Fn components of the vector source (None: zero);
-    u_bc(x, y, z, comp), p_bc(x, y, z, comp): the essential values."""
+    u_bc(x, y, z, comp), p_bc(x, y, z, comp): the essential values.
+    laplacian=True: the Laplacian formulation (stokes.cpp:80-84) -- VectorDiffusion
+    (-1, mu) and MixedGradient {u, p} (-1) instead of the stress form."""
     num = A.Numbering(mesh, [A.TRI_VP2, A.TRI_P1], [1, 1])
     S = A.LinearSystemHIP(device)
     S.upload_mesh(mesh, coloring)
@@ -169,9 +171,28 @@ def taylor_hood_system(mesh: A.Mesh, degree: int, mu: float = 1.0, convection=No
         S.add_form(A.VECTOR_CONVECTIVE_ACCELERATION, [fu], degree, [A.Fn(A.FN_CONST, float(convection))])
     S.add_form(A.MIXED_DIVERGENCE, [fp, fu], degree, [A.Fn(A.FN_CONST, 1.0)])
     S.add_form(A.VECTOR_SOURCE, [fu], degree, source or [A.Fn(A.FN_CONST, 0.0), A.Fn(A.FN_CONST, 0.0)])
-    S.add_form(A.DIV_NEWTONIAN_STRESS, [fu, fp], degree, [A.Fn(A.FN_CONST, 1.0), A.Fn(A.FN_CONST, float(mu))])
+    if laplacian:
+        S.add_form(A.VECTOR_DIFFUSION, [fu], degree, [A.Fn(A.FN_CONST, -1.0), A.Fn(A.FN_CONST, float(mu))])
+        S.add_form(A.MIXED_GRADIENT, [fu, fp], degree, [A.Fn(A.FN_CONST, -1.0)])
+    else:
+        S.add_form(A.DIV_NEWTONIAN_STRESS, [fu, fp], degree, [A.Fn(A.FN_CONST, 1.0), A.Fn(A.FN_CONST, float(mu))])
     S.build_csr()
     S.set_solution(A.essential_values(num, [u_bc, p_bc]))
+    return S, num
+
+
+def vector_laplacian_system(mesh: A.Mesh, degree: int, source, u_bc=None, coloring=None, device=0):
+    """-div(grad u) = f for the vector P2 field with strong Dirichlet values
+    (vectorFE.cpp:246-259): VectorDiffusion (1, 1) and VectorSource."""
+    num = A.Numbering(mesh, [A.TRI_VP2], [1])
+    S = A.LinearSystemHIP(device)
+    S.upload_mesh(mesh, coloring)
+    fu = S.add_field(A.TRI_VP2, num.adr[0])
+    S.set_sizes(num.nInc, num.nDofs)
+    S.add_form(A.VECTOR_DIFFUSION, [fu], degree, [A.Fn(A.FN_CONST, 1.0), A.Fn(A.FN_CONST, 1.0)])
+    S.add_form(A.VECTOR_SOURCE, [fu], degree, list(source))
+    S.build_csr()
+    S.set_solution(A.essential_values(num, [u_bc]))
     return S, num
 
 

@@ -275,7 +275,9 @@ int fe_hip_set_essential(fe_hip_ctx *ctx, const double *values);
  *   derivative of the solution from the BDF history, fe_hip_set_bdf);
  *   VECTOR_L2_ERROR, VECTOR_LINF_ERROR, VECTOR_LINF_UDOT_ERROR: the two
  *   components of the vector P2 field;
- *   SEMI_H1_ERROR: the dim components of grad u; H1_ERROR: u, then grad u.
+ *   SEMI_H1_ERROR: the dim components of grad u; H1_ERROR: u, then grad u;
+ *   VECTOR_SEMI_H1_ERROR (2D vector P2): the four components of grad u in
+ *   the reference's order d u_n / d x_m at m 2 + n.
  * The value is bitwise repeatable (fixed reduction order). */
 #define FE_NORM_L1 1
 #define FE_NORM_L1_ERROR 2
@@ -290,6 +292,7 @@ int fe_hip_set_essential(fe_hip_ctx *ctx, const double *values);
 #define FE_NORM_VECTOR_LINF_UDOT_ERROR 12
 #define FE_NORM_SEMI_H1 13
 #define FE_NORM_SEMI_H1_ERROR 14
+#define FE_NORM_VECTOR_SEMI_H1_ERROR 16
 #define FE_NORM_H1 17
 #define FE_NORM_H1_ERROR 18
 #define FE_NORM_AREA 19

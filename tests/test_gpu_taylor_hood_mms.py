@@ -1,4 +1,4 @@
-"""The reference's Taylor-Hood convergence tables through the GPU path.
+"""The reference's Taylor-Hood and vector-Laplacian convergence tables through the GPU path.
 
 stokes_MMS.output and navier_stokes_MMS.output (tests/withLinearSolver/ of
 the reference), P2-P1 divergence formulation rows: u = (x^4 y^4,
@@ -6,7 +6,9 @@ the reference), P2-P1 divergence formulation rows: u = (x^4 y^4,
 to tests/golden/meshes/data/; the scalar tables use data/mmsMeshes/), degree-8
 quadrature, u and p essential on "Bord", the manufactured source tabulated at
 the quadrature points; Newton with GMRES/ILU(0) on the device; the L2 errors
-of u (vector) and p by the device norms.  Printed by the reference with 7
+of u (vector) and p by the device norms.  Both formulations: divergence
+(the stress form) and Laplacian (vector diffusion + mixed gradient).  And
+vectorFE_vectorLaplacian.output (P2 rows).  Printed by the reference with 7
 significant digits.
 """
 import numpy as np
@@ -19,7 +21,14 @@ TABLES = {  # (velocity, pressure) L2 errors, square1..4
                ["1.054679e-02", "2.304575e-03", "5.426010e-04", "1.313177e-04"]),
     "navier_stokes": (["1.730601e-03", "2.102148e-04", "2.593958e-05", "3.221725e-06"],
                       ["1.064397e-02", "2.316230e-03", "5.440602e-04", "1.315003e-04"]),
+    "stokes_laplacian": (["1.729104e-03", "2.114311e-04", "2.643873e-05", "3.316210e-06"],
+                         ["1.055463e-02", "2.266154e-03", "5.362661e-04", "1.302836e-04"]),
+    "navier_stokes_laplacian": (["1.729202e-03", "2.115442e-04", "2.645172e-05", "3.317293e-06"],
+                                ["1.064192e-02", "2.276928e-03", "5.376581e-04", "1.304588e-04"]),
 }
+# vectorFE_vectorLaplacian.output, Lagrange P2 rows: L2 and H1 (semi-norm) errors
+VECTOR_LAPLACIAN = (["1.183067e-03", "1.454727e-04", "1.812249e-05", "2.266050e-06"],
+                    ["3.800807e-02", "9.511922e-03", "2.380397e-03", "5.955249e-04"])
 
 
 def _u(x, y):
@@ -41,13 +50,13 @@ def _close(got, ref):
     return abs(got - float(ref)) <= 1e-6 * float(ref)
 
 
-@pytest.mark.parametrize("case", ["stokes", "navier_stokes"])
+@pytest.mark.parametrize("case", ["stokes", "navier_stokes", "stokes_laplacian", "navier_stokes_laplacian"])
 def test_reference_mms_table(case):
     from feng_amd.api import FN_QP, KSP_GMRES, NORM_L2_ERROR, NORM_VECTOR_L2_ERROR, PC_ILU0, Fn, Mesh
     from feng_amd.problems import NLSolverOptions, solve_newton, taylor_hood_system
     from tests.conftest import GOLDEN
     from tests.test_oracle_time import _qp_table
-    conv = case == "navier_stokes"
+    conv = case.startswith("navier_stokes")
     eu, ep = [], []
     for lvl in range(1, 5):
         print(case, "square", lvl, flush=True)
@@ -55,7 +64,8 @@ def test_reference_mms_table(case):
         src = [Fn(FN_QP, data=_qp_table(m, 8, lambda x, y, c=c: _src(x, y, conv)[c])) for c in range(2)]
         ubc = lambda x, y, z, comp: np.where(comp == 0, x ** 4 * y ** 4, -4.0 / 5.0 * x ** 3 * y ** 5)
         pbc = lambda x, y, z, comp: x * x * y * y
-        S, num = taylor_hood_system(m, 8, 1.0, -1.0 if conv else None, src, ubc, pbc)
+        S, num = taylor_hood_system(m, 8, 1.0, -1.0 if conv else None, src, ubc, pbc,
+                                    laplacian=case.endswith("laplacian"))
         rep = solve_newton(S, NLSolverOptions(1e-12, 1e-12, 1e4, 20), ksp=KSP_GMRES, pc=PC_ILU0, rtol=1e-13,
                            restart=100, max_iter=20000)
         assert rep.converged
@@ -67,3 +77,33 @@ def test_reference_mms_table(case):
     ru, rp = TABLES[case]
     assert all(_close(g, r) for g, r in zip(eu, ru)), (eu, ru)
     assert all(_close(g, r) for g, r in zip(ep, rp)), (ep, rp)
+
+
+def test_reference_vector_laplacian_table():
+    """-div(grad u) = f, u = (a x^6 + x^5 y^2, b sin(xy)), a = 0.1234,
+    b = 1.2345, strong Dirichlet values, vector P2 (vectorFE.cpp:132-300):
+    L2 and vector semi-H1 errors."""
+    from feng_amd.api import FN_QP, KSP_GMRES, NORM_VECTOR_L2_ERROR, NORM_VECTOR_SEMI_H1_ERROR, PC_ILU0, Fn, Mesh
+    from feng_amd.problems import NLSolverOptions, solve_newton, vector_laplacian_system
+    from tests.conftest import GOLDEN
+    from tests.test_oracle_time import _qp_table
+    a, b = 0.1234, 1.2345
+    usol = (lambda x, y: a * x ** 6 + x ** 5 * y * y, lambda x, y: b * np.sin(x * y))
+    grad = (lambda x, y: 6 * a * x ** 5 + 5 * x ** 4 * y * y, lambda x, y: y * b * np.cos(x * y),
+            lambda x, y: 2 * x ** 5 * y, lambda x, y: x * b * np.cos(x * y))
+    src = (lambda x, y: 30 * a * x ** 4 + 20 * x ** 3 * y * y + 2 * x ** 5,
+           lambda x, y: -b * np.sin(x * y) * (x * x + y * y))
+    el2, eh1 = [], []
+    for lvl in range(1, 5):
+        print("vector laplacian square", lvl, flush=True)
+        m = Mesh.read_msh(GOLDEN / "meshes" / "data" / f"square{lvl}.msh")
+        S, num = vector_laplacian_system(m, 8, [Fn(FN_QP, data=_qp_table(m, 8, f)) for f in src],
+                                         lambda x, y, z, c: np.where(c == 0, usol[0](x, y), usol[1](x, y)))
+        rep = solve_newton(S, NLSolverOptions(1e-12, 1e-12, 1e4, 10), ksp=KSP_GMRES, pc=PC_ILU0, rtol=1e-13,
+                           restart=100, max_iter=20000)
+        assert rep.converged
+        el2.append(S.norm(NORM_VECTOR_L2_ERROR, 0, 8, [Fn(FN_QP, data=_qp_table(m, 8, f)) for f in usol]))
+        eh1.append(S.norm(NORM_VECTOR_SEMI_H1_ERROR, 0, 8, [Fn(FN_QP, data=_qp_table(m, 8, f)) for f in grad]))
+        S.close()
+    assert all(_close(g, r) for g, r in zip(el2, VECTOR_LAPLACIAN[0])), (el2, VECTOR_LAPLACIAN[0])
+    assert all(_close(g, r) for g, r in zip(eh1, VECTOR_LAPLACIAN[1])), (eh1, VECTOR_LAPLACIAN[1])
