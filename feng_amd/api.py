@@ -20,7 +20,7 @@ NLOC = {TRI_P1: 3, TRI_P2: 6, TRI_VP2: 12, TET_P1: 4}
 
 # weak forms (elementSystemType values, feSysElm.h:13-77)
 SOURCE, VECTOR_SOURCE, MASS, TRANSIENT_MASS, TRANSIENT_VECTOR_MASS, DIFFUSION, VECTOR_DIFFUSION = 0, 2, 6, 13, 15, 16, 18
-ADVECTION = 20
+ADVECTION, GRAD_SOURCE, VECTOR_MASS = 20, 4, 10
 VECTOR_CONVECTIVE_ACCELERATION, DIV_NEWTONIAN_STRESS = 22, 25
 MIXED_GRADIENT, MIXED_DIVERGENCE = 26, 31
 

@@ -476,6 +476,7 @@ __global__ __launch_bounds__(256) void asmScalarLanes(ScalarAsmArgs a)
 //   TransientVectorMass           feVectorSysElm.cpp:391-425
 //   VectorDiffusion               feVectorSysElm.cpp:450-503
 //   MixedGradient {u, p}          feVectorSysElm.cpp:529-576
+//   VectorMass, GradSource        feVectorSysElm.cpp:196-237, :151-172
 // 16 lanes per element: lane i < 12 owns velocity row i = 2a + c (function
 // psi_a e_c), lanes 12..14 own the pressure rows, lane 15 only loads.  The
 // element's local solution sits in LDS; every lane rebuilds the P2 gradients
@@ -488,8 +489,8 @@ struct NsAsmArgs {
   const int *adrU, *adrP;  // color-sorted element maps (12, 3 per element); adrP null: no pressure
   const int *posUU, *posDiv, *posStr, *posGrad;  // posUU: the velocity-velocity group (convection, mass, diffusion)
   const double *xyz, *tab, *u;
-  int hasConv, hasDiv, hasStr, hasSrc, hasMass, hasVDiff, hasGrad;
-  FeFn cConv, cDiv, cStr, mu, src0, src1, cMass, cVDiff, dVDiff, cGrad;
+  int hasConv, hasDiv, hasStr, hasSrc, hasMass, hasVDiff, hasGrad, hasVMass, hasGSrc;
+  FeFn cConv, cDiv, cStr, mu, src0, src1, cMass, cVDiff, dVDiff, cGrad, cVMass, gSrc;
   // time derivative u_dot = cn[0] u + sum_{j = 1}^{nsol - 1} cn[j] past[j - 1]
   int nsol;
   double cn[fe_hip_ctx::kMaxPast + 1];
@@ -636,6 +637,15 @@ __global__ __launch_bounds__(256) void asmNavierStokesFP64(NsAsmArgs a)
         const double S = ci == 0 ? feEvalFn(a.src0, x, e, nq, k) : feEvalFn(a.src1, x, e, nq, k);
         Be -= S * psi * jw;
       }
+      if(a.hasVMass) {
+        const double cm = feEvalFn(a.cVMass, x, e, nq, k);
+        if(doMat) {
+#pragma unroll
+          for(int b = 0; b < 6; ++b) Ac[2 * b + ci] += cm * psi * ps[b] * jw;
+        }
+        if(doRes) Be -= cm * (ci == 0 ? ux : uy) * psi * jw;
+      }
+      if(a.hasGSrc && doRes) Be += feEvalFn(a.gSrc, x, e, nq, k) * gi[ci] * jw;  // S div(phi_i)
       if(a.hasVDiff) {
         // grad(phi_i) : grad(phi_j) = delta(c_i, c_j) grad(psi_a) . grad(psi_b)
         const double cd = feEvalFn(a.cVDiff, x, e, nq, k) * feEvalFn(a.dVDiff, x, e, nq, k);
@@ -687,7 +697,7 @@ __global__ __launch_bounds__(256) void asmNavierStokesFP64(NsAsmArgs a)
   if(doRes && (uRow || hasP) && myDof < a.n) a.rhs[myDof] += Be;
   if(!doMat) return;
   if(uRow) {
-    if(a.hasConv || a.hasMass || a.hasVDiff)
+    if(a.hasConv || a.hasMass || a.hasVDiff || a.hasVMass)
 #pragma unroll
       for(int j = 0; j < 12; ++j) {
         const int p = a.posUU[(size_t)(lane * 12 + j) * a.E + ee];
@@ -1085,7 +1095,7 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
   {
     NsAsmArgs a{};
     int any = 0, degree = -1;
-    const int *posConv = nullptr, *posMass = nullptr, *posVDiff = nullptr;
+    const int *posConv = nullptr, *posMass = nullptr, *posVDiff = nullptr, *posVMass = nullptr;
     const int *adrPgrad = nullptr;
     for(size_t fi = 0; fi < ctx->forms.size(); ++fi) {
       FeForm &F = ctx->forms[fi];
@@ -1100,6 +1110,15 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
           FE_REQUIRE(ctx, !a.hasMass, "two vector transient mass forms");
           a.hasMass = 1; a.cMass = F.coeffs[0]; posMass = ctx->groups[F.group].d_pos;
           a.adrU = ctx->groups[F.group].d_adrI;
+          break;
+        case FE_SYS_VECTOR_MASS:
+          FE_REQUIRE(ctx, !a.hasVMass, "two vector mass forms");
+          a.hasVMass = 1; a.cVMass = F.coeffs[0]; posVMass = ctx->groups[F.group].d_pos;
+          a.adrU = ctx->groups[F.group].d_adrI;
+          break;
+        case FE_SYS_GRAD_SOURCE:
+          FE_REQUIRE(ctx, !a.hasGSrc, "two gradient sources");
+          a.hasGSrc = 1; a.gSrc = F.coeffs[0];
           break;
         case FE_SYS_VECTOR_DIFFUSION:
           FE_REQUIRE(ctx, !a.hasVDiff, "two vector diffusion forms");
@@ -1137,7 +1156,7 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
       FE_REQUIRE(ctx, a.adrU, "the vector forms need a velocity matrix form (stress, convection or transient mass)");
       FE_REQUIRE(ctx, !a.hasStr || a.hasDiv, "the stress form needs the divergence form (pressure map)");
       // convection and transient mass share the velocity-velocity layout, hence one group
-      const int *uu[3] = {posConv, posMass, posVDiff};
+      const int *uu[4] = {posConv, posMass, posVDiff, posVMass};
       for(const int *p : uu)
         if(p) {
           FE_REQUIRE(ctx, !a.posUU || a.posUU == p, "velocity forms on different groups");

@@ -346,6 +346,16 @@ int fe_hip_add_form(fe_hip_ctx *ctx, int sysType, int nfields, const int *fields
       F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = true;
       FE_REQUIRE(ctx, ncoeff == 1, "TRANSIENT_VECTOR_MASS takes one coefficient");
       break;
+    case FE_SYS_VECTOR_MASS:
+      FE_REQUIRE(ctx, nfields == 1 && k0 == FE_TRI_VP2, "VECTOR_MASS takes one vector field");
+      F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = true;
+      FE_REQUIRE(ctx, ncoeff == 1, "VECTOR_MASS takes one coefficient");
+      break;
+    case FE_SYS_GRAD_SOURCE:
+      FE_REQUIRE(ctx, nfields == 1 && k0 == FE_TRI_VP2, "GRAD_SOURCE takes one vector field");
+      F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = false;
+      FE_REQUIRE(ctx, ncoeff == 1, "GRAD_SOURCE takes one scalar function");
+      break;
     case FE_SYS_VECTOR_DIFFUSION:
       FE_REQUIRE(ctx, nfields == 1 && k0 == FE_TRI_VP2, "VECTOR_DIFFUSION takes one vector field");
       F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = true;

@@ -181,6 +181,22 @@ def taylor_hood_system(mesh: A.Mesh, degree: int, mu: float = 1.0, convection=No
     return S, num
 
 
+def weak_gradient_system(mesh: A.Mesh, degree: int, source: A.Fn, u_bc=None, coloring=None, device=0):
+    """The weak gradient u = grad f of a scalar f for the vector P2 field with
+    strong Dirichlet values (vectorFE.cpp:19-90): VectorMass (1) and
+    GradSource (f)."""
+    num = A.Numbering(mesh, [A.TRI_VP2], [1])
+    S = A.LinearSystemHIP(device)
+    S.upload_mesh(mesh, coloring)
+    fu = S.add_field(A.TRI_VP2, num.adr[0])
+    S.set_sizes(num.nInc, num.nDofs)
+    S.add_form(A.VECTOR_MASS, [fu], degree, [A.Fn(A.FN_CONST, 1.0)])
+    S.add_form(A.GRAD_SOURCE, [fu], degree, [source])
+    S.build_csr()
+    S.set_solution(A.essential_values(num, [u_bc]))
+    return S, num
+
+
 def vector_laplacian_system(mesh: A.Mesh, degree: int, source, u_bc=None, coloring=None, device=0):
     """-div(grad u) = f for the vector P2 field with strong Dirichlet values
     (vectorFE.cpp:246-259): VectorDiffusion (1, 1) and VectorSource."""

@@ -32,7 +32,9 @@ typedef struct fe_hip_ctx fe_hip_ctx;
  * for the terms the device kernels implement. */
 #define FE_SYS_SOURCE 0
 #define FE_SYS_VECTOR_SOURCE 2
+#define FE_SYS_GRAD_SOURCE 4   /* vector field: Be += S div(phi_i) */
 #define FE_SYS_MASS 6 /* coefficients (c): feSysElm_Mass; (c, FE_FN_CONST p): feSysElm_MassPower c u^p */
+#define FE_SYS_VECTOR_MASS 10
 #define FE_SYS_TRANSIENT_MASS 13
 #define FE_SYS_TRANSIENT_VECTOR_MASS 15
 #define FE_SYS_DIFFUSION 16

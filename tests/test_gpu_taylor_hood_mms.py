@@ -8,7 +8,8 @@ quadrature, u and p essential on "Bord", the manufactured source tabulated at
 the quadrature points; Newton with GMRES/ILU(0) on the device; the L2 errors
 of u (vector) and p by the device norms.  Both formulations: divergence
 (the stress form) and Laplacian (vector diffusion + mixed gradient).  And
-vectorFE_vectorLaplacian.output (P2 rows).  Printed by the reference with 7
+vectorFE_vectorLaplacian.output and vectorFE_weakGradient.output (P2 rows).
+Printed by the reference with 7
 significant digits.
 """
 import numpy as np
@@ -107,3 +108,33 @@ def test_reference_vector_laplacian_table():
         S.close()
     assert all(_close(g, r) for g, r in zip(el2, VECTOR_LAPLACIAN[0])), (el2, VECTOR_LAPLACIAN[0])
     assert all(_close(g, r) for g, r in zip(eh1, VECTOR_LAPLACIAN[1])), (eh1, VECTOR_LAPLACIAN[1])
+
+
+# vectorFE_weakGradient.output, Lagrange P2 rows
+WEAK_GRADIENT = ["3.578664e-03", "4.543448e-04", "5.759571e-05", "7.267833e-06"]
+
+
+def test_reference_weak_gradient_table():
+    """u = grad f in the weak sense (VectorMass + GradSource), vector P2,
+    strong Dirichlet values, a..d = 0.1234, 1.2345, 2.3456, 3.4567
+    (vectorFE.cpp:19-90)."""
+    from feng_amd.api import FN_QP, KSP_GMRES, NORM_VECTOR_L2_ERROR, PC_ILU0, Fn, Mesh
+    from feng_amd.problems import NLSolverOptions, solve_newton, weak_gradient_system
+    from tests.conftest import GOLDEN
+    from tests.test_oracle_time import _qp_table
+    a, b, c, d = 0.1234, 1.2345, 2.3456, 3.4567
+    usol = (lambda x, y: a * (x * y + np.cos(x)) + b * y + 3 * c * x * x,
+            lambda x, y: a / 2 * x * x + b * x + 3 * d * y * y + 2 * d * y * np.cos(y * y))
+    f = lambda x, y: -(a / 2 * x * x * y + a * np.sin(x) + b * x * y + c * x ** 3 + d * y ** 3 + d * np.sin(y * y))
+    err = []
+    for lvl in range(1, 5):
+        print("weak gradient square", lvl, flush=True)
+        m = Mesh.read_msh(GOLDEN / "meshes" / "data" / f"square{lvl}.msh")
+        S, num = weak_gradient_system(m, 8, Fn(FN_QP, data=_qp_table(m, 8, f)),
+                                      lambda x, y, z, cc: np.where(cc == 0, usol[0](x, y), usol[1](x, y)))
+        rep = solve_newton(S, NLSolverOptions(1e-12, 1e-12, 1e4, 10), ksp=KSP_GMRES, pc=PC_ILU0, rtol=1e-13,
+                           restart=100, max_iter=20000)
+        assert rep.converged
+        err.append(S.norm(NORM_VECTOR_L2_ERROR, 0, 8, [Fn(FN_QP, data=_qp_table(m, 8, g)) for g in usol]))
+        S.close()
+    assert all(_close(g, r) for g, r in zip(err, WEAK_GRADIENT)), (err, WEAK_GRADIENT)
