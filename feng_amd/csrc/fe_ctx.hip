@@ -12,10 +12,12 @@
 namespace {
 
 struct QuadBuilder {
-  double r[16], s[16], t[16], w[16];
+  double r[kMaxQuad], s[kMaxQuad], t[kMaxQuad], w[kMaxQuad];
   int n = 0;
   void pt(double a, double b, double wt) { r[n] = a; s[n] = b; t[n] = 0.; w[n] = wt; ++n; }
   void orbit3(double a, double wt) { double b = 1. - 2. * a; pt(a, a, wt); pt(a, b, wt); pt(b, a, wt); }
+  // the orbit of (a, a, b) given b, a = (1 - b) / 2
+  void orbit3b(double b, double wt) { const double a = (1. - b) / 2.; pt(a, a, wt); pt(a, b, wt); pt(b, a, wt); }
   void orbit3r(double a, double b, double c, double wt) { pt(a, b, wt); pt(c, a, wt); pt(b, c, wt); }
   void orbit6(double a, double b, double wt)
   {
@@ -62,6 +64,47 @@ bool triRule(int degree, QuadBuilder &q)
       q.orbit3(0.459292588292723156028815514494169, 0.0475458171336423123969480521942921);
       q.orbit6(0.008394777409957605337213834539296, 0.263112829634638113421785786284643,
                0.0136151570872174971324223450369544);
+      return true;
+    case 9:
+      q.pt(1. / 3., 1. / 3., 0.0485678981413994169096209912536443);
+      q.orbit3b(0.020634961602524744433, 0.0156673501135695352684274156436046);
+      q.orbit3b(0.12582081701412672546, 0.0389137705023871396583696781497019);
+      q.orbit3(0.188203535619032730240961280467335, 0.0398238694636051265164458871320226);
+      q.orbit3(0.0447295133944527098651065899662763, 0.0127888378293490156308393992794999);
+      q.orbit6(0.0368384120547362836348175987833851, 0.2219629891607656956751025276931919,
+               0.0216417696886446886446886446886446);
+      return true;
+    case 10:
+      q.pt(1. / 3., 1. / 3., 0.0454089951913767900476432975500142);
+      q.orbit3b(0.028844733232685245264984935583748, 0.0183629788782333523585030359456832);
+      q.orbit3(0.109481575485037054795458631340522, 0.0226605297177639673913028223692986);
+      q.orbit6(0.141707219414879954756683250476361, 0.307939838764120950165155022930631,
+               0.0363789584227100543021575883096803);
+      q.orbit6(0.025003534762686386073988481007746, 0.246672560639902693917276465411176,
+               0.0141636212655287424183685307910495);
+      q.orbit6(0.0095408154002994575801528096228873, 0.0668032510122002657735402127620247,
+               4.71083348186641172996373548344341E-03);
+      return true;
+    case 11:
+      q.orbit6(0.0, 0.141129718717363295960826061941652, 3.68119189165027713212944752369032E-03);
+      q.pt(1. / 3., 1. / 3., 0.0439886505811161193990465846607278);
+      q.orbit3(0.0259891409282873952600324854988407, 4.37215577686801152475821439991262E-03);
+      q.orbit3(0.0942875026479224956305697762754049, 0.0190407859969674687575121697178070);
+      q.orbit3b(0.010726449965572372516734795387128, 9.42772402806564602923839129555767E-03);
+      q.orbit3(0.207343382614511333452934024112966, 0.0360798487723697630620149942932315);
+      q.orbit3b(0.122184388599015809877869236727746, 0.0346645693527679499208828254519072);
+      q.orbit6(0.0448416775891304433090523914688007, 0.2772206675282791551488214673424523,
+               0.0205281577146442833208261574536469);
+      return true;
+    case 12:
+      q.orbit3b(2.35652204523900E-02, 1.28655332202275E-02);
+      q.orbit3b(1.20551215411079E-01, 2.18462722690190E-02);
+      q.orbit3(2.71210385012116E-01, 3.14291121089425E-02);
+      q.orbit3(1.27576145541586E-01, 1.73980564653545E-02);
+      q.orbit3(2.13173504532100E-02, 3.08313052577950E-03);
+      q.orbit6(1.15343494534698E-01, 2.75713269685514E-01, 2.01857788831905E-02);
+      q.orbit6(2.28383322222570E-02, 2.81325580989940E-01, 1.11783866011515E-02);
+      q.orbit6(2.57340505483300E-02, 1.16251915907597E-01, 8.65811555432950E-03);
       return true;
   }
   return false;

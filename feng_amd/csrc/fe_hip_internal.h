@@ -36,12 +36,13 @@
 // ---------------------------------------------------------------------------
 // Reference-element tables of one (element kind, quadrature degree) pair,
 // staged into LDS by the element kernels.
+constexpr int kMaxQuad = 36;  // quadrature points of one rule (the degree-12 triangle rule has 33)
 struct FeRefTables {
   int nq, nf, dim, nv;
-  double w[16];
-  double L[16][6];
-  double dLr[16][6], dLs[16][6], dLt[16][6];
-  double Lg[16][4];  // P1 geometric basis at the quadrature points
+  double w[kMaxQuad];
+  double L[kMaxQuad][6];
+  double dLr[kMaxQuad][6], dLs[kMaxQuad][6], dLt[kMaxQuad][6];
+  double Lg[kMaxQuad][4];  // P1 geometric basis at the quadrature points
 };
 
 int feBuildRefTables(int kind, int degree, FeRefTables *t);  // host, returns nq or <0

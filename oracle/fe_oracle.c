@@ -314,6 +314,14 @@ static int tri_p3(double *r, double *s, double *w, int off, double a, double wt)
   off = tri_pt(r, s, w, off, a, b, wt);
   return tri_pt(r, s, w, off, b, a, wt);
 }
+/* the orbit of (a, a, b) given b, a = (1 - b) / 2 (AddTriPoints3b) */
+static int tri_p3b(double *r, double *s, double *w, int off, double b, double wt)
+{
+  double a = (1. - b) / 2.;
+  off = tri_pt(r, s, w, off, a, a, wt);
+  off = tri_pt(r, s, w, off, a, b, wt);
+  return tri_pt(r, s, w, off, b, a, wt);
+}
 static int tri_p3r(double *r, double *s, double *w, int off, double a, double b, double c,
                    double wt)
 {
@@ -380,6 +388,47 @@ int or_quad_tri(int degree, double *r, double *s, double *w)
                  0.0475458171336423123969480521942921);
       o = tri_p6(r, s, w, o, 0.008394777409957605337213834539296,
                  0.263112829634638113421785786284643, 0.0136151570872174971324223450369544);
+      break;
+    case 9:
+      o = tri_pt(r, s, w, 0, 1. / 3., 1. / 3., 0.0485678981413994169096209912536443);
+      o = tri_p3b(r, s, w, o, 0.020634961602524744433, 0.0156673501135695352684274156436046);
+      o = tri_p3b(r, s, w, o, 0.12582081701412672546, 0.0389137705023871396583696781497019);
+      o = tri_p3(r, s, w, o, 0.188203535619032730240961280467335, 0.0398238694636051265164458871320226);
+      o = tri_p3(r, s, w, o, 0.0447295133944527098651065899662763, 0.0127888378293490156308393992794999);
+      o = tri_p6(r, s, w, o, 0.0368384120547362836348175987833851, 0.2219629891607656956751025276931919,
+                 0.0216417696886446886446886446886446);
+      break;
+    case 10:
+      o = tri_pt(r, s, w, 0, 1. / 3., 1. / 3., 0.0454089951913767900476432975500142);
+      o = tri_p3b(r, s, w, o, 0.028844733232685245264984935583748, 0.0183629788782333523585030359456832);
+      o = tri_p3(r, s, w, o, 0.109481575485037054795458631340522, 0.0226605297177639673913028223692986);
+      o = tri_p6(r, s, w, o, 0.141707219414879954756683250476361, 0.307939838764120950165155022930631,
+                 0.0363789584227100543021575883096803);
+      o = tri_p6(r, s, w, o, 0.025003534762686386073988481007746, 0.246672560639902693917276465411176,
+                 0.0141636212655287424183685307910495);
+      o = tri_p6(r, s, w, o, 0.0095408154002994575801528096228873, 0.0668032510122002657735402127620247,
+                 4.71083348186641172996373548344341E-03);
+      break;
+    case 11:
+      o = tri_p6(r, s, w, 0, 0.0, 0.141129718717363295960826061941652, 3.68119189165027713212944752369032E-03);
+      o = tri_pt(r, s, w, o, 1. / 3., 1. / 3., 0.0439886505811161193990465846607278);
+      o = tri_p3(r, s, w, o, 0.0259891409282873952600324854988407, 4.37215577686801152475821439991262E-03);
+      o = tri_p3(r, s, w, o, 0.0942875026479224956305697762754049, 0.0190407859969674687575121697178070);
+      o = tri_p3b(r, s, w, o, 0.010726449965572372516734795387128, 9.42772402806564602923839129555767E-03);
+      o = tri_p3(r, s, w, o, 0.207343382614511333452934024112966, 0.0360798487723697630620149942932315);
+      o = tri_p3b(r, s, w, o, 0.122184388599015809877869236727746, 0.0346645693527679499208828254519072);
+      o = tri_p6(r, s, w, o, 0.0448416775891304433090523914688007, 0.2772206675282791551488214673424523,
+                 0.0205281577146442833208261574536469);
+      break;
+    case 12:
+      o = tri_p3b(r, s, w, 0, 2.35652204523900E-02, 1.28655332202275E-02);
+      o = tri_p3b(r, s, w, o, 1.20551215411079E-01, 2.18462722690190E-02);
+      o = tri_p3(r, s, w, o, 2.71210385012116E-01, 3.14291121089425E-02);
+      o = tri_p3(r, s, w, o, 1.27576145541586E-01, 1.73980564653545E-02);
+      o = tri_p3(r, s, w, o, 2.13173504532100E-02, 3.08313052577950E-03);
+      o = tri_p6(r, s, w, o, 1.15343494534698E-01, 2.75713269685514E-01, 2.01857788831905E-02);
+      o = tri_p6(r, s, w, o, 2.28383322222570E-02, 2.81325580989940E-01, 1.11783866011515E-02);
+      o = tri_p6(r, s, w, o, 2.57340505483300E-02, 1.16251915907597E-01, 8.65811555432950E-03);
       break;
     default: return -1;
   }
@@ -530,9 +579,9 @@ static double fn_eval(or_fn f, const double *x, int e, int nq, int q)
  * order q, i, j, d), :557-583 (Be from grad u_h), :16-27 (source). */
 typedef struct {
   int nf, nq, dim, nv;
-  double qr[16], qs[16], qt[16], qw[16];
-  double L[16][12], dLr[16][12], dLs[16][12], dLt[16][12];
-  double Lg[16][4]; /* P1 geometry basis at the quadrature points */
+  double qr[36], qs[36], qt[36], qw[36];
+  double L[36][12], dLr[36][12], dLs[36][12], dLt[36][12];
+  double Lg[36][4]; /* P1 geometry basis at the quadrature points */
 } or_ref;
 
 static int ref_setup(int kind, int quad_degree, or_ref *R)

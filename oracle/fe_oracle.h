@@ -62,7 +62,7 @@ int or_number(int V, int E, int nvpe, const int *conn, int nEdges, const int *ed
 int64_t or_csr(int n, int nforms, const int *E, const int *nI, const int *nJ,
                const int *const *adrI, const int *const *adrJ, int *ia, int *ja);
 
-/* Quadrature (feQuadratureTri.cpp:874-920, feQuadratureTet.cpp:96-110). */
+/* Quadrature (feQuadratureTri.cpp:795-978 up to degree 12, feQuadratureTet.cpp:96-110). */
 int or_quad_tri(int degree, double *r, double *s, double *w);
 int or_quad_tet(int degree, double *r, double *s, double *t, double *w);
 

@@ -69,7 +69,7 @@ int or_assemble_ns(int V, const double *xyz, int E, const int *conn, const int *
                    double *rhs)
 {
   (void)V;
-  double qr[16], qs[16], qw[16];
+  double qr[36], qs[36], qw[36];
   const int nq = or_quad_tri(quad_degree, qr, qs, qw);
   if(nq <= 0) return -1;
   /* form order of the reference stack: conv, div, source, stress; each form
@@ -203,7 +203,7 @@ int or_assemble_vector_transient_mass(int V, const double *xyz, int E, const int
                                       const double *udot, double *val, double *rhs)
 {
   (void)V;
-  double qr[16], qs[16], qw[16];
+  double qr[36], qs[36], qw[36];
   const int nq = or_quad_tri(quad_degree, qr, qs, qw);
   if(nq <= 0) return -1;
   for(int e = 0; e < E; ++e) {

@@ -157,3 +157,32 @@ def test_fp32_mfma_element_matrices(kind, deg):
     r0 = S.solve(precision=PREC_FP32_IR, rtol=1e-10)
     assert r0.converged
     S.close()
+
+
+# convergenceLaplace.output (tests/withLinearSolver/ of the reference): L2
+# error of u = x^6 + y^6, k = 1, on data/square1-4.msh with the degree-12
+# triangle rule (33 points), P1 and P2 rows.
+LAPLACE_TABLE = {
+    1: [5.939644e-02, 1.536748e-02, 3.879072e-03, 9.723478e-04],
+    2: [3.209814e-03, 4.066278e-04, 5.111231e-05, 6.408018e-06],
+}
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_reference_convergence_laplace_table(order):
+    from feng_amd.api import FN_CONST, FN_QP, NORM_L2_ERROR, Mesh
+    from feng_amd.problems import NLSolverOptions, diffusion_system, solve_newton
+    from tests.conftest import GOLDEN
+    from tests.test_oracle_time import _qp_table
+    print("laplace", order, flush=True)
+    err = []
+    for lvl in range(1, 5):
+        m = Mesh.read_msh(GOLDEN / "meshes" / "data" / f"square{lvl}.msh")
+        src = _qp_table(m, 12, lambda x, y: 30.0 * (x ** 4 + y ** 4))
+        S, num = diffusion_system(m, order, 12, _fn((FN_CONST, 1.0, 0.0, None)), _fn((FN_QP, 0.0, 0.0, src)),
+                                  u_bc=lambda x, y, z, c: x ** 6 + y ** 6)
+        rep = solve_newton(S, NLSolverOptions(), rtol=1e-13)
+        assert rep.converged
+        err.append(S.norm(NORM_L2_ERROR, 0, 12, [_fn((FN_QP, 0.0, 0.0, _qp_table(m, 12, lambda x, y: x ** 6 + y ** 6)))]))
+        S.close()
+    assert all(abs(e - r) <= 1e-6 * r for e, r in zip(err, LAPLACE_TABLE[order])), err

@@ -10,10 +10,10 @@ import scipy.sparse as sp
 from tests.oracle_bind import P
 
 
-@pytest.mark.parametrize("deg", range(1, 9))
+@pytest.mark.parametrize("deg", range(1, 13))
 def test_triangle_rule_integrates_monomials_exactly(oracle, deg):
     # int_T r^a s^b = a! b! / (a+b+2)!  on the reference triangle
-    r, s, w = (np.zeros(16) for _ in range(3))
+    r, s, w = (np.zeros(36) for _ in range(3))
     nq = oracle.L.or_quad_tri(deg, P(r, C.c_double), P(s, C.c_double), P(w, C.c_double))
     assert nq > 0
     for a in range(deg + 1):

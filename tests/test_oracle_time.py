@@ -260,7 +260,7 @@ def _qp_table(mesh, deg, f):
     import ctypes as C
     from tests.oracle_bind import P, load_oracle
     o = load_oracle()
-    r, s, w = (np.zeros(16) for _ in range(3))
+    r, s, w = (np.zeros(36) for _ in range(3))
     nq = o.L.or_quad_tri(deg, P(r, C.c_double), P(s, C.c_double), P(w, C.c_double))
     xy = mesh.xyz.reshape(-1, 3)[:, :2]
     conn = mesh.conn.reshape(-1, 3)
