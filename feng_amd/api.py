@@ -20,12 +20,12 @@ NLOC = {TRI_P1: 3, TRI_P2: 6, TRI_VP2: 12, TET_P1: 4}
 
 # weak forms (elementSystemType values, feSysElm.h:13-77)
 SOURCE, VECTOR_SOURCE, MASS, TRANSIENT_MASS, TRANSIENT_VECTOR_MASS, DIFFUSION, VECTOR_DIFFUSION = 0, 2, 6, 13, 15, 16, 18
-ADVECTION, GRAD_SOURCE, VECTOR_MASS = 20, 4, 10
+ADVECTION, GRAD_SOURCE, VECTOR_MASS, NONLINEAR_DIFFUSION = 20, 4, 10, 19
 VECTOR_CONVECTIVE_ACCELERATION, DIV_NEWTONIAN_STRESS = 22, 25
 MIXED_GRADIENT, MIXED_DIVERGENCE = 26, 31
 
 # coefficient kinds
-FN_CONST, FN_SINSIN, FN_BLOCKS, FN_SMOOTH3, FN_ELEM, FN_QP, FN_KOV_U, FN_KOV_P = range(8)
+FN_CONST, FN_SINSIN, FN_BLOCKS, FN_SMOOTH3, FN_ELEM, FN_QP, FN_KOV_U, FN_KOV_P, FN_POLY_U = range(9)
 
 KSP_CG, KSP_GMRES = 0, 1
 PC_NONE, PC_JACOBI, PC_IC0, PC_ILU0, PC_BJ_ILU0, PC_LEARNED, PC_AMG = range(7)

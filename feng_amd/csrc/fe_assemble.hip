@@ -21,7 +21,8 @@ struct ScalarAsmArgs {
   const int *perm, *conn, *adr, *pos;
   const double *xyz, *tab, *u;
   FeFn coeff, src, mass, react, adv[3];
-  int hasDiff, hasSrc, hasMass, hasReact, hasAdv;
+  FeFn dcoeff;  // NONLINEAR_DIFFUSION: dk/du (coeff is then k(u))
+  int hasDiff, hasSrc, hasMass, hasReact, hasAdv, nlDiff;
   double reactPow;  // MASS form: c u^p (p = 1: the plain mass)
   // time derivative u_dot = cn[0] u + sum_{j = 1}^{nsol - 1} cn[j] past[j - 1]
   int nsol;
@@ -30,7 +31,8 @@ struct ScalarAsmArgs {
   double *val, *rhs;
 };
 
-// Scalar diffusion (feSysElm.cpp:530-583), advection (feSysElm.cpp:766-835),
+// Scalar diffusion (feSysElm.cpp:530-583) or nonlinear diffusion
+// (feSysElm.cpp:681-760), advection (feSysElm.cpp:766-835),
 // transient mass (feSysElm.cpp:426-458), mass / mass power
 // (feSysElm.cpp:186-225, :260-278) and source (feSysElm.cpp:16-27) of one
 // field and quadrature rule, fused, on one element per thread.
@@ -81,7 +83,11 @@ __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
       for(int j = 0; j < NV; ++j)
 #pragma unroll
         for(int d = 0; d < 3; ++d) x[d] += c[j][d] * T.Lg[k][j];
-      const double kc = feEvalFn(a.coeff, x, e, nq, k);
+      double uq = 0.;
+      if(a.nlDiff)
+#pragma unroll
+        for(int j = 0; j < NF; ++j) uq += T.L[k][j] * ul[j];
+      const double kc = a.nlDiff ? feEvalFnU(a.coeff, x, e, nq, k, uq) : feEvalFn(a.coeff, x, e, nq, k);
       double gp[NF][DIM];
 #pragma unroll
       for(int i = 0; i < NF; ++i) {
@@ -95,6 +101,15 @@ __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
         }
       }
       const double jw = J * T.w[k];
+      double gu[DIM];
+      if(doRes || a.nlDiff) {
+#pragma unroll
+        for(int d = 0; d < DIM; ++d) gu[d] = 0.;
+#pragma unroll
+        for(int i = 0; i < NF; ++i)
+#pragma unroll
+          for(int d = 0; d < DIM; ++d) gu[d] += ul[i] * gp[i][d];
+      }
       if(doMat) {
 #pragma unroll
         for(int i = 0; i < NF; ++i)
@@ -102,15 +117,20 @@ __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
           for(int j = 0; j < NF; ++j)
 #pragma unroll
             for(int d = 0; d < DIM; ++d) Ae[i][j] += kc * gp[i][d] * gp[j][d] * jw;
+        if(a.nlDiff) {
+          // dk/du phi_j grad(u) . grad(phi_i): the derivative of k(u)
+          const double dk = feEvalFnU(a.dcoeff, x, e, nq, k, uq);
+#pragma unroll
+          for(int i = 0; i < NF; ++i) {
+            double gi = 0.;
+#pragma unroll
+            for(int d = 0; d < DIM; ++d) gi += gu[d] * gp[i][d];
+#pragma unroll
+            for(int j = 0; j < NF; ++j) Ae[i][j] += dk * T.L[k][j] * gi * jw;
+          }
+        }
       }
       if(doRes) {
-        double gu[DIM];
-#pragma unroll
-        for(int d = 0; d < DIM; ++d) gu[d] = 0.;
-#pragma unroll
-        for(int i = 0; i < NF; ++i)
-#pragma unroll
-          for(int d = 0; d < DIM; ++d) gu[d] += ul[i] * gp[i][d];
 #pragma unroll
         for(int i = 0; i < NF; ++i)
 #pragma unroll
@@ -953,7 +973,7 @@ void launchScalar(fe_hip_ctx *ctx, ScalarAsmArgs a)
     a.begin = ctx->colorStart[c];
     a.count = ctx->colorStart[c + 1] - a.begin;
     if(a.count == 0) continue;
-    if(ctx->scalarLanes)
+    if(ctx->scalarLanes && !a.nlDiff)
       hipLaunchKernelGGL((asmScalarLanes<NF, DIM>), dim3((a.count * LPE + 255) / 256), dim3(256), 0, ctx->stream, a);
     else
       hipLaunchKernelGGL((asmScalarFP64<NF, DIM>), dim3((a.count + 255) / 256), dim3(256), 0, ctx->stream, a);
@@ -986,13 +1006,14 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
   }
   FE_HIP_TRY(ctx, hipEventRecord(ctx->asmEv[0], ctx->stream));
   ctx->asmPending = false;
-  // Scalar forms of one field and quadrature rule (at most one DIFFUSION,
-  // one TRANSIENT_MASS and one SOURCE) are fused: one element sweep computes
-  // Ae and Be of all of them.
+  // Scalar forms of one field and quadrature rule (at most one DIFFUSION or
+  // NONLINEAR_DIFFUSION, TRANSIENT_MASS, MASS, ADVECTION and SOURCE each)
+  // are fused: one element sweep computes Ae and Be of all of them; a second
+  // form of a kind gets a sweep of its own.
   std::vector<bool> done(ctx->forms.size(), false);
   bool fp32Done = false;
   auto scalarForm = [](int t) {
-    return t == FE_SYS_DIFFUSION || t == FE_SYS_TRANSIENT_MASS || t == FE_SYS_MASS || t == FE_SYS_ADVECTION ||
+    return t == FE_SYS_DIFFUSION || t == FE_SYS_NONLINEAR_DIFFUSION || t == FE_SYS_TRANSIENT_MASS || t == FE_SYS_MASS || t == FE_SYS_ADVECTION ||
            t == FE_SYS_SOURCE;
   };
   for(size_t fi = 0; fi < ctx->forms.size(); ++fi) {
@@ -1002,7 +1023,7 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
     for(size_t fj = fi; fj < ctx->forms.size(); ++fj) {
       const FeForm &G = ctx->forms[fj];
       if(done[fj] || !scalarForm(G.sysType) || G.fields[0] != field || G.degree != degree) continue;
-      int &slot = G.sysType == FE_SYS_DIFFUSION         ? iDiff
+      int &slot = G.sysType == FE_SYS_DIFFUSION || G.sysType == FE_SYS_NONLINEAR_DIFFUSION ? iDiff
                   : G.sysType == FE_SYS_TRANSIENT_MASS ? iMass
                   : G.sysType == FE_SYS_MASS           ? iReact
                   : G.sysType == FE_SYS_ADVECTION      ? iAdv
@@ -1038,6 +1059,10 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
     if(iDiff >= 0) {
       a.hasDiff = 1;
       a.coeff = ctx->forms[iDiff].coeffs[0];
+      if(ctx->forms[iDiff].sysType == FE_SYS_NONLINEAR_DIFFUSION) {
+        a.nlDiff = 1;
+        a.dcoeff = ctx->forms[iDiff].coeffs[1];
+      }
     }
     if(iMass >= 0) {
       a.hasMass = 1;
@@ -1061,7 +1086,7 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
     }
     if(!(what & 2) && iDiff < 0 && iMass < 0 && iReact < 0 && iAdv < 0) continue;  // a lone source has no matrix
     const int kind = ctx->fields[field].kind;
-    if(iDiff >= 0 && iMass < 0 && iReact < 0 && iAdv < 0 && ctx->asmKernel == 1) {
+    if(iDiff >= 0 && !a.nlDiff && iMass < 0 && iReact < 0 && iAdv < 0 && ctx->asmKernel == 1) {
       const FeForm &F = ctx->forms[iDiff];
       MfmaAsmArgs m{};
       m.E = a.E; m.n = a.n; m.what = what; m.K = F.mfmaK;

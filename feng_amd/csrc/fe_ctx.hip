@@ -350,6 +350,11 @@ int fe_hip_add_form(fe_hip_ctx *ctx, int sysType, int nfields, const int *fields
       F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = true;
       FE_REQUIRE(ctx, ncoeff == 1, "DIFFUSION takes one coefficient");
       break;
+    case FE_SYS_NONLINEAR_DIFFUSION:
+      FE_REQUIRE(ctx, nfields == 1 && k0 != FE_TRI_VP2, "NONLINEAR_DIFFUSION takes one scalar field");
+      F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = true;
+      FE_REQUIRE(ctx, ncoeff == 2, "NONLINEAR_DIFFUSION takes (conductivity k(u), its derivative dk/du)");
+      break;
     case FE_SYS_MASS:
       // feSysElm_Mass (coefficient) or feSysElm_MassPower (coefficient,
       // exponent as an FE_FN_CONST), which reuses the MASS id (feSysElm.h:408-409)
@@ -441,8 +446,17 @@ int fe_hip_add_form(fe_hip_ctx *ctx, int sysType, int nfields, const int *fields
     fn.c0 = coeffs[c].c0;
     fn.c1 = coeffs[c].c1;
     fn.data = nullptr;
-    FE_REQUIRE(ctx, fn.kind >= FE_FN_CONST && fn.kind <= FE_FN_KOV_P, "unknown function kind");
-    if(fn.kind == FE_FN_ELEM || fn.kind == FE_FN_QP) {
+    fn.n = 0;
+    FE_REQUIRE(ctx, fn.kind >= FE_FN_CONST && fn.kind <= FE_FN_POLY_U, "unknown function kind");
+    if(fn.kind == FE_FN_POLY_U) {
+      // a function of the solution: only the nonlinear diffusion evaluates it
+      FE_REQUIRE(ctx, sysType == FE_SYS_NONLINEAR_DIFFUSION, "FE_FN_POLY_U is a coefficient of NONLINEAR_DIFFUSION");
+      FE_REQUIRE(ctx, coeffs[c].data && coeffs[c].count >= 1 && coeffs[c].count <= 8,
+                 "FE_FN_POLY_U takes 1 to 8 polynomial coefficients");
+      fn.n = (int)coeffs[c].count;
+      if(feAlloc(ctx, &fn.data, (size_t)fn.n)) return FE_STATUS_OUT_OF_MEMORY;
+      FE_HIP_TRY(ctx, hipMemcpy(fn.data, coeffs[c].data, sizeof(double) * fn.n, hipMemcpyHostToDevice));
+    } else if(fn.kind == FE_FN_ELEM || fn.kind == FE_FN_QP) {
       const int64_t need = (fn.kind == FE_FN_ELEM) ? ctx->E : (int64_t)ctx->E * nq;
       FE_REQUIRE(ctx, coeffs[c].data && coeffs[c].count == need, "coefficient table has the wrong size");
       if(feAlloc(ctx, &fn.data, (size_t)need)) return FE_STATUS_OUT_OF_MEMORY;
@@ -682,6 +696,12 @@ int fe_hip_update_coeff(fe_hip_ctx *ctx, int form, int index, const double *data
   FeForm &F = ctx->forms[form];
   FE_REQUIRE(ctx, index >= 0 && index < (int)F.coeffs.size(), "unknown coefficient");
   FeFn &fn = F.coeffs[index];
+  if(fn.kind == FE_FN_POLY_U) {
+    FE_REQUIRE(ctx, count == fn.n, "a polynomial coefficient keeps its number of coefficients");
+    FE_HIP_TRY(ctx, hipMemcpyAsync(fn.data, data, sizeof(double) * fn.n, hipMemcpyHostToDevice, ctx->stream));
+    FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return FE_STATUS_OK;
+  }
   if(fn.kind != FE_FN_ELEM && fn.kind != FE_FN_QP) {
     FE_REQUIRE(ctx, count == 1, "an analytic coefficient takes one value (its c0)");
     fn.c0 = data[0];

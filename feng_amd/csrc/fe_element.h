@@ -42,6 +42,19 @@ static __device__ __forceinline__ double feEvalFn(const FeFn &f, const double *x
   return 0.;
 }
 
+// A coefficient that may depend on the solution's value u at the point
+// (feFunctionArguments::u): FE_FN_POLY_U by Horner's rule, the other kinds
+// as feEvalFn.
+static __device__ __forceinline__ double feEvalFnU(const FeFn &f, const double *x, int e, int nq, int q, double u)
+{
+  if(f.kind == FE_FN_POLY_U) {
+    double v = 0.;
+    for(int i = f.n - 1; i >= 0; --i) v = v * u + f.data[i];
+    return v;
+  }
+  return feEvalFn(f, x, e, nq, q);
+}
+
 // Straight-simplex geometry: detJ (feCncGeo.cpp:318-340 / :380-396) and the
 // constant inverse Jacobian (feCncGeo.cpp:651-692).
 template <int DIM>

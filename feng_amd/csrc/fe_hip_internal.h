@@ -50,7 +50,8 @@ int feBuildRefTables(int kind, int degree, FeRefTables *t);  // host, returns nq
 struct FeFn {
   int kind;
   double c0, c1;
-  double *data;  // device copy for FE_FN_ELEM / FE_FN_QP (mesh element order)
+  double *data;  // device copy for FE_FN_ELEM / FE_FN_QP (mesh element order) / FE_FN_POLY_U
+  int n;         // FE_FN_POLY_U: number of coefficients
 };
 
 struct FeField {

@@ -256,6 +256,7 @@ extern "C" int fe_hip_norm(fe_hip_ctx *ctx, int type, int field, int quad_degree
     f.c0 = exact[i].c0;
     f.c1 = exact[i].c1;
     f.data = nullptr;
+    f.n = 0;
     if(f.kind == FE_FN_ELEM || f.kind == FE_FN_QP) {
       f.data = ctx->d_normData + off;
       FE_HIP_TRY(ctx, hipMemcpyAsync(f.data, exact[i].data, sizeof(double) * exact[i].count, hipMemcpyHostToDevice,

@@ -36,15 +36,21 @@ class NewtonReport:
 
 
 def diffusion_system(mesh: A.Mesh, kind: int, degree: int, k: A.Fn, s: A.Fn, u_bc=None, coloring=None,
-                     device=0):
+                     device=0, dk: A.Fn | None = None):
     """Scalar -div(k grad u) + S = 0 with essential boundary, on the GPU.
-    u_bc(x, y, z, comp) gives the Dirichlet values (None: 0)."""
+    u_bc(x, y, z, comp) gives the Dirichlet values (None: 0).  dk given: the
+    conductivity depends on the solution, k(u) with derivative dk/du (e.g.
+    two FN_POLY_U), through the NonlinearDiffusion form of the reference's
+    nonlinear test (scalarFE.cpp:128-150)."""
     num = A.Numbering(mesh, [kind], [1])
     S = A.LinearSystemHIP(device)
     S.upload_mesh(mesh, coloring)
     f = S.add_field(kind, num.adr[0])
     S.set_sizes(num.nInc, num.nDofs)
-    S.add_form(A.DIFFUSION, [f], degree, [k])
+    if dk is None:
+        S.add_form(A.DIFFUSION, [f], degree, [k])
+    else:
+        S.add_form(A.NONLINEAR_DIFFUSION, [f], degree, [k, dk])
     S.add_form(A.SOURCE, [f], degree, [s])
     S.build_csr()
     S.set_solution(A.essential_values(num, [u_bc]))

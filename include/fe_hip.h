@@ -40,6 +40,7 @@ typedef struct fe_hip_ctx fe_hip_ctx;
 #define FE_SYS_DIFFUSION 16
 #define FE_SYS_ADVECTION 20 /* coefficients: the dim velocity components */
 #define FE_SYS_VECTOR_DIFFUSION 18
+#define FE_SYS_NONLINEAR_DIFFUSION 19 /* coefficients (k(u), dk/du), e.g. two FE_FN_POLY_U */
 #define FE_SYS_VECTOR_CONVECTIVE_ACCELERATION 22
 #define FE_SYS_DIV_NEWTONIAN_STRESS 25
 #define FE_SYS_MIXED_GRADIENT 26
@@ -55,11 +56,12 @@ typedef struct fe_hip_ctx fe_hip_ctx;
 #define FE_FN_QP 5      /* data[e nq + q] (count = E nq) */
 #define FE_FN_KOV_U 6   /* Kovasznay velocity component c1 (0 or 1), Re = c0 */
 #define FE_FN_KOV_P 7   /* Kovasznay pressure, Re = c0 */
+#define FE_FN_POLY_U 8  /* sum_i data[i] u^i, u the solution at the point (count <= 8; NONLINEAR_DIFFUSION only) */
 
 typedef struct {
   int kind;
   double c0, c1;
-  const double *data; /* host table for FE_FN_ELEM / FE_FN_QP, copied at add time */
+  const double *data; /* host table for FE_FN_ELEM / FE_FN_QP / FE_FN_POLY_U, copied at add time */
   int64_t count;
 } fe_hip_fn;
 

@@ -1334,4 +1334,63 @@ int or_assemble_advection(int kind, int V, const double *xyz, int E, const int *
   return 0;
 }
 
+static double poly_u(int n, const double *c, double u)
+{
+  double v = 0.;
+  for(int i = n - 1; i >= 0; --i) v = v * u + c[i];
+  return v;
+}
+
+int or_assemble_nonlinear_diffusion(int kind, int V, const double *xyz, int E, const int *conn, const int *adr,
+                                    int n, const int *ia, const int *ja, int quad_degree, int nk, const double *k,
+                                    int ndk, const double *dk, const double *u, double *val, double *rhs)
+{
+  (void)V;
+  or_ref R;
+  if(ref_setup(kind, quad_degree, &R)) return -1;
+  const int nf = R.nf, nv = R.nv, dim = R.dim;
+  double c[12], Ae[144], Be[12], ul[12], gp[36];
+  for(int e = 0; e < E; ++e) {
+    for(int j = 0; j < nv; ++j)
+      for(int d = 0; d < 3; ++d) c[3 * j + d] = xyz[3 * conn[e * nv + j] + d];
+    const int *a = adr + (size_t)e * nf;
+    for(int i = 0; i < nf; ++i) ul[i] = u[a[i]];
+    memset(Ae, 0, sizeof(Ae));
+    memset(Be, 0, sizeof(Be));
+    or_geo g;
+    if(elem_geo(&R, c, &g)) return -2 - e;
+    for(int q = 0; q < R.nq; ++q) {
+      phys_grad(&R, &g, q, gp);
+      double uq = 0., gu[3] = {0., 0., 0.};
+      for(int j = 0; j < nf; ++j) {
+        uq += R.L[q][j] * ul[j];
+        for(int d = 0; d < dim; ++d) gu[d] += ul[j] * gp[j * dim + d];
+      }
+      const double ku = poly_u(nk, k, uq), dku = poly_u(ndk, dk, uq), jw = g.J * R.qw[q];
+      for(int i = 0; i < nf; ++i) {
+        double gi_gu = 0.;
+        for(int d = 0; d < dim; ++d) gi_gu += gp[i * dim + d] * gu[d];
+        for(int j = 0; j < nf; ++j) {
+          double gi_gj = 0.;
+          for(int d = 0; d < dim; ++d) gi_gj += gp[i * dim + d] * gp[j * dim + d];
+          Ae[i * nf + j] += (ku * gi_gj + dku * R.L[q][j] * gi_gu) * jw;
+        }
+        Be[i] -= ku * gi_gu * jw;
+      }
+    }
+    for(int i = 0; i < nf; ++i) {
+      if(a[i] >= n) continue;
+      rhs[a[i]] += Be[i];
+      if(val)
+        for(int j = 0; j < nf; ++j) {
+          if(a[j] >= n) continue;
+          int p = find_pos(ia, ja, a[i], a[j]);
+          if(p < 0) return -1;
+          val[p] += Ae[i * nf + j];
+        }
+    }
+  }
+  return 0;
+}
+
 /* Taylor-Hood NS forms: implemented in fe_oracle_ns.c. */

@@ -105,6 +105,14 @@ int or_assemble_mass_power(int kind, int V, const double *xyz, int E, const int 
 int or_assemble_advection(int kind, int V, const double *xyz, int E, const int *conn, const int *adr, int n,
                           const int *ia, const int *ja, int quad_degree, or_fn cx, or_fn cy, or_fn cz,
                           const double *u, double *val, double *rhs);
+/* Nonlinear diffusion (feSysElm.cpp:681-760): conductivity k(u) and its
+ * derivative dk/du given as polynomials in the solution's value at the
+ * quadrature point (coefficients of u^0, u^1, ...):
+ *   Ae_ij += k grad(phi_i).grad(phi_j) + dk phi_j grad(u).grad(phi_i),
+ *   Be_i  -= k grad(phi_i).grad(u).  val may be NULL. */
+int or_assemble_nonlinear_diffusion(int kind, int V, const double *xyz, int E, const int *conn, const int *adr,
+                                    int n, const int *ia, const int *ja, int quad_degree, int nk, const double *k,
+                                    int ndk, const double *dk, const double *u, double *val, double *rhs);
 /* L2 error (triangles) against exact values at the quadrature points ue[E nq]. */
 double or_l2_error_qp(int kind, const double *xyz, int E, const int *conn, const int *adr, const double *u,
                       int quad_degree, const double *ue);

@@ -43,6 +43,8 @@ class Oracle:
         L.or_element_mass.argtypes = [C.c_int, dp, C.c_int, OrFn, C.c_int, dp]
         L.or_assemble_advection.argtypes = [C.c_int, C.c_int, dp, C.c_int, ip, ip, C.c_int, ip, ip, C.c_int,
                                             OrFn, OrFn, OrFn, dp, dp, dp]
+        L.or_assemble_nonlinear_diffusion.argtypes = [C.c_int, C.c_int, dp, C.c_int, ip, ip, C.c_int, ip, ip,
+                                                      C.c_int, C.c_int, dp, C.c_int, dp, dp, dp, dp]
         L.or_l2_error_qp.argtypes = [C.c_int, dp, C.c_int, ip, ip, dp, C.c_int, dp]
         L.or_l2_error_qp.restype = C.c_double
         L.or_assemble_mass_power.argtypes = [C.c_int, C.c_int, dp, C.c_int, ip, ip, C.c_int, ip, ip, C.c_int,
@@ -212,6 +214,19 @@ class Oracle:
                                           P(conn, C.c_int), P(adr, C.c_int), n, P(ia, C.c_int), P(ja, C.c_int),
                                           deg, v[0], v[1], v[2], P(u, C.c_double),
                                           None if val is None else P(val, C.c_double), P(rhs, C.c_double))
+        assert rc == 0, rc
+
+    def assemble_nonlinear_diffusion(self, kind, xyz, conn, adr, n, ia, ja, deg, k, dk, u, val, rhs):
+        """k, dk: polynomial coefficients in u (constant term first)."""
+        nvpe = 4 if kind == 4 else 3
+        k = np.ascontiguousarray(k, np.float64)
+        dk = np.ascontiguousarray(dk, np.float64)
+        u = np.ascontiguousarray(u, np.float64)
+        rc = self.L.or_assemble_nonlinear_diffusion(kind, xyz.size // 3, P(xyz, C.c_double), conn.size // nvpe,
+                                                    P(conn, C.c_int), P(adr, C.c_int), n, P(ia, C.c_int),
+                                                    P(ja, C.c_int), deg, k.size, P(k, C.c_double), dk.size,
+                                                    P(dk, C.c_double), P(u, C.c_double),
+                                                    None if val is None else P(val, C.c_double), P(rhs, C.c_double))
         assert rc == 0, rc
 
     def l2_error_qp(self, kind, xyz, conn, adr, u, deg, ue):
