@@ -15,8 +15,8 @@ import numpy as np
 from . import _native as N
 
 # element kinds (include/fe_mesh.h)
-TRI_P1, TRI_P2, TRI_VP2, TET_P1 = 1, 2, 3, 4
-NLOC = {TRI_P1: 3, TRI_P2: 6, TRI_VP2: 12, TET_P1: 4}
+TRI_P1, TRI_P2, TRI_VP2, TET_P1, TRI_P3, TRI_P4 = 1, 2, 3, 4, 5, 6
+NLOC = {TRI_P1: 3, TRI_P2: 6, TRI_VP2: 12, TET_P1: 4, TRI_P3: 10, TRI_P4: 15}
 
 # weak forms (elementSystemType values, feSysElm.h:13-77)
 SOURCE, VECTOR_SOURCE, MASS, TRANSIENT_MASS, TRANSIENT_VECTOR_MASS, DIFFUSION, VECTOR_DIFFUSION = 0, 2, 6, 13, 15, 16, 18

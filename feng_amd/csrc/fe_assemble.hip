@@ -39,11 +39,11 @@ struct ScalarAsmArgs {
 template <int NF, int DIM>
 __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
 {
-  __shared__ FeRefTables T;
+  __shared__ FeRefTablesFor<NF> T;
   {
     const double *src = a.tab;
     double *dst = (double *)&T;
-    for(int i = threadIdx.x; i < (int)(sizeof(FeRefTables) / sizeof(double)); i += blockDim.x) dst[i] = src[i];
+    for(int i = threadIdx.x; i < (int)(sizeof(T) / sizeof(double)); i += blockDim.x) dst[i] = src[i];
   }
   __syncthreads();
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -973,10 +973,14 @@ void launchScalar(fe_hip_ctx *ctx, ScalarAsmArgs a)
     a.begin = ctx->colorStart[c];
     a.count = ctx->colorStart[c + 1] - a.begin;
     if(a.count == 0) continue;
-    if(ctx->scalarLanes && !a.nlDiff)
-      hipLaunchKernelGGL((asmScalarLanes<NF, DIM>), dim3((a.count * LPE + 255) / 256), dim3(256), 0, ctx->stream, a);
-    else
-      hipLaunchKernelGGL((asmScalarFP64<NF, DIM>), dim3((a.count + 255) / 256), dim3(256), 0, ctx->stream, a);
+    if constexpr(NF <= 6) {
+      if(ctx->scalarLanes && !a.nlDiff) {
+        hipLaunchKernelGGL((asmScalarLanes<NF, DIM>), dim3((a.count * LPE + 255) / 256), dim3(256), 0, ctx->stream,
+                           a);
+        continue;
+      }
+    }
+    hipLaunchKernelGGL((asmScalarFP64<NF, DIM>), dim3((a.count + 255) / 256), dim3(256), 0, ctx->stream, a);
   }
 }
 
@@ -1086,7 +1090,8 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
     }
     if(!(what & 2) && iDiff < 0 && iMass < 0 && iReact < 0 && iAdv < 0) continue;  // a lone source has no matrix
     const int kind = ctx->fields[field].kind;
-    if(iDiff >= 0 && !a.nlDiff && iMass < 0 && iReact < 0 && iAdv < 0 && ctx->asmKernel == 1) {
+    const bool mfmaKind = kind == FE_TRI_P1 || kind == FE_TRI_P2 || kind == FE_TET_P1;
+    if(iDiff >= 0 && !a.nlDiff && iMass < 0 && iReact < 0 && iAdv < 0 && ctx->asmKernel == 1 && mfmaKind) {
       const FeForm &F = ctx->forms[iDiff];
       MfmaAsmArgs m{};
       m.E = a.E; m.n = a.n; m.what = what; m.K = F.mfmaK;
@@ -1113,6 +1118,8 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
     } else if(kind == FE_TRI_P1) launchScalar<3, 2>(ctx, a);
     else if(kind == FE_TRI_P2) launchScalar<6, 2>(ctx, a);
     else if(kind == FE_TET_P1) launchScalar<4, 3>(ctx, a);
+    else if(kind == FE_TRI_P3) launchScalar<10, 2>(ctx, a);
+    else if(kind == FE_TRI_P4) launchScalar<15, 2>(ctx, a);
     else FE_REQUIRE(ctx, false, "scalar forms need a scalar field");
   }
   // Taylor-Hood stack: all vector/mixed forms fused into one sweep per color

@@ -1,5 +1,6 @@
 // Context lifetime, uploads, fields, forms and solution I/O of libfe_hip.so.
 #include "fe_hip_internal.h"
+#include "fe_lagrange.h"
 
 #include <algorithm>
 #include <cmath>
@@ -132,7 +133,7 @@ bool tetRule(int degree, QuadBuilder &q)
 
 }  // namespace
 
-int feBuildRefTables(int kind, int degree, FeRefTables *T)
+int feBuildRefTables(int kind, int degree, FeRefTablesHO *T)
 {
   memset(T, 0, sizeof(*T));
   QuadBuilder q;
@@ -141,11 +142,14 @@ int feBuildRefTables(int kind, int degree, FeRefTables *T)
   T->nq = q.n;
   T->dim = tet ? 3 : 2;
   T->nv = tet ? 4 : 3;
-  T->nf = (kind == FE_TRI_P1) ? 3 : (tet ? 4 : 6);  // vector P2 uses the scalar P2 tables
+  // vector P2 uses the scalar P2 tables
+  T->nf = (kind == FE_TRI_P1) ? 3 : tet ? 4 : kind == FE_TRI_P3 ? 10 : kind == FE_TRI_P4 ? 15 : 6;
   for(int k = 0; k < q.n; ++k) {
     const double r = q.r[k], s = q.s[k], t = q.t[k];
     T->w[k] = q.w[k];
-    if(kind == FE_TRI_P1) {
+    if(kind == FE_TRI_P3 || kind == FE_TRI_P4) {
+      for(int i = 0; i < T->nf; ++i) feTriShape(kind, i, r, s, T->L[k][i], T->dLr[k][i], T->dLs[k][i]);
+    } else if(kind == FE_TRI_P1) {
       const double L[3] = {1. - r - s, r, s}, dr[3] = {-1., 1., 0.}, ds[3] = {-1., 0., 1.};
       for(int i = 0; i < 3; ++i) { T->L[k][i] = L[i]; T->dLr[k][i] = dr[i]; T->dLs[k][i] = ds[i]; }
     } else if(tet) {
@@ -170,6 +174,28 @@ int feBuildRefTables(int kind, int degree, FeRefTables *T)
   return q.n;
 }
 
+int feUploadRefTables(fe_hip_ctx *ctx, const FeRefTablesHO &t, double **d)
+{
+  if(t.nf > 6) {
+    if(feAlloc(ctx, d, sizeof(FeRefTablesHO) / sizeof(double) + 1)) return FE_STATUS_OUT_OF_MEMORY;
+    FE_HIP_TRY(ctx, hipMemcpy(*d, &t, sizeof(FeRefTablesHO), hipMemcpyHostToDevice));
+    return FE_STATUS_OK;
+  }
+  FeRefTables lo;
+  memset(&lo, 0, sizeof(lo));
+  lo.nq = t.nq; lo.nf = t.nf; lo.dim = t.dim; lo.nv = t.nv;
+  for(int k = 0; k < kMaxQuad; ++k) {
+    lo.w[k] = t.w[k];
+    for(int i = 0; i < 6; ++i) {
+      lo.L[k][i] = t.L[k][i]; lo.dLr[k][i] = t.dLr[k][i]; lo.dLs[k][i] = t.dLs[k][i]; lo.dLt[k][i] = t.dLt[k][i];
+    }
+    for(int i = 0; i < 4; ++i) lo.Lg[k][i] = t.Lg[k][i];
+  }
+  if(feAlloc(ctx, d, sizeof(FeRefTables) / sizeof(double) + 1)) return FE_STATUS_OUT_OF_MEMORY;
+  FE_HIP_TRY(ctx, hipMemcpy(*d, &lo, sizeof(FeRefTables), hipMemcpyHostToDevice));
+  return FE_STATUS_OK;
+}
+
 void feFree(fe_hip_ctx *ctx, void *p)
 {
   if(!p) return;
@@ -185,6 +211,8 @@ static int nlocOf(int kind)
     case FE_TRI_P2: return 6;
     case FE_TRI_VP2: return 12;
     case FE_TET_P1: return 4;
+    case FE_TRI_P3: return 10;
+    case FE_TRI_P4: return 15;
   }
   return 0;
 }
@@ -464,7 +492,7 @@ int fe_hip_add_form(fe_hip_ctx *ctx, int sysType, int nfields, const int *fields
     }
     F.coeffs.push_back(fn);
   }
-  if(sysType == FE_SYS_DIFFUSION) {
+  if(sysType == FE_SYS_DIFFUSION && F.tab.nf <= 6) {
     // Reference tensors of the GEMM form Ae_ij = sum_{q,t} c_{q,t}(e) S_{q,t,ij}:
     // t runs over the unique metric terms (a <= b) of M_ab = sum_d dr_a/dx_d
     // dr_b/dx_d, S_{q,(a,b),ij} = w_q (d_a phi_i d_b phi_j + d_b phi_i d_a phi_j)
@@ -495,8 +523,7 @@ int fe_hip_add_form(fe_hip_ctx *ctx, int sysType, int nfields, const int *fields
     FE_HIP_TRY(ctx, hipMemcpy(F.d_sref64, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice));
     FE_HIP_TRY(ctx, hipMemcpy(F.d_sref32, S32.data(), sizeof(float) * S.size(), hipMemcpyHostToDevice));
   }
-  if(feAlloc(ctx, &F.d_tab, sizeof(FeRefTables) / sizeof(double) + 1)) return FE_STATUS_OUT_OF_MEMORY;
-  FE_HIP_TRY(ctx, hipMemcpy(F.d_tab, &F.tab, sizeof(FeRefTables), hipMemcpyHostToDevice));
+  if(int rc = feUploadRefTables(ctx, F.tab, &F.d_tab)) return rc;
   for(int f : F.layoutI) F.nI += ctx->fields[f].nloc;
   for(int f : F.layoutJ) F.nJ += ctx->fields[f].nloc;
   ctx->forms.push_back(F);

@@ -37,15 +37,28 @@
 // Reference-element tables of one (element kind, quadrature degree) pair,
 // staged into LDS by the element kernels.
 constexpr int kMaxQuad = 36;  // quadrature points of one rule (the degree-12 triangle rule has 33)
-struct FeRefTables {
+// Reference-element tables of one kind and quadrature rule, with room for
+// NFM shape functions: the kernels of P1, P2 and the vector P2 pair load the
+// 6-function layout into LDS, those of P3 / P4 triangles the 15-function one.
+template <int NFM>
+struct FeRefTablesT {
   int nq, nf, dim, nv;
   double w[kMaxQuad];
-  double L[kMaxQuad][6];
-  double dLr[kMaxQuad][6], dLs[kMaxQuad][6], dLt[kMaxQuad][6];
+  double L[kMaxQuad][NFM];
+  double dLr[kMaxQuad][NFM], dLs[kMaxQuad][NFM], dLt[kMaxQuad][NFM];
   double Lg[kMaxQuad][4];  // P1 geometric basis at the quadrature points
 };
+constexpr int kMaxNfHO = 15;
+using FeRefTables = FeRefTablesT<6>;
+using FeRefTablesHO = FeRefTablesT<kMaxNfHO>;
+// The LDS layout of a kernel instantiated for NF shape functions.
+template <int NF>
+using FeRefTablesFor = FeRefTablesT<(NF > 6 ? kMaxNfHO : 6)>;
 
-int feBuildRefTables(int kind, int degree, FeRefTables *t);  // host, returns nq or <0
+int feBuildRefTables(int kind, int degree, FeRefTablesHO *t);  // host, returns nq or <0
+// Device copy of host tables in the layout their kernels load (FeRefTables
+// for nf <= 6, FeRefTablesHO otherwise); *d is owned by the context.
+int feUploadRefTables(fe_hip_ctx *ctx, const FeRefTablesHO &t, double **d);
 
 struct FeFn {
   int kind;
@@ -64,7 +77,7 @@ struct FeForm {
   std::vector<int> fields;  // indices into ctx->fields, reference order
   int degree;
   std::vector<FeFn> coeffs;
-  FeRefTables tab;
+  FeRefTablesHO tab;
   double *d_tab = nullptr;  // device copy of tab
   bool hasMatrix;
   // layouts (feSysElm createElementarySystem): field ids of rows / cols

@@ -39,12 +39,12 @@ __device__ __forceinline__ bool normIsMax(int t)
 template <int NF, int DIM, int NC>
 __global__ __launch_bounds__(256) void normElements(NormArgs a)
 {
-  __shared__ FeRefTables T;
+  __shared__ FeRefTablesFor<NF / NC> T;
   __shared__ double red[4];
   {
     const double *src = a.tab;
     double *dst = (double *)&T;
-    for(int i = threadIdx.x; i < (int)(sizeof(FeRefTables) / sizeof(double)); i += blockDim.x) dst[i] = src[i];
+    for(int i = threadIdx.x; i < (int)(sizeof(T) / sizeof(double)); i += blockDim.x) dst[i] = src[i];
   }
   __syncthreads();
   constexpr int NV = DIM + 1, NS = NF / NC;
@@ -211,11 +211,10 @@ extern "C" int fe_hip_norm(fe_hip_ctx *ctx, int type, int field, int quad_degree
   auto it = ctx->normTabs.find(key);
   if(it != ctx->normTabs.end()) tab = it->second;
   else {
-    FeRefTables T;
+    FeRefTablesHO T;
     const int nq = feBuildRefTables(kind, quad_degree, &T);
     FE_REQUIRE(ctx, nq > 0, "unsupported quadrature degree " + std::to_string(quad_degree));
-    if(feAlloc(ctx, &tab, sizeof(FeRefTables) / sizeof(double) + 1)) return FE_STATUS_OUT_OF_MEMORY;
-    FE_HIP_TRY(ctx, hipMemcpy(tab, &T, sizeof(FeRefTables), hipMemcpyHostToDevice));
+    if(int rc = feUploadRefTables(ctx, T, &tab)) return rc;
     ctx->normTabs[key] = tab;
     ctx->normNq[key] = nq;
   }
@@ -271,6 +270,8 @@ extern "C" int fe_hip_norm(fe_hip_ctx *ctx, int type, int field, int quad_degree
   else if(kind == FE_TRI_P2) hipLaunchKernelGGL((normElements<6, 2, 1>), dim3(G), dim3(256), 0, ctx->stream, a);
   else if(kind == FE_TET_P1) hipLaunchKernelGGL((normElements<4, 3, 1>), dim3(G), dim3(256), 0, ctx->stream, a);
   else if(kind == FE_TRI_VP2) hipLaunchKernelGGL((normElements<12, 2, 2>), dim3(G), dim3(256), 0, ctx->stream, a);
+  else if(kind == FE_TRI_P3) hipLaunchKernelGGL((normElements<10, 2, 1>), dim3(G), dim3(256), 0, ctx->stream, a);
+  else if(kind == FE_TRI_P4) hipLaunchKernelGGL((normElements<15, 2, 1>), dim3(G), dim3(256), 0, ctx->stream, a);
   else FE_REQUIRE(ctx, false, "field kind without a device norm");
   hipLaunchKernelGGL(normFinal, dim3(1), dim3(64), 0, ctx->stream, ctx->d_partials, G, isMax ? 1 : 0, out);
   FE_HIP_TRY(ctx, hipGetLastError());

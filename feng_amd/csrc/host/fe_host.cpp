@@ -2,6 +2,7 @@
 // coefficient fields for the GPU assemble-and-solve path (C ABI in
 // include/fe_mesh.h).  Pure host C++17.
 #include "fe_mesh.h"
+#include "../fe_lagrange.h"
 
 #include <algorithm>
 #include <cmath>
@@ -21,6 +22,7 @@ struct fe_mesh {
   std::vector<int> conn;    // E nvpe
   std::vector<int> bconn;   // nB bnv
   std::vector<int> edges;   // 3E signed tags (2D)
+  std::vector<int> edgeA;   // first vertex of each edge as its first element walks it (2D)
   std::vector<int> bedge;   // nB edge index (2D)
 };
 
@@ -61,9 +63,12 @@ int nlocOf(int kind)
     case FE_TRI_P2: return 6;
     case FE_TRI_VP2: return 12;
     case FE_TET_P1: return 4;
+    case FE_TRI_P3: return 10;
+    case FE_TRI_P4: return 15;
   }
   return 0;
 }
+
 
 // Edge tags by first encounter while walking triangles in order and local
 // edges (0,1), (1,2), (2,0): the rule of feMeshRead.cpp:1412-1456 (a std::set
@@ -72,6 +77,7 @@ int nlocOf(int kind)
 void computeEdges(fe_mesh &m)
 {
   m.edges.assign((size_t)3 * m.E, 0);
+  m.edgeA.clear();
   std::unordered_map<uint64_t, int> tag;
   tag.reserve((size_t)m.E * 2);
   int next = 1;
@@ -83,6 +89,7 @@ void computeEdges(fe_mesh &m)
       if(it == tag.end()) {
         tag.emplace(key, next);
         m.edges[3 * e + k] = next++;
+        m.edgeA.push_back((int)a);
       } else {
         m.edges[3 * e + k] = -it->second;
       }
@@ -526,7 +533,7 @@ int fe_numbering_create(const fe_mesh *m, int nfields, const int *kinds, const i
   if(!m || !kinds || !essential || !out || nfields < 1) return FE_STATUS_POINTER_ERROR;
   for(int f = 0; f < nfields; ++f) {
     int k = kinds[f];
-    if(m->dim == 2 && !(k == FE_TRI_P1 || k == FE_TRI_P2 || k == FE_TRI_VP2))
+    if(m->dim == 2 && !(k == FE_TRI_P1 || k == FE_TRI_P2 || k == FE_TRI_VP2 || k == FE_TRI_P3 || k == FE_TRI_P4))
       return FE_STATUS_FORMAT_ERROR;
     if(m->dim == 3 && k != FE_TET_P1) return FE_STATUS_FORMAT_ERROR;
   }
@@ -536,13 +543,17 @@ int fe_numbering_create(const fe_mesh *m, int nfields, const int *kinds, const i
   num->E = m->E;
   num->kinds.assign(kinds, kinds + nfields);
   const int UNK = -1, ESS = -2;
-  std::vector<std::vector<int>> vnum(nfields), enm(nfields);
-  std::vector<int> nvd(nfields), ned(nfields);
+  std::vector<std::vector<int>> vnum(nfields), enm(nfields), inm(nfields);
+  std::vector<int> nvd(nfields), ned(nfields), nin(nfields);
   for(int f = 0; f < nfields; ++f) {
-    nvd[f] = kinds[f] == FE_TRI_VP2 ? 2 : 1;
-    ned[f] = kinds[f] == FE_TRI_P2 ? 1 : (kinds[f] == FE_TRI_VP2 ? 2 : 0);
+    const int k = kinds[f];
+    nvd[f] = k == FE_TRI_VP2 ? 2 : 1;
+    // edge DOFs (components inner for the vector kind) and element DOFs
+    ned[f] = k == FE_TRI_P2 ? 1 : k == FE_TRI_VP2 ? 2 : k == FE_TRI_P3 ? 2 : k == FE_TRI_P4 ? 3 : 0;
+    nin[f] = k == FE_TRI_P3 ? 1 : k == FE_TRI_P4 ? 3 : 0;
     vnum[f].assign((size_t)m->V * nvd[f], UNK);
     enm[f].assign((size_t)m->nEdges * ned[f], UNK);
+    inm[f].assign((size_t)m->E * nin[f], UNK);
     if(essential[f]) {
       for(int b = 0; b < m->nB; ++b) {
         for(int j = 0; j < m->bnv; ++j) {
@@ -561,6 +572,7 @@ int fe_numbering_create(const fe_mesh *m, int nfields, const int *kinds, const i
   for(int f = 0; f < nfields; ++f) {
     for(int &d : vnum[f]) if(d == UNK) d = g++;
     for(int &d : enm[f]) if(d == UNK) d = g++;
+    for(int &d : inm[f]) d = g++;  // element DOFs are never on the boundary
   }
   num->nInc = g;
   for(int f = 0; f < nfields; ++f) {
@@ -591,6 +603,18 @@ int fe_numbering_create(const fe_mesh *m, int nfields, const int *kinds, const i
       } else if(kinds[f] == FE_TRI_P2) {
         for(int j = 0; j < 3; ++j) ae[j] = vnum[f][ce[j]];
         for(int k = 0; k < 3; ++k) ae[3 + k] = enm[f][std::abs(m->edges[3 * e + k]) - 1];
+      } else if(kinds[f] == FE_TRI_P3 || kinds[f] == FE_TRI_P4) {
+        // an edge's DOFs run from its first vertex (as the first element
+        // that meets it walks it); an element walking it the other way takes
+        // them in reverse
+        const int pe = ned[f];
+        for(int j = 0; j < 3; ++j) ae[j] = vnum[f][ce[j]];
+        for(int k = 0; k < 3; ++k) {
+          const int ed = std::abs(m->edges[3 * e + k]) - 1;
+          const bool fwd = m->edgeA[ed] == ce[k];
+          for(int i = 0; i < pe; ++i) ae[3 + pe * k + i] = enm[f][(size_t)ed * pe + (fwd ? i : pe - 1 - i)];
+        }
+        for(int i = 0; i < nin[f]; ++i) ae[3 + 3 * pe + i] = inm[f][(size_t)e * nin[f] + i];
       } else {
         for(int j = 0; j < 3; ++j)
           for(int c = 0; c < 2; ++c) ae[2 * j + c] = vnum[f][(size_t)ce[j] * 2 + c];
@@ -605,7 +629,14 @@ int fe_numbering_create(const fe_mesh *m, int nfields, const int *kinds, const i
         int comp = (kinds[f] == FE_TRI_VP2) ? (j % 2) : 0;
         int node = (kinds[f] == FE_TRI_VP2) ? j / 2 : j;
         double x[3];
-        if(node < m->nvpe) {
+        if(kinds[f] == FE_TRI_P3 || kinds[f] == FE_TRI_P4) {
+          double r, t;
+          feTriNodeRS(kinds[f], j, r, t);
+          for(int k = 0; k < 3; ++k) {
+            const double x0 = m->xyz[3 * (size_t)ce[0] + k];
+            x[k] = x0 + r * (m->xyz[3 * (size_t)ce[1] + k] - x0) + t * (m->xyz[3 * (size_t)ce[2] + k] - x0);
+          }
+        } else if(node < m->nvpe) {
           for(int k = 0; k < 3; ++k) x[k] = m->xyz[3 * (size_t)ce[node] + k];
         } else {
           int k0 = node - 3, va = ce[k0], vb = ce[(k0 + 1) % 3];

@@ -30,12 +30,16 @@ extern "C" {
 #define FE_STATUS_FORMAT_ERROR (-10)
 
 /* Element families: the local layouts of feSpaceTriP1 (feSpace_2D.cpp:154-159),
- * feSpaceTriP2 (:772-786), feSpaceVecTriP2<2> (:1017-1041) and feSpaceTetPn
- * with n = 1 (feSpace_3D.cpp:293-345). */
+ * feSpaceTriP2 (:772-786), feSpaceVecTriP2<2> (:1017-1041), feSpaceTetPn
+ * with n = 1 (feSpace_3D.cpp:293-345), feSpaceTriP3 (feSpace_2D.cpp:1315-1327,
+ * :1450-1480: vertices, two DOFs per edge, one element DOF) and feSpaceTriP4
+ * (:1700-1715, :1876-1912: vertices, three per edge, three element DOFs). */
 #define FE_TRI_P1 1
 #define FE_TRI_P2 2
 #define FE_TRI_VP2 3
 #define FE_TET_P1 4
+#define FE_TRI_P3 5
+#define FE_TRI_P4 6
 
 typedef struct fe_mesh fe_mesh;
 typedef struct fe_numbering fe_numbering;
@@ -85,7 +89,7 @@ int fe_mesh_get(const fe_mesh *m, double *xyz, int *conn, int *bconn, int *edges
 
 /* DOF numbering of feMetaNumber (feNumber.cpp:677-843, numberUnknowns
  * :370-426, numberEssential :428-483) for nfields fields of kind FE_TRI_P1,
- * FE_TRI_P2, FE_TRI_VP2 or FE_TET_P1 defined on the whole mesh; essential[f] != 0 makes the field's
+ * FE_TRI_P2, FE_TRI_P3, FE_TRI_P4, FE_TRI_VP2 or FE_TET_P1 defined on the whole mesh; essential[f] != 0 makes the field's
  * DOFs on the boundary essential (a Dirichlet "Bord" space).  Fields are
  * numbered in the given order (first appearance in the reference's space
  * vector). */

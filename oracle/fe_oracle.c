@@ -459,6 +459,94 @@ int or_quad_tet(int degree, double *r, double *s, double *t, double *w)
 /* ------------------------------------------------------------------------ */
 /* Reference-element bases (feSpace_2D.cpp:118-134, :536-553, :727-741;
  * feSpace_3D.cpp:104-242 for n = 1). */
+/* Cubic and quartic Lagrange triangles on equispaced nodes, written in the
+ * barycentric coordinates l = (1 - r - s, r, s) with their gradients g[i]
+ * with respect to (l0, l1, l2); node order: vertices, each edge k from
+ * vertex k to vertex k+1, element nodes (feSpace_2D.cpp:1315-1327,
+ * :1700-1715). */
+static void set3(double *g, double a, double b, double c)
+{
+  g[0] = a;
+  g[1] = b;
+  g[2] = c;
+}
+
+/* P3: vertex 1/2 l(3l-1)(3l-2); edge node nearer a: 9/2 la lb (3 la - 1);
+ * centroid 27 l0 l1 l2. */
+static int cubic_tri(const double *l, double *L, double (*g)[3])
+{
+  for(int a = 0; a < 3; ++a) {
+    const double x = l[a];
+    L[a] = 0.5 * x * (3. * x - 1.) * (3. * x - 2.);
+    double d[3] = {0., 0., 0.};
+    d[a] = 0.5 * (27. * x * x - 18. * x + 2.);
+    set3(g[a], d[0], d[1], d[2]);
+  }
+  for(int k = 0; k < 3; ++k) {
+    const int v0 = k, v1 = (k + 1) % 3;
+    for(int h = 0; h < 2; ++h) {
+      const int a = h == 0 ? v0 : v1, b = h == 0 ? v1 : v0; /* a: the nearer vertex */
+      const int i = 3 + 2 * k + h;
+      L[i] = 4.5 * l[a] * l[b] * (3. * l[a] - 1.);
+      double d[3] = {0., 0., 0.};
+      d[a] = 4.5 * l[b] * (6. * l[a] - 1.);
+      d[b] = 4.5 * l[a] * (3. * l[a] - 1.);
+      set3(g[i], d[0], d[1], d[2]);
+    }
+  }
+  L[9] = 27. * l[0] * l[1] * l[2];
+  set3(g[9], 27. * l[1] * l[2], 27. * l[0] * l[2], 27. * l[0] * l[1]);
+  return 10;
+}
+
+/* P4: vertex l(4l-1)(4l-2)(4l-3)/6; edge nodes at 1/4 and 3/4,
+ * 8/3 la lb (4 la - 1)(4 la - 2) (a the nearer vertex), midpoint
+ * 4 la lb (4 la - 1)(4 lb - 1); element nodes 32 l0 l1 l2 (4 lm - 1) with m
+ * the coordinate equal to 1/2 at the node ((1/4, 1/4): m = 0, (1/2, 1/4):
+ * m = 1, (1/4, 1/2): m = 2). */
+static int quartic_tri(const double *l, double *L, double (*g)[3])
+{
+  for(int a = 0; a < 3; ++a) {
+    const double x = l[a];
+    L[a] = x * (4. * x - 1.) * (4. * x - 2.) * (4. * x - 3.) / 6.;
+    double d[3] = {0., 0., 0.};
+    d[a] = (256. * x * x * x - 288. * x * x + 88. * x - 6.) / 6.;
+    set3(g[a], d[0], d[1], d[2]);
+  }
+  for(int k = 0; k < 3; ++k) {
+    const int v0 = k, v1 = (k + 1) % 3;
+    for(int h = 0; h < 3; ++h) {
+      const int i = 3 + 3 * k + h;
+      double d[3] = {0., 0., 0.};
+      if(h == 1) {
+        const double pa = l[v0] * (4. * l[v0] - 1.), pb = l[v1] * (4. * l[v1] - 1.);
+        L[i] = 4. * pa * pb;
+        d[v0] = 4. * (8. * l[v0] - 1.) * pb;
+        d[v1] = 4. * pa * (8. * l[v1] - 1.);
+      } else {
+        const int a = h == 0 ? v0 : v1, b = h == 0 ? v1 : v0;
+        const double x = l[a];
+        const double hx = x * (4. * x - 1.) * (4. * x - 2.), dh = 48. * x * x - 24. * x + 2.;
+        L[i] = 8. / 3. * l[b] * hx;
+        d[a] = 8. / 3. * l[b] * dh;
+        d[b] = 8. / 3. * hx;
+      }
+      set3(g[i], d[0], d[1], d[2]);
+    }
+  }
+  for(int m = 0; m < 3; ++m) {
+    const int i = 12 + m, o1 = (m + 1) % 3, o2 = (m + 2) % 3;
+    const double pm = l[m] * (4. * l[m] - 1.);
+    L[i] = 32. * pm * l[o1] * l[o2];
+    double d[3];
+    d[m] = 32. * (8. * l[m] - 1.) * l[o1] * l[o2];
+    d[o1] = 32. * pm * l[o2];
+    d[o2] = 32. * pm * l[o1];
+    set3(g[i], d[0], d[1], d[2]);
+  }
+  return 15;
+}
+
 static int basis_eval(int kind, double r, double s, double t, double *L, double *dr,
                       double *ds, double *dt)
 {
@@ -481,6 +569,16 @@ static int basis_eval(int kind, double r, double s, double t, double *L, double 
     ds[3] = -4. * r; ds[4] = 4. * r; ds[5] = 4. * (1. - r - 2. * s);
     return 6;
   }
+  if(kind == OR_TRI_P3 || kind == OR_TRI_P4) {
+    const double l[3] = {1. - r - s, r, s};
+    double g[15][3];
+    const int nf = kind == OR_TRI_P3 ? cubic_tri(l, L, g) : quartic_tri(l, L, g);
+    for(int i = 0; i < nf; ++i) {
+      dr[i] = g[i][1] - g[i][0]; /* l1 = r, l0 = 1 - r - s */
+      ds[i] = g[i][2] - g[i][0];
+    }
+    return nf;
+  }
   if(kind == OR_TET_P1) {
     L[0] = 1. - r - s - t; L[1] = r; L[2] = s; L[3] = t;
     dr[0] = -1.; dr[1] = 1.; dr[2] = 0.; dr[3] = 0.;
@@ -489,6 +587,12 @@ static int basis_eval(int kind, double r, double s, double t, double *L, double 
     return 4;
   }
   return 0;
+}
+
+int or_basis(int kind, double r, double s, double t, double *L, double *dr, double *ds, double *dt)
+{
+  double z[15];
+  return basis_eval(kind, r, s, t, L, dr, ds, dt ? dt : z);
 }
 
 /* Geometry of a straight simplex: feCncGeo.cpp:318-340 (2D detJ), :380-396
@@ -580,7 +684,7 @@ static double fn_eval(or_fn f, const double *x, int e, int nq, int q)
 typedef struct {
   int nf, nq, dim, nv;
   double qr[36], qs[36], qt[36], qw[36];
-  double L[36][12], dLr[36][12], dLs[36][12], dLt[36][12];
+  double L[36][15], dLr[36][15], dLs[36][15], dLt[36][15];
   double Lg[36][4]; /* P1 geometry basis at the quadrature points */
 } or_ref;
 
@@ -635,7 +739,7 @@ static void diffusion_elem(const or_ref *R, const double *c, int e, or_fn coeff,
   or_geo g;
   elem_geo(R, c, &g);
   const int nf = R->nf, dim = R->dim;
-  double gp[36];
+  double gp[45];
   for(int k = 0; k < R->nq; ++k) {
     double x[3];
     qp_pos(R, c, k, x);
@@ -694,7 +798,7 @@ int or_assemble_diffusion(int kind, int V, const double *xyz, int E, const int *
   or_ref R;
   if(ref_setup(kind, quad_degree, &R)) return -1;
   const int nf = R.nf, nv = R.nv;
-  double c[12], Ae[144], Be[12], ul[12];
+  double c[12], Ae[225], Be[15], ul[15];
   or_fn zero = {OR_FN_CONST, 0., 0., NULL};
   for(int pass = 0; pass < 2; ++pass) {
     for(int ee = 0; ee < E; ++ee) {
@@ -765,7 +869,7 @@ int or_assemble_mass_power(int kind, int V, const double *xyz, int E, const int 
   or_ref R;
   if(ref_setup(kind, quad_degree, &R)) return -1;
   const int nf = R.nf, nv = R.nv;
-  double c[12], Ae[144], Be[12], ul[12];
+  double c[12], Ae[225], Be[15], ul[15];
   for(int e = 0; e < E; ++e) {
     for(int j = 0; j < nv; ++j)
       for(int d = 0; d < 3; ++d) c[3 * j + d] = xyz[3 * conn[e * nv + j] + d];
@@ -820,7 +924,7 @@ int or_assemble_transient_mass(int kind, int V, const double *xyz, int E, const 
   or_ref R;
   if(ref_setup(kind, quad_degree, &R)) return -1;
   const int nf = R.nf, nv = R.nv;
-  double c[12], Ae[144], Be[12], udl[12];
+  double c[12], Ae[225], Be[15], udl[15];
   for(int e = 0; e < E; ++e) {
     for(int j = 0; j < nv; ++j)
       for(int d = 0; d < 3; ++d) c[3 * j + d] = xyz[3 * conn[e * nv + j] + d];
@@ -870,7 +974,7 @@ int or_assemble_diffusion_omp(int kind, int V, const double *xyz, int E, const i
   int err = 0;
 #pragma omp parallel num_threads(threads) reduction(| : err)
   {
-    double c[12], Ae[144], Be[12], ul[12];
+    double c[12], Ae[225], Be[15], ul[15];
     for(int col = 0; col < ncolors; ++col) {
 #pragma omp for schedule(static)
       for(int q = start[col]; q < start[col + 1]; ++q) {
@@ -1295,7 +1399,7 @@ int or_assemble_advection(int kind, int V, const double *xyz, int E, const int *
   or_ref R;
   if(ref_setup(kind, quad_degree, &R)) return -1;
   const int nf = R.nf, nv = R.nv, dim = R.dim;
-  double c[12], Ae[144], Be[12], ul[12], gp[36];
+  double c[12], Ae[225], Be[15], ul[15], gp[45];
   for(int e = 0; e < E; ++e) {
     for(int j = 0; j < nv; ++j)
       for(int d = 0; d < 3; ++d) c[3 * j + d] = xyz[3 * conn[e * nv + j] + d];
@@ -1349,7 +1453,7 @@ int or_assemble_nonlinear_diffusion(int kind, int V, const double *xyz, int E, c
   or_ref R;
   if(ref_setup(kind, quad_degree, &R)) return -1;
   const int nf = R.nf, nv = R.nv, dim = R.dim;
-  double c[12], Ae[144], Be[12], ul[12], gp[36];
+  double c[12], Ae[225], Be[15], ul[15], gp[45];
   for(int e = 0; e < E; ++e) {
     for(int j = 0; j < nv; ++j)
       for(int d = 0; d < 3; ++d) c[3 * j + d] = xyz[3 * conn[e * nv + j] + d];

@@ -18,7 +18,7 @@ extern "C" {
 #endif
 
 /* Element families (local DOF layouts of feSpace_2D.cpp / feSpace_3D.cpp). */
-enum { OR_TRI_P1 = 1, OR_TRI_P2 = 2, OR_TRI_VP2 = 3, OR_TET_P1 = 4 };
+enum { OR_TRI_P1 = 1, OR_TRI_P2 = 2, OR_TRI_VP2 = 3, OR_TET_P1 = 4, OR_TRI_P3 = 5, OR_TRI_P4 = 6 };
 
 /* Coefficient / source functions evaluated at quadrature points. */
 enum {
@@ -105,6 +105,9 @@ int or_assemble_mass_power(int kind, int V, const double *xyz, int E, const int 
 int or_assemble_advection(int kind, int V, const double *xyz, int E, const int *conn, const int *adr, int n,
                           const int *ia, const int *ja, int quad_degree, or_fn cx, or_fn cy, or_fn cz,
                           const double *u, double *val, double *rhs);
+/* Shape functions of a kind at a reference point and their r, s (t)
+ * derivatives; returns the number of functions (dt may be NULL in 2D). */
+int or_basis(int kind, double r, double s, double t, double *L, double *dr, double *ds, double *dt);
 /* Nonlinear diffusion (feSysElm.cpp:681-760): conductivity k(u) and its
  * derivative dk/du given as polynomials in the solution's value at the
  * quadrature point (coefficients of u^0, u^1, ...):

@@ -45,6 +45,7 @@ class Oracle:
                                             OrFn, OrFn, OrFn, dp, dp, dp]
         L.or_assemble_nonlinear_diffusion.argtypes = [C.c_int, C.c_int, dp, C.c_int, ip, ip, C.c_int, ip, ip,
                                                       C.c_int, C.c_int, dp, C.c_int, dp, dp, dp, dp]
+        L.or_basis.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, dp, dp, dp, dp]
         L.or_l2_error_qp.argtypes = [C.c_int, dp, C.c_int, ip, ip, dp, C.c_int, dp]
         L.or_l2_error_qp.restype = C.c_double
         L.or_assemble_mass_power.argtypes = [C.c_int, C.c_int, dp, C.c_int, ip, ip, C.c_int, ip, ip, C.c_int,
@@ -228,6 +229,14 @@ class Oracle:
                                                     P(dk, C.c_double), P(u, C.c_double),
                                                     None if val is None else P(val, C.c_double), P(rhs, C.c_double))
         assert rc == 0, rc
+
+    def basis(self, kind, r, s, t=0.0):
+        """(L, dL/dr, dL/ds) of the kind's shape functions at (r, s[, t])."""
+        L, dr, ds, dt = (np.zeros(15) for _ in range(4))
+        nf = self.L.or_basis(kind, r, s, t, P(L, C.c_double), P(dr, C.c_double), P(ds, C.c_double),
+                             P(dt, C.c_double))
+        assert nf > 0, kind
+        return L[:nf], dr[:nf], ds[:nf]
 
     def l2_error_qp(self, kind, xyz, conn, adr, u, deg, ue):
         ue = np.ascontiguousarray(ue, np.float64)

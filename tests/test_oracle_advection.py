@@ -4,7 +4,7 @@
   a constant field gives zero advective residual on interior rows (the
   conservative form integrates c . grad(phi_i) of an interior function to 0).
 - The oracle reproduces the reference's golden tables scalarFE_advdiff.output
-  and scalarFE_advdiffreact.output (P1, P2 on mmsMeshes/square1-4.msh,
+  and scalarFE_advdiffreact.output (P1 to P4 on mmsMeshes/square1-4.msh,
   degree-8 quadrature, u = sin(xy), velocity (1, 2), k = 1, reaction 7).
 """
 import numpy as np
@@ -17,6 +17,11 @@ ADVDIFF = {  # scalarFE_advdiff.output / scalarFE_advdiffreact.output, P1 and P2
     (False, 2): [8.535503e-05, 1.069797e-05, 1.337817e-06, 1.672420e-07],
     (True, 1): [6.049672e-03, 1.478301e-03, 3.672435e-04, 9.166052e-05],
     (True, 2): [8.511754e-05, 1.068925e-05, 1.337531e-06, 1.672330e-07],
+    # P3 (kind 5) and P4 (kind 6) rows
+    (False, 5): [3.656498e-06, 2.270226e-07, 1.406515e-08, 8.740671e-10],
+    (False, 6): [1.850028e-07, 5.817233e-09, 1.814484e-10, 5.656129e-12],
+    (True, 5): [3.644954e-06, 2.268197e-07, 1.405997e-08, 8.738565e-10],
+    (True, 6): [1.846470e-07, 5.814960e-09, 1.814328e-10, 5.656016e-12],
 }
 CX, CY, K, ALPHA = 1.0, 2.0, 1.0, 7.0
 
@@ -41,7 +46,7 @@ def oracle_advdiff(oracle, mesh, kind, react):
     from tests.test_oracle_time import _qp_table
     num = Numbering(mesh, [kind], [1])
     n = num.nInc
-    nf = {1: 3, 2: 6}[kind]
+    nf = {1: 3, 2: 6, 5: 10, 6: 15}[kind]
     ia, ja = oracle.csr(n, [(num.adr[0], nf, num.adr[0], nf)])
     u = essential_values(num, [lambda x, y, z, c: u_exact(x, y)])
     src = _qp_table(mesh, 8, lambda x, y: source(x, y, react))
@@ -58,7 +63,7 @@ def oracle_advdiff(oracle, mesh, kind, react):
 
 
 @pytest.mark.parametrize("react", [False, True])
-@pytest.mark.parametrize("kind", [1, 2])
+@pytest.mark.parametrize("kind", [1, 2, 5, 6])
 def test_oracle_reproduces_reference_advdiff_tables(oracle, kind, react):
     from feng_amd.api import Mesh
     from tests.conftest import GOLDEN

@@ -1,0 +1,129 @@
+"""Pins of the cubic and quartic Lagrange triangles (feSpaceTriP3 /
+feSpaceTriP4, feSpace_2D.cpp:1315-1480 and :1700-1912).
+
+- The oracle's shape functions are the Lagrange basis of their nodes: 1 at
+  their own node and 0 at the others (node coordinates of
+  feSpace_2D.cpp:1318-1320; for P4 the zero sets of :1771-1785), and they
+  reproduce every polynomial of degree <= n exactly (so a wrong node order or
+  a dropped factor fails); their derivatives equal central differences.
+- The product's numbering places every DOF of a shared edge at one position
+  from both elements, with (n+1)(n+2)/2 DOFs per element and the DOF count
+  of the structured square ((n N + 1)^2).
+- Oracle solves reproduce the P3 and P4 rows of the reference's golden tables
+  scalarFE_diffusion.output and scalarFE_nonlineardiffusion.output
+  (mmsMeshes/square1-4.msh, degree-8 quadrature).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+NODES = {
+    5: [(0, 0), (1, 0), (0, 1), (1 / 3, 0), (2 / 3, 0), (2 / 3, 1 / 3), (1 / 3, 2 / 3), (0, 2 / 3), (0, 1 / 3),
+        (1 / 3, 1 / 3)],
+    6: [(0, 0), (1, 0), (0, 1), (.25, 0), (.5, 0), (.75, 0), (.75, .25), (.5, .5), (.25, .75), (0, .75), (0, .5),
+        (0, .25), (.25, .25), (.5, .25), (.25, .5)],
+}
+DEGREE = {5: 3, 6: 4}
+# the P3 and P4 rows of the reference's golden tables
+DIFFUSION_HO = {
+    3: [3.363292e-04, 1.999892e-05, 1.215942e-06, 7.501824e-08],
+    4: [2.055642e-05, 6.588636e-07, 2.073006e-08, 6.485182e-10],
+}
+NLDIFF_HO = {
+    3: [3.364710e-04, 2.001753e-05, 1.216275e-06, 7.502345e-08],
+    4: [2.077282e-05, 6.605501e-07, 2.074489e-08, 6.486416e-10],
+}
+
+
+@pytest.mark.parametrize("kind", [5, 6])
+def test_basis_is_the_lagrange_basis_of_its_nodes(oracle, kind):
+    n = DEGREE[kind]
+    nodes = NODES[kind]
+    M = np.array([oracle.basis(kind, r, s)[0] for r, s in nodes])
+    assert M.shape == (len(nodes), len(nodes))
+    assert np.abs(M - np.eye(len(nodes))).max() <= 1e-13
+    rng = np.random.default_rng(kind)
+    for _ in range(20):
+        r, s = rng.random(2) * 0.5
+        L, dr, ds = oracle.basis(kind, r, s)
+        assert abs(L.sum() - 1) <= 1e-13 and abs(dr.sum()) <= 1e-12 and abs(ds.sum()) <= 1e-12
+        for a in range(n + 1):
+            for b in range(n + 1 - a):
+                p = np.array([x ** a * y ** b for x, y in nodes])
+                assert abs(p @ L - r ** a * s ** b) <= 1e-12
+                dpr = a * r ** (a - 1) * s ** b if a else 0.0
+                dps = b * r ** a * s ** (b - 1) if b else 0.0
+                assert abs(p @ dr - dpr) <= 1e-11 and abs(p @ ds - dps) <= 1e-11
+        h = 1e-6
+        fd_r = (oracle.basis(kind, r + h, s)[0] - oracle.basis(kind, r - h, s)[0]) / (2 * h)
+        fd_s = (oracle.basis(kind, r, s + h)[0] - oracle.basis(kind, r, s - h)[0]) / (2 * h)
+        assert np.abs(fd_r - dr).max() <= 1e-7 and np.abs(fd_s - ds).max() <= 1e-7
+
+
+@pytest.mark.parametrize("kind", [5, 6])
+def test_numbering_places_shared_edge_dofs_once(kind):
+    from feng_amd.api import NLOC, Mesh, Numbering
+    from tests.conftest import GOLDEN
+    n, nl = DEGREE[kind], NLOC[kind]
+    assert nl == (n + 1) * (n + 2) // 2
+    for m, N in ((Mesh.square(5, 0.2, 3), None), (Mesh.read_msh(GOLDEN / "meshes" / "square1.msh"), 4)):
+        num = Numbering(m, [kind], [1])
+        adr = num.adr[0].reshape(-1, nl)
+        xyz = num.dof_xyz.reshape(-1, 3)
+        X = m.xyz.reshape(-1, 3)[m.conn.reshape(-1, 3)]
+        RS = np.array(NODES[kind])
+        P = X[:, None, 0] + RS[None, :, 0, None] * (X[:, None, 1] - X[:, None, 0]) + \
+            RS[None, :, 1, None] * (X[:, None, 2] - X[:, None, 0])
+        assert np.abs(xyz[adr] - P).max() <= 1e-14
+        assert len(np.unique(adr)) == num.nDofs
+        if N is not None:
+            assert num.nDofs == (n * N + 1) ** 2
+        # the boundary DOFs are the essential ones
+        on_b = (np.abs(xyz[:, 0]) < 1e-12) | (np.abs(xyz[:, 0] - 1) < 1e-12) | \
+               (np.abs(xyz[:, 1]) < 1e-12) | (np.abs(xyz[:, 1] - 1) < 1e-12)
+        if N is not None:
+            assert np.array_equal(np.flatnonzero(on_b), np.arange(num.nInc, num.nDofs))
+
+
+def oracle_diffusion_ho(oracle, mesh, kind, nonlinear):
+    from feng_amd.api import Numbering, essential_values
+    from tests.test_oracle_nonlinear_diffusion import DK_POLY, K_POLY, source
+    from tests.test_oracle_time import _qp_table
+    num = Numbering(mesh, [kind], [1])
+    n, nl = num.nInc, {5: 10, 6: 15}[kind]
+    ia, ja = oracle.csr(n, [(num.adr[0], nl, num.adr[0], nl)])
+    sinsin = lambda x, y: np.sin(np.pi * x) * np.sin(np.pi * y)
+    u = essential_values(num, [lambda x, y, z, c: sinsin(x, y)])
+    if nonlinear:
+        src = _qp_table(mesh, 8, lambda x, y: source(x, y))
+        for _ in range(20):
+            val, rhs = oracle.assemble_diffusion(kind, mesh.xyz, mesh.conn, num.adr[0], n, ia, ja, 8,
+                                                 (0, 0.0, 0.0, None), (5, 0.0, 0.0, src), u)
+            oracle.assemble_nonlinear_diffusion(kind, mesh.xyz, mesh.conn, num.adr[0], n, ia, ja, 8, K_POLY,
+                                                DK_POLY, u, val, rhs)
+            du = spla.spsolve(sp.csr_matrix((val, ja, ia), shape=(n, n)).tocsc(), rhs)
+            u[:n] += du
+            if np.abs(du).max() <= 1e-13:
+                break
+    else:
+        val, rhs = oracle.assemble_diffusion(kind, mesh.xyz, mesh.conn, num.adr[0], n, ia, ja, 8,
+                                             (0, 1.0, 0.0, None), (1, -2 * np.pi ** 2, 0.0, None), u)
+        u[:n] += spla.spsolve(sp.csr_matrix((val, ja, ia), shape=(n, n)).tocsc(), rhs)
+    return num, u
+
+
+@pytest.mark.parametrize("nonlinear", [False, True])
+@pytest.mark.parametrize("kind", [5, 6])
+def test_oracle_reproduces_reference_high_order_rows(oracle, kind, nonlinear):
+    from feng_amd.api import Mesh
+    from tests.conftest import GOLDEN
+    from tests.test_oracle_time import _qp_table
+    err = []
+    for lvl in range(1, 5):
+        m = Mesh.read_msh(GOLDEN / "meshes" / f"square{lvl}.msh")
+        num, u = oracle_diffusion_ho(oracle, m, kind, nonlinear)
+        err.append(oracle.l2_error_qp(kind, m.xyz, m.conn, num.adr[0], u, 8,
+                                      _qp_table(m, 8, lambda x, y: np.sin(np.pi * x) * np.sin(np.pi * y))))
+    ref = (NLDIFF_HO if nonlinear else DIFFUSION_HO)[DEGREE[kind]]
+    assert all(abs(e - r) <= 1e-6 * r for e, r in zip(err, ref)), err
