@@ -4,7 +4,8 @@ tables scalarFE_diffusion.output, scalarFE_nonlineardiffusion.output,
 scalarFE_advdiff.output and scalarFE_advdiffreact.output through the fused
 scalar sweep, Newton with GMRES/ILU(0) and the device L2 norm; the assembled
 systems against the oracle; norms of polynomial fields the spaces hold
-exactly."""
+exactly; the dimension-2 rows (P1 to P4) of interpolationScalar.output
+through the device norm."""
 import math
 
 import numpy as np
@@ -149,3 +150,23 @@ def test_norms_of_polynomial_fields(kind):
     tab = _qp_table(m, 8, lambda x, y: x ** n + y)
     assert S.norm(NORM_L2_ERROR, 0, 8, [Fn(FN_QP, data=tab)]) <= 1e-13
     S.close()
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_reference_interpolation_rows(order):
+    """interpolationScalar.output, dimension-2 rows: the L2 error of the
+    nodal interpolant of x^6 + y^6 by the device norm (degree-12 rule)."""
+    from feng_amd.api import FN_CONST, FN_QP, NORM_L2_ERROR, Fn, Mesh
+    from feng_amd.problems import diffusion_system
+    from tests.conftest import GOLDEN
+    from tests.test_oracle_high_order import INTERP, KIND_OF
+    from tests.test_oracle_time import _qp_table
+    err = []
+    for lvl in range(1, 5):
+        m = Mesh.read_msh(GOLDEN / "meshes" / "data" / f"square{lvl}.msh")
+        S, num = diffusion_system(m, KIND_OF[order], 12, Fn(FN_CONST, 1.0), Fn(FN_CONST, 0.0))
+        xyz = num.dof_xyz.reshape(-1, 3)
+        S.set_solution(xyz[:, 0] ** 6 + xyz[:, 1] ** 6)
+        err.append(S.norm(NORM_L2_ERROR, 0, 12, [Fn(FN_QP, data=_qp_table(m, 12, lambda x, y: x ** 6 + y ** 6))]))
+        S.close()
+    assert _matches(err, INTERP[order]), err

@@ -12,6 +12,9 @@ feSpaceTriP4, feSpace_2D.cpp:1315-1480 and :1700-1912).
 - Oracle solves reproduce the P3 and P4 rows of the reference's golden tables
   scalarFE_diffusion.output and scalarFE_nonlineardiffusion.output
   (mmsMeshes/square1-4.msh, degree-8 quadrature).
+- The nodal interpolant of x^6 + y^6 on data/square1-4.msh reproduces the
+  dimension-2 rows (P1 to P4) of interpolationScalar.output (L2 error,
+  degree-12 quadrature): the DOF positions of every element and the bases.
 """
 import numpy as np
 import pytest
@@ -30,6 +33,14 @@ DIFFUSION_HO = {
     3: [3.363292e-04, 1.999892e-05, 1.215942e-06, 7.501824e-08],
     4: [2.055642e-05, 6.588636e-07, 2.073006e-08, 6.485182e-10],
 }
+# interpolationScalar.output, "Dimension 2" rows (data/square1-4.msh)
+INTERP = {
+    1: [5.981947e-02, 1.538087e-02, 3.872752e-03, 9.699236e-04],
+    2: [3.187652e-03, 4.081676e-04, 5.133152e-05, 6.426200e-06],
+    3: [1.567946e-04, 9.953278e-06, 6.244800e-07, 3.906751e-08],
+    4: [4.536094e-06, 1.425343e-07, 4.460280e-09, 1.394312e-10],
+}
+KIND_OF = {1: 1, 2: 2, 3: 5, 4: 6}
 NLDIFF_HO = {
     3: [3.364710e-04, 2.001753e-05, 1.216275e-06, 7.502345e-08],
     4: [2.077282e-05, 6.605501e-07, 2.074489e-08, 6.486416e-10],
@@ -127,3 +138,27 @@ def test_oracle_reproduces_reference_high_order_rows(oracle, kind, nonlinear):
                                       _qp_table(m, 8, lambda x, y: np.sin(np.pi * x) * np.sin(np.pi * y))))
     ref = (NLDIFF_HO if nonlinear else DIFFUSION_HO)[DEGREE[kind]]
     assert all(abs(e - r) <= 1e-6 * r for e, r in zip(err, ref)), err
+
+
+def interpolant(mesh, kind):
+    """Every DOF essential (the reference's test makes the domain space
+    essential): the solution is f at the DOF positions."""
+    from feng_amd.api import Numbering
+    num = Numbering(mesh, [kind], [0])
+    xyz = num.dof_xyz.reshape(-1, 3)
+    return num, xyz[:, 0] ** 6 + xyz[:, 1] ** 6
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_oracle_reproduces_reference_interpolation_rows(oracle, order):
+    from feng_amd.api import Mesh
+    from tests.conftest import GOLDEN
+    from tests.test_oracle_time import _qp_table
+    kind = KIND_OF[order]
+    err = []
+    for lvl in range(1, 5):
+        m = Mesh.read_msh(GOLDEN / "meshes" / "data" / f"square{lvl}.msh")
+        num, u = interpolant(m, kind)
+        err.append(oracle.l2_error_qp(kind, m.xyz, m.conn, num.adr[0], u, 12,
+                                      _qp_table(m, 12, lambda x, y: x ** 6 + y ** 6)))
+    assert all(abs(e - r) <= max(1e-6 * r, 5e-15) for e, r in zip(err, INTERP[order])), err
