@@ -36,6 +36,7 @@ PREC_FP64, PREC_FP32_IR = 0, 1
  NORM_LINF_UDOT_ERROR, NORM_VECTOR_LINF_ERROR, NORM_VECTOR_LINF_UDOT_ERROR, NORM_SEMI_H1, NORM_SEMI_H1_ERROR,
  NORM_VECTOR_SEMI_H1_ERROR, NORM_H1, NORM_H1_ERROR, NORM_AREA) = (1, 2, 3, 4, 5, 7, 8, 9, 10, 11, 12, 13, 14, 16, 17,
                                                                   18, 19)
+NORM_INTEGRAL, NORM_INTEGRAL_SOL_DEPENDENT_F, NORM_INTEGRAL_DT, NORM_INTEGRAL_F = 20, 21, 23, 25
 
 
 class FeError(RuntimeError):

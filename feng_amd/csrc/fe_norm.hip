@@ -3,7 +3,10 @@
 // :1399-1444, computeVectorLInfNorm :1445-1484, the time-derivative
 // L-infinity norms :1485-1545 / :1618-1642, computeLInfNorm :1593-1617,
 // computeH1SemiNorm :1643-1733, computeVectorH1SemiNorm :1794-1847,
-// computeH1Norm :1848-1853, computeArea :1862-1873).
+// computeH1Norm :1848-1853, computeArea :1862-1873, the integrals
+// computeIntegral :1875-1891, computeIntegralSolutionDependentFunction
+// :1893-1917, computeIntegralDt :1964-1977, computeIntegralUserFunction
+// :1979-1994).
 //
 // One thread per element (mesh order, no coloring: nothing is scattered):
 // the element's contribution over its quadrature points, a block reduction
@@ -50,13 +53,15 @@ __global__ __launch_bounds__(256) void normElements(NormArgs a)
   constexpr int NV = DIM + 1, NS = NF / NC;
   const int t = a.type;
   const bool isMax = normIsMax(t);
-  const bool udot = t == FE_NORM_LINF_UDOT_ERROR || t == FE_NORM_VECTOR_LINF_UDOT_ERROR;
+  const bool udot = t == FE_NORM_LINF_UDOT_ERROR || t == FE_NORM_VECTOR_LINF_UDOT_ERROR || t == FE_NORM_INTEGRAL_DT;
+  const bool integ = t == FE_NORM_INTEGRAL || t == FE_NORM_INTEGRAL_DT || t == FE_NORM_INTEGRAL_SOL_DEPENDENT_F ||
+                     t == FE_NORM_INTEGRAL_F;
   const bool err = t == FE_NORM_L1_ERROR || t == FE_NORM_L2_ERROR || t == FE_NORM_LINF_ERROR || udot ||
                    t == FE_NORM_VECTOR_L2_ERROR || t == FE_NORM_VECTOR_LINF_ERROR || t == FE_NORM_H1_ERROR;
   const bool grad = t == FE_NORM_SEMI_H1 || t == FE_NORM_SEMI_H1_ERROR || t == FE_NORM_H1 || t == FE_NORM_H1_ERROR ||
                     t == FE_NORM_VECTOR_SEMI_H1_ERROR;
   const bool val = t != FE_NORM_SEMI_H1 && t != FE_NORM_SEMI_H1_ERROR && t != FE_NORM_AREA &&
-                   t != FE_NORM_VECTOR_SEMI_H1_ERROR;
+                   t != FE_NORM_VECTOR_SEMI_H1_ERROR && !integ;
   // exact callbacks: value(s) first, then the gradient components
   const int g0 = (t == FE_NORM_H1_ERROR) ? 1 : 0;
   const bool gradErr = t == FE_NORM_SEMI_H1_ERROR || t == FE_NORM_H1_ERROR || t == FE_NORM_VECTOR_SEMI_H1_ERROR;
@@ -134,6 +139,16 @@ __global__ __launch_bounds__(256) void normElements(NormArgs a)
         }
       }
       if(t == FE_NORM_AREA) q = 1.;
+      if(integ) {
+        // signed integrands of the scalar field (u_h, du_h/dt, f(x, u_h)) or
+        // of the callback alone
+        double uh = 0.;
+#pragma unroll
+        for(int b = 0; b < NS; ++b) uh += T.L[k][b] * ul[NC * b];
+        if(t == FE_NORM_INTEGRAL_F) q = feEvalFn(a.ex[0], x, e, T.nq, k);
+        else if(t == FE_NORM_INTEGRAL_SOL_DEPENDENT_F) q = feEvalFnU(a.ex[0], x, e, T.nq, k, uh);
+        else q = uh;
+      }
       if(isMax) acc = fmax(acc, q);
       else acc += q * jw;
     }
@@ -183,6 +198,9 @@ int normExactCount(int type, int nc, int dim)
     case FE_NORM_SEMI_H1_ERROR: return nc == 1 ? dim : -1;
     case FE_NORM_VECTOR_SEMI_H1_ERROR: return nc == 2 ? 4 : -1;
     case FE_NORM_H1_ERROR: return nc == 1 ? 1 + dim : -1;
+    case FE_NORM_INTEGRAL: case FE_NORM_INTEGRAL_DT: return nc == 1 ? 0 : -1;
+    case FE_NORM_INTEGRAL_SOL_DEPENDENT_F: return nc == 1 ? 1 : -1;
+    case FE_NORM_INTEGRAL_F: return 1;
   }
   return -1;
 }
@@ -203,7 +221,7 @@ extern "C" int fe_hip_norm(fe_hip_ctx *ctx, int type, int field, int quad_degree
                                       " exact callback(s)");
   const bool isMax = type == FE_NORM_LINF || type == FE_NORM_LINF_ERROR || type == FE_NORM_LINF_UDOT_ERROR ||
                      type == FE_NORM_VECTOR_LINF_ERROR || type == FE_NORM_VECTOR_LINF_UDOT_ERROR;
-  if(type == FE_NORM_LINF_UDOT_ERROR || type == FE_NORM_VECTOR_LINF_UDOT_ERROR)
+  if(type == FE_NORM_LINF_UDOT_ERROR || type == FE_NORM_VECTOR_LINF_UDOT_ERROR || type == FE_NORM_INTEGRAL_DT)
     FE_REQUIRE(ctx, ctx->nsol == 1 || ctx->d_past[0], "the time derivative needs the solution history");
   // reference tables of (kind, degree), cached
   const int key = kind * 64 + quad_degree;
@@ -234,8 +252,14 @@ extern "C" int fe_hip_norm(fe_hip_ctx *ctx, int type, int field, int quad_degree
   // exact callbacks; tables go to one cached device buffer
   size_t tabDoubles = 0;
   for(int i = 0; i < nexact; ++i) {
-    FE_REQUIRE(ctx, exact[i].kind >= FE_FN_CONST && exact[i].kind <= FE_FN_KOV_P, "unknown function kind");
-    if(exact[i].kind == FE_FN_ELEM || exact[i].kind == FE_FN_QP) {
+    FE_REQUIRE(ctx, exact[i].kind >= FE_FN_CONST && exact[i].kind <= FE_FN_POLY_U, "unknown function kind");
+    if(exact[i].kind == FE_FN_POLY_U) {
+      FE_REQUIRE(ctx, type == FE_NORM_INTEGRAL_SOL_DEPENDENT_F,
+                 "FE_FN_POLY_U depends on the solution: only INTEGRAL_SOL_DEPENDENT_F takes it");
+      FE_REQUIRE(ctx, exact[i].data && exact[i].count >= 1 && exact[i].count <= 8,
+                 "FE_FN_POLY_U takes 1 to 8 polynomial coefficients");
+      tabDoubles += (size_t)exact[i].count;
+    } else if(exact[i].kind == FE_FN_ELEM || exact[i].kind == FE_FN_QP) {
       const int64_t cnt = exact[i].kind == FE_FN_ELEM ? ctx->E : (int64_t)ctx->E * nq;
       FE_REQUIRE(ctx, exact[i].data && exact[i].count == cnt, "exact-solution table has the wrong size");
       tabDoubles += (size_t)cnt;
@@ -255,8 +279,8 @@ extern "C" int fe_hip_norm(fe_hip_ctx *ctx, int type, int field, int quad_degree
     f.c0 = exact[i].c0;
     f.c1 = exact[i].c1;
     f.data = nullptr;
-    f.n = 0;
-    if(f.kind == FE_FN_ELEM || f.kind == FE_FN_QP) {
+    f.n = f.kind == FE_FN_POLY_U ? (int)exact[i].count : 0;
+    if(f.kind == FE_FN_ELEM || f.kind == FE_FN_QP || f.kind == FE_FN_POLY_U) {
       f.data = ctx->d_normData + off;
       FE_HIP_TRY(ctx, hipMemcpyAsync(f.data, exact[i].data, sizeof(double) * exact[i].count, hipMemcpyHostToDevice,
                                      ctx->stream));
@@ -279,7 +303,9 @@ extern "C" int fe_hip_norm(fe_hip_ctx *ctx, int type, int field, int quad_degree
   FE_HIP_TRY(ctx, hipMemcpyAsync(&r, out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   // tiny negative sums from negative weights (feNorm.cpp:390-393)
-  if(!isMax && r < 0. && fabs(r) < 1e-14) r = fabs(r);
+  const bool integ = type == FE_NORM_INTEGRAL || type == FE_NORM_INTEGRAL_DT ||
+                     type == FE_NORM_INTEGRAL_SOL_DEPENDENT_F || type == FE_NORM_INTEGRAL_F;
+  if(!isMax && !integ && r < 0. && fabs(r) < 1e-14) r = fabs(r);
   const bool root = type == FE_NORM_L2 || type == FE_NORM_L2_ERROR || type == FE_NORM_VECTOR_L2 ||
                     type == FE_NORM_VECTOR_L2_ERROR || type == FE_NORM_SEMI_H1 || type == FE_NORM_SEMI_H1_ERROR ||
                     type == FE_NORM_H1 || type == FE_NORM_H1_ERROR || type == FE_NORM_VECTOR_SEMI_H1_ERROR;

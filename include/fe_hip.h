@@ -281,8 +281,11 @@ int fe_hip_set_essential(fe_hip_ctx *ctx, const double *values);
  *   components of the vector P2 field;
  *   SEMI_H1_ERROR: the dim components of grad u; H1_ERROR: u, then grad u;
  *   VECTOR_SEMI_H1_ERROR (2D vector P2): the four components of grad u in
- *   the reference's order d u_n / d x_m at m 2 + n.
- * The value is bitwise repeatable (fixed reduction order). */
+ *   the reference's order d u_n / d x_m at m 2 + n;
+ *   INTEGRAL, INTEGRAL_DT: none; INTEGRAL_F: f; INTEGRAL_SOL_DEPENDENT_F:
+ *   f(x, u), e.g. an FE_FN_POLY_U of the field's value.
+ * The value is bitwise repeatable (fixed reduction order); integrals are
+ * signed. */
 #define FE_NORM_L1 1
 #define FE_NORM_L1_ERROR 2
 #define FE_NORM_L2 3
@@ -300,6 +303,10 @@ int fe_hip_set_essential(fe_hip_ctx *ctx, const double *values);
 #define FE_NORM_H1 17
 #define FE_NORM_H1_ERROR 18
 #define FE_NORM_AREA 19
+#define FE_NORM_INTEGRAL 20                 /* integral of u_h */
+#define FE_NORM_INTEGRAL_SOL_DEPENDENT_F 21 /* integral of f(x, u_h) */
+#define FE_NORM_INTEGRAL_DT 23              /* integral of du_h/dt from the BDF history */
+#define FE_NORM_INTEGRAL_F 25               /* integral of f(x) */
 int fe_hip_norm(fe_hip_ctx *ctx, int type, int field, int quad_degree, int nexact, const fe_hip_fn *exact,
                 double *value);
 

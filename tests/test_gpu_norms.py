@@ -129,3 +129,43 @@ def test_vector_norms_and_time_derivative_error():
     # the scalar P1 pressure field: zero solution and history
     assert S.norm(NORM_LINF_UDOT_ERROR, 1, 5, [Fn(FN_CONST, 0.25)]) == 0.25
     S.close()
+
+
+def test_integrals():
+    """INTEGRAL, INTEGRAL_F, INTEGRAL_SOL_DEPENDENT_F, INTEGRAL_DT: the
+    reference's quadratureSquare.output (the integral of xy on
+    data/square1.msh, 0.25, through the P2 interpolant and through the
+    function, degree-2 rule), closed forms, the integral of u^2 against the
+    L2 norm, signed values and the time derivative of a BDF1 history."""
+    from feng_amd.api import (FN_CONST, FN_POLY_U, FN_QP, NORM_INTEGRAL, NORM_INTEGRAL_DT, NORM_INTEGRAL_F,
+                              NORM_INTEGRAL_SOL_DEPENDENT_F, NORM_L2, FeError, Fn, Mesh)
+    from feng_amd.problems import diffusion_system
+    from tests.conftest import GOLDEN
+    from tests.test_oracle_time import _qp_table
+    m = Mesh.read_msh(GOLDEN / "meshes" / "data" / "square1.msh")
+    S, num = diffusion_system(m, 2, 2, Fn(FN_CONST, 1.0), Fn(FN_CONST, 0.0))
+    xyz = num.dof_xyz.reshape(-1, 3)
+    S.set_solution(xyz[:, 0] * xyz[:, 1])
+    assert abs(S.norm(NORM_INTEGRAL, 0, 2) - 0.25) <= 1e-15
+    assert abs(S.norm(NORM_INTEGRAL_F, 0, 2, [Fn(FN_QP, data=_qp_table(m, 2, lambda x, y: x * y))]) - 0.25) <= 1e-15
+    S.close()
+    m = Mesh.square(8, 0.2, 3)
+    S, num = diffusion_system(m, 2, 4, Fn(FN_CONST, 1.0), Fn(FN_CONST, 0.0))
+    xyz = num.dof_xyz.reshape(-1, 3)
+    g = 0.7 * xyz[:, 0] - 0.4 * xyz[:, 1] - 0.5
+    S.set_solution(g)
+    assert abs(S.norm(NORM_INTEGRAL, 0, 4) - (0.35 - 0.2 - 0.5)) <= 1e-14  # signed
+    assert abs(S.norm(NORM_INTEGRAL_F, 0, 4, [Fn(FN_CONST, 3.0)]) - 3.0) <= 1e-14
+    l2 = S.norm(NORM_L2, 0, 4)
+    sq = S.norm(NORM_INTEGRAL_SOL_DEPENDENT_F, 0, 4, [Fn(FN_POLY_U, data=[0.0, 0.0, 1.0])])
+    assert abs(sq - l2 * l2) <= 1e-14
+    # 2 + u: twice the area plus the integral of u
+    v = S.norm(NORM_INTEGRAL_SOL_DEPENDENT_F, 0, 4, [Fn(FN_POLY_U, data=[2.0, 1.0])])
+    assert abs(v - (2.0 + 0.35 - 0.2 - 0.5)) <= 1e-14
+    with pytest.raises(FeError):
+        S.norm(NORM_INTEGRAL_F, 0, 4, [Fn(FN_POLY_U, data=[1.0])])  # depends on the solution
+    # BDF1 history with slope g / dt: the integral of u_dot is that of g / dt
+    S.set_past_solution(1, np.zeros(num.nDofs))
+    S.set_bdf([50.0, -50.0])
+    assert abs(S.norm(NORM_INTEGRAL_DT, 0, 4) - 50.0 * (0.35 - 0.2 - 0.5)) <= 1e-12
+    S.close()
