@@ -21,6 +21,7 @@ NLOC = {TRI_P1: 3, TRI_P2: 6, TRI_VP2: 12, TET_P1: 4, TRI_P3: 10, TRI_P4: 15}
 # weak forms (elementSystemType values, feSysElm.h:13-77)
 SOURCE, VECTOR_SOURCE, MASS, TRANSIENT_MASS, TRANSIENT_VECTOR_MASS, DIFFUSION, VECTOR_DIFFUSION = 0, 2, 6, 13, 15, 16, 18
 ADVECTION, GRAD_SOURCE, VECTOR_MASS, NONLINEAR_DIFFUSION = 20, 4, 10, 19
+MIXED_MASS, MIXED_MASS_POWER, MIXED_GRADGRAD = 8, 9, 28
 VECTOR_CONVECTIVE_ACCELERATION, DIV_NEWTONIAN_STRESS = 22, 25
 MIXED_GRADIENT, MIXED_DIVERGENCE = 26, 31
 

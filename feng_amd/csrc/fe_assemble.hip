@@ -274,6 +274,131 @@ __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
 }
 
 
+// Mixed scalar forms of a (test field, trial field) pair of one kind and
+// quadrature rule, fused: mixed mass (feSysElm.cpp:295-337), mixed mass
+// power (:364-403) and mixed grad-grad (:899-976), one element per thread.
+struct MixedAsmArgs {
+  int begin, count, E, n, what;
+  const int *perm, *conn, *adrI, *adrJ, *pos;
+  const double *xyz, *tab, *u;
+  FeFn cMass, cPow, cGG;
+  double pw;
+  int hasMass, hasPow, hasGG;
+  double *val, *rhs;
+};
+
+template <int NF, int DIM>
+__global__ __launch_bounds__(256) void asmMixedScalarFP64(MixedAsmArgs a)
+{
+  __shared__ FeRefTablesFor<NF> T;
+  {
+    const double *src = a.tab;
+    double *dst = (double *)&T;
+    for(int i = threadIdx.x; i < (int)(sizeof(T) / sizeof(double)); i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if(t >= a.count) return;
+  const int ee = a.begin + t;
+  const int e = a.perm[ee];
+  constexpr int NV = DIM + 1;
+  double c[NV][3];
+#pragma unroll
+  for(int j = 0; j < NV; ++j) {
+    const int v = a.conn[(size_t)e * NV + j];
+#pragma unroll
+    for(int d = 0; d < 3; ++d) c[j][d] = a.xyz[3 * (size_t)v + d];
+  }
+  double J, drdx[3], drdy[3], drdz[3];
+  feGeometry<DIM>(c, J, drdx, drdy, drdz);
+  double uj[NF];
+#pragma unroll
+  for(int j = 0; j < NF; ++j) uj[j] = a.u[a.adrJ[(size_t)ee * NF + j]];
+  double Ae[NF][NF], Be[NF];
+#pragma unroll
+  for(int i = 0; i < NF; ++i) {
+    Be[i] = 0.;
+#pragma unroll
+    for(int j = 0; j < NF; ++j) Ae[i][j] = 0.;
+  }
+  const bool doMat = a.what & 1, doRes = a.what & 2;
+  for(int k = 0; k < T.nq; ++k) {
+    double x[3] = {0., 0., 0.};
+#pragma unroll
+    for(int j = 0; j < NV; ++j)
+#pragma unroll
+      for(int d = 0; d < 3; ++d) x[d] += c[j][d] * T.Lg[k][j];
+    const double jw = J * T.w[k];
+    double uq = 0.;
+#pragma unroll
+    for(int j = 0; j < NF; ++j) uq += T.L[k][j] * uj[j];
+    // value terms: r = cM u + cP u^p, with derivative dr/du
+    double r = 0., dr = 0.;
+    if(a.hasMass) {
+      const double cm = feEvalFn(a.cMass, x, e, T.nq, k);
+      r += cm * uq;
+      dr += cm;
+    }
+    if(a.hasPow) {
+      const double cp = feEvalFn(a.cPow, x, e, T.nq, k);
+      r += cp * pow(uq, a.pw);
+      dr += cp * a.pw * pow(uq, a.pw - 1.);
+    }
+    double gp[NF][DIM], gu[DIM], cg = 0.;
+    if(a.hasGG) {
+      cg = feEvalFn(a.cGG, x, e, T.nq, k);
+#pragma unroll
+      for(int i = 0; i < NF; ++i) {
+        if(DIM == 2) {
+          gp[i][0] = T.dLr[k][i] * drdx[0] + T.dLs[k][i] * drdx[1];
+          gp[i][1 % DIM] = T.dLr[k][i] * drdy[0] + T.dLs[k][i] * drdy[1];
+        } else {
+          gp[i][0] = T.dLr[k][i] * drdx[0] + T.dLs[k][i] * drdx[1] + T.dLt[k][i] * drdx[2];
+          gp[i][1 % DIM] = T.dLr[k][i] * drdy[0] + T.dLs[k][i] * drdy[1] + T.dLt[k][i] * drdy[2];
+          gp[i][2 % DIM] = T.dLr[k][i] * drdz[0] + T.dLs[k][i] * drdz[1] + T.dLt[k][i] * drdz[2];
+        }
+      }
+#pragma unroll
+      for(int d = 0; d < DIM; ++d) gu[d] = 0.;
+#pragma unroll
+      for(int j = 0; j < NF; ++j)
+#pragma unroll
+        for(int d = 0; d < DIM; ++d) gu[d] += uj[j] * gp[j][d];
+    }
+#pragma unroll
+    for(int i = 0; i < NF; ++i) {
+      if(doRes) {
+        double b = r * T.L[k][i];
+        if(a.hasGG)
+#pragma unroll
+          for(int d = 0; d < DIM; ++d) b += cg * gu[d] * gp[i][d];
+        Be[i] -= b * jw;
+      }
+      if(doMat)
+#pragma unroll
+        for(int j = 0; j < NF; ++j) {
+          double v = dr * T.L[k][i] * T.L[k][j];
+          if(a.hasGG)
+#pragma unroll
+            for(int d = 0; d < DIM; ++d) v += cg * gp[i][d] * gp[j][d];
+          Ae[i][j] += v * jw;
+        }
+    }
+  }
+#pragma unroll
+  for(int i = 0; i < NF; ++i) {
+    const int row = a.adrI[(size_t)ee * NF + i];
+    if(doRes && row < a.n) a.rhs[row] += Be[i];
+    if(doMat) {
+#pragma unroll
+      for(int j = 0; j < NF; ++j) {
+        const int p = a.pos[(size_t)(i * NF + j) * a.E + ee];
+        if(p >= 0) a.val[p] += Ae[i][j];
+      }
+    }
+  }
+}
+
 // The same fused forms with one lane per local row: LPE lanes (4 for P1, 8
 // for P2) share an element, lane i owning row i of Ae and Be.  The element's
 // vertices, DOF values and time derivatives are loaded once by the lane
@@ -984,6 +1109,17 @@ void launchScalar(fe_hip_ctx *ctx, ScalarAsmArgs a)
   }
 }
 
+template <int NF, int DIM>
+void launchMixed(fe_hip_ctx *ctx, MixedAsmArgs a)
+{
+  for(size_t c = 0; c + 1 < ctx->colorStart.size(); ++c) {
+    a.begin = ctx->colorStart[c];
+    a.count = ctx->colorStart[c + 1] - a.begin;
+    if(a.count == 0) continue;
+    hipLaunchKernelGGL((asmMixedScalarFP64<NF, DIM>), dim3((a.count + 255) / 256), dim3(256), 0, ctx->stream, a);
+  }
+}
+
 }  // namespace
 
 extern "C" int fe_hip_set_to_zero(fe_hip_ctx *ctx, int matrix, int residual)
@@ -1121,6 +1257,61 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
     else if(kind == FE_TRI_P3) launchScalar<10, 2>(ctx, a);
     else if(kind == FE_TRI_P4) launchScalar<15, 2>(ctx, a);
     else FE_REQUIRE(ctx, false, "scalar forms need a scalar field");
+  }
+  // Mixed scalar forms of one (test, trial) field pair and quadrature rule
+  // (at most one MIXED_MASS, MIXED_MASS_POWER and MIXED_GRADGRAD each) are
+  // fused into one sweep; a second form of a kind gets a sweep of its own.
+  auto mixedForm = [](int t) {
+    return t == FE_SYS_MIXED_MASS || t == FE_SYS_MIXED_MASS_POWER || t == FE_SYS_MIXED_GRADGRAD;
+  };
+  for(size_t fi = 0; fi < ctx->forms.size(); ++fi) {
+    if(done[fi] || !mixedForm(ctx->forms[fi].sysType)) continue;
+    const FeForm &F0 = ctx->forms[fi];
+    int iM = -1, iP = -1, iG = -1;
+    for(size_t fj = fi; fj < ctx->forms.size(); ++fj) {
+      const FeForm &G = ctx->forms[fj];
+      if(done[fj] || !mixedForm(G.sysType) || G.fields != F0.fields || G.degree != F0.degree) continue;
+      int &slot = G.sysType == FE_SYS_MIXED_MASS ? iM : G.sysType == FE_SYS_MIXED_MASS_POWER ? iP : iG;
+      if(slot < 0) {
+        slot = (int)fj;
+        done[fj] = true;
+      }
+    }
+    MixedAsmArgs a{};
+    a.E = ctx->E;
+    a.n = ctx->n;
+    a.what = what;
+    a.perm = ctx->d_perm;
+    a.conn = ctx->d_conn;
+    a.xyz = ctx->d_xyz;
+    a.u = ctx->d_u;
+    a.val = ctx->d_val;
+    a.rhs = ctx->d_rhs;
+    a.tab = F0.d_tab;
+    const FeGroup &grp = ctx->groups[F0.group];
+    a.adrI = grp.d_adrI;
+    a.adrJ = grp.d_adrJ;
+    a.pos = grp.d_pos;
+    if(iM >= 0) {
+      a.hasMass = 1;
+      a.cMass = ctx->forms[iM].coeffs[0];
+    }
+    if(iP >= 0) {
+      a.hasPow = 1;
+      a.cPow = ctx->forms[iP].coeffs[0];
+      a.pw = ctx->forms[iP].coeffs[1].c0;
+    }
+    if(iG >= 0) {
+      a.hasGG = 1;
+      a.cGG = ctx->forms[iG].coeffs[0];
+    }
+    const int kind = ctx->fields[F0.fields[0]].kind;
+    if(kind == FE_TRI_P1) launchMixed<3, 2>(ctx, a);
+    else if(kind == FE_TRI_P2) launchMixed<6, 2>(ctx, a);
+    else if(kind == FE_TET_P1) launchMixed<4, 3>(ctx, a);
+    else if(kind == FE_TRI_P3) launchMixed<10, 2>(ctx, a);
+    else if(kind == FE_TRI_P4) launchMixed<15, 2>(ctx, a);
+    else FE_REQUIRE(ctx, false, "mixed scalar forms need scalar fields");
   }
   // Taylor-Hood stack: all vector/mixed forms fused into one sweep per color
   // (one quadrature rule: the forms must share their degree).

@@ -437,6 +437,21 @@ int fe_hip_add_form(fe_hip_ctx *ctx, int sysType, int nfields, const int *fields
       F.layoutI = {fields[0]}; F.layoutJ = {fields[0]}; F.hasMatrix = true;
       FE_REQUIRE(ctx, ncoeff == 2, "VECTOR_DIFFUSION takes (coefficient, diffusivity)");
       break;
+    case FE_SYS_MIXED_MASS:
+    case FE_SYS_MIXED_MASS_POWER:
+    case FE_SYS_MIXED_GRADGRAD:
+      // fields {v, u}: test functions of v, unknown u, one scalar kind
+      // (feSysElm.cpp:283-293, :352-362, :886-897)
+      FE_REQUIRE(ctx, nfields == 2 && k0 != FE_TRI_VP2 && ctx->fields[fields[1]].kind == k0 &&
+                          fields[0] != fields[1],
+                 "mixed scalar forms take {test field, trial field} of one scalar kind");
+      F.layoutI = {fields[0]}; F.layoutJ = {fields[1]}; F.hasMatrix = true;
+      if(sysType == FE_SYS_MIXED_MASS_POWER)
+        FE_REQUIRE(ctx, ncoeff == 2 && coeffs[1].kind == FE_FN_CONST,
+                   "MIXED_MASS_POWER takes a coefficient and a constant exponent");
+      else
+        FE_REQUIRE(ctx, ncoeff == 1, "this mixed form takes one coefficient");
+      break;
     case FE_SYS_MIXED_GRADIENT:
       // fields {u, p}: vector test functions, scalar unknown (feSysElm_MixedGradient)
       FE_REQUIRE(ctx, nfields == 2 && k0 == FE_TRI_VP2 && ctx->fields[fields[1]].kind == FE_TRI_P1,

@@ -78,6 +78,33 @@ def advection_diffusion_system(mesh: A.Mesh, kind: int, degree: int, velocity, k
     return S, num
 
 
+def mixed_diffusion_system(mesh: A.Mesh, kind: int, degree: int, su: A.Fn, sv: A.Fn, u_bc=None, v_bc=None,
+                           coloring=None, device=0):
+    """Two coupled scalar fields u, v with essential boundaries,
+    u + v^3 - lap v + su = 0 and u + v - lap u + sv = 0: the form stack of the
+    reference's mixed diffusion test (scalarFE.cpp:455-475) -- Mass {u},
+    MixedMassPower {u, v} (1, 3), MixedGradGrad {u, v} (-1), Source {u},
+    Mass {v}, MixedMass {v, u}, MixedGradGrad {v, u} (-1), Source {v}."""
+    num = A.Numbering(mesh, [kind, kind], [1, 1])
+    S = A.LinearSystemHIP(device)
+    S.upload_mesh(mesh, coloring)
+    fu = S.add_field(kind, num.adr[0])
+    fv = S.add_field(kind, num.adr[1])
+    S.set_sizes(num.nInc, num.nDofs)
+    one, minus_one = A.Fn(A.FN_CONST, 1.0), A.Fn(A.FN_CONST, -1.0)
+    S.add_form(A.MASS, [fu], degree, [one])
+    S.add_form(A.MIXED_MASS_POWER, [fu, fv], degree, [one, A.Fn(A.FN_CONST, 3.0)])
+    S.add_form(A.MIXED_GRADGRAD, [fu, fv], degree, [minus_one])
+    S.add_form(A.SOURCE, [fu], degree, [su])
+    S.add_form(A.MASS, [fv], degree, [one])
+    S.add_form(A.MIXED_MASS, [fv, fu], degree, [one])
+    S.add_form(A.MIXED_GRADGRAD, [fv, fu], degree, [minus_one])
+    S.add_form(A.SOURCE, [fv], degree, [sv])
+    S.build_csr()
+    S.set_solution(A.essential_values(num, [u_bc, v_bc]))
+    return S, num
+
+
 def heat_system(mesh: A.Mesh, kind: int, degree: int, m: A.Fn, k: A.Fn, s: A.Fn, essential=True, coloring=None,
                 device=0, periodic=None, react=None):
     """Transient scalar m u_t - div(k grad u) + S = 0: the form stack of the

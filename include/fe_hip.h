@@ -34,6 +34,9 @@ typedef struct fe_hip_ctx fe_hip_ctx;
 #define FE_SYS_VECTOR_SOURCE 2
 #define FE_SYS_GRAD_SOURCE 4   /* vector field: Be += S div(phi_i) */
 #define FE_SYS_MASS 6 /* coefficients (c): feSysElm_Mass; (c, FE_FN_CONST p): feSysElm_MassPower c u^p */
+/* Mixed scalar forms, fields {test field, trial field} of one scalar kind: */
+#define FE_SYS_MIXED_MASS 8       /* (c): Be -= c u phi_i */
+#define FE_SYS_MIXED_MASS_POWER 9 /* (c, FE_FN_CONST p): Be -= c u^p phi_i */
 #define FE_SYS_VECTOR_MASS 10
 #define FE_SYS_TRANSIENT_MASS 13
 #define FE_SYS_TRANSIENT_VECTOR_MASS 15
@@ -44,6 +47,7 @@ typedef struct fe_hip_ctx fe_hip_ctx;
 #define FE_SYS_VECTOR_CONVECTIVE_ACCELERATION 22
 #define FE_SYS_DIV_NEWTONIAN_STRESS 25
 #define FE_SYS_MIXED_GRADIENT 26
+#define FE_SYS_MIXED_GRADGRAD 28 /* {test field, trial field}, (c): Be -= c grad(u).grad(phi_i) */
 #define FE_SYS_MIXED_DIVERGENCE 31
 
 /* Coefficient kinds: the feFunction callbacks (feFunction.h:75-110) the

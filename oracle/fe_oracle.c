@@ -1497,4 +1497,57 @@ int or_assemble_nonlinear_diffusion(int kind, int V, const double *xyz, int E, c
   return 0;
 }
 
+int or_assemble_mixed(int kind, const double *xyz, int E, const int *conn, const int *adrI, const int *adrJ, int n,
+                      const int *ia, const int *ja, int quad_degree, double cm, double cp, double p, double cg,
+                      const double *u, double *val, double *rhs)
+{
+  or_ref R;
+  if(ref_setup(kind, quad_degree, &R)) return -1;
+  const int nf = R.nf, nv = R.nv, dim = R.dim;
+  double c[12], Ae[225], Be[15], uj[15], gp[45];
+  for(int e = 0; e < E; ++e) {
+    for(int j = 0; j < nv; ++j)
+      for(int d = 0; d < 3; ++d) c[3 * j + d] = xyz[3 * conn[e * nv + j] + d];
+    const int *ai = adrI + (size_t)e * nf, *aj = adrJ + (size_t)e * nf;
+    for(int j = 0; j < nf; ++j) uj[j] = u[aj[j]];
+    memset(Ae, 0, sizeof(Ae));
+    memset(Be, 0, sizeof(Be));
+    or_geo g;
+    if(elem_geo(&R, c, &g)) return -2 - e;
+    for(int q = 0; q < R.nq; ++q) {
+      phys_grad(&R, &g, q, gp);
+      double uq = 0., gu[3] = {0., 0., 0.};
+      for(int j = 0; j < nf; ++j) {
+        uq += R.L[q][j] * uj[j];
+        for(int d = 0; d < dim; ++d) gu[d] += uj[j] * gp[j * dim + d];
+      }
+      const double jw = g.J * R.qw[q];
+      const double val_q = cm * uq + (cp != 0. ? cp * pow(uq, p) : 0.);
+      const double dval = cm + (cp != 0. ? cp * p * pow(uq, p - 1.) : 0.);
+      for(int i = 0; i < nf; ++i) {
+        double gi_gu = 0.;
+        for(int d = 0; d < dim; ++d) gi_gu += gp[i * dim + d] * gu[d];
+        Be[i] -= (val_q * R.L[q][i] + cg * gi_gu) * jw;
+        for(int j = 0; j < nf; ++j) {
+          double gi_gj = 0.;
+          for(int d = 0; d < dim; ++d) gi_gj += gp[i * dim + d] * gp[j * dim + d];
+          Ae[i * nf + j] += (dval * R.L[q][i] * R.L[q][j] + cg * gi_gj) * jw;
+        }
+      }
+    }
+    for(int i = 0; i < nf; ++i) {
+      if(ai[i] >= n) continue;
+      rhs[ai[i]] += Be[i];
+      if(val)
+        for(int j = 0; j < nf; ++j) {
+          if(aj[j] >= n) continue;
+          int pp = find_pos(ia, ja, ai[i], aj[j]);
+          if(pp < 0) return -1;
+          val[pp] += Ae[i * nf + j];
+        }
+    }
+  }
+  return 0;
+}
+
 /* Taylor-Hood NS forms: implemented in fe_oracle_ns.c. */

@@ -105,6 +105,16 @@ int or_assemble_mass_power(int kind, int V, const double *xyz, int E, const int 
 int or_assemble_advection(int kind, int V, const double *xyz, int E, const int *conn, const int *adr, int n,
                           const int *ia, const int *ja, int quad_degree, or_fn cx, or_fn cy, or_fn cz,
                           const double *u, double *val, double *rhs);
+/* Mixed scalar forms with test functions of one field (adrI) and the
+ * unknown u of another field (adrJ) of the same kind, constant coefficients:
+ * mixed mass cm (feSysElm.cpp:295-337), mixed mass power cp u^p (:364-403),
+ * mixed grad-grad cg (:899-976):
+ *   Be_i -= (cm u + cp u^p) phi_i + cg grad(u).grad(phi_i),
+ *   Ae_ij += (cm + cp p u^(p-1)) phi_i phi_j + cg grad(phi_i).grad(phi_j).
+ * Rows adrI >= n are dropped; val may be NULL. */
+int or_assemble_mixed(int kind, const double *xyz, int E, const int *conn, const int *adrI, const int *adrJ, int n,
+                      const int *ia, const int *ja, int quad_degree, double cm, double cp, double p, double cg,
+                      const double *u, double *val, double *rhs);
 /* Shape functions of a kind at a reference point and their r, s (t)
  * derivatives; returns the number of functions (dt may be NULL in 2D). */
 int or_basis(int kind, double r, double s, double t, double *L, double *dr, double *ds, double *dt);

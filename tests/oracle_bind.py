@@ -46,6 +46,8 @@ class Oracle:
         L.or_assemble_nonlinear_diffusion.argtypes = [C.c_int, C.c_int, dp, C.c_int, ip, ip, C.c_int, ip, ip,
                                                       C.c_int, C.c_int, dp, C.c_int, dp, dp, dp, dp]
         L.or_basis.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, dp, dp, dp, dp]
+        L.or_assemble_mixed.argtypes = [C.c_int, dp, C.c_int, ip, ip, ip, C.c_int, ip, ip, C.c_int, C.c_double,
+                                        C.c_double, C.c_double, C.c_double, dp, dp, dp]
         L.or_l2_error_qp.argtypes = [C.c_int, dp, C.c_int, ip, ip, dp, C.c_int, dp]
         L.or_l2_error_qp.restype = C.c_double
         L.or_assemble_mass_power.argtypes = [C.c_int, C.c_int, dp, C.c_int, ip, ip, C.c_int, ip, ip, C.c_int,
@@ -237,6 +239,16 @@ class Oracle:
                              P(dt, C.c_double))
         assert nf > 0, kind
         return L[:nf], dr[:nf], ds[:nf]
+
+    def assemble_mixed(self, kind, xyz, conn, adrI, adrJ, n, ia, ja, deg, cm, cp, p, cg, u, val, rhs):
+        """Mixed mass / mass power / grad-grad of (test adrI, trial adrJ), added into val / rhs."""
+        nvpe = 4 if kind == 4 else 3
+        u = np.ascontiguousarray(u, np.float64)
+        rc = self.L.or_assemble_mixed(kind, P(xyz, C.c_double), conn.size // nvpe, P(conn, C.c_int),
+                                      P(adrI, C.c_int), P(adrJ, C.c_int), n, P(ia, C.c_int), P(ja, C.c_int), deg,
+                                      cm, cp, p, cg, P(u, C.c_double),
+                                      None if val is None else P(val, C.c_double), P(rhs, C.c_double))
+        assert rc == 0, rc
 
     def l2_error_qp(self, kind, xyz, conn, adr, u, deg, ue):
         ue = np.ascontiguousarray(ue, np.float64)
