@@ -36,7 +36,7 @@ struct ScalarAsmArgs {
 // transient mass (feSysElm.cpp:426-458), mass / mass power
 // (feSysElm.cpp:186-225, :260-278) and source (feSysElm.cpp:16-27) of one
 // field and quadrature rule, fused, on one element per thread.
-template <int NF, int DIM>
+template <int NF, int DIM, bool RARE = true>
 __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
 {
   __shared__ FeRefTablesFor<NF> T;
@@ -84,10 +84,10 @@ __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
 #pragma unroll
         for(int d = 0; d < 3; ++d) x[d] += c[j][d] * T.Lg[k][j];
       double uq = 0.;
-      if(a.nlDiff)
+      if((RARE && a.nlDiff))
 #pragma unroll
         for(int j = 0; j < NF; ++j) uq += T.L[k][j] * ul[j];
-      const double kc = a.nlDiff ? feEvalFnU(a.coeff, x, e, nq, k, uq) : feEvalFn(a.coeff, x, e, nq, k);
+      const double kc = (RARE && a.nlDiff) ? feEvalFnU(a.coeff, x, e, nq, k, uq) : feEvalFn(a.coeff, x, e, nq, k);
       double gp[NF][DIM];
 #pragma unroll
       for(int i = 0; i < NF; ++i) {
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
       }
       const double jw = J * T.w[k];
       double gu[DIM];
-      if(doRes || a.nlDiff) {
+      if(doRes || (RARE && a.nlDiff)) {
 #pragma unroll
         for(int d = 0; d < DIM; ++d) gu[d] = 0.;
 #pragma unroll
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
           for(int j = 0; j < NF; ++j)
 #pragma unroll
             for(int d = 0; d < DIM; ++d) Ae[i][j] += kc * gp[i][d] * gp[j][d] * jw;
-        if(a.nlDiff) {
+        if((RARE && a.nlDiff)) {
           // dk/du phi_j grad(u) . grad(phi_i): the derivative of k(u)
           const double dk = feEvalFnU(a.dcoeff, x, e, nq, k, uq);
 #pragma unroll
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
       }
     }
   }
-  if(a.hasAdv) {
+  if(RARE && a.hasAdv) {
     // Ae_ij -= c . grad(phi_i) phi_j, Be_i += u c . grad(phi_i) (conservative
     // form, feSysElm.cpp:766-835)
     for(int k = 0; k < nq; ++k) {
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(256) void asmScalarFP64(ScalarAsmArgs a)
       }
     }
   }
-  if(a.hasReact) {
+  if(RARE && a.hasReact) {
     // Be -= c u^p phi_i; Ae += c p u^(p-1) phi_i phi_j (the derivative the
     // reference approximates by finite differences for p != 1)
     const double pw = a.reactPow;
@@ -1105,7 +1105,13 @@ void launchScalar(fe_hip_ctx *ctx, ScalarAsmArgs a)
         continue;
       }
     }
-    hipLaunchKernelGGL((asmScalarFP64<NF, DIM>), dim3((a.count + 255) / 256), dim3(256), 0, ctx->stream, a);
+    // the advection, mass-power and nonlinear-diffusion terms are compiled
+    // out of the common variant (fewer registers, more waves per SIMD)
+    if(a.hasAdv || a.hasReact || a.nlDiff)
+      hipLaunchKernelGGL((asmScalarFP64<NF, DIM, true>), dim3((a.count + 255) / 256), dim3(256), 0, ctx->stream, a);
+    else
+      hipLaunchKernelGGL((asmScalarFP64<NF, DIM, false>), dim3((a.count + 255) / 256), dim3(256), 0, ctx->stream,
+                         a);
   }
 }
 
