@@ -165,12 +165,17 @@ def test_fp32_mfma_element_matrices(kind, deg):
 LAPLACE_TABLE = {
     1: [5.939644e-02, 1.536748e-02, 3.879072e-03, 9.723478e-04],
     2: [3.209814e-03, 4.066278e-04, 5.111231e-05, 6.408018e-06],
+    3: [1.524938e-04, 9.733075e-06, 6.093175e-07, 3.800562e-08],
+    4: [4.353604e-06, 1.366429e-07, 4.289450e-09, 1.344400e-10],
 }
 
 
-@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
 def test_reference_convergence_laplace_table(order):
-    from feng_amd.api import FN_CONST, FN_QP, NORM_L2_ERROR, Mesh
+    """convergenceLaplace.output, P1 to P4 (P3 / P4: cubic and quartic
+    triangles, AMG-PCG; errors below 1e-9 to 5e-15 absolute, the rounding of
+    the solve in their 7th digit)."""
+    from feng_amd.api import FN_CONST, FN_QP, KSP_CG, NORM_L2_ERROR, PC_AMG, Mesh
     from feng_amd.problems import NLSolverOptions, diffusion_system, solve_newton
     from tests.conftest import GOLDEN
     from tests.test_oracle_time import _qp_table
@@ -179,10 +184,15 @@ def test_reference_convergence_laplace_table(order):
     for lvl in range(1, 5):
         m = Mesh.read_msh(GOLDEN / "meshes" / "data" / f"square{lvl}.msh")
         src = _qp_table(m, 12, lambda x, y: 30.0 * (x ** 4 + y ** 4))
-        S, num = diffusion_system(m, order, 12, _fn((FN_CONST, 1.0, 0.0, None)), _fn((FN_QP, 0.0, 0.0, src)),
+        kind = {1: 1, 2: 2, 3: 5, 4: 6}[order]
+        S, num = diffusion_system(m, kind, 12, _fn((FN_CONST, 1.0, 0.0, None)), _fn((FN_QP, 0.0, 0.0, src)),
                                   u_bc=lambda x, y, z, c: x ** 6 + y ** 6)
-        rep = solve_newton(S, NLSolverOptions(), rtol=1e-13)
+        if order <= 2:
+            rep = solve_newton(S, NLSolverOptions(), rtol=1e-13)
+        else:
+            rep = solve_newton(S, NLSolverOptions(1e-12, 1e-12, 1e4, 20), ksp=KSP_CG, pc=PC_AMG, rtol=1e-13,
+                               max_iter=5000)
         assert rep.converged
         err.append(S.norm(NORM_L2_ERROR, 0, 12, [_fn((FN_QP, 0.0, 0.0, _qp_table(m, 12, lambda x, y: x ** 6 + y ** 6)))]))
         S.close()
-    assert all(abs(e - r) <= 1e-6 * r for e, r in zip(err, LAPLACE_TABLE[order])), err
+    assert all(abs(e - r) <= max(1e-6 * r, 5e-15) for e, r in zip(err, LAPLACE_TABLE[order])), err

@@ -162,3 +162,28 @@ def test_oracle_reproduces_reference_interpolation_rows(oracle, order):
         err.append(oracle.l2_error_qp(kind, m.xyz, m.conn, num.adr[0], u, 12,
                                       _qp_table(m, 12, lambda x, y: x ** 6 + y ** 6)))
     assert all(abs(e - r) <= max(1e-6 * r, 5e-15) for e, r in zip(err, INTERP[order])), err
+
+
+@pytest.mark.parametrize("kind", [5, 6])
+def test_oracle_reproduces_reference_laplace_rows(oracle, kind):
+    """convergenceLaplace.output, P3 and P4 rows (data/square1-4.msh, degree-12
+    rule, -lap u = -30 (x^4 + y^4), u = x^6 + y^6 on the boundary)."""
+    from feng_amd.api import Mesh, Numbering, essential_values
+    from tests.conftest import GOLDEN
+    from tests.test_gpu_diffusion import LAPLACE_TABLE
+    from tests.test_oracle_time import _qp_table
+    err = []
+    for lvl in range(1, 5):
+        m = Mesh.read_msh(GOLDEN / "meshes" / "data" / f"square{lvl}.msh")
+        num = Numbering(m, [kind], [1])
+        n, nl = num.nInc, {5: 10, 6: 15}[kind]
+        ia, ja = oracle.csr(n, [(num.adr[0], nl, num.adr[0], nl)])
+        u = essential_values(num, [lambda x, y, z, c: x ** 6 + y ** 6])
+        src = _qp_table(m, 12, lambda x, y: 30.0 * (x ** 4 + y ** 4))
+        val, rhs = oracle.assemble_diffusion(kind, m.xyz, m.conn, num.adr[0], n, ia, ja, 12, (0, 1.0, 0.0, None),
+                                             (5, 0.0, 0.0, src), u)
+        u[:n] += spla.spsolve(sp.csr_matrix((val, ja, ia), shape=(n, n)).tocsc(), rhs)
+        err.append(oracle.l2_error_qp(kind, m.xyz, m.conn, num.adr[0], u, 12,
+                                      _qp_table(m, 12, lambda x, y: x ** 6 + y ** 6)))
+    ref = LAPLACE_TABLE[DEGREE[kind]]
+    assert all(abs(e - r) <= max(1e-6 * r, 5e-15) for e, r in zip(err, ref)), err
