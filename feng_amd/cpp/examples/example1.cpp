@@ -48,7 +48,12 @@ int main(int argc, char **argv)
     return 1;
   }
   // one field U, essential on the boundary (spaces {uBord, uDomaine})
-  feMetaNumber numbering(*mesh, {order == 2 ? FE_TRI_P2 : FE_TRI_P1}, {1});
+  if(order < 1 || order > 4) {
+    std::fprintf(stderr, "order must be 1 to 4\n");
+    return 1;
+  }
+  const int kinds[4] = {FE_TRI_P1, FE_TRI_P2, FE_TRI_P3, FE_TRI_P4};
+  feMetaNumber numbering(*mesh, {kinds[order - 1]}, {1});
   if(numbering.status()) return 1;
 
   // solution vector: unknowns and essential DOFs (scalarConstant::zero)

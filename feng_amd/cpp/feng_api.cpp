@@ -61,7 +61,13 @@ feMetaNumber::feMetaNumber(const feMesh &mesh, const std::vector<int> &kinds, co
   if(_status) return;
   fe_numbering_info(_h, &_nInc, &_nDofs);
   for(size_t f = 0; f < kinds.size(); ++f) {
-    const int nloc = kinds[f] == FE_TRI_P1 ? 3 : kinds[f] == FE_TRI_P2 ? 6 : kinds[f] == FE_TRI_VP2 ? 12 : 4;
+    const int k = kinds[f];
+    const int nloc = k == FE_TRI_P1 ? 3 : k == FE_TRI_P2 ? 6 : k == FE_TRI_VP2 ? 12 : k == FE_TET_P1 ? 4
+                     : k == FE_TRI_P3 ? 10 : k == FE_TRI_P4 ? 15 : 0;
+    if(nloc == 0) {
+      _status = FE_STATUS_ERROR;
+      return;
+    }
     _adr.emplace_back((size_t)mesh.nElements() * nloc);
     if((_status = fe_numbering_adr(_h, (int)f, _adr.back().data()))) return;
   }

@@ -73,7 +73,8 @@ private:
 };
 
 // Coefficient of a form: one of the device function kinds (FE_FN_*), with a
-// table for FE_FN_ELEM / FE_FN_QP.
+// table for FE_FN_ELEM / FE_FN_QP / FE_FN_POLY_U (the polynomial's
+// coefficients).
 struct feFunction {
   int kind = FE_FN_CONST;
   double c0 = 0., c1 = 0.;

@@ -17,11 +17,18 @@ MESHES = ROOT / "tests" / "golden" / "meshes"
 pytestmark = pytest.mark.gpu
 
 
+def _degree(order):
+    """A rule exact for the stiffness of the element (degree 2 (order - 1));
+    the driver's default 2 under-integrates P3 / P4 into a singular matrix."""
+    return 2 if order <= 2 else 2 * order
+
+
 def _python_driver(mesh_path, order, k=1.0):
-    from feng_amd.api import FN_CONST, TRI_P1, TRI_P2, Fn, Mesh
+    from feng_amd.api import FN_CONST, TRI_P1, TRI_P2, TRI_P3, TRI_P4, Fn, Mesh
     from feng_amd.problems import NLSolverOptions, diffusion_system, solve_newton
     m = Mesh.read_msh(mesh_path)
-    S, num = diffusion_system(m, TRI_P2 if order == 2 else TRI_P1, 2, Fn(FN_CONST, k), Fn(FN_CONST, -1.0))
+    S, num = diffusion_system(m, {1: TRI_P1, 2: TRI_P2, 3: TRI_P3, 4: TRI_P4}[order], _degree(order), Fn(FN_CONST, k),
+                              Fn(FN_CONST, -1.0))
     solve_newton(S, NLSolverOptions(), rtol=1e-12)
     u = S.get_solution()[:num.nInc]
     S.close()
@@ -29,11 +36,12 @@ def _python_driver(mesh_path, order, k=1.0):
 
 
 @pytest.mark.parametrize("mesh", ["square1.msh", "square2.msh"])
-@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
 def test_example1_cpp_matches_python_driver(mesh, order):
     assert EXE.exists(), "build the C++ layer first (python -m feng_amd.build)"
     path = MESHES / mesh
-    out = subprocess.run([str(EXE), str(path), str(order)], capture_output=True, text=True, timeout=120)
+    out = subprocess.run([str(EXE), str(path), str(order), "1", str(_degree(order))], capture_output=True, text=True,
+                         timeout=120)
     assert out.returncode == 0, out.stderr
     r = json.loads(out.stdout.strip().splitlines()[-1])
     u = _python_driver(path, order)
