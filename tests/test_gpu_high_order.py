@@ -170,3 +170,28 @@ def test_reference_interpolation_rows(order):
         err.append(S.norm(NORM_L2_ERROR, 0, 12, [Fn(FN_QP, data=_qp_table(m, 12, lambda x, y: x ** 6 + y ** 6))]))
         S.close()
     assert _matches(err, INTERP[order]), err
+
+
+def test_reference_vector_interpolation_p2_row():
+    """interpolationVector.output, dimension 2, vector P2 row: the device
+    VECTOR_L2_ERROR of the nodal interpolant of (sin xy, cos xy) on
+    data/square1-4.msh (degree-12 rule), on the Taylor-Hood velocity field."""
+    from feng_amd.api import FN_QP, NORM_VECTOR_L2_ERROR, Fn, Mesh
+    from feng_amd.problems import navier_stokes_system
+    from tests.conftest import GOLDEN
+    from tests.test_oracle_time import _qp_table
+    ref = [1.478830e-04, 1.848170e-05, 2.310065e-06, 2.887533e-07]
+    err = []
+    for lvl in range(1, 5):
+        m = Mesh.read_msh(GOLDEN / "meshes" / "data" / f"square{lvl}.msh")
+        S, num = navier_stokes_system(m, 40.0, 12)
+        xyz = num.dof_xyz.reshape(-1, 3)
+        xy = xyz[:, 0] * xyz[:, 1]
+        vel = num.dof_field == 0
+        u = np.zeros(num.nDofs)
+        u[vel] = np.where(num.dof_comp[vel] == 0, np.sin(xy[vel]), np.cos(xy[vel]))
+        S.set_solution(u)
+        err.append(S.norm(NORM_VECTOR_L2_ERROR, 0, 12, [Fn(FN_QP, data=_qp_table(m, 12, lambda x, y: np.sin(x * y))),
+                                                        Fn(FN_QP, data=_qp_table(m, 12, lambda x, y: np.cos(x * y)))]))
+        S.close()
+    assert _matches(err, ref), err
