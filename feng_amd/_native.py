@@ -51,6 +51,7 @@ HOST_API = {
     "fe_mesh_write_msh": (C.c_int, [C.c_void_p, C.c_char_p]),
     "fe_mesh_read_msh": (C.c_int, [C.c_char_p, c_void_pp]),
     "fe_mesh_destroy": (C.c_int, [C.c_void_p]),
+    "fe_mesh_check_jacobians": (C.c_int, [C.c_void_p, c_int_p, c_double_p]),
     "fe_mesh_info": (C.c_int, [C.c_void_p] + [c_int_p] * 7),
     "fe_mesh_get": (C.c_int, [C.c_void_p, c_double_p, c_int_p, c_int_p, c_int_p, c_int_p]),
     "fe_numbering_create": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, c_void_pp]),
@@ -104,6 +105,7 @@ HIP_API = {
     "fe_hip_set_past_solution": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "fe_hip_get_past_solution": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "fe_hip_set_essential": (C.c_int, [C.c_void_p, c_double_p]),
+    "fe_hip_get_system32": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "fe_hip_norm": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, c_double_p]),
     "fe_hip_set_constrained_dofs": (C.c_int, [C.c_void_p, C.c_int, c_int_p]),
     "fe_hip_constrain_essential_components": (C.c_int, [C.c_void_p]),

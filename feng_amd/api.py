@@ -92,6 +92,28 @@ class Mesh:
         _chk(N.host().fe_mesh_read_msh(str(path).encode(), C.byref(h)), "fe_mesh_read_msh")
         return Mesh(h)
 
+    @staticmethod
+    def from_arrays(dim, xyz, conn, bconn=None):
+        """A mesh from vertex coordinates (V x 3), element connectivity
+        (E x nvpe, 0-based) and optional boundary elements."""
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+        nvpe = 3 if dim == 2 else 4
+        conn = np.ascontiguousarray(conn, np.int32).reshape(-1)
+        bnv = 2 if dim == 2 else 3
+        b = np.ascontiguousarray(bconn if bconn is not None else np.zeros(1), np.int32).reshape(-1)
+        nB = 0 if bconn is None else b.size // bnv
+        h = C.c_void_p()
+        _chk(N.host().fe_mesh_from_arrays(dim, xyz.size // 3, N.ptr(xyz, C.c_double), conn.size // nvpe, nvpe,
+                                          N.ptr(conn, C.c_int), nB, bnv, N.ptr(b, C.c_int), C.byref(h)),
+             "fe_mesh_from_arrays")
+        return Mesh(h)
+
+    def check_jacobians(self):
+        """(first element with a Jacobian determinant <= 0 or -1, its determinant)."""
+        e, d = C.c_int(), C.c_double()
+        N.host().fe_mesh_check_jacobians(self.h, C.byref(e), C.byref(d))
+        return e.value, (d.value if e.value >= 0 else None)
+
     def write_msh(self, path):
         _chk(N.host().fe_mesh_write_msh(self.h, str(path).encode()), "fe_mesh_write_msh")
 
@@ -269,6 +291,12 @@ class LinearSystemHIP:
         self._c(self.L.fe_hip_get_system(self.ctx, N.ptr(val, C.c_double), N.ptr(rhs, C.c_double)),
                 "fe_hip_get_system")
         return val, rhs
+
+    def get_system32(self):
+        """The FP32 values kept by an FP32_IR assembly (fe_hip_get_system32)."""
+        val = np.zeros(self.nnz, np.float32)
+        self._c(self.L.fe_hip_get_system32(self.ctx, N.ptr(val, C.c_float)), "fe_hip_get_system32")
+        return val
 
     def set_system(self, val=None, rhs=None):
         v = np.ascontiguousarray(val, np.float64) if val is not None else None

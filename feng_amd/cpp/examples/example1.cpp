@@ -36,7 +36,15 @@ int main(int argc, char **argv)
   const std::string meshFile = argv[1];
   const int order = argc > 2 ? std::atoi(argv[2]) : 1;
   const double k = argc > 3 ? std::atof(argv[3]) : 1.;
-  const int degreeQuadrature = argc > 4 ? std::atoi(argv[4]) : 2;
+  // default rule: degree 2 integrates the P1/P2 stiffness exactly on straight
+  // triangles; cubic and quartic fields need degree 2 (order - 1) at least
+  // (a lower rule leaves the stiffness matrix singular)
+  const int degreeQuadrature = argc > 4 ? std::atoi(argv[4]) : (order >= 3 ? 2 * order : 2);
+  if(degreeQuadrature < 2 * (order - 1)) {
+    std::fprintf(stderr, "example1: quadrature degree %d is below 2 (order - 1) = %d\n", degreeQuadrature,
+                 2 * (order - 1));
+    return 1;
+  }
 
   feMesh *mesh = nullptr;
   if(feMesh::read(meshFile, mesh)) {

@@ -769,6 +769,20 @@ int feAmgSetup(fe_hip_ctx *ctx)
   FeAmg *A = new FeAmg();
   ctx->amg = A;
   A->coarseSweeps = coarseSweeps;
+  ctx->amgHost.assign(H.size(), FeAmgHostLevel());
+  ctx->amgHostSweeps = coarseSweeps;
+  ctx->amgHostMaxAgg = 0;
+  for(const HostLevel &h : H) ctx->amgHostMaxAgg = std::max(ctx->amgHostMaxAgg, h.maxAgg);
+  for(size_t l = 0; l < H.size(); ++l) {
+    FeAmgHostLevel &k = ctx->amgHost[l];
+    k.n = H[l].n;
+    k.nc = H[l].nc;
+    k.ia = H[l].ia;
+    k.ja = H[l].ja;
+    k.agg = H[l].agg;
+    k.cptr = H[l].cptr;
+    k.clist = H[l].clist;
+  }
   A->lv.resize(H.size());
   for(size_t l = 0; l < H.size(); ++l) {
     HostLevel &h = H[l];
@@ -1080,7 +1094,9 @@ int fe_hip_amg_apply(fe_hip_ctx *ctx, const double *r, double *z)
   FE_REQUIRE(ctx, ctx->csrBuilt, "build the CSR first");
   FE_HIP_TRY(ctx, hipSetDevice(ctx->device));
   if(int rc = feAmgSetup(ctx)) return rc;
-  if(int rc = feAmgNumeric(ctx)) return rc;
+  const bool nested = feAmgNestedUsable(ctx);
+  if(!nested)
+    if(int rc = feAmgNumeric(ctx)) return rc;
   const int n = ctx->n;
   double *d = nullptr;
   FE_HIP_TRY(ctx, hipMalloc(&d, sizeof(double) * 2 * (size_t)n));
@@ -1089,7 +1105,9 @@ int fe_hip_amg_apply(fe_hip_ctx *ctx, const double *r, double *z)
     ctx->setError("fe_hip_amg_apply: copy of r failed", __FILE__, __LINE__);
     rc = FE_STATUS_ERROR;
   }
-  if(rc == FE_STATUS_OK) rc = feAmgApply(ctx, (const double *)d, d + n, nullptr, ctx->pcValues32 != 0);
+  if(rc == FE_STATUS_OK)
+    rc = nested ? feAmgPcgApply(ctx, (const double *)d, d + n)
+                : feAmgApply(ctx, (const double *)d, d + n, nullptr, ctx->pcValues32 != 0);
   if(rc == FE_STATUS_OK && (hipGetLastError() != hipSuccess ||
                             hipMemcpyAsync(z, d + n, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
                             hipStreamSynchronize(ctx->stream) != hipSuccess)) {
@@ -1097,7 +1115,7 @@ int fe_hip_amg_apply(fe_hip_ctx *ctx, const double *r, double *z)
     rc = FE_STATUS_ERROR;
   }
   (void)hipFree(d);
-  if(rc == FE_STATUS_OK) rc = feAmgCheck(ctx);
+  if(rc == FE_STATUS_OK && !nested) rc = feAmgCheck(ctx);
   return rc;
 }
 

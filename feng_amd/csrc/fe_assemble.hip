@@ -1134,6 +1134,7 @@ extern "C" int fe_hip_set_to_zero(fe_hip_ctx *ctx, int matrix, int residual)
   FE_REQUIRE(ctx, ctx->csrBuilt, "build the CSR first");
   if(matrix) FE_HIP_TRY(ctx, hipMemsetAsync(ctx->d_val, 0, sizeof(double) * ctx->nnz, ctx->stream));
   if(matrix && ctx->d_val32) FE_HIP_TRY(ctx, hipMemsetAsync(ctx->d_val32, 0, sizeof(float) * ctx->nnz, ctx->stream));
+  if(matrix) ctx->matrixZeroed = true;
   if(residual) FE_HIP_TRY(ctx, hipMemsetAsync(ctx->d_rhs, 0, sizeof(double) * ctx->n, ctx->stream));
   return FE_STATUS_OK;  // stream-ordered before the next assembly
 }
@@ -1157,7 +1158,20 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
   // are fused: one element sweep computes Ae and Be of all of them; a second
   // form of a kind gets a sweep of its own.
   std::vector<bool> done(ctx->forms.size(), false);
-  bool fp32Done = false;
+  // FP32 values (FE_PREC_FP32_IR): the final FP32 rounding pass is skipped
+  // only when the matrix started from zero and every matrix sweep of this
+  // call also wrote the FP32 copy; any sweep that writes FP64 values only
+  // clears allFp32.
+  const bool fromZero = ctx->matrixZeroed;
+  if(what & 1) ctx->matrixZeroed = false;
+  bool allFp32 = true;
+  const bool ir32 = precision == FE_PREC_FP32_IR && (what & 1);
+  auto ensureVal32 = [&]() -> int {
+    if(ctx->d_val32) return FE_STATUS_OK;
+    if(feAlloc(ctx, &ctx->d_val32, (size_t)ctx->nnz)) return FE_STATUS_OUT_OF_MEMORY;
+    FE_HIP_TRY(ctx, hipMemsetAsync(ctx->d_val32, 0, sizeof(float) * ctx->nnz, ctx->stream));
+    return FE_STATUS_OK;
+  };
   auto scalarForm = [](int t) {
     return t == FE_SYS_DIFFUSION || t == FE_SYS_NONLINEAR_DIFFUSION || t == FE_SYS_TRANSIENT_MASS || t == FE_SYS_MASS || t == FE_SYS_ADVECTION ||
            t == FE_SYS_SOURCE;
@@ -1233,7 +1247,36 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
     if(!(what & 2) && iDiff < 0 && iMass < 0 && iReact < 0 && iAdv < 0) continue;  // a lone source has no matrix
     const int kind = ctx->fields[field].kind;
     const bool mfmaKind = kind == FE_TRI_P1 || kind == FE_TRI_P2 || kind == FE_TET_P1;
-    if(iDiff >= 0 && !a.nlDiff && iMass < 0 && iReact < 0 && iAdv < 0 && ctx->asmKernel == 1 && mfmaKind) {
+    const bool matSweep = (what & 1) && (iDiff >= 0 || iMass >= 0 || iReact >= 0 || iAdv >= 0);
+    if(ctx->asmKernel == 2 && mfmaKind && !a.nlDiff && iReact < 0 && iAdv < 0) {
+      bool ok = false;
+      if(int rc = feGatherPrepare(ctx, field, &ok)) return rc;
+      if(ok) {
+        FeGatherSweep gs{};
+        gs.field = field;
+        gs.what = what;
+        gs.nq = ctx->forms[fi].tab.nq;
+        gs.tab = ctx->forms[fi].d_tab;
+        gs.hasDiff = a.hasDiff;
+        gs.hasSrc = a.hasSrc;
+        gs.hasMass = a.hasMass;
+        gs.coeff = a.coeff;
+        gs.src = a.src;
+        gs.mass = a.mass;
+        gs.val32 = nullptr;
+        gs.fp32Math = false;
+        if(ir32 && matSweep) {
+          if(int rc = ensureVal32()) return rc;
+          gs.val32 = ctx->d_val32;
+          gs.fp32Math = ctx->asmFp32Math != 0;
+        }
+        if(int rc = feGatherLaunch(ctx, gs)) return rc;
+        continue;
+      }
+    }
+    const bool allBefore = allFp32;
+    if(matSweep) allFp32 = false;  // re-set below when the MFMA pair writes both copies
+    if(iDiff >= 0 && !a.nlDiff && iMass < 0 && iReact < 0 && iAdv < 0 && ctx->asmKernel >= 1 && mfmaKind) {
       const FeForm &F = ctx->forms[iDiff];
       MfmaAsmArgs m{};
       m.E = a.E; m.n = a.n; m.what = what; m.K = F.mfmaK;
@@ -1245,17 +1288,14 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
       else if(kind == FE_TRI_P2) launchMfma<double, 6, 2>(ctx, m);
       else if(kind == FE_TET_P1) launchMfma<double, 4, 3>(ctx, m);
       if(precision == FE_PREC_FP32_IR && (what & 1) && ctx->fp32Values == 1) {
-        if(!ctx->d_val32) {
-          if(feAlloc(ctx, &ctx->d_val32, (size_t)ctx->nnz)) return FE_STATUS_OUT_OF_MEMORY;
-          FE_HIP_TRY(ctx, hipMemsetAsync(ctx->d_val32, 0, sizeof(float) * ctx->nnz, ctx->stream));
-        }
+        if(int rc = ensureVal32()) return rc;
         m.what = 1;
         m.sref = F.d_sref32;
         m.val = ctx->d_val32;
         if(kind == FE_TRI_P1) launchMfma<float, 3, 2>(ctx, m);
         else if(kind == FE_TRI_P2) launchMfma<float, 6, 2>(ctx, m);
         else if(kind == FE_TET_P1) launchMfma<float, 4, 3>(ctx, m);
-        fp32Done = true;
+        allFp32 = allBefore;  // this sweep wrote both copies
       }
     } else if(kind == FE_TRI_P1) launchScalar<3, 2>(ctx, a);
     else if(kind == FE_TRI_P2) launchScalar<6, 2>(ctx, a);
@@ -1312,6 +1352,7 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
       a.cGG = ctx->forms[iG].coeffs[0];
     }
     const int kind = ctx->fields[F0.fields[0]].kind;
+    if(what & 1) allFp32 = false;
     if(kind == FE_TRI_P1) launchMixed<3, 2>(ctx, a);
     else if(kind == FE_TRI_P2) launchMixed<6, 2>(ctx, a);
     else if(kind == FE_TET_P1) launchMixed<4, 3>(ctx, a);
@@ -1407,6 +1448,7 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
       a.u = ctx->d_u;
       a.val = ctx->d_val;
       a.rhs = ctx->d_rhs;
+      if(what & 1) allFp32 = false;
       for(size_t c = 0; c + 1 < ctx->colorStart.size(); ++c) {
         a.begin = ctx->colorStart[c];
         a.count = ctx->colorStart[c + 1] - a.begin;
@@ -1419,7 +1461,7 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
   for(size_t fi = 0; fi < ctx->forms.size(); ++fi)
     FE_REQUIRE(ctx, done[fi], "form type has no device kernel");
   FE_HIP_TRY(ctx, hipGetLastError());
-  if(precision == FE_PREC_FP32_IR && (what & 1) && !fp32Done) {
+  if(ir32 && !(fromZero && allFp32 && ctx->d_val32)) {
     if(!ctx->d_val32 && feAlloc(ctx, &ctx->d_val32, (size_t)ctx->nnz)) return FE_STATUS_OUT_OF_MEMORY;
     hipLaunchKernelGGL(toFloat, dim3(2048), dim3(256), 0, ctx->stream, ctx->nnz, ctx->d_val, ctx->d_val32);
   }

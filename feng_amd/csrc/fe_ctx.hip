@@ -1,4 +1,5 @@
 // Context lifetime, uploads, fields, forms and solution I/O of libfe_hip.so.
+#include "fe_jacobian_check.h"
 #include "fe_hip_internal.h"
 #include "fe_lagrange.h"
 
@@ -174,6 +175,19 @@ int feBuildRefTables(int kind, int degree, FeRefTablesHO *T)
   return q.n;
 }
 
+void feLowerRefTables(const FeRefTablesHO &t, FeRefTables *lo)
+{
+  memset(lo, 0, sizeof(*lo));
+  lo->nq = t.nq; lo->nf = t.nf; lo->dim = t.dim; lo->nv = t.nv;
+  for(int k = 0; k < kMaxQuad; ++k) {
+    lo->w[k] = t.w[k];
+    for(int i = 0; i < 6; ++i) {
+      lo->L[k][i] = t.L[k][i]; lo->dLr[k][i] = t.dLr[k][i]; lo->dLs[k][i] = t.dLs[k][i]; lo->dLt[k][i] = t.dLt[k][i];
+    }
+    for(int i = 0; i < 4; ++i) lo->Lg[k][i] = t.Lg[k][i];
+  }
+}
+
 int feUploadRefTables(fe_hip_ctx *ctx, const FeRefTablesHO &t, double **d)
 {
   if(t.nf > 6) {
@@ -182,15 +196,7 @@ int feUploadRefTables(fe_hip_ctx *ctx, const FeRefTablesHO &t, double **d)
     return FE_STATUS_OK;
   }
   FeRefTables lo;
-  memset(&lo, 0, sizeof(lo));
-  lo.nq = t.nq; lo.nf = t.nf; lo.dim = t.dim; lo.nv = t.nv;
-  for(int k = 0; k < kMaxQuad; ++k) {
-    lo.w[k] = t.w[k];
-    for(int i = 0; i < 6; ++i) {
-      lo.L[k][i] = t.L[k][i]; lo.dLr[k][i] = t.dLr[k][i]; lo.dLs[k][i] = t.dLs[k][i]; lo.dLt[k][i] = t.dLt[k][i];
-    }
-    for(int i = 0; i < 4; ++i) lo.Lg[k][i] = t.Lg[k][i];
-  }
+  feLowerRefTables(t, &lo);
   if(feAlloc(ctx, d, sizeof(FeRefTables) / sizeof(double) + 1)) return FE_STATUS_OUT_OF_MEMORY;
   FE_HIP_TRY(ctx, hipMemcpy(*d, &lo, sizeof(FeRefTables), hipMemcpyHostToDevice));
   return FE_STATUS_OK;
@@ -261,6 +267,7 @@ int fe_hip_ctx_destroy(fe_hip_ctx *ctx)
   feMpnnRelease(ctx);
   feBatchRelease(ctx);
   feAmgRelease(ctx);
+  feAmgPcgRelease(ctx);
   for(void *p : ctx->owned) (void)hipFree(p);
   for(auto &g : ctx->graphs) (void)hipGraphExecDestroy((hipGraphExec_t)g.second);
   ctx->graphs.clear();
@@ -280,6 +287,12 @@ int fe_hip_upload_mesh(fe_hip_ctx *ctx, int dim, int V, const double *xyz, int E
   FE_REQUIRE(ctx, V > 0 && E > 0, "empty mesh");
   for(size_t i = 0; i < (size_t)E * nvpe; ++i)
     FE_REQUIRE(ctx, conn[i] >= 0 && conn[i] < V, "connectivity out of range");
+  {
+    double det = 0.;
+    const int bad = feFirstNonPositiveJacobian(dim, xyz, E, nvpe, conn, &det);
+    FE_REQUIRE(ctx, bad < 0, "Negative or zero jacobian = " + std::to_string(det) + " on element " +
+                                 std::to_string(bad) + " (feCncGeo::computeJacobians)");
+  }
   FE_HIP_TRY(ctx, hipSetDevice(ctx->device));
   ctx->dim = dim; ctx->V = V; ctx->E = E; ctx->nvpe = nvpe;
   if(feAlloc(ctx, &ctx->d_xyz, (size_t)3 * V) || feAlloc(ctx, &ctx->d_conn, (size_t)E * nvpe))
@@ -645,6 +658,16 @@ int fe_hip_get_system(fe_hip_ctx *ctx, double *val, double *rhs)
   return FE_STATUS_OK;
 }
 
+int fe_hip_get_system32(fe_hip_ctx *ctx, float *val32)
+{
+  if(!ctx || !val32) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, ctx->csrBuilt, "build the CSR first");
+  FE_REQUIRE(ctx, ctx->d_val32, "no FP32 values: assemble the matrix with FE_PREC_FP32_IR first");
+  FE_HIP_TRY(ctx, hipMemcpyAsync(val32, ctx->d_val32, sizeof(float) * ctx->nnz, hipMemcpyDeviceToHost, ctx->stream));
+  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return FE_STATUS_OK;
+}
+
 int fe_hip_set_system(fe_hip_ctx *ctx, const double *val, const double *rhs)
 {
   if(!ctx) return FE_STATUS_POINTER_ERROR;
@@ -674,8 +697,17 @@ int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value)
     FE_REQUIRE(ctx, value == 0 || value == 1, "scalar_lanes must be 0 or 1");
     ctx->scalarLanes = value;
   } else if(k == "asm_kernel") {
-    FE_REQUIRE(ctx, value == 0 || value == 1, "asm_kernel must be 0 (scalar) or 1 (MFMA)");
+    FE_REQUIRE(ctx, value >= 0 && value <= 2, "asm_kernel must be 0 (colored scalar), 1 (colored MFMA) or 2 (gather)");
     ctx->asmKernel = value;
+  } else if(k == "amg_kernel") {
+    FE_REQUIRE(ctx, value == 1 || value == 2, "amg_kernel must be 1 (per-level kernels) or 2 (nested-ordered PCG)");
+    ctx->amgKernel = value;
+  } else if(k == "gather_xcd") {
+    FE_REQUIRE(ctx, value == 0 || value == 1, "gather_xcd must be 0 or 1");
+    ctx->gatherXcd = value;
+  } else if(k == "asm_fp32_math") {
+    FE_REQUIRE(ctx, value == 0 || value == 1, "asm_fp32_math must be 0 (rounded FP64 rows) or 1 (FP32 rows)");
+    ctx->asmFp32Math = value;
   } else if(k == "fp32_values") {
     FE_REQUIRE(ctx, value == 0 || value == 1, "fp32_values must be 0 (rounded FP64) or 1 (FP32 MFMA)");
     ctx->fp32Values = value;
@@ -765,6 +797,9 @@ int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value)
   else if(k == "iter_block") *value = ctx->iterBlock;
   else if(k == "iter_grid") *value = ctx->iterGrid;
   else if(k == "asm_kernel") *value = ctx->asmKernel;
+  else if(k == "gather_xcd") *value = ctx->gatherXcd;
+  else if(k == "amg_kernel") *value = ctx->amgKernel;
+  else if(k == "asm_fp32_math") *value = ctx->asmFp32Math;
   else if(k == "scalar_lanes") *value = ctx->scalarLanes;
   else if(k == "fp32_values") *value = ctx->fp32Values;
   else if(k == "pipe_kernel") *value = ctx->pipeKernel;

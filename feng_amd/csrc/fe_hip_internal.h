@@ -59,6 +59,8 @@ int feBuildRefTables(int kind, int degree, FeRefTablesHO *t);  // host, returns 
 // Device copy of host tables in the layout their kernels load (FeRefTables
 // for nf <= 6, FeRefTablesHO otherwise); *d is owned by the context.
 int feUploadRefTables(fe_hip_ctx *ctx, const FeRefTablesHO &t, double **d);
+// The FeRefTables (nf <= 6) form of host tables, as feUploadRefTables uploads it.
+void feLowerRefTables(const FeRefTablesHO &t, FeRefTables *lo);
 
 struct FeFn {
   int kind;
@@ -101,6 +103,27 @@ struct FeGroup {
   int *d_pos = nullptr;   // (nI*nJ) * E, slot-major, -1 when filtered
 };
 
+// Row-gather assembly of the fused scalar forms (fe_gather.hip): per field,
+// the incidences (element << 4 | local index) of every row in color-sorted
+// element order, slot-major per 64-row slice.
+constexpr int kGatherMaxRow = 30;  // longest CSR row the gather kernel accumulates (5-bit slots)
+struct FeGatherField {
+  bool built = false, usable = false;
+  int nSlices = 0, maxR = 0;
+  int64_t incBytes = 0;
+  int *d_sliceOff = nullptr;
+  int2 *d_inc = nullptr;
+};
+// One fused scalar sweep handed to the gather kernel.
+struct FeGatherSweep {
+  int field, what, nq;
+  const double *tab;
+  int hasDiff, hasSrc, hasMass;
+  FeFn coeff, src, mass;
+  float *val32;   // non-null: write the FP32 rounding of the updated values
+  bool fp32Math;  // element rows computed and summed in FP32
+};
+
 struct FeKernelTimer {
   std::vector<hipEvent_t> ev;  // pairs
   size_t used = 0;
@@ -130,6 +153,16 @@ struct FePrecond {
 
 struct FeMpnn;   // learned-preconditioner state (fe_learned.hip)
 struct FeAmg;    // aggregation multigrid hierarchy (fe_amg.hip)
+struct FeAmgPcg;  // nested-ordered multigrid PCG (fe_amg_pcg.hip)
+
+// Host copy of the aggregation hierarchy of the CSR pattern (built by
+// feAmgSetup): per level the CSR pattern, the aggregate of each row (empty
+// on the coarsest) and, for levels >= 1, the entries of the previous level
+// (positions in its CSR) that sum into each entry (cptr / clist).
+struct FeAmgHostLevel {
+  int n = 0, nc = 0;
+  std::vector<int> ia, ja, agg, cptr, clist;
+};
 struct FeBatch;  // batched-solver state (fe_batch.hip)
 
 struct fe_hip_ctx {
@@ -161,6 +194,7 @@ struct fe_hip_ctx {
   float *d_val32 = nullptr;
   double *d_rhs = nullptr, *d_u = nullptr, *d_du = nullptr;
   bool csrBuilt = false;
+  bool matrixZeroed = false;  // values (and the FP32 copy) zeroed, no matrix assembled since
   // Time integration: u_dot = sum_{j < nsol} bdf[j] sol[j] with sol[0] = d_u
   // (the current iterate) and sol[j] = d_past[j - 1] (nDofs each, allocated
   // on first use); TRANSIENT_MASS matrices carry c0 = bdf[0].
@@ -187,7 +221,12 @@ struct fe_hip_ctx {
   int iterBlock = 512;
   int iterGrid = 1024;
   bool lanesFixed = false;
-  int asmKernel = 1;     // 0: scalar FP64 loop (reference order), 1: MFMA GEMM form
+  int asmKernel = 2;     // 0: colored scalar FP64 loop (reference order), 1: colored MFMA GEMM form,
+                         // 2: row gather (one launch, colored summation order) where the sweep allows it
+  int gatherXcd = 1;     // XCD-aware slice order in the gather assembly
+  int asmFp32Math = 0;   // FE_PREC_FP32_IR gather: 0 = FP32 values rounded from the FP64 rows,
+                         // 1 = element rows computed and summed in FP32
+  std::map<int, FeGatherField> gather;
   int scalarLanes = 0;   // scalar sweeps: 1 = one lane per local row (measured no faster), 0 = one thread per element
   int fp32Values = 1;    // FP32 values: 0 = rounded FP64 values, 1 = FP32 MFMA element matrices
   int pipeKernel = 2;  // pipelined PCG: 0 Chronopoulos-Gear, 1 tiled Ghysels-Vanroose, 2 by precision
@@ -205,6 +244,12 @@ struct fe_hip_ctx {
   int amgPadTiles = 1;  // restriction over aggregate-aligned tiles (0: tile-header kernel)
   int batchAmgGlobal = 0;  // batched AMG: 0 per-problem CG scalars, 1 one CG over the union system
   FeAmg *amg = nullptr;
+  std::vector<FeAmgHostLevel> amgHost;  // the hierarchy's pattern on the host (empty before the first AMG solve)
+  int amgHostSweeps = 0;                // stalled coarsening: sweeps on the coarsest level
+  int amgHostMaxAgg = 0;                // largest aggregate (rows) of any level
+  FeAmgPcg *amgPcg = nullptr;
+  int amgKernel = 2;  // FE_PC_AMG: 1 = per-level kernels over the original numbering,
+                      // 2 = nested-ordered fused PCG (fe_amg_pcg.hip)
   bool amgActive = false;  // the current CG solve applies the AMG V-cycle
   FeMpnn *mpnn = nullptr;
   FeBatch *batch = nullptr;    // Jacobi-PCG: 0 = on-the-fly D^-1 r gather, 1 = tiled (one gathered vector)
@@ -266,6 +311,12 @@ void feFree(fe_hip_ctx *ctx, void *p);
 // Returns the total through *total (host), or an error when it overflows int32.
 int feExclusiveScan(fe_hip_ctx *ctx, const int *d_in, int *d_out, int n, int64_t *total);
 
+// Gather assembly (fe_gather.hip): incidences of a field (built once per
+// pattern; *ok = false when the field or pattern does not fit the kernel),
+// and one fused scalar sweep.
+int feGatherPrepare(fe_hip_ctx *ctx, int field, bool *ok);
+int feGatherLaunch(fe_hip_ctx *ctx, const FeGatherSweep &s);
+
 // Preconditioner set-up (fe_precond.hip)
 int fePrecondSetupPattern(fe_hip_ctx *ctx);
 // sweeps = 0: exact level-scheduled ILU(0); > 0: parallel IC(0) sweeps.
@@ -289,6 +340,28 @@ int feAmgLevels(const fe_hip_ctx *ctx, int *levels, int *rows, int64_t *nnz, int
 // (0 f64/f64, 1 f64/f32, 2 f32/f32): part of the launch sequence a graph captures.
 int feAmgPlanFrom(const fe_hip_ctx *ctx, int slot);
 void feAmgRelease(fe_hip_ctx *ctx);
+// Nested-ordered multigrid PCG (fe_amg_pcg.hip): the FE_PC_AMG hierarchy with
+// the rows of every level ordered so that each aggregate is a contiguous
+// range, and standard PCG whose reductions are folded into the next kernel.
+// Solves A dx = r with FP64 refinement over inner solves of vector type
+// float (precision FE_PREC_FP32_IR) or double; dx accumulates into x64.
+struct FeAmgPcgResult {
+  int iters = 0, outer = 0, converged = 0, reason = 0, breakdowns = 0;
+  double relres = 0.;
+};
+// FP64 residual of the system: r = rhs - A x (original CSR), *out = ||r||_2^2
+// (stream synchronized).
+int feResidualNorm2(fe_hip_ctx *ctx, const double *x, double *r, double *out);
+int feAmgPcgSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double bnorm, double target, double *x64,
+                  double *r64, FeAmgPcgResult *res);
+int feAmgPcgLevels(const fe_hip_ctx *ctx, int *levels, int *rows, int64_t *nnz, int cap);
+// The nested-ordered PCG takes the hierarchy: two levels at least, a dense
+// coarsest level, aggregates that fit a tile (after feAmgSetup).
+bool feAmgNestedUsable(const fe_hip_ctx *ctx);
+// One V-cycle z = M^-1 r through the nested-ordered kernels (device arrays of
+// n doubles, original numbering), FP64 vectors, values per pcValues32.
+int feAmgPcgApply(fe_hip_ctx *ctx, const double *r, double *z);
+void feAmgPcgRelease(fe_hip_ctx *ctx);
 void feBatchRelease(fe_hip_ctx *ctx);
 
 // GMRES (fe_gmres.hip)

@@ -1692,6 +1692,13 @@ __global__ void castResidual(int n, const double *__restrict__ a, T *__restrict_
 
 }  // namespace
 
+int feResidualNorm2(fe_hip_ctx *ctx, const double *x, double *r, double *out)
+{
+  Workspace w;
+  if(int rc = ensureWorkspace(ctx, w)) return rc;
+  return residualNorm2(ctx, w, x, ctx->d_rhs, r, out);
+}
+
 int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_solve_stats *stats)
 {
   if(o->ksp == FE_KSP_GMRES) return feSolveGmres(ctx, o, du, stats);
@@ -1736,7 +1743,9 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
     FE_HIP_TRY(ctx, hipEventRecord(s0, ctx->stream));
     if(amg) {
       if(int rc = feAmgSetup(ctx)) return rc;
-      if(int rc = feAmgNumeric(ctx)) return rc;
+      // the nested-ordered PCG runs its own numeric set-up inside the solve
+      if(!feAmgNestedUsable(ctx))
+        if(int rc = feAmgNumeric(ctx)) return rc;
     } else {
       if(int rc = fePrecondFactor(ctx, o->pc == FE_PC_LEARNED ? kLearnedIcSweeps : ctx->icSweeps)) return rc;
       if(o->pc == FE_PC_LEARNED)
@@ -1756,11 +1765,25 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
     w.s64.iaU = w.s32.iaU = P.d_iaU; w.s64.jaU = w.s32.jaU = P.d_jaU;
   }
 
+  const bool nested = amg && feAmgNestedUsable(ctx);
   if(bn == 0.) {
     FE_HIP_TRY(ctx, hipMemsetAsync(ctx->d_du, 0, sizeof(double) * n, ctx->stream));
     converged = 1;
     reason = 2;
     relres = 0.;
+  } else if(nested) {
+    // multigrid PCG over the nested ordering, FP32 or FP64 inner solves under
+    // FP64 refinement (fe_amg_pcg.hip)
+    FE_HIP_TRY(ctx, hipMemsetAsync(w.x64, 0, sizeof(double) * n, ctx->stream));
+    FE_HIP_TRY(ctx, hipMemcpyAsync(w.r64, ctx->d_rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    FeAmgPcgResult R;
+    if(int rc = feAmgPcgSolve(ctx, o, bn, target, w.x64, w.r64, &R)) return rc;
+    iters = R.iters;
+    outer = R.outer;
+    converged = R.converged;
+    reason = R.reason;
+    relres = R.relres;
+    FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_du, w.x64, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
   } else if(split) {
     // split-preconditioned PCG: FP64 directly, or FP32 inner + FP64 refinement
     const bool fp32 = (o->precision == FE_PREC_FP32_IR);

@@ -1,6 +1,7 @@
 // libfe_host.so: meshes, edges, DOF numbering, coloring and synthetic
 // coefficient fields for the GPU assemble-and-solve path (C ABI in
 // include/fe_mesh.h).  Pure host C++17.
+#include "../fe_jacobian_check.h"
 #include "fe_mesh.h"
 #include "../fe_lagrange.h"
 
@@ -527,6 +528,14 @@ int fe_mesh_get(const fe_mesh *m, double *xyz, int *conn, int *bconn, int *edges
 //                   DOFs in boundary element order (the 1D element DOFs of the
 //                   boundary space, feNumber.cpp:452-466), leftover essential
 //                   edge DOFs.
+int fe_mesh_check_jacobians(const fe_mesh *m, int *elem, double *det)
+{
+  if(!m) return FE_STATUS_POINTER_ERROR;
+  const int e = feFirstNonPositiveJacobian(m->dim, m->xyz.data(), m->E, m->nvpe, m->conn.data(), det);
+  if(elem) *elem = e;
+  return e < 0 ? FE_STATUS_OK : FE_STATUS_ERROR;
+}
+
 int fe_numbering_create(const fe_mesh *m, int nfields, const int *kinds, const int *essential,
                         fe_numbering **out)
 {
@@ -537,6 +546,7 @@ int fe_numbering_create(const fe_mesh *m, int nfields, const int *kinds, const i
       return FE_STATUS_FORMAT_ERROR;
     if(m->dim == 3 && k != FE_TET_P1) return FE_STATUS_FORMAT_ERROR;
   }
+  if(fe_mesh_check_jacobians(m, nullptr, nullptr) != FE_STATUS_OK) return FE_STATUS_ERROR;
   fe_numbering *num = new(std::nothrow) fe_numbering;
   if(!num) return FE_STATUS_OUT_OF_MEMORY;
   num->nfields = nfields;

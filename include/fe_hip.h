@@ -158,12 +158,17 @@ int fe_hip_get_solution(fe_hip_ctx *ctx, double *u);
 int fe_hip_set_to_zero(fe_hip_ctx *ctx, int matrix, int residual);
 
 /* assembleMatrices / assembleResiduals / assemble (feLinearSystemPETSc.cpp:
- * 543-805): every form's element kernel on every element, one colored sweep
- * per color, adding into the CSR values and the RHS.  what: 1 = matrix,
- * 2 = residual, 3 = both (fused: one kernel per form and color computes Ae
- * and Be together).  precision: FE_PREC_FP64 computes in FP64;
- * FE_PREC_FP32_IR additionally keeps an FP32 copy of the values for the
- * inner solve. */
+ * 543-805): every form's element kernel on every element, adding into the
+ * CSR values and the RHS in the summation order of the reference's colored
+ * element loop.  Fused scalar sweeps (DIFFUSION, TRANSIENT_MASS, SOURCE) of
+ * P1/P2 triangles and P1 tets run as one row-gather launch (each CSR row
+ * sums its incident elements' rows in color order; tuning key asm_kernel =
+ * 2, the default); the other forms as one colored, atomic-free sweep per
+ * color.  what: 1 = matrix, 2 = residual, 3 = both (fused: Ae and Be
+ * together).  precision: FE_PREC_FP64 computes in FP64; FE_PREC_FP32_IR
+ * additionally keeps an FP32 copy of the values for the inner solve (the
+ * FP32 rounding of the FP64 values, or with asm_fp32_math = 1 element rows
+ * computed and summed in FP32). */
 int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision);
 
 /* getRHSMaxNorm (feLinearSystem.h:104). */
@@ -171,6 +176,10 @@ int fe_hip_rhs_max_norm(fe_hip_ctx *ctx, double *norm);
 
 /* Copy back the assembled system: val[nnz] (may be NULL), rhs[n] (may be NULL). */
 int fe_hip_get_system(fe_hip_ctx *ctx, double *val, double *rhs);
+/* Copy back the FP32 values val32[nnz] that an FE_PREC_FP32_IR assembly
+ * keeps for the inner solve (no reference counterpart: the reference
+ * assembles in FP64 only).  FE_STATUS_ERROR before any FP32 assembly. */
+int fe_hip_get_system32(fe_hip_ctx *ctx, float *val32);
 /* Overwrite the system values / RHS from the host (used for externally
  * assembled or perturbed systems). */
 int fe_hip_set_system(fe_hip_ctx *ctx, const double *val, const double *rhs);
@@ -313,6 +322,7 @@ int fe_hip_set_essential(fe_hip_ctx *ctx, const double *values);
 #define FE_NORM_INTEGRAL_F 25               /* integral of f(x) */
 int fe_hip_norm(fe_hip_ctx *ctx, int type, int field, int quad_degree, int nexact, const fe_hip_fn *exact,
                 double *value);
+
 
 /* ---- aggregation multigrid (FE_PC_AMG) ----------------------------------
  * The algebraic multigrid the reference reaches through PETSc's runtime

@@ -17,6 +17,11 @@
 extern "C" {
 #endif
 
+/* Status codes: the values of the reference's feStatus (feMessage.h:44-62).
+ * A translation unit that includes the reference's headers first gets the
+ * reference's enumerators (same values) and no macros; include these headers
+ * after the reference's, since the macros would rename its enumerators. */
+#ifndef _FEMESSAGE_
 #define FE_STATUS_OK 0
 #define FE_STATUS_ERROR (-1)
 #define FE_STATUS_FAILED (-2)
@@ -28,6 +33,7 @@ extern "C" {
 #define FE_STATUS_WRITE_ERROR (-8)
 #define FE_STATUS_RANGE_ERROR (-9)
 #define FE_STATUS_FORMAT_ERROR (-10)
+#endif
 
 /* Element families: the local layouts of feSpaceTriP1 (feSpace_2D.cpp:154-159),
  * feSpaceTriP2 (:772-786), feSpaceVecTriP2<2> (:1017-1041), feSpaceTetPn
@@ -95,6 +101,13 @@ int fe_mesh_get(const fe_mesh *m, double *xyz, int *conn, int *bconn, int *edges
  * vector). */
 int fe_numbering_create(const fe_mesh *m, int nfields, const int *kinds, const int *essential,
                         fe_numbering **out);
+/* Orientation check the reference runs when it builds a space on a mesh
+ * (feCncGeo::computeJacobians, feCncGeo.cpp:346-353, :388-395, :410-412):
+ * FE_STATUS_ERROR when any element has a Jacobian determinant <= 0, with the
+ * first such element and its determinant in *elem / *det (either may be
+ * NULL); FE_STATUS_OK and *elem = -1 otherwise.  fe_numbering_create and
+ * fe_hip_upload_mesh refuse such meshes the same way. */
+int fe_mesh_check_jacobians(const fe_mesh *m, int *elem, double *det);
 int fe_numbering_destroy(fe_numbering *num);
 /* nInc (unknowns, the system size) and nDofs (unknowns + essential). */
 int fe_numbering_info(const fe_numbering *num, int *nInc, int *nDofs);

@@ -47,12 +47,6 @@ def coeffs_of(g):
     raise ValueError(p)
 
 
-def boundary_edges(edges, bconn):
-    """Edge index of every boundary segment (matching endpoints)."""
-    conn_pairs = {}
-    return conn_pairs
-
-
 def bedge_from(conn, edges, bconn):
     E = conn.size // 3
     key = {}

@@ -47,11 +47,15 @@ def _oracle_levels(S):
 
 
 @pytest.mark.parametrize("case", ["tet_grf_12", "p2_blocks_40", "p1_square_9"])
-@pytest.mark.parametrize("variant", ["default", "header_tiles_big_block"])
+@pytest.mark.parametrize("variant", ["nested", "per_level", "header_tiles_big_block"])
 def test_vcycle_matches_oracle(case, variant):
+    """nested: the nested-ordered kernels (fe_amg_pcg.hip, the default);
+    per_level: the per-level kernels over the original numbering."""
     print("case", case, variant, flush=True)
     m, S, num = _system(case)
-    if variant != "default":
+    if variant != "nested":
+        S.set_tuning("amg_kernel", 1)
+    if variant == "header_tiles_big_block":
         # restriction through tile headers (the kernel for aggregates above
         # 256 rows) and larger levels in the single-workgroup kernel, whose
         # matrices it then reads from global memory
@@ -154,4 +158,41 @@ def test_vcycle_on_a_block_diagonal_batch_matches_oracle():
     rows, nnz = S.amg_levels()
     assert rows == [L.A.shape[0] for L in lv] and nnz == [L.A.nnz for L in lv]
     assert nrel(z, amg.vcycle(lv, r)) <= 1e-11
+    S.close()
+
+
+@pytest.mark.parametrize("case", ["tet_grf_16", "p2_blocks_64"])
+def test_nested_pcg_matches_per_level_pcg(oracle, case):
+    """The nested-ordered PCG (standard CG, reductions folded into the next
+    kernel) and the per-level pipelined PCG apply the same V-cycle: both reach
+    the FP64 target, with iteration counts within 20% of each other, in FP64
+    and with FP32 inner solves under FP64 refinement."""
+    from feng_amd.api import FN_CONST, FN_ELEM, PC_AMG, PREC_FP32_IR, Fn, Mesh
+    from feng_amd.problems import diffusion_system
+    print("nested vs per-level", case, flush=True)
+    if case == "tet_grf_16":
+        m = Mesh.cube(16)
+        k, kind, deg = Fn(FN_ELEM, data=m.coeff_grf(0.1, 1.5, 256, 2)), 4, 1
+    else:
+        m = Mesh.square(64)
+        k, kind, deg = Fn(FN_ELEM, data=m.coeff_blocks(16, 1e-2, 1e2, 9)), 2, 2
+    S, num = diffusion_system(m, kind, deg, k, Fn(FN_CONST, -1.0))
+    S.assemble(3, PREC_FP32_IR)
+    ia, ja = S.get_csr()
+    val, rhs = S.get_system()
+    out = {}
+    for kern in (2, 1):
+        S.set_tuning("amg_kernel", kern)
+        for prec in (0, PREC_FP32_IR):
+            r, du = S.solve(pc=PC_AMG, precision=prec, rtol=1e-8, inner_rtol=1e-4, want_du=True)
+            assert r.converged and r.relres <= 1e-8, (kern, prec, r)
+            res = rhs - oracle.spmv(ia, ja, val, du)
+            assert np.linalg.norm(res) <= 1.01e-8 * np.linalg.norm(rhs), (kern, prec)
+            out[(kern, prec)] = (r.iterations, du)
+    for prec in (0, PREC_FP32_IR):
+        it2, it1 = out[(2, prec)][0], out[(1, prec)][0]
+        print(case, "precision", prec, "iterations nested / per-level:", it2, it1, flush=True)
+        assert abs(it2 - it1) <= max(2, 0.2 * it1), (prec, it2, it1)
+        d2, d1 = out[(2, prec)][1], out[(1, prec)][1]
+        assert np.abs(d2 - d1).max() <= 1e-5 * np.abs(d1).max()
     S.close()

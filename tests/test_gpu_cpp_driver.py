@@ -60,3 +60,17 @@ def test_example1_error_falls_with_refinement_and_degree():
             errs[(mesh, order)] = json.loads(out.stdout.strip().splitlines()[-1])["max_nodal_error"]
     assert errs[("square2.msh", 1)] < errs[("square1.msh", 1)]
     assert errs[("square1.msh", 2)] < errs[("square1.msh", 1)]
+
+
+def test_example1_default_rule_of_cubic_fields_and_low_rule_refused():
+    """Without a degree argument the cubic and quartic runs take degree
+    2 order (a degree-2 rule would leave their stiffness singular); a rule
+    below 2 (order - 1) is refused before any solve."""
+    path = MESHES / "square1.msh"
+    out = subprocess.run([str(EXE), str(path), "3"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    u = _python_driver(path, 3)
+    assert abs(r["sum_u"] - u.sum()) <= 1e-10 * abs(u.sum())
+    bad = subprocess.run([str(EXE), str(path), "4", "1", "2"], capture_output=True, text=True, timeout=120)
+    assert bad.returncode != 0 and "below" in bad.stderr

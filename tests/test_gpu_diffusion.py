@@ -196,3 +196,20 @@ def test_reference_convergence_laplace_table(order):
         err.append(S.norm(NORM_L2_ERROR, 0, 12, [_fn((FN_QP, 0.0, 0.0, _qp_table(m, 12, lambda x, y: x ** 6 + y ** 6)))]))
         S.close()
     assert all(abs(e - r) <= max(1e-6 * r, 5e-15) for e, r in zip(err, LAPLACE_TABLE[order])), err
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_upload_refuses_negative_jacobian(dim):
+    """fe_hip_upload_mesh returns FE_STATUS_ERROR on a reversed element, as
+    feCncGeo::computeJacobians does (feCncGeo.cpp:346-353, :388-395, :410-412);
+    nothing is uploaded or launched."""
+    from feng_amd.api import FeError, LinearSystemHIP, Mesh
+    print("negative jacobian upload", dim, flush=True)
+    good = Mesh.square(4) if dim == 2 else Mesh.cube(2)
+    conn = good.conn.reshape(good.E, good.nvpe).copy()
+    conn[3, [1, 2]] = conn[3, [2, 1]]
+    m = Mesh.from_arrays(dim, good.xyz.reshape(-1, 3), conn)
+    S = LinearSystemHIP(0)
+    with pytest.raises(FeError, match="Negative or zero jacobian"):
+        S.upload_mesh(m)
+    S.close()

@@ -100,3 +100,30 @@ def test_libraries_export_every_declared_symbol():
     lib = ctypes.CDLL(str(_native.LIB_DIR / "libfe_hip.so"))
     for name in _declared("fe_hip.h"):
         assert hasattr(lib, name), name
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_negative_jacobian_is_refused(dim):
+    """A reversed element makes the mesh unusable, as feCncGeo::computeJacobians
+    (feCncGeo.cpp:346-353, :388-395, :410-412) makes the reference's space
+    creation fail: the check names the element, the numbering refuses it."""
+    from feng_amd.api import FeError, Mesh, Numbering
+    good = Mesh.square(4) if dim == 2 else Mesh.cube(2)
+    assert good.check_jacobians() == (-1, None)
+    conn = good.conn.reshape(good.E, good.nvpe).copy()
+    bad_e = 5
+    conn[bad_e, [0, 1]] = conn[bad_e, [1, 0]]  # swapped vertices: the orientation flips
+    xyz = good.xyz.reshape(-1, 3)
+    m = Mesh.from_arrays(dim, xyz, conn)
+    e, det = m.check_jacobians()
+    assert e == bad_e and det < 0.0
+    v = xyz[conn[bad_e]]
+    ref = np.linalg.det((v[1:dim + 1] - v[0])[:, :dim])  # the determinant of the element map
+    assert abs(det - ref) <= 1e-14
+    with pytest.raises(FeError):
+        Numbering(m, [1 if dim == 2 else 4], [1])
+    # a degenerate (zero-area) element is refused too
+    xyz0 = xyz.copy()
+    xyz0[conn[0, 2]] = xyz0[conn[0, 0]]
+    e0, det0 = Mesh.from_arrays(dim, xyz0, good.conn.reshape(good.E, good.nvpe)).check_jacobians()
+    assert e0 == 0 and det0 == 0.0
