@@ -107,6 +107,7 @@ HIP_API = {
     "fe_hip_set_essential": (C.c_int, [C.c_void_p, c_double_p]),
     "fe_hip_get_system32": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "fe_hip_norm": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, c_double_p]),
+    "fe_hip_norm_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, c_double_p]),
     "fe_hip_set_constrained_dofs": (C.c_int, [C.c_void_p, C.c_int, c_int_p]),
     "fe_hip_constrain_essential_components": (C.c_int, [C.c_void_p]),
     "fe_hip_set_periodic_dofs": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p]),

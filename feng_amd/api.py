@@ -375,10 +375,11 @@ class LinearSystemHIP:
         self._c(self.L.fe_hip_update_coeff(self.ctx, form, index, N.ptr(d, C.c_double), d.size),
                 "fe_hip_update_coeff")
 
-    def norm(self, type_, field, degree, exact=()):
+    def norm(self, type_, field, degree, exact=(), noalloc=False):
         """feNorm::compute on the device: norm `type_` (NORM_*) of field
         `field` with the degree-`degree` rule; exact: the Fn callbacks the
-        type takes (see fe_hip_norm)."""
+        type takes (see fe_hip_norm).  noalloc: through fe_hip_norm_noalloc
+        (no device allocation; the library's static scratch)."""
         arr = (N.FeHipFn * max(1, len(exact)))()
         keep = []
         for i, fn in enumerate(exact):
@@ -389,7 +390,9 @@ class LinearSystemHIP:
                 arr[i].data = N.ptr(d, C.c_double)
                 arr[i].count = d.size
         out = C.c_double()
-        self._c(self.L.fe_hip_norm(self.ctx, type_, field, degree, len(exact), arr, C.byref(out)), "fe_hip_norm")
+        fn = self.L.fe_hip_norm_noalloc if noalloc else self.L.fe_hip_norm
+        self._c(fn(self.ctx, type_, field, degree, len(exact), arr, C.byref(out)),
+                "fe_hip_norm_noalloc" if noalloc else "fe_hip_norm")
         return out.value
 
     # solution history of the BDF schemes (feSolutionContainer) ---------------

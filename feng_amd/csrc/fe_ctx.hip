@@ -238,6 +238,7 @@ int fe_hip_ctx_create(int device, fe_hip_ctx **out)
     delete ctx;
     return FE_STATUS_ERROR;
   }
+  feNormScratchInit(device);  // allocation-free norm: device address of its scratch on this device
   ctx->maxBlocks = 65536;
   if(feAlloc(ctx, &ctx->d_partials, (size_t)ctx->maxBlocks * 4) ||
      feAlloc(ctx, &ctx->d_counter, 64)) {

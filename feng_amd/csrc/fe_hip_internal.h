@@ -61,6 +61,10 @@ int feBuildRefTables(int kind, int degree, FeRefTablesHO *t);  // host, returns 
 int feUploadRefTables(fe_hip_ctx *ctx, const FeRefTablesHO &t, double **d);
 // The FeRefTables (nf <= 6) form of host tables, as feUploadRefTables uploads it.
 void feLowerRefTables(const FeRefTablesHO &t, FeRefTables *lo);
+// Looks up, once per device, the device address of the allocation-free
+// norm's static scratch (the device must be current); fe_hip_ctx_create
+// calls it.  A failed lookup is reported by fe_hip_norm_noalloc.
+void feNormScratchInit(int device);
 
 struct FeFn {
   int kind;

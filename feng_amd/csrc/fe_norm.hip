@@ -18,6 +18,8 @@
 #include "fe_element.h"
 
 #include <cmath>
+#include <mutex>
+#include <string>
 
 namespace {
 
@@ -207,8 +209,41 @@ int normExactCount(int type, int nc, int dim)
 
 }  // namespace
 
-extern "C" int fe_hip_norm(fe_hip_ctx *ctx, int type, int field, int quad_degree, int nexact,
-                           const fe_hip_fn *exact, double *value)
+namespace {
+
+// Scratch of the allocation-free entry point: the reference tables, the
+// block partials, the result and the exact-solution tables, at fixed offsets
+// (doubles).  One copy per GPU, held until the process ends.
+constexpr size_t kScrTab = 0;
+constexpr size_t kScrTabDoubles = ((sizeof(FeRefTablesHO) + 255) / 256) * 256 / sizeof(double);
+constexpr int kNormMaxBlocks = 1024;  // grid cap of normElements (one partial per block)
+constexpr size_t kScrPart = kScrTab + kScrTabDoubles;
+constexpr size_t kScrOut = kScrPart + kNormMaxBlocks;
+constexpr size_t kScrData = kScrOut + 32;
+constexpr size_t kScrDataDoubles = (size_t)1 << 20;  // exact-solution tables, 8 MiB
+constexpr size_t kScrDoubles = kScrData + kScrDataDoubles;
+constexpr int kScrMaxDevices = 64;
+
+static __device__ __align__(256) double scratch[kScrDoubles];
+
+std::mutex g_scrMutex;
+double *g_scrPtr[kScrMaxDevices] = {};
+hipError_t g_scrErr[kScrMaxDevices] = {};
+bool g_scrDone[kScrMaxDevices] = {};
+
+// Device buffers of one norm evaluation beyond its inputs.
+struct NormBufs {
+  const double *tab;  // reference tables in the layout of the field's kernel
+  double *data;       // exact-solution tables, dataCap doubles
+  size_t dataCap;
+  double *partials;   // >= kNormMaxBlocks doubles
+  double *out;        // the folded value
+};
+
+// Checks the arguments of fe_hip_norm / fe_hip_norm_noalloc; returns the
+// field kind and the number of exact-table doubles the call copies.
+int normCheck(fe_hip_ctx *ctx, int type, int field, int nexact, const fe_hip_fn *exact, double *value, int *kindOut,
+              size_t *tabDoubles, int nq)
 {
   if(!ctx || !value || (nexact > 0 && !exact)) return FE_STATUS_POINTER_ERROR;
   FE_REQUIRE(ctx, ctx->d_u && ctx->E > 0, "upload the mesh and set the sizes first");
@@ -219,38 +254,9 @@ extern "C" int fe_hip_norm(fe_hip_ctx *ctx, int type, int field, int quad_degree
   FE_REQUIRE(ctx, need >= 0, "norm type not available on the device for this field: " + std::to_string(type));
   FE_REQUIRE(ctx, nexact == need, "norm type " + std::to_string(type) + " takes " + std::to_string(need) +
                                       " exact callback(s)");
-  const bool isMax = type == FE_NORM_LINF || type == FE_NORM_LINF_ERROR || type == FE_NORM_LINF_UDOT_ERROR ||
-                     type == FE_NORM_VECTOR_LINF_ERROR || type == FE_NORM_VECTOR_LINF_UDOT_ERROR;
   if(type == FE_NORM_LINF_UDOT_ERROR || type == FE_NORM_VECTOR_LINF_UDOT_ERROR || type == FE_NORM_INTEGRAL_DT)
     FE_REQUIRE(ctx, ctx->nsol == 1 || ctx->d_past[0], "the time derivative needs the solution history");
-  // reference tables of (kind, degree), cached
-  const int key = kind * 64 + quad_degree;
-  double *tab = nullptr;
-  auto it = ctx->normTabs.find(key);
-  if(it != ctx->normTabs.end()) tab = it->second;
-  else {
-    FeRefTablesHO T;
-    const int nq = feBuildRefTables(kind, quad_degree, &T);
-    FE_REQUIRE(ctx, nq > 0, "unsupported quadrature degree " + std::to_string(quad_degree));
-    if(int rc = feUploadRefTables(ctx, T, &tab)) return rc;
-    ctx->normTabs[key] = tab;
-    ctx->normNq[key] = nq;
-  }
-  const int nq = ctx->normNq[key];
-  NormArgs a{};
-  a.E = ctx->E;
-  a.type = type;
-  a.nq = nq;
-  a.conn = ctx->d_conn;
-  a.adr = ctx->fields[field].adr;
-  a.xyz = ctx->d_xyz;
-  a.tab = tab;
-  a.u = ctx->d_u;
-  a.nsol = ctx->nsol;
-  for(int j = 0; j <= fe_hip_ctx::kMaxPast; ++j) a.cn[j] = ctx->bdf[j];
-  for(int j = 0; j < fe_hip_ctx::kMaxPast; ++j) a.past[j] = ctx->d_past[j];
-  // exact callbacks; tables go to one cached device buffer
-  size_t tabDoubles = 0;
+  size_t td = 0;
   for(int i = 0; i < nexact; ++i) {
     FE_REQUIRE(ctx, exact[i].kind >= FE_FN_CONST && exact[i].kind <= FE_FN_POLY_U, "unknown function kind");
     if(exact[i].kind == FE_FN_POLY_U) {
@@ -258,20 +264,37 @@ extern "C" int fe_hip_norm(fe_hip_ctx *ctx, int type, int field, int quad_degree
                  "FE_FN_POLY_U depends on the solution: only INTEGRAL_SOL_DEPENDENT_F takes it");
       FE_REQUIRE(ctx, exact[i].data && exact[i].count >= 1 && exact[i].count <= 8,
                  "FE_FN_POLY_U takes 1 to 8 polynomial coefficients");
-      tabDoubles += (size_t)exact[i].count;
+      td += (size_t)exact[i].count;
     } else if(exact[i].kind == FE_FN_ELEM || exact[i].kind == FE_FN_QP) {
       const int64_t cnt = exact[i].kind == FE_FN_ELEM ? ctx->E : (int64_t)ctx->E * nq;
       FE_REQUIRE(ctx, exact[i].data && exact[i].count == cnt, "exact-solution table has the wrong size");
-      tabDoubles += (size_t)cnt;
+      td += (size_t)cnt;
     }
   }
-  if(tabDoubles > ctx->normDataCap) {
-    if(ctx->d_normData) feFree(ctx, ctx->d_normData);
-    ctx->d_normData = nullptr;
-    ctx->normDataCap = 0;
-    if(feAlloc(ctx, &ctx->d_normData, tabDoubles)) return FE_STATUS_OUT_OF_MEMORY;
-    ctx->normDataCap = tabDoubles;
-  }
+  *kindOut = kind;
+  *tabDoubles = td;
+  return FE_STATUS_OK;
+}
+
+// Copies the exact tables into b.data, launches the element and folding
+// kernels on the context stream and returns the norm (stream synchronized).
+int normRun(fe_hip_ctx *ctx, int type, int field, int kind, int nq, int nexact, const fe_hip_fn *exact,
+            const NormBufs &b, double *value)
+{
+  const bool isMax = type == FE_NORM_LINF || type == FE_NORM_LINF_ERROR || type == FE_NORM_LINF_UDOT_ERROR ||
+                     type == FE_NORM_VECTOR_LINF_ERROR || type == FE_NORM_VECTOR_LINF_UDOT_ERROR;
+  NormArgs a{};
+  a.E = ctx->E;
+  a.type = type;
+  a.nq = nq;
+  a.conn = ctx->d_conn;
+  a.adr = ctx->fields[field].adr;
+  a.xyz = ctx->d_xyz;
+  a.tab = b.tab;
+  a.u = ctx->d_u;
+  a.nsol = ctx->nsol;
+  for(int j = 0; j <= fe_hip_ctx::kMaxPast; ++j) a.cn[j] = ctx->bdf[j];
+  for(int j = 0; j < fe_hip_ctx::kMaxPast; ++j) a.past[j] = ctx->d_past[j];
   size_t off = 0;
   for(int i = 0; i < nexact; ++i) {
     FeFn &f = a.ex[i];
@@ -281,15 +304,14 @@ extern "C" int fe_hip_norm(fe_hip_ctx *ctx, int type, int field, int quad_degree
     f.data = nullptr;
     f.n = f.kind == FE_FN_POLY_U ? (int)exact[i].count : 0;
     if(f.kind == FE_FN_ELEM || f.kind == FE_FN_QP || f.kind == FE_FN_POLY_U) {
-      f.data = ctx->d_normData + off;
+      f.data = b.data + off;
       FE_HIP_TRY(ctx, hipMemcpyAsync(f.data, exact[i].data, sizeof(double) * exact[i].count, hipMemcpyHostToDevice,
                                      ctx->stream));
       off += (size_t)exact[i].count;
     }
   }
-  const int G = std::max(1, std::min(1024, (ctx->E + 255) / 256));
-  a.partials = ctx->d_partials;
-  double *out = ctx->d_partials + 3 * ctx->maxBlocks + 8;
+  const int G = std::max(1, std::min(kNormMaxBlocks, (ctx->E + 255) / 256));
+  a.partials = b.partials;
   if(kind == FE_TRI_P1) hipLaunchKernelGGL((normElements<3, 2, 1>), dim3(G), dim3(256), 0, ctx->stream, a);
   else if(kind == FE_TRI_P2) hipLaunchKernelGGL((normElements<6, 2, 1>), dim3(G), dim3(256), 0, ctx->stream, a);
   else if(kind == FE_TET_P1) hipLaunchKernelGGL((normElements<4, 3, 1>), dim3(G), dim3(256), 0, ctx->stream, a);
@@ -297,10 +319,10 @@ extern "C" int fe_hip_norm(fe_hip_ctx *ctx, int type, int field, int quad_degree
   else if(kind == FE_TRI_P3) hipLaunchKernelGGL((normElements<10, 2, 1>), dim3(G), dim3(256), 0, ctx->stream, a);
   else if(kind == FE_TRI_P4) hipLaunchKernelGGL((normElements<15, 2, 1>), dim3(G), dim3(256), 0, ctx->stream, a);
   else FE_REQUIRE(ctx, false, "field kind without a device norm");
-  hipLaunchKernelGGL(normFinal, dim3(1), dim3(64), 0, ctx->stream, ctx->d_partials, G, isMax ? 1 : 0, out);
+  hipLaunchKernelGGL(normFinal, dim3(1), dim3(64), 0, ctx->stream, b.partials, G, isMax ? 1 : 0, b.out);
   FE_HIP_TRY(ctx, hipGetLastError());
   double r = 0.;
-  FE_HIP_TRY(ctx, hipMemcpyAsync(&r, out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  FE_HIP_TRY(ctx, hipMemcpyAsync(&r, b.out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   // tiny negative sums from negative weights (feNorm.cpp:390-393)
   const bool integ = type == FE_NORM_INTEGRAL || type == FE_NORM_INTEGRAL_DT ||
@@ -311,4 +333,96 @@ extern "C" int fe_hip_norm(fe_hip_ctx *ctx, int type, int field, int quad_degree
                     type == FE_NORM_H1 || type == FE_NORM_H1_ERROR || type == FE_NORM_VECTOR_SEMI_H1_ERROR;
   *value = root ? sqrt(r) : r;
   return FE_STATUS_OK;
+}
+
+}  // namespace
+
+void feNormScratchInit(int device)
+{
+  if(device < 0 || device >= kScrMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scrMutex);
+  if(g_scrDone[device]) return;
+  double *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+  g_scrErr[device] = e;
+  g_scrPtr[device] = e == hipSuccess ? p : nullptr;
+  g_scrDone[device] = true;
+}
+
+extern "C" int fe_hip_norm(fe_hip_ctx *ctx, int type, int field, int quad_degree, int nexact,
+                           const fe_hip_fn *exact, double *value)
+{
+  if(!ctx || !value || (nexact > 0 && !exact)) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, ctx->d_u && ctx->E > 0, "upload the mesh and set the sizes first");
+  FE_REQUIRE(ctx, field >= 0 && field < (int)ctx->fields.size(), "unknown field");
+  // reference tables of (kind, degree), cached
+  const int key = ctx->fields[field].kind * 64 + quad_degree;
+  double *tab = nullptr;
+  auto it = ctx->normTabs.find(key);
+  if(it != ctx->normTabs.end()) tab = it->second;
+  else {
+    FeRefTablesHO T;
+    const int nq = feBuildRefTables(ctx->fields[field].kind, quad_degree, &T);
+    FE_REQUIRE(ctx, nq > 0, "unsupported quadrature degree " + std::to_string(quad_degree));
+    if(int rc = feUploadRefTables(ctx, T, &tab)) return rc;
+    ctx->normTabs[key] = tab;
+    ctx->normNq[key] = nq;
+  }
+  const int nq = ctx->normNq[key];
+  int kind = 0;
+  size_t tabDoubles = 0;
+  if(int rc = normCheck(ctx, type, field, nexact, exact, value, &kind, &tabDoubles, nq)) return rc;
+  // exact tables go to one cached device buffer
+  if(tabDoubles > ctx->normDataCap) {
+    if(ctx->d_normData) feFree(ctx, ctx->d_normData);
+    ctx->d_normData = nullptr;
+    ctx->normDataCap = 0;
+    if(feAlloc(ctx, &ctx->d_normData, tabDoubles)) return FE_STATUS_OUT_OF_MEMORY;
+    ctx->normDataCap = tabDoubles;
+  }
+  NormBufs b{tab, ctx->d_normData, ctx->normDataCap, ctx->d_partials, ctx->d_partials + 3 * ctx->maxBlocks + 8};
+  return normRun(ctx, type, field, kind, nq, nexact, exact, b, value);
+}
+
+extern "C" int fe_hip_norm_noalloc(fe_hip_ctx *ctx, int type, int field, int quad_degree, int nexact,
+                                   const fe_hip_fn *exact, double *value)
+{
+  if(!ctx || !value || (nexact > 0 && !exact)) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, field >= 0 && field < (int)ctx->fields.size(), "unknown field");
+  FE_REQUIRE(ctx, ctx->device >= 0 && ctx->device < kScrMaxDevices,
+             "fe_hip_norm_noalloc: device id beyond the scratch table (" + std::to_string(kScrMaxDevices) + ")");
+  double *scr = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scrMutex);
+    FE_REQUIRE(ctx, g_scrDone[ctx->device] && g_scrPtr[ctx->device],
+               std::string("fe_hip_norm_noalloc: no device address for the scratch: ") +
+                   hipGetErrorString(g_scrErr[ctx->device]));
+    scr = g_scrPtr[ctx->device];
+  }
+  FeRefTablesHO T;
+  const int nq = feBuildRefTables(ctx->fields[field].kind, quad_degree, &T);
+  FE_REQUIRE(ctx, nq > 0, "unsupported quadrature degree " + std::to_string(quad_degree));
+  int kind = 0;
+  size_t tabDoubles = 0;
+  if(int rc = normCheck(ctx, type, field, nexact, exact, value, &kind, &tabDoubles, nq)) return rc;
+  FE_REQUIRE(ctx, tabDoubles <= kScrDataDoubles,
+             "fe_hip_norm_noalloc: exact-solution tables of " + std::to_string(tabDoubles) +
+                 " doubles exceed its scratch of " + std::to_string(kScrDataDoubles) + " doubles");
+  // the reference tables in the layout the field's kernel loads, copied
+  // into the scratch (the stream is synchronized before T goes away)
+  FeRefTables lo;
+  const void *img = &T;
+  size_t imgBytes = sizeof(FeRefTablesHO);
+  if(T.nf <= 6) {
+    feLowerRefTables(T, &lo);
+    img = &lo;
+    imgBytes = sizeof(FeRefTables);
+  }
+  static_assert(sizeof(FeRefTablesHO) <= kScrTabDoubles * sizeof(double), "table slot of the scratch");
+  FE_HIP_TRY(ctx, hipMemcpyAsync(scr + kScrTab, img, imgBytes, hipMemcpyHostToDevice, ctx->stream));
+  NormBufs b{scr + kScrTab, scr + kScrData, kScrDataDoubles, scr + kScrPart, scr + kScrOut};
+  const int rc = normRun(ctx, type, field, kind, nq, nexact, exact, b, value);
+  if(rc == FE_STATUS_OK) return rc;
+  (void)hipStreamSynchronize(ctx->stream);  // T and lo outlive every copy from them
+  return rc;
 }

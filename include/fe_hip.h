@@ -323,6 +323,22 @@ int fe_hip_set_essential(fe_hip_ctx *ctx, const double *values);
 int fe_hip_norm(fe_hip_ctx *ctx, int type, int field, int quad_degree, int nexact, const fe_hip_fn *exact,
                 double *value);
 
+/* Allocation-free fe_hip_norm.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight time-stepping or serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and value as fe_hip_norm, but it calls no
+ * hipMalloc / hipFree: the reference tables, the block partials, the result
+ * and the exact-solution tables (FE_FN_ELEM / FE_FN_QP / FE_FN_POLY_U data)
+ * live in a static device scratch of the library, one copy per GPU, about
+ * 8.1 MiB, held until the process ends.  Exact-solution tables larger than
+ * 1,048,576 doubles in total are rejected with FE_STATUS_ERROR and a
+ * message (e.g. an FE_FN_QP table of E nq > 2^20 values).  Every call on
+ * one GPU shares that GPU's scratch: calls that overlap on one GPU (two host
+ * threads, or two contexts on different streams of the same device) race and
+ * must be serialized by the caller.  Nothing of the scratch needs to start
+ * at zero: every value a call reads was written earlier in the same call. */
+int fe_hip_norm_noalloc(fe_hip_ctx *ctx, int type, int field, int quad_degree, int nexact,
+                        const fe_hip_fn *exact, double *value);
 
 /* ---- aggregation multigrid (FE_PC_AMG) ----------------------------------
  * The algebraic multigrid the reference reaches through PETSc's runtime
