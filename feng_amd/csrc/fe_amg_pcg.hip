@@ -42,9 +42,12 @@
 
 namespace {
 
-constexpr int kPTB = 256;             // threads of the level kernels
-constexpr int kPLPR = 4;              // lanes per row
-constexpr int kPRPB = kPTB / kPLPR;   // rows per block: the tile size limit
+constexpr int kPTB = 256;             // threads of the level kernels (levels >= 1)
+constexpr int kFTB = 256;             // threads of the level-0 kernels: one row per thread (about n / 256 partials)
+constexpr int kPLPR = 4;              // lanes per row of the level kernels
+constexpr int kPRPB = kPTB / kPLPR;   // rows per block of a level kernel: the tile limit of levels >= 1
+constexpr int kFRPB = kFTB;           // rows per block of a level-0 kernel: the tile limit of level 0
+constexpr int kSlice = 64;            // level-0 SELL slice: 64 rows, slot k of the slice's rows contiguous
 constexpr int kPBlockRows = 1024;     // levels of at most this many rows run in the single-workgroup kernel
 constexpr int kPBlockTB = 1024;
 constexpr int kPBlockLds = 150 * 1024;
@@ -54,8 +57,9 @@ constexpr int kPBlockLev = 8;
 template <typename T, typename TV>
 struct PLev {
   int n, nc, nTiles;
-  const int *ia, *ja;
-  const TV *val, *l1;
+  const int *ia, *ja;   // level 0: ja holds the SELL columns
+  const TV *val, *l1;   // level 0: SELL values
+  const int2 *sell;     // level 0: per row (first SELL position, length); entry k at first + 64 k
   const int *agg;       // row -> aggregate on the next level (l < coarsest)
   const int *aggStart;  // nc + 1: first row of each aggregate
   const int *tileStart, *tileAgg;  // nTiles + 1: tile rows and first aggregate
@@ -70,7 +74,9 @@ struct PBlockPlan {
   int n[kPBlockLev + 1], nc[kPBlockLev], nnz[kPBlockLev];
   int oIa[kPBlockLev], oJa[kPBlockLev], oVal[kPBlockLev], oL1[kPBlockLev], oAgg[kPBlockLev], oAs[kPBlockLev];
   int oB[kPBlockLev + 1], oU[kPBlockLev], oY[kPBlockLev], oE[kPBlockLev];
-  int oCx = 0;
+  int oCx = 0, oCi = 0;
+  int imgBytes = 0;  // the static part (matrices, aggregates, coarsest inverse) comes first: its image is
+                     // packed once per numeric set-up and copied into LDS in one pass
 };
 
 // ---------------------------------------------------------------------------
@@ -82,6 +88,31 @@ __device__ __forceinline__ double foldPartials(const double *__restrict__ part, 
 {
   double s = 0.;
   for(int g = threadIdx.x; g < G; g += TB) s += part[g];
+#pragma unroll
+  for(int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  double r = 0.;
+#pragma unroll
+  for(int w = 0; w < TB / 64; ++w) r += red[w];
+  __syncthreads();
+  return r;
+}
+
+// The two halves of foldPartials, so the partial loads can be issued with the
+// kernel's other independent loads and reduced later: this thread's strided
+// share, then the block-wide sum (the same value in every block).
+template <int TB>
+__device__ __forceinline__ double partialsShare(const double *__restrict__ part, int G)
+{
+  double s = 0.;
+#pragma unroll 4
+  for(int g = threadIdx.x; g < G; g += TB) s += part[g];
+  return s;
+}
+template <int TB>
+__device__ __forceinline__ double blockAllSum(double s, double *red)
+{
 #pragma unroll
   for(int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   if((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
@@ -109,17 +140,135 @@ __device__ __forceinline__ void blockPartial(double v, double *red, double *__re
   }
 }
 
+// Level 0, one row per thread: sum_k val f(col) over the row's SELL entries,
+// in batches of kBatch entries whose column, value and gathered loads are
+// all issued before any of them is used (a row is one or two batches), so a
+// row costs a few memory round trips instead of one per entry.
+constexpr int kBatch = 8;
+// Two row sums over one pass of the SELL columns and values.
+template <typename T, typename TV>
+__device__ __forceinline__ void sellSum2(const int *__restrict__ col, const TV *__restrict__ val, int2 bl,
+                                         const T *__restrict__ f1, const T *__restrict__ f2, T &s1, T &s2)
+{
+  s1 = (T)0;
+  s2 = (T)0;
+  for(int k0 = 0; k0 < bl.y; k0 += kBatch) {
+    int c[kBatch];
+    TV v[kBatch];
+#pragma unroll
+    for(int u = 0; u < kBatch; ++u) {
+      const bool in = k0 + u < bl.y;
+      const int q = bl.x + (in ? k0 + u : 0) * kSlice;
+      c[u] = col[q];
+      v[u] = in ? val[q] : (TV)0;
+    }
+    T g1[kBatch], g2[kBatch];
+#pragma unroll
+    for(int u = 0; u < kBatch; ++u) {
+      g1[u] = f1[c[u]];
+      g2[u] = f2[c[u]];
+    }
+#pragma unroll
+    for(int u = 0; u < kBatch; ++u) {
+      s1 += (T)v[u] * g1[u];
+      s2 += (T)v[u] * g2[u];
+    }
+  }
+}
+template <typename T, typename TV, typename F>
+__device__ __forceinline__ T sellSum(const int *__restrict__ col, const TV *__restrict__ val, int2 bl, F f)
+{
+  T acc = (T)0;
+  for(int k0 = 0; k0 < bl.y; k0 += kBatch) {
+    int c[kBatch];
+    TV v[kBatch];
+#pragma unroll
+    for(int u = 0; u < kBatch; ++u) {
+      const bool in = k0 + u < bl.y;
+      const int q = bl.x + (in ? k0 + u : 0) * kSlice;
+      c[u] = col[q];
+      v[u] = in ? val[q] : (TV)0;
+    }
+    T g[kBatch];
+#pragma unroll
+    for(int u = 0; u < kBatch; ++u) g[u] = f(c[u]);
+#pragma unroll
+    for(int u = 0; u < kBatch; ++u) acc += (T)v[u] * g[u];
+  }
+  return acc;
+}
+
 // sum_k val[k] f(ja[k]) over [kb, ke), kPLPR lanes per row (lanes sub of a
 // group of kPLPR consecutive lanes); every lane of the group gets the sum.
 template <typename T, typename TV, typename F>
 __device__ __forceinline__ T rowSum(const int *__restrict__ ja, const TV *__restrict__ val, int kb, int ke, int sub,
                                     F f)
 {
+  constexpr int U = 4;  // entries per lane per batch, loads issued together
   T acc = (T)0;
-  for(int k = kb + sub; k < ke; k += kPLPR) acc += (T)val[k] * f(ja[k]);
+  for(int k0 = kb + sub; k0 < ke; k0 += U * kPLPR) {
+    int c[U];
+    TV v[U];
+#pragma unroll
+    for(int u = 0; u < U; ++u) {
+      const int k = k0 + u * kPLPR;
+      const bool in = k < ke;
+      c[u] = ja[in ? k : kb];
+      v[u] = in ? val[k] : (TV)0;
+    }
+    T g[U];
+#pragma unroll
+    for(int u = 0; u < U; ++u) g[u] = f(c[u]);
+#pragma unroll
+    for(int u = 0; u < U; ++u) acc += (T)v[u] * g[u];
+  }
 #pragma unroll
   for(int o = 1; o < kPLPR; o <<= 1) acc += __shfl_xor(acc, o, 64);
   return acc;
+}
+
+// Row sum over LDS-staged entries [k0, k1) in batches of 8 whose loads are
+// issued together.
+template <typename T, typename TV, typename F>
+__device__ __forceinline__ T ldsRow(int k0, int k1, const unsigned short *ja, const TV *val, F f)
+{
+  T acc = (T)0;
+  for(int kb = k0; kb < k1; kb += 8) {
+    int c[8];
+    TV v[8];
+#pragma unroll
+    for(int u = 0; u < 8; ++u) {
+      const bool in = kb + u < k1;
+      c[u] = ja[in ? kb + u : k0];
+      v[u] = in ? val[kb + u] : (TV)0;
+    }
+    T g[8];
+#pragma unroll
+    for(int u = 0; u < 8; ++u) g[u] = f(c[u]);
+#pragma unroll
+    for(int u = 0; u < 8; ++u) acc += (T)v[u] * g[u];
+  }
+  return acc;
+}
+
+// Copy of count items into LDS by the whole block, B loads per thread
+// issued before their stores.
+template <int B, typename TD, typename TS>
+__device__ __forceinline__ void stageLds(TD *dst, const TS *__restrict__ src, int count)
+{
+  for(int base = threadIdx.x; base < count; base += B * blockDim.x) {
+    TS v[B];
+#pragma unroll
+    for(int u = 0; u < B; ++u) {
+      const int i = base + u * (int)blockDim.x;
+      v[u] = i < count ? src[i] : (TS)0;
+    }
+#pragma unroll
+    for(int u = 0; u < B; ++u) {
+      const int i = base + u * (int)blockDim.x;
+      if(i < count) dst[i] = (TD)v[u];
+    }
+  }
 }
 
 template <typename T>
@@ -136,30 +285,46 @@ __device__ __forceinline__ float fmaT<float>(float a, float b, float c)
 // ---------------------------------------------------------------------------
 // Kernels of one PCG iteration
 
-// rho = (r, z) from the previous up0 partials; p = z + beta p_old; q = A p.
-template <typename T, typename TV>
-__global__ __launch_bounds__(kPTB) void pcgProduct(int n, const int *__restrict__ ia, const int *__restrict__ ja,
-                                                   const TV *__restrict__ val, const T *__restrict__ z,
-                                                   const T *__restrict__ pOld, T *__restrict__ pNew,
-                                                   T *__restrict__ q, const double *__restrict__ rzPart, int Grz,
+// rho = (r, z) from the previous up0 partials; p = z + beta p_old and
+// q = A z + beta q_old (= A p; the product gathers z only, beta is applied
+// after the row sum, so the fold of the partials overlaps the gathers);
+// w = l1 q for the next restriction; partials of (p, q).  Level-0 SELL
+// values of type TV, one row per thread.
+template <typename T, typename TV, typename TL>
+__global__ __launch_bounds__(kFTB) void pcgProduct(int n, const int2 *__restrict__ sell, const int *__restrict__ col,
+                                                   const TV *__restrict__ val, const TL *__restrict__ l1,
+                                                   const T *__restrict__ z, const T *__restrict__ pOld,
+                                                   T *__restrict__ pNew, T *__restrict__ q, T *__restrict__ w,
+                                                   const double *__restrict__ rzPart, int Grz,
                                                    const double *__restrict__ rrPart, int Grr, double *scal, int slot,
                                                    int first, double tol2, int *done, int *stat, int k,
                                                    double *__restrict__ pqPart)
 {
-  __shared__ double red[kPTB / 64];
+  __shared__ double red[kFTB / 64];
+  // independent loads first: flag, row bounds, own-row values, partial shares
   const int dn = *done;
-  const int row = blockIdx.x * kPRPB + (int)threadIdx.x / kPLPR, sub = threadIdx.x % kPLPR;
-  int kb = 0, ke = 0;
+  const int row = blockIdx.x * kFTB + (int)threadIdx.x;
+  int2 bl = make_int2(0, 0);
+  T zi = (T)0, pi = (T)0, qi = (T)0, li = (T)0;
   if(row < n) {
-    kb = ia[row];
-    ke = ia[row + 1];
+    bl = sell[row];
+    zi = z[row];
+    li = (T)l1[row];
+    if(!first) {
+      pi = pOld[row];
+      qi = q[row];
+    }
   }
+  const double rzOwn = partialsShare<kFTB>(rzPart, Grz);
+  const double rrOwn = blockIdx.x == 0 ? partialsShare<kFTB>(rrPart, Grr) : 0.;
+  const double prevRho = first ? 1. : scal[slot ^ 1];
   if(dn) return;
-  const double rho = foldPartials<kPTB>(rzPart, Grz, red);
+  const T az = sellSum<T, TV>(col, val, bl, [&](int j) { return z[j]; });
+  const double rho = blockAllSum<kFTB>(rzOwn, red);
   double beta = 0.;
-  if(!first) beta = rho / scal[slot ^ 1];
+  if(!first) beta = rho / prevRho;
   if(blockIdx.x == 0) {
-    const double rr = foldPartials<kPTB>(rrPart, Grr, red);
+    const double rr = blockAllSum<kFTB>(rrOwn, red);
     if(threadIdx.x == 0) {
       scal[slot] = rho;
       scal[2] = rr;
@@ -173,73 +338,97 @@ __global__ __launch_bounds__(kPTB) void pcgProduct(int n, const int *__restrict_
     }
   }
   const T bt = (T)beta;
-  // first: p = z (the previous direction is not read)
-  const T acc = rowSum<T, TV>(ja, val, kb, ke, sub, [&](int j) { return first ? z[j] : fmaT<T>(bt, pOld[j], z[j]); });
   double pq = 0.;
-  if(row < n && sub == 0) {
-    const T p = first ? z[row] : fmaT<T>(bt, pOld[row], z[row]);
+  if(row < n) {
+    // first: p = z, q = A z (the previous direction is not read)
+    const T p = first ? zi : fmaT<T>(bt, pi, zi);
+    const T qn = first ? az : fmaT<T>(bt, qi, az);
     pNew[row] = p;
-    q[row] = acc;
-    pq = (double)p * (double)acc;
+    q[row] = qn;
+    w[row] = li * qn;
+    pq = (double)p * (double)qn;
   }
-  blockPartial<kPTB>(pq, red, pqPart);
+  blockPartial<kFTB>(pq, red, pqPart);
 }
 
-// alpha = rho / (p, q); x += alpha p; r_new = r_old - alpha q; the first
-// smoothing step and the restriction to level 1 on a tile of whole
-// aggregates; partials of (r_new, r_new).  first: alpha = 0, x untouched.
+// alpha = rho / (p, q); x += alpha p; r_new = r_old - alpha q; u = l1 r_new
+// (the first smoothing step); the restriction to level 1 of
+// r_new - A u = r_new - (A uOld - alpha A w) with uOld = l1 r_old (stored by
+// the previous call) and w = l1 q (stored by the product), so alpha is
+// applied after the gathers; tiles of whole aggregates, one row per thread;
+// partials of (r_new, r_new).  first: alpha = 0 and A (l1 r_old) is gathered
+// directly (x, p, q, w and uOld are not read).
 template <typename T, typename TV>
-__global__ __launch_bounds__(kPTB) void pcgDown0(PLev<T, TV> L, T *__restrict__ x, const T *__restrict__ p,
-                                                 const T *__restrict__ q, const T *__restrict__ rOld,
-                                                 T *__restrict__ rNew, T *__restrict__ u, T *__restrict__ bNext,
+__global__ __launch_bounds__(kFTB) void pcgDown0(PLev<T, TV> L, T *__restrict__ x, const T *__restrict__ p,
+                                                 const T *__restrict__ q, const T *__restrict__ w,
+                                                 const T *__restrict__ rOld, T *__restrict__ rNew,
+                                                 const T *__restrict__ uOld, T *__restrict__ u, T *__restrict__ bNext,
                                                  const double *__restrict__ pqPart, int Gpq, const double *scal,
                                                  int slot, int first, int *done, int *stat, int k,
                                                  double *__restrict__ rrPart)
 {
-  __shared__ double red[kPTB / 64];
-  __shared__ T res[kPRPB];
+  __shared__ double red[kFTB / 64];
+  __shared__ T res[kFRPB];
+  // independent loads first: flag, tile, row bounds, own-row values, this
+  // thread's aggregate bounds, partial shares
   const int dn = *done;
   const int t = blockIdx.x;
   const int r0 = L.tileStart[t], r1 = L.tileStart[t + 1];
-  const int rl = (int)threadIdx.x / kPLPR, sub = threadIdx.x % kPLPR;
-  const int row = r0 + rl;
-  int kb = 0, ke = 0;
+  const int I0 = L.tileAgg[t], I1 = L.tileAgg[t + 1];
+  const int row = r0 + (int)threadIdx.x;
+  int2 bl = make_int2(0, 0);
+  T ro = (T)0, qi = (T)0, pi = (T)0, xi = (T)0, li = (T)0;
   if(row < r1) {
-    kb = L.ia[row];
-    ke = L.ia[row + 1];
+    bl = L.sell[row];
+    ro = rOld[row];
+    li = (T)L.l1[row];
+    if(!first) {
+      qi = q[row];
+      pi = p[row];
+      xi = x[row];
+    }
   }
+  const int I = I0 + (int)threadIdx.x;  // tiles hold at most kFRPB = kFTB aggregates
+  int a0 = 0, a1 = 0;
+  if(I < I1) {
+    a0 = L.aggStart[I] - r0;
+    a1 = L.aggStart[I + 1] - r0;
+  }
+  const double pqOwn = first ? 0. : partialsShare<kFTB>(pqPart, Gpq);
+  const double rho = first ? 0. : scal[slot];
   if(dn) return;
+  T s1, s2 = (T)0;
+  if(first) {
+    s1 = sellSum<T, TV>(L.ja, L.val, bl, [&](int j) { return (T)L.l1[j] * rOld[j]; });
+  } else {
+    sellSum2<T, TV>(L.ja, L.val, bl, uOld, w, s1, s2);
+  }
   double alpha = 0.;
   if(!first) {
-    const double pq = foldPartials<kPTB>(pqPart, Gpq, red);
-    if(pq > 0.) alpha = scal[slot] / pq;
+    const double pq = blockAllSum<kFTB>(pqOwn, red);
+    if(pq > 0.) alpha = rho / pq;
     else if(t == 0 && threadIdx.x == 0) {
       *done = 3;  // (p, A p) <= 0: no update, the host restarts from the FP64 residual
       stat[1] = k;
     }
   }
   const T a = (T)alpha;
-  // first: r_new = r_old (q and p are not read)
-  const T acc = rowSum<T, TV>(L.ja, L.val, kb, ke, sub,
-                              [&](int j) { return (T)L.l1[j] * (first ? rOld[j] : fmaT<T>(-a, q[j], rOld[j])); });
   double rr = 0.;
-  if(row < r1 && sub == 0) {
-    const T rn = first ? rOld[row] : fmaT<T>(-a, q[row], rOld[row]);
+  if(row < r1) {
+    const T rn = first ? ro : fmaT<T>(-a, qi, ro);
     rNew[row] = rn;
-    if(!first) x[row] = fmaT<T>(a, p[row], x[row]);
-    u[row] = (T)L.l1[row] * rn;
-    res[rl] = rn - acc;
+    if(!first) x[row] = fmaT<T>(a, pi, xi);
+    u[row] = li * rn;
+    res[threadIdx.x] = rn - (first ? s1 : fmaT<T>(-a, s2, s1));
     rr = (double)rn * (double)rn;
   }
   __syncthreads();
-  const int I1 = L.tileAgg[t + 1];
-  for(int I = L.tileAgg[t] + (int)threadIdx.x; I < I1; I += kPTB) {
-    const int m1 = L.aggStart[I + 1] - r0;
-    T s = (T)0;
-    for(int m = L.aggStart[I] - r0; m < m1; ++m) s += res[m];
-    bNext[I] = s;
+  if(I < I1) {
+    T sum = (T)0;
+    for(int m = a0; m < a1; ++m) sum += res[m];
+    bNext[I] = sum;
   }
-  blockPartial<kPTB>(rr, red, rrPart);
+  blockPartial<kFTB>(rr, red, rrPart);
 }
 
 // Level l >= 1: u = S b; b_next = P^T (b - A u) over tiles of whole aggregates.
@@ -274,9 +463,13 @@ __global__ __launch_bounds__(kPTB) void pcgDownL(PLev<T, TV> L, T *__restrict__ 
   }
 }
 
-// Level l >= 1: y = u + oc e_next[agg]; e = y + S (b - A y).
+// Level l >= 1: y = u + oc e_next[agg]; e = y + S (b - A y).  On level 1
+// (y0 given), also the level-0 y0 = u0 + oc e over the rows of each
+// aggregate (contiguous in the nested order: aggStart0).
 template <typename T, typename TV>
-__global__ __launch_bounds__(kPTB) void pcgUpL(PLev<T, TV> L, const T *__restrict__ eNext, T oc, const int *done)
+__global__ __launch_bounds__(kPTB) void pcgUpL(PLev<T, TV> L, const T *__restrict__ eNext, T oc, const int *done,
+                                               T *__restrict__ y0, const T *__restrict__ u0,
+                                               const int *__restrict__ aggStart0)
 {
   const int dn = *done;
   const int row = blockIdx.x * kPRPB + (int)threadIdx.x / kPLPR, sub = threadIdx.x % kPLPR;
@@ -287,70 +480,97 @@ __global__ __launch_bounds__(kPTB) void pcgUpL(PLev<T, TV> L, const T *__restric
   }
   if(dn) return;
   const T acc = rowSum<T, TV>(L.ja, L.val, kb, ke, sub, [&](int j) { return fmaT<T>(oc, eNext[L.agg[j]], L.u[j]); });
-  if(row < L.n && sub == 0) {
+  if(row < L.n) {
     const T y = fmaT<T>(oc, eNext[L.agg[row]], L.u[row]);
-    L.e[row] = fmaT<T>((T)L.l1[row], L.b[row] - acc, y);
+    const T e = fmaT<T>((T)L.l1[row], L.b[row] - acc, y);
+    if(sub == 0) L.e[row] = e;
+    if(y0)
+      for(int m = aggStart0[row] + sub; m < aggStart0[row + 1]; m += kPLPR) y0[m] = fmaT<T>(oc, e, u0[m]);
   }
 }
 
-// Level 0: z = y + S (r - A y), y = u + oc e_1[agg]; partials of (r, z).
+// Level 0: z = y + S (r - A y) with y = u + oc P e_1 written by the level-1
+// prolongation; partials of (r, z).  One row per thread.
 template <typename T, typename TV>
-__global__ __launch_bounds__(kPTB) void pcgUp0(PLev<T, TV> L, const T *__restrict__ u, const T *__restrict__ eNext,
-                                               const T *__restrict__ r, T *__restrict__ z, T oc, const int *done,
-                                               double *__restrict__ rzPart)
+__global__ __launch_bounds__(kFTB) void pcgUp0(PLev<T, TV> L, const T *__restrict__ y, const T *__restrict__ r,
+                                               T *__restrict__ z, const int *done, double *__restrict__ rzPart)
 {
-  __shared__ double red[kPTB / 64];
+  __shared__ double red[kFTB / 64];
   const int dn = *done;
-  const int row = blockIdx.x * kPRPB + (int)threadIdx.x / kPLPR, sub = threadIdx.x % kPLPR;
-  int kb = 0, ke = 0;
+  const int row = blockIdx.x * kFTB + (int)threadIdx.x;
+  int2 bl = make_int2(0, 0);
+  T ri = (T)0, yi = (T)0, li = (T)0;
   if(row < L.n) {
-    kb = L.ia[row];
-    ke = L.ia[row + 1];
+    bl = L.sell[row];
+    ri = r[row];
+    yi = y[row];
+    li = (T)L.l1[row];
   }
   if(dn) return;
-  const T acc = rowSum<T, TV>(L.ja, L.val, kb, ke, sub, [&](int j) { return fmaT<T>(oc, eNext[L.agg[j]], u[j]); });
+  const T acc = sellSum<T, TV>(L.ja, L.val, bl, [&](int j) { return y[j]; });
   double rz = 0.;
-  if(row < L.n && sub == 0) {
-    const T y = fmaT<T>(oc, eNext[L.agg[row]], u[row]);
-    const T ri = r[row];
-    const T zi = fmaT<T>((T)L.l1[row], ri - acc, y);
+  if(row < L.n) {
+    const T zi = fmaT<T>(li, ri - acc, yi);
     z[row] = zi;
     rz = (double)ri * (double)zi;
   }
-  blockPartial<kPTB>(rz, red, rzPart);
+  blockPartial<kFTB>(rz, red, rzPart);
 }
 
 // Levels from .. coarsest in one workgroup, staged in LDS: b of level from is
 // read from global memory, its correction e written back.
+// The static part of the small-level kernel's LDS (matrices with 16-bit
+// columns, values, l1, aggregates, the coarsest inverse) packed into one
+// global image in the LDS layout, once per numeric set-up.
 template <typename T, typename TV>
-__global__ __launch_bounds__(kPBlockTB) void pcgBlock(const PLev<T, TV> *__restrict__ lv, PBlockPlan pl,
-                                                      const T *__restrict__ cinv, T oc, const int *done)
+__global__ __launch_bounds__(kPBlockTB) void pcgPackBlock(const PLev<T, TV> *__restrict__ lv, PBlockPlan pl,
+                                                          const T *__restrict__ cinvG, unsigned char *img)
 {
-  __shared__ __align__(16) unsigned char lds[kPBlockLds];
-  if(*done) return;
-  const int t = threadIdx.x, nt = blockDim.x;
   const int nLev = pl.coarsest - pl.from;
   for(int s = 0; s < nLev; ++s) {
     const PLev<T, TV> L = lv[pl.from + s];
-    int *ia = (int *)(lds + pl.oIa[s]);
-    unsigned short *ja = (unsigned short *)(lds + pl.oJa[s]);
-    TV *val = (TV *)(lds + pl.oVal[s]), *l1 = (TV *)(lds + pl.oL1[s]);
-    unsigned short *agg = (unsigned short *)(lds + pl.oAgg[s]), *as = (unsigned short *)(lds + pl.oAs[s]);
-    for(int i = t; i <= pl.n[s]; i += nt) ia[i] = L.ia[i];
-    for(int k = t; k < pl.nnz[s]; k += nt) {
-      ja[k] = (unsigned short)L.ja[k];
-      val[k] = L.val[k];
-    }
-    for(int i = t; i < pl.n[s]; i += nt) {
-      l1[i] = L.l1[i];
-      agg[i] = (unsigned short)L.agg[i];
-    }
-    for(int I = t; I <= pl.nc[s]; I += nt) as[I] = (unsigned short)L.aggStart[I];
+    stageLds<2>((int *)(img + pl.oIa[s]), L.ia, pl.n[s] + 1);
+    stageLds<8>((unsigned short *)(img + pl.oJa[s]), L.ja, pl.nnz[s]);
+    stageLds<8>((TV *)(img + pl.oVal[s]), L.val, pl.nnz[s]);
+    stageLds<2>((TV *)(img + pl.oL1[s]), L.l1, pl.n[s]);
+    stageLds<2>((unsigned short *)(img + pl.oAgg[s]), L.agg, pl.n[s]);
+    stageLds<2>((unsigned short *)(img + pl.oAs[s]), L.aggStart, pl.nc[s] + 1);
   }
+  const int nC = pl.n[nLev];
+  stageLds<2>((T *)(img + pl.oCi), cinvG, nC * nC);
+}
+
+template <typename T, typename TV>
+__global__ __launch_bounds__(kPBlockTB) void pcgBlock(const PLev<T, TV> *__restrict__ lv, PBlockPlan pl,
+                                                      const unsigned char *__restrict__ img, T oc, const int *done,
+                                                      T *__restrict__ y0, const T *__restrict__ u0)
+{
+  __shared__ __align__(16) unsigned char lds[kPBlockLds];
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int nLev = pl.coarsest - pl.from;
+  // one pass: the static image (16-byte loads, all issued before their
+  // stores) and the right-hand side of level from
   {
-    T *b = (T *)(lds + pl.oB[0]);
+    constexpr int U = 10;  // 16-byte chunks per thread per batch (160 KB per batch)
+    const int chunks = pl.imgBytes / 16;
     const T *src = lv[pl.from].b;
-    for(int i = t; i < pl.n[0]; i += nt) b[i] = src[i];
+    const int nb = pl.n[0];
+    const int dn = *done;
+    for(int c0 = t; c0 < chunks; c0 += U * nt) {
+      uint4 v[U];
+#pragma unroll
+      for(int u = 0; u < U; ++u) {
+        const int c = c0 + u * nt;
+        v[u] = c < chunks ? ((const uint4 *)img)[c] : make_uint4(0, 0, 0, 0);
+      }
+#pragma unroll
+      for(int u = 0; u < U; ++u) {
+        const int c = c0 + u * nt;
+        if(c < chunks) ((uint4 *)lds)[c] = v[u];
+      }
+    }
+    if(dn) return;
+    stageLds<2>((T *)(lds + pl.oB[0]), src, nb);
   }
   __syncthreads();
   for(int s = 0; s < nLev; ++s) {
@@ -362,8 +582,7 @@ __global__ __launch_bounds__(kPBlockTB) void pcgBlock(const PLev<T, TV> *__restr
     const T *b = (const T *)(lds + pl.oB[s]);
     T *u = (T *)(lds + pl.oU[s]), *r = (T *)(lds + pl.oY[s]), *bn = (T *)(lds + pl.oB[s + 1]);
     for(int i = t; i < n; i += nt) {
-      T acc = (T)0;
-      for(int k = ia[i]; k < ia[i + 1]; ++k) acc += (T)val[k] * ((T)l1[ja[k]] * b[ja[k]]);
+      const T acc = ldsRow<T>(ia[i], ia[i + 1], ja, val, [&](int j) { return (T)l1[j] * b[j]; });
       u[i] = (T)l1[i] * b[i];
       r[i] = b[i] - acc;
     }
@@ -378,11 +597,14 @@ __global__ __launch_bounds__(kPBlockTB) void pcgBlock(const PLev<T, TV> *__restr
   {
     const int nC = pl.n[nLev];
     const T *b = (const T *)(lds + pl.oB[nLev]);
-    T *x = nLev == 0 ? lv[pl.from].e : (T *)(lds + pl.oCx);
+    const T *cinv = (const T *)(lds + pl.oCi);  // from the image
+    T *x = (T *)(lds + pl.oCx);
     for(int i = t; i < nC; i += nt) {
       T acc = (T)0;
+#pragma unroll 8
       for(int j = 0; j < nC; ++j) acc += cinv[i * nC + j] * b[j];
       x[i] = acc;
+      if(nLev == 0) lv[pl.from].e[i] = acc;
     }
     __syncthreads();
   }
@@ -395,73 +617,115 @@ __global__ __launch_bounds__(kPBlockTB) void pcgBlock(const PLev<T, TV> *__restr
     const T *b = (const T *)(lds + pl.oB[s]), *u = (const T *)(lds + pl.oU[s]);
     T *y = (T *)(lds + pl.oY[s]);
     const T *en = (s + 1 == nLev) ? (const T *)(lds + pl.oCx) : (const T *)(lds + pl.oE[s + 1]);
-    T *e = s == 0 ? lv[pl.from].e : (T *)(lds + pl.oE[s]);
+    T *e = (T *)(lds + pl.oE[s]);
+    T *eg = s == 0 ? lv[pl.from].e : nullptr;
     for(int i = t; i < n; i += nt) y[i] = fmaT<T>(oc, en[agg[i]], u[i]);
     __syncthreads();
     for(int i = t; i < n; i += nt) {
-      T acc = (T)0;
-      for(int k = ia[i]; k < ia[i + 1]; ++k) acc += (T)val[k] * y[ja[k]];
-      e[i] = fmaT<T>((T)l1[i], b[i] - acc, y[i]);
+      const T acc = ldsRow<T>(ia[i], ia[i + 1], ja, val, [&](int j) { return y[j]; });
+      const T ei = fmaT<T>((T)l1[i], b[i] - acc, y[i]);
+      e[i] = ei;
+      if(eg) eg[i] = ei;
     }
     __syncthreads();
+  }
+  if(pl.from == 1 && y0) {
+    // level 1 is in this kernel: the level-0 y0 = u0 + oc P e_1
+    const T *e1 = (const T *)(lds + (nLev == 0 ? pl.oCx : pl.oE[0]));
+    const int n0 = lv[0].n;
+    const int *agg0 = lv[0].agg;
+    for(int m = t; m < n0; m += nt) y0[m] = fmaT<T>(oc, e1[agg0[m]], u0[m]);
   }
 }
 
 // ---------------------------------------------------------------------------
 // Numeric set-up
 
-// Level 0: permuted values (the original CSR entry of every permuted entry)
-// and l1 = c / sum |a_ij|, 4 lanes per row.
-template <typename TV>
-__global__ __launch_bounds__(kPTB) void pcgSetup0(int n, const int *__restrict__ iaP, const int *__restrict__ map,
-                                                  const double *__restrict__ val, double c, TV *__restrict__ valP,
-                                                  TV *__restrict__ l1)
+// Level 0: the SELL values (the original CSR entry of every SELL entry),
+// one row per thread, and l1 = c / sum |a_ij|, in FP32 (W32) and / or FP64
+// (W64).
+template <bool W32, bool W64>
+__global__ __launch_bounds__(kPTB) void pcgSetup0(int n, const int2 *__restrict__ sell, const int *__restrict__ map,
+                                                  const double *__restrict__ val, double c,
+                                                  float *__restrict__ val32, float *__restrict__ l1_32,
+                                                  double *__restrict__ val64, double *__restrict__ l1_64)
 {
-  const int row = blockIdx.x * kPRPB + (int)threadIdx.x / kPLPR, sub = threadIdx.x % kPLPR;
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if(row >= n) return;
+  const int2 bl = sell[row];
   double s = 0.;
-  for(int k = iaP[row] + sub; k < iaP[row + 1]; k += kPLPR) {
-    const double v = val[map[k]];
-    valP[k] = (TV)v;
+  for(int k = 0; k < bl.y; ++k) {
+    const int q = bl.x + k * kSlice;
+    const double v = val[map[q]];
+    if(W32) val32[q] = (float)v;
+    if(W64) val64[q] = v;
     s += fabs(v);
   }
-#pragma unroll
-  for(int o = 1; o < kPLPR; o <<= 1) s += __shfl_xor(s, o, 64);
-  if(sub == 0) l1[row] = (TV)(c / s);
+  if(W32) l1_32[row] = (float)(c / s);
+  if(W64) l1_64[row] = c / s;
 }
 
-// Level l >= 1: Galerkin entries as fixed-order sums of the previous level's
-// FP64 entries, their TV copy, and l1; one row per 4 lanes.
+// Level l >= 1: one thread per entry, the Galerkin entry as the fixed-order
+// sum of the previous level's FP64 entries and its TV copy; the same launch
+// sets l1 = c / sum |a_ij| of the previous level's rows when that level is a
+// coarse one (its entries were finished by the previous launch; pn = 0 for
+// level 1, whose previous level is the system).
 template <typename TV>
-__global__ __launch_bounds__(kPTB) void pcgSetupL(int n, const int *__restrict__ ia, const int *__restrict__ cptr,
-                                                  const int *__restrict__ clist, const double *__restrict__ src,
-                                                  double c, double *__restrict__ val64, TV *__restrict__ valT,
-                                                  TV *__restrict__ l1)
+__global__ __launch_bounds__(kPTB) void pcgSetupL(int nnz, const int *__restrict__ cptr, const int *__restrict__ clist,
+                                                  const double *__restrict__ src, double *__restrict__ val64,
+                                                  TV *__restrict__ valT, int pn, const int *__restrict__ pia,
+                                                  double c, TV *__restrict__ pl1)
 {
-  const int row = blockIdx.x * kPRPB + (int)threadIdx.x / kPLPR, sub = threadIdx.x % kPLPR;
-  if(row >= n) return;
-  double s = 0.;
-  for(int k = ia[row] + sub; k < ia[row + 1]; k += kPLPR) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if(t < nnz) {
+    // batches of 8 contributions: list entries, then their values, in flight together
     double v = 0.;
-    for(int m = cptr[k]; m < cptr[k + 1]; ++m) v += src[clist[m]];
-    val64[k] = v;
-    valT[k] = (TV)v;
-    s += fabs(v);
-  }
+    const int m0 = cptr[t], m1 = cptr[t + 1];
+    for(int mb = m0; mb < m1; mb += 8) {
+      int c8[8];
 #pragma unroll
-  for(int o = 1; o < kPLPR; o <<= 1) s += __shfl_xor(s, o, 64);
-  if(sub == 0) l1[row] = (TV)(c / s);
+      for(int u = 0; u < 8; ++u) c8[u] = clist[mb + u < m1 ? mb + u : m0];
+      double s8[8];
+#pragma unroll
+      for(int u = 0; u < 8; ++u) s8[u] = src[c8[u]];
+#pragma unroll
+      for(int u = 0; u < 8; ++u)
+        if(mb + u < m1) v += s8[u];
+    }
+    val64[t] = v;
+    valT[t] = (TV)v;
+  }
+  if(t < pn) {
+    double s = 0.;
+    const int k0 = pia[t], k1 = pia[t + 1];
+    for(int kb = k0; kb < k1; kb += 8) {
+      double s8[8];
+#pragma unroll
+      for(int u = 0; u < 8; ++u) s8[u] = src[kb + u < k1 ? kb + u : k0];
+#pragma unroll
+      for(int u = 0; u < 8; ++u)
+        if(kb + u < k1) s += fabs(s8[u]);
+    }
+    pl1[t] = (TV)(c / s);
+  }
 }
 
 // Dense inverse of the coarsest level (n <= kPDense) by Gauss-Jordan in LDS
 // (FP64, SPD: no pivoting); flag = 1 on a non-positive pivot.
+template <typename TV>
 __global__ __launch_bounds__(1024) void pcgDenseInverse(int n, const int *__restrict__ ia, const int *__restrict__ ja,
                                                         const double *__restrict__ val, double *__restrict__ inv64,
-                                                        float *__restrict__ inv32, int *flag)
+                                                        float *__restrict__ inv32, int *flag, double c,
+                                                        TV *__restrict__ l1)
 {
   __shared__ double M[kPDense][2 * kPDense + 1];
   __shared__ double f[kPDense];
   const int t = threadIdx.x, nt = blockDim.x, w = 2 * n;
+  for(int i = t; i < n; i += nt) {  // l1 of the coarsest level
+    double s = 0.;
+    for(int k = ia[i]; k < ia[i + 1]; ++k) s += fabs(val[k]);
+    l1[i] = (TV)(c / s);
+  }
   for(int q = t; q < n * w; q += nt) M[q / w][q % w] = (q % w) == n + q / w ? 1. : 0.;
   __syncthreads();
   for(int i = t; i < n; i += nt)
@@ -529,7 +793,8 @@ __global__ void pcgZero(int n, T *__restrict__ a, int *done, int *stat)
 
 struct FeAmgPcgLevel {
   int n = 0, nc = 0, nTiles = 0;
-  int64_t nnz = 0;
+  int64_t nnz = 0;   // level 0: SELL entries (padding included)
+  int2 *sell = nullptr;  // level 0: (first SELL position, length) per row
   int *ia = nullptr, *ja = nullptr, *agg = nullptr, *aggStart = nullptr, *tileStart = nullptr, *tileAgg = nullptr;
   int *cptr = nullptr, *clist = nullptr;  // levels >= 1
   double *val64 = nullptr;                // levels >= 1 (level 0: the values are permuted from the system's)
@@ -542,15 +807,19 @@ struct FeAmgPcgLevel {
 struct FeAmgPcg {
   std::vector<FeAmgPcgLevel> lv;
   int *perm = nullptr;  // level-0 permuted row -> original row
-  int *map = nullptr;   // level-0 permuted entry -> original entry
+  int *map = nullptr;   // level-0 SELL entry -> original CSR entry
+  int2 *sell = nullptr;  // level-0 rows: (first SELL position, length)
   float *cinv32 = nullptr;
   double *cinv64 = nullptr;
   int *flag = nullptr;  // non-positive coarsest pivot
   PBlockPlan plan[3];   // (vectors, values): 0 float/float, 1 double/float, 2 double/double
+  unsigned char *img[3] = {nullptr, nullptr, nullptr};  // packed static LDS image per plan
   void *desc[3] = {nullptr, nullptr, nullptr};
   // level-0 solver vectors (permuted), one set per vector type
-  float *x32 = nullptr, *r32[2] = {}, *p32[2] = {}, *q32 = nullptr, *z32 = nullptr, *rhs32 = nullptr;
-  double *x64v = nullptr, *r64v[2] = {}, *p64v[2] = {}, *q64v = nullptr, *z64v = nullptr, *rhs64v = nullptr;
+  float *x32 = nullptr, *r32[2] = {}, *p32[2] = {}, *u32[2] = {}, *q32 = nullptr, *z32 = nullptr, *rhs32 = nullptr,
+        *y32 = nullptr, *w32 = nullptr;
+  double *x64v = nullptr, *r64v[2] = {}, *p64v[2] = {}, *u64v[2] = {}, *q64v = nullptr, *z64v = nullptr,
+         *rhs64v = nullptr, *y64v = nullptr, *w64v = nullptr;
   double *pq = nullptr, *rr = nullptr, *rz = nullptr, *scal = nullptr, *xsave = nullptr;
   int *done = nullptr, *stat = nullptr;
   int g0 = 0;  // grid of the row kernels of level 0
@@ -573,21 +842,21 @@ PLev<T, TV> pview(const FeAmgPcgLevel &h);
 template <>
 PLev<float, float> pview<float, float>(const FeAmgPcgLevel &h)
 {
-  PLev<float, float> L{h.n, h.nc, h.nTiles, h.ia, h.ja, h.val32, h.l1_32, h.agg, h.aggStart, h.tileStart,
+  PLev<float, float> L{h.n, h.nc, h.nTiles, h.ia, h.ja, h.val32, h.l1_32, h.sell, h.agg, h.aggStart, h.tileStart,
                        h.tileAgg, h.b32, h.u32, h.e32};
   return L;
 }
 template <>
 PLev<double, float> pview<double, float>(const FeAmgPcgLevel &h)
 {
-  PLev<double, float> L{h.n, h.nc, h.nTiles, h.ia, h.ja, h.val32, h.l1_32, h.agg, h.aggStart, h.tileStart,
+  PLev<double, float> L{h.n, h.nc, h.nTiles, h.ia, h.ja, h.val32, h.l1_32, h.sell, h.agg, h.aggStart, h.tileStart,
                         h.tileAgg, h.b64, h.u64, h.e64};
   return L;
 }
 template <>
 PLev<double, double> pview<double, double>(const FeAmgPcgLevel &h)
 {
-  PLev<double, double> L{h.n, h.nc, h.nTiles, h.ia, h.ja, h.valD, h.l1_64, h.agg, h.aggStart, h.tileStart,
+  PLev<double, double> L{h.n, h.nc, h.nTiles, h.ia, h.ja, h.valD, h.l1_64, h.sell, h.agg, h.aggStart, h.tileStart,
                          h.tileAgg, h.b64, h.u64, h.e64};
   return L;
 }
@@ -611,7 +880,8 @@ PBlockPlan pcgPlan(const FeAmgPcg &A)
     pl = PBlockPlan();
     pl.from = from;
     pl.coarsest = Lc;
-    for(int l = from; l < Lc; ++l) {
+    const int nC = A.lv[Lc].n;
+    for(int l = from; l < Lc; ++l) {  // static arrays
       const int s = l - from;
       const FeAmgPcgLevel &h = A.lv[l];
       pl.n[s] = h.n;
@@ -623,12 +893,17 @@ PBlockPlan pcgPlan(const FeAmgPcg &A)
       pl.oL1[s] = take(sizeof(TV) * h.n);
       pl.oAgg[s] = take(2 * (size_t)h.n);
       pl.oAs[s] = take(2 * (size_t)(h.nc + 1));
+    }
+    pl.oCi = take(sizeof(T) * nC * nC);
+    pl.imgBytes = bytes;
+    for(int l = from; l < Lc; ++l) {  // vectors
+      const int s = l - from;
+      const FeAmgPcgLevel &h = A.lv[l];
       pl.oB[s] = take(sizeof(T) * h.n);
       pl.oU[s] = take(sizeof(T) * h.n);
       pl.oY[s] = take(sizeof(T) * h.n);
       pl.oE[s] = take(sizeof(T) * h.n);
     }
-    const int nC = A.lv[Lc].n;
     pl.n[Lc - from] = nC;
     pl.oB[Lc - from] = take(sizeof(T) * nC);
     pl.oCx = take(sizeof(T) * nC);
@@ -698,11 +973,36 @@ int pcgSetupPattern(fe_hip_ctx *ctx)
       ia[ip + 1] = ia[ip] + (int)row.size();
     }
     if(int rc = upload(ctx, &d.ia, ia)) return rc;
-    if(int rc = upload(ctx, &d.ja, ja)) return rc;
     if(l == 0) {
+      // SELL layout: slices of kSlice rows, slot k of a slice's rows
+      // contiguous; each original entry's SELL position for level 1's sums
+      const int ns = (h.n + kSlice - 1) / kSlice;
+      std::vector<int2> sell(h.n);
+      int64_t tot = 0;
+      for(int sl = 0; sl < ns; ++sl) {
+        int w = 0;
+        for(int r = sl * kSlice; r < std::min(h.n, (sl + 1) * kSlice); ++r) w = std::max(w, ia[r + 1] - ia[r]);
+        for(int r = sl * kSlice; r < std::min(h.n, (sl + 1) * kSlice); ++r)
+          sell[r] = make_int2((int)(tot + r - sl * kSlice), ia[r + 1] - ia[r]);
+        tot += (int64_t)w * kSlice;
+        FE_REQUIRE(ctx, tot < ((int64_t)1 << 31), "nested multigrid PCG: SELL layout overflows int32");
+      }
+      std::vector<int> scol(tot, 0), smap(tot, 0);
+      for(int r = 0; r < h.n; ++r)
+        for(int m = 0; m < sell[r].y; ++m) {
+          const int q = sell[r].x + m * kSlice;
+          scol[q] = ja[ia[r] + m];
+          smap[q] = ent[ia[r] + m];
+          entPos[0][ent[ia[r] + m]] = q;
+        }
+      d.nnz = tot;
+      if(int rc = upload(ctx, &d.ja, scol)) return rc;
       if(int rc = upload(ctx, &A->perm, order[0])) return rc;
-      if(int rc = upload(ctx, &A->map, ent)) return rc;
+      if(int rc = upload(ctx, &A->map, smap)) return rc;
+      if(int rc = upload(ctx, &A->sell, sell)) return rc;
+      d.sell = A->sell;
     } else {
+      if(int rc = upload(ctx, &d.ja, ja)) return rc;
       // Galerkin lists in permuted numbering: level 1 sums entries of the
       // system's original CSR, deeper levels entries of the permuted level
       // above (same list order as the original lists)
@@ -710,7 +1010,7 @@ int pcgSetupPattern(fe_hip_ctx *ctx)
       clist.reserve(h.clist.size());
       for(int kp = 0; kp < d.nnz; ++kp) {
         const int k = ent[kp];
-        for(int m = h.cptr[k]; m < h.cptr[k + 1]; ++m) clist.push_back(l == 1 ? h.clist[m] : entPos[l - 1][h.clist[m]]);
+        for(int m = h.cptr[k]; m < h.cptr[k + 1]; ++m) clist.push_back(entPos[l - 1][h.clist[m]]);
         cptr[kp + 1] = (int)clist.size();
       }
       if(int rc = upload(ctx, &d.cptr, cptr)) return rc;
@@ -723,13 +1023,15 @@ int pcgSetupPattern(fe_hip_ctx *ctx)
       for(int ip = 0; ip < h.n; ++ip) aggP[ip] = pos[l + 1][h.agg[order[l][ip]]];
       if(int rc = upload(ctx, &d.agg, aggP)) return rc;
       if(int rc = upload(ctx, &d.aggStart, aggStart[l])) return rc;
-      // tiles of whole aggregates, at most kPRPB rows
+      // tiles of whole aggregates, at most kFRPB rows on level 0 (its kernels
+      // run kFTB threads) and kPRPB rows on the others
+      const int lim = l == 0 ? kFRPB : kPRPB;
       std::vector<int> ts(1, 0), ta(1, 0);
       int rows = 0;
       for(int I = 0; I < h.nc; ++I) {
         const int sz = aggStart[l][I + 1] - aggStart[l][I];
-        FE_REQUIRE(ctx, sz <= kPRPB, "nested multigrid PCG: an aggregate is larger than a tile");
-        if(rows + sz > kPRPB) {
+        FE_REQUIRE(ctx, sz <= lim, "nested multigrid PCG: an aggregate is larger than a tile");
+        if(rows + sz > lim) {
           ts.push_back(aggStart[l][I]);
           ta.push_back(I);
           rows = 0;
@@ -764,14 +1066,18 @@ int pcgSetupPattern(fe_hip_ctx *ctx)
   for(int s = 0; s < 2; ++s) {
     if(int rc = feAlloc(ctx, &A->r32[s], n)) return rc;
     if(int rc = feAlloc(ctx, &A->p32[s], n)) return rc;
+    if(int rc = feAlloc(ctx, &A->u32[s], n)) return rc;
     if(int rc = feAlloc(ctx, &A->r64v[s], n)) return rc;
     if(int rc = feAlloc(ctx, &A->p64v[s], n)) return rc;
+    if(int rc = feAlloc(ctx, &A->u64v[s], n)) return rc;
   }
   if(feAlloc(ctx, &A->x32, n) || feAlloc(ctx, &A->q32, n) || feAlloc(ctx, &A->z32, n) || feAlloc(ctx, &A->rhs32, n) ||
+     feAlloc(ctx, &A->y32, n) || feAlloc(ctx, &A->y64v, n) || feAlloc(ctx, &A->w32, n) ||
+     feAlloc(ctx, &A->w64v, n) ||
      feAlloc(ctx, &A->x64v, n) || feAlloc(ctx, &A->q64v, n) || feAlloc(ctx, &A->z64v, n) ||
      feAlloc(ctx, &A->rhs64v, n) || feAlloc(ctx, &A->xsave, n))
     return FE_STATUS_OUT_OF_MEMORY;
-  A->g0 = (n + kPRPB - 1) / kPRPB;
+  A->g0 = (n + kFRPB - 1) / kFRPB;
   const int gMax = std::max(A->g0, A->lv[0].nTiles);
   if(feAlloc(ctx, &A->pq, gMax) || feAlloc(ctx, &A->rr, gMax) || feAlloc(ctx, &A->rz, gMax) ||
      feAlloc(ctx, &A->scal, 8) || feAlloc(ctx, &A->done, 4) || feAlloc(ctx, &A->stat, 4))
@@ -779,6 +1085,9 @@ int pcgSetupPattern(fe_hip_ctx *ctx)
   A->plan[0] = pcgPlan<float, float>(*A);
   A->plan[1] = pcgPlan<double, float>(*A);
   A->plan[2] = pcgPlan<double, double>(*A);
+  for(int s = 0; s < 3; ++s)
+    if(A->plan[s].bytes <= kPBlockLds)
+      if(int rc = feAlloc(ctx, &A->img[s], (size_t)std::max(16, A->plan[s].imgBytes))) return rc;
   {
     std::vector<PLev<float, float>> a;
     std::vector<PLev<double, float>> b;
@@ -803,76 +1112,113 @@ int pcgSetupPattern(fe_hip_ctx *ctx)
 
 // Numeric set-up for the current values (every solve): permuted level-0
 // values and l1, Galerkin sums and l1 per level, dense coarsest inverse.
-int pcgNumeric(fe_hip_ctx *ctx, bool fp64Values)
+// fp64Values: FP64 level values; fp64Product: FP64 level-0 values for the
+// A-product of FP64 inner solves (with FP32 level values otherwise).
+int pcgNumeric(fe_hip_ctx *ctx, bool fp64Values, bool fp64Product, int slot)
 {
   FeAmgPcg &A = *ctx->amgPcg;
   A.c = ctx->amgOmega;
   FeAmgPcgLevel &f = A.lv[0];
-  const int g0 = (f.n + kPRPB - 1) / kPRPB;
+  // the FP64 SELL copy is always written: level 1 sums its entries
+  const int g0 = (f.n + kPTB - 1) / kPTB;
+  (void)fp64Product;
   if(fp64Values)
-    hipLaunchKernelGGL(pcgSetup0<double>, dim3(g0), dim3(kPTB), 0, ctx->stream, f.n, f.ia, A.map, ctx->d_val, A.c,
-                       f.valD, f.l1_64);
+    hipLaunchKernelGGL((pcgSetup0<false, true>), dim3(g0), dim3(kPTB), 0, ctx->stream, f.n, A.sell, A.map, ctx->d_val,
+                       A.c, f.val32, f.l1_32, f.valD, f.l1_64);
   else
-    hipLaunchKernelGGL(pcgSetup0<float>, dim3(g0), dim3(kPTB), 0, ctx->stream, f.n, f.ia, A.map, ctx->d_val, A.c,
-                       f.val32, f.l1_32);
+    hipLaunchKernelGGL((pcgSetup0<true, true>), dim3(g0), dim3(kPTB), 0, ctx->stream, f.n, A.sell, A.map, ctx->d_val,
+                       A.c, f.val32, f.l1_32, f.valD, f.l1_64);
   for(size_t l = 1; l < A.lv.size(); ++l) {
     FeAmgPcgLevel &c = A.lv[l];
-    const double *src = l == 1 ? ctx->d_val : A.lv[l - 1].val64;
-    const int g = (c.n + kPRPB - 1) / kPRPB;
+    const FeAmgPcgLevel &p = A.lv[l - 1];
+    const double *src = l == 1 ? p.valD : p.val64;  // level 1: the FP64 SELL values
+    const int pn = l == 1 ? 0 : p.n;
+    const int g = (int)((std::max<int64_t>(c.nnz, pn) + kPTB - 1) / kPTB);
     if(fp64Values)
-      hipLaunchKernelGGL(pcgSetupL<double>, dim3(g), dim3(kPTB), 0, ctx->stream, c.n, c.ia, c.cptr, c.clist, src,
-                         A.c, c.val64, c.valD, c.l1_64);
+      hipLaunchKernelGGL(pcgSetupL<double>, dim3(g), dim3(kPTB), 0, ctx->stream, (int)c.nnz, c.cptr, c.clist, src,
+                         c.val64, c.valD, pn, p.ia, A.c, p.l1_64);
     else
-      hipLaunchKernelGGL(pcgSetupL<float>, dim3(g), dim3(kPTB), 0, ctx->stream, c.n, c.ia, c.cptr, c.clist, src, A.c,
-                         c.val64, c.val32, c.l1_32);
+      hipLaunchKernelGGL(pcgSetupL<float>, dim3(g), dim3(kPTB), 0, ctx->stream, (int)c.nnz, c.cptr, c.clist, src,
+                         c.val64, c.val32, pn, p.ia, A.c, p.l1_32);
   }
   const FeAmgPcgLevel &C = A.lv.back();
   hipLaunchKernelGGL(pcgZero<int>, dim3(1), dim3(64), 0, ctx->stream, 1, A.flag, A.done, A.stat);
-  hipLaunchKernelGGL(pcgDenseInverse, dim3(1), dim3(1024), 0, ctx->stream, C.n, C.ia, C.ja, C.val64, A.cinv64,
-                     A.cinv32, A.flag);
+  if(fp64Values)
+    hipLaunchKernelGGL(pcgDenseInverse<double>, dim3(1), dim3(1024), 0, ctx->stream, C.n, C.ia, C.ja, C.val64,
+                       A.cinv64, A.cinv32, A.flag, A.c, C.l1_64);
+  else
+    hipLaunchKernelGGL(pcgDenseInverse<float>, dim3(1), dim3(1024), 0, ctx->stream, C.n, C.ia, C.ja, C.val64,
+                       A.cinv64, A.cinv32, A.flag, A.c, C.l1_32);
+  // the static LDS image of the small-level kernel for this slot
+  const PBlockPlan &pl = A.plan[slot];
+  if(slot == 0)
+    hipLaunchKernelGGL((pcgPackBlock<float, float>), dim3(1), dim3(kPBlockTB), 0, ctx->stream,
+                       (const PLev<float, float> *)A.desc[0], pl, (const float *)A.cinv32, A.img[0]);
+  else if(slot == 1)
+    hipLaunchKernelGGL((pcgPackBlock<double, float>), dim3(1), dim3(kPBlockTB), 0, ctx->stream,
+                       (const PLev<double, float> *)A.desc[1], pl, (const double *)A.cinv64, A.img[1]);
+  else
+    hipLaunchKernelGGL((pcgPackBlock<double, double>), dim3(1), dim3(kPBlockTB), 0, ctx->stream,
+                       (const PLev<double, double> *)A.desc[2], pl, (const double *)A.cinv64, A.img[2]);
   FE_HIP_TRY(ctx, hipGetLastError());
   return FE_STATUS_OK;
 }
 
 template <typename T>
+const T *productValues(const FeAmgPcgLevel &h);
+template <>
+const float *productValues<float>(const FeAmgPcgLevel &h)
+{
+  return h.val32;
+}
+template <>
+const double *productValues<double>(const FeAmgPcgLevel &h)
+{
+  return h.valD;
+}
+
+template <typename T>
 struct PcgVecs {
-  T *x, *r[2], *p[2], *q, *z, *rhs;
+  T *x, *r[2], *p[2], *u[2], *q, *z, *rhs, *y, *w;
   const T *cinv;
 };
 PcgVecs<float> vecs32(FeAmgPcg &A)
 {
-  return PcgVecs<float>{A.x32, {A.r32[0], A.r32[1]}, {A.p32[0], A.p32[1]}, A.q32, A.z32, A.rhs32, A.cinv32};
+  return PcgVecs<float>{A.x32,  {A.r32[0], A.r32[1]}, {A.p32[0], A.p32[1]}, {A.u32[0], A.u32[1]}, A.q32,
+                        A.z32,  A.rhs32,              A.y32,                A.w32,                A.cinv32};
 }
 PcgVecs<double> vecs64(FeAmgPcg &A)
 {
-  return PcgVecs<double>{A.x64v, {A.r64v[0], A.r64v[1]}, {A.p64v[0], A.p64v[1]}, A.q64v, A.z64v, A.rhs64v, A.cinv64};
+  return PcgVecs<double>{A.x64v,  {A.r64v[0], A.r64v[1]}, {A.p64v[0], A.p64v[1]}, {A.u64v[0], A.u64v[1]}, A.q64v,
+                         A.z64v,  A.rhs64v,               A.y64v,                 A.w64v,                 A.cinv64};
 }
 
 // Launches the V-cycle after down0: down_l (levels 1 .. B-1), the block
-// kernel, up_l (levels B-1 .. 1), then up0 on rNew into z.
+// kernel, up_l (levels B-1 .. 1; level 1 also writes the level-0 y), then
+// up0 on rNew into z.
 template <typename T, typename TV>
-void vcycleRest(fe_hip_ctx *ctx, FeAmgPcg &A, int slot, const PcgVecs<T> &v, const T *rNew)
+void vcycleRest(fe_hip_ctx *ctx, FeAmgPcg &A, int slot, const PcgVecs<T> &v, const T *rNew, const T *u0)
 {
   const PBlockPlan &pl = A.plan[slot];
   const int B = pl.from;
   const T oc = (T)ctx->amgOverCorrect;
   const PLev<T, TV> *d = (const PLev<T, TV> *)A.desc[slot];
+  const PLev<T, TV> L0 = pview<T, TV>(A.lv[0]);
   for(int l = 1; l < B; ++l) {
     const PLev<T, TV> L = pview<T, TV>(A.lv[l]);
     const PLev<T, TV> N = pview<T, TV>(A.lv[l + 1]);
     hipLaunchKernelGGL((pcgDownL<T, TV>), dim3(L.nTiles), dim3(kPTB), 0, ctx->stream, L, N.b, A.done);
   }
-  hipLaunchKernelGGL((pcgBlock<T, TV>), dim3(1), dim3(kPBlockTB), 0, ctx->stream, d, pl, v.cinv, oc, A.done);
+  hipLaunchKernelGGL((pcgBlock<T, TV>), dim3(1), dim3(kPBlockTB), 0, ctx->stream, d, pl,
+                     (const unsigned char *)A.img[slot], oc, A.done, v.y, u0);
   for(int l = B - 1; l >= 1; --l) {
     const PLev<T, TV> L = pview<T, TV>(A.lv[l]);
     const PLev<T, TV> N = pview<T, TV>(A.lv[l + 1]);
     hipLaunchKernelGGL((pcgUpL<T, TV>), dim3((L.n + kPRPB - 1) / kPRPB), dim3(kPTB), 0, ctx->stream, L, N.e, oc,
-                       A.done);
+                       A.done, l == 1 ? v.y : (T *)nullptr, u0, L0.aggStart);
   }
-  const PLev<T, TV> L0 = pview<T, TV>(A.lv[0]);
-  const PLev<T, TV> L1 = pview<T, TV>(A.lv[1]);
-  hipLaunchKernelGGL((pcgUp0<T, TV>), dim3(A.g0), dim3(kPTB), 0, ctx->stream, L0, (const T *)L0.u, (const T *)L1.e,
-                     rNew, v.z, oc, A.done, A.rz);
+  hipLaunchKernelGGL((pcgUp0<T, TV>), dim3(A.g0), dim3(kFTB), 0, ctx->stream, L0, (const T *)v.y, rNew, v.z, A.done,
+                     A.rz);
 }
 
 // One inner solve of A x = rhs (permuted, vector type T) from x = 0 until
@@ -887,10 +1233,11 @@ int innerSolve(fe_hip_ctx *ctx, FeAmgPcg &A, int slot, const PcgVecs<T> &v, doub
   const PLev<T, TV> L1 = pview<T, TV>(A.lv[1]);
   hipLaunchKernelGGL(pcgZero<T>, dim3(256), dim3(256), 0, ctx->stream, n, v.x, A.done, A.stat);
   // z_0 = M r_0 (down0 with alpha = 0 copies r_0 into r[0])
-  hipLaunchKernelGGL((pcgDown0<T, TV>), dim3(L0.nTiles), dim3(kPTB), 0, ctx->stream, L0, v.x, (const T *)v.p[0],
-                     (const T *)v.q, (const T *)v.rhs, v.r[0], L0.u, L1.b, (const double *)A.pq, A.g0,
-                     (const double *)A.scal, 0, 1, A.done, A.stat, 0, A.rr);
-  vcycleRest<T, TV>(ctx, A, slot, v, v.r[0]);
+  hipLaunchKernelGGL((pcgDown0<T, TV>), dim3(L0.nTiles), dim3(kFTB), 0, ctx->stream, L0, v.x, (const T *)v.p[0],
+                     (const T *)v.q, (const T *)v.w, (const T *)v.rhs, v.r[0], (const T *)v.u[1], v.u[0], L1.b,
+                     (const double *)A.pq, A.g0, (const double *)A.scal, 0, 1, A.done, A.stat, 0, A.rr);
+  vcycleRest<T, TV>(ctx, A, slot, v, v.r[0], v.u[0]);
+  const T *prodVal = productValues<T>(A.lv[0]);
   int *hs = (int *)ctx->h_state;  // pinned: stat[0..1], done
   int k = 0, batch = 12, kPrev = -1;
   double rrPrev = -1.;
@@ -898,15 +1245,18 @@ int innerSolve(fe_hip_ctx *ctx, FeAmgPcg &A, int slot, const PcgVecs<T> &v, doub
     const int kEnd = std::min(maxit, k + batch);
     for(; k < kEnd; ++k) {
       const int s = k & 1;
-      // p_k from z_k and p_k-1; q_k = A p_k
-      hipLaunchKernelGGL((pcgProduct<T, TV>), dim3(A.g0), dim3(kPTB), 0, ctx->stream, n, L0.ia, L0.ja, L0.val,
-                         (const T *)v.z, (const T *)v.p[s ^ 1], v.p[s], v.q, (const double *)A.rz, A.g0,
-                         (const double *)A.rr, L0.nTiles, A.scal, s, k == 0 ? 1 : 0, tol2, A.done, A.stat, k, A.pq);
-      // r_k+1 = r_k - alpha q_k into r[(k + 1) & 1], x += alpha p_k, V-cycle down
-      hipLaunchKernelGGL((pcgDown0<T, TV>), dim3(L0.nTiles), dim3(kPTB), 0, ctx->stream, L0, v.x, (const T *)v.p[s],
-                         (const T *)v.q, (const T *)v.r[s], v.r[s ^ 1], L0.u, L1.b, (const double *)A.pq, A.g0,
-                         (const double *)A.scal, s, 0, A.done, A.stat, k, A.rr);
-      vcycleRest<T, TV>(ctx, A, slot, v, v.r[s ^ 1]);
+      // p_k = z_k + beta p_k-1, q_k = A z_k + beta q_k-1 (level-0 values in the
+      // vector precision), w = l1 q_k
+      hipLaunchKernelGGL((pcgProduct<T, T, TV>), dim3(A.g0), dim3(kFTB), 0, ctx->stream, n, (const int2 *)L0.sell,
+                         L0.ja, prodVal, L0.l1, (const T *)v.z, (const T *)v.p[s ^ 1], v.p[s], v.q, v.w,
+                         (const double *)A.rz, A.g0, (const double *)A.rr, L0.nTiles, A.scal, s, k == 0 ? 1 : 0,
+                         tol2, A.done, A.stat, k, A.pq);
+      // r_k+1 = r_k - alpha q_k into r[(k + 1) & 1], x += alpha p_k, u = l1 r_k+1
+      // into u[(k + 1) & 1] (u[k & 1] = l1 r_k), V-cycle down
+      hipLaunchKernelGGL((pcgDown0<T, TV>), dim3(L0.nTiles), dim3(kFTB), 0, ctx->stream, L0, v.x, (const T *)v.p[s],
+                         (const T *)v.q, (const T *)v.w, (const T *)v.r[s], v.r[s ^ 1], (const T *)v.u[s], v.u[s ^ 1],
+                         L1.b, (const double *)A.pq, A.g0, (const double *)A.scal, s, 0, A.done, A.stat, k, A.rr);
+      vcycleRest<T, TV>(ctx, A, slot, v, v.r[s ^ 1], v.u[s ^ 1]);
     }
     FE_HIP_TRY(ctx, hipGetLastError());
     FE_HIP_TRY(ctx, hipMemcpyAsync(hs, A.stat, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -954,7 +1304,7 @@ int feAmgPcgSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double bnorm, do
   const bool val64 = !fp32 && !ctx->pcValues32;
   const int slot = fp32 ? 0 : (val64 ? 2 : 1);
   FE_REQUIRE(ctx, A.plan[slot].bytes <= kPBlockLds, "nested multigrid PCG: the small levels do not fit the LDS");
-  if(int rc = pcgNumeric(ctx, val64)) return rc;
+  if(int rc = pcgNumeric(ctx, val64, !fp32, slot)) return rc;
   if(!ctx->h_state) FE_HIP_TRY(ctx, hipHostMalloc(&ctx->h_state, 4096));
   const int n = ctx->n;
   double rn = bnorm;
@@ -1002,8 +1352,11 @@ int feAmgPcgSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double bnorm, do
     if(code == 3) res->breakdowns++;
     if(inner32 && !(rnew < 0.9 * rn)) {
       // the FP32 correction did not reduce the FP64 residual enough: continue
-      // with FP64 inner solves, from the previous iterate when it grew
+      // with FP64 inner solves (FP64 product values set up now), from the
+      // previous iterate when it grew
       inner32 = false;
+      FE_REQUIRE(ctx, A.plan[1].bytes <= kPBlockLds, "nested multigrid PCG: the small levels do not fit the LDS");
+      if(int rc = pcgNumeric(ctx, false, true, 1)) return rc;
       if(!(rnew <= rn)) {
         FE_HIP_TRY(ctx, hipMemcpyAsync(x64, A.xsave, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
         double rr0 = 0.;
@@ -1055,7 +1408,7 @@ int feAmgPcgApply(fe_hip_ctx *ctx, const double *r, double *z)
   const bool val64 = !ctx->pcValues32;
   const int slot = val64 ? 2 : 1;
   FE_REQUIRE(ctx, A.plan[slot].bytes <= kPBlockLds, "nested multigrid PCG: the small levels do not fit the LDS");
-  if(int rc = pcgNumeric(ctx, val64)) return rc;
+  if(int rc = pcgNumeric(ctx, val64, false, slot)) return rc;
   const int n = A.lv[0].n;
   const int gp = std::min(1024, (n + 255) / 256);
   PcgVecs<double> v = vecs64(A);
@@ -1063,16 +1416,18 @@ int feAmgPcgApply(fe_hip_ctx *ctx, const double *r, double *z)
   hipLaunchKernelGGL(pcgZero<double>, dim3(gp), dim3(256), 0, ctx->stream, n, v.x, A.done, A.stat);
   if(val64) {
     const PLev<double, double> L0 = pview<double, double>(A.lv[0]), L1 = pview<double, double>(A.lv[1]);
-    hipLaunchKernelGGL((pcgDown0<double, double>), dim3(L0.nTiles), dim3(kPTB), 0, ctx->stream, L0, v.x,
-                       (const double *)v.p[0], (const double *)v.q, (const double *)v.rhs, v.r[0], L0.u, L1.b,
-                       (const double *)A.pq, A.g0, (const double *)A.scal, 0, 1, A.done, A.stat, 0, A.rr);
-    vcycleRest<double, double>(ctx, A, 2, v, v.r[0]);
+    hipLaunchKernelGGL((pcgDown0<double, double>), dim3(L0.nTiles), dim3(kFTB), 0, ctx->stream, L0, v.x,
+                       (const double *)v.p[0], (const double *)v.q, (const double *)v.w, (const double *)v.rhs, v.r[0],
+                       (const double *)v.u[1], v.u[0], L1.b, (const double *)A.pq, A.g0, (const double *)A.scal, 0, 1,
+                       A.done, A.stat, 0, A.rr);
+    vcycleRest<double, double>(ctx, A, 2, v, v.r[0], v.u[0]);
   } else {
     const PLev<double, float> L0 = pview<double, float>(A.lv[0]), L1 = pview<double, float>(A.lv[1]);
-    hipLaunchKernelGGL((pcgDown0<double, float>), dim3(L0.nTiles), dim3(kPTB), 0, ctx->stream, L0, v.x,
-                       (const double *)v.p[0], (const double *)v.q, (const double *)v.rhs, v.r[0], L0.u, L1.b,
-                       (const double *)A.pq, A.g0, (const double *)A.scal, 0, 1, A.done, A.stat, 0, A.rr);
-    vcycleRest<double, float>(ctx, A, 1, v, v.r[0]);
+    hipLaunchKernelGGL((pcgDown0<double, float>), dim3(L0.nTiles), dim3(kFTB), 0, ctx->stream, L0, v.x,
+                       (const double *)v.p[0], (const double *)v.q, (const double *)v.w, (const double *)v.rhs, v.r[0],
+                       (const double *)v.u[1], v.u[0], L1.b, (const double *)A.pq, A.g0, (const double *)A.scal, 0, 1,
+                       A.done, A.stat, 0, A.rr);
+    vcycleRest<double, float>(ctx, A, 1, v, v.r[0], v.u[0]);
   }
   hipLaunchKernelGGL(pcgScatterBack<double>, dim3(gp), dim3(256), 0, ctx->stream, n, A.perm, (const double *)v.z, z);
   FE_HIP_TRY(ctx, hipGetLastError());
