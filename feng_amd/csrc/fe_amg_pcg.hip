@@ -31,8 +31,12 @@
 // The product kernel's first block also checks the residual against the
 // inner tolerance and breakdowns (rho <= 0, (p, q) <= 0) and raises a flag
 // the later kernels read first, so a batch of iterations stops where it
-// converged.  FP32 inner solves run under FP64 iterative refinement on the
-// original system.
+// converged.  FP32 iterations run under FP64 iterative refinement on the
+// original system with residual replacement: when the recursive residual has
+// dropped by the inner tolerance, the FP32 iterate is added to the FP64 one,
+// the FP64 residual replaces the recursive one and the stopped iteration is
+// redone from the kept search direction (no restart, the Krylov history
+// carries over).
 #include "fe_hip_internal.h"
 
 #include <algorithm>
@@ -79,8 +83,33 @@ struct PBlockPlan {
                      // packed once per numeric set-up and copied into LDS in one pass
 };
 
+// Host-mapped (pinned, coherent) words the product kernel posts to, so the
+// host can keep a bounded number of iterations queued ahead of the device and
+// stop launching as soon as the device flags a stop, without synchronizing
+// the stream: progress = the iteration whose product last started, done = the
+// stop code (1 converged, 3 breakdown), stopK = its iteration, rr = the
+// recursive (r, r) at the stop.
+struct PcgMailbox {
+  int progress, done, stopK, pad;
+  double rr;
+};
+
 // ---------------------------------------------------------------------------
 // Device helpers
+
+// System-scope stores into the mailbox (vector stores, no ordering: the host
+// reads rr only after synchronizing the stream, and progress and done are
+// independent hints).
+__device__ __forceinline__ void mbPost(int *p, int v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void mbStop(PcgMailbox *mb, int code, int k, double rr)
+{
+  mb->rr = rr;
+  mbPost(&mb->stopK, k);
+  mbPost(&mb->done, code);
+}
 
 // Sum of G block partials in a fixed order (every block gets the same value).
 template <int TB>
@@ -287,18 +316,21 @@ __device__ __forceinline__ float fmaT<float>(float a, float b, float c)
 
 // rho = (r, z) from the previous up0 partials; p = z + beta p_old and
 // q = A z + beta q_old (= A p; the product gathers z only, beta is applied
-// after the row sum, so the fold of the partials overlaps the gathers);
+// after the row sum, so the fold of the partials overlaps the gathers; p and
+// q are double-buffered, so an iteration can be redone from a replaced
+// residual with the previous direction);
 // w = l1 q for the next restriction; partials of (p, q).  Level-0 SELL
 // values of type TV, one row per thread.
 template <typename T, typename TV, typename TL>
 __global__ __launch_bounds__(kFTB) void pcgProduct(int n, const int2 *__restrict__ sell, const int *__restrict__ col,
                                                    const TV *__restrict__ val, const TL *__restrict__ l1,
                                                    const T *__restrict__ z, const T *__restrict__ pOld,
-                                                   T *__restrict__ pNew, T *__restrict__ q, T *__restrict__ w,
+                                                   T *__restrict__ pNew, const T *__restrict__ qOld,
+                                                   T *__restrict__ q, T *__restrict__ w,
                                                    const double *__restrict__ rzPart, int Grz,
                                                    const double *__restrict__ rrPart, int Grr, double *scal, int slot,
-                                                   int first, double tol2, int *done, int *stat, int k,
-                                                   double *__restrict__ pqPart)
+                                                   int first, double tol2, int *done, int *stat,
+                                                   PcgMailbox *mb, int k, double *__restrict__ pqPart)
 {
   __shared__ double red[kFTB / 64];
   // independent loads first: flag, row bounds, own-row values, partial shares
@@ -312,7 +344,7 @@ __global__ __launch_bounds__(kFTB) void pcgProduct(int n, const int2 *__restrict
     li = (T)l1[row];
     if(!first) {
       pi = pOld[row];
-      qi = q[row];
+      qi = qOld[row];
     }
   }
   const double rzOwn = partialsShare<kFTB>(rzPart, Grz);
@@ -328,13 +360,13 @@ __global__ __launch_bounds__(kFTB) void pcgProduct(int n, const int2 *__restrict
     if(threadIdx.x == 0) {
       scal[slot] = rho;
       scal[2] = rr;
-      if(!(rho > 0.) || !(rr == rr)) {
-        *done = 3;
+      const int code = !(rho > 0.) || !(rr == rr) ? 3 : (!first && rr <= tol2 ? 1 : 0);
+      if(code) {
+        *done = code;
         stat[1] = k;
-      } else if(!first && rr <= tol2) {
-        *done = 1;
-        stat[1] = k;
+        mbStop(mb, code, k, rr);
       }
+      mbPost(&mb->progress, k);
     }
   }
   const T bt = (T)beta;
@@ -364,8 +396,8 @@ __global__ __launch_bounds__(kFTB) void pcgDown0(PLev<T, TV> L, T *__restrict__ 
                                                  const T *__restrict__ rOld, T *__restrict__ rNew,
                                                  const T *__restrict__ uOld, T *__restrict__ u, T *__restrict__ bNext,
                                                  const double *__restrict__ pqPart, int Gpq, const double *scal,
-                                                 int slot, int first, int *done, int *stat, int k,
-                                                 double *__restrict__ rrPart)
+                                                 int slot, int first, int *done, int *stat, PcgMailbox *mb,
+                                                 int k, double *__restrict__ rrPart)
 {
   __shared__ double red[kFTB / 64];
   __shared__ T res[kFRPB];
@@ -410,6 +442,7 @@ __global__ __launch_bounds__(kFTB) void pcgDown0(PLev<T, TV> L, T *__restrict__ 
     else if(t == 0 && threadIdx.x == 0) {
       *done = 3;  // (p, A p) <= 0: no update, the host restarts from the FP64 residual
       stat[1] = k;
+      mbStop(mb, 3, k, 0.);
     }
   }
   const T a = (T)alpha;
@@ -816,12 +849,13 @@ struct FeAmgPcg {
   unsigned char *img[3] = {nullptr, nullptr, nullptr};  // packed static LDS image per plan
   void *desc[3] = {nullptr, nullptr, nullptr};
   // level-0 solver vectors (permuted), one set per vector type
-  float *x32 = nullptr, *r32[2] = {}, *p32[2] = {}, *u32[2] = {}, *q32 = nullptr, *z32 = nullptr, *rhs32 = nullptr,
+  float *x32 = nullptr, *r32[2] = {}, *p32[2] = {}, *u32[2] = {}, *q32[2] = {}, *z32 = nullptr, *rhs32 = nullptr,
         *y32 = nullptr, *w32 = nullptr;
-  double *x64v = nullptr, *r64v[2] = {}, *p64v[2] = {}, *u64v[2] = {}, *q64v = nullptr, *z64v = nullptr,
+  double *x64v = nullptr, *r64v[2] = {}, *p64v[2] = {}, *u64v[2] = {}, *q64v[2] = {}, *z64v = nullptr,
          *rhs64v = nullptr, *y64v = nullptr, *w64v = nullptr;
   double *pq = nullptr, *rr = nullptr, *rz = nullptr, *scal = nullptr, *xsave = nullptr;
   int *done = nullptr, *stat = nullptr;
+  PcgMailbox *mbHost = nullptr, *mbDev = nullptr;  // pinned, mapped: host and device addresses
   int g0 = 0;  // grid of the row kernels of level 0
   double c = 0.;
   double valuesFrom = -1.;  // unused marker
@@ -1070,11 +1104,13 @@ int pcgSetupPattern(fe_hip_ctx *ctx)
     if(int rc = feAlloc(ctx, &A->r64v[s], n)) return rc;
     if(int rc = feAlloc(ctx, &A->p64v[s], n)) return rc;
     if(int rc = feAlloc(ctx, &A->u64v[s], n)) return rc;
+    if(int rc = feAlloc(ctx, &A->q32[s], n)) return rc;
+    if(int rc = feAlloc(ctx, &A->q64v[s], n)) return rc;
   }
-  if(feAlloc(ctx, &A->x32, n) || feAlloc(ctx, &A->q32, n) || feAlloc(ctx, &A->z32, n) || feAlloc(ctx, &A->rhs32, n) ||
+  if(feAlloc(ctx, &A->x32, n) || feAlloc(ctx, &A->z32, n) || feAlloc(ctx, &A->rhs32, n) ||
      feAlloc(ctx, &A->y32, n) || feAlloc(ctx, &A->y64v, n) || feAlloc(ctx, &A->w32, n) ||
      feAlloc(ctx, &A->w64v, n) ||
-     feAlloc(ctx, &A->x64v, n) || feAlloc(ctx, &A->q64v, n) || feAlloc(ctx, &A->z64v, n) ||
+     feAlloc(ctx, &A->x64v, n) || feAlloc(ctx, &A->z64v, n) ||
      feAlloc(ctx, &A->rhs64v, n) || feAlloc(ctx, &A->xsave, n))
     return FE_STATUS_OUT_OF_MEMORY;
   A->g0 = (n + kFRPB - 1) / kFRPB;
@@ -1082,6 +1118,8 @@ int pcgSetupPattern(fe_hip_ctx *ctx)
   if(feAlloc(ctx, &A->pq, gMax) || feAlloc(ctx, &A->rr, gMax) || feAlloc(ctx, &A->rz, gMax) ||
      feAlloc(ctx, &A->scal, 8) || feAlloc(ctx, &A->done, 4) || feAlloc(ctx, &A->stat, 4))
     return FE_STATUS_OUT_OF_MEMORY;
+  FE_HIP_TRY(ctx, hipHostMalloc((void **)&A->mbHost, sizeof(PcgMailbox), hipHostMallocMapped | hipHostMallocCoherent));
+  FE_HIP_TRY(ctx, hipHostGetDevicePointer((void **)&A->mbDev, A->mbHost, 0));
   A->plan[0] = pcgPlan<float, float>(*A);
   A->plan[1] = pcgPlan<double, float>(*A);
   A->plan[2] = pcgPlan<double, double>(*A);
@@ -1179,17 +1217,17 @@ const double *productValues<double>(const FeAmgPcgLevel &h)
 
 template <typename T>
 struct PcgVecs {
-  T *x, *r[2], *p[2], *u[2], *q, *z, *rhs, *y, *w;
+  T *x, *r[2], *p[2], *u[2], *q[2], *z, *rhs, *y, *w;
   const T *cinv;
 };
 PcgVecs<float> vecs32(FeAmgPcg &A)
 {
-  return PcgVecs<float>{A.x32,  {A.r32[0], A.r32[1]}, {A.p32[0], A.p32[1]}, {A.u32[0], A.u32[1]}, A.q32,
+  return PcgVecs<float>{A.x32,  {A.r32[0], A.r32[1]}, {A.p32[0], A.p32[1]}, {A.u32[0], A.u32[1]}, {A.q32[0], A.q32[1]},
                         A.z32,  A.rhs32,              A.y32,                A.w32,                A.cinv32};
 }
 PcgVecs<double> vecs64(FeAmgPcg &A)
 {
-  return PcgVecs<double>{A.x64v,  {A.r64v[0], A.r64v[1]}, {A.p64v[0], A.p64v[1]}, {A.u64v[0], A.u64v[1]}, A.q64v,
+  return PcgVecs<double>{A.x64v,  {A.r64v[0], A.r64v[1]}, {A.p64v[0], A.p64v[1]}, {A.u64v[0], A.u64v[1]}, {A.q64v[0], A.q64v[1]},
                          A.z64v,  A.rhs64v,               A.y64v,                 A.w64v,                 A.cinv64};
 }
 
@@ -1221,72 +1259,91 @@ void vcycleRest(fe_hip_ctx *ctx, FeAmgPcg &A, int slot, const PcgVecs<T> &v, con
                      A.rz);
 }
 
-// One inner solve of A x = rhs (permuted, vector type T) from x = 0 until
-// (r, r) <= tol2, maxit iterations or a breakdown.  Returns the iterations
-// in *iters and the stop code (1 converged, 2 iteration cap, 3 breakdown).
+// Iterations queued ahead of the one whose product the device last started:
+// enough to keep it busy while the host launches, and the most that run as
+// early exits after a stop.
+constexpr int kPcgAhead = 2;
+
+// State of one solve across its segments: the next iteration (its parity
+// picks the double buffers).
+struct PcgRun {
+  int k = 0;
+};
+
+// One segment of PCG iterations on A x = rhs (permuted, vector type T), from
+// x = 0, until (r, r) <= tol2, maxit iterations or a breakdown.
+//   resume = false: a fresh solve, iteration 0 with r = rhs.
+//   resume = true:  a residual replacement.  r = rhs (the caller's FP64
+//     residual of the iterate it has accumulated), z = M r, and iteration
+//     run.k, at which the previous segment stopped, is redone from the kept
+//     p_k-1, q_k-1 = A p_k-1 and rho_k-1, so the Krylov history carries over
+//     instead of restarting.
+// The host launches iteration k once the device has started the product of
+// iteration k - kPcgAhead, and stops launching when the mailbox shows a stop.
+// Returns the segment's iterations in *iters, the stop code (1 converged,
+// 2 iteration cap, 3 breakdown) and the recursive (r, r) at a stop in *rrLast;
+// run.k is left at the iteration that stopped.
 template <typename T, typename TV>
-int innerSolve(fe_hip_ctx *ctx, FeAmgPcg &A, int slot, const PcgVecs<T> &v, double tol2, int maxit, int *iters,
-               int *code)
+int pcgSegment(fe_hip_ctx *ctx, FeAmgPcg &A, int slot, const PcgVecs<T> &v, double tol2, int maxit, bool resume,
+               PcgRun &run, int *iters, int *code, double *rrLast)
 {
   const int n = A.lv[0].n;
   const PLev<T, TV> L0 = pview<T, TV>(A.lv[0]);
   const PLev<T, TV> L1 = pview<T, TV>(A.lv[1]);
-  hipLaunchKernelGGL(pcgZero<T>, dim3(256), dim3(256), 0, ctx->stream, n, v.x, A.done, A.stat);
-  // z_0 = M r_0 (down0 with alpha = 0 copies r_0 into r[0])
-  hipLaunchKernelGGL((pcgDown0<T, TV>), dim3(L0.nTiles), dim3(kFTB), 0, ctx->stream, L0, v.x, (const T *)v.p[0],
-                     (const T *)v.q, (const T *)v.w, (const T *)v.rhs, v.r[0], (const T *)v.u[1], v.u[0], L1.b,
-                     (const double *)A.pq, A.g0, (const double *)A.scal, 0, 1, A.done, A.stat, 0, A.rr);
-  vcycleRest<T, TV>(ctx, A, slot, v, v.r[0], v.u[0]);
-  const T *prodVal = productValues<T>(A.lv[0]);
-  int *hs = (int *)ctx->h_state;  // pinned: stat[0..1], done
-  int k = 0, batch = 12, kPrev = -1;
-  double rrPrev = -1.;
-  for(;;) {
-    const int kEnd = std::min(maxit, k + batch);
-    for(; k < kEnd; ++k) {
-      const int s = k & 1;
-      // p_k = z_k + beta p_k-1, q_k = A z_k + beta q_k-1 (level-0 values in the
-      // vector precision), w = l1 q_k
-      hipLaunchKernelGGL((pcgProduct<T, T, TV>), dim3(A.g0), dim3(kFTB), 0, ctx->stream, n, (const int2 *)L0.sell,
-                         L0.ja, prodVal, L0.l1, (const T *)v.z, (const T *)v.p[s ^ 1], v.p[s], v.q, v.w,
-                         (const double *)A.rz, A.g0, (const double *)A.rr, L0.nTiles, A.scal, s, k == 0 ? 1 : 0,
-                         tol2, A.done, A.stat, k, A.pq);
-      // r_k+1 = r_k - alpha q_k into r[(k + 1) & 1], x += alpha p_k, u = l1 r_k+1
-      // into u[(k + 1) & 1] (u[k & 1] = l1 r_k), V-cycle down
-      hipLaunchKernelGGL((pcgDown0<T, TV>), dim3(L0.nTiles), dim3(kFTB), 0, ctx->stream, L0, v.x, (const T *)v.p[s],
-                         (const T *)v.q, (const T *)v.w, (const T *)v.r[s], v.r[s ^ 1], (const T *)v.u[s], v.u[s ^ 1],
-                         L1.b, (const double *)A.pq, A.g0, (const double *)A.scal, s, 0, A.done, A.stat, k, A.rr);
-      vcycleRest<T, TV>(ctx, A, slot, v, v.r[s ^ 1], v.u[s ^ 1]);
-    }
-    FE_HIP_TRY(ctx, hipGetLastError());
-    FE_HIP_TRY(ctx, hipMemcpyAsync(hs, A.stat, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    FE_HIP_TRY(ctx, hipMemcpyAsync(hs + 2, A.done, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    FE_HIP_TRY(ctx, hipMemcpyAsync((char *)ctx->h_state + 64, A.scal + 2, sizeof(double), hipMemcpyDeviceToHost,
-                                   ctx->stream));
-    FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    const int dn = hs[2];
-    if(dn) {
-      *code = dn;
-      *iters = hs[1] >= 0 ? hs[1] : k;
-      return FE_STATUS_OK;
-    }
-    if(k >= maxit) {
-      *code = 2;
-      *iters = k;
-      return FE_STATUS_OK;
-    }
-    // next batch from the observed reduction of (r, r) (the value at the
-    // start of the last iteration launched)
-    const double rr = *(double *)((char *)ctx->h_state + 64);
-    int next = 8;
-    if(kPrev >= 0 && rr > 0. && rr < rrPrev && k > kPrev) {
-      const double rate = std::log(rr / rrPrev) / (double)(k - kPrev);  // per iteration, of (r, r)
-      next = (int)std::ceil(std::log(tol2 / rr) / rate) + 1;
-    }
-    kPrev = k;
-    rrPrev = rr;
-    batch = std::max(2, std::min(64, next));
+  if(!resume) run.k = 0;
+  const int k0 = run.k;
+  volatile PcgMailbox *mb = A.mbHost;  // no kernel of this context is running: the stream is idle here
+  mb->done = 0;
+  mb->stopK = -1;
+  mb->rr = 0.;
+  mb->progress = k0 - 1;
+  {
+    const int s = k0 & 1;
+    hipLaunchKernelGGL(pcgZero<T>, dim3(256), dim3(256), 0, ctx->stream, n, v.x, A.done, A.stat);
+    // z = M r (down0 in first mode copies rhs into r[s] and forms u[s] = l1 r)
+    hipLaunchKernelGGL((pcgDown0<T, TV>), dim3(L0.nTiles), dim3(kFTB), 0, ctx->stream, L0, v.x, (const T *)v.p[s],
+                       (const T *)v.q[s], (const T *)v.w, (const T *)v.rhs, v.r[s], (const T *)v.u[s ^ 1], v.u[s],
+                       L1.b, (const double *)A.pq, A.g0, (const double *)A.scal, 0, 1, A.done, A.stat, A.mbDev, k0, A.rr);
+    vcycleRest<T, TV>(ctx, A, slot, v, v.r[s], v.u[s]);
   }
+  const T *prodVal = productValues<T>(A.lv[0]);
+  const int kMax = k0 + std::max(0, maxit);
+  int k = k0;
+  for(; k < kMax; ++k) {
+    // wait until the device is at most kPcgAhead iterations behind; the
+    // stream draining ends the wait too (nothing left to wait for)
+    for(int spin = 1; mb->progress < k - kPcgAhead && !mb->done; ++spin)
+      if((spin & 255) == 0 && hipStreamQuery(ctx->stream) != hipErrorNotReady) break;
+    if(mb->done) break;
+    const int s = k & 1;
+    // p_k = z_k + beta p_k-1, q_k = A z_k + beta q_k-1 (level-0 values in the
+    // vector precision), w = l1 q_k
+    hipLaunchKernelGGL((pcgProduct<T, T, TV>), dim3(A.g0), dim3(kFTB), 0, ctx->stream, n, (const int2 *)L0.sell,
+                       L0.ja, prodVal, L0.l1, (const T *)v.z, (const T *)v.p[s ^ 1], v.p[s], (const T *)v.q[s ^ 1],
+                       v.q[s], v.w, (const double *)A.rz, A.g0, (const double *)A.rr, L0.nTiles, A.scal, s,
+                       k == 0 ? 1 : 0, tol2, A.done, A.stat, A.mbDev, k, A.pq);
+    // r_k+1 = r_k - alpha q_k into r[(k + 1) & 1], x += alpha p_k, u = l1 r_k+1
+    // into u[(k + 1) & 1] (u[k & 1] = l1 r_k), V-cycle down
+    hipLaunchKernelGGL((pcgDown0<T, TV>), dim3(L0.nTiles), dim3(kFTB), 0, ctx->stream, L0, v.x, (const T *)v.p[s],
+                       (const T *)v.q[s], (const T *)v.w, (const T *)v.r[s], v.r[s ^ 1], (const T *)v.u[s], v.u[s ^ 1],
+                       L1.b, (const double *)A.pq, A.g0, (const double *)A.scal, s, 0, A.done, A.stat, A.mbDev, k,
+                       A.rr);
+    vcycleRest<T, TV>(ctx, A, slot, v, v.r[s ^ 1], v.u[s ^ 1]);
+  }
+  FE_HIP_TRY(ctx, hipGetLastError());
+  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  const int dn = mb->done;
+  if(dn) {
+    run.k = mb->stopK >= k0 ? mb->stopK : k;
+    *code = dn;
+    *rrLast = mb->rr;
+  } else {
+    run.k = k;
+    *code = 2;
+    *rrLast = 0.;
+  }
+  *iters = run.k - k0;
+  return FE_STATUS_OK;
 }
 
 }  // namespace
@@ -1308,8 +1365,9 @@ int feAmgPcgSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double bnorm, do
   if(!ctx->h_state) FE_HIP_TRY(ctx, hipHostMalloc(&ctx->h_state, 4096));
   const int n = ctx->n;
   double rn = bnorm;
-  bool inner32 = fp32;
-  int iters = 0, outer = 0;
+  bool inner32 = fp32, resume = false;
+  PcgRun run;
+  int iters = 0, outer = 0, lastIt = 0;
   while(outer < 60) {
     res->relres = rn / bnorm;
     if(rn <= target) {
@@ -1321,24 +1379,33 @@ int feAmgPcgSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double bnorm, do
       res->reason = -3;
       break;
     }
-    // inner target: FP32 inner solves reduce by inner_rtol; FP64 ones aim at
-    // the outer target directly
-    double tol2 = inner32 ? (o->inner_rtol * rn) * (o->inner_rtol * rn) : 0.25 * target * target;
-    tol2 = std::max(tol2, 0.01 * target * target);
+    // segment target: FP32 segments reduce the residual by inner_rtol, FP64
+    // ones aim at the outer target directly; neither goes below half of it
+    const double t = inner32 ? std::max(o->inner_rtol * rn, 0.5 * target) : 0.5 * target;
+    // a resumed segment that needs more than twice its predecessor's
+    // iterations has lost the conjugacy it carried over: it is stopped and
+    // the next segment restarts
+    int segMax = o->max_iter - iters;
+    if(resume) segMax = std::min(segMax, 2 * lastIt + 10);
     int it = 0, code = 0;
+    double rrRec = 0.;
     const int gp = std::min(1024, (n + 255) / 256);
     FE_HIP_TRY(ctx, hipMemcpyAsync(A.xsave, x64, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
     if(inner32) {
       PcgVecs<float> v = vecs32(A);
       hipLaunchKernelGGL(pcgPermuteRhs<float>, dim3(gp), dim3(256), 0, ctx->stream, n, A.perm, r64, v.rhs);
-      if(int rc = innerSolve<float, float>(ctx, A, 0, v, tol2, o->max_iter - iters, &it, &code)) return rc;
+      if(int rc = pcgSegment<float, float>(ctx, A, 0, v, t * t, segMax, resume, run, &it, &code, &rrRec))
+        return rc;
       hipLaunchKernelGGL(pcgAddBack<float>, dim3(gp), dim3(256), 0, ctx->stream, n, A.perm, (const float *)v.x, x64);
     } else {
       PcgVecs<double> v = vecs64(A);
       hipLaunchKernelGGL(pcgPermuteRhs<double>, dim3(gp), dim3(256), 0, ctx->stream, n, A.perm, r64, v.rhs);
       if(val64) {
-        if(int rc = innerSolve<double, double>(ctx, A, 2, v, tol2, o->max_iter - iters, &it, &code)) return rc;
-      } else if(int rc = innerSolve<double, float>(ctx, A, 1, v, tol2, o->max_iter - iters, &it, &code)) {
+        if(int rc = pcgSegment<double, double>(ctx, A, 2, v, t * t, segMax, resume, run, &it, &code,
+                                               &rrRec))
+          return rc;
+      } else if(int rc = pcgSegment<double, float>(ctx, A, 1, v, t * t, segMax, resume, run, &it, &code,
+                                                   &rrRec)) {
         return rc;
       }
       hipLaunchKernelGGL(pcgAddBack<double>, dim3(gp), dim3(256), 0, ctx->stream, n, A.perm, (const double *)v.x, x64);
@@ -1350,11 +1417,18 @@ int feAmgPcgSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double bnorm, do
     if(int rc = feResidualNorm2(ctx, x64, r64, &rr)) return rc;
     const double rnew = std::sqrt(rr);
     if(code == 3) res->breakdowns++;
+    // a segment that reached its target continues with the replaced
+    // residual when the FP64 residual is close to the recursive one (the
+    // directions carried over are then still conjugate enough); after a
+    // breakdown, the cap or a larger gap the next segment starts afresh
+    resume = code == 1 && rnew <= 2. * std::sqrt(std::max(rrRec, 0.));
+    lastIt = it;
     if(inner32 && !(rnew < 0.9 * rn)) {
-      // the FP32 correction did not reduce the FP64 residual enough: continue
+      // the FP32 segment did not reduce the FP64 residual enough: continue
       // with FP64 inner solves (FP64 product values set up now), from the
       // previous iterate when it grew
       inner32 = false;
+      resume = false;
       FE_REQUIRE(ctx, A.plan[1].bytes <= kPBlockLds, "nested multigrid PCG: the small levels do not fit the LDS");
       if(int rc = pcgNumeric(ctx, false, true, 1)) return rc;
       if(!(rnew <= rn)) {
@@ -1417,16 +1491,16 @@ int feAmgPcgApply(fe_hip_ctx *ctx, const double *r, double *z)
   if(val64) {
     const PLev<double, double> L0 = pview<double, double>(A.lv[0]), L1 = pview<double, double>(A.lv[1]);
     hipLaunchKernelGGL((pcgDown0<double, double>), dim3(L0.nTiles), dim3(kFTB), 0, ctx->stream, L0, v.x,
-                       (const double *)v.p[0], (const double *)v.q, (const double *)v.w, (const double *)v.rhs, v.r[0],
+                       (const double *)v.p[0], (const double *)v.q[0], (const double *)v.w, (const double *)v.rhs, v.r[0],
                        (const double *)v.u[1], v.u[0], L1.b, (const double *)A.pq, A.g0, (const double *)A.scal, 0, 1,
-                       A.done, A.stat, 0, A.rr);
+                       A.done, A.stat, A.mbDev, 0, A.rr);
     vcycleRest<double, double>(ctx, A, 2, v, v.r[0], v.u[0]);
   } else {
     const PLev<double, float> L0 = pview<double, float>(A.lv[0]), L1 = pview<double, float>(A.lv[1]);
     hipLaunchKernelGGL((pcgDown0<double, float>), dim3(L0.nTiles), dim3(kFTB), 0, ctx->stream, L0, v.x,
-                       (const double *)v.p[0], (const double *)v.q, (const double *)v.w, (const double *)v.rhs, v.r[0],
+                       (const double *)v.p[0], (const double *)v.q[0], (const double *)v.w, (const double *)v.rhs, v.r[0],
                        (const double *)v.u[1], v.u[0], L1.b, (const double *)A.pq, A.g0, (const double *)A.scal, 0, 1,
-                       A.done, A.stat, 0, A.rr);
+                       A.done, A.stat, A.mbDev, 0, A.rr);
     vcycleRest<double, float>(ctx, A, 1, v, v.r[0], v.u[0]);
   }
   hipLaunchKernelGGL(pcgScatterBack<double>, dim3(gp), dim3(256), 0, ctx->stream, n, A.perm, (const double *)v.z, z);
@@ -1453,6 +1527,7 @@ int feAmgPcgLevels(const fe_hip_ctx *ctx, int *levels, int *rows, int64_t *nnz, 
 
 void feAmgPcgRelease(fe_hip_ctx *ctx)
 {
+  if(ctx->amgPcg && ctx->amgPcg->mbHost) (void)hipHostFree(ctx->amgPcg->mbHost);
   delete ctx->amgPcg;  // device buffers are owned by the context
   ctx->amgPcg = nullptr;
 }
