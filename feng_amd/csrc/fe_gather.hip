@@ -27,6 +27,7 @@
 #include "fe_element.h"
 
 #include <algorithm>
+#include <numeric>
 #include <cmath>
 #include <vector>
 
@@ -39,6 +40,7 @@ struct GatherArgs {
   int n, nSlices, what, nq;
   const int *ia, *sliceOff, *conn, *adr;  // adr: E x NF, mesh element order
   const int2 *inc;                        // (element << 4 | local index, packed CSR slots)
+  const int *order;                       // slice of each (block, wavefront) slot
   const double *xyz, *tab, *u;
   FeFn coeff, src, mass;
   int hasDiff, hasSrc, hasMass;
@@ -50,14 +52,41 @@ struct GatherArgs {
   int xcd;
 };
 
-// Slice of this wavefront: with xcd set and a grid that is a multiple of 8,
-// each of the 8 XCDs (block b runs on XCD b % 8) takes one contiguous range
-// of slices, so the elements shared by neighbouring rows stay in one L2.
+// Slot of this block: with xcd set and a grid that is a multiple of 8, each
+// of the 8 XCDs (block b runs on XCD b % 8) takes one contiguous range of
+// slots.  Slots map to slices in Morton order of the slices' element
+// centroids, so the blocks an XCD runs together cover a compact region of the
+// mesh and the elements their rows share stay in that XCD's L2.
 __device__ __forceinline__ int gatherBlock(int xcd)
 {
   const int G = gridDim.x;
   if(!xcd || (G & 7)) return blockIdx.x;
   return (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
+}
+
+// N consecutive ints of one element in the widest aligned loads: 16-byte
+// loads for 4 (element rows of 16 bytes), 8-byte loads for even N (rows of
+// 8k bytes), single ints otherwise.
+template <int N>
+__device__ __forceinline__ void gatherInts(const int *__restrict__ p, int (&v)[N])
+{
+  if constexpr(N == 4) {
+    const int4 w = *(const int4 *)p;
+    v[0] = w.x;
+    v[1] = w.y;
+    v[2] = w.z;
+    v[3] = w.w;
+  } else if constexpr(N % 2 == 0) {
+#pragma unroll
+    for(int i = 0; i < N / 2; ++i) {
+      const int2 w = ((const int2 *)p)[i];
+      v[2 * i] = w.x;
+      v[2 * i + 1] = w.y;
+    }
+  } else {
+#pragma unroll
+    for(int j = 0; j < N; ++j) v[j] = p[j];
+  }
 }
 
 // One lane per CSR row.  TC: the arithmetic type of the element rows and of
@@ -79,9 +108,10 @@ __global__ __launch_bounds__(kGatherTB) void asmGather(GatherArgs a)
   __syncthreads();
   constexpr int NV = DIM + 1;
   constexpr int WPB = kGatherTB / kGatherSlice;
-  const int slice = gatherBlock(a.xcd) * WPB + (int)(threadIdx.x / kGatherSlice);
+  const int slot = gatherBlock(a.xcd) * WPB + (int)(threadIdx.x / kGatherSlice);
   const int lane = threadIdx.x % kGatherSlice;
-  if(slice >= a.nSlices) return;
+  if(slot >= a.nSlices) return;
+  const int slice = a.order[slot];
   const int row = slice * kGatherSlice + lane;
   int k0 = 0, len = 0;
   if(row < a.n) {
@@ -99,22 +129,19 @@ __global__ __launch_bounds__(kGatherTB) void asmGather(GatherArgs a)
     const int p = pw.x;
     if(p < 0) continue;
     const int e = p >> 4, ra = p & 15;
+    int cv[NV], ad[NF];
+    gatherInts<NV>(a.conn + (size_t)e * NV, cv);
+    gatherInts<NF>(a.adr + (size_t)e * NF, ad);
     double c[NV][3];
 #pragma unroll
-    for(int j = 0; j < NV; ++j) {
-      const int v = a.conn[(size_t)e * NV + j];
+    for(int j = 0; j < NV; ++j)
 #pragma unroll
-      for(int d = 0; d < 3; ++d) c[j][d] = a.xyz[3 * (size_t)v + d];
-    }
+      for(int d = 0; d < 3; ++d) c[j][d] = a.xyz[3 * (size_t)cv[j] + d];
     double J, drdx[3], drdy[3], drdz[3];
     feGeometry<DIM>(c, J, drdx, drdy, drdz);
-    int ad[NF];
     double ul[NF];
 #pragma unroll
-    for(int b = 0; b < NF; ++b) {
-      ad[b] = a.adr[(size_t)e * NF + b];
-      ul[b] = doRes ? a.u[ad[b]] : 0.;
-    }
+    for(int b = 0; b < NF; ++b) ul[b] = doRes ? a.u[ad[b]] : 0.;
     TC rowv[NF];
 #pragma unroll
     for(int b = 0; b < NF; ++b) rowv[b] = (TC)0;
@@ -316,9 +343,58 @@ int feGatherPrepare(fe_hip_ctx *ctx, int field, bool *ok)
           make_int2(e << 4 | b, (int)slots);
     }
   }
-  if(feAlloc(ctx, &g.d_sliceOff, off.size()) || feAlloc(ctx, &g.d_inc, inc.size())) return FE_STATUS_OUT_OF_MEMORY;
+  // slice order: Morton code of the mean centroid of the elements incident to
+  // the slice's rows, so the slices that run together on one XCD share
+  // elements and their rows' elements stay in that XCD's L2
+  std::vector<int> order(nSlices);
+  {
+    const int nv = ctx->nvpe;
+    std::vector<int> conn((size_t)E * nv);
+    std::vector<double> xyz((size_t)ctx->V * 3);
+    FE_HIP_TRY(ctx, hipMemcpy(conn.data(), ctx->d_conn, sizeof(int) * conn.size(), hipMemcpyDeviceToHost));
+    FE_HIP_TRY(ctx, hipMemcpy(xyz.data(), ctx->d_xyz, sizeof(double) * xyz.size(), hipMemcpyDeviceToHost));
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for(int v = 0; v < ctx->V; ++v)
+      for(int d = 0; d < 3; ++d) {
+        lo[d] = std::min(lo[d], xyz[3 * (size_t)v + d]);
+        hi[d] = std::max(hi[d], xyz[3 * (size_t)v + d]);
+      }
+    std::vector<double> cen((size_t)nSlices * 3, 0.);
+    std::vector<int> num(nSlices, 0);
+    for(int e = 0; e < E; ++e) {
+      double ce[3] = {0., 0., 0.};
+      for(int j = 0; j < nv; ++j)
+        for(int d = 0; d < 3; ++d) ce[d] += xyz[3 * (size_t)conn[(size_t)e * nv + j] + d] / nv;
+      for(int b = 0; b < nf; ++b) {
+        const int d = adr[(size_t)e * nf + b];
+        if(d >= n) continue;
+        const int sl = d / kGatherSlice;
+        for(int x = 0; x < 3; ++x) cen[3 * (size_t)sl + x] += ce[x];
+        num[sl]++;
+      }
+    }
+    std::vector<uint64_t> key(nSlices);
+    for(int sl = 0; sl < nSlices; ++sl) {
+      uint64_t code = 0;
+      unsigned q[3];
+      for(int d = 0; d < 3; ++d) {
+        const double ext = hi[d] - lo[d];
+        const double t = num[sl] && ext > 0. ? (cen[3 * (size_t)sl + d] / num[sl] - lo[d]) / ext : 0.;
+        q[d] = (unsigned)std::min(1023., std::max(0., t * 1024.));
+      }
+      for(int bit = 9; bit >= 0; --bit)
+        for(int d = 0; d < 3; ++d) code = code << 1 | ((q[d] >> bit) & 1u);
+      key[sl] = code;
+    }
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key[x] < key[y]; });
+  }
+  if(feAlloc(ctx, &g.d_sliceOff, off.size()) || feAlloc(ctx, &g.d_inc, inc.size()) ||
+     feAlloc(ctx, &g.d_order, order.size()))
+    return FE_STATUS_OUT_OF_MEMORY;
   FE_HIP_TRY(ctx, hipMemcpy(g.d_sliceOff, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
   FE_HIP_TRY(ctx, hipMemcpy(g.d_inc, inc.data(), sizeof(int2) * inc.size(), hipMemcpyHostToDevice));
+  FE_HIP_TRY(ctx, hipMemcpy(g.d_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
   g.nSlices = nSlices;
   g.maxR = maxR;
   g.incBytes = (int64_t)inc.size() * 8;
@@ -339,6 +415,7 @@ int feGatherLaunch(fe_hip_ctx *ctx, const FeGatherSweep &s)
   a.ia = ctx->d_ia;
   a.sliceOff = g.d_sliceOff;
   a.inc = g.d_inc;
+  a.order = g.d_order;
   a.conn = ctx->d_conn;
   a.adr = F.adr;
   a.xyz = ctx->d_xyz;
@@ -358,9 +435,12 @@ int feGatherLaunch(fe_hip_ctx *ctx, const FeGatherSweep &s)
   a.val32 = s.val32;
   a.xcd = ctx->gatherXcd;
   const bool f32 = s.fp32Math;
-  if(F.kind == FE_TRI_P1) f32 ? gatherLaunchR<float, 3, 2>(ctx, a, g.maxR) : gatherLaunchR<double, 3, 2>(ctx, a, g.maxR);
-  else if(F.kind == FE_TRI_P2) f32 ? gatherLaunchR<float, 6, 2>(ctx, a, g.maxR) : gatherLaunchR<double, 6, 2>(ctx, a, g.maxR);
-  else if(F.kind == FE_TET_P1) f32 ? gatherLaunchR<float, 4, 3>(ctx, a, g.maxR) : gatherLaunchR<double, 4, 3>(ctx, a, g.maxR);
+  if(F.kind == FE_TRI_P1)
+    f32 ? gatherLaunchR<float, 3, 2>(ctx, a, g.maxR) : gatherLaunchR<double, 3, 2>(ctx, a, g.maxR);
+  else if(F.kind == FE_TRI_P2)
+    f32 ? gatherLaunchR<float, 6, 2>(ctx, a, g.maxR) : gatherLaunchR<double, 6, 2>(ctx, a, g.maxR);
+  else if(F.kind == FE_TET_P1)
+    f32 ? gatherLaunchR<float, 4, 3>(ctx, a, g.maxR) : gatherLaunchR<double, 4, 3>(ctx, a, g.maxR);
   else FE_REQUIRE(ctx, false, "gather assembly: unsupported field kind");
   FE_HIP_TRY(ctx, hipGetLastError());
   return FE_STATUS_OK;

@@ -117,6 +117,7 @@ struct FeGatherField {
   int64_t incBytes = 0;
   int *d_sliceOff = nullptr;
   int2 *d_inc = nullptr;
+  int *d_order = nullptr;  // slices in Morton order of their incident elements' centroids
 };
 // One fused scalar sweep handed to the gather kernel.
 struct FeGatherSweep {
