@@ -7,25 +7,28 @@ n = 250,047 unknowns), k = exp(GRF) with length scale 0.1 and sigma 1.5
 Dirichlet boundary, preconditioned CG stopped on the FP64 true residual
 ||b - A x||_2 / ||b||_2 <= 1e-8.
 
-One step = zero + assemble (all element kernels, colored scatter into the
-device CSR) + solve, inputs already resident in HBM.  Paths:
-  plain   : FP64 assembly + FP64 Jacobi-PCG (pipelined, one kernel/iteration)
-  fast    : FP64 + FP32 (matrix-core) assembly, FP32 Jacobi-PCG inner solves,
-            FP64 iterative refinement
+One step = zero + assemble (the one-launch row-gather assembly of every
+element term into the device CSR and RHS, with the FP32 copy of the values
+on the FP32 paths) + solve, inputs already resident in HBM.  Paths:
+  plain   : FP64 Jacobi-PCG (pipelined, one kernel/iteration)
+  fast    : FP32 Jacobi-PCG inner solves, FP64 iterative refinement
   learned : as fast, with the learned IC(0)-pattern split preconditioner
             (weights from feng_amd/weights/, skipped when absent)
   learned64 : FP64 split PCG with the learned preconditioner
   ic0     : FP64 split IC(0)-pattern PCG (the learned path without the net)
-`value` is the DOF/s of the fastest of fast/learned/learned64 (named in
-config.path);
-every path is reported in config.paths.
+  amg     : FP64 PCG preconditioned by the aggregation-multigrid V-cycle over
+            the nested ordering (FP32 level values)
+  amg32   : the same in FP32, FP64 refinement by residual replacement
+`value` is the DOF/s of the fastest non-plain path (named in config.path);
+every path is reported in config.paths, with its speed-up over plain.
 
 --workload batched runs the 256-problem batched workload instead;
 --workload transient runs BDF2 steps of the heat equation on the same cube;
 --copy-probe runs a plain device copy (counter calibration for rocprofv3).
 
-The cpu_baseline leg times the CPU oracle (a single-threaded C restatement of
-the reference element loop + CG/Jacobi) on the same problem.
+The cpu_baseline leg times the CPU oracle (a C restatement of the reference
+element loop + Jacobi-CG) on the same problem, on all cores of the process's
+affinity (value) and single-threaded (value_1core).
 """
 from __future__ import annotations
 
@@ -77,7 +80,7 @@ PATHS = {
     "learned64": ("fp64", "learned", 1e-3),
     "ic0": ("fp64", "ic0", 1e-3),
     "amg": ("fp64", "amg", 1e-3),
-    "amg32": ("fp32", "amg", 1e-2),
+    "amg32": ("fp32", "amg", 1e-3),
 }
 
 # Timed kernel of each path (the A product of one PCG iteration) and its
@@ -90,13 +93,55 @@ ITER_KERNEL = {
     "learned": ("cgSpmv<float> (split PCG, A product)", 4),
     "learned64": ("spIter<double> (split PCG, A product + updates)", 8),
     "ic0": ("spIter<double> (split PCG, A product + updates)", 8),
-    "amg": ("spIter<double> (AMG PCG, A product + updates)", 8),
-    "amg32": ("spIter<float> (AMG PCG, A product + updates)", 4),
+    "amg": ("pcgDown0<double, float> (level-0 CG update + smoothing + restriction)", 8),
+    "amg32": ("pcgDown0<float, float> (level-0 CG update + smoothing + restriction)", 4),
 }
 
 
 def iter_bytes(s, n, nnz):
     return (s + 4) * nnz + 4 * (n + 1) + 18 * s * n
+
+
+def mg_down0_bytes(s, v, n, nnz, n1):
+    """Algorithmic bytes of one pcgDown0 launch (level 0 of the nested
+    multigrid PCG): SELL values and columns once ((v + 4) nnz, the two row
+    sums share the pass), row descriptors 8 n, the vectors x (read and
+    written), p, q, r_old, r_new, u (written), u_old and w (gathered, each
+    once): 9 s n, l1 v n, and the level-1 right-hand side plus its aggregate
+    bounds (s + 4) n1.  s, v: vector and value bytes."""
+    return (v + 4) * nnz + 8 * n + 9 * s * n + v * n + (s + 4) * n1
+
+
+def mg_iter_bytes(s, v, vprod, rows, nnzs, block_rows=1024):
+    """Algorithmic bytes of one nested multigrid PCG iteration: product
+    ((vprod + 4) nnz + 8 n + 6 s n + v n: z gathered, p_old and q_old read,
+    p, q, w written, l1), down0 (mg_down0_bytes), up0 ((v + 4) nnz + 8 n +
+    3 s n + v n), the level-0 prolongation written by level 1 (2 s n), and per
+    level l >= 1 below the single-workgroup kernel a restriction and a
+    prolongation ((v + 4) nnz_l + 4 n_l + 2 s n_l + v n_l + (s + 4) n_l+1 and
+    (v + 4) nnz_l + 8 n_l + 3 s n_l + v n_l + s n_l+1); the levels inside the
+    single-workgroup kernel count their values and columns once ((v + 2)
+    nnz_l)."""
+    n, nnz = rows[0], nnzs[0]
+    b = (vprod + 4) * nnz + 8 * n + 6 * s * n + v * n
+    b += mg_down0_bytes(s, v, n, nnz, rows[1] if len(rows) > 1 else 0)
+    b += (v + 4) * nnz + 8 * n + 3 * s * n + v * n + 2 * s * n
+    for l in range(1, len(rows)):
+        nl, zl = rows[l], nnzs[l]
+        nn = rows[l + 1] if l + 1 < len(rows) else 0
+        if nl <= block_rows:
+            b += (v + 2) * zl
+            continue
+        b += (v + 4) * zl + 4 * nl + 2 * s * nl + v * nl + (s + 4) * nn
+        b += (v + 4) * zl + 8 * nl + 3 * s * nl + v * nl + s * nn
+    return b
+
+
+def asm_bytes(E, V, n, nnz, nloc=4, dim=3):
+    """Algorithmic bytes of one assembly (the metric's definition): geometric
+    connectivity 4 nloc E, coordinates 8 dim V, per-element coefficient 8 E,
+    CSR values written once 8 nnz, RHS 8 n."""
+    return 4 * nloc * E + 8 * dim * V + 8 * E + 8 * nnz + 8 * n
 
 
 def run_step(S, path, rtol, profile=False):
@@ -113,8 +158,9 @@ def run_step(S, path, rtol, profile=False):
     r = S.solve(pc=pc, precision=prec, rtol=rtol, inner_rtol=inner, check_every=32)
     wall = time.perf_counter() - t0
     spmv_ms, launches = S.kernel_stats(0)
+    iter_ms, _ = S.kernel_stats(2)
     return dict(wall_s=wall, asm_ms=asm_ms, solve_ms=r.ms_solve, iters=r.iterations, outer=r.outer_iterations,
-                relres=r.relres, converged=r.converged, spmv_ms=spmv_ms, spmv_launches=launches)
+                relres=r.relres, converged=r.converged, spmv_ms=spmv_ms, spmv_launches=launches, iter_ms=iter_ms)
 
 
 def cpu_baseline(m, k, num, rtol, budget_s=25.0):
@@ -123,12 +169,12 @@ def cpu_baseline(m, k, num, rtol, budget_s=25.0):
     oracle's own iteration count when it converges, else with the measured
     per-iteration time times the GPU plain path's iteration count.  Timed
     twice: single-threaded, and with the colored OpenMP assembly and threaded
-    CG on all cores of this process's affinity (at most 16)."""
+    CG on all cores of this process's affinity."""
     from tests.oracle_bind import load_oracle
 
     O = load_oracle()
     n = num.nInc
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    threads = max(1, len(os.sched_getaffinity(0)))
     t0 = time.perf_counter()
     ia, ja = O.csr(n, [(num.adr[0], 4, num.adr[0], 4)])
     t_csr = time.perf_counter() - t0
@@ -384,10 +430,28 @@ def main():
     # passes of s n.  FP32 inner solve: s = 4.
     kern_ms = B["prof"]["spmv_ms"]
     kname, sz = ITER_KERNEL[best]
-    alg_bytes = iter_bytes(sz, n, nnz)
-    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    per_iter = None
+    if PATHS[best][1] == "amg":
+        # dominant kernel by profiled time: the level-0 restriction pcgDown0
+        rows_l, nnz_l = B["extra"]["amg_rows"], B["extra"]["amg_nnz"]
+        v = 4  # level values FP32 (pc_values32 = 1)
+        alg_bytes = mg_down0_bytes(sz, v, n, nnz, rows_l[1])
+        it_bytes = mg_iter_bytes(sz, v, 8 if sz == 8 else 4, rows_l, nnz_l)
+        it_ms = B["prof"]["iter_ms"]
+        per_iter = {"alg_bytes": it_bytes, "avg_ms": it_ms,
+                    "frac": it_bytes / (it_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if it_ms else None}
+    else:
+        alg_bytes = iter_bytes(sz, n, nnz)
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9 if kern_ms else 0.
     p_kern = plain["prof"]["spmv_ms"]
     p_alg = iter_bytes(8, n, nnz)
+    b_asm = asm_bytes(m.E, m.V, n, nnz)
+    asm_ms = float(np.mean([x["asm_ms"] for x in B["steps"]]))
+    per_step = None
+    if per_iter:
+        step_bytes = b_asm + per_iter["alg_bytes"] * B["last"]["iters"]
+        per_step = {"alg_bytes": step_bytes, "ms": B["ms"],
+                    "frac": step_bytes / (B["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
     cpu = None
     if not args.no_cpu:
@@ -401,9 +465,10 @@ def main():
         ca = c["all"]
         cpu = {"value": v_all, "unit": "DOF/s", "cores": c["threads"], "kind": "port",
                "value_1core": v_1,
-               "sample": (f"oracle on {c['threads']} threads: colored OpenMP assembly of all {m.E} tets "
-                          f"({ca['t_asm']:.3f}s; coloring {c['t_color']:.2f}s excluded like the CSR build) + threaded "
-                          f"CG/Jacobi {ca['iters']} iterations ({ca['t_cg']:.2f}s, "
+               "sample": (f"oracle on {c['threads']} threads (all cores of the affinity): colored OpenMP "
+                          f"assembly of all {m.E} tets ({ca['t_asm']:.3f}s; coloring {c['t_color']:.2f}s excluded "
+                          f"like the CSR build) + threaded Jacobi-CG (FP64 CG preconditioned by the diagonal, the "
+                          f"plain path's algorithm) {ca['iters']} iterations ({ca['t_cg']:.2f}s, "
                           f"{'converged' if ca['converged'] else 'bounded, extrapolated to'} {it_all} iterations); "
                           f"value_1core: the serial oracle on the same problem")}
         log("cpu", c)
@@ -411,9 +476,10 @@ def main():
     def summary(p):
         r = res[p]
         return {"ms": r["ms"], "dofs": r["dofs"], "iterations": r["last"]["iters"], "outer": r["last"]["outer"],
-                "relres": r["last"]["relres"], "asm_ms": r["prof"]["asm_ms"],
+                "relres": r["last"]["relres"], "asm_ms": float(np.mean([x["asm_ms"] for x in r["steps"]])),
                 "solve_ms": float(np.mean([x["solve_ms"] for x in r["steps"]])),
-                "iter_kernel_ms": r["prof"]["spmv_ms"], **r.get("extra", {})}
+                "iter_kernel_ms": r["prof"]["spmv_ms"], "iteration_ms": r["prof"]["iter_ms"] or None,
+                **r.get("extra", {})}
 
     out = {
         "metric": "DOF/s assemble+solve to FP64 rel-res 1e-8; learned/FP32 vs plain FP64 PCG",
@@ -433,15 +499,20 @@ def main():
                    "path": {"fast": "FP32 Jacobi-PCG + FP64 iterative refinement",
                             "learned": "FP32 learned-IC(0) split PCG + FP64 iterative refinement",
                             "learned64": "FP64 learned-IC(0) split PCG",
-                            "amg": "FP64 PCG with the aggregation-multigrid V-cycle",
-                            "amg32": "FP32 PCG with the aggregation-multigrid V-cycle + FP64 iterative refinement",
+                            "amg": "FP64 PCG with the aggregation-multigrid V-cycle (FP32 level values)",
+                            "amg32": "FP32 PCG with the aggregation-multigrid V-cycle, FP64 refinement by "
+                                     "residual replacement",
                             "plain": "FP64 Jacobi-PCG"}[best],
-                   "paths": {p: summary(p) for p in res},
+                   "paths": {p: {**summary(p), "speedup_vs_plain_fp64": plain["ms"] / res[p]["ms"]} for p in res},
                    "speedup_vs_plain_fp64": plain["ms"] / B["ms"],
                    "plain_iter_roofline_frac": p_alg / (p_kern * 1e-3) / 1e9 / HBM_PEAK_GBS if p_kern else None},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(kname),
-                     "avg_launch_ms": kern_ms, "alg_bytes_per_launch": alg_bytes},
+                     "avg_launch_ms": kern_ms, "alg_bytes_per_launch": alg_bytes,
+                     "per_iteration": per_iter, "per_step": per_step,
+                     "assembly": {"kernel": "asmGather<double, 4, 3, 16>", "ms": asm_ms, "alg_bytes": b_asm,
+                                  "frac": b_asm / (asm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                  "traffic": TRAFFIC.get("asmGather<double, 4, 3, 16>")}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

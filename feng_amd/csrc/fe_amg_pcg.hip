@@ -1308,6 +1308,19 @@ int pcgSegment(fe_hip_ctx *ctx, FeAmgPcg &A, int slot, const PcgVecs<T> &v, doub
   }
   const T *prodVal = productValues<T>(A.lv[0]);
   const int kMax = k0 + std::max(0, maxit);
+  // profiling: events around every down0 and around every whole iteration
+  FeKernelTimer &tk = ctx->spmvTimer, &ti = ctx->iterTimer;
+  tk.used = ti.used = 0;
+  auto mark = [&](FeKernelTimer &t) -> int {
+    if(t.used + 1 > t.ev.size())
+      for(int q = 0; q < 64; ++q) {
+        hipEvent_t e;
+        FE_HIP_TRY(ctx, hipEventCreate(&e));
+        t.ev.push_back(e);
+      }
+    FE_HIP_TRY(ctx, hipEventRecord(t.ev[t.used++], ctx->stream));
+    return FE_STATUS_OK;
+  };
   int k = k0;
   for(; k < kMax; ++k) {
     // wait until the device is at most kPcgAhead iterations behind; the
@@ -1316,19 +1329,27 @@ int pcgSegment(fe_hip_ctx *ctx, FeAmgPcg &A, int slot, const PcgVecs<T> &v, doub
       if((spin & 255) == 0 && hipStreamQuery(ctx->stream) != hipErrorNotReady) break;
     if(mb->done) break;
     const int s = k & 1;
+    if(ctx->profiling)
+      if(int rc = mark(ti)) return rc;
     // p_k = z_k + beta p_k-1, q_k = A z_k + beta q_k-1 (level-0 values in the
     // vector precision), w = l1 q_k
     hipLaunchKernelGGL((pcgProduct<T, T, TV>), dim3(A.g0), dim3(kFTB), 0, ctx->stream, n, (const int2 *)L0.sell,
                        L0.ja, prodVal, L0.l1, (const T *)v.z, (const T *)v.p[s ^ 1], v.p[s], (const T *)v.q[s ^ 1],
                        v.q[s], v.w, (const double *)A.rz, A.g0, (const double *)A.rr, L0.nTiles, A.scal, s,
                        k == 0 ? 1 : 0, tol2, A.done, A.stat, A.mbDev, k, A.pq);
+    if(ctx->profiling)
+      if(int rc = mark(tk)) return rc;
     // r_k+1 = r_k - alpha q_k into r[(k + 1) & 1], x += alpha p_k, u = l1 r_k+1
     // into u[(k + 1) & 1] (u[k & 1] = l1 r_k), V-cycle down
     hipLaunchKernelGGL((pcgDown0<T, TV>), dim3(L0.nTiles), dim3(kFTB), 0, ctx->stream, L0, v.x, (const T *)v.p[s],
                        (const T *)v.q[s], (const T *)v.w, (const T *)v.r[s], v.r[s ^ 1], (const T *)v.u[s], v.u[s ^ 1],
                        L1.b, (const double *)A.pq, A.g0, (const double *)A.scal, s, 0, A.done, A.stat, A.mbDev, k,
                        A.rr);
+    if(ctx->profiling)
+      if(int rc = mark(tk)) return rc;
     vcycleRest<T, TV>(ctx, A, slot, v, v.r[s ^ 1], v.u[s ^ 1]);
+    if(ctx->profiling)
+      if(int rc = mark(ti)) return rc;
   }
   FE_HIP_TRY(ctx, hipGetLastError());
   FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -1343,6 +1364,19 @@ int pcgSegment(fe_hip_ctx *ctx, FeAmgPcg &A, int slot, const PcgVecs<T> &v, doub
     *rrLast = 0.;
   }
   *iters = run.k - k0;
+  if(ctx->profiling) {
+    // the iterations that ran in full (the launches after a stop exit at once)
+    const size_t full = (size_t)std::max(0, run.k - k0);
+    for(FeKernelTimer *t : {&tk, &ti}) {
+      for(size_t q = 0; q < std::min(full, t->used / 2); ++q) {
+        float ms = 0.f;
+        FE_HIP_TRY(ctx, hipEventElapsedTime(&ms, t->ev[2 * q], t->ev[2 * q + 1]));
+        t->total_ms += ms;
+        t->launches++;
+      }
+      t->used = 0;
+    }
+  }
   return FE_STATUS_OK;
 }
 

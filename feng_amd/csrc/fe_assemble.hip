@@ -1165,6 +1165,7 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
   const bool fromZero = ctx->matrixZeroed;
   if(what & 1) ctx->matrixZeroed = false;
   bool allFp32 = true;
+  bool valClean = fromZero;  // no sweep of this call has written the values yet
   const bool ir32 = precision == FE_PREC_FP32_IR && (what & 1);
   auto ensureVal32 = [&]() -> int {
     if(ctx->d_val32) return FE_STATUS_OK;
@@ -1265,6 +1266,10 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
         gs.mass = a.mass;
         gs.val32 = nullptr;
         gs.fp32Math = false;
+        // the first matrix sweep after a zeroing stores its sums outright
+        // (later sweeps of the call add to them, stream-ordered)
+        gs.freshVal = matSweep && valClean;
+        if(matSweep) valClean = false;
         if(ir32 && matSweep) {
           if(int rc = ensureVal32()) return rc;
           gs.val32 = ctx->d_val32;
@@ -1275,6 +1280,7 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
       }
     }
     const bool allBefore = allFp32;
+    if(matSweep) valClean = false;
     if(matSweep) allFp32 = false;  // re-set below when the MFMA pair writes both copies
     if(iDiff >= 0 && !a.nlDiff && iMass < 0 && iReact < 0 && iAdv < 0 && ctx->asmKernel >= 1 && mfmaKind) {
       const FeForm &F = ctx->forms[iDiff];

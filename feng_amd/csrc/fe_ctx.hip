@@ -263,6 +263,7 @@ int fe_hip_ctx_destroy(fe_hip_ctx *ctx)
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   for(hipEvent_t e : ctx->spmvTimer.ev) (void)hipEventDestroy(e);
+  for(hipEvent_t e : ctx->iterTimer.ev) (void)hipEventDestroy(e);
   for(hipEvent_t e : ctx->asmEv)
     if(e) (void)hipEventDestroy(e);
   feMpnnRelease(ctx);
@@ -674,6 +675,7 @@ int fe_hip_set_system(fe_hip_ctx *ctx, const double *val, const double *rhs)
   if(!ctx) return FE_STATUS_POINTER_ERROR;
   FE_REQUIRE(ctx, ctx->csrBuilt, "build the CSR first");
   if(val) FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_val, val, sizeof(double) * ctx->nnz, hipMemcpyHostToDevice, ctx->stream));
+  if(val) ctx->matrixZeroed = false;  // the next assembly adds to these values
   if(rhs) FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_rhs, rhs, sizeof(double) * ctx->n, hipMemcpyHostToDevice, ctx->stream));
   FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return FE_STATUS_OK;
@@ -834,6 +836,9 @@ int fe_hip_kernel_stats(fe_hip_ctx *ctx, int which, double *avg_ms, int64_t *lau
   if(which == 0) {
     if(avg_ms) *avg_ms = ctx->spmvTimer.launches ? ctx->spmvTimer.total_ms / ctx->spmvTimer.launches : 0.;
     if(launches) *launches = ctx->spmvTimer.launches;
+  } else if(which == 2) {
+    if(avg_ms) *avg_ms = ctx->iterTimer.launches ? ctx->iterTimer.total_ms / ctx->iterTimer.launches : 0.;
+    if(launches) *launches = ctx->iterTimer.launches;
   } else if(which == 1) {
     if(ctx->asmPending) {
       float ms = 0.f;

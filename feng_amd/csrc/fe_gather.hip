@@ -49,6 +49,7 @@ struct GatherArgs {
   const double *past[fe_hip_ctx::kMaxPast];
   double *val, *rhs;
   float *val32;  // non-null: also write the FP32 rounding of every updated value
+  int fresh;     // the values are zero and this sweep is their only writer: stored, not read and added
   int xcd;
 };
 
@@ -125,7 +126,11 @@ __global__ __launch_bounds__(kGatherTB) void asmGather(GatherArgs a)
   const int off = a.sliceOff[slice], W = (a.sliceOff[slice + 1] - off) / kGatherSlice;
   const int nq = a.nq;
   for(int k = 0; k < W; ++k) {
-    const int2 pw = a.inc[off + k * kGatherSlice + lane];
+    // the incidence words stream through once: non-temporal, so they do not
+    // displace the element data the neighbouring rows reuse from L2
+    const unsigned long long pk =
+        __builtin_nontemporal_load((const unsigned long long *)&a.inc[off + k * kGatherSlice + lane]);
+    const int2 pw = make_int2((int)(unsigned)pk, (int)(unsigned)(pk >> 32));
     const int p = pw.x;
     if(p < 0) continue;
     const int e = p >> 4, ra = p & 15;
@@ -252,7 +257,7 @@ __global__ __launch_bounds__(kGatherTB) void asmGather(GatherArgs a)
   if(doRes) a.rhs[row] += racc;
   if(doMat) {
     for(int s = 0; s < len; ++s) {
-      const double v = a.val[k0 + s] + (double)acc[s * kGatherTB + threadIdx.x];
+      const double v = (a.fresh ? 0. : a.val[k0 + s]) + (double)acc[s * kGatherTB + threadIdx.x];
       a.val[k0 + s] = v;
       if(a.val32) a.val32[k0 + s] = (float)v;
     }
@@ -433,6 +438,7 @@ int feGatherLaunch(fe_hip_ctx *ctx, const FeGatherSweep &s)
   a.val = ctx->d_val;
   a.rhs = ctx->d_rhs;
   a.val32 = s.val32;
+  a.fresh = s.freshVal ? 1 : 0;
   a.xcd = ctx->gatherXcd;
   const bool f32 = s.fp32Math;
   if(F.kind == FE_TRI_P1)

@@ -127,6 +127,7 @@ struct FeGatherSweep {
   FeFn coeff, src, mass;
   float *val32;   // non-null: write the FP32 rounding of the updated values
   bool fp32Math;  // element rows computed and summed in FP32
+  bool freshVal;  // the values are zero and no earlier sweep of the call wrote them: stored, not read
 };
 
 struct FeKernelTimer {
@@ -279,6 +280,7 @@ struct fe_hip_ctx {
   // measurement
   bool profiling = false;
   FeKernelTimer spmvTimer;
+  FeKernelTimer iterTimer;  // whole iterations of the nested multigrid PCG (all their launches)
   double lastAsmMs = 0.;
   hipEvent_t asmEv[2] = {nullptr, nullptr};  // bracket the last fe_hip_assemble, read lazily
   bool asmPending = false;

@@ -1723,6 +1723,8 @@ int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_sol
   FE_HIP_TRY(ctx, hipEventRecord(t0, ctx->stream));
   ctx->spmvTimer.total_ms = 0.;
   ctx->spmvTimer.launches = 0;
+  ctx->iterTimer.total_ms = 0.;
+  ctx->iterTimer.launches = 0;
 
   hipLaunchKernelGGL(sumSquares, dim3(kGrid), dim3(kTB), 0, ctx->stream, n, ctx->d_rhs, w.part);
   hipLaunchKernelGGL(finalizeSums, dim3(1), dim3(kTB), 0, ctx->stream, w.part, kGrid, 1, w.res + 8);

@@ -229,8 +229,12 @@ int fe_hip_get_factor(fe_hip_ctx *ctx, double *lu);
 
 /* ---- measurement -------------------------------------------------------- */
 
-/* Average device time (ms, hipEvents on the context stream) of the last
- * `fe_hip_solve`'s dominant kernel (the SpMV sweep) and its launch count. */
+/* Average device time (ms, hipEvents on the context stream) and launch count
+ * of the last `fe_hip_solve`'s dominant kernel (which = 0: the iteration
+ * kernel of the Jacobi, IC(0) and learned paths, the level-0 restriction
+ * `pcgDown0` of the nested multigrid PCG), of the last assembly (which = 1),
+ * or of one whole nested multigrid PCG iteration with all its launches
+ * (which = 2).  Kernel and iteration times are taken with profiling on. */
 int fe_hip_kernel_stats(fe_hip_ctx *ctx, int which, double *avg_ms, int64_t *launches);
 /* Enable per-launch event timing of the dominant kernel (adds two events per
  * launch). */

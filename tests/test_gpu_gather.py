@@ -95,6 +95,33 @@ def test_gather_matches_oracle_and_colored_sweep(oracle, name):
     S.close()
 
 
+def test_gather_stores_after_a_zeroing_and_adds_otherwise():
+    """The first matrix sweep after setToZero stores its sums without reading
+    the values; every other assembly adds to what is there: a second assembly
+    without zeroing doubles the values, and values set from the host are
+    added to."""
+    from feng_amd.problems import diffusion_system
+    print("gather store / add", flush=True)
+    m, kind, deg, k, s = _case("tet_grf_12")
+    S, num = diffusion_system(m, kind, deg, _fn(k), _fn(s))
+    S.setToZero()
+    S.assemble(3)
+    v1, r1 = S.get_system()
+    S.assemble(1)
+    v2, _ = S.get_system()
+    assert np.array_equal(v2, 2.0 * v1)
+    base = np.linspace(-1.0, 1.0, v1.size)
+    S.set_system(base, r1)
+    S.assemble(1)
+    v3, _ = S.get_system()
+    assert np.array_equal(v3, base + v1)
+    S.setToZero()
+    S.assemble(1)
+    v4, _ = S.get_system()
+    assert np.array_equal(v4, v1)
+    S.close()
+
+
 def test_gather_transient_mass_matches_colored_sweep():
     from feng_amd.api import FN_CONST, FN_ELEM, Fn, Mesh
     from feng_amd.problems import heat_system
