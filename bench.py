@@ -24,6 +24,7 @@ every path is reported in config.paths, with its speed-up over plain.
 
 --workload batched runs the 256-problem batched workload instead;
 --workload transient runs BDF2 steps of the heat equation on the same cube;
+--workload ns runs Newton solves of the Kovasznay flow (Taylor-Hood, 512^2);
 --copy-probe runs a plain device copy (counter calibration for rocprofv3).
 
 The cpu_baseline leg times the CPU oracle (a C restatement of the reference
@@ -359,6 +360,115 @@ def bench_transient(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_ns(args):
+    """Steady Kovasznay flow (Re = 40) on the jittered N^2 Taylor-Hood P2-P1
+    mesh (the Navier-Stokes configuration: N = 512, n = 2,354,179).  One step
+    = a full Newton solve from the boundary-lifted zero interior state:
+    per Newton iteration the residual and matrix sweeps of the four forms and
+    GMRES(50) with the block Schur preconditioner to rel-res 1e-8, until
+    ||NL(u)||_inf <= 1e-9.  value = unknowns / (wall time of a step).
+    Roofline: the dominant kernel by profiled time, the projection multi-dot
+    of the Arnoldi steps (j + 2 vectors of n doubles read by the j-th step's
+    two launches)."""
+    from feng_amd.api import KSP_GMRES, PC_FIELDSPLIT, Mesh, essential_values
+    from feng_amd.problems import NLSolverOptions, kovasznay, navier_stokes_system, solve_newton
+
+    N = args.N
+    m = Mesh.square(N, 0.2, 0)
+    S, num = navier_stokes_system(m, 40.0, 5)
+    n, nnz = S.n, S.nnz
+    _, ufn, pfn = kovasznay(40.0)
+    u0 = essential_values(num, [ufn, pfn])
+    log(f"ns setup: N={N} n={n} nnz={nnz}")
+    restart = 50
+
+    def step(profile=False):
+        S.set_solution(u0)
+        S.set_profiling(profile)
+        lin = []
+        t0 = time.perf_counter()
+        rep = solve_newton(S, NLSolverOptions(1e-9, 1e-12, 1e4, 12), progress=lambda it, r: lin.append(r),
+                           ksp=KSP_GMRES, pc=PC_FIELDSPLIT, rtol=1e-8, restart=restart, max_iter=10000)
+        wall = time.perf_counter() - t0
+        if not rep.converged or not all(r.converged and r.relres <= 1e-8 for r in lin):
+            raise SystemExit(f"ns: Newton or a GMRES solve did not converge: {rep}, {[r.relres for r in lin]}")
+        return dict(wall=wall, newton=rep.iterations, gmres=[r.iterations for r in lin],
+                    solve_ms=[r.ms_solve for r in lin], asm_ms=rep.asm_ms)
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    runs = [step() for _ in range(max(1, args.steps))]
+    wall = float(np.mean([r["wall"] for r in runs]))
+    # one extra solve with events around the dominant kernel (its last linear solve's figures)
+    prof = step(profile=True)
+    kern_ms, launches = S.kernel_stats(0)
+    it_ms, _ = S.kernel_stats(2)
+    last_iters = prof["gmres"][-1]
+    js = [t % restart for t in range(last_iters)]
+    alg_bytes = 8 * n * float(np.mean([j + 2 for j in js]))  # per launch: (j + 1) basis vectors and w
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9 if kern_ms else 0.
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_ns(m, num, S, runs[-1], args.cpu_budget)
+    last = runs[-1]
+    out = {"metric": "DOF/s of a Navier-Stokes Newton solve (assemble + GMRES(50) to rel-res 1e-8 per step)",
+           "value": n / wall, "unit": "DOF/s", "n_gpus": 1, "steps": len(runs), "warmup": args.warmup,
+           "ms_per_step": 1e3 * wall, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic (jittered unit-square mesh, Kovasznay boundary values)",
+           "config": {"workload": f"Kovasznay Re=40, Taylor-Hood P2-P1, jittered {N}^2 square, Newton + GMRES(50) "
+                                  f"with the block Schur preconditioner (field split)", "n": n, "nnz": nnz,
+                      "newton_iterations": last["newton"], "gmres_iterations": last["gmres"],
+                      "solve_ms": last["solve_ms"], "asm_ms": last["asm_ms"]},
+           "roofline": {"bound": "hbm", "kernel": "multiDot (Arnoldi CGS2 projections)", "achieved": achieved,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": None, "avg_launch_ms": kern_ms, "alg_bytes_per_launch": alg_bytes,
+                        "launches_timed": launches, "arnoldi_step_ms": it_ms or None},
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    S.close()
+
+
+def cpu_ns(m, num, S, gpu, budget_s):
+    """The oracle on the host: the Navier-Stokes assembly of one Newton
+    iteration (serial C restatement of the reference's form stack), its
+    ILU(0) and GMRES(50)-ILU(0) (the reference's PETSc combination) for a
+    bounded number of iterations; the step's time extrapolated with the GPU
+    path's Newton and GMRES iteration counts (ILU(0) needs many more
+    iterations than the GPU preconditioner, so the figure favours the CPU)."""
+    import ctypes as C
+
+    from tests.oracle_bind import P, load_oracle
+
+    O = load_oracle()
+    n = S.n
+    ia, ja = S.get_csr()
+    uU, pP = num.adr
+    u = S.get_solution()
+    val = np.zeros(ja.size)
+    rhs = np.zeros(n)
+    t0 = time.perf_counter()
+    rc = O.L.or_assemble_ns(m.V, P(m.xyz, C.c_double), m.E, P(m.conn, C.c_int), P(uU, C.c_int), P(pP, C.c_int), n,
+                            P(ia, C.c_int), P(ja, C.c_int), 5, 1.0 / 40.0, P(u, C.c_double), None,
+                            P(val, C.c_double), P(rhs, C.c_double))
+    t_asm = time.perf_counter() - t0
+    if rc != 0:
+        return None
+    t0 = time.perf_counter()
+    lu, rc = O.ilu0(ia, ja, val)
+    t_ilu = time.perf_counter() - t0
+    probe = 5
+    t0 = time.perf_counter()
+    _, it, rr = O.gmres_ilu0(ia, ja, val, lu, rhs, 50, 1e-30, probe)
+    t_it = (time.perf_counter() - t0) / max(1, it)
+    total_gmres = int(sum(gpu["gmres"]))
+    newton = gpu["newton"]
+    t_step = newton * (t_asm + t_ilu) + t_it * total_gmres
+    return {"value": n / t_step, "unit": "DOF/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle, one thread: Navier-Stokes assembly {t_asm:.2f}s and ILU(0) {t_ilu:.2f}s of one Newton "
+                       f"iteration, GMRES(50)-ILU(0) {it} iterations at {1e3 * t_it:.0f} ms each; step = "
+                       f"{newton} x (assembly + ILU(0)) + {total_gmres} GMRES iterations (the GPU path's counts)")}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -369,7 +479,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=25.0)
-    ap.add_argument("--workload", default="cube", choices=["cube", "batched", "transient"])
+    ap.add_argument("--workload", default="cube", choices=["cube", "batched", "transient", "ns"])
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--paths", default=None, help="comma-separated subset of " + ",".join(PATHS))
     args = ap.parse_args()
@@ -383,6 +493,10 @@ def main():
         return bench_batched(args)
     if args.workload == "transient":
         return bench_transient(args)
+    if args.workload == "ns":
+        if args.N == 64:
+            args.N = 512  # the Navier-Stokes configuration's mesh
+        return bench_ns(args)
     rtol = 1e-8
     m, k, S, num = make_problem(args.N, args.nfeat, args.seed)
     n, nnz = num.nInc, S.nnz

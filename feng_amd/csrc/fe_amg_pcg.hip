@@ -1508,7 +1508,7 @@ bool feAmgNestedUsable(const fe_hip_ctx *ctx)
          ctx->amgHost.back().n <= kPDense && ctx->amgHostMaxAgg <= kPRPB;
 }
 
-int feAmgPcgApply(fe_hip_ctx *ctx, const double *r, double *z)
+int feAmgPcgPrepare(fe_hip_ctx *ctx)
 {
   if(!ctx->amgPcg)
     if(int rc = pcgSetupPattern(ctx)) return rc;
@@ -1517,6 +1517,18 @@ int feAmgPcgApply(fe_hip_ctx *ctx, const double *r, double *z)
   const int slot = val64 ? 2 : 1;
   FE_REQUIRE(ctx, A.plan[slot].bytes <= kPBlockLds, "nested multigrid PCG: the small levels do not fit the LDS");
   if(int rc = pcgNumeric(ctx, val64, false, slot)) return rc;
+  int flag = 0;
+  FE_HIP_TRY(ctx, hipMemcpyAsync(&flag, A.flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  FE_REQUIRE(ctx, flag == 0, "AMG: the coarsest operator is not positive definite");
+  return FE_STATUS_OK;
+}
+
+int feAmgPcgVcycle(fe_hip_ctx *ctx, const double *r, double *z)
+{
+  FE_REQUIRE(ctx, ctx->amgPcg, "nested multigrid: set up before the V-cycle");
+  FeAmgPcg &A = *ctx->amgPcg;
+  const bool val64 = !ctx->pcValues32;
   const int n = A.lv[0].n;
   const int gp = std::min(1024, (n + 255) / 256);
   PcgVecs<double> v = vecs64(A);
@@ -1539,24 +1551,13 @@ int feAmgPcgApply(fe_hip_ctx *ctx, const double *r, double *z)
   }
   hipLaunchKernelGGL(pcgScatterBack<double>, dim3(gp), dim3(256), 0, ctx->stream, n, A.perm, (const double *)v.z, z);
   FE_HIP_TRY(ctx, hipGetLastError());
-  int flag = 0;
-  FE_HIP_TRY(ctx, hipMemcpyAsync(&flag, A.flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  FE_REQUIRE(ctx, flag == 0, "AMG: the coarsest operator is not positive definite");
   return FE_STATUS_OK;
 }
 
-int feAmgPcgLevels(const fe_hip_ctx *ctx, int *levels, int *rows, int64_t *nnz, int cap)
+int feAmgPcgApply(fe_hip_ctx *ctx, const double *r, double *z)
 {
-  *levels = 0;
-  if(!ctx->amgPcg) return FE_STATUS_OK;
-  const FeAmgPcg &A = *ctx->amgPcg;
-  *levels = (int)A.lv.size();
-  for(int l = 0; l < (int)A.lv.size() && l < cap; ++l) {
-    rows[l] = A.lv[l].n;
-    nnz[l] = A.lv[l].nnz;
-  }
-  return FE_STATUS_OK;
+  if(int rc = feAmgPcgPrepare(ctx)) return rc;
+  return feAmgPcgVcycle(ctx, r, z);
 }
 
 void feAmgPcgRelease(fe_hip_ctx *ctx)

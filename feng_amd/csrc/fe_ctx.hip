@@ -270,11 +270,12 @@ int fe_hip_ctx_destroy(fe_hip_ctx *ctx)
   feBatchRelease(ctx);
   feAmgRelease(ctx);
   feAmgPcgRelease(ctx);
+  feFieldSplitRelease(ctx);
   for(void *p : ctx->owned) (void)hipFree(p);
   for(auto &g : ctx->graphs) (void)hipGraphExecDestroy((hipGraphExec_t)g.second);
   ctx->graphs.clear();
   if(ctx->h_state) (void)hipHostFree(ctx->h_state);
-  (void)hipStreamDestroy(ctx->stream);
+  if(!ctx->borrowedStream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return FE_STATUS_OK;
 }
@@ -725,7 +726,8 @@ int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value)
     FE_REQUIRE(ctx, value == 0 || value == 1, "pc_fp32 must be 0 or 1");
     ctx->pcFp32 = value;
   } else if(k == "ilu_sweeps") {
-    FE_REQUIRE(ctx, value >= 1 && value <= 64, "ilu_sweeps must be in [1, 64]");
+    FE_REQUIRE(ctx, value >= 0 && value <= 64,
+               "ilu_sweeps must be 0 (exact triangular solves) or a sweep count in [1, 64]");
     ctx->pc.sweeps = value;
   } else if(k == "ic0_sweeps") {
     FE_REQUIRE(ctx, value >= 0 && value <= 64, "ic0_sweeps must be 0 (exact ILU(0)) or 1..64");

@@ -10,9 +10,13 @@
 // host reads j+2 scalars per Arnoldi step.
 //
 // Preconditioners: Jacobi, or ILU(0) (exact incomplete factors from
-// fe_precond.hip) applied with `sweeps` Jacobi iterations per triangular
-// factor -- the parallel approximate triangular solve: y <- r - L_strict y,
-// z <- D^-1 (y - U_strict z).
+// fe_precond.hip, PETSc's PCILU) applied by exact triangular solves
+// scheduled by dependency levels -- y = L^-1 r over the levels of the lower
+// triangle, z = U^-1 y over those of the upper one; the rows of a level are
+// independent, a level of many rows is one launch, a run of consecutive
+// levels of few rows is one workgroup with a barrier between levels -- or,
+// with `ilu_sweeps` = k > 0, by k Jacobi iterations per factor (the parallel
+// approximate triangular solve: y <- r - L_strict y, z <- D^-1 (y - U_strict z)).
 #include "fe_hip_internal.h"
 
 #include <algorithm>
@@ -58,6 +62,81 @@ __global__ __launch_bounds__(kTB) void factorSweep(int n, const int *__restrict_
     }
     out[i] = s;
   }
+}
+
+constexpr int kTriLPR = 8;      // lanes per row of the exact triangular solves
+constexpr int kTriRunTB = 1024;  // workgroup of a run of small levels
+constexpr int kTriSmall = 256;   // levels of at most this many rows join a run
+
+// Row i of an exact triangular solve: out_i = (scale ? scale_i in0_i : in0_i)
+// + sum_{j != i} g_ij out_j, the out_j of earlier levels.  kTriLPR lanes per
+// row; every lane of the group takes part in the reduction.
+__device__ __forceinline__ void triRow(bool on, int i, int sub, const int *__restrict__ ia, const int *__restrict__ ja,
+                                       const double *__restrict__ g, const double *__restrict__ scale,
+                                       const double *__restrict__ in0, double *out)
+{
+  double s = 0.;
+  if(on)
+    for(int k = ia[i] + sub; k < ia[i + 1]; k += kTriLPR) {
+      const int j = ja[k];
+      if(j != i) s += g[k] * out[j];
+    }
+#pragma unroll
+  for(int o = kTriLPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kTriLPR);
+  if(on && sub == 0) out[i] = (scale ? scale[i] * in0[i] : in0[i]) + s;
+}
+
+// One level of many rows.
+__global__ __launch_bounds__(256) void triLevel(const int *__restrict__ rows, int count, const int *__restrict__ ia,
+                                                const int *__restrict__ ja, const double *__restrict__ g,
+                                                const double *__restrict__ scale, const double *__restrict__ in0,
+                                                double *out)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = t / kTriLPR, sub = t % kTriLPR;
+  const bool on = r < count;
+  triRow(on, on ? rows[r] : 0, sub, ia, ja, g, scale, in0, out);
+}
+
+// Levels l0 .. l1 - 1, each of few rows, in one workgroup: a barrier between
+// levels orders the writes of one level before the reads of the next.
+__global__ __launch_bounds__(kTriRunTB) void triLevelRun(const int *__restrict__ levelPtr, int l0, int l1,
+                                                         const int *__restrict__ rows, const int *__restrict__ ia,
+                                                         const int *__restrict__ ja, const double *__restrict__ g,
+                                                         const double *__restrict__ scale,
+                                                         const double *__restrict__ in0, double *out)
+{
+  constexpr int RPP = kTriRunTB / kTriLPR;  // rows per pass
+  const int sub = threadIdx.x % kTriLPR;
+  for(int l = l0; l < l1; ++l) {
+    const int b = levelPtr[l], cnt = levelPtr[l + 1] - b;
+    for(int r0 = 0; r0 < cnt; r0 += RPP) {
+      const int r = r0 + (int)threadIdx.x / kTriLPR;
+      const bool on = r < cnt;
+      triRow(on, on ? rows[b + r] : 0, sub, ia, ja, g, scale, in0, out);
+    }
+    __syncthreads();
+  }
+}
+
+// Segments of the exact solves over the levels in levelPtr.
+std::vector<FePrecond::TriSeg> triPlan(const std::vector<int> &levelPtr)
+{
+  std::vector<FePrecond::TriSeg> plan;
+  const int nl = (int)levelPtr.size() - 1;
+  int runStart = -1;
+  for(int l = 0; l < nl; ++l) {
+    const int cnt = levelPtr[l + 1] - levelPtr[l];
+    if(cnt <= kTriSmall) {
+      if(runStart < 0) runStart = l;
+      continue;
+    }
+    if(runStart >= 0) plan.push_back({runStart, l, true});
+    runStart = -1;
+    plan.push_back({l, l + 1, false});
+  }
+  if(runStart >= 0) plan.push_back({runStart, nl, true});
+  return plan;
 }
 
 __global__ void scaleVec(int n, const double *__restrict__ d, const double *__restrict__ x, double *__restrict__ y)
@@ -162,17 +241,41 @@ struct Gm {
   }
 
   // z = M^-1 v
-  void precond(const double *v, double *out)
+  int precond(const double *v, double *out)
   {
     if(pc == FE_PC_JACOBI) {
       hipLaunchKernelGGL(scaleVec, dim3(kG), dim3(kTB), 0, ctx->stream, n, dinvJ, v, out);
-      return;
+      return FE_STATUS_OK;
     }
     if(pc == FE_PC_NONE) {
-      (void)hipMemcpyAsync(out, v, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream);
-      return;
+      FE_HIP_TRY(ctx, hipMemcpyAsync(out, v, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+      return FE_STATUS_OK;
     }
-    const FePrecond &P = ctx->pc;
+    if(pc == FE_PC_FIELDSPLIT) return feFieldSplitApply(ctx, v, out);
+    FePrecond &P = ctx->pc;
+    if(sweeps == 0) {
+      // exact: y = L^-1 v (G_L = I - L_strict: y_i = v_i + sum_j<i g_ij y_j), then
+      // z = U^-1 y (z_i = y_i / u_ii + sum_j>i g_ij z_j with g_ij = -u_ij / u_ii)
+      if(P.triL.empty()) P.triL = triPlan(P.levelPtr);
+      if(P.triU.empty()) P.triU = triPlan(P.levelPtrU);
+      auto solve = [&](const std::vector<FePrecond::TriSeg> &plan, const std::vector<int> &lp, const int *dLp,
+                       const int *rows, const int *tia, const int *tja, const double *tg, const double *scale,
+                       const double *in0, double *o) {
+        for(const FePrecond::TriSeg &sg : plan) {
+          if(sg.run) {
+            hipLaunchKernelGGL(triLevelRun, dim3(1), dim3(kTriRunTB), 0, ctx->stream, dLp, sg.l0, sg.l1, rows, tia,
+                               tja, tg, scale, in0, o);
+          } else {
+            const int b = lp[sg.l0], cnt = lp[sg.l0 + 1] - b;
+            hipLaunchKernelGGL(triLevel, dim3((cnt * kTriLPR + 255) / 256), dim3(256), 0, ctx->stream, rows + b, cnt,
+                               tia, tja, tg, scale, in0, o);
+          }
+        }
+      };
+      solve(P.triL, P.levelPtr, P.d_levelPtr, P.d_levelRows, P.d_iaL, P.d_jaL, P.d_gL64, nullptr, v, t1);
+      solve(P.triU, P.levelPtrU, P.d_levelPtrU, P.d_levelRowsU, P.d_iaU, P.d_jaU, P.d_gU64, P.d_dinv64, t1, out);
+      return FE_STATUS_OK;
+    }
     // L sweeps: y_{k+1} = v + G_L,strict y_k (G_L = -L_strict), y_0 = v
     const double *cur = v;
     double *bufs[2] = {t1, t2};
@@ -195,17 +298,55 @@ struct Gm {
       zi ^= 1;
     }
     if(zb[zi] != out)
-      (void)hipMemcpyAsync(out, zb[zi], sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream);
+      FE_HIP_TRY(ctx, hipMemcpyAsync(out, zb[zi], sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    return FE_STATUS_OK;
+  }
+
+  // profiling: an event pair around a launch (timed: the projection
+  // multi-dots of the Arnoldi steps, the dominant kernel, and whole steps)
+  int mark(FeKernelTimer &t)
+  {
+    if(t.used + 1 > t.ev.size())
+      for(int q = 0; q < 64; ++q) {
+        hipEvent_t e;
+        FE_HIP_TRY(ctx, hipEventCreate(&e));
+        t.ev.push_back(e);
+      }
+    FE_HIP_TRY(ctx, hipEventRecord(t.ev[t.used++], ctx->stream));
+    return FE_STATUS_OK;
+  }
+  // complete pairs into the totals (the stream has been synchronized); an
+  // open pair's first event moves to the front
+  int collect(FeKernelTimer &t)
+  {
+    size_t q = 0;
+    for(; q + 1 < t.used; q += 2) {
+      float ms = 0.f;
+      FE_HIP_TRY(ctx, hipEventElapsedTime(&ms, t.ev[q], t.ev[q + 1]));
+      t.total_ms += ms;
+      t.launches++;
+    }
+    if(q < t.used) std::swap(t.ev[0], t.ev[q]);
+    t.used -= q;
+    return FE_STATUS_OK;
   }
 
   // host copies of the first nv folded values
-  int dots(int nv, const double *Vb, const double *w_, double *hHost)
+  int dots(int nv, const double *Vb, const double *w_, double *hHost, bool timed = false)
   {
+    if(timed && ctx->profiling)
+      if(int rc = mark(ctx->spmvTimer)) return rc;
     hipLaunchKernelGGL(multiDot, dim3(kG), dim3(kTB), 0, ctx->stream, n, nv, Vb, ld, w_, part);
+    if(timed && ctx->profiling)
+      if(int rc = mark(ctx->spmvTimer)) return rc;
     hipLaunchKernelGGL(foldMany, dim3(nv), dim3(kTB), 0, ctx->stream, part, kG, h);
     FE_HIP_TRY(ctx, hipGetLastError());
     FE_HIP_TRY(ctx, hipMemcpyAsync(hHost, h, sizeof(double) * nv, hipMemcpyDeviceToHost, ctx->stream));
     FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if(ctx->profiling) {
+      if(int rc = collect(ctx->spmvTimer)) return rc;
+      if(int rc = collect(ctx->iterTimer)) return rc;
+    }
     return FE_STATUS_OK;
   }
 };
@@ -215,8 +356,10 @@ struct Gm {
 int feSolveGmres(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_solve_stats *stats)
 {
   FE_REQUIRE(ctx, o->precision == FE_PREC_FP64, "GMRES runs in FP64");
-  FE_REQUIRE(ctx, o->pc == FE_PC_NONE || o->pc == FE_PC_JACOBI || o->pc == FE_PC_ILU0 || o->pc == FE_PC_BJ_ILU0,
-             "GMRES takes the NONE, JACOBI or ILU0 preconditioner");
+  FE_REQUIRE(ctx,
+             o->pc == FE_PC_NONE || o->pc == FE_PC_JACOBI || o->pc == FE_PC_ILU0 || o->pc == FE_PC_BJ_ILU0 ||
+                 o->pc == FE_PC_FIELDSPLIT,
+             "GMRES takes the NONE, JACOBI, ILU0 or FIELDSPLIT preconditioner");
   const int n = ctx->n;
   const int m = std::max(2, std::min(o->restart, 120));
   Gm g;
@@ -225,7 +368,7 @@ int feSolveGmres(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hi
   g.m = m;
   g.ld = ((size_t)n + 63) & ~(size_t)63;
   g.pc = (o->pc == FE_PC_BJ_ILU0) ? FE_PC_ILU0 : o->pc;
-  g.sweeps = std::max(1, ctx->pc.sweeps);
+  g.sweeps = std::max(0, ctx->pc.sweeps);
   double *work = nullptr;
   const size_t need = g.ld * (m + 1) + 7 * g.ld + (size_t)kG * 128 + 256;
   if(feAlloc(ctx, &work, need)) return FE_STATUS_OUT_OF_MEMORY;
@@ -244,9 +387,13 @@ int feSolveGmres(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hi
   FE_HIP_TRY(ctx, hipEventCreate(&t0));
   FE_HIP_TRY(ctx, hipEventCreate(&t1));
   FE_HIP_TRY(ctx, hipEventRecord(t0, ctx->stream));
+  ctx->spmvTimer = FeKernelTimer{std::move(ctx->spmvTimer.ev)};
+  ctx->iterTimer = FeKernelTimer{std::move(ctx->iterTimer.ev)};
   double setupMs = 0.;
   if(g.pc == FE_PC_ILU0) {
     if(int rc = fePrecondFactor(ctx)) { feFree(ctx, work); return rc; }
+  } else if(g.pc == FE_PC_FIELDSPLIT) {
+    if(int rc = feFieldSplitSetup(ctx)) { feFree(ctx, work); return rc; }
   } else if(g.pc == FE_PC_JACOBI) {
     hipLaunchKernelGGL(jacobiDiag, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, ctx->d_diag, ctx->d_val, g.dinvJ);
   }
@@ -288,12 +435,14 @@ int feSolveGmres(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hi
     bool cycleDone = false;
     for(; j < m && iters < o->max_iter; ++j) {
       double *vj = g.V + g.ld * j;
-      g.precond(vj, g.z);
+      if(ctx->profiling)
+        if(int rc = g.mark(ctx->iterTimer)) return rc;
+      if(int rc = g.precond(vj, g.z)) { feFree(ctx, work); return rc; }
       g.spmv(g.z, g.w);
       // CGS2
-      if(int rc = g.dots(j + 1, g.V, g.w, hh.data())) return rc;
+      if(int rc = g.dots(j + 1, g.V, g.w, hh.data(), true)) return rc;
       hipLaunchKernelGGL(multiAxpy, dim3(kG), dim3(kTB), 0, ctx->stream, n, j + 1, g.V, g.ld, g.h, g.w);
-      if(int rc = g.dots(j + 1, g.V, g.w, hh2.data())) return rc;
+      if(int rc = g.dots(j + 1, g.V, g.w, hh2.data(), true)) return rc;
       hipLaunchKernelGGL(multiAxpy, dim3(kG), dim3(kTB), 0, ctx->stream, n, j + 1, g.V, g.ld, g.h, g.w);
       double wn = 0.;
       if(int rc = g.dots(1, g.w, g.w, &wn)) return rc;
@@ -302,6 +451,8 @@ int feSolveGmres(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hi
       H[(size_t)(j + 1) * m + j] = wn;
       if(wn > 0.)
         hipLaunchKernelGGL(scaleInto, dim3(kG), dim3(kTB), 0, ctx->stream, n, 1. / wn, g.w, g.V + g.ld * (j + 1));
+      if(ctx->profiling)
+        if(int rc = g.mark(ctx->iterTimer)) return rc;
       for(int k = 0; k < j; ++k) {
         const double a = H[(size_t)k * m + j], b = H[(size_t)(k + 1) * m + j];
         H[(size_t)k * m + j] = cs[k] * a + sn[k] * b;
@@ -331,7 +482,7 @@ int feSolveGmres(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hi
     }
     FE_HIP_TRY(ctx, hipMemcpyAsync(g.h, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, ctx->stream));
     hipLaunchKernelGGL(multiCombine, dim3(kG), dim3(kTB), 0, ctx->stream, n, j, g.V, g.ld, g.h, g.w);
-    g.precond(g.w, g.z);
+    if(int rc = g.precond(g.w, g.z)) { feFree(ctx, work); return rc; }
     hipLaunchKernelGGL(axpyKernel, dim3(kG), dim3(kTB), 0, ctx->stream, n, 1., g.z, g.x);
     FE_HIP_TRY(ctx, hipGetLastError());
   }
@@ -351,6 +502,11 @@ int feSolveGmres(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hi
   FE_HIP_TRY(ctx, hipEventElapsedTime(&ms, t0, t1));
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
+  if(ctx->profiling) {
+    if(int rc = g.collect(ctx->spmvTimer)) return rc;
+    if(int rc = g.collect(ctx->iterTimer)) return rc;
+    ctx->spmvTimer.used = ctx->iterTimer.used = 0;
+  }
   if(stats) {
     memset(stats, 0, sizeof(*stats));
     // infinity norms: du, b, A du - b

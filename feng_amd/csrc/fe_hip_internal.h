@@ -154,12 +154,25 @@ struct FePrecond {
   double *d_icL = nullptr, *d_icD = nullptr;  // parallel IC(0) iterates (two of each)
   int lastShifts = 0;
   double lastSetupMs = 0.;
-  int sweeps = 2;  // Jacobi sweeps per triangular factor in GMRES/ILU(0)
+  int sweeps = 0;  // GMRES/ILU(0): 0 = exact level-scheduled triangular solves, k = k Jacobi sweeps per factor
+  // dependency levels of the upper triangle (row i after every j > i of its
+  // row): the order of the exact backward solve; device copies of both level
+  // pointers for the kernels that run several small levels in one workgroup
+  std::vector<int> levelPtrU;
+  int *d_levelRowsU = nullptr, *d_levelPtr = nullptr, *d_levelPtrU = nullptr;
+  // launches of the exact solves: a level of many rows on its own grid, or
+  // a run of consecutive small levels in one workgroup
+  struct TriSeg {
+    int l0, l1;
+    bool run;
+  };
+  std::vector<TriSeg> triL, triU;
 };
 
 struct FeMpnn;   // learned-preconditioner state (fe_learned.hip)
 struct FeAmg;    // aggregation multigrid hierarchy (fe_amg.hip)
 struct FeAmgPcg;  // nested-ordered multigrid PCG (fe_amg_pcg.hip)
+struct FeFieldSplit;  // Schur block preconditioner of saddle-point systems (fe_fieldsplit.hip)
 
 // Host copy of the aggregation hierarchy of the CSR pattern (built by
 // feAmgSetup): per level the CSR pattern, the aggregate of each row (empty
@@ -174,7 +187,9 @@ struct FeBatch;  // batched-solver state (fe_batch.hip)
 struct fe_hip_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  bool borrowedStream = false;  // a sub-context on its parent's stream: not destroyed with it
   std::string err;
+  FeFieldSplit *fieldSplit = nullptr;
 
   // mesh
   int dim = 0, V = 0, E = 0, nvpe = 0;
@@ -366,8 +381,19 @@ int feAmgPcgLevels(const fe_hip_ctx *ctx, int *levels, int *rows, int64_t *nnz, 
 // coarsest level, aggregates that fit a tile (after feAmgSetup).
 bool feAmgNestedUsable(const fe_hip_ctx *ctx);
 // One V-cycle z = M^-1 r through the nested-ordered kernels (device arrays of
-// n doubles, original numbering), FP64 vectors, values per pcValues32.
+// n doubles, original numbering), FP64 vectors, values per pcValues32:
+// feAmgPcgApply = feAmgPcgPrepare (pattern once, numeric set-up of the
+// current values) + feAmgPcgVcycle (the V-cycle alone).
 int feAmgPcgApply(fe_hip_ctx *ctx, const double *r, double *z);
+int feAmgPcgPrepare(fe_hip_ctx *ctx);
+int feAmgPcgVcycle(fe_hip_ctx *ctx, const double *r, double *z);
+
+// Block upper-triangular Schur preconditioner of a saddle-point system
+// (fe_fieldsplit.hip): set-up of the current values, then z = P^-1 r on
+// device vectors of n doubles.
+int feFieldSplitSetup(fe_hip_ctx *ctx);
+int feFieldSplitApply(fe_hip_ctx *ctx, const double *r, double *z);
+void feFieldSplitRelease(fe_hip_ctx *ctx);
 void feAmgPcgRelease(fe_hip_ctx *ctx);
 void feBatchRelease(fe_hip_ctx *ctx);
 

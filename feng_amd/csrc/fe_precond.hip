@@ -257,17 +257,52 @@ int feComputeLevels(fe_hip_ctx *ctx, std::vector<int> &levelPtr, std::vector<int
   return FE_STATUS_OK;
 }
 
+// Host: dependency levels of the upper-triangular DAG (level(i) = 1 + max
+// level of the columns j > i of row i), rows grouped by level.
+int feComputeLevelsUpper(fe_hip_ctx *ctx, std::vector<int> &levelPtr, std::vector<int> &levelRows)
+{
+  const int n = ctx->n;
+  std::vector<int> ia(n + 1), ja(ctx->nnz);
+  FE_HIP_TRY(ctx, hipMemcpy(ia.data(), ctx->d_ia, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+  FE_HIP_TRY(ctx, hipMemcpy(ja.data(), ctx->d_ja, sizeof(int) * ctx->nnz, hipMemcpyDeviceToHost));
+  std::vector<int> lev(n, 0);
+  int nl = 0;
+  for(int i = n - 1; i >= 0; --i) {
+    int l = 0;
+    for(int p = ia[i + 1] - 1; p >= ia[i]; --p) {
+      const int j = ja[p];
+      if(j <= i) break;
+      l = std::max(l, lev[j] + 1);
+    }
+    lev[i] = l;
+    nl = std::max(nl, l + 1);
+  }
+  levelPtr.assign(nl + 1, 0);
+  for(int i = 0; i < n; ++i) levelPtr[lev[i] + 1]++;
+  for(int l = 0; l < nl; ++l) levelPtr[l + 1] += levelPtr[l];
+  std::vector<int> pos(levelPtr.begin(), levelPtr.end() - 1);
+  levelRows.assign(n, 0);
+  for(int i = 0; i < n; ++i) levelRows[pos[lev[i]]++] = i;
+  return FE_STATUS_OK;
+}
+
 int fePrecondSetupPattern(fe_hip_ctx *ctx)
 {
   FePrecond &P = ctx->pc;
   if(P.patternReady) return FE_STATUS_OK;
-  std::vector<int> lrows;
+  std::vector<int> lrows, urows;
   if(int rc = feComputeLevels(ctx, P.levelPtr, lrows)) return rc;
+  if(int rc = feComputeLevelsUpper(ctx, P.levelPtrU, urows)) return rc;
   const int n = ctx->n;
   if(feAlloc(ctx, &P.d_levelRows, n) || feAlloc(ctx, &P.d_lu, (size_t)ctx->nnz) || feAlloc(ctx, &P.d_iaL, n + 1) ||
-     feAlloc(ctx, &P.d_iaU, n + 1) || feAlloc(ctx, &P.d_nShift, 4))
+     feAlloc(ctx, &P.d_iaU, n + 1) || feAlloc(ctx, &P.d_nShift, 4) || feAlloc(ctx, &P.d_levelRowsU, n) ||
+     feAlloc(ctx, &P.d_levelPtr, P.levelPtr.size()) || feAlloc(ctx, &P.d_levelPtrU, P.levelPtrU.size()))
     return FE_STATUS_OUT_OF_MEMORY;
   FE_HIP_TRY(ctx, hipMemcpy(P.d_levelRows, lrows.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  FE_HIP_TRY(ctx, hipMemcpy(P.d_levelRowsU, urows.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  FE_HIP_TRY(ctx, hipMemcpy(P.d_levelPtr, P.levelPtr.data(), sizeof(int) * P.levelPtr.size(), hipMemcpyHostToDevice));
+  FE_HIP_TRY(ctx, hipMemcpy(P.d_levelPtrU, P.levelPtrU.data(), sizeof(int) * P.levelPtrU.size(),
+                            hipMemcpyHostToDevice));
   if(feAlloc(ctx, &P.d_tposA, (size_t)ctx->nnz) || feAlloc(ctx, &P.d_rowOf, (size_t)ctx->nnz))
     return FE_STATUS_OUT_OF_MEMORY;
   hipLaunchKernelGGL(rowIndex, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, ctx->d_ia, P.d_rowOf);

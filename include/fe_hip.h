@@ -81,6 +81,14 @@ typedef struct {
 #define FE_PC_BJ_ILU0 4
 #define FE_PC_LEARNED 5
 #define FE_PC_AMG 6 /* aggregation multigrid V-cycle (PETSc -pc_type gamg role) */
+/* GMRES on saddle-point systems (a field whose diagonal block is zero, e.g.
+ * the pressure of Taylor-Hood Navier-Stokes): block upper-triangular Schur
+ * preconditioner, the role of PETSc's -pc_type fieldsplit
+ * -pc_fieldsplit_type schur -pc_fieldsplit_schur_fact_type upper
+ * -pc_fieldsplit_schur_precondition selfp; the other block by one
+ * aggregation-multigrid V-cycle on its same-component entries, the Schur
+ * complement by the diagonal of A_pu diag(A_uu)^-1 A_up */
+#define FE_PC_FIELDSPLIT 7
 
 #define FE_PREC_FP64 0    /* FP64 Krylov */
 #define FE_PREC_FP32_IR 1 /* FP32 inner Krylov + FP64 iterative refinement */
