@@ -533,6 +533,15 @@ def main():
             res[path]["extra"] = {"amg_rows": rows_l, "amg_nnz": nnz_l}
         log(path, f"{1e3 * wall:.3f} ms/step", steps[-1], res[path].get("extra", ""))
 
+    # FP32 values of the FP32 paths against the FP64 values (those are the
+    # oracle's assembly to 1e-12, as the GPU tests pin)
+    from feng_amd.api import PREC_FP32_IR
+
+    S.setToZero()
+    S.assemble(3, PREC_FP32_IR)
+    v64, _ = S.get_system()
+    v32 = S.get_system32()
+    fp32_err = float(np.abs(v32.astype(np.float64) - v64).max() / np.abs(v64).max())
     fast_paths = [p for p in ("fast", "learned", "learned64", "amg", "amg32") if p in res] or ["plain"]
     best = min(fast_paths, key=lambda p: res[p]["ms"])
     B = res[best]
@@ -619,6 +628,7 @@ def main():
                             "plain": "FP64 Jacobi-PCG"}[best],
                    "paths": {p: {**summary(p), "speedup_vs_plain_fp64": plain["ms"] / res[p]["ms"]} for p in res},
                    "speedup_vs_plain_fp64": plain["ms"] / B["ms"],
+                   "fp32_value_error_vs_fp64": fp32_err,
                    "plain_iter_roofline_frac": p_alg / (p_kern * 1e-3) / 1e9 / HBM_PEAK_GBS if p_kern else None},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(kname),

@@ -2,7 +2,10 @@
 element degree, per-problem Jacobi-PCG in shared launches.  Each problem's
 block of the union system must equal the problem assembled alone (CSR
 bit-exact, values to 1e-12), and each problem's solution must match its own
-single-system solve."""
+single-system solve.  On the full 256-problem draw of the batched workload,
+every problem's block equals the oracle's assembly of that problem
+(or_assemble_diffusion, feSysElm.cpp:16-27,520-589) and every problem's
+solution meets rel-res 1e-8 on the oracle's system (or_spmv)."""
 import numpy as np
 import pytest
 
@@ -80,3 +83,58 @@ def test_batched_amg_matches_individual_systems(kind):
         assert relerr(u[b.dof_maps[p][:num.nInc]], u1) <= 1e-6
         S1.close()
     b.S.close()
+
+
+@pytest.mark.timeout(900)
+def test_batched_256_problem_draw_vs_oracle(oracle):
+    """draw_problems(256, 1), both degrees: the FP32-assembly, FP32-inner +
+    FP64-refined batched Jacobi-PCG (the batched configuration) brings every
+    problem to FP64 rel-res <= 1e-8; every problem's CSR block is the
+    oracle's pattern, its values and RHS the oracle's assembly to 1e-12, and
+    its solution has rel-res <= 1.01e-8 on the oracle's system; P1 solutions
+    are non-negative (the split-square P1 stiffness is an M-matrix, so the
+    discrete maximum principle holds for a non-negative load)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from feng_amd.api import PREC_FP32_IR, Numbering
+    from feng_amd.batched import build_batch, draw_problems, solve_batch
+    probs = draw_problems(256, 1)
+    checked = 0
+    for kind in (1, 2):
+        b = build_batch(probs, kind, keep_meshes=True)
+        print(f"kind {kind}: {len(b.nInc)} problems, n = {int(b.rows[-1])}", flush=True)
+        res = solve_batch(b, precision=PREC_FP32_IR, rtol=1e-8)
+        assert res["converged"].all() and (res["relres"] <= 1e-8).all(), res["relres"].max()
+        ia, ja = b.S.get_csr()
+        val, rhs = b.S.get_system()
+        b.S.correctSolution()
+        u = b.S.get_solution()
+
+        def check(p):
+            m, k = b.meshes[p], b.coeffs[p]
+            num = Numbering(m, [kind], [1])
+            nl = num.adr[0].size // m.E
+            ia0, ja0 = oracle.csr(num.nInc, [(num.adr[0], nl, num.adr[0], nl)])
+            val0, rhs0 = oracle.assemble_diffusion(kind, m.xyz, m.conn, num.adr[0], num.nInc, ia0, ja0, 2,
+                                                   (4, 0.0, 0.0, k), (0, -1.0, 0.0, None), np.zeros(num.nDofs))
+            r0, r1 = int(b.rows[p]), int(b.rows[p + 1])
+            blk = slice(ia[r0], ia[r1])
+            ok_csr = np.array_equal(ia[r0:r1 + 1] - ia[r0], ia0) and np.array_equal(ja[blk] - r0, ja0)
+            e_val, e_rhs = relerr(val[blk], val0), relerr(rhs[r0:r1], rhs0)
+            up = u[b.dof_maps[p][:num.nInc]]
+            rr = np.linalg.norm(rhs0 - oracle.spmv(ia0, ja0, val0, up)) / np.linalg.norm(rhs0)
+            return ok_csr, e_val, e_rhs, rr, up.min()
+
+        with ThreadPoolExecutor(8) as ex:
+            out = list(ex.map(check, range(len(b.nInc))))
+        for p, (ok_csr, e_val, e_rhs, rr, umin) in enumerate(out):
+            assert ok_csr, p
+            assert e_val <= 1e-12 and e_rhs <= 1e-12, (p, e_val, e_rhs)
+            assert rr <= 1.01e-8, (p, rr)
+            if kind == 1:
+                assert umin >= -1e-12, (p, umin)
+        checked += len(out)
+        print(f"kind {kind}: blocks and solutions of {len(out)} problems checked against the oracle, "
+              f"max rel-res on the oracle's systems {max(o[3] for o in out):.2e}", flush=True)
+        b.S.close()
+    assert checked == 256
