@@ -231,7 +231,10 @@ def bench_batched(args):
     probs = draw_problems(256, args.seed)
     out = {"workload": "batched", "problems": len(probs)}
     methods = {"jacobi": dict(precision=PREC_FP32_IR), "amg": dict(precision=PREC_FP64, pc=PC_AMG)}
+    if args.methods:
+        methods = {k: v for k, v in methods.items() if k in args.methods.split(",")}
     tot_n = 0
+    kern = {}
     tot_ms = {k: 0.0 for k in methods}
     detail = {k: {} for k in methods}
     for kind, name in ((1, "P1"), (2, "P2")):
@@ -251,14 +254,60 @@ def bench_batched(args):
                                   "warmup_s": round(setup_s, 2)}
             tot_ms[meth] += ms
             log(name, meth, detail[meth][name])
+        if "amg" in methods:
+            # one more solve with events around the level-0 prolongation of the
+            # multigrid V-cycle (the dominant kernel of the batched AMG solves)
+            rp = solve_batch(b, rtol=1e-8, autotune=False, profile=True, **methods["amg"])
+            rows_l, nnz_l = b.S.amg_levels()
+            n0, z0 = rows_l[0], nnz_l[0]
+            n1 = rows_l[1] if len(rows_l) > 1 else 0
+            # (4 + 4) nnz (FP32 values, columns) + 4 (n + 1) + 4 s n + 4 n + s n_1, s = 8
+            ub = 8 * z0 + 4 * (n0 + 1) + 32 * n0 + 4 * n0 + 8 * n1
+            kern[name] = {"avg_launch_ms": rp["kernel_ms"], "launches": rp["kernel_launches"], "alg_bytes": ub}
         tot_n += r["n"]
         b.S.close()
     best = min(tot_ms, key=tot_ms.get)
+    roof = None
+    if "P2" in kern and kern["P2"]["avg_launch_ms"]:
+        kp = kern["P2"]
+        ach = kp["alg_bytes"] / (kp["avg_launch_ms"] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": "amgUp<double, float, 4, 256> (level 0 of the P2 batch's V-cycle)",
+                "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                "avg_launch_ms": kp["avg_launch_ms"], "alg_bytes_per_launch": kp["alg_bytes"], "per_batch": kern}
+    cpu = None if args.no_cpu else cpu_batched(probs)
     out.update({"metric": "DOF/s assemble+solve to FP64 rel-res 1e-8 (256 batched problems)",
                 "value": tot_n / (tot_ms[best] / 1e3), "unit": "DOF/s", "ms_per_step": tot_ms[best], "n_total": tot_n,
                 "method": best, "dofs_per_method": {k: tot_n / (v / 1e3) for k, v in tot_ms.items()},
-                "batches": detail, "dtype": "f64" if best == "amg" else "fp32-inner/fp64-refine", "n_gpus": 1})
+                "batches": detail, "dtype": "f64" if best == "amg" else "fp32-inner/fp64-refine", "n_gpus": 1,
+                "roofline": roof, "cpu_baseline": cpu})
     print(json.dumps(out), flush=True)
+
+
+def cpu_batched(probs, sample=24):
+    """The oracle on the host, on the first `sample` problems of the draw: the
+    serial C assembly of each problem and its Jacobi-CG to FP64 rel-res 1e-8
+    (the batched path's algorithm), DOF/s = the sample's unknowns over its
+    time."""
+    from feng_amd.api import Mesh, Numbering
+    from tests.oracle_bind import load_oracle
+
+    O = load_oracle()
+    n_tot, t_tot = 0, 0.0
+    for N_, kind, seed in probs[:sample]:
+        m = Mesh.square(N_)
+        num = Numbering(m, [kind], [1])
+        k = m.coeff_blocks(8, 0.1, 10.0, seed)
+        nl = num.adr[0].size // m.E
+        ia, ja = O.csr(num.nInc, [(num.adr[0], nl, num.adr[0], nl)])
+        t0 = time.perf_counter()
+        val, rhs = O.assemble_diffusion(kind, m.xyz, m.conn, num.adr[0], num.nInc, ia, ja, 2, (4, 0.0, 0.0, k),
+                                        (0, -1.0, 0.0, None), np.zeros(num.nDofs))
+        O.cg_jacobi(ia, ja, val, rhs, 1e-8, 100000)
+        t_tot += time.perf_counter() - t0
+        n_tot += num.nInc
+    return {"value": n_tot / t_tot, "unit": "DOF/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle, one thread, on the first {sample} problems of the draw ({n_tot} unknowns): assembly "
+                       f"and Jacobi-CG to rel-res 1e-8 of each, {t_tot:.1f}s")}
 
 
 def transient_alg_bytes(E, V, nDofs, n, nnz, nsol, matrix=True):
@@ -482,6 +531,7 @@ def main():
     ap.add_argument("--workload", default="cube", choices=["cube", "batched", "transient", "ns"])
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--paths", default=None, help="comma-separated subset of " + ",".join(PATHS))
+    ap.add_argument("--methods", default=None, help="batched workload: comma-separated subset of jacobi,amg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the solver has no multi-GPU path "

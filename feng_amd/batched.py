@@ -100,13 +100,14 @@ def build_batch(problems, kind, device=0, keep_meshes=False) -> Batch:
 
 
 def solve_batch(b: Batch, precision=A.PREC_FP32_IR, rtol=1e-8, inner_rtol=1e-2, max_iter=100000, autotune=True,
-                check=True, pc=A.PC_JACOBI):
+                check=True, pc=A.PC_JACOBI, profile=False):
     """Assemble every problem (zero + all element kernels + scatter) and solve
-    them all; returns timings and per-problem statistics."""
+    them all; returns timings and per-problem statistics.  profile: events
+    around the dominant kernel's launches (kernel_ms, kernel_launches)."""
     L, S = b.S.L, b.S
     rows = np.ascontiguousarray(b.rows, np.int32)
     nprob = len(b.nInc)
-    S.set_profiling(True)
+    S.set_profiling(profile)
     S.setToZero()
     S.assemble(3, precision)
     asm_ms = S.kernel_stats(1)[0]
@@ -127,5 +128,6 @@ def solve_batch(b: Batch, precision=A.PREC_FP32_IR, rtol=1e-8, inner_rtol=1e-2, 
     its = np.array([s.iterations for s in st])
     rel = np.array([s.relres_fp64 for s in st])
     conv = np.array([bool(s.converged) for s in st])
+    kms, kl = S.kernel_stats(0)
     return dict(asm_ms=asm_ms, solve_ms=ms.value, iterations=its, relres=rel, converged=conv, tuned=tuned,
-                n=int(b.rows[-1]), nnz=S.nnz)
+                n=int(b.rows[-1]), nnz=S.nnz, kernel_ms=kms, kernel_launches=kl)

@@ -1032,6 +1032,10 @@ int amgApplyT(fe_hip_ctx *ctx, const T *in, T *out, const int *done, int slot)
     int G = std::max(1, (h.n + kAmgTB - 1) / kAmgTB);
     if(ctx->amgXcd) G = (G + 7) & ~7;
     const T *e = pick<T>(A.lv[l + 1].x64, A.lv[l + 1].x32);
+    // profiling: events around the level-0 prolongation (the dominant kernel
+    // of the batched multigrid solves)
+    if(l == 0 && ctx->profiling)
+      if(int rc = feTimerMark(ctx, ctx->spmvTimer)) return rc;
     if(ctx->amgLanes == 2)
       hipLaunchKernelGGL((amgUp<T, TV, 2, kAmgTB>), dim3(G), dim3(kAmgTB), 0, ctx->stream, L, b, e, oc, x, ctx->amgXcd,
                          done);
@@ -1041,6 +1045,8 @@ int amgApplyT(fe_hip_ctx *ctx, const T *in, T *out, const int *done, int slot)
     else
       hipLaunchKernelGGL((amgUp<T, TV, kAmgLPR, kAmgTB>), dim3(G), dim3(kAmgTB), 0, ctx->stream, L, b, e, oc, x,
                          ctx->amgXcd, done);
+    if(l == 0 && ctx->profiling)
+      if(int rc = feTimerMark(ctx, ctx->spmvTimer)) return rc;
   }
   return FE_STATUS_OK;
 }

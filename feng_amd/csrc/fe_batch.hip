@@ -782,6 +782,10 @@ int solveBatchedAmgPerProblem(fe_hip_ctx *ctx, FeBatch &B, const fe_hip_solver_o
   FE_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_du, B.x64, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
   FE_HIP_TRY(ctx, hipEventRecord(t1, ctx->stream));
   FE_HIP_TRY(ctx, hipEventSynchronize(t1));
+  if(ctx->profiling) {
+    if(int rc = feTimerCollect(ctx, ctx->spmvTimer)) return rc;
+    ctx->spmvTimer.used = 0;
+  }
   if(int rc = feAmgCheck(ctx)) return rc;
   float ms = 0.f;
   FE_HIP_TRY(ctx, hipEventElapsedTime(&ms, t0, t1));
@@ -808,6 +812,7 @@ extern "C" int fe_hip_solve_batched(fe_hip_ctx *ctx, int nprob, const int *rows,
   const bool fp32 = (o->precision == FE_PREC_FP32_IR);
   FE_REQUIRE(ctx, !fp32 || ctx->d_val32, "assemble with FE_PREC_FP32_IR to create the FP32 values");
   FE_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  ctx->spmvTimer = FeKernelTimer{std::move(ctx->spmvTimer.ev)};  // profiling: this solve's launches only
   FeBatch *Bp = nullptr;
   if(int rc = setupBatch(ctx, nprob, rows, Bp)) return rc;
   FeBatch &B = *Bp;

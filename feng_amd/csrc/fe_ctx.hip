@@ -825,6 +825,34 @@ int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value)
   return FE_STATUS_OK;
 }
 
+extern "C++" {
+int feTimerMark(fe_hip_ctx *ctx, FeKernelTimer &t)
+{
+  if(t.used + 1 > t.ev.size())
+    for(int q = 0; q < 64; ++q) {
+      hipEvent_t e;
+      FE_HIP_TRY(ctx, hipEventCreate(&e));
+      t.ev.push_back(e);
+    }
+  FE_HIP_TRY(ctx, hipEventRecord(t.ev[t.used++], ctx->stream));
+  return FE_STATUS_OK;
+}
+
+int feTimerCollect(fe_hip_ctx *ctx, FeKernelTimer &t)
+{
+  size_t q = 0;
+  for(; q + 1 < t.used; q += 2) {
+    float ms = 0.f;
+    FE_HIP_TRY(ctx, hipEventElapsedTime(&ms, t.ev[q], t.ev[q + 1]));
+    t.total_ms += ms;
+    t.launches++;
+  }
+  if(q < t.used) std::swap(t.ev[0], t.ev[q]);
+  t.used -= q;
+  return FE_STATUS_OK;
+}
+}  // extern "C++"
+
 int fe_hip_set_profiling(fe_hip_ctx *ctx, int on)
 {
   if(!ctx) return FE_STATUS_POINTER_ERROR;

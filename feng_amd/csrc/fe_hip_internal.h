@@ -136,6 +136,11 @@ struct FeKernelTimer {
   double total_ms = 0.;
   int64_t launches = 0;
 };
+// Records the next event of t on the stream (growing the pool); after the
+// stream has been synchronized, feTimerCollect adds the complete pairs to
+// the totals (an open pair's first event is kept).
+int feTimerMark(fe_hip_ctx *ctx, FeKernelTimer &t);
+int feTimerCollect(fe_hip_ctx *ctx, FeKernelTimer &t);
 
 // Preconditioner state: ILU(0)/IC(0) factor on A's pattern, row levels, and
 // the split factors G (lower CSR, unit diagonal) / G^T (upper CSR) / D^-1.
