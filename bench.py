@@ -25,7 +25,8 @@ every path is reported in config.paths, with its speed-up over plain.
 --workload batched runs the 256-problem batched workload instead;
 --workload transient runs BDF2 steps of the heat equation on the same cube;
 --workload ns runs Newton solves of the Kovasznay flow (Taylor-Hood, 512^2);
---copy-probe runs a plain device copy (counter calibration for rocprofv3).
+--copy-probe runs a plain device copy (counter calibration for rocprofv3);
+--mfma-probe runs the matrix-core kernels (MFMA counter passes).
 
 The cpu_baseline leg times the CPU oracle (a C restatement of the reference
 element loop + Jacobi-CG) on the same problem, on all cores of the process's
@@ -48,12 +49,14 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernels from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (scaled by the same counters on a plain copy of known size).
-TRAFFIC = {  # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE (cube 64^3):
-    # profiles/cube64_learned64_hbm_traffic.md (learned FP64 path),
-    # profiles/cube64_amg_hbm_traffic.md (multigrid FP64 path)
+TRAFFIC = {  # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE (cube 64^3), scaled by the same
+    # counters on a 512 MiB copy (x2.00, x1.00): profiles/cube64_hbm_traffic.md (assembly, nested multigrid PCG,
+    # plain), profiles/cube64_learned64_hbm_traffic.md (learned FP64 path)
     "spIter<double> (split PCG, A product + updates)": 79.6e6,
-    "tileIter<double>": 72.8e6,
-    "spIter<double> (AMG PCG, A product + updates)": 76.6e6,
+    "tileIter<double>": 75.9e6,
+    "pcgDown0<double, float> (level-0 CG update + smoothing + restriction)": 76.2e6,
+    "pcgDown0<float, float> (level-0 CG update + smoothing + restriction)": 53.0e6,
+    "asmGather<double, 4, 3, 16>": 370.5e6,
 }
 
 
@@ -219,6 +222,30 @@ def copy_probe(mib=512, reps=5):
     print(json.dumps({"copy_probe_mib": mib, "reps": reps}), flush=True)
 
 
+def mfma_probe():
+    """The matrix-core kernels for counter passes (rocprofv3 --pmc): the
+    colored MFMA element sweep of the headline cube in FP64 and FP32
+    (asm_kernel = 1, fp32_values = 1), the learned net's inference inside one
+    learned-PCG solve, and three training steps of the net on a 32^3 cube."""
+    from feng_amd.api import PC_LEARNED, PREC_FP32_IR, PREC_FP64
+    from feng_amd.learned import load_weights, train
+
+    m, k, S, num = make_problem(64, 512, 1)
+    S.set_tuning("asm_kernel", 1)
+    S.set_tuning("fp32_values", 1)
+    for prec in (PREC_FP64, PREC_FP64, PREC_FP32_IR, PREC_FP32_IR):
+        S.setToZero()
+        S.assemble(3, prec)
+    S.set_tuning("asm_kernel", 2)
+    load_weights(S)
+    S.setToZero()
+    S.assemble(3, PREC_FP64)
+    r = S.solve(pc=PC_LEARNED, rtol=1e-8)
+    S.close()
+    train(N=32, instances=3, log_every=1)
+    print(json.dumps({"mfma_probe": True, "learned_iterations": r.iterations}), flush=True)
+
+
 def bench_batched(args):
     """256 independent P1/P2 diffusion problems (N_p in [32, 256]) solved to
     FP64 rel-res 1e-8 per problem, two ways: FP32 assembly + batched FP32
@@ -312,13 +339,11 @@ def cpu_batched(probs, sample=24):
 
 def transient_alg_bytes(E, V, nDofs, n, nnz, nsol, matrix=True):
     """Algorithmic HBM bytes of one sweep of TRANSIENT_MASS + DIFFUSION +
-    SOURCE over P1 tets: per element the color permutation (4), connectivity
-    (16), row map (16) and coefficient (8), with the matrix the scatter
-    positions (64); the vertex coordinates (24 V), the current and past
-    solutions (8 nDofs nsol) read once; the RHS (8 n) and, with the matrix,
-    the CSR values (8 nnz) written once."""
-    per_elem = 4 + 16 + 16 + 8 + (64 if matrix else 0)
-    return E * per_elem + 24 * V + 8 * nDofs * nsol + 8 * n + (8 * nnz if matrix else 0)
+    SOURCE over P1 tets (the assembly's definition plus the solutions the
+    transient term reads): connectivity 16 E, coefficient 8 E, coordinates
+    24 V, the current and past solutions 8 nDofs nsol, the RHS 8 n and, with
+    the matrix, the CSR values 8 nnz written once."""
+    return 16 * E + 8 * E + 24 * V + 8 * nDofs * nsol + 8 * n + (8 * nnz if matrix else 0)
 
 
 def cpu_transient_step(m, k, num, dt, threads):
@@ -389,7 +414,7 @@ def bench_transient(args):
     achieved = alg / (B["asm_ms"] * 1e-3) / 1e9
     cpu = None
     if not args.no_cpu:
-        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        threads = max(1, len(os.sched_getaffinity(0)))
         c = cpu_transient_step(m, k, num, dt, threads)
         cpu = {"value": n / (c["t_asm"] + c["t_cg"]), "unit": "DOF-steps/s", "cores": threads, "kind": "port",
                "sample": (f"one BDF1 step of the same problem by the oracle: OpenMP assembly + serial transient "
@@ -402,7 +427,7 @@ def bench_transient(args):
            "config": {"workload": f"transient P1 tet heat equation, unit cube {args.N}^3 x6 Kuhn tets, "
                                   f"k=exp(GRF l=0.1 s=1.5), BDF2 dt={dt}", "n": n, "nnz": nnz, "elements": m.E,
                       "path": best, "paths": res},
-           "roofline": {"bound": "hbm", "kernel": "asmScalarFP64<4, 3> (a step's two sweeps, all colors)",
+           "roofline": {"bound": "hbm", "kernel": "asmGather<double, 4, 3, 16> (a step's two sweeps)",
                         "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                         "traffic": None, "sweeps_ms_per_step": B["asm_ms"], "alg_bytes_per_step": alg},
            "cpu_baseline": cpu}
@@ -530,6 +555,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=25.0)
     ap.add_argument("--workload", default="cube", choices=["cube", "batched", "transient", "ns"])
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--mfma-probe", action="store_true")
     ap.add_argument("--paths", default=None, help="comma-separated subset of " + ",".join(PATHS))
     ap.add_argument("--methods", default=None, help="batched workload: comma-separated subset of jacobi,amg")
     args = ap.parse_args()
@@ -539,6 +565,8 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         return copy_probe()
+    if args.mfma_probe:
+        return mfma_probe()
     if args.workload == "batched":
         return bench_batched(args)
     if args.workload == "transient":
