@@ -92,6 +92,16 @@ class Mesh:
         _chk(N.host().fe_mesh_read_msh(str(path).encode(), C.byref(h)), "fe_mesh_read_msh")
         return Mesh(h)
 
+    def boundary_groups(self):
+        """(names of the boundary physical groups, group of each boundary element)."""
+        n = C.c_int()
+        buf = C.create_string_buffer(4096)
+        bg = np.zeros(max(1, self.nB), np.int32)
+        _chk(N.host().fe_mesh_boundary_groups(self.h, C.byref(n), buf, 4096, N.ptr(bg, C.c_int)),
+             "fe_mesh_boundary_groups")
+        names = buf.value.decode().split("\n") if n.value else []
+        return names, bg[:self.nB]
+
     @staticmethod
     def from_arrays(dim, xyz, conn, bconn=None):
         """A mesh from vertex coordinates (V x 3), element connectivity
@@ -144,16 +154,32 @@ class Mesh:
 
 
 class Numbering:
-    """feMetaNumber counterpart (libfe_host.so)."""
+    """feMetaNumber counterpart (libfe_host.so).  essential[f]: 0 / 1 (no /
+    every boundary group essential for field f), or the names of the boundary
+    physical groups on which field f is essential (the reference's essential
+    spaces of the field, one per group)."""
 
     def __init__(self, mesh: Mesh, kinds, essential):
         self.mesh = mesh
         self.kinds = list(kinds)
         k = np.array(kinds, np.int32)
-        e = np.array(essential, np.int32)
         self.h = C.c_void_p()
-        _chk(N.host().fe_numbering_create(mesh.h, len(kinds), N.ptr(k, C.c_int), N.ptr(e, C.c_int),
-                                          C.byref(self.h)), "fe_numbering_create")
+        if any(isinstance(x, (list, tuple)) for x in essential):
+            names, _ = mesh.boundary_groups()
+            mask = np.zeros(len(kinds), np.uint32)
+            for f, x in enumerate(essential):
+                if isinstance(x, (list, tuple)):
+                    for g in x:
+                        mask[f] |= np.uint32(1 << names.index(g))
+                elif x:
+                    mask[f] = np.uint32(0xFFFFFFFF)
+            _chk(N.host().fe_numbering_create_groups(mesh.h, len(kinds), N.ptr(k, C.c_int),
+                                                     mask.ctypes.data_as(C.POINTER(C.c_uint)), C.byref(self.h)),
+                 "fe_numbering_create_groups")
+        else:
+            e = np.array(essential, np.int32)
+            _chk(N.host().fe_numbering_create(mesh.h, len(kinds), N.ptr(k, C.c_int), N.ptr(e, C.c_int),
+                                              C.byref(self.h)), "fe_numbering_create")
         a, b = C.c_int(), C.c_int()
         N.host().fe_numbering_info(self.h, C.byref(a), C.byref(b))
         self.nInc, self.nDofs = a.value, b.value

@@ -25,6 +25,10 @@ struct fe_mesh {
   std::vector<int> edges;   // 3E signed tags (2D)
   std::vector<int> edgeA;   // first vertex of each edge as its first element walks it (2D)
   std::vector<int> bedge;   // nB edge index (2D)
+  // boundary physical groups of a mesh read from a file (physical-tag order)
+  // and the group of each boundary element; empty: one group "Bord"
+  std::vector<std::string> groups;
+  std::vector<int> bgroup;
 };
 
 struct fe_numbering {
@@ -316,9 +320,13 @@ int fe_mesh_write_msh(const fe_mesh *m, const char *path)
   return rc;
 }
 
-// Reader for .msh v4.1 ASCII files with physical groups "Bord" (dim-1) and
-// "Domaine" (dim): vertex ids in $Nodes order (feMeshRead.cpp:933-958);
-// domain elements ordered by geometric entity (dim, tag) then file order
+// Reader for .msh v4.1 ASCII files (feMeshRead.cpp:1709-2005): the domain is
+// every element of the highest dimension in a physical group ("Domaine",
+// "Domain", ...), the boundary every element one dimension below in a
+// physical group ("Bord", or "Inlet", "Outlet", "NoSlip", ...), each
+// boundary element tagged with its group (groups in physical-tag order).
+// Vertex ids in $Nodes order (feMeshRead.cpp:933-958); domain elements
+// ordered by geometric entity (dim, tag) then file order
 // (feMeshRead.cpp:1777-1808); boundary elements in file order (their global
 // element order, which drives essential edge numbering).
 int fe_mesh_read_msh(const char *path, fe_mesh **out)
@@ -441,15 +449,25 @@ int fe_mesh_read_msh(const char *path, fe_mesh **out)
   // Domain triangles in file order (edge tags are assigned in this order),
   // then reordered by entity tag for the element numbering.
   std::vector<std::pair<int, size_t>> domBlocks;
+  // boundary groups: the named physical groups of dimension maxDim - 1
+  std::map<int, int> groupOfTag;  // physical tag -> group index
+  for(auto &pn : physName)
+    if(pn.first.first == maxDim - 1) {
+      groupOfTag[pn.first.second] = (int)m->groups.size();
+      m->groups.push_back(pn.second);
+    }
   for(size_t b = 0; b < blocks.size(); ++b) {
     const Block &B = blocks[b];
-    if(B.dim == maxDim && physOf(B.dim, B.tag) == "Domaine") domBlocks.push_back({B.tag, b});
-    if(B.dim == maxDim - 1 && physOf(B.dim, B.tag) == "Bord")
+    if(B.dim == maxDim && !physOf(B.dim, B.tag).empty()) domBlocks.push_back({B.tag, b});
+    if(B.dim == maxDim - 1 && !physOf(B.dim, B.tag).empty()) {
+      const int grp = groupOfTag.at(entityPhys.at({B.dim, B.tag}));
       for(long v : B.nodes) {
         auto it = nodeId.find(v);
         if(it == nodeId.end()) { delete m; return FE_STATUS_READ_ERROR; }
         m->bconn.push_back(it->second);
       }
+      for(size_t e = 0; e < B.nodes.size() / (size_t)B.nper; ++e) m->bgroup.push_back(grp);
+    }
   }
   m->nB = (int)(m->bconn.size() / m->bnv);
   std::vector<int> fileConn;
@@ -536,10 +554,38 @@ int fe_mesh_check_jacobians(const fe_mesh *m, int *elem, double *det)
   return e < 0 ? FE_STATUS_OK : FE_STATUS_ERROR;
 }
 
+int fe_mesh_boundary_groups(const fe_mesh *m, int *ngroups, char *names, int len, int *bgroup)
+{
+  if(!m) return FE_STATUS_POINTER_ERROR;
+  const std::vector<std::string> dflt = {"Bord"};
+  const std::vector<std::string> &g = m->groups.empty() ? dflt : m->groups;
+  if(ngroups) *ngroups = (int)g.size();
+  if(names) {
+    std::string all;
+    for(size_t i = 0; i < g.size(); ++i) all += (i ? "\n" : "") + g[i];
+    if((int)all.size() + 1 > len) return FE_STATUS_RANGE_ERROR;
+    std::copy(all.begin(), all.end(), names);
+    names[all.size()] = 0;
+  }
+  if(bgroup)
+    for(int b = 0; b < m->nB; ++b) bgroup[b] = m->bgroup.empty() ? 0 : m->bgroup[b];
+  return FE_STATUS_OK;
+}
+
 int fe_numbering_create(const fe_mesh *m, int nfields, const int *kinds, const int *essential,
                         fe_numbering **out)
 {
   if(!m || !kinds || !essential || !out || nfields < 1) return FE_STATUS_POINTER_ERROR;
+  std::vector<unsigned> mask(nfields);
+  for(int f = 0; f < nfields; ++f) mask[f] = essential[f] ? ~0u : 0u;
+  return fe_numbering_create_groups(m, nfields, kinds, mask.data(), out);
+}
+
+int fe_numbering_create_groups(const fe_mesh *m, int nfields, const int *kinds, const unsigned *essentialGroups,
+                               fe_numbering **out)
+{
+  if(!m || !kinds || !essentialGroups || !out || nfields < 1) return FE_STATUS_POINTER_ERROR;
+  auto groupOf = [&](int b) { return m->bgroup.empty() ? 0 : m->bgroup[b]; };
   for(int f = 0; f < nfields; ++f) {
     int k = kinds[f];
     if(m->dim == 2 && !(k == FE_TRI_P1 || k == FE_TRI_P2 || k == FE_TRI_VP2 || k == FE_TRI_P3 || k == FE_TRI_P4))
@@ -564,8 +610,10 @@ int fe_numbering_create(const fe_mesh *m, int nfields, const int *kinds, const i
     vnum[f].assign((size_t)m->V * nvd[f], UNK);
     enm[f].assign((size_t)m->nEdges * ned[f], UNK);
     inm[f].assign((size_t)m->E * nin[f], UNK);
-    if(essential[f]) {
+    if(essentialGroups[f]) {
       for(int b = 0; b < m->nB; ++b) {
+        const int grp = groupOf(b);
+        if(grp >= 32 || !((essentialGroups[f] >> grp) & 1u)) continue;
         for(int j = 0; j < m->bnv; ++j) {
           int v = m->bconn[(size_t)b * m->bnv + j];
           for(int c = 0; c < nvd[f]; ++c) vnum[f][(size_t)v * nvd[f] + c] = ESS;

@@ -73,10 +73,14 @@ int fe_mesh_cube(int N, fe_mesh **out);
 int fe_mesh_from_arrays(int dim, int V, const double *xyz, int E, int nvpe, const int *conn,
                         int nB, int bnv, const int *bconn, fe_mesh **out);
 
-/* Gmsh .msh v4.1 ASCII: physical "Bord" (boundary, dim-1) and "Domaine"
+/* Gmsh .msh v4.1 ASCII writer: physical "Bord" (boundary, dim-1) and "Domaine"
  * (dim), boundary elements written first.  feMeshRead.cpp:645-1622 reads it
  * back with vertex ids = file order, element order = file order. */
 int fe_mesh_write_msh(const fe_mesh *m, const char *path);
+/* Gmsh .msh v4.1 ASCII reader (feMeshRead.cpp:1709-2005): the domain is every
+ * element of the highest dimension in a named physical group, the boundary
+ * every element one dimension below in a named physical group, each tagged
+ * with its group (fe_mesh_boundary_groups). */
 int fe_mesh_read_msh(const char *path, fe_mesh **out);
 
 int fe_mesh_destroy(fe_mesh *m);
@@ -101,6 +105,19 @@ int fe_mesh_get(const fe_mesh *m, double *xyz, int *conn, int *bconn, int *edges
  * vector). */
 int fe_numbering_create(const fe_mesh *m, int nfields, const int *kinds, const int *essential,
                         fe_numbering **out);
+/* fe_numbering_create with the essential boundary of each field given per
+ * boundary physical group: bit g of essentialGroups[f] makes the DOFs of
+ * field f on the boundary elements of group g essential (the reference's
+ * essential spaces, one per field and physical group, feNumber.cpp:677-843);
+ * fe_numbering_create is this with every group for essential[f] != 0. */
+int fe_numbering_create_groups(const fe_mesh *m, int nfields, const int *kinds, const unsigned *essentialGroups,
+                               fe_numbering **out);
+/* Boundary physical groups of a mesh (dimension dim - 1; a mesh read from a
+ * .msh file has its named groups in physical-tag order, a generated mesh the
+ * one group "Bord"): their count, their names separated by '\n' in names
+ * (capacity len, NULL: not wanted) and the group of each boundary element in
+ * bgroup[nB] (NULL: not wanted). */
+int fe_mesh_boundary_groups(const fe_mesh *m, int *ngroups, char *names, int len, int *bgroup);
 /* Orientation check the reference runs when it builds a space on a mesh
  * (feCncGeo::computeJacobians, feCncGeo.cpp:346-353, :388-395, :410-412):
  * FE_STATUS_ERROR when any element has a Jacobian determinant <= 0, with the

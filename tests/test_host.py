@@ -127,3 +127,39 @@ def test_negative_jacobian_is_refused(dim):
     xyz0[conn[0, 2]] = xyz0[conn[0, 0]]
     e0, det0 = Mesh.from_arrays(dim, xyz0, good.conn.reshape(good.E, good.nvpe)).check_jacobians()
     assert e0 == 0 and det0 == 0.0
+
+
+def test_msh_reader_keeps_named_boundary_groups():
+    """The reference's Poiseuille mesh (data/poiseuille0.msh: "Domain" and the
+    boundary groups "NoSlip", "Inlet", "Outlet", feMeshRead.cpp:1709-2005):
+    every named group is read, each boundary segment keeps its group, and an
+    essential set given per group (the reference's essential spaces uInlet,
+    uNoSlip of navier_stokes.cpp:213-221) marks exactly the DOFs on those
+    lines; generated meshes have the one group "Bord"."""
+    import numpy as np
+
+    from feng_amd.api import TRI_P1, TRI_VP2, Mesh, Numbering
+    from tests.conftest import GOLDEN
+    m = Mesh.read_msh(GOLDEN / "meshes" / "data" / "poiseuille0.msh")
+    names, bg = m.boundary_groups()
+    assert names == ["NoSlip", "Inlet", "Outlet"]
+    assert m.E > 0 and bg.size == m.nB and set(bg.tolist()) == {0, 1, 2}
+    xyz = m.xyz.reshape(-1, 3)
+    seg = m.bconn.reshape(-1, 2)
+    for g, on_line in enumerate([lambda p: np.isclose(p[:, 1], 0) | np.isclose(p[:, 1], 1),
+                                 lambda p: np.isclose(p[:, 0], 0), lambda p: np.isclose(p[:, 0], 5)]):
+        pts = xyz[seg[bg == g].ravel()]
+        assert on_line(pts).all(), names[g]
+    num = Numbering(m, [TRI_VP2, TRI_P1], [["Inlet", "NoSlip"], []])
+    d = num.dof_xyz.reshape(-1, 3)
+    ess = np.arange(num.nDofs) >= num.nInc
+    wall = np.isclose(d[:, 0], 0) | np.isclose(d[:, 1], 0) | np.isclose(d[:, 1], 1)
+    u = num.dof_field == 0
+    assert np.array_equal(ess[u], wall[u])  # velocity: essential exactly on Inlet and NoSlip
+    assert not ess[num.dof_field == 1].any()  # pressure: no essential DOF
+    assert (np.isclose(d[:, 0], 5) & u & ~ess).any()  # the outlet's velocity DOFs stay unknowns
+    # every group of the old meaning (essential = 1) equals the union of the names
+    n1 = Numbering(m, [TRI_VP2, TRI_P1], [1, 0])
+    n2 = Numbering(m, [TRI_VP2, TRI_P1], [["NoSlip", "Inlet", "Outlet"], []])
+    assert n1.nInc == n2.nInc and all(np.array_equal(a, b) for a, b in zip(n1.adr, n2.adr))
+    assert Mesh.square(4).boundary_groups()[0] == ["Bord"]
