@@ -42,6 +42,7 @@
 #include <algorithm>
 #include <cmath>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -88,9 +89,10 @@ struct PBlockPlan {
 // stop launching as soon as the device flags a stop, without synchronizing
 // the stream: progress = the iteration whose product last started, done = the
 // stop code (1 converged, 3 breakdown), stopK = its iteration, rr = the
-// recursive (r, r) at the stop.
+// recursive (r, r) at the stop; barErr = a grid barrier of the fused
+// coarse-level launch timed out.
 struct PcgMailbox {
-  int progress, done, stopK, pad;
+  int progress, done, stopK, barErr;
   double rr;
 };
 
@@ -397,8 +399,9 @@ __global__ __launch_bounds__(kFTB) void pcgDown0(PLev<T, TV> L, T *__restrict__ 
                                                  const T *__restrict__ uOld, T *__restrict__ u, T *__restrict__ bNext,
                                                  const double *__restrict__ pqPart, int Gpq, const double *scal,
                                                  int slot, int first, int *done, int *stat, PcgMailbox *mb,
-                                                 int k, double *__restrict__ rrPart)
+                                                 int k, double *__restrict__ rrPart, unsigned *bar)
 {
+  if(blockIdx.x == 0 && threadIdx.x == 0) *bar = 0u;  // grid-barrier counter of the fused coarse-level launch
   __shared__ double red[kFTB / 64];
   __shared__ T res[kFRPB];
   // independent loads first: flag, tile, row bounds, own-row values, this
@@ -573,38 +576,39 @@ __global__ __launch_bounds__(kPBlockTB) void pcgPackBlock(const PLev<T, TV> *__r
   stageLds<2>((T *)(img + pl.oCi), cinvG, nC * nC);
 }
 
-template <typename T, typename TV>
-__global__ __launch_bounds__(kPBlockTB) void pcgBlock(const PLev<T, TV> *__restrict__ lv, PBlockPlan pl,
-                                                      const unsigned char *__restrict__ img, T oc, const int *done,
-                                                      T *__restrict__ y0, const T *__restrict__ u0)
+// Static image of the single-workgroup levels into LDS: 16-byte loads, all
+// of a batch issued before their stores.
+__device__ __forceinline__ void blockImage(const PBlockPlan &pl, const unsigned char *__restrict__ img,
+                                           unsigned char *lds)
 {
-  __shared__ __align__(16) unsigned char lds[kPBlockLds];
+  constexpr int U = 10;  // 16-byte chunks per thread per batch (160 KB per batch)
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int chunks = pl.imgBytes / 16;
+  for(int c0 = t; c0 < chunks; c0 += U * nt) {
+    uint4 v[U];
+#pragma unroll
+    for(int u = 0; u < U; ++u) {
+      const int c = c0 + u * nt;
+      v[u] = c < chunks ? ((const uint4 *)img)[c] : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for(int u = 0; u < U; ++u) {
+      const int c = c0 + u * nt;
+      if(c < chunks) ((uint4 *)lds)[c] = v[u];
+    }
+  }
+}
+
+// Levels from .. coarsest with the static image in LDS: b of level from is
+// read from global memory, its correction e written back (and, from = 1,
+// the level-0 y0 = u0 + oc P e_1).
+template <typename T, typename TV>
+__device__ __forceinline__ void blockSolve(const PLev<T, TV> *__restrict__ lv, const PBlockPlan &pl,
+                                           unsigned char *lds, T oc, T *__restrict__ y0, const T *__restrict__ u0)
+{
   const int t = threadIdx.x, nt = blockDim.x;
   const int nLev = pl.coarsest - pl.from;
-  // one pass: the static image (16-byte loads, all issued before their
-  // stores) and the right-hand side of level from
-  {
-    constexpr int U = 10;  // 16-byte chunks per thread per batch (160 KB per batch)
-    const int chunks = pl.imgBytes / 16;
-    const T *src = lv[pl.from].b;
-    const int nb = pl.n[0];
-    const int dn = *done;
-    for(int c0 = t; c0 < chunks; c0 += U * nt) {
-      uint4 v[U];
-#pragma unroll
-      for(int u = 0; u < U; ++u) {
-        const int c = c0 + u * nt;
-        v[u] = c < chunks ? ((const uint4 *)img)[c] : make_uint4(0, 0, 0, 0);
-      }
-#pragma unroll
-      for(int u = 0; u < U; ++u) {
-        const int c = c0 + u * nt;
-        if(c < chunks) ((uint4 *)lds)[c] = v[u];
-      }
-    }
-    if(dn) return;
-    stageLds<2>((T *)(lds + pl.oB[0]), src, nb);
-  }
+  stageLds<2>((T *)(lds + pl.oB[0]), lv[pl.from].b, pl.n[0]);
   __syncthreads();
   for(int s = 0; s < nLev; ++s) {
     const int n = pl.n[s], nc = pl.nc[s];
@@ -668,6 +672,552 @@ __global__ __launch_bounds__(kPBlockTB) void pcgBlock(const PLev<T, TV> *__restr
     const int n0 = lv[0].n;
     const int *agg0 = lv[0].agg;
     for(int m = t; m < n0; m += nt) y0[m] = fmaT<T>(oc, e1[agg0[m]], u0[m]);
+  }
+}
+
+template <typename T, typename TV>
+__global__ __launch_bounds__(kPBlockTB) void pcgBlock(const PLev<T, TV> *__restrict__ lv, PBlockPlan pl,
+                                                      const unsigned char *__restrict__ img, T oc, const int *done,
+                                                      T *__restrict__ y0, const T *__restrict__ u0)
+{
+  __shared__ __align__(16) unsigned char lds[kPBlockLds];
+  const int dn = *done;
+  blockImage(pl, img, lds);
+  if(dn) return;
+  blockSolve<T, TV>(lv, pl, lds, oc, y0, u0);
+}
+
+// ---------------------------------------------------------------------------
+// Levels 1 .. coarsest in one launch
+//
+// The restriction and prolongation of the levels above the single-workgroup
+// ones (levels 1 .. B-1) and the single-workgroup levels (B .. coarsest) run
+// in one launch of G workgroups separated by grid barriers, instead of
+// 2 (B - 1) + 1 dependent launches.  Workgroup w owns, on every level l < B,
+// the rows nested in its contiguous share of level-B rows (tiles[l][w] ..
+// tiles[l][w + 1]): its restrictions are sums inside its own rows, and the
+// neighbour values a row needs (l1 b, u + oc e_next) are recomputed from
+// vectors completed before the last barrier.  Workgroup 0 loads the
+// single-workgroup LDS image first and solves levels B .. coarsest between
+// the down and up sweeps.  The grid is small (G <= kFuseMaxG) and every
+// workgroup holds one CU, so all are resident; each barrier wait is bounded
+// and a timeout raises a flag (no wait is endless).
+constexpr int kFuseMaxG = 64;
+constexpr int kFuseRows = 1280;  // rows of one level per workgroup (LDS partial residuals)
+constexpr int kFuseLds = kPBlockLds + kFuseRows * 8;
+constexpr int kFuseSpin = 1 << 21;
+
+struct FusePlan {
+  int G = 0, B = 0;  // G = 0: not used
+};
+
+__device__ __forceinline__ void gridBarrier(unsigned *cnt, unsigned target, int *err)
+{
+  __syncthreads();
+  if(threadIdx.x == 0) {
+    __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    int spins = 0;
+    while(__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if(++spins > kFuseSpin) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  __syncthreads();
+}
+
+// tiles: (B + 1) x (G + 1) row starts, level l at l * (G + 1).  bar: counter
+// zeroed by the preceding down0 launch.  err: flag raised on a barrier timeout
+// (host-mapped).  y0 = u0 + oc P e_1 for the level-0 rows of the own level-1
+// rows.
+template <typename T, typename TV>
+__global__ __launch_bounds__(kPBlockTB) void pcgCoarseV(const PLev<T, TV> *__restrict__ lv, PBlockPlan pl,
+                                                        const unsigned char *__restrict__ img,
+                                                        const int *__restrict__ tiles, int G, T oc, const int *done,
+                                                        unsigned *bar, int *err, T *__restrict__ y0,
+                                                        const T *__restrict__ u0, const int *__restrict__ aggStart0)
+{
+  __shared__ __align__(16) unsigned char lds[kFuseLds];
+  const int t = threadIdx.x, w = blockIdx.x, B = pl.from;
+  const int dn = *done;
+  if(w == 0) blockImage(pl, img, lds);
+  if(dn) return;
+  T *res = (T *)(lds + kPBlockLds);
+  const int rl = t / kPLPR, sub = t % kPLPR;
+  constexpr int RPR = kPBlockTB / kPLPR;  // rows per round
+  unsigned target = 0;
+  for(int l = 1; l < B; ++l) {  // restrictions: b_l+1 = P^T (b_l - A (l1 b_l))
+    const PLev<T, TV> L = lv[l];
+    T *bn = lv[l + 1].b;
+    const int r0 = tiles[l * (G + 1) + w], r1 = tiles[l * (G + 1) + w + 1];
+    for(int base = r0; base < r1; base += RPR) {
+      const int row = base + rl;
+      int kb = 0, ke = 0;
+      if(row < r1) {
+        kb = L.ia[row];
+        ke = L.ia[row + 1];
+      }
+      const T acc = rowSum<T, TV>(L.ja, L.val, kb, ke, sub, [&](int j) { return (T)L.l1[j] * L.b[j]; });
+      if(row < r1 && sub == 0) {
+        const T bi = L.b[row];
+        L.u[row] = (T)L.l1[row] * bi;
+        res[row - r0] = bi - acc;
+      }
+    }
+    __syncthreads();
+    const int I0 = tiles[(l + 1) * (G + 1) + w], I1 = tiles[(l + 1) * (G + 1) + w + 1];
+    for(int I = I0 + t; I < I1; I += kPBlockTB) {
+      T s = (T)0;
+      for(int m = L.aggStart[I] - r0; m < L.aggStart[I + 1] - r0; ++m) s += res[m];
+      bn[I] = s;
+    }
+    gridBarrier(bar, target += G, err);
+  }
+  if(w == 0) blockSolve<T, TV>(lv, pl, lds, oc, (T *)nullptr, (const T *)nullptr);
+  gridBarrier(bar, target += G, err);
+  for(int l = B - 1; l >= 1; --l) {  // prolongations: e_l = y + S (b_l - A y), y = u + oc P e_l+1
+    const PLev<T, TV> L = lv[l];
+    const T *eNext = lv[l + 1].e;
+    const int r0 = tiles[l * (G + 1) + w], r1 = tiles[l * (G + 1) + w + 1];
+    for(int base = r0; base < r1; base += RPR) {
+      const int row = base + rl;
+      int kb = 0, ke = 0;
+      if(row < r1) {
+        kb = L.ia[row];
+        ke = L.ia[row + 1];
+      }
+      const T acc =
+          rowSum<T, TV>(L.ja, L.val, kb, ke, sub, [&](int j) { return fmaT<T>(oc, eNext[L.agg[j]], L.u[j]); });
+      if(row < r1) {
+        const T y = fmaT<T>(oc, eNext[L.agg[row]], L.u[row]);
+        const T e = fmaT<T>((T)L.l1[row], L.b[row] - acc, y);
+        if(l > 1) {
+          if(sub == 0) L.e[row] = e;
+        } else {
+          for(int m = aggStart0[row] + sub; m < aggStart0[row + 1]; m += kPLPR) y0[m] = fmaT<T>(oc, e, u0[m]);
+        }
+      }
+    }
+    if(l > 1) gridBarrier(bar, target += G, err);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// W-cycle of the coarse levels in one workgroup
+//
+// Levels R .. D-1 (one or two "register levels") are each solved by two
+// cycles, e = e1 + C(b - A e1), C the cycle of the level over the solve of the
+// next one; level D (at most kWDense rows) by its dense inverse.  One launch
+// replaces the restriction, single-workgroup and prolongation launches of
+// those levels, and the second cycles cut the PCG iterations (the coarse
+// problem of level R - 1 is solved almost exactly).
+//
+// A register level's matrix is held in the registers of the workgroup: its
+// rows, each padded to whole groups of kWGroup entries, are cut into runs of
+// K entries, one run per thread (slots 4g .. 4g + 3 of thread t are the
+// 16-byte word g * kWTB + t).  A word holds the LDS byte address of the
+// gathered entry of the level's X vector (low 16 bits) and the binary16 value
+// h_ij = a_ij w_i (high 16 bits), w the level's l1 weights rounded to
+// bfloat16; the operator is diag(w)^-1 H: r = b - (H x) / w (the reciprocal
+// of w to 1 ulp), e = y + w b - H y.  Per thread: its first row, whether that row began in the
+// previous thread, and one bit per group where a row ends.  A row spans at
+// most two threads (K >= its padded length); the two partial sums are added
+// after a barrier, in a fixed order.  Vectors, weights, aggregate starts and
+// the dense inverse are in LDS.
+constexpr int kWTB = 512;          // threads of the W-cycle kernel
+constexpr int kWGroup = 4;         // row entries per group (rows end on a group boundary)
+constexpr int kWLds = 160 * 1024;  // LDS of the W-cycle kernel
+constexpr int kWDense = 128;       // its dense level: at most this many rows
+constexpr int kWKA[2] = {64, 160};  // slots per thread of the first register level (kernel variants)
+constexpr int kWKB = 32;            // slots per thread of the second register level
+
+struct WLev {
+  int n = 0, nc = 0;                    // rows; rows of the next level (= aggregates)
+  int oX = 0, oB = 0, oE = 0, oR = 0;   // LDS byte offsets of the float vectors
+  int oW = 0, oAs = 0;                  // bfloat16 weights, nc + 1 aggregate starts (static image)
+};
+// Steps of the W-cycle, run in order by the kernel (ops[k] = step | level << 4);
+// one instance of each level's pass code serves all its passes, which keeps
+// the kernel inside the instruction cache.
+enum WStep : unsigned char {
+  kWScaleB,      // X = w B                                (u of the first cycle)
+  kWPassE,       // E = S = H X (raw row sums)
+  kWPassR,       // R = S
+  kWPassB,       // B = S
+  kWRestrictE,   // next rhs = P^T (B - E / w)             (r = b - A u)
+  kWRestrictB,   // next rhs = P^T (R - B / w)             (r' = b' - A u')
+  kWProlong,     // X += oc P e_next                       (y)
+  kWPost1,       // E = X + w B - E; X = E                 (e1)
+  kWResRScale,   // R = B - R / w; X = w R                 (b' = b - A e1, u')
+  kWPost2,       // E += X + w R - B                       (e1 + e2)
+  kWDenseSolve   // e_D = Dinv b_D
+};
+constexpr int kWMaxOps = 48;
+
+struct WPlan {
+  int R = 0, D = 0, nReg = 0, KA = 0, KB = 0;  // R = 0: the kernel is not used
+  WLev lv[2];
+  int nOps = 0;
+  unsigned char ops[kWMaxOps];
+  int nD = 0, oDinv = 0, oBD = 0, oED = 0, oCarry = 0;
+  int oImg = 0, imgBytes = 0, bytes = 0;  // static image: LDS bytes [oImg, oImg + imgBytes)
+};
+
+__device__ __forceinline__ float wBf16(unsigned short b)
+{
+  return __uint_as_float((uint32_t)b << 16);
+}
+__device__ __forceinline__ unsigned short bf16Bits(float x)  // nearest, ties to even
+{
+  const uint32_t u = __float_as_uint(x);
+  return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+// Decoding of a register word, kept inside each pass: the words stay in
+// registers for the whole kernel, and decoded copies hoisted out of the
+// passes would need registers of their own (volatile, so not merged across
+// passes; ALU only).
+__device__ __forceinline__ uint32_t wAddr(uint32_t w)  // LDS byte address (low 16 bits)
+{
+  uint32_t a;
+  asm volatile("v_and_b32 %0, 0xffff, %1" : "=v"(a) : "v"(w));
+  return a;
+}
+__device__ __forceinline__ float wFma(uint32_t w, float x, float acc)  // acc + h x, h the high binary16
+{
+  asm volatile("v_fma_mix_f32 %0, %1, %2, %0 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(acc) : "v"(w), "v"(x));
+  return acc;
+}
+__device__ __forceinline__ uint32_t wBit(uint32_t m, uint32_t bit)
+{
+  uint32_t r;
+  asm volatile("v_and_b32 %0, %1, %2" : "=v"(r) : "v"(m), "v"(bit));
+  return r;
+}
+
+// S_i = sum_j h_ij X_j over the thread's run, stored to dst[i] (the row the
+// thread continues after the barrier).  Gathers run one group ahead; the
+// row ends only store.
+template <int K>
+__device__ __forceinline__ void wPass(const uint4 (&w)[K / kWGroup], int4 info, const unsigned char *lds, float *carry,
+                                      float *dst)
+{
+  constexpr int G = K / kWGroup;
+  const bool cont = info.y != 0;
+  int row = info.x;
+  float acc = 0.f, head = 0.f;
+  float xs[2][kWGroup];
+  auto gather = [&](const uint4 &q, float *x) {
+    x[0] = *(const float *)(lds + wAddr(q.x));
+    x[1] = *(const float *)(lds + wAddr(q.y));
+    x[2] = *(const float *)(lds + wAddr(q.z));
+    x[3] = *(const float *)(lds + wAddr(q.w));
+  };
+  gather(w[0], xs[0]);
+#pragma unroll
+  for(int g = 0; g < G; ++g) {
+    if(g + 1 < G) gather(w[g + 1], xs[(g + 1) & 1]);
+    acc = wFma(w[g].x, xs[g & 1][0], acc);
+    acc = wFma(w[g].y, xs[g & 1][1], acc);
+    acc = wFma(w[g].z, xs[g & 1][2], acc);
+    acc = wFma(w[g].w, xs[g & 1][3], acc);
+    if(wBit(g < 32 ? (uint32_t)info.z : (uint32_t)info.w, 1u << (g & 31))) {
+      if(cont && row == info.x) head = acc;
+      else dst[row] = acc;
+      ++row;
+      acc = 0.f;
+    }
+  }
+  carry[threadIdx.x] = acc;
+  __syncthreads();
+  if(cont) dst[info.x] = carry[threadIdx.x - 1] + head;
+  __syncthreads();
+}
+
+// Rows i = t, t + kWTB, ... of a level in batches of kWU: every load of a
+// batch is issued before its stores (LDS latency once per batch).
+constexpr int kWU = 4;
+template <int NV, typename Load, typename Store>
+__device__ __forceinline__ void wRows(int n, Load load, Store store)
+{
+  for(int base = threadIdx.x; base < n; base += kWU * kWTB) {
+    float v[kWU][NV];
+#pragma unroll
+    for(int u = 0; u < kWU; ++u) {
+      const int i = base + u * kWTB;
+      if(i < n) load(i, v[u]);
+    }
+#pragma unroll
+    for(int u = 0; u < kWU; ++u) {
+      const int i = base + u * kWTB;
+      if(i < n) store(i, v[u]);
+    }
+  }
+  __syncthreads();
+}
+
+// Aggregates I = t, t + kWTB, ...: f(I, i0, i1) over member rows [i0, i1).
+template <typename F>
+__device__ __forceinline__ void wAggs(int nc, const int *as, F f)
+{
+  for(int I = threadIdx.x; I < nc; I += kWTB) f(I, as[I], as[I + 1]);
+  __syncthreads();
+}
+
+// e = Dinv b on the dense level, 4 lanes per row, 8 entries per batch.
+__device__ __forceinline__ void wDense(const float *Dinv, int nD, const float *b, float *e)
+{
+  const int t = threadIdx.x, i = t >> 2, sub = t & 3;
+  float s = 0.f;
+  if(i < nD)
+    for(int j0 = sub; j0 < nD; j0 += 32) {
+      float d[8], x[8];
+#pragma unroll
+      for(int u = 0; u < 8; ++u) {
+        const int j = j0 + 4 * u;
+        d[u] = j < nD ? Dinv[i * nD + j] : 0.f;
+        x[u] = j < nD ? b[j] : 0.f;
+      }
+#pragma unroll
+      for(int u = 0; u < 8; ++u) s = __builtin_fmaf(d[u], x[u], s);
+    }
+  s += __shfl_xor(s, 1, 64);
+  s += __shfl_xor(s, 2, 64);
+  if(i < nD && sub == 0) e[i] = s;
+  __syncthreads();
+}
+
+// The W-cycle kernel: b_R (T) in, e_R (T) out; with R = 1 also the level-0
+// y0 = u0 + oc P e_1.  KB = 0: level R + 1 is the dense level.  Each
+// register level is solved by two cycles on its rhs B (solution E, work
+// vectors R and X), the next level's rhs and solution being its B and E or
+// the dense level's b_D and e_D; the steps come from the plan's op list.
+template <typename T, int KA, int KB>
+__global__ __launch_bounds__(kWTB) void pcgCoarseW(WPlan pl, const uint4 *__restrict__ wA,
+                                                   const int4 *__restrict__ infoA, const uint4 *__restrict__ wB,
+                                                   const int4 *__restrict__ infoB, const unsigned char *__restrict__ img,
+                                                   float oc, const int *done, const T *__restrict__ bR,
+                                                   T *__restrict__ eR, T *__restrict__ y0, const T *__restrict__ u0,
+                                                   const int *__restrict__ aggStart0)
+{
+  __shared__ __align__(16) unsigned char lds[kWLds];
+  const int t = threadIdx.x;
+  const int dn = *done;
+  uint4 a[KA / kWGroup];
+#pragma unroll
+  for(int g = 0; g < KA / kWGroup; ++g) a[g] = wA[g * kWTB + t];
+  constexpr int GB = KB > 0 ? KB / kWGroup : 1;
+  uint4 b[GB];
+  int4 ib = make_int4(0, 0, 0, 0);
+  if constexpr(KB > 0) {
+#pragma unroll
+    for(int g = 0; g < GB; ++g) b[g] = wB[g * kWTB + t];
+    ib = infoB[t];
+  }
+  const int4 ia = infoA[t];
+  {
+    constexpr int U = 4;
+    const int chunks = pl.imgBytes / 16;
+    uint4 *dst = (uint4 *)(lds + pl.oImg);
+    for(int c0 = t; c0 < chunks; c0 += U * kWTB) {
+      uint4 v[U];
+#pragma unroll
+      for(int u = 0; u < U; ++u) {
+        const int c = c0 + u * kWTB;
+        v[u] = c < chunks ? ((const uint4 *)img)[c] : make_uint4(0, 0, 0, 0);
+      }
+#pragma unroll
+      for(int u = 0; u < U; ++u)
+        if(c0 + u * kWTB < chunks) dst[c0 + u * kWTB] = v[u];
+    }
+  }
+  if(dn) return;
+  for(int i = t; i < pl.lv[0].n; i += kWTB) ((float *)(lds + pl.lv[0].oB))[i] = (float)bR[i];
+  __syncthreads();
+  float *carry = (float *)(lds + pl.oCarry);
+  float *bD = (float *)(lds + pl.oBD), *eD = (float *)(lds + pl.oED);
+  for(int k = 0; k < pl.nOps; ++k) {
+    const int step = pl.ops[k] & 15, L = pl.ops[k] >> 4;
+    if(step == kWDenseSolve) {
+      wDense((const float *)(lds + pl.oDinv), pl.nD, bD, eD);
+      continue;
+    }
+    const WLev &V = pl.lv[L];
+    const int n = V.n, nc = V.nc;
+    float *X = (float *)(lds + V.oX), *B = (float *)(lds + V.oB), *E = (float *)(lds + V.oE), *R = (float *)(lds + V.oR);
+    const unsigned short *wt = (const unsigned short *)(lds + V.oW);
+    const int *as = (const int *)(lds + V.oAs);
+    const bool regChild = L + 1 < pl.nReg;
+    float *cB = regChild ? (float *)(lds + pl.lv[L + 1].oB) : bD;
+    const float *cE = regChild ? (const float *)(lds + pl.lv[L + 1].oE) : eD;
+    switch(step) {
+    case kWScaleB:
+      wRows<2>(
+          n, [&](int i, float *v) { v[0] = wBf16(wt[i]); v[1] = B[i]; }, [&](int i, const float *v) { X[i] = v[0] * v[1]; });
+      break;
+    case kWPost1:
+      wRows<4>(
+          n, [&](int i, float *v) { v[0] = wBf16(wt[i]); v[1] = B[i]; v[2] = X[i]; v[3] = E[i]; },
+          [&](int i, const float *v) {
+            const float e = v[2] + v[0] * v[1] - v[3];
+            E[i] = e;
+            X[i] = e;
+          });
+      break;
+    case kWResRScale:
+      wRows<3>(
+          n, [&](int i, float *v) { v[0] = wBf16(wt[i]); v[1] = B[i]; v[2] = R[i]; },
+          [&](int i, const float *v) {
+            const float r = v[1] - v[2] * __builtin_amdgcn_rcpf(v[0]);
+            R[i] = r;
+            X[i] = v[0] * r;
+          });
+      break;
+    case kWPost2:
+      wRows<5>(
+          n, [&](int i, float *v) { v[0] = wBf16(wt[i]); v[1] = R[i]; v[2] = X[i]; v[3] = B[i]; v[4] = E[i]; },
+          [&](int i, const float *v) { E[i] = v[4] + (v[2] + v[0] * v[1] - v[3]); });
+      break;
+    case kWRestrictE:
+    case kWRestrictB: {
+      const float *Y = step == kWRestrictE ? B : R, *Sm = step == kWRestrictE ? E : B;
+      wAggs(nc, as, [&](int I, int i0, int i1) {
+        float s = 0.f;
+        for(int b0 = i0; b0 < i1; b0 += 8) {
+          float y[8], sv[8], w[8];
+#pragma unroll
+          for(int u = 0; u < 8; ++u) {
+            const int i = b0 + u < i1 ? b0 + u : i0;
+            y[u] = Y[i];
+            sv[u] = Sm[i];
+            w[u] = wBf16(wt[i]);
+          }
+#pragma unroll
+          for(int u = 0; u < 8; ++u)
+            if(b0 + u < i1) s += y[u] - sv[u] * __builtin_amdgcn_rcpf(w[u]);
+        }
+        cB[I] = s;
+      });
+      break;
+    }
+    case kWProlong:
+      wAggs(nc, as, [&](int I, int i0, int i1) {
+        const float e = oc * cE[I];
+        for(int b0 = i0; b0 < i1; b0 += 8) {
+          float x[8];
+#pragma unroll
+          for(int u = 0; u < 8; ++u) x[u] = X[b0 + u < i1 ? b0 + u : i0];
+#pragma unroll
+          for(int u = 0; u < 8; ++u)
+            if(b0 + u < i1) X[b0 + u] = x[u] + e;
+        }
+      });
+      break;
+    default: {  // a pass over the level's matrix: raw sums into E, R or B
+      float *dst = step == kWPassE ? E : step == kWPassR ? R : B;
+      if constexpr(KB > 0) {
+        if(L == 1) {
+          wPass<KB>(b, ib, lds, carry, dst);
+          break;
+        }
+      }
+      wPass<KA>(a, ia, lds, carry, dst);
+      break;
+    }
+    }
+  }
+  const float *E = (const float *)(lds + pl.lv[0].oE);
+  for(int i = t; i < pl.lv[0].n; i += kWTB) eR[i] = (T)E[i];
+  if(y0)
+    for(int I = t; I < pl.lv[0].n; I += kWTB) {
+      const T e = (T)oc * (T)E[I];
+      for(int m = aggStart0[I]; m < aggStart0[I + 1]; ++m) y0[m] = u0[m] + e;
+    }
+}
+
+// Register words and static image of one register level from the FP32
+// values and l1 of the level (every numeric set-up): map[q] = (CSR entry or
+// -1, row | byte address << 16) per word slot q.
+__global__ void pcgWPack(int nq, const int2 *__restrict__ map, const float *__restrict__ val32,
+                         const float *__restrict__ l132, uint32_t *__restrict__ words, int n,
+                         unsigned short *__restrict__ wImg, int nc1, const int *__restrict__ as, int *__restrict__ asImg)
+{
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if(q < nq) {
+    const int2 m = map[q];
+    uint32_t word = (uint32_t)m.y >> 16;  // padding: value 0 at the level's X[0]
+    if(m.x >= 0) {
+      const float w = wBf16(bf16Bits(l132[m.y & 0xFFFF]));
+      const _Float16 h = (_Float16)(val32[m.x] * w);
+      word |= (uint32_t)__builtin_bit_cast(unsigned short, h) << 16;
+    }
+    words[q] = word;
+  }
+  if(q < n) wImg[q] = bf16Bits(l132[q]);
+  if(q < nc1) asImg[q] = as[q];
+}
+
+// FP32 inverse of the dense level (n <= kWDense) by Gauss-Jordan in FP64
+// (SPD: no pivoting) with the matrix in registers: thread t holds row t >> 3,
+// columns (t & 7) + 8k; each step's pivot column and normalised pivot row go
+// through LDS.  flag = 1 on a non-positive pivot.
+__global__ __launch_bounds__(1024) void pcgWDenseInverse(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                                         const double *__restrict__ val, float *__restrict__ inv,
+                                                         int *flag)
+{
+  __shared__ double Md[kWDense * kWDense];
+  __shared__ double rowP[kWDense], colP[kWDense];
+  const int t = threadIdx.x, i = t >> 3, c0 = t & 7;
+  constexpr int KC = kWDense / 8;
+  for(int q = t; q < n * n; q += 1024) Md[q] = 0.;
+  __syncthreads();
+  for(int r = t; r < n; r += 1024)
+    for(int k = ia[r]; k < ia[r + 1]; ++k) Md[r * n + ja[k]] = val[k];
+  __syncthreads();
+  double m[KC];
+#pragma unroll
+  for(int k = 0; k < KC; ++k) {
+    const int c = c0 + 8 * k;
+    m[k] = i < n && c < n ? Md[i * n + c] : 0.;
+  }
+  for(int p = 0; p < n; ++p) {
+    double mp = 0.;
+#pragma unroll
+    for(int k = 0; k < KC; ++k)
+      if(k == (p >> 3)) mp = m[k];
+    if(c0 == (p & 7) && i < n) colP[i] = mp;  // column p before the step
+    __syncthreads();
+    const double piv = colP[p];
+    if(!(piv > 0.)) {
+      if(t == 0) *flag = 1;
+      return;
+    }
+    if(i == p) {
+      const double rp = 1. / piv;
+#pragma unroll
+      for(int k = 0; k < KC; ++k) {
+        const int c = c0 + 8 * k;
+        m[k] = (c == p ? 1. : m[k]) * rp;
+        if(c < n) rowP[c] = m[k];
+      }
+    }
+    __syncthreads();
+    if(i != p && i < n) {
+      const double f = colP[i];
+#pragma unroll
+      for(int k = 0; k < KC; ++k) {
+        const int c = c0 + 8 * k;
+        if(c < n) m[k] = (c == p ? 0. : m[k]) - f * rowP[c];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for(int k = 0; k < KC; ++k) {
+    const int c = c0 + 8 * k;
+    if(i < n && c < n) inv[i * n + c] = (float)m[k];
   }
 }
 
@@ -856,6 +1406,19 @@ struct FeAmgPcg {
   double *pq = nullptr, *rr = nullptr, *rz = nullptr, *scal = nullptr, *xsave = nullptr;
   int *done = nullptr, *stat = nullptr;
   PcgMailbox *mbHost = nullptr, *mbDev = nullptr;  // pinned, mapped: host and device addresses
+  // W-cycle kernel of the coarse levels (wplan.R = 0: not used): per register
+  // level the words, per-thread (first row, continued, group-end mask) and
+  // the word -> (entry, row | byte address << 16) map; the static LDS image
+  WPlan wplan;
+  uint32_t *wWords[2] = {nullptr, nullptr};
+  int4 *wInfo[2] = {nullptr, nullptr};
+  int2 *wMap[2] = {nullptr, nullptr};
+  unsigned char *wImg = nullptr;
+  // fused coarse-level launch per slot (fuseG = 0: not used): tiles of
+  // (B + 1) x (G + 1) row starts; the grid-barrier counter
+  int fuseG[3] = {0, 0, 0};
+  int *fuseTiles[3] = {nullptr, nullptr, nullptr};
+  unsigned *bar = nullptr;
   int g0 = 0;  // grid of the row kernels of level 0
   double c = 0.;
   double valuesFrom = -1.;  // unused marker
@@ -948,6 +1511,167 @@ PBlockPlan pcgPlan(const FeAmgPcg &A)
   return pl;
 }
 
+// Steps of the two cycles on register level L (the next level solved in
+// between, recursively, or the dense level).
+void wProgram(WPlan &pl, int L)
+{
+  auto add = [&](int step, int lev) { pl.ops[pl.nOps++] = (unsigned char)(step | lev << 4); };
+  auto child = [&]() {
+    if(L + 1 < pl.nReg) wProgram(pl, L + 1);
+    else add(kWDenseSolve, 0);
+  };
+  add(kWScaleB, L);
+  add(kWPassE, L);
+  add(kWRestrictE, L);
+  child();
+  add(kWProlong, L);
+  add(kWPassE, L);
+  add(kWPost1, L);
+  add(kWPassR, L);
+  add(kWResRScale, L);
+  add(kWPassB, L);
+  add(kWRestrictB, L);
+  child();
+  add(kWProlong, L);
+  add(kWPassB, L);
+  add(kWPost2, L);
+}
+
+// Plan, register maps and thread data of the W-cycle kernel (once per CSR
+// pattern): R is the first level whose matrix (and, when the level below it
+// has more than kWDense rows, that level's too) fits the registers and whose
+// vectors, weights, aggregate starts and dense inverse fit the LDS; none: R = 0.
+int wSetup(fe_hip_ctx *ctx, FeAmgPcg &A, const std::vector<std::vector<int>> &hIa,
+           const std::vector<std::vector<int>> &hJa)
+{
+  const int Lc = (int)A.lv.size() - 1;
+  for(int R = 1; R < Lc; ++R) {
+    const int nReg = A.lv[R + 1].n <= kWDense ? 1 : 2;
+    if(nReg == 2 && (R + 2 > Lc || A.lv[R + 2].n > kWDense)) continue;
+    int K[2] = {0, 0};
+    bool ok = true;
+    for(int r = 0; r < nReg && ok; ++r) {
+      const std::vector<int> &ia = hIa[R + r];
+      int64_t P = 0;
+      int maxPad = 0;
+      for(size_t i = 0; i + 1 < ia.size(); ++i) {
+        const int pd = (ia[i + 1] - ia[i] + kWGroup - 1) / kWGroup * kWGroup;
+        P += pd;
+        maxPad = std::max(maxPad, pd);
+      }
+      const int need = std::max(maxPad, (int)((P + kWTB - 1) / kWTB + kWGroup - 1) / kWGroup * kWGroup);
+      K[r] = r == 0 ? (need <= kWKA[0] ? kWKA[0] : need <= kWKA[1] ? kWKA[1] : 0) : (need <= kWKB ? kWKB : 0);
+      ok = K[r] > 0;
+    }
+    if(!ok) continue;
+    WPlan pl;
+    pl.R = R;
+    pl.D = R + nReg;
+    pl.nReg = nReg;
+    pl.KA = K[0];
+    pl.KB = nReg == 2 ? K[1] : 0;
+    int bytes = 0;
+    auto take = [&](size_t b) {
+      const int o = bytes;
+      bytes += (int)((b + 15) & ~(size_t)15);
+      return o;
+    };
+    for(int r = 0; r < nReg; ++r) {
+      pl.lv[r].n = A.lv[R + r].n;
+      pl.lv[r].nc = A.lv[R + r].nc;
+      pl.lv[r].oX = take(sizeof(float) * pl.lv[r].n);
+    }
+    if(bytes > 65536) continue;  // gathered entries are addressed with 16 bits
+    pl.oImg = bytes;
+    for(int r = 0; r < nReg; ++r) {
+      pl.lv[r].oW = take(2 * (size_t)pl.lv[r].n);
+      pl.lv[r].oAs = take(sizeof(int) * (size_t)(pl.lv[r].nc + 1));
+    }
+    pl.nD = A.lv[pl.D].n;
+    pl.oDinv = take(sizeof(float) * (size_t)pl.nD * pl.nD);
+    pl.imgBytes = bytes - pl.oImg;
+    for(int r = 0; r < nReg; ++r) {
+      pl.lv[r].oB = take(sizeof(float) * pl.lv[r].n);
+      pl.lv[r].oE = take(sizeof(float) * pl.lv[r].n);
+      pl.lv[r].oR = take(sizeof(float) * pl.lv[r].n);
+    }
+    pl.oBD = take(sizeof(float) * pl.nD);
+    pl.oED = take(sizeof(float) * pl.nD);
+    pl.oCarry = take(sizeof(float) * kWTB);
+    pl.bytes = bytes;
+    if(bytes > kWLds) continue;
+    wProgram(pl, 0);
+    for(int r = 0; r < nReg; ++r) {
+      const int Kr = K[r], n = pl.lv[r].n, oX = pl.lv[r].oX;
+      const std::vector<int> &ia = hIa[R + r], &ja = hJa[R + r];
+      std::vector<int2> map((size_t)Kr * kWTB, make_int2(-1, oX << 16));
+      std::vector<int> first(kWTB, n), cont(kWTB, 0);
+      std::vector<unsigned long long> mask(kWTB, 0ull);
+      auto word = [&](int64_t q) {  // padded position -> word slot
+        const int t = (int)(q / Kr), s = (int)(q % Kr);
+        return ((size_t)(s / kWGroup) * kWTB + t) * kWGroup + s % kWGroup;
+      };
+      int64_t p = 0;
+      for(int i = 0; i < n; ++i) {
+        const int len = ia[i + 1] - ia[i];
+        const int pd = (len + kWGroup - 1) / kWGroup * kWGroup;
+        for(int m = 0; m < len; ++m)
+          map[word(p + m)] = make_int2(ia[i] + m, i | ((oX + (int)sizeof(float) * ja[ia[i] + m]) << 16));
+        const int64_t e = p + pd - 1;
+        mask[e / Kr] |= 1ull << ((e % Kr) / kWGroup);
+        for(int64_t t = (p + Kr - 1) / Kr; t * Kr < p + pd; ++t) {
+          first[t] = i;
+          cont[t] = t * Kr > p;
+        }
+        p += pd;
+      }
+      FE_REQUIRE(ctx, p <= (int64_t)Kr * kWTB && cont[0] == 0, "nested multigrid PCG: W-cycle layout overflow");
+      std::vector<int4> info(kWTB);
+      for(int t = 0; t < kWTB; ++t)
+        info[t] = make_int4(first[t], cont[t], (int)(uint32_t)mask[t], (int)(uint32_t)(mask[t] >> 32));
+      if(int rc = upload(ctx, &A.wMap[r], map)) return rc;
+      if(int rc = upload(ctx, &A.wInfo[r], info)) return rc;
+      if(int rc = feAlloc(ctx, &A.wWords[r], (size_t)Kr * kWTB)) return rc;
+    }
+    if(int rc = feAlloc(ctx, &A.wImg, (size_t)pl.imgBytes)) return rc;
+    A.wplan = pl;
+    return FE_STATUS_OK;
+  }
+  return FE_STATUS_OK;
+}
+
+// Tiles of the fused coarse-level launch for a single-workgroup start level
+// B: the level-B rows cut into contiguous shares so that no level l < B gets
+// more than kFuseRows rows in a share; level-l tiles are the rows nested in
+// each share.  Returns G (0: B <= 1, or a share would exceed the limits).
+int fuseTilesFor(const std::vector<std::vector<int>> &aggStart, const std::vector<FeAmgPcgLevel> &lv, int B,
+                 std::vector<int> &tiles)
+{
+  if(B <= 1 || B >= (int)lv.size()) return 0;
+  auto start = [&](int l, int r) {  // first level-l row nested in level-B row r
+    for(int m = B - 1; m >= l; --m) r = aggStart[m][r];
+    return r;
+  };
+  std::vector<int> cuts(1, 0);
+  const int nB = lv[B].n;
+  for(int r = 1; r <= nB; ++r) {
+    bool over = false;
+    for(int l = 1; l < B; ++l) over = over || start(l, r) - start(l, cuts.back()) > kFuseRows;
+    if(over) {
+      if(r - 1 == cuts.back()) return 0;  // one level-B row alone is too large
+      cuts.push_back(r - 1);
+      --r;
+    }
+  }
+  cuts.push_back(nB);
+  const int G = (int)cuts.size() - 1;
+  if(G > kFuseMaxG) return 0;
+  tiles.assign((size_t)(B + 1) * (G + 1), 0);
+  for(int l = 1; l <= B; ++l)
+    for(int w = 0; w <= G; ++w) tiles[(size_t)l * (G + 1) + w] = l == B ? cuts[w] : start(l, cuts[w]);
+  return G;
+}
+
 // Nested-ordered copy of the host hierarchy (once per CSR pattern).
 int pcgSetupPattern(fe_hip_ctx *ctx)
 {
@@ -983,7 +1707,7 @@ int pcgSetupPattern(fe_hip_ctx *ctx)
   ctx->amgPcg = A;
   A->lv.resize(Lc + 1);
   // permuted CSR of every level and, per level, original entry -> permuted entry
-  std::vector<std::vector<int>> entPos(Lc + 1);
+  std::vector<std::vector<int>> entPos(Lc + 1), hIa(Lc + 1), hJa(Lc + 1);
   for(int l = 0; l <= Lc; ++l) {
     const FeAmgHostLevel &h = H[l];
     FeAmgPcgLevel &d = A->lv[l];
@@ -1007,6 +1731,10 @@ int pcgSetupPattern(fe_hip_ctx *ctx)
       ia[ip + 1] = ia[ip] + (int)row.size();
     }
     if(int rc = upload(ctx, &d.ia, ia)) return rc;
+    if(l > 0) {
+      hIa[l] = ia;
+      hJa[l] = ja;
+    }
     if(l == 0) {
       // SELL layout: slices of kSlice rows, slot k of a slice's rows
       // contiguous; each original entry's SELL position for level 1's sums
@@ -1126,6 +1854,14 @@ int pcgSetupPattern(fe_hip_ctx *ctx)
   for(int s = 0; s < 3; ++s)
     if(A->plan[s].bytes <= kPBlockLds)
       if(int rc = feAlloc(ctx, &A->img[s], (size_t)std::max(16, A->plan[s].imgBytes))) return rc;
+  for(int s = 0; s < 3; ++s) {
+    std::vector<int> tiles;
+    if(A->plan[s].bytes > kPBlockLds) continue;
+    A->fuseG[s] = fuseTilesFor(aggStart, A->lv, A->plan[s].from, tiles);
+    if(A->fuseG[s] > 0)
+      if(int rc = upload(ctx, &A->fuseTiles[s], tiles)) return rc;
+  }
+  if(int rc = feAlloc(ctx, &A->bar, 1)) return rc;
   {
     std::vector<PLev<float, float>> a;
     std::vector<PLev<double, float>> b;
@@ -1145,7 +1881,7 @@ int pcgSetupPattern(fe_hip_ctx *ctx)
     A->desc[1] = db;
     A->desc[2] = dc;
   }
-  return FE_STATUS_OK;
+  return wSetup(ctx, *A, hIa, hJa);
 }
 
 // Numeric set-up for the current values (every solve): permuted level-0
@@ -1187,6 +1923,22 @@ int pcgNumeric(fe_hip_ctx *ctx, bool fp64Values, bool fp64Product, int slot)
   else
     hipLaunchKernelGGL(pcgDenseInverse<float>, dim3(1), dim3(1024), 0, ctx->stream, C.n, C.ia, C.ja, C.val64,
                        A.cinv64, A.cinv32, A.flag, A.c, C.l1_32);
+  // the W-cycle kernel's register words and static image (FP32 level values)
+  const WPlan &W = A.wplan;
+  if(W.R > 0 && !fp64Values && ctx->amgWcycle) {
+    for(int r = 0; r < W.nReg; ++r) {
+      const FeAmgPcgLevel &h = A.lv[W.R + r];
+      const int nq = (r == 0 ? W.KA : W.KB) * kWTB;
+      const int cnt = std::max(nq, std::max(h.n, h.nc + 1));
+      hipLaunchKernelGGL(pcgWPack, dim3((cnt + kPTB - 1) / kPTB), dim3(kPTB), 0, ctx->stream, nq,
+                         (const int2 *)A.wMap[r], (const float *)h.val32, (const float *)h.l1_32, A.wWords[r], h.n,
+                         (unsigned short *)(A.wImg + W.lv[r].oW - W.oImg), h.nc + 1, (const int *)h.aggStart,
+                         (int *)(A.wImg + W.lv[r].oAs - W.oImg));
+    }
+    const FeAmgPcgLevel &Dl = A.lv[W.D];
+    hipLaunchKernelGGL(pcgWDenseInverse, dim3(1), dim3(1024), 0, ctx->stream, Dl.n, (const int *)Dl.ia,
+                       (const int *)Dl.ja, (const double *)Dl.val64, (float *)(A.wImg + W.oDinv - W.oImg), A.flag);
+  }
   // the static LDS image of the small-level kernel for this slot
   const PBlockPlan &pl = A.plan[slot];
   if(slot == 0)
@@ -1231,15 +1983,46 @@ PcgVecs<double> vecs64(FeAmgPcg &A)
                          A.z64v,  A.rhs64v,               A.y64v,                 A.w64v,                 A.cinv64};
 }
 
+// The W-cycle kernel on levels R .. D (FP32 level values): b_R in, e_R out,
+// with R = 1 also the level-0 y.
+template <typename T>
+void launchW(fe_hip_ctx *ctx, FeAmgPcg &A, const PcgVecs<T> &v, const T *u0)
+{
+  const WPlan &W = A.wplan;
+  const PLev<T, float> LR = pview<T, float>(A.lv[W.R]);
+  T *y0 = W.R == 1 ? v.y : nullptr;
+  const float oc = (float)ctx->amgOverCorrect;
+#define FE_W_LAUNCH(KA_, KB_)                                                                                         \
+  hipLaunchKernelGGL((pcgCoarseW<T, KA_, KB_>), dim3(1), dim3(kWTB), 0, ctx->stream, W, (const uint4 *)A.wWords[0],  \
+                     (const int4 *)A.wInfo[0], (const uint4 *)A.wWords[1], (const int4 *)A.wInfo[1],                  \
+                     (const unsigned char *)A.wImg, oc, (const int *)A.done, (const T *)LR.b, LR.e, y0, u0,          \
+                     (const int *)A.lv[0].aggStart)
+  if(W.KA == kWKA[0] && W.KB == 0) FE_W_LAUNCH(kWKA[0], 0);
+  else if(W.KA == kWKA[0]) FE_W_LAUNCH(kWKA[0], kWKB);
+  else if(W.KB == 0) FE_W_LAUNCH(kWKA[1], 0);
+  else FE_W_LAUNCH(kWKA[1], kWKB);
+#undef FE_W_LAUNCH
+}
+
 // Launches the V-cycle after down0: down_l (levels 1 .. B-1), the block
-// kernel, up_l (levels B-1 .. 1; level 1 also writes the level-0 y), then
-// up0 on rNew into z.
+// kernel (or the W-cycle kernel from level B = R), up_l (levels B-1 .. 1;
+// level 1 also writes the level-0 y), then up0 on rNew into z.
 template <typename T, typename TV>
 void vcycleRest(fe_hip_ctx *ctx, FeAmgPcg &A, int slot, const PcgVecs<T> &v, const T *rNew, const T *u0)
 {
   const PBlockPlan &pl = A.plan[slot];
-  const int B = pl.from;
+  const bool useW = std::is_same<TV, float>::value && A.wplan.R > 0 && ctx->amgWcycle;
+  const int B = useW ? A.wplan.R : pl.from;
   const T oc = (T)ctx->amgOverCorrect;
+  if(!useW && ctx->amgFuse && A.fuseG[slot] > 0) {
+    hipLaunchKernelGGL((pcgCoarseV<T, TV>), dim3(A.fuseG[slot]), dim3(kPBlockTB), 0, ctx->stream,
+                       (const PLev<T, TV> *)A.desc[slot], pl, (const unsigned char *)A.img[slot],
+                       (const int *)A.fuseTiles[slot], A.fuseG[slot], oc, (const int *)A.done, A.bar, &A.mbDev->barErr,
+                       v.y, u0, (const int *)A.lv[0].aggStart);
+    hipLaunchKernelGGL((pcgUp0<T, TV>), dim3(A.g0), dim3(kFTB), 0, ctx->stream, pview<T, TV>(A.lv[0]), (const T *)v.y,
+                       rNew, v.z, A.done, A.rz);
+    return;
+  }
   const PLev<T, TV> *d = (const PLev<T, TV> *)A.desc[slot];
   const PLev<T, TV> L0 = pview<T, TV>(A.lv[0]);
   for(int l = 1; l < B; ++l) {
@@ -1247,8 +2030,11 @@ void vcycleRest(fe_hip_ctx *ctx, FeAmgPcg &A, int slot, const PcgVecs<T> &v, con
     const PLev<T, TV> N = pview<T, TV>(A.lv[l + 1]);
     hipLaunchKernelGGL((pcgDownL<T, TV>), dim3(L.nTiles), dim3(kPTB), 0, ctx->stream, L, N.b, A.done);
   }
-  hipLaunchKernelGGL((pcgBlock<T, TV>), dim3(1), dim3(kPBlockTB), 0, ctx->stream, d, pl,
-                     (const unsigned char *)A.img[slot], oc, A.done, v.y, u0);
+  if(useW)
+    launchW<T>(ctx, A, v, u0);
+  else
+    hipLaunchKernelGGL((pcgBlock<T, TV>), dim3(1), dim3(kPBlockTB), 0, ctx->stream, d, pl,
+                       (const unsigned char *)A.img[slot], oc, A.done, v.y, u0);
   for(int l = B - 1; l >= 1; --l) {
     const PLev<T, TV> L = pview<T, TV>(A.lv[l]);
     const PLev<T, TV> N = pview<T, TV>(A.lv[l + 1]);
@@ -1297,13 +2083,14 @@ int pcgSegment(fe_hip_ctx *ctx, FeAmgPcg &A, int slot, const PcgVecs<T> &v, doub
   mb->stopK = -1;
   mb->rr = 0.;
   mb->progress = k0 - 1;
+  mb->barErr = 0;
   {
     const int s = k0 & 1;
     hipLaunchKernelGGL(pcgZero<T>, dim3(256), dim3(256), 0, ctx->stream, n, v.x, A.done, A.stat);
     // z = M r (down0 in first mode copies rhs into r[s] and forms u[s] = l1 r)
     hipLaunchKernelGGL((pcgDown0<T, TV>), dim3(L0.nTiles), dim3(kFTB), 0, ctx->stream, L0, v.x, (const T *)v.p[s],
                        (const T *)v.q[s], (const T *)v.w, (const T *)v.rhs, v.r[s], (const T *)v.u[s ^ 1], v.u[s],
-                       L1.b, (const double *)A.pq, A.g0, (const double *)A.scal, 0, 1, A.done, A.stat, A.mbDev, k0, A.rr);
+                       L1.b, (const double *)A.pq, A.g0, (const double *)A.scal, 0, 1, A.done, A.stat, A.mbDev, k0, A.rr, A.bar);
     vcycleRest<T, TV>(ctx, A, slot, v, v.r[s], v.u[s]);
   }
   const T *prodVal = productValues<T>(A.lv[0]);
@@ -1344,7 +2131,7 @@ int pcgSegment(fe_hip_ctx *ctx, FeAmgPcg &A, int slot, const PcgVecs<T> &v, doub
     hipLaunchKernelGGL((pcgDown0<T, TV>), dim3(L0.nTiles), dim3(kFTB), 0, ctx->stream, L0, v.x, (const T *)v.p[s],
                        (const T *)v.q[s], (const T *)v.w, (const T *)v.r[s], v.r[s ^ 1], (const T *)v.u[s], v.u[s ^ 1],
                        L1.b, (const double *)A.pq, A.g0, (const double *)A.scal, s, 0, A.done, A.stat, A.mbDev, k,
-                       A.rr);
+                       A.rr, A.bar);
     if(ctx->profiling)
       if(int rc = mark(tk)) return rc;
     vcycleRest<T, TV>(ctx, A, slot, v, v.r[s ^ 1], v.u[s ^ 1]);
@@ -1353,6 +2140,7 @@ int pcgSegment(fe_hip_ctx *ctx, FeAmgPcg &A, int slot, const PcgVecs<T> &v, doub
   }
   FE_HIP_TRY(ctx, hipGetLastError());
   FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  FE_REQUIRE(ctx, !mb->barErr, "nested multigrid PCG: a grid barrier of the coarse-level launch timed out");
   const int dn = mb->done;
   if(dn) {
     run.k = mb->stopK >= k0 ? mb->stopK : k;
@@ -1394,7 +2182,8 @@ int feAmgPcgSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double bnorm, do
   const bool fp32 = o->precision == FE_PREC_FP32_IR;
   const bool val64 = !fp32 && !ctx->pcValues32;
   const int slot = fp32 ? 0 : (val64 ? 2 : 1);
-  FE_REQUIRE(ctx, A.plan[slot].bytes <= kPBlockLds, "nested multigrid PCG: the small levels do not fit the LDS");
+  FE_REQUIRE(ctx, (slot != 2 && A.wplan.R > 0 && ctx->amgWcycle) || A.plan[slot].bytes <= kPBlockLds,
+             "nested multigrid PCG: the small levels do not fit the LDS");
   if(int rc = pcgNumeric(ctx, val64, !fp32, slot)) return rc;
   if(!ctx->h_state) FE_HIP_TRY(ctx, hipHostMalloc(&ctx->h_state, 4096));
   const int n = ctx->n;
@@ -1502,6 +2291,23 @@ int feAmgPcgSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double bnorm, do
   return FE_STATUS_OK;
 }
 
+int feAmgPcgWLevel(const fe_hip_ctx *ctx)
+{
+  return ctx->amgPcg && ctx->amgWcycle ? ctx->amgPcg->wplan.R : 0;
+}
+
+int feAmgPcgFuseGrid(const fe_hip_ctx *ctx)
+{
+  if(!ctx->amgPcg || !ctx->amgFuse) return 0;
+  const int slot = ctx->pcValues32 ? 1 : 2;
+  return ctx->amgPcg->fuseG[slot];
+}
+
+int feAmgPcgWDense(const fe_hip_ctx *ctx)
+{
+  return ctx->amgPcg && ctx->amgWcycle ? ctx->amgPcg->wplan.D : 0;
+}
+
 bool feAmgNestedUsable(const fe_hip_ctx *ctx)
 {
   return ctx->amgKernel == 2 && ctx->amgHostSweeps == 0 && ctx->amgHost.size() >= 2 &&
@@ -1515,7 +2321,8 @@ int feAmgPcgPrepare(fe_hip_ctx *ctx)
   FeAmgPcg &A = *ctx->amgPcg;
   const bool val64 = !ctx->pcValues32;
   const int slot = val64 ? 2 : 1;
-  FE_REQUIRE(ctx, A.plan[slot].bytes <= kPBlockLds, "nested multigrid PCG: the small levels do not fit the LDS");
+  FE_REQUIRE(ctx, (slot != 2 && A.wplan.R > 0 && ctx->amgWcycle) || A.plan[slot].bytes <= kPBlockLds,
+             "nested multigrid PCG: the small levels do not fit the LDS");
   if(int rc = pcgNumeric(ctx, val64, false, slot)) return rc;
   int flag = 0;
   FE_HIP_TRY(ctx, hipMemcpyAsync(&flag, A.flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -1528,6 +2335,8 @@ int feAmgPcgVcycle(fe_hip_ctx *ctx, const double *r, double *z)
 {
   FE_REQUIRE(ctx, ctx->amgPcg, "nested multigrid: set up before the V-cycle");
   FeAmgPcg &A = *ctx->amgPcg;
+  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // the mailbox is written by the device
+  ((volatile PcgMailbox *)A.mbHost)->barErr = 0;
   const bool val64 = !ctx->pcValues32;
   const int n = A.lv[0].n;
   const int gp = std::min(1024, (n + 255) / 256);
@@ -1539,18 +2348,20 @@ int feAmgPcgVcycle(fe_hip_ctx *ctx, const double *r, double *z)
     hipLaunchKernelGGL((pcgDown0<double, double>), dim3(L0.nTiles), dim3(kFTB), 0, ctx->stream, L0, v.x,
                        (const double *)v.p[0], (const double *)v.q[0], (const double *)v.w, (const double *)v.rhs, v.r[0],
                        (const double *)v.u[1], v.u[0], L1.b, (const double *)A.pq, A.g0, (const double *)A.scal, 0, 1,
-                       A.done, A.stat, A.mbDev, 0, A.rr);
+                       A.done, A.stat, A.mbDev, 0, A.rr, A.bar);
     vcycleRest<double, double>(ctx, A, 2, v, v.r[0], v.u[0]);
   } else {
     const PLev<double, float> L0 = pview<double, float>(A.lv[0]), L1 = pview<double, float>(A.lv[1]);
     hipLaunchKernelGGL((pcgDown0<double, float>), dim3(L0.nTiles), dim3(kFTB), 0, ctx->stream, L0, v.x,
                        (const double *)v.p[0], (const double *)v.q[0], (const double *)v.w, (const double *)v.rhs, v.r[0],
                        (const double *)v.u[1], v.u[0], L1.b, (const double *)A.pq, A.g0, (const double *)A.scal, 0, 1,
-                       A.done, A.stat, A.mbDev, 0, A.rr);
+                       A.done, A.stat, A.mbDev, 0, A.rr, A.bar);
     vcycleRest<double, float>(ctx, A, 1, v, v.r[0], v.u[0]);
   }
   hipLaunchKernelGGL(pcgScatterBack<double>, dim3(gp), dim3(256), 0, ctx->stream, n, A.perm, (const double *)v.z, z);
   FE_HIP_TRY(ctx, hipGetLastError());
+  FE_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  FE_REQUIRE(ctx, !A.mbHost->barErr, "nested multigrid: a grid barrier of the coarse-level launch timed out");
   return FE_STATUS_OK;
 }
 

@@ -706,6 +706,12 @@ int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value)
   } else if(k == "amg_kernel") {
     FE_REQUIRE(ctx, value == 1 || value == 2, "amg_kernel must be 1 (per-level kernels) or 2 (nested-ordered PCG)");
     ctx->amgKernel = value;
+  } else if(k == "amg_fuse") {
+    FE_REQUIRE(ctx, value == 0 || value == 1, "amg_fuse must be 0 (one launch per level) or 1 (levels 1 .. coarsest in one launch)");
+    ctx->amgFuse = value;
+  } else if(k == "amg_wcycle") {
+    FE_REQUIRE(ctx, value == 0 || value == 1, "amg_wcycle must be 0 (V-cycle) or 1 (W-cycle kernel on the coarse levels)");
+    ctx->amgWcycle = value;
   } else if(k == "gather_xcd") {
     FE_REQUIRE(ctx, value == 0 || value == 1, "gather_xcd must be 0 or 1");
     ctx->gatherXcd = value;
@@ -804,6 +810,11 @@ int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value)
   else if(k == "asm_kernel") *value = ctx->asmKernel;
   else if(k == "gather_xcd") *value = ctx->gatherXcd;
   else if(k == "amg_kernel") *value = ctx->amgKernel;
+  else if(k == "amg_wcycle") *value = ctx->amgWcycle;
+  else if(k == "amg_fuse") *value = ctx->amgFuse;
+  else if(k == "amg_fuse_grid") *value = feAmgPcgFuseGrid(ctx);
+  else if(k == "amg_wcycle_level") *value = feAmgPcgWLevel(ctx);
+  else if(k == "amg_wcycle_dense") *value = feAmgPcgWDense(ctx);
   else if(k == "asm_fp32_math") *value = ctx->asmFp32Math;
   else if(k == "scalar_lanes") *value = ctx->scalarLanes;
   else if(k == "fp32_values") *value = ctx->fp32Values;

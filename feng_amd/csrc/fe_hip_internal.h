@@ -274,6 +274,8 @@ struct fe_hip_ctx {
   int amgHostSweeps = 0;                // stalled coarsening: sweeps on the coarsest level
   int amgHostMaxAgg = 0;                // largest aggregate (rows) of any level
   FeAmgPcg *amgPcg = nullptr;
+  int amgWcycle = 0;  // nested PCG with FP32 level values: W-cycle kernel for the coarse levels (1) or V-cycle (0)
+  int amgFuse = 0;    // nested PCG: levels 1 .. coarsest in one launch with grid barriers (1) or one launch per level (0)
   int amgKernel = 2;  // FE_PC_AMG: 1 = per-level kernels over the original numbering,
                       // 2 = nested-ordered fused PCG (fe_amg_pcg.hip)
   bool amgActive = false;  // the current CG solve applies the AMG V-cycle
@@ -385,6 +387,9 @@ int feAmgPcgLevels(const fe_hip_ctx *ctx, int *levels, int *rows, int64_t *nnz, 
 // The nested-ordered PCG takes the hierarchy: two levels at least, a dense
 // coarsest level, aggregates that fit a tile (after feAmgSetup).
 bool feAmgNestedUsable(const fe_hip_ctx *ctx);
+int feAmgPcgWLevel(const fe_hip_ctx *ctx);  // first level of the W-cycle kernel, 0 if none
+int feAmgPcgWDense(const fe_hip_ctx *ctx);  // its dense level, 0 if none
+int feAmgPcgFuseGrid(const fe_hip_ctx *ctx);  // workgroups of the fused coarse-level launch, 0 if not used
 // One V-cycle z = M^-1 r through the nested-ordered kernels (device arrays of
 // n doubles, original numbering), FP64 vectors, values per pcValues32:
 // feAmgPcgApply = feAmgPcgPrepare (pattern once, numeric set-up of the

@@ -132,3 +132,77 @@ def vcycle(levels, b, l=0):
     e = vcycle(levels, bc, l + 1)
     y = u + L.oc * e[L.agg]
     return y + L.l1 * (b - L.A @ y)
+
+
+# ---------------------------------------------------------------------------
+# W-cycle of the coarse levels (the single-workgroup kernel of the nested
+# multigrid PCG, fe_amg_pcg.hip).  Levels 0 .. R-1 take the V-cycle above;
+# each level l of R .. D-1 is solved by `gamma` cycles (the second on the
+# residual b - A e of the first, e = e1 + C(b - A e1)); level D is solved with
+# its exact inverse.  rounded=True models the kernel's data: the FP32 values
+# and l1 weights (the FP32 rounding of the FP64 ones), the weights w rounded
+# to bfloat16 and the row-scaled values h_ij = a_ij w_i rounded to binary16;
+# the operator is then A~ = diag(w)^-1 H and every formula of the cycle uses
+# H and w:  r = b - (H x) / w,  e = y + w b - H y.  Pinned in
+# tests/test_oracle_amg.py: gamma = 1 with exact data is the V-cycle, the
+# two-cycle solve has the error propagation (I - C A~)^2 of its single cycle
+# C, and the bfloat16 rounding is the nearest-even one.
+
+def round_bf16(x):
+    """float32 values rounded to the nearest bfloat16 (ties to even), as float32."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32)
+
+
+def _wdata(L, rounded):
+    """(H, w): row-scaled values as a CSR matrix and the weights, float64."""
+    key = "_wd%d" % int(rounded)
+    if not hasattr(L, key):
+        A = L.A.tocsr()
+        rows = np.repeat(np.arange(A.shape[0]), np.diff(A.indptr))
+        if rounded:
+            w32 = round_bf16(L.l1.astype(np.float32))
+            h = (A.data.astype(np.float32) * w32[rows]).astype(np.float16).astype(np.float64)
+            w = w32.astype(np.float64)
+        else:
+            w = L.l1.copy()
+            h = A.data * w[rows]
+        setattr(L, key, (sp.csr_matrix((h, A.indices, A.indptr), shape=A.shape), w))
+    return getattr(L, key)
+
+
+def _wsolve(levels, l, b, D, rounded, gamma):
+    L = levels[l]
+    if l == D:
+        if not hasattr(L, "AinvW"):
+            L.AinvW = np.linalg.inv(L.A.toarray())
+        return L.AinvW @ b
+    H, w = _wdata(L, rounded)
+
+    def cycle(rhs):
+        x = w * rhs
+        r = rhs - (H @ x) / w
+        ec = _wsolve(levels, l + 1, np.bincount(L.agg, weights=r, minlength=L.nc), D, rounded, gamma)
+        y = x + L.oc * ec[L.agg]
+        return y + w * rhs - H @ y
+
+    e = cycle(b)
+    for _ in range(gamma - 1):
+        e = e + cycle(b - (H @ e) / w)
+    return e
+
+
+def wcycle(levels, b, R, D, rounded=True, gamma=2):
+    """V-cycle on levels 0 .. R-1 whose level-R coarse problem is solved by
+    gamma cycles per level down to the exact level D (R <= D)."""
+    def down(l, rhs):
+        if l == R:
+            return _wsolve(levels, l, rhs, D, rounded, gamma)
+        L = levels[l]
+        u = L.l1 * rhs
+        r = rhs - L.A @ u
+        e = down(l + 1, np.bincount(L.agg, weights=r, minlength=L.nc))
+        y = u + L.oc * e[L.agg]
+        return y + L.l1 * (rhs - L.A @ y)
+    return down(0, np.asarray(b, dtype=np.float64))

@@ -6,7 +6,10 @@ match exactly.  One V-cycle with FP64 level values matches the oracle to
 1e-11 relative (the device sums the Galerkin entries, the l1 norms and the
 row products in other orders, and inverts the coarsest level by Gauss-Jordan
 instead of LAPACK); with FP32 level values (the default) to 1e-5, the FP32
-rounding of the values carried through the cycle.
+rounding of the values carried through the cycle.  With FP32 level values the
+nested PCG solves the coarse levels R .. D with the single-workgroup W-cycle
+kernel (the levels it holds rounded to binary16 / bfloat16 as the oracle's
+wcycle models them); its cycle matches oracle.wcycle to 1e-5.
 """
 import numpy as np
 import pytest
@@ -75,6 +78,48 @@ def test_vcycle_matches_oracle(case, variant):
     S.set_tuning("pc_values32", 1)
     z32 = S.amg_apply(r)
     assert nrel(z32, zo) <= 1e-5, nrel(z32, zo)
+    if variant == "nested":
+        S.set_tuning("amg_wcycle", 1)
+        z32w = S.amg_apply(r)
+        R, D = S.get_tuning("amg_wcycle_level"), S.get_tuning("amg_wcycle_dense")
+        if len(lv) > 2:
+            assert R >= 1 and D > R, (R, D)
+            zw = amg.wcycle(lv, r, R, D)
+            assert nrel(z32w, zw) <= 1e-5, nrel(z32w, zw)
+    S.close()
+
+
+@pytest.mark.parametrize("N, R, D, kernel", [(40, 2, 4, "K64 + K32"), (64, 2, 4, "K160 + K32")])
+def test_wcycle_kernel_matches_oracle_on_cubes(N, R, D, kernel):
+    """The W-cycle kernel with two register levels (levels 2 and 3 of the P1
+    tet GRF cubes 40^3 and 64^3, dense level 4) against oracle.wcycle;
+    switched off (amg_wcycle 0, the default) the nested path is the V-cycle,
+    with levels 1 .. coarsest in one launch (amg_fuse 1) bitwise equal to one
+    launch per level (amg_fuse 0) and to the oracle's V-cycle to 1e-5."""
+    from feng_amd.api import FN_CONST, FN_ELEM, Fn, Mesh
+    from feng_amd.problems import diffusion_system
+    print("W-cycle kernel", N, kernel, flush=True)
+    m = Mesh.cube(N)
+    S, num = diffusion_system(m, 4, 1, Fn(FN_ELEM, data=m.coeff_grf(0.1, 1.5, 512, 1)), Fn(FN_CONST, -1.0))
+    S.assemble(3)
+    A, rhs, lv = _oracle_levels(S)
+    r = np.random.default_rng(11).normal(size=A.shape[0])
+    S.set_tuning("amg_wcycle", 1)
+    z = S.amg_apply(r)
+    assert (S.get_tuning("amg_wcycle_level"), S.get_tuning("amg_wcycle_dense")) == (R, D)
+    zo = amg.wcycle(lv, r, R, D)
+    err = nrel(z, zo)
+    print("W-cycle vs oracle", N, err, flush=True)
+    assert err <= 1e-5, err
+    S.set_tuning("amg_wcycle", 0)
+    assert S.get_tuning("amg_wcycle_level") == 0
+    S.set_tuning("amg_fuse", 1)
+    zv = S.amg_apply(r)
+    assert S.get_tuning("amg_fuse_grid") >= 2
+    assert nrel(zv, amg.vcycle(lv, r)) <= 1e-5
+    S.set_tuning("amg_fuse", 0)
+    assert S.get_tuning("amg_fuse_grid") == 0
+    assert np.array_equal(S.amg_apply(r), zv)
     S.close()
 
 
@@ -164,9 +209,10 @@ def test_vcycle_on_a_block_diagonal_batch_matches_oracle():
 @pytest.mark.parametrize("case", ["tet_grf_16", "p2_blocks_64"])
 def test_nested_pcg_matches_per_level_pcg(oracle, case):
     """The nested-ordered PCG (standard CG, reductions folded into the next
-    kernel) and the per-level pipelined PCG apply the same V-cycle: both reach
-    the FP64 target, with iteration counts within 20% of each other, in FP64
-    and with FP32 inner solves under FP64 refinement."""
+    kernel) and the per-level pipelined PCG: both reach the FP64 target, in
+    FP64 and with FP32 inner solves under FP64 refinement; with the V-cycle
+    (amg_wcycle 0) their iteration counts are within 20% of each other, and
+    the W-cycle kernel takes no more iterations than the V-cycle."""
     from feng_amd.api import FN_CONST, FN_ELEM, PC_AMG, PREC_FP32_IR, Fn, Mesh
     from feng_amd.problems import diffusion_system
     print("nested vs per-level", case, flush=True)
@@ -181,8 +227,9 @@ def test_nested_pcg_matches_per_level_pcg(oracle, case):
     ia, ja = S.get_csr()
     val, rhs = S.get_system()
     out = {}
-    for kern in (2, 1):
-        S.set_tuning("amg_kernel", kern)
+    for kern in (2, 1, 3):  # 3: nested with the V-cycle
+        S.set_tuning("amg_kernel", 2 if kern == 3 else kern)
+        S.set_tuning("amg_wcycle", 0 if kern == 3 else 1)
         for prec in (0, PREC_FP32_IR):
             r, du = S.solve(pc=PC_AMG, precision=prec, rtol=1e-8, inner_rtol=1e-4, want_du=True)
             assert r.converged and r.relres <= 1e-8, (kern, prec, r)
@@ -190,9 +237,11 @@ def test_nested_pcg_matches_per_level_pcg(oracle, case):
             assert np.linalg.norm(res) <= 1.01e-8 * np.linalg.norm(rhs), (kern, prec)
             out[(kern, prec)] = (r.iterations, du)
     for prec in (0, PREC_FP32_IR):
-        it2, it1 = out[(2, prec)][0], out[(1, prec)][0]
-        print(case, "precision", prec, "iterations nested / per-level:", it2, it1, flush=True)
-        assert abs(it2 - it1) <= max(2, 0.2 * it1), (prec, it2, it1)
-        d2, d1 = out[(2, prec)][1], out[(1, prec)][1]
-        assert np.abs(d2 - d1).max() <= 1e-5 * np.abs(d1).max()
+        itW, it1, itV = out[(2, prec)][0], out[(1, prec)][0], out[(3, prec)][0]
+        print(case, "precision", prec, "iterations nested W / per-level / nested V:", itW, it1, itV, flush=True)
+        assert abs(itV - it1) <= max(2, 0.2 * it1), (prec, itV, it1)
+        assert itW <= itV + 1, (prec, itW, itV)
+        for kern in (2, 3):
+            d2, d1 = out[(kern, prec)][1], out[(1, prec)][1]
+            assert np.abs(d2 - d1).max() <= 1e-5 * np.abs(d1).max()
     S.close()

@@ -164,3 +164,95 @@ def test_stalled_coarsening_on_block_diagonal_matrix_is_spd():
         m = B.shape[0]
         assert np.abs(M[r0:r0 + m, :r0]).max(initial=0) == 0 and np.abs(M[r0:r0 + m, r0 + m:]).max(initial=0) == 0
         r0 += m
+
+
+def _bf16_by_search(x):
+    """Nearest bfloat16 (ties to even) of one float32, from the two bfloat16
+    neighbours that bracket it (truncation and the next one away from zero)."""
+    u = int(np.float32(x).view(np.uint32))
+    lo = u & 0xFFFF0000
+    hi = lo + 0x10000
+    f = lambda b: float(np.uint32(b).view(np.float32))
+    dl, dh = abs(float(x) - f(lo)), abs(f(hi) - float(x))
+    if dl < dh or (dl == dh and ((lo >> 16) & 1) == 0):
+        return f(lo)
+    return f(hi)
+
+
+def test_bf16_rounding_is_nearest_even():
+    rng = np.random.default_rng(7)
+    x = (rng.normal(size=4000) * np.exp(rng.normal(size=4000) * 8)).astype(np.float32)
+    # exact ties of both parities
+    x[:4] = np.array([1 + 2 ** -8, 1 + 3 * 2 ** -8, -(1 + 2 ** -8), 3 * (1 + 5 * 2 ** -8)], np.float32)
+    r = amg.round_bf16(x)
+    assert (r.view(np.uint32) & 0xFFFF == 0).all()
+    assert np.array_equal(r, np.array([_bf16_by_search(v) for v in x], np.float32))
+    assert list(r[:2]) == [1.0, 1 + 2 ** -6]
+
+
+def test_wcycle_with_one_cycle_per_level_is_the_vcycle():
+    A = csr(30)
+    lv = amg.hierarchy(A)
+    assert len(lv) >= 3
+    b = np.random.default_rng(5).normal(size=A.shape[0])
+    z = amg.vcycle(lv, b)
+    for R in range(len(lv)):
+        zw = amg.wcycle(lv, b, R, len(lv) - 1, rounded=False, gamma=1)
+        assert np.linalg.norm(zw - z) <= 1e-13 * np.linalg.norm(z), R
+
+
+def test_two_cycle_solve_squares_the_error_propagation():
+    """gamma = 2 on level R: I - W A~ = (I - C A~)^2 with C the single cycle
+    of that level over the same coarser solves, on the rounded operator
+    A~ = diag(w)^-1 H; with gamma = 2 below too, and down to a dense level
+    above the coarsest."""
+    A = csr(30)
+    lv = amg.hierarchy(A)
+    R = 1
+    for D in (len(lv) - 1, 2):
+        L = lv[R]
+        n = L.A.shape[0]
+        H, w = amg._wdata(L, True)
+        At = (H.toarray().T / w).T
+        W = np.column_stack([amg._wsolve(lv, R, e, D, True, 2) for e in np.eye(n)])
+        C = np.column_stack([amg._wsolve(lv, R, e, D, True, 1) for e in np.eye(n)])
+        # C here already solves level R + 1 with gamma = 1; the identity needs
+        # the same coarse solve in both, so compare on a hierarchy where R + 1
+        # is the exact level
+        if D == R + 1:
+            E1 = np.eye(n) - C @ At
+            assert np.abs((np.eye(n) - W @ At) - E1 @ E1).max() <= 1e-10
+        # the rounded operator differs from the FP64 one by the rounding only
+        assert np.abs(At - L.A.toarray()).max() <= 2e-3 * np.abs(L.A.toarray()).max()
+        # gamma = 2 contracts more than gamma = 1 on the level's own operator
+        rho = lambda M: np.abs(np.linalg.eigvals(np.eye(n) - M @ At)).max()
+        assert rho(W) < rho(C) < 1
+
+
+def test_wcycle_pcg_beats_the_vcycle_on_a_3d_problem(oracle):
+    """Iterations of PCG to 1e-8 on a P1 tet GRF problem: the W-cycle from
+    level 1 needs fewer than the V-cycle."""
+    import feng_amd.api as Api
+    m = Api.Mesh.cube(12)
+    k = m.coeff_grf(0.1, 1.5, 64, 5)
+    num = Api.Numbering(m, [4], [1])
+    nl = num.adr[0].size // m.E
+    ia, ja = oracle.csr(num.nInc, [(num.adr[0], nl, num.adr[0], nl)])
+    val, rhs = oracle.assemble_diffusion(4, m.xyz, m.conn, num.adr[0], num.nInc, ia, ja, 1, (4, 0.0, 0.0, k),
+                                         (0, -1.0, 0.0, None), np.zeros(num.nDofs))
+    n = len(ia) - 1
+    A = sp.csr_matrix((val, ja, ia), shape=(n, n))
+    lv = amg.hierarchy(A)
+
+    def pcg(M):
+        x = np.zeros(n); r = rhs.copy(); z = M(r); p = z.copy(); rz = r @ z
+        for it in range(1, 500):
+            q = A @ p; a = rz / (p @ q); x += a * p; r -= a * q
+            if np.linalg.norm(r) <= 1e-8 * np.linalg.norm(rhs):
+                return it
+            z = M(r); rz2 = r @ z; p = z + rz2 / rz * p; rz = rz2
+        return 500
+    itV = pcg(lambda r: amg.vcycle(lv, r))
+    itW = pcg(lambda r: amg.wcycle(lv, r, 1, len(lv) - 1))
+    print("V", itV, "W", itW)
+    assert itW < itV
