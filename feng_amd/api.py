@@ -28,7 +28,7 @@ MIXED_GRADIENT, MIXED_DIVERGENCE = 26, 31
 # coefficient kinds
 FN_CONST, FN_SINSIN, FN_BLOCKS, FN_SMOOTH3, FN_ELEM, FN_QP, FN_KOV_U, FN_KOV_P, FN_POLY_U = range(9)
 
-KSP_CG, KSP_GMRES = 0, 1
+KSP_CG, KSP_GMRES, KSP_DIRECT = 0, 1, 2
 PC_NONE, PC_JACOBI, PC_IC0, PC_ILU0, PC_BJ_ILU0, PC_LEARNED, PC_AMG, PC_FIELDSPLIT = range(8)
 PREC_FP64, PREC_FP32_IR = 0, 1
 

@@ -23,7 +23,7 @@ INC = ROOT / "include"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["fe_ctx.hip", "fe_csr.hip", "fe_assemble.hip", "fe_krylov.hip", "fe_precond.hip", "fe_gmres.hip", "fe_learned.hip", "fe_batch.hip", "fe_amg.hip", "fe_norm.hip", "fe_constraints.hip", "fe_gather.hip", "fe_amg_pcg.hip", "fe_fieldsplit.hip"]
+HIP_SOURCES = ["fe_ctx.hip", "fe_csr.hip", "fe_assemble.hip", "fe_krylov.hip", "fe_precond.hip", "fe_gmres.hip", "fe_learned.hip", "fe_batch.hip", "fe_amg.hip", "fe_norm.hip", "fe_constraints.hip", "fe_gather.hip", "fe_amg_pcg.hip", "fe_fieldsplit.hip", "fe_direct.hip"]
 
 # Packed FP32 instructions are disabled for every HIP translation unit.
 HIP_FLAGS = [
@@ -66,7 +66,7 @@ def build_hip(jobs: int = 4) -> Path:
         if p.wait() != 0:
             raise RuntimeError("hipcc failed")
     out = LIB / "libfe_hip.so"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out, *objs, "-lrocsolver", "-lrocblas"])
     return out
 
 

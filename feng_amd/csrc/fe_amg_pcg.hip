@@ -688,12 +688,14 @@ __global__ __launch_bounds__(kPBlockTB) void pcgBlock(const PLev<T, TV> *__restr
 }
 
 // ---------------------------------------------------------------------------
-// Levels 1 .. coarsest in one launch
+// Levels F .. coarsest in one launch
 //
-// The restriction and prolongation of the levels above the single-workgroup
-// ones (levels 1 .. B-1) and the single-workgroup levels (B .. coarsest) run
-// in one launch of G workgroups separated by grid barriers, instead of
-// 2 (B - 1) + 1 dependent launches.  Workgroup w owns, on every level l < B,
+// The restriction and prolongation of the levels F .. B-1 above the
+// single-workgroup ones and the single-workgroup levels (B .. coarsest) run in
+// one launch of G workgroups separated by grid barriers, instead of
+// 2 (B - F) + 1 dependent launches; F is the first level whose shares fit one
+// round of the workgroup (levels 1 .. F-1 keep their own launches, where the
+// rows of a level are spread over many more workgroups).  Workgroup w owns, on every level F <= l < B,
 // the rows nested in its contiguous share of level-B rows (tiles[l][w] ..
 // tiles[l][w + 1]): its restrictions are sums inside its own rows, and the
 // neighbour values a row needs (l1 b, u + oc e_next) are recomputed from
@@ -703,7 +705,7 @@ __global__ __launch_bounds__(kPBlockTB) void pcgBlock(const PLev<T, TV> *__restr
 // workgroup holds one CU, so all are resident; each barrier wait is bounded
 // and a timeout raises a flag (no wait is endless).
 constexpr int kFuseMaxG = 64;
-constexpr int kFuseRows = 1280;  // rows of one level per workgroup (LDS partial residuals)
+constexpr int kFuseRows = kPBlockTB / kPLPR;  // rows of one level per workgroup: one round of kPLPR lanes per row
 constexpr int kFuseLds = kPBlockLds + kFuseRows * 8;
 constexpr int kFuseSpin = 1 << 21;
 
@@ -735,7 +737,8 @@ __device__ __forceinline__ void gridBarrier(unsigned *cnt, unsigned target, int 
 template <typename T, typename TV>
 __global__ __launch_bounds__(kPBlockTB) void pcgCoarseV(const PLev<T, TV> *__restrict__ lv, PBlockPlan pl,
                                                         const unsigned char *__restrict__ img,
-                                                        const int *__restrict__ tiles, int G, T oc, const int *done,
+                                                        const int *__restrict__ tiles, int G, int F, T oc,
+                                                        const int *done,
                                                         unsigned *bar, int *err, T *__restrict__ y0,
                                                         const T *__restrict__ u0, const int *__restrict__ aggStart0)
 {
@@ -748,7 +751,7 @@ __global__ __launch_bounds__(kPBlockTB) void pcgCoarseV(const PLev<T, TV> *__res
   const int rl = t / kPLPR, sub = t % kPLPR;
   constexpr int RPR = kPBlockTB / kPLPR;  // rows per round
   unsigned target = 0;
-  for(int l = 1; l < B; ++l) {  // restrictions: b_l+1 = P^T (b_l - A (l1 b_l))
+  for(int l = F; l < B; ++l) {  // restrictions: b_l+1 = P^T (b_l - A (l1 b_l))
     const PLev<T, TV> L = lv[l];
     T *bn = lv[l + 1].b;
     const int r0 = tiles[l * (G + 1) + w], r1 = tiles[l * (G + 1) + w + 1];
@@ -777,7 +780,7 @@ __global__ __launch_bounds__(kPBlockTB) void pcgCoarseV(const PLev<T, TV> *__res
   }
   if(w == 0) blockSolve<T, TV>(lv, pl, lds, oc, (T *)nullptr, (const T *)nullptr);
   gridBarrier(bar, target += G, err);
-  for(int l = B - 1; l >= 1; --l) {  // prolongations: e_l = y + S (b_l - A y), y = u + oc P e_l+1
+  for(int l = B - 1; l >= F; --l) {  // prolongations: e_l = y + S (b_l - A y), y = u + oc P e_l+1
     const PLev<T, TV> L = lv[l];
     const T *eNext = lv[l + 1].e;
     const int r0 = tiles[l * (G + 1) + w], r1 = tiles[l * (G + 1) + w + 1];
@@ -800,7 +803,7 @@ __global__ __launch_bounds__(kPBlockTB) void pcgCoarseV(const PLev<T, TV> *__res
         }
       }
     }
-    if(l > 1) gridBarrier(bar, target += G, err);
+    if(l > F) gridBarrier(bar, target += G, err);
   }
 }
 
@@ -1416,7 +1419,7 @@ struct FeAmgPcg {
   unsigned char *wImg = nullptr;
   // fused coarse-level launch per slot (fuseG = 0: not used): tiles of
   // (B + 1) x (G + 1) row starts; the grid-barrier counter
-  int fuseG[3] = {0, 0, 0};
+  int fuseG[3] = {0, 0, 0}, fuseF[3] = {0, 0, 0};
   int *fuseTiles[3] = {nullptr, nullptr, nullptr};
   unsigned *bar = nullptr;
   int g0 = 0;  // grid of the row kernels of level 0
@@ -1641,35 +1644,42 @@ int wSetup(fe_hip_ctx *ctx, FeAmgPcg &A, const std::vector<std::vector<int>> &hI
 }
 
 // Tiles of the fused coarse-level launch for a single-workgroup start level
-// B: the level-B rows cut into contiguous shares so that no level l < B gets
-// more than kFuseRows rows in a share; level-l tiles are the rows nested in
-// each share.  Returns G (0: B <= 1, or a share would exceed the limits).
+// B: the level-B rows cut into contiguous shares so that no level F <= l < B
+// gets more than kFuseRows rows in a share; level-l tiles are the rows nested
+// in each share, for l = F .. B.  F is the lowest level (>= 1) with at most
+// kFuseMaxG shares.  Returns G (0: B <= 1 or no such level).
 int fuseTilesFor(const std::vector<std::vector<int>> &aggStart, const std::vector<FeAmgPcgLevel> &lv, int B,
-                 std::vector<int> &tiles)
+                 std::vector<int> &tiles, int *Fout)
 {
   if(B <= 1 || B >= (int)lv.size()) return 0;
   auto start = [&](int l, int r) {  // first level-l row nested in level-B row r
     for(int m = B - 1; m >= l; --m) r = aggStart[m][r];
     return r;
   };
-  std::vector<int> cuts(1, 0);
   const int nB = lv[B].n;
-  for(int r = 1; r <= nB; ++r) {
-    bool over = false;
-    for(int l = 1; l < B; ++l) over = over || start(l, r) - start(l, cuts.back()) > kFuseRows;
-    if(over) {
-      if(r - 1 == cuts.back()) return 0;  // one level-B row alone is too large
-      cuts.push_back(r - 1);
-      --r;
+  for(int F = 1; F < B; ++F) {
+    std::vector<int> cuts(1, 0);
+    bool bad = false;
+    for(int r = 1; r <= nB && !bad; ++r) {
+      bool over = false;
+      for(int l = F; l < B; ++l) over = over || start(l, r) - start(l, cuts.back()) > kFuseRows;
+      if(over) {
+        if(r - 1 == cuts.back()) bad = true;  // one level-B row alone is too large
+        cuts.push_back(r - 1);
+        --r;
+      }
     }
+    if(bad) continue;
+    cuts.push_back(nB);
+    const int G = (int)cuts.size() - 1;
+    if(G > kFuseMaxG) continue;
+    tiles.assign((size_t)(B + 1) * (G + 1), 0);
+    for(int l = F; l <= B; ++l)
+      for(int w = 0; w <= G; ++w) tiles[(size_t)l * (G + 1) + w] = l == B ? cuts[w] : start(l, cuts[w]);
+    *Fout = F;
+    return G;
   }
-  cuts.push_back(nB);
-  const int G = (int)cuts.size() - 1;
-  if(G > kFuseMaxG) return 0;
-  tiles.assign((size_t)(B + 1) * (G + 1), 0);
-  for(int l = 1; l <= B; ++l)
-    for(int w = 0; w <= G; ++w) tiles[(size_t)l * (G + 1) + w] = l == B ? cuts[w] : start(l, cuts[w]);
-  return G;
+  return 0;
 }
 
 // Nested-ordered copy of the host hierarchy (once per CSR pattern).
@@ -1857,7 +1867,7 @@ int pcgSetupPattern(fe_hip_ctx *ctx)
   for(int s = 0; s < 3; ++s) {
     std::vector<int> tiles;
     if(A->plan[s].bytes > kPBlockLds) continue;
-    A->fuseG[s] = fuseTilesFor(aggStart, A->lv, A->plan[s].from, tiles);
+    A->fuseG[s] = fuseTilesFor(aggStart, A->lv, A->plan[s].from, tiles, &A->fuseF[s]);
     if(A->fuseG[s] > 0)
       if(int rc = upload(ctx, &A->fuseTiles[s], tiles)) return rc;
   }
@@ -2015,12 +2025,25 @@ void vcycleRest(fe_hip_ctx *ctx, FeAmgPcg &A, int slot, const PcgVecs<T> &v, con
   const int B = useW ? A.wplan.R : pl.from;
   const T oc = (T)ctx->amgOverCorrect;
   if(!useW && ctx->amgFuse && A.fuseG[slot] > 0) {
+    const int F = A.fuseF[slot];
+    const PLev<T, TV> L0 = pview<T, TV>(A.lv[0]);
+    for(int l = 1; l < F; ++l) {
+      const PLev<T, TV> L = pview<T, TV>(A.lv[l]);
+      const PLev<T, TV> N = pview<T, TV>(A.lv[l + 1]);
+      hipLaunchKernelGGL((pcgDownL<T, TV>), dim3(L.nTiles), dim3(kPTB), 0, ctx->stream, L, N.b, A.done);
+    }
     hipLaunchKernelGGL((pcgCoarseV<T, TV>), dim3(A.fuseG[slot]), dim3(kPBlockTB), 0, ctx->stream,
                        (const PLev<T, TV> *)A.desc[slot], pl, (const unsigned char *)A.img[slot],
-                       (const int *)A.fuseTiles[slot], A.fuseG[slot], oc, (const int *)A.done, A.bar, &A.mbDev->barErr,
-                       v.y, u0, (const int *)A.lv[0].aggStart);
-    hipLaunchKernelGGL((pcgUp0<T, TV>), dim3(A.g0), dim3(kFTB), 0, ctx->stream, pview<T, TV>(A.lv[0]), (const T *)v.y,
-                       rNew, v.z, A.done, A.rz);
+                       (const int *)A.fuseTiles[slot], A.fuseG[slot], F, oc, (const int *)A.done, A.bar,
+                       &A.mbDev->barErr, v.y, u0, (const int *)L0.aggStart);
+    for(int l = F - 1; l >= 1; --l) {
+      const PLev<T, TV> L = pview<T, TV>(A.lv[l]);
+      const PLev<T, TV> N = pview<T, TV>(A.lv[l + 1]);
+      hipLaunchKernelGGL((pcgUpL<T, TV>), dim3((L.n + kPRPB - 1) / kPRPB), dim3(kPTB), 0, ctx->stream, L, N.e, oc,
+                         A.done, l == 1 ? v.y : (T *)nullptr, u0, L0.aggStart);
+    }
+    hipLaunchKernelGGL((pcgUp0<T, TV>), dim3(A.g0), dim3(kFTB), 0, ctx->stream, L0, (const T *)v.y, rNew, v.z,
+                       A.done, A.rz);
     return;
   }
   const PLev<T, TV> *d = (const PLev<T, TV> *)A.desc[slot];

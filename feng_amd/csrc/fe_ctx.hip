@@ -271,6 +271,7 @@ int fe_hip_ctx_destroy(fe_hip_ctx *ctx)
   feAmgRelease(ctx);
   feAmgPcgRelease(ctx);
   feFieldSplitRelease(ctx);
+  feDirectRelease(ctx);
   for(void *p : ctx->owned) (void)hipFree(p);
   for(auto &g : ctx->graphs) (void)hipGraphExecDestroy((hipGraphExec_t)g.second);
   ctx->graphs.clear();

@@ -178,6 +178,7 @@ struct FeMpnn;   // learned-preconditioner state (fe_learned.hip)
 struct FeAmg;    // aggregation multigrid hierarchy (fe_amg.hip)
 struct FeAmgPcg;  // nested-ordered multigrid PCG (fe_amg_pcg.hip)
 struct FeFieldSplit;  // Schur block preconditioner of saddle-point systems (fe_fieldsplit.hip)
+struct FeDirect;      // dense LU of the direct solve (fe_direct.hip)
 
 // Host copy of the aggregation hierarchy of the CSR pattern (built by
 // feAmgSetup): per level the CSR pattern, the aggregate of each row (empty
@@ -195,6 +196,7 @@ struct fe_hip_ctx {
   bool borrowedStream = false;  // a sub-context on its parent's stream: not destroyed with it
   std::string err;
   FeFieldSplit *fieldSplit = nullptr;
+  FeDirect *direct = nullptr;
 
   // mesh
   int dim = 0, V = 0, E = 0, nvpe = 0;
@@ -381,6 +383,8 @@ struct FeAmgPcgResult {
 // FP64 residual of the system: r = rhs - A x (original CSR), *out = ||r||_2^2
 // (stream synchronized).
 int feResidualNorm2(fe_hip_ctx *ctx, const double *x, double *r, double *out);
+int feSolveDirect(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_solve_stats *stats);
+void feDirectRelease(fe_hip_ctx *ctx);
 int feAmgPcgSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double bnorm, double target, double *x64,
                   double *r64, FeAmgPcgResult *res);
 int feAmgPcgLevels(const fe_hip_ctx *ctx, int *levels, int *rows, int64_t *nnz, int cap);

@@ -1702,6 +1702,7 @@ int feResidualNorm2(fe_hip_ctx *ctx, const double *x, double *r, double *out)
 int feSolve(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hip_solve_stats *stats)
 {
   if(o->ksp == FE_KSP_GMRES) return feSolveGmres(ctx, o, du, stats);
+  if(o->ksp == FE_KSP_DIRECT) return feSolveDirect(ctx, o, du, stats);
   FE_REQUIRE(ctx, o->ksp == FE_KSP_CG, "unknown Krylov method");
   FE_REQUIRE(ctx, o->pc == FE_PC_JACOBI || o->pc == FE_PC_IC0 || o->pc == FE_PC_LEARNED || o->pc == FE_PC_AMG,
              "CG takes the JACOBI, IC0, LEARNED or AMG preconditioner");

@@ -73,6 +73,14 @@ typedef struct {
  * reference selects through PETSc options, feLinearSystemPETSc.cpp:900-1031). */
 #define FE_KSP_CG 0
 #define FE_KSP_GMRES 1
+/* Direct solve: LU with partial pivoting of the dense copy of the system on
+ * the device (rocSOLVER getrf / getrs), up to 46000 unknowns; the role of
+ * the reference's direct back ends (MKL Pardiso,
+ * feLinearSystemMklPardiso.cpp:1154-1222; PETSc -pc_type lu with MUMPS,
+ * tests/withLinearSolver/navier_stokes.cpp:101-108).  pc, precision and the
+ * iteration options are ignored; the FP64 true residual is checked against
+ * rtol / atol; iterations = 1. */
+#define FE_KSP_DIRECT 2
 
 #define FE_PC_NONE 0
 #define FE_PC_JACOBI 1

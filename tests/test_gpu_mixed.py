@@ -6,10 +6,10 @@ coupled system against the oracle on P1, P2, P3 triangles and P1 tets.
 
 The coupled system's diagonal blocks are mass matrices (O(h^2)) against O(1)
 grad-grad couplings: ILU(0) pivots on them and the device GMRES/ILU(0)
-converges only on the coarsest mesh with P1 / P2, where it is checked; the table's
-Newton steps solve the device-assembled systems with a host direct solver,
-in place of the reference's direct solver, which has no device counterpart
-here."""
+converges only on the coarsest mesh with P1 / P2, where it is checked; the
+table's Newton steps solve on the device with the direct solve (FE_KSP_DIRECT,
+dense LU with partial pivoting), the role of the reference's direct solver;
+the direct solve equals a host sparse LU of the same system."""
 import numpy as np
 import pytest
 
@@ -27,23 +27,39 @@ def _system(m, kind, deg=8):
                                   lambda x, y, z, c: u_exact(x, y), lambda x, y, z, c: v_exact(x, y))
 
 
-def _host_newton(S, n, tol=1e-13):
-    """Newton on the device-assembled residual and Jacobian, each step solved
-    by a host direct solver."""
+def _direct_newton(S, tol=1e-13):
+    """Newton on the device (feNonLinearSolver's loop), every step solved by
+    the device direct solve."""
+    from feng_amd.api import KSP_DIRECT
+    from feng_amd.problems import NLSolverOptions, solve_newton
+    rep = solve_newton(S, NLSolverOptions(tol, tol, 1e4, 20), ksp=KSP_DIRECT, rtol=1e-12)
+    assert rep.converged and all(r.converged for r in rep.linear)
+    return rep.iterations
+
+
+def test_direct_solve_equals_host_sparse_lu():
+    """One Newton system of the P2 mixed problem on square3.msh (1922
+    unknowns, nonsymmetric): the device direct solve equals scipy's sparse LU
+    to 1e-12 and its FP64 residual is at round-off."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spla
+    from feng_amd.api import KSP_DIRECT, Mesh
+    from tests.conftest import GOLDEN
+    m = Mesh.read_msh(GOLDEN / "meshes" / "square3.msh")
+    S, num = _system(m, KIND_OF[2])
+    n = num.nInc
+    S.setToZero()
+    S.assemble(3)
     ia, ja = S.get_csr()
-    for it in range(20):
-        S.setToZero()
-        S.assemble(3)
-        val, rhs = S.get_system()
-        du = spla.spsolve(sp.csr_matrix((val, ja, ia), shape=(n, n)).tocsc(), rhs)
-        u = S.get_solution()
-        u[:n] += du
-        S.set_solution(u)
-        if np.abs(du).max() <= tol:
-            return it + 1
-    raise AssertionError("Newton did not converge")
+    val, rhs = S.get_system()
+    A = sp.csr_matrix((val, ja, ia), shape=(n, n))
+    assert abs(A - A.T).max() > 1e-8 * abs(A).max()  # a nonsymmetric system
+    r, du = S.solve(ksp=KSP_DIRECT, rtol=1e-12, want_du=True)
+    ref = spla.spsolve(A.tocsc(), rhs)
+    print("direct vs host LU", np.abs(du - ref).max() / np.abs(ref).max(), "relres", r.relres, flush=True)
+    assert r.converged and r.iterations == 1 and r.relres <= 1e-13
+    assert np.abs(du - ref).max() <= 1e-12 * np.abs(ref).max()
+    S.close()
 
 
 @pytest.mark.parametrize("order", [1, 2, 3, 4])
@@ -58,7 +74,7 @@ def test_reference_mixed_diffusion_table(order):
     for lvl in range(1, 5):
         m = Mesh.read_msh(GOLDEN / "meshes" / f"square{lvl}.msh")
         S, num = _system(m, kind)
-        assert _host_newton(S, num.nInc) > 2  # a nonlinear problem
+        assert _direct_newton(S) > 2  # a nonlinear problem
         for f, ex in ((0, u_exact), (1, v_exact)):
             err[f].append(S.norm(NORM_L2_ERROR, f, 8, [Fn(FN_QP, data=_qp_table(m, 8, ex))]))
         S.close()
