@@ -57,6 +57,8 @@ HOST_API = {
     "fe_numbering_create": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, c_void_pp]),
     "fe_numbering_create_groups": (C.c_int, [C.c_void_p, C.c_int, c_int_p, C.POINTER(C.c_uint), c_void_pp]),
     "fe_mesh_boundary_groups": (C.c_int, [C.c_void_p, c_int_p, C.c_char_p, C.c_int, c_int_p]),
+    "fe_numbering_periodic": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                        c_int_p, c_int_p, C.c_int, c_int_p]),
     "fe_numbering_destroy": (C.c_int, [C.c_void_p]),
     "fe_numbering_info": (C.c_int, [C.c_void_p, c_int_p, c_int_p]),
     "fe_numbering_adr": (C.c_int, [C.c_void_p, C.c_int, c_int_p]),

@@ -194,6 +194,23 @@ class Numbering:
         N.host().fe_numbering_dof_info(self.h, N.ptr(self.dof_xyz, C.c_double),
                                        N.ptr(self.dof_field, C.c_int), N.ptr(self.dof_comp, C.c_int))
 
+    def periodic_dofs(self, field, master_group, slave_group, offset):
+        """feNumber::applyPeriodicity: (master, slave) DOF arrays of `field`
+        between two boundary groups (names or indices), the slave side at
+        master + offset; for LinearSystemHIP.set_periodic_dofs."""
+        names, _ = self.mesh.boundary_groups()
+        gi = lambda g: names.index(g) if isinstance(g, str) else int(g)
+        off = np.zeros(3)
+        off[:len(offset)] = offset
+        cnt = C.c_int()
+        fn = N.host().fe_numbering_periodic
+        _chk(fn(self.mesh.h, self.h, field, gi(master_group), gi(slave_group), N.ptr(off, C.c_double), None, None, 0,
+                C.byref(cnt)), "fe_numbering_periodic")
+        master, slave = np.zeros(cnt.value, np.int32), np.zeros(cnt.value, np.int32)
+        _chk(fn(self.mesh.h, self.h, field, gi(master_group), gi(slave_group), N.ptr(off, C.c_double),
+                N.ptr(master, C.c_int), N.ptr(slave, C.c_int), cnt.value, C.byref(cnt)), "fe_numbering_periodic")
+        return master, slave
+
     def __del__(self):
         try:
             if self.h:

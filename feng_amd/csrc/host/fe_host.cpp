@@ -37,6 +37,11 @@ struct fe_numbering {
   std::vector<std::vector<int>> adr;
   std::vector<double> dofXyz;
   std::vector<int> dofField, dofComp;
+  // per field: DOFs of each vertex (nvd per vertex: components) and of each
+  // edge (ned per edge: components for the vector kind, else positions from
+  // the edge's first vertex)
+  std::vector<std::vector<int>> vnum, enm;
+  std::vector<int> nvd, ned;
 };
 
 namespace {
@@ -645,6 +650,10 @@ int fe_numbering_create_groups(const fe_mesh *m, int nfields, const int *kinds, 
     }
   }
   num->nDofs = g;
+  num->vnum = vnum;
+  num->enm = enm;
+  num->nvd = nvd;
+  num->ned = ned;
   num->adr.resize(nfields);
   num->dofXyz.assign((size_t)3 * g, 0.);
   num->dofField.assign(g, -1);
@@ -707,6 +716,90 @@ int fe_numbering_create_groups(const fe_mesh *m, int nfields, const int *kinds, 
     }
   }
   *out = num;
+  return FE_STATUS_OK;
+}
+
+int fe_numbering_periodic(const fe_mesh *m, const fe_numbering *num, int field, int masterGroup, int slaveGroup,
+                          const double *offset, int *master, int *slave, int cap, int *count)
+{
+  if(!m || !num || !offset || !count || field < 0 || field >= num->nfields) return FE_STATUS_POINTER_ERROR;
+  if(m->dim != 2 || num->vnum.empty()) return FE_STATUS_FORMAT_ERROR;
+  const int ngr = m->groups.empty() ? 1 : (int)m->groups.size();
+  if(masterGroup < 0 || masterGroup >= ngr || slaveGroup < 0 || slaveGroup >= ngr) return FE_STATUS_POINTER_ERROR;
+  const double tol = 1e-8;  // the reference's matching tolerance
+  auto groupOf = [&](int b) { return m->bgroup.empty() ? 0 : m->bgroup[b]; };
+  auto X = [&](int v, int k) { return m->xyz[3 * (size_t)v + k]; };
+  auto near = [&](int v0, int v1) {  // x(v1) = x(v0) + offset
+    double d = 0.;
+    for(int k = 0; k < 3; ++k) d += (X(v1, k) - X(v0, k) - offset[k]) * (X(v1, k) - X(v0, k) - offset[k]);
+    return std::sqrt(d) < tol;
+  };
+  // vertices and edges of each group, in boundary-element order (unique)
+  std::vector<int> v0, v1, e0, e1;
+  std::vector<char> seenV(m->V, 0), seenE(m->nEdges, 0);
+  for(int pass = 0; pass < 2; ++pass) {
+    std::fill(seenV.begin(), seenV.end(), 0);
+    std::fill(seenE.begin(), seenE.end(), 0);
+    const int grp = pass == 0 ? masterGroup : slaveGroup;
+    std::vector<int> &vs = pass == 0 ? v0 : v1, &es = pass == 0 ? e0 : e1;
+    for(int b = 0; b < m->nB; ++b) {
+      if(groupOf(b) != grp) continue;
+      for(int j = 0; j < m->bnv; ++j) {
+        const int v = m->bconn[(size_t)b * m->bnv + j];
+        if(!seenV[v]) { seenV[v] = 1; vs.push_back(v); }
+      }
+      const int ed = m->bedge[b];
+      if(ed >= 0 && !seenE[ed]) { seenE[ed] = 1; es.push_back(ed); }
+    }
+  }
+  const int k = num->kinds[field], nvd = num->nvd[field], ned = num->ned[field];
+  const bool vec = k == FE_TRI_VP2;  // edge DOFs are components, not positions
+  const std::vector<int> &vn = num->vnum[field], &en = num->enm[field];
+  std::map<int, int> pairs;  // master -> slave, the first pair of a master kept
+  int nv = 0, ne = 0;
+  for(int a : v0)
+    for(int c : v1)
+      if(near(a, c)) {
+        ++nv;
+        for(int i = 0; i < nvd; ++i) pairs.insert({vn[(size_t)a * nvd + i], vn[(size_t)c * nvd + i]});
+        break;
+      }
+  for(int a : e0) {
+    // the edge's two vertices (its first vertex, as its DOFs run, first)
+    const int a0 = m->edgeA[a];
+    int a1 = -1;
+    for(int b = 0; b < m->nB && a1 < 0; ++b)
+      if(m->bedge[b] == a)
+        for(int j = 0; j < 2; ++j)
+          if(m->bconn[(size_t)b * m->bnv + j] != a0) a1 = m->bconn[(size_t)b * m->bnv + j];
+    for(int c : e1) {
+      const int c0 = m->edgeA[c];
+      int c1 = -1;
+      for(int b = 0; b < m->nB && c1 < 0; ++b)
+        if(m->bedge[b] == c)
+          for(int j = 0; j < 2; ++j)
+            if(m->bconn[(size_t)b * m->bnv + j] != c0) c1 = m->bconn[(size_t)b * m->bnv + j];
+      const bool same = near(a0, c0) && near(a1, c1), rev = near(a0, c1) && near(a1, c0);
+      if(!same && !rev) continue;
+      ++ne;
+      for(int i = 0; i < ned; ++i) {
+        const int j = (vec || same) ? i : ned - 1 - i;  // positions matched along the edge
+        pairs.insert({en[(size_t)a * ned + i], en[(size_t)c * ned + j]});
+      }
+      break;
+    }
+  }
+  if(nv == 0 && ne == 0) return FE_STATUS_ERROR;  // no matching vertex or edge (feNumber.cpp:299-306)
+  *count = (int)pairs.size();
+  if(master && slave) {
+    if((int)pairs.size() > cap) return FE_STATUS_ERROR;
+    int q = 0;
+    for(const auto &p : pairs) {
+      master[q] = p.first;
+      slave[q] = p.second;
+      ++q;
+    }
+  }
   return FE_STATUS_OK;
 }
 

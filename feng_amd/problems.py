@@ -110,13 +110,15 @@ def heat_system(mesh: A.Mesh, kind: int, degree: int, m: A.Fn, k: A.Fn, s: A.Fn,
     """Transient scalar m u_t - div(k grad u) + S = 0: the form stack of the
     reference's unsteady scalar tests (TransientMass, Diffusion, Source --
     scalarFEunsteady.cpp:270-300), on the GPU.  essential: Dirichlet boundary
-    (True) or none (False); periodic: (master, slave) DOF pairs entering the
+    (True), none (False) or the names of its boundary groups; periodic:
+    (master, slave) DOF pairs (Numbering.periodic_dofs) entering the
     pattern.  k = None leaves out the diffusion form: with no
     essential boundary that is the ODE m u_t + S = 0 of
     scalarFEunsteady.cpp:54-80.  react = (c, p): a MassPower form c u^p (with
     m u_t alone, the nonlinear ODE of scalarFEunsteady.cpp:134-165).  Returns (S, num, forms) with forms =
     {"mass", "diffusion", "source"} ids for fe_hip_update_coeff."""
-    num = A.Numbering(mesh, [kind], [1 if essential else 0])
+    ess = list(essential) if isinstance(essential, (list, tuple)) else (1 if essential else 0)
+    num = A.Numbering(mesh, [kind], [ess])
     S = A.LinearSystemHIP(device)
     S.upload_mesh(mesh, coloring)
     f = S.add_field(kind, num.adr[0])

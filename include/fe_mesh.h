@@ -125,6 +125,20 @@ int fe_mesh_boundary_groups(const fe_mesh *m, int *ngroups, char *names, int len
  * NULL); FE_STATUS_OK and *elem = -1 otherwise.  fe_numbering_create and
  * fe_hip_upload_mesh refuse such meshes the same way. */
 int fe_mesh_check_jacobians(const fe_mesh *m, int *elem, double *det);
+/* feNumber::applyPeriodicity (feNumber.cpp:217-307): the (master, slave)
+ * DOF pairs of field `field` between boundary groups masterGroup and
+ * slaveGroup (indices of fe_mesh_boundary_groups) of a triangle mesh: every
+ * master-group vertex whose translate by offset[3] is a slave-group vertex
+ * (distance < 1e-8) pairs its DOFs with that vertex's, every master-group edge
+ * whose two translated vertices are a slave-group edge's pairs its edge DOFs
+ * (components in order; for P3/P4 the positions along the edge that match
+ * after the translation).  Pairs sorted by master DOF, one per master (the
+ * reference's std::map).  *count = the number of pairs; master / slave
+ * (cap entries) filled when both are non-NULL (FE_STATUS_ERROR when cap is
+ * smaller).  FE_STATUS_ERROR when nothing matches, as the reference's error
+ * exit.  Feed the pairs to fe_hip_set_periodic_dofs. */
+int fe_numbering_periodic(const fe_mesh *m, const fe_numbering *num, int field, int masterGroup, int slaveGroup,
+                          const double *offset, int *master, int *slave, int cap, int *count);
 int fe_numbering_destroy(fe_numbering *num);
 /* nInc (unknowns, the system size) and nDofs (unknowns + essential). */
 int fe_numbering_info(const fe_numbering *num, int *nInc, int *nDofs);

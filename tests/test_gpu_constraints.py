@@ -89,3 +89,49 @@ def test_periodic_pattern_system_and_solution(oracle):
     assert r.converged
     assert np.abs(du[slave] - du[master]).max() <= 1e-10 * np.abs(du).max()
     S.close()
+
+
+def test_periodic_channel_with_pairs_from_the_numbering(oracle):
+    """Inlet -> Outlet periodicity on the reference's Poiseuille mesh (P2,
+    walls essential, an x-dependent source): the pairs of
+    fe_numbering_periodic (feNumber::applyPeriodicity) enter the device
+    pattern and system exactly as the oracle's restatement of the reference's
+    CSR and applyPeriodicity loops build them; the device direct solve ties
+    every slave to its master and equals the oracle's system solved on the
+    host."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    from feng_amd.api import FN_CONST, FN_SINSIN, KSP_DIRECT, Fn, Mesh, Numbering, TRI_P2
+    from feng_amd.problems import heat_system
+    from tests.conftest import GOLDEN
+    print("periodic channel", flush=True)
+    m = Mesh.read_msh(GOLDEN / "meshes" / "data" / "poiseuille0.msh")
+    master, slave = Numbering(m, [TRI_P2], [["NoSlip"]]).periodic_dofs(0, "Inlet", "Outlet", (5.0, 0.0))
+    S, num, forms = heat_system(m, TRI_P2, 4, Fn(FN_CONST, 1.0), Fn(FN_CONST, 1.0), Fn(FN_SINSIN, -3.0),
+                                essential=["NoSlip"], periodic=(master, slave))
+    n = num.nInc
+    pairs = list(zip(master, slave))
+    inner = [(a, b) for a, b in pairs if a < n and b < n]
+    assert len(inner) >= 3  # the walls' corner DOFs are essential, the others tie
+    ia0, ja0 = oracle.csr(n, [(num.adr[0], 6, num.adr[0], 6)])
+    ia, ja = oc.csr_add_periodic(ia0, ja0, n, pairs)
+    ia_d, ja_d = S.get_csr()
+    assert np.array_equal(ia_d, ia) and np.array_equal(ja_d, ja) and ja.size > ja0.size
+    S.set_bdf([10.0])
+    S.setToZero()
+    S.assemble(3)
+    S.applyPeriodicity()
+    val, rhs = S.get_system()
+    v0, r0 = oracle.assemble_diffusion(2, m.xyz, m.conn, num.adr[0], n, ia, ja, 4, (0, 1.0, 0.0, None),
+                                       (1, -3.0, 0.0, None), np.zeros(num.nDofs))
+    oracle.assemble_transient_mass(2, m.xyz, m.conn, num.adr[0], n, ia, ja, 4, (0, 1.0, 0.0, None), 10.0,
+                                   np.zeros(num.nDofs), v0, r0)
+    oc.apply_periodicity(ia, ja, v0, r0, pairs, n)
+    assert np.linalg.norm(val - v0) <= 1e-12 * np.linalg.norm(v0)
+    assert np.linalg.norm(rhs - r0) <= 1e-12 * np.linalg.norm(r0)
+    r, du = S.solve(ksp=KSP_DIRECT, rtol=1e-12, want_du=True)
+    ref = spla.spsolve(sp.csr_matrix((v0, ja, ia), shape=(n, n)).tocsc(), r0)
+    assert r.converged and np.abs(du - ref).max() <= 1e-10 * np.abs(ref).max()
+    ms, ss = np.array(inner).T
+    assert np.abs(du[ss] - du[ms]).max() <= 1e-12 * np.abs(du).max()
+    S.close()

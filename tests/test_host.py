@@ -163,3 +163,28 @@ def test_msh_reader_keeps_named_boundary_groups():
     n2 = Numbering(m, [TRI_VP2, TRI_P1], [["NoSlip", "Inlet", "Outlet"], []])
     assert n1.nInc == n2.nInc and all(np.array_equal(a, b) for a, b in zip(n1.adr, n2.adr))
     assert Mesh.square(4).boundary_groups()[0] == ["Bord"]
+
+
+@pytest.mark.parametrize("kind", ["P1", "P2", "P3", "VP2"])
+def test_numbering_periodic_pairs_between_boundary_groups(kind):
+    """fe_numbering_periodic (feNumber::applyPeriodicity, feNumber.cpp:217-307)
+    on the reference's Poiseuille mesh, Inlet (x = 0) -> Outlet (x = 5): the
+    4 matching vertices pair their DOFs (per component), the 3 matching edges
+    their edge DOFs; every slave DOF sits at its master's position + offset,
+    same component; masters sorted and unique; an offset that matches nothing
+    is an error, as the reference's exit."""
+    from feng_amd import api as A
+    from tests.conftest import GOLDEN
+    m = A.Mesh.read_msh(GOLDEN / "meshes" / "data" / "poiseuille0.msh")
+    k = {"P1": A.TRI_P1, "P2": A.TRI_P2, "P3": A.TRI_P3, "VP2": A.TRI_VP2}[kind]
+    num = A.Numbering(m, [k], [["NoSlip"]])
+    master, slave = num.periodic_dofs(0, "Inlet", "Outlet", (5.0, 0.0))
+    nv, ne = {"P1": (1, 0), "P2": (1, 1), "P3": (1, 2), "VP2": (2, 2)}[kind]
+    assert master.size == 4 * nv + 3 * ne
+    assert np.all(np.diff(master) > 0)
+    xyz = num.dof_xyz.reshape(-1, 3)
+    assert np.abs(xyz[slave] - xyz[master] - [5.0, 0.0, 0.0]).max() <= 1e-8
+    assert np.abs(xyz[master, 0]).max() <= 1e-12
+    assert np.array_equal(num.dof_comp[slave], num.dof_comp[master])
+    with pytest.raises(A.FeError):
+        num.periodic_dofs(0, "Inlet", "Outlet", (4.9, 0.0))
