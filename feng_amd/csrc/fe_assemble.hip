@@ -920,12 +920,12 @@ template <typename T, int NF, int DIM>
 __global__ __launch_bounds__(256) void asmDiffusionMfma(MfmaAsmArgs a)
 {
   constexpr int NV = DIM + 1, NE = NF * NF, NT = (NE + 15) / 16, NTERM = DIM == 2 ? 3 : 6;
-  __shared__ FeRefTables Tb;
+  __shared__ FeRefTablesFor<NF> Tb;
   __shared__ T aeLds[4][16][NE];
   {
     const double *src = a.tab;
     double *dst = (double *)&Tb;
-    for(int i = threadIdx.x; i < (int)(sizeof(FeRefTables) / sizeof(double)); i += blockDim.x) dst[i] = src[i];
+    for(int i = threadIdx.x; i < (int)(sizeof(Tb) / sizeof(double)); i += blockDim.x) dst[i] = src[i];
   }
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1282,7 +1282,9 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
     const bool allBefore = allFp32;
     if(matSweep) valClean = false;
     if(matSweep) allFp32 = false;  // re-set below when the MFMA pair writes both copies
-    if(iDiff >= 0 && !a.nlDiff && iMass < 0 && iReact < 0 && iAdv < 0 && ctx->asmKernel >= 1 && mfmaKind) {
+    const bool mfmaHigh = kind == FE_TRI_P3 || kind == FE_TRI_P4;  // cubic / quartic: colored MFMA sweep
+    if(iDiff >= 0 && !a.nlDiff && iMass < 0 && iReact < 0 && iAdv < 0 && ctx->asmKernel >= 1 &&
+       (mfmaKind || mfmaHigh)) {
       const FeForm &F = ctx->forms[iDiff];
       MfmaAsmArgs m{};
       m.E = a.E; m.n = a.n; m.what = what; m.K = F.mfmaK;
@@ -1293,6 +1295,8 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
       if(kind == FE_TRI_P1) launchMfma<double, 3, 2>(ctx, m);
       else if(kind == FE_TRI_P2) launchMfma<double, 6, 2>(ctx, m);
       else if(kind == FE_TET_P1) launchMfma<double, 4, 3>(ctx, m);
+      else if(kind == FE_TRI_P3) launchMfma<double, 10, 2>(ctx, m);
+      else launchMfma<double, 15, 2>(ctx, m);
       if(precision == FE_PREC_FP32_IR && (what & 1) && ctx->fp32Values == 1) {
         if(int rc = ensureVal32()) return rc;
         m.what = 1;
@@ -1301,6 +1305,8 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
         if(kind == FE_TRI_P1) launchMfma<float, 3, 2>(ctx, m);
         else if(kind == FE_TRI_P2) launchMfma<float, 6, 2>(ctx, m);
         else if(kind == FE_TET_P1) launchMfma<float, 4, 3>(ctx, m);
+        else if(kind == FE_TRI_P3) launchMfma<float, 10, 2>(ctx, m);
+        else launchMfma<float, 15, 2>(ctx, m);
         allFp32 = allBefore;  // this sweep wrote both copies
       }
     } else if(kind == FE_TRI_P1) launchScalar<3, 2>(ctx, a);

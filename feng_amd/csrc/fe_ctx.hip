@@ -524,7 +524,7 @@ int fe_hip_add_form(fe_hip_ctx *ctx, int sysType, int nfields, const int *fields
     }
     F.coeffs.push_back(fn);
   }
-  if(sysType == FE_SYS_DIFFUSION && F.tab.nf <= 6) {
+  if(sysType == FE_SYS_DIFFUSION && F.tab.nf <= 15) {
     // Reference tensors of the GEMM form Ae_ij = sum_{q,t} c_{q,t}(e) S_{q,t,ij}:
     // t runs over the unique metric terms (a <= b) of M_ab = sum_d dr_a/dx_d
     // dr_b/dx_d, S_{q,(a,b),ij} = w_q (d_a phi_i d_b phi_j + d_b phi_i d_a phi_j)

@@ -195,3 +195,54 @@ def test_reference_vector_interpolation_p2_row():
                                                         Fn(FN_QP, data=_qp_table(m, 12, lambda x, y: np.cos(x * y)))]))
         S.close()
     assert _matches(err, ref), err
+
+
+@pytest.mark.parametrize("kind", [5, 6])
+def test_cubic_and_quartic_diffusion_on_the_matrix_cores(oracle, kind):
+    """Diffusion + source on cubic / quartic triangles takes the MFMA sweep
+    (GEMM form, 7 / 15 tiles of 16 entries per element, asmDiffusionMfma):
+    values and RHS equal the oracle's element loop to 1e-12 and the scalar
+    sweep (asm_kernel 0) to 1e-13; its FP32 instance (element matrices
+    computed in FP32 on the matrix cores) is within 1e-6 of the FP64 values."""
+    from feng_amd.api import DIFFUSION, FN_ELEM, FN_QP, PREC_FP32_IR, SOURCE, Fn, LinearSystemHIP, Mesh, Numbering
+    from tests.test_oracle_time import _qp_table
+    print("high-order MFMA diffusion", kind, flush=True)
+    m = Mesh.square(9, 0.2, 3)
+    deg = 2 * (kind - 3)  # 4 (P3), 6 (P4): exact for the stiffness of a straight element
+    num = Numbering(m, [kind], [1])
+    n = num.nInc
+    k = m.coeff_blocks(3, 0.1, 10.0, 7)
+    tab = _qp_table(m, deg, lambda x, y: np.sin(3 * x) * np.cos(2 * y))
+    S = LinearSystemHIP(0)
+    S.upload_mesh(m)
+    f = S.add_field(kind, num.adr[0])
+    S.set_sizes(n, num.nDofs)
+    S.add_form(DIFFUSION, [f], deg, [Fn(FN_ELEM, data=k)])
+    S.add_form(SOURCE, [f], deg, [Fn(FN_QP, data=tab)])
+    S.build_csr()
+    u = np.random.default_rng(kind).normal(size=num.nDofs)
+    S.set_solution(u)
+    out = {}
+    for ak in (1, 0):
+        S.set_tuning("asm_kernel", ak)
+        S.setToZero()
+        S.assemble(3)
+        out[ak] = S.get_system()
+    ia, ja = S.get_csr()
+    v0, r0 = oracle.assemble_diffusion(kind, m.xyz, m.conn, num.adr[0], n, ia, ja, deg, (4, 0.0, 0.0, k),
+                                       (5, 0.0, 0.0, tab), u)
+    (vm, rm), (vs, rs) = out[1], out[0]
+    print("MFMA vs oracle", np.linalg.norm(vm - v0) / np.linalg.norm(v0), "vs scalar",
+          np.linalg.norm(vm - vs) / np.linalg.norm(vs), flush=True)
+    assert np.linalg.norm(vm - v0) <= 1e-12 * np.linalg.norm(v0)
+    assert np.linalg.norm(rm - r0) <= 1e-12 * np.linalg.norm(r0)
+    assert np.linalg.norm(vm - vs) <= 1e-13 * np.linalg.norm(vs)
+    assert np.linalg.norm(rm - rs) <= 1e-13 * np.linalg.norm(rs)
+    S.set_tuning("asm_kernel", 1)
+    S.setToZero()
+    S.assemble(3, PREC_FP32_IR)
+    v32 = S.get_system32()
+    err = np.abs(v32.astype(np.float64) - v0).max() / np.abs(v0).max()
+    print("FP32 MFMA values vs FP64 oracle", err, flush=True)
+    assert err <= 1e-6 and not np.array_equal(v32, vm.astype(np.float32))
+    S.close()
