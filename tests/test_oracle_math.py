@@ -107,3 +107,102 @@ def test_gmres_nonsymmetric_convergence(oracle):
     b = rng.random(n)
     x, it, rr = oracle.gmres_ilu0(ia, ja, val, lu, b, 10, 1e-10, 500)
     assert np.linalg.norm(A @ x - b) <= 1e-9 * np.linalg.norm(b)
+
+
+def _ilu0_textbook(A):
+    """ILU(0) by the IKJ elimination restricted to the pattern (Saad,
+    Iterative Methods for Sparse Linear Systems, alg. 10.4), dense storage."""
+    n = A.shape[0]
+    M = A.toarray().astype(float)
+    P = A.toarray() != 0
+    for i in range(1, n):
+        for k in range(i):
+            if not P[i, k]:
+                continue
+            M[i, k] /= M[k, k]
+            for j in range(k + 1, n):
+                if P[i, j]:
+                    M[i, j] -= M[i, k] * M[k, j]
+    return M
+
+
+def _fe_matrices():
+    from tests.test_oracle_mpnn import grid_matrix
+    rng = np.random.default_rng(5)
+    ia, ja, val = grid_matrix(9, 2)
+    A2 = sp.csr_matrix((val, ja, ia), shape=(len(ia) - 1,) * 2)
+    # a nonsymmetric matrix on the same pattern (an advection-like skew part)
+    S = A2.copy()
+    S.data = rng.normal(size=S.nnz) * 0.2 * np.abs(A2.data).max()
+    N2 = (A2 + (S - S.T)).tocsr()
+    from feng_amd.api import Mesh, Numbering, TET_P1
+    m = Mesh.cube(3)
+    num = Numbering(m, [TET_P1], [0])
+    rows, cols, vals = [], [], []
+    conn = m.conn.reshape(-1, 4)
+    for e in range(conn.shape[0]):
+        d = num.adr[0][4 * e:4 * e + 4]
+        K = rng.normal(size=(4, 4))
+        K = K @ K.T + 4 * np.eye(4)
+        for a in range(4):
+            for b in range(4):
+                rows.append(d[a]); cols.append(d[b]); vals.append(K[a, b])
+    A3 = sp.csr_matrix((vals, (rows, cols)), shape=(num.nDofs,) * 2)
+    out = []
+    for A in (A2, N2, A3):
+        A = A.tocsr()
+        A.sum_duplicates()
+        A.sort_indices()
+        out.append(A)
+    return out
+
+
+def test_ilu0_factor_reproduces_the_matrix_on_its_pattern(oracle):
+    """The defining property of ILU(0): (L U)_ij = a_ij on the pattern of A
+    (L unit lower, U upper, both on that pattern), with fill outside it, on a
+    2D FE matrix, a nonsymmetric one on the same pattern and a 3D tet matrix;
+    and the factor equals an independent textbook ILU(0) (IKJ elimination
+    restricted to the pattern)."""
+    for A in _fe_matrices():
+        n = A.shape[0]
+        ia, ja, val = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64)
+        lu, rc = oracle.ilu0(ia, ja, val)
+        assert rc == 0
+        LU = sp.csr_matrix((lu, ja, ia), shape=A.shape).toarray()
+        L = np.tril(LU, -1) + np.eye(n)
+        U = np.triu(LU)
+        prod = L @ U
+        P = A.toarray() != 0
+        scale = np.abs(A.toarray()).max()
+        assert np.abs(prod[P] - A.toarray()[P]).max() <= 1e-12 * scale
+        assert np.abs(prod[~P]).max() > 1e-6 * scale  # ILU(0) is not the exact LU here
+        ref = _ilu0_textbook(A)
+        assert np.abs(LU[P] - ref[P]).max() <= 1e-12 * np.abs(ref[P]).max()
+
+
+def test_fp64_residual_floor_of_the_high_contrast_p2_problem(oracle):
+    """Why the mixed-precision workload (P2, k log-uniform in [1e-2, 1e2] on
+    16 x 16 blocks) is solved to rel-res 1e-8 and not 1e-10: the FP64 solution
+    of a sparse direct LU, its residual evaluated in extended precision, has a
+    relative residual that grows about 4x per refinement (the floor scales as
+    || |A| |x| || / ||b||, ~h^-2): 3e-12, 1.4e-11, 7e-11 at 32^2, 64^2, 128^2
+    -- extrapolated, ~1e-8 at 1024^2."""
+    import scipy.sparse.linalg as spla
+    from feng_amd.api import Mesh, Numbering, TRI_P2
+    rel = []
+    for N in (32, 64, 128):
+        m = Mesh.square(N)
+        k = m.coeff_blocks(16, 1e-2, 1e2, 3)
+        num = Numbering(m, [TRI_P2], [1])
+        n = num.nInc
+        ia, ja = oracle.csr(n, [(num.adr[0], 6, num.adr[0], 6)])
+        val, rhs = oracle.assemble_diffusion(TRI_P2, m.xyz, m.conn, num.adr[0], n, ia, ja, 2, (4, 0.0, 0.0, k),
+                                             (0, -1.0, 0.0, None), np.zeros(num.nDofs))
+        A = sp.csr_matrix((val, ja, ia), shape=(n, n))
+        x = spla.spsolve(A.tocsc(), rhs)
+        Al = sp.csr_matrix((val.astype(np.longdouble), ja, ia), shape=(n, n))
+        r = rhs.astype(np.longdouble) - Al @ x.astype(np.longdouble)
+        rel.append(float(np.sqrt((r * r).sum()) / np.sqrt((rhs.astype(np.longdouble) ** 2).sum())))
+    print("FP64 direct-solve relative residuals at 32^2, 64^2, 128^2:", rel)
+    assert rel[1] > 3 * rel[0] and rel[2] > 3 * rel[1]
+    assert rel[2] > 3e-11
