@@ -1037,6 +1037,150 @@ __global__ __launch_bounds__(256) void asmDiffusionMfma(MfmaAsmArgs a)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Taylor-Hood Jacobian of the convection, Newtonian-stress and divergence
+// forms on the matrix cores (matrix-only sweeps).  With B_q the 7 x 15 basis
+// quantities at quadrature point q -- rows: the two velocity values, the four
+// velocity gradients d/dx_m of component n at 2 + 2 m + n, the pressure
+// value; columns: the 12 velocity and 3 pressure functions -- and D_q the
+// 7 x 7 coupling of the forms there (test quantity r, trial quantity s, the
+// quadrature weight and Jacobian folded in), the element matrix is
+// A = sum_q B_q^T D_q B_q: one v_mfma_f64_16x16x4f64 tile per element over
+// K = 7 nq, A operand L[(q,s)][i] = sum_r D_q[r][s] B_q[r][i] and B operand
+// B_q[s][j] formed per lane.  One wavefront per element; lanes < nq build the
+// D_q in LDS.
+constexpr int kNsMfmaQ = 16;  // quadrature points the LDS couplings hold
+
+__global__ __launch_bounds__(256) void asmNsMfma(NsAsmArgs a)
+{
+  __shared__ FeRefTables T;
+  __shared__ double Dq[4][kNsMfmaQ][7][7];
+  __shared__ double su[4][16], sx[4][6], sg[4][5];
+  {
+    const double *src = a.tab;
+    double *dst = (double *)&T;
+    for(int i = threadIdx.x; i < (int)(sizeof(FeRefTables) / sizeof(double)); i += blockDim.x) dst[i] = src[i];
+  }
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int t = blockIdx.x * 4 + w;
+  const bool valid = t < a.count;
+  const int ee = a.begin + (valid ? t : 0);
+  const int e = a.perm[ee];
+  if(valid) {
+    if(l < 12) su[w][l] = a.u[a.adrU[(size_t)ee * 12 + l]];
+    else if(l < 15) su[w][l] = a.u[a.adrP[(size_t)ee * 3 + l - 12]];
+    if(l < 6) sx[w][l] = a.xyz[3 * (size_t)a.conn[(size_t)e * 3 + l / 2] + (l & 1)];
+  }
+  __syncthreads();
+  if(valid && l == 0) {
+    const double *c = sx[w];
+    const double dxdr0 = -c[0] + c[2], dxdr1 = -c[1] + c[3];
+    const double dxds0 = -c[0] + c[4], dxds1 = -c[1] + c[5];
+    const double J = dxdr0 * dxds1 - dxdr1 * dxds0;
+    sg[w][0] = J;
+    sg[w][1] = dxds1 / J;
+    sg[w][2] = -dxdr1 / J;
+    sg[w][3] = -dxds0 / J;
+    sg[w][4] = dxdr0 / J;
+  }
+  __syncthreads();
+  const int nq = T.nq;
+  if(valid && l < nq) {
+    const int q = l;
+    const double J = sg[w][0], drdx0 = sg[w][1], drdx1 = sg[w][2], drdy0 = sg[w][3], drdy1 = sg[w][4];
+    double ux = 0., uy = 0., g[4] = {0., 0., 0., 0.};  // g[m * 2 + n] = d u_n / d x_m
+#pragma unroll
+    for(int b = 0; b < 6; ++b) {
+      const double gx = T.dLr[q][b] * drdx0 + T.dLs[q][b] * drdx1, gy = T.dLr[q][b] * drdy0 + T.dLs[q][b] * drdy1;
+      const double vx = su[w][2 * b], vy = su[w][2 * b + 1];
+      ux += vx * T.L[q][b];
+      uy += vy * T.L[q][b];
+      g[0] += vx * gx;
+      g[1] += vy * gx;
+      g[2] += vx * gy;
+      g[3] += vy * gy;
+    }
+    double x[3] = {0., 0., 0.};
+#pragma unroll
+    for(int j = 0; j < 3; ++j) {
+      x[0] += sx[w][2 * j] * T.Lg[q][j];
+      x[1] += sx[w][2 * j + 1] * T.Lg[q][j];
+    }
+    const double jw = J * T.w[q];
+    double D[7][7];
+#pragma unroll
+    for(int r = 0; r < 7; ++r)
+#pragma unroll
+      for(int c = 0; c < 7; ++c) D[r][c] = 0.;
+    if(a.hasConv) {  // phi_i . ((u . grad) phi_j + (phi_j . grad) u)
+      const double cc = feEvalFn(a.cConv, x, e, nq, q) * jw;
+#pragma unroll
+      for(int ci = 0; ci < 2; ++ci) {
+        D[ci][2 + ci] += cc * ux;
+        D[ci][4 + ci] += cc * uy;
+#pragma unroll
+        for(int cj = 0; cj < 2; ++cj) D[ci][cj] += cc * g[cj * 2 + ci];
+      }
+    }
+    {  // -mu grad phi_i : (grad phi_j + grad phi_j^T) and p div phi_i
+      const double cs = feEvalFn(a.cStr, x, e, nq, q), f = -cs * feEvalFn(a.mu, x, e, nq, q) * jw;
+#pragma unroll
+      for(int m = 0; m < 2; ++m)
+#pragma unroll
+        for(int n = 0; n < 2; ++n) {
+          D[2 + 2 * m + n][2 + 2 * m + n] += f;
+          D[2 + 2 * n + m][2 + 2 * m + n] += f;
+        }
+      D[2][6] += cs * jw;
+      D[5][6] += cs * jw;
+    }
+    {  // psi_k div phi_j
+      const double cd = feEvalFn(a.cDiv, x, e, nq, q) * jw;
+      D[6][2] += cd;
+      D[6][5] += cd;
+    }
+#pragma unroll
+    for(int r = 0; r < 7; ++r)
+#pragma unroll
+      for(int c = 0; c < 7; ++c) Dq[w][q][r][c] = D[r][c];
+  }
+  __syncthreads();
+  if(!valid) return;
+  const double drdx0 = sg[w][1], drdx1 = sg[w][2], drdy0 = sg[w][3], drdy1 = sg[w][4];
+  const int i = l & 15, kq = l >> 4;  // A row / B column i; k offset kq
+  const int ai = i >> 1, ci = i & 1;
+  feDouble4 acc = {0., 0., 0., 0.};
+  const int K = 7 * nq;
+  for(int k0 = 0; k0 < K; k0 += 4) {
+    const int kk = k0 + kq;
+    double av = 0., bv = 0.;
+    if(kk < K && i < 15) {
+      const int q = kk / 7, s = kk - 7 * q;
+      if(i < 12) {
+        const double ps = T.L[q][ai];
+        const double gx = T.dLr[q][ai] * drdx0 + T.dLs[q][ai] * drdx1;
+        const double gy = T.dLr[q][ai] * drdy0 + T.dLs[q][ai] * drdy1;
+        av = Dq[w][q][ci][s] * ps + Dq[w][q][2 + ci][s] * gx + Dq[w][q][4 + ci][s] * gy;
+        if(s < 2) bv = ci == s ? ps : 0.;
+        else if(s < 6) bv = ci == ((s - 2) & 1) ? ((s - 2) >> 1 == 0 ? gx : gy) : 0.;
+      } else {
+        const double pp = T.Lg[q][i - 12];
+        av = Dq[w][q][6][s] * pp;
+        bv = s == 6 ? pp : 0.;
+      }
+    }
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for(int v = 0; v < 4; ++v) {
+    const int row = kq + 4 * v, col = i;
+    if(row >= 15 || col >= 15 || (row >= 12 && col >= 12)) continue;
+    const int p = row < 12 ? a.posStr[(size_t)(row * 15 + col) * a.E + ee]
+                           : a.posDiv[(size_t)((row - 12) * 12 + col) * a.E + ee];
+    if(p >= 0) a.val[p] += acc[v];
+  }
+}
+
 template <typename T, int NF, int DIM>
 void launchMfma(fe_hip_ctx *ctx, MfmaAsmArgs a)
 {
@@ -1376,7 +1520,7 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
   // (one quadrature rule: the forms must share their degree).
   {
     NsAsmArgs a{};
-    int any = 0, degree = -1;
+    int any = 0, degree = -1, thNq = 0;
     const int *posConv = nullptr, *posMass = nullptr, *posVDiff = nullptr, *posVMass = nullptr;
     const int *adrPgrad = nullptr;
     for(size_t fi = 0; fi < ctx->forms.size(); ++fi) {
@@ -1430,7 +1574,10 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
       }
       FE_REQUIRE(ctx, degree < 0 || F.degree == degree, "the vector forms must share one quadrature degree");
       degree = F.degree;
-      if(!a.tab) a.tab = F.d_tab;
+      if(!a.tab) {
+        a.tab = F.d_tab;
+        thNq = F.tab.nq;
+      }
       done[fi] = true;
       any = 1;
     }
@@ -1461,12 +1608,19 @@ extern "C" int fe_hip_assemble(fe_hip_ctx *ctx, int what, int precision)
       a.val = ctx->d_val;
       a.rhs = ctx->d_rhs;
       if(what & 1) allFp32 = false;
+      // the MFMA form of the Jacobian: matrix-only sweeps of exactly the
+      // stress + divergence (+ convection) stack
+      const bool nsMfma = ctx->nsMfma && what == 1 && a.hasStr && a.hasDiv && a.adrP && !a.hasMass && !a.hasVDiff &&
+                          !a.hasGrad && !a.hasVMass && thNq <= kNsMfmaQ;
       for(size_t c = 0; c + 1 < ctx->colorStart.size(); ++c) {
         a.begin = ctx->colorStart[c];
         a.count = ctx->colorStart[c + 1] - a.begin;
         if(a.count == 0) continue;
-        hipLaunchKernelGGL(asmNavierStokesFP64, dim3((a.count + kNsElemPerBlock - 1) / kNsElemPerBlock), dim3(256), 0,
-                           ctx->stream, a);
+        if(nsMfma)
+          hipLaunchKernelGGL(asmNsMfma, dim3((a.count + 3) / 4), dim3(256), 0, ctx->stream, a);
+        else
+          hipLaunchKernelGGL(asmNavierStokesFP64, dim3((a.count + kNsElemPerBlock - 1) / kNsElemPerBlock), dim3(256),
+                             0, ctx->stream, a);
       }
     }
   }

@@ -707,6 +707,9 @@ int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value)
   } else if(k == "amg_kernel") {
     FE_REQUIRE(ctx, value == 1 || value == 2, "amg_kernel must be 1 (per-level kernels) or 2 (nested-ordered PCG)");
     ctx->amgKernel = value;
+  } else if(k == "ns_mfma") {
+    FE_REQUIRE(ctx, value == 0 || value == 1, "ns_mfma must be 0 (scalar sweep) or 1 (matrix cores)");
+    ctx->nsMfma = value;
   } else if(k == "amg_fuse") {
     FE_REQUIRE(ctx, value == 0 || value == 1, "amg_fuse must be 0 (one launch per level) or 1 (levels 1 .. coarsest in one launch)");
     ctx->amgFuse = value;
@@ -813,6 +816,7 @@ int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value)
   else if(k == "amg_kernel") *value = ctx->amgKernel;
   else if(k == "amg_wcycle") *value = ctx->amgWcycle;
   else if(k == "amg_fuse") *value = ctx->amgFuse;
+  else if(k == "ns_mfma") *value = ctx->nsMfma;
   else if(k == "amg_fuse_grid") *value = feAmgPcgFuseGrid(ctx);
   else if(k == "amg_wcycle_level") *value = feAmgPcgWLevel(ctx);
   else if(k == "amg_wcycle_dense") *value = feAmgPcgWDense(ctx);
