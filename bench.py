@@ -268,10 +268,11 @@ def bench_batched(args):
         b = build_batch(probs, kind)
         for meth, kw in methods.items():
             t0 = time.perf_counter()
-            solve_batch(b, rtol=1e-8, autotune=(meth == "jacobi"), **kw)  # warmup: autotune / hierarchy
+            warm = solve_batch(b, rtol=1e-8, autotune=True, **kw)  # warmup: autotuners / hierarchy
             setup_s = time.perf_counter() - t0
             runs = [solve_batch(b, rtol=1e-8, autotune=False, **kw) for _ in range(max(1, args.steps))]
             r = runs[-1]
+            r["tuned"] = warm["tuned"]
             if not r["converged"].all():
                 raise SystemExit(f"batched {name} {meth}: {int((~r['converged']).sum())} problems did not converge")
             ms = float(np.mean([x["asm_ms"] + x["solve_ms"] for x in runs]))

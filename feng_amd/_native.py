@@ -103,6 +103,7 @@ HIP_API = {
     "fe_hip_solve_batched": (C.c_int, [C.c_void_p, C.c_int, c_int_p, C.POINTER(FeHipSolverOpts),
                                        C.POINTER(FeHipBatchStats), c_double_p]),
     "fe_hip_autotune_batched": (C.c_int, [C.c_void_p, C.c_int, c_int_p, C.c_int, c_int_p, c_int_p, c_double_p]),
+    "fe_hip_autotune_assembly": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, c_int_p, c_double_p]),
     "fe_hip_update_coeff": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_double_p, C.c_int64]),
     "fe_hip_set_bdf": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "fe_hip_rotate_solutions": (C.c_int, [C.c_void_p]),

@@ -102,21 +102,29 @@ def build_batch(problems, kind, device=0, keep_meshes=False) -> Batch:
 def solve_batch(b: Batch, precision=A.PREC_FP32_IR, rtol=1e-8, inner_rtol=1e-2, max_iter=100000, autotune=True,
                 check=True, pc=A.PC_JACOBI, profile=False):
     """Assemble every problem (zero + all element kernels + scatter) and solve
-    them all; returns timings and per-problem statistics.  profile: events
-    around the dominant kernel's launches (kernel_ms, kernel_launches)."""
+    them all; returns timings and per-problem statistics.  autotune: the
+    assembly kernel variant (fe_hip_autotune_assembly) and, for the Jacobi
+    method, the iteration's launch shape are timed and the fastest kept.
+    profile: events around the dominant kernel's launches (kernel_ms,
+    kernel_launches)."""
     L, S = b.S.L, b.S
     rows = np.ascontiguousarray(b.rows, np.int32)
     nprob = len(b.nInc)
+    tuned = None
+    if autotune:
+        ak, sl, gx, ams = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        S._c(L.fe_hip_autotune_assembly(S.ctx, precision, C.byref(ak), C.byref(sl), C.byref(gx), C.byref(ams)),
+             "fe_hip_autotune_assembly")
+        tuned = dict(asm_kernel=ak.value, scalar_lanes=sl.value, gather_xcd=gx.value, asm_ms=ams.value)
     S.set_profiling(profile)
     S.setToZero()
     S.assemble(3, precision)
     asm_ms = S.kernel_stats(1)[0]
-    tuned = None
     if autotune and pc == A.PC_JACOBI:
         lanes, block, ms = C.c_int(), C.c_int(), C.c_double()
         S._c(L.fe_hip_autotune_batched(S.ctx, nprob, N.ptr(rows, C.c_int), precision, C.byref(lanes),
                                        C.byref(block), C.byref(ms)), "fe_hip_autotune_batched")
-        tuned = dict(lanes=lanes.value, block=block.value, ms_per_iter=ms.value)
+        tuned.update(lanes=lanes.value, block=block.value, ms_per_iter=ms.value)
     o = N.FeHipSolverOpts()
     L.fe_hip_default_opts(C.byref(o))
     o.precision, o.rtol, o.atol, o.inner_rtol, o.max_iter = precision, rtol, 1e-300, inner_rtol, max_iter

@@ -1658,3 +1658,48 @@ extern "C" int fe_hip_correct_solution(fe_hip_ctx *ctx)
   FE_HIP_TRY(ctx, hipGetLastError());  // stream-ordered: the next read of u waits for it
   return FE_STATUS_OK;
 }
+
+// Times one assembly (zero + sweeps) of the current forms for every kernel
+// variant that applies -- the row gather (two slice orders), the colored
+// sweep with one thread per element or one lane per local row, the colored
+// MFMA sweep (16 elements per wavefront) -- best of three each, and keeps
+// the fastest in the context's tuning (asm_kernel, scalar_lanes, gather_xcd).
+// The system holds the last variant's assembly afterwards.
+extern "C" int fe_hip_autotune_assembly(fe_hip_ctx *ctx, int precision, int *asm_kernel, int *scalar_lanes,
+                                        int *gather_xcd, double *best_ms)
+{
+  if(!ctx) return FE_STATUS_POINTER_ERROR;
+  FE_REQUIRE(ctx, ctx->csrBuilt, "build the CSR first");
+  struct Cand {
+    int kernel, lanes, xcd;
+  };
+  const Cand cands[] = {{2, 0, 1}, {2, 0, 0}, {1, 0, 1}, {0, 0, 1}, {0, 1, 1}};
+  const Cand keep{ctx->asmKernel, ctx->scalarLanes, ctx->gatherXcd};
+  Cand best = keep;
+  double bestMs = 1e300;
+  for(const Cand &c : cands) {
+    ctx->asmKernel = c.kernel;
+    ctx->scalarLanes = c.lanes;
+    ctx->gatherXcd = c.xcd;
+    double ms = 1e300;
+    for(int rep = 0; rep < 3; ++rep) {
+      if(int rc = fe_hip_set_to_zero(ctx, 1, 1)) return rc;
+      if(int rc = fe_hip_assemble(ctx, 3, precision)) return rc;
+      double m = 0.;
+      if(int rc = fe_hip_kernel_stats(ctx, 1, &m, nullptr)) return rc;
+      ms = std::min(ms, m);
+    }
+    if(ms < bestMs) {
+      bestMs = ms;
+      best = c;
+    }
+  }
+  ctx->asmKernel = best.kernel;
+  ctx->scalarLanes = best.lanes;
+  ctx->gatherXcd = best.xcd;
+  if(asm_kernel) *asm_kernel = best.kernel;
+  if(scalar_lanes) *scalar_lanes = best.lanes;
+  if(gather_xcd) *gather_xcd = best.xcd;
+  if(best_ms) *best_ms = bestMs;
+  return FE_STATUS_OK;
+}

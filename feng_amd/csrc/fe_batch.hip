@@ -38,6 +38,11 @@ struct FeBatch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graphKey[5] = {-1, -1, -1, -1, -1};  // TB, lanes, precision, maxit, active tiles of the captured graph
+  // the per-problem multigrid PCG: kBatchGraphIters iterations (V-cycle +
+  // update each) and the iteration-base advance captured as one graph
+  hipGraph_t ampGraph = nullptr;
+  hipGraphExec_t ampExec = nullptr;
+  int ampKey[4] = {-1, -1, -1, -1};  // active tiles, iteration budget, block, FP32 level values
   size_t cap = 0;  // allocated tiles
   std::vector<int> hPtile;  // host copy of the per-problem tile ranges
   int *act = nullptr;       // tiles of the problems still iterating (the iteration grid)
@@ -49,6 +54,8 @@ void feBatchRelease(fe_hip_ctx *ctx)
   if(!ctx->batch) return;
   if(ctx->batch->exec) (void)hipGraphExecDestroy(ctx->batch->exec);
   if(ctx->batch->graph) (void)hipGraphDestroy(ctx->batch->graph);
+  if(ctx->batch->ampExec) (void)hipGraphExecDestroy(ctx->batch->ampExec);
+  if(ctx->batch->ampGraph) (void)hipGraphDestroy(ctx->batch->ampGraph);
   delete ctx->batch;
   ctx->batch = nullptr;
 }
@@ -367,9 +374,11 @@ __global__ __launch_bounds__(TB) void bSpInit(const int *__restrict__ ia, const 
 template <int LPR, int TB>
 __global__ __launch_bounds__(TB) void bSpIter(const int *__restrict__ ia, const int *__restrict__ ja,
                                               const double *__restrict__ val, BSpVecs v, BTiles bt, BParts P,
-                                              BState S, const int *__restrict__ act, int it, int maxit)
+                                              BState S, const int *__restrict__ act,
+                                              const int *__restrict__ itBase, int k, int maxit)
 {
   __shared__ double ldsY[TB];
+  const int it = *itBase + k;  // the iteration: a device base (advanced by bAdvance) + the offset in the chunk
   const int tile = act[blockIdx.x], p = bt.prob[tile];
   if(S.done[p]) return;
   const int base = bt.row0[tile], nEnd = base + bt.rows[tile];
@@ -752,11 +761,41 @@ int solveBatchedAmgPerProblem(fe_hip_ctx *ctx, FeBatch &B, const fe_hip_solver_o
     int budget = 0;
     for(int p = 0; p < nprob; ++p)
       if(!outerDone[p]) budget = std::max(budget, o->max_iter - its[p]);
-    for(int it = 0; it <= budget;) {
-      for(int k = 0; k < 16; ++k, ++it) {
+    // one chunk: kBatchGraphIters iterations and the advance of the device
+    // iteration base; replayed from a captured graph unless profiling (the
+    // same launches either way)
+    auto chunk = [&]() -> int {
+      for(int k = 0; k < kBatchGraphIters; ++k) {
         if(int rc = feAmgApply(ctx, (const double *)v.w, v.m, nullptr, v32)) return rc;
         FE_SP_B(bSpIter, B.nAct, ctx->d_ia, ctx->d_ja, (const double *)ctx->d_val, v, bt, P, S, (const int *)B.act,
-                it, budget);
+                (const int *)B.itBase, k, budget);
+      }
+      hipLaunchKernelGGL(bAdvance, dim3(1), dim3(64), 0, ctx->stream, B.itBase, kBatchGraphIters);
+      return FE_STATUS_OK;
+    };
+    const bool useGraph = ctx->batchGraph && !ctx->profiling;
+    FE_HIP_TRY(ctx, hipMemsetAsync(B.itBase, 0, sizeof(int), ctx->stream));
+    for(int it = 0; it <= budget; it += kBatchGraphIters) {
+      if(useGraph) {
+        const int key[4] = {B.nAct, budget, TB, v32 ? 1 : 0};
+        if(!B.ampExec || !std::equal(key, key + 4, B.ampKey)) {
+          if(B.ampExec) (void)hipGraphExecDestroy(B.ampExec);
+          if(B.ampGraph) (void)hipGraphDestroy(B.ampGraph);
+          B.ampExec = nullptr;
+          B.ampGraph = nullptr;
+          FE_HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+          const int rc = chunk();
+          hipGraph_t g = nullptr;
+          const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+          if(rc) return rc;
+          FE_HIP_TRY(ctx, e);
+          B.ampGraph = g;
+          FE_HIP_TRY(ctx, hipGraphInstantiate(&B.ampExec, g, nullptr, nullptr, 0));
+          std::copy(key, key + 4, B.ampKey);
+        }
+        FE_HIP_TRY(ctx, hipGraphLaunch(B.ampExec, ctx->stream));
+      } else if(int rc = chunk()) {
+        return rc;
       }
       FE_HIP_TRY(ctx, hipGetLastError());
       FE_HIP_TRY(ctx, hipMemcpyAsync(hDone.data(), B.done, sizeof(int) * nprob, hipMemcpyDeviceToHost, ctx->stream));

@@ -278,6 +278,7 @@ struct fe_hip_ctx {
   FeAmgPcg *amgPcg = nullptr;
   int amgWcycle = 0;  // nested PCG with FP32 level values: W-cycle kernel for the coarse levels (1) or V-cycle (0)
   int amgFuse = 0;    // nested PCG: levels 1 .. coarsest in one launch with grid barriers (1) or one launch per level (0)
+  int batchGraph = 1;  // batched multigrid PCG: chunks of iterations replayed from a captured graph
   int nsMfma = 1;     // Taylor-Hood Jacobian sweeps (stress + divergence + convection) on the matrix cores
   int amgKernel = 2;  // FE_PC_AMG: 1 = per-level kernels over the original numbering,
                       // 2 = nested-ordered fused PCG (fe_amg_pcg.hip)

@@ -399,6 +399,16 @@ int fe_hip_solve_batched(fe_hip_ctx *ctx, int nprob, const int *rows, const fe_h
  * block 256/512/1024) and keeps the fastest in the context's tuning. */
 int fe_hip_autotune_batched(fe_hip_ctx *ctx, int nprob, const int *rows, int precision, int *lanes, int *block,
                             double *best_ms);
+/* Times one assembly of the current forms (zero + sweeps, best of three) for
+ * every kernel variant -- the row gather in two slice orders, the colored
+ * sweep with one thread per element or one lane per local row, the colored
+ * MFMA sweep with 16 elements per wavefront -- and keeps the fastest in the
+ * context's tuning (asm_kernel, scalar_lanes, gather_xcd); the variants that
+ * do not apply to a form fall back to the sweep that does.  The system holds
+ * an assembly afterwards.  No reference counterpart (the reference has one
+ * element loop). */
+int fe_hip_autotune_assembly(fe_hip_ctx *ctx, int precision, int *asm_kernel, int *scalar_lanes, int *gather_xcd,
+                             double *best_ms);
 
 /* ---- learned IC(0)-pattern correction (FE_PC_LEARNED) --------------------
  * No reference counterpart (the reference's PCG uses PETSc's IC(0),

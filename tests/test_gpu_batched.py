@@ -85,6 +85,69 @@ def test_batched_amg_matches_individual_systems(kind):
     b.S.close()
 
 
+@pytest.mark.parametrize("kind", [1, 2])
+def test_assembly_autotuner_keeps_a_variant_with_the_same_system(kind):
+    """fe_hip_autotune_assembly times the row gather (two slice orders), the
+    colored thread-per-element and lane-per-row sweeps and the colored MFMA
+    sweep, and keeps the fastest in the tuning; every variant assembles the
+    same system (values and RHS to 1e-13), so the choice changes time only."""
+    import ctypes as C
+    from feng_amd.api import PREC_FP64
+    from feng_amd.batched import build_batch
+    print("assembly autotune", kind, flush=True)
+    probs = [(20, kind, 41), (33, kind, 42), (27, kind, 43)]
+    b = build_batch(probs, kind)
+    S = b.S
+    systems = []
+    for ak, sl in ((2, 0), (1, 0), (0, 0), (0, 1)):
+        S.set_tuning("asm_kernel", ak)
+        S.set_tuning("scalar_lanes", sl)
+        S.setToZero()
+        S.assemble(3, PREC_FP64)
+        systems.append(S.get_system())
+    v0, r0 = systems[0]
+    for v, r in systems[1:]:
+        assert np.linalg.norm(v - v0) <= 1e-13 * np.linalg.norm(v0) and np.linalg.norm(r - r0) <= 1e-13 * np.linalg.norm(r0)
+    ak, sl, gx, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+    S._c(S.L.fe_hip_autotune_assembly(S.ctx, PREC_FP64, C.byref(ak), C.byref(sl), C.byref(gx), C.byref(ms)),
+         "fe_hip_autotune_assembly")
+    print("autotuned assembly: asm_kernel", ak.value, "scalar_lanes", sl.value, "gather_xcd", gx.value,
+          f"{ms.value:.3f} ms", flush=True)
+    assert (S.get_tuning("asm_kernel"), S.get_tuning("scalar_lanes"), S.get_tuning("gather_xcd")) == \
+        (ak.value, sl.value, gx.value)
+    assert 0 < ms.value < 1e3
+    S.setToZero()
+    S.assemble(3, PREC_FP64)
+    v, r = S.get_system()
+    assert np.linalg.norm(v - v0) <= 1e-13 * np.linalg.norm(v0)
+    S.close()
+
+
+def test_batched_amg_graph_replays_equal_direct_launches():
+    """The per-problem multigrid PCG replays its chunks of 16 iterations from a
+    captured graph (batch_graph 1, the default): solutions, iterations and
+    residuals equal direct launches (batch_graph 0) bit for bit, over many
+    replays, with eager host work between them and a recapture each time
+    problems drop out of the active tiles."""
+    from feng_amd.api import PC_AMG, PREC_FP64
+    from feng_amd.batched import build_batch, solve_batch
+    print("batched AMG graph", flush=True)
+    probs = [(9, 2, 31), (40, 2, 32), (17, 2, 33), (64, 2, 34), (25, 2, 35), (48, 2, 36)]
+    b = build_batch(probs, 2)
+    out = {}
+    for g in (1, 0, 1):
+        b.S.set_tuning("batch_graph", g)
+        b.S.set_solution(np.zeros(b.S.nDofs))
+        res = solve_batch(b, precision=PREC_FP64, rtol=1e-10, pc=PC_AMG)
+        assert res["converged"].all()
+        b.S.correctSolution()
+        out.setdefault(g, []).append((b.S.get_solution().copy(), res["iterations"].copy(), res["relres"].copy()))
+    assert len(set(out[0][0][1])) > 1  # problems stop at different iterations: the active set shrinks
+    for x, it, rr in out[1]:
+        assert np.array_equal(x, out[0][0][0]) and np.array_equal(it, out[0][0][1]) and np.array_equal(rr, out[0][0][2])
+    b.S.close()
+
+
 @pytest.mark.timeout(900)
 def test_batched_256_problem_draw_vs_oracle(oracle):
     """draw_problems(256, 1), both degrees: the FP32-assembly, FP32-inner +
