@@ -9,7 +9,11 @@ problem by problem: problem p's unknowns are the rows [rows[p], rows[p+1]) of
 a block-diagonal system (its reference numbering shifted), the essential DOFs
 of all problems follow.  One assembly then covers every problem, and
 fe_hip_solve_batched runs an independent Jacobi-PCG per problem in shared
-launches.
+launches.  build_union_batch puts the problems of BOTH degrees into one
+system (a P1 and a P2 field on the union mesh, each mapping the other
+degree's elements to one extra essential DOF), so that one assembly and one
+batched solve cover the whole draw; the batched multigrid PCG's V-cycle then
+skips the tiles of problems that have stopped.
 """
 from __future__ import annotations
 
@@ -93,6 +97,56 @@ def build_batch(problems, kind, device=0, keep_meshes=False) -> Batch:
     S.build_csr()
     S.set_solution(np.zeros(NU + int(eoff[-1])))
     b = Batch(kind, [N_ for N_, _ in probs], [s for _, s in probs], S, uoff.astype(np.int32), nInc, maps)
+    if keep_meshes:
+        b.meshes, b.coeffs = meshes, ks
+    b.setup_s = time.perf_counter() - t0
+    return b
+
+
+def build_union_batch(problems, device=0, keep_meshes=False) -> Batch:
+    """One device system holding every problem of the draw, P1 and P2
+    together: the disjoint union mesh carries a P1 and a P2 field; problem p's
+    unknowns are rows [rows[p], rows[p+1]) in draw order, numbered by its own
+    field.  On an element of the other degree a field's map points at one
+    extra essential DOF (value 0), so that element adds no row, column or
+    gather incidence of that field.  One assembly and one batched solve then
+    cover the whole draw."""
+    t0 = time.perf_counter()
+    meshes, nums, ks, kinds = [], [], [], []
+    for N_, kind, s in problems:
+        m = A.Mesh.square(N_)
+        meshes.append(m)
+        nums.append(A.Numbering(m, [kind], [1]))
+        ks.append(m.coeff_blocks(8, 0.1, 10.0, s))
+        kinds.append(kind)
+    nInc = [nm.nInc for nm in nums]
+    nEss = [nm.nDofs - nm.nInc for nm in nums]
+    NU = int(sum(nInc))
+    uoff = np.concatenate([[0], np.cumsum(nInc)]).astype(np.int64)
+    eoff = np.concatenate([[0], np.cumsum(nEss)]).astype(np.int64)
+    voff = np.concatenate([[0], np.cumsum([m.V for m in meshes])]).astype(np.int64)
+    dummy = NU + int(eoff[-1])  # the essential DOF of the other degree's elements
+    xyz = np.concatenate([m.xyz for m in meshes])
+    conn = np.concatenate([m.conn.astype(np.int64) + voff[p] for p, m in enumerate(meshes)]).astype(np.int32)
+    maps, adr = [], {A.TRI_P1: [], A.TRI_P2: []}
+    for p, nm in enumerate(nums):
+        loc = np.arange(nm.nDofs)
+        gm = np.where(loc < nm.nInc, uoff[p] + loc, NU + eoff[p] + (loc - nm.nInc))
+        maps.append(gm)
+        for kind, nl in ((A.TRI_P1, 3), (A.TRI_P2, 6)):
+            adr[kind].append(gm[nm.adr[0]] if kinds[p] == kind else np.full(meshes[p].E * nl, dummy))
+    k = np.concatenate(ks)
+    um = UnionMesh(2, int(voff[-1]), int(conn.size // 3), 3, xyz, conn)
+    S = A.LinearSystemHIP(device)
+    S.upload_mesh(um)
+    S.set_sizes(NU, dummy + 1)
+    for kind in (A.TRI_P1, A.TRI_P2):
+        f = S.add_field(kind, np.concatenate(adr[kind]).astype(np.int32))
+        S.add_form(A.DIFFUSION, [f], 2, [A.Fn(A.FN_ELEM, data=k)])
+        S.add_form(A.SOURCE, [f], 2, [A.Fn(A.FN_CONST, -1.0)])
+    S.build_csr()
+    S.set_solution(np.zeros(dummy + 1))
+    b = Batch(0, [N_ for N_, _, _ in problems], [s for _, _, s in problems], S, uoff.astype(np.int32), nInc, maps)
     if keep_meshes:
         b.meshes, b.coeffs = meshes, ks
     b.setup_s = time.perf_counter() - t0

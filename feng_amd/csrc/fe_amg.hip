@@ -78,6 +78,15 @@ __device__ __forceinline__ int amgTile(int xcd)
   return (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
 }
 
+// A tile whose problems [r.x, r.y] have all stopped (act: prefix sums of the
+// still-iterating flags) is skipped.
+__device__ __forceinline__ bool amgSkip(const int2 *__restrict__ tp, const int *__restrict__ act, int tile)
+{
+  if(!act) return false;
+  const int2 r = tp[tile];
+  return act[r.y + 1] == act[r.x];
+}
+
 template <typename T, typename TV>
 struct AmgLev {
   int n, nc;
@@ -108,6 +117,11 @@ struct FeAmgLevel {
   int *cptr = nullptr, *clist = nullptr;                  // Galerkin sums from level - 1
   int nTiles = 0;
   int *tileRow = nullptr, *tileAgg = nullptr;
+  // problems touched by each padded tile, aggregate tile and row block of
+  // the level kernels (block-diagonal systems of independent problems)
+  int2 *probPad = nullptr, *probTile = nullptr, *probUp = nullptr;
+  std::vector<int2> hPadAgg;   // first and last aggregate of each padded tile
+  std::vector<int> hTileAgg;   // aggregate range boundaries of the aggregate tiles
   double *b64 = nullptr, *u64 = nullptr, *x64 = nullptr, *r64 = nullptr;
   float *b32 = nullptr, *u32 = nullptr, *x32 = nullptr, *r32 = nullptr;
   int maxAgg = 0;
@@ -122,6 +136,8 @@ struct FeAmg {
   void *desc[3] = {nullptr, nullptr, nullptr};  // AmgLev<double,double>, <double,float>, <float,float>
   int *flag = nullptr;
   double c = 0., oc = 0.;
+  const int *act = nullptr;    // prefix sums of the still-iterating problems (tile skipping on)
+  std::vector<int> probRows;   // the problem rows the tile ranges were built for
 };
 
 namespace {
@@ -367,14 +383,15 @@ __global__ __launch_bounds__(1024) void amgDenseInverse(int n, const int *__rest
 template <typename T, typename TV, int LPR, int TB>
 __global__ __launch_bounds__(TB) void amgDownP(AmgLev<T, TV> L, const T *__restrict__ b, T *__restrict__ bNext,
                                                const int *__restrict__ prow, const int2 *__restrict__ prg,
-                                               const int *__restrict__ pagg, int nTiles, int xcd, const int *done)
+                                               const int *__restrict__ pagg, int nTiles, int xcd, const int *done,
+                                               const int2 *__restrict__ tp, const int *__restrict__ act)
 {
   __shared__ T ldsY[TB];
   __shared__ T rb[TB];
   __shared__ int ag[TB];
   if(done && *done) return;
   const int tile = amgTile(xcd);
-  if(tile >= nTiles) return;
+  if(tile >= nTiles || amgSkip(tp, act, tile)) return;
   constexpr int RPB = TB / LPR;
   const int sub = threadIdx.x % LPR, g = threadIdx.x / LPR;
   const int base = tile * TB;
@@ -413,7 +430,7 @@ __global__ __launch_bounds__(TB) void amgDownP(AmgLev<T, TV> L, const T *__restr
 template <typename T, typename TV, int LPR, int TB>
 __global__ __launch_bounds__(TB) void amgDown(AmgLev<T, TV> L, const T *__restrict__ b, T *__restrict__ bNext,
                                               const int *__restrict__ tileRow, const int *__restrict__ tileAgg,
-                                              const int *done)
+                                              const int *done, const int2 *__restrict__ tp, const int *__restrict__ act)
 {
   __shared__ T ldsY[TB];
   __shared__ T rb[kAmgTileMax];
@@ -421,6 +438,7 @@ __global__ __launch_bounds__(TB) void amgDown(AmgLev<T, TV> L, const T *__restri
   constexpr int RPB = TB / LPR;
   const int sub = threadIdx.x % LPR, g = threadIdx.x / LPR;
   const int tile = blockIdx.x;
+  if(amgSkip(tp, act, tile)) return;
   const int r0 = tileRow[tile], r1 = tileRow[tile + 1];
   for(int pass = r0; pass < r1; pass += TB) {
     int kb[LPR], ke[LPR];
@@ -458,13 +476,15 @@ __global__ __launch_bounds__(TB) void amgDown(AmgLev<T, TV> L, const T *__restri
 // y_i = u_i + oc e[agg_i], x_i = y_i + l1_i (b_i - sum_j a_ij y_j).
 template <typename T, typename TV, int LPR, int TB>
 __global__ __launch_bounds__(TB) void amgUp(AmgLev<T, TV> L, const T *__restrict__ b, const T *__restrict__ e, T oc,
-                                            T *__restrict__ x, int xcd, const int *done)
+                                            T *__restrict__ x, int xcd, const int *done, const int2 *__restrict__ tp,
+                                            const int *__restrict__ act)
 {
   __shared__ T ldsY[TB];
   if(done && *done) return;
   const int n = L.n;
   const int nTiles = (n + TB - 1) / TB;
   for(int tile = amgTile(xcd); tile < nTiles; tile += gridDim.x) {
+    if(amgSkip(tp, act, tile)) continue;  // uniform over the block
     const int base = tile * TB;
     int kb[LPR], ke[LPR];
     tileBounds<LPR, TB>(n, base, L.ia, kb, ke);
@@ -834,6 +854,13 @@ int feAmgSetup(fe_hip_ctx *ctx)
           prg.push_back(make_int2(0, 0));
         }
         d.padTiles = (int)(prow.size() / kAmgTB);
+        d.hPadAgg.assign(d.padTiles, make_int2(-1, -1));
+        for(size_t q = 0; q < pagg.size(); ++q)
+          if(pagg[q] >= 0) {
+            int2 &r = d.hPadAgg[q / kAmgTB];
+            if(r.x < 0) r.x = pagg[q];
+            r.y = pagg[q];
+          }
         if(int rc = upload(ctx, &d.prow, prow)) return rc;
         if(int rc = upload(ctx, &d.pagg, pagg)) return rc;
         if(int rc = upload(ctx, &d.prg, prg)) return rc;
@@ -846,6 +873,7 @@ int feAmgSetup(fe_hip_ctx *ctx)
       if(int rc = upload(ctx, &d.tileRow, h.tileRow)) return rc;
       if(int rc = upload(ctx, &d.tileAgg, h.tileAgg)) return rc;
       d.nTiles = (int)h.tileRow.size() - 1;
+      d.hTileAgg = h.tileAgg;
     }
     // level 0 reads the Krylov vectors for b and x
     if(l > 0) {
@@ -866,6 +894,81 @@ int feAmgSetup(fe_hip_ctx *ctx)
   if(int rc = amgUploadDesc<double, double>(ctx, *A, 0)) return rc;
   if(int rc = amgUploadDesc<double, float>(ctx, *A, 1)) return rc;
   if(int rc = amgUploadDesc<float, float>(ctx, *A, 2)) return rc;
+  return FE_STATUS_OK;
+}
+
+int feAmgProblemTiles(fe_hip_ctx *ctx, int nprob, const int *rows, const int *act)
+{
+  if(!act) {
+    if(ctx->amg) ctx->amg->act = nullptr;
+    return FE_STATUS_OK;
+  }
+  FE_REQUIRE(ctx, ctx->amg, "AMG problem tiles: set up the hierarchy first");
+  FeAmg &A = *ctx->amg;
+  A.act = nullptr;
+  FE_REQUIRE(ctx, nprob >= 1 && rows, "AMG problem tiles: no problems");
+  const std::vector<int> key(rows, rows + nprob + 1);
+  if(key != A.probRows) {
+    FE_REQUIRE(ctx, key[0] == 0 && key[nprob] == A.lv[0].n && std::is_sorted(key.begin(), key.end()),
+               "AMG problem tiles: the problem rows do not partition the system");
+    FE_REQUIRE(ctx, ctx->amgHost.size() == A.lv.size(), "AMG problem tiles: no host hierarchy");
+    // problem of row r of a level with problem offsets o (empty problems own no row)
+    auto probOf = [nprob](const std::vector<int> &o, int r) {
+      return std::min(nprob - 1, (int)(std::upper_bound(o.begin(), o.end(), r) - o.begin()) - 1);
+    };
+    auto put = [ctx](int2 **d, const std::vector<int2> &h) -> int {
+      if(h.empty()) return FE_STATUS_OK;
+      if(!*d)
+        if(int rc = feAlloc(ctx, d, h.size())) return rc;
+      FE_HIP_TRY(ctx, hipMemcpy(*d, h.data(), sizeof(int2) * h.size(), hipMemcpyHostToDevice));
+      return FE_STATUS_OK;
+    };
+    std::vector<int> off = key;
+    for(size_t l = 0; l < A.lv.size(); ++l) {
+      FeAmgLevel &d = A.lv[l];
+      std::vector<int> offN;
+      if(l + 1 < A.lv.size()) {
+        // roots are numbered in row order and aggregates never join rows of
+        // two problems (the system is block-diagonal): problem p's aggregates
+        // are the range [offN[p], offN[p + 1]) of the next level
+        const std::vector<int> &agg = ctx->amgHost[l].agg;
+        offN.assign(nprob + 1, d.nc);
+        for(int p = 0; p < nprob; ++p)
+          for(int i = off[p]; i < off[p + 1]; ++i) offN[p] = std::min(offN[p], agg[i]);
+        for(int p = nprob - 1; p >= 0; --p) offN[p] = std::min(offN[p], offN[p + 1]);
+        for(int p = 0; p < nprob; ++p)
+          for(int i = off[p]; i < off[p + 1]; ++i)
+            FE_REQUIRE(ctx, agg[i] >= offN[p] && agg[i] < offN[p + 1],
+                       "AMG problem tiles: an aggregate joins rows of two problems");
+        std::vector<int2> pad(d.padTiles), til(d.nTiles);
+        for(int t = 0; t < d.padTiles; ++t) {
+          const int2 r = d.hPadAgg[t];
+          pad[t] = r.x < 0 ? make_int2(0, nprob - 1) : make_int2(probOf(offN, r.x), probOf(offN, r.y));
+        }
+        for(int t = 0; t < d.nTiles; ++t)
+          til[t] = make_int2(probOf(offN, d.hTileAgg[t]), probOf(offN, std::max(d.hTileAgg[t], d.hTileAgg[t + 1] - 1)));
+        if(int rc = put(&d.probPad, pad)) return rc;
+        if(int rc = put(&d.probTile, til)) return rc;
+      }
+      const int nUp = (d.n + kAmgTB - 1) / kAmgTB;
+      std::vector<int2> up(nUp);
+      for(int t = 0; t < nUp; ++t)
+        up[t] = make_int2(probOf(off, t * kAmgTB), probOf(off, std::min(d.n, (t + 1) * kAmgTB) - 1));
+      if(int rc = put(&d.probUp, up)) return rc;
+      off.swap(offN);
+    }
+    // rows of stopped problems keep their last values; start them finite
+    // (the coarsest dense inverse multiplies them by exact zeros)
+    for(size_t l = 0; l < A.lv.size(); ++l) {
+      FeAmgLevel &d = A.lv[l];
+      for(double *q : {d.b64, d.x64, d.u64, d.r64})
+        if(q) FE_HIP_TRY(ctx, hipMemsetAsync(q, 0, sizeof(double) * d.n, ctx->stream));
+      for(float *q : {d.b32, d.x32, d.u32, d.r32})
+        if(q) FE_HIP_TRY(ctx, hipMemsetAsync(q, 0, sizeof(float) * d.n, ctx->stream));
+    }
+    A.probRows = key;
+  }
+  A.act = act;
   return FE_STATUS_OK;
 }
 
@@ -1001,6 +1104,7 @@ int amgApplyT(fe_hip_ctx *ctx, const T *in, T *out, const int *done, int slot)
   const AmgBlockPlan &pl = A.plan[slot];
   const int B = pl.from;
   const T oc = (T)A.oc;
+  const int *act = A.act;
   for(int l = 0; l < B; ++l) {
     const FeAmgLevel &h = A.lv[l];
     AmgLev<T, TV> L = amgView<T, TV>(h);
@@ -1008,18 +1112,19 @@ int amgApplyT(fe_hip_ctx *ctx, const T *in, T *out, const int *done, int slot)
     T *bn = pick<T>(A.lv[l + 1].b64, A.lv[l + 1].b32);
     const dim3 gp(ctx->amgXcd ? (h.padTiles + 7) & ~7 : h.padTiles);
     const bool pad = h.padTiles > 0 && ctx->amgPadTiles;
+    const int2 *tp = pad ? h.probPad : h.probTile;
     if(pad && ctx->amgLanes == 2)
       hipLaunchKernelGGL((amgDownP<T, TV, 2, kAmgTB>), gp, dim3(kAmgTB), 0, ctx->stream, L, b, bn, h.prow, h.prg,
-                         h.pagg, h.padTiles, ctx->amgXcd, done);
+                         h.pagg, h.padTiles, ctx->amgXcd, done, tp, act);
     else if(pad && ctx->amgLanes == 8)
       hipLaunchKernelGGL((amgDownP<T, TV, 8, kAmgTB>), gp, dim3(kAmgTB), 0, ctx->stream, L, b, bn, h.prow, h.prg,
-                         h.pagg, h.padTiles, ctx->amgXcd, done);
+                         h.pagg, h.padTiles, ctx->amgXcd, done, tp, act);
     else if(pad)
       hipLaunchKernelGGL((amgDownP<T, TV, kAmgLPR, kAmgTB>), gp, dim3(kAmgTB), 0, ctx->stream, L, b, bn, h.prow,
-                         h.prg, h.pagg, h.padTiles, ctx->amgXcd, done);
+                         h.prg, h.pagg, h.padTiles, ctx->amgXcd, done, tp, act);
     else
       hipLaunchKernelGGL((amgDown<T, TV, kAmgLPR, kAmgTB>), dim3(h.nTiles), dim3(kAmgTB), 0, ctx->stream, L, b, bn,
-                         h.tileRow, h.tileAgg, done);
+                         h.tileRow, h.tileAgg, done, tp, act);
   }
   hipLaunchKernelGGL((amgBlock<T, TV>), dim3(1), dim3(kAmgBlockTB), 0, ctx->stream,
                      (const AmgLev<T, TV> *)A.desc[slot], pl, (const T *)pick<T>(A.cinv64, A.cinv32), oc, in, out,
@@ -1038,13 +1143,13 @@ int amgApplyT(fe_hip_ctx *ctx, const T *in, T *out, const int *done, int slot)
       if(int rc = feTimerMark(ctx, ctx->spmvTimer)) return rc;
     if(ctx->amgLanes == 2)
       hipLaunchKernelGGL((amgUp<T, TV, 2, kAmgTB>), dim3(G), dim3(kAmgTB), 0, ctx->stream, L, b, e, oc, x, ctx->amgXcd,
-                         done);
+                         done, (const int2 *)h.probUp, act);
     else if(ctx->amgLanes == 8)
       hipLaunchKernelGGL((amgUp<T, TV, 8, kAmgTB>), dim3(G), dim3(kAmgTB), 0, ctx->stream, L, b, e, oc, x, ctx->amgXcd,
-                         done);
+                         done, (const int2 *)h.probUp, act);
     else
       hipLaunchKernelGGL((amgUp<T, TV, kAmgLPR, kAmgTB>), dim3(G), dim3(kAmgTB), 0, ctx->stream, L, b, e, oc, x,
-                         ctx->amgXcd, done);
+                         ctx->amgXcd, done, (const int2 *)h.probUp, act);
     if(l == 0 && ctx->profiling)
       if(int rc = feTimerMark(ctx, ctx->spmvTimer)) return rc;
   }

@@ -47,6 +47,8 @@ struct FeBatch {
   std::vector<int> hPtile;  // host copy of the per-problem tile ranges
   int *act = nullptr;       // tiles of the problems still iterating (the iteration grid)
   int nAct = 0;
+  int *actPrefix = nullptr;     // nprob + 1 prefix sums of the still-iterating flags
+  std::vector<int> hActPrefix;  // its host copy (the source of the upload)
 };
 
 void feBatchRelease(fe_hip_ctx *ctx)
@@ -511,7 +513,8 @@ int setupBatch(fe_hip_ctx *ctx, int nprob, const int *rows, FeBatch *&out)
   if(feAlloc(ctx, &B.pTile, nprob + 1) || feAlloc(ctx, &B.done, nprob) || feAlloc(ctx, &B.iter, nprob) ||
      feAlloc(ctx, &B.outerDone, nprob) || feAlloc(ctx, &B.itBase, 1) || feAlloc(ctx, &B.gamma, 2 * nprob) ||
      feAlloc(ctx, &B.alpha, 2 * nprob) || feAlloc(ctx, &B.tol2, nprob) || feAlloc(ctx, &B.rr64, nprob) ||
-     feAlloc(ctx, &B.x64, n) || feAlloc(ctx, &B.r64, n) || feAlloc(ctx, &B.bak64, n))
+     feAlloc(ctx, &B.x64, n) || feAlloc(ctx, &B.r64, n) || feAlloc(ctx, &B.bak64, n) ||
+     feAlloc(ctx, &B.actPrefix, nprob + 1))
     return FE_STATUS_OUT_OF_MEMORY;
   double **a[] = {&B.v64.x, &B.v64.r, &B.v64.w, &B.v64.s, &B.v64.z, &B.v64.p, &B.v64.dinv, &B.v64.m[0], &B.v64.m[1]};
   for(auto pp : a)
@@ -605,6 +608,10 @@ int setActive(fe_hip_ctx *ctx, FeBatch &B, const std::vector<int> &done)
   B.nAct = std::max(1, (int)act.size());
   if(act.empty()) act.push_back(0);  // a stopped problem's tile returns at once
   FE_HIP_TRY(ctx, hipMemcpyAsync(B.act, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, ctx->stream));
+  B.hActPrefix.assign(B.nprob + 1, 0);
+  for(int p = 0; p < B.nprob; ++p) B.hActPrefix[p + 1] = B.hActPrefix[p] + (done[p] ? 0 : 1);
+  FE_HIP_TRY(ctx, hipMemcpyAsync(B.actPrefix, B.hActPrefix.data(), sizeof(int) * (B.nprob + 1), hipMemcpyHostToDevice,
+                                 ctx->stream));
   return FE_STATUS_OK;
 }
 
@@ -720,6 +727,12 @@ int solveBatchedAmgPerProblem(fe_hip_ctx *ctx, FeBatch &B, const fe_hip_solver_o
     if(!q && feAlloc(ctx, &q, n)) return FE_STATUS_OUT_OF_MEMORY;
   BSpVecs v{B.sp[0], B.sp[1], B.sp[2], B.sp[3], B.sp[4], B.sp[5], B.sp[6], B.sp[7], B.sp[8]};
   if(int rc = feAmgSetup(ctx)) return rc;
+  // the V-cycle skips the tiles of stopped problems while this solve runs
+  if(int rc = feAmgProblemTiles(ctx, nprob, B.rows.data(), ctx->batchSkip ? B.actPrefix : nullptr)) return rc;
+  struct Detach {
+    fe_hip_ctx *c;
+    ~Detach() { (void)feAmgProblemTiles(c, 0, nullptr, nullptr); }
+  } detach{ctx};
   hipEvent_t t0, t1;
   FE_HIP_TRY(ctx, hipEventCreate(&t0));
   FE_HIP_TRY(ctx, hipEventCreate(&t1));
@@ -748,6 +761,7 @@ int solveBatchedAmgPerProblem(fe_hip_ctx *ctx, FeBatch &B, const fe_hip_solver_o
     if(!any) break;
     FE_HIP_TRY(ctx, hipMemcpyAsync(B.outerDone, outerDone.data(), sizeof(int) * nprob, hipMemcpyHostToDevice, ctx->stream));
     FE_HIP_TRY(ctx, hipMemcpyAsync(B.tol2, tol2.data(), sizeof(double) * nprob, hipMemcpyHostToDevice, ctx->stream));
+    if(int rc = setActive(ctx, B, outerDone)) return rc;  // before the first V-cycle: it reads the flags
     hipLaunchKernelGGL(bSpStart, dim3(B.nTiles), dim3(TB), 0, ctx->stream, B.r64, v, bt, S);
     if(int rc = feAmgApply(ctx, (const double *)v.r, v.u, nullptr, v32)) return rc;
 #define FE_SP_B(KER, GRID, ...)                                                                                     \
@@ -757,7 +771,6 @@ int solveBatchedAmgPerProblem(fe_hip_ctx *ctx, FeBatch &B, const fe_hip_solver_o
     else hipLaunchKernelGGL((KER<4, 1024>), dim3(GRID), dim3(1024), 0, ctx->stream, __VA_ARGS__);                \
   } while(0)
     FE_SP_B(bSpInit, B.nTiles, ctx->d_ia, ctx->d_ja, (const double *)ctx->d_val, v, bt, P, S);
-    if(int rc = setActive(ctx, B, outerDone)) return rc;
     int budget = 0;
     for(int p = 0; p < nprob; ++p)
       if(!outerDone[p]) budget = std::max(budget, o->max_iter - its[p]);

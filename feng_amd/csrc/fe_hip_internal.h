@@ -279,6 +279,7 @@ struct fe_hip_ctx {
   int amgWcycle = 0;  // nested PCG with FP32 level values: W-cycle kernel for the coarse levels (1) or V-cycle (0)
   int amgFuse = 0;    // nested PCG: levels 1 .. coarsest in one launch with grid barriers (1) or one launch per level (0)
   int batchGraph = 1;  // batched multigrid PCG: chunks of iterations replayed from a captured graph
+  int batchSkip = 1;   // batched multigrid PCG: the V-cycle skips the tiles of stopped problems
   int nsMfma = 1;     // Taylor-Hood Jacobian sweeps (stress + divergence + convection) on the matrix cores
   int amgKernel = 2;  // FE_PC_AMG: 1 = per-level kernels over the original numbering,
                       // 2 = nested-ordered fused PCG (fe_amg_pcg.hip)
@@ -369,6 +370,12 @@ int feAmgCheck(fe_hip_ctx *ctx);
 int feAmgApply(fe_hip_ctx *ctx, const double *in, double *out, const int *done, bool values32);
 int feAmgApply(fe_hip_ctx *ctx, const float *in, float *out, const int *done, bool values32);
 int feAmgLevels(const fe_hip_ctx *ctx, int *levels, int *rows, int64_t *nnz, int cap);
+// Block-diagonal systems of independent problems (problem p: rows
+// [rows[p], rows[p + 1]) of level 0): each tile of the level kernels gets the
+// range of problems it touches, and with act (device, nprob + 1 prefix sums
+// of the still-iterating flags) the V-cycle's level launches skip the tiles
+// whose problems have all stopped.  act = nullptr detaches (every tile runs).
+int feAmgProblemTiles(fe_hip_ctx *ctx, int nprob, const int *rows, const int *act);
 // First level of the single-workgroup kernel for a (vector, value) type slot
 // (0 f64/f64, 1 f64/f32, 2 f32/f32): part of the launch sequence a graph captures.
 int feAmgPlanFrom(const fe_hip_ctx *ctx, int slot);

@@ -1,5 +1,6 @@
 """Batched independent problems (fe_batch.hip): one disjoint-mesh system per
-element degree, per-problem Jacobi-PCG in shared launches.  Each problem's
+element degree or one for both degrees, per-problem Jacobi- or multigrid-PCG
+in shared launches.  Each problem's
 block of the union system must equal the problem assembled alone (CSR
 bit-exact, values to 1e-12), and each problem's solution must match its own
 single-system solve.  On the full 256-problem draw of the batched workload,
@@ -148,18 +149,119 @@ def test_batched_amg_graph_replays_equal_direct_launches():
     b.S.close()
 
 
-@pytest.mark.timeout(900)
-def test_batched_256_problem_draw_vs_oracle(oracle):
-    """draw_problems(256, 1), both degrees: the FP32-assembly, FP32-inner +
-    FP64-refined batched Jacobi-PCG (the batched configuration) brings every
-    problem to FP64 rel-res <= 1e-8; every problem's CSR block is the
-    oracle's pattern, its values and RHS the oracle's assembly to 1e-12, and
-    its solution has rel-res <= 1.01e-8 on the oracle's system; P1 solutions
-    are non-negative (the split-square P1 stiffness is an M-matrix, so the
+def test_batched_amg_tile_skipping_equals_every_tile():
+    """The V-cycle's level launches skip the tiles whose problems have all
+    stopped (batch_skip 1, the default): each row only couples to rows of its
+    own problem on every level, so solutions, iterations and residuals equal
+    the launches over every tile (batch_skip 0) bit for bit, on one degree
+    and on the union system of both degrees."""
+    from feng_amd.api import PC_AMG, PREC_FP64
+    from feng_amd.batched import build_batch, build_union_batch, solve_batch
+    print("batched AMG tile skipping", flush=True)
+    probs = [(9, 2, 41), (40, 1, 42), (17, 2, 43), (64, 2, 44), (25, 1, 45), (48, 2, 46), (33, 1, 47)]
+    for b in (build_batch(probs, 2), build_union_batch(probs)):
+        out = {}
+        for skip in (1, 0):
+            b.S.set_tuning("batch_skip", skip)
+            b.S.set_solution(np.zeros(b.S.nDofs))
+            res = solve_batch(b, precision=PREC_FP64, rtol=1e-10, pc=PC_AMG)
+            assert res["converged"].all()
+            b.S.correctSolution()
+            out[skip] = (b.S.get_solution().copy(), res["iterations"].copy(), res["relres"].copy())
+        assert len(set(out[0][1])) > 1  # problems stop at different iterations
+        for a_, b_ in zip(out[1], out[0]):
+            assert np.array_equal(a_, b_)
+        b.S.close()
+
+
+@pytest.mark.parametrize("method", ["jacobi", "amg"])
+def test_union_batch_of_both_degrees_matches_problems_alone(method):
+    """One system for P1 and P2 problems together (build_union_batch): every
+    problem's block equals the problem assembled alone (CSR bit-exact, values
+    and RHS to 1e-12) and its solution, by the batched Jacobi- or
+    multigrid-PCG, its single-system solve to 1e-6."""
+    from feng_amd.api import PC_AMG, PC_JACOBI, PREC_FP64
+    from feng_amd.batched import build_union_batch, solve_batch
+    print(f"union batch {method}", flush=True)
+    pc = PC_AMG if method == "amg" else PC_JACOBI
+    probs = [(9, 1, 51), (17, 2, 52), (12, 2, 53), (24, 1, 54), (31, 2, 55), (20, 1, 56)]
+    b = build_union_batch(probs)
+    res = solve_batch(b, precision=PREC_FP64, rtol=1e-10, pc=pc)
+    assert res["converged"].all() and (res["relres"] <= 1e-10).all(), res["relres"]
+    ia, ja = b.S.get_csr()
+    val, rhs = b.S.get_system()
+    b.S.correctSolution()
+    u = b.S.get_solution()
+    for p, (N_, kind, seed) in enumerate(probs):
+        S1, num = _alone(N_, kind, seed)
+        S1.setToZero()
+        S1.assemble(3, PREC_FP64)
+        ia1, ja1 = S1.get_csr()
+        v1, r1 = S1.get_system()
+        r0, r1_ = b.rows[p], b.rows[p + 1]
+        assert r1_ - r0 == num.nInc
+        blk = slice(ia[r0], ia[r1_])
+        assert np.array_equal(ia[r0:r1_ + 1] - ia[r0], ia1)
+        assert np.array_equal(ja[blk] - r0, ja1)
+        assert relerr(val[blk], v1) <= 1e-12 and relerr(rhs[r0:r1_], r1) <= 1e-12
+        S1.solve(rtol=1e-10)
+        S1.correctSolution()
+        u1 = S1.get_solution()[:num.nInc]
+        assert relerr(u[b.dof_maps[p][:num.nInc]], u1) <= 1e-6
+        S1.close()
+    b.S.close()
+
+
+def _check_blocks_vs_oracle(oracle, b, kinds, res, label):
+    """Every problem of batch b (element kind kinds[p]): CSR block = the
+    oracle's pattern, values and RHS = the oracle's assembly to 1e-12, the
+    solution's rel-res on the oracle's system <= 1.01e-8, P1 solutions
+    non-negative (the split-square P1 stiffness is an M-matrix, so the
     discrete maximum principle holds for a non-negative load)."""
     from concurrent.futures import ThreadPoolExecutor
 
-    from feng_amd.api import PREC_FP32_IR, Numbering
+    from feng_amd.api import Numbering
+    assert res["converged"].all() and (res["relres"] <= 1e-8).all(), res["relres"].max()
+    ia, ja = b.S.get_csr()
+    val, rhs = b.S.get_system()
+    b.S.correctSolution()
+    u = b.S.get_solution()
+
+    def check(p):
+        m, k, kind = b.meshes[p], b.coeffs[p], kinds[p]
+        num = Numbering(m, [kind], [1])
+        nl = num.adr[0].size // m.E
+        ia0, ja0 = oracle.csr(num.nInc, [(num.adr[0], nl, num.adr[0], nl)])
+        val0, rhs0 = oracle.assemble_diffusion(kind, m.xyz, m.conn, num.adr[0], num.nInc, ia0, ja0, 2,
+                                               (4, 0.0, 0.0, k), (0, -1.0, 0.0, None), np.zeros(num.nDofs))
+        r0, r1 = int(b.rows[p]), int(b.rows[p + 1])
+        blk = slice(ia[r0], ia[r1])
+        ok_csr = np.array_equal(ia[r0:r1 + 1] - ia[r0], ia0) and np.array_equal(ja[blk] - r0, ja0)
+        e_val, e_rhs = relerr(val[blk], val0), relerr(rhs[r0:r1], rhs0)
+        up = u[b.dof_maps[p][:num.nInc]]
+        rr = np.linalg.norm(rhs0 - oracle.spmv(ia0, ja0, val0, up)) / np.linalg.norm(rhs0)
+        return ok_csr, e_val, e_rhs, rr, up.min()
+
+    with ThreadPoolExecutor(8) as ex:
+        out = list(ex.map(check, range(len(b.nInc))))
+    for p, (ok_csr, e_val, e_rhs, rr, umin) in enumerate(out):
+        assert ok_csr, p
+        assert e_val <= 1e-12 and e_rhs <= 1e-12, (p, e_val, e_rhs)
+        assert rr <= 1.01e-8, (p, rr)
+        if kinds[p] == 1:
+            assert umin >= -1e-12, (p, umin)
+    print(f"{label}: blocks and solutions of {len(out)} problems checked against the oracle, "
+          f"max rel-res on the oracle's systems {max(o[3] for o in out):.2e}", flush=True)
+    return len(out)
+
+
+@pytest.mark.timeout(900)
+def test_batched_256_problem_draw_vs_oracle(oracle):
+    """draw_problems(256, 1), one system per degree: the FP32-assembly,
+    FP32-inner + FP64-refined batched Jacobi-PCG brings every problem to FP64
+    rel-res <= 1e-8, and every problem's block and solution pass the oracle
+    checks (_check_blocks_vs_oracle)."""
+    from feng_amd.api import PREC_FP32_IR
     from feng_amd.batched import build_batch, draw_problems, solve_batch
     probs = draw_problems(256, 1)
     checked = 0
@@ -167,37 +269,24 @@ def test_batched_256_problem_draw_vs_oracle(oracle):
         b = build_batch(probs, kind, keep_meshes=True)
         print(f"kind {kind}: {len(b.nInc)} problems, n = {int(b.rows[-1])}", flush=True)
         res = solve_batch(b, precision=PREC_FP32_IR, rtol=1e-8)
-        assert res["converged"].all() and (res["relres"] <= 1e-8).all(), res["relres"].max()
-        ia, ja = b.S.get_csr()
-        val, rhs = b.S.get_system()
-        b.S.correctSolution()
-        u = b.S.get_solution()
-
-        def check(p):
-            m, k = b.meshes[p], b.coeffs[p]
-            num = Numbering(m, [kind], [1])
-            nl = num.adr[0].size // m.E
-            ia0, ja0 = oracle.csr(num.nInc, [(num.adr[0], nl, num.adr[0], nl)])
-            val0, rhs0 = oracle.assemble_diffusion(kind, m.xyz, m.conn, num.adr[0], num.nInc, ia0, ja0, 2,
-                                                   (4, 0.0, 0.0, k), (0, -1.0, 0.0, None), np.zeros(num.nDofs))
-            r0, r1 = int(b.rows[p]), int(b.rows[p + 1])
-            blk = slice(ia[r0], ia[r1])
-            ok_csr = np.array_equal(ia[r0:r1 + 1] - ia[r0], ia0) and np.array_equal(ja[blk] - r0, ja0)
-            e_val, e_rhs = relerr(val[blk], val0), relerr(rhs[r0:r1], rhs0)
-            up = u[b.dof_maps[p][:num.nInc]]
-            rr = np.linalg.norm(rhs0 - oracle.spmv(ia0, ja0, val0, up)) / np.linalg.norm(rhs0)
-            return ok_csr, e_val, e_rhs, rr, up.min()
-
-        with ThreadPoolExecutor(8) as ex:
-            out = list(ex.map(check, range(len(b.nInc))))
-        for p, (ok_csr, e_val, e_rhs, rr, umin) in enumerate(out):
-            assert ok_csr, p
-            assert e_val <= 1e-12 and e_rhs <= 1e-12, (p, e_val, e_rhs)
-            assert rr <= 1.01e-8, (p, rr)
-            if kind == 1:
-                assert umin >= -1e-12, (p, umin)
-        checked += len(out)
-        print(f"kind {kind}: blocks and solutions of {len(out)} problems checked against the oracle, "
-              f"max rel-res on the oracle's systems {max(o[3] for o in out):.2e}", flush=True)
+        checked += _check_blocks_vs_oracle(oracle, b, [kind] * len(b.nInc), res, f"kind {kind}")
         b.S.close()
     assert checked == 256
+
+
+@pytest.mark.timeout(900)
+def test_union_256_problem_draw_amg_vs_oracle(oracle):
+    """draw_problems(256, 2) as ONE system of both degrees (the batched
+    workload's layout), solved by the batched multigrid PCG with the tiles of
+    stopped problems skipped: every problem reaches FP64 rel-res <= 1e-8 and
+    its block and solution pass the oracle checks."""
+    from feng_amd.api import PC_AMG, PREC_FP64
+    from feng_amd.batched import build_union_batch, draw_problems, solve_batch
+    probs = draw_problems(256, 2)
+    b = build_union_batch(probs, keep_meshes=True)
+    print(f"union: {len(b.nInc)} problems, n = {int(b.rows[-1])}", flush=True)
+    res = solve_batch(b, precision=PREC_FP64, rtol=1e-8, pc=PC_AMG)
+    print(f"union AMG: iterations {res['iterations'].min()}..{res['iterations'].max()}, "
+          f"solve {res['solve_ms']:.0f} ms", flush=True)
+    assert _check_blocks_vs_oracle(oracle, b, [k for _, k, _ in probs], res, "union") == 256
+    b.S.close()
