@@ -597,7 +597,19 @@ int captureGraph(fe_hip_ctx *ctx, FeBatch &B, const T *val, TileVecs<T> &v, int 
   return FE_STATUS_OK;
 }
 
-// Uploads the tiles of the problems with done[p] == 0 as the iteration grid.
+// Uploads the prefix sums of the flags done[p] == 0: the multigrid V-cycle
+// skips the tiles of the other problems.
+int setSkip(fe_hip_ctx *ctx, FeBatch &B, const std::vector<int> &done)
+{
+  B.hActPrefix.assign(B.nprob + 1, 0);
+  for(int p = 0; p < B.nprob; ++p) B.hActPrefix[p + 1] = B.hActPrefix[p] + (done[p] ? 0 : 1);
+  FE_HIP_TRY(ctx, hipMemcpyAsync(B.actPrefix, B.hActPrefix.data(), sizeof(int) * (B.nprob + 1), hipMemcpyHostToDevice,
+                                 ctx->stream));
+  return FE_STATUS_OK;
+}
+
+// Uploads the tiles of the problems with done[p] == 0 as the iteration grid
+// (and the skip flags of the V-cycle).
 int setActive(fe_hip_ctx *ctx, FeBatch &B, const std::vector<int> &done)
 {
   std::vector<int> act;
@@ -608,11 +620,7 @@ int setActive(fe_hip_ctx *ctx, FeBatch &B, const std::vector<int> &done)
   B.nAct = std::max(1, (int)act.size());
   if(act.empty()) act.push_back(0);  // a stopped problem's tile returns at once
   FE_HIP_TRY(ctx, hipMemcpyAsync(B.act, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, ctx->stream));
-  B.hActPrefix.assign(B.nprob + 1, 0);
-  for(int p = 0; p < B.nprob; ++p) B.hActPrefix[p + 1] = B.hActPrefix[p] + (done[p] ? 0 : 1);
-  FE_HIP_TRY(ctx, hipMemcpyAsync(B.actPrefix, B.hActPrefix.data(), sizeof(int) * (B.nprob + 1), hipMemcpyHostToDevice,
-                                 ctx->stream));
-  return FE_STATUS_OK;
+  return setSkip(ctx, B, done);
 }
 
 // Runs inner iterations until every problem has stopped (or maxit).  The
@@ -817,8 +825,14 @@ int solveBatchedAmgPerProblem(fe_hip_ctx *ctx, FeBatch &B, const fe_hip_solver_o
       int active = 0;
       for(int p = 0; p < nprob; ++p)
         if(!hDone[p]) active += B.hPtile[p + 1] - B.hPtile[p];
-      if(active < (3 * B.nAct) / 4)
+      // the iteration grid shrinks (a recapture) below 3/4 of its tiles; the
+      // V-cycle's skip flags follow every check (stopped problems stay
+      // stopped until the next restart, so their vectors are not read again)
+      if(active < (3 * B.nAct) / 4) {
         if(int rc = setActive(ctx, B, hDone)) return rc;
+      } else if(int rc = setSkip(ctx, B, hDone)) {
+        return rc;
+      }
     }
 #undef FE_SP_B
     FE_HIP_TRY(ctx, hipMemcpyAsync(hIter.data(), B.iter, sizeof(int) * nprob, hipMemcpyDeviceToHost, ctx->stream));
