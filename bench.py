@@ -270,6 +270,8 @@ def bench_batched(args):
     systems = [(None, "P1+P2")] if args.batch_layout == "union" else [(1, "P1"), (2, "P2")]
     for kind, name in systems:
         b = build_union_batch(probs) if kind is None else build_batch(probs, kind)
+        if args.no_batch_graph:
+            b.S.set_tuning("batch_graph", 0)  # the multigrid chunks as direct launches
         for meth, kw in methods.items():
             t0 = time.perf_counter()
             warm = solve_batch(b, rtol=1e-8, autotune=True, **kw)  # warmup: autotuners / hierarchy
@@ -297,9 +299,16 @@ def bench_batched(args):
             rows_l, nnz_l = b.S.amg_levels()
             n0, z0 = rows_l[0], nnz_l[0]
             n1 = rows_l[1] if len(rows_l) > 1 else 0
-            # (4 + 4) nnz (FP32 values, columns) + 4 (n + 1) + 4 s n + 4 n + s n_1, s = 8
-            ub = 8 * z0 + 4 * (n0 + 1) + 32 * n0 + 4 * n0 + 8 * n1
-            kern[name] = {"avg_launch_ms": rp["kernel_ms"], "launches": rp["kernel_launches"], "alg_bytes": ub}
+            pre = 0 < b.S.get_tuning("amg_pre_rows") <= n0
+            if pre:
+                # the product over y = u + oc e[agg] stored by the row pass:
+                # (4 + 4) nnz + 4 (n + 1) + y, b, x (8 each) and l1 (4) per row
+                ub = 8 * z0 + 4 * (n0 + 1) + 28 * n0
+            else:
+                # (4 + 4) nnz (FP32 values, columns) + 4 (n + 1) + 4 s n + 4 n + s n_1, s = 8
+                ub = 8 * z0 + 4 * (n0 + 1) + 32 * n0 + 4 * n0 + 8 * n1
+            kern[name] = {"avg_launch_ms": rp["kernel_ms"], "launches": rp["kernel_launches"], "alg_bytes": ub,
+                          "kernel": "amgUp<double, float, 4, 256, true>" if pre else "amgUp<double, float, 4, 256>"}
         tot_n += r["n"]
         b.S.close()
     best = min(tot_ms, key=tot_ms.get)
@@ -308,7 +317,7 @@ def bench_batched(args):
     if big in kern and kern[big]["avg_launch_ms"]:
         kp = kern[big]
         ach = kp["alg_bytes"] / (kp["avg_launch_ms"] * 1e-3) / 1e9
-        roof = {"bound": "hbm", "kernel": f"amgUp<double, float, 4, 256> (level 0 of the {big} system's V-cycle)",
+        roof = {"bound": "hbm", "kernel": f"{kp['kernel']} (level 0 of the {big} system's V-cycle)",
                 "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
                 "avg_launch_ms": kp["avg_launch_ms"], "alg_bytes_per_launch": kp["alg_bytes"], "per_batch": kern}
     cpu = None if args.no_cpu else cpu_batched(probs)
@@ -568,6 +577,8 @@ def main():
     ap.add_argument("--mfma-probe", action="store_true")
     ap.add_argument("--paths", default=None, help="comma-separated subset of " + ",".join(PATHS))
     ap.add_argument("--methods", default=None, help="batched workload: comma-separated subset of jacobi,amg")
+    ap.add_argument("--no-batch-graph", action="store_true",
+                    help="batched workload: launch the multigrid method's chunks directly, not from a captured graph")
     ap.add_argument("--batch-layout", default="union", choices=["union", "degree"],
                     help="batched workload: one system for both degrees, or one per degree")
     args = ap.parse_args()

@@ -380,7 +380,7 @@ __global__ __launch_bounds__(1024) void amgDenseInverse(int n, const int *__rest
 // aggregate sums its members into bNext.  The position data are the first
 // loads of the kernel, so the row products start one load later than
 // through a tile header.
-template <typename T, typename TV, int LPR, int TB>
+template <typename T, typename TV, int LPR, int TB, bool PRE = false>
 __global__ __launch_bounds__(TB) void amgDownP(AmgLev<T, TV> L, const T *__restrict__ b, T *__restrict__ bNext,
                                                const int *__restrict__ prow, const int2 *__restrict__ prg,
                                                const int *__restrict__ pagg, int nTiles, int xcd, const int *done,
@@ -409,10 +409,15 @@ __global__ __launch_bounds__(TB) void amgDownP(AmgLev<T, TV> L, const T *__restr
     bi = b[i];
     li = (T)L.l1[i];
   }
-  tileProductF<T, LPR, TB>(L.ja, L.val, kb, ke, ldsY, [&](int j) { return (T)L.l1[j] * b[j]; });
+  // PRE: u = l1 b is already stored for every row (amgScaleRows), one
+  // gather per entry instead of two
+  if constexpr(PRE)
+    tileProductF<T, LPR, TB>(L.ja, L.val, kb, ke, ldsY, [&](int j) { return L.u[j]; });
+  else
+    tileProductF<T, LPR, TB>(L.ja, L.val, kb, ke, ldsY, [&](int j) { return (T)L.l1[j] * b[j]; });
   __syncthreads();
   if(i >= 0) {
-    L.u[i] = li * bi;
+    if constexpr(!PRE) L.u[i] = li * bi;
     rb[t] = bi - ldsY[t];
   }
   ag[t] = a;
@@ -474,7 +479,7 @@ __global__ __launch_bounds__(TB) void amgDown(AmgLev<T, TV> L, const T *__restri
 
 // Prolongation fused with the second smoothing step, over row tiles:
 // y_i = u_i + oc e[agg_i], x_i = y_i + l1_i (b_i - sum_j a_ij y_j).
-template <typename T, typename TV, int LPR, int TB>
+template <typename T, typename TV, int LPR, int TB, bool PRE = false>
 __global__ __launch_bounds__(TB) void amgUp(AmgLev<T, TV> L, const T *__restrict__ b, const T *__restrict__ e, T oc,
                                             T *__restrict__ x, int xcd, const int *done, const int2 *__restrict__ tp,
                                             const int *__restrict__ act)
@@ -488,14 +493,49 @@ __global__ __launch_bounds__(TB) void amgUp(AmgLev<T, TV> L, const T *__restrict
     const int base = tile * TB;
     int kb[LPR], ke[LPR];
     tileBounds<LPR, TB>(n, base, L.ia, kb, ke);
-    tileProductF<T, LPR, TB>(L.ja, L.val, kb, ke, ldsY, [&](int j) { return L.u[j] + oc * e[L.agg[j]]; });
+    // PRE: y = u + oc e[agg] is already stored in L.r (amgUpStart), one
+    // gather per entry instead of three
+    if constexpr(PRE)
+      tileProductF<T, LPR, TB>(L.ja, L.val, kb, ke, ldsY, [&](int j) { return L.r[j]; });
+    else
+      tileProductF<T, LPR, TB>(L.ja, L.val, kb, ke, ldsY, [&](int j) { return L.u[j] + oc * e[L.agg[j]]; });
     __syncthreads();
     const int row = base + threadIdx.x;
     if(row < n) {
-      const T yi = L.u[row] + oc * e[L.agg[row]];
+      const T yi = PRE ? L.r[row] : L.u[row] + oc * e[L.agg[row]];
       x[row] = yi + (T)L.l1[row] * (b[row] - ldsY[threadIdx.x]);
     }
     __syncthreads();
+  }
+}
+
+// Row passes of the large levels (the gathered vector stored once, so that
+// the level kernels' products gather one vector per entry): over the same row
+// blocks, skip flags and block order as amgUp.
+template <typename T, typename TV, int TB>
+__global__ __launch_bounds__(TB) void amgScaleRows(AmgLev<T, TV> L, const T *__restrict__ b, int xcd, const int *done,
+                                                   const int2 *__restrict__ tp, const int *__restrict__ act)
+{
+  if(done && *done) return;
+  const int nTiles = (L.n + TB - 1) / TB;
+  for(int tile = amgTile(xcd); tile < nTiles; tile += gridDim.x) {
+    if(amgSkip(tp, act, tile)) continue;
+    const int row = tile * TB + threadIdx.x;
+    if(row < L.n) L.u[row] = (T)L.l1[row] * b[row];
+  }
+}
+
+template <typename T, typename TV, int TB>
+__global__ __launch_bounds__(TB) void amgUpStart(AmgLev<T, TV> L, const T *__restrict__ e, T oc, int xcd,
+                                                 const int *done, const int2 *__restrict__ tp,
+                                                 const int *__restrict__ act)
+{
+  if(done && *done) return;
+  const int nTiles = (L.n + TB - 1) / TB;
+  for(int tile = amgTile(xcd); tile < nTiles; tile += gridDim.x) {
+    if(amgSkip(tp, act, tile)) continue;
+    const int row = tile * TB + threadIdx.x;
+    if(row < L.n) L.r[row] = L.u[row] + oc * e[L.agg[row]];
   }
 }
 
@@ -1113,7 +1153,14 @@ int amgApplyT(fe_hip_ctx *ctx, const T *in, T *out, const int *done, int slot)
     const dim3 gp(ctx->amgXcd ? (h.padTiles + 7) & ~7 : h.padTiles);
     const bool pad = h.padTiles > 0 && ctx->amgPadTiles;
     const int2 *tp = pad ? h.probPad : h.probTile;
-    if(pad && ctx->amgLanes == 2)
+    int Gr = std::max(1, (h.n + kAmgTB - 1) / kAmgTB);
+    if(ctx->amgXcd) Gr = (Gr + 7) & ~7;
+    if(pad && ctx->amgPreRows > 0 && h.n >= ctx->amgPreRows && ctx->amgLanes == kAmgLPR) {
+      hipLaunchKernelGGL((amgScaleRows<T, TV, kAmgTB>), dim3(Gr), dim3(kAmgTB), 0, ctx->stream, L, b, ctx->amgXcd, done,
+                         (const int2 *)h.probUp, act);
+      hipLaunchKernelGGL((amgDownP<T, TV, kAmgLPR, kAmgTB, true>), gp, dim3(kAmgTB), 0, ctx->stream, L, b, bn, h.prow,
+                         h.prg, h.pagg, h.padTiles, ctx->amgXcd, done, tp, act);
+    } else if(pad && ctx->amgLanes == 2)
       hipLaunchKernelGGL((amgDownP<T, TV, 2, kAmgTB>), gp, dim3(kAmgTB), 0, ctx->stream, L, b, bn, h.prow, h.prg,
                          h.pagg, h.padTiles, ctx->amgXcd, done, tp, act);
     else if(pad && ctx->amgLanes == 8)
@@ -1137,11 +1184,18 @@ int amgApplyT(fe_hip_ctx *ctx, const T *in, T *out, const int *done, int slot)
     int G = std::max(1, (h.n + kAmgTB - 1) / kAmgTB);
     if(ctx->amgXcd) G = (G + 7) & ~7;
     const T *e = pick<T>(A.lv[l + 1].x64, A.lv[l + 1].x32);
-    // profiling: events around the level-0 prolongation (the dominant kernel
-    // of the batched multigrid solves)
+    // profiling: events around the level-0 prolongation kernel (the dominant
+    // kernel of the batched multigrid solves; its row pass stays outside)
+    const bool pre = ctx->amgPreRows > 0 && h.n >= ctx->amgPreRows && ctx->amgLanes == kAmgLPR;
+    if(pre)
+      hipLaunchKernelGGL((amgUpStart<T, TV, kAmgTB>), dim3(G), dim3(kAmgTB), 0, ctx->stream, L, e, oc, ctx->amgXcd, done,
+                         (const int2 *)h.probUp, act);
     if(l == 0 && ctx->profiling)
       if(int rc = feTimerMark(ctx, ctx->spmvTimer)) return rc;
-    if(ctx->amgLanes == 2)
+    if(pre) {
+      hipLaunchKernelGGL((amgUp<T, TV, kAmgLPR, kAmgTB, true>), dim3(G), dim3(kAmgTB), 0, ctx->stream, L, b, e, oc, x,
+                         ctx->amgXcd, done, (const int2 *)h.probUp, act);
+    } else if(ctx->amgLanes == 2)
       hipLaunchKernelGGL((amgUp<T, TV, 2, kAmgTB>), dim3(G), dim3(kAmgTB), 0, ctx->stream, L, b, e, oc, x, ctx->amgXcd,
                          done, (const int2 *)h.probUp, act);
     else if(ctx->amgLanes == 8)

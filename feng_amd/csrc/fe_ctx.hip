@@ -707,6 +707,9 @@ int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value)
   } else if(k == "amg_kernel") {
     FE_REQUIRE(ctx, value == 1 || value == 2, "amg_kernel must be 1 (per-level kernels) or 2 (nested-ordered PCG)");
     ctx->amgKernel = value;
+  } else if(k == "amg_pre_rows") {
+    FE_REQUIRE(ctx, value >= 0, "amg_pre_rows must be >= 0 (0: no row passes)");
+    ctx->amgPreRows = value;
   } else if(k == "batch_skip") {
     FE_REQUIRE(ctx, value == 0 || value == 1, "batch_skip must be 0 (every tile) or 1 (skip stopped problems)");
     ctx->batchSkip = value;
@@ -825,6 +828,7 @@ int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value)
   else if(k == "ns_mfma") *value = ctx->nsMfma;
   else if(k == "batch_graph") *value = ctx->batchGraph;
   else if(k == "batch_skip") *value = ctx->batchSkip;
+  else if(k == "amg_pre_rows") *value = ctx->amgPreRows;
   else if(k == "amg_fuse_grid") *value = feAmgPcgFuseGrid(ctx);
   else if(k == "amg_wcycle_level") *value = feAmgPcgWLevel(ctx);
   else if(k == "amg_wcycle_dense") *value = feAmgPcgWDense(ctx);

@@ -270,6 +270,9 @@ struct fe_hip_ctx {
   int xcdTiles = 0;  // XCD-aware tile order in the row-tile PCG kernels (measured slower at 512-row tiles)
   int amgLanes = 4;  // lanes per row in the multigrid level kernels
   int amgPadTiles = 1;  // restriction over aggregate-aligned tiles (0: tile-header kernel)
+  // per-level multigrid: levels of at least this many rows store the gathered
+  // vectors (l1 b, u + oc e[agg]) in a row pass first (0: never)
+  int amgPreRows = 1 << 20;
   int batchAmgGlobal = 0;  // batched AMG: 0 per-problem CG scalars, 1 one CG over the union system
   FeAmg *amg = nullptr;
   std::vector<FeAmgHostLevel> amgHost;  // the hierarchy's pattern on the host (empty before the first AMG solve)

@@ -154,23 +154,27 @@ def test_batched_amg_tile_skipping_equals_every_tile():
     stopped (batch_skip 1, the default): each row only couples to rows of its
     own problem on every level, so solutions, iterations and residuals equal
     the launches over every tile (batch_skip 0) bit for bit, on one degree
-    and on the union system of both degrees."""
+    and on the union system of both degrees.  The same holds with the row
+    passes that store the gathered vectors of every level first
+    (amg_pre_rows 1) and without them (amg_pre_rows 0)."""
     from feng_amd.api import PC_AMG, PREC_FP64
     from feng_amd.batched import build_batch, build_union_batch, solve_batch
     print("batched AMG tile skipping", flush=True)
     probs = [(9, 2, 41), (40, 1, 42), (17, 2, 43), (64, 2, 44), (25, 1, 45), (48, 2, 46), (33, 1, 47)]
     for b in (build_batch(probs, 2), build_union_batch(probs)):
         out = {}
-        for skip in (1, 0):
+        for skip, pre in ((1, 1), (0, 0), (1, 0), (0, 1)):
             b.S.set_tuning("batch_skip", skip)
+            b.S.set_tuning("amg_pre_rows", pre)
             b.S.set_solution(np.zeros(b.S.nDofs))
             res = solve_batch(b, precision=PREC_FP64, rtol=1e-10, pc=PC_AMG)
             assert res["converged"].all()
             b.S.correctSolution()
-            out[skip] = (b.S.get_solution().copy(), res["iterations"].copy(), res["relres"].copy())
-        assert len(set(out[0][1])) > 1  # problems stop at different iterations
-        for a_, b_ in zip(out[1], out[0]):
-            assert np.array_equal(a_, b_)
+            out[skip, pre] = (b.S.get_solution().copy(), res["iterations"].copy(), res["relres"].copy())
+        assert len(set(out[0, 0][1])) > 1  # problems stop at different iterations
+        for key in out:
+            for a_, b_ in zip(out[key], out[0, 0]):
+                assert np.array_equal(a_, b_), key
         b.S.close()
 
 
