@@ -40,7 +40,7 @@ class UnionMesh:
 
 @dataclass
 class Batch:
-    kind: int
+    kind: int                 # element kind of every problem; 0: both degrees (build_union_batch)
     sizes: list
     seeds: list
     S: A.LinearSystemHIP
