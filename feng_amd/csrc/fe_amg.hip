@@ -78,13 +78,18 @@ __device__ __forceinline__ int amgTile(int xcd)
   return (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
 }
 
-// A tile whose problems [r.x, r.y] have all stopped (act: prefix sums of the
-// still-iterating flags) is skipped.
-__device__ __forceinline__ bool amgSkip(const int2 *__restrict__ tp, const int *__restrict__ act, int tile)
+// A tile whose problems [r.x, r.y] have all stopped (stop[p] != 0, the
+// batched solver's per-problem flags) is skipped; a tile over more than
+// kAmgSkipSpan problems (a coarse level) always runs.
+constexpr int kAmgSkipSpan = 16;
+__device__ __forceinline__ bool amgSkip(const int2 *__restrict__ tp, const int *__restrict__ stop, int tile)
 {
-  if(!act) return false;
+  if(!stop) return false;
   const int2 r = tp[tile];
-  return act[r.y + 1] == act[r.x];
+  if(r.y - r.x >= kAmgSkipSpan) return false;
+  for(int p = r.x; p <= r.y; ++p)
+    if(!stop[p]) return false;
+  return true;
 }
 
 template <typename T, typename TV>
@@ -136,7 +141,7 @@ struct FeAmg {
   void *desc[3] = {nullptr, nullptr, nullptr};  // AmgLev<double,double>, <double,float>, <float,float>
   int *flag = nullptr;
   double c = 0., oc = 0.;
-  const int *act = nullptr;    // prefix sums of the still-iterating problems (tile skipping on)
+  const int *act = nullptr;    // per-problem stop flags (tile skipping on)
   std::vector<int> probRows;   // the problem rows the tile ranges were built for
 };
 

@@ -47,8 +47,6 @@ struct FeBatch {
   std::vector<int> hPtile;  // host copy of the per-problem tile ranges
   int *act = nullptr;       // tiles of the problems still iterating (the iteration grid)
   int nAct = 0;
-  int *actPrefix = nullptr;     // nprob + 1 prefix sums of the still-iterating flags
-  std::vector<int> hActPrefix;  // its host copy (the source of the upload)
 };
 
 void feBatchRelease(fe_hip_ctx *ctx)
@@ -323,6 +321,9 @@ __global__ void bSpStart(const double *__restrict__ b, BSpVecs v, BTiles bt, BSt
     v.x[row] = 0.;
     v.r[row] = S.outerDone[p] ? 0. : b[row];
   }
+  // the stop flags of this pass's first V-cycle (which skips the tiles of
+  // stopped problems); bSpInit sets them again
+  if(threadIdx.x == 0 && tile == bt.ptile[p]) S.done[p] = S.outerDone[p];
 }
 
 // w = A u, z = q = s = p = 0, partials of iteration 0, per-problem start values.
@@ -513,8 +514,7 @@ int setupBatch(fe_hip_ctx *ctx, int nprob, const int *rows, FeBatch *&out)
   if(feAlloc(ctx, &B.pTile, nprob + 1) || feAlloc(ctx, &B.done, nprob) || feAlloc(ctx, &B.iter, nprob) ||
      feAlloc(ctx, &B.outerDone, nprob) || feAlloc(ctx, &B.itBase, 1) || feAlloc(ctx, &B.gamma, 2 * nprob) ||
      feAlloc(ctx, &B.alpha, 2 * nprob) || feAlloc(ctx, &B.tol2, nprob) || feAlloc(ctx, &B.rr64, nprob) ||
-     feAlloc(ctx, &B.x64, n) || feAlloc(ctx, &B.r64, n) || feAlloc(ctx, &B.bak64, n) ||
-     feAlloc(ctx, &B.actPrefix, nprob + 1))
+     feAlloc(ctx, &B.x64, n) || feAlloc(ctx, &B.r64, n) || feAlloc(ctx, &B.bak64, n))
     return FE_STATUS_OUT_OF_MEMORY;
   double **a[] = {&B.v64.x, &B.v64.r, &B.v64.w, &B.v64.s, &B.v64.z, &B.v64.p, &B.v64.dinv, &B.v64.m[0], &B.v64.m[1]};
   for(auto pp : a)
@@ -597,19 +597,7 @@ int captureGraph(fe_hip_ctx *ctx, FeBatch &B, const T *val, TileVecs<T> &v, int 
   return FE_STATUS_OK;
 }
 
-// Uploads the prefix sums of the flags done[p] == 0: the multigrid V-cycle
-// skips the tiles of the other problems.
-int setSkip(fe_hip_ctx *ctx, FeBatch &B, const std::vector<int> &done)
-{
-  B.hActPrefix.assign(B.nprob + 1, 0);
-  for(int p = 0; p < B.nprob; ++p) B.hActPrefix[p + 1] = B.hActPrefix[p] + (done[p] ? 0 : 1);
-  FE_HIP_TRY(ctx, hipMemcpyAsync(B.actPrefix, B.hActPrefix.data(), sizeof(int) * (B.nprob + 1), hipMemcpyHostToDevice,
-                                 ctx->stream));
-  return FE_STATUS_OK;
-}
-
-// Uploads the tiles of the problems with done[p] == 0 as the iteration grid
-// (and the skip flags of the V-cycle).
+// Uploads the tiles of the problems with done[p] == 0 as the iteration grid.
 int setActive(fe_hip_ctx *ctx, FeBatch &B, const std::vector<int> &done)
 {
   std::vector<int> act;
@@ -620,7 +608,7 @@ int setActive(fe_hip_ctx *ctx, FeBatch &B, const std::vector<int> &done)
   B.nAct = std::max(1, (int)act.size());
   if(act.empty()) act.push_back(0);  // a stopped problem's tile returns at once
   FE_HIP_TRY(ctx, hipMemcpyAsync(B.act, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, ctx->stream));
-  return setSkip(ctx, B, done);
+  return FE_STATUS_OK;
 }
 
 // Runs inner iterations until every problem has stopped (or maxit).  The
@@ -735,8 +723,9 @@ int solveBatchedAmgPerProblem(fe_hip_ctx *ctx, FeBatch &B, const fe_hip_solver_o
     if(!q && feAlloc(ctx, &q, n)) return FE_STATUS_OUT_OF_MEMORY;
   BSpVecs v{B.sp[0], B.sp[1], B.sp[2], B.sp[3], B.sp[4], B.sp[5], B.sp[6], B.sp[7], B.sp[8]};
   if(int rc = feAmgSetup(ctx)) return rc;
-  // the V-cycle skips the tiles of stopped problems while this solve runs
-  if(int rc = feAmgProblemTiles(ctx, nprob, B.rows.data(), ctx->batchSkip ? B.actPrefix : nullptr)) return rc;
+  // the V-cycle skips the tiles of stopped problems (their done flags) while
+  // this solve runs
+  if(int rc = feAmgProblemTiles(ctx, nprob, B.rows.data(), ctx->batchSkip ? B.done : nullptr)) return rc;
   struct Detach {
     fe_hip_ctx *c;
     ~Detach() { (void)feAmgProblemTiles(c, 0, nullptr, nullptr); }
@@ -769,7 +758,7 @@ int solveBatchedAmgPerProblem(fe_hip_ctx *ctx, FeBatch &B, const fe_hip_solver_o
     if(!any) break;
     FE_HIP_TRY(ctx, hipMemcpyAsync(B.outerDone, outerDone.data(), sizeof(int) * nprob, hipMemcpyHostToDevice, ctx->stream));
     FE_HIP_TRY(ctx, hipMemcpyAsync(B.tol2, tol2.data(), sizeof(double) * nprob, hipMemcpyHostToDevice, ctx->stream));
-    if(int rc = setActive(ctx, B, outerDone)) return rc;  // before the first V-cycle: it reads the flags
+    if(int rc = setActive(ctx, B, outerDone)) return rc;
     hipLaunchKernelGGL(bSpStart, dim3(B.nTiles), dim3(TB), 0, ctx->stream, B.r64, v, bt, S);
     if(int rc = feAmgApply(ctx, (const double *)v.r, v.u, nullptr, v32)) return rc;
 #define FE_SP_B(KER, GRID, ...)                                                                                     \
@@ -825,14 +814,9 @@ int solveBatchedAmgPerProblem(fe_hip_ctx *ctx, FeBatch &B, const fe_hip_solver_o
       int active = 0;
       for(int p = 0; p < nprob; ++p)
         if(!hDone[p]) active += B.hPtile[p + 1] - B.hPtile[p];
-      // the iteration grid shrinks (a recapture) below 3/4 of its tiles; the
-      // V-cycle's skip flags follow every check (stopped problems stay
-      // stopped until the next restart, so their vectors are not read again)
-      if(active < (3 * B.nAct) / 4) {
+      // the iteration grid shrinks (a recapture) below 3/4 of its tiles
+      if(active < (3 * B.nAct) / 4)
         if(int rc = setActive(ctx, B, hDone)) return rc;
-      } else if(int rc = setSkip(ctx, B, hDone)) {
-        return rc;
-      }
     }
 #undef FE_SP_B
     FE_HIP_TRY(ctx, hipMemcpyAsync(hIter.data(), B.iter, sizeof(int) * nprob, hipMemcpyDeviceToHost, ctx->stream));

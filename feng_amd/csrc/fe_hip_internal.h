@@ -375,9 +375,9 @@ int feAmgApply(fe_hip_ctx *ctx, const float *in, float *out, const int *done, bo
 int feAmgLevels(const fe_hip_ctx *ctx, int *levels, int *rows, int64_t *nnz, int cap);
 // Block-diagonal systems of independent problems (problem p: rows
 // [rows[p], rows[p + 1]) of level 0): each tile of the level kernels gets the
-// range of problems it touches, and with act (device, nprob + 1 prefix sums
-// of the still-iterating flags) the V-cycle's level launches skip the tiles
-// whose problems have all stopped.  act = nullptr detaches (every tile runs).
+// range of problems it touches, and with act (device, nprob flags, nonzero:
+// the problem has stopped) the V-cycle's level launches skip the tiles whose
+// problems have all stopped.  act = nullptr detaches (every tile runs).
 int feAmgProblemTiles(fe_hip_ctx *ctx, int nprob, const int *rows, const int *act);
 // First level of the single-workgroup kernel for a (vector, value) type slot
 // (0 f64/f64, 1 f64/f32, 2 f32/f32): part of the launch sequence a graph captures.
