@@ -57,6 +57,9 @@ TRAFFIC = {  # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE (cube
     "pcgDown0<double, float> (level-0 CG update + smoothing + restriction)": 76.2e6,
     "pcgDown0<float, float> (level-0 CG update + smoothing + restriction)": 53.0e6,
     "asmGather<double, 4, 3, 16>": 370.5e6,
+    # the batched workload's level-0 prolongation product over the whole level
+    # (draw_problems(256, 1) as one system): profiles/batched256_hbm_traffic.md
+    "amgUp<double, float, 4, 256, true>": 2012.9e6,
 }
 
 
@@ -318,7 +321,9 @@ def bench_batched(args):
         kp = kern[big]
         ach = kp["alg_bytes"] / (kp["avg_launch_ms"] * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": f"{kp['kernel']} (level 0 of the {big} system's V-cycle)",
-                "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                "traffic": TRAFFIC.get(kp["kernel"]) if (args.seed, len(probs), args.batch_layout) == (1, 256, "union")
+                else None,
                 "avg_launch_ms": kp["avg_launch_ms"], "alg_bytes_per_launch": kp["alg_bytes"], "per_batch": kern}
     cpu = None if args.no_cpu else cpu_batched(probs)
     out.update({"metric": "DOF/s assemble+solve to FP64 rel-res 1e-8 (256 batched problems)",
