@@ -52,6 +52,11 @@ class Batch:
     setup_s: float = 0.0
 
 
+# over-correction factors (x100) the multigrid method's autotuner times; the
+# last is the library default
+AMG_OC_CHOICES = (140, 160, 180)
+
+
 def draw_problems(nprob=256, seed=0, nmin=32, nmax=256):
     """N_p uniform in [nmin, nmax], degree P1/P2 50/50, coefficient seeds."""
     rng = np.random.default_rng(seed)
@@ -158,7 +163,9 @@ def solve_batch(b: Batch, precision=A.PREC_FP32_IR, rtol=1e-8, inner_rtol=1e-2, 
     """Assemble every problem (zero + all element kernels + scatter) and solve
     them all; returns timings and per-problem statistics.  autotune: the
     assembly kernel variant (fe_hip_autotune_assembly) and, for the Jacobi
-    method, the iteration's launch shape are timed and the fastest kept.
+    method, the iteration's launch shape are timed and the fastest kept; for
+    the multigrid method, the coarse correction's over-correction factor
+    (amg_oc_x100 in AMG_OC_CHOICES) is chosen by timing one solve with each.
     profile: events around the dominant kernel's launches (kernel_ms,
     kernel_launches)."""
     L, S = b.S.L, b.S
@@ -185,6 +192,17 @@ def solve_batch(b: Batch, precision=A.PREC_FP32_IR, rtol=1e-8, inner_rtol=1e-2, 
     o.pc = pc
     st = (N.FeHipBatchStats * nprob)()
     ms = C.c_double()
+    if autotune and pc == A.PC_AMG:
+        trial = {}
+        for oc in AMG_OC_CHOICES:
+            S.set_tuning("amg_oc_x100", oc)
+            S._c(L.fe_hip_solve_batched(S.ctx, nprob, N.ptr(rows, C.c_int), C.byref(o), st, C.byref(ms)),
+                 "fe_hip_solve_batched")
+            if all(x.converged for x in st):
+                trial[oc] = ms.value
+        oc = min(trial, key=trial.get) if trial else AMG_OC_CHOICES[-1]
+        S.set_tuning("amg_oc_x100", oc)
+        tuned.update(amg_oc_x100=oc, amg_oc_ms={k: round(v, 1) for k, v in trial.items()})
     S._c(L.fe_hip_solve_batched(S.ctx, nprob, N.ptr(rows, C.c_int), C.byref(o), st, C.byref(ms)),
          "fe_hip_solve_batched")
     its = np.array([s.iterations for s in st])

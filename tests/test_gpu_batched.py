@@ -139,7 +139,7 @@ def test_batched_amg_graph_replays_equal_direct_launches():
     for g in (1, 0, 1):
         b.S.set_tuning("batch_graph", g)
         b.S.set_solution(np.zeros(b.S.nDofs))
-        res = solve_batch(b, precision=PREC_FP64, rtol=1e-10, pc=PC_AMG)
+        res = solve_batch(b, precision=PREC_FP64, rtol=1e-10, pc=PC_AMG, autotune=False)
         assert res["converged"].all()
         b.S.correctSolution()
         out.setdefault(g, []).append((b.S.get_solution().copy(), res["iterations"].copy(), res["relres"].copy()))
@@ -167,7 +167,7 @@ def test_batched_amg_tile_skipping_equals_every_tile():
             b.S.set_tuning("batch_skip", skip)
             b.S.set_tuning("amg_pre_rows", pre)
             b.S.set_solution(np.zeros(b.S.nDofs))
-            res = solve_batch(b, precision=PREC_FP64, rtol=1e-10, pc=PC_AMG)
+            res = solve_batch(b, precision=PREC_FP64, rtol=1e-10, pc=PC_AMG, autotune=False)
             assert res["converged"].all()
             b.S.correctSolution()
             out[skip, pre] = (b.S.get_solution().copy(), res["iterations"].copy(), res["relres"].copy())
@@ -291,6 +291,6 @@ def test_union_256_problem_draw_amg_vs_oracle(oracle):
     print(f"union: {len(b.nInc)} problems, n = {int(b.rows[-1])}", flush=True)
     res = solve_batch(b, precision=PREC_FP64, rtol=1e-8, pc=PC_AMG)
     print(f"union AMG: iterations {res['iterations'].min()}..{res['iterations'].max()}, "
-          f"solve {res['solve_ms']:.0f} ms", flush=True)
+          f"solve {res['solve_ms']:.0f} ms, tuned {res['tuned']}", flush=True)
     assert _check_blocks_vs_oracle(oracle, b, [k for _, k, _ in probs], res, "union") == 256
     b.S.close()
