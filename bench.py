@@ -302,7 +302,8 @@ def bench_batched(args):
             rows_l, nnz_l = b.S.amg_levels()
             n0, z0 = rows_l[0], nnz_l[0]
             n1 = rows_l[1] if len(rows_l) > 1 else 0
-            pre = 0 < b.S.get_tuning("amg_pre_rows") <= n0
+            lanes = b.S.get_tuning("amg_lanes")
+            pre = 0 < b.S.get_tuning("amg_pre_rows") <= n0 and lanes in (2, 4)
             if pre:
                 # the product over y = u + oc e[agg] stored by the row pass:
                 # (4 + 4) nnz + 4 (n + 1) + y, b, x (8 each) and l1 (4) per row
@@ -311,7 +312,8 @@ def bench_batched(args):
                 # (4 + 4) nnz (FP32 values, columns) + 4 (n + 1) + 4 s n + 4 n + s n_1, s = 8
                 ub = 8 * z0 + 4 * (n0 + 1) + 32 * n0 + 4 * n0 + 8 * n1
             kern[name] = {"avg_launch_ms": rp["kernel_ms"], "launches": rp["kernel_launches"], "alg_bytes": ub,
-                          "kernel": "amgUp<double, float, 4, 256, true>" if pre else "amgUp<double, float, 4, 256>"}
+                          "kernel": f"amgUp<double, float, {lanes}, 256, true>" if pre
+                          else f"amgUp<double, float, {lanes}, 256>"}
         tot_n += r["n"]
         b.S.close()
     best = min(tot_ms, key=tot_ms.get)

@@ -193,13 +193,16 @@ def solve_batch(b: Batch, precision=A.PREC_FP32_IR, rtol=1e-8, inner_rtol=1e-2, 
     st = (N.FeHipBatchStats * nprob)()
     ms = C.c_double()
     if autotune and pc == A.PC_AMG:
+        def timed():
+            S._c(L.fe_hip_solve_batched(S.ctx, nprob, N.ptr(rows, C.c_int), C.byref(o), st, C.byref(ms)),
+                 "fe_hip_solve_batched")
+            return ms.value if all(x.converged for x in st) else None
         trial = {}
         for oc in AMG_OC_CHOICES:
             S.set_tuning("amg_oc_x100", oc)
-            S._c(L.fe_hip_solve_batched(S.ctx, nprob, N.ptr(rows, C.c_int), C.byref(o), st, C.byref(ms)),
-                 "fe_hip_solve_batched")
-            if all(x.converged for x in st):
-                trial[oc] = ms.value
+            t = timed()
+            if t is not None:
+                trial[oc] = t
         oc = min(trial, key=trial.get) if trial else AMG_OC_CHOICES[-1]
         S.set_tuning("amg_oc_x100", oc)
         tuned.update(amg_oc_x100=oc, amg_oc_ms={k: round(v, 1) for k, v in trial.items()})

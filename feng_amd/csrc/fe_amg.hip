@@ -1160,11 +1160,16 @@ int amgApplyT(fe_hip_ctx *ctx, const T *in, T *out, const int *done, int slot)
     const int2 *tp = pad ? h.probPad : h.probTile;
     int Gr = std::max(1, (h.n + kAmgTB - 1) / kAmgTB);
     if(ctx->amgXcd) Gr = (Gr + 7) & ~7;
-    if(pad && ctx->amgPreRows > 0 && h.n >= ctx->amgPreRows && ctx->amgLanes == kAmgLPR) {
+    const bool preD = ctx->amgPreRows > 0 && h.n >= ctx->amgPreRows && (ctx->amgLanes == 2 || ctx->amgLanes == 4);
+    if(pad && preD) {
       hipLaunchKernelGGL((amgScaleRows<T, TV, kAmgTB>), dim3(Gr), dim3(kAmgTB), 0, ctx->stream, L, b, ctx->amgXcd, done,
                          (const int2 *)h.probUp, act);
-      hipLaunchKernelGGL((amgDownP<T, TV, kAmgLPR, kAmgTB, true>), gp, dim3(kAmgTB), 0, ctx->stream, L, b, bn, h.prow,
-                         h.prg, h.pagg, h.padTiles, ctx->amgXcd, done, tp, act);
+      if(ctx->amgLanes == 2)
+        hipLaunchKernelGGL((amgDownP<T, TV, 2, kAmgTB, true>), gp, dim3(kAmgTB), 0, ctx->stream, L, b, bn, h.prow,
+                           h.prg, h.pagg, h.padTiles, ctx->amgXcd, done, tp, act);
+      else
+        hipLaunchKernelGGL((amgDownP<T, TV, 4, kAmgTB, true>), gp, dim3(kAmgTB), 0, ctx->stream, L, b, bn, h.prow,
+                           h.prg, h.pagg, h.padTiles, ctx->amgXcd, done, tp, act);
     } else if(pad && ctx->amgLanes == 2)
       hipLaunchKernelGGL((amgDownP<T, TV, 2, kAmgTB>), gp, dim3(kAmgTB), 0, ctx->stream, L, b, bn, h.prow, h.prg,
                          h.pagg, h.padTiles, ctx->amgXcd, done, tp, act);
@@ -1191,14 +1196,17 @@ int amgApplyT(fe_hip_ctx *ctx, const T *in, T *out, const int *done, int slot)
     const T *e = pick<T>(A.lv[l + 1].x64, A.lv[l + 1].x32);
     // profiling: events around the level-0 prolongation kernel (the dominant
     // kernel of the batched multigrid solves; its row pass stays outside)
-    const bool pre = ctx->amgPreRows > 0 && h.n >= ctx->amgPreRows && ctx->amgLanes == kAmgLPR;
+    const bool pre = ctx->amgPreRows > 0 && h.n >= ctx->amgPreRows && (ctx->amgLanes == 2 || ctx->amgLanes == 4);
     if(pre)
       hipLaunchKernelGGL((amgUpStart<T, TV, kAmgTB>), dim3(G), dim3(kAmgTB), 0, ctx->stream, L, e, oc, ctx->amgXcd, done,
                          (const int2 *)h.probUp, act);
     if(l == 0 && ctx->profiling)
       if(int rc = feTimerMark(ctx, ctx->spmvTimer)) return rc;
-    if(pre) {
-      hipLaunchKernelGGL((amgUp<T, TV, kAmgLPR, kAmgTB, true>), dim3(G), dim3(kAmgTB), 0, ctx->stream, L, b, e, oc, x,
+    if(pre && ctx->amgLanes == 2) {
+      hipLaunchKernelGGL((amgUp<T, TV, 2, kAmgTB, true>), dim3(G), dim3(kAmgTB), 0, ctx->stream, L, b, e, oc, x,
+                         ctx->amgXcd, done, (const int2 *)h.probUp, act);
+    } else if(pre) {
+      hipLaunchKernelGGL((amgUp<T, TV, 4, kAmgTB, true>), dim3(G), dim3(kAmgTB), 0, ctx->stream, L, b, e, oc, x,
                          ctx->amgXcd, done, (const int2 *)h.probUp, act);
     } else if(ctx->amgLanes == 2)
       hipLaunchKernelGGL((amgUp<T, TV, 2, kAmgTB>), dim3(G), dim3(kAmgTB), 0, ctx->stream, L, b, e, oc, x, ctx->amgXcd,

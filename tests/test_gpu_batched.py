@@ -156,7 +156,8 @@ def test_batched_amg_tile_skipping_equals_every_tile():
     the launches over every tile (batch_skip 0) bit for bit, on one degree
     and on the union system of both degrees.  The same holds with the row
     passes that store the gathered vectors of every level first
-    (amg_pre_rows 1) and without them (amg_pre_rows 0)."""
+    (amg_pre_rows 1) and without them (amg_pre_rows 0), with four and with
+    two lanes per row."""
     from feng_amd.api import PC_AMG, PREC_FP64
     from feng_amd.batched import build_batch, build_union_batch, solve_batch
     print("batched AMG tile skipping", flush=True)
@@ -175,6 +176,18 @@ def test_batched_amg_tile_skipping_equals_every_tile():
         for key in out:
             for a_, b_ in zip(out[key], out[0, 0]):
                 assert np.array_equal(a_, b_), key
+        # two lanes per row (the other summation order): row passes on and off
+        b.S.set_tuning("amg_lanes", 2)
+        two = []
+        for pre in (1, 0):
+            b.S.set_tuning("amg_pre_rows", pre)
+            b.S.set_solution(np.zeros(b.S.nDofs))
+            res = solve_batch(b, precision=PREC_FP64, rtol=1e-10, pc=PC_AMG, autotune=False)
+            assert res["converged"].all()
+            b.S.correctSolution()
+            two.append((b.S.get_solution().copy(), res["iterations"].copy(), res["relres"].copy()))
+        for a_, b_ in zip(two[0], two[1]):
+            assert np.array_equal(a_, b_)
         b.S.close()
 
 
