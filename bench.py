@@ -229,9 +229,13 @@ def mfma_probe():
     """The matrix-core kernels for counter passes (rocprofv3 --pmc): the
     colored MFMA element sweep of the headline cube in FP64 and FP32
     (asm_kernel = 1, fp32_values = 1), the learned net's inference inside one
-    learned-PCG solve, and three training steps of the net on a 32^3 cube."""
-    from feng_amd.api import PC_LEARNED, PREC_FP32_IR, PREC_FP64
+    learned-PCG solve, and three training steps of the net on a 32^3 cube;
+    then the MFMA sweeps of P2 (512^2), P3 and P4 (256^2) triangle diffusion
+    and the Taylor-Hood Newton Jacobian (256^2 Kovasznay)."""
+    from feng_amd.api import (FN_CONST, FN_ELEM, PC_LEARNED, PREC_FP32_IR, PREC_FP64, TRI_P2, TRI_P3, TRI_P4, Fn,
+                              Mesh, essential_values)
     from feng_amd.learned import load_weights, train
+    from feng_amd.problems import diffusion_system, kovasznay, navier_stokes_system
 
     m, k, S, num = make_problem(64, 512, 1)
     S.set_tuning("asm_kernel", 1)
@@ -246,6 +250,23 @@ def mfma_probe():
     r = S.solve(pc=PC_LEARNED, rtol=1e-8)
     S.close()
     train(N=32, instances=3, log_every=1)
+    for kind, N in ((TRI_P2, 512), (TRI_P3, 256), (TRI_P4, 256)):
+        mt = Mesh.square(N)
+        deg = 2 if kind == TRI_P2 else 2 * (kind - 3)  # exact for the stiffness of a straight element
+        St, _ = diffusion_system(mt, kind, deg, Fn(FN_ELEM, data=mt.coeff_blocks(8, 0.1, 10.0, 3)), Fn(FN_CONST, -1.0))
+        St.set_tuning("asm_kernel", 1)
+        for _ in range(3):
+            St.setToZero()
+            St.assemble(3, PREC_FP64)
+        St.close()
+    mn = Mesh.square(256, 0.2, 0)
+    Sn, numn = navier_stokes_system(mn, 40.0, 5)
+    _, ufn, pfn = kovasznay(40.0)
+    Sn.set_solution(essential_values(numn, [ufn, pfn]))
+    for _ in range(3):  # the Jacobian alone: the matrix-core kernel (ns_mfma 1) assembles matrices only
+        Sn.setToZero()
+        Sn.assemble(1, PREC_FP64)
+    Sn.close()
     print(json.dumps({"mfma_probe": True, "learned_iterations": r.iterations}), flush=True)
 
 
