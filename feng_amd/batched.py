@@ -108,15 +108,13 @@ def build_batch(problems, kind, device=0, keep_meshes=False) -> Batch:
     return b
 
 
-def build_union_batch(problems, device=0, keep_meshes=False) -> Batch:
-    """One device system holding every problem of the draw, P1 and P2
-    together: the disjoint union mesh carries a P1 and a P2 field; problem p's
-    unknowns are rows [rows[p], rows[p+1]) in draw order, numbered by its own
-    field.  On an element of the other degree a field's map points at one
-    extra essential DOF (value 0), so that element adds no row, column or
-    gather incidence of that field.  One assembly and one batched solve then
-    cover the whole draw."""
-    t0 = time.perf_counter()
+def union_layout(problems):
+    """Host side of build_union_batch (no device work): per problem its mesh,
+    numbering and block coefficient; the union mesh (xyz, conn); the unknown
+    rows (rows[p]..rows[p+1] for problem p, in draw order) and per problem the
+    union DOF of each of its local DOFs (maps); per element kind the union DOF
+    map of the field over every element of the union mesh (adr), the other
+    degree's elements mapped to the extra essential DOF `dummy`."""
     meshes, nums, ks, kinds = [], [], [], []
     for N_, kind, s in problems:
         m = A.Mesh.square(N_)
@@ -140,20 +138,34 @@ def build_union_batch(problems, device=0, keep_meshes=False) -> Batch:
         maps.append(gm)
         for kind, nl in ((A.TRI_P1, 3), (A.TRI_P2, 6)):
             adr[kind].append(gm[nm.adr[0]] if kinds[p] == kind else np.full(meshes[p].E * nl, dummy))
-    k = np.concatenate(ks)
-    um = UnionMesh(2, int(voff[-1]), int(conn.size // 3), 3, xyz, conn)
+    adr = {kind: np.concatenate(v).astype(np.int32) for kind, v in adr.items()}
+    return dict(meshes=meshes, nums=nums, coeffs=ks, kinds=kinds, nInc=nInc, NU=NU, rows=uoff.astype(np.int32),
+                dummy=dummy, xyz=xyz, conn=conn, maps=maps, adr=adr, coeff=np.concatenate(ks))
+
+
+def build_union_batch(problems, device=0, keep_meshes=False) -> Batch:
+    """One device system holding every problem of the draw, P1 and P2
+    together: the disjoint union mesh carries a P1 and a P2 field; problem p's
+    unknowns are rows [rows[p], rows[p+1]) in draw order, numbered by its own
+    field.  On an element of the other degree a field's map points at one
+    extra essential DOF (value 0), so that element adds no row, column or
+    gather incidence of that field.  One assembly and one batched solve then
+    cover the whole draw."""
+    t0 = time.perf_counter()
+    U = union_layout(problems)
+    um = UnionMesh(2, int(U["xyz"].shape[0]), int(U["conn"].size // 3), 3, U["xyz"], U["conn"])
     S = A.LinearSystemHIP(device)
     S.upload_mesh(um)
-    S.set_sizes(NU, dummy + 1)
+    S.set_sizes(U["NU"], U["dummy"] + 1)
     for kind in (A.TRI_P1, A.TRI_P2):
-        f = S.add_field(kind, np.concatenate(adr[kind]).astype(np.int32))
-        S.add_form(A.DIFFUSION, [f], 2, [A.Fn(A.FN_ELEM, data=k)])
+        f = S.add_field(kind, U["adr"][kind])
+        S.add_form(A.DIFFUSION, [f], 2, [A.Fn(A.FN_ELEM, data=U["coeff"])])
         S.add_form(A.SOURCE, [f], 2, [A.Fn(A.FN_CONST, -1.0)])
     S.build_csr()
-    S.set_solution(np.zeros(dummy + 1))
-    b = Batch(0, [N_ for N_, _, _ in problems], [s for _, _, s in problems], S, uoff.astype(np.int32), nInc, maps)
+    S.set_solution(np.zeros(U["dummy"] + 1))
+    b = Batch(0, [N_ for N_, _, _ in problems], [s for _, _, s in problems], S, U["rows"], U["nInc"], U["maps"])
     if keep_meshes:
-        b.meshes, b.coeffs = meshes, ks
+        b.meshes, b.coeffs = U["meshes"], U["coeffs"]
     b.setup_s = time.perf_counter() - t0
     return b
 
