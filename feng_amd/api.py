@@ -256,6 +256,9 @@ class LinearSystemHIP:
         _chk(rc, what, self.ctx)
 
     def upload_mesh(self, mesh: Mesh, coloring=None):
+        # the C-ABI reads 3 V coordinates and E nvpe vertex ids through raw pointers
+        if np.asarray(mesh.xyz).size != 3 * mesh.V or np.asarray(mesh.conn).size != mesh.E * mesh.nvpe:
+            raise ValueError(f"mesh arrays do not match V = {mesh.V}, E = {mesh.E}, nvpe = {mesh.nvpe}")
         self.E = mesh.E
         self._c(self.L.fe_hip_upload_mesh(self.ctx, mesh.dim, mesh.V, N.ptr(mesh.xyz, C.c_double), mesh.E,
                                           mesh.nvpe, N.ptr(mesh.conn, C.c_int)), "fe_hip_upload_mesh")

@@ -110,7 +110,8 @@ def build_batch(problems, kind, device=0, keep_meshes=False) -> Batch:
 
 def union_layout(problems):
     """Host side of build_union_batch (no device work): per problem its mesh,
-    numbering and block coefficient; the union mesh (xyz, conn); the unknown
+    numbering and block coefficient; the union mesh (V vertices, 3 V
+    coordinates xyz, conn); the unknown
     rows (rows[p]..rows[p+1] for problem p, in draw order) and per problem the
     union DOF of each of its local DOFs (maps); per element kind the union DOF
     map of the field over every element of the union mesh (adr), the other
@@ -140,7 +141,7 @@ def union_layout(problems):
             adr[kind].append(gm[nm.adr[0]] if kinds[p] == kind else np.full(meshes[p].E * nl, dummy))
     adr = {kind: np.concatenate(v).astype(np.int32) for kind, v in adr.items()}
     return dict(meshes=meshes, nums=nums, coeffs=ks, kinds=kinds, nInc=nInc, NU=NU, rows=uoff.astype(np.int32),
-                dummy=dummy, xyz=xyz, conn=conn, maps=maps, adr=adr, coeff=np.concatenate(ks))
+                dummy=dummy, V=int(voff[-1]), xyz=xyz, conn=conn, maps=maps, adr=adr, coeff=np.concatenate(ks))
 
 
 def build_union_batch(problems, device=0, keep_meshes=False) -> Batch:
@@ -153,7 +154,7 @@ def build_union_batch(problems, device=0, keep_meshes=False) -> Batch:
     cover the whole draw."""
     t0 = time.perf_counter()
     U = union_layout(problems)
-    um = UnionMesh(2, int(U["xyz"].shape[0]), int(U["conn"].size // 3), 3, U["xyz"], U["conn"])
+    um = UnionMesh(2, U["V"], int(U["conn"].size // 3), 3, U["xyz"], U["conn"])
     S = A.LinearSystemHIP(device)
     S.upload_mesh(um)
     S.set_sizes(U["NU"], U["dummy"] + 1)

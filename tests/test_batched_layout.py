@@ -34,7 +34,8 @@ def test_union_layout_partitions_rows_and_maps_the_other_degree_to_one_dof():
             assert np.all((es >= ess[p]) & (es < ess[p + 1]))
         e0 += m.E
     assert e0 * 3 == U["adr"][A.TRI_P1].size and e0 * 6 == U["adr"][A.TRI_P2].size
-    assert U["conn"].max() < U["xyz"].shape[0] and U["coeff"].size == e0
+    assert U["V"] == sum(m.V for m in U["meshes"]) and U["xyz"].size == 3 * U["V"]
+    assert U["conn"].min() >= 0 and U["conn"].max() < U["V"] and U["coeff"].size == e0
 
 
 def test_union_pattern_is_block_diagonal_with_each_problems_own_pattern(oracle):
