@@ -744,6 +744,9 @@ int fe_hip_set_tuning(fe_hip_ctx *ctx, const char *key, int value)
   } else if(k == "pc_fp32") {
     FE_REQUIRE(ctx, value == 0 || value == 1, "pc_fp32 must be 0 or 1");
     ctx->pcFp32 = value;
+  } else if(k == "gmres_orth") {
+    FE_REQUIRE(ctx, value == 1 || value == 2, "gmres_orth must be 1 (classical Gram-Schmidt) or 2 (CGS2)");
+    ctx->gmresOrth = value;
   } else if(k == "ilu_sweeps") {
     FE_REQUIRE(ctx, value >= 0 && value <= 64,
                "ilu_sweeps must be 0 (exact triangular solves) or a sweep count in [1, 64]");
@@ -838,6 +841,7 @@ int fe_hip_get_tuning(fe_hip_ctx *ctx, const char *key, int *value)
   else if(k == "pipe_kernel") *value = ctx->pipeKernel;
   else if(k == "ic0_sweeps") *value = ctx->icSweeps;
   else if(k == "ilu_sweeps") *value = ctx->pc.sweeps;
+  else if(k == "gmres_orth") *value = ctx->gmresOrth;
   else if(k == "pc_fp32") *value = ctx->pcFp32;
   else if(k == "pc_values32") *value = ctx->pcValues32;
   else if(k == "amg_omega_x100") *value = (int)std::lround(ctx->amgOmega * 100.);

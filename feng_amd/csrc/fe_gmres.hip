@@ -3,9 +3,10 @@
 // PETSc default, 50 in the configs), FP64, right preconditioned so the
 // monitored residual is the true residual of A x = b.
 //
-// Arnoldi uses classical Gram-Schmidt with one reorthogonalization (CGS2):
-// all j+1 projections of a step come from one multi-dot kernel and one
-// multi-axpy kernel, instead of j+1 dependent dot products.  The small
+// Arnoldi uses classical Gram-Schmidt (PETSc's default orthogonalization,
+// no refinement), or with one reorthogonalization (CGS2, `gmres_orth` 2): all j+1
+// projections of a step come from one multi-dot kernel and one multi-axpy
+// kernel, instead of j+1 dependent dot products.  The small
 // Hessenberg least-squares problem (Givens rotations) lives on the host; the
 // host reads j+2 scalars per Arnoldi step.
 //
@@ -439,11 +440,15 @@ int feSolveGmres(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hi
         if(int rc = g.mark(ctx->iterTimer)) return rc;
       if(int rc = g.precond(vj, g.z)) { feFree(ctx, work); return rc; }
       g.spmv(g.z, g.w);
-      // CGS2
+      // classical Gram-Schmidt, twice (CGS2) when gmres_orth = 2
       if(int rc = g.dots(j + 1, g.V, g.w, hh.data(), true)) return rc;
       hipLaunchKernelGGL(multiAxpy, dim3(kG), dim3(kTB), 0, ctx->stream, n, j + 1, g.V, g.ld, g.h, g.w);
-      if(int rc = g.dots(j + 1, g.V, g.w, hh2.data(), true)) return rc;
-      hipLaunchKernelGGL(multiAxpy, dim3(kG), dim3(kTB), 0, ctx->stream, n, j + 1, g.V, g.ld, g.h, g.w);
+      if(ctx->gmresOrth == 2) {
+        if(int rc = g.dots(j + 1, g.V, g.w, hh2.data(), true)) return rc;
+        hipLaunchKernelGGL(multiAxpy, dim3(kG), dim3(kTB), 0, ctx->stream, n, j + 1, g.V, g.ld, g.h, g.w);
+      } else {
+        std::fill(hh2.begin(), hh2.begin() + j + 1, 0.);
+      }
       double wn = 0.;
       if(int rc = g.dots(1, g.w, g.w, &wn)) return rc;
       wn = std::sqrt(wn);

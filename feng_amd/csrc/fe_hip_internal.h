@@ -283,6 +283,7 @@ struct fe_hip_ctx {
   int amgFuse = 0;    // nested PCG: levels 1 .. coarsest in one launch with grid barriers (1) or one launch per level (0)
   int batchGraph = 1;  // batched multigrid PCG: chunks of iterations replayed from a captured graph
   int batchSkip = 1;   // batched multigrid PCG: the V-cycle skips the tiles of stopped problems
+  int gmresOrth = 1;   // GMRES Arnoldi: 1 = classical Gram-Schmidt once (PETSc's default), 2 = twice (CGS2)
   int nsMfma = 1;     // Taylor-Hood Jacobian sweeps (stress + divergence + convection) on the matrix cores
   int amgKernel = 2;  // FE_PC_AMG: 1 = per-level kernels over the original numbering,
                       // 2 = nested-ordered fused PCG (fe_amg_pcg.hip)
