@@ -84,7 +84,8 @@ def test_fieldsplit_gmres_matches_oracle_gmres_ilu0(oracle):
     """GMRES(50) with the block Schur preconditioner (FE_PC_FIELDSPLIT) on one
     Newton system of the jittered 24^2 Kovasznay problem: the solution of the
     same (values, RHS) as the oracle's GMRES(50) + ILU(0) (or_gmres_ilu0, the
-    reference's PETSc combination restated), both to tight tolerances."""
+    reference's PETSc combination restated), both to tight tolerances; with
+    classical Gram-Schmidt (the default, PETSc's) and with CGS2."""
     from feng_amd.api import KSP_GMRES, PC_FIELDSPLIT, Mesh
     from feng_amd.problems import navier_stokes_system
     print("fieldsplit vs oracle gmres-ilu0", flush=True)
@@ -105,6 +106,14 @@ def test_fieldsplit_gmres_matches_oracle_gmres_ilu0(oracle):
     assert err <= 1e-7
     res = rhs - oracle.spmv(ia, ja, val, du)
     assert np.linalg.norm(res) <= 1.01e-11 * np.linalg.norm(rhs)
+    # the Arnoldi steps with one reorthogonalization (CGS2, gmres_orth 2)
+    # instead of classical Gram-Schmidt: the same solution
+    S.set_tuning("gmres_orth", 2)
+    r2, du2 = S.solve(ksp=KSP_GMRES, pc=PC_FIELDSPLIT, rtol=1e-11, restart=50, max_iter=5000, want_du=True)
+    S.set_tuning("gmres_orth", 1)
+    assert r2.converged and r2.relres <= 1e-11
+    assert abs(r2.iterations - r.iterations) <= max(2, r.iterations // 20), (r.iterations, r2.iterations)
+    assert np.abs(du2 - x0).max() / np.abs(x0).max() <= 1e-7
     S.close()
 
 
