@@ -303,8 +303,8 @@ struct Gm {
     return FE_STATUS_OK;
   }
 
-  // profiling: an event pair around a launch (timed: the projection
-  // multi-dots of the Arnoldi steps, the dominant kernel, and whole steps)
+  // profiling: an event pair around a launch (timed: the A products of the
+  // Arnoldi steps, the dominant kernel, and whole steps)
   int mark(FeKernelTimer &t)
   {
     if(t.used + 1 > t.ev.size())
@@ -439,12 +439,18 @@ int feSolveGmres(fe_hip_ctx *ctx, const fe_hip_solver_opts *o, double *du, fe_hi
       if(ctx->profiling)
         if(int rc = g.mark(ctx->iterTimer)) return rc;
       if(int rc = g.precond(vj, g.z)) { feFree(ctx, work); return rc; }
+      // profiling: events around the A product (the dominant kernel of the
+      // Arnoldi steps), collected at the next host read
+      if(ctx->profiling)
+        if(int rc = g.mark(ctx->spmvTimer)) return rc;
       g.spmv(g.z, g.w);
+      if(ctx->profiling)
+        if(int rc = g.mark(ctx->spmvTimer)) return rc;
       // classical Gram-Schmidt, twice (CGS2) when gmres_orth = 2
-      if(int rc = g.dots(j + 1, g.V, g.w, hh.data(), true)) return rc;
+      if(int rc = g.dots(j + 1, g.V, g.w, hh.data())) return rc;
       hipLaunchKernelGGL(multiAxpy, dim3(kG), dim3(kTB), 0, ctx->stream, n, j + 1, g.V, g.ld, g.h, g.w);
       if(ctx->gmresOrth == 2) {
-        if(int rc = g.dots(j + 1, g.V, g.w, hh2.data(), true)) return rc;
+        if(int rc = g.dots(j + 1, g.V, g.w, hh2.data())) return rc;
         hipLaunchKernelGGL(multiAxpy, dim3(kG), dim3(kTB), 0, ctx->stream, n, j + 1, g.V, g.ld, g.h, g.w);
       } else {
         std::fill(hh2.begin(), hh2.begin() + j + 1, 0.);
